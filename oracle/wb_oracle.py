@@ -1,0 +1,1008 @@
+"""FP64 NumPy oracle for the Berry-quantity path of wannier-berri.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: `tests/`,
+`__graft_entry__.smoke()` and the `cpu_baseline` leg of `bench.py` may import
+it; the product package (`wannierberri_amd`) never does.
+
+It restates, in plain NumPy/FP64 and in the reference's own per-(k, group)
+loop structure, the path
+
+    System_R -> Data_K_R (R->k Fourier, eigh, U^+ X U) -> covariant formulas
+    -> StaticCalculator Fermi-level scan -> k-sum over K-blocks
+
+Every function cites the reference file:line it follows (paths relative to
+the wannier-berri source tree).  How each part is pinned (reference fixtures,
+closed forms, invariants) is recorded in the header of
+`tests/test_oracle_pins.py`; all functions listed there carry a pin.
+"""
+
+from collections import defaultdict
+from functools import cached_property
+from math import ceil
+import os
+
+import numpy as np
+from scipy.constants import (elementary_charge as _e, hbar as _hbar,
+                             electron_mass as _me, physical_constants as _pc,
+                             angstrom as _ang)
+
+# ---------------------------------------------------------------------------
+# constants and index maps
+# ---------------------------------------------------------------------------
+
+# Levi-Civita index maps: component c of a curl is d_alpha X_beta - d_beta X_alpha
+# (wannierberri/utility.py:45-46)
+ALPHA = np.array([1, 2, 0])
+BETA = np.array([2, 0, 1])
+
+# physical unit factors (wannierberri/factors.py:6-28), SciPy CODATA values
+BOHR_MAGNETON = _e * _hbar / (2 * _me)
+BOHR = _pc['Bohr radius'][0] / _ang
+EV_AU = _pc['electron volt-hartree relationship'][0]
+FACTOR_MORB_EVA2_TO_MUB = -(_e * _ang ** 2) * _e / (2 * _hbar) / BOHR_MAGNETON
+FACTOR_GME_ORB = FACTOR_MORB_EVA2_TO_MUB * BOHR_MAGNETON / _ang ** 2
+FACTOR_AHC = -(_e ** 2 / _hbar / _ang)
+FACTOR_NLAHC = _e ** 3 / _hbar ** 2 * 1e-15
+
+# dEig_inv threshold (wannierberri/data_K/data_K.py:292)
+DEIG_THRESHOLD = 1e-7
+
+
+# ---------------------------------------------------------------------------
+# System (R-space model)
+# ---------------------------------------------------------------------------
+
+class OracleSystem:
+    """R-space tight-binding / Wannier model.
+
+    Matrices are stored as `[nR, nw, nw, 3^m]` complex arrays, as in
+    `System_R._XX_R` (wannierberri/system/system_R.py:117-175).  The
+    derivative shifts are the Wannier centres (`shifts_left_red =
+    wannier_centers_red`, wannierberri/system/system_R.py:776-780).
+    """
+
+    def __init__(self, real_lattice, iRvec, wannier_centers_cart, matrices,
+                 periodic=(True, True, True), force_internal_terms_only=False):
+        self.real_lattice = np.array(real_lattice, dtype=float)
+        self.iRvec = np.array(iRvec, dtype=int)
+        self.wannier_centers_cart = np.array(wannier_centers_cart, dtype=float)
+        self.matrices = {k: np.array(v, dtype=complex) for k, v in matrices.items()}
+        self.periodic = np.array(periodic, dtype=bool)
+        self.force_internal_terms_only = force_internal_terms_only
+        self.num_wann = self.matrices['Ham'].shape[1]
+        assert self.matrices['Ham'].shape[0] == self.iRvec.shape[0]
+
+    @property
+    def nRvec(self):
+        return self.iRvec.shape[0]
+
+    @cached_property
+    def cell_volume(self):
+        # wannierberri/system/system.py:173-181
+        return abs(np.linalg.det(self.real_lattice))
+
+    @cached_property
+    def recip_lattice(self):
+        # wannierberri/utility.py:50-51
+        return 2 * np.pi * np.linalg.inv(self.real_lattice).T
+
+    @cached_property
+    def wannier_centers_red(self):
+        # wannierberri/system/system_R.py:608-610
+        return self.wannier_centers_cart.dot(np.linalg.inv(self.real_lattice))
+
+    @cached_property
+    def cRvec_shifted(self):
+        """R + t_n - t_m for element (m, n): shape [nR, nw, nw, 3].
+
+        wannierberri/fourier/rvectors.py:351-369 (cRvec, shifts_diff_cart);
+        the shift is -shift_left[m] + shift_right[n] with both equal to the
+        Wannier centres.
+        """
+        shifts_cart = self.wannier_centers_red.dot(self.real_lattice)
+        cR = self.iRvec.dot(self.real_lattice)
+        diff = -shifts_cart[:, None, :] + shifts_cart[None, :, :]
+        return cR[:, None, None, :] + diff[None, :, :, :]
+
+    def has(self, key):
+        return key in self.matrices
+
+    def NKFFT_recommended(self):
+        # wannierberri/fourier/rvectors.py:280-292
+        nk = np.ones(3, dtype=int)
+        for i in range(3):
+            R = self.iRvec[:, i]
+            if (R > 0).any():
+                nk[i] += R.max()
+            if (R < 0).any():
+                nk[i] -= R.min()
+        return nk
+
+
+def conj_XX_R(iRvec, X):
+    """X'(R) = X(-R)^dagger (wannierberri/fourier/rvectors.py:397-430)."""
+    index = {tuple(R): i for i, R in enumerate(iRvec)}
+    out = np.zeros_like(X)
+    for i, R in enumerate(iRvec):
+        j = index.get(tuple(-R))
+        if j is not None:
+            out[i] = X[j]
+    return out.swapaxes(1, 2).conj()
+
+
+def load_system_dir(path, legacy=False, force_internal_terms_only=False, matrices=None):
+    """Read a directory of `.npz` files in the `System_R.to_npz` layout.
+
+    wannierberri/system/system_R.py:733-799.  Matrix files are named either
+    `_XX_R_<key>.npz` (current layout) or `<key>.npz` (older test fixtures).
+    With `legacy=True` matrices are stored as `[m, n, R, ...]` and are
+    transposed to `[R, m, n, ...]` (system_R.py:795-796).
+    """
+    def arr(name):
+        with np.load(os.path.join(path, name + ".npz"), allow_pickle=False) as f:
+            return f['arr_0']
+    real_lattice = arr('real_lattice')
+    iRvec = arr('iRvec')
+    if os.path.exists(os.path.join(path, 'wannier_centers_cart.npz')):
+        wcc = arr('wannier_centers_cart')
+    else:
+        wcc = arr('wannier_centers_red').dot(real_lattice)
+    periodic = arr('periodic') if os.path.exists(os.path.join(path, 'periodic.npz')) else (True, True, True)
+    if matrices is None:
+        matrices = []
+        for f in sorted(os.listdir(path)):
+            if f.startswith('_XX_R_'):
+                matrices.append(f[6:-4])
+            elif f[:-4] in ('Ham', 'AA', 'BB', 'CC', 'OO', 'SS'):
+                matrices.append(f[:-4])
+    mats = {}
+    for key in matrices:
+        fn = '_XX_R_' + key if os.path.exists(os.path.join(path, '_XX_R_' + key + '.npz')) else key
+        a = arr(fn)
+        if legacy:
+            a = np.transpose(a, (2, 0, 1) + tuple(range(3, a.ndim)))
+        mats[key] = a
+    return OracleSystem(real_lattice, iRvec, wcc, mats, periodic=periodic,
+                        force_internal_terms_only=force_internal_terms_only)
+
+
+# ---------------------------------------------------------------------------
+# R -> k
+# ---------------------------------------------------------------------------
+
+def r2k_grid(X_R, iRvec, NKFFT, hermitian):
+    """sum_R X_R exp(2 pi i k.R) on the NKFFT grid (phase exp(2 pi i dK.R) already applied).
+
+    wannierberri/fourier/fft.py:172-189: scatter to `iRvec mod NKFFT`,
+    inverse FFT times prod(NKFFT), optional 0.5 (X + X^dagger), reshape
+    to `[nk, ...]` with the z index running fastest.
+    """
+    NKFFT = tuple(int(n) for n in NKFFT)
+    grid = np.zeros(NKFFT + X_R.shape[1:], dtype=complex)
+    for ir, r in enumerate(iRvec % np.array(NKFFT)):
+        grid[tuple(r)] += X_R[ir]
+    grid = np.fft.ifftn(grid, axes=(0, 1, 2)) * np.prod(NKFFT)
+    if hermitian:
+        grid = 0.5 * (grid + grid.swapaxes(3, 4).conj())
+    return grid.reshape((int(np.prod(NKFFT)),) + X_R.shape[1:])
+
+
+def r2k_grid_slow(X_R, iRvec, NKFFT, hermitian):
+    """Brute-force DFT over the same grid (wannierberri/fourier/fft.py:158-169)."""
+    NKFFT = np.array(NKFFT)
+    ks = np.array([[i, j, l] for i in range(NKFFT[0]) for j in range(NKFFT[1])
+                   for l in range(NKFFT[2])], dtype=float) / NKFFT[None, :]
+    out = r2k_klist(X_R, iRvec, ks, hermitian=False)
+    if hermitian:
+        out = 0.5 * (out + out.swapaxes(1, 2).conj())
+    return out
+
+
+def r2k_klist(X_R, iRvec, k_list, hermitian):
+    """Phase-matrix transform for an arbitrary k-list.
+
+    wannierberri/fourier/fft.py:125-130,170-171 (exponent_k_list, einsum
+    'ij,j...->i...'), hermitised over axes (1, 2) (fft.py:183-184).
+    """
+    P = np.exp(2j * np.pi * (np.asarray(k_list) @ iRvec.T))
+    out = np.einsum('ij,j...->i...', P, X_R)
+    if hermitian:
+        out = 0.5 * (out + out.swapaxes(1, 2).conj())
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Data_K
+# ---------------------------------------------------------------------------
+
+class OracleDataK:
+    """Per-K-block evaluation (wannierberri/data_K/data_K.py:49-334,
+    wannierberri/data_K/data_K_R.py:8-97).
+
+    Either `dK` + `NKFFT` (grid mode; k = (j/NKFFT + dK) mod 1) or `k_list`
+    (arbitrary k-points, no expdK phase: rvectors.py:481-482).
+    """
+
+    def __init__(self, system, dK=None, NKFFT=None, k_list=None):
+        assert (dK is None) != (k_list is None)
+        self.system = system
+        self.num_wann = system.num_wann
+        self.cell_volume = system.cell_volume
+        self.force_internal_terms_only = system.force_internal_terms_only
+        self.k_list = None if k_list is None else np.array(k_list, dtype=float)
+        if k_list is None:
+            self.NKFFT = np.array(NKFFT, dtype=int)
+            self.dK = np.array(dK, dtype=float)
+            # rvectors.py:468
+            self.expdK = np.exp(2j * np.pi * system.iRvec.dot(self.dK))
+        self._Rmat = {}
+        self._bar = {}
+        self._cov = {}
+
+    @cached_property
+    def nk(self):
+        # data_K.py:153-158
+        if self.k_list is None:
+            return int(np.prod(self.NKFFT))
+        return len(self.k_list)
+
+    @cached_property
+    def kpoints_all(self):
+        # data_K.py:146-151, grid.py:68-75
+        if self.k_list is not None:
+            return self.k_list
+        N = self.NKFFT
+        pts = np.array([[i / N[0], j / N[1], l / N[2]] for i in range(N[0])
+                        for j in range(N[1]) for l in range(N[2])])
+        return (pts + self.dK[None]) % 1
+
+    # -- R-space matrices -------------------------------------------------
+    def get_R_mat(self, key):
+        """data_K_R.py:28-51 and rvectors.py:477-485."""
+        if key in self._Rmat:
+            return self._Rmat[key]
+        if key == 'rotAA':
+            # data_K_R.py:48-51 (the phase is already inside AA)
+            dA = self.derivative(self.get_R_mat('AA'))
+            res = dA[:, :, :, BETA, ALPHA] - dA[:, :, :, ALPHA, BETA]
+        else:
+            X = self.system.matrices[key]
+            if self.k_list is None:
+                res = X * self.expdK.reshape((-1,) + (1,) * (X.ndim - 1))
+            else:
+                res = X
+        self._Rmat[key] = res
+        return res
+
+    def derivative(self, X):
+        """i (R + t_n - t_m) X, appending one Cartesian axis (rvectors.py:487-494)."""
+        c = self.system.cRvec_shifted
+        shape = c.shape[:3] + (1,) * (X.ndim - 3) + (3,)
+        return 1j * X[..., None] * c.reshape(shape)
+
+    def R_to_k(self, X, der=0, hermitian=True):
+        """rvectors.py:496-506."""
+        for _ in range(der):
+            X = self.derivative(X)
+        if self.k_list is None:
+            return r2k_grid(X, self.system.iRvec, self.NKFFT, hermitian)
+        return r2k_klist(X, self.system.iRvec, self.k_list, hermitian)
+
+    # -- eigenproblem -----------------------------------------------------
+    @cached_property
+    def HH_K(self):
+        # data_K_R.py:24-26
+        return self.R_to_k(self.get_R_mat('Ham'), hermitian=True)
+
+    @cached_property
+    def _eig(self):
+        # data_K.py:211-218
+        return np.linalg.eigh(self.HH_K)
+
+    @property
+    def E_K(self):
+        return self._eig[0]
+
+    @property
+    def UU_K(self):
+        return self._eig[1]
+
+    def _rotate(self, mat):
+        # data_K.py:130-132
+        U = self.UU_K
+        return np.einsum('kba,kbc...,kcd->kad...', U.conj(), mat, U, optimize=True)
+
+    def Xbar(self, name, der=0):
+        """data_K_R.py:69-97: R->k (hermitised for AA/SS/OO/rotAA) then U^+ X U."""
+        key = (name, der)
+        if key not in self._bar:
+            X = self.R_to_k(self.get_R_mat(name).copy(), der=der,
+                            hermitian=(name in ('AA', 'SS', 'OO', 'rotAA')))
+            self._bar[key] = self._rotate(X)
+        return self._bar[key]
+
+    @cached_property
+    def dEig_inv(self):
+        # data_K.py:290-298
+        dE = self.E_K[:, :, None] - self.E_K[:, None, :]
+        sel = abs(dE) < DEIG_THRESHOLD
+        dE[sel] = DEIG_THRESHOLD
+        dE = 1. / dE
+        dE[sel] = 0.
+        return dE
+
+    @cached_property
+    def D_H(self):
+        # data_K.py:324-326
+        return -self.Xbar('Ham', 1) * self.dEig_inv[:, :, :, None]
+
+    @cached_property
+    def Dcov(self):
+        # data_K.py:286-288
+        return Dcov(self)
+
+    def covariant(self, name, commader=0, gender=0):
+        """data_K.py:244-284."""
+        key = (name, commader, gender)
+        if key not in self._cov:
+            if gender == 0:
+                res = Matrix_ln(self.Xbar(name, commader))
+            elif name == 'Ham':
+                res = _VelocityLn(self.Xbar('Ham', 1))
+            else:
+                res = Matrix_GenDer_ln(self.covariant(name), self.covariant(name, commader=1), self.Dcov)
+            self._cov[key] = res
+        return self._cov[key]
+
+    # -- band groups ------------------------------------------------------
+    def get_bands_in_range_groups_ik(self, ik, emin, emax, degen_thresh=-1, degen_Kramers=False, sea=False):
+        """data_K.py:172-186 with grid/tetrahedron.py:132-162."""
+        E = self.E_K[ik]
+        groups = band_groups(E, degen_thresh, degen_Kramers)
+        in_range = [(a, b) for a, b in groups if E[a:b].max() >= emin and E[a:b].min() <= emax]
+        weights = {(a, b): E[a:b].mean() for a, b in in_range}
+        if sea:
+            below = np.where(E < emin)[0]
+            bandmax = below[-1] + 1 if len(below) > 0 else 0
+            if len(in_range) > 0:
+                bandmax = min(bandmax, in_range[0][0])
+            if bandmax > 0:
+                weights[(0, bandmax)] = -np.inf
+        return weights
+
+    def get_bands_in_range_groups(self, emin, emax, degen_thresh=-1, degen_Kramers=False, sea=False):
+        return [self.get_bands_in_range_groups_ik(ik, emin, emax, degen_thresh, degen_Kramers, sea)
+                for ik in range(self.nk)]
+
+
+def band_groups(E, degen_thresh, degen_Kramers=False):
+    """Split sorted energies where the gap exceeds degen_thresh (tetrahedron.py:132-136)."""
+    borders = [0] + list(np.where((E[1:] - E[:-1]) > degen_thresh)[0] + 1) + [len(E)]
+    if degen_Kramers:
+        borders = [i for i in borders if i % 2 == 0]
+    return list(zip(borders, borders[1:]))
+
+
+# ---------------------------------------------------------------------------
+# covariant formula algebra (wannierberri/formula/*.py)
+# ---------------------------------------------------------------------------
+
+class Formula_ln:
+    """Sub-block access for an (inn, out) split (formula/formula.py:31-78)."""
+    ndim = 0
+    additive = True
+    internal_terms = True
+    external_terms = True
+
+    def _set_terms(self, data_K, internal_terms=True, external_terms=True):
+        # formula.py:11-25
+        self.internal_terms = internal_terms
+        self.external_terms = external_terms and not data_K.force_internal_terms_only
+
+    def nl(self, ik, inn, out):
+        return self.ln(ik, out, inn)
+
+    def ll(self, ik, inn, out):
+        return self.nn(ik, out, inn)
+
+    def trace(self, ik, inn, out):
+        return np.einsum("nn...->...", self.nn(ik, inn, out)).real
+
+
+class Matrix_ln(Formula_ln):
+    # formula.py:81-92
+    def __init__(self, matrix):
+        self.matrix = matrix
+        self.ndim = matrix.ndim - 3
+
+    def ln(self, ik, inn, out):
+        return self.matrix[ik][out][:, inn]
+
+    def nn(self, ik, inn, out):
+        return self.matrix[ik][inn][:, inn]
+
+
+class _VelocityLn(Matrix_ln):
+    # data_K.py:273-284: the velocity has no ln part
+    def ln(self, ik, inn, out):
+        return np.zeros((len(out), len(inn), 3), dtype=complex)
+
+
+class Matrix_GenDer_ln(Formula_ln):
+    """Generalised derivative dA - D.A + A.D (formula.py:95-118)."""
+
+    def __init__(self, A, dA, D):
+        self.A, self.dA, self.D = A, dA, D
+        self.ndim = A.ndim + 1
+
+    def nn(self, ik, inn, out):
+        s = self.dA.nn(ik, inn, out).copy()
+        s -= np.einsum("mld,lnb...->mnb...d", self.D.nl(ik, inn, out), self.A.ln(ik, inn, out))
+        s += np.einsum("mlb...,lnd->mnb...d", self.A.nl(ik, inn, out), self.D.ln(ik, inn, out))
+        return s
+
+    def ln(self, ik, inn, out):
+        s = self.dA.ln(ik, inn, out).copy()
+        s -= np.einsum("mld,lnb...->mnb...d", self.D.ln(ik, inn, out), self.A.nn(ik, inn, out))
+        s += np.einsum("mlb...,lnd->mnb...d", self.A.ll(ik, inn, out), self.D.ln(ik, inn, out))
+        return s
+
+
+class Dcov(Matrix_ln):
+    # formula/elementary.py:46-52
+    def __init__(self, data_K):
+        super().__init__(data_K.D_H)
+
+    def nn(self, ik, inn, out):
+        raise ValueError("D is not defined within the inner states")
+
+
+class DEinv_ln(Matrix_ln):
+    # formula/elementary.py:18-25
+    def __init__(self, data_K):
+        super().__init__(data_K.dEig_inv)
+
+
+class Eavln(Matrix_ln):
+    # formula/elementary.py:8-15
+    def __init__(self, data_K):
+        super().__init__(0.5 * (data_K.E_K[:, :, None] + data_K.E_K[:, None, :]))
+        self.ndim = 0
+
+
+class DerDcov(Dcov):
+    """k-derivative of D (formula/elementary.py:55-70)."""
+
+    def __init__(self, data_K):
+        self.W = data_K.covariant('Ham', commader=2)
+        self.V = data_K.covariant('Ham', gender=1)
+        self.D = data_K.Dcov
+        self.dEinv = DEinv_ln(data_K)
+        self.ndim = 2
+
+    def ln(self, ik, inn, out):
+        s = self.W.ln(ik, inn, out).copy()
+        t = np.einsum("lpb,pnd->lnbd", self.V.ll(ik, inn, out), self.D.ln(ik, inn, out))
+        s += t + t.swapaxes(2, 3)
+        t = -np.einsum("lmb,mnd->lnbd", self.D.ln(ik, inn, out), self.V.nn(ik, inn, out))
+        s += t + t.swapaxes(2, 3)
+        s *= -self.dEinv.ln(ik, inn, out)[:, :, None, None]
+        return s
+
+
+class InvMass(Matrix_GenDer_ln):
+    # formula/elementary.py:28-33
+    def __init__(self, data_K):
+        super().__init__(data_K.covariant('Ham', commader=1), data_K.covariant('Ham', commader=2), data_K.Dcov)
+
+
+class Identity(Formula_ln):
+    # formula/covariant.py:10-22
+    def __init__(self, data_K=None, **kw):
+        self.ndim = 0
+
+    def nn(self, ik, inn, out):
+        return np.eye(len(inn))
+
+    def ln(self, ik, inn, out):
+        return np.zeros((len(out), len(inn)))
+
+
+class Omega(Formula_ln):
+    """Berry curvature (formula/covariant.py:161-203)."""
+
+    def __init__(self, data_K, **kw):
+        self._set_terms(data_K, **kw)
+        self.D = data_K.Dcov
+        if self.external_terms:
+            self.A = data_K.covariant('AA')
+            self.O = data_K.covariant('rotAA')
+        self.ndim = 1
+
+    def nn(self, ik, inn, out):
+        s = np.zeros((len(inn), len(inn), 3), dtype=complex)
+        Dnl = self.D.nl(ik, inn, out)
+        Dln = self.D.ln(ik, inn, out)
+        if self.internal_terms:
+            s += -1j * np.einsum("mlc,lnc->mnc", Dnl[:, :, ALPHA], Dln[:, :, BETA])
+        if self.external_terms:
+            Aln = self.A.ln(ik, inn, out)
+            Ann = self.A.nn(ik, inn, out)
+            s += 0.5 * self.O.nn(ik, inn, out)
+            s -= np.einsum("mlc,lnc->mnc", Dnl[:, :, ALPHA], Aln[:, :, BETA])
+            s += np.einsum("mlc,lnc->mnc", Dnl[:, :, BETA], Aln[:, :, ALPHA])
+            s += -1j * np.einsum("mlc,lnc->mnc", Ann[:, :, ALPHA], Ann[:, :, BETA])
+        return s + s.swapaxes(0, 1).conj()
+
+
+class DerOmega(Formula_ln):
+    """k-derivative of the Berry curvature (formula/covariant.py:212-259)."""
+
+    def __init__(self, data_K, **kw):
+        self._set_terms(data_K, **kw)
+        self.dD = DerDcov(data_K)
+        self.D = data_K.Dcov
+        if self.external_terms:
+            self.A = data_K.covariant('AA')
+            self.dA = data_K.covariant('AA', gender=1)
+            self.dO = data_K.covariant('rotAA', gender=1)
+        self.ndim = 2
+
+    def nn(self, ik, inn, out):
+        s = np.zeros((len(inn), len(inn), 3, 3), dtype=complex)
+        if self.external_terms:
+            s += 0.5 * self.dO.nn(ik, inn, out)
+        Dnl = self.D.nl(ik, inn, out)
+        dDln = self.dD.ln(ik, inn, out)
+        if self.external_terms:
+            dAln = self.dA.ln(ik, inn, out)
+            dDnl = self.dD.nl(ik, inn, out)
+            Aln = self.A.ln(ik, inn, out)
+            Ann = self.A.nn(ik, inn, out)
+            dAnn = self.dA.nn(ik, inn, out)
+        for sg, a, b in ((+1, ALPHA, BETA), (-1, BETA, ALPHA)):
+            if self.internal_terms:
+                s += -1j * sg * np.einsum("mlc,lncd->mncd", Dnl[:, :, a], dDln[:, :, b])
+            if self.external_terms:
+                s += -sg * np.einsum("mlc,lncd->mncd", Dnl[:, :, a], dAln[:, :, b, :])
+                s += -sg * np.einsum("mlcd,lnc->mncd", dDnl[:, :, a, :], Aln[:, :, b])
+                s += -1j * sg * np.einsum("mlc,lncd->mncd", Ann[:, :, a], dAnn[:, :, b, :])
+        return s + s.swapaxes(0, 1).conj()
+
+
+class Morb_H(Formula_ln):
+    """eps_abc <d_a u| H |d_b u> (formula/covariant.py:375-421)."""
+    additive = False
+
+    def __init__(self, data_K, **kw):
+        self._set_terms(data_K, **kw)
+        if self.external_terms:
+            self.A = data_K.covariant('AA')
+            self.B = data_K.covariant('BB')
+            self.C = data_K.covariant('CC')
+        self.D = data_K.Dcov
+        self.E = data_K.E_K
+        self.ndim = 1
+
+    def nn(self, ik, inn, out):
+        s = np.zeros((len(inn), len(inn), 3), dtype=complex)
+        Dnl = self.D.nl(ik, inn, out)
+        Dln = self.D.ln(ik, inn, out)
+        if self.internal_terms:
+            s += -1j * np.einsum("mlc,lnc->mnc", Dnl[:, :, ALPHA] * self.E[ik][out][None, :, None],
+                                 Dln[:, :, BETA])
+        if self.external_terms:
+            Bln = self.B.ln(ik, inn, out)
+            Ann = self.A.nn(ik, inn, out)
+            s += 0.5 * self.C.nn(ik, inn, out)
+            s -= np.einsum("mlc,lnc->mnc", Dnl[:, :, ALPHA], Bln[:, :, BETA])
+            s += np.einsum("mlc,lnc->mnc", Dnl[:, :, BETA], Bln[:, :, ALPHA])
+            s += -1j * np.einsum("mlc,lnc->mnc", Ann[:, :, ALPHA] * self.E[ik][inn][None, :, None],
+                                 Ann[:, :, BETA])
+        return s + s.swapaxes(0, 1).conj()
+
+
+class Morb_Hpm(Formula_ln):
+    """Morb_H +- (E_n + E_m)/2 Omega (formula/covariant.py:424-449)."""
+    additive = False
+
+    def __init__(self, data_K, sign=+1, **kw):
+        self.H = Morb_H(data_K, **kw)
+        self.sign = sign
+        if sign != 0:
+            self.O = Omega(data_K, **kw)
+            self.Eav = Eavln(data_K)
+        self.ndim = 1
+
+    def nn(self, ik, inn, out):
+        r = self.H.nn(ik, inn, out)
+        if self.sign != 0:
+            r = r + self.sign * self.Eav.nn(ik, inn, out)[:, :, None] * self.O.nn(ik, inn, out)
+        return r
+
+
+class morb(Morb_Hpm):
+    def __init__(self, data_K, **kw):
+        super().__init__(data_K, sign=-1, **kw)
+
+
+class DerMorb_H(Formula_ln):
+    """k-derivative of Morb_H (formula/covariant.py:463-519)."""
+    additive = False
+
+    def __init__(self, data_K, **kw):
+        self._set_terms(data_K, **kw)
+        self.dD = DerDcov(data_K)
+        self.D = data_K.Dcov
+        self.V = data_K.covariant('Ham', commader=1)
+        self.E = data_K.E_K
+        if self.external_terms:
+            self.A = data_K.covariant('AA')
+            self.dA = data_K.covariant('AA', gender=1)
+            self.B = data_K.covariant('BB')
+            self.dB = data_K.covariant('BB', gender=1)
+            self.dH = data_K.covariant('CC', gender=1)
+        self.ndim = 2
+
+    def nn(self, ik, inn, out):
+        s = np.zeros((len(inn), len(inn), 3, 3), dtype=complex)
+        Dnl = self.D.nl(ik, inn, out)
+        Dln = self.D.ln(ik, inn, out)
+        if self.internal_terms:
+            s += -2j * np.einsum("mpc,pld,lnc->mncd", Dnl[:, :, ALPHA], self.V.ll(ik, inn, out),
+                                 Dln[:, :, BETA])
+            dDln = self.dD.ln(ik, inn, out)
+            for sg, a, b in ((+1, ALPHA, BETA), (-1, BETA, ALPHA)):
+                s += -2j * sg * np.einsum("mlc,lncd->mncd", Dnl[:, :, a],
+                                          self.E[ik][out][:, None, None, None] * dDln[:, :, b])
+        if self.external_terms:
+            Ann = self.A.nn(ik, inn, out)
+            dAnn = self.dA.nn(ik, inn, out)
+            Bln = self.B.ln(ik, inn, out)
+            dBln = self.dB.ln(ik, inn, out)
+            dDln = self.dD.ln(ik, inn, out)
+            s += self.dH.nn(ik, inn, out)
+            s += -2j * np.einsum("mpc,pld,lnc->mncd", Ann[:, :, ALPHA], self.V.nn(ik, inn, out),
+                                 Ann[:, :, BETA])
+            for sg, a, b in ((+1, ALPHA, BETA), (-1, BETA, ALPHA)):
+                s += -2j * sg * np.einsum("mlc,lncd->mncd", Ann[:, :, a] * self.E[ik][inn][None, :, None],
+                                          dAnn[:, :, b, :])
+                s += -2 * sg * np.einsum("mlc,lncd->mncd", Dnl[:, :, a], dBln[:, :, b, :])
+                s += -2 * sg * np.einsum("mlc,lncd->mncd", Bln[:, :, a].transpose(1, 0, 2).conj(),
+                                         dDln[:, :, b, :])
+        return (s + s.swapaxes(0, 1).conj()) / 2
+
+
+class DerMorb(Formula_ln):
+    """DerMorb_H + sign * d[(E_n+E_m)/2 Omega] (formula/covariant.py:522-552)."""
+    additive = False
+
+    def __init__(self, data_K, sign=+1, **kw):
+        self.dermorb_H = DerMorb_H(data_K, **kw)
+        self.sign = sign
+        if sign != 0:
+            self.V = data_K.covariant('Ham', commader=1)
+            self.dO = DerOmega(data_K, **kw)
+            self.O = Omega(data_K, **kw)
+            self.Eav = Eavln(data_K)
+        self.ndim = 2
+
+    def nn(self, ik, inn, out):
+        r = self.dermorb_H.nn(ik, inn, out)
+        if self.sign != 0:
+            On = self.O.nn(ik, inn, out)
+            Vn = self.V.nn(ik, inn, out)
+            t = self.Eav.nn(ik, inn, out)[:, :, None, None] * self.dO.nn(ik, inn, out)
+            t = t + 0.5 * np.einsum("mlc,lnd->mncd", On, Vn)
+            t = t + 0.5 * np.einsum("mld,lnc->mncd", Vn, On)
+            r = r + self.sign * (t + t.swapaxes(0, 1).conj()) / 2
+        return r
+
+
+class Dermorb(DerMorb):
+    def __init__(self, data_K, **kw):
+        super().__init__(data_K, sign=-1, **kw)
+
+
+class FormulaProduct(Formula_ln):
+    """Product of nn blocks (formula/formula.py:121-148)."""
+
+    def __init__(self, formulae):
+        self.formulae = formulae
+        ndims = [f.ndim for f in formulae]
+        self.ndim = sum(ndims)
+        letters = "abcdefghijklmnopqrstuvw"
+        self.lines = []
+        dim = ndims[0]
+        for d in ndims[1:]:
+            self.lines.append("LM" + letters[:dim] + ",MN" + letters[dim:dim + d] + "->LN" + letters[:dim + d])
+            dim += d
+
+    def nn(self, ik, inn, out):
+        mats = [f.nn(ik, inn, out) for f in self.formulae]
+        r = mats[0]
+        for m, line in zip(mats[1:], self.lines):
+            r = np.einsum(line, r, m)
+        return np.array(r, dtype=complex)
+
+
+def VelOmega(data_K, **kw):
+    # formula/covariant.py:798-801
+    return FormulaProduct([data_K.covariant('Ham', commader=1), Omega(data_K, **kw)])
+
+
+def VelHplus(data_K, **kw):
+    # formula/covariant.py:804-808
+    return FormulaProduct([data_K.covariant('Ham', commader=1), Morb_Hpm(data_K, sign=+1, **kw)])
+
+
+# ---------------------------------------------------------------------------
+# results
+# ---------------------------------------------------------------------------
+
+class OracleResult:
+    """Minimal EnergyResult: data [nEF, 3^rank] (result/energyresult.py:131-194)."""
+
+    def __init__(self, Efermi, data):
+        self.Efermi = np.array(Efermi)
+        self.data = np.array(data)
+
+    def __add__(self, other):
+        if other is None or (isinstance(other, (int, float)) and other == 0):
+            return self
+        return OracleResult(self.Efermi, self.data + other.data)
+
+    __radd__ = __add__
+
+    def __mul__(self, x):
+        return OracleResult(self.Efermi, self.data * x)
+
+    __rmul__ = __mul__
+
+    def __sub__(self, other):
+        return OracleResult(self.Efermi, self.data - other.data)
+
+    def mul_array(self, arr):
+        shape = (-1,) + (1,) * (self.data.ndim - 1)
+        return OracleResult(self.Efermi, self.data * np.asarray(arr).reshape(shape))
+
+
+# ---------------------------------------------------------------------------
+# static calculators (wannierberri/calculators/static.py)
+# ---------------------------------------------------------------------------
+
+class StaticCalculator:
+    """Fermi-level scan integrator (calculators/static.py:24-169, calculator.py:20-26).
+
+    Only the non-tetrahedron, non-k-resolved, all-bands branch is restated.
+    """
+    Formula = None
+    fder = None
+    constant_factor_default = 1.
+
+    def __init__(self, Efermi, constant_factor=None, use_factor=True, kwargs_formula=None,
+                 degen_thresh=1e-4, degen_Kramers=False, hole_like=False, Formula=None, fder=None):
+        self.Efermi = np.array(Efermi, dtype=float)
+        if Formula is not None:
+            self.Formula = Formula
+        if fder is not None:
+            self.fder = fder
+        self.kwargs_formula = dict(kwargs_formula or {})
+        self.degen_thresh = degen_thresh
+        self.degen_Kramers = degen_Kramers
+        self.use_factor = use_factor
+        self.constant_factor = self.constant_factor_default if constant_factor is None else constant_factor
+        if hole_like and self.fder == 0:
+            self.constant_factor *= -1
+        self.extraEf = {0: 0, 1: 1, 2: 1, 3: 2}[self.fder]
+        self.dEF = self.Efermi[1] - self.Efermi[0] if len(self.Efermi) > 1 else 0.001
+        self.EFmin = self.Efermi[0] - self.extraEf * self.dEF
+        self.EFmax = self.Efermi[-1] + self.extraEf * self.dEF
+        self.nEF_extra = self.Efermi.shape[0] + 2 * self.extraEf
+
+    def group_values(self, data_K):
+        """Per-k dict {(ib1, ib2): (weight E, trace value)} (static.py:60-117)."""
+        NB = data_K.num_wann
+        formula = self.Formula(data_K, **self.kwargs_formula)
+        weights = data_K.get_bands_in_range_groups(self.EFmin, self.EFmax, degen_thresh=self.degen_thresh,
+                                                   degen_Kramers=self.degen_Kramers, sea=(self.fder == 0))
+        out = []
+        for ik, bnd in enumerate(weights):
+            vals = {}
+            if formula.additive:
+                for n in bnd:
+                    inn = np.arange(n[0], n[1])
+                    outb = np.concatenate((np.arange(0, n[0]), np.arange(n[1], NB)))
+                    vals[n] = formula.trace(ik, inn, outb)
+            else:
+                cuts = set(x for n in bnd for x in n)
+                cut_vals = {c: formula.trace(ik, np.arange(0, c), np.arange(c, NB)) for c in cuts}
+                for n in bnd:
+                    vals[n] = cut_vals[n[1]] - cut_vals[n[0]]
+            out.append({n: (E, vals[n]) for n, E in bnd.items()})
+        return out, formula.ndim
+
+    def __call__(self, data_K):
+        per_k, ndim = self.group_values(data_K)
+        restot = np.zeros((self.nEF_extra,) + (3,) * ndim)
+        # Fermi scan (static.py:126-136)
+        for groups in per_k:
+            for n, (E, v) in sorted(groups.items()):
+                if E < self.EFmin:
+                    restot += v[None]
+                elif E <= self.EFmax:
+                    iEf = ceil((E - self.EFmin) / self.dEF)
+                    restot[iEf:] += v
+        restot = fder_stencil(restot, self.fder, self.dEF)
+        restot /= data_K.cell_volume
+        restot /= data_K.nk
+        restot *= self.constant_factor if self.use_factor else np.sign(self.constant_factor)
+        return OracleResult(self.Efermi, restot)
+
+
+def fder_stencil(restot, fder, dEF):
+    """Finite differences over the extended Fermi grid (static.py:137-147)."""
+    if fder == 0:
+        return restot
+    if fder == 1:
+        return (restot[2:] - restot[:-2]) / (2 * dEF)
+    if fder == 2:
+        return (restot[2:] + restot[:-2] - 2 * restot[1:-1]) / (dEF ** 2)
+    if fder == 3:
+        return (restot[4:] - restot[:-4] - 2 * (restot[3:-1] - restot[1:-3])) / (2 * dEF ** 3)
+    raise NotImplementedError(fder)
+
+
+class AHC(StaticCalculator):
+    # static.py:353-364
+    Formula = Omega
+    fder = 0
+    constant_factor_default = FACTOR_AHC
+
+
+class BerryDipole_FermiSea(StaticCalculator):
+    # static.py:472-487
+    Formula = DerOmega
+    fder = 0
+
+    def __call__(self, data_K):
+        r = super().__call__(data_K)
+        r.data = r.data.swapaxes(1, 2)
+        return r
+
+
+class NLAHC_FermiSea(BerryDipole_FermiSea):
+    # static.py:490-498
+    constant_factor_default = FACTOR_NLAHC
+
+
+class BerryDipole_FermiSurf(StaticCalculator):
+    # static.py:449-458
+    Formula = staticmethod(VelOmega)
+    fder = 1
+
+
+class NLAHC_FermiSurf(BerryDipole_FermiSurf):
+    # static.py:461-469
+    constant_factor_default = FACTOR_NLAHC
+
+
+class CumDOS(StaticCalculator):
+    # static.py:192-214
+    Formula = Identity
+    fder = 0
+
+    def __call__(self, data_K):
+        return super().__call__(data_K) * data_K.cell_volume
+
+
+class DOS(CumDOS):
+    fder = 1
+
+
+class Morb(StaticCalculator):
+    # static.py:231-247
+    Formula = Morb_Hpm
+    fder = 0
+    constant_factor_default = -EV_AU / BOHR ** 2
+
+    def __init__(self, Efermi, **kw):
+        super().__init__(Efermi, **kw)
+        kw = dict(kw)
+        kw.setdefault('constant_factor', self.constant_factor)
+        self.AHC = AHC(Efermi, **kw)
+
+    def __call__(self, data_K):
+        Hplus = super().__call__(data_K)
+        Om = self.AHC(data_K).mul_array(self.Efermi)
+        return OracleResult(self.Efermi, (Hplus.data - 2 * Om.data) * data_K.cell_volume)
+
+
+class GME_orb_FermiSea(StaticCalculator):
+    # static.py:284-300
+    Formula = DerMorb
+    fder = 0
+    constant_factor_default = FACTOR_GME_ORB
+
+    def __init__(self, Efermi, **kw):
+        super().__init__(Efermi, **kw)
+        kw = dict(kw)
+        kw.setdefault('constant_factor', self.constant_factor)
+        self.BerryDipole = BerryDipole_FermiSea(Efermi, **kw)
+
+    def __call__(self, data_K):
+        H = super().__call__(data_K)
+        H.data = H.data.swapaxes(1, 2)
+        Om = self.BerryDipole(data_K).mul_array(self.Efermi)
+        return H - 2 * Om
+
+
+class GME_orb_FermiSurf(StaticCalculator):
+    # static.py:266-281
+    Formula = staticmethod(VelHplus)
+    fder = 1
+    constant_factor_default = FACTOR_GME_ORB
+
+    def __init__(self, Efermi, **kw):
+        super().__init__(Efermi, **kw)
+        kw = dict(kw)
+        kw.setdefault('constant_factor', self.constant_factor)
+        self.BerryDipole = BerryDipole_FermiSurf(Efermi, **kw)
+
+    def __call__(self, data_K):
+        H = super().__call__(data_K)
+        Om = self.BerryDipole(data_K).mul_array(self.Efermi)
+        return H - 2 * Om
+
+
+# ---------------------------------------------------------------------------
+# grid and driver
+# ---------------------------------------------------------------------------
+
+def grid_blocks(NK=None, NKFFT=None, NKdiv=None, periodic=(True, True, True)):
+    """NKdiv x NKFFT decomposition and K-block shifts (grid/grid.py:149-189,215-266).
+
+    Returns (NKdiv, NKFFT, list of (dK = K/NKFFT, factor)).
+    """
+    if NKdiv is None:
+        NKdiv = np.array(np.round(np.array(NK) / np.array(NKFFT)), dtype=int)
+        NKdiv[NKdiv <= 0] = 1
+    NKdiv = np.array(NKdiv, dtype=int)
+    NKFFT = np.array(NKFFT, dtype=int)
+    notper = np.logical_not(np.array(periodic, dtype=bool))
+    NKdiv[notper] = 1
+    NKFFT[notper] = 1
+    factor = 1. / np.prod(NKdiv)
+    blocks = []
+    for x in range(NKdiv[0]):
+        for y in range(NKdiv[1]):
+            for z in range(NKdiv[2]):
+                K = np.array([x, y, z]) / NKdiv
+                blocks.append((K / NKFFT, factor))   # Kpoint.py:75-77
+    return NKdiv, NKFFT, blocks
+
+
+def run(system, calculators, NK=None, NKFFT=None, NKdiv=None, blocks=None):
+    """Serial run() without symmetry or refinement (run_grid.py:32-115,258-265).
+
+    k-sum: sum over K-blocks of result * factor.
+    """
+    if blocks is None:
+        _, NKFFT, blocks = grid_blocks(NK, NKFFT, NKdiv, system.periodic)
+    total = {k: None for k in calculators}
+    for dK, factor in blocks:
+        data = OracleDataK(system, dK=dK, NKFFT=NKFFT)
+        for key, calc in calculators.items():
+            res = calc(data) * factor
+            total[key] = res if total[key] is None else total[key] + res
+    return total
+
+
+def load_golden_system(path):
+    """Read a system written by tests/golden/make_golden.py (already `[R, m, n, ...]`)."""
+    with np.load(path, allow_pickle=False) as f:
+        mats = {k: f[k] for k in f.files if k in ('Ham', 'AA', 'BB', 'CC', 'OO', 'SS')}
+        periodic = f['periodic'] if 'periodic' in f.files else (True, True, True)
+        fito = bool(f['force_internal_terms_only']) if 'force_internal_terms_only' in f.files else False
+        return OracleSystem(f['real_lattice'], f['iRvec'], f['wannier_centers_cart'], mats,
+                            periodic=periodic, force_internal_terms_only=fito)

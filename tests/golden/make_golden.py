@@ -1,0 +1,86 @@
+"""Extract the golden fixtures used by the parity tests.
+
+Run in the development container, where the wannier-berri source tree is at
+/root/reference.  It reads the reference's own committed test data with
+`np.load(allow_pickle=False)` (numeric arrays only) and writes compact `.npz`
+files here: model inputs in the `[R, m, n, ...]` layout and the expected
+outputs.  Nothing from the reference is imported or executed.
+
+    python tests/golden/make_golden.py
+"""
+import os
+import numpy as np
+
+REF = "/root/reference/tests"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _arr(path, name):
+    with np.load(os.path.join(path, name + ".npz"), allow_pickle=False) as f:
+        return f["arr_0"]
+
+
+def system(dirname, out, keys, legacy, extra=None):
+    path = os.path.join(REF, dirname)
+    d = {"real_lattice": _arr(path, "real_lattice"), "iRvec": _arr(path, "iRvec").astype(np.int32)}
+    if os.path.exists(os.path.join(path, "wannier_centers_cart.npz")):
+        d["wannier_centers_cart"] = _arr(path, "wannier_centers_cart")
+    else:
+        d["wannier_centers_cart"] = _arr(path, "wannier_centers_red").dot(d["real_lattice"])
+    if os.path.exists(os.path.join(path, "periodic.npz")):
+        d["periodic"] = _arr(path, "periodic")
+    for k in keys:
+        fn = "_XX_R_" + k if os.path.exists(os.path.join(path, "_XX_R_" + k + ".npz")) else k
+        a = _arr(path, fn)
+        if legacy:
+            a = np.transpose(a, (2, 0, 1) + tuple(range(3, a.ndim)))
+        d[k] = a
+    if extra:
+        d.update(extra)
+    np.savez_compressed(os.path.join(HERE, out), **d)
+
+
+def results(pairs, out):
+    d = {}
+    for key, rel in pairs.items():
+        with np.load(os.path.join(REF, rel), allow_pickle=False) as f:
+            d[key] = f["data"]
+            d[key + "__Efermi"] = f["Energies_0"]
+    np.savez_compressed(os.path.join(HERE, out), **d)
+
+
+def main():
+    I = "reference/integrate_files/"
+    system("reference/systems/Haldane", "sys_haldane.npz", ["Ham"], legacy=True,
+           extra={"force_internal_terms_only": True})
+    system("reference/systems/Chiral_left", "sys_chiral_left.npz", ["Ham"], legacy=True,
+           extra={"force_internal_terms_only": True})
+    system("data/random", "sys_random.npz", ["Ham", "AA", "BB", "CC"], legacy=True)
+    system("reference/systems/Fe_W90", "sys_fe_w90.npz", ["Ham", "AA", "BB", "CC"], legacy=True)
+    system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz", ["Ham", "AA", "BB", "CC"], legacy=False)
+    results({"haldane_ahc": I + "Haldane-ahc_iter-0000.npz",
+             "chiral_ahc": I + "Chiral-ahc_iter-0000.npz",
+             "chiral_berry_dipole": I + "Chiral-berry_dipole_iter-0000.npz",
+             "chiral_berry_dipole_fsurf": I + "Chiral-berry_dipole_fsurf_iter-0000.npz",
+             "random_ahc": I + "random-ahc_iter-0000.npz",
+             "random_Morb": I + "random-Morb_iter-0000.npz",
+             "fe_ident_fder0": "reference/calculators/calculator-Fe-ident-fder=0.npz",
+             "fe_ident_fder1": "reference/calculators/calculator-Fe-ident-fder=1.npz",
+             "fe_ident_fder2": "reference/calculators/calculator-Fe-ident-fder=2.npz",
+             "fe_ident_fder3": "reference/calculators/calculator-Fe-ident-fder=3.npz",
+             }, "results_run.npz")
+    # per-formula gauge-invariant traces (tests/reference/formula/*.npz)
+    forms = {}
+    for name in ["Dcov", "DerDcov", "DEinv_ln", "InvMass", "Omega", "DerOmega", "Morb_H", "morb",
+                 "DerMorb_upper", "Dermorb_lower", "VelOmega", "Hamiltonian", "Velocity"]:
+        with np.load(os.path.join(REF, "reference/formula", name + ".npz"), allow_pickle=False) as f:
+            for k in f.files:
+                forms[name + "/" + k] = f[k]
+    np.savez_compressed(os.path.join(HERE, "results_formula.npz"), **forms)
+    with np.load(os.path.join(REF, "reference/evaluate_k.npz"), allow_pickle=False) as f:
+        np.savez_compressed(os.path.join(HERE, "results_evaluate_k.npz"),
+                            **{k: f[k] for k in ["kpoint", "energy", "berry_curvature", "ahc", "Morb"]})
+
+
+if __name__ == "__main__":
+    main()

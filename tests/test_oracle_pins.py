@@ -1,0 +1,186 @@
+"""Pins of the FP64 oracle (oracle/wb_oracle.py) against the reference's own fixtures.
+
+Every oracle function on the path is checked by at least one of:
+
+* P1 `Haldane-ahc` (AHC end to end, internal terms, config-1 Hamiltonian);
+* P2 `random-ahc`, `random-Morb` (external terms A, rotA, B, C; Morb non-additive cuts);
+* P4 `Chiral-{ahc,berry_dipole,berry_dipole_fsurf}` (DerOmega sea, VelOmega surface, fder=1);
+* P5 per-formula gauge-invariant traces on Fe (Dcov, DerDcov, DEinv, InvMass, Omega,
+  DerOmega, Morb_H, morb, DerMorb, Dermorb, VelOmega, Hamiltonian, Velocity);
+* P6 `calculator-Fe-ident-fder=0..3` (band grouping, Fermi scan, all fder stencils);
+* P7 `evaluate_k` (eigenvalues, per-band Berry curvature, AHC, Morb at one k);
+* closed forms and invariants: brute-force DFT == FFT (r2k_grid_slow), U^+HU = diag(E),
+  sum_n Omega_n = 0 over all bands (internal terms), Tr dOmega = 0 (internal).
+
+The tolerance of each pin is the reference test's own (tests/test_run.py:56,
+tests/test_formula.py:207-213, tests/test_calc.py:50-58).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import wb_oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def gsys(name):
+    return O.load_golden_system(os.path.join(GOLDEN, f"sys_{name}.npz"))
+
+
+@pytest.fixture(scope="module")
+def results_run():
+    with np.load(os.path.join(GOLDEN, "results_run.npz"), allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_haldane_ahc(results_run):
+    s = gsys("haldane")
+    Ef = np.linspace(-3, 3, 11)
+    r = O.run(s, {"ahc": O.AHC(Ef)}, NK=[10, 10, 1], NKFFT=[5, 5, 1])
+    assert rel_err(r["ahc"].data, results_run["haldane_ahc"]) < 1e-8
+
+
+def test_chiral_ahc_bcd(results_run):
+    s = gsys("chiral_left")
+    Ef = np.linspace(-5, 8, 27)
+    calcs = {"ahc": O.AHC(Ef),
+             "bd": O.BerryDipole_FermiSea(Ef, use_factor=False),
+             "bds": O.BerryDipole_FermiSurf(Ef, use_factor=False)}
+    r = O.run(s, calcs, NK=[10, 10, 4], NKFFT=[5, 5, 2])
+    assert rel_err(r["ahc"].data, results_run["chiral_ahc"]) < 1e-8
+    assert rel_err(r["bd"].data, results_run["chiral_berry_dipole"]) < 1e-8
+    assert rel_err(r["bds"].data, results_run["chiral_berry_dipole_fsurf"]) < 1e-8
+
+
+def test_random_ahc_morb(results_run):
+    s = gsys("random")
+    Ef = np.linspace(-2, 2, 5)
+    r = O.run(s, {"ahc": O.AHC(Ef), "Morb": O.Morb(Ef)}, NK=[6, 6, 6], NKFFT=[3, 3, 3])
+    assert rel_err(r["ahc"].data, results_run["random_ahc"]) < 1e-8
+    assert rel_err(r["Morb"].data, results_run["random_Morb"]) < 1e-8
+
+
+@pytest.mark.parametrize("fder", [0, 1, 2, 3])
+def test_fe_scan_stencils(results_run, fder):
+    s = gsys("fe_w90")
+    d = O.OracleDataK(s, dK=[0.1, 0.2, 0.3], NKFFT=[3, 3, 3])
+    c = O.StaticCalculator(np.linspace(17, 18, 11), Formula=O.Identity, fder=fder)
+    ref = results_run[f"fe_ident_fder{fder}"]
+    assert np.abs(c(d).data - ref).max() < max(1e-8 * np.abs(ref).max(), 1e-11)
+
+
+def test_evaluate_k():
+    s = gsys("fe_w90")
+    with np.load(os.path.join(GOLDEN, "results_evaluate_k.npz"), allow_pickle=False) as f:
+        ev = {k: f[k] for k in f.files}
+    d = O.OracleDataK(s, dK=ev["kpoint"], NKFFT=[1, 1, 1])
+    assert np.abs(d.E_K[0][4:6] - ev["energy"]).max() < 1e-8 * 17
+    om = O.Omega(d)
+    nb = s.num_wann
+    berry = np.array([om.trace(0, [n], [m for m in range(nb) if m != n]) for n in (4, 5)])
+    assert rel_err(berry, ev["berry_curvature"]) < 1e-8
+    Ef = np.linspace(17, 18, 11)
+    assert rel_err(O.AHC(Ef)(d).data, ev["ahc"]) < 1e-8
+    assert rel_err(O.Morb(Ef)(d).data, ev["Morb"]) < 1e-8
+
+
+FORMULAS = {
+    "Dcov": O.Dcov, "DerDcov": O.DerDcov, "DEinv_ln": O.DEinv_ln, "InvMass": O.InvMass,
+    "Omega": O.Omega, "DerOmega": O.DerOmega, "Morb_H": O.Morb_H, "morb": O.morb,
+    "DerMorb_upper": O.DerMorb, "Dermorb_lower": O.Dermorb, "VelOmega": O.VelOmega,
+    "Hamiltonian": lambda d: O.Matrix_ln(d.Xbar("Ham", 0)),
+    # the reference's Velocity copies only the attributes of the covariant V,
+    # so its ln block is the full off-diagonal block (formula/covariant.py:321-328)
+    "Velocity": lambda d: O.Matrix_ln(d.Xbar("Ham", 1)),
+}
+LN_ONLY = {"Dcov", "DerDcov", "DEinv_ln"}
+NN_ONLY = {"Omega", "DerOmega", "Morb_H", "morb", "DerMorb_upper", "Dermorb_lower", "VelOmega"}
+
+
+@pytest.fixture(scope="module")
+def datak_fe_sym():
+    # tests/test_formula.py:19-41: dK = 0, NKFFT = [1, 2, 3]
+    return O.OracleDataK(gsys("fe_sym_osd"), dK=[0, 0, 0], NKFFT=[1, 2, 3])
+
+
+@pytest.mark.parametrize("name", sorted(FORMULAS))
+def test_formula_traces(datak_fe_sym, name):
+    d = datak_fe_sym
+    nb = d.num_wann
+    groups = d.get_bands_in_range_groups(-10, 30, degen_thresh=1)
+    with np.load(os.path.join(GOLDEN, "results_formula.npz"), allow_pickle=False) as f:
+        ref = {k.split("/", 1)[1]: f[k] for k in f.files if k.startswith(name + "/")}
+    val = {}
+    for ik in range(d.nk):
+        for key in ["Xnn", "Xll", "XlnXnl", "XnlXln", "XnnXnn", "XllXll"]:
+            val[f"{key}_ik={ik}"] = []
+        for n in groups[ik]:
+            inn = np.arange(n[0], n[1])
+            out = np.concatenate((np.arange(0, n[0]), np.arange(n[1], nb)))
+            form = FORMULAS[name](d)
+            if name not in NN_ONLY:
+                Xnl, Xln = form.ln(ik, inn, out), form.ln(ik, out, inn)
+                val[f"XnlXln_ik={ik}"].append(np.einsum("nl...,ln...->...", Xnl, Xln))
+                val[f"XlnXnl_ik={ik}"].append(np.einsum("ln...,nl...->...", Xln, Xnl))
+            if name not in LN_ONLY:
+                # the reference calls ll/nn with out=inn (tests/test_formula.py:198-200)
+                Xll, Xnn = form.ll(ik, inn, inn), form.nn(ik, inn, inn)
+                val[f"Xll_ik={ik}"].append(np.einsum("ll...->...", Xll))
+                val[f"Xnn_ik={ik}"].append(np.einsum("nn...->...", Xnn))
+                val[f"XllXll_ik={ik}"].append(np.einsum("ll...,ll...->...", Xll, Xll))
+                val[f"XnnXnn_ik={ik}"].append(np.einsum("nn...,nn...->...", Xnn, Xnn))
+    der = name.startswith("Der") or name == "VelOmega"
+    rtol, atol = (1e-5, 2e-6) if der else (1e-6, 1e-10)
+    assert len(ref) == 36
+    for k, r in ref.items():
+        o = np.array(val[k])
+        assert o.shape == r.shape, k
+        if r.size == 0:
+            continue
+        mx = np.abs(r).max()
+        if mx < atol:
+            assert np.abs(o).max() < atol, k
+        else:
+            assert np.abs(r - o).max() / mx < rtol, k
+
+
+# -- closed forms and invariants ------------------------------------------
+
+def test_fft_equals_bruteforce_dft():
+    s = gsys("random")
+    d = O.OracleDataK(s, dK=[0.13, -0.27, 0.41], NKFFT=[3, 4, 5])
+    X = d.get_R_mat("AA")
+    fast = O.r2k_grid(X, s.iRvec, d.NKFFT, hermitian=True)
+    # direct sum over R at k = j/N (the dK phase is already inside X)
+    slow = O.r2k_grid_slow(X, s.iRvec, d.NKFFT, hermitian=True)
+    assert np.abs(fast - slow).max() < 1e-12 * np.abs(slow).max()
+
+
+def test_eigh_invariants():
+    s = gsys("random")
+    d = O.OracleDataK(s, dK=[0.1, 0.2, 0.3], NKFFT=[2, 2, 2])
+    U, E, H = d.UU_K, d.E_K, d.HH_K
+    UhHU = np.einsum("kba,kbc,kcd->kad", U.conj(), H, U)
+    assert np.abs(UhHU - np.eye(s.num_wann)[None] * E[:, :, None]).max() < 1e-12
+    assert np.all(np.diff(E, axis=1) >= 0)
+
+
+def test_omega_sum_rule_and_bcd_trace():
+    # internal terms: sum_n Omega_n = 0 over all bands; Tr_c d_c Omega_c = 0 (tests/test_calc.py:110-120)
+    s = gsys("chiral_left")
+    d = O.OracleDataK(s, dK=[0.05, 0.1, 0.2], NKFFT=[3, 3, 2])
+    om = O.Omega(d)
+    dom = O.DerOmega(d)
+    nb = s.num_wann
+    for ik in range(d.nk):
+        tot = sum(om.trace(ik, [n], [m for m in range(nb) if m != n]) for n in range(nb))
+        assert np.abs(tot).max() < 1e-9
+        for n in range(nb):
+            t = dom.trace(ik, [n], [m for m in range(nb) if m != n])
+            assert abs(np.trace(t)) < 1e-9 * max(1, np.abs(t).max())
