@@ -1,0 +1,173 @@
+/*
+ * wbmi.h -- C ABI of the MI355X Berry-quantity library (libwbmi.so, gfx950).
+ *
+ * The library evaluates one K-block of the wannier-berri Berry path:
+ *
+ *     R-space matrices --(R->k)--> k-space matrices --(eigh)--> E, U
+ *     --(U^+ X U)--> band-basis matrices --(covariant formula + Fermi scan)-->
+ *     FP64 Fermi-level histogram (accumulated: this is the k-sum)
+ *
+ * Conventions
+ *   - Every array argument named *_dev is a caller-owned DEVICE pointer
+ *     (e.g. a torch tensor's data_ptr()), contiguous, row-major.  Arrays named
+ *     *_host are host pointers read during the call.
+ *   - Complex numbers are interleaved (re, im).  `prec` selects the element
+ *     type of complex k-space arrays: WBMI_F64 = complex128, WBMI_F32 = complex64.
+ *     Energies, histograms and all inputs of the Fermi scan are always FP64.
+ *   - Work is enqueued on the context's stream; calls return after enqueueing
+ *     (wbmi_sync waits).  Return 0 on success, a WBMI_ERR_* code otherwise;
+ *     wbmi_last_error() gives the message.  Argument errors mirror the
+ *     reference's ValueError / AssertionError cases and are detected on the
+ *     host before any launch.
+ *   - One context per host thread and stream; contexts share nothing.
+ */
+#ifndef WBMI_H
+#define WBMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WBMI_OK 0
+#define WBMI_ERR_ARG 1
+#define WBMI_ERR_HIP 2
+#define WBMI_ERR_UNSUPPORTED 3
+
+#define WBMI_F64 0
+#define WBMI_F32 1
+
+/* formula ids for wbmi_static_scan */
+#define WBMI_FORMULA_IDENTITY 0   /* formula/covariant.py:10-22            */
+#define WBMI_FORMULA_OMEGA 1      /* formula/covariant.py:161-203 (AHC)    */
+#define WBMI_FORMULA_DEROMEGA 2   /* formula/covariant.py:212-259 (BCD)    */
+#define WBMI_FORMULA_VELOMEGA 3   /* formula/covariant.py:798-801          */
+#define WBMI_FORMULA_MORB_HPM 4   /* formula/covariant.py:424-449          */
+#define WBMI_FORMULA_DERMORB 5    /* formula/covariant.py:522-552          */
+#define WBMI_FORMULA_VELHPLUS 6   /* formula/covariant.py:804-808          */
+
+/* channel roles: index into wbmi_scan_params.ch (value = channel slot in the
+ * per-k stack, -1 = absent).  Two-index roles are row-major over (x, y). */
+#define WBMI_ROLE_V 0     /* Xbar('Ham',1)[..,a]            3 */
+#define WBMI_ROLE_W 3     /* Xbar('Ham',2)[..,a,b]          9 */
+#define WBMI_ROLE_A 12    /* Xbar('AA',0)[..,a]             3 */
+#define WBMI_ROLE_AD 15   /* Xbar('AA',1)[..,a,d]           9 */
+#define WBMI_ROLE_O 24    /* Xbar('rotAA',0)[..,c]          3 */
+#define WBMI_ROLE_OD 27   /* Xbar('rotAA',1)[..,c,d]        9 */
+#define WBMI_ROLE_B 36    /* Xbar('BB',0)[..,a]             3 */
+#define WBMI_ROLE_BD 39   /* Xbar('BB',1)[..,a,d]           9 */
+#define WBMI_ROLE_C 48    /* Xbar('CC',0)[..,a]             3 */
+#define WBMI_ROLE_CD 51   /* Xbar('CC',1)[..,a,d]           9 */
+#define WBMI_NROLES 60
+
+typedef struct wbmi_ctx wbmi_ctx;
+typedef struct wbmi_r2k_plan wbmi_r2k_plan;
+
+/* Parameters of one Fermi-level scan (calculators/static.py:26-58). */
+typedef struct wbmi_scan_params {
+    int formula;          /* WBMI_FORMULA_*                                   */
+    int sign;             /* Morb_Hpm / DerMorb sign (+1, -1, 0)              */
+    int internal_terms;   /* formula/formula.py:11-25                         */
+    int external_terms;   /* already 0 when force_internal_terms_only         */
+    int fder;             /* 0 = Fermi sea (adds the below-range group)       */
+    int nEF_extra;        /* number of extended Fermi levels                  */
+    double EFmin, EFmax, dEF;
+    double degen_thresh;  /* calculator.py:20                                 */
+    int degen_Kramers;
+    int ch[WBMI_NROLES];
+} wbmi_scan_params;
+
+/* ---- context ------------------------------------------------------------ */
+int wbmi_ctx_create(int device, void* stream, wbmi_ctx** out);
+int wbmi_ctx_destroy(wbmi_ctx* ctx);
+int wbmi_ctx_set_stream(wbmi_ctx* ctx, void* stream);
+int wbmi_sync(wbmi_ctx* ctx);
+const char* wbmi_last_error(const wbmi_ctx* ctx);
+int wbmi_version(void);
+
+/* ---- R-space channel stack ---------------------------------------------
+ * Builds out[R, ch, m, n] = sum_t sign_t * base[R, m, n, a_t] * prod_(d in t) i r_d
+ * with r = R_cart + t_n - t_m (the Wannier-centre-shifted R vector).
+ * Replaces Rvectors.derivative (fourier/rvectors.py:487-494), the rotAA curl
+ * (data_K/data_K_R.py:48-51) and the per-quantity copies of get_R_mat.
+ *   cRshift_dev : float64 [nR, nw, nw, 3]
+ *   base_dev    : complex128 [nR, nw, nw, nbase]
+ *   spec_host   : int32 [nch, 2, 4] = (sign, a, d1, d2) per term; sign 0 =
+ *                 unused term, d = -1 = no derivative.
+ *   out_dev     : complex128 [nR, nch, nw, nw]                                  */
+int wbmi_build_channels(wbmi_ctx* ctx, int nR, int nw, const double* cRshift_dev,
+                        const void* base_dev, int nbase, const int32_t* spec_host,
+                        int nch, void* out_dev);
+
+/* ---- R -> k on an FFT sub-grid -------------------------------------------
+ * Replaces FFT_R_to_k grid mode (fourier/fft.py:58-192) together with
+ * Rvectors.apply_expdK (fourier/rvectors.py:468,477-485):
+ *   out[k, e] = sum_R X[R, e] exp(2 pi i (j/NKFFT + dK).R),  k = (j0*N1+j1)*N2+j2
+ * as three separable DFT passes over the R bounding box.  `nelem` is the
+ * number of complex elements per R (nch*nw*nw).                             */
+int wbmi_r2k_plan_create(wbmi_ctx* ctx, int nR, const int32_t* iRvec_host,
+                         const int32_t NKFFT[3], int64_t nelem, wbmi_r2k_plan** out);
+int wbmi_r2k_plan_destroy(wbmi_r2k_plan* plan);
+/* bytes of box storage (complex128) and of pass workspace (complex of `prec`) */
+int wbmi_r2k_plan_sizes(const wbmi_r2k_plan* plan, size_t* box_bytes, size_t* work_bytes);
+/* scatter X[R, e] (complex128) into the box; needed once per system when no
+ * axis aliases, and is repeated by wbmi_r2k_exec otherwise                  */
+int wbmi_r2k_box(wbmi_ctx* ctx, const wbmi_r2k_plan* plan, const void* XR_dev, void* box_dev);
+/* out_dev: [nk, nelem] complex of `prec`; XR_dev is read only when an axis
+ * aliases (R extent > NKFFT), in which case box_dev is rebuilt per call.    */
+int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* plan, const void* XR_dev,
+                  void* box_dev, const double dK[3], void* work_dev, void* out_dev, int prec);
+
+/* ---- R -> k on an arbitrary k-list ---------------------------------------
+ * Phase-matrix transform (fourier/fft.py:125-130,170-171), no expdK phase.
+ *   klist_dev : float64 [nk, 3];  iRvec_dev : int32 [nR, 3]
+ *   XR_dev    : complex128 [nR, nelem];  out_dev : [nk, nelem] (prec)       */
+int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nelem,
+                   const void* XR_dev, int nk, const double* klist_dev, void* out_dev, int prec);
+
+/* ---- batched Hermitian eigensolver ----------------------------------------
+ * Replaces np.linalg.eigh in Data_K.E_K (data_K/data_K.py:211-218).
+ * H is read as 0.5 (H + H^+) (data_K_R.py:26 hermitises HH_K).
+ *   H_dev : [nk, Hstride] complex (prec), matrix k at H_dev + k*Hstride
+ *   E_dev : float64 [nk, nw] ascending;  U_dev : [nk, nw, nw] complex (prec),
+ *   column j = eigenvector of E[j].  nw <= 64.
+ *   refine (prec = WBMI_F32 only): 1 = FP32 Jacobi then one FP64 Ogita-Aishima
+ *   step against the FP64 Hamiltonian H64_dev (complex128, same stride);
+ *   E and the refined U are then FP64 (U_dev is complex128).               */
+int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride,
+              double* E_dev, void* U_dev, int prec, int refine, const void* H64_dev);
+
+/* ---- gauge rotation U^+ X U -----------------------------------------------
+ * Replaces Data_K._rotate (data_K/data_K.py:130-132) for the channels of a
+ * per-k stack X[k, nch, nw, nw] (in place).  herm_host[c] = 1 forces the
+ * channel Hermitian first (0.5 (X + X^+), fourier/fft.py:183-186,
+ * data_K_R.py:87).  Channels with skip_host[c] = 1 are left untouched.     */
+int wbmi_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_dev,
+                void* X_dev, const int32_t* herm_host, const int32_t* skip_host, int prec);
+
+/* ---- covariant formula traces + Fermi scan --------------------------------
+ * Replaces the per-(k, band-group) loop and Fermi scan of
+ * StaticCalculator.__call__ (calculators/static.py:60-136), band grouping
+ * Data_K.get_bands_in_range_groups (data_K/data_K.py:172-194,
+ * grid/tetrahedron.py:132-162) and the formula traces.
+ *   E_dev    : float64 [nk, nw]
+ *   Xbar_dev : [nk, nch, nw, nw] rotated channels (prec)
+ *   hist_dev : float64 [nEF_extra, 3^ndim], accumulated with weight * value
+ *              at bin ceil((E-EFmin)/dEF) (bin 0 for the below-range group).
+ * `weight` carries the k-sum factor (K-block factor / nk).                   */
+int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                     const void* Xbar_dev, int prec, const wbmi_scan_params* params,
+                     double weight, double* hist_dev);
+
+/* ---- histogram -> EnergyResult data ---------------------------------------
+ * Prefix sum over the Fermi levels, fder finite-difference stencil and scaling
+ * (calculators/static.py:126-155).  out_dev: float64 [nEF_extra-2*extra, ncomp]. */
+int wbmi_finalize(wbmi_ctx* ctx, const double* hist_dev, int nEF_extra, int ncomp,
+                  int fder, double dEF, double scale, double* out_dev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WBMI_H */

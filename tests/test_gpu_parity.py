@@ -1,0 +1,253 @@
+"""GPU parity: libwbmi (through the C ABI) against the FP64 oracle and the reference fixtures.
+
+Tolerances: the FP64 path must match the reference's own bar,
+max|delta| < 1e-8 * max|ref| per calculator (tests/test_run.py:56); stage
+outputs (R->k, eigh, rotation) are compared at 1e-11..1e-12 relative.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import wb_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def wb():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import wannierberri_amd
+    return wannierberri_amd
+
+
+@pytest.fixture(scope="module")
+def ctx(wb):
+    from wannierberri_amd import _lib
+    return _lib.get_context(0)
+
+
+@pytest.fixture(scope="module")
+def results_run():
+    with np.load(os.path.join(GOLDEN, "results_run.npz"), allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def gsys(wb, name):
+    return wb.System_R.from_arrays_npz(os.path.join(GOLDEN, f"sys_{name}.npz"))
+
+
+def osys(name):
+    return O.load_golden_system(os.path.join(GOLDEN, f"sys_{name}.npz"))
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def run_gpu(wb, system, calcs, NK, NKFFT, precision="fp64"):
+    grid = wb.Grid(system=system, NK=NK, NKFFT=NKFFT)
+    return wb.run(system, grid, calcs, parameters_K={"precision": precision}, save_files=False, verbose=False)
+
+
+# -- stage parity --------------------------------------------------------------
+
+@pytest.mark.parametrize("NKFFT", [(3, 3, 3), (9, 9, 9), (2, 4, 3)])
+def test_r2k_channels(wb, ctx, NKFFT):
+    """R->k of Ham, Ham', AA, rotAA, AA' (phase, derivatives, curl) vs the oracle's FFT."""
+    import torch
+    say(f"test_r2k_channels NKFFT={NKFFT}")
+    from wannierberri_amd.data_k import device_system, CHANNEL_NCOMP
+    s = gsys(wb, "random")
+    o = osys("random")
+    keys = [("Ham", 0), ("Ham", 1), ("Ham", 2), ("AA", 0), ("AA", 1), ("rotAA", 0), ("rotAA", 1)]
+    dev = device_system(s, ctx)
+    stack = dev.stack(keys)
+    plan, box, aliased = dev.plan(keys, NKFFT)
+    dK = np.array([0.13, -0.21, 0.37])
+    nk = int(np.prod(NKFFT))
+    nw = s.num_wann
+    X = torch.empty((nk, stack.shape[1], nw, nw), dtype=torch.complex128, device="cuda")
+    work = torch.empty(plan.work_bytes, dtype=torch.uint8, device="cuda")
+    plan.execute(stack if aliased else None, box, dK, work, X, 0)
+    X = X.cpu().numpy()
+    d = O.OracleDataK(o, dK=dK, NKFFT=NKFFT)
+    sl = 0
+    for name, der in keys:
+        ref = d.R_to_k(d.get_R_mat(name).copy(), der=der, hermitian=False)
+        n = CHANNEL_NCOMP[(name, der)]
+        got = np.moveaxis(X[:, sl:sl + n], 1, -1).reshape(ref.shape)
+        assert rel_err(got, ref) < 1e-12, (name, der)
+        sl += n
+
+
+def test_r2k_klist(wb, ctx):
+    import torch
+    say("test_r2k_klist")
+    from wannierberri_amd.data_k import device_system
+    s = gsys(wb, "random")
+    o = osys("random")
+    dev = device_system(s, ctx)
+    stack = dev.stack([("Ham", 0), ("AA", 0)])
+    kl = np.random.default_rng(0).uniform(-1, 1, (37, 3))
+    X = torch.empty((37, stack.shape[1], s.num_wann, s.num_wann), dtype=torch.complex128, device="cuda")
+    ctx.r2k_klist(dev.iRvec_dev, stack, torch.from_numpy(kl).cuda(), X, 0)
+    X = X.cpu().numpy()
+    refH = O.r2k_klist(o.matrices["Ham"], o.iRvec, kl, hermitian=False)
+    refA = O.r2k_klist(o.matrices["AA"], o.iRvec, kl, hermitian=False)
+    assert rel_err(X[:, 0], refH) < 1e-12
+    assert rel_err(np.moveaxis(X[:, 1:4], 1, -1), refA) < 1e-12
+
+
+@pytest.mark.parametrize("nw", [1, 2, 5, 18, 31, 32, 64])
+def test_eigh(wb, ctx, nw):
+    """Jacobi eigensolver vs np.linalg.eigh: eigenvalues, U^+HU = diag(E), U^+U = 1."""
+    import torch
+    say(f"test_eigh nw={nw}")
+    rng = np.random.default_rng(nw)
+    nk = 50
+    H = rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw))
+    H = H + H.conj().swapaxes(1, 2)
+    H[0] = np.diag(np.repeat(np.arange(nw // 2 + 1), 2)[:nw]).astype(complex)  # exact degeneracies
+    Ht = torch.from_numpy(H).cuda()
+    E = torch.empty((nk, nw), dtype=torch.float64, device="cuda")
+    U = torch.empty((nk, nw, nw), dtype=torch.complex128, device="cuda")
+    ctx.eigh(Ht, nw * nw, E, U, 0, nw)
+    E, U = E.cpu().numpy(), U.cpu().numpy()
+    Eref = np.linalg.eigvalsh(H)
+    scale = np.abs(Eref).max()
+    assert np.abs(E - Eref).max() < 1e-12 * scale
+    UhHU = np.einsum("kba,kbc,kcd->kad", U.conj(), H, U)
+    assert np.abs(UhHU - np.eye(nw)[None] * E[:, :, None]).max() < 1e-12 * scale
+    assert np.abs(np.einsum("kba,kbc->kac", U.conj(), U) - np.eye(nw)[None]).max() < 1e-12
+
+
+def test_rotate(wb, ctx):
+    import torch
+    say("test_rotate")
+    rng = np.random.default_rng(5)
+    nk, nch, nw = 7, 5, 13
+    X = rng.standard_normal((nk, nch, nw, nw)) + 1j * rng.standard_normal((nk, nch, nw, nw))
+    U = np.linalg.qr(rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw)))[0]
+    herm = np.array([0, 1, 0, 1, 0], dtype=np.int32)
+    skip = np.array([0, 0, 1, 0, 0], dtype=np.int32)
+    Xt = torch.from_numpy(X.copy()).cuda()
+    ctx.rotate(torch.from_numpy(U).cuda(), Xt, herm, skip, 0)
+    got = Xt.cpu().numpy()
+    for c in range(nch):
+        x = X[:, c]
+        if skip[c]:
+            ref = x
+        else:
+            if herm[c]:
+                x = 0.5 * (x + x.conj().swapaxes(1, 2))
+            ref = np.einsum("kba,kbc,kcd->kad", U.conj(), x, U)
+        assert rel_err(got[:, c], ref) < 1e-12, c
+
+
+# -- end to end against the reference fixtures ----------------------------------
+
+def test_haldane_ahc(wb, results_run):
+    say("test_haldane_ahc")
+    s = gsys(wb, "haldane")
+    Ef = np.linspace(-3, 3, 11)
+    r = run_gpu(wb, s, {"ahc": wb.calculators.static.AHC(Efermi=Ef)}, [10, 10, 1], [5, 5, 1])
+    assert rel_err(r.results["ahc"].data, results_run["haldane_ahc"]) < 1e-8
+
+
+def test_chiral_ahc_bcd(wb, results_run):
+    say("test_chiral_ahc_bcd")
+    st = wb.calculators.static
+    s = gsys(wb, "chiral_left")
+    Ef = np.linspace(-5, 8, 27)
+    calcs = {"ahc": st.AHC(Efermi=Ef), "bd": st.BerryDipole_FermiSea(Efermi=Ef, use_factor=False),
+             "bds": st.BerryDipole_FermiSurf(Efermi=Ef, use_factor=False)}
+    r = run_gpu(wb, s, calcs, [10, 10, 4], [5, 5, 2])
+    assert rel_err(r.results["ahc"].data, results_run["chiral_ahc"]) < 1e-8
+    assert rel_err(r.results["bd"].data, results_run["chiral_berry_dipole"]) < 1e-8
+    assert rel_err(r.results["bds"].data, results_run["chiral_berry_dipole_fsurf"]) < 1e-8
+
+
+def test_random_ahc_morb(wb, results_run):
+    say("test_random_ahc_morb")
+    st = wb.calculators.static
+    s = gsys(wb, "random")
+    Ef = np.linspace(-2, 2, 5)
+    r = run_gpu(wb, s, {"ahc": st.AHC(Efermi=Ef), "Morb": st.Morb(Efermi=Ef)}, [6, 6, 6], [3, 3, 3])
+    assert rel_err(r.results["ahc"].data, results_run["random_ahc"]) < 1e-8
+    assert rel_err(r.results["Morb"].data, results_run["random_Morb"]) < 1e-8
+
+
+@pytest.mark.parametrize("fder", [0, 1, 2, 3])
+def test_fe_scan_stencils(wb, results_run, fder):
+    say(f"test_fe_scan_stencils fder={fder}")
+    from wannierberri_amd import _lib
+    st = wb.calculators.static
+
+    class Ident(st.StaticCalculator):
+        formula = _lib.FORMULA_IDENTITY
+        ndim = 0
+
+    s = gsys(wb, "fe_w90")
+    grid = wb.Grid(system=s, NKFFT=[3, 3, 3], NKdiv=1)
+    data = wb.Data_K(s, dK=[0.1, 0.2, 0.3], grid=grid)
+    res = Ident(Efermi=np.linspace(17, 18, 11), fder=fder)(data).data
+    ref = results_run[f"fe_ident_fder{fder}"]
+    assert np.abs(res - ref).max() < max(1e-8 * np.abs(ref).max(), 1e-11)
+
+
+# -- synthetic model against the oracle ------------------------------------------
+
+@pytest.fixture(scope="module")
+def small_synth(wb):
+    from wannierberri_amd import models
+    s = models.synthetic(num_wann=8, Rmax=2, seed=11, with_BBCC=True)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA", "BB", "CC")})
+    return s, o
+
+
+def test_synthetic_all_calculators(wb, small_synth):
+    """AHC, BCD (sea, surf), Morb, CumDOS on a random nw=8 model vs the oracle."""
+    say("test_synthetic_all_calculators")
+    st = wb.calculators.static
+    s, o = small_synth
+    E = np.linalg.eigvalsh(s.get_R_mat("Ham")[np.where((s.rvec.iRvec == 0).all(1))[0][0]])
+    Ef = np.linspace(E.min() - 1, E.max() + 1, 41)
+    NK, NKFFT = [10, 10, 10], [5, 5, 5]
+    calcs = {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef),
+             "bcds": st.BerryDipole_FermiSurf(Efermi=Ef), "morb": st.Morb(Efermi=Ef),
+             "cumdos": st.CumDOS(Efermi=Ef)}
+    r = run_gpu(wb, s, calcs, NK, NKFFT)
+    ocalcs = {"ahc": O.AHC(Ef), "bcd": O.BerryDipole_FermiSea(Ef), "bcds": O.BerryDipole_FermiSurf(Ef),
+              "morb": O.Morb(Ef), "cumdos": O.CumDOS(Ef)}
+    ro = O.run(o, ocalcs, NK=NK, NKFFT=NKFFT)
+    for k in calcs:
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  {k}: rel err {e:.2e}")
+        assert e < 1e-8, k
+
+
+def test_synthetic_fp32(wb, small_synth):
+    """Complex64 channels with FP64 energies/denominators/k-sum: error vs the FP64 oracle."""
+    say("test_synthetic_fp32")
+    st = wb.calculators.static
+    s, o = small_synth
+    Ef = np.linspace(-6, 6, 41)
+    NK, NKFFT = [10, 10, 10], [5, 5, 5]
+    r = run_gpu(wb, s, {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef)}, NK, NKFFT, "fp32")
+    ro = O.run(o, {"ahc": O.AHC(Ef), "bcd": O.BerryDipole_FermiSea(Ef)}, NK=NK, NKFFT=NKFFT)
+    for k in ("ahc", "bcd"):
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  fp32 {k}: rel err {e:.2e}")
+        assert e < 1e-3, k

@@ -1,0 +1,17 @@
+"""wannierberri_amd: MI355X-native engine for the Berry-quantity path of wannier-berri.
+
+The API surface of the path is the reference's: System_R -> Data_K ->
+calculators / run().  All numerics run in the HIP library libwbmi.so
+(include/wbmi.h); see `wannierberri_amd.build_lib` to build it.
+"""
+from . import calculators, factors
+from .calculators import static
+from .data_k import Data_K
+from .grid import Grid, KpointBZparallel
+from .result import EnergyResult, ResultDict, VoidResult
+from .run_grid import run
+from .system import System_R
+
+__version__ = "0.1.0"
+__all__ = ["System_R", "Grid", "Data_K", "run", "calculators", "EnergyResult", "ResultDict", "VoidResult",
+           "KpointBZparallel", "factors"]

@@ -1,0 +1,228 @@
+"""ctypes binding of libwbmi.so (the C ABI declared in include/wbmi.h).
+
+The library is required: importing the GPU path without it raises
+`WbmiLibraryError`; there is no CPU fallback.
+"""
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwbmi.so")
+
+WBMI_OK = 0
+WBMI_ERR_ARG = 1
+WBMI_ERR_HIP = 2
+WBMI_ERR_UNSUPPORTED = 3
+WBMI_F64 = 0
+WBMI_F32 = 1
+
+FORMULA_IDENTITY = 0
+FORMULA_OMEGA = 1
+FORMULA_DEROMEGA = 2
+FORMULA_VELOMEGA = 3
+FORMULA_MORB_HPM = 4
+FORMULA_DERMORB = 5
+FORMULA_VELHPLUS = 6
+
+ROLE_V, ROLE_W, ROLE_A, ROLE_AD, ROLE_O, ROLE_OD = 0, 3, 12, 15, 24, 27
+ROLE_B, ROLE_BD, ROLE_C, ROLE_CD = 36, 39, 48, 51
+NROLES = 60
+
+# exported symbols (must match include/wbmi.h)
+SYMBOLS = [
+    "wbmi_version", "wbmi_ctx_create", "wbmi_ctx_destroy", "wbmi_ctx_set_stream", "wbmi_sync",
+    "wbmi_last_error", "wbmi_build_channels", "wbmi_r2k_plan_create", "wbmi_r2k_plan_destroy",
+    "wbmi_r2k_plan_sizes", "wbmi_r2k_box", "wbmi_r2k_exec", "wbmi_r2k_klist", "wbmi_eigh",
+    "wbmi_rotate", "wbmi_static_scan", "wbmi_finalize",
+]
+
+
+class WbmiLibraryError(RuntimeError):
+    pass
+
+
+class WbmiError(RuntimeError):
+    pass
+
+
+class ScanParams(ctypes.Structure):
+    _fields_ = [
+        ("formula", ctypes.c_int), ("sign", ctypes.c_int), ("internal_terms", ctypes.c_int),
+        ("external_terms", ctypes.c_int), ("fder", ctypes.c_int), ("nEF_extra", ctypes.c_int),
+        ("EFmin", ctypes.c_double), ("EFmax", ctypes.c_double), ("dEF", ctypes.c_double),
+        ("degen_thresh", ctypes.c_double), ("degen_Kramers", ctypes.c_int),
+        ("ch", ctypes.c_int * NROLES),
+    ]
+
+
+_lib = None
+_lock = threading.Lock()
+
+_vp, _i, _i64, _d = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+_PROTOS = {
+    "wbmi_version": (_i, []),
+    "wbmi_ctx_create": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
+    "wbmi_ctx_destroy": (_i, [_vp]),
+    "wbmi_ctx_set_stream": (_i, [_vp, _vp]),
+    "wbmi_sync": (_i, [_vp]),
+    "wbmi_last_error": (ctypes.c_char_p, [_vp]),
+    "wbmi_build_channels": (_i, [_vp, _i, _i, _vp, _vp, _i, _vp, _i, _vp]),
+    "wbmi_r2k_plan_create": (_i, [_vp, _i, _vp, _vp, _i64, ctypes.POINTER(_vp)]),
+    "wbmi_r2k_plan_destroy": (_i, [_vp]),
+    "wbmi_r2k_plan_sizes": (_i, [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "wbmi_r2k_box": (_i, [_vp, _vp, _vp, _vp]),
+    "wbmi_r2k_exec": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "wbmi_r2k_klist": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _vp, _i]),
+    "wbmi_eigh": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp]),
+    "wbmi_rotate": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i]),
+    "wbmi_static_scan": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _d, _vp]),
+    "wbmi_finalize": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _vp]),
+}
+
+
+def lib():
+    """Load libwbmi.so (once).  Raises WbmiLibraryError if it is not built."""
+    global _lib
+    with _lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise WbmiLibraryError(
+                    f"{LIB_PATH} not found: build it with `python -m wannierberri_amd.build_lib` "
+                    "(the MI355X path has no CPU fallback)")
+            L = ctypes.CDLL(LIB_PATH)
+            for name, (res, args) in _PROTOS.items():
+                f = getattr(L, name)
+                f.restype = res
+                f.argtypes = args
+            _lib = L
+        return _lib
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _host_i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Context:
+    """One library context bound to a device and a HIP stream (torch's current stream by default)."""
+
+    def __init__(self, device=0, stream=None):
+        import torch
+        self.torch = torch
+        self.device = torch.device("cuda", device)
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        self.stream = stream
+        h = ctypes.c_void_p()
+        rc = lib().wbmi_ctx_create(device, ctypes.c_void_p(stream.cuda_stream), ctypes.byref(h))
+        if rc != WBMI_OK:
+            raise WbmiError(f"wbmi_ctx_create failed with code {rc} (device {device}; gfx950 required)")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and _lib is not None:
+                _lib.wbmi_ctx_destroy(self.h)
+        except Exception:
+            pass
+
+    def check(self, rc, what):
+        if rc != WBMI_OK:
+            msg = lib().wbmi_last_error(self.h).decode(errors="replace")
+            if rc == WBMI_ERR_ARG:
+                raise ValueError(f"{what}: {msg}")
+            if rc == WBMI_ERR_UNSUPPORTED:
+                raise NotImplementedError(f"{what}: {msg}")
+            raise WbmiError(f"{what}: {msg}")
+
+    def sync(self):
+        self.check(lib().wbmi_sync(self.h), "wbmi_sync")
+
+    # -- stages --------------------------------------------------------------
+    def build_channels(self, cRshift, base, spec, out):
+        nR, nw = cRshift.shape[0], cRshift.shape[1]
+        spec_a, spec_p = _host_i32(spec)
+        self.check(lib().wbmi_build_channels(self.h, nR, nw, _ptr(cRshift), _ptr(base), base.shape[3], spec_p,
+                                             spec_a.shape[0], _ptr(out)), "wbmi_build_channels")
+
+    def r2k_plan(self, iRvec, NKFFT, nelem):
+        return R2KPlan(self, iRvec, NKFFT, nelem)
+
+    def r2k_klist(self, iRvec_dev, XR, klist_dev, out, prec):
+        nR = iRvec_dev.shape[0]
+        nelem = XR[0].numel()
+        self.check(lib().wbmi_r2k_klist(self.h, nR, _ptr(iRvec_dev), nelem, _ptr(XR), klist_dev.shape[0],
+                                        _ptr(klist_dev), _ptr(out), prec), "wbmi_r2k_klist")
+
+    def eigh(self, H, Hstride, E, U, prec, nw):
+        self.check(lib().wbmi_eigh(self.h, E.shape[0], nw, _ptr(H), Hstride, _ptr(E), _ptr(U), prec, 0, None),
+                   "wbmi_eigh")
+
+    def rotate(self, U, X, herm, skip, prec):
+        nk, nch, nw = X.shape[0], X.shape[1], X.shape[2]
+        h_a, h_p = _host_i32(herm)
+        s_a, s_p = _host_i32(skip)
+        self.check(lib().wbmi_rotate(self.h, nk, nw, nch, _ptr(U), _ptr(X), h_p, s_p, prec), "wbmi_rotate")
+
+    def static_scan(self, E, X, prec, params, weight, hist):
+        nk, nw = E.shape
+        nch = X.shape[1] if X is not None else 1
+        self.check(lib().wbmi_static_scan(self.h, nk, nw, nch, _ptr(E), _ptr(X), prec, ctypes.byref(params),
+                                          float(weight), _ptr(hist)), "wbmi_static_scan")
+
+    def finalize(self, hist, fder, dEF, scale, out):
+        nEF, ncomp = hist.shape
+        self.check(lib().wbmi_finalize(self.h, _ptr(hist), nEF, ncomp, fder, float(dEF), float(scale), _ptr(out)),
+                   "wbmi_finalize")
+
+
+class R2KPlan:
+    def __init__(self, ctx, iRvec, NKFFT, nelem):
+        self.ctx = ctx
+        iR_a, iR_p = _host_i32(np.asarray(iRvec).reshape(-1, 3))
+        nk_a = (ctypes.c_int32 * 3)(*[int(x) for x in NKFFT])
+        h = ctypes.c_void_p()
+        ctx.check(lib().wbmi_r2k_plan_create(ctx.h, iR_a.shape[0], iR_p, nk_a, int(nelem), ctypes.byref(h)),
+                  "wbmi_r2k_plan_create")
+        self.h = h
+        b = ctypes.c_size_t()
+        w = ctypes.c_size_t()
+        ctx.check(lib().wbmi_r2k_plan_sizes(h, ctypes.byref(b), ctypes.byref(w)), "wbmi_r2k_plan_sizes")
+        self.box_bytes, self.work_bytes = b.value, w.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and _lib is not None:
+                _lib.wbmi_r2k_plan_destroy(self.h)
+        except Exception:
+            pass
+
+    def box(self, XR, box):
+        self.ctx.check(lib().wbmi_r2k_box(self.ctx.h, self.h, _ptr(XR), _ptr(box)), "wbmi_r2k_box")
+
+    def execute(self, XR, box, dK, work, out, prec):
+        dk = (ctypes.c_double * 3)(*[float(x) for x in dK])
+        self.ctx.check(lib().wbmi_r2k_exec(self.ctx.h, self.h, _ptr(XR), _ptr(box), dk, _ptr(work), _ptr(out),
+                                           prec), "wbmi_r2k_exec")
+
+
+_contexts = {}
+
+
+def get_context(device=0):
+    """Per-device context on torch's current stream."""
+    import torch
+    key = (device, torch.cuda.current_stream(device).cuda_stream)
+    if key not in _contexts:
+        _contexts[key] = Context(device)
+    return _contexts[key]

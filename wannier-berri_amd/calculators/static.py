@@ -1,0 +1,233 @@
+"""Static (Fermi-level scan) calculators on the MI355X.
+
+Same constructor signatures and outputs as wannierberri/calculators/static.py;
+each call runs the fused formula-trace + Fermi-scan kernel (wbmi_static_scan)
+on the K-block and the prefix-sum / fder-stencil / scaling kernel
+(wbmi_finalize).  The result stays on the GPU until read.
+"""
+import numpy as np
+
+from .. import _lib
+from .. import factors
+from ..result import EnergyResult, VoidResult
+
+
+class Calculator:
+    """calculators/calculator.py:6-43."""
+
+    def __init__(self, degen_thresh=1e-4, degen_Kramers=False, save_mode="bin+txt", print_comment=False):
+        self.degen_thresh = degen_thresh
+        self.degen_Kramers = degen_Kramers
+        self.save_mode = save_mode
+        if not hasattr(self, "comment"):
+            self.comment = self.__doc__ if self.__doc__ is not None else "calculator not described"
+        if print_comment:
+            print(self.comment)
+
+    @property
+    def allow_path(self):
+        return False
+
+    @property
+    def allow_grid(self):
+        return True
+
+
+def _external(kwargs_formula, system):
+    ext = kwargs_formula.get("external_terms", True)
+    return bool(ext) and not system.force_internal_terms_only
+
+
+class StaticCalculator(Calculator):
+    """Fermi-level scan integrator (calculators/static.py:24-169).
+
+    Subclasses set `formula` (a WBMI_FORMULA_* id), `ndim` and `fder`.
+    Not supported: tetra, k_resolved, select_bands, finite Emin/Emax.
+    """
+    formula = None
+    ndim = 0
+    sign = 0
+
+    def __init__(self, Efermi, tetra=False, smoother=None, constant_factor=1., use_factor=True, kwargs_formula=None,
+                 Emin=-np.inf, Emax=np.inf, hole_like=False, k_resolved=False, fder=None, select_bands=None,
+                 **kwargs):
+        super().__init__(**kwargs)
+        if tetra:
+            raise NotImplementedError("the tetrahedron method is not available on the MI355X path")
+        if k_resolved:
+            raise NotImplementedError("k-resolved output is not available on the MI355X path")
+        if select_bands is not None:
+            raise NotImplementedError("select_bands is not available on the MI355X path")
+        if np.isfinite(Emin) or np.isfinite(Emax):
+            raise NotImplementedError("Emin/Emax band windows are not available on the MI355X path")
+        self.kwargs_formula = dict(kwargs_formula or {})
+        for k in self.kwargs_formula:
+            if k not in ("internal_terms", "external_terms", "sign"):
+                raise NotImplementedError(f"kwargs_formula option '{k}' is not available on the MI355X path")
+        self.Efermi = np.array(Efermi, dtype=float)
+        self.smoother = smoother
+        self.use_factor = use_factor
+        self.hole_like = hole_like
+        if fder is not None:
+            self.fder = fder
+        self.constant_factor = constant_factor
+        if hole_like and self.fder == 0:
+            self.constant_factor *= -1
+        self.extraEf = {0: 0, 1: 1, 2: 1, 3: 2}[self.fder]
+        self.dEF = self.Efermi[1] - self.Efermi[0] if len(self.Efermi) > 1 else 0.001
+        self.EFmin = self.Efermi[0] - self.extraEf * self.dEF
+        self.EFmax = self.Efermi[-1] + self.extraEf * self.dEF
+        self.nEF_extra = self.Efermi.shape[0] + 2 * self.extraEf
+
+    @property
+    def ncomp(self):
+        return 3 ** self.ndim
+
+    def needed_channels(self, system):
+        """Channels Xbar(name, der) the formula reads."""
+        ext = _external(self.kwargs_formula, system)
+        f = self.formula
+        if f == _lib.FORMULA_IDENTITY:
+            return {("Ham", 0)}
+        need = {("Ham", 0), ("Ham", 1)}
+        if f == _lib.FORMULA_DEROMEGA:
+            need.add(("Ham", 2))
+        if ext:
+            need |= {("AA", 0), ("rotAA", 0)}
+            if f == _lib.FORMULA_DEROMEGA:
+                need |= {("AA", 1), ("rotAA", 1)}
+            if f == _lib.FORMULA_MORB_HPM:
+                need |= {("BB", 0), ("CC", 0)}
+        return need
+
+    def scan_params(self, system):
+        p = _lib.ScanParams()
+        p.formula = self.formula
+        p.sign = int(self.kwargs_formula.get("sign", self.sign))
+        p.internal_terms = int(bool(self.kwargs_formula.get("internal_terms", True)))
+        p.external_terms = int(_external(self.kwargs_formula, system))
+        p.fder = self.fder
+        p.nEF_extra = self.nEF_extra
+        p.EFmin, p.EFmax, p.dEF = float(self.EFmin), float(self.EFmax), float(self.dEF)
+        p.degen_thresh = float(self.degen_thresh)
+        p.degen_Kramers = int(bool(self.degen_Kramers))
+        return p
+
+    def __call__(self, data_K):
+        torch = data_K.torch
+        data_K.ensure(self.needed_channels(data_K.system))
+        dev = data_K.ctx.device
+        hist = torch.zeros((self.nEF_extra, self.ncomp), dtype=torch.float64, device=dev)
+        data_K.static_scan(self.scan_params(data_K.system), hist, 1.0)
+        nout = self.Efermi.shape[0]
+        out = torch.empty((nout, self.ncomp), dtype=torch.float64, device=dev)
+        cf = self.constant_factor if self.use_factor else float(np.sign(self.constant_factor))
+        scale = cf / data_K.cell_volume / data_K.nk
+        data_K.ctx.finalize(hist, self.fder, self.dEF, scale, out)
+        data = out.reshape((nout,) + (3,) * self.ndim)
+        return EnergyResult(self.Efermi, data, smoothers=[self.smoother], comment=self.comment,
+                            save_mode=self.save_mode)
+
+
+# -- implemented calculators (calculators/static.py:192-498) --------------------
+
+class _DOS(StaticCalculator):
+    formula = _lib.FORMULA_IDENTITY
+    ndim = 0
+
+    def __init__(self, fder, **kwargs):
+        self.fder = fder
+        super().__init__(**kwargs)
+
+    def __call__(self, data_K):
+        return super().__call__(data_K) * data_K.cell_volume
+
+
+class DOS(_DOS):
+    r"""Density of states"""
+
+    def __init__(self, **kwargs):
+        super().__init__(fder=1, **kwargs)
+
+
+class CumDOS(_DOS):
+    r"""Cumulative density of states"""
+
+    def __init__(self, **kwargs):
+        super().__init__(fder=0, **kwargs)
+
+
+class AHC(StaticCalculator):
+    r"""Anomalous Hall conductivity (:math:`s^3 \cdot A^2 / (kg \cdot m^3) = S/m`)
+
+        | Output: :math:`O = -e^2/\hbar \int [dk] \Omega f`"""
+    formula = _lib.FORMULA_OMEGA
+    ndim = 1
+    fder = 0
+
+    def __init__(self, constant_factor=factors.factor_ahc, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class BerryDipole_FermiSurf(StaticCalculator):
+    r"""Berry curvature dipole (dimensionless), Fermi surface
+
+        | Output: :math:`D_{\beta\delta} = -\int [dk] v_\beta \Omega_\delta f'`"""
+    formula = _lib.FORMULA_VELOMEGA
+    ndim = 2
+    fder = 1
+
+
+class NLAHC_FermiSurf(BerryDipole_FermiSurf):
+    r"""Nonlinear anomalous Hall conductivity (:math:`S^2/A`), Fermi surface"""
+
+    def __init__(self, constant_factor=factors.factor_nlahc, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class BerryDipole_FermiSea(StaticCalculator):
+    r"""Berry curvature dipole (dimensionless), Fermi sea
+
+        | Output: :math:`D_{\beta\delta} = \int [dk] \partial_\beta \Omega_\delta f`"""
+    formula = _lib.FORMULA_DEROMEGA
+    ndim = 2
+    fder = 0
+
+    def __call__(self, data_K):
+        # swap axes to match eq. (29) of DOI:10.1038/s41524-021-00498-5 (static.py:483-487)
+        return super().__call__(data_K).swapaxes(1, 2)
+
+
+class NLAHC_FermiSea(BerryDipole_FermiSea):
+    r"""Nonlinear anomalous Hall conductivity (:math:`S^2/A`), Fermi sea"""
+
+    def __init__(self, constant_factor=factors.factor_nlahc, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class Morb(StaticCalculator):
+    r"""Orbital magnetic moment per unit cell (:math:`\mu_B`)
+
+        | Output: :math:`M = -\int [dk] (H + G - 2E_f \cdot \Omega) f`"""
+    formula = _lib.FORMULA_MORB_HPM
+    ndim = 1
+    fder = 0
+    sign = 1
+
+    def __init__(self, constant_factor=-factors.eV_au / factors.bohr ** 2, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+        self.AHC = AHC(constant_factor=constant_factor, print_comment=False, **kwargs)
+
+    def needed_channels(self, system):
+        return super().needed_channels(system) | self.AHC.needed_channels(system)
+
+    def __call__(self, data_K):
+        data_K.ensure(self.needed_channels(data_K.system))
+        Hplus = super().__call__(data_K)
+        Omega = self.AHC(data_K).mul_array(self.Efermi, axes=0)
+        Hplus.add(-2 * Omega)
+        return Hplus * data_K.cell_volume
+
+
+__all__ = ["Calculator", "StaticCalculator", "DOS", "CumDOS", "AHC", "BerryDipole_FermiSurf", "NLAHC_FermiSurf",
+           "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "VoidResult"]

@@ -1,0 +1,404 @@
+// R -> k transforms of libwbmi.
+//
+//  * wbmi_build_channels: R-space channel stack (derivatives, curl) once per system.
+//  * wbmi_r2k_*: the K-block transform on an NKFFT grid as three separable DFT
+//    passes over the R bounding box.  Every pass is HBM-bound: it streams the
+//    [A][L][B] input once and writes [A][N][B]; threads run along the
+//    contiguous B index so loads and stores are fully coalesced.
+//  * wbmi_r2k_klist: phase-matrix transform for arbitrary k-points.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "wbmi_common.h"
+
+// ---------------------------------------------------------------------------
+// channel stack
+// ---------------------------------------------------------------------------
+
+#define WBMI_MAX_CH 128
+
+struct ChanSpec {
+    int8_t s[WBMI_MAX_CH * 8];  // per channel: 2 terms x (sign, a, d1, d2)
+};
+
+__global__ void k_build_channels(int nR, int nw, int nch, int nbase, const double* __restrict__ cR,
+                                 const cplx<double>* __restrict__ base, ChanSpec spec,
+                                 cplx<double>* __restrict__ out) {
+    int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t nw2 = (int64_t)nw * nw;
+    int64_t total = (int64_t)nR * nch * nw2;
+    if (idx >= total) return;
+    int64_t mn = idx % nw2;
+    int ch = (int)((idx / nw2) % nch);
+    int64_t R = idx / (nw2 * nch);
+    int64_t e = R * nw2 + mn;  // element (R, m, n)
+    const double* r = cR + e * 3;
+    cplx<double> acc = mk(0.0, 0.0);
+    for (int t = 0; t < 2; ++t) {
+        const int8_t* q = spec.s + ch * 8 + t * 4;
+        int sg = q[0];
+        if (sg == 0) continue;
+        cplx<double> x = base[e * nbase + q[1]];
+        for (int k = 2; k < 4; ++k) {
+            int d = q[k];
+            if (d >= 0) x = r[d] * times_i(x);
+        }
+        if (sg > 0) acc += x; else acc -= x;
+    }
+    out[idx] = acc;
+}
+
+extern "C" int wbmi_build_channels(wbmi_ctx* ctx, int nR, int nw, const double* cRshift_dev,
+                                   const void* base_dev, int nbase, const int32_t* spec_host, int nch,
+                                   void* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (nR <= 0 || nw <= 0 || nbase <= 0 || nch <= 0 || nch > WBMI_MAX_CH || !cRshift_dev || !base_dev ||
+        !spec_host || !out_dev)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_build_channels: bad arguments (nch <= %d)", WBMI_MAX_CH);
+    ChanSpec spec;
+    for (int c = 0; c < nch; ++c)
+        for (int t = 0; t < 2; ++t) {
+            const int32_t* q = spec_host + (c * 2 + t) * 4;
+            if (q[0] < -1 || q[0] > 1 || (q[0] != 0 && (q[1] < 0 || q[1] >= nbase)) || q[2] < -1 || q[2] > 2 ||
+                q[3] < -1 || q[3] > 2)
+                return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_build_channels: bad spec for channel %d", c);
+            for (int k = 0; k < 4; ++k) spec.s[c * 8 + t * 4 + k] = (int8_t)q[k];
+        }
+    int64_t total = (int64_t)nR * nch * nw * nw;
+    hipLaunchKernelGGL(k_build_channels, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, ctx->stream, nR, nw,
+                       nch, nbase, cRshift_dev, (const cplx<double>*)base_dev, spec, (cplx<double>*)out_dev);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// separable DFT on the NKFFT grid
+// ---------------------------------------------------------------------------
+
+struct wbmi_r2k_plan {
+    int nR;
+    int N[3];
+    int L[3];        // box extent per axis
+    int off[3];      // R value of box index 0 (non-aliased axes)
+    int alias[3];    // 1 when the R extent exceeds NKFFT on this axis
+    int any_alias;
+    int64_t nelem;
+    int64_t nslot;
+    int32_t* slot_start;  // device CSR: R indices per box slot
+    int32_t* slot_list;
+    int32_t* iR;          // device copy of iRvec
+    cplx<double>* phase;  // device [3][NMAX*LMAX] pass phase tables
+    int tab;              // stride of one axis table
+};
+
+// box[s][e] = sum_{R in slot s} X[R][e] exp(2 pi i sum_{aliased axes} dK_i R_i)
+__global__ void k_box_gather(int64_t nslot, int64_t nelem, const int32_t* __restrict__ start,
+                             const int32_t* __restrict__ list, const int32_t* __restrict__ iR,
+                             const cplx<double>* __restrict__ X, double dk0, double dk1, double dk2,
+                             cplx<double>* __restrict__ box) {
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t s = blockIdx.y + (int64_t)blockIdx.z * gridDim.y;
+    if (e >= nelem || s >= nslot) return;
+    cplx<double> acc = mk(0.0, 0.0);
+    for (int q = start[s]; q < start[s + 1]; ++q) {
+        int r = list[q];
+        double ph = dk0 * iR[3 * r] + dk1 * iR[3 * r + 1] + dk2 * iR[3 * r + 2];
+        ph -= floor(ph);
+        double sn, cs;
+        sincospi(2.0 * ph, &sn, &cs);
+        acc += mk(cs, sn) * X[(int64_t)r * nelem + e];
+    }
+    box[s * nelem + e] = acc;
+}
+
+// phase[ax][j*L + b] = exp(2 pi i ((j*(b+off)) mod N / N + dKpass*(b+off)))
+__global__ void k_pass_phases(int N0, int N1, int N2, int L0, int L1, int L2, int o0, int o1, int o2, double d0,
+                              double d1, double d2, int tab, cplx<double>* __restrict__ phase) {
+    int ax = blockIdx.y;
+    int N = ax == 0 ? N0 : ax == 1 ? N1 : N2;
+    int L = ax == 0 ? L0 : ax == 1 ? L1 : L2;
+    int off = ax == 0 ? o0 : ax == 1 ? o1 : o2;
+    double dk = ax == 0 ? d0 : ax == 1 ? d1 : d2;
+    for (int t = threadIdx.x + blockIdx.x * blockDim.x; t < N * L; t += blockDim.x * gridDim.x) {
+        int j = t / L, b = t % L;
+        long long R = (long long)b + off;
+        long long jr = ((long long)j * R) % N;
+        if (jr < 0) jr += N;
+        double ph = (double)jr / N;
+        double f = dk * (double)R;
+        f -= floor(f);
+        ph += f;
+        double sn, cs;
+        sincospi(2.0 * ph, &sn, &cs);
+        phase[ax * tab + t] = mk(cs, sn);
+    }
+}
+
+// out[a][j][b] = sum_l P[j][l] in[a][l][b]; one thread per (a, b), inputs held in registers
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(256) k_dft_pass(const cplx<Tin>* __restrict__ in, cplx<Tout>* __restrict__ out,
+                                                  const cplx<double>* __restrict__ P, int64_t A, int L, int N,
+                                                  int64_t B) {
+    extern __shared__ cplx<double> sP[];
+    for (int t = threadIdx.x; t < N * L; t += blockDim.x) sP[t] = P[t];
+    __syncthreads();
+    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t a = blockIdx.y + (int64_t)blockIdx.z * gridDim.y;
+    if (b >= B || a >= A) return;
+    const cplx<Tin>* src = in + (a * L) * B + b;
+    cplx<Tout>* dst = out + (a * N) * B + b;
+    constexpr int LC = 16;
+    for (int l0 = 0; l0 < L; l0 += LC) {
+        cplx<double> x[LC];
+#pragma unroll
+        for (int l = 0; l < LC; ++l) {
+            if (l0 + l < L) {
+                cplx<Tin> v = src[(int64_t)(l0 + l) * B];
+                x[l] = mk((double)v.re, (double)v.im);
+            } else {
+                x[l] = mk(0.0, 0.0);
+            }
+        }
+        for (int j = 0; j < N; ++j) {
+            cplx<double> acc = mk(0.0, 0.0);
+            if (l0 > 0) {
+                cplx<Tout> p = dst[(int64_t)j * B];
+                acc = mk((double)p.re, (double)p.im);
+            }
+            const cplx<double>* pj = sP + j * L + l0;
+#pragma unroll
+            for (int l = 0; l < LC; ++l)
+                if (l0 + l < L) acc += pj[l] * x[l];
+            dst[(int64_t)j * B] = mk((Tout)acc.re, (Tout)acc.im);
+        }
+    }
+}
+
+static void grid_2d(int64_t A, dim3& g) {
+    // split A over gridDim.y * gridDim.z (y <= 65535)
+    int64_t gy = A < 65535 ? A : 65535;
+    int64_t gz = ceil_div(A, gy);
+    g.y = (unsigned)gy;
+    g.z = (unsigned)gz;
+}
+
+template <typename Tin, typename Tout>
+static int launch_pass(wbmi_ctx* ctx, const void* in, void* out, const cplx<double>* P, int64_t A, int L, int N,
+                       int64_t B) {
+    dim3 g((unsigned)ceil_div(B, 256));
+    grid_2d(A, g);
+    size_t lds = sizeof(cplx<double>) * (size_t)N * L;
+    hipLaunchKernelGGL((k_dft_pass<Tin, Tout>), g, dim3(256), lds, ctx->stream, (const cplx<Tin>*)in,
+                       (cplx<Tout>*)out, P, A, L, N, B);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+extern "C" {
+
+int wbmi_r2k_plan_create(wbmi_ctx* ctx, int nR, const int32_t* iRvec_host, const int32_t NKFFT[3], int64_t nelem,
+                         wbmi_r2k_plan** out) {
+    if (!ctx || !out) return WBMI_ERR_ARG;
+    *out = nullptr;
+    if (nR <= 0 || !iRvec_host || nelem <= 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_plan_create: bad arguments");
+    wbmi_r2k_plan* p = new wbmi_r2k_plan();
+    p->nR = nR;
+    p->nelem = nelem;
+    p->any_alias = 0;
+    for (int i = 0; i < 3; ++i) {
+        if (NKFFT[i] <= 0) {
+            delete p;
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_plan_create: NKFFT must be positive");
+        }
+        int lo = iRvec_host[i], hi = iRvec_host[i];
+        for (int r = 0; r < nR; ++r) {
+            lo = std::min(lo, iRvec_host[3 * r + i]);
+            hi = std::max(hi, iRvec_host[3 * r + i]);
+        }
+        p->N[i] = NKFFT[i];
+        int ext = hi - lo + 1;
+        p->alias[i] = ext > NKFFT[i];
+        p->L[i] = p->alias[i] ? NKFFT[i] : ext;
+        p->off[i] = p->alias[i] ? 0 : lo;
+        p->any_alias |= p->alias[i];
+    }
+    p->nslot = (int64_t)p->L[0] * p->L[1] * p->L[2];
+    std::vector<std::vector<int32_t>> lists(p->nslot);
+    for (int r = 0; r < nR; ++r) {
+        int64_t s = 0;
+        for (int i = 0; i < 3; ++i) {
+            int R = iRvec_host[3 * r + i];
+            int b = p->alias[i] ? ((R % p->N[i]) + p->N[i]) % p->N[i] : R - p->off[i];
+            s = s * p->L[i] + b;
+        }
+        lists[s].push_back(r);
+    }
+    std::vector<int32_t> start(p->nslot + 1, 0), flat;
+    flat.reserve(nR);
+    for (int64_t s = 0; s < p->nslot; ++s) {
+        start[s] = (int32_t)flat.size();
+        flat.insert(flat.end(), lists[s].begin(), lists[s].end());
+    }
+    start[p->nslot] = (int32_t)flat.size();
+    int NM = std::max(p->N[0], std::max(p->N[1], p->N[2]));
+    int LM = std::max(p->L[0], std::max(p->L[1], p->L[2]));
+    p->tab = NM * LM;
+    hipError_t e = hipMalloc(&p->slot_start, sizeof(int32_t) * (p->nslot + 1));
+    if (e == hipSuccess) e = hipMalloc(&p->slot_list, sizeof(int32_t) * nR);
+    if (e == hipSuccess) e = hipMalloc(&p->iR, sizeof(int32_t) * 3 * nR);
+    if (e == hipSuccess) e = hipMalloc(&p->phase, sizeof(cplx<double>) * 3 * p->tab);
+    if (e == hipSuccess) e = hipMemcpy(p->slot_start, start.data(), sizeof(int32_t) * (p->nslot + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->slot_list, flat.data(), sizeof(int32_t) * nR, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->iR, iRvec_host, sizeof(int32_t) * 3 * nR, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        wbmi_r2k_plan_destroy(p);
+        return wbmi_fail(ctx, WBMI_ERR_HIP, "wbmi_r2k_plan_create: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return WBMI_OK;
+}
+
+int wbmi_r2k_plan_destroy(wbmi_r2k_plan* p) {
+    if (!p) return WBMI_OK;
+    if (p->slot_start) (void)hipFree(p->slot_start);
+    if (p->slot_list) (void)hipFree(p->slot_list);
+    if (p->iR) (void)hipFree(p->iR);
+    if (p->phase) (void)hipFree(p->phase);
+    delete p;
+    return WBMI_OK;
+}
+
+int wbmi_r2k_plan_sizes(const wbmi_r2k_plan* p, size_t* box_bytes, size_t* work_bytes) {
+    if (!p) return WBMI_ERR_ARG;
+    if (box_bytes) *box_bytes = sizeof(cplx<double>) * (size_t)p->nslot * p->nelem;
+    // two intermediate pass outputs, sized for complex128 (also enough for complex64)
+    size_t w1 = (size_t)p->L[0] * p->L[1] * p->N[2];
+    size_t w2 = (size_t)p->L[0] * p->N[1] * p->N[2];
+    if (work_bytes) *work_bytes = sizeof(cplx<double>) * (w1 + w2) * p->nelem;
+    return WBMI_OK;
+}
+
+int wbmi_r2k_box(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* XR_dev, void* box_dev) {
+    if (!ctx || !p || !XR_dev || !box_dev) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_box: null argument");
+    dim3 g((unsigned)ceil_div(p->nelem, 256));
+    grid_2d(p->nslot, g);
+    hipLaunchKernelGGL(k_box_gather, g, dim3(256), 0, ctx->stream, p->nslot, p->nelem, p->slot_start, p->slot_list,
+                       p->iR, (const cplx<double>*)XR_dev, 0.0, 0.0, 0.0, (cplx<double>*)box_dev);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* XR_dev, void* box_dev, const double dK[3],
+                  void* work_dev, void* out_dev, int prec) {
+    if (!ctx || !p || !box_dev || !work_dev || !out_dev || !dK)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_exec: null argument");
+    if (prec != WBMI_F64 && prec != WBMI_F32) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_exec: bad precision");
+    double dpass[3];
+    if (p->any_alias) {
+        if (!XR_dev) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_exec: aliased plan needs XR_dev");
+        dim3 g((unsigned)ceil_div(p->nelem, 256));
+        grid_2d(p->nslot, g);
+        hipLaunchKernelGGL(k_box_gather, g, dim3(256), 0, ctx->stream, p->nslot, p->nelem, p->slot_start,
+                           p->slot_list, p->iR, (const cplx<double>*)XR_dev, p->alias[0] ? dK[0] : 0.0,
+                           p->alias[1] ? dK[1] : 0.0, p->alias[2] ? dK[2] : 0.0, (cplx<double>*)box_dev);
+        WBMI_LAUNCH_CHECK(ctx);
+    }
+    for (int i = 0; i < 3; ++i) dpass[i] = p->alias[i] ? 0.0 : dK[i];
+    hipLaunchKernelGGL(k_pass_phases, dim3(4, 3), dim3(256), 0, ctx->stream, p->N[0], p->N[1], p->N[2], p->L[0],
+                       p->L[1], p->L[2], p->off[0], p->off[1], p->off[2], dpass[0], dpass[1], dpass[2], p->tab,
+                       p->phase);
+    WBMI_LAUNCH_CHECK(ctx);
+    const int64_t E = p->nelem;
+    const int L0 = p->L[0], L1 = p->L[1], L2 = p->L[2], N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
+    size_t esz = prec == WBMI_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+    char* w1 = (char*)work_dev;
+    char* w2 = w1 + esz * (size_t)L0 * L1 * N2 * E;
+    int rc;
+    // z pass: [L0 L1][L2][E] -> [L0 L1][N2][E]
+    if (prec == WBMI_F64) {
+        rc = launch_pass<double, double>(ctx, box_dev, w1, p->phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
+        if (rc) return rc;
+        rc = launch_pass<double, double>(ctx, w1, w2, p->phase + 1 * p->tab, L0, L1, N1, (int64_t)N2 * E);
+        if (rc) return rc;
+        rc = launch_pass<double, double>(ctx, w2, out_dev, p->phase, 1, L0, N0, (int64_t)N1 * N2 * E);
+    } else {
+        rc = launch_pass<double, float>(ctx, box_dev, w1, p->phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
+        if (rc) return rc;
+        rc = launch_pass<float, float>(ctx, w1, w2, p->phase + 1 * p->tab, L0, L1, N1, (int64_t)N2 * E);
+        if (rc) return rc;
+        rc = launch_pass<float, float>(ctx, w2, out_dev, p->phase, 1, L0, N0, (int64_t)N1 * N2 * E);
+    }
+    return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// arbitrary k-list: out[k][e] = sum_R exp(2 pi i k.R) X[R][e]
+// ---------------------------------------------------------------------------
+
+#define KL_TK 8
+#define KL_TR 128
+
+template <typename Tout>
+__global__ void __launch_bounds__(256) k_r2k_klist(int nR, const int32_t* __restrict__ iR, int64_t nelem,
+                                                   const cplx<double>* __restrict__ X, int nk,
+                                                   const double* __restrict__ kl, cplx<Tout>* __restrict__ out) {
+    __shared__ cplx<double> ph[KL_TK][KL_TR];
+    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int k0 = blockIdx.y * KL_TK;
+    cplx<double> acc[KL_TK];
+#pragma unroll
+    for (int q = 0; q < KL_TK; ++q) acc[q] = mk(0.0, 0.0);
+    for (int r0 = 0; r0 < nR; r0 += KL_TR) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < KL_TK * KL_TR; t += blockDim.x) {
+            int q = t / KL_TR, rr = t % KL_TR;
+            int k = k0 + q, r = r0 + rr;
+            cplx<double> v = mk(0.0, 0.0);
+            if (k < nk && r < nR) {
+                double f = kl[3 * k] * iR[3 * r] + kl[3 * k + 1] * iR[3 * r + 1] + kl[3 * k + 2] * iR[3 * r + 2];
+                f -= floor(f);
+                double sn, cs;
+                sincospi(2.0 * f, &sn, &cs);
+                v = mk(cs, sn);
+            }
+            ph[q][rr] = v;
+        }
+        __syncthreads();
+        if (e < nelem) {
+            int rmax = nR - r0 < KL_TR ? nR - r0 : KL_TR;
+            for (int rr = 0; rr < rmax; ++rr) {
+                cplx<double> x = X[(int64_t)(r0 + rr) * nelem + e];
+#pragma unroll
+                for (int q = 0; q < KL_TK; ++q) acc[q] += ph[q][rr] * x;
+            }
+        }
+    }
+    if (e < nelem) {
+#pragma unroll
+        for (int q = 0; q < KL_TK; ++q)
+            if (k0 + q < nk) out[(int64_t)(k0 + q) * nelem + e] = mk((Tout)acc[q].re, (Tout)acc[q].im);
+    }
+}
+
+extern "C" int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nelem, const void* XR_dev,
+                              int nk, const double* klist_dev, void* out_dev, int prec) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (nR <= 0 || nelem <= 0 || nk <= 0 || !iRvec_dev || !XR_dev || !klist_dev || !out_dev)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: bad arguments");
+    if (ceil_div(nk, KL_TK) > 65535) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: too many k-points per call");
+    dim3 g((unsigned)ceil_div(nelem, 256), (unsigned)ceil_div(nk, KL_TK));
+    if (prec == WBMI_F64)
+        hipLaunchKernelGGL(k_r2k_klist<double>, g, dim3(256), 0, ctx->stream, nR, iRvec_dev, nelem,
+                           (const cplx<double>*)XR_dev, nk, klist_dev, (cplx<double>*)out_dev);
+    else if (prec == WBMI_F32)
+        hipLaunchKernelGGL(k_r2k_klist<float>, g, dim3(256), 0, ctx->stream, nR, iRvec_dev, nelem,
+                           (const cplx<double>*)XR_dev, nk, klist_dev, (cplx<float>*)out_dev);
+    else
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: bad precision");
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}

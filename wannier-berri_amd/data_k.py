@@ -1,0 +1,295 @@
+"""Data_K: one K-block of the Berry path on the MI355X (the `data_k_class` of run()).
+
+Constructed as the reference constructs its data_K per K-block
+(run_grid.py:258-265, data_K/data_K_R.py:8-22):
+
+    Data_K(system, dK=Kp.Kp_fullBZ, grid=grid, Kpoint=Kp, **parameters_K)
+
+and consumed by the calculators of `wannierberri_amd.calculators`.  All
+numerics run in libwbmi (HIP, gfx950):
+
+  R-space channel stack (once per system)  -> wbmi_build_channels
+  R -> k on the NKFFT sub-grid              -> wbmi_r2k_exec  (k-list: wbmi_r2k_klist)
+  eigh of 0.5 (H + H^+)                     -> wbmi_eigh
+  U^+ X U of every other channel            -> wbmi_rotate
+  formula traces + Fermi scan               -> wbmi_static_scan (called by calculators)
+
+A "channel" is one complex nw x nw matrix per k: a component of
+Xbar(name, der) for name in Ham, AA, rotAA, BB, CC and der in 0, 1 (2 for Ham).
+"""
+import numpy as np
+
+from . import _lib
+from ._lib import WBMI_F32, WBMI_F64
+
+# components of each Xbar(name, der)
+CHANNEL_NCOMP = {("Ham", 0): 1, ("Ham", 1): 3, ("Ham", 2): 9,
+                 ("AA", 0): 3, ("AA", 1): 9, ("rotAA", 0): 3, ("rotAA", 1): 9,
+                 ("BB", 0): 3, ("BB", 1): 9, ("CC", 0): 3, ("CC", 1): 9}
+CHANNEL_ORDER = list(CHANNEL_NCOMP)
+# matrices forced Hermitian after R->k (data_K/data_K_R.py:87)
+HERMITIAN = {"AA", "rotAA"}
+BASES = ["Ham", "AA", "BB", "CC"]
+_ALPHA = [1, 2, 0]
+_BETA = [2, 0, 1]
+
+
+def channel_terms(name, der, base_offset):
+    """Per component: two terms (sign, base component, d1, d2) of X_R * prod(i r_d).
+
+    Ham/AA/BB/CC derivatives: rvectors.py:487-506 (one Cartesian axis per order);
+    rotAA_c = i r_alpha A_beta - i r_beta A_alpha: data_K_R.py:48-51.
+    """
+    out = []
+    if name == "rotAA":
+        a0 = base_offset["AA"]
+        for c in range(3):
+            al, be = _ALPHA[c], _BETA[c]
+            for d in ([-1] if der == 0 else [0, 1, 2]):
+                out.append([[1, a0 + be, al, d], [-1, a0 + al, be, d]])
+        return out
+    b0 = base_offset[name]
+    if name == "Ham":
+        if der == 0:
+            return [[[1, b0, -1, -1], [0, 0, -1, -1]]]
+        if der == 1:
+            return [[[1, b0, a, -1], [0, 0, -1, -1]] for a in range(3)]
+        return [[[1, b0, a, b], [0, 0, -1, -1]] for a in range(3) for b in range(3)]
+    if der == 0:
+        return [[[1, b0 + a, -1, -1], [0, 0, -1, -1]] for a in range(3)]
+    return [[[1, b0 + a, d, -1], [0, 0, -1, -1]] for a in range(3) for d in range(3)]
+
+
+class DeviceSystem:
+    """Device copies of a System_R's R-space data (cached per system and device)."""
+
+    def __init__(self, system, ctx):
+        import torch
+        self.torch = torch
+        self.ctx = ctx
+        dev = ctx.device
+        self.system = system
+        self.nw = system.num_wann
+        self.nR = system.nRvec
+        present = [b for b in BASES if system.has_R_mat(b)]
+        self.base_offset = {}
+        comps = []
+        for b in present:
+            X = system.get_R_mat(b)
+            X = X.reshape(X.shape[:3] + (-1,))
+            self.base_offset[b] = sum(c.shape[3] for c in comps)
+            comps.append(X)
+        base = np.concatenate(comps, axis=3)
+        self.base = torch.from_numpy(np.ascontiguousarray(base)).to(dev)
+        self.cR = torch.from_numpy(np.ascontiguousarray(system.cRvec_shifted)).to(dev)
+        self.iRvec = np.ascontiguousarray(system.rvec.iRvec, dtype=np.int32)
+        self.iRvec_dev = torch.from_numpy(self.iRvec).to(dev)
+        self._stacks = {}
+        self._plans = {}
+
+    def has(self, name):
+        return name in self.base_offset or (name == "rotAA" and "AA" in self.base_offset)
+
+    def stack(self, keys):
+        """R-space channel stack [nR, nch, nw, nw] (complex128) for the channel keys."""
+        keys = tuple(keys)
+        if keys not in self._stacks:
+            spec = []
+            for name, der in keys:
+                if not self.has(name):
+                    raise ValueError(f"The real-space matrix elements '{name}' are not set in the system, "
+                                     "but are required for the current calculation")
+                spec += channel_terms(name, der, self.base_offset)
+            spec = np.array(spec, dtype=np.int32)
+            out = self.torch.empty((self.nR, len(spec), self.nw, self.nw), dtype=self.torch.complex128,
+                                   device=self.ctx.device)
+            self.ctx.build_channels(self.cR, self.base, spec, out)
+            self._stacks[keys] = out
+        return self._stacks[keys]
+
+    def plan(self, keys, NKFFT):
+        keys = tuple(keys)
+        key = (keys, tuple(int(x) for x in NKFFT))
+        if key not in self._plans:
+            st = self.stack(keys)
+            nelem = st[0].numel()
+            plan = self.ctx.r2k_plan(self.iRvec, NKFFT, nelem)
+            box = self.torch.empty(plan.box_bytes // 16, dtype=self.torch.complex128, device=self.ctx.device)
+            aliased = _is_aliased(self.iRvec, NKFFT)
+            if not aliased:
+                plan.box(st, box)
+            self._plans[key] = (plan, box, aliased)
+        return self._plans[key]
+
+
+def _is_aliased(iRvec, NKFFT):
+    ext = iRvec.max(axis=0) - iRvec.min(axis=0) + 1
+    return bool(np.any(ext > np.array(NKFFT)))
+
+
+def device_system(system, ctx):
+    key = (ctx.device.index, ctx.stream.cuda_stream)
+    cache = system._device_cache
+    if key not in cache:
+        cache[key] = DeviceSystem(system, ctx)
+    return cache[key]
+
+
+class Data_K:
+    """K-block data on the GPU (data_K/data_K.py:49-334, data_K/data_K_R.py:8-97).
+
+    Parameters (beyond the reference's): `precision` = 'fp64' (default) or
+    'fp32' (complex64 k-space channels; energies, denominators and sums stay
+    FP64); `channels` = iterable of (name, der) computed together on first use.
+    """
+
+    accepts_channels = True
+
+    def __init__(self, system, dK=None, grid=None, Kpoint=None, k_list=None, Emin=-np.inf, Emax=np.inf,
+                 fftlib=None, precision="fp64", channels=None, device=0, random_gauge=False, **unused):
+        import torch
+        self.torch = torch
+        if not torch.cuda.is_available():
+            raise _lib.WbmiLibraryError("the MI355X path needs a GPU (torch.cuda is not available)")
+        if np.isfinite(Emin) or np.isfinite(Emax):
+            raise NotImplementedError("band selection by Emin/Emax is not supported")
+        if random_gauge:
+            raise NotImplementedError("random_gauge is a test-only option of the reference and is not supported")
+        if k_list is None and dK is None:
+            raise AssertionError("either dK or k_list must be specified, but not both")
+        self.system = system
+        self.ctx = _lib.get_context(device)
+        self.dev = device_system(system, self.ctx)
+        self.num_wann = system.num_wann
+        self.nbands = self.num_wann
+        self.cell_volume = system.cell_volume
+        self.force_internal_terms_only = system.force_internal_terms_only
+        self.grid = grid
+        self.Kpoint = Kpoint
+        self.k_list = None if k_list is None else np.array(k_list, dtype=float).reshape(-1, 3)
+        if self.k_list is None:
+            self.dK = np.array(dK, dtype=float)
+            self.NKFFT = np.array(grid.FFT, dtype=int)
+            self.nk = int(np.prod(self.NKFFT))
+        else:
+            self.dK = None
+            self.NKFFT = None
+            self.nk = len(self.k_list)
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(f"precision must be 'fp64' or 'fp32', not {precision}")
+        self.precision = precision
+        self.prec = WBMI_F64 if precision == "fp64" else WBMI_F32
+        self.cdtype = torch.complex128 if precision == "fp64" else torch.complex64
+        self._requested = set(channels) if channels else set()
+        self._slots = None
+        self.Xk = None
+        self.E_dev = None
+        self.U_dev = None
+
+    # -- evaluation -------------------------------------------------------------
+    def request(self, keys):
+        self._requested |= set(keys)
+
+    def ensure(self, keys):
+        """Make sure the channels `keys` are available (rotated to the band basis)."""
+        keys = set(keys) | {("Ham", 0)}
+        if self._slots is not None and keys <= set(self._slots):
+            return
+        self._requested |= keys
+        if self._slots is not None:
+            self._requested |= set(self._slots)
+        self._evaluate(sorted(self._requested, key=CHANNEL_ORDER.index))
+
+    def _evaluate(self, keys):
+        torch = self.torch
+        nw, nk = self.num_wann, self.nk
+        slots, s = {}, 0
+        for key in keys:
+            slots[key] = s
+            s += CHANNEL_NCOMP[key]
+        nch = s
+        stack = self.dev.stack(keys)
+        X = torch.empty((nk, nch, nw, nw), dtype=self.cdtype, device=self.ctx.device)
+        if self.k_list is None:
+            plan, box, aliased = self.dev.plan(keys, self.NKFFT)
+            work = torch.empty(plan.work_bytes, dtype=torch.uint8, device=self.ctx.device)
+            plan.execute(stack if aliased else None, box, self.dK, work, X, self.prec)
+            del work
+        else:
+            kl = torch.from_numpy(self.k_list).to(self.ctx.device)
+            self.ctx.r2k_klist(self.dev.iRvec_dev, stack, kl, X, self.prec)
+        E = torch.empty((nk, nw), dtype=torch.float64, device=self.ctx.device)
+        U = torch.empty((nk, nw, nw), dtype=self.cdtype, device=self.ctx.device)
+        h0 = slots[("Ham", 0)]
+        self.ctx.eigh(X[:, h0], nch * nw * nw, E, U, self.prec, nw)
+        herm = np.zeros(nch, dtype=np.int32)
+        skip = np.zeros(nch, dtype=np.int32)
+        for key, sl in slots.items():
+            n = CHANNEL_NCOMP[key]
+            if key[0] in HERMITIAN:
+                herm[sl:sl + n] = 1
+        skip[h0] = 1
+        self.ctx.rotate(U, X, herm, skip, self.prec)
+        self.Xk, self.E_dev, self.U_dev, self._slots, self.nch = X, E, U, slots, nch
+
+    def role_table(self):
+        """Channel slot per formula role (include/wbmi.h WBMI_ROLE_*), -1 if absent."""
+        ch = [-1] * _lib.NROLES
+        s = self._slots or {}
+
+        def put(role, key, n):
+            if key in s:
+                for i in range(n):
+                    ch[role + i] = s[key] + i
+        put(_lib.ROLE_V, ("Ham", 1), 3)
+        put(_lib.ROLE_W, ("Ham", 2), 9)
+        put(_lib.ROLE_A, ("AA", 0), 3)
+        put(_lib.ROLE_AD, ("AA", 1), 9)
+        put(_lib.ROLE_O, ("rotAA", 0), 3)
+        put(_lib.ROLE_OD, ("rotAA", 1), 9)
+        put(_lib.ROLE_B, ("BB", 0), 3)
+        put(_lib.ROLE_BD, ("BB", 1), 9)
+        put(_lib.ROLE_C, ("CC", 0), 3)
+        put(_lib.ROLE_CD, ("CC", 1), 9)
+        return ch
+
+    def static_scan(self, params, hist, weight=1.0):
+        """Accumulate weight * (group traces) into hist [nEF_extra, ncomp] (FP64, device)."""
+        for i, v in enumerate(self.role_table()):
+            params.ch[i] = v
+        self.ctx.static_scan(self.E_dev, self.Xk, self.prec, params, weight, hist)
+
+    # -- reference-style accessors (synchronising; for tests and users) ---------
+    @property
+    def E_K(self):
+        self.ensure(set())
+        return self.E_dev.cpu().numpy()
+
+    @property
+    def UU_K(self):
+        self.ensure(set())
+        return self.U_dev.cpu().numpy()
+
+    def Xbar(self, name, der=0):
+        """Rotated matrix [nk, nw, nw, 3^der...] as NumPy (data_K_R.py:69-89)."""
+        key = (name, der)
+        self.ensure({key})
+        if key == ("Ham", 0):
+            # U^+ H U = diag(E): the Hamiltonian channel itself is consumed by eigh
+            E = self.E_K
+            return E[:, :, None] * np.eye(self.num_wann)[None]
+        sl = self._slots[key]
+        n = CHANNEL_NCOMP[key]
+        x = self.Xk[:, sl:sl + n].cpu().numpy()
+        x = np.moveaxis(x, 1, -1)
+        if n == 9:
+            x = x.reshape(x.shape[:3] + (3, 3))
+        elif n == 1:
+            x = x[..., 0]
+        return x
+
+    @property
+    def kpoints_all(self):
+        if self.k_list is not None:
+            return self.k_list
+        return (self.grid.points_FFT + self.dK[None]) % 1

@@ -1,0 +1,15 @@
+"""Physical unit factors of the Berry-quantity calculators.
+
+Same definitions as wannierberri/factors.py:6-28 (SciPy CODATA constants), so
+that results agree with the reference to the last bit of the constants.
+"""
+from scipy.constants import elementary_charge, hbar, electron_mass, physical_constants, angstrom
+
+TAU_UNIT = 1E-15
+bohr_magneton = elementary_charge * hbar / (2 * electron_mass)
+bohr = physical_constants['Bohr radius'][0] / angstrom
+eV_au = physical_constants['electron volt-hartree relationship'][0]
+factor_morb_evA2_to_muB = -(elementary_charge * angstrom ** 2) * elementary_charge / (2 * hbar) / bohr_magneton
+factor_gme_orb = factor_morb_evA2_to_muB * bohr_magneton / angstrom ** 2
+factor_ahc = -(elementary_charge ** 2 / hbar / angstrom)
+factor_nlahc = elementary_charge ** 3 / hbar ** 2 * TAU_UNIT

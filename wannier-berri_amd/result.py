@@ -1,0 +1,215 @@
+"""Result containers: EnergyResult, ResultDict, VoidResult.
+
+Interface of wannierberri/result/{energyresult,resultdict,result}.py
+(`+`, `*` by a number, `mul_array`, `.max`, `savedata`, `from_npz`).  The
+data of a K-block result stays on the GPU (a torch tensor) so the k-sum over
+K-blocks never synchronises; `.data` materialises a NumPy array.
+"""
+import os
+
+import numpy as np
+
+
+def _is_tensor(x):
+    return hasattr(x, "device") and hasattr(x, "cpu") and not isinstance(x, np.ndarray)
+
+
+class Result:
+    def __radd__(self, other):
+        return self + other
+
+    def __rmul__(self, other):
+        return self * other
+
+
+class VoidResult(Result):
+    """Identically zero result (result/result.py:73-103)."""
+
+    def __add__(self, other):
+        return other
+
+    def __mul__(self, other):
+        return self
+
+    def __sub__(self, other):
+        return (-1) * other
+
+    @property
+    def max(self):
+        return np.zeros(3)
+
+    def savedata(self, name, prefix, suffix, i_iter):
+        pass
+
+
+class EnergyResult(Result):
+    """Data on a Fermi-level grid: data[iEF, 3, 3, ...] (result/energyresult.py:11-278).
+
+    Parameters follow the reference; `data` may be a NumPy array or a torch tensor.
+    """
+
+    def __init__(self, Energies=None, data=None, smoothers=None, transformTR=None, transformInv=None,
+                 rank=None, E_titles=("Efermi", "Omega"), comment="undocumented", save_mode="bin+txt"):
+        if not isinstance(Energies, (list, tuple)):
+            Energies = [Energies]
+        self.Energies = [np.asarray(E) for E in Energies]
+        self.N_energies = len(self.Energies)
+        E_titles = list(E_titles) if isinstance(E_titles, (list, tuple)) else [E_titles]
+        self.E_titles = (E_titles + ["???"] * self.N_energies)[:self.N_energies]
+        self._data = data
+        self._np = None
+        nd = len(data.shape)
+        self.rank = nd - self.N_energies if rank is None else rank
+        for i in range(self.N_energies):
+            assert self.Energies[i].shape[0] == data.shape[i], "Energy grid does not match the data"
+        self.smoothers = smoothers
+        self.transformTR = transformTR
+        self.transformInv = transformInv
+        self.comment = comment
+        self.save_mode = set(s for s in ("bin", "txt") if s in save_mode) if isinstance(save_mode, str) else set(save_mode)
+
+    # -- data access ----------------------------------------------------------
+    @property
+    def data(self):
+        if self._np is None:
+            d = self._data
+            self._np = d.detach().cpu().numpy() if _is_tensor(d) else np.asarray(d)
+        return self._np
+
+    @data.setter
+    def data(self, value):
+        self._data = value
+        self._np = None
+
+    @property
+    def raw(self):
+        """The underlying array (torch tensor on the GPU, or NumPy)."""
+        return self._data
+
+    def _new(self, data):
+        return EnergyResult(self.Energies, data, smoothers=self.smoothers, transformTR=self.transformTR,
+                            transformInv=self.transformInv, rank=self.rank, E_titles=self.E_titles,
+                            comment=self.comment, save_mode=self.save_mode)
+
+    # -- algebra (energyresult.py:131-200) ------------------------------------
+    def __mul__(self, number):
+        if isinstance(number, (int, float, np.floating, np.integer)):
+            return self._new(self._data * float(number))
+        raise TypeError("result can only be multiplied by a number")
+
+    def __truediv__(self, number):
+        return self * (1. / number)
+
+    def __add__(self, other):
+        if other is None or isinstance(other, VoidResult) or (isinstance(other, (int, float)) and other == 0):
+            return self
+        for i in range(self.N_energies):
+            if np.linalg.norm(self.Energies[i] - other.Energies[i]) > 1e-8:
+                raise RuntimeError(f"Adding results with different energies {i} ({self.E_titles[i]}) - not allowed")
+        a, b = self._data, other._data
+        if _is_tensor(a) != _is_tensor(b):
+            a, b = self.data, other.data
+        return self._new(a + b)
+
+    def __sub__(self, other):
+        return self + (-1) * other
+
+    def add(self, other):
+        self.data = self._data + other._data
+
+    def mul_array(self, other, axes=None):
+        other = np.asarray(other)
+        if isinstance(axes, int):
+            axes = (axes,)
+        if axes is None:
+            axes = tuple(range(other.ndim))
+        nd = len(self._data.shape)
+        reshape = tuple((self._data.shape[i] if i in axes else 1) for i in range(nd))
+        o = other.reshape(reshape)
+        if _is_tensor(self._data):
+            import torch
+            o = torch.as_tensor(o, dtype=self._data.dtype, device=self._data.device)
+        return self._new(self._data * o)
+
+    def swapaxes(self, a, b):
+        d = self._data
+        return self._new(d.transpose(a, b).contiguous() if _is_tensor(d) else d.swapaxes(a, b))
+
+    # -- refinement criteria (energyresult.py:250-264) ------------------------
+    @property
+    def dataSmooth(self):
+        return self.data
+
+    @property
+    def max(self):
+        d = self.dataSmooth
+        return np.array([np.abs(d).max(), np.linalg.norm(d), np.linalg.norm(d[1:] - d[:-1])])
+
+    # -- output (energyresult.py:202-248) -------------------------------------
+    def as_dict(self):
+        d = dict(E_titles=np.array(self.E_titles), data=self.data, rank=self.rank, comment=self.comment)
+        for i, E in enumerate(self.Energies):
+            d[f"Energies_{i}"] = E
+        return d
+
+    def save(self, name):
+        with open(name + ".npz", "wb") as f:
+            np.savez_compressed(f, **self.as_dict())
+
+    def savetxt(self, name):
+        data = self.data
+        head = "".join("#### " + s + "\n" for s in self.comment.split("\n"))
+        labels = ["".join("xyz"[i] for i in c) for c in np.ndindex(*([3] * self.rank))] if self.rank > 0 else ["value"]
+        head += "#" + "    ".join(f"{s:^15s}" for s in self.E_titles) + " " * 8 + \
+            "    ".join(f"{b:^15s}" for b in labels * 2) + "\n"
+        lines = []
+        flat = data.reshape(data.shape[0], -1)
+        for E, row in zip(self.Energies[0], flat):
+            vals = list(row) + list(row)
+            lines.append(f"{E:15.6e}    " + "    ".join(f"{x:15.6e}" for x in vals))
+        with open(name, "w") as f:
+            f.write(head + "\n".join(lines))
+
+    def savedata(self, name, prefix, suffix, i_iter):
+        suffix = "-" + suffix if len(suffix) > 0 else ""
+        prefix = prefix + "-" if len(prefix) > 0 else ""
+        filename = prefix + name + suffix + f"_iter-{i_iter:04d}"
+        if "bin" in self.save_mode:
+            self.save(filename)
+        if "txt" in self.save_mode:
+            self.savetxt(filename + ".dat")
+
+    @classmethod
+    def from_npz(cls, file_npz):
+        with np.load(file_npz, allow_pickle=False) as f:
+            E = [f[k] for k in sorted(f.files) if k.startswith("Energies_")]
+            return cls(Energies=E, data=f["data"], rank=int(f["rank"]) if "rank" in f.files else None)
+
+
+class ResultDict(Result):
+    """Dictionary of results (result/resultdict.py:18-66)."""
+
+    def __init__(self, results, save_mode="bin+txt"):
+        self.results = results
+
+    def __mul__(self, number):
+        return ResultDict({k: v * number for k, v in self.results.items()})
+
+    def __truediv__(self, number):
+        return ResultDict({k: v / number for k, v in self.results.items()})
+
+    def __add__(self, other):
+        if other is None or (isinstance(other, (int, float)) and other == 0):
+            return self
+        return ResultDict({k: self.results[k] + other.results[k] for k in self.results if k in other.results})
+
+    def __sub__(self, other):
+        return self + (-1) * other
+
+    def savedata(self, prefix, suffix, i_iter):
+        for k, v in self.results.items():
+            v.savedata(k, prefix, suffix, i_iter)
+
+    @property
+    def max(self):
+        return np.array([x for v in self.results.values() for x in v.max])

@@ -1,0 +1,96 @@
+"""run(): integrate calculators over a Grid, K-block by K-block, on the MI355X.
+
+Mirrors wannierberri/run_grid.py:32-403 for the serial, symmetry-free case:
+the K-list of the Grid, one data_k_class per K-block, the k-sum
+`sum_K result * Kp.factor`, optional adaptive refinement
+(run_grid.py:370-389, Kpoint.divide) and per-iteration output files
+`{fout_name}-{calc}{-suffix}_iter-{i:04d}.npz/.dat`.
+"""
+import time
+from collections.abc import Iterable
+
+import numpy as np
+
+from .data_k import Data_K
+from .result import ResultDict
+
+
+def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetrize=True, fout_name="result",
+        suffix="", parameters_K=None, file_Klist_path=None, restart=False, allow_restart=False, dump_results=False,
+        restart_iteration=-1, Klist_part=10, parallel=True, print_Kpoints=False, adpt_mesh=2, adpt_fac=1,
+        print_progress_step_time=5, print_progress_step_percent=1, data_k_class=None, k_batch=50,
+        save_files=True, verbose=True):
+    """Integrate `calculators` (dict name -> calculator) over `grid`.
+
+    The system carries the identity point group, so `use_irred_kpt` and
+    `symmetrize` do not change the K-list or the result.  `parallel` is
+    accepted for compatibility: one process drives one GPU.
+    """
+    if restart or dump_results:
+        raise NotImplementedError("restart / dump_results are not available on the MI355X path")
+    if data_k_class is None:
+        data_k_class = Data_K
+    parameters_K = dict(parameters_K or {})
+    for key, calc in calculators.items():
+        if not calc.allow_grid:
+            raise ValueError(f"Calculation on Grid is running, but calculator `{key}` is not compatible with a Grid")
+    if getattr(data_k_class, "accepts_channels", False):
+        need = set()
+        for calc in calculators.values():
+            if hasattr(calc, "needed_channels"):
+                need |= set(calc.needed_channels(system))
+        parameters_K.setdefault("channels", need)
+
+    K_list = grid.get_K_list(use_symmetry=use_irred_kpt)
+    if adpt_num_iter < 0:
+        adpt_num_iter = -adpt_num_iter * np.prod(grid.div) / np.prod(adpt_mesh) / adpt_fac / 3
+    adpt_num_iter = int(round(adpt_num_iter))
+    if (adpt_mesh is None) or np.max(adpt_mesh) <= 1:
+        adpt_num_iter = 0
+    else:
+        if not isinstance(adpt_mesh, Iterable):
+            adpt_mesh = [adpt_mesh] * 3
+        adpt_mesh = np.array(adpt_mesh)
+
+    result_all = None
+    factors = np.array([Kp.factor for Kp in K_list])
+    t0 = time.time()
+    counter = 0
+    for i_iter in range(adpt_num_iter + 1):
+        result_iter = None
+        for Kp in K_list:
+            if Kp.was_evaluated_flag:
+                continue
+            data = data_k_class(system, dK=Kp.Kp_fullBZ, grid=grid, Kpoint=Kp, **parameters_K)
+            res = ResultDict({k: v(data) for k, v in calculators.items()})
+            del data
+            res = system.pointgroup.symmetrize(res) if symmetrize else res
+            if adpt_num_iter > 0:
+                Kp.set_result(res)
+            else:
+                Kp.was_evaluated_flag = True
+            contrib = res * Kp.factor
+            result_iter = contrib if result_iter is None else result_iter + contrib
+            counter += 1
+        if result_all is None:
+            result_all = result_iter
+        else:
+            factors_old = factors
+            factors = np.array([kp.factor for kp in K_list])
+            diff = factors[:len(factors_old)] - factors_old
+            if result_iter is not None:
+                result_all = result_all + result_iter
+            for i, fac in enumerate(diff):
+                if abs(fac) > 1e-8:
+                    result_all = result_all + K_list[i].get_result() * fac
+        if save_files:
+            result_all.savedata(prefix=fout_name, suffix=suffix, i_iter=i_iter)
+        if i_iter >= adpt_num_iter:
+            break
+        Kmax = np.array([K.max for K in K_list]).T
+        select_points = set().union(*(np.argsort(Km)[-adpt_fac:] for Km in Kmax))
+        for iK in select_points:
+            K_list += K_list[iK].divide(ndiv=adpt_mesh, periodic=system.periodic)
+    if verbose:
+        print(f"run(): processed {counter} K-blocks in {time.time() - t0:.3f} s", flush=True)
+    return result_all

@@ -1,0 +1,206 @@
+"""System_R: an R-space Wannier / tight-binding model for the MI355X path.
+
+Keeps the reference's input semantics (wannierberri/system/system_R.py):
+matrices `_XX_R[key]` of shape `[nR, nw, nw, 3^m]`, `rvec.iRvec`, Wannier
+centres used as the derivative shifts (system_R.py:776-780), `periodic`,
+`force_internal_terms_only`, and the directory-of-npz layout of
+`System_R.from_npz` (system_R.py:713-799, legacy `[m, n, R]` transposed at
+795-796).  Device copies of the R-space data are made once per system and
+device and cached here.
+"""
+import glob
+import os
+
+import numpy as np
+
+
+class _Rvec:
+    """Minimal `Rvectors` view: iRvec, nRvec, iR0 (fourier/rvectors.py:385-434)."""
+
+    def __init__(self, iRvec):
+        self.iRvec = np.array(iRvec, dtype=int)
+
+    @property
+    def nRvec(self):
+        return self.iRvec.shape[0]
+
+    @property
+    def iR0(self):
+        return self.iRvec.tolist().index([0, 0, 0])
+
+    def iR(self, R):
+        return self.iRvec.tolist().index(np.array(np.round(R), dtype=int).tolist())
+
+
+class _IdentityPointGroup:
+    """The trivial point group: symmetrize() is the identity (point_symmetry.py:337-338)."""
+
+    def __init__(self, real_lattice):
+        self.real_lattice = real_lattice
+        self.recip_lattice = 2 * np.pi * np.linalg.inv(real_lattice).T
+
+    def symmetrize(self, result):
+        return result
+
+    def symmetric_grid(self, nk):
+        return True
+
+    def star(self, K):
+        return np.array([K])
+
+
+class System_R:
+    """R-space model (system/system_R.py:18-175)."""
+
+    def __init__(self, real_lattice=None, iRvec=None, wannier_centers_cart=None, matrices=None,
+                 periodic=(True, True, True), force_internal_terms_only=False, name="wberri", NKFFT=None):
+        self.name = name
+        self._XX_R = {}
+        self.force_internal_terms_only = force_internal_terms_only
+        self.periodic = np.zeros(3, dtype=bool)
+        per = np.array(periodic, dtype=bool).ravel()
+        self.periodic[:len(per)] = per
+        self.is_phonon = False
+        if NKFFT is not None:
+            self._NKFFT_recommended = np.array(NKFFT)
+        self._device_cache = {}
+        if real_lattice is not None:
+            self.real_lattice = np.array(real_lattice, dtype=float)
+            self.pointgroup = _IdentityPointGroup(self.real_lattice)
+        if iRvec is not None:
+            self.rvec = _Rvec(iRvec)
+        if matrices:
+            for k, v in matrices.items():
+                self.set_R_mat(k, v)
+        if wannier_centers_cart is not None:
+            self.wannier_centers_cart = np.array(wannier_centers_cart, dtype=float)
+        elif self.has_R_mat("Ham"):
+            self.wannier_centers_cart = np.zeros((self.num_wann, 3))
+
+    # -- matrices -------------------------------------------------------------
+    def get_R_mat(self, key):
+        try:
+            return self._XX_R[key]
+        except KeyError:
+            raise ValueError(f"The real-space matrix elements '{key}' are not set in the system, but are "
+                             "required for the current calculation")
+
+    def has_R_mat(self, key):
+        return key in self._XX_R
+
+    def set_R_mat(self, key, value, reset=False, Hermitian=False):
+        """Set a full `[nR, nw, nw, ...]` matrix (system_R.py:117-175, R=None branch)."""
+        value = np.array(value, dtype=complex)
+        if hasattr(self, "rvec"):
+            assert value.shape[0] == self.rvec.nRvec, f"the 0th dimension should be nR={self.rvec.nRvec}"
+        if Hermitian:
+            value = 0.5 * (value + self.conj_XX_R(value))
+        if key in self._XX_R and not reset:
+            raise RuntimeError(f"setting {key} for the second time without explicit permission")
+        self._XX_R[key] = value
+        self._device_cache.clear()
+
+    def conj_XX_R(self, X):
+        """X(-R)^dagger (fourier/rvectors.py:424-430)."""
+        index = {tuple(R): i for i, R in enumerate(self.rvec.iRvec)}
+        out = np.zeros_like(X)
+        for i, R in enumerate(self.rvec.iRvec):
+            j = index.get(tuple(-R))
+            if j is not None:
+                out[i] = X[j]
+        return out.swapaxes(1, 2).conj()
+
+    # -- geometry -------------------------------------------------------------
+    @property
+    def num_wann(self):
+        return self._XX_R["Ham"].shape[1]
+
+    @property
+    def nRvec(self):
+        return self.rvec.nRvec
+
+    @property
+    def cell_volume(self):
+        return abs(np.linalg.det(self.real_lattice))   # system/system.py:173-181
+
+    @property
+    def recip_lattice(self):
+        return 2 * np.pi * np.linalg.inv(self.real_lattice).T
+
+    @property
+    def wannier_centers_red(self):
+        return self.wannier_centers_cart.dot(np.linalg.inv(self.real_lattice))   # system_R.py:608-610
+
+    @property
+    def cRvec_shifted(self):
+        """R + t_n - t_m for element (m, n), shape [nR, nw, nw, 3] (rvectors.py:351-369)."""
+        shifts = self.wannier_centers_red.dot(self.real_lattice)
+        cR = self.rvec.iRvec.dot(self.real_lattice)
+        return cR[:, None, None, :] + (-shifts[:, None, :] + shifts[None, :, :])[None]
+
+    @property
+    def NKFFT_recommended(self):
+        """Smallest FFT grid on which the R-vectors do not overlap (rvectors.py:280-292)."""
+        if hasattr(self, "_NKFFT_recommended"):
+            return self._NKFFT_recommended
+        nk = np.ones(3, dtype=int)
+        for i in range(3):
+            R = self.rvec.iRvec[:, i]
+            if (R > 0).any():
+                nk[i] += R.max()
+            if (R < 0).any():
+                nk[i] -= R.min()
+        return nk
+
+    # -- loaders --------------------------------------------------------------
+    @classmethod
+    def from_npz(cls, path, legacy=False, matrices=None, force_internal_terms_only=False):
+        """Directory of `.npz` files as written by System_R.to_npz (system_R.py:713-799).
+
+        Matrix files are `_XX_R_<key>.npz` (or `<key>.npz` in older fixtures).
+        """
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"directory {path} does not exist")
+
+        def arr(name):
+            with np.load(os.path.join(path, name + ".npz"), allow_pickle=False) as f:
+                return f["arr_0"]
+        real_lattice = arr("real_lattice")
+        iRvec = arr("iRvec")
+        if os.path.exists(os.path.join(path, "wannier_centers_cart.npz")):
+            wcc = arr("wannier_centers_cart")
+        elif os.path.exists(os.path.join(path, "wannier_centers_red.npz")):
+            wcc = arr("wannier_centers_red").dot(real_lattice)
+        else:
+            wcc = None
+        periodic = arr("periodic") if os.path.exists(os.path.join(path, "periodic.npz")) else (True, True, True)
+        if matrices is None:
+            names = [os.path.splitext(os.path.basename(f))[0] for f in glob.glob(os.path.join(path, "*.npz"))]
+            matrices = sorted(set(n[6:] for n in names if n.startswith("_XX_R_")) |
+                              set(n for n in names if n in ("Ham", "AA", "BB", "CC", "OO", "SS")))
+        mats = {}
+        for key in matrices:
+            fn = "_XX_R_" + key if os.path.exists(os.path.join(path, "_XX_R_" + key + ".npz")) else key
+            a = arr(fn)
+            if legacy:
+                a = np.transpose(a, (2, 0, 1) + tuple(range(3, a.ndim)))
+            mats[key] = a
+        return cls(real_lattice, iRvec, wcc, mats, periodic=periodic,
+                   force_internal_terms_only=force_internal_terms_only)
+
+    @classmethod
+    def from_arrays_npz(cls, file_npz):
+        """Single `.npz` with keys real_lattice, iRvec, wannier_centers_cart, Ham, AA, ... (`[R, m, n, ...]`)."""
+        with np.load(file_npz, allow_pickle=False) as f:
+            mats = {k: f[k] for k in f.files if k in ("Ham", "AA", "BB", "CC", "OO", "SS")}
+            periodic = f["periodic"] if "periodic" in f.files else (True, True, True)
+            fito = bool(f["force_internal_terms_only"]) if "force_internal_terms_only" in f.files else False
+            return cls(f["real_lattice"], f["iRvec"], f["wannier_centers_cart"], mats, periodic=periodic,
+                       force_internal_terms_only=fito)
+
+    def to_arrays_npz(self, file_npz):
+        d = dict(real_lattice=self.real_lattice, iRvec=self.rvec.iRvec.astype(np.int32),
+                 wannier_centers_cart=self.wannier_centers_cart, periodic=self.periodic,
+                 force_internal_terms_only=self.force_internal_terms_only)
+        d.update(self._XX_R)
+        np.savez_compressed(file_npz, **d)
