@@ -1,0 +1,226 @@
+"""Benchmark: k-points/s of AHC + Berry-curvature dipole (Fermi sea) at nw=32 on one MI355X.
+
+Workload (the bcc nw=32 synthetic model): num_wann=32, bcc lattice a=3 A,
+R in {-4..4}^3 (729) with Wigner-Seitz weights, seed 2; AHC + BCD_sea with
+external terms (AA_R present) on a 96^3 grid (NKFFT 16^3 x NKdiv 6^3) at 2001
+Fermi levels spanning the bands, linspace(-22, 22, 2001) eV.
+
+A step is one full run() over the 96^3 grid: R->k, eigh, U^+XU, formula
+traces, Fermi scan and k-sum of all 216 K-blocks.  The FP64 form is timed;
+the complex64-channel form (FP64 energies, denominators and k-sum) is timed
+too and its error vs the FP64 form reported.
+
+Prints one JSON line (rank 0).  `--gpus N` with N > 1 exits: this build runs
+on one GPU.
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np
+
+NW, RMAX, SEED = 32, 4, 2
+NK = 96
+EFERMI = (-22.0, 22.0, 2001)
+# algorithmic work per k-point (AHC + BCD, external terms, nw=32, 37 channels)
+FLOPS_ROTATE_PER_K_PER_CH = 2 * 8 * NW ** 3          # U^+ (X U): two complex nw^3 products
+FLOPS_SCAN_BCD_PER_K = 18 * 8 * NW ** 3               # DerDcov V.D and D.V products (9 + 9)
+FLOPS_SCAN_AHC_PER_K = 3 * 3 * 8 * NW ** 2            # Omega: 3 comps x 3 terms x nw^2 cmacs
+FLOPS_EIGH_PER_K = int((16 / 3 + 8) * NW ** 3)        # LAPACK-equivalent Hermitian eigensolver
+PEAK_HBM = 8000.0        # GB/s
+PEAK_FP64 = 78.6         # TFLOP/s (dense MFMA and vector FP64)
+PEAK_FP32 = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_system():
+    from wannierberri_amd import models
+    return models.synthetic(num_wann=NW, Rmax=RMAX, seed=SEED)
+
+
+def calculators(wb):
+    Ef = np.linspace(*EFERMI)
+    st = wb.calculators.static
+    return {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef)}
+
+
+# -- CPU baseline: the FP64 oracle, one worker process per host core ------------
+
+def _cpu_worker(args):
+    dK, nkfft = args
+    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+    from oracle import wb_oracle as O
+    s = make_system()
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {"Ham": s.get_R_mat("Ham"), "AA": s.get_R_mat("AA")})
+    Ef = np.linspace(*EFERMI)
+    t = time.time()
+    d = O.OracleDataK(o, dK=dK, NKFFT=nkfft)
+    ahc = O.AHC(Ef)(d).data
+    bcd = O.BerryDipole_FermiSea(Ef)(d).data
+    return time.time() - t, d.nk, ahc, bcd
+
+
+def cpu_baseline(nproc, nkfft, blocks_per_proc=1):
+    rng = np.random.default_rng(7)
+    dKs = [rng.uniform(0, 1, 3) / np.array(nkfft) for _ in range(nproc * blocks_per_proc)]
+    ctx = mp.get_context("spawn")
+    t = time.time()
+    with ctx.Pool(nproc) as pool:
+        out = pool.map(_cpu_worker, [(dk, nkfft) for dk in dKs])
+    wall = time.time() - t
+    nk = sum(o[1] for o in out)
+    return dict(wall=wall, nk=nk, kps=nk / wall, dKs=dKs, ahc=[o[2] for o in out], bcd=[o[3] for o in out])
+
+
+# -- GPU -------------------------------------------------------------------------
+
+def gpu_step(wb, system, grid, calcs, precision):
+    return wb.run(system, grid, calcs, parameters_K={"precision": precision}, save_files=False, verbose=False)
+
+
+def stage_stats(timer):
+    """{stage: (calls, total_ms, [units])} from the event timer."""
+    out = {}
+    for stage, lst in timer.items():
+        ms = [e0.elapsed_time(e1) for e0, e1, _ in lst]
+        out[stage] = (len(ms), float(np.sum(ms)), [u for _, _, u in lst])
+    return out
+
+
+def roofline(stage, calls, total_ms, units, prec):
+    avg_s = total_ms / calls / 1e3
+    peak_f = PEAK_FP64 if prec == "fp64" else PEAK_FP32
+    if stage == "rotate":
+        nk, nw, nrot = units[0]
+        work = nk * nrot * FLOPS_ROTATE_PER_K_PER_CH
+        return dict(bound="mfma", achieved=work / avg_s / 1e12, peak=peak_f, unit="TFLOP/s")
+    if stage.startswith("scan_f2"):
+        nk = units[0][0]
+        return dict(bound="mfma", achieved=nk * FLOPS_SCAN_BCD_PER_K / avg_s / 1e12, peak=PEAK_FP64, unit="TFLOP/s")
+    if stage.startswith("scan_f1"):
+        nk = units[0][0]
+        return dict(bound="mfma", achieved=nk * FLOPS_SCAN_AHC_PER_K / avg_s / 1e12, peak=PEAK_FP64, unit="TFLOP/s")
+    if stage == "eigh":
+        nk = units[0][0]
+        return dict(bound="mfma", achieved=nk * FLOPS_EIGH_PER_K / avg_s / 1e12, peak=peak_f, unit="TFLOP/s")
+    if stage == "r2k":
+        nk, nelem, box_bytes, work_bytes = units[0]
+        esz = 16 if prec == "fp64" else 8
+        # algorithmic: read the R-space box once, write the k-space channels once
+        byt = box_bytes + nk * nelem * esz
+        return dict(bound="hbm", achieved=byt / avg_s / 1e9, peak=PEAK_HBM, unit="GB/s")
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--nkfft", type=int, default=16)
+    ap.add_argument("--precisions", default="fp64,fp32")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="0 = all host cores (up to 16)")
+    ap.add_argument("--cpu-nkfft", default="6,6,4")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import wannierberri_amd as wb
+    from wannierberri_amd import _lib
+
+    system = make_system()
+    nkfft = [args.nkfft] * 3
+    assert NK % args.nkfft == 0
+    grid = wb.Grid(system=system, NKFFT=nkfft, NKdiv=[NK // args.nkfft] * 3)
+    calcs = calculators(wb)
+    nk_total = NK ** 3
+    ctx = _lib.get_context(0)
+    precs = args.precisions.split(",")
+    res = {}
+    for prec in precs:
+        log(f"[bench] {prec}: warmup {args.warmup}")
+        for _ in range(args.warmup):
+            r = gpu_step(wb, system, grid, calcs, prec)
+            torch.cuda.synchronize()
+        ctx.timer = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            r = gpu_step(wb, system, grid, calcs, prec)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        timer, ctx.timer = ctx.timer, None
+        stats = stage_stats(timer)
+        res[prec] = dict(ms=dt * 1e3, kps=nk_total / dt, stats=stats,
+                         ahc=r.results["ahc"].data.copy(), bcd=r.results["bcd"].data.copy())
+        log(f"[bench] {prec}: {dt * 1e3:.1f} ms/step, {nk_total / dt:.4g} k-pts/s")
+        for st, (n, ms, _) in sorted(stats.items(), key=lambda x: -x[1][1]):
+            log(f"    {st:12s} calls/step={n // args.steps:5d} ms/step={ms / args.steps:9.2f} avg={ms / n:.3f} ms")
+
+    main_prec = "fp64" if "fp64" in res else precs[0]
+    m = res[main_prec]
+    dom = max(m["stats"].items(), key=lambda x: x[1][1])
+    rl = roofline(dom[0], dom[1][0], dom[1][1], dom[1][2], main_prec)
+    rl["frac"] = rl["achieved"] / rl["peak"]
+    rl["kernel"] = dom[0]
+    rl["traffic"] = None
+
+    out = {
+        "metric": "k-pts/s AHC+BCD at nw=32: mixed-prec vs FP64 on 1 MI355X, rel err<=1e-6",
+        "value": m["kps"], "unit": "k-points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": m["ms"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if main_prec == "fp64" else "c64+f64", "data": "synthetic",
+        "config": {"workload": f"bcc nw={NW} synthetic (seed {SEED}), R cube |R_i|<={RMAX} (729), AHC+BCD_sea, "
+                               f"{NK}^3 grid as NKFFT {args.nkfft}^3 x NKdiv {NK // args.nkfft}^3, "
+                               f"{EFERMI[2]} Fermi levels in [{EFERMI[0]}, {EFERMI[1]}] eV",
+                   "nk": nk_total, "num_wann": NW, "nR": system.nRvec, "precision": main_prec},
+        "roofline": rl,
+    }
+    if "fp64" in res and "fp32" in res:
+        ref = res["fp64"]
+        errs = {q: float(np.abs(res["fp32"][q] - ref[q]).max() / np.abs(ref[q]).max()) for q in ("ahc", "bcd")}
+        out["mixed"] = {"value": res["fp32"]["kps"], "ms_per_step": res["fp32"]["ms"],
+                        "speedup_vs_fp64": res["fp32"]["kps"] / ref["kps"],
+                        "rel_err_vs_fp64_gpu": errs, "meets_1e-6": all(e <= 1e-6 for e in errs.values())}
+
+    if not args.no_cpu:
+        nproc = args.cpu_procs or min(16, os.cpu_count() or 1)
+        cnk = [int(x) for x in args.cpu_nkfft.split(",")]
+        log(f"[bench] cpu baseline: oracle on {nproc} processes, one {cnk} K-block each")
+        cb = cpu_baseline(nproc, cnk)
+        # GPU on the same K-blocks: error of each GPU form vs the FP64 oracle
+        g = wb.Grid(system=system, NKFFT=cnk, NKdiv=[1, 1, 1])
+        err = {}
+        for prec in res:
+            ea, eb = 0.0, 0.0
+            for dK, a, b in zip(cb["dKs"], cb["ahc"], cb["bcd"]):
+                d = wb.Data_K(system, dK=dK, grid=g, precision=prec)
+                ra = calcs["ahc"](d).data
+                rb = calcs["bcd"](d).data
+                ea = max(ea, np.abs(ra - a).max() / np.abs(a).max())
+                eb = max(eb, np.abs(rb - b).max() / np.abs(b).max())
+            err[prec] = {"ahc": ea, "bcd": eb}
+        out["rel_err_vs_oracle_on_cpu_sample"] = err
+        out["cpu_baseline"] = {"value": cb["kps"], "unit": "k-points/s", "cores": nproc, "kind": "port",
+                               "sample": f"{nproc} K-blocks of {cnk[0]}x{cnk[1]}x{cnk[2]} k-points (same model, "
+                                         f"AHC+BCD, 2001 levels), FP64 NumPy oracle, one process per core, "
+                                         f"OPENBLAS_NUM_THREADS=1, wall {cb['wall']:.1f} s"}
+        out["gpu_over_cpu"] = m["kps"] / cb["kps"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -128,6 +128,7 @@ class Context:
         if rc != WBMI_OK:
             raise WbmiError(f"wbmi_ctx_create failed with code {rc} (device {device}; gfx950 required)")
         self.h = h
+        self.timer = None   # optional {stage: [(start_event, end_event, units)]}
 
     def __del__(self):
         try:
@@ -135,6 +136,19 @@ class Context:
                 _lib.wbmi_ctx_destroy(self.h)
         except Exception:
             pass
+
+    def _timed(self, stage, units, fn, *args):
+        """Run a library call; with a timer set, bracket it with events on this stream."""
+        if self.timer is None:
+            return fn(*args)
+        torch = self.torch
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(self.stream)
+        rc = fn(*args)
+        e1.record(self.stream)
+        self.timer.setdefault(stage, []).append((e0, e1, units))
+        return rc
 
     def check(self, rc, what):
         if rc != WBMI_OK:
@@ -161,24 +175,28 @@ class Context:
     def r2k_klist(self, iRvec_dev, XR, klist_dev, out, prec):
         nR = iRvec_dev.shape[0]
         nelem = XR[0].numel()
-        self.check(lib().wbmi_r2k_klist(self.h, nR, _ptr(iRvec_dev), nelem, _ptr(XR), klist_dev.shape[0],
-                                        _ptr(klist_dev), _ptr(out), prec), "wbmi_r2k_klist")
+        self.check(self._timed("r2k_klist", (klist_dev.shape[0], nelem), lib().wbmi_r2k_klist, self.h, nR,
+                               _ptr(iRvec_dev), nelem, _ptr(XR), klist_dev.shape[0], _ptr(klist_dev), _ptr(out),
+                               prec), "wbmi_r2k_klist")
 
     def eigh(self, H, Hstride, E, U, prec, nw):
-        self.check(lib().wbmi_eigh(self.h, E.shape[0], nw, _ptr(H), Hstride, _ptr(E), _ptr(U), prec, 0, None),
-                   "wbmi_eigh")
+        self.check(self._timed("eigh", (E.shape[0], nw), lib().wbmi_eigh, self.h, E.shape[0], nw, _ptr(H), Hstride,
+                               _ptr(E), _ptr(U), prec, 0, None), "wbmi_eigh")
 
     def rotate(self, U, X, herm, skip, prec):
         nk, nch, nw = X.shape[0], X.shape[1], X.shape[2]
         h_a, h_p = _host_i32(herm)
         s_a, s_p = _host_i32(skip)
-        self.check(lib().wbmi_rotate(self.h, nk, nw, nch, _ptr(U), _ptr(X), h_p, s_p, prec), "wbmi_rotate")
+        nrot = int(nch - np.count_nonzero(s_a))
+        self.check(self._timed("rotate", (nk, nw, nrot), lib().wbmi_rotate, self.h, nk, nw, nch, _ptr(U), _ptr(X),
+                               h_p, s_p, prec), "wbmi_rotate")
 
     def static_scan(self, E, X, prec, params, weight, hist):
         nk, nw = E.shape
         nch = X.shape[1] if X is not None else 1
-        self.check(lib().wbmi_static_scan(self.h, nk, nw, nch, _ptr(E), _ptr(X), prec, ctypes.byref(params),
-                                          float(weight), _ptr(hist)), "wbmi_static_scan")
+        self.check(self._timed(f"scan_f{params.formula}", (nk, nw), lib().wbmi_static_scan, self.h, nk, nw, nch,
+                               _ptr(E), _ptr(X), prec, ctypes.byref(params), float(weight), _ptr(hist)),
+                   "wbmi_static_scan")
 
     def finalize(self, hist, fder, dEF, scale, out):
         nEF, ncomp = hist.shape
@@ -212,8 +230,9 @@ class R2KPlan:
 
     def execute(self, XR, box, dK, work, out, prec):
         dk = (ctypes.c_double * 3)(*[float(x) for x in dK])
-        self.ctx.check(lib().wbmi_r2k_exec(self.ctx.h, self.h, _ptr(XR), _ptr(box), dk, _ptr(work), _ptr(out),
-                                           prec), "wbmi_r2k_exec")
+        self.ctx.check(self.ctx._timed("r2k", (out.shape[0], out[0].numel(), self.box_bytes, self.work_bytes),
+                                       lib().wbmi_r2k_exec, self.ctx.h, self.h, _ptr(XR), _ptr(box), dk, _ptr(work),
+                                       _ptr(out), prec), "wbmi_r2k_exec")
 
 
 _contexts = {}
