@@ -132,18 +132,24 @@ def test_eigh(wb, ctx, nw):
     assert np.abs(np.einsum("kba,kbc->kac", U.conj(), U) - np.eye(nw)[None]).max() < 1e-12
 
 
-def test_rotate(wb, ctx):
+@pytest.mark.parametrize("nw,prec", [(13, 0), (32, 0), (64, 0), (32, 1), (64, 1)])
+def test_rotate(wb, ctx, nw, prec):
     import torch
-    say("test_rotate")
+    say(f"test_rotate nw={nw} prec={prec}")
     rng = np.random.default_rng(5)
-    nk, nch, nw = 7, 5, 13
+    nk, nch = 7, 5
     X = rng.standard_normal((nk, nch, nw, nw)) + 1j * rng.standard_normal((nk, nch, nw, nw))
     U = np.linalg.qr(rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw)))[0]
     herm = np.array([0, 1, 0, 1, 0], dtype=np.int32)
     skip = np.array([0, 0, 1, 0, 0], dtype=np.int32)
-    Xt = torch.from_numpy(X.copy()).cuda()
-    ctx.rotate(torch.from_numpy(U).cuda(), Xt, herm, skip, 0)
-    got = Xt.cpu().numpy()
+    cdt = torch.complex128 if prec == 0 else torch.complex64
+    if prec == 1:
+        X = X.astype(np.complex64).astype(np.complex128)
+        U = U.astype(np.complex64).astype(np.complex128)
+    Xt = torch.from_numpy(X.copy()).to(cdt).cuda()
+    ctx.rotate(torch.from_numpy(U).to(cdt).cuda(), Xt, herm, skip, prec)
+    got = Xt.cpu().numpy().astype(np.complex128)
+    tol = 1e-12 if prec == 0 else 1e-5
     for c in range(nch):
         x = X[:, c]
         if skip[c]:
@@ -152,7 +158,7 @@ def test_rotate(wb, ctx):
             if herm[c]:
                 x = 0.5 * (x + x.conj().swapaxes(1, 2))
             ref = np.einsum("kba,kbc,kcd->kad", U.conj(), x, U)
-        assert rel_err(got[:, c], ref) < 1e-12, c
+        assert rel_err(got[:, c], ref) < tol, c
 
 
 # -- end to end against the reference fixtures ----------------------------------
@@ -251,3 +257,22 @@ def test_synthetic_fp32(wb, small_synth):
         e = rel_err(r.results[k].data, ro[k].data)
         say(f"  fp32 {k}: rel err {e:.2e}")
         assert e < 1e-3, k
+
+
+def test_synthetic_nw40_global_operands(wb):
+    """nw > 32: the formula kernel reads V and A from HBM and uses 4x4 product tiles."""
+    say("test_synthetic_nw40_global_operands")
+    from wannierberri_amd import models
+    st = wb.calculators.static
+    s = models.synthetic(num_wann=40, Rmax=1, seed=13)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA")})
+    Ef = np.linspace(-8, 8, 33)
+    NK, NKFFT = [3, 3, 3], [3, 3, 3]
+    calcs = {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef)}
+    r = run_gpu(wb, s, calcs, NK, NKFFT)
+    ro = O.run(o, {"ahc": O.AHC(Ef), "bcd": O.BerryDipole_FermiSea(Ef)}, NK=NK, NKFFT=NKFFT)
+    for k in calcs:
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  {k}: rel err {e:.2e}")
+        assert e < 1e-8, k

@@ -37,9 +37,10 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<T>* __res
                                                     int max_sweeps) {
     extern __shared__ char smem[];
     const int n = nw + (nw & 1);  // padded to even; the dummy index is decoupled
+    const int ld = n + 1;         // LDS row stride (one element of padding against bank conflicts)
     cplx<T>* A = (cplx<T>*)smem;
-    cplx<T>* V = A + n * n;
-    T* prm = (T*)(V + n * n);  // per pair: c, s*cos(phi), s*sin(phi)
+    cplx<T>* V = A + n * ld;
+    T* prm = (T*)(V + n * ld);  // per pair: c, s*cos(phi), s*sin(phi)
     int* flag = (int*)(prm + 3 * (n / 2));
     const int tid = threadIdx.x;
     const int64_t k = blockIdx.x;
@@ -52,8 +53,8 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<T>* __res
             cplx<T> x = Hk[i * nw + j], y = Hk[j * nw + i];
             a = mk((T)0.5 * (x.re + y.re), (T)0.5 * (x.im - y.im));
         }
-        A[t] = a;
-        V[t] = mk((T)(i == j ? 1 : 0), (T)0);
+        A[i * ld + j] = a;
+        V[i * ld + j] = mk((T)(i == j ? 1 : 0), (T)0);
     }
     __syncthreads();
 
@@ -65,9 +66,9 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<T>* __res
             if (tid < n / 2) {
                 int p, q;
                 rr_pair(n, r, tid, p, q);
-                cplx<T> apq = A[p * n + q];
+                cplx<T> apq = A[p * ld + q];
                 T g = sqrt(apq.re * apq.re + apq.im * apq.im);
-                T app = A[p * n + p].re, aqq = A[q * n + q].re;
+                T app = A[p * ld + p].re, aqq = A[q * ld + q].re;
                 T c = 1, sc = 0, ss = 0;
                 T scale = fabs(app) > fabs(aqq) ? fabs(app) : fabs(aqq);
                 if (g > (T)0.5 * eps * scale && g > (T)(sizeof(T) == 8 ? 1e-300 : 1e-36)) {
@@ -92,12 +93,12 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<T>* __res
                 T c = prm[3 * pr];
                 cplx<T> se = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
                 if (se.re == 0 && se.im == 0) continue;
-                cplx<T> ap = A[i * n + p], aq = A[i * n + q];
-                A[i * n + p] = c * ap - mulc(aq, se);
-                A[i * n + q] = se * ap + c * aq;
-                cplx<T> vp = V[i * n + p], vq = V[i * n + q];
-                V[i * n + p] = c * vp - mulc(vq, se);
-                V[i * n + q] = se * vp + c * vq;
+                cplx<T> ap = A[i * ld + p], aq = A[i * ld + q];
+                A[i * ld + p] = c * ap - mulc(aq, se);
+                A[i * ld + q] = se * ap + c * aq;
+                cplx<T> vp = V[i * ld + p], vq = V[i * ld + q];
+                V[i * ld + p] = c * vp - mulc(vq, se);
+                V[i * ld + q] = se * vp + c * vq;
             }
             __syncthreads();
             // rows of A:  row_p <- c row_p - s e^{i phi} row_q ; row_q <- s e^{-i phi} row_p + c row_q
@@ -108,9 +109,9 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<T>* __res
                 T c = prm[3 * pr];
                 cplx<T> se = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
                 if (se.re == 0 && se.im == 0) continue;
-                cplx<T> ap = A[p * n + j], aq = A[q * n + j];
-                A[p * n + j] = c * ap - se * aq;
-                A[q * n + j] = mulc(ap, se) + c * aq;
+                cplx<T> ap = A[p * ld + j], aq = A[q * ld + j];
+                A[p * ld + j] = c * ap - se * aq;
+                A[q * ld + j] = mulc(ap, se) + c * aq;
             }
             __syncthreads();
         }
@@ -119,14 +120,14 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<T>* __res
     }
     // ascending order: rank of each eigenvalue (ties broken by index)
     for (int i = tid; i < nw; i += 64) {
-        T ei = A[i * n + i].re;
+        T ei = A[i * ld + i].re;
         int rank = 0;
         for (int j = 0; j < nw; ++j) {
-            T ej = A[j * n + j].re;
+            T ej = A[j * ld + j].re;
             rank += (ej < ei) || (ej == ei && j < i);
         }
         Eout[k * nw + rank] = (double)ei;
-        for (int m = 0; m < nw; ++m) Uout[(k * nw + m) * nw + rank] = V[m * n + i];
+        for (int m = 0; m < nw; ++m) Uout[(k * nw + m) * nw + rank] = V[m * ld + i];
     }
 }
 
@@ -140,13 +141,13 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
     int n = nw + (nw & 1);
     const int sweeps = 40;
     if (prec == WBMI_F64) {
-        size_t lds = 2 * sizeof(cplx<double>) * n * n + 3 * sizeof(double) * (n / 2) + 16;
+        size_t lds = 2 * sizeof(cplx<double>) * n * (n + 1) + 3 * sizeof(double) * (n / 2) + 16;
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<double>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_eigh_jacobi<double>, dim3(nk), dim3(64), lds, ctx->stream, nw,
                            (const cplx<double>*)H_dev, Hstride, E_dev, (cplx<double>*)U_dev, sweeps);
     } else if (prec == WBMI_F32) {
-        size_t lds = 2 * sizeof(cplx<float>) * n * n + 3 * sizeof(float) * (n / 2) + 16;
+        size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_eigh_jacobi<float>, dim3(nk), dim3(64), lds, ctx->stream, nw,

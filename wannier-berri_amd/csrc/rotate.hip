@@ -1,67 +1,108 @@
 // Gauge rotation Xbar = U^+ X U of libwbmi (data_K/data_K.py:130-132).
 //
-// One 256-thread workgroup per k-point; U and the intermediate T = X U stay
-// in LDS while the workgroup walks over the channels of the per-k stack.
-// Each thread owns a strip of one row (JT threads per row) so that every X
-// element it loads from HBM feeds nw/JT multiply-adds.
+// One 256-thread workgroup per k-point.  U stays in LDS for all channels of
+// the k-point; each channel is loaded once from HBM with coalesced reads into
+// LDS (Hermitised there when flagged), T = X U is formed in LDS and
+// Xbar = U^+ T is written straight back from registers.  Both products use
+// TS x TS register tiles on a 16 x 16 thread grid (TS = 2 for nw <= 32,
+// 4 for nw <= 64).  LDS rows are padded by one element.
 #include "wbmi_common.h"
 
 #define ROT_THREADS 256
-#define ROT_JT 8
 
+template <typename T, int TS>
+__device__ inline void rot_channel(int nw, int ld, const cplx<T>* sU, const cplx<T>* sX, int xld, cplx<T>* sT,
+                                   bool herm, cplx<T>* Xc) {
+    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+    const int r0 = TS * tr, c0 = TS * tc;
+    const bool active = r0 < nw && c0 < nw;
+    cplx<T> acc[TS][TS];
+    // T = X U
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < TS; ++i)
+#pragma unroll
+            for (int j = 0; j < TS; ++j) acc[i][j] = mk((T)0, (T)0);
+        for (int m = 0; m < nw; ++m) {
+            cplx<T> x[TS], u[TS];
+#pragma unroll
+            for (int i = 0; i < TS; ++i) {
+                const int r = r0 + i < nw ? r0 + i : nw - 1;
+                cplx<T> v = sX[r * xld + m];
+                if (herm) {
+                    const cplx<T> w = sX[m * xld + r];
+                    v = mk((T)0.5 * (v.re + w.re), (T)0.5 * (v.im - w.im));
+                }
+                x[i] = v;
+            }
+#pragma unroll
+            for (int j = 0; j < TS; ++j) u[j] = sU[m * ld + (c0 + j < nw ? c0 + j : nw - 1)];
+#pragma unroll
+            for (int i = 0; i < TS; ++i)
+#pragma unroll
+                for (int j = 0; j < TS; ++j) acc[i][j] += x[i] * u[j];
+        }
+#pragma unroll
+        for (int i = 0; i < TS; ++i)
+#pragma unroll
+            for (int j = 0; j < TS; ++j)
+                if (r0 + i < nw && c0 + j < nw) sT[(r0 + i) * ld + c0 + j] = acc[i][j];
+    }
+    __syncthreads();
+    // Xbar = U^+ T
+    if (active) {
+#pragma unroll
+        for (int i = 0; i < TS; ++i)
+#pragma unroll
+            for (int j = 0; j < TS; ++j) acc[i][j] = mk((T)0, (T)0);
+        for (int m = 0; m < nw; ++m) {
+            cplx<T> u[TS], t[TS];
+#pragma unroll
+            for (int i = 0; i < TS; ++i) u[i] = sU[m * ld + (r0 + i < nw ? r0 + i : nw - 1)];
+#pragma unroll
+            for (int j = 0; j < TS; ++j) t[j] = sT[m * ld + (c0 + j < nw ? c0 + j : nw - 1)];
+#pragma unroll
+            for (int i = 0; i < TS; ++i)
+#pragma unroll
+                for (int j = 0; j < TS; ++j) acc[i][j] += cmul(u[i], t[j]);
+        }
+#pragma unroll
+        for (int i = 0; i < TS; ++i)
+#pragma unroll
+            for (int j = 0; j < TS; ++j)
+                if (r0 + i < nw && c0 + j < nw) Xc[(r0 + i) * nw + c0 + j] = acc[i][j];
+    }
+}
+
+// stage_x = 0: X is read from HBM (three padded nw x nw tiles would not fit in LDS)
 template <typename T>
 __global__ void __launch_bounds__(ROT_THREADS) k_rotate(int nw, int nch, const cplx<T>* __restrict__ U,
-                                                        cplx<T>* __restrict__ X, uint64_t herm_lo, uint64_t herm_hi,
-                                                        uint64_t skip_lo, uint64_t skip_hi) {
-    extern __shared__ char smem[];
+                                                        cplx<T>* X, uint64_t herm_lo, uint64_t herm_hi,
+                                                        uint64_t skip_lo, uint64_t skip_hi, int stage_x) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int ld = nw + 1;
     cplx<T>* sU = (cplx<T>*)smem;
-    cplx<T>* sT = sU + nw * nw;
+    cplx<T>* sT = sU + nw * ld;
+    cplx<T>* sX = sT + nw * ld;
     const int64_t k = blockIdx.x;
     const int tid = threadIdx.x;
     const int nw2 = nw * nw;
     const cplx<T>* Uk = U + k * nw2;
-    for (int t = tid; t < nw2; t += ROT_THREADS) sU[t] = Uk[t];
-    __syncthreads();
-    const int jt = tid % ROT_JT;
-    const int i0 = tid / ROT_JT;
-    const int istep = ROT_THREADS / ROT_JT;
+    for (int t = tid; t < nw2; t += ROT_THREADS) sU[(t / nw) * ld + t % nw] = Uk[t];
     for (int c = 0; c < nch; ++c) {
-        bool skip = c < 64 ? (skip_lo >> c) & 1 : (skip_hi >> (c - 64)) & 1;
+        const bool skip = c < 64 ? (skip_lo >> c) & 1 : (skip_hi >> (c - 64)) & 1;
         if (skip) continue;
-        bool herm = c < 64 ? (herm_lo >> c) & 1 : (herm_hi >> (c - 64)) & 1;
+        const bool herm = c < 64 ? (herm_lo >> c) & 1 : (herm_hi >> (c - 64)) & 1;
         cplx<T>* Xc = X + (k * nch + c) * nw2;
-        // T = X U
-        for (int i = i0; i < nw; i += istep) {
-            cplx<T> acc[64 / ROT_JT];
-#pragma unroll
-            for (int q = 0; q < 64 / ROT_JT; ++q) acc[q] = mk((T)0, (T)0);
-            for (int m = 0; m < nw; ++m) {
-                cplx<T> x = Xc[i * nw + m];
-                if (herm) {
-                    cplx<T> y = Xc[m * nw + i];
-                    x = mk((T)0.5 * (x.re + y.re), (T)0.5 * (x.im - y.im));
-                }
-#pragma unroll
-                for (int q = 0; q < 64 / ROT_JT; ++q) {
-                    int j = jt + q * ROT_JT;
-                    if (j < nw) acc[q] += x * sU[m * nw + j];
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 64 / ROT_JT; ++q) {
-                int j = jt + q * ROT_JT;
-                if (j < nw) sT[i * nw + j] = acc[q];
-            }
-        }
+        if (stage_x)
+            for (int t = tid; t < nw2; t += ROT_THREADS) sX[(t / nw) * ld + t % nw] = Xc[t];
         __syncthreads();
-        // Xbar = U^+ T
-        for (int i = i0; i < nw; i += istep) {
-            for (int j = jt; j < nw; j += ROT_JT) {
-                cplx<T> acc = mk((T)0, (T)0);
-                for (int m = 0; m < nw; ++m) acc += cmul(sU[m * nw + i], sT[m * nw + j]);
-                Xc[i * nw + j] = acc;
-            }
-        }
+        const cplx<T>* xs = stage_x ? sX : Xc;
+        const int xld = stage_x ? ld : nw;
+        if (nw <= 32)
+            rot_channel<T, 2>(nw, ld, sU, xs, xld, sT, herm, Xc);
+        else
+            rot_channel<T, 4>(nw, ld, sU, xs, xld, sT, herm, Xc);
         __syncthreads();
     }
 }
@@ -77,17 +118,21 @@ extern "C" int wbmi_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U
         if (skip_host && skip_host[c]) s[c / 64] |= 1ull << (c % 64);
     }
     if (prec == WBMI_F64) {
-        size_t lds = 2 * sizeof(cplx<double>) * nw * nw;
+        size_t tile = sizeof(cplx<double>) * nw * (nw + 1);
+        int stage_x = 3 * tile <= 160 * 1024;
+        size_t lds = (stage_x ? 3 : 2) * tile;
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_rotate<double>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_rotate<double>, dim3(nk), dim3(ROT_THREADS), lds, ctx->stream, nw, nch,
-                           (const cplx<double>*)U_dev, (cplx<double>*)X_dev, h[0], h[1], s[0], s[1]);
+                           (const cplx<double>*)U_dev, (cplx<double>*)X_dev, h[0], h[1], s[0], s[1], stage_x);
     } else if (prec == WBMI_F32) {
-        size_t lds = 2 * sizeof(cplx<float>) * nw * nw;
+        size_t tile = sizeof(cplx<float>) * nw * (nw + 1);
+        int stage_x = 3 * tile <= 160 * 1024;
+        size_t lds = (stage_x ? 3 : 2) * tile;
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_rotate<float>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL(k_rotate<float>, dim3(nk), dim3(ROT_THREADS), lds, ctx->stream, nw, nch,
-                           (const cplx<float>*)U_dev, (cplx<float>*)X_dev, h[0], h[1], s[0], s[1]);
+                           (const cplx<float>*)U_dev, (cplx<float>*)X_dev, h[0], h[1], s[0], s[1], stage_x);
     } else {
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_rotate: bad precision");
     }

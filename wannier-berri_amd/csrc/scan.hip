@@ -1,7 +1,8 @@
 // Covariant formula traces + Fermi-level scan of libwbmi.
 //
 // Replaces the per-(k, band group) loop of StaticCalculator.__call__
-// (calculators/static.py:60-136).  One 256-thread workgroup per k-point.
+// (calculators/static.py:60-136).  One 256-thread workgroup per k-point; the
+// formula is a template parameter, so every calculator gets its own kernel.
 //
 // Band partition.  The reference evaluates, for every band group G, the
 // formula with inn = G and out = all other bands.  Every sum in the formulas
@@ -11,11 +12,21 @@
 //     Off(x, y) = lab[x] != lab[y]      (x, y in different groups)
 //     On(x, y)  = lab[x] == lab[y]
 // and the masked matrices
-//     D'^a = -V^a o dEinv o Off,  V'' = V o On,  A'' = A o On.
+//     D'^a = -V^a o M,  M = dEinv o Off,  V'' = V o On,  A'' = A o On.
 // The Fermi-sea group (all bands below EFmin, data_K.py:178-185) and the
 // cumulative cuts [0, c) of non-additive formulas (static.py:109-117) are
 // two-set partitions evaluated the same way.
 //
+// DerOmega.  With G^{bd} = V^b D'^d, H^{ad} = D'^a V^d and K^{bd} = A^b D'^d,
+//   dD.ln = X1^{bd} = -M o [W^{bd} + G^{bd} + G^{db} - D'^b V''^d - D'^d V''^b]
+//   dD.nl = X2^{ad} = -M o [W^{ad} + V''^a D'^d + V''^d D'^a - H^{ad} - H^{da}]
+//   dA.ln = Y1^{bd} = A'^{bd} - D'^d A''^b + K^{bd}
+// (formula/elementary.py:63-70, formula/formula.py:108-118).  Each of the 27
+// dense products is computed once into LDS with register blocking and
+// consumed at once by the diagonal dot products of the traces; the sparse
+// (within-group) terms are added elementwise.
+//
+// Operands V and A stay in LDS (rows padded by one element) when nw <= 32.
 // All arithmetic is FP64 (energies, dEinv, traces, histogram); channels may
 // be stored in FP32 or FP64.
 #include <cmath>
@@ -24,45 +35,153 @@
 
 #define SC_THREADS 256
 #define SC_NWMAX 64
+#define SC_LDS_NW 32  // V and A live in LDS up to this nw
 
 __constant__ int c_alpha[3] = {1, 2, 0};  // utility.py:45-46
 __constant__ int c_beta[3] = {2, 0, 1};
 
 struct ScanArgs {
     wbmi_scan_params p;
-    int nw, nch;
+    int nw, nch, stage_va;
     double weight;
 };
 
+// dynamic shared-memory layout
+struct Smem {
+    cplx<double>* V;    // [3][nw][ld] (staged) or null
+    cplx<double>* A;    // [3][nw][ld] (staged, external terms) or null
+    cplx<double>* P;    // [nw][ld] product buffer
+    cplx<double>* acc;  // [nw][9]
+    double* M;          // [nw][ld] dEinv o Off
+    double* E;
+    double* cut;        // [nw+1][9]
+    double* gw;         // group weights (mean energy)
+    int* lab;
+    int* gs;
+    int* ge;
+    int* act;
+    int* gb;    // group borders [nw+1]
+    int* gin;   // group in range
+    int* misc;  // [0] = ngroups, [1] = bandmax
+    int* ch;    // channel slot per role [WBMI_NROLES]
+};
+
+__host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
+
+__host__ __device__ inline size_t smem_layout(int nw, bool stage_v, bool stage_a, char* base, Smem* s) {
+    const int ld = nw + 1;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        size_t o = off;
+        off = al16(off + bytes);
+        return base ? base + o : nullptr;
+    };
+    char* pV = stage_v ? take(sizeof(cplx<double>) * 3 * nw * ld) : nullptr;
+    char* pA = stage_a ? take(sizeof(cplx<double>) * 3 * nw * ld) : nullptr;
+    char* pP = take(sizeof(cplx<double>) * nw * ld);
+    char* pacc = take(sizeof(cplx<double>) * nw * 9);
+    char* pM = take(sizeof(double) * nw * ld);
+    char* pE = take(sizeof(double) * nw);
+    char* pcut = take(sizeof(double) * (nw + 1) * 9);
+    char* pgw = take(sizeof(double) * nw);
+    char* pint = take(sizeof(int) * (6 * nw + 1 + 4 + WBMI_NROLES));
+    if (s) {
+        s->V = (cplx<double>*)pV;
+        s->A = (cplx<double>*)pA;
+        s->P = (cplx<double>*)pP;
+        s->acc = (cplx<double>*)pacc;
+        s->M = (double*)pM;
+        s->E = (double*)pE;
+        s->cut = (double*)pcut;
+        s->gw = (double*)pgw;
+        int* q = (int*)pint;
+        s->lab = q;
+        s->gs = q + nw;
+        s->ge = q + 2 * nw;
+        s->act = q + 3 * nw;
+        s->gb = q + 4 * nw;
+        s->gin = q + 5 * nw + 1;
+        s->misc = q + 6 * nw + 1;
+        s->ch = q + 6 * nw + 5;
+    }
+    return off;
+}
+
+// Element access to the channels of one k-point (LDS copies of V and A when staged).
 template <typename T>
 struct KCtx {
     const cplx<T>* X;  // channels of this k: [nch][nw][nw]
-    int nw;
+    int nw, ld;
     const int* ch;
     const double* E;
-    const double* M;  // dEinv o Off
+    const double* M;  // dEinv o Off, row stride ld
     const int* lab;
     const int* gs;
     const int* ge;
+    const cplx<double>* sV;
+    const cplx<double>* sA;
     __device__ inline cplx<double> get(int role, int x, int y) const {
         cplx<T> v = X[((int64_t)ch[role] * nw + x) * nw + y];
         return mk((double)v.re, (double)v.im);
     }
-    __device__ inline cplx<double> V(int a, int x, int y) const { return get(WBMI_ROLE_V + a, x, y); }
+    __device__ inline cplx<double> V(int a, int x, int y) const {
+        return sV ? sV[(a * nw + x) * ld + y] : get(WBMI_ROLE_V + a, x, y);
+    }
+    __device__ inline cplx<double> A(int a, int x, int y) const {
+        return sA ? sA[(a * nw + x) * ld + y] : get(WBMI_ROLE_A + a, x, y);
+    }
     __device__ inline cplx<double> W(int a, int b, int x, int y) const { return get(WBMI_ROLE_W + 3 * a + b, x, y); }
-    __device__ inline cplx<double> A(int a, int x, int y) const { return get(WBMI_ROLE_A + a, x, y); }
     __device__ inline cplx<double> Ad(int a, int d, int x, int y) const { return get(WBMI_ROLE_AD + 3 * a + d, x, y); }
     __device__ inline cplx<double> O(int c, int x, int y) const { return get(WBMI_ROLE_O + c, x, y); }
     __device__ inline cplx<double> Od(int c, int d, int x, int y) const { return get(WBMI_ROLE_OD + 3 * c + d, x, y); }
     __device__ inline cplx<double> B(int a, int x, int y) const { return get(WBMI_ROLE_B + a, x, y); }
     __device__ inline cplx<double> C(int a, int x, int y) const { return get(WBMI_ROLE_C + a, x, y); }
+    __device__ inline double m(int x, int y) const { return M[x * ld + y]; }
     // D'^a_xy = -V^a_xy dEinv_xy Off(x, y)
     __device__ inline cplx<double> Dp(int a, int x, int y) const {
-        double m = M[x * nw + y];
-        if (m == 0.0) return mk(0.0, 0.0);
-        return (-m) * V(a, x, y);
+        double mm = M[x * ld + y];
+        if (mm == 0.0) return mk(0.0, 0.0);
+        return (-mm) * V(a, x, y);
     }
 };
+
+// P[r][c] = sum_k X(r, k) Y(k, c) for r, c < nw with TS x TS register tiles
+// on a 16 x 16 thread grid (TS = 2 covers nw <= 32, TS = 4 covers nw <= 64).
+template <int TS, class FX, class FY>
+__device__ inline void prod_tiles(cplx<double>* P, int nw, int ld, FX X, FY Y) {
+    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+    const int r0 = TS * tr, c0 = TS * tc;
+    if (r0 >= nw || c0 >= nw) return;
+    cplx<double> acc[TS][TS];
+#pragma unroll
+    for (int i = 0; i < TS; ++i)
+#pragma unroll
+        for (int j = 0; j < TS; ++j) acc[i][j] = mk(0.0, 0.0);
+    for (int k = 0; k < nw; ++k) {
+        cplx<double> x[TS], y[TS];
+#pragma unroll
+        for (int i = 0; i < TS; ++i) x[i] = (r0 + i < nw) ? X(r0 + i, k) : mk(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < TS; ++j) y[j] = (c0 + j < nw) ? Y(k, c0 + j) : mk(0.0, 0.0);
+#pragma unroll
+        for (int i = 0; i < TS; ++i)
+#pragma unroll
+            for (int j = 0; j < TS; ++j) acc[i][j] += x[i] * y[j];
+    }
+#pragma unroll
+    for (int i = 0; i < TS; ++i)
+#pragma unroll
+        for (int j = 0; j < TS; ++j)
+            if (r0 + i < nw && c0 + j < nw) P[(r0 + i) * ld + c0 + j] = acc[i][j];
+}
+
+template <class FX, class FY>
+__device__ inline void prod(cplx<double>* P, int nw, int ld, FX X, FY Y) {
+    if (nw <= 32)
+        prod_tiles<2>(P, nw, ld, X, Y);
+    else
+        prod_tiles<4>(P, nw, ld, X, Y);
+}
 
 // pre-Hermitian Omega summand s^c_xy for x, y in the same group
 // (formula/covariant.py:175-200): Omega.nn = s + s^+
@@ -102,74 +221,42 @@ __device__ cplx<double> morbh_s(const KCtx<T>& K, int c, int x, int y, bool inte
     return s;
 }
 
-// Shared-memory layout (dynamic)
-struct Smem {
-    double* E;
-    double* M;
-    cplx<double>* P;
-    cplx<double>* acc;  // [nw][9]
-    double* cut;        // [nw+1][9]
-    double* gw;         // group weights (mean energy)
-    int* lab;
-    int* gs;
-    int* ge;
-    int* act;
-    int* gb;   // group borders [nw+1]
-    int* gin;  // group in range
-    int* misc; // [0] = ngroups, [1] = bandmax
-    int* ch;   // channel slot per role [WBMI_NROLES]
+template <int F>
+__host__ __device__ constexpr int ncomp_of() {
+    return F == WBMI_FORMULA_IDENTITY ? 1 : (F == WBMI_FORMULA_OMEGA || F == WBMI_FORMULA_MORB_HPM) ? 3 : 9;
+}
+
+template <int F>
+__host__ __device__ constexpr bool uses_A() {
+    return F == WBMI_FORMULA_OMEGA || F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
+           F == WBMI_FORMULA_MORB_HPM;
+}
+
+// The two curl components c whose index pair (alpha_c, beta_c) contains x.
+struct Pair {
+    int c, partner;
+    double sig;
 };
-
-__device__ inline void* align16(void* p) { return (void*)(((uintptr_t)p + 15) & ~(uintptr_t)15); }
-
-__device__ Smem carve(char* base, int nw) {
-    Smem s;
-    char* p = base;
-    s.P = (cplx<double>*)align16(p);
-    p = (char*)(s.P + nw * nw);
-    s.acc = (cplx<double>*)align16(p);
-    p = (char*)(s.acc + nw * 9);
-    s.M = (double*)align16(p);
-    p = (char*)(s.M + nw * nw);
-    s.E = (double*)p;
-    p = (char*)(s.E + nw);
-    s.cut = (double*)p;
-    p = (char*)(s.cut + (nw + 1) * 9);
-    s.gw = (double*)p;
-    p = (char*)(s.gw + nw);
-    s.lab = (int*)p;
-    s.gs = s.lab + nw;
-    s.ge = s.gs + nw;
-    s.act = s.ge + nw;
-    s.gb = s.act + nw;
-    s.gin = s.gb + nw + 1;
-    s.misc = s.gin + nw;
-    s.ch = s.misc + 4;
-    return s;
-}
-
-static size_t smem_bytes(int nw) {
-    size_t b = 0;
-    b += 16 + sizeof(cplx<double>) * nw * nw;
-    b += 16 + sizeof(cplx<double>) * nw * 9;
-    b += 16 + sizeof(double) * nw * nw;
-    b += sizeof(double) * nw;
-    b += sizeof(double) * (nw + 1) * 9;
-    b += sizeof(double) * nw;
-    b += sizeof(int) * (5 * nw + 1 + nw + 4 + WBMI_NROLES);
-    return b + 64;
-}
-
-__device__ inline int ncomp_of(int f) {
-    switch (f) {
-        case WBMI_FORMULA_IDENTITY: return 1;
-        case WBMI_FORMULA_OMEGA: case WBMI_FORMULA_MORB_HPM: return 3;
-        default: return 9;
+// x in the second slot for sigma=+1: (sigma=+1: beta_c == x, a = alpha_c; sigma=-1: alpha_c == x, a = beta_c)
+__device__ inline Pair pair_second(int x, int which) {
+    for (int c = 0; c < 3; ++c) {
+        if (which == 0 && c_beta[c] == x) return Pair{c, c_alpha[c], 1.0};
+        if (which == 1 && c_alpha[c] == x) return Pair{c, c_beta[c], -1.0};
     }
+    return Pair{0, 0, 0.0};
+}
+// x in the first slot for sigma=+1: (sigma=+1: alpha_c == x, b = beta_c; sigma=-1: beta_c == x, b = alpha_c)
+__device__ inline Pair pair_first(int x, int which) {
+    for (int c = 0; c < 3; ++c) {
+        if (which == 0 && c_alpha[c] == x) return Pair{c, c_beta[c], 1.0};
+        if (which == 1 && c_beta[c] == x) return Pair{c, c_alpha[c], -1.0};
+    }
+    return Pair{0, 0, 0.0};
 }
 
 // build M = dEinv o Off for the current labels (data_K.py:290-298)
 __device__ void build_M(const Smem& S, int nw) {
+    const int ld = nw + 1;
     for (int t = threadIdx.x; t < nw * nw; t += SC_THREADS) {
         int x = t / nw, y = t % nw;
         double m = 0.0;
@@ -177,13 +264,132 @@ __device__ void build_M(const Smem& S, int nw) {
             double de = S.E[x] - S.E[y];
             if (fabs(de) >= 1e-7) m = 1.0 / de;
         }
-        S.M[t] = m;
+        S.M[x * ld + y] = m;
     }
 }
 
-// Per-band values for the current partition into S.acc[n][comp] (real part
-// kept in .re).  Only bands with act[n] are evaluated.
+// DerOmega per-band values (formula/covariant.py:227-256) into S.acc[n][c*3+d]
 template <typename T>
+__device__ void eval_deromega(const Smem& S, const KCtx<T>& K, bool in, bool ex) {
+    const int nw = K.nw, ld = K.ld, tid = threadIdx.x;
+    // elementwise terms: T5 and T4 (external) and the non-product parts of T1, T2, T3
+    for (int t = tid; t < nw * 9; t += SC_THREADS) {
+        int n = t / 9, c = (t % 9) / 3, d = t % 3;
+        if (!S.act[n]) continue;
+        const int g0 = S.gs[n], g1 = S.ge[n];
+        cplx<double> s = mk(0.0, 0.0);
+        for (int sg = 0; sg < 2; ++sg) {
+            const int a = sg == 0 ? c_alpha[c] : c_beta[c];
+            const int b = sg == 0 ? c_beta[c] : c_alpha[c];
+            const double sig = sg == 0 ? 1.0 : -1.0;
+            cplx<double> t1 = mk(0.0, 0.0), t2 = mk(0.0, 0.0), t3 = mk(0.0, 0.0);
+            for (int l = 0; l < nw; ++l) {
+                if (K.lab[l] == K.lab[n]) continue;
+                const cplx<double> dnl = K.Dp(a, n, l);
+                if (in) {
+                    // X1 non-product part: -M_ln [W^{bd} - D'^b V''^d - D'^d V''^b]_ln
+                    cplx<double> z = K.W(b, d, l, n);
+                    for (int q = g0; q < g1; ++q) z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
+                    t1 += dnl * ((-K.m(l, n)) * z);
+                }
+                if (ex) {
+                    // Y1 non-product part: A'^{bd} - D'^d A''^b
+                    cplx<double> y = K.Ad(b, d, l, n);
+                    for (int q = g0; q < g1; ++q) y -= K.Dp(d, l, q) * K.A(b, q, n);
+                    t2 += dnl * y;
+                    // X2 non-product part: -M_nl [W^{ad} + V''^a D'^d + V''^d D'^a]_nl, times A^b_ln
+                    cplx<double> z = K.W(a, d, n, l);
+                    for (int q = g0; q < g1; ++q) z += K.V(a, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(a, q, l);
+                    t3 += ((-K.m(n, l)) * z) * K.A(b, l, n);
+                }
+            }
+            s += times_i(-(sig * t1));
+            s -= sig * t2;
+            s -= sig * t3;
+            if (ex) {
+                // T4: -i sigma sum_{n' in G} A^a_nn' dA.nn^{bd}_n'n
+                for (int m = g0; m < g1; ++m) {
+                    cplx<double> y2 = K.Ad(b, d, m, n);
+                    for (int l = 0; l < nw; ++l) y2 += K.A(b, m, l) * K.Dp(d, l, n) - K.Dp(d, m, l) * K.A(b, l, n);
+                    s += times_i(-(sig * (K.A(a, n, m) * y2)));
+                }
+            }
+        }
+        if (ex) {
+            // T5: 0.5 dO.nn = 0.5 [O'^{cd}_nn - (D'^d O^c)_nn + (O^c D'^d)_nn]
+            cplx<double> t5 = K.Od(c, d, n, n);
+            for (int l = 0; l < nw; ++l) t5 += K.O(c, n, l) * K.Dp(d, l, n) - K.Dp(d, n, l) * K.O(c, l, n);
+            s += 0.5 * t5;
+        }
+        S.acc[t] = s;
+    }
+    __syncthreads();
+    cplx<double>* P = S.P;
+    for (int b = 0; b < 3; ++b) {
+        for (int d = 0; d < 3; ++d) {
+            if (in) {
+                // G^{bd} = V^b D'^d: in X1^{bd} (derivative d, D-index b) and X1^{db} (derivative b, D-index d)
+                prod(P, nw, ld, [&](int r, int k) { return K.V(b, r, k); },
+                     [&](int k, int c) { return K.Dp(d, k, c); });
+                __syncthreads();
+                for (int t = tid; t < nw * 4; t += SC_THREADS) {
+                    const int n = t / 4, w = t % 4;
+                    if (!S.act[n]) continue;
+                    const Pair p = pair_second(w < 2 ? b : d, w % 2);
+                    const int der = w < 2 ? d : b;
+                    cplx<double> s = mk(0.0, 0.0);
+                    for (int l = 0; l < nw; ++l) {
+                        const double mln = K.m(l, n);
+                        if (mln == 0.0) continue;
+                        s += K.Dp(p.partner, n, l) * ((-mln) * P[l * ld + n]);
+                    }
+                    S.acc[n * 9 + p.c * 3 + der] += times_i(-(p.sig * s));
+                }
+                __syncthreads();
+            }
+            if (ex) {
+                // H^{bd} = D'^b V^d: in X2^{bd} (derivative d, D-index b) and X2^{db} (derivative b, D-index d)
+                prod(P, nw, ld, [&](int r, int k) { return K.Dp(b, r, k); },
+                     [&](int k, int c) { return K.V(d, k, c); });
+                __syncthreads();
+                for (int t = tid; t < nw * 4; t += SC_THREADS) {
+                    const int n = t / 4, w = t % 4;
+                    if (!S.act[n]) continue;
+                    const Pair p = pair_first(w < 2 ? b : d, w % 2);
+                    const int der = w < 2 ? d : b;
+                    cplx<double> s = mk(0.0, 0.0);
+                    for (int l = 0; l < nw; ++l) {
+                        const double mnl = K.m(n, l);
+                        if (mnl == 0.0) continue;
+                        s += (mnl * P[n * ld + l]) * K.A(p.partner, l, n);
+                    }
+                    S.acc[n * 9 + p.c * 3 + der] -= p.sig * s;
+                }
+                __syncthreads();
+                // K^{bd} = A^b D'^d: in Y1^{bd} (derivative d, A-index b)
+                prod(P, nw, ld, [&](int r, int k) { return K.A(b, r, k); },
+                     [&](int k, int c) { return K.Dp(d, k, c); });
+                __syncthreads();
+                for (int t = tid; t < nw * 2; t += SC_THREADS) {
+                    const int n = t / 2, w = t % 2;
+                    if (!S.act[n]) continue;
+                    const Pair p = pair_second(b, w);
+                    cplx<double> s = mk(0.0, 0.0);
+                    for (int l = 0; l < nw; ++l) {
+                        if (K.lab[l] == K.lab[n]) continue;
+                        s += K.Dp(p.partner, n, l) * P[l * ld + n];
+                    }
+                    S.acc[n * 9 + p.c * 3 + d] -= p.sig * s;
+                }
+                __syncthreads();
+            }
+        }
+    }
+    for (int t = tid; t < nw * 9; t += SC_THREADS) S.acc[t] = mk(2.0 * S.acc[t].re, 0.0);
+}
+
+// Per-band values for the current partition into S.acc[n][comp] (.re).
+template <int F, typename T>
 __device__ void eval_partition(const Smem& S, const KCtx<T>& K, const wbmi_scan_params& p) {
     const int nw = K.nw;
     const int tid = threadIdx.x;
@@ -191,179 +397,37 @@ __device__ void eval_partition(const Smem& S, const KCtx<T>& K, const wbmi_scan_
     build_M(S, nw);
     for (int t = tid; t < nw * 9; t += SC_THREADS) S.acc[t] = mk(0.0, 0.0);
     __syncthreads();
-
-    switch (p.formula) {
-        case WBMI_FORMULA_IDENTITY: {
-            for (int n = tid; n < nw; n += SC_THREADS)
-                if (S.act[n]) S.acc[n * 9] = mk(1.0, 0.0);
-            break;
+    if (F == WBMI_FORMULA_IDENTITY) {
+        for (int n = tid; n < nw; n += SC_THREADS)
+            if (S.act[n]) S.acc[n * 9] = mk(1.0, 0.0);
+    } else if (F == WBMI_FORMULA_OMEGA) {
+        for (int t = tid; t < nw * 3; t += SC_THREADS) {
+            int n = t / 3, c = t % 3;
+            if (!S.act[n]) continue;
+            S.acc[n * 9 + c] = mk(2.0 * omega_s(K, c, n, n, in, ex).re, 0.0);
         }
-        case WBMI_FORMULA_OMEGA: {
-            for (int t = tid; t < nw * 3; t += SC_THREADS) {
-                int n = t / 3, c = t % 3;
-                if (!S.act[n]) continue;
-                S.acc[n * 9 + c] = mk(2.0 * omega_s(K, c, n, n, in, ex).re, 0.0);
-            }
-            break;
+    } else if (F == WBMI_FORMULA_MORB_HPM) {
+        for (int t = tid; t < nw * 3; t += SC_THREADS) {
+            int n = t / 3, c = t % 3;
+            if (!S.act[n]) continue;
+            double v = 2.0 * morbh_s(K, c, n, n, in, ex).re;
+            if (p.sign != 0) v += p.sign * S.E[n] * 2.0 * omega_s(K, c, n, n, in, ex).re;
+            S.acc[n * 9 + c] = mk(v, 0.0);
         }
-        case WBMI_FORMULA_MORB_HPM: {
-            for (int t = tid; t < nw * 3; t += SC_THREADS) {
-                int n = t / 3, c = t % 3;
-                if (!S.act[n]) continue;
-                double v = 2.0 * morbh_s(K, c, n, n, in, ex).re;
-                if (p.sign != 0) v += p.sign * S.E[n] * 2.0 * omega_s(K, c, n, n, in, ex).re;
-                S.acc[n * 9 + c] = mk(v, 0.0);
+    } else if (F == WBMI_FORMULA_VELOMEGA) {
+        // Re sum_{n' in G} V^a_nn' Omega^b_n'n, Omega = s + s^+ (formula/formula.py:141-148)
+        for (int t = tid; t < nw * 9; t += SC_THREADS) {
+            int n = t / 9, a = (t % 9) / 3, b = t % 3;
+            if (!S.act[n]) continue;
+            cplx<double> v = mk(0.0, 0.0);
+            for (int m = S.gs[n]; m < S.ge[n]; ++m) {
+                cplx<double> om = omega_s(K, b, m, n, in, ex) + conj(omega_s(K, b, n, m, in, ex));
+                v += K.V(a, n, m) * om;
             }
-            break;
+            S.acc[t] = mk(v.re, 0.0);
         }
-        case WBMI_FORMULA_VELOMEGA: {
-            // Re sum_{n' in G} V^a_nn' Omega^b_n'n, Omega = s + s^+ (formula/formula.py:141-148)
-            for (int t = tid; t < nw * 9; t += SC_THREADS) {
-                int n = t / 9, a = (t % 9) / 3, b = t % 3;
-                if (!S.act[n]) continue;
-                cplx<double> v = mk(0.0, 0.0);
-                for (int m = S.gs[n]; m < S.ge[n]; ++m) {
-                    cplx<double> om = omega_s(K, b, m, n, in, ex) + conj(omega_s(K, b, n, m, in, ex));
-                    v += K.V(a, n, m) * om;
-                }
-                S.acc[t] = mk(v.re, 0.0);
-            }
-            break;
-        }
-        case WBMI_FORMULA_DEROMEGA: {
-            // formula/covariant.py:227-256.  acc holds the complex pre-Hermitian
-            // diagonal sum_terms S^{cd}_nn; the trace of S + S^+ is 2 Re.
-            if (ex) {
-                for (int t = tid; t < nw * 9; t += SC_THREADS) {
-                    int n = t / 9, c = (t % 9) / 3, d = t % 3;
-                    if (!S.act[n]) continue;
-                    // 0.5 dO.nn: O'^{cd}_nn - (D'^d O^c)_nn + (O^c D'^d)_nn
-                    cplx<double> t5 = K.Od(c, d, n, n);
-                    for (int l = 0; l < nw; ++l) t5 += K.O(c, n, l) * K.Dp(d, l, n) - K.Dp(d, n, l) * K.O(c, l, n);
-                    cplx<double> s = 0.5 * t5;
-                    // -i sigma sum_{n' in G} A^a_nn' dA.nn^{bd}_n'n
-                    for (int sg = 0; sg < 2; ++sg) {
-                        int a = sg == 0 ? c_alpha[c] : c_beta[c];
-                        int b = sg == 0 ? c_beta[c] : c_alpha[c];
-                        double sig = sg == 0 ? 1.0 : -1.0;
-                        for (int m = S.gs[n]; m < S.ge[n]; ++m) {
-                            cplx<double> y2 = K.Ad(b, d, m, n);
-                            for (int l = 0; l < nw; ++l)
-                                y2 += K.A(b, m, l) * K.Dp(d, l, n) - K.Dp(d, m, l) * K.A(b, l, n);
-                            s += times_i(-(sig * (K.A(a, n, m) * y2)));
-                        }
-                    }
-                    S.acc[t] = s;
-                }
-            }
-            __syncthreads();
-            for (int d = 0; d < 3; ++d) {
-                for (int b = 0; b < 3; ++b) {
-                    // the two components c that pair D^a with index b:
-                    //   sigma=+1: beta_c == b, a = alpha_c;  sigma=-1: alpha_c == b, a = beta_c
-                    int c1 = -1, c2 = -1;
-                    for (int c = 0; c < 3; ++c) {
-                        if (c_beta[c] == b) c1 = c;
-                        if (c_alpha[c] == b) c2 = c;
-                    }
-                    if (in) {
-                        // X1^{bd}_ln = dD.ln = -dEinv_ln [W + V^b D'^d + V^d D'^b - D'^b V''^d - D'^d V''^b]_ln
-                        for (int t = tid; t < nw * nw; t += SC_THREADS) {
-                            int l = t / nw, n = t % nw;
-                            cplx<double> z = mk(0.0, 0.0);
-                            double m = S.M[l * nw + n];
-                            if (S.act[n] && m != 0.0) {
-                                z = K.W(b, d, l, n);
-                                for (int q = 0; q < nw; ++q) z += K.V(b, l, q) * K.Dp(d, q, n) + K.V(d, l, q) * K.Dp(b, q, n);
-                                for (int q = S.gs[n]; q < S.ge[n]; ++q)
-                                    z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
-                                z = (-m) * z;
-                            }
-                            S.P[t] = z;
-                        }
-                        __syncthreads();
-                        // -i sigma sum_l D'^a_nl X1_ln
-                        for (int t = tid; t < nw * 2; t += SC_THREADS) {
-                            int n = t / 2, w = t % 2;
-                            int c = w == 0 ? c1 : c2;
-                            if (!S.act[n]) continue;
-                            int a = w == 0 ? c_alpha[c] : c_beta[c];
-                            double sig = w == 0 ? 1.0 : -1.0;
-                            cplx<double> acc = mk(0.0, 0.0);
-                            for (int l = 0; l < nw; ++l) acc += K.Dp(a, n, l) * S.P[l * nw + n];
-                            S.acc[n * 9 + c * 3 + d] += times_i(-(sig * acc));
-                        }
-                        __syncthreads();
-                    }
-                    if (ex) {
-                        // Y1^{bd}_ln = dA.ln = A'^{bd} - D'^d A''^b + A^b D'^d
-                        for (int t = tid; t < nw * nw; t += SC_THREADS) {
-                            int l = t / nw, n = t % nw;
-                            cplx<double> z = mk(0.0, 0.0);
-                            if (S.act[n] && S.lab[l] != S.lab[n]) {
-                                z = K.Ad(b, d, l, n);
-                                for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
-                                for (int q = 0; q < nw; ++q) z += K.A(b, l, q) * K.Dp(d, q, n);
-                            }
-                            S.P[t] = z;
-                        }
-                        __syncthreads();
-                        // -sigma sum_l D'^a_nl Y1_ln
-                        for (int t = tid; t < nw * 2; t += SC_THREADS) {
-                            int n = t / 2, w = t % 2;
-                            int c = w == 0 ? c1 : c2;
-                            if (!S.act[n]) continue;
-                            int a = w == 0 ? c_alpha[c] : c_beta[c];
-                            double sig = w == 0 ? 1.0 : -1.0;
-                            cplx<double> acc = mk(0.0, 0.0);
-                            for (int l = 0; l < nw; ++l) acc += K.Dp(a, n, l) * S.P[l * nw + n];
-                            S.acc[n * 9 + c * 3 + d] -= sig * acc;
-                        }
-                        __syncthreads();
-                    }
-                }
-                if (ex) {
-                    for (int a = 0; a < 3; ++a) {
-                        int c1 = -1, c2 = -1;  // sigma=+1: alpha_c == a ; sigma=-1: beta_c == a
-                        for (int c = 0; c < 3; ++c) {
-                            if (c_alpha[c] == a) c1 = c;
-                            if (c_beta[c] == a) c2 = c;
-                        }
-                        // X2^{ad}_nl = dD.nl = -dEinv_nl [W + V''^a D'^d + V''^d D'^a - D'^a V^d - D'^d V^a]_nl
-                        for (int t = tid; t < nw * nw; t += SC_THREADS) {
-                            int n = t / nw, l = t % nw;
-                            cplx<double> z = mk(0.0, 0.0);
-                            double m = S.M[n * nw + l];
-                            if (S.act[n] && m != 0.0) {
-                                z = K.W(a, d, n, l);
-                                for (int q = S.gs[n]; q < S.ge[n]; ++q)
-                                    z += K.V(a, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(a, q, l);
-                                for (int q = 0; q < nw; ++q) z -= K.Dp(a, n, q) * K.V(d, q, l) + K.Dp(d, n, q) * K.V(a, q, l);
-                                z = (-m) * z;
-                            }
-                            S.P[t] = z;
-                        }
-                        __syncthreads();
-                        // -sigma sum_l X2_nl A^b_ln
-                        for (int t = tid; t < nw * 2; t += SC_THREADS) {
-                            int n = t / 2, w = t % 2;
-                            int c = w == 0 ? c1 : c2;
-                            if (!S.act[n]) continue;
-                            int b = w == 0 ? c_beta[c] : c_alpha[c];
-                            double sig = w == 0 ? 1.0 : -1.0;
-                            cplx<double> acc = mk(0.0, 0.0);
-                            for (int l = 0; l < nw; ++l) acc += S.P[n * nw + l] * K.A(b, l, n);
-                            S.acc[n * 9 + c * 3 + d] -= sig * acc;
-                        }
-                        __syncthreads();
-                    }
-                }
-            }
-            for (int t = tid; t < nw * 9; t += SC_THREADS) S.acc[t] = mk(2.0 * S.acc[t].re, 0.0);
-            break;
-        }
-        default:
-            break;
+    } else if (F == WBMI_FORMULA_DEROMEGA) {
+        eval_deromega(S, K, in, ex);
     }
     __syncthreads();
 }
@@ -382,26 +446,42 @@ __device__ inline void scatter(double* hist, const wbmi_scan_params& p, double E
     for (int c = 0; c < ncomp; ++c) unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, weight * v[c]);
 }
 
-template <typename T>
+template <typename T, int F>
 __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
                                                             const cplx<T>* __restrict__ Xall,
                                                             double* __restrict__ hist) {
-    extern __shared__ char smem_raw[];
+    extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     const wbmi_scan_params& p = args.p;
-    const int nw = args.nw;
+    const int nw = args.nw, ld = nw + 1;
     const int64_t k = blockIdx.x;
     const int tid = threadIdx.x;
-    Smem S = carve(smem_raw, nw);
+    const bool stage_v = args.stage_va && F != WBMI_FORMULA_IDENTITY;
+    const bool stage_a = args.stage_va && uses_A<F>() && p.external_terms;
+    Smem S;
+    smem_layout(nw, stage_v, stage_a, smem_raw, &S);
     for (int n = tid; n < nw; n += SC_THREADS) S.E[n] = Eall[k * nw + n];
     for (int r = tid; r < WBMI_NROLES; r += SC_THREADS) S.ch[r] = p.ch[r];
     __syncthreads();
-    const int ncomp = ncomp_of(p.formula);
-    const bool additive = p.formula == WBMI_FORMULA_IDENTITY || p.formula == WBMI_FORMULA_OMEGA ||
-                          p.formula == WBMI_FORMULA_DEROMEGA || p.formula == WBMI_FORMULA_VELOMEGA;
+    const cplx<T>* Xk = Xall + k * (int64_t)args.nch * nw * nw;
+    if (stage_v || stage_a) {
+        for (int t = tid; t < 3 * nw * nw; t += SC_THREADS) {
+            int a = t / (nw * nw), x = (t / nw) % nw, y = t % nw;
+            if (stage_v) {
+                cplx<T> v = Xk[((int64_t)S.ch[WBMI_ROLE_V + a] * nw + x) * nw + y];
+                S.V[(a * nw + x) * ld + y] = mk((double)v.re, (double)v.im);
+            }
+            if (stage_a) {
+                cplx<T> v = Xk[((int64_t)S.ch[WBMI_ROLE_A + a] * nw + x) * nw + y];
+                S.A[(a * nw + x) * ld + y] = mk((double)v.re, (double)v.im);
+            }
+        }
+    }
+    constexpr int ncomp = ncomp_of<F>();
+    constexpr bool additive = F == WBMI_FORMULA_IDENTITY || F == WBMI_FORMULA_OMEGA ||
+                              F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA;
 
     // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186)
     if (tid == 0) {
-        int ng = 0;
         S.gb[0] = 0;
         int nb = 1;
         for (int i = 1; i < nw; ++i)
@@ -413,7 +493,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 if (S.gb[i] % 2 == 0) S.gb[m++] = S.gb[i];
             nb = m;
         }
-        ng = nb - 1;
+        const int ng = nb - 1;
         int first_in = -1;
         for (int g = 0; g < ng; ++g) {
             int a = S.gb[g], b = S.gb[g + 1];
@@ -438,14 +518,17 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const int bandmax = S.misc[1];
 
     KCtx<T> K;
-    K.X = Xall + k * (int64_t)args.nch * nw * nw;
+    K.X = Xk;
     K.nw = nw;
+    K.ld = ld;
     K.ch = S.ch;
     K.E = S.E;
     K.M = S.M;
     K.lab = S.lab;
     K.gs = S.gs;
     K.ge = S.ge;
+    K.sV = stage_v ? S.V : nullptr;
+    K.sA = stage_a ? S.A : nullptr;
 
     double v[9];
     if (additive) {
@@ -458,7 +541,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 S.act[n] = S.gin[g];
             }
         __syncthreads();
-        eval_partition(S, K, p);
+        eval_partition<F>(S, K, p);
         if (tid < ng && S.gin[tid]) {
             int g = tid;
             for (int c = 0; c < ncomp; ++c) {
@@ -472,14 +555,14 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
         if (bandmax > 0) {
             // Fermi-sea group (0, bandmax) with weight -inf
             for (int n = tid; n < nw; n += SC_THREADS) {
-                int in = n < bandmax;
-                S.lab[n] = in ? 0 : 1;
-                S.gs[n] = in ? 0 : bandmax;
-                S.ge[n] = in ? bandmax : nw;
-                S.act[n] = in;
+                int inn = n < bandmax;
+                S.lab[n] = inn ? 0 : 1;
+                S.gs[n] = inn ? 0 : bandmax;
+                S.ge[n] = inn ? bandmax : nw;
+                S.act[n] = inn;
             }
             __syncthreads();
-            eval_partition(S, K, p);
+            eval_partition<F>(S, K, p);
             if (tid == 0) {
                 for (int c = 0; c < ncomp; ++c) {
                     double s = 0.0;
@@ -498,14 +581,14 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 if (S.gin[g] && (S.gb[g] == cidx || S.gb[g + 1] == cidx)) need = true;
             if (!need) continue;
             for (int n = tid; n < nw; n += SC_THREADS) {
-                int in = n < cidx;
-                S.lab[n] = in ? 0 : 1;
-                S.gs[n] = in ? 0 : cidx;
-                S.ge[n] = in ? cidx : nw;
-                S.act[n] = in;
+                int inn = n < cidx;
+                S.lab[n] = inn ? 0 : 1;
+                S.gs[n] = inn ? 0 : cidx;
+                S.ge[n] = inn ? cidx : nw;
+                S.act[n] = inn;
             }
             __syncthreads();
-            eval_partition(S, K, p);
+            eval_partition<F>(S, K, p);
             if (tid == 0)
                 for (int c = 0; c < ncomp; ++c) {
                     double s = 0.0;
@@ -538,7 +621,7 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
         return WBMI_OK;
     };
     int rc = WBMI_OK;
-    bool in = p->internal_terms, ex = p->external_terms;
+    bool ex = p->external_terms;
     switch (p->formula) {
         case WBMI_FORMULA_IDENTITY: break;
         case WBMI_FORMULA_OMEGA:
@@ -561,8 +644,33 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
         default:
             return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d is not available", p->formula);
     }
-    (void)in;
     return WBMI_OK;
+}
+
+template <typename T, int F>
+static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
+    const bool stage_v = a.stage_va && F != WBMI_FORMULA_IDENTITY;
+    const bool stage_a = a.stage_va && uses_A<F>() && a.p.external_terms;
+    size_t lds = smem_layout(a.nw, stage_v, stage_a, nullptr, nullptr);
+    if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_static_scan<T, F>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
+                       (const cplx<T>*)X, hist);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+template <typename T>
+static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
+    switch (a.p.formula) {
+        case WBMI_FORMULA_IDENTITY: return launch_scan<T, WBMI_FORMULA_IDENTITY>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_OMEGA: return launch_scan<T, WBMI_FORMULA_OMEGA>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_DEROMEGA: return launch_scan<T, WBMI_FORMULA_DEROMEGA>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_VELOMEGA: return launch_scan<T, WBMI_FORMULA_VELOMEGA>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_MORB_HPM: return launch_scan<T, WBMI_FORMULA_MORB_HPM>(ctx, nk, a, E, X, hist);
+        default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
+    }
 }
 
 extern "C" int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
@@ -579,22 +687,11 @@ extern "C" int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const do
     a.p = *params;
     a.nw = nw;
     a.nch = nch;
+    a.stage_va = nw <= SC_LDS_NW;
     a.weight = weight;
-    size_t lds = smem_bytes(nw);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<double>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<float>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (prec == WBMI_F64)
-        hipLaunchKernelGGL(k_static_scan<double>, dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E_dev,
-                           (const cplx<double>*)Xbar_dev, hist_dev);
-    else if (prec == WBMI_F32)
-        hipLaunchKernelGGL(k_static_scan<float>, dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E_dev,
-                           (const cplx<float>*)Xbar_dev, hist_dev);
-    else
-        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: bad precision");
-    WBMI_LAUNCH_CHECK(ctx);
-    return WBMI_OK;
+    if (prec == WBMI_F64) return dispatch_scan<double>(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
+    if (prec == WBMI_F32) return dispatch_scan<float>(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: bad precision");
 }
 
 // ---------------------------------------------------------------------------
