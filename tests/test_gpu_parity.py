@@ -276,3 +276,27 @@ def test_synthetic_nw40_global_operands(wb):
         e = rel_err(r.results[k].data, ro[k].data)
         say(f"  {k}: rel err {e:.2e}")
         assert e < 1e-8, k
+
+
+@pytest.mark.parametrize("nw", [5, 18, 32, 40])
+def test_eigh_fp32_refined(wb, ctx, nw):
+    """FP32 Jacobi + one FP64 Ogita-Aishima step: FP64-accurate eigenvalues and residuals."""
+    import torch
+    say(f"test_eigh_fp32_refined nw={nw}")
+    rng = np.random.default_rng(100 + nw)
+    nk = 40
+    H = rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw))
+    H = 3 * (H + H.conj().swapaxes(1, 2))
+    w, v = np.linalg.eigh(H[1])
+    w[3] = w[2] + 2e-4          # near-degenerate pair above the refinement threshold
+    H[1] = (v * w) @ v.conj().T
+    E = torch.empty((nk, nw), dtype=torch.float64, device="cuda")
+    U = torch.empty((nk, nw, nw), dtype=torch.complex64, device="cuda")
+    ctx.eigh(None, nw * nw, E, U, 1, nw, refine=True, H64=torch.from_numpy(H).cuda())
+    E = E.cpu().numpy()
+    U = U.cpu().numpy().astype(np.complex128)
+    Eref = np.linalg.eigvalsh(H)
+    scale = np.abs(Eref).max()
+    assert np.abs(E - Eref).max() < 1e-12 * scale
+    res = np.abs(np.einsum("kab,kbc->kac", H, U) - U * E[:, None, :]).max()
+    assert res < 1e-5 * scale        # vectors are stored in complex64

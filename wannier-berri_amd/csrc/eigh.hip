@@ -31,8 +31,8 @@ __device__ inline void rr_pair(int n, int r, int i, int& p, int& q) {
     q = ((n - 1 - i + r - 1) % m) + 1;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<T>* __restrict__ H, int64_t Hstride,
+template <typename T, typename Tin>
+__global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __restrict__ H, int64_t Hstride,
                                                     double* __restrict__ Eout, cplx<T>* __restrict__ Uout,
                                                     int max_sweeps) {
     extern __shared__ char smem[];
@@ -44,14 +44,14 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<T>* __res
     int* flag = (int*)(prm + 3 * (n / 2));
     const int tid = threadIdx.x;
     const int64_t k = blockIdx.x;
-    const cplx<T>* Hk = H + k * Hstride;
+    const cplx<Tin>* Hk = H + k * Hstride;
 
     for (int t = tid; t < n * n; t += 64) {
         int i = t / n, j = t % n;
         cplx<T> a = mk((T)0, (T)0);
         if (i < nw && j < nw) {
-            cplx<T> x = Hk[i * nw + j], y = Hk[j * nw + i];
-            a = mk((T)0.5 * (x.re + y.re), (T)0.5 * (x.im - y.im));
+            cplx<Tin> x = Hk[i * nw + j], y = Hk[j * nw + i];
+            a = mk((T)(0.5 * (x.re + y.re)), (T)(0.5 * (x.im - y.im)));
         }
         A[i * ld + j] = a;
         V[i * ld + j] = mk((T)(i == j ? 1 : 0), (T)0);
@@ -131,27 +131,184 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<T>* __res
     }
 }
 
+// ---------------------------------------------------------------------------
+// One FP64 refinement step of approximate eigenvectors (Ogita & Aishima,
+// "Iterative refinement for symmetric eigenvalue decomposition", Japan J.
+// Indust. Appl. Math. 35 (2018), Algorithm 1, written for Hermitian matrices):
+//   R = I - X^+ X,  S = X^+ A X,  lam_i = S_ii / (1 - R_ii),
+//   delta = 2 (||S - diag(lam)|| + max|lam| ||R||)   (Frobenius bounds),
+//   E_ij = (S_ij + lam_j R_ij) / (lam_j - lam_i) if |lam_j - lam_i| > delta, else R_ij / 2,
+//   X <- X + X E.
+// Eigenvalues come out FP64-accurate; vectors of separated eigenvalues gain a
+// factor ~eps32; clusters below delta are re-orthogonalised only.
+// ---------------------------------------------------------------------------
+
+#define OA_THREADS 256
+
+template <int TS, class FX, class FY, class FO>
+__device__ __attribute__((always_inline)) void oa_gemm(int nw, FX X, FY Y, FO out) {
+    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
+    const int r0 = TS * tr, c0 = TS * tc;
+    if (r0 >= nw || c0 >= nw) return;
+    cplx<double> acc[TS][TS];
+#pragma unroll
+    for (int i = 0; i < TS; ++i)
+#pragma unroll
+        for (int j = 0; j < TS; ++j) acc[i][j] = mk(0.0, 0.0);
+    for (int m = 0; m < nw; ++m) {
+        cplx<double> x[TS], y[TS];
+#pragma unroll
+        for (int i = 0; i < TS; ++i) x[i] = X(r0 + i < nw ? r0 + i : nw - 1, m);
+#pragma unroll
+        for (int j = 0; j < TS; ++j) y[j] = Y(m, c0 + j < nw ? c0 + j : nw - 1);
+#pragma unroll
+        for (int i = 0; i < TS; ++i)
+#pragma unroll
+            for (int j = 0; j < TS; ++j) acc[i][j] += x[i] * y[j];
+    }
+#pragma unroll
+    for (int i = 0; i < TS; ++i)
+#pragma unroll
+        for (int j = 0; j < TS; ++j)
+            if (r0 + i < nw && c0 + j < nw) out(r0 + i, c0 + j, acc[i][j]);
+}
+
+template <typename TU, int TS>
+__global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<double>* __restrict__ H64,
+                                                            int64_t Hstride, double* __restrict__ Eout,
+                                                            cplx<TU>* __restrict__ U) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int ld = nw + 1;
+    cplx<double>* sX = (cplx<double>*)smem;   // X (approximate eigenvectors)
+    cplx<double>* sA = sX + nw * ld;          // A, later R
+    cplx<double>* sT = sA + nw * ld;          // A X, later X + X E
+    cplx<double>* sS = sT + nw * ld;          // S, later E
+    double* lam = (double*)(sS + nw * ld);
+    double* red = lam + nw;                   // [0] ||S-D||^2, [1] ||R||^2, [2] max|lam|
+    int* rank = (int*)(red + 4);
+    const int tid = threadIdx.x;
+    const int64_t k = blockIdx.x;
+    const cplx<double>* Hk = H64 + k * Hstride;
+    cplx<TU>* Uk = U + k * nw * nw;
+    for (int t = tid; t < nw * nw; t += OA_THREADS) {
+        const int i = t / nw, j = t % nw;
+        const cplx<TU> u = Uk[t];
+        sX[i * ld + j] = mk((double)u.re, (double)u.im);
+        const cplx<double> x = Hk[i * nw + j], y = Hk[j * nw + i];
+        sA[i * ld + j] = mk(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
+    }
+    if (tid < 4) red[tid] = 0.0;
+    __syncthreads();
+    // T = A X
+    oa_gemm<TS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
+                [&](int r, int c, cplx<double> v) { sT[r * ld + c] = v; });
+    __syncthreads();
+    // S = X^+ T ; R = I - X^+ X (into sA)
+    oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); }, [&](int m, int c) { return sT[m * ld + c]; },
+                [&](int r, int c, cplx<double> v) { sS[r * ld + c] = v; });
+    __syncthreads();
+    oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); }, [&](int m, int c) { return sX[m * ld + c]; },
+                [&](int r, int c, cplx<double> v) { sA[r * ld + c] = mk((r == c ? 1.0 : 0.0) - v.re, -v.im); });
+    __syncthreads();
+    for (int i = tid; i < nw; i += OA_THREADS) lam[i] = sS[i * ld + i].re / (1.0 - sA[i * ld + i].re);
+    __syncthreads();
+    // Frobenius norms and max|lam|
+    double s2 = 0.0, r2 = 0.0, lm = 0.0;
+    for (int t = tid; t < nw * nw; t += OA_THREADS) {
+        const int i = t / nw, j = t % nw;
+        cplx<double> sv = sS[i * ld + j];
+        if (i == j) sv.re -= lam[i];
+        s2 += sv.re * sv.re + sv.im * sv.im;
+        const cplx<double> rv = sA[i * ld + j];
+        r2 += rv.re * rv.re + rv.im * rv.im;
+    }
+    for (int i = tid; i < nw; i += OA_THREADS) lm = fmax(lm, fabs(lam[i]));
+    for (int off = 32; off > 0; off >>= 1) {
+        s2 += __shfl_down(s2, off);
+        r2 += __shfl_down(r2, off);
+        lm = fmax(lm, __shfl_down(lm, off));
+    }
+    if ((tid & 63) == 0) {
+        atomicAdd(&red[0], s2);
+        atomicAdd(&red[1], r2);
+        atomicMax((unsigned long long*)&red[2], (unsigned long long)__double_as_longlong(lm));
+    }
+    __syncthreads();
+    const double delta = 2.0 * (sqrt(red[0]) + red[2] * sqrt(red[1]));
+    // E into sS
+    for (int t = tid; t < nw * nw; t += OA_THREADS) {
+        const int i = t / nw, j = t % nw;
+        const double dl = lam[j] - lam[i];
+        const cplx<double> r = sA[i * ld + j];
+        cplx<double> e;
+        if (fabs(dl) > delta) {
+            const cplx<double> sv = sS[i * ld + j];
+            e = (1.0 / dl) * mk(sv.re + lam[j] * r.re, sv.im + lam[j] * r.im);
+        } else {
+            e = 0.5 * r;
+        }
+        sS[i * ld + j] = e;
+    }
+    // ascending order of the refined eigenvalues (ties by index)
+    for (int i = tid; i < nw; i += OA_THREADS) {
+        int rk = 0;
+        for (int j = 0; j < nw; ++j) rk += (lam[j] < lam[i]) || (lam[j] == lam[i] && j < i);
+        rank[i] = rk;
+    }
+    __syncthreads();
+    // X + X E, written to column rank[c]
+    oa_gemm<TS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sS[m * ld + c]; },
+                [&](int r, int c, cplx<double> v) {
+                    const cplx<double> x = sX[r * ld + c] + v;
+                    Uk[r * nw + rank[c]] = mk((TU)x.re, (TU)x.im);
+                });
+    for (int i = tid; i < nw; i += OA_THREADS) Eout[k * nw + rank[i]] = lam[i];
+}
+
 extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride, double* E_dev,
                          void* U_dev, int prec, int refine, const void* H64_dev) {
     if (!ctx) return WBMI_ERR_ARG;
-    if (nk <= 0 || nw <= 0 || nw > 64 || !H_dev || !E_dev || !U_dev || Hstride < (int64_t)nw * nw)
+    if (nk <= 0 || nw <= 0 || nw > 64 || !E_dev || !U_dev || Hstride < (int64_t)nw * nw)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad arguments (nw <= 64, Hstride >= nw*nw)");
-    if (refine) return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_eigh: refinement is not available in this build");
-    (void)H64_dev;
+    if (refine && (prec != WBMI_F32 || !H64_dev))
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement needs prec=WBMI_F32 and the FP64 Hamiltonian");
+    if (!refine && !H_dev) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: H_dev is null");
     int n = nw + (nw & 1);
     const int sweeps = 40;
     if (prec == WBMI_F64) {
         size_t lds = 2 * sizeof(cplx<double>) * n * (n + 1) + 3 * sizeof(double) * (n / 2) + 16;
-        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<double>,
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<double, double>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_eigh_jacobi<double>, dim3(nk), dim3(64), lds, ctx->stream, nw,
+        hipLaunchKernelGGL((k_eigh_jacobi<double, double>), dim3(nk), dim3(64), lds, ctx->stream, nw,
                            (const cplx<double>*)H_dev, Hstride, E_dev, (cplx<double>*)U_dev, sweeps);
     } else if (prec == WBMI_F32) {
         size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
-        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_eigh_jacobi<float>, dim3(nk), dim3(64), lds, ctx->stream, nw,
-                           (const cplx<float>*)H_dev, Hstride, E_dev, (cplx<float>*)U_dev, sweeps);
+        if (refine) {
+            // FP32 solve of the FP64 Hamiltonian, then one FP64 refinement step
+            WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((k_eigh_jacobi<float, double>), dim3(nk), dim3(64), lds, ctx->stream, nw,
+                               (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev, sweeps);
+            WBMI_LAUNCH_CHECK(ctx);
+            size_t lds2 = 4 * sizeof(cplx<double>) * nw * (nw + 1) + sizeof(double) * (nw + 4) + sizeof(int) * nw + 16;
+            if (lds2 > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement needs nw <= 44");
+            if (nw <= 32) {
+                WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<float, 2>,
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+                hipLaunchKernelGGL((k_eigh_refine<float, 2>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw,
+                                   (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev);
+            } else {
+                WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<float, 4>,
+                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+                hipLaunchKernelGGL((k_eigh_refine<float, 4>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw,
+                                   (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev);
+            }
+        } else {
+            WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, float>,
+                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL((k_eigh_jacobi<float, float>), dim3(nk), dim3(64), lds, ctx->stream, nw,
+                               (const cplx<float>*)H_dev, Hstride, E_dev, (cplx<float>*)U_dev, sweeps);
+        }
     } else {
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad precision");
     }

@@ -22,11 +22,12 @@
 //   dD.nl = X2^{ad} = -M o [W^{ad} + V''^a D'^d + V''^d D'^a - H^{ad} - H^{da}]
 //   dA.ln = Y1^{bd} = A'^{bd} - D'^d A''^b + K^{bd}
 // (formula/elementary.py:63-70, formula/formula.py:108-118).  Each of the 27
-// dense products is computed once into LDS with register blocking and
-// consumed at once by the diagonal dot products of the traces; the sparse
-// (within-group) terms are added elementwise.
+// dense products is computed once into LDS with register blocking, the
+// elementwise terms are folded into it, and it is consumed at once by the
+// diagonal dot products of the traces.
 //
-// Operands V and A stay in LDS (rows padded by one element) when nw <= 32.
+// For DerOmega the operands V and A stay in LDS (rows padded by one element)
+// when nw <= 32.
 // All arithmetic is FP64 (energies, dEinv, traces, histogram); channels may
 // be stored in FP32 or FP64.
 #include <cmath>
@@ -68,43 +69,58 @@ struct Smem {
 
 __host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-__host__ __device__ inline size_t smem_layout(int nw, bool stage_v, bool stage_a, char* base, Smem* s) {
-    const int ld = nw + 1;
+// byte offsets of the LDS regions; off[11] = total size
+struct SmemOffsets {
+    size_t V, A, P, acc, M, E, cut, gw, ints, total;
+};
+
+__host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool stage_a) {
+    const size_t ld = nw + 1;
+    SmemOffsets o;
     size_t off = 0;
-    auto take = [&](size_t bytes) {
-        size_t o = off;
-        off = al16(off + bytes);
-        return base ? base + o : nullptr;
-    };
-    char* pV = stage_v ? take(sizeof(cplx<double>) * 3 * nw * ld) : nullptr;
-    char* pA = stage_a ? take(sizeof(cplx<double>) * 3 * nw * ld) : nullptr;
-    char* pP = take(sizeof(cplx<double>) * nw * ld);
-    char* pacc = take(sizeof(cplx<double>) * nw * 9);
-    char* pM = take(sizeof(double) * nw * ld);
-    char* pE = take(sizeof(double) * nw);
-    char* pcut = take(sizeof(double) * (nw + 1) * 9);
-    char* pgw = take(sizeof(double) * nw);
-    char* pint = take(sizeof(int) * (6 * nw + 1 + 4 + WBMI_NROLES));
-    if (s) {
-        s->V = (cplx<double>*)pV;
-        s->A = (cplx<double>*)pA;
-        s->P = (cplx<double>*)pP;
-        s->acc = (cplx<double>*)pacc;
-        s->M = (double*)pM;
-        s->E = (double*)pE;
-        s->cut = (double*)pcut;
-        s->gw = (double*)pgw;
-        int* q = (int*)pint;
-        s->lab = q;
-        s->gs = q + nw;
-        s->ge = q + 2 * nw;
-        s->act = q + 3 * nw;
-        s->gb = q + 4 * nw;
-        s->gin = q + 5 * nw + 1;
-        s->misc = q + 6 * nw + 1;
-        s->ch = q + 6 * nw + 5;
-    }
-    return off;
+    o.V = off;
+    off = al16(off + (stage_v ? sizeof(cplx<double>) * 3 * nw * ld : 0));
+    o.A = off;
+    off = al16(off + (stage_a ? sizeof(cplx<double>) * 3 * nw * ld : 0));
+    o.P = off;
+    off = al16(off + sizeof(cplx<double>) * nw * ld);
+    o.acc = off;
+    off = al16(off + sizeof(cplx<double>) * nw * 9);
+    o.M = off;
+    off = al16(off + sizeof(double) * nw * ld);
+    o.E = off;
+    off = al16(off + sizeof(double) * nw);
+    o.cut = off;
+    off = al16(off + sizeof(double) * (nw + 1) * 9);
+    o.gw = off;
+    off = al16(off + sizeof(double) * nw);
+    o.ints = off;
+    off = al16(off + sizeof(int) * (6 * nw + 1 + 4 + WBMI_NROLES));
+    o.total = off;
+    return o;
+}
+
+__device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int nw, bool stage_v, bool stage_a) {
+    const SmemOffsets o = smem_offsets(nw, stage_v, stage_a);
+    Smem s;
+    s.V = stage_v ? (cplx<double>*)(base + o.V) : nullptr;
+    s.A = stage_a ? (cplx<double>*)(base + o.A) : nullptr;
+    s.P = (cplx<double>*)(base + o.P);
+    s.acc = (cplx<double>*)(base + o.acc);
+    s.M = (double*)(base + o.M);
+    s.E = (double*)(base + o.E);
+    s.cut = (double*)(base + o.cut);
+    s.gw = (double*)(base + o.gw);
+    int* q = (int*)(base + o.ints);
+    s.lab = q;
+    s.gs = q + nw;
+    s.ge = q + 2 * nw;
+    s.act = q + 3 * nw;
+    s.gb = q + 4 * nw;
+    s.gin = q + 5 * nw + 1;
+    s.misc = q + 6 * nw + 1;
+    s.ch = q + 6 * nw + 5;
+    return s;
 }
 
 // Element access to the channels of one k-point (LDS copies of V and A when staged).
@@ -148,7 +164,7 @@ struct KCtx {
 // P[r][c] = sum_k X(r, k) Y(k, c) for r, c < nw with TS x TS register tiles
 // on a 16 x 16 thread grid (TS = 2 covers nw <= 32, TS = 4 covers nw <= 64).
 template <int TS, class FX, class FY>
-__device__ inline void prod_tiles(cplx<double>* P, int nw, int ld, FX X, FY Y) {
+__device__ __attribute__((always_inline)) void prod_tiles(cplx<double>* P, int nw, int ld, FX X, FY Y) {
     const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
     const int r0 = TS * tr, c0 = TS * tc;
     if (r0 >= nw || c0 >= nw) return;
@@ -175,18 +191,15 @@ __device__ inline void prod_tiles(cplx<double>* P, int nw, int ld, FX X, FY Y) {
             if (r0 + i < nw && c0 + j < nw) P[(r0 + i) * ld + c0 + j] = acc[i][j];
 }
 
-template <class FX, class FY>
-__device__ inline void prod(cplx<double>* P, int nw, int ld, FX X, FY Y) {
-    if (nw <= 32)
-        prod_tiles<2>(P, nw, ld, X, Y);
-    else
-        prod_tiles<4>(P, nw, ld, X, Y);
+template <int TS, class FX, class FY>
+__device__ __attribute__((always_inline)) void prod(cplx<double>* P, int nw, int ld, FX X, FY Y) {
+    prod_tiles<TS>(P, nw, ld, X, Y);
 }
 
 // pre-Hermitian Omega summand s^c_xy for x, y in the same group
 // (formula/covariant.py:175-200): Omega.nn = s + s^+
 template <typename T>
-__device__ cplx<double> omega_s(const KCtx<T>& K, int c, int x, int y, bool internal, bool external) {
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T>& K, int c, int x, int y, bool internal, bool external) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
     for (int l = 0; l < K.nw; ++l) {
@@ -205,7 +218,7 @@ __device__ cplx<double> omega_s(const KCtx<T>& K, int c, int x, int y, bool inte
 // pre-Hermitian Morb_H summand h^c_xy for x, y in the same group
 // (formula/covariant.py:390-414)
 template <typename T>
-__device__ cplx<double> morbh_s(const KCtx<T>& K, int c, int x, int y, bool internal, bool external) {
+__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T>& K, int c, int x, int y, bool internal, bool external) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
     for (int l = 0; l < K.nw; ++l) {
@@ -255,7 +268,7 @@ __device__ inline Pair pair_first(int x, int which) {
 }
 
 // build M = dEinv o Off for the current labels (data_K.py:290-298)
-__device__ void build_M(const Smem& S, int nw) {
+__device__ __attribute__((always_inline)) void build_M(const Smem& S, int nw) {
     const int ld = nw + 1;
     for (int t = threadIdx.x; t < nw * nw; t += SC_THREADS) {
         int x = t / nw, y = t % nw;
@@ -268,113 +281,128 @@ __device__ void build_M(const Smem& S, int nw) {
     }
 }
 
-// DerOmega per-band values (formula/covariant.py:227-256) into S.acc[n][c*3+d]
-template <typename T>
-__device__ void eval_deromega(const Smem& S, const KCtx<T>& K, bool in, bool ex) {
+// DerOmega per-band values (formula/covariant.py:227-256) into S.acc[n][c*3+d].
+// X1 and X2 are symmetric in their two indices, X1^{bd} = -M o (Q1^{bd} + Q1^{db}) and
+// X2^{ad} = -M o (Q2^{ad} + Q2^{da}), so the elementwise terms (W, within-group sums)
+// are folded half-and-half into the product buffers Q1 = G + (W - S)/2 and
+// Q2 = -H + (W + R)/2; Y1 = K + A' - D'A'' is formed the same way.  Every buffer
+// is consumed by the diagonal dot products of the traces right away.
+template <int TS, typename T>
+__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T>& K, bool in, bool ex) {
     const int nw = K.nw, ld = K.ld, tid = threadIdx.x;
-    // elementwise terms: T5 and T4 (external) and the non-product parts of T1, T2, T3
+    // T4 and T5 (external), lanes running over n
     for (int t = tid; t < nw * 9; t += SC_THREADS) {
-        int n = t / 9, c = (t % 9) / 3, d = t % 3;
-        if (!S.act[n]) continue;
-        const int g0 = S.gs[n], g1 = S.ge[n];
+        const int n = t % nw, c = (t / nw) / 3, d = (t / nw) % 3;
         cplx<double> s = mk(0.0, 0.0);
-        for (int sg = 0; sg < 2; ++sg) {
-            const int a = sg == 0 ? c_alpha[c] : c_beta[c];
-            const int b = sg == 0 ? c_beta[c] : c_alpha[c];
-            const double sig = sg == 0 ? 1.0 : -1.0;
-            cplx<double> t1 = mk(0.0, 0.0), t2 = mk(0.0, 0.0), t3 = mk(0.0, 0.0);
-            for (int l = 0; l < nw; ++l) {
-                if (K.lab[l] == K.lab[n]) continue;
-                const cplx<double> dnl = K.Dp(a, n, l);
-                if (in) {
-                    // X1 non-product part: -M_ln [W^{bd} - D'^b V''^d - D'^d V''^b]_ln
-                    cplx<double> z = K.W(b, d, l, n);
-                    for (int q = g0; q < g1; ++q) z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
-                    t1 += dnl * ((-K.m(l, n)) * z);
-                }
-                if (ex) {
-                    // Y1 non-product part: A'^{bd} - D'^d A''^b
-                    cplx<double> y = K.Ad(b, d, l, n);
-                    for (int q = g0; q < g1; ++q) y -= K.Dp(d, l, q) * K.A(b, q, n);
-                    t2 += dnl * y;
-                    // X2 non-product part: -M_nl [W^{ad} + V''^a D'^d + V''^d D'^a]_nl, times A^b_ln
-                    cplx<double> z = K.W(a, d, n, l);
-                    for (int q = g0; q < g1; ++q) z += K.V(a, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(a, q, l);
-                    t3 += ((-K.m(n, l)) * z) * K.A(b, l, n);
-                }
-            }
-            s += times_i(-(sig * t1));
-            s -= sig * t2;
-            s -= sig * t3;
-            if (ex) {
+        if (S.act[n] && ex) {
+            const int g0 = S.gs[n], g1 = S.ge[n];
+            for (int sg = 0; sg < 2; ++sg) {
+                const int a = sg == 0 ? c_alpha[c] : c_beta[c];
+                const int b = sg == 0 ? c_beta[c] : c_alpha[c];
+                const double sig = sg == 0 ? 1.0 : -1.0;
                 // T4: -i sigma sum_{n' in G} A^a_nn' dA.nn^{bd}_n'n
                 for (int m = g0; m < g1; ++m) {
                     cplx<double> y2 = K.Ad(b, d, m, n);
+#pragma unroll 4
                     for (int l = 0; l < nw; ++l) y2 += K.A(b, m, l) * K.Dp(d, l, n) - K.Dp(d, m, l) * K.A(b, l, n);
                     s += times_i(-(sig * (K.A(a, n, m) * y2)));
                 }
             }
-        }
-        if (ex) {
             // T5: 0.5 dO.nn = 0.5 [O'^{cd}_nn - (D'^d O^c)_nn + (O^c D'^d)_nn]
             cplx<double> t5 = K.Od(c, d, n, n);
+#pragma unroll 4
             for (int l = 0; l < nw; ++l) t5 += K.O(c, n, l) * K.Dp(d, l, n) - K.Dp(d, n, l) * K.O(c, l, n);
             s += 0.5 * t5;
         }
-        S.acc[t] = s;
+        if (n < nw) S.acc[n * 9 + c * 3 + d] = s;
     }
     __syncthreads();
     cplx<double>* P = S.P;
     for (int b = 0; b < 3; ++b) {
         for (int d = 0; d < 3; ++d) {
+            // both targets coincide for b == d: one lane adds twice the value
+            const int nt = b == d ? 2 : 4;
+            const double f = b == d ? 2.0 : 1.0;
             if (in) {
-                // G^{bd} = V^b D'^d: in X1^{bd} (derivative d, D-index b) and X1^{db} (derivative b, D-index d)
-                prod(P, nw, ld, [&](int r, int k) { return K.V(b, r, k); },
+                // Q1^{bd} = V^b D'^d + (W^{bd} - D'^b V''^d - D'^d V''^b) / 2
+                prod<TS>(P, nw, ld, [&](int r, int k) { return K.V(b, r, k); },
                      [&](int k, int c) { return K.Dp(d, k, c); });
                 __syncthreads();
-                for (int t = tid; t < nw * 4; t += SC_THREADS) {
-                    const int n = t / 4, w = t % 4;
+                for (int t = tid; t < nw * nw; t += SC_THREADS) {
+                    const int l = t / nw, n = t % nw;
+                    if (!S.act[n] || K.lab[l] == K.lab[n]) continue;
+                    cplx<double> z = K.W(b, d, l, n);
+                    for (int q = S.gs[n]; q < S.ge[n]; ++q)
+                        z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
+                    P[l * ld + n] += 0.5 * z;
+                }
+                __syncthreads();
+                // T1: -i sigma sum_l D'^a_nl X1_ln, X1 = -M o (Q1^{bd} + Q1^{db})
+                for (int t = tid; t < nw * nt; t += SC_THREADS) {
+                    const int n = t / nt, w = t % nt;
                     if (!S.act[n]) continue;
                     const Pair p = pair_second(w < 2 ? b : d, w % 2);
                     const int der = w < 2 ? d : b;
                     cplx<double> s = mk(0.0, 0.0);
+#pragma unroll 4
                     for (int l = 0; l < nw; ++l) {
                         const double mln = K.m(l, n);
                         if (mln == 0.0) continue;
                         s += K.Dp(p.partner, n, l) * ((-mln) * P[l * ld + n]);
                     }
-                    S.acc[n * 9 + p.c * 3 + der] += times_i(-(p.sig * s));
+                    S.acc[n * 9 + p.c * 3 + der] += times_i(-((f * p.sig) * s));
                 }
                 __syncthreads();
             }
             if (ex) {
-                // H^{bd} = D'^b V^d: in X2^{bd} (derivative d, D-index b) and X2^{db} (derivative b, D-index d)
-                prod(P, nw, ld, [&](int r, int k) { return K.Dp(b, r, k); },
+                // Q2^{bd} = -D'^b V^d + (W^{bd} + V''^b D'^d + V''^d D'^b) / 2
+                prod<TS>(P, nw, ld, [&](int r, int k) { return K.Dp(b, r, k); },
                      [&](int k, int c) { return K.V(d, k, c); });
                 __syncthreads();
-                for (int t = tid; t < nw * 4; t += SC_THREADS) {
-                    const int n = t / 4, w = t % 4;
+                for (int t = tid; t < nw * nw; t += SC_THREADS) {
+                    const int n = t / nw, l = t % nw;
+                    if (!S.act[n] || K.lab[l] == K.lab[n]) continue;
+                    cplx<double> z = K.W(b, d, n, l);
+                    for (int q = S.gs[n]; q < S.ge[n]; ++q)
+                        z += K.V(b, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(b, q, l);
+                    P[n * ld + l] = 0.5 * z - P[n * ld + l];
+                }
+                __syncthreads();
+                // T3: -sigma sum_l X2_nl A^b_ln, X2 = -M o (Q2^{bd} + Q2^{db})
+                for (int t = tid; t < nw * nt; t += SC_THREADS) {
+                    const int n = t / nt, w = t % nt;
                     if (!S.act[n]) continue;
                     const Pair p = pair_first(w < 2 ? b : d, w % 2);
                     const int der = w < 2 ? d : b;
                     cplx<double> s = mk(0.0, 0.0);
+#pragma unroll 4
                     for (int l = 0; l < nw; ++l) {
                         const double mnl = K.m(n, l);
                         if (mnl == 0.0) continue;
-                        s += (mnl * P[n * ld + l]) * K.A(p.partner, l, n);
+                        s += ((-mnl) * P[n * ld + l]) * K.A(p.partner, l, n);
                     }
-                    S.acc[n * 9 + p.c * 3 + der] -= p.sig * s;
+                    S.acc[n * 9 + p.c * 3 + der] -= (f * p.sig) * s;
                 }
                 __syncthreads();
-                // K^{bd} = A^b D'^d: in Y1^{bd} (derivative d, A-index b)
-                prod(P, nw, ld, [&](int r, int k) { return K.A(b, r, k); },
+                // Y1^{bd} = A^b D'^d + A'^{bd} - D'^d A''^b
+                prod<TS>(P, nw, ld, [&](int r, int k) { return K.A(b, r, k); },
                      [&](int k, int c) { return K.Dp(d, k, c); });
                 __syncthreads();
+                for (int t = tid; t < nw * nw; t += SC_THREADS) {
+                    const int l = t / nw, n = t % nw;
+                    if (!S.act[n] || K.lab[l] == K.lab[n]) continue;
+                    cplx<double> z = K.Ad(b, d, l, n);
+                    for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
+                    P[l * ld + n] += z;
+                }
+                __syncthreads();
+                // T2: -sigma sum_l D'^a_nl Y1_ln
                 for (int t = tid; t < nw * 2; t += SC_THREADS) {
                     const int n = t / 2, w = t % 2;
                     if (!S.act[n]) continue;
                     const Pair p = pair_second(b, w);
                     cplx<double> s = mk(0.0, 0.0);
+#pragma unroll 4
                     for (int l = 0; l < nw; ++l) {
                         if (K.lab[l] == K.lab[n]) continue;
                         s += K.Dp(p.partner, n, l) * P[l * ld + n];
@@ -389,8 +417,8 @@ __device__ void eval_deromega(const Smem& S, const KCtx<T>& K, bool in, bool ex)
 }
 
 // Per-band values for the current partition into S.acc[n][comp] (.re).
-template <int F, typename T>
-__device__ void eval_partition(const Smem& S, const KCtx<T>& K, const wbmi_scan_params& p) {
+template <int F, int TS, typename T>
+__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T>& K, const wbmi_scan_params& p) {
     const int nw = K.nw;
     const int tid = threadIdx.x;
     const bool in = p.internal_terms, ex = p.external_terms;
@@ -427,26 +455,24 @@ __device__ void eval_partition(const Smem& S, const KCtx<T>& K, const wbmi_scan_
             S.acc[t] = mk(v.re, 0.0);
         }
     } else if (F == WBMI_FORMULA_DEROMEGA) {
-        eval_deromega(S, K, in, ex);
+        eval_deromega<TS>(S, K, in, ex);
     }
     __syncthreads();
 }
 
-__device__ inline void scatter(double* hist, const wbmi_scan_params& p, double E, const double* v, int ncomp,
-                               double weight) {
-    int bin;
-    if (E < p.EFmin) {
-        bin = 0;
-    } else if (E <= p.EFmax) {
-        bin = (int)ceil((E - p.EFmin) / p.dEF);
-        if (bin >= p.nEF_extra) return;
-    } else {
-        return;
+// Fermi-level bin of a group with mean energy E (static.py:131-136): 0 for
+// E < EFmin (added to every level), ceil((E - EFmin)/dEF) up to EFmax, -1 = none.
+__device__ inline int fermi_bin(const wbmi_scan_params& p, double E) {
+    if (E < p.EFmin) return 0;
+    if (E <= p.EFmax) {
+        const int bin = (int)ceil((E - p.EFmin) / p.dEF);
+        return bin < p.nEF_extra ? bin : -1;
     }
-    for (int c = 0; c < ncomp; ++c) unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, weight * v[c]);
+    return -1;
 }
 
-template <typename T, int F>
+// TS: product tile (2 for nw <= 32, 4 for nw <= 64)
+template <typename T, int F, int TS>
 __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
                                                             const cplx<T>* __restrict__ Xall,
                                                             double* __restrict__ hist) {
@@ -455,10 +481,9 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const int nw = args.nw, ld = nw + 1;
     const int64_t k = blockIdx.x;
     const int tid = threadIdx.x;
-    const bool stage_v = args.stage_va && F != WBMI_FORMULA_IDENTITY;
-    const bool stage_a = args.stage_va && uses_A<F>() && p.external_terms;
-    Smem S;
-    smem_layout(nw, stage_v, stage_a, smem_raw, &S);
+    const bool stage_v = args.stage_va && F == WBMI_FORMULA_DEROMEGA;
+    const bool stage_a = stage_v && p.external_terms;
+    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a);
     for (int n = tid; n < nw; n += SC_THREADS) S.E[n] = Eall[k * nw + n];
     for (int r = tid; r < WBMI_NROLES; r += SC_THREADS) S.ch[r] = p.ch[r];
     __syncthreads();
@@ -530,7 +555,6 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     K.sV = stage_v ? S.V : nullptr;
     K.sA = stage_a ? S.A : nullptr;
 
-    double v[9];
     if (additive) {
         // main partition: every group at once
         for (int g = 0; g < ng; ++g)
@@ -541,15 +565,14 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 S.act[n] = S.gin[g];
             }
         __syncthreads();
-        eval_partition<F>(S, K, p);
-        if (tid < ng && S.gin[tid]) {
-            int g = tid;
-            for (int c = 0; c < ncomp; ++c) {
-                double s = 0.0;
-                for (int n = S.gb[g]; n < S.gb[g + 1]; ++n) s += S.acc[n * 9 + c].re;
-                v[c] = s;
-            }
-            scatter(hist, p, S.gw[g], v, ncomp, args.weight);
+        eval_partition<F, TS>(S, K, p);
+        for (int t = tid; t < ng * ncomp; t += SC_THREADS) {
+            const int g = t / ncomp, c = t % ncomp;
+            const int bin = S.gin[g] ? fermi_bin(p, S.gw[g]) : -1;
+            if (bin < 0) continue;
+            double s = 0.0;
+            for (int n = S.gb[g]; n < S.gb[g + 1]; ++n) s += S.acc[n * 9 + c].re;
+            unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s);
         }
         __syncthreads();
         if (bandmax > 0) {
@@ -562,14 +585,11 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 S.act[n] = inn;
             }
             __syncthreads();
-            eval_partition<F>(S, K, p);
-            if (tid == 0) {
-                for (int c = 0; c < ncomp; ++c) {
-                    double s = 0.0;
-                    for (int n = 0; n < bandmax; ++n) s += S.acc[n * 9 + c].re;
-                    v[c] = s;
-                }
-                scatter(hist, p, -INFINITY, v, ncomp, args.weight);
+            eval_partition<F, TS>(S, K, p);
+            for (int c = tid; c < ncomp; c += SC_THREADS) {
+                double s = 0.0;
+                for (int n = 0; n < bandmax; ++n) s += S.acc[n * 9 + c].re;
+                unsafeAtomicAdd(hist + c, args.weight * s);  // bin 0: below EFmin
             }
             __syncthreads();
         }
@@ -588,7 +608,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 S.act[n] = inn;
             }
             __syncthreads();
-            eval_partition<F>(S, K, p);
+            eval_partition<F, TS>(S, K, p);
             if (tid == 0)
                 for (int c = 0; c < ncomp; ++c) {
                     double s = 0.0;
@@ -597,18 +617,18 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 }
             __syncthreads();
         }
-        if (tid == 0) {
-            for (int c = 0; c < ncomp; ++c) S.cut[c] = 0.0;
-            for (int g = 0; g < ng; ++g) {
-                if (!S.gin[g]) continue;
-                for (int c = 0; c < ncomp; ++c) v[c] = S.cut[S.gb[g + 1] * 9 + c] - S.cut[S.gb[g] * 9 + c];
-                scatter(hist, p, S.gw[g], v, ncomp, args.weight);
-            }
-            if (bandmax > 0) {
-                for (int c = 0; c < ncomp; ++c) v[c] = S.cut[bandmax * 9 + c];
-                scatter(hist, p, -INFINITY, v, ncomp, args.weight);
-            }
+        for (int c = tid; c < ncomp; c += SC_THREADS) S.cut[c] = 0.0;
+        __syncthreads();
+        for (int t = tid; t < ng * ncomp; t += SC_THREADS) {
+            const int g = t / ncomp, c = t % ncomp;
+            const int bin = S.gin[g] ? fermi_bin(p, S.gw[g]) : -1;
+            if (bin < 0) continue;
+            const double s = S.cut[S.gb[g + 1] * 9 + c] - S.cut[S.gb[g] * 9 + c];
+            unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s);
         }
+        if (bandmax > 0)
+            for (int c = tid; c < ncomp; c += SC_THREADS)
+                unsafeAtomicAdd(hist + c, args.weight * S.cut[bandmax * 9 + c]);
     }
 }
 
@@ -647,18 +667,25 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
     return WBMI_OK;
 }
 
-template <typename T, int F>
-static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    const bool stage_v = a.stage_va && F != WBMI_FORMULA_IDENTITY;
-    const bool stage_a = a.stage_va && uses_A<F>() && a.p.external_terms;
-    size_t lds = smem_layout(a.nw, stage_v, stage_a, nullptr, nullptr);
+template <typename T, int F, int TS>
+static int launch_scan_ts(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
+    const bool stage_v = a.stage_va && F == WBMI_FORMULA_DEROMEGA;
+    const bool stage_a = stage_v && a.p.external_terms;
+    size_t lds = smem_offsets(a.nw, stage_v, stage_a).total;
     if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F>,
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, TS>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_static_scan<T, F>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
+    hipLaunchKernelGGL((k_static_scan<T, F, TS>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
                        (const cplx<T>*)X, hist);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
+}
+
+template <typename T, int F>
+static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
+    // only DerOmega forms dense products; the other formulas need no tile variants
+    if (F == WBMI_FORMULA_DEROMEGA && a.nw > 32) return launch_scan_ts<T, F, 4>(ctx, nk, a, E, X, hist);
+    return launch_scan_ts<T, F, 2>(ctx, nk, a, E, X, hist);
 }
 
 template <typename T>

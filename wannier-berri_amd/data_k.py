@@ -193,43 +193,66 @@ class Data_K:
     def ensure(self, keys):
         """Make sure the channels `keys` are available (rotated to the band basis)."""
         keys = set(keys) | {("Ham", 0)}
-        if self._slots is not None and keys <= set(self._slots):
+        if self._slots is not None and (keys - {("Ham", 0)}) <= set(self._slots):
             return
         self._requested |= keys
         if self._slots is not None:
             self._requested |= set(self._slots)
         self._evaluate(sorted(self._requested, key=CHANNEL_ORDER.index))
 
-    def _evaluate(self, keys):
+    def _transform(self, keys, dtype, prec):
+        """R -> k of the channel keys into a new [nk, nch, nw, nw] tensor."""
         torch = self.torch
         nw, nk = self.num_wann, self.nk
-        slots, s = {}, 0
-        for key in keys:
-            slots[key] = s
-            s += CHANNEL_NCOMP[key]
-        nch = s
         stack = self.dev.stack(keys)
-        X = torch.empty((nk, nch, nw, nw), dtype=self.cdtype, device=self.ctx.device)
+        X = torch.empty((nk, stack.shape[1], nw, nw), dtype=dtype, device=self.ctx.device)
         if self.k_list is None:
             plan, box, aliased = self.dev.plan(keys, self.NKFFT)
             work = torch.empty(plan.work_bytes, dtype=torch.uint8, device=self.ctx.device)
-            plan.execute(stack if aliased else None, box, self.dK, work, X, self.prec)
+            plan.execute(stack if aliased else None, box, self.dK, work, X, prec)
             del work
         else:
             kl = torch.from_numpy(self.k_list).to(self.ctx.device)
-            self.ctx.r2k_klist(self.dev.iRvec_dev, stack, kl, X, self.prec)
+            self.ctx.r2k_klist(self.dev.iRvec_dev, stack, kl, X, prec)
+        return X
+
+    def _evaluate(self, keys):
+        """Channels `keys` (always including the Hamiltonian) -> E, U and the rotated stack.
+
+        FP64 form: one complex128 stack, FP64 Jacobi.  Mixed form: the
+        Hamiltonian is transformed in FP64 on its own, the other channels in
+        complex64; FP32 Jacobi + one FP64 Ogita-Aishima step give FP64
+        energies and refined eigenvectors (stored as complex64).
+        """
+        torch = self.torch
+        nw, nk = self.num_wann, self.nk
+        mixed = self.prec == WBMI_F32
+        main = [k for k in keys if not (mixed and k == ("Ham", 0))]
+        slots, s = {}, 0
+        for key in main:
+            slots[key] = s
+            s += CHANNEL_NCOMP[key]
+        nch = s
         E = torch.empty((nk, nw), dtype=torch.float64, device=self.ctx.device)
         U = torch.empty((nk, nw, nw), dtype=self.cdtype, device=self.ctx.device)
-        h0 = slots[("Ham", 0)]
-        self.ctx.eigh(X[:, h0], nch * nw * nw, E, U, self.prec, nw)
-        herm = np.zeros(nch, dtype=np.int32)
-        skip = np.zeros(nch, dtype=np.int32)
-        for key, sl in slots.items():
-            n = CHANNEL_NCOMP[key]
-            if key[0] in HERMITIAN:
-                herm[sl:sl + n] = 1
-        skip[h0] = 1
-        self.ctx.rotate(U, X, herm, skip, self.prec)
+        X = self._transform(main, self.cdtype, self.prec) if main else None
+        if mixed:
+            H64 = self._transform([("Ham", 0)], torch.complex128, WBMI_F64)
+            self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=True, H64=H64)
+            del H64
+        else:
+            h0 = slots[("Ham", 0)]
+            self.ctx.eigh(X[:, h0], nch * nw * nw, E, U, self.prec, nw)
+        if X is not None:
+            herm = np.zeros(nch, dtype=np.int32)
+            skip = np.zeros(nch, dtype=np.int32)
+            for key, sl in slots.items():
+                if key[0] in HERMITIAN:
+                    herm[sl:sl + CHANNEL_NCOMP[key]] = 1
+            if ("Ham", 0) in slots:
+                skip[slots[("Ham", 0)]] = 1
+            if skip.sum() < nch:
+                self.ctx.rotate(U, X, herm, skip, self.prec)
         self.Xk, self.E_dev, self.U_dev, self._slots, self.nch = X, E, U, slots, nch
 
     def role_table(self):
