@@ -245,7 +245,10 @@ def test_synthetic_all_calculators(wb, small_synth):
 
 
 def test_synthetic_fp32(wb, small_synth):
-    """Complex64 channels with FP64 energies/denominators/k-sum: error vs the FP64 oracle."""
+    """Mixed form vs the FP64 oracle: the metric's bar, 1e-6 relative for AHC (BCD reported, 5e-6 bar).
+
+    complex64 channels and FP32 rotation; FP64 Hamiltonian, refined FP64
+    energies, FP64 denominators, traces and k-sum."""
     say("test_synthetic_fp32")
     st = wb.calculators.static
     s, o = small_synth
@@ -256,7 +259,7 @@ def test_synthetic_fp32(wb, small_synth):
     for k in ("ahc", "bcd"):
         e = rel_err(r.results[k].data, ro[k].data)
         say(f"  fp32 {k}: rel err {e:.2e}")
-        assert e < 1e-3, k
+        assert e < {"ahc": 1e-6, "bcd": 5e-6}[k], k
 
 
 def test_synthetic_nw40_global_operands(wb):
@@ -280,7 +283,11 @@ def test_synthetic_nw40_global_operands(wb):
 
 @pytest.mark.parametrize("nw", [5, 18, 32, 40])
 def test_eigh_fp32_refined(wb, ctx, nw):
-    """FP32 Jacobi + one FP64 Ogita-Aishima step: FP64-accurate eigenvalues and residuals."""
+    """FP32 Jacobi + FP64 Ogita-Aishima steps: FP64-accurate eigenvalues, complex64-accurate vectors.
+
+    k = 1 holds a pair 2e-4 apart (relative gap 4e-6), closer than the first
+    step's threshold, which only re-orthogonalises it; the following steps
+    separate it (six to eight in all).  All others stop after two."""
     import torch
     say(f"test_eigh_fp32_refined nw={nw}")
     rng = np.random.default_rng(100 + nw)
@@ -288,15 +295,19 @@ def test_eigh_fp32_refined(wb, ctx, nw):
     H = rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw))
     H = 3 * (H + H.conj().swapaxes(1, 2))
     w, v = np.linalg.eigh(H[1])
-    w[3] = w[2] + 2e-4          # near-degenerate pair above the refinement threshold
+    w[3] = w[2] + 2e-4          # near-degenerate pair (relative gap 4e-6)
     H[1] = (v * w) @ v.conj().T
     E = torch.empty((nk, nw), dtype=torch.float64, device="cuda")
     U = torch.empty((nk, nw, nw), dtype=torch.complex64, device="cuda")
-    ctx.eigh(None, nw * nw, E, U, 1, nw, refine=True, H64=torch.from_numpy(H).cuda())
+    ctx.eigh(None, nw * nw, E, U, 1, nw, refine=8, H64=torch.from_numpy(H).cuda())
     E = E.cpu().numpy()
     U = U.cpu().numpy().astype(np.complex128)
     Eref = np.linalg.eigvalsh(H)
     scale = np.abs(Eref).max()
-    assert np.abs(E - Eref).max() < 1e-12 * scale
+    err = np.abs(E - Eref).max(axis=1) / scale
+    say(f"  E rel err: pair k=1 {err[1]:.2e}, others {np.delete(err, 1).max():.2e}")
+    assert err.max() < 1e-13
     res = np.abs(np.einsum("kab,kbc->kac", H, U) - U * E[:, None, :]).max()
-    assert res < 1e-5 * scale        # vectors are stored in complex64
+    assert res < 1e-6 * scale        # vectors are stored in complex64
+    orth = np.abs(np.einsum("kji,kjl->kil", U.conj(), U) - np.eye(nw)).max()
+    assert orth < 1e-6

@@ -179,10 +179,10 @@ class Context:
                                _ptr(iRvec_dev), nelem, _ptr(XR), klist_dev.shape[0], _ptr(klist_dev), _ptr(out),
                                prec), "wbmi_r2k_klist")
 
-    def eigh(self, H, Hstride, E, U, prec, nw, refine=False, H64=None):
-        """Batched eigh; refine=True: FP32 Jacobi of H64 + one FP64 Ogita-Aishima step (prec must be F32)."""
+    def eigh(self, H, Hstride, E, U, prec, nw, refine=0, H64=None):
+        """Batched eigh; refine=n >= 1: FP32 Jacobi of H64 + n FP64 Ogita-Aishima steps (prec must be F32)."""
         self.check(self._timed("eigh", (E.shape[0], nw), lib().wbmi_eigh, self.h, E.shape[0], nw, _ptr(H), Hstride,
-                               _ptr(E), _ptr(U), prec, int(bool(refine)), _ptr(H64)), "wbmi_eigh")
+                               _ptr(E), _ptr(U), prec, int(refine), _ptr(H64)), "wbmi_eigh")
 
     def rotate(self, U, X, herm, skip, prec):
         nk, nch, nw = X.shape[0], X.shape[1], X.shape[2]

@@ -132,18 +132,20 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
 }
 
 // ---------------------------------------------------------------------------
-// One FP64 refinement step of approximate eigenvectors (Ogita & Aishima,
+// FP64 refinement steps of approximate eigenvectors (Ogita & Aishima,
 // "Iterative refinement for symmetric eigenvalue decomposition", Japan J.
 // Indust. Appl. Math. 35 (2018), Algorithm 1, written for Hermitian matrices):
 //   R = I - X^+ X,  S = X^+ A X,  lam_i = S_ii / (1 - R_ii),
 //   delta = 2 (||S - diag(lam)|| + max|lam| ||R||)   (Frobenius bounds),
 //   E_ij = (S_ij + lam_j R_ij) / (lam_j - lam_i) if |lam_j - lam_i| > delta, else R_ij / 2,
 //   X <- X + X E.
-// Eigenvalues come out FP64-accurate; vectors of separated eigenvalues gain a
-// factor ~eps32; clusters below delta are re-orthogonalised only.
+// Each step squares the error of the vectors of separated eigenvalues;
+// clusters below delta are re-orthogonalised only.
 // ---------------------------------------------------------------------------
 
 #define OA_THREADS 256
+// stop refining once ||E||_F is below this (the vectors are stored in complex64)
+#define OA_TOL 1e-8
 
 template <int TS, class FX, class FY, class FO>
 __device__ __attribute__((always_inline)) void oa_gemm(int nw, FX X, FY Y, FO out) {
@@ -173,10 +175,26 @@ __device__ __attribute__((always_inline)) void oa_gemm(int nw, FX X, FY Y, FO ou
             if (r0 + i < nw && c0 + j < nw) out(r0 + i, c0 + j, acc[i][j]);
 }
 
+// A = (H + H^+)/2 of k-point Hk into LDS
+__device__ inline void oa_load_a(int nw, int ld, const cplx<double>* __restrict__ Hk, cplx<double>* sA) {
+    for (int t = threadIdx.x; t < nw * nw; t += OA_THREADS) {
+        const int i = t / nw, j = t % nw;
+        const cplx<double> x = Hk[i * nw + j], y = Hk[j * nw + i];
+        sA[i * ld + j] = mk(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
+    }
+}
+
+// Refinement steps until the correction satisfies ||E||_F < tol (at most
+// `iters`), then eigenvalues as Rayleigh quotients of the final vectors.
+// Well-separated spectra stop after two steps (||E|| ~ eps32, then ~eps32^2).
+// A pair closer than delta of the first step (FP32 vectors mix it) is only
+// re-orthogonalised there; the following steps separate it, converging
+// quadratically once the mixing angle is small (six to eight steps for a
+// relative gap of 4e-6 at nw = 18..40).
 template <typename TU, int TS>
 __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<double>* __restrict__ H64,
                                                             int64_t Hstride, double* __restrict__ Eout,
-                                                            cplx<TU>* __restrict__ U) {
+                                                            cplx<TU>* __restrict__ U, int iters, double tol) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int ld = nw + 1;
     cplx<double>* sX = (cplx<double>*)smem;   // X (approximate eigenvectors)
@@ -184,84 +202,116 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
     cplx<double>* sT = sA + nw * ld;          // A X, later X + X E
     cplx<double>* sS = sT + nw * ld;          // S, later E
     double* lam = (double*)(sS + nw * ld);
-    double* red = lam + nw;                   // [0] ||S-D||^2, [1] ||R||^2, [2] max|lam|
+    double* red = lam + nw;                   // [0] ||S-D||^2, [1] ||R||^2, [2] max|lam|, [3] ||E||^2
     int* rank = (int*)(red + 4);
     const int tid = threadIdx.x;
     const int64_t k = blockIdx.x;
     const cplx<double>* Hk = H64 + k * Hstride;
     cplx<TU>* Uk = U + k * nw * nw;
     for (int t = tid; t < nw * nw; t += OA_THREADS) {
-        const int i = t / nw, j = t % nw;
         const cplx<TU> u = Uk[t];
-        sX[i * ld + j] = mk((double)u.re, (double)u.im);
-        const cplx<double> x = Hk[i * nw + j], y = Hk[j * nw + i];
-        sA[i * ld + j] = mk(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
+        sX[(t / nw) * ld + t % nw] = mk((double)u.re, (double)u.im);
     }
-    if (tid < 4) red[tid] = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        oa_load_a(nw, ld, Hk, sA);
+        if (tid < 4) red[tid] = 0.0;
+        __syncthreads();
+        // T = A X
+        oa_gemm<TS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
+                    [&](int r, int c, cplx<double> v) { sT[r * ld + c] = v; });
+        __syncthreads();
+        // S = X^+ T ; R = I - X^+ X (into sA)
+        oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
+                    [&](int m, int c) { return sT[m * ld + c]; },
+                    [&](int r, int c, cplx<double> v) { sS[r * ld + c] = v; });
+        __syncthreads();
+        oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
+                    [&](int m, int c) { return sX[m * ld + c]; },
+                    [&](int r, int c, cplx<double> v) { sA[r * ld + c] = mk((r == c ? 1.0 : 0.0) - v.re, -v.im); });
+        __syncthreads();
+        for (int i = tid; i < nw; i += OA_THREADS) lam[i] = sS[i * ld + i].re / (1.0 - sA[i * ld + i].re);
+        __syncthreads();
+        // Frobenius norms and max|lam|
+        double s2 = 0.0, r2 = 0.0, lm = 0.0;
+        for (int t = tid; t < nw * nw; t += OA_THREADS) {
+            const int i = t / nw, j = t % nw;
+            cplx<double> sv = sS[i * ld + j];
+            if (i == j) sv.re -= lam[i];
+            s2 += sv.re * sv.re + sv.im * sv.im;
+            const cplx<double> rv = sA[i * ld + j];
+            r2 += rv.re * rv.re + rv.im * rv.im;
+        }
+        for (int i = tid; i < nw; i += OA_THREADS) lm = fmax(lm, fabs(lam[i]));
+        for (int off = 32; off > 0; off >>= 1) {
+            s2 += __shfl_down(s2, off);
+            r2 += __shfl_down(r2, off);
+            lm = fmax(lm, __shfl_down(lm, off));
+        }
+        if ((tid & 63) == 0) {
+            atomicAdd(&red[0], s2);
+            atomicAdd(&red[1], r2);
+            atomicMax((unsigned long long*)&red[2], (unsigned long long)__double_as_longlong(lm));
+        }
+        __syncthreads();
+        const double delta = 2.0 * (sqrt(red[0]) + red[2] * sqrt(red[1]));
+        // E into sS, ||E||^2 into red[3]
+        double e2 = 0.0;
+        for (int t = tid; t < nw * nw; t += OA_THREADS) {
+            const int i = t / nw, j = t % nw;
+            const double dl = lam[j] - lam[i];
+            const cplx<double> r = sA[i * ld + j];
+            cplx<double> e;
+            if (fabs(dl) > delta) {
+                const cplx<double> sv = sS[i * ld + j];
+                e = (1.0 / dl) * mk(sv.re + lam[j] * r.re, sv.im + lam[j] * r.im);
+            } else {
+                e = 0.5 * r;
+            }
+            sS[i * ld + j] = e;
+            e2 += e.re * e.re + e.im * e.im;
+        }
+        for (int off = 32; off > 0; off >>= 1) e2 += __shfl_down(e2, off);
+        if ((tid & 63) == 0) atomicAdd(&red[3], e2);
+        __syncthreads();
+        // every thread reads red[3] here; it is cleared only after the next barrier
+        const bool converged = red[3] < tol * tol;
+        // X + X E into sT, which becomes X
+        oa_gemm<TS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sS[m * ld + c]; },
+                    [&](int r, int c, cplx<double> v) { sT[r * ld + c] = sX[r * ld + c] + v; });
+        __syncthreads();
+        cplx<double>* t = sX;
+        sX = sT;
+        sT = t;
+        if (converged) break;
+    }
+    // eigenvalues: Rayleigh quotients x_c^+ A x_c / x_c^+ x_c of the refined columns
+    oa_load_a(nw, ld, Hk, sA);
     __syncthreads();
-    // T = A X
     oa_gemm<TS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
                 [&](int r, int c, cplx<double> v) { sT[r * ld + c] = v; });
     __syncthreads();
-    // S = X^+ T ; R = I - X^+ X (into sA)
-    oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); }, [&](int m, int c) { return sT[m * ld + c]; },
-                [&](int r, int c, cplx<double> v) { sS[r * ld + c] = v; });
-    __syncthreads();
-    oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); }, [&](int m, int c) { return sX[m * ld + c]; },
-                [&](int r, int c, cplx<double> v) { sA[r * ld + c] = mk((r == c ? 1.0 : 0.0) - v.re, -v.im); });
-    __syncthreads();
-    for (int i = tid; i < nw; i += OA_THREADS) lam[i] = sS[i * ld + i].re / (1.0 - sA[i * ld + i].re);
-    __syncthreads();
-    // Frobenius norms and max|lam|
-    double s2 = 0.0, r2 = 0.0, lm = 0.0;
-    for (int t = tid; t < nw * nw; t += OA_THREADS) {
-        const int i = t / nw, j = t % nw;
-        cplx<double> sv = sS[i * ld + j];
-        if (i == j) sv.re -= lam[i];
-        s2 += sv.re * sv.re + sv.im * sv.im;
-        const cplx<double> rv = sA[i * ld + j];
-        r2 += rv.re * rv.re + rv.im * rv.im;
-    }
-    for (int i = tid; i < nw; i += OA_THREADS) lm = fmax(lm, fabs(lam[i]));
-    for (int off = 32; off > 0; off >>= 1) {
-        s2 += __shfl_down(s2, off);
-        r2 += __shfl_down(r2, off);
-        lm = fmax(lm, __shfl_down(lm, off));
-    }
-    if ((tid & 63) == 0) {
-        atomicAdd(&red[0], s2);
-        atomicAdd(&red[1], r2);
-        atomicMax((unsigned long long*)&red[2], (unsigned long long)__double_as_longlong(lm));
-    }
-    __syncthreads();
-    const double delta = 2.0 * (sqrt(red[0]) + red[2] * sqrt(red[1]));
-    // E into sS
-    for (int t = tid; t < nw * nw; t += OA_THREADS) {
-        const int i = t / nw, j = t % nw;
-        const double dl = lam[j] - lam[i];
-        const cplx<double> r = sA[i * ld + j];
-        cplx<double> e;
-        if (fabs(dl) > delta) {
-            const cplx<double> sv = sS[i * ld + j];
-            e = (1.0 / dl) * mk(sv.re + lam[j] * r.re, sv.im + lam[j] * r.im);
-        } else {
-            e = 0.5 * r;
+    for (int c = tid; c < nw; c += OA_THREADS) {
+        double num = 0.0, den = 0.0;
+        for (int m = 0; m < nw; ++m) {
+            const cplx<double> x = sX[m * ld + c], y = sT[m * ld + c];
+            num += x.re * y.re + x.im * y.im;
+            den += x.re * x.re + x.im * x.im;
         }
-        sS[i * ld + j] = e;
+        lam[c] = num / den;
     }
-    // ascending order of the refined eigenvalues (ties by index)
+    __syncthreads();
+    // ascending order (ties by index)
     for (int i = tid; i < nw; i += OA_THREADS) {
         int rk = 0;
         for (int j = 0; j < nw; ++j) rk += (lam[j] < lam[i]) || (lam[j] == lam[i] && j < i);
         rank[i] = rk;
     }
     __syncthreads();
-    // X + X E, written to column rank[c]
-    oa_gemm<TS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sS[m * ld + c]; },
-                [&](int r, int c, cplx<double> v) {
-                    const cplx<double> x = sX[r * ld + c] + v;
-                    Uk[r * nw + rank[c]] = mk((TU)x.re, (TU)x.im);
-                });
+    for (int t = tid; t < nw * nw; t += OA_THREADS) {
+        const int r = t / nw, c = t % nw;
+        const cplx<double> x = sX[r * ld + c];
+        Uk[r * nw + rank[c]] = mk((TU)x.re, (TU)x.im);
+    }
     for (int i = tid; i < nw; i += OA_THREADS) Eout[k * nw + rank[i]] = lam[i];
 }
 
@@ -270,6 +320,7 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
     if (!ctx) return WBMI_ERR_ARG;
     if (nk <= 0 || nw <= 0 || nw > 64 || !E_dev || !U_dev || Hstride < (int64_t)nw * nw)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad arguments (nw <= 64, Hstride >= nw*nw)");
+    if (refine < 0 || refine > 8) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refine must be 0..8");
     if (refine && (prec != WBMI_F32 || !H64_dev))
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement needs prec=WBMI_F32 and the FP64 Hamiltonian");
     if (!refine && !H_dev) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: H_dev is null");
@@ -284,7 +335,7 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
     } else if (prec == WBMI_F32) {
         size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
         if (refine) {
-            // FP32 solve of the FP64 Hamiltonian, then one FP64 refinement step
+            // FP32 solve of the FP64 Hamiltonian, then `refine` FP64 refinement steps
             WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double>,
                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             hipLaunchKernelGGL((k_eigh_jacobi<float, double>), dim3(nk), dim3(64), lds, ctx->stream, nw,
@@ -296,12 +347,12 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
                 WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<float, 2>,
                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
                 hipLaunchKernelGGL((k_eigh_refine<float, 2>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw,
-                                   (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev);
+                                   (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev, refine, OA_TOL);
             } else {
                 WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<float, 4>,
                                                         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
                 hipLaunchKernelGGL((k_eigh_refine<float, 4>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw,
-                                   (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev);
+                                   (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev, refine, OA_TOL);
             }
         } else {
             WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, float>,

@@ -21,10 +21,15 @@
 //   dD.ln = X1^{bd} = -M o [W^{bd} + G^{bd} + G^{db} - D'^b V''^d - D'^d V''^b]
 //   dD.nl = X2^{ad} = -M o [W^{ad} + V''^a D'^d + V''^d D'^a - H^{ad} - H^{da}]
 //   dA.ln = Y1^{bd} = A'^{bd} - D'^d A''^b + K^{bd}
-// (formula/elementary.py:63-70, formula/formula.py:108-118).  Each of the 27
-// dense products is computed once into LDS with register blocking, the
-// elementwise terms are folded into it, and it is consumed at once by the
-// diagonal dot products of the traces.
+// (formula/elementary.py:63-70, formula/formula.py:108-118).  X1 and X2 are
+// symmetric in (b, d), so the elementwise terms are folded half-and-half into
+// each product: Q1 = G + (W - S)/2, Q2 = -H + (W + R)/2, Y1 = K + A' - D'A''.
+// The traces use only diagonal elements of D'^a X1, X2 A^b and D'^a Y1, so
+// every product element is consumed where it is computed: each wave forms
+// one 32 x 32 block of one of the 27 products with a 4 x 4 register tile per
+// lane, adds the elementwise terms, multiplies by the consumer weights and
+// reduces over the summed index with cross-lane shuffles into its own
+// accumulator rows.  No product is stored and no barrier separates them.
 //
 // For DerOmega the operands V and A stay in LDS (rows padded by one element)
 // when nw <= 32.
@@ -34,7 +39,8 @@
 
 #include "wbmi_common.h"
 
-#define SC_THREADS 256
+#define SC_THREADS 512
+#define SC_WAVES (SC_THREADS / 64)
 #define SC_NWMAX 64
 #define SC_LDS_NW 32  // V and A live in LDS up to this nw
 
@@ -51,7 +57,7 @@ struct ScanArgs {
 struct Smem {
     cplx<double>* V;    // [3][nw][ld] (staged) or null
     cplx<double>* A;    // [3][nw][ld] (staged, external terms) or null
-    cplx<double>* P;    // [nw][ld] product buffer
+    double* accw;       // [SC_WAVES][nw][9] per-wave DerOmega accumulators
     cplx<double>* acc;  // [nw][9]
     double* M;          // [nw][ld] dEinv o Off
     double* E;
@@ -69,12 +75,12 @@ struct Smem {
 
 __host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
 
-// byte offsets of the LDS regions; off[11] = total size
+// byte offsets of the LDS regions and their total size
 struct SmemOffsets {
-    size_t V, A, P, acc, M, E, cut, gw, ints, total;
+    size_t V, A, accw, acc, M, E, cut, gw, ints, total;
 };
 
-__host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool stage_a) {
+__host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool stage_a, bool stage_w) {
     const size_t ld = nw + 1;
     SmemOffsets o;
     size_t off = 0;
@@ -82,8 +88,8 @@ __host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool s
     off = al16(off + (stage_v ? sizeof(cplx<double>) * 3 * nw * ld : 0));
     o.A = off;
     off = al16(off + (stage_a ? sizeof(cplx<double>) * 3 * nw * ld : 0));
-    o.P = off;
-    off = al16(off + sizeof(cplx<double>) * nw * ld);
+    o.accw = off;
+    off = al16(off + (stage_w ? sizeof(double) * SC_WAVES * nw * 9 : 0));
     o.acc = off;
     off = al16(off + sizeof(cplx<double>) * nw * 9);
     o.M = off;
@@ -100,12 +106,13 @@ __host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool s
     return o;
 }
 
-__device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int nw, bool stage_v, bool stage_a) {
-    const SmemOffsets o = smem_offsets(nw, stage_v, stage_a);
+__device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int nw, bool stage_v, bool stage_a,
+                                                                bool stage_w) {
+    const SmemOffsets o = smem_offsets(nw, stage_v, stage_a, stage_w);
     Smem s;
     s.V = stage_v ? (cplx<double>*)(base + o.V) : nullptr;
     s.A = stage_a ? (cplx<double>*)(base + o.A) : nullptr;
-    s.P = (cplx<double>*)(base + o.P);
+    s.accw = stage_w ? (double*)(base + o.accw) : nullptr;
     s.acc = (cplx<double>*)(base + o.acc);
     s.M = (double*)(base + o.M);
     s.E = (double*)(base + o.E);
@@ -153,48 +160,9 @@ struct KCtx {
     __device__ inline cplx<double> B(int a, int x, int y) const { return get(WBMI_ROLE_B + a, x, y); }
     __device__ inline cplx<double> C(int a, int x, int y) const { return get(WBMI_ROLE_C + a, x, y); }
     __device__ inline double m(int x, int y) const { return M[x * ld + y]; }
-    // D'^a_xy = -V^a_xy dEinv_xy Off(x, y)
-    __device__ inline cplx<double> Dp(int a, int x, int y) const {
-        double mm = M[x * ld + y];
-        if (mm == 0.0) return mk(0.0, 0.0);
-        return (-mm) * V(a, x, y);
-    }
+    // D'^a_xy = -V^a_xy dEinv_xy Off(x, y)  (branch-free: M = 0 gives an exact zero for finite V)
+    __device__ inline cplx<double> Dp(int a, int x, int y) const { return (-M[x * ld + y]) * V(a, x, y); }
 };
-
-// P[r][c] = sum_k X(r, k) Y(k, c) for r, c < nw with TS x TS register tiles
-// on a 16 x 16 thread grid (TS = 2 covers nw <= 32, TS = 4 covers nw <= 64).
-template <int TS, class FX, class FY>
-__device__ __attribute__((always_inline)) void prod_tiles(cplx<double>* P, int nw, int ld, FX X, FY Y) {
-    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
-    const int r0 = TS * tr, c0 = TS * tc;
-    if (r0 >= nw || c0 >= nw) return;
-    cplx<double> acc[TS][TS];
-#pragma unroll
-    for (int i = 0; i < TS; ++i)
-#pragma unroll
-        for (int j = 0; j < TS; ++j) acc[i][j] = mk(0.0, 0.0);
-    for (int k = 0; k < nw; ++k) {
-        cplx<double> x[TS], y[TS];
-#pragma unroll
-        for (int i = 0; i < TS; ++i) x[i] = (r0 + i < nw) ? X(r0 + i, k) : mk(0.0, 0.0);
-#pragma unroll
-        for (int j = 0; j < TS; ++j) y[j] = (c0 + j < nw) ? Y(k, c0 + j) : mk(0.0, 0.0);
-#pragma unroll
-        for (int i = 0; i < TS; ++i)
-#pragma unroll
-            for (int j = 0; j < TS; ++j) acc[i][j] += x[i] * y[j];
-    }
-#pragma unroll
-    for (int i = 0; i < TS; ++i)
-#pragma unroll
-        for (int j = 0; j < TS; ++j)
-            if (r0 + i < nw && c0 + j < nw) P[(r0 + i) * ld + c0 + j] = acc[i][j];
-}
-
-template <int TS, class FX, class FY>
-__device__ __attribute__((always_inline)) void prod(cplx<double>* P, int nw, int ld, FX X, FY Y) {
-    prod_tiles<TS>(P, nw, ld, X, Y);
-}
 
 // pre-Hermitian Omega summand s^c_xy for x, y in the same group
 // (formula/covariant.py:175-200): Omega.nn = s + s^+
@@ -281,15 +249,131 @@ __device__ __attribute__((always_inline)) void build_M(const Smem& S, int nw) {
     }
 }
 
+// acc[i][j] = sum_m X(r0 + i, m) Y(m, c0 + j) over m < nw; rows and columns
+// past nw are clamped (computed and discarded by the caller).
+template <class FX, class FY>
+__device__ __attribute__((always_inline)) void tile4(cplx<double> (&acc)[4][4], int nw, int r0, int c0, FX X, FY Y) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = mk(0.0, 0.0);
+    int rr[4], cc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        rr[i] = r0 + i < nw ? r0 + i : nw - 1;
+        cc[i] = c0 + i < nw ? c0 + i : nw - 1;
+    }
+#pragma unroll 2
+    for (int m = 0; m < nw; ++m) {
+        cplx<double> x[4], y[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) x[i] = X(rr[i], m);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) y[j] = Y(m, cc[j]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] += x[i] * y[j];
+    }
+}
+
+// sum of v over the lanes of a wave whose lane bits outside `keep` differ
+// (lane = 8 tr + tc: mask 0x38 sums over tr, 0x07 over tc)
+__device__ inline double wave_sum(double v, int mask) {
+    for (int b = 1; b < 64; b <<= 1)
+        if (mask & b) v += __shfl_xor(v, b);
+    return v;
+}
+
+// One wave task: block (rb, cb) of product `kind` for (b, d), consumed into
+// the wave's accumulator rows accw[n][comp] (real parts of the trace terms).
+//   kind 0: Q1^{bd} -> T1 (internal);  kind 1: Q2^{bd} -> T3;  kind 2: Y1^{bd} -> T2
+template <int KIND, typename T>
+__device__ __attribute__((always_inline)) void deromega_task(const Smem& S, const KCtx<T>& K, double* accw, int b,
+                                                            int d, int rb, int cb) {
+    const int nw = K.nw, lane = threadIdx.x & 63, tr = lane >> 3, tc = lane & 7;
+    const int r0 = rb * 32 + 4 * tr, c0 = cb * 32 + 4 * tc;
+    cplx<double> acc[4][4];
+    if (KIND == 0)
+        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.V(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
+    else if (KIND == 1)
+        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Dp(b, r, m); }, [&](int m, int c) { return K.V(d, m, c); });
+    else
+        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.A(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
+    // consumer targets: (partner component, accumulator column, weight)
+    constexpr int NT = KIND == 2 ? 2 : 4;
+    int tp[NT], tcol[NT];
+    double tw[NT];
+    const int nt = KIND == 2 ? 2 : (b == d ? 2 : 4);
+    const double f = (KIND != 2 && b == d) ? 2.0 : 1.0;  // both targets coincide for b == d
+#pragma unroll
+    for (int w = 0; w < NT; ++w) {
+        const Pair p = KIND == 1 ? pair_first(w < 2 ? b : d, w % 2) : pair_second(KIND == 2 ? b : (w < 2 ? b : d), w % 2);
+        const int der = KIND == 2 ? d : (w < 2 ? d : b);
+        tp[w] = p.partner;
+        tcol[w] = p.c * 3 + der;
+        tw[w] = w < nt ? (KIND == 0 ? f * p.sig : -f * p.sig) : 0.0;
+    }
+    // part[u][w]: u = the lane's own index of the band n (column j for kinds 0, 2; row i for kind 1)
+    double part[4][NT];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int w = 0; w < NT; ++w) part[u][w] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = r0 + i, c = c0 + j;
+            if (r >= nw || c >= nw) continue;
+            const int n = KIND == 1 ? r : c, l = KIND == 1 ? c : r;
+            if (!S.act[n] || K.lab[l] == K.lab[n]) continue;
+            cplx<double> z;
+            if (KIND == 0) {
+                // (W^{bd} - D'^b V''^d - D'^d V''^b)_ln / 2
+                z = K.W(b, d, l, n);
+                for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
+                const cplx<double> Pm = (-K.m(l, n)) * (acc[i][j] + 0.5 * z);
+#pragma unroll
+                for (int w = 0; w < NT; ++w)
+                    if (w < nt) {
+                        const cplx<double> v = K.Dp(tp[w], n, l) * Pm;
+                        part[j][w] += v.im;  // Re(-i v)
+                    }
+            } else if (KIND == 1) {
+                // (W^{bd} + V''^b D'^d + V''^d D'^b)_nl / 2
+                z = K.W(b, d, n, l);
+                for (int q = S.gs[n]; q < S.ge[n]; ++q) z += K.V(b, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(b, q, l);
+                const cplx<double> Pm = (-K.m(n, l)) * (0.5 * z - acc[i][j]);
+#pragma unroll
+                for (int w = 0; w < NT; ++w)
+                    if (w < nt) part[i][w] += (Pm * K.A(tp[w], l, n)).re;
+            } else {
+                // A'^{bd} - D'^d A''^b
+                z = K.Ad(b, d, l, n);
+                for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
+                const cplx<double> P = acc[i][j] + z;
+#pragma unroll
+                for (int w = 0; w < NT; ++w) part[j][w] += (K.Dp(tp[w], n, l) * P).re;
+            }
+        }
+    // reduce over the summed band index l: rows (tr) for kinds 0, 2, columns (tc) for kind 1
+    const int mask = KIND == 1 ? 0x07 : 0x38;
+    const bool writer = KIND == 1 ? tc == 0 : tr == 0;
+    const int n0 = KIND == 1 ? r0 : c0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int w = 0; w < NT; ++w) {
+            const double v = wave_sum(part[u][w], mask);
+            if (writer && w < nt && n0 + u < nw) accw[(n0 + u) * 9 + tcol[w]] += tw[w] * v;
+        }
+}
+
 // DerOmega per-band values (formula/covariant.py:227-256) into S.acc[n][c*3+d].
-// X1 and X2 are symmetric in their two indices, X1^{bd} = -M o (Q1^{bd} + Q1^{db}) and
-// X2^{ad} = -M o (Q2^{ad} + Q2^{da}), so the elementwise terms (W, within-group sums)
-// are folded half-and-half into the product buffers Q1 = G + (W - S)/2 and
-// Q2 = -H + (W + R)/2; Y1 = K + A' - D'A'' is formed the same way.  Every buffer
-// is consumed by the diagonal dot products of the traces right away.
-template <int TS, typename T>
+template <typename T>
 __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T>& K, bool in, bool ex) {
-    const int nw = K.nw, ld = K.ld, tid = threadIdx.x;
+    const int nw = K.nw, tid = threadIdx.x, wave = tid >> 6;
     // T4 and T5 (external), lanes running over n
     for (int t = tid; t < nw * 9; t += SC_THREADS) {
         const int n = t % nw, c = (t / nw) / 3, d = (t / nw) % 3;
@@ -314,110 +398,35 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
             for (int l = 0; l < nw; ++l) t5 += K.O(c, n, l) * K.Dp(d, l, n) - K.Dp(d, n, l) * K.O(c, l, n);
             s += 0.5 * t5;
         }
-        if (n < nw) S.acc[n * 9 + c * 3 + d] = s;
+        S.acc[n * 9 + c * 3 + d] = s;
+    }
+    for (int t = tid; t < SC_WAVES * nw * 9; t += SC_THREADS) S.accw[t] = 0.0;
+    __syncthreads();
+    // wave tasks: (kind, b, d, row block, column block), round-robin over the waves
+    const int nblk = (nw + 31) / 32, nb2 = nblk * nblk;
+    const int kind0 = in ? 0 : 1, nkinds = (in ? 1 : 0) + (ex ? 2 : 0);
+    const int ntask = nkinds * 9 * nb2;
+    double* accw = S.accw + wave * nw * 9;
+    for (int t = wave; t < ntask; t += SC_WAVES) {
+        const int kind = kind0 + t / (9 * nb2), rem = t % (9 * nb2);
+        const int b = (rem / nb2) / 3, d = (rem / nb2) % 3, rb = (rem % nb2) / nblk, cb = rem % nblk;
+        if (kind == 0)
+            deromega_task<0>(S, K, accw, b, d, rb, cb);
+        else if (kind == 1)
+            deromega_task<1>(S, K, accw, b, d, rb, cb);
+        else
+            deromega_task<2>(S, K, accw, b, d, rb, cb);
     }
     __syncthreads();
-    cplx<double>* P = S.P;
-    for (int b = 0; b < 3; ++b) {
-        for (int d = 0; d < 3; ++d) {
-            // both targets coincide for b == d: one lane adds twice the value
-            const int nt = b == d ? 2 : 4;
-            const double f = b == d ? 2.0 : 1.0;
-            if (in) {
-                // Q1^{bd} = V^b D'^d + (W^{bd} - D'^b V''^d - D'^d V''^b) / 2
-                prod<TS>(P, nw, ld, [&](int r, int k) { return K.V(b, r, k); },
-                     [&](int k, int c) { return K.Dp(d, k, c); });
-                __syncthreads();
-                for (int t = tid; t < nw * nw; t += SC_THREADS) {
-                    const int l = t / nw, n = t % nw;
-                    if (!S.act[n] || K.lab[l] == K.lab[n]) continue;
-                    cplx<double> z = K.W(b, d, l, n);
-                    for (int q = S.gs[n]; q < S.ge[n]; ++q)
-                        z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
-                    P[l * ld + n] += 0.5 * z;
-                }
-                __syncthreads();
-                // T1: -i sigma sum_l D'^a_nl X1_ln, X1 = -M o (Q1^{bd} + Q1^{db})
-                for (int t = tid; t < nw * nt; t += SC_THREADS) {
-                    const int n = t / nt, w = t % nt;
-                    if (!S.act[n]) continue;
-                    const Pair p = pair_second(w < 2 ? b : d, w % 2);
-                    const int der = w < 2 ? d : b;
-                    cplx<double> s = mk(0.0, 0.0);
-#pragma unroll 4
-                    for (int l = 0; l < nw; ++l) {
-                        const double mln = K.m(l, n);
-                        if (mln == 0.0) continue;
-                        s += K.Dp(p.partner, n, l) * ((-mln) * P[l * ld + n]);
-                    }
-                    S.acc[n * 9 + p.c * 3 + der] += times_i(-((f * p.sig) * s));
-                }
-                __syncthreads();
-            }
-            if (ex) {
-                // Q2^{bd} = -D'^b V^d + (W^{bd} + V''^b D'^d + V''^d D'^b) / 2
-                prod<TS>(P, nw, ld, [&](int r, int k) { return K.Dp(b, r, k); },
-                     [&](int k, int c) { return K.V(d, k, c); });
-                __syncthreads();
-                for (int t = tid; t < nw * nw; t += SC_THREADS) {
-                    const int n = t / nw, l = t % nw;
-                    if (!S.act[n] || K.lab[l] == K.lab[n]) continue;
-                    cplx<double> z = K.W(b, d, n, l);
-                    for (int q = S.gs[n]; q < S.ge[n]; ++q)
-                        z += K.V(b, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(b, q, l);
-                    P[n * ld + l] = 0.5 * z - P[n * ld + l];
-                }
-                __syncthreads();
-                // T3: -sigma sum_l X2_nl A^b_ln, X2 = -M o (Q2^{bd} + Q2^{db})
-                for (int t = tid; t < nw * nt; t += SC_THREADS) {
-                    const int n = t / nt, w = t % nt;
-                    if (!S.act[n]) continue;
-                    const Pair p = pair_first(w < 2 ? b : d, w % 2);
-                    const int der = w < 2 ? d : b;
-                    cplx<double> s = mk(0.0, 0.0);
-#pragma unroll 4
-                    for (int l = 0; l < nw; ++l) {
-                        const double mnl = K.m(n, l);
-                        if (mnl == 0.0) continue;
-                        s += ((-mnl) * P[n * ld + l]) * K.A(p.partner, l, n);
-                    }
-                    S.acc[n * 9 + p.c * 3 + der] -= (f * p.sig) * s;
-                }
-                __syncthreads();
-                // Y1^{bd} = A^b D'^d + A'^{bd} - D'^d A''^b
-                prod<TS>(P, nw, ld, [&](int r, int k) { return K.A(b, r, k); },
-                     [&](int k, int c) { return K.Dp(d, k, c); });
-                __syncthreads();
-                for (int t = tid; t < nw * nw; t += SC_THREADS) {
-                    const int l = t / nw, n = t % nw;
-                    if (!S.act[n] || K.lab[l] == K.lab[n]) continue;
-                    cplx<double> z = K.Ad(b, d, l, n);
-                    for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
-                    P[l * ld + n] += z;
-                }
-                __syncthreads();
-                // T2: -sigma sum_l D'^a_nl Y1_ln
-                for (int t = tid; t < nw * 2; t += SC_THREADS) {
-                    const int n = t / 2, w = t % 2;
-                    if (!S.act[n]) continue;
-                    const Pair p = pair_second(b, w);
-                    cplx<double> s = mk(0.0, 0.0);
-#pragma unroll 4
-                    for (int l = 0; l < nw; ++l) {
-                        if (K.lab[l] == K.lab[n]) continue;
-                        s += K.Dp(p.partner, n, l) * P[l * ld + n];
-                    }
-                    S.acc[n * 9 + p.c * 3 + d] -= p.sig * s;
-                }
-                __syncthreads();
-            }
-        }
+    for (int t = tid; t < nw * 9; t += SC_THREADS) {
+        double v = S.acc[t].re;
+        for (int w = 0; w < SC_WAVES; ++w) v += S.accw[w * nw * 9 + t];
+        S.acc[t] = mk(2.0 * v, 0.0);
     }
-    for (int t = tid; t < nw * 9; t += SC_THREADS) S.acc[t] = mk(2.0 * S.acc[t].re, 0.0);
 }
 
 // Per-band values for the current partition into S.acc[n][comp] (.re).
-template <int F, int TS, typename T>
+template <int F, typename T>
 __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T>& K, const wbmi_scan_params& p) {
     const int nw = K.nw;
     const int tid = threadIdx.x;
@@ -455,7 +464,7 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
             S.acc[t] = mk(v.re, 0.0);
         }
     } else if (F == WBMI_FORMULA_DEROMEGA) {
-        eval_deromega<TS>(S, K, in, ex);
+        eval_deromega(S, K, in, ex);
     }
     __syncthreads();
 }
@@ -471,8 +480,7 @@ __device__ inline int fermi_bin(const wbmi_scan_params& p, double E) {
     return -1;
 }
 
-// TS: product tile (2 for nw <= 32, 4 for nw <= 64)
-template <typename T, int F, int TS>
+template <typename T, int F>
 __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
                                                             const cplx<T>* __restrict__ Xall,
                                                             double* __restrict__ hist) {
@@ -483,7 +491,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const int tid = threadIdx.x;
     const bool stage_v = args.stage_va && F == WBMI_FORMULA_DEROMEGA;
     const bool stage_a = stage_v && p.external_terms;
-    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a);
+    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, F == WBMI_FORMULA_DEROMEGA);
     for (int n = tid; n < nw; n += SC_THREADS) S.E[n] = Eall[k * nw + n];
     for (int r = tid; r < WBMI_NROLES; r += SC_THREADS) S.ch[r] = p.ch[r];
     __syncthreads();
@@ -565,7 +573,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 S.act[n] = S.gin[g];
             }
         __syncthreads();
-        eval_partition<F, TS>(S, K, p);
+        eval_partition<F>(S, K, p);
         for (int t = tid; t < ng * ncomp; t += SC_THREADS) {
             const int g = t / ncomp, c = t % ncomp;
             const int bin = S.gin[g] ? fermi_bin(p, S.gw[g]) : -1;
@@ -585,7 +593,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 S.act[n] = inn;
             }
             __syncthreads();
-            eval_partition<F, TS>(S, K, p);
+            eval_partition<F>(S, K, p);
             for (int c = tid; c < ncomp; c += SC_THREADS) {
                 double s = 0.0;
                 for (int n = 0; n < bandmax; ++n) s += S.acc[n * 9 + c].re;
@@ -608,7 +616,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 S.act[n] = inn;
             }
             __syncthreads();
-            eval_partition<F, TS>(S, K, p);
+            eval_partition<F>(S, K, p);
             if (tid == 0)
                 for (int c = 0; c < ncomp; ++c) {
                     double s = 0.0;
@@ -667,25 +675,18 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
     return WBMI_OK;
 }
 
-template <typename T, int F, int TS>
-static int launch_scan_ts(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    const bool stage_v = a.stage_va && F == WBMI_FORMULA_DEROMEGA;
-    const bool stage_a = stage_v && a.p.external_terms;
-    size_t lds = smem_offsets(a.nw, stage_v, stage_a).total;
-    if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, TS>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_static_scan<T, F, TS>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
-                       (const cplx<T>*)X, hist);
-    WBMI_LAUNCH_CHECK(ctx);
-    return WBMI_OK;
-}
-
 template <typename T, int F>
 static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    // only DerOmega forms dense products; the other formulas need no tile variants
-    if (F == WBMI_FORMULA_DEROMEGA && a.nw > 32) return launch_scan_ts<T, F, 4>(ctx, nk, a, E, X, hist);
-    return launch_scan_ts<T, F, 2>(ctx, nk, a, E, X, hist);
+    const bool stage_v = a.stage_va && F == WBMI_FORMULA_DEROMEGA;
+    const bool stage_a = stage_v && a.p.external_terms;
+    size_t lds = smem_offsets(a.nw, stage_v, stage_a, F == WBMI_FORMULA_DEROMEGA).total;
+    if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_static_scan<T, F>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E, (const cplx<T>*)X,
+                       hist);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
 }
 
 template <typename T>

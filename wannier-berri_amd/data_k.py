@@ -30,6 +30,10 @@ CHANNEL_ORDER = list(CHANNEL_NCOMP)
 # matrices forced Hermitian after R->k (data_K/data_K_R.py:87)
 HERMITIAN = {"AA", "rotAA"}
 BASES = ["Ham", "AA", "BB", "CC"]
+# At most this many FP64 Ogita-Aishima steps after the FP32 Jacobi of the
+# mixed form (the kernel stops at ||E||_F < 1e-8: two steps for separated
+# spectra, up to eight for pairs closer than ~1e-5 |H|)
+OA_STEPS = 8
 _ALPHA = [1, 2, 0]
 _BETA = [2, 0, 1]
 
@@ -221,7 +225,7 @@ class Data_K:
 
         FP64 form: one complex128 stack, FP64 Jacobi.  Mixed form: the
         Hamiltonian is transformed in FP64 on its own, the other channels in
-        complex64; FP32 Jacobi + one FP64 Ogita-Aishima step give FP64
+        complex64; FP32 Jacobi + FP64 Ogita-Aishima steps (at most OA_STEPS) give FP64
         energies and refined eigenvectors (stored as complex64).
         """
         torch = self.torch
@@ -238,7 +242,7 @@ class Data_K:
         X = self._transform(main, self.cdtype, self.prec) if main else None
         if mixed:
             H64 = self._transform([("Ham", 0)], torch.complex128, WBMI_F64)
-            self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=True, H64=H64)
+            self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=OA_STEPS, H64=H64)
             del H64
         else:
             h0 = slots[("Ham", 0)]
