@@ -147,9 +147,13 @@ int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride,
  * Replaces Data_K._rotate (data_K/data_K.py:130-132) for the channels of a
  * per-k stack X[k, nch, nw, nw] (in place).  herm_host[c] = 1 forces the
  * channel Hermitian first (0.5 (X + X^+), fourier/fft.py:183-186,
- * data_K_R.py:87).  Channels with skip_host[c] = 1 are left untouched.     */
+ * data_K_R.py:87).  Channels with skip_host[c] = 1 are left untouched.
+ * fp64_host[c] = 1 (prec = WBMI_F32): rotate that complex64 channel with FP64
+ * arithmetic (only the stored result is rounded); other complex64 channels
+ * use FP32 arithmetic.  Any of the three flag arrays may be NULL (all 0).  */
 int wbmi_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_dev,
-                void* X_dev, const int32_t* herm_host, const int32_t* skip_host, int prec);
+                void* X_dev, const int32_t* herm_host, const int32_t* skip_host,
+                const int32_t* fp64_host, int prec);
 
 /* ---- covariant formula traces + Fermi scan --------------------------------
  * Replaces the per-(k, band-group) loop and Fermi scan of

@@ -90,14 +90,16 @@ class StaticCalculator(Calculator):
         if f == _lib.FORMULA_IDENTITY:
             return {("Ham", 0)}
         need = {("Ham", 0), ("Ham", 1)}
-        if f == _lib.FORMULA_DEROMEGA:
+        if f in (_lib.FORMULA_DEROMEGA, _lib.FORMULA_DERMORB):
             need.add(("Ham", 2))
         if ext:
             need |= {("AA", 0), ("rotAA", 0)}
-            if f == _lib.FORMULA_DEROMEGA:
+            if f in (_lib.FORMULA_DEROMEGA, _lib.FORMULA_DERMORB):
                 need |= {("AA", 1), ("rotAA", 1)}
-            if f == _lib.FORMULA_MORB_HPM:
+            if f in (_lib.FORMULA_MORB_HPM, _lib.FORMULA_VELHPLUS, _lib.FORMULA_DERMORB):
                 need |= {("BB", 0), ("CC", 0)}
+            if f == _lib.FORMULA_DERMORB:
+                need |= {("BB", 1), ("CC", 1)}
         return need
 
     def scan_params(self, system):
@@ -229,5 +231,53 @@ class Morb(StaticCalculator):
         return Hplus * data_K.cell_volume
 
 
+class GME_orb_FermiSurf(StaticCalculator):
+    r"""Gyrotropic tensor, orbital part, as a Fermi-surface integral (calculators/static.py:266-281)
+
+        | Output: :math:`K^{orb}_{\alpha :\mu} = \int [dk] v_\alpha m^{orb}_\mu f'`,
+        | :math:`m^{orb} = H + G - 2E_f \cdot \Omega`"""
+    formula = _lib.FORMULA_VELHPLUS
+    ndim = 2
+    fder = 1
+    sign = 1
+
+    def __init__(self, constant_factor=factors.factor_gme_orb, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+        self.BerryDipole = BerryDipole_FermiSurf(constant_factor=constant_factor, print_comment=False, **kwargs)
+
+    def needed_channels(self, system):
+        return super().needed_channels(system) | self.BerryDipole.needed_channels(system)
+
+    def __call__(self, data_K):
+        data_K.ensure(self.needed_channels(data_K.system))
+        Hplus = super().__call__(data_K)
+        Omega = self.BerryDipole(data_K).mul_array(self.Efermi)
+        return Hplus - 2 * Omega
+
+
+class GME_orb_FermiSea(StaticCalculator):
+    r"""Gyrotropic tensor, orbital part, as a Fermi-sea integral (calculators/static.py:284-300)
+
+        | Output: :math:`K^{orb}_{\alpha :\mu} = -\int [dk] \partial_\alpha m_\mu f`,
+        | :math:`m = H + G - 2E_f \cdot \Omega`"""
+    formula = _lib.FORMULA_DERMORB
+    ndim = 2
+    fder = 0
+    sign = 1
+
+    def __init__(self, constant_factor=factors.factor_gme_orb, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+        self.BerryDipole = BerryDipole_FermiSea(constant_factor=constant_factor, print_comment=False, **kwargs)
+
+    def needed_channels(self, system):
+        return super().needed_channels(system) | self.BerryDipole.needed_channels(system)
+
+    def __call__(self, data_K):
+        data_K.ensure(self.needed_channels(data_K.system))
+        Hplus = super().__call__(data_K).swapaxes(1, 2)
+        Omega = self.BerryDipole(data_K).mul_array(self.Efermi)
+        return Hplus - 2 * Omega
+
+
 __all__ = ["Calculator", "StaticCalculator", "DOS", "CumDOS", "AHC", "BerryDipole_FermiSurf", "NLAHC_FermiSurf",
-           "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "VoidResult"]
+           "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "GME_orb_FermiSurf", "GME_orb_FermiSea", "VoidResult"]

@@ -24,12 +24,33 @@ struct Eps<float> {
     static constexpr float v = 1.1920929e-07f;
 };
 
-// pair i of round r in the circle ordering of n (even) players
+// pair i of round r (0 <= i < n/2, 0 <= r < n-1) in the circle ordering of n
+// (even) players: i + r - 1 < 2(n-1) and n - 2 - i + r < 2(n-1), so one
+// conditional subtraction replaces the modulo.
 __device__ inline void rr_pair(int n, int r, int i, int& p, int& q) {
-    int m = n - 1;
-    p = (i == 0) ? 0 : ((i + r - 1) % m) + 1;
-    q = ((n - 1 - i + r - 1) % m) + 1;
+    const int m = n - 1;
+    int a = i + r - 1;
+    a = a >= m ? a - m : a;
+    p = (i == 0) ? 0 : a + 1;
+    int b = n - 2 - i + r;
+    b = b >= m ? b - m : b;
+    q = b + 1;
 }
+
+// Items (pair, index) of a round, pair-major, dealt to the 64 lanes:
+// lane t starts at item t and steps by 64 items.
+struct ItemWalk {
+    int pr, i, dp, di;
+    __device__ ItemWalk(int n, int tid) : pr(tid / n), i(tid % n), dp(64 / n), di(64 % n) {}
+    __device__ inline void next(int n) {
+        pr += dp;
+        i += di;
+        if (i >= n) {
+            i -= n;
+            ++pr;
+        }
+    }
+};
 
 template <typename T, typename Tin>
 __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __restrict__ H, int64_t Hstride,
@@ -38,24 +59,26 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
     extern __shared__ char smem[];
     const int n = nw + (nw & 1);  // padded to even; the dummy index is decoupled
     const int ld = n + 1;         // LDS row stride (one element of padding against bank conflicts)
+    const int np = n / 2;
     cplx<T>* A = (cplx<T>*)smem;
     cplx<T>* V = A + n * ld;
-    T* prm = (T*)(V + n * ld);  // per pair: c, s*cos(phi), s*sin(phi)
-    int* flag = (int*)(prm + 3 * (n / 2));
+    T* prm = (T*)(V + n * ld);        // per pair: c, s*cos(phi), s*sin(phi)
+    int* ppq = (int*)(prm + 3 * np);  // per pair: p, q
+    int* flag = ppq + 2 * np;
     const int tid = threadIdx.x;
     const int64_t k = blockIdx.x;
     const cplx<Tin>* Hk = H + k * Hstride;
 
-    for (int t = tid; t < n * n; t += 64) {
-        int i = t / n, j = t % n;
-        cplx<T> a = mk((T)0, (T)0);
-        if (i < nw && j < nw) {
-            cplx<Tin> x = Hk[i * nw + j], y = Hk[j * nw + i];
-            a = mk((T)(0.5 * (x.re + y.re)), (T)(0.5 * (x.im - y.im)));
+    for (int i = tid; i < n; i += 64)
+        for (int j = 0; j < n; ++j) {
+            cplx<T> a = mk((T)0, (T)0);
+            if (i < nw && j < nw) {
+                cplx<Tin> x = Hk[i * nw + j], y = Hk[j * nw + i];
+                a = mk((T)(0.5 * (x.re + y.re)), (T)(0.5 * (x.im - y.im)));
+            }
+            A[i * ld + j] = a;
+            V[i * ld + j] = mk((T)(i == j ? 1 : 0), (T)0);
         }
-        A[i * ld + j] = a;
-        V[i * ld + j] = mk((T)(i == j ? 1 : 0), (T)0);
-    }
     __syncthreads();
 
     const T eps = Eps<T>::v;
@@ -63,7 +86,7 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
         if (tid == 0) *flag = 0;
         __syncthreads();
         for (int r = 0; r < n - 1; ++r) {
-            if (tid < n / 2) {
+            if (tid < np) {
                 int p, q;
                 rr_pair(n, r, tid, p, q);
                 cplx<T> apq = A[p * ld + q];
@@ -83,16 +106,16 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
                 prm[3 * tid] = c;
                 prm[3 * tid + 1] = sc;
                 prm[3 * tid + 2] = ss;
+                ppq[2 * tid] = p;
+                ppq[2 * tid + 1] = q;
             }
             __syncthreads();
             // columns of A and V:  col_p <- c col_p - s e^{-i phi} col_q ; col_q <- s e^{i phi} col_p + c col_q
-            for (int t = tid; t < n * (n / 2); t += 64) {
-                int pr = t / n, i = t % n;
-                int p, q;
-                rr_pair(n, r, pr, p, q);
-                T c = prm[3 * pr];
-                cplx<T> se = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
+            for (ItemWalk w(n, tid); w.pr < np; w.next(n)) {
+                const cplx<T> se = mk(prm[3 * w.pr + 1], prm[3 * w.pr + 2]);
                 if (se.re == 0 && se.im == 0) continue;
+                const T c = prm[3 * w.pr];
+                const int p = ppq[2 * w.pr], q = ppq[2 * w.pr + 1], i = w.i;
                 cplx<T> ap = A[i * ld + p], aq = A[i * ld + q];
                 A[i * ld + p] = c * ap - mulc(aq, se);
                 A[i * ld + q] = se * ap + c * aq;
@@ -102,13 +125,11 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
             }
             __syncthreads();
             // rows of A:  row_p <- c row_p - s e^{i phi} row_q ; row_q <- s e^{-i phi} row_p + c row_q
-            for (int t = tid; t < n * (n / 2); t += 64) {
-                int pr = t / n, j = t % n;
-                int p, q;
-                rr_pair(n, r, pr, p, q);
-                T c = prm[3 * pr];
-                cplx<T> se = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
+            for (ItemWalk w(n, tid); w.pr < np; w.next(n)) {
+                const cplx<T> se = mk(prm[3 * w.pr + 1], prm[3 * w.pr + 2]);
                 if (se.re == 0 && se.im == 0) continue;
+                const T c = prm[3 * w.pr];
+                const int p = ppq[2 * w.pr], q = ppq[2 * w.pr + 1], j = w.i;
                 cplx<T> ap = A[p * ld + j], aq = A[q * ld + j];
                 A[p * ld + j] = c * ap - se * aq;
                 A[q * ld + j] = mulc(ap, se) + c * aq;
@@ -327,13 +348,13 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
     int n = nw + (nw & 1);
     const int sweeps = 40;
     if (prec == WBMI_F64) {
-        size_t lds = 2 * sizeof(cplx<double>) * n * (n + 1) + 3 * sizeof(double) * (n / 2) + 16;
+        size_t lds = 2 * sizeof(cplx<double>) * n * (n + 1) + 3 * sizeof(double) * (n / 2) + 2 * sizeof(int) * (n / 2) + 16;
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<double, double>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL((k_eigh_jacobi<double, double>), dim3(nk), dim3(64), lds, ctx->stream, nw,
                            (const cplx<double>*)H_dev, Hstride, E_dev, (cplx<double>*)U_dev, sweeps);
     } else if (prec == WBMI_F32) {
-        size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
+        size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 2 * sizeof(int) * (n / 2) + 16;
         if (refine) {
             // FP32 solve of the FP64 Hamiltonian, then `refine` FP64 refinement steps
             WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double>,

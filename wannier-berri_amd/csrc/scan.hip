@@ -159,6 +159,11 @@ struct KCtx {
     __device__ inline cplx<double> Od(int c, int d, int x, int y) const { return get(WBMI_ROLE_OD + 3 * c + d, x, y); }
     __device__ inline cplx<double> B(int a, int x, int y) const { return get(WBMI_ROLE_B + a, x, y); }
     __device__ inline cplx<double> C(int a, int x, int y) const { return get(WBMI_ROLE_C + a, x, y); }
+    __device__ inline cplx<double> Bd(int a, int d, int x, int y) const { return get(WBMI_ROLE_BD + 3 * a + d, x, y); }
+    __device__ inline cplx<double> Cd(int c, int d, int x, int y) const { return get(WBMI_ROLE_CD + 3 * c + d, x, y); }
+    // X o On: elements between bands of the same group
+    __device__ inline cplx<double> Ain(int a, int x, int y) const { return lab[x] == lab[y] ? A(a, x, y) : mk(0.0, 0.0); }
+    __device__ inline cplx<double> Vin(int a, int x, int y) const { return lab[x] == lab[y] ? V(a, x, y) : mk(0.0, 0.0); }
     __device__ inline double m(int x, int y) const { return M[x * ld + y]; }
     // D'^a_xy = -V^a_xy dEinv_xy Off(x, y)  (branch-free: M = 0 gives an exact zero for finite V)
     __device__ inline cplx<double> Dp(int a, int x, int y) const { return (-M[x * ld + y]) * V(a, x, y); }
@@ -210,7 +215,7 @@ __host__ __device__ constexpr int ncomp_of() {
 template <int F>
 __host__ __device__ constexpr bool uses_A() {
     return F == WBMI_FORMULA_OMEGA || F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
-           F == WBMI_FORMULA_MORB_HPM;
+           F == WBMI_FORMULA_MORB_HPM || F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERMORB;
 }
 
 // The two curl components c whose index pair (alpha_c, beta_c) contains x.
@@ -285,36 +290,133 @@ __device__ inline double wave_sum(double v, int mask) {
     return v;
 }
 
-// One wave task: block (rb, cb) of product `kind` for (b, d), consumed into
-// the wave's accumulator rows accw[n][comp] (real parts of the trace terms).
-//   kind 0: Q1^{bd} -> T1 (internal);  kind 1: Q2^{bd} -> T3;  kind 2: Y1^{bd} -> T2
+// Wave tasks.  One wave forms block (rb, cb) (32 x 32) of a dense product
+// P = X Y with a 4 x 4 register tile per lane, adds the elementwise terms of
+// each element (x, y) and accumulates the consumers' weighted elements into
+// its accumulator rows accw[band][comp] after a shuffle reduction over the
+// summed index.  The band of an element is its column y, except for kind 1
+// (row x).  Kinds (task indices b, d; for kinds 5, 6 b = c):
+//   0  Q1^{bd} = V^b D'^d + (W - D'^b V''^d - D'^d V''^b)/2, y in the partition, x off-side:
+//      DerOmega T1; DerMorb s2 (x 2 E_x), s7 (conj B^a_xy), s1 (raw V^b D'^d)
+//   1  Q2^{bd} = -D'^b V^d + (W + V''^b D'^d + V''^d D'^b)/2, x in, y off-side: DerOmega T3
+//   2  Y1^{bd} = A^b D'^d + A' - D'^d A''^b, x off-side: DerOmega T2;
+//      x same-side (DerMorb): s5 terms of A' + A^b D'^d
+//   3  YB^{bd} = B^b D'^d + B' - D'^d B''^b, x off-side: DerMorb s6
+//   4  DA^{bd} = D'^d A^b, same side: DerMorb s5 (-D'^d A^b) and the Omega block (-D'^al A^be + D'^be A^al)
+//   5  DD^c = D'^al D'^be, same side: Omega block, internal term
+//   6  AA^c = A''^al A''^be, same side: Omega block, external term
+//   7  VA^{d be} = V''^d A''^be, same side: DerMorb s4
+// DerOmega contributions carry the band weight dw(n) = 2 (DerOmega) or
+// 2 sign E_n (DerMorb's Eav o dOmega term; only cut partitions are summed, so a
+// same-side element may be credited to either of its bands).
+struct TaskCtx {
+    bool dm, in, ex;
+    double sign;
+};
+
+template <int KIND>
+__host__ __device__ constexpr int task_nt() {
+    return KIND == 0 ? 5 : KIND == 1 ? 4 : KIND == 2 ? 2 : KIND == 3 ? 2 : KIND == 4 ? 5 : KIND == 7 ? 1 : 3;
+}
+
 template <int KIND, typename T>
-__device__ __attribute__((always_inline)) void deromega_task(const Smem& S, const KCtx<T>& K, double* accw, int b,
-                                                            int d, int rb, int cb) {
+__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T>& K, const TaskCtx& tc_,
+                                                           double* accw, int b, int d, int rb, int cb) {
     const int nw = K.nw, lane = threadIdx.x & 63, tr = lane >> 3, tc = lane & 7;
     const int r0 = rb * 32 + 4 * tr, c0 = cb * 32 + 4 * tc;
+    const bool dm = tc_.dm, in = tc_.in, ex = tc_.ex;
+    const double sign = tc_.sign;
+    const int al = c_alpha[b], be = c_beta[b];  // kinds 5, 6: b = c
     cplx<double> acc[4][4];
-    if (KIND == 0)
+    if constexpr (KIND == 0)
         tile4(acc, nw, r0, c0, [&](int r, int m) { return K.V(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
-    else if (KIND == 1)
+    else if constexpr (KIND == 1)
         tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Dp(b, r, m); }, [&](int m, int c) { return K.V(d, m, c); });
-    else
+    else if constexpr (KIND == 2)
         tile4(acc, nw, r0, c0, [&](int r, int m) { return K.A(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
-    // consumer targets: (partner component, accumulator column, weight)
-    constexpr int NT = KIND == 2 ? 2 : 4;
+    else if constexpr (KIND == 3)
+        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.B(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
+    else if constexpr (KIND == 4)
+        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Dp(d, r, m); }, [&](int m, int c) { return K.A(b, m, c); });
+    else if constexpr (KIND == 5)
+        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Dp(al, r, m); }, [&](int m, int c) { return K.Dp(be, m, c); });
+    else if constexpr (KIND == 6)
+        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Ain(al, r, m); }, [&](int m, int c) { return K.Ain(be, m, c); });
+    else
+        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Vin(b, r, m); }, [&](int m, int c) { return K.Ain(d, m, c); });
+
+    // targets: partner component tp, accumulator column tcol, factor tw
+    constexpr int NT = task_nt<KIND>();
     int tp[NT], tcol[NT];
     double tw[NT];
-    const int nt = KIND == 2 ? 2 : (b == d ? 2 : 4);
-    const double f = (KIND != 2 && b == d) ? 2.0 : 1.0;  // both targets coincide for b == d
 #pragma unroll
     for (int w = 0; w < NT; ++w) {
-        const Pair p = KIND == 1 ? pair_first(w < 2 ? b : d, w % 2) : pair_second(KIND == 2 ? b : (w < 2 ? b : d), w % 2);
-        const int der = KIND == 2 ? d : (w < 2 ? d : b);
-        tp[w] = p.partner;
-        tcol[w] = p.c * 3 + der;
-        tw[w] = w < nt ? (KIND == 0 ? f * p.sig : -f * p.sig) : 0.0;
+        tp[w] = 0;
+        tcol[w] = 0;
+        tw[w] = 0.0;
     }
-    // part[u][w]: u = the lane's own index of the band n (column j for kinds 0, 2; row i for kind 1)
+    if constexpr (KIND == 0 || KIND == 1) {
+        // the two index pairs of X1 / X2: targets coincide for b == d (one lane adds twice)
+        const int nt = b == d ? 2 : 4;
+        const double f = b == d ? 2.0 : 1.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const Pair p = KIND == 1 ? pair_first(w < 2 ? b : d, w % 2) : pair_second(w < 2 ? b : d, w % 2);
+            tp[w] = p.partner;
+            tcol[w] = p.c * 3 + (w < 2 ? d : b);
+            tw[w] = w < nt ? f * p.sig : 0.0;
+        }
+        if constexpr (KIND == 0) {
+            // s1: the c with beta_c = d, partner alpha_c, column (c, b)
+            for (int c = 0; c < 3; ++c)
+                if (c_beta[c] == d) {
+                    tp[4] = c_alpha[c];
+                    tcol[4] = c * 3 + b;
+                    tw[4] = 2.0;
+                }
+        }
+    } else if constexpr (KIND == 2 || KIND == 3) {
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const Pair p = pair_second(b, w);
+            tp[w] = p.partner;
+            tcol[w] = p.c * 3 + d;
+            tw[w] = p.sig;
+        }
+    } else if constexpr (KIND == 4) {
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const Pair p = pair_second(b, w);
+            tp[w] = p.partner;
+            tcol[w] = p.c * 3 + d;
+            tw[w] = p.sig;
+        }
+        // Omega block: D'^d A^b enters s^c with -1 for (al_c, be_c) = (d, b), +1 for (b, d)
+        for (int c = 0; c < 3; ++c) {
+            const double coef = (c_alpha[c] == d && c_beta[c] == b) ? -1.0 : (c_alpha[c] == b && c_beta[c] == d) ? 1.0 : 0.0;
+            if (coef != 0.0)
+                for (int e = 0; e < 3; ++e) {
+                    tp[2 + e] = e;
+                    tcol[2 + e] = c * 3 + e;
+                    tw[2 + e] = 2.0 * sign * coef;
+                }
+        }
+    } else if constexpr (KIND == 5 || KIND == 6) {
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            tp[e] = e;
+            tcol[e] = b * 3 + e;
+            tw[e] = 2.0 * sign;
+        }
+    } else {
+        for (int c = 0; c < 3; ++c)
+            if (c_beta[c] == d) {
+                tp[0] = c_alpha[c];
+                tcol[0] = c * 3 + b;
+                tw[0] = 2.0;
+            }
+    }
+
     double part[4][NT];
 #pragma unroll
     for (int u = 0; u < 4; ++u)
@@ -324,40 +426,75 @@ __device__ __attribute__((always_inline)) void deromega_task(const Smem& S, cons
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const int r = r0 + i, c = c0 + j;
-            if (r >= nw || c >= nw) continue;
-            const int n = KIND == 1 ? r : c, l = KIND == 1 ? c : r;
-            if (!S.act[n] || K.lab[l] == K.lab[n]) continue;
-            cplx<double> z;
-            if (KIND == 0) {
+            const int x = r0 + i, y = c0 + j;
+            if (x >= nw || y >= nw) continue;
+            const int n = KIND == 1 ? x : y, l = KIND == 1 ? y : x;
+            if (!S.act[n]) continue;
+            const bool same = K.lab[l] == K.lab[n];
+            const double dw = dm ? 2.0 * sign * K.E[n] : 2.0;
+            if constexpr (KIND == 0) {
+                if (same) continue;
                 // (W^{bd} - D'^b V''^d - D'^d V''^b)_ln / 2
-                z = K.W(b, d, l, n);
+                cplx<double> z = K.W(b, d, l, n);
                 for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
                 const cplx<double> Pm = (-K.m(l, n)) * (acc[i][j] + 0.5 * z);
+                const double wD = (in ? dw : 0.0) + (dm && in ? 2.0 * K.E[l] : 0.0);
 #pragma unroll
-                for (int w = 0; w < NT; ++w)
-                    if (w < nt) {
-                        const cplx<double> v = K.Dp(tp[w], n, l) * Pm;
-                        part[j][w] += v.im;  // Re(-i v)
-                    }
-            } else if (KIND == 1) {
+                for (int w = 0; w < 4; ++w) {
+                    double v = wD * (K.Dp(tp[w], n, l) * Pm).im;  // Re(-i D' X1) (T1, s2)
+                    if (dm && ex) v -= 2.0 * (conj(K.B(tp[w], l, n)) * Pm).re;  // s7
+                    part[j][w] += v;
+                }
+                if (dm && in) part[j][4] += (K.Dp(tp[4], n, l) * acc[i][j]).im;  // s1: Re(-i D'^al V^d D'^be)
+            } else if constexpr (KIND == 1) {
+                if (same) continue;
                 // (W^{bd} + V''^b D'^d + V''^d D'^b)_nl / 2
-                z = K.W(b, d, n, l);
+                cplx<double> z = K.W(b, d, n, l);
                 for (int q = S.gs[n]; q < S.ge[n]; ++q) z += K.V(b, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(b, q, l);
                 const cplx<double> Pm = (-K.m(n, l)) * (0.5 * z - acc[i][j]);
 #pragma unroll
-                for (int w = 0; w < NT; ++w)
-                    if (w < nt) part[i][w] += (Pm * K.A(tp[w], l, n)).re;
-            } else {
-                // A'^{bd} - D'^d A''^b
-                z = K.Ad(b, d, l, n);
-                for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
+                for (int w = 0; w < 4; ++w) part[i][w] -= dw * (Pm * K.A(tp[w], l, n)).re;
+            } else if constexpr (KIND == 2) {
+                if (!same) {
+                    // A'^{bd} - D'^d A''^b
+                    cplx<double> z = K.Ad(b, d, l, n);
+                    for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
+                    const cplx<double> P = acc[i][j] + z;
+#pragma unroll
+                    for (int w = 0; w < 2; ++w) part[j][w] -= dw * (K.Dp(tp[w], n, l) * P).re;
+                } else if (dm) {
+                    // s5: Re(-2i sig A^a_nl E_l (A' + A^b D'^d)_ln)
+                    const cplx<double> P = acc[i][j] + K.Ad(b, d, l, n);
+#pragma unroll
+                    for (int w = 0; w < 2; ++w) part[j][w] += 2.0 * K.E[l] * (K.A(tp[w], n, l) * P).im;
+                }
+            } else if constexpr (KIND == 3) {
+                if (same) continue;
+                // B'^{bd} - D'^d B''^b
+                cplx<double> z = K.Bd(b, d, l, n);
+                for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.B(b, q, n);
                 const cplx<double> P = acc[i][j] + z;
 #pragma unroll
-                for (int w = 0; w < NT; ++w) part[j][w] += (K.Dp(tp[w], n, l) * P).re;
+                for (int w = 0; w < 2; ++w) part[j][w] -= 2.0 * (K.Dp(tp[w], n, l) * P).re;  // s6
+            } else if constexpr (KIND == 4) {
+                if (!same) continue;
+                // s5: -Re(-2i sig A^a_nl E_l (D'^d A^b)_ln) ; Omega block: 2 sign coef Re(DA_ln V^e_nl)
+#pragma unroll
+                for (int w = 0; w < 2; ++w) part[j][w] -= 2.0 * K.E[l] * (K.A(tp[w], n, l) * acc[i][j]).im;
+#pragma unroll
+                for (int w = 2; w < 5; ++w) part[j][w] += (acc[i][j] * K.V(tp[w], n, l)).re;
+            } else if constexpr (KIND == 5 || KIND == 6) {
+                if (!same) continue;
+                // Omega block term -i P_ln: 2 sign Re(-i P_ln V^e_nl)
+#pragma unroll
+                for (int w = 0; w < 3; ++w) part[j][w] += (acc[i][j] * K.V(tp[w], n, l)).im;
+            } else {
+                if (!same) continue;
+                // s4: Re(-2i A^al_np (V''^d A''^be)_pn)
+                part[j][0] += (K.A(tp[0], n, l) * acc[i][j]).im;
             }
         }
-    // reduce over the summed band index l: rows (tr) for kinds 0, 2, columns (tc) for kind 1
+    // reduce over the summed index: rows (tr), or columns (tc) for kind 1
     const int mask = KIND == 1 ? 0x07 : 0x38;
     const bool writer = KIND == 1 ? tc == 0 : tr == 0;
     const int n0 = KIND == 1 ? r0 : c0;
@@ -366,19 +503,25 @@ __device__ __attribute__((always_inline)) void deromega_task(const Smem& S, cons
 #pragma unroll
         for (int w = 0; w < NT; ++w) {
             const double v = wave_sum(part[u][w], mask);
-            if (writer && w < nt && n0 + u < nw) accw[(n0 + u) * 9 + tcol[w]] += tw[w] * v;
+            if (writer && tw[w] != 0.0 && n0 + u < nw) accw[(n0 + u) * 9 + tcol[w]] += tw[w] * v;
         }
 }
 
-// DerOmega per-band values (formula/covariant.py:227-256) into S.acc[n][c*3+d].
+// DerOmega (formula/covariant.py:212-259) or, dm = true, DerMorb
+// (formula/covariant.py:463-552) per-band values into S.acc[n][c*3+d].
+// DerMorb = DerMorb_H + sign [Eav o dOmega + (Omega V + V Omega)/2] on cut
+// partitions; its trace terms (s1 ... s7, the Omega block) are listed at
+// formula_task.
 template <typename T>
-__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T>& K, bool in, bool ex) {
+__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T>& K, bool in, bool ex,
+                                                            bool dm, int sign) {
     const int nw = K.nw, tid = threadIdx.x, wave = tid >> 6;
-    // T4 and T5 (external), lanes running over n
+    // per-band terms, lanes running over (n, c, d)
     for (int t = tid; t < nw * 9; t += SC_THREADS) {
         const int n = t % nw, c = (t / nw) / 3, d = (t / nw) % 3;
-        cplx<double> s = mk(0.0, 0.0);
+        double v = 0.0;
         if (S.act[n] && ex) {
+            cplx<double> s = mk(0.0, 0.0);
             const int g0 = S.gs[n], g1 = S.ge[n];
             for (int sg = 0; sg < 2; ++sg) {
                 const int a = sg == 0 ? c_alpha[c] : c_beta[c];
@@ -397,31 +540,69 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
 #pragma unroll 4
             for (int l = 0; l < nw; ++l) t5 += K.O(c, n, l) * K.Dp(d, l, n) - K.Dp(d, n, l) * K.O(c, l, n);
             s += 0.5 * t5;
+            v = (dm ? 2.0 * sign * S.E[n] : 2.0) * s.re;
+            if (dm) {
+                // s3: dH.nn = C'^{cd}_nn - (D'^d C^c)_nn + (C^c D'^d)_nn
+                cplx<double> s3 = K.Cd(c, d, n, n);
+                // Omega block, elementwise part: 2 sign Re sum_{l in G} (O^c_nl / 2) V^d_ln
+                cplx<double> ov = mk(0.0, 0.0);
+                for (int l = 0; l < nw; ++l) {
+                    s3 += K.C(c, n, l) * K.Dp(d, l, n) - K.Dp(d, n, l) * K.C(c, l, n);
+                    if (K.lab[l] == K.lab[n]) ov += K.O(c, n, l) * K.V(d, l, n);
+                }
+                v += s3.re + sign * ov.re;
+            }
         }
-        S.acc[n * 9 + c * 3 + d] = s;
+        S.acc[n * 9 + c * 3 + d] = mk(v, 0.0);
     }
     for (int t = tid; t < SC_WAVES * nw * 9; t += SC_THREADS) S.accw[t] = 0.0;
     __syncthreads();
-    // wave tasks: (kind, b, d, row block, column block), round-robin over the waves
+    // wave tasks (kind, b, d, row block, column block), round-robin over the waves
+    const TaskCtx tcx{dm, in, ex, (double)sign};
+    int cnt[8];
+    cnt[0] = (in || (dm && ex)) ? 9 : 0;
+    cnt[1] = ex ? 9 : 0;
+    cnt[2] = ex ? 9 : 0;
+    cnt[3] = dm && ex ? 9 : 0;
+    cnt[4] = dm && ex ? 9 : 0;
+    cnt[5] = dm && in && sign != 0 ? 3 : 0;
+    cnt[6] = dm && ex && sign != 0 ? 3 : 0;
+    cnt[7] = dm && ex ? 9 : 0;
+    if (dm && sign == 0) cnt[1] = 0;  // T3 feeds only the dOmega term
     const int nblk = (nw + 31) / 32, nb2 = nblk * nblk;
-    const int kind0 = in ? 0 : 1, nkinds = (in ? 1 : 0) + (ex ? 2 : 0);
-    const int ntask = nkinds * 9 * nb2;
+    int ntask = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) ntask += cnt[q] * nb2;
     double* accw = S.accw + wave * nw * 9;
     for (int t = wave; t < ntask; t += SC_WAVES) {
-        const int kind = kind0 + t / (9 * nb2), rem = t % (9 * nb2);
-        const int b = (rem / nb2) / 3, d = (rem / nb2) % 3, rb = (rem % nb2) / nblk, cb = rem % nblk;
-        if (kind == 0)
-            deromega_task<0>(S, K, accw, b, d, rb, cb);
-        else if (kind == 1)
-            deromega_task<1>(S, K, accw, b, d, rb, cb);
-        else
-            deromega_task<2>(S, K, accw, b, d, rb, cb);
+        int kind = -1, rem = 0, base = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int nq = cnt[q] * nb2;
+            if (kind < 0 && t - base < nq) {
+                kind = q;
+                rem = t - base;
+            }
+            base += nq;
+        }
+        const int idx = rem / nb2, rb = (rem % nb2) / nblk, cb = rem % nblk;
+        const int b = kind == 5 || kind == 6 ? idx : idx / 3, d = kind == 5 || kind == 6 ? 0 : idx % 3;
+        switch (kind) {
+            case 0: formula_task<0>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 1: formula_task<1>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 2: formula_task<2>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 3: formula_task<3>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 4: formula_task<4>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 5: formula_task<5>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 6: formula_task<6>(S, K, tcx, accw, b, d, rb, cb); break;
+            default: formula_task<7>(S, K, tcx, accw, b, d, rb, cb); break;
+        }
     }
     __syncthreads();
     for (int t = tid; t < nw * 9; t += SC_THREADS) {
         double v = S.acc[t].re;
         for (int w = 0; w < SC_WAVES; ++w) v += S.accw[w * nw * 9 + t];
-        S.acc[t] = mk(2.0 * v, 0.0);
+        S.acc[t] = mk(v, 0.0);
     }
 }
 
@@ -463,8 +644,27 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
             }
             S.acc[t] = mk(v.re, 0.0);
         }
+    } else if (F == WBMI_FORMULA_VELHPLUS) {
+        // Re sum_{n' in G} V^a_nn' Hpm^b_n'n with the group block of Morb_Hpm,
+        // Hpm = Morb_H.nn + sign Eav o Omega.nn (formula/covariant.py:424-449,
+        // formula/formula.py:141-148; FormulaProduct is additive)
+        for (int t = tid; t < nw * 9; t += SC_THREADS) {
+            int n = t / 9, a = (t % 9) / 3, b = t % 3;
+            if (!S.act[n]) continue;
+            cplx<double> v = mk(0.0, 0.0);
+            for (int m = S.gs[n]; m < S.ge[n]; ++m) {
+                cplx<double> h = morbh_s(K, b, m, n, in, ex) + conj(morbh_s(K, b, n, m, in, ex));
+                if (p.sign != 0)
+                    h += (p.sign * 0.5 * (S.E[m] + S.E[n])) *
+                         (omega_s(K, b, m, n, in, ex) + conj(omega_s(K, b, n, m, in, ex)));
+                v += K.V(a, n, m) * h;
+            }
+            S.acc[t] = mk(v.re, 0.0);
+        }
     } else if (F == WBMI_FORMULA_DEROMEGA) {
-        eval_deromega(S, K, in, ex);
+        eval_deromega(S, K, in, ex, false, 0);
+    } else if (F == WBMI_FORMULA_DERMORB) {
+        eval_deromega(S, K, in, ex, true, p.sign);
     }
     __syncthreads();
 }
@@ -489,9 +689,10 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const int nw = args.nw, ld = nw + 1;
     const int64_t k = blockIdx.x;
     const int tid = threadIdx.x;
-    const bool stage_v = args.stage_va && F == WBMI_FORMULA_DEROMEGA;
+    constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
+    const bool stage_v = args.stage_va && tasks;
     const bool stage_a = stage_v && p.external_terms;
-    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, F == WBMI_FORMULA_DEROMEGA);
+    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tasks);
     for (int n = tid; n < nw; n += SC_THREADS) S.E[n] = Eall[k * nw + n];
     for (int r = tid; r < WBMI_NROLES; r += SC_THREADS) S.ch[r] = p.ch[r];
     __syncthreads();
@@ -511,7 +712,8 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     }
     constexpr int ncomp = ncomp_of<F>();
     constexpr bool additive = F == WBMI_FORMULA_IDENTITY || F == WBMI_FORMULA_OMEGA ||
-                              F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA;
+                              F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
+                              F == WBMI_FORMULA_VELHPLUS;
 
     // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186)
     if (tid == 0) {
@@ -663,7 +865,15 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
                        (rc = need(WBMI_ROLE_OD, 9))))
                 return rc;
             break;
+        case WBMI_FORMULA_DERMORB:
+            if ((rc = need(WBMI_ROLE_V, 3)) || (rc = need(WBMI_ROLE_W, 9))) return rc;
+            if (ex && ((rc = need(WBMI_ROLE_A, 3)) || (rc = need(WBMI_ROLE_AD, 9)) || (rc = need(WBMI_ROLE_O, 3)) ||
+                       (rc = need(WBMI_ROLE_OD, 9)) || (rc = need(WBMI_ROLE_B, 3)) || (rc = need(WBMI_ROLE_BD, 9)) ||
+                       (rc = need(WBMI_ROLE_C, 3)) || (rc = need(WBMI_ROLE_CD, 9))))
+                return rc;
+            break;
         case WBMI_FORMULA_MORB_HPM:
+        case WBMI_FORMULA_VELHPLUS:
             if ((rc = need(WBMI_ROLE_V, 3))) return rc;
             if (ex && ((rc = need(WBMI_ROLE_A, 3)) || (rc = need(WBMI_ROLE_O, 3)) || (rc = need(WBMI_ROLE_B, 3)) ||
                        (rc = need(WBMI_ROLE_C, 3))))
@@ -677,9 +887,10 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
 
 template <typename T, int F>
 static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    const bool stage_v = a.stage_va && F == WBMI_FORMULA_DEROMEGA;
+    constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
+    const bool stage_v = a.stage_va && tasks;
     const bool stage_a = stage_v && a.p.external_terms;
-    size_t lds = smem_offsets(a.nw, stage_v, stage_a, F == WBMI_FORMULA_DEROMEGA).total;
+    size_t lds = smem_offsets(a.nw, stage_v, stage_a, tasks).total;
     if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
     WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -697,6 +908,8 @@ static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
         case WBMI_FORMULA_DEROMEGA: return launch_scan<T, WBMI_FORMULA_DEROMEGA>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_VELOMEGA: return launch_scan<T, WBMI_FORMULA_VELOMEGA>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_MORB_HPM: return launch_scan<T, WBMI_FORMULA_MORB_HPM>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_VELHPLUS: return launch_scan<T, WBMI_FORMULA_VELHPLUS>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_DERMORB: return launch_scan<T, WBMI_FORMULA_DERMORB>(ctx, nk, a, E, X, hist);
         default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
     }
 }

@@ -34,6 +34,11 @@ BASES = ["Ham", "AA", "BB", "CC"]
 # mixed form (the kernel stops at ||E||_F < 1e-8: two steps for separated
 # spectra, up to eight for pairs closer than ~1e-5 |H|)
 OA_STEPS = 8
+# Mixed form: channels rotated with FP64 arithmetic on their complex64 values.
+# They feed the first-order formulas (Omega, Morb) and the products of
+# DerOmega; FP32 accumulation would leave a normwise error ~eps32 |X| in
+# small elements between close bands, which 1/(E_n - E_m)^2 amplifies.
+FP64_ROTATION = {("Ham", 1), ("AA", 0), ("rotAA", 0), ("BB", 0), ("CC", 0)}
 _ALPHA = [1, 2, 0]
 _BETA = [2, 0, 1]
 
@@ -250,13 +255,16 @@ class Data_K:
         if X is not None:
             herm = np.zeros(nch, dtype=np.int32)
             skip = np.zeros(nch, dtype=np.int32)
+            fp64 = np.zeros(nch, dtype=np.int32)
             for key, sl in slots.items():
                 if key[0] in HERMITIAN:
                     herm[sl:sl + CHANNEL_NCOMP[key]] = 1
+                if key in FP64_ROTATION:
+                    fp64[sl:sl + CHANNEL_NCOMP[key]] = 1
             if ("Ham", 0) in slots:
                 skip[slots[("Ham", 0)]] = 1
             if skip.sum() < nch:
-                self.ctx.rotate(U, X, herm, skip, self.prec)
+                self.ctx.rotate(U, X, herm, skip, self.prec, fp64)
         self.Xk, self.E_dev, self.U_dev, self._slots, self.nch = X, E, U, slots, nch
 
     def role_table(self):
