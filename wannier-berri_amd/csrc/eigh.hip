@@ -13,6 +13,9 @@
 
 #include "wbmi_common.h"
 
+// update items per lane per batch (loads of a batch are issued together)
+#define JB 8
+
 template <typename T>
 struct Eps;
 template <>
@@ -62,9 +65,8 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
     const int np = n / 2;
     cplx<T>* A = (cplx<T>*)smem;
     cplx<T>* V = A + n * ld;
-    T* prm = (T*)(V + n * ld);        // per pair: c, s*cos(phi), s*sin(phi)
-    int* ppq = (int*)(prm + 3 * np);  // per pair: p, q
-    int* flag = ppq + 2 * np;
+    T* prm = (T*)(V + n * ld);  // per pair: c, s*cos(phi), s*sin(phi)
+    int* flag = (int*)(prm + 3 * np);
     const int tid = threadIdx.x;
     const int64_t k = blockIdx.x;
     const cplx<Tin>* Hk = H + k * Hstride;
@@ -86,6 +88,7 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
         if (tid == 0) *flag = 0;
         __syncthreads();
         for (int r = 0; r < n - 1; ++r) {
+            bool rot = false;
             if (tid < np) {
                 int p, q;
                 rr_pair(n, r, tid, p, q);
@@ -101,38 +104,71 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
                     T s = t * c;
                     sc = s * apq.re / g;
                     ss = s * apq.im / g;
+                    rot = true;
                     *flag = 1;
                 }
                 prm[3 * tid] = c;
                 prm[3 * tid + 1] = sc;
                 prm[3 * tid + 2] = ss;
-                ppq[2 * tid] = p;
-                ppq[2 * tid + 1] = q;
             }
+            // the workgroup is one wavefront: __any is uniform over it
+            if (!__any(rot)) continue;
             __syncthreads();
             // columns of A and V:  col_p <- c col_p - s e^{-i phi} col_q ; col_q <- s e^{i phi} col_p + c col_q
-            for (ItemWalk w(n, tid); w.pr < np; w.next(n)) {
-                const cplx<T> se = mk(prm[3 * w.pr + 1], prm[3 * w.pr + 2]);
-                if (se.re == 0 && se.im == 0) continue;
-                const T c = prm[3 * w.pr];
-                const int p = ppq[2 * w.pr], q = ppq[2 * w.pr + 1], i = w.i;
-                cplx<T> ap = A[i * ld + p], aq = A[i * ld + q];
-                A[i * ld + p] = c * ap - mulc(aq, se);
-                A[i * ld + q] = se * ap + c * aq;
-                cplx<T> vp = V[i * ld + p], vq = V[i * ld + q];
-                V[i * ld + p] = c * vp - mulc(vq, se);
-                V[i * ld + q] = se * vp + c * vq;
+            // Items of a phase touch disjoint elements: each lane loads JB items, then rotates and stores them.
+            for (ItemWalk w(n, tid); w.pr < np;) {
+                int ii[JB], pp[JB], qq[JB];
+                bool ok[JB];
+                T cc[JB];
+                cplx<T> se[JB], ap[JB], aq[JB], vp[JB], vq[JB];
+#pragma unroll
+                for (int u = 0; u < JB; ++u) {
+                    ok[u] = w.pr < np;
+                    const int pr = ok[u] ? w.pr : 0;
+                    rr_pair(n, r, pr, pp[u], qq[u]);
+                    ii[u] = w.i;
+                    w.next(n);
+                    cc[u] = prm[3 * pr];
+                    se[u] = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
+                    ap[u] = A[ii[u] * ld + pp[u]];
+                    aq[u] = A[ii[u] * ld + qq[u]];
+                    vp[u] = V[ii[u] * ld + pp[u]];
+                    vq[u] = V[ii[u] * ld + qq[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < JB; ++u) {
+                    if (!ok[u]) continue;
+                    A[ii[u] * ld + pp[u]] = cc[u] * ap[u] - mulc(aq[u], se[u]);
+                    A[ii[u] * ld + qq[u]] = se[u] * ap[u] + cc[u] * aq[u];
+                    V[ii[u] * ld + pp[u]] = cc[u] * vp[u] - mulc(vq[u], se[u]);
+                    V[ii[u] * ld + qq[u]] = se[u] * vp[u] + cc[u] * vq[u];
+                }
             }
             __syncthreads();
             // rows of A:  row_p <- c row_p - s e^{i phi} row_q ; row_q <- s e^{-i phi} row_p + c row_q
-            for (ItemWalk w(n, tid); w.pr < np; w.next(n)) {
-                const cplx<T> se = mk(prm[3 * w.pr + 1], prm[3 * w.pr + 2]);
-                if (se.re == 0 && se.im == 0) continue;
-                const T c = prm[3 * w.pr];
-                const int p = ppq[2 * w.pr], q = ppq[2 * w.pr + 1], j = w.i;
-                cplx<T> ap = A[p * ld + j], aq = A[q * ld + j];
-                A[p * ld + j] = c * ap - se * aq;
-                A[q * ld + j] = mulc(ap, se) + c * aq;
+            for (ItemWalk w(n, tid); w.pr < np;) {
+                int jj[JB], pp[JB], qq[JB];
+                bool ok[JB];
+                T cc[JB];
+                cplx<T> se[JB], ap[JB], aq[JB];
+#pragma unroll
+                for (int u = 0; u < JB; ++u) {
+                    ok[u] = w.pr < np;
+                    const int pr = ok[u] ? w.pr : 0;
+                    rr_pair(n, r, pr, pp[u], qq[u]);
+                    jj[u] = w.i;
+                    w.next(n);
+                    cc[u] = prm[3 * pr];
+                    se[u] = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
+                    ap[u] = A[pp[u] * ld + jj[u]];
+                    aq[u] = A[qq[u] * ld + jj[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < JB; ++u) {
+                    if (!ok[u]) continue;
+                    A[pp[u] * ld + jj[u]] = cc[u] * ap[u] - se[u] * aq[u];
+                    A[qq[u] * ld + jj[u]] = mulc(ap[u], se[u]) + cc[u] * aq[u];
+                }
             }
             __syncthreads();
         }
@@ -187,7 +223,7 @@ __device__ __attribute__((always_inline)) void oa_gemm(int nw, FX X, FY Y, FO ou
 #pragma unroll
         for (int i = 0; i < TS; ++i)
 #pragma unroll
-            for (int j = 0; j < TS; ++j) acc[i][j] += x[i] * y[j];
+            for (int j = 0; j < TS; ++j) cfma(acc[i][j], x[i], y[j]);
     }
 #pragma unroll
     for (int i = 0; i < TS; ++i)
@@ -348,13 +384,13 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
     int n = nw + (nw & 1);
     const int sweeps = 40;
     if (prec == WBMI_F64) {
-        size_t lds = 2 * sizeof(cplx<double>) * n * (n + 1) + 3 * sizeof(double) * (n / 2) + 2 * sizeof(int) * (n / 2) + 16;
+        size_t lds = 2 * sizeof(cplx<double>) * n * (n + 1) + 3 * sizeof(double) * (n / 2) + 16;
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<double, double>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL((k_eigh_jacobi<double, double>), dim3(nk), dim3(64), lds, ctx->stream, nw,
                            (const cplx<double>*)H_dev, Hstride, E_dev, (cplx<double>*)U_dev, sweeps);
     } else if (prec == WBMI_F32) {
-        size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 2 * sizeof(int) * (n / 2) + 16;
+        size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
         if (refine) {
             // FP32 solve of the FP64 Hamiltonian, then `refine` FP64 refinement steps
             WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double>,

@@ -51,7 +51,7 @@ __device__ inline void rot_channel(int nw, int ld, const cplx<T>* sU, const cplx
 #pragma unroll
             for (int i = 0; i < TS; ++i)
 #pragma unroll
-                for (int j = 0; j < TS; ++j) acc[i][j] += x[i] * u[j];
+                for (int j = 0; j < TS; ++j) cfma(acc[i][j], x[i], u[j]);
         }
 #pragma unroll
         for (int i = 0; i < TS; ++i)
@@ -78,7 +78,7 @@ __device__ inline void rot_channel(int nw, int ld, const cplx<T>* sU, const cplx
 #pragma unroll
             for (int i = 0; i < TS; ++i)
 #pragma unroll
-                for (int j = 0; j < TS; ++j) acc[i][j] += cmul(u[i], t[j]);
+                for (int j = 0; j < TS; ++j) cfma_cj(acc[i][j], u[i], t[j]);
         }
 #pragma unroll
         for (int i = 0; i < TS; ++i)
@@ -131,9 +131,147 @@ __global__ void __launch_bounds__(ROT_THREADS) k_rotate(int nw, int nch, const c
     }
 }
 
+// nw <= 32: each wave rotates its own channels (ROTW_WAVES per round, one per
+// wave) with a 4 x 4 register tile per lane on an 8 x 8 lane grid.  U and the
+// tiles are held in FP64 whatever the storage type (without packed math FP32
+// and FP64 FMAs issue at the same rate, so complex64 channels are rotated
+// with FP64 arithmetic at no cost).  A wave owns one LDS tile: X is staged
+// there, T = X U is formed in registers (Hermitising X on the fly when
+// flagged), overwrites the tile after a barrier, and Xbar = U^+ T is stored
+// from registers.  One tile per wave lets eight waves share the CU's LDS.
+// Barriers stay in uniform control flow.
+#define ROTW_WAVES 8
+
+struct Tile4Idx {
+    int r0, c0, rr[4], cc[4];
+    __device__ Tile4Idx(int nw) {
+        const int lane = threadIdx.x & 63;
+        r0 = 4 * (lane >> 3);
+        c0 = 4 * (lane & 7);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            rr[i] = r0 + i < nw ? r0 + i : nw - 1;
+            cc[i] = c0 + i < nw ? c0 + i : nw - 1;
+        }
+    }
+};
+
+__device__ inline bool bit_of(uint64_t lo, uint64_t hi, int c) { return c < 64 ? (lo >> c) & 1 : (hi >> (c - 64)) & 1; }
+
+__host__ __device__ inline size_t rotw_tile_bytes(int nw) { return sizeof(cplx<double>) * nw * (nw + 1); }
+
+template <typename T>
+__global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, const cplx<T>* __restrict__ U,
+                                                             cplx<T>* X, uint64_t herm_lo, uint64_t herm_hi,
+                                                             uint64_t skip_lo, uint64_t skip_hi) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int ld = nw + 1, nw2 = nw * nw;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    cplx<double>* sU = (cplx<double>*)smem;
+    cplx<double>* tile = sU + (1 + wave) * nw * ld;
+    const int64_t k = blockIdx.x;
+    const cplx<T>* Uk = U + k * nw2;
+    for (int t = tid; t < nw2; t += 64 * ROTW_WAVES) {
+        const cplx<T> u = Uk[t];
+        sU[(t / nw) * ld + t % nw] = mk((double)u.re, (double)u.im);
+    }
+    int nrot = 0;
+    for (int c = 0; c < nch; ++c) nrot += !bit_of(skip_lo, skip_hi, c);
+    // channel of rotation index `want` (scan of the skip mask)
+    auto channel = [&](int want) {
+        int seen = 0;
+        for (int c = 0; c < nch; ++c) {
+            if (bit_of(skip_lo, skip_hi, c)) continue;
+            if (seen == want) return c;
+            ++seen;
+        }
+        return -1;
+    };
+    const Tile4Idx ix(nw);
+    for (int base = 0; base < nrot; base += ROTW_WAVES) {
+        const int c = base + wave < nrot ? channel(base + wave) : -1;
+        cplx<T>* Xc = X + (k * nch + (c < 0 ? 0 : c)) * nw2;
+        const bool herm = c >= 0 && bit_of(herm_lo, herm_hi, c);
+        __syncthreads();  // sU staged; the previous round's tiles consumed
+        if (c >= 0)
+            for (int t = lane; t < nw2; t += 64) {
+                const cplx<T> x = Xc[t];
+                tile[(t / nw) * ld + t % nw] = mk((double)x.re, (double)x.im);
+            }
+        __syncthreads();
+        cplx<double> acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = mk(0.0, 0.0);
+        if (c >= 0) {
+            // T = X U, X = 0.5 (X + X^+) when flagged
+#pragma unroll 2
+            for (int m = 0; m < nw; ++m) {
+                cplx<double> x[4], u[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    x[i] = tile[ix.rr[i] * ld + m];
+                    if (herm) {
+                        const cplx<double> w = tile[m * ld + ix.rr[i]];
+                        x[i] = mk(0.5 * (x[i].re + w.re), 0.5 * (x[i].im - w.im));
+                    }
+                    u[i] = sU[m * ld + ix.cc[i]];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) cfma(acc[i][j], x[i], u[j]);
+            }
+        }
+        __syncthreads();  // every read of X is done before T overwrites the tile
+        if (c >= 0)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (ix.r0 + i < nw && ix.c0 + j < nw) tile[(ix.r0 + i) * ld + ix.c0 + j] = acc[i][j];
+                    acc[i][j] = mk(0.0, 0.0);
+                }
+        __syncthreads();
+        if (c >= 0) {
+            // Xbar = U^+ T
+#pragma unroll 2
+            for (int m = 0; m < nw; ++m) {
+                cplx<double> u[4], t[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    u[i] = sU[m * ld + ix.rr[i]];
+                    t[i] = tile[m * ld + ix.cc[i]];
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) cfma_cj(acc[i][j], u[i], t[j]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (ix.r0 + i < nw && ix.c0 + j < nw)
+                        Xc[(ix.r0 + i) * nw + ix.c0 + j] = mk((T)acc[i][j].re, (T)acc[i][j].im);
+        }
+    }
+}
+
 template <typename T>
 static int launch_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_dev, void* X_dev, const uint64_t* h,
                          const uint64_t* s, const uint64_t* f) {
+    if (nw <= 32) {
+        // every channel in FP64 arithmetic here (fp64 flags are implied)
+        const size_t lds = (1 + ROTW_WAVES) * rotw_tile_bytes(nw);
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_rotate_w<T>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_rotate_w<T>, dim3(nk), dim3(64 * ROTW_WAVES), lds, ctx->stream, nw, nch,
+                           (const cplx<T>*)U_dev, (cplx<T>*)X_dev, h[0], h[1], s[0], s[1]);
+        WBMI_LAUNCH_CHECK(ctx);
+        return WBMI_OK;
+    }
     const size_t tile = sizeof(cplx<T>) * nw * (nw + 1);
     const size_t ttile = sizeof(cplx<double>) * nw * (nw + 1);  // T = X U (FP64 for flagged channels)
     const int stage_x = 2 * tile + ttile <= 160 * 1024;

@@ -278,7 +278,7 @@ __device__ __attribute__((always_inline)) void tile4(cplx<double> (&acc)[4][4], 
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] += x[i] * y[j];
+            for (int j = 0; j < 4; ++j) cfma(acc[i][j], x[i], y[j]);
     }
 }
 

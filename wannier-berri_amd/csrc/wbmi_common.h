@@ -64,6 +64,22 @@ template <typename T>
 __host__ __device__ inline cplx<T> times_i(cplx<T> a) { return mk(-a.im, a.re); }
 template <typename T>
 __host__ __device__ inline cplx<T>& operator+=(cplx<T>& a, cplx<T> b) { a.re += b.re; a.im += b.im; return a; }
+// acc += a b and acc += conj(a) b as four fused multiply-adds (a product
+// followed by an add would round twice and cost six operations)
+template <typename T>
+__host__ __device__ inline void cfma(cplx<T>& acc, cplx<T> a, cplx<T> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(-a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(a.im, b.re, acc.im);
+}
+template <typename T>
+__host__ __device__ inline void cfma_cj(cplx<T>& acc, cplx<T> a, cplx<T> b) {
+    acc.re = fma(a.re, b.re, acc.re);
+    acc.re = fma(a.im, b.im, acc.re);
+    acc.im = fma(a.re, b.im, acc.im);
+    acc.im = fma(-a.im, b.re, acc.im);
+}
 template <typename T>
 __host__ __device__ inline cplx<T>& operator-=(cplx<T>& a, cplx<T> b) { a.re -= b.re; a.im -= b.im; return a; }
 

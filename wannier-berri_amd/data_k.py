@@ -34,10 +34,12 @@ BASES = ["Ham", "AA", "BB", "CC"]
 # mixed form (the kernel stops at ||E||_F < 1e-8: two steps for separated
 # spectra, up to eight for pairs closer than ~1e-5 |H|)
 OA_STEPS = 8
-# Mixed form: channels rotated with FP64 arithmetic on their complex64 values.
-# They feed the first-order formulas (Omega, Morb) and the products of
-# DerOmega; FP32 accumulation would leave a normwise error ~eps32 |X| in
-# small elements between close bands, which 1/(E_n - E_m)^2 amplifies.
+# Mixed form, nw > 32: channels rotated with FP64 arithmetic on their
+# complex64 values (for nw <= 32 the rotation kernel uses FP64 arithmetic for
+# every channel).  They feed the first-order formulas (Omega, Morb) and the
+# products of DerOmega; FP32 accumulation would leave a normwise error
+# ~eps32 |X| in small elements between close bands, which 1/(E_n - E_m)^2
+# amplifies.
 FP64_ROTATION = {("Ham", 1), ("AA", 0), ("rotAA", 0), ("BB", 0), ("CC", 0)}
 _ALPHA = [1, 2, 0]
 _BETA = [2, 0, 1]
