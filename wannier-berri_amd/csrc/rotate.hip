@@ -13,6 +13,7 @@
 // the 1/(E_n - E_m)^2 of the Berry curvature amplifies for small elements
 // between close bands; FP64 arithmetic leaves only the componentwise rounding
 // of the stored result.
+#include "mfma_c64.h"
 #include "wbmi_common.h"
 
 #define ROT_THREADS 256
@@ -132,29 +133,15 @@ __global__ void __launch_bounds__(ROT_THREADS) k_rotate(int nw, int nch, const c
 }
 
 // nw <= 32: each wave rotates its own channels (ROTW_WAVES per round, one per
-// wave) with a 4 x 4 register tile per lane on an 8 x 8 lane grid.  U and the
+// wave) on the matrix cores (FP64 MFMA, mfma_c64.h).  U and the
 // tiles are held in FP64 whatever the storage type (without packed math FP32
 // and FP64 FMAs issue at the same rate, so complex64 channels are rotated
 // with FP64 arithmetic at no cost).  A wave owns one LDS tile: X is staged
-// there, T = X U is formed in registers (Hermitising X on the fly when
-// flagged), overwrites the tile after a barrier, and Xbar = U^+ T is stored
-// from registers.  One tile per wave lets eight waves share the CU's LDS.
+// there, T = X U is formed in the MFMA accumulators (Hermitising X on the fly
+// when flagged), overwrites the tile after a barrier, and Xbar = U^+ T is
+// stored from the accumulators.  One tile per wave lets eight waves share the CU's LDS.
 // Barriers stay in uniform control flow.
 #define ROTW_WAVES 8
-
-struct Tile4Idx {
-    int r0, c0, rr[4], cc[4];
-    __device__ Tile4Idx(int nw) {
-        const int lane = threadIdx.x & 63;
-        r0 = 4 * (lane >> 3);
-        c0 = 4 * (lane & 7);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            rr[i] = r0 + i < nw ? r0 + i : nw - 1;
-            cc[i] = c0 + i < nw ? c0 + i : nw - 1;
-        }
-    }
-};
 
 __device__ inline bool bit_of(uint64_t lo, uint64_t hi, int c) { return c < 64 ? (lo >> c) & 1 : (hi >> (c - 64)) & 1; }
 
@@ -187,74 +174,70 @@ __global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, c
         }
         return -1;
     };
-    const Tile4Idx ix(nw);
     for (int base = 0; base < nrot; base += ROTW_WAVES) {
         const int c = base + wave < nrot ? channel(base + wave) : -1;
         cplx<T>* Xc = X + (k * nch + (c < 0 ? 0 : c)) * nw2;
         const bool herm = c >= 0 && bit_of(herm_lo, herm_hi, c);
         __syncthreads();  // sU staged; the previous round's tiles consumed
-        if (c >= 0)
-            for (int t = lane; t < nw2; t += 64) {
-                const cplx<T> x = Xc[t];
-                tile[(t / nw) * ld + t % nw] = mk((double)x.re, (double)x.im);
-            }
-        __syncthreads();
-        cplx<double> acc[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = mk(0.0, 0.0);
         if (c >= 0) {
-            // T = X U, X = 0.5 (X + X^+) when flagged
-#pragma unroll 2
-            for (int m = 0; m < nw; ++m) {
-                cplx<double> x[4], u[4];
+            // all of the lane's loads in flight before the first LDS store (nw2 <= 1024 = 16 x 64)
+            cplx<T> buf[16];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    x[i] = tile[ix.rr[i] * ld + m];
-                    if (herm) {
-                        const cplx<double> w = tile[m * ld + ix.rr[i]];
-                        x[i] = mk(0.5 * (x[i].re + w.re), 0.5 * (x[i].im - w.im));
-                    }
-                    u[i] = sU[m * ld + ix.cc[i]];
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) cfma(acc[i][j], x[i], u[j]);
+            for (int u = 0; u < 16; ++u) {
+                const int t = lane + 64 * u;
+                if (t < nw2) buf[u] = Xc[t];
             }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int t = lane + 64 * u;
+                if (t < nw2) tile[(t / nw) * ld + t % nw] = mk((double)buf[u].re, (double)buf[u].im);
+            }
+        }
+        __syncthreads();
+        MfmaBlock acc;
+        acc.zero();
+        if (c >= 0) {
+            // T = X U, X = 0.5 (X + X^+) when flagged (matrix cores, FP64)
+            if (herm)
+                mfma_block(acc, nw, nw, nw, 0, 0,
+                           [&](int r, int m) {
+                               const cplx<double> x = tile[r * ld + m], w = tile[m * ld + r];
+                               return mk(0.5 * (x.re + w.re), 0.5 * (x.im - w.im));
+                           },
+                           [&](int m, int cc) { return sU[m * ld + cc]; });
+            else
+                mfma_block(acc, nw, nw, nw, 0, 0, [&](int r, int m) { return tile[r * ld + m]; },
+                           [&](int m, int cc) { return sU[m * ld + cc]; });
         }
         __syncthreads();  // every read of X is done before T overwrites the tile
         if (c >= 0)
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (ix.r0 + i < nw && ix.c0 + j < nw) tile[(ix.r0 + i) * ld + ix.c0 + j] = acc[i][j];
-                    acc[i][j] = mk(0.0, 0.0);
-                }
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = mfma_row(bi, v), cc = mfma_col(bj);
+                        if (r < nw && cc < nw) tile[r * ld + cc] = acc.get(bi, bj, v);
+                    }
         __syncthreads();
         if (c >= 0) {
             // Xbar = U^+ T
-#pragma unroll 2
-            for (int m = 0; m < nw; ++m) {
-                cplx<double> u[4], t[4];
+            acc.zero();
+            mfma_block(acc, nw, nw, nw, 0, 0, [&](int r, int m) { return conj(sU[m * ld + r]); },
+                       [&](int m, int cc) { return tile[m * ld + cc]; });
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    u[i] = sU[m * ld + ix.rr[i]];
-                    t[i] = tile[m * ld + ix.cc[i]];
-                }
+            for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int i = 0; i < 4; ++i)
+                for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-                    for (int j = 0; j < 4; ++j) cfma_cj(acc[i][j], u[i], t[j]);
-            }
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (ix.r0 + i < nw && ix.c0 + j < nw)
-                        Xc[(ix.r0 + i) * nw + ix.c0 + j] = mk((T)acc[i][j].re, (T)acc[i][j].im);
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = mfma_row(bi, v), cc = mfma_col(bj);
+                        if (r < nw && cc < nw) {
+                            const cplx<double> x = acc.get(bi, bj, v);
+                            Xc[r * nw + cc] = mk((T)x.re, (T)x.im);
+                        }
+                    }
         }
     }
 }

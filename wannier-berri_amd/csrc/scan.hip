@@ -37,6 +37,7 @@
 // be stored in FP32 or FP64.
 #include <cmath>
 
+#include "mfma_c64.h"
 #include "wbmi_common.h"
 
 #define SC_THREADS 512
@@ -254,36 +255,7 @@ __device__ __attribute__((always_inline)) void build_M(const Smem& S, int nw) {
     }
 }
 
-// acc[i][j] = sum_m X(r0 + i, m) Y(m, c0 + j) over m < nw; rows and columns
-// past nw are clamped (computed and discarded by the caller).
-template <class FX, class FY>
-__device__ __attribute__((always_inline)) void tile4(cplx<double> (&acc)[4][4], int nw, int r0, int c0, FX X, FY Y) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = mk(0.0, 0.0);
-    int rr[4], cc[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        rr[i] = r0 + i < nw ? r0 + i : nw - 1;
-        cc[i] = c0 + i < nw ? c0 + i : nw - 1;
-    }
-#pragma unroll 2
-    for (int m = 0; m < nw; ++m) {
-        cplx<double> x[4], y[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) x[i] = X(rr[i], m);
-#pragma unroll
-        for (int j = 0; j < 4; ++j) y[j] = Y(m, cc[j]);
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) cfma(acc[i][j], x[i], y[j]);
-    }
-}
-
-// sum of v over the lanes of a wave whose lane bits outside `keep` differ
-// (lane = 8 tr + tc: mask 0x38 sums over tr, 0x07 over tc)
+// sum of v over the lanes of a wave that differ only in the lane bits of `mask`
 __device__ inline double wave_sum(double v, int mask) {
     for (int b = 1; b < 64; b <<= 1)
         if (mask & b) v += __shfl_xor(v, b);
@@ -291,7 +263,7 @@ __device__ inline double wave_sum(double v, int mask) {
 }
 
 // Wave tasks.  One wave forms block (rb, cb) (32 x 32) of a dense product
-// P = X Y with a 4 x 4 register tile per lane, adds the elementwise terms of
+// P = X Y on the matrix cores (FP64 MFMA, mfma_c64.h), adds the elementwise terms of
 // each element (x, y) and accumulates the consumers' weighted elements into
 // its accumulator rows accw[band][comp] after a shuffle reduction over the
 // summed index.  The band of an element is its column y, except for kind 1
@@ -322,28 +294,29 @@ __host__ __device__ constexpr int task_nt() {
 template <int KIND, typename T>
 __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T>& K, const TaskCtx& tc_,
                                                            double* accw, int b, int d, int rb, int cb) {
-    const int nw = K.nw, lane = threadIdx.x & 63, tr = lane >> 3, tc = lane & 7;
-    const int r0 = rb * 32 + 4 * tr, c0 = cb * 32 + 4 * tc;
+    const int nw = K.nw, lane = threadIdx.x & 63;
+    const int r0 = rb * 32, c0 = cb * 32;
     const bool dm = tc_.dm, in = tc_.in, ex = tc_.ex;
     const double sign = tc_.sign;
     const int al = c_alpha[b], be = c_beta[b];  // kinds 5, 6: b = c
-    cplx<double> acc[4][4];
+    MfmaBlock P;
+    P.zero();
     if constexpr (KIND == 0)
-        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.V(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
+        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.V(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
     else if constexpr (KIND == 1)
-        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Dp(b, r, m); }, [&](int m, int c) { return K.V(d, m, c); });
+        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(b, r, m); }, [&](int m, int c) { return K.V(d, m, c); });
     else if constexpr (KIND == 2)
-        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.A(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
+        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.A(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
     else if constexpr (KIND == 3)
-        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.B(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
+        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.B(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
     else if constexpr (KIND == 4)
-        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Dp(d, r, m); }, [&](int m, int c) { return K.A(b, m, c); });
+        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(d, r, m); }, [&](int m, int c) { return K.A(b, m, c); });
     else if constexpr (KIND == 5)
-        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Dp(al, r, m); }, [&](int m, int c) { return K.Dp(be, m, c); });
+        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(al, r, m); }, [&](int m, int c) { return K.Dp(be, m, c); });
     else if constexpr (KIND == 6)
-        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Ain(al, r, m); }, [&](int m, int c) { return K.Ain(be, m, c); });
+        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Ain(al, r, m); }, [&](int m, int c) { return K.Ain(be, m, c); });
     else
-        tile4(acc, nw, r0, c0, [&](int r, int m) { return K.Vin(b, r, m); }, [&](int m, int c) { return K.Ain(d, m, c); });
+        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Vin(b, r, m); }, [&](int m, int c) { return K.Ain(d, m, c); });
 
     // targets: partner component tp, accumulator column tcol, factor tw
     constexpr int NT = task_nt<KIND>();
@@ -417,17 +390,23 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
             }
     }
 
-    double part[4][NT];
+    // part[u][w]: u = the lane's band slot (column bj for kinds != 1, row (bi, v) for kind 1)
+    constexpr int NU = KIND == 1 ? 8 : 2;
+    double part[NU][NT];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < NU; ++u)
 #pragma unroll
         for (int w = 0; w < NT; ++w) part[u][w] = 0.0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int x = r0 + i, y = c0 + j;
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int x = r0 + mfma_row(bi, v), y = c0 + mfma_col(bj);
             if (x >= nw || y >= nw) continue;
+            const int i = KIND == 1 ? bi * 4 + v : bj, j = i;  // part slot
+            const cplx<double> pv = P.get(bi, bj, v);
             const int n = KIND == 1 ? x : y, l = KIND == 1 ? y : x;
             if (!S.act[n]) continue;
             const bool same = K.lab[l] == K.lab[n];
@@ -437,21 +416,21 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
                 // (W^{bd} - D'^b V''^d - D'^d V''^b)_ln / 2
                 cplx<double> z = K.W(b, d, l, n);
                 for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
-                const cplx<double> Pm = (-K.m(l, n)) * (acc[i][j] + 0.5 * z);
+                const cplx<double> Pm = (-K.m(l, n)) * (pv + 0.5 * z);
                 const double wD = (in ? dw : 0.0) + (dm && in ? 2.0 * K.E[l] : 0.0);
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
-                    double v = wD * (K.Dp(tp[w], n, l) * Pm).im;  // Re(-i D' X1) (T1, s2)
-                    if (dm && ex) v -= 2.0 * (conj(K.B(tp[w], l, n)) * Pm).re;  // s7
-                    part[j][w] += v;
+                    double t = wD * (K.Dp(tp[w], n, l) * Pm).im;  // Re(-i D' X1) (T1, s2)
+                    if (dm && ex) t -= 2.0 * (conj(K.B(tp[w], l, n)) * Pm).re;  // s7
+                    part[j][w] += t;
                 }
-                if (dm && in) part[j][4] += (K.Dp(tp[4], n, l) * acc[i][j]).im;  // s1: Re(-i D'^al V^d D'^be)
+                if (dm && in) part[j][4] += (K.Dp(tp[4], n, l) * pv).im;  // s1: Re(-i D'^al V^d D'^be)
             } else if constexpr (KIND == 1) {
                 if (same) continue;
                 // (W^{bd} + V''^b D'^d + V''^d D'^b)_nl / 2
                 cplx<double> z = K.W(b, d, n, l);
                 for (int q = S.gs[n]; q < S.ge[n]; ++q) z += K.V(b, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(b, q, l);
-                const cplx<double> Pm = (-K.m(n, l)) * (0.5 * z - acc[i][j]);
+                const cplx<double> Pm = (-K.m(n, l)) * (0.5 * z - pv);
 #pragma unroll
                 for (int w = 0; w < 4; ++w) part[i][w] -= dw * (Pm * K.A(tp[w], l, n)).re;
             } else if constexpr (KIND == 2) {
@@ -459,52 +438,53 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
                     // A'^{bd} - D'^d A''^b
                     cplx<double> z = K.Ad(b, d, l, n);
                     for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
-                    const cplx<double> P = acc[i][j] + z;
+                    const cplx<double> Y1 = pv + z;
 #pragma unroll
-                    for (int w = 0; w < 2; ++w) part[j][w] -= dw * (K.Dp(tp[w], n, l) * P).re;
+                    for (int w = 0; w < 2; ++w) part[j][w] -= dw * (K.Dp(tp[w], n, l) * Y1).re;
                 } else if (dm) {
                     // s5: Re(-2i sig A^a_nl E_l (A' + A^b D'^d)_ln)
-                    const cplx<double> P = acc[i][j] + K.Ad(b, d, l, n);
+                    const cplx<double> dA = pv + K.Ad(b, d, l, n);
 #pragma unroll
-                    for (int w = 0; w < 2; ++w) part[j][w] += 2.0 * K.E[l] * (K.A(tp[w], n, l) * P).im;
+                    for (int w = 0; w < 2; ++w) part[j][w] += 2.0 * K.E[l] * (K.A(tp[w], n, l) * dA).im;
                 }
             } else if constexpr (KIND == 3) {
                 if (same) continue;
                 // B'^{bd} - D'^d B''^b
                 cplx<double> z = K.Bd(b, d, l, n);
                 for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.B(b, q, n);
-                const cplx<double> P = acc[i][j] + z;
+                const cplx<double> YB = pv + z;
 #pragma unroll
-                for (int w = 0; w < 2; ++w) part[j][w] -= 2.0 * (K.Dp(tp[w], n, l) * P).re;  // s6
+                for (int w = 0; w < 2; ++w) part[j][w] -= 2.0 * (K.Dp(tp[w], n, l) * YB).re;  // s6
             } else if constexpr (KIND == 4) {
                 if (!same) continue;
                 // s5: -Re(-2i sig A^a_nl E_l (D'^d A^b)_ln) ; Omega block: 2 sign coef Re(DA_ln V^e_nl)
 #pragma unroll
-                for (int w = 0; w < 2; ++w) part[j][w] -= 2.0 * K.E[l] * (K.A(tp[w], n, l) * acc[i][j]).im;
+                for (int w = 0; w < 2; ++w) part[j][w] -= 2.0 * K.E[l] * (K.A(tp[w], n, l) * pv).im;
 #pragma unroll
-                for (int w = 2; w < 5; ++w) part[j][w] += (acc[i][j] * K.V(tp[w], n, l)).re;
+                for (int w = 2; w < 5; ++w) part[j][w] += (pv * K.V(tp[w], n, l)).re;
             } else if constexpr (KIND == 5 || KIND == 6) {
                 if (!same) continue;
                 // Omega block term -i P_ln: 2 sign Re(-i P_ln V^e_nl)
 #pragma unroll
-                for (int w = 0; w < 3; ++w) part[j][w] += (acc[i][j] * K.V(tp[w], n, l)).im;
+                for (int w = 0; w < 3; ++w) part[j][w] += (pv * K.V(tp[w], n, l)).im;
             } else {
                 if (!same) continue;
                 // s4: Re(-2i A^al_np (V''^d A''^be)_pn)
-                part[j][0] += (K.A(tp[0], n, l) * acc[i][j]).im;
+                part[j][0] += (K.A(tp[0], n, l) * pv).im;
             }
         }
-    // reduce over the summed index: rows (tr), or columns (tc) for kind 1
-    const int mask = KIND == 1 ? 0x07 : 0x38;
-    const bool writer = KIND == 1 ? tc == 0 : tr == 0;
-    const int n0 = KIND == 1 ? r0 : c0;
+    // reduce over the summed index: rows (lane bits 4-5), or columns (bits 0-3) for kind 1
+    const int mask = KIND == 1 ? 0x0f : 0x30;
+    const bool writer = KIND == 1 ? (lane & 15) == 0 : (lane >> 4) == 0;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < NU; ++u) {
+        const int n = KIND == 1 ? r0 + mfma_row(u / 4, u % 4) : c0 + mfma_col(u);
 #pragma unroll
         for (int w = 0; w < NT; ++w) {
-            const double v = wave_sum(part[u][w], mask);
-            if (writer && tw[w] != 0.0 && n0 + u < nw) accw[(n0 + u) * 9 + tcol[w]] += tw[w] * v;
+            const double sum = wave_sum(part[u][w], mask);
+            if (writer && tw[w] != 0.0 && n < nw) accw[n * 9 + tcol[w]] += tw[w] * sum;
         }
+    }
 }
 
 // DerOmega (formula/covariant.py:212-259) or, dm = true, DerMorb
@@ -698,15 +678,27 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     __syncthreads();
     const cplx<T>* Xk = Xall + k * (int64_t)args.nch * nw * nw;
     if (stage_v || stage_a) {
-        for (int t = tid; t < 3 * nw * nw; t += SC_THREADS) {
-            int a = t / (nw * nw), x = (t / nw) % nw, y = t % nw;
-            if (stage_v) {
-                cplx<T> v = Xk[((int64_t)S.ch[WBMI_ROLE_V + a] * nw + x) * nw + y];
-                S.V[(a * nw + x) * ld + y] = mk((double)v.re, (double)v.im);
+        // staged only for nw <= SC_LDS_NW: 3 nw^2 <= 3072 = 6 x SC_THREADS elements per array;
+        // all of a thread's loads are issued before its LDS stores
+        constexpr int NS = (3 * SC_LDS_NW * SC_LDS_NW + SC_THREADS - 1) / SC_THREADS;
+        const int nw2 = nw * nw;
+        cplx<T> bv[NS], ba[NS];
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int t = tid + SC_THREADS * u;
+            if (t < 3 * nw2) {
+                const int a = t / nw2, xy = t % nw2;
+                if (stage_v) bv[u] = Xk[(int64_t)S.ch[WBMI_ROLE_V + a] * nw2 + xy];
+                if (stage_a) ba[u] = Xk[(int64_t)S.ch[WBMI_ROLE_A + a] * nw2 + xy];
             }
-            if (stage_a) {
-                cplx<T> v = Xk[((int64_t)S.ch[WBMI_ROLE_A + a] * nw + x) * nw + y];
-                S.A[(a * nw + x) * ld + y] = mk((double)v.re, (double)v.im);
+        }
+#pragma unroll
+        for (int u = 0; u < NS; ++u) {
+            const int t = tid + SC_THREADS * u;
+            if (t < 3 * nw2) {
+                const int a = t / nw2, x = (t % nw2) / nw, y = t % nw;
+                if (stage_v) S.V[(a * nw + x) * ld + y] = mk((double)bv[u].re, (double)bv[u].im);
+                if (stage_a) S.A[(a * nw + x) * ld + y] = mk((double)ba[u].re, (double)ba[u].im);
             }
         }
     }
