@@ -133,13 +133,15 @@ int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nele
  *   H_dev : [nk, Hstride] complex (prec), matrix k at H_dev + k*Hstride
  *   E_dev : float64 [nk, nw] ascending;  U_dev : [nk, nw, nw] complex (prec),
  *   column j = eigenvector of E[j].  nw <= 64.
- *   refine (prec = WBMI_F32 only): n in 1..8 = FP32 Jacobi of the FP64
- *   Hamiltonian H64_dev (complex128, same stride; H_dev unused), then FP64
- *   Ogita-Aishima steps (Japan J. Indust. Appl. Math. 35 (2018), Alg. 1)
- *   until the correction has ||E||_F < 1e-8, at most n of them.  E are the
- *   FP64 Rayleigh quotients of the refined vectors, which are stored as
- *   complex64.  Separated spectra take 2 steps; a pair closer than the first
- *   step's threshold (~1e-4 |H| at nw = 32) up to 8.  Needs nw <= 44.    */
+ *   refine = n in 1..8: FP32 Jacobi of the FP64 Hamiltonian (H_dev for
+ *   prec = WBMI_F64; H64_dev, complex128 with the same stride, for
+ *   prec = WBMI_F32), then FP64 Ogita-Aishima steps (Japan J. Indust. Appl.
+ *   Math. 35 (2018), Alg. 1) until the correction has ||E||_F below 1e-8
+ *   (complex64 vectors) or 1e-10 (complex128 vectors), at most n of them.
+ *   E are the FP64 Rayleigh quotients of the refined vectors.  Separated
+ *   spectra take 2-3 steps; a pair closer than the first step's threshold
+ *   (~1e-4 |H| at nw = 32) up to 8.  Needs nw <= 44.
+ *   refine = 0: plain Jacobi in the arithmetic of `prec`.                   */
 int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride,
               double* E_dev, void* U_dev, int prec, int refine, const void* H64_dev);
 

@@ -55,9 +55,10 @@ struct ItemWalk {
     }
 };
 
-template <typename T, typename Tin>
+// T: arithmetic and LDS type; Tin: type of H; TO: type of the stored eigenvectors
+template <typename T, typename Tin, typename TO = T>
 __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __restrict__ H, int64_t Hstride,
-                                                    double* __restrict__ Eout, cplx<T>* __restrict__ Uout,
+                                                    double* __restrict__ Eout, cplx<TO>* __restrict__ Uout,
                                                     int max_sweeps) {
     extern __shared__ char smem[];
     const int n = nw + (nw & 1);  // padded to even; the dummy index is decoupled
@@ -184,7 +185,10 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
             rank += (ej < ei) || (ej == ei && j < i);
         }
         Eout[k * nw + rank] = (double)ei;
-        for (int m = 0; m < nw; ++m) Uout[(k * nw + m) * nw + rank] = V[m * ld + i];
+        for (int m = 0; m < nw; ++m) {
+            const cplx<T> v = V[m * ld + i];
+            Uout[(k * nw + m) * nw + rank] = mk((TO)v.re, (TO)v.im);
+        }
     }
 }
 
@@ -201,8 +205,11 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
 // ---------------------------------------------------------------------------
 
 #define OA_THREADS 256
-// stop refining once ||E||_F is below this (the vectors are stored in complex64)
-#define OA_TOL 1e-8
+// stop refining once ||E||_F is below this: complex64 vectors are limited by
+// their storage; complex128 vectors after a step with ||E|| < 1e-10 are
+// accurate to ~||E||^2 ||H|| / gap, i.e. to FP64 rounding
+#define OA_TOL_C64 1e-8
+#define OA_TOL_C128 1e-10
 
 template <int TS, class FX, class FY, class FO>
 __device__ __attribute__((always_inline)) void oa_gemm(int nw, FX X, FY Y, FO out) {
@@ -341,6 +348,21 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         sT = t;
         if (converged) break;
     }
+    if (sizeof(TU) == 8) {
+        // complex128 output: one Newton-Schulz step X <- X (I + R/2), R = I - X^+ X, so that the
+        // basis is orthonormal to rounding also where a close pair leaves the vectors themselves
+        // accurate only to ~eps |H| / gap (the correction steps do not preserve orthogonality)
+        oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
+                    [&](int m, int c) { return sX[m * ld + c]; },
+                    [&](int r, int c, cplx<double> v) { sA[r * ld + c] = mk(0.5 * ((r == c ? 1.0 : 0.0) - v.re), -0.5 * v.im); });
+        __syncthreads();
+        oa_gemm<TS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sA[m * ld + c]; },
+                    [&](int r, int c, cplx<double> v) { sT[r * ld + c] = sX[r * ld + c] + v; });
+        __syncthreads();
+        cplx<double>* t = sX;
+        sX = sT;
+        sT = t;
+    }
     // eigenvalues: Rayleigh quotients x_c^+ A x_c / x_c^+ x_c of the refined columns
     oa_load_a(nw, ld, Hk, sA);
     __syncthreads();
@@ -372,16 +394,54 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
     for (int i = tid; i < nw; i += OA_THREADS) Eout[k * nw + rank[i]] = lam[i];
 }
 
+// FP32 Jacobi of the FP64 Hamiltonian, then FP64 Ogita-Aishima steps; eigenvectors stored as TU
+template <typename TU>
+static int eigh_refined(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* H64, int64_t Hstride, double* E,
+                        cplx<TU>* U, int steps) {
+    const int n = nw + (nw & 1);
+    const int sweeps = 40;
+    const size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
+    const size_t lds2 = 4 * sizeof(cplx<double>) * nw * (nw + 1) + sizeof(double) * (nw + 4) + sizeof(int) * nw + 16;
+    if (lds2 > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement needs nw <= 44");
+    const double tol = sizeof(TU) == 4 ? OA_TOL_C64 : OA_TOL_C128;
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double, TU>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_eigh_jacobi<float, double, TU>), dim3(nk), dim3(64), lds, ctx->stream, nw, H64, Hstride, E,
+                       U, sweeps);
+    WBMI_LAUNCH_CHECK(ctx);
+    if (nw <= 32) {
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<TU, 2>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+        hipLaunchKernelGGL((k_eigh_refine<TU, 2>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw, H64, Hstride, E,
+                           U, steps, tol);
+    } else {
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<TU, 4>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+        hipLaunchKernelGGL((k_eigh_refine<TU, 4>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw, H64, Hstride, E,
+                           U, steps, tol);
+    }
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
 extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride, double* E_dev,
                          void* U_dev, int prec, int refine, const void* H64_dev) {
     if (!ctx) return WBMI_ERR_ARG;
     if (nk <= 0 || nw <= 0 || nw > 64 || !E_dev || !U_dev || Hstride < (int64_t)nw * nw)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad arguments (nw <= 64, Hstride >= nw*nw)");
     if (refine < 0 || refine > 8) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refine must be 0..8");
-    if (refine && (prec != WBMI_F32 || !H64_dev))
-        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement needs prec=WBMI_F32 and the FP64 Hamiltonian");
-    if (!refine && !H_dev) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: H_dev is null");
-    int n = nw + (nw & 1);
+    if (prec != WBMI_F64 && prec != WBMI_F32) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad precision");
+    if (refine) {
+        // the FP64 Hamiltonian: H_dev (complex128) for prec = WBMI_F64, H64_dev for prec = WBMI_F32
+        const void* H64 = prec == WBMI_F64 ? H_dev : H64_dev;
+        if (!H64) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement needs the FP64 Hamiltonian");
+        if (prec == WBMI_F64)
+            return eigh_refined<double>(ctx, nk, nw, (const cplx<double>*)H64, Hstride, E_dev, (cplx<double>*)U_dev,
+                                        refine);
+        return eigh_refined<float>(ctx, nk, nw, (const cplx<double>*)H64, Hstride, E_dev, (cplx<float>*)U_dev, refine);
+    }
+    if (!H_dev) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: H_dev is null");
+    const int n = nw + (nw & 1);
     const int sweeps = 40;
     if (prec == WBMI_F64) {
         size_t lds = 2 * sizeof(cplx<double>) * n * (n + 1) + 3 * sizeof(double) * (n / 2) + 16;
@@ -389,36 +449,12 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         hipLaunchKernelGGL((k_eigh_jacobi<double, double>), dim3(nk), dim3(64), lds, ctx->stream, nw,
                            (const cplx<double>*)H_dev, Hstride, E_dev, (cplx<double>*)U_dev, sweeps);
-    } else if (prec == WBMI_F32) {
-        size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
-        if (refine) {
-            // FP32 solve of the FP64 Hamiltonian, then `refine` FP64 refinement steps
-            WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double>,
-                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((k_eigh_jacobi<float, double>), dim3(nk), dim3(64), lds, ctx->stream, nw,
-                               (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev, sweeps);
-            WBMI_LAUNCH_CHECK(ctx);
-            size_t lds2 = 4 * sizeof(cplx<double>) * nw * (nw + 1) + sizeof(double) * (nw + 4) + sizeof(int) * nw + 16;
-            if (lds2 > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement needs nw <= 44");
-            if (nw <= 32) {
-                WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<float, 2>,
-                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-                hipLaunchKernelGGL((k_eigh_refine<float, 2>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw,
-                                   (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev, refine, OA_TOL);
-            } else {
-                WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<float, 4>,
-                                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-                hipLaunchKernelGGL((k_eigh_refine<float, 4>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw,
-                                   (const cplx<double>*)H64_dev, Hstride, E_dev, (cplx<float>*)U_dev, refine, OA_TOL);
-            }
-        } else {
-            WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, float>,
-                                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL((k_eigh_jacobi<float, float>), dim3(nk), dim3(64), lds, ctx->stream, nw,
-                               (const cplx<float>*)H_dev, Hstride, E_dev, (cplx<float>*)U_dev, sweeps);
-        }
     } else {
-        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad precision");
+        size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, float>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_eigh_jacobi<float, float>), dim3(nk), dim3(64), lds, ctx->stream, nw,
+                           (const cplx<float>*)H_dev, Hstride, E_dev, (cplx<float>*)U_dev, sweeps);
     }
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;

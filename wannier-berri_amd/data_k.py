@@ -230,7 +230,9 @@ class Data_K:
     def _evaluate(self, keys):
         """Channels `keys` (always including the Hamiltonian) -> E, U and the rotated stack.
 
-        FP64 form: one complex128 stack, FP64 Jacobi.  Mixed form: the
+        FP64 form: one complex128 stack; FP32 Jacobi + FP64 Ogita-Aishima
+        refinement to FP64-accurate energies and complex128 vectors (plain
+        FP64 Jacobi for nw > 44).  Mixed form: the
         Hamiltonian is transformed in FP64 on its own, the other channels in
         complex64; FP32 Jacobi + FP64 Ogita-Aishima steps (at most OA_STEPS) give FP64
         energies and refined eigenvectors (stored as complex64).
@@ -252,8 +254,9 @@ class Data_K:
             self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=OA_STEPS, H64=H64)
             del H64
         else:
+            # FP32 Jacobi + FP64 refinement to complex128 vectors (nw <= 44); plain FP64 Jacobi above
             h0 = slots[("Ham", 0)]
-            self.ctx.eigh(X[:, h0], nch * nw * nw, E, U, self.prec, nw)
+            self.ctx.eigh(X[:, h0], nch * nw * nw, E, U, self.prec, nw, refine=OA_STEPS if nw <= 44 else 0)
         if X is not None:
             herm = np.zeros(nch, dtype=np.int32)
             skip = np.zeros(nch, dtype=np.int32)
