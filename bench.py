@@ -42,7 +42,6 @@ FLOPS_SCAN_AHC_PER_K = 3 * 3 * 8 * NW ** 2            # Omega: 3 comps x 3 terms
 FLOPS_EIGH_PER_K = int((16 / 3 + 8) * NW ** 3)        # LAPACK-equivalent Hermitian eigensolver
 PEAK_HBM = 8000.0        # GB/s
 PEAK_FP64 = 78.6         # TFLOP/s (dense MFMA and vector FP64)
-PEAK_FP32 = 157.3
 
 
 def log(*a):
@@ -105,8 +104,10 @@ def stage_stats(timer):
 
 
 def roofline(stage, calls, total_ms, units, prec):
+    # rotation, formula products and the eigensolver's refinement run in FP64
+    # arithmetic in both forms (FP64 MFMA); the forms differ in storage
     avg_s = total_ms / calls / 1e3
-    peak_f = PEAK_FP64 if prec == "fp64" else PEAK_FP32
+    peak_f = PEAK_FP64
     if stage == "rotate":
         nk, nw, nrot = units[0]
         work = nk * nrot * FLOPS_ROTATE_PER_K_PER_CH
