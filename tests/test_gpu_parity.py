@@ -134,7 +134,7 @@ def test_eigh(wb, ctx, nw):
 
 @pytest.mark.parametrize("nw,prec", [(13, 0), (32, 0), (64, 0), (32, 1), (64, 1)])
 def test_rotate(wb, ctx, nw, prec):
-    """U^+ X U with Hermitisation/skip flags; complex64 channels flagged fp64 are exact up to the stored rounding."""
+    """U^+ X U with Hermitisation/skip flags; complex64 channels (FP64 arithmetic) are exact up to the stored rounding."""
     import torch
     say(f"test_rotate nw={nw} prec={prec}")
     rng = np.random.default_rng(5)
@@ -143,13 +143,12 @@ def test_rotate(wb, ctx, nw, prec):
     U = np.linalg.qr(rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw)))[0]
     herm = np.array([0, 1, 0, 1, 0], dtype=np.int32)
     skip = np.array([0, 0, 1, 0, 0], dtype=np.int32)
-    fp64 = np.array([0, 0, 0, 1, 1], dtype=np.int32)
     cdt = torch.complex128 if prec == 0 else torch.complex64
     if prec == 1:
         X = X.astype(np.complex64).astype(np.complex128)
         U = U.astype(np.complex64).astype(np.complex128)
     Xt = torch.from_numpy(X.copy()).to(cdt).cuda()
-    ctx.rotate(torch.from_numpy(U).to(cdt).cuda(), Xt, herm, skip, prec, fp64)
+    ctx.rotate(torch.from_numpy(U).to(cdt).cuda(), Xt, herm, skip, prec)
     got = Xt.cpu().numpy().astype(np.complex128)
     for c in range(nch):
         x = X[:, c]
@@ -161,11 +160,9 @@ def test_rotate(wb, ctx, nw, prec):
             ref = np.einsum("kba,kbc,kcd->kad", U.conj(), x, U)
         if prec == 0:
             assert rel_err(got[:, c], ref) < 1e-12, c
-        elif fp64[c]:
+        else:
             # componentwise: only the final rounding to complex64 (2^-24 per real part)
             assert (np.abs(got[:, c] - ref) / np.abs(ref)).max() < 1.2e-7, c
-        else:
-            assert rel_err(got[:, c], ref) < 1e-5, c
 
 
 # -- end to end against the reference fixtures ----------------------------------
@@ -329,3 +326,37 @@ def test_eigh_fp32_refined(wb, ctx, nw, prec):
     # complex64 storage: ~1e-7; complex128: FP64 rounding
     assert res < (1e-6 if prec == 1 else 1e-13)
     assert orth < (1e-6 if prec == 1 else 1e-13)
+
+
+@pytest.fixture(scope="module")
+def spinor64(wb):
+    """nw = 64 spinor model (Rmax 1), its Fermi levels and the oracle's results (computed once)."""
+    from wannierberri_amd import models
+    s = models.spinor(num_wann_half=32, Rmax=1, seed=3)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA")})
+    E0 = np.linalg.eigvalsh(s.get_R_mat("Ham")[np.where((s.rvec.iRvec == 0).all(1))[0][0]])
+    Ef = np.linspace(E0.min() - 1, E0.max() + 1, 21)
+    ki = {"external_terms": False}
+    ro = O.run(o, {"ahc": O.AHC(Ef), "morb": O.Morb(Ef, kwargs_formula=ki),
+                   "gme": O.GME_orb_FermiSea(Ef, kwargs_formula=ki),
+                   "gmes": O.GME_orb_FermiSurf(Ef, kwargs_formula=ki)}, NK=[2, 2, 2], NKFFT=[2, 2, 2])
+    return s, Ef, ki, ro
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_spinor_nw64(wb, spinor64, precision):
+    """nw = 64 (two 32-band spin blocks with SOC): AHC with external terms, Morb and GME (sea, surface)
+    with internal terms (the model has no BB/CC); 32 x 32 product blocks, FP64 Jacobi."""
+    say(f"test_spinor_nw64 {precision}")
+    st = wb.calculators.static
+    s, Ef, ki, ro = spinor64
+    calcs = {"ahc": st.AHC(Efermi=Ef), "morb": st.Morb(Efermi=Ef, kwargs_formula=ki),
+             "gme": st.GME_orb_FermiSea(Efermi=Ef, kwargs_formula=ki),
+             "gmes": st.GME_orb_FermiSurf(Efermi=Ef, kwargs_formula=ki)}
+    r = run_gpu(wb, s, calcs, [2, 2, 2], [2, 2, 2], precision)
+    tol = 1e-8 if precision == "fp64" else 1e-5
+    for k in calcs:
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  {precision} {k}: rel err {e:.2e}")
+        assert e < tol, k

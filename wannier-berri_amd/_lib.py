@@ -77,7 +77,7 @@ _PROTOS = {
     "wbmi_r2k_exec": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
     "wbmi_r2k_klist": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _vp, _i]),
     "wbmi_eigh": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp]),
-    "wbmi_rotate": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _i]),
+    "wbmi_rotate": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i]),
     "wbmi_static_scan": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _d, _vp]),
     "wbmi_finalize": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _vp]),
 }
@@ -186,14 +186,13 @@ class Context:
         self.check(self._timed("eigh", (E.shape[0], nw), lib().wbmi_eigh, self.h, E.shape[0], nw, _ptr(H), Hstride,
                                _ptr(E), _ptr(U), prec, int(refine), _ptr(H64)), "wbmi_eigh")
 
-    def rotate(self, U, X, herm, skip, prec, fp64=None):
+    def rotate(self, U, X, herm, skip, prec):
         nk, nch, nw = X.shape[0], X.shape[1], X.shape[2]
         h_a, h_p = _host_i32(herm)
         s_a, s_p = _host_i32(skip)
-        f_a, f_p = _host_i32(fp64)
         nrot = int(nch - (0 if s_a is None else np.count_nonzero(s_a)))
         self.check(self._timed("rotate", (nk, nw, nrot), lib().wbmi_rotate, self.h, nk, nw, nch, _ptr(U), _ptr(X),
-                               h_p, s_p, f_p, prec), "wbmi_rotate")
+                               h_p, s_p, prec), "wbmi_rotate")
 
     def static_scan(self, E, X, prec, params, weight, hist):
         nk, nw = E.shape

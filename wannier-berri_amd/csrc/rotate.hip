@@ -7,12 +7,11 @@
 // TS x TS register tiles on a 16 x 16 thread grid (TS = 2 for nw <= 32,
 // 4 for nw <= 64).  LDS rows are padded by one element.
 //
-// Complex64 storage: each channel is rotated either in FP32 arithmetic or, when
-// flagged, in FP64 arithmetic on the complex64 operands (T kept in FP64).  FP32
-// accumulation leaves a normwise error ~eps32 |X| |U| in every element, which
-// the 1/(E_n - E_m)^2 of the Berry curvature amplifies for small elements
-// between close bands; FP64 arithmetic leaves only the componentwise rounding
-// of the stored result.
+// Complex64 storage: channels are rotated with FP64 arithmetic on the
+// complex64 operands (only the stored result is rounded).  FP32 accumulation
+// would leave a normwise error ~eps32 |X| |U| in every element, which the
+// 1/(E_n - E_m)^2 of the Berry curvature amplifies for small elements between
+// close bands; without packed math FP32 and FP64 FMAs issue at the same rate.
 #include "mfma_c64.h"
 #include "wbmi_common.h"
 
@@ -94,8 +93,7 @@ __device__ inline void rot_channel(int nw, int ld, const cplx<T>* sU, const cplx
 template <typename T>
 __global__ void __launch_bounds__(ROT_THREADS) k_rotate(int nw, int nch, const cplx<T>* __restrict__ U,
                                                         cplx<T>* X, uint64_t herm_lo, uint64_t herm_hi,
-                                                        uint64_t skip_lo, uint64_t skip_hi, uint64_t f64_lo,
-                                                        uint64_t f64_hi, int stage_x) {
+                                                        uint64_t skip_lo, uint64_t skip_hi, int stage_x) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int ld = nw + 1;
     cplx<T>* sU = (cplx<T>*)smem;
@@ -110,14 +108,13 @@ __global__ void __launch_bounds__(ROT_THREADS) k_rotate(int nw, int nch, const c
         const bool skip = c < 64 ? (skip_lo >> c) & 1 : (skip_hi >> (c - 64)) & 1;
         if (skip) continue;
         const bool herm = c < 64 ? (herm_lo >> c) & 1 : (herm_hi >> (c - 64)) & 1;
-        const bool f64 = c < 64 ? (f64_lo >> c) & 1 : (f64_hi >> (c - 64)) & 1;
         cplx<T>* Xc = X + (k * nch + c) * nw2;
         if (stage_x)
             for (int t = tid; t < nw2; t += ROT_THREADS) sX[(t / nw) * ld + t % nw] = Xc[t];
         __syncthreads();
         const cplx<T>* xs = stage_x ? sX : Xc;
         const int xld = stage_x ? ld : nw;
-        if (f64 && sizeof(T) == 4) {
+        if (sizeof(T) == 4) {
             if (nw <= 32)
                 rot_channel<T, double, 2>(nw, ld, sU, xs, xld, (cplx<double>*)sT, herm, Xc);
             else
@@ -244,9 +241,8 @@ __global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, c
 
 template <typename T>
 static int launch_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_dev, void* X_dev, const uint64_t* h,
-                         const uint64_t* s, const uint64_t* f) {
+                         const uint64_t* s) {
     if (nw <= 32) {
-        // every channel in FP64 arithmetic here (fp64 flags are implied)
         const size_t lds = (1 + ROTW_WAVES) * rotw_tile_bytes(nw);
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_rotate_w<T>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -262,23 +258,22 @@ static int launch_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_d
     WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_rotate<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                             (int)lds));
     hipLaunchKernelGGL(k_rotate<T>, dim3(nk), dim3(ROT_THREADS), lds, ctx->stream, nw, nch, (const cplx<T>*)U_dev,
-                       (cplx<T>*)X_dev, h[0], h[1], s[0], s[1], f[0], f[1], stage_x);
+                       (cplx<T>*)X_dev, h[0], h[1], s[0], s[1], stage_x);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
 }
 
 extern "C" int wbmi_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_dev, void* X_dev,
-                           const int32_t* herm_host, const int32_t* skip_host, const int32_t* fp64_host, int prec) {
+                           const int32_t* herm_host, const int32_t* skip_host, int prec) {
     if (!ctx) return WBMI_ERR_ARG;
     if (nk <= 0 || nw <= 0 || nw > 64 || nch <= 0 || nch > 128 || !U_dev || !X_dev)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_rotate: bad arguments (nw <= 64, nch <= 128)");
-    uint64_t h[2] = {0, 0}, s[2] = {0, 0}, f[2] = {0, 0};
+    uint64_t h[2] = {0, 0}, s[2] = {0, 0};
     for (int c = 0; c < nch; ++c) {
         if (herm_host && herm_host[c]) h[c / 64] |= 1ull << (c % 64);
         if (skip_host && skip_host[c]) s[c / 64] |= 1ull << (c % 64);
-        if (fp64_host && fp64_host[c]) f[c / 64] |= 1ull << (c % 64);
     }
-    if (prec == WBMI_F64) return launch_rotate<double>(ctx, nk, nw, nch, U_dev, X_dev, h, s, f);
-    if (prec == WBMI_F32) return launch_rotate<float>(ctx, nk, nw, nch, U_dev, X_dev, h, s, f);
+    if (prec == WBMI_F64) return launch_rotate<double>(ctx, nk, nw, nch, U_dev, X_dev, h, s);
+    if (prec == WBMI_F32) return launch_rotate<float>(ctx, nk, nw, nch, U_dev, X_dev, h, s);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_rotate: bad precision");
 }

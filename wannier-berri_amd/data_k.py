@@ -34,13 +34,6 @@ BASES = ["Ham", "AA", "BB", "CC"]
 # mixed form (the kernel stops at ||E||_F < 1e-8: two steps for separated
 # spectra, up to eight for pairs closer than ~1e-5 |H|)
 OA_STEPS = 8
-# Mixed form, nw > 32: channels rotated with FP64 arithmetic on their
-# complex64 values (for nw <= 32 the rotation kernel uses FP64 arithmetic for
-# every channel).  They feed the first-order formulas (Omega, Morb) and the
-# products of DerOmega; FP32 accumulation would leave a normwise error
-# ~eps32 |X| in small elements between close bands, which 1/(E_n - E_m)^2
-# amplifies.
-FP64_ROTATION = {("Ham", 1), ("AA", 0), ("rotAA", 0), ("BB", 0), ("CC", 0)}
 _ALPHA = [1, 2, 0]
 _BETA = [2, 0, 1]
 
@@ -251,7 +244,14 @@ class Data_K:
         X = self._transform(main, self.cdtype, self.prec) if main else None
         if mixed:
             H64 = self._transform([("Ham", 0)], torch.complex128, WBMI_F64)
-            self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=OA_STEPS, H64=H64)
+            if nw <= 44:
+                self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=OA_STEPS, H64=H64)
+            else:
+                # refinement needs nw <= 44: FP64 Jacobi, vectors rounded to complex64
+                U64 = torch.empty((nk, nw, nw), dtype=torch.complex128, device=self.ctx.device)
+                self.ctx.eigh(H64, nw * nw, E, U64, WBMI_F64, nw)
+                U.copy_(U64)
+                del U64
             del H64
         else:
             # FP32 Jacobi + FP64 refinement to complex128 vectors (nw <= 44); plain FP64 Jacobi above
@@ -260,16 +260,13 @@ class Data_K:
         if X is not None:
             herm = np.zeros(nch, dtype=np.int32)
             skip = np.zeros(nch, dtype=np.int32)
-            fp64 = np.zeros(nch, dtype=np.int32)
             for key, sl in slots.items():
                 if key[0] in HERMITIAN:
                     herm[sl:sl + CHANNEL_NCOMP[key]] = 1
-                if key in FP64_ROTATION:
-                    fp64[sl:sl + CHANNEL_NCOMP[key]] = 1
             if ("Ham", 0) in slots:
                 skip[slots[("Ham", 0)]] = 1
             if skip.sum() < nch:
-                self.ctx.rotate(U, X, herm, skip, self.prec, fp64)
+                self.ctx.rotate(U, X, herm, skip, self.prec)
         self.Xk, self.E_dev, self.U_dev, self._slots, self.nch = X, E, U, slots, nch
 
     def role_table(self):
