@@ -132,18 +132,25 @@ int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nele
  * H is read as 0.5 (H + H^+) (data_K_R.py:26 hermitises HH_K).
  *   H_dev : [nk, Hstride] complex (prec), matrix k at H_dev + k*Hstride
  *   E_dev : float64 [nk, nw] ascending;  U_dev : [nk, nw, nw] complex (prec),
- *   column j = eigenvector of E[j].  nw <= 64.
+ *   column j = eigenvector of E[j].  nw <= 128 (nw <= 64 without refinement).
  *   refine = n in 1..8: FP32 Jacobi of the FP64 Hamiltonian (H_dev for
  *   prec = WBMI_F64; H64_dev, complex128 with the same stride, for
  *   prec = WBMI_F32), then FP64 Ogita-Aishima steps (Japan J. Indust. Appl.
  *   Math. 35 (2018), Alg. 1) until the correction has ||E||_F below 1e-8
- *   (complex64 vectors) or 1e-10 (complex128 vectors), at most n of them.
+ *   (complex64 vectors) or 1e-10 (complex128 vectors), at most n of them;
+ *   complex128 vectors get a final Newton-Schulz orthonormalisation.
  *   E are the FP64 Rayleigh quotients of the refined vectors.  Separated
  *   spectra take 2-3 steps; a pair closer than the first step's threshold
- *   (~1e-4 |H| at nw = 32) up to 8.  Needs nw <= 44.
- *   refine = 0: plain Jacobi in the arithmetic of `prec`.                   */
+ *   (~1e-4 |H| at nw = 32) up to 8.  For nw > 64 the FP32 Jacobi runs one
+ *   workgroup per chunk of eigenvector rows.
+ *   refine = 0: plain Jacobi in the arithmetic of `prec` (nw <= 64).
+ *   work_dev / work_bytes: device workspace of at least
+ *   wbmi_eigh_work_bytes(...) bytes (nonzero only for refinement at
+ *   nw > 44); may be NULL / 0 otherwise.                                   */
 int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride,
-              double* E_dev, void* U_dev, int prec, int refine, const void* H64_dev);
+              double* E_dev, void* U_dev, int prec, int refine, const void* H64_dev,
+              void* work_dev, size_t work_bytes);
+size_t wbmi_eigh_work_bytes(int nk, int nw, int prec, int refine);
 
 /* ---- gauge rotation U^+ X U -----------------------------------------------
  * Replaces Data_K._rotate (data_K/data_K.py:130-132) for the channels of a
@@ -151,9 +158,13 @@ int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride,
  * channel Hermitian first (0.5 (X + X^+), fourier/fft.py:183-186,
  * data_K_R.py:87).  Channels with skip_host[c] = 1 are left untouched.
  * Complex64 channels (prec = WBMI_F32) are rotated with FP64 arithmetic;
- * only the stored result is rounded.  Flag arrays may be NULL (all 0).     */
+ * only the stored result is rounded.  Flag arrays may be NULL (all 0).
+ * nw <= 128; work_dev / work_bytes: device workspace of at least
+ * wbmi_rotate_work_bytes(...) bytes (nonzero only for nw > 64).            */
 int wbmi_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_dev,
-                void* X_dev, const int32_t* herm_host, const int32_t* skip_host, int prec);
+                void* X_dev, const int32_t* herm_host, const int32_t* skip_host, int prec,
+                void* work_dev, size_t work_bytes);
+size_t wbmi_rotate_work_bytes(int nk, int nw, int prec);
 
 /* ---- covariant formula traces + Fermi scan --------------------------------
  * Replaces the per-(k, band-group) loop and Fermi scan of
@@ -171,8 +182,9 @@ int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev
 
 /* ---- histogram -> EnergyResult data ---------------------------------------
  * Prefix sum over the Fermi levels, fder finite-difference stencil and scaling
- * (calculators/static.py:126-155).  out_dev: float64 [nEF_extra-2*extra, ncomp]. */
-int wbmi_finalize(wbmi_ctx* ctx, const double* hist_dev, int nEF_extra, int ncomp,
+ * (calculators/static.py:126-155).  out_dev: float64 [nEF_extra-2*extra, ncomp].
+ * hist_dev is overwritten with its prefix sums (any nEF_extra).             */
+int wbmi_finalize(wbmi_ctx* ctx, double* hist_dev, int nEF_extra, int ncomp,
                   int fder, double dEF, double scale, double* out_dev);
 
 #ifdef __cplusplus

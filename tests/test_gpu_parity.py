@@ -132,7 +132,7 @@ def test_eigh(wb, ctx, nw):
     assert np.abs(np.einsum("kba,kbc->kac", U.conj(), U) - np.eye(nw)[None]).max() < 1e-12
 
 
-@pytest.mark.parametrize("nw,prec", [(13, 0), (32, 0), (64, 0), (32, 1), (64, 1)])
+@pytest.mark.parametrize("nw,prec", [(13, 0), (32, 0), (64, 0), (32, 1), (64, 1), (80, 0), (128, 1)])
 def test_rotate(wb, ctx, nw, prec):
     """U^+ X U with Hermitisation/skip flags; complex64 channels (FP64 arithmetic) are exact up to the stored rounding."""
     import torch
@@ -288,7 +288,7 @@ def test_synthetic_nw40_global_operands(wb):
 
 
 @pytest.mark.parametrize("prec", [1, 0])
-@pytest.mark.parametrize("nw", [5, 18, 32, 40])
+@pytest.mark.parametrize("nw", [5, 18, 32, 40, 80, 128])
 def test_eigh_fp32_refined(wb, ctx, nw, prec):
     """FP32 Jacobi + FP64 Ogita-Aishima steps: FP64-accurate eigenvalues; vectors accurate to their
     storage (complex64 for prec=1, complex128 for prec=0).
@@ -360,3 +360,39 @@ def test_spinor_nw64(wb, spinor64, precision):
         e = rel_err(r.results[k].data, ro[k].data)
         say(f"  {precision} {k}: rel err {e:.2e}")
         assert e < tol, k
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_synthetic_nw80(wb, precision):
+    """64 < nw <= 128: row-chunked FP32 Jacobi + workspace refinement, global-operand rotation,
+    dEinv formed on the fly in the scan; AHC and BCD vs the oracle."""
+    say(f"test_synthetic_nw80 {precision}")
+    from wannierberri_amd import models
+    st = wb.calculators.static
+    s = models.synthetic(num_wann=80, Rmax=1, seed=17)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA")})
+    Ef = np.linspace(-8, 8, 33)
+    NK, NKFFT = [2, 2, 2], [2, 2, 2]
+    r = run_gpu(wb, s, {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef)}, NK, NKFFT, precision)
+    ro = O.run(o, {"ahc": O.AHC(Ef), "bcd": O.BerryDipole_FermiSea(Ef)}, NK=NK, NKFFT=NKFFT)
+    tol = {"fp64": 1e-8, "fp32": 1e-6}[precision]
+    for k in ("ahc", "bcd"):
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  {precision} {k}: rel err {e:.2e}")
+        assert e < tol, k
+
+
+def test_finalize_4001_levels(wb, small_synth):
+    """4001 Fermi levels (histogram larger than LDS): AHC (fder 0) and BCD surface (fder 1) vs the oracle."""
+    say("test_finalize_4001_levels")
+    st = wb.calculators.static
+    s, o = small_synth
+    Ef = np.linspace(-6, 6, 4001)
+    NK, NKFFT = [4, 4, 4], [4, 4, 4]
+    r = run_gpu(wb, s, {"ahc": st.AHC(Efermi=Ef), "bcds": st.BerryDipole_FermiSurf(Efermi=Ef)}, NK, NKFFT)
+    ro = O.run(o, {"ahc": O.AHC(Ef), "bcds": O.BerryDipole_FermiSurf(Ef)}, NK=NK, NKFFT=NKFFT)
+    for k in ("ahc", "bcds"):
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  {k}: rel err {e:.2e}")
+        assert e < 1e-8, k

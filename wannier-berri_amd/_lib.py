@@ -36,7 +36,7 @@ SYMBOLS = [
     "wbmi_version", "wbmi_ctx_create", "wbmi_ctx_destroy", "wbmi_ctx_set_stream", "wbmi_sync",
     "wbmi_last_error", "wbmi_build_channels", "wbmi_r2k_plan_create", "wbmi_r2k_plan_destroy",
     "wbmi_r2k_plan_sizes", "wbmi_r2k_box", "wbmi_r2k_exec", "wbmi_r2k_klist", "wbmi_eigh",
-    "wbmi_rotate", "wbmi_static_scan", "wbmi_finalize",
+    "wbmi_eigh_work_bytes", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
 ]
 
 
@@ -76,8 +76,10 @@ _PROTOS = {
     "wbmi_r2k_box": (_i, [_vp, _vp, _vp, _vp]),
     "wbmi_r2k_exec": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
     "wbmi_r2k_klist": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _vp, _i]),
-    "wbmi_eigh": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp]),
-    "wbmi_rotate": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i]),
+    "wbmi_eigh": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp, _vp, ctypes.c_size_t]),
+    "wbmi_eigh_work_bytes": (ctypes.c_size_t, [_i, _i, _i, _i]),
+    "wbmi_rotate": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, ctypes.c_size_t]),
+    "wbmi_rotate_work_bytes": (ctypes.c_size_t, [_i, _i, _i]),
     "wbmi_static_scan": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _d, _vp]),
     "wbmi_finalize": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _vp]),
 }
@@ -131,6 +133,7 @@ class Context:
             raise WbmiError(f"wbmi_ctx_create failed with code {rc} (device {device}; gfx950 required)")
         self.h = h
         self.timer = None   # optional {stage: [(start_event, end_event, units)]}
+        self._work = None   # device workspace shared by the calls on this context's stream
 
     def __del__(self):
         try:
@@ -181,18 +184,33 @@ class Context:
                                _ptr(iRvec_dev), nelem, _ptr(XR), klist_dev.shape[0], _ptr(klist_dev), _ptr(out),
                                prec), "wbmi_r2k_klist")
 
+    def _workspace(self, nbytes):
+        """Device workspace of at least nbytes (kept and grown on demand; None for 0)."""
+        if nbytes <= 0:
+            return None
+        import torch
+        if self._work is None or self._work.numel() < nbytes:
+            self._work = None
+            self._work = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+        return self._work
+
     def eigh(self, H, Hstride, E, U, prec, nw, refine=0, H64=None):
         """Batched eigh; refine=n >= 1: FP32 Jacobi of H64 + n FP64 Ogita-Aishima steps (prec must be F32)."""
-        self.check(self._timed("eigh", (E.shape[0], nw), lib().wbmi_eigh, self.h, E.shape[0], nw, _ptr(H), Hstride,
-                               _ptr(E), _ptr(U), prec, int(refine), _ptr(H64)), "wbmi_eigh")
+        nk = E.shape[0]
+        wbytes = int(lib().wbmi_eigh_work_bytes(nk, nw, prec, int(refine)))
+        work = self._workspace(wbytes)
+        self.check(self._timed("eigh", (nk, nw), lib().wbmi_eigh, self.h, nk, nw, _ptr(H), Hstride,
+                               _ptr(E), _ptr(U), prec, int(refine), _ptr(H64), _ptr(work), wbytes), "wbmi_eigh")
 
     def rotate(self, U, X, herm, skip, prec):
         nk, nch, nw = X.shape[0], X.shape[1], X.shape[2]
         h_a, h_p = _host_i32(herm)
         s_a, s_p = _host_i32(skip)
         nrot = int(nch - (0 if s_a is None else np.count_nonzero(s_a)))
+        wbytes = int(lib().wbmi_rotate_work_bytes(nk, nw, prec))
+        work = self._workspace(wbytes)
         self.check(self._timed("rotate", (nk, nw, nrot), lib().wbmi_rotate, self.h, nk, nw, nch, _ptr(U), _ptr(X),
-                               h_p, s_p, prec), "wbmi_rotate")
+                               h_p, s_p, prec, _ptr(work), wbytes), "wbmi_rotate")
 
     def static_scan(self, E, X, prec, params, weight, hist):
         nk, nw = E.shape

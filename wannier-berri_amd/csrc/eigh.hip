@@ -11,6 +11,7 @@
 // eigenvectors as columns, as np.linalg.eigh does (data_K/data_K.py:213).
 #include <cfloat>
 
+#include "mfma_c64.h"
 #include "wbmi_common.h"
 
 // update items per lane per batch (loads of a batch are issued together)
@@ -189,6 +190,182 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
             const cplx<T> v = V[m * ld + i];
             Uout[(k * nw + m) * nw + rank] = mk((TO)v.re, (TO)v.im);
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 64 < n <= 128: FP32 Jacobi with A in LDS (132 KB at n = 128) and the
+// eigenvectors in row chunks.  The rotations act on V from the right, so the
+// rows of V are independent; every workgroup of the grid row blockIdx.y runs
+// the same (deterministic) Jacobi on A and accumulates rows [v0, v0 + vr) of
+// V.  Uniform control flow: the round flag is written by wave 0 and read by
+// all waves after a barrier, double-buffered by round parity.
+// ---------------------------------------------------------------------------
+#define JROWS_THREADS 256
+
+struct ItemWalkN {
+    int pr, i, dp, di;
+    __device__ ItemWalkN(int rows, int tid, int nth) : pr(tid / rows), i(tid % rows), dp(nth / rows), di(nth % rows) {}
+    __device__ inline void next(int rows) {
+        pr += dp;
+        i += di;
+        if (i >= rows) {
+            i -= rows;
+            ++pr;
+        }
+    }
+};
+
+__host__ __device__ inline size_t jrows_lds(int n, int vr) {
+    const size_t ld = n + 1;
+    return sizeof(cplx<float>) * (n + vr) * ld + sizeof(float) * 3 * (n / 2) + sizeof(int) * (n + 4) + 16;
+}
+
+template <typename Tin, typename TO>
+__global__ void __launch_bounds__(JROWS_THREADS) k_eigh_jacobi_rows(int nw, const cplx<Tin>* __restrict__ H,
+                                                                   int64_t Hstride, double* __restrict__ Eout,
+                                                                   cplx<TO>* __restrict__ Uout, int max_sweeps,
+                                                                   int vr) {
+    typedef float T;
+    extern __shared__ char smem[];
+    const int n = nw + (nw & 1), ld = n + 1, np = n / 2;
+    cplx<T>* A = (cplx<T>*)smem;
+    cplx<T>* V = A + n * ld;  // rows v0 .. v0 + nv - 1 of V
+    T* prm = (T*)(V + vr * ld);
+    int* rank = (int*)(prm + 3 * np);
+    int* flags = rank + n;  // [0] sweep, [1 + r % 2] round
+    const int tid = threadIdx.x;
+    const int64_t k = blockIdx.x;
+    const int v0 = blockIdx.y * vr;
+    const int nv = n - v0 < vr ? n - v0 : vr;
+    const cplx<Tin>* Hk = H + k * Hstride;
+    for (int t = tid; t < n * n; t += JROWS_THREADS) {
+        const int i = t / n, j = t % n;
+        cplx<T> a = mk((T)0, (T)0);
+        if (i < nw && j < nw) {
+            const cplx<Tin> x = Hk[i * nw + j], y = Hk[j * nw + i];
+            a = mk((T)(0.5 * (x.re + y.re)), (T)(0.5 * (x.im - y.im)));
+        }
+        A[i * ld + j] = a;
+    }
+    for (int t = tid; t < nv * n; t += JROWS_THREADS) {
+        const int r = t / n, j = t % n;
+        V[r * ld + j] = mk((T)(v0 + r == j ? 1 : 0), (T)0);
+    }
+    __syncthreads();
+    const T eps = Eps<T>::v;
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        if (tid == 0) flags[0] = 0;
+        __syncthreads();
+        for (int r = 0; r < n - 1; ++r) {
+            bool rot = false;
+            if (tid < np) {
+                int p, q;
+                rr_pair(n, r, tid, p, q);
+                const cplx<T> apq = A[p * ld + q];
+                const T g = sqrt(apq.re * apq.re + apq.im * apq.im);
+                const T app = A[p * ld + p].re, aqq = A[q * ld + q].re;
+                T c = 1, sc = 0, ss = 0;
+                const T scale = fabs(app) > fabs(aqq) ? fabs(app) : fabs(aqq);
+                if (g > (T)0.5 * eps * scale && g > (T)1e-36) {
+                    const T tau = (aqq - app) / (2 * g);
+                    const T t = (tau >= 0 ? (T)1 : (T)-1) / (fabs(tau) + sqrt((T)1 + tau * tau));
+                    c = (T)1 / sqrt((T)1 + t * t);
+                    const T sn = t * c;
+                    sc = sn * apq.re / g;
+                    ss = sn * apq.im / g;
+                    rot = true;
+                }
+                prm[3 * tid] = c;
+                prm[3 * tid + 1] = sc;
+                prm[3 * tid + 2] = ss;
+            }
+            if (tid < 64) {
+                const bool any = __any(rot);
+                if (tid == 0) {
+                    flags[1 + (r & 1)] = any;
+                    if (any) flags[0] = 1;
+                }
+            }
+            __syncthreads();
+            if (!flags[1 + (r & 1)]) continue;
+            // columns of A (rows i < n) and of the V chunk (rows < nv)
+            for (int part = 0; part < 2; ++part) {
+                cplx<T>* M = part == 0 ? A : V;
+                const int rows = part == 0 ? n : nv;
+                for (ItemWalkN w(rows, tid, JROWS_THREADS); w.pr < np;) {
+                    int ii[JB], pp[JB], qq[JB];
+                    bool ok[JB];
+                    T cc[JB];
+                    cplx<T> se[JB], ap[JB], aq[JB];
+#pragma unroll
+                    for (int u = 0; u < JB; ++u) {
+                        ok[u] = w.pr < np;
+                        const int pr = ok[u] ? w.pr : 0;
+                        rr_pair(n, r, pr, pp[u], qq[u]);
+                        ii[u] = w.i;
+                        w.next(rows);
+                        cc[u] = prm[3 * pr];
+                        se[u] = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
+                        ap[u] = M[ii[u] * ld + pp[u]];
+                        aq[u] = M[ii[u] * ld + qq[u]];
+                    }
+#pragma unroll
+                    for (int u = 0; u < JB; ++u) {
+                        if (!ok[u]) continue;
+                        M[ii[u] * ld + pp[u]] = cc[u] * ap[u] - mulc(aq[u], se[u]);
+                        M[ii[u] * ld + qq[u]] = se[u] * ap[u] + cc[u] * aq[u];
+                    }
+                }
+            }
+            __syncthreads();
+            // rows of A
+            for (ItemWalkN w(n, tid, JROWS_THREADS); w.pr < np;) {
+                int jj[JB], pp[JB], qq[JB];
+                bool ok[JB];
+                T cc[JB];
+                cplx<T> se[JB], ap[JB], aq[JB];
+#pragma unroll
+                for (int u = 0; u < JB; ++u) {
+                    ok[u] = w.pr < np;
+                    const int pr = ok[u] ? w.pr : 0;
+                    rr_pair(n, r, pr, pp[u], qq[u]);
+                    jj[u] = w.i;
+                    w.next(n);
+                    cc[u] = prm[3 * pr];
+                    se[u] = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
+                    ap[u] = A[pp[u] * ld + jj[u]];
+                    aq[u] = A[qq[u] * ld + jj[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < JB; ++u) {
+                    if (!ok[u]) continue;
+                    A[pp[u] * ld + jj[u]] = cc[u] * ap[u] - se[u] * aq[u];
+                    A[qq[u] * ld + jj[u]] = mulc(ap[u], se[u]) + cc[u] * aq[u];
+                }
+            }
+            __syncthreads();
+        }
+        if (flags[0] == 0) break;
+        __syncthreads();
+    }
+    // ascending order (ties by index); every chunk computes the same ranks
+    for (int i = tid; i < nw; i += JROWS_THREADS) {
+        const T ei = A[i * ld + i].re;
+        int rk = 0;
+        for (int j = 0; j < nw; ++j) {
+            const T ej = A[j * ld + j].re;
+            rk += (ej < ei) || (ej == ei && j < i);
+        }
+        rank[i] = rk;
+        if (blockIdx.y == 0) Eout[k * nw + rk] = (double)ei;
+    }
+    __syncthreads();
+    for (int t = tid; t < nv * nw; t += JROWS_THREADS) {
+        const int r = t / nw, i = t % nw;
+        if (v0 + r >= nw) continue;
+        const cplx<T> v = V[r * ld + i];
+        Uout[(k * nw + v0 + r) * nw + rank[i]] = mk((TO)v.re, (TO)v.im);
     }
 }
 
@@ -394,21 +571,213 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
     for (int i = tid; i < nw; i += OA_THREADS) Eout[k * nw + rank[i]] = lam[i];
 }
 
+// ---------------------------------------------------------------------------
+// Refinement for 44 < nw <= 128: the same steps with the four nw x nw matrices
+// (X, T, S/E, R) in a per-k workspace of global memory and every product as
+// 32 x 32 wave tasks on the FP64 matrix cores (operands read through L2).
+// ---------------------------------------------------------------------------
+#define OAG_THREADS 512
+#define OAG_NWMAX 128
+
+// C = P Q (n x n) as wave tasks; out(r, c, value) for r, c < n
+template <class FP, class FQ, class FO>
+__device__ __attribute__((always_inline)) void wg_gemm(int n, FP P, FQ Q, FO out) {
+    const int wave = threadIdx.x >> 6, nwave = OAG_THREADS / 64;
+    const int nb = (n + 31) / 32;
+    for (int t = wave; t < nb * nb; t += nwave) {
+        const int rb = t / nb, cb = t % nb;
+        MfmaBlock C;
+        C.zero();
+        mfma_block(C, n, n, n, rb * 32, cb * 32, P, Q);
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = rb * 32 + mfma_row(bi, v), c = cb * 32 + mfma_col(bj);
+                    if (r < n && c < n) out(r, c, C.get(bi, bj, v));
+                }
+    }
+}
+
+__device__ inline double block_sum_to(double v, double* slot) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if ((threadIdx.x & 63) == 0) atomicAdd(slot, v);
+    return v;
+}
+
+template <typename TU>
+__global__ void __launch_bounds__(OAG_THREADS) k_eigh_refine_g(int n, const cplx<double>* __restrict__ H64,
+                                                              int64_t Hstride, double* __restrict__ Eout,
+                                                              cplx<TU>* __restrict__ U, int iters, double tol,
+                                                              cplx<double>* __restrict__ work) {
+    __shared__ double lam[OAG_NWMAX];
+    __shared__ double red[4];  // [0] ||S-D||^2, [1] ||R||^2, [2] max|lam|, [3] ||E||^2
+    __shared__ int rank[OAG_NWMAX];
+    const int tid = threadIdx.x, n2 = n * n;
+    const int64_t k = blockIdx.x;
+    const cplx<double>* Hk = H64 + k * Hstride;
+    cplx<TU>* Uk = U + k * n2;
+    cplx<double>* X = work + k * 4 * (int64_t)n2;
+    cplx<double>* T = X + n2;
+    cplx<double>* S = T + n2;
+    cplx<double>* R = S + n2;
+    auto Aop = [&](int r, int m) {
+        const cplx<double> x = Hk[r * n + m], y = Hk[m * n + r];
+        return mk(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
+    };
+    for (int t = tid; t < n2; t += OAG_THREADS) {
+        const cplx<TU> u = Uk[t];
+        X[t] = mk((double)u.re, (double)u.im);
+    }
+    __syncthreads();
+    for (int it = 0; it < iters; ++it) {
+        if (tid < 4) red[tid] = 0.0;
+        wg_gemm(n, Aop, [&](int m, int c) { return X[m * n + c]; }, [&](int r, int c, cplx<double> v) { T[r * n + c] = v; });
+        __syncthreads();
+        wg_gemm(n, [&](int r, int m) { return conj(X[m * n + r]); }, [&](int m, int c) { return T[m * n + c]; },
+                [&](int r, int c, cplx<double> v) { S[r * n + c] = v; });
+        wg_gemm(n, [&](int r, int m) { return conj(X[m * n + r]); }, [&](int m, int c) { return X[m * n + c]; },
+                [&](int r, int c, cplx<double> v) { R[r * n + c] = mk((r == c ? 1.0 : 0.0) - v.re, -v.im); });
+        __syncthreads();
+        for (int i = tid; i < n; i += OAG_THREADS) lam[i] = S[i * n + i].re / (1.0 - R[i * n + i].re);
+        __syncthreads();
+        double s2 = 0.0, r2 = 0.0, lm = 0.0;
+        for (int t = tid; t < n2; t += OAG_THREADS) {
+            const int i = t / n, j = t % n;
+            cplx<double> sv = S[t];
+            if (i == j) sv.re -= lam[i];
+            s2 += sv.re * sv.re + sv.im * sv.im;
+            r2 += R[t].re * R[t].re + R[t].im * R[t].im;
+        }
+        for (int i = tid; i < n; i += OAG_THREADS) lm = fmax(lm, fabs(lam[i]));
+        for (int off = 32; off > 0; off >>= 1) lm = fmax(lm, __shfl_down(lm, off));
+        block_sum_to(s2, &red[0]);
+        block_sum_to(r2, &red[1]);
+        if ((tid & 63) == 0) atomicMax((unsigned long long*)&red[2], (unsigned long long)__double_as_longlong(lm));
+        __syncthreads();
+        const double delta = 2.0 * (sqrt(red[0]) + red[2] * sqrt(red[1]));
+        double e2 = 0.0;
+        for (int t = tid; t < n2; t += OAG_THREADS) {
+            const int i = t / n, j = t % n;
+            const double dl = lam[j] - lam[i];
+            const cplx<double> r = R[t];
+            cplx<double> e;
+            if (fabs(dl) > delta) {
+                const cplx<double> sv = S[t];
+                e = (1.0 / dl) * mk(sv.re + lam[j] * r.re, sv.im + lam[j] * r.im);
+            } else {
+                e = 0.5 * r;
+            }
+            S[t] = e;
+            e2 += e.re * e.re + e.im * e.im;
+        }
+        block_sum_to(e2, &red[3]);
+        __syncthreads();
+        // every thread reads red[3] here; it is cleared only after the next barrier
+        const bool converged = red[3] < tol * tol;
+        wg_gemm(n, [&](int r, int m) { return X[r * n + m]; }, [&](int m, int c) { return S[m * n + c]; },
+                [&](int r, int c, cplx<double> v) { T[r * n + c] = X[r * n + c] + v; });
+        __syncthreads();
+        cplx<double>* t = X;
+        X = T;
+        T = t;
+        if (converged) break;
+    }
+    if (sizeof(TU) == 8) {
+        // Newton-Schulz orthonormalisation X <- X (I + R/2), R = I - X^+ X
+        wg_gemm(n, [&](int r, int m) { return conj(X[m * n + r]); }, [&](int m, int c) { return X[m * n + c]; },
+                [&](int r, int c, cplx<double> v) { R[r * n + c] = mk(0.5 * ((r == c ? 1.0 : 0.0) - v.re), -0.5 * v.im); });
+        __syncthreads();
+        wg_gemm(n, [&](int r, int m) { return X[r * n + m]; }, [&](int m, int c) { return R[m * n + c]; },
+                [&](int r, int c, cplx<double> v) { T[r * n + c] = X[r * n + c] + v; });
+        __syncthreads();
+        cplx<double>* t = X;
+        X = T;
+        T = t;
+    }
+    // Rayleigh quotients of the refined columns
+    wg_gemm(n, Aop, [&](int m, int c) { return X[m * n + c]; }, [&](int r, int c, cplx<double> v) { T[r * n + c] = v; });
+    __syncthreads();
+    for (int c = tid; c < n; c += OAG_THREADS) {
+        double num = 0.0, den = 0.0;
+        for (int m = 0; m < n; ++m) {
+            const cplx<double> x = X[m * n + c], y = T[m * n + c];
+            num += x.re * y.re + x.im * y.im;
+            den += x.re * x.re + x.im * x.im;
+        }
+        lam[c] = num / den;
+    }
+    __syncthreads();
+    for (int i = tid; i < n; i += OAG_THREADS) {
+        int rk = 0;
+        for (int j = 0; j < n; ++j) rk += (lam[j] < lam[i]) || (lam[j] == lam[i] && j < i);
+        rank[i] = rk;
+    }
+    __syncthreads();
+    for (int t = tid; t < n2; t += OAG_THREADS) {
+        const int r = t / n, c = t % n;
+        const cplx<double> x = X[t];
+        Uk[r * n + rank[c]] = mk((TU)x.re, (TU)x.im);
+    }
+    for (int i = tid; i < n; i += OAG_THREADS) Eout[k * n + rank[i]] = lam[i];
+}
+
+// bytes of global workspace wbmi_eigh needs (refinement for nw > 44)
+static size_t eigh_work_bytes(int nk, int nw, int refine) {
+    return (refine && nw > 44) ? sizeof(cplx<double>) * 4 * (size_t)nw * nw * nk : 0;
+}
+
+extern "C" size_t wbmi_eigh_work_bytes(int nk, int nw, int prec, int refine) {
+    (void)prec;
+    return nk > 0 && nw > 0 ? eigh_work_bytes(nk, nw, refine) : 0;
+}
+
+// FP32 Jacobi of the FP64 Hamiltonian (eigenvectors stored as TU)
+template <typename TU>
+static int jacobi_f32_of_f64(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* H64, int64_t Hstride, double* E,
+                             cplx<TU>* U) {
+    const int n = nw + (nw & 1);
+    const int sweeps = 40;
+    if (n <= 64) {
+        const size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double, TU>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_eigh_jacobi<float, double, TU>), dim3(nk), dim3(64), lds, ctx->stream, nw, H64, Hstride,
+                           E, U, sweeps);
+    } else {
+        // rows of V per workgroup: what the LDS leaves after A
+        int vr = 1;
+        while (vr < n && jrows_lds(n, vr + 1) <= 160 * 1024) ++vr;
+        const int nchunk = (n + vr - 1) / vr;
+        const size_t lds = jrows_lds(n, vr);
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi_rows<double, TU>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_eigh_jacobi_rows<double, TU>), dim3(nk, nchunk), dim3(JROWS_THREADS), lds, ctx->stream,
+                           nw, H64, Hstride, E, U, sweeps, vr);
+    }
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
 // FP32 Jacobi of the FP64 Hamiltonian, then FP64 Ogita-Aishima steps; eigenvectors stored as TU
 template <typename TU>
 static int eigh_refined(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* H64, int64_t Hstride, double* E,
-                        cplx<TU>* U, int steps) {
-    const int n = nw + (nw & 1);
-    const int sweeps = 40;
-    const size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
-    const size_t lds2 = 4 * sizeof(cplx<double>) * nw * (nw + 1) + sizeof(double) * (nw + 4) + sizeof(int) * nw + 16;
-    if (lds2 > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement needs nw <= 44");
+                        cplx<TU>* U, int steps, void* work, size_t work_bytes) {
     const double tol = sizeof(TU) == 4 ? OA_TOL_C64 : OA_TOL_C128;
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double, TU>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_eigh_jacobi<float, double, TU>), dim3(nk), dim3(64), lds, ctx->stream, nw, H64, Hstride, E,
-                       U, sweeps);
-    WBMI_LAUNCH_CHECK(ctx);
+    int rc = jacobi_f32_of_f64<TU>(ctx, nk, nw, H64, Hstride, E, U);
+    if (rc) return rc;
+    if (nw > 44) {
+        if (!work || work_bytes < eigh_work_bytes(nk, nw, steps))
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement at nw = %d needs %zu bytes of workspace", nw,
+                             eigh_work_bytes(nk, nw, steps));
+        hipLaunchKernelGGL((k_eigh_refine_g<TU>), dim3(nk), dim3(OAG_THREADS), 0, ctx->stream, nw, H64, Hstride, E, U,
+                           steps, tol, (cplx<double>*)work);
+        WBMI_LAUNCH_CHECK(ctx);
+        return WBMI_OK;
+    }
+    const size_t lds2 = 4 * sizeof(cplx<double>) * nw * (nw + 1) + sizeof(double) * (nw + 4) + sizeof(int) * nw + 16;
     if (nw <= 32) {
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<TU, 2>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
@@ -425,10 +794,10 @@ static int eigh_refined(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* H64, 
 }
 
 extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride, double* E_dev,
-                         void* U_dev, int prec, int refine, const void* H64_dev) {
+                         void* U_dev, int prec, int refine, const void* H64_dev, void* work_dev, size_t work_bytes) {
     if (!ctx) return WBMI_ERR_ARG;
-    if (nk <= 0 || nw <= 0 || nw > 64 || !E_dev || !U_dev || Hstride < (int64_t)nw * nw)
-        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad arguments (nw <= 64, Hstride >= nw*nw)");
+    if (nk <= 0 || nw <= 0 || nw > 128 || !E_dev || !U_dev || Hstride < (int64_t)nw * nw)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad arguments (nw <= 128, Hstride >= nw*nw)");
     if (refine < 0 || refine > 8) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refine must be 0..8");
     if (prec != WBMI_F64 && prec != WBMI_F32) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad precision");
     if (refine) {
@@ -437,10 +806,12 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
         if (!H64) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refinement needs the FP64 Hamiltonian");
         if (prec == WBMI_F64)
             return eigh_refined<double>(ctx, nk, nw, (const cplx<double>*)H64, Hstride, E_dev, (cplx<double>*)U_dev,
-                                        refine);
-        return eigh_refined<float>(ctx, nk, nw, (const cplx<double>*)H64, Hstride, E_dev, (cplx<float>*)U_dev, refine);
+                                        refine, work_dev, work_bytes);
+        return eigh_refined<float>(ctx, nk, nw, (const cplx<double>*)H64, Hstride, E_dev, (cplx<float>*)U_dev, refine,
+                                   work_dev, work_bytes);
     }
     if (!H_dev) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: H_dev is null");
+    if (nw > 64) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: nw > 64 needs refine >= 1");
     const int n = nw + (nw & 1);
     const int sweeps = 40;
     if (prec == WBMI_F64) {

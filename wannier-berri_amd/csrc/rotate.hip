@@ -239,9 +239,95 @@ __global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, c
     }
 }
 
+// 64 < nw <= 128: one 512-thread workgroup per k, channels in turn.  Both
+// products are 32 x 32 wave tasks on the FP64 matrix cores with operands read
+// from global memory (L2): T = X U into a per-k FP64 workspace, then
+// Xbar = U^+ T over the channel in place.
+#define ROTG_THREADS 512
+template <typename T>
+__global__ void __launch_bounds__(ROTG_THREADS) k_rotate_g(int nw, int nch, const cplx<T>* __restrict__ U, cplx<T>* X,
+                                                           uint64_t herm_lo, uint64_t herm_hi, uint64_t skip_lo,
+                                                           uint64_t skip_hi, cplx<double>* __restrict__ work) {
+    const int n = nw, n2 = nw * nw, wave = threadIdx.x >> 6, nwave = ROTG_THREADS / 64;
+    const int nb = (n + 31) / 32;
+    const int64_t k = blockIdx.x;
+    const cplx<T>* Uk = U + k * n2;
+    cplx<double>* Tk = work + k * (int64_t)n2;
+    auto u = [&](int r, int c) {
+        const cplx<T> v = Uk[r * n + c];
+        return mk((double)v.re, (double)v.im);
+    };
+    for (int c = 0; c < nch; ++c) {
+        if (bit_of(skip_lo, skip_hi, c)) continue;
+        const bool herm = bit_of(herm_lo, herm_hi, c);
+        cplx<T>* Xc = X + (k * nch + c) * n2;
+        auto x = [&](int r, int m) {
+            const cplx<T> v = Xc[r * n + m];
+            cplx<double> a = mk((double)v.re, (double)v.im);
+            if (herm) {
+                const cplx<T> w = Xc[m * n + r];
+                a = mk(0.5 * (a.re + (double)w.re), 0.5 * (a.im - (double)w.im));
+            }
+            return a;
+        };
+        for (int t = wave; t < nb * nb; t += nwave) {
+            const int rb = t / nb, cb = t % nb;
+            MfmaBlock C;
+            C.zero();
+            mfma_block(C, n, n, n, rb * 32, cb * 32, x, u);
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = rb * 32 + mfma_row(bi, v), cc = cb * 32 + mfma_col(bj);
+                        if (r < n && cc < n) Tk[r * n + cc] = C.get(bi, bj, v);
+                    }
+        }
+        __syncthreads();  // T complete; every read of X done
+        for (int t = wave; t < nb * nb; t += nwave) {
+            const int rb = t / nb, cb = t % nb;
+            MfmaBlock C;
+            C.zero();
+            mfma_block(C, n, n, n, rb * 32, cb * 32, [&](int r, int m) { return conj(u(m, r)); },
+                       [&](int m, int cc) { return Tk[m * n + cc]; });
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int r = rb * 32 + mfma_row(bi, v), cc = cb * 32 + mfma_col(bj);
+                        if (r < n && cc < n) {
+                            const cplx<double> val = C.get(bi, bj, v);
+                            Xc[r * n + cc] = mk((T)val.re, (T)val.im);
+                        }
+                    }
+        }
+        __syncthreads();  // the workspace is reused by the next channel
+    }
+}
+
+static size_t rotate_work_bytes(int nk, int nw) { return nw > 64 ? sizeof(cplx<double>) * (size_t)nw * nw * nk : 0; }
+
+extern "C" size_t wbmi_rotate_work_bytes(int nk, int nw, int prec) {
+    (void)prec;
+    return nk > 0 && nw > 0 ? rotate_work_bytes(nk, nw) : 0;
+}
+
 template <typename T>
 static int launch_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_dev, void* X_dev, const uint64_t* h,
-                         const uint64_t* s) {
+                         const uint64_t* s, void* work, size_t work_bytes) {
+    if (nw > 64) {
+        if (!work || work_bytes < rotate_work_bytes(nk, nw))
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_rotate: nw = %d needs %zu bytes of workspace", nw,
+                             rotate_work_bytes(nk, nw));
+        hipLaunchKernelGGL(k_rotate_g<T>, dim3(nk), dim3(ROTG_THREADS), 0, ctx->stream, nw, nch, (const cplx<T>*)U_dev,
+                           (cplx<T>*)X_dev, h[0], h[1], s[0], s[1], (cplx<double>*)work);
+        WBMI_LAUNCH_CHECK(ctx);
+        return WBMI_OK;
+    }
     if (nw <= 32) {
         const size_t lds = (1 + ROTW_WAVES) * rotw_tile_bytes(nw);
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_rotate_w<T>,
@@ -264,16 +350,17 @@ static int launch_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_d
 }
 
 extern "C" int wbmi_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_dev, void* X_dev,
-                           const int32_t* herm_host, const int32_t* skip_host, int prec) {
+                           const int32_t* herm_host, const int32_t* skip_host, int prec, void* work_dev,
+                           size_t work_bytes) {
     if (!ctx) return WBMI_ERR_ARG;
-    if (nk <= 0 || nw <= 0 || nw > 64 || nch <= 0 || nch > 128 || !U_dev || !X_dev)
-        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_rotate: bad arguments (nw <= 64, nch <= 128)");
+    if (nk <= 0 || nw <= 0 || nw > 128 || nch <= 0 || nch > 128 || !U_dev || !X_dev)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_rotate: bad arguments (nw <= 128, nch <= 128)");
     uint64_t h[2] = {0, 0}, s[2] = {0, 0};
     for (int c = 0; c < nch; ++c) {
         if (herm_host && herm_host[c]) h[c / 64] |= 1ull << (c % 64);
         if (skip_host && skip_host[c]) s[c / 64] |= 1ull << (c % 64);
     }
-    if (prec == WBMI_F64) return launch_rotate<double>(ctx, nk, nw, nch, U_dev, X_dev, h, s);
-    if (prec == WBMI_F32) return launch_rotate<float>(ctx, nk, nw, nch, U_dev, X_dev, h, s);
+    if (prec == WBMI_F64) return launch_rotate<double>(ctx, nk, nw, nch, U_dev, X_dev, h, s, work_dev, work_bytes);
+    if (prec == WBMI_F32) return launch_rotate<float>(ctx, nk, nw, nch, U_dev, X_dev, h, s, work_dev, work_bytes);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_rotate: bad precision");
 }

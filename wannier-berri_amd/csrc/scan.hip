@@ -42,7 +42,8 @@
 
 #define SC_THREADS 512
 #define SC_WAVES (SC_THREADS / 64)
-#define SC_NWMAX 64
+#define SC_NWMAX 128
+#define SC_M_NW 64    // dEinv o Off lives in LDS up to this nw, else it is formed on the fly
 #define SC_LDS_NW 32  // V and A live in LDS up to this nw
 
 __constant__ int c_alpha[3] = {1, 2, 0};  // utility.py:45-46
@@ -94,7 +95,7 @@ __host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool s
     o.acc = off;
     off = al16(off + sizeof(cplx<double>) * nw * 9);
     o.M = off;
-    off = al16(off + sizeof(double) * nw * ld);
+    off = al16(off + (nw <= SC_M_NW ? sizeof(double) * nw * ld : 0));
     o.E = off;
     off = al16(off + sizeof(double) * nw);
     o.cut = off;
@@ -111,11 +112,12 @@ __device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int 
                                                                 bool stage_w) {
     const SmemOffsets o = smem_offsets(nw, stage_v, stage_a, stage_w);
     Smem s;
-    s.V = stage_v ? (cplx<double>*)(base + o.V) : nullptr;
-    s.A = stage_a ? (cplx<double>*)(base + o.A) : nullptr;
-    s.accw = stage_w ? (double*)(base + o.accw) : nullptr;
+    // LDS addresses even where a region is empty (never read then): no null tests on LDS pointers
+    s.V = (cplx<double>*)(base + o.V);
+    s.A = (cplx<double>*)(base + o.A);
+    s.accw = (double*)(base + o.accw);
     s.acc = (cplx<double>*)(base + o.acc);
-    s.M = (double*)(base + o.M);
+    s.M = (double*)(base + o.M);  // empty region (never read) when nw > SC_M_NW
     s.E = (double*)(base + o.E);
     s.cut = (double*)(base + o.cut);
     s.gw = (double*)(base + o.gw);
@@ -132,7 +134,8 @@ __device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int 
 }
 
 // Element access to the channels of one k-point (LDS copies of V and A when staged).
-template <typename T>
+// LM: dEinv o Off read from LDS (nw <= SC_M_NW) or formed from E and the labels
+template <typename T, bool LM>
 struct KCtx {
     const cplx<T>* X;  // channels of this k: [nch][nw][nw]
     int nw, ld;
@@ -142,17 +145,18 @@ struct KCtx {
     const int* lab;
     const int* gs;
     const int* ge;
-    const cplx<double>* sV;
+    const cplx<double>* sV;  // LDS copies of V and A, used when stV / stA
     const cplx<double>* sA;
+    bool stV, stA;
     __device__ inline cplx<double> get(int role, int x, int y) const {
         cplx<T> v = X[((int64_t)ch[role] * nw + x) * nw + y];
         return mk((double)v.re, (double)v.im);
     }
     __device__ inline cplx<double> V(int a, int x, int y) const {
-        return sV ? sV[(a * nw + x) * ld + y] : get(WBMI_ROLE_V + a, x, y);
+        return stV ? sV[(a * nw + x) * ld + y] : get(WBMI_ROLE_V + a, x, y);
     }
     __device__ inline cplx<double> A(int a, int x, int y) const {
-        return sA ? sA[(a * nw + x) * ld + y] : get(WBMI_ROLE_A + a, x, y);
+        return stA ? sA[(a * nw + x) * ld + y] : get(WBMI_ROLE_A + a, x, y);
     }
     __device__ inline cplx<double> W(int a, int b, int x, int y) const { return get(WBMI_ROLE_W + 3 * a + b, x, y); }
     __device__ inline cplx<double> Ad(int a, int d, int x, int y) const { return get(WBMI_ROLE_AD + 3 * a + d, x, y); }
@@ -165,15 +169,23 @@ struct KCtx {
     // X o On: elements between bands of the same group
     __device__ inline cplx<double> Ain(int a, int x, int y) const { return lab[x] == lab[y] ? A(a, x, y) : mk(0.0, 0.0); }
     __device__ inline cplx<double> Vin(int a, int x, int y) const { return lab[x] == lab[y] ? V(a, x, y) : mk(0.0, 0.0); }
-    __device__ inline double m(int x, int y) const { return M[x * ld + y]; }
+    // dEinv_xy Off(x, y) (data_K.py:290-298): from LDS, or formed from E and the labels
+    __device__ inline double m(int x, int y) const {
+        if constexpr (LM) {
+            return M[x * ld + y];
+        } else {
+            const double de = E[x] - E[y];
+            return (lab[x] != lab[y] && fabs(de) >= 1e-7) ? 1.0 / de : 0.0;
+        }
+    }
     // D'^a_xy = -V^a_xy dEinv_xy Off(x, y)  (branch-free: M = 0 gives an exact zero for finite V)
-    __device__ inline cplx<double> Dp(int a, int x, int y) const { return (-M[x * ld + y]) * V(a, x, y); }
+    __device__ inline cplx<double> Dp(int a, int x, int y) const { return (-m(x, y)) * V(a, x, y); }
 };
 
 // pre-Hermitian Omega summand s^c_xy for x, y in the same group
 // (formula/covariant.py:175-200): Omega.nn = s + s^+
-template <typename T>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T>& K, int c, int x, int y, bool internal, bool external) {
+template <typename T, bool LM>
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM>& K, int c, int x, int y, bool internal, bool external) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
     for (int l = 0; l < K.nw; ++l) {
@@ -191,8 +203,8 @@ __device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T>& K,
 
 // pre-Hermitian Morb_H summand h^c_xy for x, y in the same group
 // (formula/covariant.py:390-414)
-template <typename T>
-__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T>& K, int c, int x, int y, bool internal, bool external) {
+template <typename T, bool LM>
+__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM>& K, int c, int x, int y, bool internal, bool external) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
     for (int l = 0; l < K.nw; ++l) {
@@ -291,8 +303,8 @@ __host__ __device__ constexpr int task_nt() {
     return KIND == 0 ? 5 : KIND == 1 ? 4 : KIND == 2 ? 2 : KIND == 3 ? 2 : KIND == 4 ? 5 : KIND == 7 ? 1 : 3;
 }
 
-template <int KIND, typename T>
-__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T>& K, const TaskCtx& tc_,
+template <int KIND, typename T, bool LM>
+__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM>& K, const TaskCtx& tc_,
                                                            double* accw, int b, int d, int rb, int cb) {
     const int nw = K.nw, lane = threadIdx.x & 63;
     const int r0 = rb * 32, c0 = cb * 32;
@@ -492,8 +504,8 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
 // DerMorb = DerMorb_H + sign [Eav o dOmega + (Omega V + V Omega)/2] on cut
 // partitions; its trace terms (s1 ... s7, the Omega block) are listed at
 // formula_task.
-template <typename T>
-__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T>& K, bool in, bool ex,
+template <typename T, bool LM>
+__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM>& K, bool in, bool ex,
                                                             bool dm, int sign) {
     const int nw = K.nw, tid = threadIdx.x, wave = tid >> 6;
     // per-band terms, lanes running over (n, c, d)
@@ -587,12 +599,12 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
 }
 
 // Per-band values for the current partition into S.acc[n][comp] (.re).
-template <int F, typename T>
-__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T>& K, const wbmi_scan_params& p) {
+template <int F, typename T, bool LM>
+__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM>& K, const wbmi_scan_params& p) {
     const int nw = K.nw;
     const int tid = threadIdx.x;
     const bool in = p.internal_terms, ex = p.external_terms;
-    build_M(S, nw);
+    if constexpr (LM) build_M(S, nw);
     for (int t = tid; t < nw * 9; t += SC_THREADS) S.acc[t] = mk(0.0, 0.0);
     __syncthreads();
     if (F == WBMI_FORMULA_IDENTITY) {
@@ -660,7 +672,7 @@ __device__ inline int fermi_bin(const wbmi_scan_params& p, double E) {
     return -1;
 }
 
-template <typename T, int F>
+template <typename T, int F, bool LM>
 __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
                                                             const cplx<T>* __restrict__ Xall,
                                                             double* __restrict__ hist) {
@@ -744,7 +756,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const int ng = S.misc[0];
     const int bandmax = S.misc[1];
 
-    KCtx<T> K;
+    KCtx<T, LM> K;
     K.X = Xk;
     K.nw = nw;
     K.ld = ld;
@@ -754,8 +766,10 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     K.lab = S.lab;
     K.gs = S.gs;
     K.ge = S.ge;
-    K.sV = stage_v ? S.V : nullptr;
-    K.sA = stage_a ? S.A : nullptr;
+    K.sV = S.V;
+    K.sA = S.A;
+    K.stV = stage_v;
+    K.stA = stage_a;
 
     if (additive) {
         // main partition: every group at once
@@ -877,19 +891,26 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
     return WBMI_OK;
 }
 
-template <typename T, int F>
-static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
+template <typename T, int F, bool LM>
+static int launch_scan_m(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
     const bool stage_v = a.stage_va && tasks;
     const bool stage_a = stage_v && a.p.external_terms;
     size_t lds = smem_offsets(a.nw, stage_v, stage_a, tasks).total;
     if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F>,
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_static_scan<T, F>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E, (const cplx<T>*)X,
-                       hist);
+    hipLaunchKernelGGL((k_static_scan<T, F, LM>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
+                       (const cplx<T>*)X, hist);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
+}
+
+// dEinv o Off in LDS up to SC_M_NW (the layout of smem_offsets), formed on the fly above
+template <typename T, int F>
+static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
+    if (a.nw <= SC_M_NW) return launch_scan_m<T, F, true>(ctx, nk, a, E, X, hist);
+    return launch_scan_m<T, F, false>(ctx, nk, a, E, X, hist);
 }
 
 template <typename T>
@@ -931,43 +952,60 @@ extern "C" int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const do
 // histogram -> result (static.py:137-155)
 // ---------------------------------------------------------------------------
 
-__global__ void k_finalize(const double* __restrict__ hist, int nEF, int ncomp, int fder, double dEF, double scale,
-                           double* __restrict__ out) {
-    extern __shared__ double cum[];  // [nEF][ncomp] prefix sums
-    for (int c = threadIdx.x; c < ncomp; c += blockDim.x) {
-        double s = 0.0;
-        for (int i = 0; i < nEF; ++i) {
-            s += hist[i * ncomp + c];
-            cum[i * ncomp + c] = s;
+#define FIN_THREADS 256
+// One workgroup: per component, an in-place block prefix sum over the Fermi
+// levels (each thread owns a contiguous run), then the fder stencil.
+__global__ void __launch_bounds__(FIN_THREADS) k_finalize(double* __restrict__ hist, int nEF, int ncomp, int fder,
+                                                         double dEF, double scale, double* __restrict__ out) {
+    __shared__ double part[FIN_THREADS];
+    const int tid = threadIdx.x;
+    const int L = (nEF + FIN_THREADS - 1) / FIN_THREADS;
+    const int i0 = tid * L < nEF ? tid * L : nEF, i1 = i0 + L < nEF ? i0 + L : nEF;
+    for (int c = 0; c < ncomp; ++c) {
+        double sum = 0.0;
+        for (int i = i0; i < i1; ++i) sum += hist[(int64_t)i * ncomp + c];
+        part[tid] = sum;
+        __syncthreads();
+        // inclusive scan of the per-thread sums (Hillis-Steele)
+        for (int off = 1; off < FIN_THREADS; off <<= 1) {
+            const double v = tid >= off ? part[tid - off] : 0.0;
+            __syncthreads();
+            part[tid] += v;
+            __syncthreads();
         }
+        double run = tid > 0 ? part[tid - 1] : 0.0;
+        for (int i = i0; i < i1; ++i) {
+            run += hist[(int64_t)i * ncomp + c];
+            hist[(int64_t)i * ncomp + c] = run;
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    int extra = fder == 0 ? 0 : (fder == 3 ? 2 : 1);
-    int nout = nEF - 2 * extra;
-    for (int t = threadIdx.x; t < nout * ncomp; t += blockDim.x) {
-        int i = t / ncomp, c = t % ncomp;
-        const double* r = cum + c;
+    const int extra = fder == 0 ? 0 : (fder == 3 ? 2 : 1);
+    const int nout = nEF - 2 * extra;
+    for (int t = tid; t < nout * ncomp; t += FIN_THREADS) {
+        const int i = t / ncomp, c = t % ncomp;
+        const double* r = hist + c;
         double v;
-        if (fder == 0) v = r[i * ncomp];
-        else if (fder == 1) v = (r[(i + 2) * ncomp] - r[i * ncomp]) / (2 * dEF);
-        else if (fder == 2) v = (r[(i + 2) * ncomp] + r[i * ncomp] - 2 * r[(i + 1) * ncomp]) / (dEF * dEF);
-        else v = (r[(i + 4) * ncomp] - r[i * ncomp] - 2 * (r[(i + 3) * ncomp] - r[(i + 1) * ncomp])) / (2 * dEF * dEF * dEF);
+        if (fder == 0) v = r[(int64_t)i * ncomp];
+        else if (fder == 1) v = (r[(int64_t)(i + 2) * ncomp] - r[(int64_t)i * ncomp]) / (2 * dEF);
+        else if (fder == 2)
+            v = (r[(int64_t)(i + 2) * ncomp] + r[(int64_t)i * ncomp] - 2 * r[(int64_t)(i + 1) * ncomp]) / (dEF * dEF);
+        else
+            v = (r[(int64_t)(i + 4) * ncomp] - r[(int64_t)i * ncomp] -
+                 2 * (r[(int64_t)(i + 3) * ncomp] - r[(int64_t)(i + 1) * ncomp])) /
+                (2 * dEF * dEF * dEF);
         out[t] = v * scale;
     }
 }
 
-extern "C" int wbmi_finalize(wbmi_ctx* ctx, const double* hist_dev, int nEF_extra, int ncomp, int fder, double dEF,
+extern "C" int wbmi_finalize(wbmi_ctx* ctx, double* hist_dev, int nEF_extra, int ncomp, int fder, double dEF,
                              double scale, double* out_dev) {
     if (!ctx) return WBMI_ERR_ARG;
     int extra = fder == 0 ? 0 : (fder == 3 ? 2 : 1);
     if (!hist_dev || !out_dev || ncomp <= 0 || fder < 0 || fder > 3 || nEF_extra - 2 * extra <= 0)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize: bad arguments");
-    size_t lds = sizeof(double) * (size_t)nEF_extra * ncomp;
-    if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize: too many Fermi levels for one pass");
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_finalize, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            (int)lds));
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), lds, ctx->stream, hist_dev, nEF_extra, ncomp, fder, dEF, scale,
-                       out_dev);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF_extra, ncomp, fder, dEF,
+                       scale, out_dev);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
 }

@@ -224,8 +224,8 @@ class Data_K:
         """Channels `keys` (always including the Hamiltonian) -> E, U and the rotated stack.
 
         FP64 form: one complex128 stack; FP32 Jacobi + FP64 Ogita-Aishima
-        refinement to FP64-accurate energies and complex128 vectors (plain
-        FP64 Jacobi for nw > 44).  Mixed form: the
+        refinement to FP64-accurate energies and complex128 vectors.  Mixed
+        form: the
         Hamiltonian is transformed in FP64 on its own, the other channels in
         complex64; FP32 Jacobi + FP64 Ogita-Aishima steps (at most OA_STEPS) give FP64
         energies and refined eigenvectors (stored as complex64).
@@ -244,19 +244,12 @@ class Data_K:
         X = self._transform(main, self.cdtype, self.prec) if main else None
         if mixed:
             H64 = self._transform([("Ham", 0)], torch.complex128, WBMI_F64)
-            if nw <= 44:
-                self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=OA_STEPS, H64=H64)
-            else:
-                # refinement needs nw <= 44: FP64 Jacobi, vectors rounded to complex64
-                U64 = torch.empty((nk, nw, nw), dtype=torch.complex128, device=self.ctx.device)
-                self.ctx.eigh(H64, nw * nw, E, U64, WBMI_F64, nw)
-                U.copy_(U64)
-                del U64
+            self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=OA_STEPS, H64=H64)
             del H64
         else:
-            # FP32 Jacobi + FP64 refinement to complex128 vectors (nw <= 44); plain FP64 Jacobi above
+            # FP32 Jacobi + FP64 refinement to complex128 vectors
             h0 = slots[("Ham", 0)]
-            self.ctx.eigh(X[:, h0], nch * nw * nw, E, U, self.prec, nw, refine=OA_STEPS if nw <= 44 else 0)
+            self.ctx.eigh(X[:, h0], nch * nw * nw, E, U, self.prec, nw, refine=OA_STEPS)
         if X is not None:
             herm = np.zeros(nch, dtype=np.int32)
             skip = np.zeros(nch, dtype=np.int32)
