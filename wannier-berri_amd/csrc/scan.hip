@@ -278,8 +278,8 @@ __device__ inline double wave_sum(double v, int mask) {
 // P = X Y on the matrix cores (FP64 MFMA, mfma_c64.h), adds the elementwise terms of
 // each element (x, y) and accumulates the consumers' weighted elements into
 // its accumulator rows accw[band][comp] after a shuffle reduction over the
-// summed index.  The band of an element is its column y, except for kind 1
-// (row x).  Kinds (task indices b, d; for kinds 5, 6 b = c):
+// summed index.  The band of an element is its column y (kind 1 forms its
+// product transposed for this).  Kinds (task indices b, d; for kinds 5, 6 b = c):
 //   0  Q1^{bd} = V^b D'^d + (W - D'^b V''^d - D'^d V''^b)/2, y in the partition, x off-side:
 //      DerOmega T1; DerMorb s2 (x 2 E_x), s7 (conj B^a_xy), s1 (raw V^b D'^d)
 //   1  Q2^{bd} = -D'^b V^d + (W + V''^b D'^d + V''^d D'^b)/2, x in, y off-side: DerOmega T3
@@ -315,8 +315,8 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
     P.zero();
     if constexpr (KIND == 0)
         mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.V(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
-    else if constexpr (KIND == 1)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(b, r, m); }, [&](int m, int c) { return K.V(d, m, c); });
+    else if constexpr (KIND == 1)  // transposed: P_xy = (D'^b V^d)_yx, so the band n = y is a column as for the others
+        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.V(d, m, r); }, [&](int m, int c) { return K.Dp(b, c, m); });
     else if constexpr (KIND == 2)
         mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.A(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
     else if constexpr (KIND == 3)
@@ -402,57 +402,93 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
             }
     }
 
-    // part[u][w]: u = the lane's band slot (column bj for kinds != 1, row (bi, v) for kind 1)
-    constexpr int NU = KIND == 1 ? 8 : 2;
+    // part[u][w]: u = the lane's band slot, its column bj (the band n is the column y of every element)
+    constexpr int NU = 2;
     double part[NU][NT];
 #pragma unroll
     for (int u = 0; u < NU; ++u)
 #pragma unroll
         for (int w = 0; w < NT; ++w) part[u][w] = 0.0;
 #pragma unroll
-    for (int bi = 0; bi < 2; ++bi)
+    for (int bi = 0; bi < 2; ++bi) {
+        // the elementwise channel (W, A', B') of this row half: all loads issued before use
+        cplx<double> gz[2][4];
+        if constexpr (KIND <= 3) {
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int x = r0 + mfma_row(bi, v), y = c0 + mfma_col(bj);
+                    const bool ok = x < nw && y < nw;
+                    const int xx = ok ? x : 0, yy = ok ? y : 0;
+                    if constexpr (KIND == 0)
+                        gz[bj][v] = K.W(b, d, xx, yy);  // (l, n)
+                    else if constexpr (KIND == 1)
+                        gz[bj][v] = K.W(b, d, yy, xx);  // (n, l)
+                    else if constexpr (KIND == 2)
+                        gz[bj][v] = ex ? K.Ad(b, d, xx, yy) : mk(0.0, 0.0);
+                    else
+                        gz[bj][v] = K.Bd(b, d, xx, yy);
+                }
+        }
 #pragma unroll
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int x = r0 + mfma_row(bi, v), y = c0 + mfma_col(bj);
             if (x >= nw || y >= nw) continue;
-            const int i = KIND == 1 ? bi * 4 + v : bj, j = i;  // part slot
+            const int i = bj, j = bj;  // part slot
             const cplx<double> pv = P.get(bi, bj, v);
-            const int n = KIND == 1 ? x : y, l = KIND == 1 ? y : x;
+            const int n = y, l = x;
             if (!S.act[n]) continue;
             const bool same = K.lab[l] == K.lab[n];
+            const bool single = S.ge[n] - S.gs[n] == 1;  // group {n}: the within-group sums have one term
             const double dw = dm ? 2.0 * sign * K.E[n] : 2.0;
             if constexpr (KIND == 0) {
                 if (same) continue;
-                // (W^{bd} - D'^b V''^d - D'^d V''^b)_ln / 2
-                cplx<double> z = K.W(b, d, l, n);
-                for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
-                const cplx<double> Pm = (-K.m(l, n)) * (pv + 0.5 * z);
+                // (W^{bd} - D'^b V''^d - D'^d V''^b)_ln / 2, with D'_xy = -m_xy V_xy and m_nl = -m_ln
+                const double mln = K.m(l, n);
+                cplx<double> z = gz[bj][v];
+                if (single)
+                    z += mln * (K.V(b, l, n) * K.V(d, n, n) + K.V(d, l, n) * K.V(b, n, n));
+                else
+                    for (int q = S.gs[n]; q < S.ge[n]; ++q)
+                        z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
+                const cplx<double> Pm = (-mln) * (pv + 0.5 * z);
                 const double wD = (in ? dw : 0.0) + (dm && in ? 2.0 * K.E[l] : 0.0);
 #pragma unroll
                 for (int w = 0; w < 4; ++w) {
-                    double t = wD * (K.Dp(tp[w], n, l) * Pm).im;  // Re(-i D' X1) (T1, s2)
+                    const cplx<double> Dnl = mln * K.V(tp[w], n, l);
+                    double t = wD * (Dnl * Pm).im;  // Re(-i D' X1) (T1, s2)
                     if (dm && ex) t -= 2.0 * (conj(K.B(tp[w], l, n)) * Pm).re;  // s7
                     part[j][w] += t;
                 }
-                if (dm && in) part[j][4] += (K.Dp(tp[4], n, l) * pv).im;  // s1: Re(-i D'^al V^d D'^be)
+                if (dm && in) part[j][4] += ((mln * K.V(tp[4], n, l)) * pv).im;  // s1: Re(-i D'^al V^d D'^be)
             } else if constexpr (KIND == 1) {
                 if (same) continue;
                 // (W^{bd} + V''^b D'^d + V''^d D'^b)_nl / 2
-                cplx<double> z = K.W(b, d, n, l);
-                for (int q = S.gs[n]; q < S.ge[n]; ++q) z += K.V(b, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(b, q, l);
-                const cplx<double> Pm = (-K.m(n, l)) * (0.5 * z - pv);
+                const double mnl = K.m(n, l);
+                cplx<double> z = gz[bj][v];
+                if (single)
+                    z -= mnl * (K.V(b, n, n) * K.V(d, n, l) + K.V(d, n, n) * K.V(b, n, l));
+                else
+                    for (int q = S.gs[n]; q < S.ge[n]; ++q)
+                        z += K.V(b, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(b, q, l);
+                const cplx<double> Pm = (-mnl) * (0.5 * z - pv);
 #pragma unroll
                 for (int w = 0; w < 4; ++w) part[i][w] -= dw * (Pm * K.A(tp[w], l, n)).re;
             } else if constexpr (KIND == 2) {
                 if (!same) {
                     // A'^{bd} - D'^d A''^b
-                    cplx<double> z = K.Ad(b, d, l, n);
-                    for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
+                    const double mln = K.m(l, n);
+                    cplx<double> z = gz[bj][v];
+                    if (single)
+                        z += mln * (K.V(d, l, n) * K.A(b, n, n));
+                    else
+                        for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
                     const cplx<double> Y1 = pv + z;
 #pragma unroll
-                    for (int w = 0; w < 2; ++w) part[j][w] -= dw * (K.Dp(tp[w], n, l) * Y1).re;
+                    for (int w = 0; w < 2; ++w) part[j][w] -= dw * ((mln * K.V(tp[w], n, l)) * Y1).re;
                 } else if (dm) {
                     // s5: Re(-2i sig A^a_nl E_l (A' + A^b D'^d)_ln)
                     const cplx<double> dA = pv + K.Ad(b, d, l, n);
@@ -462,7 +498,7 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
             } else if constexpr (KIND == 3) {
                 if (same) continue;
                 // B'^{bd} - D'^d B''^b
-                cplx<double> z = K.Bd(b, d, l, n);
+                cplx<double> z = gz[bj][v];
                 for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.B(b, q, n);
                 const cplx<double> YB = pv + z;
 #pragma unroll
@@ -485,12 +521,13 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
                 part[j][0] += (K.A(tp[0], n, l) * pv).im;
             }
         }
-    // reduce over the summed index: rows (lane bits 4-5), or columns (bits 0-3) for kind 1
-    const int mask = KIND == 1 ? 0x0f : 0x30;
-    const bool writer = KIND == 1 ? (lane & 15) == 0 : (lane >> 4) == 0;
+    }
+    // reduce over the summed index l: the rows, lane bits 4-5
+    const int mask = 0x30;
+    const bool writer = (lane >> 4) == 0;
 #pragma unroll
     for (int u = 0; u < NU; ++u) {
-        const int n = KIND == 1 ? r0 + mfma_row(u / 4, u % 4) : c0 + mfma_col(u);
+        const int n = c0 + mfma_col(u);
 #pragma unroll
         for (int w = 0; w < NT; ++w) {
             const double sum = wave_sum(part[u][w], mask);
