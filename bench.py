@@ -151,8 +151,87 @@ def traffic_of(stage, prec):
     return table.get(f"{stage}@{prec}", {}).get("bytes_per_launch")
 
 
+KL_NW = (8, 16, 24, 36, 48)     # the five k-list models (seeds 40..44)
+
+
+def bench_klists(args):
+    """The k-list workload: 512 k-lists (sizes log-uniform in [8, 4096], points
+    ~ N(centre, 0.02) around 16 centres, seed 4), list i on model i % 5
+    (synthetic recipe, nw = 8, 16, 24, 36, 48, R cube |R_i| <= 4), AHC with
+    external terms at 2001 levels, every point weighted 1 / (its model's
+    points).  A step evaluates all lists of all five models with run_klists."""
+    import torch
+    import wannierberri_amd as wb
+    from wannierberri_amd import _lib, models
+
+    lists = models.klist_batch()
+    systems = [models.synthetic(num_wann=nw, Rmax=RMAX, seed=40 + i) for i, nw in enumerate(KL_NW)]
+    groups = [[lists[i] for i in range(len(lists)) if i % len(KL_NW) == m] for m in range(len(KL_NW))]
+    npts = sum(len(k) for k in lists)
+    Ef = np.linspace(*EFERMI)
+    calcs = {"ahc": wb.calculators.static.AHC(Efermi=Ef)}
+    ctx = _lib.get_context(0)
+    prec = args.precisions.split(",")[0]
+
+    def step():
+        return [wb.run_klists(s, g, calcs, parameters_K={"precision": prec}) for s, g in zip(systems, groups)]
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ctx.timer = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    stats = stage_stats(ctx.timer)
+    ctx.timer = None
+    for st, (n, ms, _) in sorted(stats.items(), key=lambda x: -x[1][1]):
+        log(f"    {st:12s} calls/step={n // args.steps:5d} ms/step={ms / args.steps:9.2f} avg={ms / n:.3f} ms")
+    n, ms, units = stats["r2k_klist"]
+    flops = sum(8.0 * nk * nelem * nR for nk, nelem, nR in units)
+    ach = flops / (ms / 1e3) / 1e12
+    cb = None
+    if not args.no_cpu:
+        # oracle on (at most 200 points of) the first list of every model, one process
+        from oracle import wb_oracle as O
+        t = time.time()
+        nk_cpu, err = 0, 0.0
+        for s, g in zip(systems, groups):
+            o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                               {"Ham": s.get_R_mat("Ham"), "AA": s.get_R_mat("AA")})
+            sample = g[0][:200]
+            ref = O.AHC(Ef)(O.OracleDataK(o, k_list=sample)).data
+            nk_cpu += len(sample)
+            got = wb.run_klists(s, [sample], calcs, parameters_K={"precision": prec}).results["ahc"].data
+            err = max(err, float(np.abs(got - ref).max() / np.abs(ref).max()))
+        wall = time.time() - t
+        cb = {"value": nk_cpu / wall, "unit": "k-points/s", "cores": 1, "kind": "port",
+              "sample": f"<= 200 points of the first list of each model ({nk_cpu} points), FP64 NumPy "
+                        f"oracle, one process, "
+                        f"wall {wall:.1f} s; GPU rel err on it {err:.2e}"}
+    out = {
+        "metric": "k-pts/s AHC over 512 weighted k-lists, 5 models nw 8-48, on 1 MI355X",
+        "value": npts / dt, "unit": "k-points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": f"512 k-lists (log-uniform 8..4096 points, {npts} total) on 5 synthetic models "
+                               f"nw {KL_NW}, R cube |R_i|<={RMAX} (729), AHC at {EFERMI[2]} levels",
+                   "nk": npts, "precision": "mixed" if prec == "fp32" else "fp64"},
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
+                     "frac": ach / PEAK_FP64, "kernel": "r2k_klist", "traffic": None},
+        "stages_ms_per_step": {st: v[1] / args.steps for st, v in stats.items()},
+    }
+    if cb is not None:
+        out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["grid", "klists"], default="grid",
+                    help="grid: AHC+BCD on the 96^3 grid (default); klists: AHC over 512 k-lists")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
@@ -168,6 +247,9 @@ def main():
         sys.exit(2)
     if args.calib_copy:
         calib_copy()
+        return
+    if args.workload == "klists":
+        bench_klists(args)
         return
 
     import torch

@@ -766,6 +766,13 @@ class OracleResult:
         shape = (-1,) + (1,) * (self.data.ndim - 1)
         return OracleResult(self.Efermi, self.data * np.asarray(arr).reshape(shape))
 
+    @property
+    def max(self):
+        """Refinement metrics [max|d|, ||d||, ||d[1:]-d[:-1]||] of the unsmoothed data
+        (result/energyresult.py:250-264)."""
+        d = self.data
+        return np.array([np.abs(d).max(), np.linalg.norm(d), np.linalg.norm(d[1:] - d[:-1])])
+
 
 # ---------------------------------------------------------------------------
 # static calculators (wannierberri/calculators/static.py)
@@ -996,6 +1003,65 @@ def run(system, calculators, NK=None, NKFFT=None, NKdiv=None, blocks=None):
             res = calc(data) * factor
             total[key] = res if total[key] is None else total[key] + res
     return total
+
+
+def run_adaptive(system, calculators, NK=None, NKFFT=None, adpt_num_iter=1, adpt_mesh=2, adpt_fac=1):
+    """run() with adaptive refinement, identity point group (run_grid.py:258-265,352-389).
+
+    A K-block is [K (units of 1/NKFFT), dK, factor, results]; the metric of an
+    evaluated block is factor * concat(result.max over calculators)
+    (Kpoint.py:86-88, resultdict.py:62-63).  After each iteration the
+    `adpt_fac` blocks with the largest value of each metric are divided
+    (Kpoint.py:145-175): ndiv = adpt_mesh along periodic axes, sub-blocks
+    at K + (dK_sub - dK)/2 + dK_sub*[x,y,z], factor / prod(ndiv), the parent's
+    factor set to 0.  The running sum adds the new blocks and corrects the
+    divided ones by (new factor - old factor) * result (run_grid.py:352-363).
+    Returns (results, list of (K, dK, factor)).
+    """
+    NKdiv, NKFFT, _ = grid_blocks(NK, NKFFT, None, system.periodic)
+    blocks = []
+    fac0 = 1. / np.prod(NKdiv)
+    for x in range(NKdiv[0]):
+        for y in range(NKdiv[1]):
+            for z in range(NKdiv[2]):
+                blocks.append([np.array([x, y, z]) / NKdiv, 1. / NKdiv, fac0, None])
+    mesh = np.array([adpt_mesh] * 3 if np.isscalar(adpt_mesh) else adpt_mesh, dtype=int)
+    mesh[np.logical_not(np.array(system.periodic, dtype=bool))] = 1
+    total = None
+    factors = None
+    for i_iter in range(adpt_num_iter + 1):
+        it_sum = None
+        for b in blocks:
+            if b[3] is not None:
+                continue
+            data = OracleDataK(system, dK=b[0] / NKFFT, NKFFT=NKFFT)
+            b[3] = {k: c(data) for k, c in calculators.items()}
+            contrib = {k: v * b[2] for k, v in b[3].items()}
+            it_sum = contrib if it_sum is None else {k: it_sum[k] + contrib[k] for k in contrib}
+        if total is None:
+            total = it_sum
+        else:
+            new = np.array([b[2] for b in blocks[:len(factors)]])
+            if it_sum is not None:
+                total = {k: total[k] + it_sum[k] for k in total}
+            for i, df in enumerate(new - factors):
+                if abs(df) > 1e-8:
+                    total = {k: total[k] + blocks[i][3][k] * df for k in total}
+        factors = np.array([b[2] for b in blocks])
+        if i_iter >= adpt_num_iter:
+            break
+        kmax = np.array([np.concatenate([r.max for r in b[3].values()]) * b[2] for b in blocks]).T
+        select = set().union(*(np.argsort(km)[-adpt_fac:] for km in kmax))
+        for i in select:
+            K, dK, fac, _ = blocks[i]
+            dsub = dK / mesh
+            shift = (dsub - dK) / 2
+            for x in range(mesh[0]):
+                for y in range(mesh[1]):
+                    for z in range(mesh[2]):
+                        blocks.append([K + shift + dsub * np.array([x, y, z]), dsub, fac / np.prod(mesh), None])
+            blocks[i][2] = 0.
+    return total, [(b[0], b[1], b[2]) for b in blocks]
 
 
 def load_golden_system(path):

@@ -109,6 +109,29 @@ def test_r2k_klist(wb, ctx):
     assert rel_err(np.moveaxis(X[:, 1:4], 1, -1), refA) < 1e-12
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_r2k_klist_tiles(wb, ctx, prec):
+    """Phase GEMM over several R chunks (nR = 125), k tiles (nk = 150) and
+    element tiles (nelem = 4 * 17^2, ragged), complex128 and complex64 output."""
+    import torch
+    say(f"test_r2k_klist_tiles prec={prec}")
+    from wannierberri_amd import models
+    from wannierberri_amd.data_k import device_system
+    s = models.synthetic(num_wann=17, Rmax=2, seed=5)
+    dev = device_system(s, ctx)
+    stack = dev.stack([("Ham", 0), ("AA", 0)])
+    kl = np.random.default_rng(1).uniform(-1, 1, (150, 3))
+    dt = torch.complex128 if prec == 0 else torch.complex64
+    X = torch.empty((150, stack.shape[1], 17, 17), dtype=dt, device="cuda")
+    ctx.r2k_klist(dev.iRvec_dev, stack, torch.from_numpy(kl).cuda(), X, prec)
+    X = X.cpu().numpy()
+    refH = O.r2k_klist(s.get_R_mat("Ham"), s.rvec.iRvec, kl, hermitian=False)
+    refA = O.r2k_klist(s.get_R_mat("AA"), s.rvec.iRvec, kl, hermitian=False)
+    tol = 1e-12 if prec == 0 else 1e-6
+    assert rel_err(X[:, 0], refH) < tol
+    assert rel_err(np.moveaxis(X[:, 1:4], 1, -1), refA) < tol
+
+
 @pytest.mark.parametrize("nw", [1, 2, 5, 18, 31, 32, 64])
 def test_eigh(wb, ctx, nw):
     """Jacobi eigensolver vs np.linalg.eigh: eigenvalues, U^+HU = diag(E), U^+U = 1."""
@@ -198,6 +221,41 @@ def test_random_ahc_morb(wb, results_run):
     assert rel_err(r.results["Morb"].data, results_run["random_Morb"]) < 1e-8
 
 
+def test_refinement_random(wb):
+    """Adaptive refinement (two iterations, 2x2x2 sub-blocks) against the oracle's
+    restatement; the selected blocks' metrics are separated by >= 2 %, so both
+    sides divide the same blocks."""
+    say("test_refinement_random")
+    st = wb.calculators.static
+    Ef = np.linspace(-2, 2, 5)
+    ref, _ = O.run_adaptive(osys("random"), {"ahc": O.AHC(Ef), "Morb": O.Morb(Ef)}, NK=[6, 6, 6],
+                            NKFFT=[3, 3, 3], adpt_num_iter=2)
+    s = gsys(wb, "random")
+    grid = wb.Grid(system=s, NK=[6, 6, 6], NKFFT=[3, 3, 3])
+    r = wb.run(s, grid, {"ahc": st.AHC(Efermi=Ef), "Morb": st.Morb(Efermi=Ef)}, adpt_num_iter=2,
+               save_files=False, verbose=False)
+    assert rel_err(r.results["ahc"].data, ref["ahc"].data) < 1e-8
+    assert rel_err(r.results["Morb"].data, ref["Morb"].data) < 1e-8
+
+
+E2_H_ANGSTROM = 1.602176634e-19 ** 2 / 6.62607015e-34 / 1e-10   # e^2/(h * 1 A) in S/m
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 1e-6)])
+def test_haldane_200x200_chern(wb, precision, tol):
+    """Haldane at 200x200 with 1001 Fermi levels: the in-gap plateau is the
+    closed form e^2/(h * 1 A) (Chern number 1); in-plane components vanish."""
+    say(f"test_haldane_200x200_chern {precision}")
+    s = gsys(wb, "haldane")
+    Ef = np.linspace(-3, 3, 1001)
+    r = run_gpu(wb, s, {"ahc": wb.calculators.static.AHC(Efermi=Ef)}, [200, 200, 1], [50, 50, 1],
+                precision=precision)
+    d = r.results["ahc"].data
+    gap = np.abs(Ef) < 0.5            # the gap is (-0.579, 0.579) eV
+    assert np.abs(d[gap, 2] / E2_H_ANGSTROM - 1).max() < tol
+    assert np.abs(d[:, :2]).max() < tol * E2_H_ANGSTROM
+
+
 @pytest.mark.parametrize("fder", [0, 1, 2, 3])
 def test_fe_scan_stencils(wb, results_run, fder):
     say(f"test_fe_scan_stencils fder={fder}")
@@ -266,6 +324,37 @@ def test_synthetic_fp32(wb, small_synth):
         e = rel_err(r.results[k].data, ro[k].data)
         say(f"  fp32 {k}: rel err {e:.2e}")
         assert e < {"ahc": 1e-6, "bcd": 5e-6}[k], k
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-6)])
+def test_klist_batches(wb, precision, tol):
+    """Weighted k-lists of log-uniform sizes around a few centres, two models of
+    different nw, packed into batches of <= 128 points (one list longer than a
+    batch), against the oracle list by list: the per-point-weighted total, the
+    per-list results on batch slices and explicit per-list weights."""
+    say(f"test_klist_batches {precision}")
+    from wannierberri_amd import models
+    st = wb.calculators.static
+    lists = models.klist_batch(n_lists=10, nmin=8, nmax=160, n_centres=3, seed=41)
+    n = np.array([len(k) for k in lists])
+    assert n.max() > 128
+    Ef = np.linspace(-8, 8, 33)
+    w = np.linspace(0.5, 1.5, len(lists))
+    for nw, seed in ((8, 40), (16, 42)):
+        s = models.synthetic(num_wann=nw, Rmax=2, seed=seed)
+        o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                           {k: s.get_R_mat(k) for k in ("Ham", "AA")})
+        ref = [O.AHC(Ef)(O.OracleDataK(o, k_list=kl)) for kl in lists]
+        ref_tot = sum(r * (ni / n.sum()) for r, ni in zip(ref, n))
+        calcs = {"ahc": st.AHC(Efermi=Ef)}
+        pk = {"precision": precision}
+        tot = wb.run_klists(s, lists, calcs, parameters_K=pk, batch_points=128)
+        assert rel_err(tot.results["ahc"].data, ref_tot.data) < tol, nw
+        totw, each = wb.run_klists(s, lists, calcs, weights=w, parameters_K=pk, batch_points=128, per_list=True)
+        for i, r in enumerate(ref):
+            assert rel_err(each[i].results["ahc"].data, r.data) < tol, (nw, i)
+        ref_w = sum(r * wi for r, wi in zip(ref, w))
+        assert rel_err(totw.results["ahc"].data, ref_w.data) < tol, nw
 
 
 def test_synthetic_nw40_global_operands(wb):

@@ -46,6 +46,54 @@ def test_haldane_ahc(results_run):
     assert rel_err(r["ahc"].data, results_run["haldane_ahc"]) < 1e-8
 
 
+E2_H_ANGSTROM = 1.602176634e-19 ** 2 / 6.62607015e-34 / 1e-10   # e^2/(h * 1 A) in S/m
+
+
+def test_haldane_chern_plateau():
+    """Closed form: in the gap the Haldane layer (Chern number 1, interlayer 1 A)
+    carries sigma_xy = e^2/(h * 1 A); the grid sum converges exponentially
+    (10x10: 1.6e-3 off, the reference's own 3.86796e5; 40x40: ~1e-13)."""
+    s = gsys("haldane")
+    Ef = np.linspace(-3, 3, 1001)
+    r = O.run(s, {"ahc": O.AHC(Ef)}, NK=[40, 40, 1], NKFFT=[20, 20, 1])
+    gap = np.abs(Ef) < 0.5            # the gap is (-0.579, 0.579) eV
+    z = r["ahc"].data[gap, 2]
+    assert np.abs(z / E2_H_ANGSTROM - 1).max() < 1e-10
+    # a 2-D layer has no in-plane Berry curvature
+    assert np.abs(r["ahc"].data[:, :2]).max() < 1e-10 * E2_H_ANGSTROM
+
+
+def test_run_adaptive_invariants():
+    """Refinement bookkeeping (run_grid.py:352-389, Kpoint.py:145-175).
+
+    Every K-block is a full shifted NKFFT grid, so any weights that sum to one
+    give a valid estimate: at NKFFT 40x40 each block carries the Chern plateau
+    to ~1e-13, and a lost or doubled block (factor sum != 1, a missing
+    correction of a divided block) moves it.  The running sum must equal the
+    direct sum over the final factors, and the divided blocks must be the
+    ones with the largest factor-weighted metrics.
+    """
+    s = gsys("haldane")
+    Ef = np.linspace(-3, 3, 61)
+    calcs = {"ahc": O.AHC(Ef), "dos": O.CumDOS(Ef)}
+    tot, blocks = O.run_adaptive(s, calcs, NK=[80, 80, 1], NKFFT=[40, 40, 1], adpt_num_iter=2)
+    fac = np.array([b[2] for b in blocks])
+    ndiv = int((fac == 0).sum())
+    assert 2 <= ndiv <= 2 * 6                     # 1..6 metrics select per iteration
+    assert len(blocks) == 4 + 4 * ndiv            # 2x2 sub-blocks (z is not periodic)
+    assert abs(fac.sum() - 1) < 1e-14
+    gap = np.abs(Ef) < 0.5
+    assert np.abs(tot["ahc"].data[gap, 2] / E2_H_ANGSTROM - 1).max() < 1e-10
+    direct = None
+    for K, dK, f in blocks:
+        if f == 0:
+            continue
+        d = O.OracleDataK(s, dK=K / np.array([40, 40, 1]), NKFFT=np.array([40, 40, 1]))
+        r = calcs["ahc"](d) * f
+        direct = r if direct is None else direct + r
+    assert np.abs(direct.data - tot["ahc"].data).max() < 1e-12 * np.abs(tot["ahc"].data).max()
+
+
 def test_chiral_ahc_bcd(results_run):
     s = gsys("chiral_left")
     Ef = np.linspace(-5, 8, 27)

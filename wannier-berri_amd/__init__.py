@@ -9,9 +9,9 @@ from .calculators import static
 from .data_k import Data_K
 from .grid import Grid, KpointBZparallel
 from .result import EnergyResult, ResultDict, VoidResult
-from .run_grid import run
+from .run_grid import run, run_klists
 from .system import System_R
 
 __version__ = "0.1.0"
-__all__ = ["System_R", "Grid", "Data_K", "run", "calculators", "EnergyResult", "ResultDict", "VoidResult",
+__all__ = ["System_R", "Grid", "Data_K", "run", "run_klists", "calculators", "EnergyResult", "ResultDict", "VoidResult",
            "KpointBZparallel", "factors"]

@@ -180,7 +180,7 @@ class Context:
     def r2k_klist(self, iRvec_dev, XR, klist_dev, out, prec):
         nR = iRvec_dev.shape[0]
         nelem = XR[0].numel()
-        self.check(self._timed("r2k_klist", (klist_dev.shape[0], nelem), lib().wbmi_r2k_klist, self.h, nR,
+        self.check(self._timed("r2k_klist", (klist_dev.shape[0], nelem, nR), lib().wbmi_r2k_klist, self.h, nR,
                                _ptr(iRvec_dev), nelem, _ptr(XR), klist_dev.shape[0], _ptr(klist_dev), _ptr(out),
                                prec), "wbmi_r2k_klist")
 

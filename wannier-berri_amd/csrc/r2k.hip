@@ -5,11 +5,12 @@
 //    passes over the R bounding box.  Every pass is HBM-bound: it streams the
 //    [A][L][B] input once and writes [A][N][B]; threads run along the
 //    contiguous B index so loads and stores are fully coalesced.
-//  * wbmi_r2k_klist: phase-matrix transform for arbitrary k-points.
+//  * wbmi_r2k_klist: phase-matrix transform for arbitrary k-points (FP64 MFMA GEMM).
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
+#include "mfma_c64.h"
 #include "wbmi_common.h"
 
 // ---------------------------------------------------------------------------
@@ -337,26 +338,41 @@ int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* XR_dev, voi
 
 // ---------------------------------------------------------------------------
 // arbitrary k-list: out[k][e] = sum_R exp(2 pi i k.R) X[R][e]
+//
+// A complex GEMM (M = nk, N = nelem, K = nR) on the FP64 matrix cores.  A
+// workgroup owns 64 k x 128 elements (8 waves of 32 x 32, mfma_c64.h); the
+// phase factors of its 64 k-points are formed in LDS, KLM_RC R-vectors at a
+// time, and the X operand is read straight from HBM (each 16-element row
+// segment is 256 contiguous bytes).  blockIdx.x runs over k-tiles, so the
+// workgroups resident on one XCD at a time share their X columns
+// (nR x 128 x 16 B) through that XCD's L2.
 // ---------------------------------------------------------------------------
 
-#define KL_TK 8
-#define KL_TR 128
+#define KLM_THREADS 512
+#define KLM_TK 64
+#define KLM_TE 128
+#define KLM_RC 32
 
 template <typename Tout>
-__global__ void __launch_bounds__(256) k_r2k_klist(int nR, const int32_t* __restrict__ iR, int64_t nelem,
-                                                   const cplx<double>* __restrict__ X, int nk,
-                                                   const double* __restrict__ kl, cplx<Tout>* __restrict__ out) {
-    __shared__ cplx<double> ph[KL_TK][KL_TR];
-    int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int k0 = blockIdx.y * KL_TK;
-    cplx<double> acc[KL_TK];
-#pragma unroll
-    for (int q = 0; q < KL_TK; ++q) acc[q] = mk(0.0, 0.0);
-    for (int r0 = 0; r0 < nR; r0 += KL_TR) {
+__global__ void __launch_bounds__(KLM_THREADS) k_r2k_klist(int nR, const int32_t* __restrict__ iR, int64_t nelem,
+                                                           const cplx<double>* __restrict__ X, int nk,
+                                                           const double* __restrict__ kl,
+                                                           cplx<Tout>* __restrict__ out) {
+    __shared__ cplx<double> ph[KLM_TK][KLM_RC + 1];
+    const int wave = threadIdx.x >> 6;
+    const int k0 = blockIdx.x * KLM_TK;
+    const int64_t e0 = (int64_t)blockIdx.y * KLM_TE;
+    const int kw = k0 + 32 * (wave >> 2);          // first k of this wave's tile
+    const int64_t ew = e0 + 32 * (wave & 3);       // first element of this wave's tile
+    const int nrow = nk - kw < 32 ? nk - kw : 32;
+    const int ncol = nelem - ew < 32 ? (int)(nelem - ew) : 32;
+    MfmaBlock C;
+    C.zero();
+    for (int r0 = 0; r0 < nR; r0 += KLM_RC) {
         __syncthreads();
-        for (int t = threadIdx.x; t < KL_TK * KL_TR; t += blockDim.x) {
-            int q = t / KL_TR, rr = t % KL_TR;
-            int k = k0 + q, r = r0 + rr;
+        for (int t = threadIdx.x; t < KLM_TK * KLM_RC; t += KLM_THREADS) {
+            const int q = t / KLM_RC, rr = t % KLM_RC;
+            const int k = k0 + q, r = r0 + rr;
             cplx<double> v = mk(0.0, 0.0);
             if (k < nk && r < nR) {
                 double f = kl[3 * k] * iR[3 * r] + kl[3 * k + 1] * iR[3 * r + 1] + kl[3 * k + 2] * iR[3 * r + 2];
@@ -368,20 +384,28 @@ __global__ void __launch_bounds__(256) k_r2k_klist(int nR, const int32_t* __rest
             ph[q][rr] = v;
         }
         __syncthreads();
-        if (e < nelem) {
-            int rmax = nR - r0 < KL_TR ? nR - r0 : KL_TR;
-            for (int rr = 0; rr < rmax; ++rr) {
-                cplx<double> x = X[(int64_t)(r0 + rr) * nelem + e];
-#pragma unroll
-                for (int q = 0; q < KL_TK; ++q) acc[q] += ph[q][rr] * x;
-            }
+        if (nrow > 0 && ncol > 0) {
+            const int rc = nR - r0 < KLM_RC ? nR - r0 : KLM_RC;
+            const int prow = 32 * (wave >> 2);
+            const cplx<double>* Xr = X + (int64_t)r0 * nelem + ew;
+            mfma_block(
+                C, nrow, ncol, rc, 0, 0, [&](int r, int k) { return ph[prow + r][k]; },
+                [&](int k, int c) { return Xr[(int64_t)k * nelem + c]; });
         }
     }
-    if (e < nelem) {
+    if (nrow <= 0 || ncol <= 0) return;
 #pragma unroll
-        for (int q = 0; q < KL_TK; ++q)
-            if (k0 + q < nk) out[(int64_t)(k0 + q) * nelem + e] = mk((Tout)acc[q].re, (Tout)acc[q].im);
-    }
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = mfma_row(bi, v), c = mfma_col(bj);
+                if (r < nrow && c < ncol) {
+                    const cplx<double> z = C.get(bi, bj, v);
+                    out[(int64_t)(kw + r) * nelem + ew + c] = mk((Tout)z.re, (Tout)z.im);
+                }
+            }
 }
 
 extern "C" int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nelem, const void* XR_dev,
@@ -389,13 +413,15 @@ extern "C" int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, i
     if (!ctx) return WBMI_ERR_ARG;
     if (nR <= 0 || nelem <= 0 || nk <= 0 || !iRvec_dev || !XR_dev || !klist_dev || !out_dev)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: bad arguments");
-    if (ceil_div(nk, KL_TK) > 65535) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: too many k-points per call");
-    dim3 g((unsigned)ceil_div(nelem, 256), (unsigned)ceil_div(nk, KL_TK));
+    if (ceil_div(nelem, KLM_TE) > 65535)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: more than %d elements per k-point",
+                         65535 * KLM_TE);
+    dim3 g((unsigned)ceil_div(nk, KLM_TK), (unsigned)ceil_div(nelem, KLM_TE));
     if (prec == WBMI_F64)
-        hipLaunchKernelGGL(k_r2k_klist<double>, g, dim3(256), 0, ctx->stream, nR, iRvec_dev, nelem,
+        hipLaunchKernelGGL(k_r2k_klist<double>, g, dim3(KLM_THREADS), 0, ctx->stream, nR, iRvec_dev, nelem,
                            (const cplx<double>*)XR_dev, nk, klist_dev, (cplx<double>*)out_dev);
     else if (prec == WBMI_F32)
-        hipLaunchKernelGGL(k_r2k_klist<float>, g, dim3(256), 0, ctx->stream, nR, iRvec_dev, nelem,
+        hipLaunchKernelGGL(k_r2k_klist<float>, g, dim3(KLM_THREADS), 0, ctx->stream, nR, iRvec_dev, nelem,
                            (const cplx<double>*)XR_dev, nk, klist_dev, (cplx<float>*)out_dev);
     else
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: bad precision");

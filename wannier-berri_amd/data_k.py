@@ -289,6 +289,28 @@ class Data_K:
             params.ch[i] = v
         self.ctx.static_scan(self.E_dev, self.Xk, self.prec, params, weight, hist)
 
+    def subset(self, k0, k1):
+        """The k-points [k0, k1) of an evaluated k-list block as a Data_K of their own.
+
+        Shares (slices) the device arrays of this block, so calculators on the
+        subset cost one scan launch each; channels that this block did not
+        evaluate are computed for the subset's points alone.
+        """
+        if self.k_list is None:
+            raise ValueError("subset() needs a k-list block")
+        if not 0 <= k0 < k1 <= self.nk:
+            raise ValueError(f"subset [{k0}, {k1}) out of range for {self.nk} k-points")
+        self.ensure(set())
+        sub = object.__new__(type(self))
+        sub.__dict__.update(self.__dict__)
+        sub.k_list = self.k_list[k0:k1]
+        sub.nk = k1 - k0
+        sub._requested = set(self._requested)
+        sub.E_dev = self.E_dev[k0:k1]
+        sub.U_dev = self.U_dev[k0:k1]
+        sub.Xk = None if self.Xk is None else self.Xk[k0:k1]
+        return sub
+
     # -- reference-style accessors (synchronising; for tests and users) ---------
     @property
     def E_K(self):

@@ -120,6 +120,16 @@ def synthetic(num_wann=32, Rmax=4, seed=2, lattice=None, decay=3.0, h_std=1.0, o
     return System_R(lattice, iRvec, wcc_red.dot(lattice), mats, name=f"synthetic_nw{nw}_seed{seed}")
 
 
+def klist_batch(n_lists=512, nmin=8, nmax=4096, n_centres=16, sigma=0.02, seed=4):
+    """Refinement-like k-lists: sizes log-uniform in [nmin, nmax], points
+    ~ N(centre, sigma) (reduced coordinates, wrapped into [0, 1)) around
+    `n_centres` centres uniform in the cell; list i uses centre i % n_centres."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    centres = rng.uniform(0.0, 1.0, (n_centres, 3))
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax + 1), n_lists)).astype(int).clip(nmin, nmax)
+    return [(centres[i % n_centres] + sigma * rng.standard_normal((n, 3))) % 1.0 for i, n in enumerate(sizes)]
+
+
 def spinor(num_wann_half=32, Rmax=5, seed=3, soc_std=0.05, decay=3.0, a_std=0.1):
     """2 x num_wann_half spin-block model: both spin blocks carry the same random
     Hamiltonian (synthetic recipe) and a random SOC-like block couples them

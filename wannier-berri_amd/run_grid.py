@@ -94,3 +94,63 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
     if verbose:
         print(f"run(): processed {counter} K-blocks in {time.time() - t0:.3f} s", flush=True)
     return result_all
+
+
+def run_klists(system, k_lists, calculators, weights=None, parameters_K=None, batch_points=1 << 15,
+               per_list=False):
+    """Static calculators over many k-lists, batched into few large launches.
+
+    Each list i (reduced coordinates [n_i, 3]) is one "K-point" in the sense
+    of run(): a calculator sees it as a Data_K with k_list = list i (so
+    results carry the calculator's 1/n_i), and the total is
+    sum_i weights[i] * result_i, as run() sums result * Kp.factor
+    (run_grid.py:97-99).  `weights` defaults to n_i / sum_j n_j, i.e. every
+    point weighted 1 / (total number of points).
+
+    Consecutive lists are packed into batches of up to `batch_points` points
+    (a longer list is a batch of its own): R -> k, eigh and rotation run once
+    per batch.  With default weights and per_list=False the calculators run
+    once per batch (weight n_batch / n_total); otherwise once per list on a
+    slice of the batch.  Returns the total ResultDict, and with per_list=True
+    also the list of per-list ResultDicts.
+    """
+    parameters_K = dict(parameters_K or {})
+    need = set()
+    for calc in calculators.values():
+        if hasattr(calc, "needed_channels"):
+            need |= set(calc.needed_channels(system))
+    parameters_K.setdefault("channels", need)
+    k_lists = [np.asarray(k, dtype=float).reshape(-1, 3) for k in k_lists]
+    sizes = np.array([len(k) for k in k_lists], dtype=np.int64)
+    if len(k_lists) == 0 or sizes.min() <= 0:
+        raise ValueError("run_klists needs at least one list and no empty list")
+    ntot = int(sizes.sum())
+    uniform = weights is None
+    weights = sizes / ntot if uniform else np.asarray(weights, dtype=float).reshape(-1)
+    if weights.shape[0] != len(k_lists):
+        raise ValueError(f"{weights.shape[0]} weights for {len(k_lists)} k-lists")
+    batches, cur, npts = [], [], 0
+    for i, n in enumerate(sizes):
+        if cur and npts + n > batch_points:
+            batches.append(cur)
+            cur, npts = [], 0
+        cur.append(i)
+        npts += int(n)
+    batches.append(cur)
+    total, each = None, [None] * len(k_lists)
+    for b in batches:
+        data = Data_K(system, k_list=np.concatenate([k_lists[i] for i in b]), **parameters_K)
+        if uniform and not per_list:
+            res = ResultDict({k: c(data) for k, c in calculators.items()}) * (data.nk / ntot)
+            total = res if total is None else total + res
+        else:
+            off = 0
+            for i in b:
+                sub = data.subset(off, off + int(sizes[i]))
+                off += int(sizes[i])
+                res = ResultDict({k: c(sub) for k, c in calculators.items()})
+                each[i] = res
+                contrib = res * float(weights[i])
+                total = contrib if total is None else total + contrib
+        del data
+    return (total, each) if per_list else total
