@@ -133,7 +133,7 @@ int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nele
  *   H_dev : [nk, Hstride] complex (prec), matrix k at H_dev + k*Hstride
  *   E_dev : float64 [nk, nw] ascending;  U_dev : [nk, nw, nw] complex (prec),
  *   column j = eigenvector of E[j].  nw <= 128 (nw <= 64 without refinement).
- *   refine = n in 1..8: FP32 Jacobi of the FP64 Hamiltonian (H_dev for
+ *   refine = n in 1..16: FP32 Jacobi of the FP64 Hamiltonian (H_dev for
  *   prec = WBMI_F64; H64_dev, complex128 with the same stride, for
  *   prec = WBMI_F32), then FP64 Ogita-Aishima steps (Japan J. Indust. Appl.
  *   Math. 35 (2018), Alg. 1) until the correction has ||E||_F below 1e-8
@@ -141,7 +141,7 @@ int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nele
  *   complex128 vectors get a final Newton-Schulz orthonormalisation.
  *   E are the FP64 Rayleigh quotients of the refined vectors.  Separated
  *   spectra take 2-3 steps; a pair closer than the first step's threshold
- *   (~1e-4 |H| at nw = 32) up to 8.  For nw > 64 the FP32 Jacobi runs one
+ *   (~1e-4 |H| at nw = 32) about ten.  For nw > 64 the FP32 Jacobi runs one
  *   workgroup per chunk of eigenvector rows.
  *   refine = 0: plain Jacobi in the arithmetic of `prec` (nw <= 64).
  *   work_dev / work_bytes: device workspace of at least

@@ -194,6 +194,155 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi(int nw, const cplx<Tin>* __r
 }
 
 // ---------------------------------------------------------------------------
+// FP32 Jacobi for padded sizes n = NT in {16, 32, 64}: the rotations of
+// k_eigh_jacobi, in the same order, with the size known at compile time.  A
+// round's parameters are stored once per pair as {c, s cos phi, s sin phi,
+// p | q << 16}, so an update item is one 16-byte parameter read plus its
+// element loads and stores (four of each in the column phase, two in the row
+// phase), with no per-item index arithmetic beyond two adds.  Lane t owns row
+// (column phase) or column (row phase) t % NT of the pairs t / NT,
+// t / NT + 64 / NT, ...
+// ---------------------------------------------------------------------------
+template <int NT>
+__host__ __device__ constexpr size_t jacobi_c_lds() {
+    return 2 * sizeof(cplx<float>) * NT * (NT + 1) + sizeof(float4) * (NT / 2) + 16;
+}
+
+template <int NT, typename Tin, typename TO>
+__global__ void __launch_bounds__(64) k_eigh_jacobi_c(int nw, const cplx<Tin>* __restrict__ H, int64_t Hstride,
+                                                      double* __restrict__ Eout, cplx<TO>* __restrict__ Uout,
+                                                      int max_sweeps) {
+    typedef float T;
+    constexpr int n = NT, ld = NT + 1, np = NT / 2;
+    constexpr int LPR = 64 / NT;          // pair stride of a lane
+    constexpr int IPL = np / LPR;         // update items per lane and phase
+    constexpr int B = IPL < 8 ? IPL : 8;  // items per load batch
+    static_assert(NT % 2 == 0 && 64 % NT == 0 && IPL % B == 0, "NT must be 16, 32 or 64");
+    extern __shared__ char smem[];
+    cplx<T>* A = (cplx<T>*)smem;
+    cplx<T>* V = A + n * ld;
+    float4* prm = (float4*)(V + n * ld);  // 16-byte aligned: 16 n (n + 1) bytes precede it
+    int* flag = (int*)(prm + np);
+    const int tid = threadIdx.x;
+    const int64_t k = blockIdx.x;
+    const cplx<Tin>* Hk = H + k * Hstride;
+
+    for (int i = tid; i < n; i += 64)
+        for (int j = 0; j < n; ++j) {
+            cplx<T> a = mk((T)0, (T)0);
+            if (i < nw && j < nw) {
+                cplx<Tin> x = Hk[i * nw + j], y = Hk[j * nw + i];
+                a = mk((T)(0.5 * (x.re + y.re)), (T)(0.5 * (x.im - y.im)));
+            }
+            A[i * ld + j] = a;
+            V[i * ld + j] = mk((T)(i == j ? 1 : 0), (T)0);
+        }
+    __syncthreads();
+
+    const int lr = tid % NT;
+    const int pr0 = tid / NT;
+    const T eps = Eps<T>::v;
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        if (tid == 0) *flag = 0;
+        __syncthreads();
+        for (int r = 0; r < n - 1; ++r) {
+            bool rot = false;
+            if (tid < np) {
+                int p, q;
+                rr_pair(n, r, tid, p, q);
+                cplx<T> apq = A[p * ld + q];
+                T g = sqrt(apq.re * apq.re + apq.im * apq.im);
+                T app = A[p * ld + p].re, aqq = A[q * ld + q].re;
+                T c = 1, sc = 0, ss = 0;
+                T scale = fabs(app) > fabs(aqq) ? fabs(app) : fabs(aqq);
+                if (g > (T)0.5 * eps * scale && g > (T)1e-36) {
+                    T tau = (aqq - app) / (2 * g);
+                    T t = (tau >= 0 ? (T)1 : (T)-1) / (fabs(tau) + sqrt((T)1 + tau * tau));
+                    c = (T)1 / sqrt((T)1 + t * t);
+                    T s = t * c;
+                    sc = s * apq.re / g;
+                    ss = s * apq.im / g;
+                    rot = true;
+                    *flag = 1;
+                }
+                prm[tid] = make_float4(c, sc, ss, __int_as_float(p | (q << 16)));
+            }
+            // the workgroup is one wavefront: __any is uniform over it
+            if (!__any(rot)) continue;
+            __syncthreads();
+            // columns of A and V:  col_p <- c col_p - s e^{-i phi} col_q ; col_q <- s e^{i phi} col_p + c col_q
+#pragma unroll
+            for (int b0 = 0; b0 < IPL; b0 += B) {
+                float4 P[B];
+                int op[B], oq[B];
+                cplx<T> ap[B], aq[B], vp[B], vq[B];
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    P[u] = prm[pr0 + LPR * (b0 + u)];
+                    const int pq = __float_as_int(P[u].w);
+                    op[u] = lr * ld + (pq & 0xffff);
+                    oq[u] = lr * ld + (pq >> 16);
+                    ap[u] = A[op[u]];
+                    aq[u] = A[oq[u]];
+                    vp[u] = V[op[u]];
+                    vq[u] = V[oq[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    const T c = P[u].x;
+                    const cplx<T> se = mk(P[u].y, P[u].z);
+                    A[op[u]] = c * ap[u] - mulc(aq[u], se);
+                    A[oq[u]] = se * ap[u] + c * aq[u];
+                    V[op[u]] = c * vp[u] - mulc(vq[u], se);
+                    V[oq[u]] = se * vp[u] + c * vq[u];
+                }
+            }
+            __syncthreads();
+            // rows of A:  row_p <- c row_p - s e^{i phi} row_q ; row_q <- s e^{-i phi} row_p + c row_q
+#pragma unroll
+            for (int b0 = 0; b0 < IPL; b0 += B) {
+                float4 P[B];
+                int op[B], oq[B];
+                cplx<T> ap[B], aq[B];
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    P[u] = prm[pr0 + LPR * (b0 + u)];
+                    const int pq = __float_as_int(P[u].w);
+                    op[u] = (pq & 0xffff) * ld + lr;
+                    oq[u] = (pq >> 16) * ld + lr;
+                    ap[u] = A[op[u]];
+                    aq[u] = A[oq[u]];
+                }
+#pragma unroll
+                for (int u = 0; u < B; ++u) {
+                    const T c = P[u].x;
+                    const cplx<T> se = mk(P[u].y, P[u].z);
+                    A[op[u]] = c * ap[u] - se * aq[u];
+                    A[oq[u]] = mulc(ap[u], se) + c * aq[u];
+                }
+            }
+            __syncthreads();
+        }
+        if (*flag == 0) break;
+        __syncthreads();
+    }
+    // ascending order: rank of each eigenvalue (ties broken by index)
+    for (int i = tid; i < nw; i += 64) {
+        T ei = A[i * ld + i].re;
+        int rank = 0;
+        for (int j = 0; j < nw; ++j) {
+            T ej = A[j * ld + j].re;
+            rank += (ej < ei) || (ej == ei && j < i);
+        }
+        Eout[k * nw + rank] = (double)ei;
+        for (int m = 0; m < nw; ++m) {
+            const cplx<T> v = V[m * ld + i];
+            Uout[(k * nw + m) * nw + rank] = mk((TO)v.re, (TO)v.im);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // 64 < n <= 128: FP32 Jacobi with A in LDS (132 KB at n = 128) and the
 // eigenvectors in row chunks.  The rotations act on V from the right, so the
 // rows of V are independent; every workgroup of the grid row blockIdx.y runs
@@ -425,6 +574,101 @@ __device__ inline void oa_load_a(int nw, int ld, const cplx<double>* __restrict_
     }
 }
 
+// Scratch of cluster_jacobi in LDS: per pair {c, s cos phi, s sin phi} and
+// (p, q), plus a round flag pair.
+__host__ __device__ inline size_t cj_lds(int n) {
+    const int np = (n + 1) / 2;
+    return sizeof(double) * 3 * np + sizeof(int) * (2 * np + 4);
+}
+
+// Finishes the refinement where it stalls: a pair closer than the
+// Ogita-Aishima threshold delta is only re-orthogonalised by the correction
+// step, so its two vectors can stay any unitary mix of the pair's
+// eigenvectors.  With S = X^+ A X of the refined X (Hermitian, diagonal to
+// the refinement's accuracy for separated eigenvalues), cyclic Jacobi sweeps
+// (round-robin order, rotations of Golub & Van Loan, sec. 8.5.2, as in
+// k_eigh_jacobi) rotate only the pairs with
+//   |S_pq| > floor_abs  and  |S_pq| > theta |S_qq - S_pp|,
+// i.e. whose mixing angle exceeds theta: none after a converged refinement of
+// a separated spectrum, the unresolved pairs (and larger clusters) otherwise.
+// The rotations update S (columns and rows) and the columns of X.
+// Uniform control flow: the round flag is set by the rotating threads and
+// read by every thread after a barrier (double-buffered by round parity).
+template <int NTH, class FS, class FX>
+__device__ void cluster_jacobi(int n, FS S, FX X, char* scratch, double theta, double floor_abs, int max_sweeps) {
+    const int ne = n + (n & 1), np = ne / 2;
+    double* prm = (double*)scratch;
+    int* pq = (int*)(prm + 3 * np);
+    int* flags = pq + 2 * np;  // [0] sweep, [1 + (r & 1)] round
+    const int tid = threadIdx.x;
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        if (tid == 0) flags[0] = flags[1] = flags[2] = 0;
+        __syncthreads();
+        for (int r = 0; r < ne - 1; ++r) {
+            // the next round's flag: its last readers (round r - 1) passed a barrier since, or read 0
+            if (tid == 0) flags[1 + ((r + 1) & 1)] = 0;
+            if (tid < np) {
+                int p, q;
+                rr_pair(ne, r, tid, p, q);
+                double c = 1.0, sc = 0.0, ss = 0.0;
+                if (p < n && q < n) {
+                    const cplx<double> apq = S(p, q);
+                    const double g = sqrt(apq.re * apq.re + apq.im * apq.im);
+                    const double app = S(p, p).re, aqq = S(q, q).re;
+                    if (g > floor_abs && g > theta * fabs(aqq - app)) {
+                        const double tau = (aqq - app) / (2.0 * g);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                        c = 1.0 / sqrt(1.0 + t * t);
+                        const double sn = t * c;
+                        sc = sn * apq.re / g;
+                        ss = sn * apq.im / g;
+                        flags[1 + (r & 1)] = 1;
+                        flags[0] = 1;
+                    }
+                }
+                prm[3 * tid] = c;
+                prm[3 * tid + 1] = sc;
+                prm[3 * tid + 2] = ss;
+                pq[2 * tid] = p;
+                pq[2 * tid + 1] = q;
+            }
+            __syncthreads();
+            if (flags[1 + (r & 1)] == 0) continue;
+            // columns of S and X: col_p <- c col_p - s e^{-i phi} col_q ; col_q <- s e^{i phi} col_p + c col_q
+            for (int t = tid; t < np * n; t += NTH) {
+                const int pr = t / n, i = t % n;
+                const int p = pq[2 * pr], q = pq[2 * pr + 1];
+                if (p >= n || q >= n) continue;
+                const double c = prm[3 * pr];
+                const cplx<double> se = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
+                if (c == 1.0 && se.re == 0.0 && se.im == 0.0) continue;
+                const cplx<double> ap = S(i, p), aq = S(i, q), xp = X(i, p), xq = X(i, q);
+                S(i, p) = c * ap - mulc(aq, se);
+                S(i, q) = se * ap + c * aq;
+                X(i, p) = c * xp - mulc(xq, se);
+                X(i, q) = se * xp + c * xq;
+            }
+            __syncthreads();
+            // rows of S: row_p <- c row_p - s e^{i phi} row_q ; row_q <- s e^{-i phi} row_p + c row_q
+            for (int t = tid; t < np * n; t += NTH) {
+                const int pr = t / n, j = t % n;
+                const int p = pq[2 * pr], q = pq[2 * pr + 1];
+                if (p >= n || q >= n) continue;
+                const double c = prm[3 * pr];
+                const cplx<double> se = mk(prm[3 * pr + 1], prm[3 * pr + 2]);
+                if (c == 1.0 && se.re == 0.0 && se.im == 0.0) continue;
+                const cplx<double> ap = S(p, j), aq = S(q, j);
+                S(p, j) = c * ap - se * aq;
+                S(q, j) = mulc(ap, se) + c * aq;
+            }
+            __syncthreads();
+        }
+        const bool more = flags[0] != 0;
+        __syncthreads();
+        if (!more) break;
+    }
+}
+
 // Refinement steps until the correction satisfies ||E||_F < tol (at most
 // `iters`), then eigenvalues as Rayleigh quotients of the final vectors.
 // Well-separated spectra stop after two steps (||E|| ~ eps32, then ~eps32^2).
@@ -444,7 +688,8 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
     cplx<double>* sS = sT + nw * ld;          // S, later E
     double* lam = (double*)(sS + nw * ld);
     double* red = lam + nw;                   // [0] ||S-D||^2, [1] ||R||^2, [2] max|lam|, [3] ||E||^2
-    int* rank = (int*)(red + 4);
+    char* cjs = (char*)(red + 4);             // cluster_jacobi scratch
+    int* rank = (int*)(cjs + cj_lds(nw));
     const int tid = threadIdx.x;
     const int64_t k = blockIdx.x;
     const cplx<double>* Hk = H64 + k * Hstride;
@@ -540,20 +785,28 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         sX = sT;
         sT = t;
     }
-    // eigenvalues: Rayleigh quotients x_c^+ A x_c / x_c^+ x_c of the refined columns
+    // S = X^+ A X of the refined columns; pairs left mixed are rotated apart
+    const double scale = red[2];  // max |lam| of the last step
     oa_load_a(nw, ld, Hk, sA);
     __syncthreads();
     oa_gemm<TS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
                 [&](int r, int c, cplx<double> v) { sT[r * ld + c] = v; });
     __syncthreads();
+    oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
+                [&](int m, int c) { return sT[m * ld + c]; },
+                [&](int r, int c, cplx<double> v) { sS[r * ld + c] = v; });
+    __syncthreads();
+    cluster_jacobi<OA_THREADS>(
+        nw, [&](int i, int j) -> cplx<double>& { return sS[i * ld + j]; },
+        [&](int i, int j) -> cplx<double>& { return sX[i * ld + j]; }, cjs, tol, 64.0 * DBL_EPSILON * scale, 10);
+    // eigenvalues: Rayleigh quotients x_c^+ A x_c / x_c^+ x_c = S_cc / x_c^+ x_c
     for (int c = tid; c < nw; c += OA_THREADS) {
-        double num = 0.0, den = 0.0;
+        double den = 0.0;
         for (int m = 0; m < nw; ++m) {
-            const cplx<double> x = sX[m * ld + c], y = sT[m * ld + c];
-            num += x.re * y.re + x.im * y.im;
+            const cplx<double> x = sX[m * ld + c];
             den += x.re * x.re + x.im * x.im;
         }
-        lam[c] = num / den;
+        lam[c] = sS[c * ld + c].re / den;
     }
     __syncthreads();
     // ascending order (ties by index)
@@ -615,6 +868,8 @@ __global__ void __launch_bounds__(OAG_THREADS) k_eigh_refine_g(int n, const cplx
     __shared__ double lam[OAG_NWMAX];
     __shared__ double red[4];  // [0] ||S-D||^2, [1] ||R||^2, [2] max|lam|, [3] ||E||^2
     __shared__ int rank[OAG_NWMAX];
+    __shared__ __attribute__((aligned(16))) char cjs[sizeof(double) * 3 * (OAG_NWMAX / 2) +
+                                                     sizeof(int) * (OAG_NWMAX + 4)];
     const int tid = threadIdx.x, n2 = n * n;
     const int64_t k = blockIdx.x;
     const cplx<double>* Hk = H64 + k * Hstride;
@@ -697,17 +952,24 @@ __global__ void __launch_bounds__(OAG_THREADS) k_eigh_refine_g(int n, const cplx
         X = T;
         T = t;
     }
-    // Rayleigh quotients of the refined columns
+    // S = X^+ A X of the refined columns; pairs left mixed are rotated apart
+    const double scale = red[2];  // max |lam| of the last step
     wg_gemm(n, Aop, [&](int m, int c) { return X[m * n + c]; }, [&](int r, int c, cplx<double> v) { T[r * n + c] = v; });
     __syncthreads();
+    wg_gemm(n, [&](int r, int m) { return conj(X[m * n + r]); }, [&](int m, int c) { return T[m * n + c]; },
+            [&](int r, int c, cplx<double> v) { S[r * n + c] = v; });
+    __syncthreads();
+    cluster_jacobi<OAG_THREADS>(
+        n, [&](int i, int j) -> cplx<double>& { return S[i * n + j]; },
+        [&](int i, int j) -> cplx<double>& { return X[i * n + j]; }, cjs, tol, 64.0 * DBL_EPSILON * scale, 10);
+    // Rayleigh quotients x_c^+ A x_c / x_c^+ x_c = S_cc / x_c^+ x_c
     for (int c = tid; c < n; c += OAG_THREADS) {
-        double num = 0.0, den = 0.0;
+        double den = 0.0;
         for (int m = 0; m < n; ++m) {
-            const cplx<double> x = X[m * n + c], y = T[m * n + c];
-            num += x.re * y.re + x.im * y.im;
+            const cplx<double> x = X[m * n + c];
             den += x.re * x.re + x.im * x.im;
         }
-        lam[c] = num / den;
+        lam[c] = S[c * n + c].re / den;
     }
     __syncthreads();
     for (int i = tid; i < n; i += OAG_THREADS) {
@@ -740,7 +1002,22 @@ static int jacobi_f32_of_f64(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* 
                              cplx<TU>* U) {
     const int n = nw + (nw & 1);
     const int sweeps = 40;
-    if (n <= 64) {
+    if (n == 16 || n == 32 || n == 64) {
+        const void* fn = n == 16 ? (const void*)k_eigh_jacobi_c<16, double, TU>
+                       : n == 32 ? (const void*)k_eigh_jacobi_c<32, double, TU>
+                                 : (const void*)k_eigh_jacobi_c<64, double, TU>;
+        const size_t lds = n == 16 ? jacobi_c_lds<16>() : n == 32 ? jacobi_c_lds<32>() : jacobi_c_lds<64>();
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (n == 16)
+            hipLaunchKernelGGL((k_eigh_jacobi_c<16, double, TU>), dim3(nk), dim3(64), lds, ctx->stream, nw, H64,
+                               Hstride, E, U, sweeps);
+        else if (n == 32)
+            hipLaunchKernelGGL((k_eigh_jacobi_c<32, double, TU>), dim3(nk), dim3(64), lds, ctx->stream, nw, H64,
+                               Hstride, E, U, sweeps);
+        else
+            hipLaunchKernelGGL((k_eigh_jacobi_c<64, double, TU>), dim3(nk), dim3(64), lds, ctx->stream, nw, H64,
+                               Hstride, E, U, sweeps);
+    } else if (n <= 64) {
         const size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double, TU>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -777,7 +1054,8 @@ static int eigh_refined(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* H64, 
         WBMI_LAUNCH_CHECK(ctx);
         return WBMI_OK;
     }
-    const size_t lds2 = 4 * sizeof(cplx<double>) * nw * (nw + 1) + sizeof(double) * (nw + 4) + sizeof(int) * nw + 16;
+    const size_t lds2 =
+        4 * sizeof(cplx<double>) * nw * (nw + 1) + sizeof(double) * (nw + 4) + cj_lds(nw) + sizeof(int) * nw + 16;
     if (nw <= 32) {
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<TU, 2>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
@@ -798,7 +1076,7 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
     if (!ctx) return WBMI_ERR_ARG;
     if (nk <= 0 || nw <= 0 || nw > 128 || !E_dev || !U_dev || Hstride < (int64_t)nw * nw)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad arguments (nw <= 128, Hstride >= nw*nw)");
-    if (refine < 0 || refine > 8) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refine must be 0..8");
+    if (refine < 0 || refine > 16) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: refine must be 0..16");
     if (prec != WBMI_F64 && prec != WBMI_F32) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh: bad precision");
     if (refine) {
         // the FP64 Hamiltonian: H_dev (complex128) for prec = WBMI_F64, H64_dev for prec = WBMI_F32

@@ -30,10 +30,11 @@ CHANNEL_ORDER = list(CHANNEL_NCOMP)
 # matrices forced Hermitian after R->k (data_K/data_K_R.py:87)
 HERMITIAN = {"AA", "rotAA"}
 BASES = ["Ham", "AA", "BB", "CC"]
-# At most this many FP64 Ogita-Aishima steps after the FP32 Jacobi of the
-# mixed form (the kernel stops at ||E||_F < 1e-8: two steps for separated
-# spectra, up to eight for pairs closer than ~1e-5 |H|)
-OA_STEPS = 8
+# At most this many FP64 Ogita-Aishima steps after the FP32 Jacobi (the
+# kernel stops at ||E||_F < 1e-8 (complex64 vectors) or 1e-10 (complex128):
+# two or three steps for separated spectra, up to about ten for a pair
+# closer than ~1e-5 |H|, whose first-order correction is large)
+OA_STEPS = 16
 _ALPHA = [1, 2, 0]
 _BETA = [2, 0, 1]
 
