@@ -220,7 +220,7 @@ def bench_klists(args):
                                f"nw {KL_NW}, R cube |R_i|<={RMAX} (729), AHC at {EFERMI[2]} levels",
                    "nk": npts, "precision": "mixed" if prec == "fp32" else "fp64"},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
-                     "frac": ach / PEAK_FP64, "kernel": "r2k_klist", "traffic": None},
+                     "frac": ach / PEAK_FP64, "kernel": "r2k_klist", "traffic": traffic_of("r2k_klist", prec)},
         "stages_ms_per_step": {st: v[1] / args.steps for st, v in stats.items()},
     }
     if cb is not None:
