@@ -133,9 +133,12 @@ __device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int 
     return s;
 }
 
-// Element access to the channels of one k-point (LDS copies of V and A when staged).
-// LM: dEinv o Off read from LDS (nw <= SC_M_NW) or formed from E and the labels
-template <typename T, bool LM>
+// Element access to the channels of one k-point.
+// LM: dEinv o Off read from LDS (nw <= SC_M_NW) or formed from E and the labels.
+// ST: V and A read from their LDS copies (formulas with wave tasks, nw <= SC_LDS_NW;
+// A only with external terms, the only case that reads it) or from the channels.
+// Both are compile-time so that every access is a plain LDS or global load.
+template <typename T, bool LM, bool ST>
 struct KCtx {
     const cplx<T>* X;  // channels of this k: [nch][nw][nw]
     int nw, ld;
@@ -145,18 +148,23 @@ struct KCtx {
     const int* lab;
     const int* gs;
     const int* ge;
-    const cplx<double>* sV;  // LDS copies of V and A, used when stV / stA
+    const cplx<double>* sV;  // LDS copies of V and A (ST)
     const cplx<double>* sA;
-    bool stV, stA;
     __device__ inline cplx<double> get(int role, int x, int y) const {
         cplx<T> v = X[((int64_t)ch[role] * nw + x) * nw + y];
         return mk((double)v.re, (double)v.im);
     }
     __device__ inline cplx<double> V(int a, int x, int y) const {
-        return stV ? sV[(a * nw + x) * ld + y] : get(WBMI_ROLE_V + a, x, y);
+        if constexpr (ST)
+            return sV[(a * nw + x) * ld + y];
+        else
+            return get(WBMI_ROLE_V + a, x, y);
     }
     __device__ inline cplx<double> A(int a, int x, int y) const {
-        return stA ? sA[(a * nw + x) * ld + y] : get(WBMI_ROLE_A + a, x, y);
+        if constexpr (ST)
+            return sA[(a * nw + x) * ld + y];
+        else
+            return get(WBMI_ROLE_A + a, x, y);
     }
     __device__ inline cplx<double> W(int a, int b, int x, int y) const { return get(WBMI_ROLE_W + 3 * a + b, x, y); }
     __device__ inline cplx<double> Ad(int a, int d, int x, int y) const { return get(WBMI_ROLE_AD + 3 * a + d, x, y); }
@@ -184,8 +192,8 @@ struct KCtx {
 
 // pre-Hermitian Omega summand s^c_xy for x, y in the same group
 // (formula/covariant.py:175-200): Omega.nn = s + s^+
-template <typename T, bool LM>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM>& K, int c, int x, int y, bool internal, bool external) {
+template <typename T, bool LM, bool ST>
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal, bool external) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
     for (int l = 0; l < K.nw; ++l) {
@@ -203,8 +211,8 @@ __device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM>
 
 // pre-Hermitian Morb_H summand h^c_xy for x, y in the same group
 // (formula/covariant.py:390-414)
-template <typename T, bool LM>
-__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM>& K, int c, int x, int y, bool internal, bool external) {
+template <typename T, bool LM, bool ST>
+__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal, bool external) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
     for (int l = 0; l < K.nw; ++l) {
@@ -303,8 +311,8 @@ __host__ __device__ constexpr int task_nt() {
     return KIND == 0 ? 5 : KIND == 1 ? 4 : KIND == 2 ? 2 : KIND == 3 ? 2 : KIND == 4 ? 5 : KIND == 7 ? 1 : 3;
 }
 
-template <int KIND, typename T, bool LM>
-__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM>& K, const TaskCtx& tc_,
+template <int KIND, typename T, bool LM, bool ST>
+__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST>& K, const TaskCtx& tc_,
                                                            double* accw, int b, int d, int rb, int cb) {
     const int nw = K.nw, lane = threadIdx.x & 63;
     const int r0 = rb * 32, c0 = cb * 32;
@@ -424,7 +432,7 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
                     if constexpr (KIND == 0)
                         gz[bj][v] = K.W(b, d, xx, yy);  // (l, n)
                     else if constexpr (KIND == 1)
-                        gz[bj][v] = K.W(b, d, yy, xx);  // (n, l)
+                        gz[bj][v] = conj(K.W(b, d, xx, yy));  // (n, l) of the Hermitian W, read along rows
                     else if constexpr (KIND == 2)
                         gz[bj][v] = ex ? K.Ad(b, d, xx, yy) : mk(0.0, 0.0);
                     else
@@ -541,8 +549,8 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
 // DerMorb = DerMorb_H + sign [Eav o dOmega + (Omega V + V Omega)/2] on cut
 // partitions; its trace terms (s1 ... s7, the Omega block) are listed at
 // formula_task.
-template <typename T, bool LM>
-__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM>& K, bool in, bool ex,
+template <typename T, bool LM, bool ST>
+__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST>& K, bool in, bool ex,
                                                             bool dm, int sign) {
     const int nw = K.nw, tid = threadIdx.x, wave = tid >> 6;
     // per-band terms, lanes running over (n, c, d)
@@ -564,11 +572,14 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
                     s += times_i(-(sig * (K.A(a, n, m) * y2)));
                 }
             }
-            // T5: 0.5 dO.nn = 0.5 [O'^{cd}_nn - (D'^d O^c)_nn + (O^c D'^d)_nn]
-            cplx<double> t5 = K.Od(c, d, n, n);
-#pragma unroll 4
-            for (int l = 0; l < nw; ++l) t5 += K.O(c, n, l) * K.Dp(d, l, n) - K.Dp(d, n, l) * K.O(c, l, n);
-            s += 0.5 * t5;
+            // T5: 0.5 dO.nn = 0.5 [O'^{cd}_nn - (D'^d O^c)_nn + (O^c D'^d)_nn].  O is Hermitian
+            // and D' anti-Hermitian, so (D'^d O^c)_nn = -conj((O^c D'^d)_nn) and only the real part,
+            // Re O'_nn + 2 Re sum_l conj(O_ln) D'_ln, is used: column n of O, which consecutive
+            // threads (consecutive n) read from consecutive addresses
+            double t5 = K.Od(c, d, n, n).re;
+#pragma unroll 8
+            for (int l = 0; l < nw; ++l) t5 += 2.0 * (conj(K.O(c, l, n)) * K.Dp(d, l, n)).re;
+            s.re += 0.5 * t5;
             v = (dm ? 2.0 * sign * S.E[n] : 2.0) * s.re;
             if (dm) {
                 // s3: dH.nn = C'^{cd}_nn - (D'^d C^c)_nn + (C^c D'^d)_nn
@@ -636,8 +647,8 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
 }
 
 // Per-band values for the current partition into S.acc[n][comp] (.re).
-template <int F, typename T, bool LM>
-__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM>& K, const wbmi_scan_params& p) {
+template <int F, typename T, bool LM, bool ST>
+__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM, ST>& K, const wbmi_scan_params& p) {
     const int nw = K.nw;
     const int tid = threadIdx.x;
     const bool in = p.internal_terms, ex = p.external_terms;
@@ -709,7 +720,7 @@ __device__ inline int fermi_bin(const wbmi_scan_params& p, double E) {
     return -1;
 }
 
-template <typename T, int F, bool LM>
+template <typename T, int F, bool LM, bool ST>
 __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
                                                             const cplx<T>* __restrict__ Xall,
                                                             double* __restrict__ hist) {
@@ -719,8 +730,9 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const int64_t k = blockIdx.x;
     const int tid = threadIdx.x;
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
-    const bool stage_v = args.stage_va && tasks;
-    const bool stage_a = stage_v && p.external_terms;
+    static_assert(tasks || !ST, "V and A are staged only for the formulas with wave tasks");
+    constexpr bool stage_v = ST;
+    const bool stage_a = ST && p.external_terms;
     const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tasks);
     for (int n = tid; n < nw; n += SC_THREADS) S.E[n] = Eall[k * nw + n];
     for (int r = tid; r < WBMI_NROLES; r += SC_THREADS) S.ch[r] = p.ch[r];
@@ -793,7 +805,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const int ng = S.misc[0];
     const int bandmax = S.misc[1];
 
-    KCtx<T, LM> K;
+    KCtx<T, LM, ST> K;
     K.X = Xk;
     K.nw = nw;
     K.ld = ld;
@@ -805,8 +817,6 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     K.ge = S.ge;
     K.sV = S.V;
     K.sA = S.A;
-    K.stV = stage_v;
-    K.stA = stage_a;
 
     if (additive) {
         // main partition: every group at once
@@ -928,26 +938,29 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
     return WBMI_OK;
 }
 
-template <typename T, int F, bool LM>
+template <typename T, int F, bool LM, bool ST>
 static int launch_scan_m(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
-    const bool stage_v = a.stage_va && tasks;
-    const bool stage_a = stage_v && a.p.external_terms;
-    size_t lds = smem_offsets(a.nw, stage_v, stage_a, tasks).total;
+    const bool stage_a = ST && a.p.external_terms;
+    size_t lds = smem_offsets(a.nw, ST, stage_a, tasks).total;
     if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM>,
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM, ST>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_static_scan<T, F, LM>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
+    hipLaunchKernelGGL((k_static_scan<T, F, LM, ST>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
                        (const cplx<T>*)X, hist);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
 }
 
-// dEinv o Off in LDS up to SC_M_NW (the layout of smem_offsets), formed on the fly above
+// dEinv o Off in LDS up to SC_M_NW (the layout of smem_offsets), formed on the fly above;
+// V and A in LDS up to SC_LDS_NW for the formulas with wave tasks
 template <typename T, int F>
 static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    if (a.nw <= SC_M_NW) return launch_scan_m<T, F, true>(ctx, nk, a, E, X, hist);
-    return launch_scan_m<T, F, false>(ctx, nk, a, E, X, hist);
+    constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
+    if constexpr (tasks)
+        if (a.stage_va) return launch_scan_m<T, F, true, true>(ctx, nk, a, E, X, hist);
+    if (a.nw <= SC_M_NW) return launch_scan_m<T, F, true, false>(ctx, nk, a, E, X, hist);
+    return launch_scan_m<T, F, false, false>(ctx, nk, a, E, X, hist);
 }
 
 template <typename T>
