@@ -171,12 +171,16 @@ __global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, c
         }
         return -1;
     };
+    __syncthreads();  // sU staged
+    // From here on every wave works on its own tile: the phases of a channel are
+    // ordered by wave-level fences only, so the waves drift apart and one wave's
+    // loads and stores overlap the other waves' products on the matrix cores.
     for (int base = 0; base < nrot; base += ROTW_WAVES) {
         const int c = base + wave < nrot ? channel(base + wave) : -1;
-        cplx<T>* Xc = X + (k * nch + (c < 0 ? 0 : c)) * nw2;
-        const bool herm = c >= 0 && bit_of(herm_lo, herm_hi, c);
-        __syncthreads();  // sU staged; the previous round's tiles consumed
-        if (c >= 0) {
+        if (c < 0) break;
+        cplx<T>* Xc = X + (k * nch + c) * nw2;
+        const bool herm = bit_of(herm_lo, herm_hi, c);
+        {
             // all of the lane's loads in flight before the first LDS store (nw2 <= 1024 = 16 x 64)
             cplx<T> buf[16];
 #pragma unroll
@@ -190,10 +194,10 @@ __global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, c
                 if (t < nw2) tile[(t / nw) * ld + t % nw] = mk((double)buf[u].re, (double)buf[u].im);
             }
         }
-        __syncthreads();
+        wave_lds_sync();
         MfmaBlock acc;
         acc.zero();
-        if (c >= 0) {
+        {
             // T = X U, X = 0.5 (X + X^+) when flagged (matrix cores, FP64)
             if (herm)
                 mfma_block(acc, nw, nw, nw, 0, 0,
@@ -206,19 +210,18 @@ __global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, c
                 mfma_block(acc, nw, nw, nw, 0, 0, [&](int r, int m) { return tile[r * ld + m]; },
                            [&](int m, int cc) { return sU[m * ld + cc]; });
         }
-        __syncthreads();  // every read of X is done before T overwrites the tile
-        if (c >= 0)
+        wave_lds_sync();  // every read of X is done before T overwrites the tile
 #pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
+        for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
-                for (int bj = 0; bj < 2; ++bj)
+            for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int r = mfma_row(bi, v), cc = mfma_col(bj);
-                        if (r < nw && cc < nw) tile[r * ld + cc] = acc.get(bi, bj, v);
-                    }
-        __syncthreads();
-        if (c >= 0) {
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mfma_row(bi, v), cc = mfma_col(bj);
+                    if (r < nw && cc < nw) tile[r * ld + cc] = acc.get(bi, bj, v);
+                }
+        wave_lds_sync();
+        {
             // Xbar = U^+ T
             acc.zero();
             mfma_block(acc, nw, nw, nw, 0, 0, [&](int r, int m) { return conj(sU[m * ld + r]); },
@@ -236,6 +239,7 @@ __global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, c
                         }
                     }
         }
+        wave_lds_sync();  // every read of T is done before the next channel overwrites the tile
     }
 }
 

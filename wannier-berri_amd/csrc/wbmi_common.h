@@ -28,6 +28,16 @@ int wbmi_fail(wbmi_ctx* ctx, int code, const char* fmt, ...);
 
 #define WBMI_LAUNCH_CHECK(ctx) WBMI_HIP_CHECK(ctx, hipGetLastError())
 
+// Orders one wave's own LDS accesses between phases (LDS written by some lanes
+// and then read by other lanes of the same wave): the lanes of a wave execute
+// together, so no s_barrier is needed; the fences wait for the wave's
+// outstanding LDS operations and keep the compiler from moving accesses across.
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // Complex number with natural alignment for vector loads.
 template <typename T>
 struct __attribute__((aligned(2 * sizeof(T)))) cplx {
