@@ -256,12 +256,16 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi_c(int nw, const cplx<Tin>* _
                 T c = 1, sc = 0, ss = 0;
                 T scale = fabs(app) > fabs(aqq) ? fabs(app) : fabs(aqq);
                 if (g > (T)0.5 * eps * scale && g > (T)1e-36) {
-                    T tau = (aqq - app) / (2 * g);
-                    T t = (tau >= 0 ? (T)1 : (T)-1) / (fabs(tau) + sqrt((T)1 + tau * tau));
-                    c = (T)1 / sqrt((T)1 + t * t);
-                    T s = t * c;
-                    sc = s * apq.re / g;
-                    ss = s * apq.im / g;
+                    // hardware reciprocal and reciprocal square root (1 ulp): c^2 + s^2 = 1 to
+                    // rounding as with correctly rounded division, in far fewer instructions
+                    const T rg = __builtin_amdgcn_rcpf(g);
+                    const T tau = (aqq - app) * (T)0.5 * rg;
+                    const T ta = __builtin_amdgcn_rcpf(fabs(tau) + sqrt((T)1 + tau * tau));
+                    const T t = tau >= 0 ? ta : -ta;
+                    c = __builtin_amdgcn_rsqf((T)1 + t * t);
+                    const T s = t * c;
+                    sc = s * apq.re * rg;
+                    ss = s * apq.im * rg;
                     rot = true;
                     *flag = 1;
                 }
@@ -537,32 +541,36 @@ __global__ void __launch_bounds__(JROWS_THREADS) k_eigh_jacobi_rows(int nw, cons
 #define OA_TOL_C64 1e-8
 #define OA_TOL_C128 1e-10
 
-template <int TS, class FX, class FY, class FO>
+// C = X Y (nw x nw, complex FP64) on the matrix cores: 16 x 16 output blocks
+// dealt to the waves of the workgroup, four independent accumulators per block
+// (re re, re im, im re, im im) so that consecutive MFMAs never wait on each
+// other; out(r, c, value) for r, c < nw.  Lane layout as in mfma_c64.h.
+template <int NTHREADS, class FX, class FY, class FO>
 __device__ __attribute__((always_inline)) void oa_gemm(int nw, FX X, FY Y, FO out) {
-    const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
-    const int r0 = TS * tr, c0 = TS * tc;
-    if (r0 >= nw || c0 >= nw) return;
-    cplx<double> acc[TS][TS];
+    const int wave = threadIdx.x >> 6, nwave = NTHREADS / 64;
+    const int li = threadIdx.x & 15, lk = (threadIdx.x & 63) >> 4;
+    const int nb = (nw + 15) / 16;
+    for (int t = wave; t < nb * nb; t += nwave) {
+        const int r0 = (t / nb) * 16, c0 = (t % nb) * 16;
+        const bool rok = r0 + li < nw, cok = c0 + li < nw;
+        wbmi_v4d rr = {0.0, 0.0, 0.0, 0.0}, ri = rr, ir = rr, ii = rr;
+#pragma unroll 2
+        for (int k0 = 0; k0 < nw; k0 += 4) {
+            const int k = k0 + lk;
+            const bool kok = k < nw;
+            const cplx<double> a = (kok && rok) ? X(r0 + li, k) : mk(0.0, 0.0);
+            const cplx<double> b = (kok && cok) ? Y(k, c0 + li) : mk(0.0, 0.0);
+            rr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.re, rr, 0, 0, 0);
+            ri = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b.im, ri, 0, 0, 0);
+            ir = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.re, ir, 0, 0, 0);
+            ii = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b.im, ii, 0, 0, 0);
+        }
 #pragma unroll
-    for (int i = 0; i < TS; ++i)
-#pragma unroll
-        for (int j = 0; j < TS; ++j) acc[i][j] = mk(0.0, 0.0);
-    for (int m = 0; m < nw; ++m) {
-        cplx<double> x[TS], y[TS];
-#pragma unroll
-        for (int i = 0; i < TS; ++i) x[i] = X(r0 + i < nw ? r0 + i : nw - 1, m);
-#pragma unroll
-        for (int j = 0; j < TS; ++j) y[j] = Y(m, c0 + j < nw ? c0 + j : nw - 1);
-#pragma unroll
-        for (int i = 0; i < TS; ++i)
-#pragma unroll
-            for (int j = 0; j < TS; ++j) cfma(acc[i][j], x[i], y[j]);
+        for (int v = 0; v < 4; ++v) {
+            const int r = r0 + lk + 4 * v, c = c0 + li;
+            if (r < nw && c < nw) out(r, c, mk(rr[v] - ii[v], ri[v] + ir[v]));
+        }
     }
-#pragma unroll
-    for (int i = 0; i < TS; ++i)
-#pragma unroll
-        for (int j = 0; j < TS; ++j)
-            if (r0 + i < nw && c0 + j < nw) out(r0 + i, c0 + j, acc[i][j]);
 }
 
 // A = (H + H^+)/2 of k-point Hk into LDS
@@ -676,7 +684,7 @@ __device__ void cluster_jacobi(int n, FS S, FX X, char* scratch, double theta, d
 // re-orthogonalised there; the following steps separate it, converging
 // quadratically once the mixing angle is small (six to eight steps for a
 // relative gap of 4e-6 at nw = 18..40).
-template <typename TU, int TS>
+template <typename TU>
 __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<double>* __restrict__ H64,
                                                             int64_t Hstride, double* __restrict__ Eout,
                                                             cplx<TU>* __restrict__ U, int iters, double tol) {
@@ -703,15 +711,15 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         if (tid < 4) red[tid] = 0.0;
         __syncthreads();
         // T = A X
-        oa_gemm<TS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
+        oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sT[r * ld + c] = v; });
         __syncthreads();
         // S = X^+ T ; R = I - X^+ X (into sA)
-        oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
+        oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
                     [&](int m, int c) { return sT[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sS[r * ld + c] = v; });
         __syncthreads();
-        oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
+        oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
                     [&](int m, int c) { return sX[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sA[r * ld + c] = mk((r == c ? 1.0 : 0.0) - v.re, -v.im); });
         __syncthreads();
@@ -762,7 +770,7 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         // every thread reads red[3] here; it is cleared only after the next barrier
         const bool converged = red[3] < tol * tol;
         // X + X E into sT, which becomes X
-        oa_gemm<TS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sS[m * ld + c]; },
+        oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sS[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sT[r * ld + c] = sX[r * ld + c] + v; });
         __syncthreads();
         cplx<double>* t = sX;
@@ -774,11 +782,11 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         // complex128 output: one Newton-Schulz step X <- X (I + R/2), R = I - X^+ X, so that the
         // basis is orthonormal to rounding also where a close pair leaves the vectors themselves
         // accurate only to ~eps |H| / gap (the correction steps do not preserve orthogonality)
-        oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
+        oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
                     [&](int m, int c) { return sX[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sA[r * ld + c] = mk(0.5 * ((r == c ? 1.0 : 0.0) - v.re), -0.5 * v.im); });
         __syncthreads();
-        oa_gemm<TS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sA[m * ld + c]; },
+        oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sA[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sT[r * ld + c] = sX[r * ld + c] + v; });
         __syncthreads();
         cplx<double>* t = sX;
@@ -789,10 +797,10 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
     const double scale = red[2];  // max |lam| of the last step
     oa_load_a(nw, ld, Hk, sA);
     __syncthreads();
-    oa_gemm<TS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
+    oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
                 [&](int r, int c, cplx<double> v) { sT[r * ld + c] = v; });
     __syncthreads();
-    oa_gemm<TS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
+    oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
                 [&](int m, int c) { return sT[m * ld + c]; },
                 [&](int r, int c, cplx<double> v) { sS[r * ld + c] = v; });
     __syncthreads();
@@ -1056,17 +1064,10 @@ static int eigh_refined(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* H64, 
     }
     const size_t lds2 =
         4 * sizeof(cplx<double>) * nw * (nw + 1) + sizeof(double) * (nw + 4) + cj_lds(nw) + sizeof(int) * nw + 16;
-    if (nw <= 32) {
-        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<TU, 2>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-        hipLaunchKernelGGL((k_eigh_refine<TU, 2>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw, H64, Hstride, E,
-                           U, steps, tol);
-    } else {
-        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<TU, 4>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-        hipLaunchKernelGGL((k_eigh_refine<TU, 4>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw, H64, Hstride, E,
-                           U, steps, tol);
-    }
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_refine<TU>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+    hipLaunchKernelGGL((k_eigh_refine<TU>), dim3(nk), dim3(OA_THREADS), lds2, ctx->stream, nw, H64, Hstride, E, U,
+                       steps, tol);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
 }
