@@ -68,6 +68,13 @@ def main():
              "fe_ident_fder1": "reference/calculators/calculator-Fe-ident-fder=1.npz",
              "fe_ident_fder2": "reference/calculators/calculator-Fe-ident-fder=2.npz",
              "fe_ident_fder3": "reference/calculators/calculator-Fe-ident-fder=3.npz",
+             # Haldane with the C3z point group, irreducible K-points, symmetrised results and
+             # one refinement iteration (adpt_mesh 2, resp. 3): tests/test_run.py:691-745
+             "haldane_sym_ahc_iter1": I + "Haldane-ahc_iter-0001.npz",
+             "haldane_sym_dos_iter1": I + "Haldane-dos_iter-0001.npz",
+             "haldane_adpt3_ahc_iter0": I + "Haldane-ahc-PythTB-adpt3-sym_iter-0000.npz",
+             "haldane_adpt3_ahc_iter1": I + "Haldane-ahc-PythTB-adpt3-sym_iter-0001.npz",
+             "haldane_adpt3_dos_iter1": I + "Haldane-dos-PythTB-adpt3-sym_iter-0001.npz",
              }, "results_run.npz")
     # per-formula gauge-invariant traces (tests/reference/formula/*.npz)
     forms = {}

@@ -238,6 +238,24 @@ def test_refinement_random(wb):
     assert rel_err(r.results["Morb"].data, ref["Morb"].data) < 1e-8
 
 
+@pytest.mark.parametrize("mesh,keys", [(2, ("haldane_sym_ahc_iter1", "haldane_sym_dos_iter1")),
+                                       (3, ("haldane_adpt3_ahc_iter1", "haldane_adpt3_dos_iter1"))])
+def test_haldane_symmetric_refinement(wb, results_run, mesh, keys):
+    """C3z point group: irreducible K-blocks, symmetrised results, one refinement
+    iteration with equivalent sub-blocks merged, against the reference's own
+    fixtures (tests/test_run.py:691-745)."""
+    say(f"test_haldane_symmetric_refinement mesh={mesh}")
+    st = wb.calculators.static
+    s = gsys(wb, "haldane")
+    s.set_pointgroup(["C3z"])
+    Ef = np.linspace(-3, 3, 11)
+    grid = wb.Grid(system=s, NK=[10, 10, 1], NKFFT=[5, 5, 1])
+    r = wb.run(s, grid, {"ahc": st.AHC(Efermi=Ef), "dos": st.DOS(Efermi=Ef)}, adpt_num_iter=1, adpt_mesh=mesh,
+               use_irred_kpt=True, symmetrize=True, save_files=False, verbose=False)
+    assert rel_err(r.results["ahc"].data, results_run[keys[0]]) < 1e-8
+    assert rel_err(r.results["dos"].data, results_run[keys[1]]) < 1e-8
+
+
 E2_H_ANGSTROM = 1.602176634e-19 ** 2 / 6.62607015e-34 / 1e-10   # e^2/(h * 1 A) in S/m
 
 

@@ -1,0 +1,119 @@
+"""Point-group symmetry of the host driver (no GPU): the group closure, stars,
+irreducible K-lists, symmetrised refinement.
+
+The end-to-end check runs the package's own `run()` (irreducible K-points,
+symmetrisation, refinement with merging of equivalent sub-blocks) with the
+oracle evaluating each K-block, against the reference's fixtures for the
+Haldane model with the C3z group (tests/test_run.py:691-745):
+`Haldane-ahc_iter-0001` (adpt_mesh 2) and `Haldane-*-PythTB-adpt3-sym_iter-000*`
+(adpt_mesh 3).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import wb_oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def wb():
+    import wannierberri_amd
+    return wannierberri_amd
+
+
+@pytest.fixture(scope="module")
+def results_run():
+    with np.load(os.path.join(GOLDEN, "results_run.npz"), allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_group_closure(wb):
+    S = wb.symmetry
+    lat = np.eye(3)
+    assert S.PointGroup(["C3z"], real_lattice=[[1, 0, 0], [0.5, np.sqrt(3) / 2, 0], [0, 0, 1]]).size == 3
+    assert S.PointGroup(["C4z", "Inversion"], real_lattice=lat).size == 8
+    # the Fe group of the reference tests: Inversion, C4z, TimeReversal*C2x
+    g = S.PointGroup(["Inversion", "C4z", "TimeReversal*C2x"], real_lattice=lat)
+    assert g.size == 16
+    assert sum(s.TR for s in g.symmetries) == 8 and sum(s.Inv for s in g.symmetries) == 8
+    with pytest.raises(AssertionError):
+        S.PointGroup(["C3z"], real_lattice=lat)          # not a symmetry of the cubic lattice
+
+
+def test_transform_tensor(wb):
+    S = wb.symmetry
+    v = np.array([[1.0, 2.0, 3.0]])
+    # C4z: x -> y; axial (TR-odd) vector under TimeReversal flips
+    assert np.allclose(S.C4z.transform_tensor(v, 1), [[-2.0, 1.0, 3.0]])
+    assert np.allclose(S.TimeReversal.transform_tensor(v, 1, factor_TR=-1), -v)
+    assert np.allclose(S.Inversion.transform_tensor(v, 1, factor_Inv=-1), -v)
+    assert np.allclose(S.Inversion.transform_tensor(v, 1, factor_Inv=1), v)
+    t = np.arange(9.0).reshape(1, 3, 3)
+    R = S.C3z.R
+    assert np.allclose(S.C3z.transform_tensor(t, 2)[0], R @ t[0] @ R.T)
+
+
+def haldane(wb, sym):
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_haldane.npz"))
+    if sym:
+        s.set_pointgroup(["C3z"])
+    return s
+
+
+def test_irreducible_klist(wb):
+    s = haldane(wb, True)
+    g = wb.Grid(system=s, NK=[10, 10, 1], NKFFT=[5, 5, 1])
+    K = g.get_K_list(use_symmetry=True)
+    # 2x2 K-blocks: Gamma and the three M points (one star)
+    assert len(K) == 2
+    assert sorted(round(k.factor, 12) for k in K) == [0.25, 0.75]
+    assert len(g.get_K_list(use_symmetry=False)) == 4
+    s.set_pointgroup(["C3z"])
+    g6 = wb.Grid(system=s, NK=[24, 24, 1], NKFFT=[4, 4, 1])
+    K6 = g6.get_K_list(use_symmetry=True)
+    assert abs(sum(k.factor for k in K6) - 1) < 1e-14
+    assert len(K6) < 36
+
+
+class _OracleBlock:
+    """data_k_class for run(): the oracle evaluates the K-block."""
+    osys = None
+
+    def __init__(self, system, dK=None, grid=None, Kpoint=None, **kw):
+        self.d = O.OracleDataK(self.osys, dK=dK, NKFFT=grid.FFT)
+
+
+class _OracleCalc:
+    allow_grid = True
+
+    def __init__(self, wb, calc, tr, inv):
+        self.wb, self.calc, self.tr, self.inv = wb, calc, tr, inv
+
+    def __call__(self, data):
+        r = self.calc(data.d)
+        return self.wb.EnergyResult(r.Efermi, r.data, transformTR=self.tr, transformInv=self.inv)
+
+
+@pytest.mark.parametrize("mesh,keys", [(2, ("haldane_sym_ahc_iter1", "haldane_sym_dos_iter1")),
+                                       (3, ("haldane_adpt3_ahc_iter1", "haldane_adpt3_dos_iter1"))])
+def test_symmetric_refinement_haldane(wb, results_run, mesh, keys):
+    s = haldane(wb, True)
+    _OracleBlock.osys = O.load_golden_system(os.path.join(GOLDEN, "sys_haldane.npz"))
+    Ef = np.linspace(-3, 3, 11)
+    calcs = {"ahc": _OracleCalc(wb, O.AHC(Ef), -1, 1), "dos": _OracleCalc(wb, O.DOS(Ef), 1, 1)}
+    grid = wb.Grid(system=s, NK=[10, 10, 1], NKFFT=[5, 5, 1])
+    r = wb.run(s, grid, calcs, adpt_num_iter=1, adpt_mesh=mesh, use_irred_kpt=True, symmetrize=True,
+               data_k_class=_OracleBlock, save_files=False, verbose=False)
+    assert rel_err(r.results["ahc"].data, results_run[keys[0]]) < 1e-8
+    assert rel_err(r.results["dos"].data, results_run[keys[1]]) < 1e-8
+    if mesh == 3:
+        r0 = wb.run(s, grid, calcs, adpt_num_iter=0, use_irred_kpt=True, data_k_class=_OracleBlock,
+                    save_files=False, verbose=False)
+        assert rel_err(r0.results["ahc"].data, results_run["haldane_adpt3_ahc_iter0"]) < 1e-8

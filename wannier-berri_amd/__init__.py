@@ -4,7 +4,7 @@ The API surface of the path is the reference's: System_R -> Data_K ->
 calculators / run().  All numerics run in the HIP library libwbmi.so
 (include/wbmi.h); see `wannierberri_amd.build_lib` to build it.
 """
-from . import calculators, factors
+from . import calculators, factors, symmetry
 from .calculators import static
 from .data_k import Data_K
 from .grid import Grid, KpointBZparallel
@@ -14,4 +14,4 @@ from .system import System_R
 
 __version__ = "0.1.0"
 __all__ = ["System_R", "Grid", "Data_K", "run", "run_klists", "calculators", "EnergyResult", "ResultDict", "VoidResult",
-           "KpointBZparallel", "factors"]
+           "KpointBZparallel", "factors", "symmetry"]

@@ -33,6 +33,15 @@ class Calculator:
         return True
 
 
+# parity (+1 even, -1 odd) under time reversal and inversion of each formula's trace
+# (formula/covariant.py: Identity 15-16, Omega 172-173, DerOmega 224-225,
+# Morb_Hpm 435-436, DerMorb 535-536; VelOmega and VelHplus are products with the
+# velocity, odd under both, formula/formula.py:127-128, data_K.py:16-46)
+_PARITY = {_lib.FORMULA_IDENTITY: (1, 1), _lib.FORMULA_OMEGA: (-1, 1), _lib.FORMULA_DEROMEGA: (1, -1),
+           _lib.FORMULA_VELOMEGA: (1, -1), _lib.FORMULA_MORB_HPM: (-1, 1), _lib.FORMULA_DERMORB: (1, -1),
+           _lib.FORMULA_VELHPLUS: (1, -1)}
+
+
 def _external(kwargs_formula, system):
     ext = kwargs_formula.get("external_terms", True)
     return bool(ext) and not system.force_internal_terms_only
@@ -127,8 +136,9 @@ class StaticCalculator(Calculator):
         scale = cf / data_K.cell_volume / data_K.nk
         data_K.ctx.finalize(hist, self.fder, self.dEF, scale, out)
         data = out.reshape((nout,) + (3,) * self.ndim)
-        return EnergyResult(self.Efermi, data, smoothers=[self.smoother], comment=self.comment,
-                            save_mode=self.save_mode)
+        tr, inv = _PARITY[self.formula]
+        return EnergyResult(self.Efermi, data, smoothers=[self.smoother], transformTR=tr, transformInv=inv,
+                            comment=self.comment, save_mode=self.save_mode)
 
 
 # -- implemented calculators (calculators/static.py:192-498) --------------------

@@ -1,13 +1,17 @@
 """K-grid: NK = NKdiv x NKFFT mixed decomposition and K-block objects.
 
 Follows wannierberri/grid/grid.py:86-266 (Grid, determineNK, autoNK) and
-grid/Kpoint.py:9-181 (KpointBZparallel, divide) for systems without
-point-group symmetry (the identity group: every K-block is irreducible).
+grid/Kpoint.py:9-216 (KpointBZparallel: star, absorb, equiv, divide;
+exclude_equiv_points): with a point group the K-list holds one K-block per
+star, weighted by the star's size, and refinement keeps only inequivalent
+sub-blocks.
 """
 from collections.abc import Iterable
 import warnings
 
 import numpy as np
+
+from .symmetry import SYMMETRY_PRECISION, PointGroup
 
 
 def one2three(nk):
@@ -26,9 +30,13 @@ def _iterate_vector(v1, v2):
     return ((x, y, z) for x in range(v1[0], v2[0]) for y in range(v1[1], v2[1]) for z in range(v1[2], v2[2]))
 
 
-def autoNK(NK, NKFFTrec):
-    """FFT grid between NKFFTrec and 2*NKFFTrec that best divides NK (grid.py:196-212)."""
-    cand = np.array([f for f in _iterate_vector(NKFFTrec, NKFFTrec * 2)])
+def autoNK(NK, NKFFTrec, pointgroup=None):
+    """Symmetric FFT grid between the smallest symmetric one (>= NKFFTrec) and twice it
+    that best divides NK (grid.py:196-212)."""
+    sym = (lambda fft: True) if pointgroup is None else pointgroup.symmetric_grid
+    fft_sym = np.array([f for f in _iterate_vector(NKFFTrec, NKFFTrec * 3) if sym(f)])
+    NKFFTmin = fft_sym[np.argmin(fft_sym.prod(axis=1))]
+    cand = np.array([f for f in _iterate_vector(NKFFTmin, NKFFTmin * 2) if sym(f)])
     NKdiv = np.array(np.round(NK[None, :] / cand), dtype=int)
     NKdiv[NKdiv <= 0] = 1
     change = NKdiv * cand / NK[None, :]
@@ -40,13 +48,18 @@ def autoNK(NK, NKFFTrec):
     return NKdiv, FFT
 
 
-def determineNK(periodic, NKdiv, NKFFT, NK, NKFFT_recommended, recip_lattice, length=None, length_FFT=None):
+def determineNK(periodic, NKdiv, NKFFT, NK, NKFFT_recommended, recip_lattice, length=None, length_FFT=None,
+                pointgroup=None):
     """grid.py:215-266."""
     NKdiv, NKFFT, NK = one2three(NKdiv), one2three(NKFFT), one2three(NK)
     if length is not None and NK is None:
         NK = np.array(np.round(length / (2 * np.pi) * np.linalg.norm(recip_lattice, axis=1)), dtype=int)
     if length_FFT is not None and NKFFT is None:
         NKFFT = np.array(np.round(length_FFT / (2 * np.pi) * np.linalg.norm(recip_lattice, axis=1)), dtype=int)
+    if pointgroup is not None:
+        for nkname, nk in (("NKdiv", NKdiv), ("NK", NK), ("NKFFT", NKFFT)):
+            if nk is not None:
+                assert pointgroup.symmetric_grid(nk), f" {nkname}={nk} is not consistent with the given symmetry "
     if (NKdiv is not None) and (NKFFT is not None):
         pass
     elif NK is not None:
@@ -54,7 +67,7 @@ def determineNK(periodic, NKdiv, NKFFT, NK, NKFFT_recommended, recip_lattice, le
             NKdiv = np.array(np.round(NK / NKFFT), dtype=int)
             NKdiv[NKdiv <= 0] = 1
         else:
-            NKdiv, NKFFT = autoNK(NK, np.array(NKFFT_recommended))
+            NKdiv, NKFFT = autoNK(NK, np.array(NKFFT_recommended), pointgroup)
     else:
         raise ValueError("you need to specify either NK or a pair (NKdiv,NKFFT) or (NK,NKFFT). "
                          f"found NK={NK}, NKdiv={NKdiv}, NKFFT={NKFFT} ")
@@ -71,15 +84,18 @@ def determineNK(periodic, NKdiv, NKFFT, NK, NKFFT_recommended, recip_lattice, le
 class KpointBZparallel:
     """A K-block: the NKFFT sub-grid shifted by K/NKFFT (Kpoint.py:9-181)."""
 
-    def __init__(self, K=np.zeros(3), dK=np.ones(3), NKFFT=np.ones(3), factor=1., refinement_level=0):
+    def __init__(self, K=np.zeros(3), dK=np.ones(3), NKFFT=np.ones(3), factor=1., pointgroup=None,
+                 refinement_level=0):
         self.K = np.array(K, dtype=float)
         self.dK = np.array(dK, dtype=float)
         self.NKFFT = np.array(NKFFT, dtype=int)
         self.factor = factor
+        self.pointgroup = pointgroup
         self.refinement_level = refinement_level
         self.result = None
         self.was_evaluated_flag = False
         self._max = None
+        self._star = None
 
     @property
     def Kp_fullBZ(self):
@@ -88,6 +104,20 @@ class KpointBZparallel:
     @property
     def dK_fullBZ(self):
         return self.dK / self.NKFFT
+
+    @property
+    def star(self):
+        """Images of K under the point group (Kpoint.py:117-122)."""
+        if self._star is None:
+            self._star = np.array([self.K]) if self.pointgroup is None else self.pointgroup.star(self.K)
+        return self._star
+
+    @property
+    def distGamma(self):
+        """Distance of K from the nearest reciprocal lattice vector (Kpoint.py:177-181)."""
+        sh = np.arange(-3, 4)
+        corners = np.array([[x, y, z] for x in sh for y in sh for z in sh])
+        return np.linalg.norm(((self.K % 1)[None, :] - corners).dot(self.pointgroup.recip_lattice), axis=1).min()
 
     def set_result(self, res):
         self.result = res
@@ -105,13 +135,34 @@ class KpointBZparallel:
     def set_factor(self, factor):
         self.factor = factor
 
+    def add_factor(self, factor):
+        self.factor += factor
+
     @property
     def max(self):
         assert self.was_evaluated_flag, "max is called for a K-point which is not evaluated"
         return self._max * self.factor
 
-    def divide(self, ndiv, periodic, use_symmetry=False):
-        """Split into ndiv^3 sub-blocks (Kpoint.py:145-175)."""
+    def absorb(self, other):
+        """Take over the weight (and an evaluated result) of an equivalent K-point (Kpoint.py:124-133)."""
+        if other is None:
+            return
+        if other.was_evaluated_flag:
+            if self.was_evaluated_flag:
+                raise RuntimeError(f"combining two evaluated K-points {self} and {other} should not happen")
+            self.set_result(other.get_result())
+        self.add_factor(other.factor)
+
+    def equiv(self, other):
+        """Same refinement level and K in each other's star (Kpoint.py:135-142)."""
+        if self.refinement_level != other.refinement_level:
+            return False
+        dif = self.star[:, None, :] - other.star[None, :, :]
+        return bool(np.linalg.norm(dif - np.round(dif), axis=2).min() < SYMMETRY_PRECISION)
+
+    def divide(self, ndiv, periodic, use_symmetry=True):
+        """Split into ndiv^3 sub-blocks (Kpoint.py:144-175); with symmetry, equivalent
+        sub-blocks are merged."""
         ndiv = np.array(ndiv, dtype=int)
         ndiv[np.logical_not(periodic)] = 1
         dK_adpt = self.dK / ndiv
@@ -122,9 +173,11 @@ class KpointBZparallel:
             for y in range(ndiv[1]):
                 for z in range(ndiv[2]):
                     out.append(KpointBZparallel(K=self.K + shift + dK_adpt * np.array([x, y, z]), dK=dK_adpt,
-                                                NKFFT=self.NKFFT, factor=newfac,
+                                                NKFFT=self.NKFFT, factor=newfac, pointgroup=self.pointgroup,
                                                 refinement_level=self.refinement_level + 1))
         self.set_factor(0)
+        if use_symmetry and self.pointgroup is not None:
+            exclude_equiv_points(out)
         return out
 
     def __str__(self):
@@ -132,29 +185,72 @@ class KpointBZparallel:
                 f" ], refinement level:{self.refinement_level}, factor = {self.factor}")
 
 
-class Grid:
-    """k-grid NK = NKdiv * NKFFT (grid/grid.py:86-189).
+def exclude_equiv_points(K_list, new_points=None):
+    """Merge equivalent K-points in place (Kpoint.py:184-216).
 
-    Exactly one of `system`, `real_lattice`, `recip_lattice` gives the lattice.
+    Points are bucketed by their distance from Gamma; within a bucket a later
+    point equivalent to an earlier one is absorbed by it, except that two of
+    the first n - new_points (old) points are never merged.
+    """
+    n = len(K_list)
+    if new_points is None:
+        new_points = n
+    dist = np.array([K.distGamma for K in K_list])
+    order = np.argsort(dist)
+    dist = dist[order]
+    walls = [0] + list(np.where(dist[1:] - dist[:-1] > 1e-4)[0] + 1) + [n]
+    exclude = []
+    for start, end in zip(walls[:-1], walls[1:]):
+        for a in range(start, end):
+            i = order[a]
+            if i in exclude:
+                continue
+            for b in range(start, end):
+                j = order[b]
+                if i >= j:
+                    continue
+                if i < n - new_points and j < n - new_points:
+                    continue
+                if j not in exclude and K_list[i].equiv(K_list[j]):
+                    exclude.append(j)
+                    K_list[i].absorb(K_list[j])
+    for i in sorted(exclude)[::-1]:
+        del K_list[i]
+
+
+class Grid:
+    """k-grid NK = NKdiv * NKFFT (grid/grid.py:13-189).
+
+    Exactly one of `system`, `real_lattice`, `recip_lattice`, `pointgroup`
+    gives the lattice.  With use_symmetry (default) the point group is the
+    system's (or the given one), otherwise the trivial group of the lattice
+    (grid.py:44-60).
     """
 
-    def __init__(self, system=None, real_lattice=None, recip_lattice=None, length=None, NKdiv=None,
-                 NKFFT=None, NK=None, length_FFT=None, use_symmetry=False, pointgroup=None):
-        found = [p is not None for p in (system, real_lattice, recip_lattice)]
-        assert sum(found) == 1, "Exactly one of system, real_lattice, recip_lattice should be provided"
+    def __init__(self, system=None, real_lattice=None, recip_lattice=None, pointgroup=None, length=None,
+                 NKdiv=None, NKFFT=None, NK=None, length_FFT=None, use_symmetry=True):
+        found = [p is not None for p in (system, real_lattice, recip_lattice, pointgroup)]
+        assert sum(found) == 1, "Exactly one of system, real_lattice, recip_lattice, pointgroup should be provided"
+        periodic = np.ones(3, dtype=bool)
+        NKFFT_rec = np.ones(3, dtype=int)
         if system is not None:
             real_lattice = system.real_lattice
             periodic = system.periodic
             NKFFT_rec = np.array(system.NKFFT_recommended)
-        else:
-            if real_lattice is None:
-                real_lattice = 2 * np.pi * np.linalg.inv(np.array(recip_lattice)).T
-            periodic = np.ones(3, dtype=bool)
-            NKFFT_rec = np.ones(3, dtype=int)
+        elif pointgroup is not None:
+            real_lattice = pointgroup.real_lattice
+        elif real_lattice is None:
+            real_lattice = 2 * np.pi * np.linalg.inv(np.array(recip_lattice)).T
         self.real_lattice = np.array(real_lattice)
         self.recip_lattice = 2 * np.pi * np.linalg.inv(self.real_lattice).T
+        if use_symmetry and system is not None:
+            self.pointgroup = system.pointgroup
+        elif use_symmetry and pointgroup is not None:
+            self.pointgroup = pointgroup
+        else:
+            self.pointgroup = PointGroup([], real_lattice=self.real_lattice)
         self.div, self.FFT = determineNK(periodic, NKdiv, NKFFT, NK, NKFFT_rec, self.recip_lattice,
-                                         length=length, length_FFT=length_FFT)
+                                         length=length, length_FFT=length_FFT, pointgroup=self.pointgroup)
 
     @property
     def dense(self):
@@ -170,8 +266,24 @@ class Grid:
         return np.array([[i / N[0], j / N[1], l / N[2]] for i in range(N[0]) for j in range(N[1])
                          for l in range(N[2])])
 
-    def get_K_list(self, use_symmetry=False, k_batch=None):
+    def get_K_list(self, use_symmetry=True, k_batch=None):
+        """K-blocks of the grid; with use_symmetry one per star of the point group,
+        carrying the star's weight (grid.py:148-189)."""
         dK = 1. / self.div
         factor = 1. / np.prod(self.div)
-        return [KpointBZparallel(K=np.array([x, y, z]) * dK, dK=dK, NKFFT=self.FFT, factor=factor, refinement_level=0)
-                for x in range(self.div[0]) for y in range(self.div[1]) for z in range(self.div[2])]
+        K = [[[KpointBZparallel(K=np.array([x, y, z]) * dK, dK=dK, NKFFT=self.FFT, factor=factor,
+                                pointgroup=self.pointgroup, refinement_level=0)
+               for z in range(self.div[2])] for y in range(self.div[1])] for x in range(self.div[0])]
+        if use_symmetry:
+            for x in range(self.div[0]):
+                for y in range(self.div[1]):
+                    for z in range(self.div[2]):
+                        KP = K[x][y][z]
+                        if KP is None:
+                            continue
+                        star = [tuple(k) for k in np.array(np.round(KP.star * self.div), dtype=int) % self.div]
+                        for k in star:
+                            if k != (x, y, z):
+                                KP.absorb(K[k[0]][k[1]][k[2]])
+                                K[k[0]][k[1]][k[2]] = None
+        return [Kp for Kyz in K for Kz in Kyz for Kp in Kz if Kp is not None]

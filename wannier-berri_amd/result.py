@@ -41,6 +41,9 @@ class VoidResult(Result):
     def savedata(self, name, prefix, suffix, i_iter):
         pass
 
+    def transform(self, sym):
+        return self
+
 
 class EnergyResult(Result):
     """Data on a Fermi-level grid: data[iEF, 3, 3, ...] (result/energyresult.py:11-278).
@@ -135,6 +138,17 @@ class EnergyResult(Result):
         d = self._data
         return self._new(d.transpose(a, b).contiguous() if _is_tensor(d) else d.swapaxes(a, b))
 
+    def transform(self, sym):
+        """Image under a point operation (energyresult.py:266-278): the last `rank` axes
+        rotated, times the TR / inversion parity (transformTR, transformInv: +1 or -1)
+        where the operation contains time reversal / inversion."""
+        for flag, f, what in ((sym.TR, self.transformTR, "time reversal"), (sym.Inv, self.transformInv, "inversion")):
+            if flag and f is None:
+                raise ValueError(f"the parity of '{self.comment}' under {what} is not known")
+        fTR = self.transformTR if self.transformTR is not None else 1
+        fI = self.transformInv if self.transformInv is not None else 1
+        return self._new(sym.transform_tensor(self._data, self.rank, fTR, fI))
+
     # -- refinement criteria (energyresult.py:250-264) ------------------------
     @property
     def dataSmooth(self):
@@ -209,6 +223,9 @@ class ResultDict(Result):
     def savedata(self, prefix, suffix, i_iter):
         for k, v in self.results.items():
             v.savedata(k, prefix, suffix, i_iter)
+
+    def transform(self, sym):
+        return ResultDict({k: v.transform(sym) for k, v in self.results.items()})
 
     @property
     def max(self):

@@ -1,9 +1,10 @@
 """run(): integrate calculators over a Grid, K-block by K-block, on the MI355X.
 
-Mirrors wannierberri/run_grid.py:32-403 for the serial, symmetry-free case:
-the K-list of the Grid, one data_k_class per K-block, the k-sum
-`sum_K result * Kp.factor`, optional adaptive refinement
-(run_grid.py:370-389, Kpoint.divide) and per-iteration output files
+Mirrors wannierberri/run_grid.py:32-403 for one process: the (irreducible)
+K-list of the Grid, one data_k_class per K-block, point-group symmetrisation
+of each K-block's results, the k-sum `sum_K result * Kp.factor`, optional
+adaptive refinement (run_grid.py:352-389, Kpoint.divide,
+exclude_equiv_points) and per-iteration output files
 `{fout_name}-{calc}{-suffix}_iter-{i:04d}.npz/.dat`.
 """
 import time
@@ -12,6 +13,7 @@ from collections.abc import Iterable
 import numpy as np
 
 from .data_k import Data_K
+from .grid import exclude_equiv_points
 from .result import ResultDict
 
 
@@ -22,14 +24,18 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
         save_files=True, verbose=True):
     """Integrate `calculators` (dict name -> calculator) over `grid`.
 
-    The system carries the identity point group, so `use_irred_kpt` and
-    `symmetrize` do not change the K-list or the result.  `parallel` is
-    accepted for compatibility: one process drives one GPU.
+    With `use_irred_kpt` the K-list holds one K-block per star of the grid's
+    point group (and refinement merges equivalent sub-blocks), which implies
+    `symmetrize`: every K-block's result is averaged over the group
+    (run_grid.py:225-236,258-265,382-386).  `parallel` is accepted for
+    compatibility: one process drives one GPU.
     """
     if restart or dump_results:
         raise NotImplementedError("restart / dump_results are not available on the MI355X path")
     if data_k_class is None:
         data_k_class = Data_K
+    if use_irred_kpt:
+        symmetrize = True
     parameters_K = dict(parameters_K or {})
     for key, calc in calculators.items():
         if not calc.allow_grid:
@@ -89,8 +95,11 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
             break
         Kmax = np.array([K.max for K in K_list]).T
         select_points = set().union(*(np.argsort(Km)[-adpt_fac:] for Km in Kmax))
+        l1 = len(K_list)
         for iK in select_points:
-            K_list += K_list[iK].divide(ndiv=adpt_mesh, periodic=system.periodic)
+            K_list += K_list[iK].divide(ndiv=adpt_mesh, periodic=system.periodic, use_symmetry=use_irred_kpt)
+        if use_irred_kpt:
+            exclude_equiv_points(K_list, new_points=len(K_list) - l1)
     if verbose:
         print(f"run(): processed {counter} K-blocks in {time.time() - t0:.3f} s", flush=True)
     return result_all

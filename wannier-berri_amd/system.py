@@ -13,6 +13,8 @@ import os
 
 import numpy as np
 
+from .symmetry import PointGroup
+
 
 class _Rvec:
     """Minimal `Rvectors` view: iRvec, nRvec, iR0 (fourier/rvectors.py:385-434)."""
@@ -32,23 +34,6 @@ class _Rvec:
         return self.iRvec.tolist().index(np.array(np.round(R), dtype=int).tolist())
 
 
-class _IdentityPointGroup:
-    """The trivial point group: symmetrize() is the identity (point_symmetry.py:337-338)."""
-
-    def __init__(self, real_lattice):
-        self.real_lattice = real_lattice
-        self.recip_lattice = 2 * np.pi * np.linalg.inv(real_lattice).T
-
-    def symmetrize(self, result):
-        return result
-
-    def symmetric_grid(self, nk):
-        return True
-
-    def star(self, K):
-        return np.array([K])
-
-
 class System_R:
     """R-space model (system/system_R.py:18-175)."""
 
@@ -66,7 +51,7 @@ class System_R:
         self._device_cache = {}
         if real_lattice is not None:
             self.real_lattice = np.array(real_lattice, dtype=float)
-            self.pointgroup = _IdentityPointGroup(self.real_lattice)
+            self.pointgroup = PointGroup([], real_lattice=self.real_lattice)
         if iRvec is not None:
             self.rvec = _Rvec(iRvec)
         if matrices:
@@ -76,6 +61,25 @@ class System_R:
             self.wannier_centers_cart = np.array(wannier_centers_cart, dtype=float)
         elif self.has_R_mat("Ham"):
             self.wannier_centers_cart = np.zeros((self.num_wann, 3))
+
+    def set_pointgroup(self, symmetry_gen=(), pointgroup=None, spacegroup=None):
+        """Point group used for irreducible K-points and symmetrisation (system/system.py:140-170).
+
+        `symmetry_gen`: generators (symmetry.PointSymmetry objects or names
+        such as 'C3z', 'TimeReversal*C2x'); the group is closed under products.
+        """
+        if spacegroup is not None:
+            raise NotImplementedError("point groups from a space group (irrep) are not available")
+        if pointgroup is not None:
+            assert np.allclose(pointgroup.recip_lattice, self.recip_lattice), \
+                "the provided pointgroup has a different recip_lattice than the system"
+            self.pointgroup = pointgroup
+        else:
+            self.pointgroup = PointGroup(symmetry_gen, real_lattice=self.real_lattice)
+
+    def set_symmetry(self, symmetry_gen=(), pointgroup=None, spacegroup=None):
+        """Deprecated name of set_pointgroup (system/system.py:130-137)."""
+        self.set_pointgroup(symmetry_gen=symmetry_gen, pointgroup=pointgroup, spacegroup=spacegroup)
 
     # -- matrices -------------------------------------------------------------
     def get_R_mat(self, key):
