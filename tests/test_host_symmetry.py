@@ -117,3 +117,32 @@ def test_symmetric_refinement_haldane(wb, results_run, mesh, keys):
         r0 = wb.run(s, grid, calcs, adpt_num_iter=0, use_irred_kpt=True, data_k_class=_OracleBlock,
                     save_files=False, verbose=False)
         assert rel_err(r0.results["ahc"].data, results_run["haldane_adpt3_ahc_iter0"]) < 1e-8
+
+
+def test_system_npz_roundtrip(wb, tmp_path):
+    """System_R.to_npz -> from_npz keeps matrices, centres, periodicity and the point group."""
+    s = haldane(wb, True)
+    d = str(tmp_path / "sys")
+    s.to_npz(d)
+    t = wb.System_R.from_npz(d)
+    assert np.array_equal(t.get_R_mat("Ham"), s.get_R_mat("Ham"))
+    assert np.array_equal(t.rvec.iRvec, s.rvec.iRvec)
+    assert np.allclose(t.wannier_centers_cart, s.wannier_centers_cart)
+    assert np.array_equal(t.periodic, s.periodic)
+    assert t.pointgroup.size == 3
+
+
+def test_energyresult_npz_roundtrip(wb, tmp_path):
+    """savedata -> from_npz (energyresult.py:83-112,202-248); files hold no pickled objects."""
+    Ef = np.linspace(-1, 1, 5)
+    data = np.arange(45.0).reshape(5, 3, 3)
+    r = wb.EnergyResult(Ef, data, transformTR=1, transformInv=-1, comment="test")
+    r.savedata("bcd", str(tmp_path / "out"), "sfx", 2)
+    fn = str(tmp_path / "out-bcd-sfx_iter-0002.npz")
+    with np.load(fn, allow_pickle=False) as f:
+        assert np.array_equal(f["data"], data)
+    back = wb.EnergyResult.from_npz(fn)
+    assert np.array_equal(back.data, data) and back.rank == 2
+    assert np.array_equal(back.Energies[0], Ef)
+    txt = open(str(tmp_path / "out-bcd-sfx_iter-0002.dat")).read().splitlines()
+    assert txt[0].startswith("#### test") and len(txt) == 2 + 5

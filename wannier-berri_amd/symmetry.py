@@ -179,6 +179,20 @@ class PointGroup:
     def size(self):
         return len(self.symmetries)
 
+    def as_dict(self):
+        """Keys real_lattice, nsym, symm{i}_R (improper matrices kept improper), symm{i}_TR
+        (point_symmetry.py:294-302): the `pointgroup.npz` of a saved system."""
+        d = dict(real_lattice=self.real_lattice, nsym=self.size)
+        for i, sym in enumerate(self.symmetries):
+            d[f"symm{i}_R"] = sym.R * sym.iInv
+            d[f"symm{i}_TR"] = sym.TR
+        return d
+
+    @classmethod
+    def from_dict(cls, d):
+        ops = [PointSymmetry(d[f"symm{i}_R"], bool(d[f"symm{i}_TR"])) for i in range(int(d["nsym"]))]
+        return cls(ops, real_lattice=d["real_lattice"])
+
     def check_basis_symmetry(self, basis, tol=1e-6, rel_tol=None):
         """True if every operation maps the basis vectors onto integer combinations of them."""
         if rel_tol is not None:

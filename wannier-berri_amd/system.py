@@ -189,8 +189,25 @@ class System_R:
             if legacy:
                 a = np.transpose(a, (2, 0, 1) + tuple(range(3, a.ndim)))
             mats[key] = a
-        return cls(real_lattice, iRvec, wcc, mats, periodic=periodic,
-                   force_internal_terms_only=force_internal_terms_only)
+        system = cls(real_lattice, iRvec, wcc, mats, periodic=periodic,
+                     force_internal_terms_only=force_internal_terms_only)
+        pg = os.path.join(path, "pointgroup.npz")
+        if os.path.exists(pg):
+            with np.load(pg, allow_pickle=False) as f:
+                system.set_pointgroup(pointgroup=PointGroup.from_dict({k: f[k] for k in f.files}))
+        return system
+
+    def to_npz(self, path, R_matrices=None, overwrite=True):
+        """Write the directory layout that System_R.from_npz reads (system_R.py:658-711):
+        real_lattice, iRvec, wannier_centers_cart, periodic, pointgroup and
+        compressed `_XX_R_<key>.npz` matrices `[nR, nw, nw, ...]`."""
+        os.makedirs(path, exist_ok=overwrite)
+        for key, val in (("real_lattice", self.real_lattice), ("iRvec", self.rvec.iRvec),
+                         ("wannier_centers_cart", self.wannier_centers_cart), ("periodic", self.periodic)):
+            np.savez(os.path.join(path, key + ".npz"), val)
+        np.savez(os.path.join(path, "pointgroup.npz"), **self.pointgroup.as_dict())
+        for key in (self._XX_R if R_matrices is None else R_matrices):
+            np.savez_compressed(os.path.join(path, "_XX_R_" + key + ".npz"), self.get_R_mat(key))
 
     @classmethod
     def from_arrays_npz(cls, file_npz):
