@@ -311,7 +311,12 @@ __host__ __device__ constexpr int task_nt() {
     return KIND == 0 ? 5 : KIND == 1 ? 4 : KIND == 2 ? 2 : KIND == 3 ? 2 : KIND == 4 ? 5 : KIND == 7 ? 1 : 3;
 }
 
-template <int KIND, typename T, bool LM, bool ST>
+// M1 (kind 0 only): the task also carries the kind-1 task (d, b).  With V
+// Hermitian and D' anti-Hermitian, (D'^d V^b)_yx = -conj((V^b D'^d)_xy), so
+// kind 1's transposed product is -conj of kind 0's at the same element, and
+// its elementwise channel W^{db}_yx = conj(W^{bd}_xy): one product and one
+// load serve both consumers.
+template <int KIND, bool M1, typename T, bool LM, bool ST>
 __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST>& K, const TaskCtx& tc_,
                                                            double* accw, int b, int d, int rb, int cb) {
     const int nw = K.nw, lane = threadIdx.x & 63;
@@ -409,14 +414,39 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
                 tw[0] = 2.0;
             }
     }
+    // targets of the carried kind-1 task (b1, d1) = (d, b)
+    constexpr int NT1 = M1 ? 4 : 1;
+    const int b1 = d, d1 = b;
+    int tp1[NT1], tcol1[NT1];
+    double tw1[NT1];
+#pragma unroll
+    for (int w = 0; w < NT1; ++w) {
+        tp1[w] = 0;
+        tcol1[w] = 0;
+        tw1[w] = 0.0;
+    }
+    if constexpr (M1) {
+        const int nt = b1 == d1 ? 2 : 4;
+        const double f = b1 == d1 ? 2.0 : 1.0;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const Pair p = pair_first(w < 2 ? b1 : d1, w % 2);
+            tp1[w] = p.partner;
+            tcol1[w] = p.c * 3 + (w < 2 ? d1 : b1);
+            tw1[w] = w < nt ? f * p.sig : 0.0;
+        }
+    }
 
     // part[u][w]: u = the lane's band slot, its column bj (the band n is the column y of every element)
     constexpr int NU = 2;
-    double part[NU][NT];
+    double part[NU][NT], part1[NU][NT1];
 #pragma unroll
-    for (int u = 0; u < NU; ++u)
+    for (int u = 0; u < NU; ++u) {
 #pragma unroll
         for (int w = 0; w < NT; ++w) part[u][w] = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT1; ++w) part1[u][w] = 0.0;
+    }
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) {
         // the elementwise channel (W, A', B') of this row half: all loads issued before use
@@ -472,6 +502,19 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
                     part[j][w] += t;
                 }
                 if (dm && in) part[j][4] += ((mln * K.V(tp[4], n, l)) * pv).im;  // s1: Re(-i D'^al V^d D'^be)
+                if constexpr (M1) {
+                    // kind 1 (b1, d1): (W^{b1 d1} + V''^b1 D'^d1 + V''^d1 D'^b1)_nl / 2, with P1 = -conj(pv)
+                    const double mnl = -mln;
+                    cplx<double> z1 = conj(gz[bj][v]);
+                    if (single)
+                        z1 -= mnl * (K.V(b1, n, n) * K.V(d1, n, l) + K.V(d1, n, n) * K.V(b1, n, l));
+                    else
+                        for (int q = S.gs[n]; q < S.ge[n]; ++q)
+                            z1 += K.V(b1, n, q) * K.Dp(d1, q, l) + K.V(d1, n, q) * K.Dp(b1, q, l);
+                    const cplx<double> Pm1 = (-mnl) * (0.5 * z1 + conj(pv));
+#pragma unroll
+                    for (int w = 0; w < 4; ++w) part1[i][w] -= dw * (Pm1 * K.A(tp1[w], l, n)).re;
+                }
             } else if constexpr (KIND == 1) {
                 if (same) continue;
                 // (W^{bd} + V''^b D'^d + V''^d D'^b)_nl / 2
@@ -541,6 +584,13 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
             const double sum = wave_sum(part[u][w], mask);
             if (writer && tw[w] != 0.0 && n < nw) accw[n * 9 + tcol[w]] += tw[w] * sum;
         }
+        if constexpr (M1) {
+#pragma unroll
+            for (int w = 0; w < NT1; ++w) {
+                const double sum = wave_sum(part1[u][w], mask);
+                if (writer && tw1[w] != 0.0 && n < nw) accw[n * 9 + tcol1[w]] += tw1[w] * sum;
+            }
+        }
     }
 }
 
@@ -609,6 +659,9 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
     cnt[6] = dm && ex && sign != 0 ? 3 : 0;
     cnt[7] = dm && ex ? 9 : 0;
     if (dm && sign == 0) cnt[1] = 0;  // T3 feeds only the dOmega term
+    // kind-1 tasks ride on the kind-0 tasks of the transposed index pair when both run
+    const bool merge01 = cnt[0] > 0 && cnt[1] > 0;
+    if (merge01) cnt[1] = 0;
     const int nblk = (nw + 31) / 32, nb2 = nblk * nblk;
     int ntask = 0;
 #pragma unroll
@@ -628,14 +681,19 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
         const int idx = rem / nb2, rb = (rem % nb2) / nblk, cb = rem % nblk;
         const int b = kind == 5 || kind == 6 ? idx : idx / 3, d = kind == 5 || kind == 6 ? 0 : idx % 3;
         switch (kind) {
-            case 0: formula_task<0>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 1: formula_task<1>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 2: formula_task<2>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 3: formula_task<3>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 4: formula_task<4>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 5: formula_task<5>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 6: formula_task<6>(S, K, tcx, accw, b, d, rb, cb); break;
-            default: formula_task<7>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 0:
+                if (merge01)
+                    formula_task<0, true>(S, K, tcx, accw, b, d, rb, cb);
+                else
+                    formula_task<0, false>(S, K, tcx, accw, b, d, rb, cb);
+                break;
+            case 1: formula_task<1, false>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 2: formula_task<2, false>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 3: formula_task<3, false>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 4: formula_task<4, false>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 5: formula_task<5, false>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 6: formula_task<6, false>(S, K, tcx, accw, b, d, rb, cb); break;
+            default: formula_task<7, false>(S, K, tcx, accw, b, d, rb, cb); break;
         }
     }
     __syncthreads();
