@@ -152,6 +152,20 @@ int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride,
               void* work_dev, size_t work_bytes);
 size_t wbmi_eigh_work_bytes(int nk, int nw, int prec, int refine);
 
+/* ---- batched eigensolver with a starting basis ---------------------------
+ * No counterpart in the reference (its eigh starts from nothing,
+ * data_K.py:211-218): the same result as wbmi_eigh with refine >= 1, started
+ * from U0_dev, [nk, nw, nw] complex of `prec` (columns = approximate
+ * eigenvectors, e.g. the previous K-block's U at the same k index, which is
+ * one grid step away).  For nw + (nw & 1) in {16, 32} the FP32 Jacobi runs on
+ * U0^+ H U0 with the rotations accumulated onto U0 (about half the sweeps);
+ * other sizes start from the identity.  Any unitary U0 gives the same
+ * eigen-decomposition.  U0_dev may equal U_dev (each k-point reads its guess
+ * before it writes); U0_dev = NULL is wbmi_eigh.                           */
+int wbmi_eigh_guess(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride, double* E_dev,
+                    void* U_dev, int prec, int refine, const void* H64_dev, const void* U0_dev, void* work_dev,
+                    size_t work_bytes);
+
 /* ---- gauge rotation U^+ X U -----------------------------------------------
  * Replaces Data_K._rotate (data_K/data_K.py:130-132) for the channels of a
  * per-k stack X[k, nch, nw, nw] (in place).  herm_host[c] = 1 forces the

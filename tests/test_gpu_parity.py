@@ -294,6 +294,49 @@ def test_fe_scan_stencils(wb, results_run, fder):
 
 # -- synthetic model against the oracle ------------------------------------------
 
+@pytest.mark.parametrize("nw", [16, 18, 31, 32])
+@pytest.mark.parametrize("prec", [1, 0])
+def test_eigh_guess(wb, ctx, nw, prec):
+    """Warm-started eigensolver: from the eigenvectors of a neighbouring matrix, from a
+    random unitary basis, and in place (U0 is U), the result meets the bars of the
+    cold start (nw = 18 ignores the guess)."""
+    import torch
+    say(f"test_eigh_guess nw={nw} prec={prec}")
+    rng = np.random.default_rng(300 + nw)
+    nk = 24
+    H = rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw))
+    H = 3 * (H + H.conj().swapaxes(1, 2))
+    D = rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw))
+    Hn = H + 0.05 * (D + D.conj().swapaxes(1, 2))          # the neighbour
+    cdt = torch.complex64 if prec == 1 else torch.complex128
+    Eref = np.linalg.eigvalsh(H)
+    scale = np.abs(Eref).max()
+    guesses = {"neighbour": np.linalg.eigh(Hn)[1],
+               "random": np.linalg.qr(rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw)))[0]}
+    Hd = torch.from_numpy(H).cuda()
+    for name, G in list(guesses.items()) + [("in place", None)]:
+        E = torch.empty((nk, nw), dtype=torch.float64, device="cuda")
+        if G is None:
+            U = torch.from_numpy(guesses["neighbour"]).to(cdt).cuda()
+            U0 = U
+        else:
+            U = torch.empty((nk, nw, nw), dtype=cdt, device="cuda")
+            U0 = torch.from_numpy(G).to(cdt).cuda()
+        if prec == 1:
+            ctx.eigh(None, nw * nw, E, U, 1, nw, refine=16, H64=Hd, U0=U0)
+        else:
+            ctx.eigh(Hd, nw * nw, E, U, 0, nw, refine=16, U0=U0)
+        Eg = E.cpu().numpy()
+        Ug = U.cpu().numpy().astype(np.complex128)
+        err = np.abs(Eg - Eref).max() / scale
+        res = np.abs(np.einsum("kab,kbc->kac", H, Ug) - Ug * Eg[:, None, :]).max() / scale
+        orth = np.abs(np.einsum("kji,kjl->kil", Ug.conj(), Ug) - np.eye(nw)).max()
+        say(f"  {name}: E {err:.2e} residual {res:.2e} orthogonality {orth:.2e}")
+        assert err < 1e-13, name
+        assert res < (1e-6 if prec == 1 else 1e-13), name
+        assert orth < (1e-6 if prec == 1 else 1e-13), name
+
+
 @pytest.fixture(scope="module")
 def small_synth(wb):
     from wannierberri_amd import models

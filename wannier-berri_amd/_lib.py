@@ -36,7 +36,7 @@ SYMBOLS = [
     "wbmi_version", "wbmi_ctx_create", "wbmi_ctx_destroy", "wbmi_ctx_set_stream", "wbmi_sync",
     "wbmi_last_error", "wbmi_build_channels", "wbmi_r2k_plan_create", "wbmi_r2k_plan_destroy",
     "wbmi_r2k_plan_sizes", "wbmi_r2k_box", "wbmi_r2k_exec", "wbmi_r2k_klist", "wbmi_eigh",
-    "wbmi_eigh_work_bytes", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
+    "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
 ]
 
 
@@ -78,6 +78,7 @@ _PROTOS = {
     "wbmi_r2k_klist": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _vp, _i]),
     "wbmi_eigh": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp, _vp, ctypes.c_size_t]),
     "wbmi_eigh_work_bytes": (ctypes.c_size_t, [_i, _i, _i, _i]),
+    "wbmi_eigh_guess": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp, _vp, _vp, ctypes.c_size_t]),
     "wbmi_rotate": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, ctypes.c_size_t]),
     "wbmi_rotate_work_bytes": (ctypes.c_size_t, [_i, _i, _i]),
     "wbmi_static_scan": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _d, _vp]),
@@ -194,13 +195,21 @@ class Context:
             self._work = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
         return self._work
 
-    def eigh(self, H, Hstride, E, U, prec, nw, refine=0, H64=None):
-        """Batched eigh; refine=n >= 1: FP32 Jacobi of H64 + n FP64 Ogita-Aishima steps (prec must be F32)."""
+    def eigh(self, H, Hstride, E, U, prec, nw, refine=0, H64=None, U0=None):
+        """Batched eigh; refine=n >= 1: FP32 Jacobi + up to n FP64 Ogita-Aishima steps.
+        U0: optional starting eigenvectors (same shape and dtype as U; may be U itself)."""
         nk = E.shape[0]
         wbytes = int(lib().wbmi_eigh_work_bytes(nk, nw, prec, int(refine)))
         work = self._workspace(wbytes)
-        self.check(self._timed("eigh", (nk, nw), lib().wbmi_eigh, self.h, nk, nw, _ptr(H), Hstride,
-                               _ptr(E), _ptr(U), prec, int(refine), _ptr(H64), _ptr(work), wbytes), "wbmi_eigh")
+        if U0 is None:
+            self.check(self._timed("eigh", (nk, nw), lib().wbmi_eigh, self.h, nk, nw, _ptr(H), Hstride,
+                                   _ptr(E), _ptr(U), prec, int(refine), _ptr(H64), _ptr(work), wbytes), "wbmi_eigh")
+        else:
+            if tuple(U0.shape) != tuple(U.shape) or U0.dtype != U.dtype:
+                raise ValueError("eigh: U0 must have the shape and dtype of U")
+            self.check(self._timed("eigh", (nk, nw), lib().wbmi_eigh_guess, self.h, nk, nw, _ptr(H), Hstride,
+                                   _ptr(E), _ptr(U), prec, int(refine), _ptr(H64), _ptr(U0), _ptr(work), wbytes),
+                       "wbmi_eigh_guess")
 
     def rotate(self, U, X, herm, skip, prec):
         nk, nch, nw = X.shape[0], X.shape[1], X.shape[2]

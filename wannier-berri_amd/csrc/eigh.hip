@@ -208,10 +208,16 @@ __host__ __device__ constexpr size_t jacobi_c_lds() {
     return 2 * sizeof(cplx<float>) * NT * (NT + 1) + sizeof(float4) * (NT / 2) + 16;
 }
 
+// U0 (optional, NT <= 32): approximate eigenvectors (columns), e.g. those of
+// the same k-point index in the neighbouring K-block.  The sweeps then run on
+// A = U0^+ H U0 (both products on the FP64 matrix cores, stored in FP32) with
+// the rotations accumulated onto V = U0: a nearly diagonal start needs about
+// three active sweeps instead of six or seven.  Any unitary U0 gives the same
+// eigen-decomposition; a poor one only costs sweeps.
 template <int NT, typename Tin, typename TO>
 __global__ void __launch_bounds__(64) k_eigh_jacobi_c(int nw, const cplx<Tin>* __restrict__ H, int64_t Hstride,
-                                                      double* __restrict__ Eout, cplx<TO>* __restrict__ Uout,
-                                                      int max_sweeps) {
+                                                      double* __restrict__ Eout, cplx<TO>* Uout,
+                                                      int max_sweeps, const cplx<TO>* U0) {
     typedef float T;
     constexpr int n = NT, ld = NT + 1, np = NT / 2;
     constexpr int LPR = 64 / NT;          // pair stride of a lane
@@ -227,17 +233,70 @@ __global__ void __launch_bounds__(64) k_eigh_jacobi_c(int nw, const cplx<Tin>* _
     const int64_t k = blockIdx.x;
     const cplx<Tin>* Hk = H + k * Hstride;
 
-    for (int i = tid; i < n; i += 64)
-        for (int j = 0; j < n; ++j) {
-            cplx<T> a = mk((T)0, (T)0);
-            if (i < nw && j < nw) {
-                cplx<Tin> x = Hk[i * nw + j], y = Hk[j * nw + i];
-                a = mk((T)(0.5 * (x.re + y.re)), (T)(0.5 * (x.im - y.im)));
+    bool warm = false;
+    if constexpr (NT <= 32) warm = U0 != nullptr;
+    if (!warm) {
+        for (int i = tid; i < n; i += 64)
+            for (int j = 0; j < n; ++j) {
+                cplx<T> a = mk((T)0, (T)0);
+                if (i < nw && j < nw) {
+                    cplx<Tin> x = Hk[i * nw + j], y = Hk[j * nw + i];
+                    a = mk((T)(0.5 * (x.re + y.re)), (T)(0.5 * (x.im - y.im)));
+                }
+                A[i * ld + j] = a;
+                V[i * ld + j] = mk((T)(i == j ? 1 : 0), (T)0);
             }
-            A[i * ld + j] = a;
-            V[i * ld + j] = mk((T)(i == j ? 1 : 0), (T)0);
+        __syncthreads();
+    } else if constexpr (NT <= 32) {
+        // V = U0 (identity on the padding index of odd nw)
+        const cplx<TO>* U0k = U0 + k * nw * nw;
+        for (int t = tid; t < n * n; t += 64) {
+            const int i = t / n, j = t % n;
+            cplx<T> v = mk((T)(i == j ? 1 : 0), (T)0);
+            if (i < nw && j < nw) {
+                const cplx<TO> u = U0k[i * nw + j];
+                v = mk((T)u.re, (T)u.im);
+            }
+            V[i * ld + j] = v;
         }
-    __syncthreads();
+        __syncthreads();
+        // T = H U0 into A, then A = U0^+ T (one 32 x 32 block per product)
+        MfmaBlock C;
+        C.zero();
+        mfma_block(
+            C, n, n, n, 0, 0,
+            [&](int r, int m) {
+                if (r >= nw || m >= nw) return mk(0.0, 0.0);
+                const cplx<Tin> x = Hk[r * nw + m], y = Hk[m * nw + r];
+                return mk(0.5 * ((double)x.re + (double)y.re), 0.5 * ((double)x.im - (double)y.im));
+            },
+            [&](int m, int c) { return mk((double)V[m * ld + c].re, (double)V[m * ld + c].im); });
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mfma_row(bi, v), c = mfma_col(bj);
+                    if (r < n && c < n) A[r * ld + c] = mk((T)C.re[bi][bj][v], (T)C.im[bi][bj][v]);
+                }
+        __syncthreads();
+        C.zero();
+        mfma_block(
+            C, n, n, n, 0, 0, [&](int r, int m) { return mk((double)V[m * ld + r].re, -(double)V[m * ld + r].im); },
+            [&](int m, int c) { return mk((double)A[m * ld + c].re, (double)A[m * ld + c].im); });
+        __syncthreads();  // every read of T is done
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = mfma_row(bi, v), c = mfma_col(bj);
+                    if (r < n && c < n) A[r * ld + c] = mk((T)C.re[bi][bj][v], (T)C.im[bi][bj][v]);
+                }
+        __syncthreads();
+    }
 
     const int lr = tid % NT;
     const int pr0 = tid / NT;
@@ -1007,7 +1066,7 @@ extern "C" size_t wbmi_eigh_work_bytes(int nk, int nw, int prec, int refine) {
 // FP32 Jacobi of the FP64 Hamiltonian (eigenvectors stored as TU)
 template <typename TU>
 static int jacobi_f32_of_f64(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* H64, int64_t Hstride, double* E,
-                             cplx<TU>* U) {
+                             cplx<TU>* U, const cplx<TU>* U0 = nullptr) {
     const int n = nw + (nw & 1);
     const int sweeps = 40;
     if (n == 16 || n == 32 || n == 64) {
@@ -1018,13 +1077,13 @@ static int jacobi_f32_of_f64(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* 
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
         if (n == 16)
             hipLaunchKernelGGL((k_eigh_jacobi_c<16, double, TU>), dim3(nk), dim3(64), lds, ctx->stream, nw, H64,
-                               Hstride, E, U, sweeps);
+                               Hstride, E, U, sweeps, U0);
         else if (n == 32)
             hipLaunchKernelGGL((k_eigh_jacobi_c<32, double, TU>), dim3(nk), dim3(64), lds, ctx->stream, nw, H64,
-                               Hstride, E, U, sweeps);
+                               Hstride, E, U, sweeps, U0);
         else
             hipLaunchKernelGGL((k_eigh_jacobi_c<64, double, TU>), dim3(nk), dim3(64), lds, ctx->stream, nw, H64,
-                               Hstride, E, U, sweeps);
+                               Hstride, E, U, sweeps, (const cplx<TU>*)nullptr);
     } else if (n <= 64) {
         const size_t lds = 2 * sizeof(cplx<float>) * n * (n + 1) + 3 * sizeof(float) * (n / 2) + 16;
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_eigh_jacobi<float, double, TU>,
@@ -1049,9 +1108,9 @@ static int jacobi_f32_of_f64(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* 
 // FP32 Jacobi of the FP64 Hamiltonian, then FP64 Ogita-Aishima steps; eigenvectors stored as TU
 template <typename TU>
 static int eigh_refined(wbmi_ctx* ctx, int nk, int nw, const cplx<double>* H64, int64_t Hstride, double* E,
-                        cplx<TU>* U, int steps, void* work, size_t work_bytes) {
+                        cplx<TU>* U, int steps, void* work, size_t work_bytes, const cplx<TU>* U0 = nullptr) {
     const double tol = sizeof(TU) == 4 ? OA_TOL_C64 : OA_TOL_C128;
-    int rc = jacobi_f32_of_f64<TU>(ctx, nk, nw, H64, Hstride, E, U);
+    int rc = jacobi_f32_of_f64<TU>(ctx, nk, nw, H64, Hstride, E, U, U0);
     if (rc) return rc;
     if (nw > 44) {
         if (!work || work_bytes < eigh_work_bytes(nk, nw, steps))
@@ -1108,4 +1167,23 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
     }
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
+}
+
+extern "C" int wbmi_eigh_guess(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride, double* E_dev,
+                               void* U_dev, int prec, int refine, const void* H64_dev, const void* U0_dev,
+                               void* work_dev, size_t work_bytes) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (!U0_dev)
+        return wbmi_eigh(ctx, nk, nw, H_dev, Hstride, E_dev, U_dev, prec, refine, H64_dev, work_dev, work_bytes);
+    if (nk <= 0 || nw <= 0 || nw > 128 || !E_dev || !U_dev || Hstride < (int64_t)nw * nw)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh_guess: bad arguments (nw <= 128, Hstride >= nw*nw)");
+    if (refine < 1 || refine > 16) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh_guess: refine must be 1..16");
+    if (prec != WBMI_F64 && prec != WBMI_F32) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh_guess: bad precision");
+    const void* H64 = prec == WBMI_F64 ? H_dev : H64_dev;
+    if (!H64) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_eigh_guess: refinement needs the FP64 Hamiltonian");
+    if (prec == WBMI_F64)
+        return eigh_refined<double>(ctx, nk, nw, (const cplx<double>*)H64, Hstride, E_dev, (cplx<double>*)U_dev,
+                                    refine, work_dev, work_bytes, (const cplx<double>*)U0_dev);
+    return eigh_refined<float>(ctx, nk, nw, (const cplx<double>*)H64, Hstride, E_dev, (cplx<float>*)U_dev, refine,
+                               work_dev, work_bytes, (const cplx<float>*)U0_dev);
 }

@@ -91,6 +91,7 @@ class DeviceSystem:
         self.iRvec_dev = torch.from_numpy(self.iRvec).to(dev)
         self._stacks = {}
         self._plans = {}
+        self.eig_hint = None   # the last grid K-block's eigenvectors (starting basis of the next)
 
     def has(self, name):
         return name in self.base_offset or (name == "rotAA" and "AA" in self.base_offset)
@@ -243,14 +244,22 @@ class Data_K:
         E = torch.empty((nk, nw), dtype=torch.float64, device=self.ctx.device)
         U = torch.empty((nk, nw, nw), dtype=self.cdtype, device=self.ctx.device)
         X = self._transform(main, self.cdtype, self.prec) if main else None
+        # starting eigenvectors: the previous K-block's at the same k index (one grid step
+        # away on the NK grid), when it was a grid block of the same shape and form
+        U0 = None
+        hint = getattr(self.dev, "eig_hint", None)
+        if self.k_list is None and hint is not None and hint.shape == U.shape and hint.dtype == U.dtype:
+            U0 = hint
         if mixed:
             H64 = self._transform([("Ham", 0)], torch.complex128, WBMI_F64)
-            self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=OA_STEPS, H64=H64)
+            self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=OA_STEPS, H64=H64, U0=U0)
             del H64
         else:
             # FP32 Jacobi + FP64 refinement to complex128 vectors
             h0 = slots[("Ham", 0)]
-            self.ctx.eigh(X[:, h0], nch * nw * nw, E, U, self.prec, nw, refine=OA_STEPS)
+            self.ctx.eigh(X[:, h0], nch * nw * nw, E, U, self.prec, nw, refine=OA_STEPS, U0=U0)
+        if self.k_list is None:
+            self.dev.eig_hint = U
         if X is not None:
             herm = np.zeros(nch, dtype=np.int32)
             skip = np.zeros(nch, dtype=np.int32)
