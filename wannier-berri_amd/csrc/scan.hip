@@ -135,7 +135,7 @@ __device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int 
 
 // Element access to the channels of one k-point.
 // LM: dEinv o Off read from LDS (nw <= SC_M_NW) or formed from E and the labels.
-// ST: V and A read from their LDS copies (formulas with wave tasks, nw <= SC_LDS_NW;
+// ST: V and A read from their LDS copies (DerOmega, DerMorb and Omega at nw <= SC_LDS_NW;
 // A only with external terms, the only case that reads it) or from the channels.
 // Both are compile-time so that every access is a plain LDS or global load.
 template <typename T, bool LM, bool ST>
@@ -191,22 +191,29 @@ struct KCtx {
 };
 
 // pre-Hermitian Omega summand s^c_xy for x, y in the same group
-// (formula/covariant.py:175-200): Omega.nn = s + s^+
+// (formula/covariant.py:175-200): Omega.nn = s + s^+.  The sum over the
+// off-group bands l runs over [l0, l1); `tail` adds the within-group and O terms.
 template <typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal, bool external) {
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal,
+                                                              bool external, int l0, int l1, bool tail) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
-    for (int l = 0; l < K.nw; ++l) {
+    for (int l = l0; l < l1; ++l) {
         if (K.lab[l] == K.lab[x]) continue;
         cplx<double> dxa = K.Dp(al, x, l);
         if (internal) s += times_i(-(dxa * K.Dp(be, l, y)));
         if (external) s += K.Dp(be, x, l) * K.A(al, l, y) - dxa * K.A(be, l, y);
     }
-    if (external) {
+    if (external && tail) {
         for (int m = K.gs[x]; m < K.ge[x]; ++m) s += times_i(-(K.A(al, x, m) * K.A(be, m, y)));
         s += 0.5 * K.O(c, x, y);
     }
     return s;
+}
+template <typename T, bool LM, bool ST>
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal,
+                                                              bool external) {
+    return omega_s(K, c, x, y, internal, external, 0, K.nw, true);
 }
 
 // pre-Hermitian Morb_H summand h^c_xy for x, y in the same group
@@ -717,10 +724,30 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
         for (int n = tid; n < nw; n += SC_THREADS)
             if (S.act[n]) S.acc[n * 9] = mk(1.0, 0.0);
     } else if (F == WBMI_FORMULA_OMEGA) {
-        for (int t = tid; t < nw * 3; t += SC_THREADS) {
-            int n = t / 3, c = t % 3;
-            if (!S.act[n]) continue;
-            S.acc[n * 9 + c] = mk(2.0 * omega_s(K, c, n, n, in, ex).re, 0.0);
+        if constexpr (ST) {
+            // V and A in LDS: four adjacent lanes share (n, c) and split the sum over l,
+            // reduced by two shuffles (a uniform loop, so every lane reaches them)
+            const int L4 = (nw + 3) / 4;
+            for (int t0 = 0; t0 < nw * 12; t0 += SC_THREADS) {
+                const int t = t0 + tid;
+                const bool ok = t < nw * 12;
+                const int n = ok ? t / 12 : 0, c = (t / 4) % 3, qq = t % 4;
+                const bool act = ok && S.act[n];
+                double v = 0.0;
+                if (act) {
+                    const int l0 = qq * L4, l1 = l0 + L4 < nw ? l0 + L4 : nw;
+                    v = omega_s(K, c, n, n, in, ex, l0, l1, qq == 0).re;
+                }
+                v += __shfl_xor(v, 1);
+                v += __shfl_xor(v, 2);
+                if (act && qq == 0) S.acc[n * 9 + c] = mk(2.0 * v, 0.0);
+            }
+        } else {
+            for (int t = tid; t < nw * 3; t += SC_THREADS) {
+                int n = t / 3, c = t % 3;
+                if (!S.act[n]) continue;
+                S.acc[n * 9 + c] = mk(2.0 * omega_s(K, c, n, n, in, ex).re, 0.0);
+            }
         }
     } else if (F == WBMI_FORMULA_MORB_HPM) {
         for (int t = tid; t < nw * 3; t += SC_THREADS) {
@@ -788,7 +815,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const int64_t k = blockIdx.x;
     const int tid = threadIdx.x;
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
-    static_assert(tasks || !ST, "V and A are staged only for the formulas with wave tasks");
+    static_assert(tasks || F == WBMI_FORMULA_OMEGA || !ST, "V and A are staged for DerOmega, DerMorb and Omega");
     constexpr bool stage_v = ST;
     const bool stage_a = ST && p.external_terms;
     const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tasks);
@@ -1014,8 +1041,8 @@ static int launch_scan_m(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
 // V and A in LDS up to SC_LDS_NW for the formulas with wave tasks
 template <typename T, int F>
 static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
-    if constexpr (tasks)
+    constexpr bool staged = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB || F == WBMI_FORMULA_OMEGA;
+    if constexpr (staged)
         if (a.stage_va) return launch_scan_m<T, F, true, true>(ctx, nk, a, E, X, hist);
     if (a.nw <= SC_M_NW) return launch_scan_m<T, F, true, false>(ctx, nk, a, E, X, hist);
     return launch_scan_m<T, F, false, false>(ctx, nk, a, E, X, hist);
