@@ -79,6 +79,21 @@ typedef struct wbmi_scan_params {
     int ch[WBMI_NROLES];
 } wbmi_scan_params;
 
+/* Parameters of the Kubo optical conductivity (calculators/dynamic.py:26-196). */
+typedef struct wbmi_optcond_params {
+    int nEF, nomega;
+    const double* Efermi; /* device [nEF]                                     */
+    const double* omega;  /* device [nomega]                                  */
+    double EFmin, EFmax;  /* min / max of Efermi (occupation window)          */
+    double kBT;           /* 0: step occupations (utility.py:172-182)         */
+    double eta;           /* smr_fixed_width                                  */
+    int smr_gaussian;     /* 1: Gaussian, 0: Lorentzian smearing              */
+    double degen_thresh;
+    int degen_Kramers;
+    int external_terms;   /* A = i D_H + Abar (data_K.py:329-334)             */
+    int ch[WBMI_NROLES];  /* channel slots: WBMI_ROLE_V (and WBMI_ROLE_A)     */
+} wbmi_optcond_params;
+
 /* ---- context ------------------------------------------------------------ */
 int wbmi_ctx_create(int device, void* stream, wbmi_ctx** out);
 int wbmi_ctx_destroy(wbmi_ctx* ctx);
@@ -165,6 +180,17 @@ size_t wbmi_eigh_work_bytes(int nk, int nw, int prec, int refine);
 int wbmi_eigh_guess(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride, double* E_dev,
                     void* U_dev, int prec, int refine, const void* H64_dev, const void* U0_dev, void* work_dev,
                     size_t work_bytes);
+
+/* ---- Kubo optical conductivity ---------------------------------------------
+ * Replaces OpticalConductivity.__call__ (calculators/dynamic.py:26-196 with
+ * Formula_OptCond 170-181) on one K-block: for every pair (m, n) of band
+ * groups, M^{ab} = sum_{i in m, j in n} i A^a_ij A^b_ji, A = i D_H (+ Abar),
+ * added as (f(E_n) - f(E_m)) (E_n - E_m) / (E_n - E_m - w - i eta) M^{ab}
+ * (Gaussian: imaginary part pi G) into out_dev [nEF, nomega, 3, 3] complex128
+ * (interleaved doubles), accumulated across calls; the caller scales by
+ * factor_opt / (nk V).  nw <= 32.                                          */
+int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                              int prec, const wbmi_optcond_params* params, double* out_dev);
 
 /* ---- gauge rotation U^+ X U -----------------------------------------------
  * Replaces Data_K._rotate (data_K/data_K.py:130-132) for the channels of a

@@ -966,6 +966,94 @@ class GME_orb_FermiSurf(StaticCalculator):
 # grid and driver
 # ---------------------------------------------------------------------------
 
+# ---------------------------------------------------------------------------
+# dynamic calculators (wannierberri/calculators/dynamic.py)
+# ---------------------------------------------------------------------------
+
+FACTOR_OPT = -FACTOR_AHC   # factors.py:36
+
+
+def fermi_dirac(E, mu, kBT):
+    """Occupation of level E for the Fermi levels mu (utility.py:172-182)."""
+    if kBT == 0:
+        return 1.0 * (E <= mu)
+    res = np.zeros(mu.shape)
+    res[mu > E + 30 * kBT] = 1.0
+    sel = np.abs(mu - E) <= 30 * kBT
+    res[sel] = 1.0 / (np.exp((E - mu[sel]) / kBT) + 1)
+    return res
+
+
+def gaussian(x, width):
+    """1/(sqrt(pi) w) exp(-(x/w)^2), zero beyond |x| = w sqrt(200) (utility.py:151-168)."""
+    out = np.zeros(np.shape(x))
+    sel = np.abs(x) < width * np.sqrt(200.0)
+    out[sel] = 1.0 / (np.sqrt(np.pi) * width) * np.exp(-(np.asarray(x)[sel] / width) ** 2)
+    return out
+
+
+def lorentzian(x, width):
+    """utility.py:147-148."""
+    return 1.0 / (np.pi * width) * width ** 2 / (x ** 2 + width ** 2)
+
+
+class OpticalConductivity:
+    """Kubo interband optical conductivity (dynamic.py:26-196, Formula_OptCond 170-181).
+
+    Per k, every pair (m, n) of band groups (all groups, degen_thresh) with
+    some occupation change in the Fermi range contributes
+    M^{ab} = sum_{i in m, j in n} i A^a_ij A^b_ji, A = i D_H (+ Abar with
+    external terms), times (f(E_n) - f(E_m)) per Fermi level and
+    (E_n - E_m) / (E_n - E_m - omega - i eta) per frequency (Gaussian
+    smearing replaces the imaginary part by pi G(E_n - E_m - omega)).
+    Result [nEF, nomega, 3, 3] complex, x factor_opt / (nk V).
+    """
+
+    def __init__(self, Efermi, omega, kBT=0, smr_fixed_width=0.1, smr_type="Lorentzian", degen_thresh=1e-4,
+                 degen_Kramers=False, kwargs_formula=None):
+        self.Efermi = np.array(Efermi, dtype=float).reshape(-1)
+        self.omega = np.array(omega, dtype=float)
+        self.kBT = kBT
+        self.eta = smr_fixed_width
+        self.smr_type = smr_type
+        self.degen_thresh = degen_thresh
+        self.degen_Kramers = degen_Kramers
+        self.kwargs_formula = dict(kwargs_formula or {})
+        self.eocc1max = self.Efermi.min() - 30 * kBT
+        self.eocc0min = self.Efermi.max() + 30 * kBT
+
+    def nonzero(self, E1, E2):
+        return not ((E1 < self.eocc1max and E2 < self.eocc1max) or (E1 > self.eocc0min and E2 > self.eocc0min))
+
+    def factor_omega(self, E1, E2):
+        arg = E2 - E1 - self.omega
+        cfac = 1. / (arg - 1j * self.eta)
+        if self.smr_type != "Lorentzian":
+            cfac.imag = np.pi * gaussian(arg, self.eta)
+        return (E2 - E1) * cfac
+
+    def __call__(self, data):
+        ext = self.kwargs_formula.get("external_terms", True) and not data.system.force_internal_terms_only
+        A = 1j * data.D_H
+        if ext:
+            A = A + data.Xbar("AA")
+        AA = 1j * A[:, :, :, :, None] * A.swapaxes(1, 2)[:, :, :, None, :]
+        restot = np.zeros((len(self.omega), len(self.Efermi) * 9), dtype=complex)
+        for ik in range(data.nk):
+            groups = data.get_bands_in_range_groups_ik(ik, -np.inf, np.inf, degen_thresh=self.degen_thresh,
+                                                       degen_Kramers=self.degen_Kramers)
+            pairs = [(m, n, Em, En) for m, Em in groups.items() for n, En in groups.items() if self.nonzero(Em, En)]
+            if not pairs:
+                continue
+            M = np.array([AA[ik, p[0][0]:p[0][1]].sum(axis=0)[p[1][0]:p[1][1]].sum(axis=0) for p in pairs])
+            fE = np.array([fermi_dirac(p[3], self.Efermi, self.kBT) - fermi_dirac(p[2], self.Efermi, self.kBT)
+                           for p in pairs])
+            fw = np.array([self.factor_omega(p[2], p[3]) for p in pairs]).T
+            restot += fw @ (fE[:, :, None] * M.reshape(len(pairs), -1)[:, None, :]).reshape(len(pairs), -1)
+        restot = restot.reshape((len(self.omega), len(self.Efermi), 3, 3)).swapaxes(0, 1)
+        return OracleResult(self.Efermi, restot * FACTOR_OPT / (data.nk * data.system.cell_volume))
+
+
 def grid_blocks(NK=None, NKFFT=None, NKdiv=None, periodic=(True, True, True)):
     """NKdiv x NKFFT decomposition and K-block shifts (grid/grid.py:149-189,215-266).
 

@@ -75,6 +75,8 @@ def main():
              "haldane_adpt3_ahc_iter0": I + "Haldane-ahc-PythTB-adpt3-sym_iter-0000.npz",
              "haldane_adpt3_ahc_iter1": I + "Haldane-ahc-PythTB-adpt3-sym_iter-0001.npz",
              "haldane_adpt3_dos_iter1": I + "Haldane-dos-PythTB-adpt3-sym_iter-0001.npz",
+             # Kubo optical conductivity, Chiral model (tests/test_run.py:183-199,799-832)
+             "chiral_opt_conductivity": I + "Chiral-opt_conductivity_iter-0000.npz",
              }, "results_run.npz")
     # per-formula gauge-invariant traces (tests/reference/formula/*.npz)
     forms = {}

@@ -274,6 +274,34 @@ def test_haldane_200x200_chern(wb, precision, tol):
     assert np.abs(d[:, :2]).max() < tol * E2_H_ANGSTROM
 
 
+def test_chiral_optical_conductivity(wb, results_run):
+    """Kubo optical conductivity against the reference's fixture (Chiral model, Gaussian
+    smearing 0.2 eV, 27 Fermi levels x 11 frequencies; tests/test_run.py:183-199,799-832)."""
+    say("test_chiral_optical_conductivity")
+    s = gsys(wb, "chiral_left")
+    calc = wb.calculators.dynamic.OpticalConductivity(Efermi=np.linspace(-5, 8, 27), omega=np.linspace(0, 1., 11),
+                                                      smr_fixed_width=0.20, smr_type="Gaussian")
+    r = run_gpu(wb, s, {"oc": calc}, [10, 10, 4], [5, 5, 2])
+    assert r.results["oc"].data.shape == (27, 11, 3, 3)
+    assert rel_err(r.results["oc"].data, results_run["chiral_opt_conductivity"]) < 1e-8
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-6)])
+def test_synthetic_optical_conductivity(wb, small_synth, precision, tol):
+    """External terms (A = i D_H + Abar), Lorentzian smearing, finite temperature, vs the oracle."""
+    say(f"test_synthetic_optical_conductivity {precision}")
+    s, o = small_synth
+    Ef = np.linspace(-4, 4, 9)
+    om = np.linspace(0.1, 3, 7)
+    kw = dict(omega=om, smr_fixed_width=0.1, smr_type="Lorentzian", kBT=0.05)
+    calc = wb.calculators.dynamic.OpticalConductivity(Efermi=Ef, **kw)
+    r = run_gpu(wb, s, {"oc": calc}, [4, 4, 4], [2, 2, 2], precision=precision)
+    ro = O.run(o, {"oc": O.OpticalConductivity(Ef, **kw)}, NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    e = rel_err(r.results["oc"].data, ro["oc"].data)
+    say(f"  rel err {e:.2e}")
+    assert e < tol
+
+
 @pytest.mark.parametrize("fder", [0, 1, 2, 3])
 def test_fe_scan_stencils(wb, results_run, fder):
     say(f"test_fe_scan_stencils fder={fder}")

@@ -232,3 +232,13 @@ def test_omega_sum_rule_and_bcd_trace():
         for n in range(nb):
             t = dom.trace(ik, [n], [m for m in range(nb) if m != n])
             assert abs(np.trace(t)) < 1e-9 * max(1, np.abs(t).max())
+
+
+def test_chiral_optical_conductivity(results_run):
+    """Kubo optical conductivity (dynamic.py:26-196) on the Chiral model, Gaussian
+    smearing 0.2 eV, 27 Fermi levels x 11 frequencies (tests/test_run.py:183-199)."""
+    s = gsys("chiral_left")
+    calc = O.OpticalConductivity(np.linspace(-5, 8, 27), np.linspace(0, 1, 11), smr_fixed_width=0.2,
+                                 smr_type="Gaussian")
+    r = O.run(s, {"oc": calc}, NK=[10, 10, 4], NKFFT=[5, 5, 2])
+    assert rel_err(r["oc"].data, results_run["chiral_opt_conductivity"]) < 1e-8

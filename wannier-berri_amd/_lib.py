@@ -36,7 +36,7 @@ SYMBOLS = [
     "wbmi_version", "wbmi_ctx_create", "wbmi_ctx_destroy", "wbmi_ctx_set_stream", "wbmi_sync",
     "wbmi_last_error", "wbmi_build_channels", "wbmi_r2k_plan_create", "wbmi_r2k_plan_destroy",
     "wbmi_r2k_plan_sizes", "wbmi_r2k_box", "wbmi_r2k_exec", "wbmi_r2k_klist", "wbmi_eigh",
-    "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
+    "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_optical_conductivity", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
 ]
 
 
@@ -55,6 +55,15 @@ class ScanParams(ctypes.Structure):
         ("EFmin", ctypes.c_double), ("EFmax", ctypes.c_double), ("dEF", ctypes.c_double),
         ("degen_thresh", ctypes.c_double), ("degen_Kramers", ctypes.c_int),
         ("ch", ctypes.c_int * NROLES),
+    ]
+
+
+class OptcondParams(ctypes.Structure):
+    _fields_ = [
+        ("nEF", ctypes.c_int), ("nomega", ctypes.c_int), ("Efermi", ctypes.c_void_p), ("omega", ctypes.c_void_p),
+        ("EFmin", ctypes.c_double), ("EFmax", ctypes.c_double), ("kBT", ctypes.c_double), ("eta", ctypes.c_double),
+        ("smr_gaussian", ctypes.c_int), ("degen_thresh", ctypes.c_double), ("degen_Kramers", ctypes.c_int),
+        ("external_terms", ctypes.c_int), ("ch", ctypes.c_int * NROLES),
     ]
 
 
@@ -79,6 +88,7 @@ _PROTOS = {
     "wbmi_eigh": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp, _vp, ctypes.c_size_t]),
     "wbmi_eigh_work_bytes": (ctypes.c_size_t, [_i, _i, _i, _i]),
     "wbmi_eigh_guess": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp, _vp, _vp, ctypes.c_size_t]),
+    "wbmi_optical_conductivity": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(OptcondParams), _vp]),
     "wbmi_rotate": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _vp, _i, _vp, ctypes.c_size_t]),
     "wbmi_rotate_work_bytes": (ctypes.c_size_t, [_i, _i, _i]),
     "wbmi_static_scan": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _d, _vp]),
@@ -227,6 +237,12 @@ class Context:
         self.check(self._timed(f"scan_f{params.formula}", (nk, nw), lib().wbmi_static_scan, self.h, nk, nw, nch,
                                _ptr(E), _ptr(X), prec, ctypes.byref(params), float(weight), _ptr(hist)),
                    "wbmi_static_scan")
+
+    def optical_conductivity(self, E, X, prec, params, out):
+        nk, nw = E.shape
+        self.check(self._timed("optcond", (nk, nw), lib().wbmi_optical_conductivity, self.h, nk, nw, X.shape[1],
+                               _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(out)),
+                   "wbmi_optical_conductivity")
 
     def finalize(self, hist, fder, dEF, scale, out):
         nEF, ncomp = hist.shape

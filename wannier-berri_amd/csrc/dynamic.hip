@@ -1,0 +1,194 @@
+// Kubo interband optical conductivity of libwbmi (calculators/dynamic.py:26-196,
+// Formula_OptCond 170-181).
+//
+// Per k: the generalized Berry connection A^a = i D_H^a (+ Abar^a with external
+// terms), D_H = -V o dEinv (data_K.py:290-334); for every ordered pair (m, n)
+// of band groups (degen_thresh) whose occupations can differ inside the Fermi
+// range, M^{ab} = sum_{i in m, j in n} i A^a_ij A^b_ji, and
+//   out[EF][w][a][b] += (f(E_n) - f(E_m)) (E_n - E_m) / (E_n - E_m - w - i eta) M^{ab}
+// (Gaussian smearing: the imaginary part is pi G(E_n - E_m - w)).
+//
+// Workgroup = (a tile of 256 (EF, w) outputs, one per thread with nine complex
+// accumulators) x (a chunk of k-points).  For each k the workgroup forms A in
+// LDS and walks the group pairs 256 at a time: every thread forms one pair's
+// M into an LDS slot, then every thread adds all slots into its outputs.  The
+// tile is added to `out` once, with FP64 atomics, at the end.
+#include <cmath>
+
+#include "wbmi_common.h"
+
+#define OC_THREADS 256
+#define OC_KCHUNK 16
+#define OC_NWMAX 32
+
+struct OcArgs {
+    wbmi_optcond_params p;
+    int nw, nch;
+};
+
+__device__ inline double fermi_dirac(double E, double mu, double kBT) {
+    if (kBT == 0.0) return E <= mu ? 1.0 : 0.0;
+    if (mu > E + 30.0 * kBT) return 1.0;
+    if (mu < E - 30.0 * kBT) return 0.0;
+    return 1.0 / (exp((E - mu) / kBT) + 1.0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(OC_THREADS) k_optcond(int nk, OcArgs a, const double* __restrict__ Eall,
+                                                        const cplx<T>* __restrict__ Xall, double* __restrict__ out) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const wbmi_optcond_params& p = a.p;
+    const int nw = a.nw, nw2 = nw * nw, tid = threadIdx.x;
+    cplx<double>* sA = (cplx<double>*)smem;  // [3][nw][nw]
+    cplx<double>* sM = sA + 3 * nw2;         // [OC_THREADS][9]
+    double* sEm = (double*)(sM + OC_THREADS * 9);
+    double* sEn = sEm + OC_THREADS;
+    double* sE = sEn + OC_THREADS;
+    double* sGm = sE + OC_NWMAX;
+    int* sOk = (int*)(sGm + OC_NWMAX);
+    int* sGb = sOk + OC_THREADS;
+    int& sNg = sGb[OC_NWMAX + 1];
+    const int o = blockIdx.x * OC_THREADS + tid;  // output index EF * nomega + w
+    const bool has_out = o < p.nEF * p.nomega;
+    const double mu = has_out ? p.Efermi[o / p.nomega] : 0.0;
+    const double w = has_out ? p.omega[o % p.nomega] : 0.0;
+    const double occ_lo = p.EFmin - 30.0 * p.kBT, occ_hi = p.EFmax + 30.0 * p.kBT;
+    cplx<double> acc[9];
+#pragma unroll
+    for (int ab = 0; ab < 9; ++ab) acc[ab] = mk(0.0, 0.0);
+    const int k1 = min(nk, (int)(blockIdx.y + 1) * OC_KCHUNK);
+    for (int k = blockIdx.y * OC_KCHUNK; k < k1; ++k) {
+        for (int i = tid; i < nw; i += OC_THREADS) sE[i] = Eall[(int64_t)k * nw + i];
+        __syncthreads();
+        if (tid == 0) {
+            // band groups (grid/tetrahedron.py:132-136) and their mean energies
+            int nb = 0;
+            sGb[nb++] = 0;
+            for (int i = 1; i < nw; ++i)
+                if (sE[i] - sE[i - 1] > p.degen_thresh) sGb[nb++] = i;
+            sGb[nb++] = nw;
+            if (p.degen_Kramers) {
+                int m = 0;
+                for (int i = 0; i < nb; ++i)
+                    if (sGb[i] % 2 == 0) sGb[m++] = sGb[i];
+                nb = m;
+            }
+            sNg = nb - 1;
+            for (int g = 0; g < nb - 1; ++g) {
+                double s = 0.0;
+                for (int i = sGb[g]; i < sGb[g + 1]; ++i) s += sE[i];
+                sGm[g] = s / (sGb[g + 1] - sGb[g]);
+            }
+        }
+        // A^a_ij = i D_H^a_ij (+ Abar^a_ij), D_H = -V dEinv (0 where |E_i - E_j| < 1e-7)
+        const cplx<T>* Xk = Xall + (int64_t)k * a.nch * nw2;
+        for (int t = tid; t < 3 * nw2; t += OC_THREADS) {
+            const int c = t / nw2, i = (t % nw2) / nw, j = t % nw;
+            const double de = sE[i] - sE[j];
+            const double m = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+            const cplx<T> v = Xk[(int64_t)p.ch[WBMI_ROLE_V + c] * nw2 + i * nw + j];
+            cplx<double> av = mk((double)v.im * m, -(double)v.re * m);  // i * (-m V)
+            if (p.external_terms) {
+                const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + i * nw + j];
+                av.re += (double)x.re;
+                av.im += (double)x.im;
+            }
+            sA[t] = av;
+        }
+        __syncthreads();
+        const int ng = sNg, ncand = ng * ng;
+        for (int c0 = 0; c0 < ncand; c0 += OC_THREADS) {
+            const int cnd = c0 + tid;
+            int ok = 0;
+            if (cnd < ncand) {
+                const int gm = cnd / ng, gn = cnd % ng;
+                const double Em = sGm[gm], En = sGm[gn];
+                ok = !((Em < occ_lo && En < occ_lo) || (Em > occ_hi && En > occ_hi));
+                if (ok) {
+                    sEm[tid] = Em;
+                    sEn[tid] = En;
+                    for (int ab = 0; ab < 9; ++ab) {
+                        const int ca = ab / 3, cb = ab % 3;
+                        cplx<double> s = mk(0.0, 0.0);
+                        for (int i = sGb[gm]; i < sGb[gm + 1]; ++i)
+                            for (int j = sGb[gn]; j < sGb[gn + 1]; ++j)
+                                s += sA[(ca * nw + i) * nw + j] * sA[(cb * nw + j) * nw + i];
+                        sM[tid * 9 + ab] = times_i(s);
+                    }
+                }
+            }
+            sOk[tid] = ok;
+            __syncthreads();
+            const int ns = ncand - c0 < OC_THREADS ? ncand - c0 : OC_THREADS;
+            if (has_out) {
+                for (int s = 0; s < ns; ++s) {
+                    if (!sOk[s]) continue;
+                    const double Em = sEm[s], En = sEn[s];
+                    const double fE = fermi_dirac(En, mu, p.kBT) - fermi_dirac(Em, mu, p.kBT);
+                    if (fE == 0.0) continue;
+                    // (E_n - E_m) / (x - i eta), x = E_n - E_m - w
+                    const double x = En - Em - w;
+                    const double den = 1.0 / (x * x + p.eta * p.eta);
+                    double fr = x * den, fi = p.eta * den;
+                    if (p.smr_gaussian) {
+                        const double y = x / p.eta;
+                        fi = fabs(x) < p.eta * sqrt(200.0) ? M_PI / (sqrt(M_PI) * p.eta) * exp(-y * y) : 0.0;
+                    }
+                    const double sc = (En - Em) * fE;
+                    const cplx<double> f = mk(sc * fr, sc * fi);
+#pragma unroll
+                    for (int ab = 0; ab < 9; ++ab) acc[ab] += f * sM[s * 9 + ab];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (has_out) {
+#pragma unroll
+        for (int ab = 0; ab < 9; ++ab) {
+            unsafeAtomicAdd(out + ((int64_t)o * 9 + ab) * 2, acc[ab].re);
+            unsafeAtomicAdd(out + ((int64_t)o * 9 + ab) * 2 + 1, acc[ab].im);
+        }
+    }
+}
+
+extern "C" int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                                         const void* Xbar_dev, int prec, const wbmi_optcond_params* params,
+                                         double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (nk <= 0 || nw <= 0 || nch <= 0 || !E_dev || !Xbar_dev || !params || !out_dev || !params->Efermi ||
+        !params->omega || params->nEF <= 0 || params->nomega <= 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_optical_conductivity: bad arguments");
+    if (nw > OC_NWMAX)
+        return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_optical_conductivity: nw <= %d", OC_NWMAX);
+    if (!(params->eta > 0) || params->kBT < 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_optical_conductivity: smearing width must be positive");
+    for (int c = 0; c < 3; ++c) {
+        const int v = params->ch[WBMI_ROLE_V + c], x = params->ch[WBMI_ROLE_A + c];
+        if (v < 0 || v >= nch || (params->external_terms && (x < 0 || x >= nch)))
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_optical_conductivity: missing V or A channel");
+    }
+    OcArgs a;
+    a.p = *params;
+    a.nw = nw;
+    a.nch = nch;
+    const int64_t nout = (int64_t)params->nEF * params->nomega;
+    dim3 g((unsigned)ceil_div(nout, OC_THREADS), (unsigned)ceil_div(nk, OC_KCHUNK));
+    const size_t lds = sizeof(cplx<double>) * (3 * (size_t)nw * nw + OC_THREADS * 9) +
+                       sizeof(double) * (2 * OC_THREADS + 2 * OC_NWMAX) + sizeof(int) * (OC_THREADS + OC_NWMAX + 2);
+    if (prec == WBMI_F64) {
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_optcond<double>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_optcond<double>, g, dim3(OC_THREADS), lds, ctx->stream, nk, a, E_dev,
+                           (const cplx<double>*)Xbar_dev, out_dev);
+    } else if (prec == WBMI_F32) {
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_optcond<float>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_optcond<float>, g, dim3(OC_THREADS), lds, ctx->stream, nk, a, E_dev,
+                           (const cplx<float>*)Xbar_dev, out_dev);
+    } else {
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_optical_conductivity: bad precision");
+    }
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}

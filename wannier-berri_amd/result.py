@@ -171,16 +171,21 @@ class EnergyResult(Result):
             np.savez_compressed(f, **self.as_dict())
 
     def savetxt(self, name):
+        """One line per point of the energy grid(s): the energies, then the data and the
+        smoothed data (here the same), complex values as `re im` (energyresult.py:202-225)."""
         data = self.data
+        cpx = np.iscomplexobj(data)
+        frmt = "{0:^31s}" if cpx else "{0:^15s}"
         head = "".join("#### " + s + "\n" for s in self.comment.split("\n"))
         labels = ["".join("xyz"[i] for i in c) for c in np.ndindex(*([3] * self.rank))] if self.rank > 0 else ["value"]
         head += "#" + "    ".join(f"{s:^15s}" for s in self.E_titles) + " " * 8 + \
-            "    ".join(f"{b:^15s}" for b in labels * 2) + "\n"
+            "    ".join(frmt.format(b) for b in labels * 2) + "\n"
         lines = []
-        flat = data.reshape(data.shape[0], -1)
-        for E, row in zip(self.Energies[0], flat):
-            vals = list(row) + list(row)
-            lines.append(f"{E:15.6e}    " + "    ".join(f"{x:15.6e}" for x in vals))
+        for idx in np.ndindex(*data.shape[:self.N_energies]):
+            row = list(data[idx].reshape(-1)) * 2
+            Es = "    ".join(f"{self.Energies[i][j]:15.6e}" for i, j in enumerate(idx))
+            vals = "    ".join((f"{x.real:15.6e} {x.imag:15.6e}" if cpx else f"{x:15.6e}") for x in row)
+            lines.append(Es + "    " + vals)
         with open(name, "w") as f:
             f.write(head + "\n".join(lines))
 
