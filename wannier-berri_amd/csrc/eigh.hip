@@ -599,6 +599,12 @@ __global__ void __launch_bounds__(JROWS_THREADS) k_eigh_jacobi_rows(int nw, cons
 // accurate to ~||E||^2 ||H|| / gap, i.e. to FP64 rounding
 #define OA_TOL_C64 1e-8
 #define OA_TOL_C128 1e-10
+// complex64 vectors: a correction with ||E||_F below this is the last one.  The
+// step converges quadratically, so the vectors it leaves are accurate to about
+// ||E||^2 = 1e-8 (well below complex64 rounding) where the gaps are not small,
+// and the cluster pass that follows rotates every pair still mixed by more than
+// OA_TOL_C64 (close pairs converge more slowly).
+#define OA_LAST_C64 1e-4
 
 // C = X Y (nw x nw, complex FP64) on the matrix cores: 16 x 16 output blocks
 // dealt to the waves of the workgroup, four independent accumulators per block
@@ -827,7 +833,7 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         if ((tid & 63) == 0) atomicAdd(&red[3], e2);
         __syncthreads();
         // every thread reads red[3] here; it is cleared only after the next barrier
-        const bool converged = red[3] < tol * tol;
+        const bool converged = red[3] < tol * tol || (sizeof(TU) == 4 && red[3] < OA_LAST_C64 * OA_LAST_C64);
         // X + X E into sT, which becomes X
         oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sS[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sT[r * ld + c] = sX[r * ld + c] + v; });
@@ -998,7 +1004,7 @@ __global__ void __launch_bounds__(OAG_THREADS) k_eigh_refine_g(int n, const cplx
         block_sum_to(e2, &red[3]);
         __syncthreads();
         // every thread reads red[3] here; it is cleared only after the next barrier
-        const bool converged = red[3] < tol * tol;
+        const bool converged = red[3] < tol * tol || (sizeof(TU) == 4 && red[3] < OA_LAST_C64 * OA_LAST_C64);
         wg_gemm(n, [&](int r, int m) { return X[r * n + m]; }, [&](int m, int c) { return S[m * n + c]; },
                 [&](int r, int c, cplx<double> v) { T[r * n + c] = X[r * n + c] + v; });
         __syncthreads();
