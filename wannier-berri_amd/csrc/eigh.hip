@@ -756,9 +756,9 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int ld = nw + 1;
     cplx<double>* sX = (cplx<double>*)smem;   // X (approximate eigenvectors)
-    cplx<double>* sA = sX + nw * ld;          // A, later R
-    cplx<double>* sT = sA + nw * ld;          // A X, later X + X E
-    cplx<double>* sS = sT + nw * ld;          // S, later E
+    cplx<double>* sA = sX + nw * ld;          // A (for the whole solve)
+    cplx<double>* sT = sA + nw * ld;          // A X, then R, then X + X E
+    cplx<double>* sS = sT + nw * ld;          // S, then E
     double* lam = (double*)(sS + nw * ld);
     double* red = lam + nw;                   // [0] ||S-D||^2, [1] ||R||^2, [2] max|lam|, [3] ||E||^2
     char* cjs = (char*)(red + 4);             // cluster_jacobi scratch
@@ -771,24 +771,24 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         const cplx<TU> u = Uk[t];
         sX[(t / nw) * ld + t % nw] = mk((double)u.re, (double)u.im);
     }
+    oa_load_a(nw, ld, Hk, sA);
     for (int it = 0; it < iters; ++it) {
-        oa_load_a(nw, ld, Hk, sA);
         if (tid < 4) red[tid] = 0.0;
         __syncthreads();
         // T = A X
         oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sT[r * ld + c] = v; });
         __syncthreads();
-        // S = X^+ T ; R = I - X^+ X (into sA)
+        // S = X^+ T ; then R = I - X^+ X into sT
         oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
                     [&](int m, int c) { return sT[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sS[r * ld + c] = v; });
         __syncthreads();
         oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
                     [&](int m, int c) { return sX[m * ld + c]; },
-                    [&](int r, int c, cplx<double> v) { sA[r * ld + c] = mk((r == c ? 1.0 : 0.0) - v.re, -v.im); });
+                    [&](int r, int c, cplx<double> v) { sT[r * ld + c] = mk((r == c ? 1.0 : 0.0) - v.re, -v.im); });
         __syncthreads();
-        for (int i = tid; i < nw; i += OA_THREADS) lam[i] = sS[i * ld + i].re / (1.0 - sA[i * ld + i].re);
+        for (int i = tid; i < nw; i += OA_THREADS) lam[i] = sS[i * ld + i].re / (1.0 - sT[i * ld + i].re);
         __syncthreads();
         // Frobenius norms and max|lam|
         double s2 = 0.0, r2 = 0.0, lm = 0.0;
@@ -797,7 +797,7 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
             cplx<double> sv = sS[i * ld + j];
             if (i == j) sv.re -= lam[i];
             s2 += sv.re * sv.re + sv.im * sv.im;
-            const cplx<double> rv = sA[i * ld + j];
+            const cplx<double> rv = sT[i * ld + j];
             r2 += rv.re * rv.re + rv.im * rv.im;
         }
         for (int i = tid; i < nw; i += OA_THREADS) lm = fmax(lm, fabs(lam[i]));
@@ -818,7 +818,7 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         for (int t = tid; t < nw * nw; t += OA_THREADS) {
             const int i = t / nw, j = t % nw;
             const double dl = lam[j] - lam[i];
-            const cplx<double> r = sA[i * ld + j];
+            const cplx<double> r = sT[i * ld + j];
             cplx<double> e;
             if (fabs(dl) > delta) {
                 const cplx<double> sv = sS[i * ld + j];
@@ -834,7 +834,7 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         __syncthreads();
         // every thread reads red[3] here; it is cleared only after the next barrier
         const bool converged = red[3] < tol * tol || (sizeof(TU) == 4 && red[3] < OA_LAST_C64 * OA_LAST_C64);
-        // X + X E into sT, which becomes X
+        // X + X E into sT (R is no longer read), which becomes X
         oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sS[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sT[r * ld + c] = sX[r * ld + c] + v; });
         __syncthreads();
@@ -849,9 +849,9 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
         // accurate only to ~eps |H| / gap (the correction steps do not preserve orthogonality)
         oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return conj(sX[m * ld + r]); },
                     [&](int m, int c) { return sX[m * ld + c]; },
-                    [&](int r, int c, cplx<double> v) { sA[r * ld + c] = mk(0.5 * ((r == c ? 1.0 : 0.0) - v.re), -0.5 * v.im); });
+                    [&](int r, int c, cplx<double> v) { sS[r * ld + c] = mk(0.5 * ((r == c ? 1.0 : 0.0) - v.re), -0.5 * v.im); });
         __syncthreads();
-        oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sA[m * ld + c]; },
+        oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sX[r * ld + m]; }, [&](int m, int c) { return sS[m * ld + c]; },
                     [&](int r, int c, cplx<double> v) { sT[r * ld + c] = sX[r * ld + c] + v; });
         __syncthreads();
         cplx<double>* t = sX;
@@ -860,8 +860,6 @@ __global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<d
     }
     // S = X^+ A X of the refined columns; pairs left mixed are rotated apart
     const double scale = red[2];  // max |lam| of the last step
-    oa_load_a(nw, ld, Hk, sA);
-    __syncthreads();
     oa_gemm<OA_THREADS>(nw, [&](int r, int m) { return sA[r * ld + m]; }, [&](int m, int c) { return sX[m * ld + c]; },
                 [&](int r, int c, cplx<double> v) { sT[r * ld + c] = v; });
     __syncthreads();
