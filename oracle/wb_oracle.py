@@ -223,9 +223,12 @@ class OracleDataK:
     (arbitrary k-points, no expdK phase: rvectors.py:481-482).
     """
 
-    def __init__(self, system, dK=None, NKFFT=None, k_list=None):
+    def __init__(self, system, dK=None, NKFFT=None, k_list=None, dK_block=None):
         assert (dK is None) != (k_list is None)
         self.system = system
+        # size of the K-block in the full BZ (Kpoint.dK); the tetrahedron corners
+        # lie at +-dK_block / NKFFT / 2 around each k (Kpoint.py:107-109)
+        self.dK_block = None if dK_block is None else np.array(dK_block, dtype=float)
         self.num_wann = system.num_wann
         self.cell_volume = system.cell_volume
         self.force_internal_terms_only = system.force_internal_terms_only
@@ -302,6 +305,24 @@ class OracleDataK:
     @property
     def E_K(self):
         return self._eig[0]
+
+    @cached_property
+    def E_K_corners_parallel(self):
+        """Band energies [nk, 2, 2, 2, nw] at the corners k + ((2i-1), (2j-1), (2l-1)) * dK_fullBZ/2
+        of the parallelepiped around each k (data_K_R.py:126-141): the Hamiltonian with the
+        extra phase exp(2 pi i R.s), on the same grid, then eigvalsh."""
+        assert self.k_list is None and self.dK_block is not None
+        half = self.dK_block / self.NKFFT / 2
+        H_R = self.get_R_mat('Ham')
+        out = np.zeros((self.nk, 2, 2, 2, self.num_wann))
+        for ix in (0, 1):
+            for iy in (0, 1):
+                for iz in (0, 1):
+                    s = np.array([2 * ix - 1, 2 * iy - 1, 2 * iz - 1]) * half
+                    ph = np.exp(2j * np.pi * self.system.iRvec.dot(s))
+                    Hk = r2k_grid(H_R * ph[:, None, None], self.system.iRvec, self.NKFFT, True)
+                    out[:, ix, iy, iz, :] = np.linalg.eigvalsh(Hk)
+        return out
 
     @property
     def UU_K(self):
@@ -381,6 +402,127 @@ def band_groups(E, degen_thresh, degen_Kramers=False):
     if degen_Kramers:
         borders = [i for i in borders if i % 2 == 0]
     return list(zip(borders, borders[1:]))
+
+
+# ---------------------------------------------------------------------------
+# tetrahedron method (wannierberri/grid/tetrahedron.py)
+# ---------------------------------------------------------------------------
+
+TETRA_SPLIT = 1e-12   # minimum separation of the sorted corner energies (tetrahedron.py:17-21)
+
+
+def tetra_occupation(Efermi, corners, der=0):
+    """Occupied volume fraction (der=0) of a tetrahedron whose energy is linear between
+    the corner values `corners` (4), or its der-th derivative in the Fermi level, at each
+    of `Efermi` (grid/tetrahedron.py:15-128).
+
+    The sorted corners e1 <= e2 <= e3 <= e4 are split to at least TETRA_SPLIT.  Below e1
+    the fraction is 0 and from e4 on it is 1 (derivatives 0).  der=0 evaluates the three
+    cubic pieces as products of ratios (30-46): (x-e1)^3 / [(e2-e1)(e3-e1)(e4-e1)] on
+    [e1, e2), the two-term form of the a_ij = (x-e_i)/(e_j-e_i) on [e2, e3), and
+    1 - prod (x-e4)/(e_i-e4) on [e3, e4).  der>=1 differentiates the same cubics in
+    their power-series form c0 + c1 x + c2 x^2 + c3 x^3, with the coefficients grouped
+    as in 48-79 so that the rounding is the reference's.
+    """
+    e = sorted(float(x) for x in corners)
+    for i in range(3):
+        if e[i + 1] - e[i] < TETRA_SPLIT:
+            e[i + 1] = e[i] + TETRA_SPLIT
+    e1, e2, e3, e4 = e
+    x = np.asarray(Efermi, dtype=float)
+    out = np.zeros(x.shape)
+    lo, mid, hi = (x >= e1) & (x < e2), (x >= e2) & (x < e3), (x >= e3) & (x < e4)
+    if der == 0:
+        out[x >= e4] = 1.0
+        t = x[lo]
+        out[lo] = ((t - e1) / (e2 - e1)) * ((t - e1) / (e3 - e1)) * ((t - e1) / (e4 - e1))
+        t = x[mid]
+        a13, a14 = (t - e1) / (e3 - e1), (t - e1) / (e4 - e1)
+        a23, a24 = (t - e2) / (e3 - e2), (t - e2) / (e4 - e2)
+        out[mid] = a23 * a24 + a13 * (a14 * (1 - a24) + a24 * (1 - a23))
+        t = x[hi]
+        out[hi] = 1 - ((t - e4) / (e1 - e4)) * ((t - e4) / (e2 - e4)) * ((t - e4) / (e3 - e4))
+        return out
+    c = tetra_cubics(e1, e2, e3, e4)
+    for sel, (c0, c1, c2, c3) in zip((lo, mid, hi), c):
+        t = x[sel]
+        if der == 1:
+            out[sel] = c1 + t * (2 * c2 + 3 * c3 * t)
+        elif der == 2:
+            out[sel] = 2 * c2 + 6 * c3 * t
+        elif der == 3:
+            out[sel] = 6 * c3
+        else:
+            raise NotImplementedError(der)
+    return out
+
+
+def tetra_cubics(e1, e2, e3, e4):
+    """Power-series coefficients (c0, c1, c2, c3) of the occupied fraction on [e1, e2),
+    [e2, e3), [e3, e4) (tetrahedron.py:48-79).  Pieces 1 and 3 are the expansions of
+    (x-e1)^3/D1 and 1 + (x-e4)^3/D3 with D1 = (e2-e1)(e3-e1)(e4-e1),
+    D3 = (e4-e1)(e4-e2)(e4-e3); piece 2 is the middle cubic over
+    D2 = (e3-e1)(e4-e1)(e3-e2)(e4-e2)."""
+    r1 = 1. / ((e2 - e1) * (e3 - e1) * (e4 - e1))
+    r2 = 1. / ((e3 - e1) * (e4 - e1) * (e3 - e2) * (e4 - e2))
+    r3 = 1. / ((e4 - e1) * (e4 - e2) * (e4 - e3))
+    p1 = (-e1 ** 3 * r1, 3 * e1 ** 2 * r1, -3 * e1 * r1, r1)
+    p3 = (-e4 ** 3 * r3 + 1., 3 * e4 ** 2 * r3, -3 * e4 * r3, r3)
+    p2 = ((e1 ** 2 * (e3 - e2) * (e4 - e2) - (e2 ** 2 * e4) * (e1 - e3) - e1 * e2 * e3 * (e2 - e4)) * r2,
+          (-2 * e1 * ((e3 - e2) * (e4 - e2)) + (2 * e2 * e4 + e2 ** 2) * (e1 - e3)
+           + (e1 * e2 + e2 * e3 + e1 * e3) * (e2 - e4)) * r2,
+          (((e3 - e2) * (e4 - e2)) - (e1 - e3) * (2 * e2 + e4) - (e3 + e1 + e2) * (e2 - e4)) * r2,
+          r2 * (e1 + e2 - e3 - e4))
+    return p1, p2, p3
+
+
+def tetra_band_weight(Efermi, e_center, e_corners, der=0):
+    """Weight of one band at one k: the average over the 12 tetrahedra that join the
+    centre to the two triangles of each face of the parallelepiped (TetraWeightsParal,
+    tetrahedron.py:269-281).  e_corners[ix, iy, iz]; der=-1 gives 1 - occupation."""
+    if der == -1:
+        return 1 - tetra_band_weight(Efermi, e_center, e_corners, 0)
+    occ = np.zeros(np.shape(Efermi))
+    for side in (0, 1):
+        for F in (e_corners[side, :, :], e_corners[:, side, :], e_corners[:, :, side]):
+            occ += tetra_occupation(Efermi, (e_center, F[0, 0], F[0, 1], F[1, 1]), der)
+            occ += tetra_occupation(Efermi, (e_center, F[0, 0], F[1, 0], F[1, 1]), der)
+    return occ / 12.
+
+
+def tetra_group_weights(data_K, Efermi, der, degen_thresh=-1, degen_Kramers=False):
+    """Per k: {(ib1, ib2): weight over Efermi} (TetraWeights.weights_all_band_groups,
+    tetrahedron.py:212-255).  Groups are split on the central energies; a group is kept
+    when its energies over the centre and corners reach [Efermi[0], Efermi[-1]] (140-152);
+    its weight is the mean of its bands' weights.  der=0 adds the group of all bands
+    entirely below Efermi[0] with weight 1 (155-162), der=-1 the group above Efermi[-1]."""
+    Ec = data_K.E_K
+    Ecorn = data_K.E_K_corners_parallel
+    out = []
+    for ik in range(data_K.nk):
+        Eall = np.concatenate((Ec[ik][None], Ecorn[ik].reshape(8, -1)), axis=0)
+        Emin, Emax = Eall.min(axis=0), Eall.max(axis=0)
+        nb = len(Ec[ik])
+        in_range = [(a, b) for a, b in band_groups(Ec[ik], degen_thresh, degen_Kramers)
+                    if Emax[a:b].max() >= Efermi[0] and Emin[a:b].min() <= Efermi[-1]]
+        w = {(a, b): sum(tetra_band_weight(Efermi, Ec[ik, i], Ecorn[ik, :, :, :, i], der)
+                         for i in range(a, b)) / (b - a) for a, b in in_range}
+        if der == 0:
+            below = np.where(Emax < Efermi[0])[0]
+            bandmax = below[-1] + 1 if len(below) else 0
+            if in_range:
+                bandmax = min(bandmax, in_range[0][0])
+            if bandmax > 0:
+                w[(0, bandmax)] = np.ones(len(Efermi))
+        if der == -1:
+            above = np.where(Emin > Efermi[-1])[0]
+            bandmin = above[0] if len(above) else nb
+            if in_range:
+                bandmin = max(bandmin, in_range[-1][1])
+            if nb > bandmin:
+                w[(bandmin, nb)] = np.ones(len(Efermi))
+        out.append(w)
+    return out
 
 
 # ---------------------------------------------------------------------------
@@ -788,8 +930,11 @@ class StaticCalculator:
     constant_factor_default = 1.
 
     def __init__(self, Efermi, constant_factor=None, use_factor=True, kwargs_formula=None,
-                 degen_thresh=1e-4, degen_Kramers=False, hole_like=False, Formula=None, fder=None):
+                 degen_thresh=1e-4, degen_Kramers=False, hole_like=False, Formula=None, fder=None,
+                 tetra=False):
         self.Efermi = np.array(Efermi, dtype=float)
+        self.tetra = tetra
+        self.hole_like = hole_like
         if Formula is not None:
             self.Formula = Formula
         if fder is not None:
@@ -808,11 +953,16 @@ class StaticCalculator:
         self.nEF_extra = self.Efermi.shape[0] + 2 * self.extraEf
 
     def group_values(self, data_K):
-        """Per-k dict {(ib1, ib2): (weight E, trace value)} (static.py:60-117)."""
+        """Per-k dict {(ib1, ib2): (weight, trace value)} (static.py:60-117); the weight is
+        the group's mean energy, or its tetrahedron weight over Efermi when tetra."""
         NB = data_K.num_wann
         formula = self.Formula(data_K, **self.kwargs_formula)
-        weights = data_K.get_bands_in_range_groups(self.EFmin, self.EFmax, degen_thresh=self.degen_thresh,
-                                                   degen_Kramers=self.degen_Kramers, sea=(self.fder == 0))
+        if self.tetra:
+            weights = tetra_group_weights(data_K, self.Efermi, -1 if self.hole_like else self.fder,
+                                          self.degen_thresh, self.degen_Kramers)
+        else:
+            weights = data_K.get_bands_in_range_groups(self.EFmin, self.EFmax, degen_thresh=self.degen_thresh,
+                                                       degen_Kramers=self.degen_Kramers, sea=(self.fder == 0))
         out = []
         for ik, bnd in enumerate(weights):
             vals = {}
@@ -831,6 +981,16 @@ class StaticCalculator:
 
     def __call__(self, data_K):
         per_k, ndim = self.group_values(data_K)
+        if self.tetra:
+            # tetrahedron weights: sum_k sum_groups w(Efermi) * value (static.py:119-125)
+            restot = np.zeros((len(self.Efermi),) + (3,) * ndim)
+            for groups in per_k:
+                for n, (w, v) in groups.items():
+                    restot += w.reshape((-1,) + (1,) * ndim) * v[None]
+            restot /= data_K.cell_volume
+            restot /= data_K.nk
+            restot *= self.constant_factor if self.use_factor else np.sign(self.constant_factor)
+            return OracleResult(self.Efermi, restot)
         restot = np.zeros((self.nEF_extra,) + (3,) * ndim)
         # Fermi scan (static.py:126-136)
         for groups in per_k:

@@ -77,6 +77,17 @@ def main():
              "haldane_adpt3_dos_iter1": I + "Haldane-dos-PythTB-adpt3-sym_iter-0001.npz",
              # Kubo optical conductivity, Chiral model (tests/test_run.py:183-199,799-832)
              "chiral_opt_conductivity": I + "Chiral-opt_conductivity_iter-0000.npz",
+             # tetrahedron method: Chiral model with C3z (tests/test_run.py:227-238,835-845)
+             # and the Haldane C3z refinement with adpt_mesh 3 (tests/test_run.py:730-735)
+             "chiral_tetra_ahc": I + "Chiral_tetra-ahc_iter-0000.npz",
+             "chiral_tetra_berry_dipole": I + "Chiral_tetra-berry_dipole_iter-0000.npz",
+             "chiral_tetra_berry_dipole_fsurf": I + "Chiral_tetra-berry_dipole_fsurf_iter-0000.npz",
+             "chiral_tetra_dos": I + "Chiral_tetra-dos_iter-0000.npz",
+             "chiral_tetra_cumdos": I + "Chiral_tetra-cumdos_iter-0000.npz",
+             "haldane_adpt3_tetra_ahc_iter0": I + "Haldane-ahc-PythTB-adpt3-tetra-sym_iter-0000.npz",
+             "haldane_adpt3_tetra_ahc_iter1": I + "Haldane-ahc-PythTB-adpt3-tetra-sym_iter-0001.npz",
+             "haldane_adpt3_tetra_dos_iter0": I + "Haldane-dos-PythTB-adpt3-tetra-sym_iter-0000.npz",
+             "haldane_adpt3_tetra_dos_iter1": I + "Haldane-dos-PythTB-adpt3-tetra-sym_iter-0001.npz",
              }, "results_run.npz")
     # per-formula gauge-invariant traces (tests/reference/formula/*.npz)
     forms = {}

@@ -1,0 +1,129 @@
+"""Pins of the oracle's tetrahedron method (no GPU).
+
+* `tetra_occupation` der=0 against the occupied volume fraction of a tetrahedron,
+  counted by sampling (uniform barycentric points), and der=1..3 against central
+  differences of the order below;
+* `tetra_band_weight` (12 tetrahedra around the centre of the parallelepiped) against
+  the exact fraction of a cube below a plane, for a linear band, where the linear
+  interpolation is exact;
+* the whole chain (corner energies, group weights, weighted sum, symmetrisation,
+  refinement) through the package's `run()` with the oracle evaluating each K-block,
+  against the reference's fixtures `Chiral_tetra-*` (C3z, tests/test_run.py:835-845)
+  and `Haldane-*-PythTB-adpt3-tetra-sym_iter-000*` (tests/test_run.py:730-735).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import wb_oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def wb():
+    import wannierberri_amd
+    return wannierberri_amd
+
+
+@pytest.fixture(scope="module")
+def results_run():
+    with np.load(os.path.join(GOLDEN, "results_run.npz"), allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("corners", [(0.3, -0.2, 1.1, 0.45), (0.0, 0.0, 1.0, 2.0), (-1.0, 0.5, 0.5, 0.5)])
+def test_tetra_occupation_volume(corners):
+    rng = np.random.default_rng(7)
+    lam = rng.dirichlet(np.ones(4), size=400000)
+    f = lam @ np.array(corners)
+    ef = np.linspace(min(corners) - 0.1, max(corners) + 0.1, 41)
+    sampled = (f[None, :] < ef[:, None]).mean(axis=1)
+    assert np.abs(O.tetra_occupation(ef, corners, 0) - sampled).max() < 3e-3
+
+
+@pytest.mark.parametrize("der", [1, 2, 3])
+def test_tetra_occupation_derivatives(der):
+    corners = (0.3, -0.2, 1.1, 0.45)
+    h = 1e-5
+    # points away from the corner energies, where each piece is smooth
+    ef = np.array([-0.1, 0.0, 0.2, 0.35, 0.4, 0.6, 0.9, 1.0])
+    fd = (O.tetra_occupation(ef + h, corners, der - 1) - O.tetra_occupation(ef - h, corners, der - 1)) / (2 * h)
+    assert np.abs(O.tetra_occupation(ef, corners, der) - fd).max() < 1e-5 * max(1, np.abs(fd).max())
+
+
+def test_tetra_band_weight_linear_band():
+    """E(k) = E0 + v.k on the cube [-1/2, 1/2]^3: the 12 tetrahedra cover the cube once
+    and the interpolation is exact, so the weight is the cube's volume fraction below EF."""
+    v = np.array([0.7, -0.4, 1.3])
+    E0 = 0.2
+    s = np.array([-0.5, 0.5])
+    corners = E0 + v[0] * s[:, None, None] + v[1] * s[None, :, None] + v[2] * s[None, None, :]
+    n = 120
+    g = (np.arange(n) + 0.5) / n - 0.5
+    E = E0 + v[0] * g[:, None, None] + v[1] * g[None, :, None] + v[2] * g[None, None, :]
+    ef = np.linspace(-1.2, 1.6, 29)
+    exact = np.array([(E < e).mean() for e in ef])
+    w = O.tetra_band_weight(ef, E0, corners, 0)
+    assert np.abs(w - exact).max() < 2e-3
+    # density of states: the derivative integrates to one over the band
+    efd = np.linspace(-1.5, 1.9, 20001)
+    dos = O.tetra_band_weight(efd, E0, corners, 1)
+    assert abs(np.trapezoid(dos, efd) - 1) < 1e-4
+    assert np.array_equal(O.tetra_band_weight(ef, E0, corners, -1), 1 - w)
+
+
+class _OracleBlock:
+    """data_k_class for run(): the oracle evaluates the K-block, corners included."""
+    osys = None
+
+    def __init__(self, system, dK=None, grid=None, Kpoint=None, **kw):
+        self.d = O.OracleDataK(self.osys, dK=dK, NKFFT=grid.FFT, dK_block=Kpoint.dK)
+
+
+class _OracleCalc:
+    allow_grid = True
+
+    def __init__(self, wb, calc, tr, inv):
+        self.wb, self.calc, self.tr, self.inv = wb, calc, tr, inv
+
+    def __call__(self, data):
+        r = self.calc(data.d)
+        return self.wb.EnergyResult(r.Efermi, r.data, transformTR=self.tr, transformInv=self.inv)
+
+
+def test_chiral_tetra(wb, results_run):
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_chiral_left.npz"))
+    s.set_pointgroup(["C3z"])
+    _OracleBlock.osys = O.load_golden_system(os.path.join(GOLDEN, "sys_chiral_left.npz"))
+    Ef = np.linspace(-5, 8, 27)
+    calcs = {"ahc": _OracleCalc(wb, O.AHC(Ef, tetra=True), -1, 1),
+             "berry_dipole": _OracleCalc(wb, O.BerryDipole_FermiSea(Ef, tetra=True, use_factor=False), 1, -1),
+             "berry_dipole_fsurf": _OracleCalc(wb, O.BerryDipole_FermiSurf(Ef, tetra=True, use_factor=False), 1, -1),
+             "dos": _OracleCalc(wb, O.DOS(Ef, tetra=True), 1, 1),
+             "cumdos": _OracleCalc(wb, O.CumDOS(Ef, tetra=True), 1, 1)}
+    grid = wb.Grid(system=s, NK=[10, 10, 4], NKFFT=[5, 5, 2])
+    r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, data_k_class=_OracleBlock,
+               save_files=False, verbose=False)
+    for k in calcs:
+        assert rel_err(r.results[k].data, results_run["chiral_tetra_" + k]) < 1e-8, k
+
+
+def test_haldane_tetra_refinement(wb, results_run):
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_haldane.npz"))
+    s.set_pointgroup(["C3z"])
+    _OracleBlock.osys = O.load_golden_system(os.path.join(GOLDEN, "sys_haldane.npz"))
+    Ef = np.linspace(-3, 3, 11)
+    calcs = {"ahc": _OracleCalc(wb, O.AHC(Ef, tetra=True), -1, 1),
+             "dos": _OracleCalc(wb, O.DOS(Ef, tetra=True), 1, 1)}
+    grid = wb.Grid(system=s, NK=[10, 10, 1], NKFFT=[5, 5, 1])
+    for it in (0, 1):
+        r = wb.run(s, grid, calcs, adpt_num_iter=it, adpt_mesh=3, use_irred_kpt=True, symmetrize=True,
+                   data_k_class=_OracleBlock, save_files=False, verbose=False)
+        for k in calcs:
+            assert rel_err(r.results[k].data, results_run[f"haldane_adpt3_tetra_{k}_iter{it}"]) < 1e-8, (k, it)

@@ -275,9 +275,12 @@ class Grid:
                                 pointgroup=self.pointgroup, refinement_level=0)
                for z in range(self.div[2])] for y in range(self.div[1])] for x in range(self.div[0])]
         if use_symmetry:
-            for x in range(self.div[0]):
+            # x fastest, as grid.py:172-181: the first point of each star met in this order
+            # represents it, which matters where the block's own result is not symmetric
+            # (tetrahedron corners)
+            for z in range(self.div[2]):
                 for y in range(self.div[1]):
-                    for z in range(self.div[2]):
+                    for x in range(self.div[0]):
                         KP = K[x][y][z]
                         if KP is None:
                             continue
