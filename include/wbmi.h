@@ -220,6 +220,34 @@ int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev
                      const void* Xbar_dev, int prec, const wbmi_scan_params* params,
                      double weight, double* hist_dev);
 
+/* ---- the same with tetrahedron weights -----------------------------------
+ * Replaces the tetra=True branch of StaticCalculator.__call__
+ * (calculators/static.py:80-85,119-125) with TetraWeightsParal
+ * (grid/tetrahedron.py:185-281): each group's trace is weighted over the
+ * Fermi levels by the mean of its bands' 12-tetrahedron occupations (fder 0)
+ * or their fder-th derivative (1..3).  Groups are kept, and the fder-0 group
+ * of bands below the range is formed, from each band's energy range over the
+ * centre and the 8 corners.
+ *   Ecorners_dev : float64 [8, nk, nw] band energies at k + ((2i-1), (2j-1),
+ *                  (2l-1)) * dK_fullBZ / 2, corner i*4 + j*2 + l
+ *                  (data_K/data_K_R.py:126-141)
+ *   Efermi_dev   : float64 [nEF], ascending
+ *   params       : fder = 0..3; EFmin = Efermi[0], EFmax = Efermi[nEF-1];
+ *                  nEF_extra and dEF are not read
+ *   hist_dev     : float64 [2 * nEF, 3^ndim]: rows [0, nEF) are steps
+ *                  (prefix-summed by wbmi_finalize_tetra), rows [nEF, 2 nEF)
+ *                  are added as they are.                                    */
+int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                           const double* Ecorners_dev, const void* Xbar_dev, int prec,
+                           const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
+                           double weight, double* hist_dev);
+
+/* out_dev [nEF, ncomp] = scale * (prefix sums of rows [0, nEF) + rows [nEF, 2 nEF))
+ * of a wbmi_static_scan_tetra histogram (static.py:149-155, no fder stencil).
+ * The step rows of hist_dev are overwritten with their prefix sums.         */
+int wbmi_finalize_tetra(wbmi_ctx* ctx, double* hist_dev, int nEF, int ncomp, double scale,
+                        double* out_dev);
+
 /* ---- histogram -> EnergyResult data ---------------------------------------
  * Prefix sum over the Fermi levels, fder finite-difference stencil and scaling
  * (calculators/static.py:126-155).  out_dev: float64 [nEF_extra-2*extra, ncomp].

@@ -1,0 +1,124 @@
+"""GPU parity of the tetrahedron method (tetra=True): corner energies, the
+tetrahedron-weighted scan (wbmi_static_scan_tetra) and its finalisation.
+
+* the reference's fixtures: Chiral model with C3z, irreducible K-points and
+  symmetrisation (`Chiral_tetra-*`, tests/test_run.py:227-238,835-845), and the
+  Haldane C3z refinement with adpt_mesh 3 (`Haldane-*-PythTB-adpt3-tetra-sym`,
+  tests/test_run.py:730-735), at the reference's bar 1e-8 * max|ref|;
+* the oracle on a random Hermitian 8-band model with external terms: non-additive
+  formulas (Morb, GME sea), the Fermi-surface form (fder 1) and fder 2, 3 on
+  unevenly spaced Fermi levels, at 1e-10 * max (FP64) and 2e-6 (mixed form).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import wb_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def wb():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import wannierberri_amd
+    return wannierberri_amd
+
+
+@pytest.fixture(scope="module")
+def results_run():
+    with np.load(os.path.join(GOLDEN, "results_run.npz"), allow_pickle=False) as f:
+        return {k: f[k] for k in f.files}
+
+
+def rel_err(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_chiral_tetra(wb, results_run, precision, tol):
+    say(f"test_chiral_tetra {precision}")
+    st = wb.calculators.static
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_chiral_left.npz"))
+    s.set_pointgroup(["C3z"])
+    Ef = np.linspace(-5, 8, 27)
+    calcs = {"ahc": st.AHC(Efermi=Ef, tetra=True),
+             "berry_dipole": st.BerryDipole_FermiSea(Efermi=Ef, tetra=True, use_factor=False),
+             "berry_dipole_fsurf": st.BerryDipole_FermiSurf(Efermi=Ef, tetra=True, use_factor=False),
+             "dos": st.DOS(Efermi=Ef, tetra=True),
+             "cumdos": st.CumDOS(Efermi=Ef, tetra=True)}
+    grid = wb.Grid(system=s, NK=[10, 10, 4], NKFFT=[5, 5, 2])
+    r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, parameters_K={"precision": precision},
+               save_files=False, verbose=False)
+    for k in calcs:
+        err = rel_err(r.results[k].data, results_run["chiral_tetra_" + k])
+        say(f"  {k}: {err:.2e}")
+        assert err < tol, k
+
+
+def test_haldane_tetra_refinement(wb, results_run):
+    say("test_haldane_tetra_refinement")
+    st = wb.calculators.static
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_haldane.npz"))
+    s.set_pointgroup(["C3z"])
+    Ef = np.linspace(-3, 3, 11)
+    grid = wb.Grid(system=s, NK=[10, 10, 1], NKFFT=[5, 5, 1])
+    for it in (0, 1):
+        calcs = {"ahc": st.AHC(Efermi=Ef, tetra=True), "dos": st.DOS(Efermi=Ef, tetra=True)}
+        r = wb.run(s, grid, calcs, adpt_num_iter=it, adpt_mesh=3, use_irred_kpt=True, symmetrize=True,
+                   save_files=False, verbose=False)
+        for k in calcs:
+            assert rel_err(r.results[k].data, results_run[f"haldane_adpt3_tetra_{k}_iter{it}"]) < 1e-8, (k, it)
+
+
+def _oracle_tetra(osys, calcs, NK, NKFFT):
+    NKdiv, NKFFT, blocks = O.grid_blocks(NK, NKFFT)
+    tot = {}
+    for dK, factor in blocks:
+        d = O.OracleDataK(osys, dK=dK, NKFFT=NKFFT, dK_block=1. / NKdiv)
+        for k, c in calcs.items():
+            r = c(d) * factor
+            tot[k] = r if k not in tot else tot[k] + r
+    return tot
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32", 2e-6)])
+def test_tetra_random_vs_oracle(wb, precision, tol):
+    say(f"test_tetra_random_vs_oracle {precision}")
+    st = wb.calculators.static
+    from wannierberri_amd import models
+    s = models.synthetic(num_wann=8, Rmax=2, seed=11, with_BBCC=True)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA", "BB", "CC")})
+    E0 = np.linalg.eigvalsh(s.get_R_mat("Ham")[np.where((s.rvec.iRvec == 0).all(1))[0][0]])
+    E = np.linspace(E0.min() - 1, E0.max() + 1, 23)
+    Ef = np.sort(np.concatenate([E, E[:-1] + 0.37 * (E[1] - E[0])]))   # uneven spacing
+    gpu = {"morb": st.Morb(Efermi=Ef, tetra=True),
+           "gme_sea": st.GME_orb_FermiSea(Efermi=Ef, tetra=True),
+           "bcd_surf": st.BerryDipole_FermiSurf(Efermi=Ef, tetra=True),
+           "d2": st._DOS(fder=2, Efermi=Ef, tetra=True),
+           "d3": st._DOS(fder=3, Efermi=Ef, tetra=True)}
+    ora = {"morb": O.Morb(Ef, tetra=True),
+           "gme_sea": O.GME_orb_FermiSea(Ef, tetra=True),
+           "bcd_surf": O.BerryDipole_FermiSurf(Ef, tetra=True),
+           "d2": O.CumDOS(Ef, fder=2, tetra=True),
+           "d3": O.CumDOS(Ef, fder=3, tetra=True)}
+    NK, NKFFT = [4, 4, 4], [2, 2, 2]
+    grid = wb.Grid(system=s, NK=NK, NKFFT=NKFFT)
+    r = wb.run(s, grid, gpu, parameters_K={"precision": precision}, save_files=False, verbose=False)
+    ref = _oracle_tetra(o, ora, NK, NKFFT)
+    for k in gpu:
+        err = rel_err(r.results[k].data, ref[k].data)
+        say(f"  {k}: {err:.2e}")
+        assert err < tol, k

@@ -37,6 +37,7 @@ SYMBOLS = [
     "wbmi_last_error", "wbmi_build_channels", "wbmi_r2k_plan_create", "wbmi_r2k_plan_destroy",
     "wbmi_r2k_plan_sizes", "wbmi_r2k_box", "wbmi_r2k_exec", "wbmi_r2k_klist", "wbmi_eigh",
     "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_optical_conductivity", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
+    "wbmi_static_scan_tetra", "wbmi_finalize_tetra",
 ]
 
 
@@ -93,6 +94,9 @@ _PROTOS = {
     "wbmi_rotate_work_bytes": (ctypes.c_size_t, [_i, _i, _i]),
     "wbmi_static_scan": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _d, _vp]),
     "wbmi_finalize": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _vp]),
+    "wbmi_static_scan_tetra": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp, _i, _d,
+                                    _vp]),
+    "wbmi_finalize_tetra": (_i, [_vp, _vp, _i, _i, _d, _vp]),
 }
 
 
@@ -237,6 +241,23 @@ class Context:
         self.check(self._timed(f"scan_f{params.formula}", (nk, nw), lib().wbmi_static_scan, self.h, nk, nw, nch,
                                _ptr(E), _ptr(X), prec, ctypes.byref(params), float(weight), _ptr(hist)),
                    "wbmi_static_scan")
+
+    def static_scan_tetra(self, E, Ec, X, prec, params, Efermi, weight, hist):
+        """Tetrahedron-weighted scan: Ec [8, nk, nw], Efermi [nEF] (device, ascending),
+        hist [2 nEF, ncomp]."""
+        nk, nw = E.shape
+        nch = X.shape[1] if X is not None else 1
+        if tuple(Ec.shape) != (8, nk, nw) or tuple(hist.shape) != (2 * Efermi.shape[0], hist.shape[1]):
+            raise ValueError("static_scan_tetra: Ec must be [8, nk, nw] and hist [2 nEF, ncomp]")
+        self.check(self._timed(f"scan_tetra_f{params.formula}", (nk, nw), lib().wbmi_static_scan_tetra, self.h, nk,
+                               nw, nch, _ptr(E), _ptr(Ec), _ptr(X), prec, ctypes.byref(params), _ptr(Efermi),
+                               int(Efermi.shape[0]), float(weight), _ptr(hist)),
+                   "wbmi_static_scan_tetra")
+
+    def finalize_tetra(self, hist, scale, out):
+        nEF2, ncomp = hist.shape
+        self.check(lib().wbmi_finalize_tetra(self.h, _ptr(hist), nEF2 // 2, ncomp, float(scale), _ptr(out)),
+                   "wbmi_finalize_tetra")
 
     def optical_conductivity(self, E, X, prec, params, out):
         nk, nw = E.shape

@@ -3,7 +3,9 @@
 Same constructor signatures and outputs as wannierberri/calculators/static.py;
 each call runs the fused formula-trace + Fermi-scan kernel (wbmi_static_scan)
 on the K-block and the prefix-sum / fder-stencil / scaling kernel
-(wbmi_finalize).  The result stays on the GPU until read.
+(wbmi_finalize); with tetra=True the tetrahedron-weighted scan
+(wbmi_static_scan_tetra, corner energies from Data_K.corner_energies) and
+wbmi_finalize_tetra.  The result stays on the GPU until read.
 """
 import numpy as np
 
@@ -51,7 +53,7 @@ class StaticCalculator(Calculator):
     """Fermi-level scan integrator (calculators/static.py:24-169).
 
     Subclasses set `formula` (a WBMI_FORMULA_* id), `ndim` and `fder`.
-    Not supported: tetra, k_resolved, select_bands, finite Emin/Emax.
+    Not supported: k_resolved, select_bands, finite Emin/Emax, hole_like with tetra.
     """
     formula = None
     ndim = 0
@@ -61,8 +63,8 @@ class StaticCalculator(Calculator):
                  Emin=-np.inf, Emax=np.inf, hole_like=False, k_resolved=False, fder=None, select_bands=None,
                  **kwargs):
         super().__init__(**kwargs)
-        if tetra:
-            raise NotImplementedError("the tetrahedron method is not available on the MI355X path")
+        if tetra and hole_like:
+            raise NotImplementedError("hole_like with the tetrahedron method is not available on the MI355X path")
         if k_resolved:
             raise NotImplementedError("k-resolved output is not available on the MI355X path")
         if select_bands is not None:
@@ -74,6 +76,10 @@ class StaticCalculator(Calculator):
             if k not in ("internal_terms", "external_terms", "sign"):
                 raise NotImplementedError(f"kwargs_formula option '{k}' is not available on the MI355X path")
         self.Efermi = np.array(Efermi, dtype=float)
+        self.tetra = bool(tetra)
+        if self.tetra and np.any(np.diff(self.Efermi) < 0):
+            raise NotImplementedError("the tetrahedron method on the MI355X path needs ascending Fermi levels")
+        self._Ef_dev = {}
         self.smoother = smoother
         self.use_factor = use_factor
         self.hole_like = hole_like
@@ -124,9 +130,33 @@ class StaticCalculator(Calculator):
         p.degen_Kramers = int(bool(self.degen_Kramers))
         return p
 
+    def _call_tetra(self, data_K):
+        """Tetrahedron weights (static.py:80-85,119-125): no extended Fermi grid, no stencil."""
+        torch = data_K.torch
+        dev = data_K.ctx.device
+        Ef = self._Ef_dev.get(dev)
+        if Ef is None:
+            Ef = self._Ef_dev[dev] = torch.from_numpy(self.Efermi).to(dev)
+        nEF = self.Efermi.shape[0]
+        p = self.scan_params(data_K.system)
+        p.EFmin, p.EFmax = float(self.Efermi[0]), float(self.Efermi[-1])
+        p.nEF_extra = nEF
+        hist = torch.zeros((2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
+        data_K.static_scan_tetra(p, Ef, hist, 1.0)
+        out = torch.empty((nEF, self.ncomp), dtype=torch.float64, device=dev)
+        cf = self.constant_factor if self.use_factor else float(np.sign(self.constant_factor))
+        data_K.ctx.finalize_tetra(hist, cf / data_K.cell_volume / data_K.nk, out)
+        return out
+
     def __call__(self, data_K):
         torch = data_K.torch
         data_K.ensure(self.needed_channels(data_K.system))
+        if self.tetra:
+            out = self._call_tetra(data_K)
+            data = out.reshape((self.Efermi.shape[0],) + (3,) * self.ndim)
+            tr, inv = _PARITY[self.formula]
+            return EnergyResult(self.Efermi, data, smoothers=[self.smoother], transformTR=tr, transformInv=inv,
+                                comment=self.comment, save_mode=self.save_mode)
         dev = data_K.ctx.device
         hist = torch.zeros((self.nEF_extra, self.ncomp), dtype=torch.float64, device=dev)
         data_K.static_scan(self.scan_params(data_K.system), hist, 1.0)

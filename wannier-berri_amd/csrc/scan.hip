@@ -1,984 +1,8 @@
-// Covariant formula traces + Fermi-level scan of libwbmi.
-//
-// Replaces the per-(k, band group) loop of StaticCalculator.__call__
-// (calculators/static.py:60-136).  One 256-thread workgroup per k-point; the
-// formula is a template parameter, so every calculator gets its own kernel.
-//
-// Band partition.  The reference evaluates, for every band group G, the
-// formula with inn = G and out = all other bands.  Every sum in the formulas
-// is then either over l in out (l not in G) or over n' in G, so with a label
-// per band (lab[x] = group of x) the traces of ALL groups come from one set
-// of per-band values v_n, n in G, using the masks
-//     Off(x, y) = lab[x] != lab[y]      (x, y in different groups)
-//     On(x, y)  = lab[x] == lab[y]
-// and the masked matrices
-//     D'^a = -V^a o M,  M = dEinv o Off,  V'' = V o On,  A'' = A o On.
-// The Fermi-sea group (all bands below EFmin, data_K.py:178-185) and the
-// cumulative cuts [0, c) of non-additive formulas (static.py:109-117) are
-// two-set partitions evaluated the same way.
-//
-// DerOmega.  With G^{bd} = V^b D'^d, H^{ad} = D'^a V^d and K^{bd} = A^b D'^d,
-//   dD.ln = X1^{bd} = -M o [W^{bd} + G^{bd} + G^{db} - D'^b V''^d - D'^d V''^b]
-//   dD.nl = X2^{ad} = -M o [W^{ad} + V''^a D'^d + V''^d D'^a - H^{ad} - H^{da}]
-//   dA.ln = Y1^{bd} = A'^{bd} - D'^d A''^b + K^{bd}
-// (formula/elementary.py:63-70, formula/formula.py:108-118).  X1 and X2 are
-// symmetric in (b, d), so the elementwise terms are folded half-and-half into
-// each product: Q1 = G + (W - S)/2, Q2 = -H + (W + R)/2, Y1 = K + A' - D'A''.
-// The traces use only diagonal elements of D'^a X1, X2 A^b and D'^a Y1, so
-// every product element is consumed where it is computed: each wave forms
-// one 32 x 32 block of one of the 27 products with a 4 x 4 register tile per
-// lane, adds the elementwise terms, multiplies by the consumer weights and
-// reduces over the summed index with cross-lane shuffles into its own
-// accumulator rows.  No product is stored and no barrier separates them.
-//
-// For DerOmega the operands V and A stay in LDS (rows padded by one element)
-// when nw <= 32.
-// All arithmetic is FP64 (energies, dEinv, traces, histogram); channels may
-// be stored in FP32 or FP64.
-#include <cmath>
-
-#include "mfma_c64.h"
-#include "wbmi_common.h"
-
-#define SC_THREADS 512
-#define SC_WAVES (SC_THREADS / 64)
-#define SC_NWMAX 128
-#define SC_M_NW 64    // dEinv o Off lives in LDS up to this nw, else it is formed on the fly
-#define SC_LDS_NW 32  // V and A live in LDS up to this nw
-
-__constant__ int c_alpha[3] = {1, 2, 0};  // utility.py:45-46
-__constant__ int c_beta[3] = {2, 0, 1};
-
-struct ScanArgs {
-    wbmi_scan_params p;
-    int nw, nch, stage_va;
-    double weight;
-};
-
-// dynamic shared-memory layout
-struct Smem {
-    cplx<double>* V;    // [3][nw][ld] (staged) or null
-    cplx<double>* A;    // [3][nw][ld] (staged, external terms) or null
-    double* accw;       // [SC_WAVES][nw][9] per-wave DerOmega accumulators
-    cplx<double>* acc;  // [nw][9]
-    double* M;          // [nw][ld] dEinv o Off
-    double* E;
-    double* cut;        // [nw+1][9]
-    double* gw;         // group weights (mean energy)
-    int* lab;
-    int* gs;
-    int* ge;
-    int* act;
-    int* gb;    // group borders [nw+1]
-    int* gin;   // group in range
-    int* misc;  // [0] = ngroups, [1] = bandmax
-    int* ch;    // channel slot per role [WBMI_NROLES]
-};
-
-__host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15; }
-
-// byte offsets of the LDS regions and their total size
-struct SmemOffsets {
-    size_t V, A, accw, acc, M, E, cut, gw, ints, total;
-};
-
-__host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool stage_a, bool stage_w) {
-    const size_t ld = nw + 1;
-    SmemOffsets o;
-    size_t off = 0;
-    o.V = off;
-    off = al16(off + (stage_v ? sizeof(cplx<double>) * 3 * nw * ld : 0));
-    o.A = off;
-    off = al16(off + (stage_a ? sizeof(cplx<double>) * 3 * nw * ld : 0));
-    o.accw = off;
-    off = al16(off + (stage_w ? sizeof(double) * SC_WAVES * nw * 9 : 0));
-    o.acc = off;
-    off = al16(off + sizeof(cplx<double>) * nw * 9);
-    o.M = off;
-    off = al16(off + (nw <= SC_M_NW ? sizeof(double) * nw * ld : 0));
-    o.E = off;
-    off = al16(off + sizeof(double) * nw);
-    o.cut = off;
-    off = al16(off + sizeof(double) * (nw + 1) * 9);
-    o.gw = off;
-    off = al16(off + sizeof(double) * nw);
-    o.ints = off;
-    off = al16(off + sizeof(int) * (6 * nw + 1 + 4 + WBMI_NROLES));
-    o.total = off;
-    return o;
-}
-
-__device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int nw, bool stage_v, bool stage_a,
-                                                                bool stage_w) {
-    const SmemOffsets o = smem_offsets(nw, stage_v, stage_a, stage_w);
-    Smem s;
-    // LDS addresses even where a region is empty (never read then): no null tests on LDS pointers
-    s.V = (cplx<double>*)(base + o.V);
-    s.A = (cplx<double>*)(base + o.A);
-    s.accw = (double*)(base + o.accw);
-    s.acc = (cplx<double>*)(base + o.acc);
-    s.M = (double*)(base + o.M);  // empty region (never read) when nw > SC_M_NW
-    s.E = (double*)(base + o.E);
-    s.cut = (double*)(base + o.cut);
-    s.gw = (double*)(base + o.gw);
-    int* q = (int*)(base + o.ints);
-    s.lab = q;
-    s.gs = q + nw;
-    s.ge = q + 2 * nw;
-    s.act = q + 3 * nw;
-    s.gb = q + 4 * nw;
-    s.gin = q + 5 * nw + 1;
-    s.misc = q + 6 * nw + 1;
-    s.ch = q + 6 * nw + 5;
-    return s;
-}
-
-// Element access to the channels of one k-point.
-// LM: dEinv o Off read from LDS (nw <= SC_M_NW) or formed from E and the labels.
-// ST: V and A read from their LDS copies (DerOmega, DerMorb and Omega at nw <= SC_LDS_NW;
-// A only with external terms, the only case that reads it) or from the channels.
-// Both are compile-time so that every access is a plain LDS or global load.
-template <typename T, bool LM, bool ST>
-struct KCtx {
-    const cplx<T>* X;  // channels of this k: [nch][nw][nw]
-    int nw, ld;
-    const int* ch;
-    const double* E;
-    const double* M;  // dEinv o Off, row stride ld
-    const int* lab;
-    const int* gs;
-    const int* ge;
-    const cplx<double>* sV;  // LDS copies of V and A (ST)
-    const cplx<double>* sA;
-    __device__ inline cplx<double> get(int role, int x, int y) const {
-        cplx<T> v = X[((int64_t)ch[role] * nw + x) * nw + y];
-        return mk((double)v.re, (double)v.im);
-    }
-    __device__ inline cplx<double> V(int a, int x, int y) const {
-        if constexpr (ST)
-            return sV[(a * nw + x) * ld + y];
-        else
-            return get(WBMI_ROLE_V + a, x, y);
-    }
-    __device__ inline cplx<double> A(int a, int x, int y) const {
-        if constexpr (ST)
-            return sA[(a * nw + x) * ld + y];
-        else
-            return get(WBMI_ROLE_A + a, x, y);
-    }
-    __device__ inline cplx<double> W(int a, int b, int x, int y) const { return get(WBMI_ROLE_W + 3 * a + b, x, y); }
-    __device__ inline cplx<double> Ad(int a, int d, int x, int y) const { return get(WBMI_ROLE_AD + 3 * a + d, x, y); }
-    __device__ inline cplx<double> O(int c, int x, int y) const { return get(WBMI_ROLE_O + c, x, y); }
-    __device__ inline cplx<double> Od(int c, int d, int x, int y) const { return get(WBMI_ROLE_OD + 3 * c + d, x, y); }
-    __device__ inline cplx<double> B(int a, int x, int y) const { return get(WBMI_ROLE_B + a, x, y); }
-    __device__ inline cplx<double> C(int a, int x, int y) const { return get(WBMI_ROLE_C + a, x, y); }
-    __device__ inline cplx<double> Bd(int a, int d, int x, int y) const { return get(WBMI_ROLE_BD + 3 * a + d, x, y); }
-    __device__ inline cplx<double> Cd(int c, int d, int x, int y) const { return get(WBMI_ROLE_CD + 3 * c + d, x, y); }
-    // X o On: elements between bands of the same group
-    __device__ inline cplx<double> Ain(int a, int x, int y) const { return lab[x] == lab[y] ? A(a, x, y) : mk(0.0, 0.0); }
-    __device__ inline cplx<double> Vin(int a, int x, int y) const { return lab[x] == lab[y] ? V(a, x, y) : mk(0.0, 0.0); }
-    // dEinv_xy Off(x, y) (data_K.py:290-298): from LDS, or formed from E and the labels
-    __device__ inline double m(int x, int y) const {
-        if constexpr (LM) {
-            return M[x * ld + y];
-        } else {
-            const double de = E[x] - E[y];
-            return (lab[x] != lab[y] && fabs(de) >= 1e-7) ? 1.0 / de : 0.0;
-        }
-    }
-    // D'^a_xy = -V^a_xy dEinv_xy Off(x, y)  (branch-free: M = 0 gives an exact zero for finite V)
-    __device__ inline cplx<double> Dp(int a, int x, int y) const { return (-m(x, y)) * V(a, x, y); }
-};
-
-// pre-Hermitian Omega summand s^c_xy for x, y in the same group
-// (formula/covariant.py:175-200): Omega.nn = s + s^+.  The sum over the
-// off-group bands l runs over [l0, l1); `tail` adds the within-group and O terms.
-template <typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal,
-                                                              bool external, int l0, int l1, bool tail) {
-    const int al = c_alpha[c], be = c_beta[c];
-    cplx<double> s = mk(0.0, 0.0);
-    for (int l = l0; l < l1; ++l) {
-        if (K.lab[l] == K.lab[x]) continue;
-        cplx<double> dxa = K.Dp(al, x, l);
-        if (internal) s += times_i(-(dxa * K.Dp(be, l, y)));
-        if (external) s += K.Dp(be, x, l) * K.A(al, l, y) - dxa * K.A(be, l, y);
-    }
-    if (external && tail) {
-        for (int m = K.gs[x]; m < K.ge[x]; ++m) s += times_i(-(K.A(al, x, m) * K.A(be, m, y)));
-        s += 0.5 * K.O(c, x, y);
-    }
-    return s;
-}
-template <typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal,
-                                                              bool external) {
-    return omega_s(K, c, x, y, internal, external, 0, K.nw, true);
-}
-
-// pre-Hermitian Morb_H summand h^c_xy for x, y in the same group
-// (formula/covariant.py:390-414)
-template <typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal, bool external) {
-    const int al = c_alpha[c], be = c_beta[c];
-    cplx<double> s = mk(0.0, 0.0);
-    for (int l = 0; l < K.nw; ++l) {
-        if (K.lab[l] == K.lab[x]) continue;
-        cplx<double> dxa = K.Dp(al, x, l);
-        if (internal) s += times_i(-(K.E[l] * (dxa * K.Dp(be, l, y))));
-        if (external) s += K.Dp(be, x, l) * K.B(al, l, y) - dxa * K.B(be, l, y);
-    }
-    if (external) {
-        for (int m = K.gs[x]; m < K.ge[x]; ++m) s += times_i(-(K.E[m] * (K.A(al, x, m) * K.A(be, m, y))));
-        s += 0.5 * K.C(c, x, y);
-    }
-    return s;
-}
-
-template <int F>
-__host__ __device__ constexpr int ncomp_of() {
-    return F == WBMI_FORMULA_IDENTITY ? 1 : (F == WBMI_FORMULA_OMEGA || F == WBMI_FORMULA_MORB_HPM) ? 3 : 9;
-}
-
-template <int F>
-__host__ __device__ constexpr bool uses_A() {
-    return F == WBMI_FORMULA_OMEGA || F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
-           F == WBMI_FORMULA_MORB_HPM || F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERMORB;
-}
-
-// The two curl components c whose index pair (alpha_c, beta_c) contains x.
-struct Pair {
-    int c, partner;
-    double sig;
-};
-// x in the second slot for sigma=+1: (sigma=+1: beta_c == x, a = alpha_c; sigma=-1: alpha_c == x, a = beta_c)
-__device__ inline Pair pair_second(int x, int which) {
-    for (int c = 0; c < 3; ++c) {
-        if (which == 0 && c_beta[c] == x) return Pair{c, c_alpha[c], 1.0};
-        if (which == 1 && c_alpha[c] == x) return Pair{c, c_beta[c], -1.0};
-    }
-    return Pair{0, 0, 0.0};
-}
-// x in the first slot for sigma=+1: (sigma=+1: alpha_c == x, b = beta_c; sigma=-1: beta_c == x, b = alpha_c)
-__device__ inline Pair pair_first(int x, int which) {
-    for (int c = 0; c < 3; ++c) {
-        if (which == 0 && c_alpha[c] == x) return Pair{c, c_beta[c], 1.0};
-        if (which == 1 && c_beta[c] == x) return Pair{c, c_alpha[c], -1.0};
-    }
-    return Pair{0, 0, 0.0};
-}
-
-// build M = dEinv o Off for the current labels (data_K.py:290-298)
-__device__ __attribute__((always_inline)) void build_M(const Smem& S, int nw) {
-    const int ld = nw + 1;
-    for (int t = threadIdx.x; t < nw * nw; t += SC_THREADS) {
-        int x = t / nw, y = t % nw;
-        double m = 0.0;
-        if (S.lab[x] != S.lab[y]) {
-            double de = S.E[x] - S.E[y];
-            if (fabs(de) >= 1e-7) m = 1.0 / de;
-        }
-        S.M[x * ld + y] = m;
-    }
-}
-
-// sum of v over the lanes of a wave that differ only in the lane bits of `mask`
-__device__ inline double wave_sum(double v, int mask) {
-    for (int b = 1; b < 64; b <<= 1)
-        if (mask & b) v += __shfl_xor(v, b);
-    return v;
-}
-
-// Wave tasks.  One wave forms block (rb, cb) (32 x 32) of a dense product
-// P = X Y on the matrix cores (FP64 MFMA, mfma_c64.h), adds the elementwise terms of
-// each element (x, y) and accumulates the consumers' weighted elements into
-// its accumulator rows accw[band][comp] after a shuffle reduction over the
-// summed index.  The band of an element is its column y (kind 1 forms its
-// product transposed for this).  Kinds (task indices b, d; for kinds 5, 6 b = c):
-//   0  Q1^{bd} = V^b D'^d + (W - D'^b V''^d - D'^d V''^b)/2, y in the partition, x off-side:
-//      DerOmega T1; DerMorb s2 (x 2 E_x), s7 (conj B^a_xy), s1 (raw V^b D'^d)
-//   1  Q2^{bd} = -D'^b V^d + (W + V''^b D'^d + V''^d D'^b)/2, x in, y off-side: DerOmega T3
-//   2  Y1^{bd} = A^b D'^d + A' - D'^d A''^b, x off-side: DerOmega T2;
-//      x same-side (DerMorb): s5 terms of A' + A^b D'^d
-//   3  YB^{bd} = B^b D'^d + B' - D'^d B''^b, x off-side: DerMorb s6
-//   4  DA^{bd} = D'^d A^b, same side: DerMorb s5 (-D'^d A^b) and the Omega block (-D'^al A^be + D'^be A^al)
-//   5  DD^c = D'^al D'^be, same side: Omega block, internal term
-//   6  AA^c = A''^al A''^be, same side: Omega block, external term
-//   7  VA^{d be} = V''^d A''^be, same side: DerMorb s4
-// DerOmega contributions carry the band weight dw(n) = 2 (DerOmega) or
-// 2 sign E_n (DerMorb's Eav o dOmega term; only cut partitions are summed, so a
-// same-side element may be credited to either of its bands).
-struct TaskCtx {
-    bool dm, in, ex;
-    double sign;
-};
-
-template <int KIND>
-__host__ __device__ constexpr int task_nt() {
-    return KIND == 0 ? 5 : KIND == 1 ? 4 : KIND == 2 ? 2 : KIND == 3 ? 2 : KIND == 4 ? 5 : KIND == 7 ? 1 : 3;
-}
-
-// M1 (kind 0 only): the task also carries the kind-1 task (d, b).  With V
-// Hermitian and D' anti-Hermitian, (D'^d V^b)_yx = -conj((V^b D'^d)_xy), so
-// kind 1's transposed product is -conj of kind 0's at the same element, and
-// its elementwise channel W^{db}_yx = conj(W^{bd}_xy): one product and one
-// load serve both consumers.
-template <int KIND, bool M1, typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST>& K, const TaskCtx& tc_,
-                                                           double* accw, int b, int d, int rb, int cb) {
-    const int nw = K.nw, lane = threadIdx.x & 63;
-    const int r0 = rb * 32, c0 = cb * 32;
-    const bool dm = tc_.dm, in = tc_.in, ex = tc_.ex;
-    const double sign = tc_.sign;
-    const int al = c_alpha[b], be = c_beta[b];  // kinds 5, 6: b = c
-    MfmaBlock P;
-    P.zero();
-    if constexpr (KIND == 0)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.V(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
-    else if constexpr (KIND == 1)  // transposed: P_xy = (D'^b V^d)_yx, so the band n = y is a column as for the others
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.V(d, m, r); }, [&](int m, int c) { return K.Dp(b, c, m); });
-    else if constexpr (KIND == 2)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.A(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
-    else if constexpr (KIND == 3)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.B(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
-    else if constexpr (KIND == 4)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(d, r, m); }, [&](int m, int c) { return K.A(b, m, c); });
-    else if constexpr (KIND == 5)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(al, r, m); }, [&](int m, int c) { return K.Dp(be, m, c); });
-    else if constexpr (KIND == 6)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Ain(al, r, m); }, [&](int m, int c) { return K.Ain(be, m, c); });
-    else
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Vin(b, r, m); }, [&](int m, int c) { return K.Ain(d, m, c); });
-
-    // targets: partner component tp, accumulator column tcol, factor tw
-    constexpr int NT = task_nt<KIND>();
-    int tp[NT], tcol[NT];
-    double tw[NT];
-#pragma unroll
-    for (int w = 0; w < NT; ++w) {
-        tp[w] = 0;
-        tcol[w] = 0;
-        tw[w] = 0.0;
-    }
-    if constexpr (KIND == 0 || KIND == 1) {
-        // the two index pairs of X1 / X2: targets coincide for b == d (one lane adds twice)
-        const int nt = b == d ? 2 : 4;
-        const double f = b == d ? 2.0 : 1.0;
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            const Pair p = KIND == 1 ? pair_first(w < 2 ? b : d, w % 2) : pair_second(w < 2 ? b : d, w % 2);
-            tp[w] = p.partner;
-            tcol[w] = p.c * 3 + (w < 2 ? d : b);
-            tw[w] = w < nt ? f * p.sig : 0.0;
-        }
-        if constexpr (KIND == 0) {
-            // s1: the c with beta_c = d, partner alpha_c, column (c, b)
-            for (int c = 0; c < 3; ++c)
-                if (c_beta[c] == d) {
-                    tp[4] = c_alpha[c];
-                    tcol[4] = c * 3 + b;
-                    tw[4] = 2.0;
-                }
-        }
-    } else if constexpr (KIND == 2 || KIND == 3) {
-#pragma unroll
-        for (int w = 0; w < 2; ++w) {
-            const Pair p = pair_second(b, w);
-            tp[w] = p.partner;
-            tcol[w] = p.c * 3 + d;
-            tw[w] = p.sig;
-        }
-    } else if constexpr (KIND == 4) {
-#pragma unroll
-        for (int w = 0; w < 2; ++w) {
-            const Pair p = pair_second(b, w);
-            tp[w] = p.partner;
-            tcol[w] = p.c * 3 + d;
-            tw[w] = p.sig;
-        }
-        // Omega block: D'^d A^b enters s^c with -1 for (al_c, be_c) = (d, b), +1 for (b, d)
-        for (int c = 0; c < 3; ++c) {
-            const double coef = (c_alpha[c] == d && c_beta[c] == b) ? -1.0 : (c_alpha[c] == b && c_beta[c] == d) ? 1.0 : 0.0;
-            if (coef != 0.0)
-                for (int e = 0; e < 3; ++e) {
-                    tp[2 + e] = e;
-                    tcol[2 + e] = c * 3 + e;
-                    tw[2 + e] = 2.0 * sign * coef;
-                }
-        }
-    } else if constexpr (KIND == 5 || KIND == 6) {
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-            tp[e] = e;
-            tcol[e] = b * 3 + e;
-            tw[e] = 2.0 * sign;
-        }
-    } else {
-        for (int c = 0; c < 3; ++c)
-            if (c_beta[c] == d) {
-                tp[0] = c_alpha[c];
-                tcol[0] = c * 3 + b;
-                tw[0] = 2.0;
-            }
-    }
-    // targets of the carried kind-1 task (b1, d1) = (d, b)
-    constexpr int NT1 = M1 ? 4 : 1;
-    const int b1 = d, d1 = b;
-    int tp1[NT1], tcol1[NT1];
-    double tw1[NT1];
-#pragma unroll
-    for (int w = 0; w < NT1; ++w) {
-        tp1[w] = 0;
-        tcol1[w] = 0;
-        tw1[w] = 0.0;
-    }
-    if constexpr (M1) {
-        const int nt = b1 == d1 ? 2 : 4;
-        const double f = b1 == d1 ? 2.0 : 1.0;
-#pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            const Pair p = pair_first(w < 2 ? b1 : d1, w % 2);
-            tp1[w] = p.partner;
-            tcol1[w] = p.c * 3 + (w < 2 ? d1 : b1);
-            tw1[w] = w < nt ? f * p.sig : 0.0;
-        }
-    }
-
-    // part[u][w]: u = the lane's band slot, its column bj (the band n is the column y of every element)
-    constexpr int NU = 2;
-    double part[NU][NT], part1[NU][NT1];
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-#pragma unroll
-        for (int w = 0; w < NT; ++w) part[u][w] = 0.0;
-#pragma unroll
-        for (int w = 0; w < NT1; ++w) part1[u][w] = 0.0;
-    }
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi) {
-        // the elementwise channel (W, A', B') of this row half: all loads issued before use
-        cplx<double> gz[2][4];
-        if constexpr (KIND <= 3) {
-#pragma unroll
-            for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int x = r0 + mfma_row(bi, v), y = c0 + mfma_col(bj);
-                    const bool ok = x < nw && y < nw;
-                    const int xx = ok ? x : 0, yy = ok ? y : 0;
-                    if constexpr (KIND == 0)
-                        gz[bj][v] = K.W(b, d, xx, yy);  // (l, n)
-                    else if constexpr (KIND == 1)
-                        gz[bj][v] = conj(K.W(b, d, xx, yy));  // (n, l) of the Hermitian W, read along rows
-                    else if constexpr (KIND == 2)
-                        gz[bj][v] = ex ? K.Ad(b, d, xx, yy) : mk(0.0, 0.0);
-                    else
-                        gz[bj][v] = K.Bd(b, d, xx, yy);
-                }
-        }
-#pragma unroll
-        for (int bj = 0; bj < 2; ++bj)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int x = r0 + mfma_row(bi, v), y = c0 + mfma_col(bj);
-            if (x >= nw || y >= nw) continue;
-            const int i = bj, j = bj;  // part slot
-            const cplx<double> pv = P.get(bi, bj, v);
-            const int n = y, l = x;
-            if (!S.act[n]) continue;
-            const bool same = K.lab[l] == K.lab[n];
-            const bool single = S.ge[n] - S.gs[n] == 1;  // group {n}: the within-group sums have one term
-            const double dw = dm ? 2.0 * sign * K.E[n] : 2.0;
-            if constexpr (KIND == 0) {
-                if (same) continue;
-                // (W^{bd} - D'^b V''^d - D'^d V''^b)_ln / 2, with D'_xy = -m_xy V_xy and m_nl = -m_ln
-                const double mln = K.m(l, n);
-                cplx<double> z = gz[bj][v];
-                if (single)
-                    z += mln * (K.V(b, l, n) * K.V(d, n, n) + K.V(d, l, n) * K.V(b, n, n));
-                else
-                    for (int q = S.gs[n]; q < S.ge[n]; ++q)
-                        z -= K.Dp(b, l, q) * K.V(d, q, n) + K.Dp(d, l, q) * K.V(b, q, n);
-                const cplx<double> Pm = (-mln) * (pv + 0.5 * z);
-                const double wD = (in ? dw : 0.0) + (dm && in ? 2.0 * K.E[l] : 0.0);
-#pragma unroll
-                for (int w = 0; w < 4; ++w) {
-                    const cplx<double> Dnl = mln * K.V(tp[w], n, l);
-                    double t = wD * (Dnl * Pm).im;  // Re(-i D' X1) (T1, s2)
-                    if (dm && ex) t -= 2.0 * (conj(K.B(tp[w], l, n)) * Pm).re;  // s7
-                    part[j][w] += t;
-                }
-                if (dm && in) part[j][4] += ((mln * K.V(tp[4], n, l)) * pv).im;  // s1: Re(-i D'^al V^d D'^be)
-                if constexpr (M1) {
-                    // kind 1 (b1, d1): (W^{b1 d1} + V''^b1 D'^d1 + V''^d1 D'^b1)_nl / 2, with P1 = -conj(pv)
-                    const double mnl = -mln;
-                    cplx<double> z1 = conj(gz[bj][v]);
-                    if (single)
-                        z1 -= mnl * (K.V(b1, n, n) * K.V(d1, n, l) + K.V(d1, n, n) * K.V(b1, n, l));
-                    else
-                        for (int q = S.gs[n]; q < S.ge[n]; ++q)
-                            z1 += K.V(b1, n, q) * K.Dp(d1, q, l) + K.V(d1, n, q) * K.Dp(b1, q, l);
-                    const cplx<double> Pm1 = (-mnl) * (0.5 * z1 + conj(pv));
-#pragma unroll
-                    for (int w = 0; w < 4; ++w) part1[i][w] -= dw * (Pm1 * K.A(tp1[w], l, n)).re;
-                }
-            } else if constexpr (KIND == 1) {
-                if (same) continue;
-                // (W^{bd} + V''^b D'^d + V''^d D'^b)_nl / 2
-                const double mnl = K.m(n, l);
-                cplx<double> z = gz[bj][v];
-                if (single)
-                    z -= mnl * (K.V(b, n, n) * K.V(d, n, l) + K.V(d, n, n) * K.V(b, n, l));
-                else
-                    for (int q = S.gs[n]; q < S.ge[n]; ++q)
-                        z += K.V(b, n, q) * K.Dp(d, q, l) + K.V(d, n, q) * K.Dp(b, q, l);
-                const cplx<double> Pm = (-mnl) * (0.5 * z - pv);
-#pragma unroll
-                for (int w = 0; w < 4; ++w) part[i][w] -= dw * (Pm * K.A(tp[w], l, n)).re;
-            } else if constexpr (KIND == 2) {
-                if (!same) {
-                    // A'^{bd} - D'^d A''^b
-                    const double mln = K.m(l, n);
-                    cplx<double> z = gz[bj][v];
-                    if (single)
-                        z += mln * (K.V(d, l, n) * K.A(b, n, n));
-                    else
-                        for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.A(b, q, n);
-                    const cplx<double> Y1 = pv + z;
-#pragma unroll
-                    for (int w = 0; w < 2; ++w) part[j][w] -= dw * ((mln * K.V(tp[w], n, l)) * Y1).re;
-                } else if (dm) {
-                    // s5: Re(-2i sig A^a_nl E_l (A' + A^b D'^d)_ln)
-                    const cplx<double> dA = pv + K.Ad(b, d, l, n);
-#pragma unroll
-                    for (int w = 0; w < 2; ++w) part[j][w] += 2.0 * K.E[l] * (K.A(tp[w], n, l) * dA).im;
-                }
-            } else if constexpr (KIND == 3) {
-                if (same) continue;
-                // B'^{bd} - D'^d B''^b
-                cplx<double> z = gz[bj][v];
-                for (int q = S.gs[n]; q < S.ge[n]; ++q) z -= K.Dp(d, l, q) * K.B(b, q, n);
-                const cplx<double> YB = pv + z;
-#pragma unroll
-                for (int w = 0; w < 2; ++w) part[j][w] -= 2.0 * (K.Dp(tp[w], n, l) * YB).re;  // s6
-            } else if constexpr (KIND == 4) {
-                if (!same) continue;
-                // s5: -Re(-2i sig A^a_nl E_l (D'^d A^b)_ln) ; Omega block: 2 sign coef Re(DA_ln V^e_nl)
-#pragma unroll
-                for (int w = 0; w < 2; ++w) part[j][w] -= 2.0 * K.E[l] * (K.A(tp[w], n, l) * pv).im;
-#pragma unroll
-                for (int w = 2; w < 5; ++w) part[j][w] += (pv * K.V(tp[w], n, l)).re;
-            } else if constexpr (KIND == 5 || KIND == 6) {
-                if (!same) continue;
-                // Omega block term -i P_ln: 2 sign Re(-i P_ln V^e_nl)
-#pragma unroll
-                for (int w = 0; w < 3; ++w) part[j][w] += (pv * K.V(tp[w], n, l)).im;
-            } else {
-                if (!same) continue;
-                // s4: Re(-2i A^al_np (V''^d A''^be)_pn)
-                part[j][0] += (K.A(tp[0], n, l) * pv).im;
-            }
-        }
-    }
-    // reduce over the summed index l: the rows, lane bits 4-5
-    const int mask = 0x30;
-    const bool writer = (lane >> 4) == 0;
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-        const int n = c0 + mfma_col(u);
-#pragma unroll
-        for (int w = 0; w < NT; ++w) {
-            const double sum = wave_sum(part[u][w], mask);
-            if (writer && tw[w] != 0.0 && n < nw) accw[n * 9 + tcol[w]] += tw[w] * sum;
-        }
-        if constexpr (M1) {
-#pragma unroll
-            for (int w = 0; w < NT1; ++w) {
-                const double sum = wave_sum(part1[u][w], mask);
-                if (writer && tw1[w] != 0.0 && n < nw) accw[n * 9 + tcol1[w]] += tw1[w] * sum;
-            }
-        }
-    }
-}
-
-// DerOmega (formula/covariant.py:212-259) or, dm = true, DerMorb
-// (formula/covariant.py:463-552) per-band values into S.acc[n][c*3+d].
-// DerMorb = DerMorb_H + sign [Eav o dOmega + (Omega V + V Omega)/2] on cut
-// partitions; its trace terms (s1 ... s7, the Omega block) are listed at
-// formula_task.
-template <typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST>& K, bool in, bool ex,
-                                                            bool dm, int sign) {
-    const int nw = K.nw, tid = threadIdx.x, wave = tid >> 6;
-    // per-band terms, lanes running over (n, c, d)
-    for (int t = tid; t < nw * 9; t += SC_THREADS) {
-        const int n = t % nw, c = (t / nw) / 3, d = (t / nw) % 3;
-        double v = 0.0;
-        if (S.act[n] && ex) {
-            cplx<double> s = mk(0.0, 0.0);
-            const int g0 = S.gs[n], g1 = S.ge[n];
-            for (int sg = 0; sg < 2; ++sg) {
-                const int a = sg == 0 ? c_alpha[c] : c_beta[c];
-                const int b = sg == 0 ? c_beta[c] : c_alpha[c];
-                const double sig = sg == 0 ? 1.0 : -1.0;
-                // T4: -i sigma sum_{n' in G} A^a_nn' dA.nn^{bd}_n'n
-                for (int m = g0; m < g1; ++m) {
-                    cplx<double> y2 = K.Ad(b, d, m, n);
-#pragma unroll 4
-                    for (int l = 0; l < nw; ++l) y2 += K.A(b, m, l) * K.Dp(d, l, n) - K.Dp(d, m, l) * K.A(b, l, n);
-                    s += times_i(-(sig * (K.A(a, n, m) * y2)));
-                }
-            }
-            // T5: 0.5 dO.nn = 0.5 [O'^{cd}_nn - (D'^d O^c)_nn + (O^c D'^d)_nn].  O is Hermitian
-            // and D' anti-Hermitian, so (D'^d O^c)_nn = -conj((O^c D'^d)_nn) and only the real part,
-            // Re O'_nn + 2 Re sum_l conj(O_ln) D'_ln, is used: column n of O, which consecutive
-            // threads (consecutive n) read from consecutive addresses
-            double t5 = K.Od(c, d, n, n).re;
-#pragma unroll 8
-            for (int l = 0; l < nw; ++l) t5 += 2.0 * (conj(K.O(c, l, n)) * K.Dp(d, l, n)).re;
-            s.re += 0.5 * t5;
-            v = (dm ? 2.0 * sign * S.E[n] : 2.0) * s.re;
-            if (dm) {
-                // s3: dH.nn = C'^{cd}_nn - (D'^d C^c)_nn + (C^c D'^d)_nn
-                cplx<double> s3 = K.Cd(c, d, n, n);
-                // Omega block, elementwise part: 2 sign Re sum_{l in G} (O^c_nl / 2) V^d_ln
-                cplx<double> ov = mk(0.0, 0.0);
-                for (int l = 0; l < nw; ++l) {
-                    s3 += K.C(c, n, l) * K.Dp(d, l, n) - K.Dp(d, n, l) * K.C(c, l, n);
-                    if (K.lab[l] == K.lab[n]) ov += K.O(c, n, l) * K.V(d, l, n);
-                }
-                v += s3.re + sign * ov.re;
-            }
-        }
-        S.acc[n * 9 + c * 3 + d] = mk(v, 0.0);
-    }
-    for (int t = tid; t < SC_WAVES * nw * 9; t += SC_THREADS) S.accw[t] = 0.0;
-    __syncthreads();
-    // wave tasks (kind, b, d, row block, column block), round-robin over the waves
-    const TaskCtx tcx{dm, in, ex, (double)sign};
-    int cnt[8];
-    cnt[0] = (in || (dm && ex)) ? 9 : 0;
-    cnt[1] = ex ? 9 : 0;
-    cnt[2] = ex ? 9 : 0;
-    cnt[3] = dm && ex ? 9 : 0;
-    cnt[4] = dm && ex ? 9 : 0;
-    cnt[5] = dm && in && sign != 0 ? 3 : 0;
-    cnt[6] = dm && ex && sign != 0 ? 3 : 0;
-    cnt[7] = dm && ex ? 9 : 0;
-    if (dm && sign == 0) cnt[1] = 0;  // T3 feeds only the dOmega term
-    // kind-1 tasks ride on the kind-0 tasks of the transposed index pair when both run
-    const bool merge01 = cnt[0] > 0 && cnt[1] > 0;
-    if (merge01) cnt[1] = 0;
-    const int nblk = (nw + 31) / 32, nb2 = nblk * nblk;
-    int ntask = 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) ntask += cnt[q] * nb2;
-    double* accw = S.accw + wave * nw * 9;
-    for (int t = wave; t < ntask; t += SC_WAVES) {
-        int kind = -1, rem = 0, base = 0;
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int nq = cnt[q] * nb2;
-            if (kind < 0 && t - base < nq) {
-                kind = q;
-                rem = t - base;
-            }
-            base += nq;
-        }
-        const int idx = rem / nb2, rb = (rem % nb2) / nblk, cb = rem % nblk;
-        const int b = kind == 5 || kind == 6 ? idx : idx / 3, d = kind == 5 || kind == 6 ? 0 : idx % 3;
-        switch (kind) {
-            case 0:
-                if (merge01)
-                    formula_task<0, true>(S, K, tcx, accw, b, d, rb, cb);
-                else
-                    formula_task<0, false>(S, K, tcx, accw, b, d, rb, cb);
-                break;
-            case 1: formula_task<1, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 2: formula_task<2, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 3: formula_task<3, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 4: formula_task<4, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 5: formula_task<5, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 6: formula_task<6, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            default: formula_task<7, false>(S, K, tcx, accw, b, d, rb, cb); break;
-        }
-    }
-    __syncthreads();
-    for (int t = tid; t < nw * 9; t += SC_THREADS) {
-        double v = S.acc[t].re;
-        for (int w = 0; w < SC_WAVES; ++w) v += S.accw[w * nw * 9 + t];
-        S.acc[t] = mk(v, 0.0);
-    }
-}
-
-// Per-band values for the current partition into S.acc[n][comp] (.re).
-template <int F, typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM, ST>& K, const wbmi_scan_params& p) {
-    const int nw = K.nw;
-    const int tid = threadIdx.x;
-    const bool in = p.internal_terms, ex = p.external_terms;
-    if constexpr (LM) build_M(S, nw);
-    for (int t = tid; t < nw * 9; t += SC_THREADS) S.acc[t] = mk(0.0, 0.0);
-    __syncthreads();
-    if (F == WBMI_FORMULA_IDENTITY) {
-        for (int n = tid; n < nw; n += SC_THREADS)
-            if (S.act[n]) S.acc[n * 9] = mk(1.0, 0.0);
-    } else if (F == WBMI_FORMULA_OMEGA) {
-        if constexpr (ST) {
-            // V and A in LDS: four adjacent lanes share (n, c) and split the sum over l,
-            // reduced by two shuffles (a uniform loop, so every lane reaches them)
-            const int L4 = (nw + 3) / 4;
-            for (int t0 = 0; t0 < nw * 12; t0 += SC_THREADS) {
-                const int t = t0 + tid;
-                const bool ok = t < nw * 12;
-                const int n = ok ? t / 12 : 0, c = (t / 4) % 3, qq = t % 4;
-                const bool act = ok && S.act[n];
-                double v = 0.0;
-                if (act) {
-                    const int l0 = qq * L4, l1 = l0 + L4 < nw ? l0 + L4 : nw;
-                    v = omega_s(K, c, n, n, in, ex, l0, l1, qq == 0).re;
-                }
-                v += __shfl_xor(v, 1);
-                v += __shfl_xor(v, 2);
-                if (act && qq == 0) S.acc[n * 9 + c] = mk(2.0 * v, 0.0);
-            }
-        } else {
-            for (int t = tid; t < nw * 3; t += SC_THREADS) {
-                int n = t / 3, c = t % 3;
-                if (!S.act[n]) continue;
-                S.acc[n * 9 + c] = mk(2.0 * omega_s(K, c, n, n, in, ex).re, 0.0);
-            }
-        }
-    } else if (F == WBMI_FORMULA_MORB_HPM) {
-        for (int t = tid; t < nw * 3; t += SC_THREADS) {
-            int n = t / 3, c = t % 3;
-            if (!S.act[n]) continue;
-            double v = 2.0 * morbh_s(K, c, n, n, in, ex).re;
-            if (p.sign != 0) v += p.sign * S.E[n] * 2.0 * omega_s(K, c, n, n, in, ex).re;
-            S.acc[n * 9 + c] = mk(v, 0.0);
-        }
-    } else if (F == WBMI_FORMULA_VELOMEGA) {
-        // Re sum_{n' in G} V^a_nn' Omega^b_n'n, Omega = s + s^+ (formula/formula.py:141-148)
-        for (int t = tid; t < nw * 9; t += SC_THREADS) {
-            int n = t / 9, a = (t % 9) / 3, b = t % 3;
-            if (!S.act[n]) continue;
-            cplx<double> v = mk(0.0, 0.0);
-            for (int m = S.gs[n]; m < S.ge[n]; ++m) {
-                cplx<double> om = omega_s(K, b, m, n, in, ex) + conj(omega_s(K, b, n, m, in, ex));
-                v += K.V(a, n, m) * om;
-            }
-            S.acc[t] = mk(v.re, 0.0);
-        }
-    } else if (F == WBMI_FORMULA_VELHPLUS) {
-        // Re sum_{n' in G} V^a_nn' Hpm^b_n'n with the group block of Morb_Hpm,
-        // Hpm = Morb_H.nn + sign Eav o Omega.nn (formula/covariant.py:424-449,
-        // formula/formula.py:141-148; FormulaProduct is additive)
-        for (int t = tid; t < nw * 9; t += SC_THREADS) {
-            int n = t / 9, a = (t % 9) / 3, b = t % 3;
-            if (!S.act[n]) continue;
-            cplx<double> v = mk(0.0, 0.0);
-            for (int m = S.gs[n]; m < S.ge[n]; ++m) {
-                cplx<double> h = morbh_s(K, b, m, n, in, ex) + conj(morbh_s(K, b, n, m, in, ex));
-                if (p.sign != 0)
-                    h += (p.sign * 0.5 * (S.E[m] + S.E[n])) *
-                         (omega_s(K, b, m, n, in, ex) + conj(omega_s(K, b, n, m, in, ex)));
-                v += K.V(a, n, m) * h;
-            }
-            S.acc[t] = mk(v.re, 0.0);
-        }
-    } else if (F == WBMI_FORMULA_DEROMEGA) {
-        eval_deromega(S, K, in, ex, false, 0);
-    } else if (F == WBMI_FORMULA_DERMORB) {
-        eval_deromega(S, K, in, ex, true, p.sign);
-    }
-    __syncthreads();
-}
-
-// Fermi-level bin of a group with mean energy E (static.py:131-136): 0 for
-// E < EFmin (added to every level), ceil((E - EFmin)/dEF) up to EFmax, -1 = none.
-__device__ inline int fermi_bin(const wbmi_scan_params& p, double E) {
-    if (E < p.EFmin) return 0;
-    if (E <= p.EFmax) {
-        const int bin = (int)ceil((E - p.EFmin) / p.dEF);
-        return bin < p.nEF_extra ? bin : -1;
-    }
-    return -1;
-}
-
-template <typename T, int F, bool LM, bool ST>
-__global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
-                                                            const cplx<T>* __restrict__ Xall,
-                                                            double* __restrict__ hist) {
-    extern __shared__ __attribute__((aligned(16))) char smem_raw[];
-    const wbmi_scan_params& p = args.p;
-    const int nw = args.nw, ld = nw + 1;
-    const int64_t k = blockIdx.x;
-    const int tid = threadIdx.x;
-    constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
-    static_assert(tasks || F == WBMI_FORMULA_OMEGA || !ST, "V and A are staged for DerOmega, DerMorb and Omega");
-    constexpr bool stage_v = ST;
-    const bool stage_a = ST && p.external_terms;
-    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tasks);
-    for (int n = tid; n < nw; n += SC_THREADS) S.E[n] = Eall[k * nw + n];
-    for (int r = tid; r < WBMI_NROLES; r += SC_THREADS) S.ch[r] = p.ch[r];
-    __syncthreads();
-    const cplx<T>* Xk = Xall + k * (int64_t)args.nch * nw * nw;
-    if (stage_v || stage_a) {
-        // staged only for nw <= SC_LDS_NW: 3 nw^2 <= 3072 = 6 x SC_THREADS elements per array;
-        // all of a thread's loads are issued before its LDS stores
-        constexpr int NS = (3 * SC_LDS_NW * SC_LDS_NW + SC_THREADS - 1) / SC_THREADS;
-        const int nw2 = nw * nw;
-        cplx<T> bv[NS], ba[NS];
-#pragma unroll
-        for (int u = 0; u < NS; ++u) {
-            const int t = tid + SC_THREADS * u;
-            if (t < 3 * nw2) {
-                const int a = t / nw2, xy = t % nw2;
-                if (stage_v) bv[u] = Xk[(int64_t)S.ch[WBMI_ROLE_V + a] * nw2 + xy];
-                if (stage_a) ba[u] = Xk[(int64_t)S.ch[WBMI_ROLE_A + a] * nw2 + xy];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < NS; ++u) {
-            const int t = tid + SC_THREADS * u;
-            if (t < 3 * nw2) {
-                const int a = t / nw2, x = (t % nw2) / nw, y = t % nw;
-                if (stage_v) S.V[(a * nw + x) * ld + y] = mk((double)bv[u].re, (double)bv[u].im);
-                if (stage_a) S.A[(a * nw + x) * ld + y] = mk((double)ba[u].re, (double)ba[u].im);
-            }
-        }
-    }
-    constexpr int ncomp = ncomp_of<F>();
-    constexpr bool additive = F == WBMI_FORMULA_IDENTITY || F == WBMI_FORMULA_OMEGA ||
-                              F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
-                              F == WBMI_FORMULA_VELHPLUS;
-
-    // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186)
-    if (tid == 0) {
-        S.gb[0] = 0;
-        int nb = 1;
-        for (int i = 1; i < nw; ++i)
-            if (S.E[i] - S.E[i - 1] > p.degen_thresh) S.gb[nb++] = i;
-        S.gb[nb++] = nw;
-        if (p.degen_Kramers) {
-            int m = 0;
-            for (int i = 0; i < nb; ++i)
-                if (S.gb[i] % 2 == 0) S.gb[m++] = S.gb[i];
-            nb = m;
-        }
-        const int ng = nb - 1;
-        int first_in = -1;
-        for (int g = 0; g < ng; ++g) {
-            int a = S.gb[g], b = S.gb[g + 1];
-            int inr = (S.E[b - 1] >= p.EFmin) && (S.E[a] <= p.EFmax);
-            S.gin[g] = inr;
-            if (inr && first_in < 0) first_in = a;
-            double sum = 0.0;
-            for (int i = a; i < b; ++i) sum += S.E[i];
-            S.gw[g] = sum / (b - a);
-        }
-        int bandmax = 0;
-        if (p.fder == 0) {
-            for (int i = 0; i < nw; ++i)
-                if (S.E[i] < p.EFmin) bandmax = i + 1;
-            if (first_in >= 0 && first_in < bandmax) bandmax = first_in;
-        }
-        S.misc[0] = ng;
-        S.misc[1] = bandmax;
-    }
-    __syncthreads();
-    const int ng = S.misc[0];
-    const int bandmax = S.misc[1];
-
-    KCtx<T, LM, ST> K;
-    K.X = Xk;
-    K.nw = nw;
-    K.ld = ld;
-    K.ch = S.ch;
-    K.E = S.E;
-    K.M = S.M;
-    K.lab = S.lab;
-    K.gs = S.gs;
-    K.ge = S.ge;
-    K.sV = S.V;
-    K.sA = S.A;
-
-    if (additive) {
-        // main partition: every group at once
-        for (int g = 0; g < ng; ++g)
-            for (int n = S.gb[g] + tid; n < S.gb[g + 1]; n += SC_THREADS) {
-                S.lab[n] = g;
-                S.gs[n] = S.gb[g];
-                S.ge[n] = S.gb[g + 1];
-                S.act[n] = S.gin[g];
-            }
-        __syncthreads();
-        eval_partition<F>(S, K, p);
-        for (int t = tid; t < ng * ncomp; t += SC_THREADS) {
-            const int g = t / ncomp, c = t % ncomp;
-            const int bin = S.gin[g] ? fermi_bin(p, S.gw[g]) : -1;
-            if (bin < 0) continue;
-            double s = 0.0;
-            for (int n = S.gb[g]; n < S.gb[g + 1]; ++n) s += S.acc[n * 9 + c].re;
-            unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s);
-        }
-        __syncthreads();
-        if (bandmax > 0) {
-            // Fermi-sea group (0, bandmax) with weight -inf
-            for (int n = tid; n < nw; n += SC_THREADS) {
-                int inn = n < bandmax;
-                S.lab[n] = inn ? 0 : 1;
-                S.gs[n] = inn ? 0 : bandmax;
-                S.ge[n] = inn ? bandmax : nw;
-                S.act[n] = inn;
-            }
-            __syncthreads();
-            eval_partition<F>(S, K, p);
-            for (int c = tid; c < ncomp; c += SC_THREADS) {
-                double s = 0.0;
-                for (int n = 0; n < bandmax; ++n) s += S.acc[n * 9 + c].re;
-                unsafeAtomicAdd(hist + c, args.weight * s);  // bin 0: below EFmin
-            }
-            __syncthreads();
-        }
-    } else {
-        // non-additive: traces of the cumulative cuts [0, c) (static.py:109-117)
-        for (int cidx = 1; cidx <= nw; ++cidx) {
-            bool need = (bandmax == cidx);
-            for (int g = 0; g < ng; ++g)
-                if (S.gin[g] && (S.gb[g] == cidx || S.gb[g + 1] == cidx)) need = true;
-            if (!need) continue;
-            for (int n = tid; n < nw; n += SC_THREADS) {
-                int inn = n < cidx;
-                S.lab[n] = inn ? 0 : 1;
-                S.gs[n] = inn ? 0 : cidx;
-                S.ge[n] = inn ? cidx : nw;
-                S.act[n] = inn;
-            }
-            __syncthreads();
-            eval_partition<F>(S, K, p);
-            if (tid == 0)
-                for (int c = 0; c < ncomp; ++c) {
-                    double s = 0.0;
-                    for (int n = 0; n < cidx; ++n) s += S.acc[n * 9 + c].re;
-                    S.cut[cidx * 9 + c] = s;
-                }
-            __syncthreads();
-        }
-        for (int c = tid; c < ncomp; c += SC_THREADS) S.cut[c] = 0.0;
-        __syncthreads();
-        for (int t = tid; t < ng * ncomp; t += SC_THREADS) {
-            const int g = t / ncomp, c = t % ncomp;
-            const int bin = S.gin[g] ? fermi_bin(p, S.gw[g]) : -1;
-            if (bin < 0) continue;
-            const double s = S.cut[S.gb[g + 1] * 9 + c] - S.cut[S.gb[g] * 9 + c];
-            unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s);
-        }
-        if (bandmax > 0)
-            for (int c = tid; c < ncomp; c += SC_THREADS)
-                unsafeAtomicAdd(hist + c, args.weight * S.cut[bandmax * 9 + c]);
-    }
-}
+// Entry points of the covariant formula traces + Fermi-level scan of libwbmi
+// (calculators/static.py:60-169) and the histogram finalisation.  The kernels
+// are in scan_kernels.h; scan_f64.hip, scan_f32.hip, scan_tetra_f64.hip and
+// scan_tetra_f32.hip instantiate them.
+#include "scan_kernels.h"
 
 static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
     auto need = [&](int base, int n) -> int {
@@ -1023,45 +47,6 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
     return WBMI_OK;
 }
 
-template <typename T, int F, bool LM, bool ST>
-static int launch_scan_m(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
-    const bool stage_a = ST && a.p.external_terms;
-    size_t lds = smem_offsets(a.nw, ST, stage_a, tasks).total;
-    if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM, ST>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_static_scan<T, F, LM, ST>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
-                       (const cplx<T>*)X, hist);
-    WBMI_LAUNCH_CHECK(ctx);
-    return WBMI_OK;
-}
-
-// dEinv o Off in LDS up to SC_M_NW (the layout of smem_offsets), formed on the fly above;
-// V and A in LDS up to SC_LDS_NW for the formulas with wave tasks
-template <typename T, int F>
-static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    constexpr bool staged = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB || F == WBMI_FORMULA_OMEGA;
-    if constexpr (staged)
-        if (a.stage_va) return launch_scan_m<T, F, true, true>(ctx, nk, a, E, X, hist);
-    if (a.nw <= SC_M_NW) return launch_scan_m<T, F, true, false>(ctx, nk, a, E, X, hist);
-    return launch_scan_m<T, F, false, false>(ctx, nk, a, E, X, hist);
-}
-
-template <typename T>
-static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    switch (a.p.formula) {
-        case WBMI_FORMULA_IDENTITY: return launch_scan<T, WBMI_FORMULA_IDENTITY>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_OMEGA: return launch_scan<T, WBMI_FORMULA_OMEGA>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_DEROMEGA: return launch_scan<T, WBMI_FORMULA_DEROMEGA>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_VELOMEGA: return launch_scan<T, WBMI_FORMULA_VELOMEGA>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_MORB_HPM: return launch_scan<T, WBMI_FORMULA_MORB_HPM>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_VELHPLUS: return launch_scan<T, WBMI_FORMULA_VELHPLUS>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_DERMORB: return launch_scan<T, WBMI_FORMULA_DERMORB>(ctx, nk, a, E, X, hist);
-        default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
-    }
-}
-
 extern "C" int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
                                 int prec, const wbmi_scan_params* params, double weight, double* hist_dev) {
     if (!ctx) return WBMI_ERR_ARG;
@@ -1078,9 +63,42 @@ extern "C" int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const do
     a.nch = nch;
     a.stage_va = nw <= SC_LDS_NW;
     a.weight = weight;
-    if (prec == WBMI_F64) return dispatch_scan<double>(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
-    if (prec == WBMI_F32) return dispatch_scan<float>(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
+    a.tetra = 0;
+    a.nEF = 0;
+    a.Ec = nullptr;
+    a.ec_stride = 0;
+    a.Ef = nullptr;
+    if (prec == WBMI_F64) return scan_dispatch_f64(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
+    if (prec == WBMI_F32) return scan_dispatch_f32(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: bad precision");
+}
+
+extern "C" int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                                      const double* Ecorners_dev, const void* Xbar_dev, int prec,
+                                      const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
+                                      double weight, double* hist_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (nk <= 0 || nw <= 0 || nw > SC_NWMAX || nch <= 0 || !E_dev || !Ecorners_dev || !params || !hist_dev ||
+        !Efermi_dev || nEF <= 0 || (params->formula != WBMI_FORMULA_IDENTITY && !Xbar_dev))
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad arguments (nw <= %d)", SC_NWMAX);
+    if (params->fder < 0 || params->fder > 3 || !(params->EFmax >= params->EFmin))
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad fder or Fermi range");
+    int rc = check_roles(ctx, params, nch);
+    if (rc) return rc;
+    ScanArgs a;
+    a.p = *params;
+    a.nw = nw;
+    a.nch = nch;
+    a.stage_va = nw <= SC_LDS_NW;
+    a.weight = weight;
+    a.tetra = 1;
+    a.nEF = nEF;
+    a.Ec = Ecorners_dev;
+    a.ec_stride = (int64_t)nk * nw;
+    a.Ef = Efermi_dev;
+    if (prec == WBMI_F64) return scan_dispatch_tetra_f64(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
+    if (prec == WBMI_F32) return scan_dispatch_tetra_f32(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad precision");
 }
 
 // ---------------------------------------------------------------------------
@@ -1131,6 +149,27 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(double* __restrict__ h
                 (2 * dEF * dEF * dEF);
         out[t] = v * scale;
     }
+}
+
+// out += scale * add (the directly accumulated rows of the tetrahedron histogram)
+__global__ void k_add_scaled(double* __restrict__ out, const double* __restrict__ add, double scale, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += scale * add[i];
+}
+
+extern "C" int wbmi_finalize_tetra(wbmi_ctx* ctx, double* hist_dev, int nEF, int ncomp, double scale,
+                                   double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (!hist_dev || !out_dev || ncomp <= 0 || nEF <= 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize_tetra: bad arguments");
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF, ncomp, 0, 1.0, scale,
+                       out_dev);
+    WBMI_LAUNCH_CHECK(ctx);
+    const int64_t n = (int64_t)nEF * ncomp;
+    hipLaunchKernelGGL(k_add_scaled, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, out_dev,
+                       hist_dev + n, scale, n);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
 }
 
 extern "C" int wbmi_finalize(wbmi_ctx* ctx, double* hist_dev, int nEF_extra, int ncomp, int fder, double dEF,

@@ -191,6 +191,7 @@ class Data_K:
         self.Xk = None
         self.E_dev = None
         self.U_dev = None
+        self._Ec = None
 
     # -- evaluation -------------------------------------------------------------
     def request(self, keys):
@@ -206,8 +207,9 @@ class Data_K:
             self._requested |= set(self._slots)
         self._evaluate(sorted(self._requested, key=CHANNEL_ORDER.index))
 
-    def _transform(self, keys, dtype, prec):
-        """R -> k of the channel keys into a new [nk, nch, nw, nw] tensor."""
+    def _transform(self, keys, dtype, prec, shift=None):
+        """R -> k of the channel keys into a new [nk, nch, nw, nw] tensor (grid blocks:
+        at the points j/NKFFT + dK, or + shift in place of dK)."""
         torch = self.torch
         nw, nk = self.num_wann, self.nk
         stack = self.dev.stack(keys)
@@ -215,7 +217,7 @@ class Data_K:
         if self.k_list is None:
             plan, box, aliased = self.dev.plan(keys, self.NKFFT)
             work = torch.empty(plan.work_bytes, dtype=torch.uint8, device=self.ctx.device)
-            plan.execute(stack if aliased else None, box, self.dK, work, X, prec)
+            plan.execute(stack if aliased else None, box, self.dK if shift is None else shift, work, X, prec)
             del work
         else:
             kl = torch.from_numpy(self.k_list).to(self.ctx.device)
@@ -271,6 +273,36 @@ class Data_K:
             if skip.sum() < nch:
                 self.ctx.rotate(U, X, herm, skip, self.prec)
         self.Xk, self.E_dev, self.U_dev, self._slots, self.nch = X, E, U, slots, nch
+
+    def corner_energies(self):
+        """Band energies [8, nk, nw] (FP64, device) at the corners
+        k + ((2i-1), (2j-1), (2l-1)) * dK_fullBZ / 2 of the parallelepiped around each k,
+        corner i*4 + j*2 + l (data_K_R.py:126-141): the Hamiltonian transformed to the
+        shifted grid and diagonalised (FP32 Jacobi + FP64 refinement, FP64-accurate
+        energies) once per corner."""
+        if self._Ec is None:
+            if self.k_list is not None or self.Kpoint is None:
+                raise ValueError("the tetrahedron method needs a grid K-block (Kpoint with dK)")
+            self.ensure(set())
+            torch = self.torch
+            nw, nk = self.num_wann, self.nk
+            half = np.asarray(self.Kpoint.dK_fullBZ, dtype=float) / 2
+            Ec = torch.empty((8, nk, nw), dtype=torch.float64, device=self.ctx.device)
+            U = torch.empty((nk, nw, nw), dtype=torch.complex128, device=self.ctx.device)
+            for c in range(8):
+                sgn = np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1]) * 2 - 1
+                H = self._transform([("Ham", 0)], torch.complex128, WBMI_F64, shift=self.dK + sgn * half)
+                self.ctx.eigh(H[:, 0], nw * nw, Ec[c], U, WBMI_F64, nw, refine=OA_STEPS)
+                del H
+            self._Ec = Ec
+        return self._Ec
+
+    def static_scan_tetra(self, params, Efermi, hist, weight=1.0):
+        """Tetrahedron-weighted scan into hist [2 nEF, ncomp] (see wbmi_static_scan_tetra)."""
+        Ec = self.corner_energies()
+        for i, v in enumerate(self.role_table()):
+            params.ch[i] = v
+        self.ctx.static_scan_tetra(self.E_dev, Ec, self.Xk, self.prec, params, Efermi, weight, hist)
 
     def role_table(self):
         """Channel slot per formula role (include/wbmi.h WBMI_ROLE_*), -1 if absent."""
