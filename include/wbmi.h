@@ -85,6 +85,7 @@ typedef struct wbmi_optcond_params {
     const double* Efermi; /* device [nEF]                                     */
     const double* omega;  /* device [nomega]                                  */
     double EFmin, EFmax;  /* min / max of Efermi (occupation window)          */
+    double omega_min, omega_max; /* min / max of omega (JDOS pair window)      */
     double kBT;           /* 0: step occupations (utility.py:172-182)         */
     double eta;           /* smr_fixed_width                                  */
     int smr_gaussian;     /* 1: Gaussian, 0: Lorentzian smearing              */
@@ -189,6 +190,16 @@ int wbmi_eigh_guess(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hs
  * (Gaussian: imaginary part pi G) into out_dev [nEF, nomega, 3, 3] complex128
  * (interleaved doubles), accumulated across calls; the caller scales by
  * factor_opt / (nk V).  nw <= 32.                                          */
+/* Joint density of states (calculators/dynamic.py:145-167): per k, band-group
+ * pairs (m, n) with E_m < EFmax, E_n > EFmin and omega_min - 5 eta <
+ * E_m - E_n < omega_max + 5 eta add |m| |n| (f(E_n) - f(E_m)) smear(E_m - E_n - w).
+ *   E_dev   : float64 [nk, nw]
+ *   params  : the optical-conductivity parameters (channels not read)
+ *   out_dev : float64 [nEF, nomega], accumulated (zero it first); the caller
+ *             scales by 1 / (nk V).                                           */
+int wbmi_jdos(wbmi_ctx* ctx, int nk, int nw, const double* E_dev, const wbmi_optcond_params* params,
+              double* out_dev);
+
 int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
                               int prec, const wbmi_optcond_params* params, double* out_dev);
 

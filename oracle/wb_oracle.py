@@ -1131,6 +1131,7 @@ class GME_orb_FermiSurf(StaticCalculator):
 # ---------------------------------------------------------------------------
 
 FACTOR_OPT = -FACTOR_AHC   # factors.py:36
+FACTOR_SHC = -FACTOR_AHC   # factors.py:37
 
 
 def fermi_dirac(E, mu, kBT):
@@ -1212,6 +1213,98 @@ class OpticalConductivity:
             restot += fw @ (fE[:, :, None] * M.reshape(len(pairs), -1)[:, None, :]).reshape(len(pairs), -1)
         restot = restot.reshape((len(self.omega), len(self.Efermi), 3, 3)).swapaxes(0, 1)
         return OracleResult(self.Efermi, restot * FACTOR_OPT / (data.nk * data.system.cell_volume))
+
+
+class SHC(OpticalConductivity):
+    """Kubo spin Hall conductivity (dynamic.py:204-241) with the Ryoo spin current
+    (formula/covariant.py:689-756).
+
+    J^{as} = (J' + J'^+)/2, J'^{as}_mn = -i (E_n SA^{as}_mn - SHA^{as}_mn) + sum_l SS^s_ml V^a_ln;
+    B^b = -i A^b, A = i D_H (+ Abar); per pair (m, n) of band groups
+    M^{abs} = sum_{i in m, j in n} Im(J^{as}_ij B^b_ji), times (f(E_n) - f(E_m)) per
+    Fermi level and 1 / (2 (E_m - E_n - omega - i eta)) per frequency (Gaussian: the
+    imaginary part pi G / 2).  Result [nEF, nomega, 3, 3, 3] complex, x factor_shc / (nk V).
+    """
+
+    def __init__(self, Efermi, omega, SHC_type="ryoo", **kw):
+        super().__init__(Efermi, omega, **kw)
+        if SHC_type != "ryoo":
+            raise NotImplementedError(SHC_type)
+
+    def factor_omega(self, E1, E2):
+        arg = E1 - E2 - self.omega
+        cfac = 1. / (arg - 1j * self.eta)
+        if self.smr_type != "Lorentzian":
+            cfac.imag = np.pi * gaussian(arg, self.eta)
+        return cfac / 2
+
+    def __call__(self, data):
+        ext = self.kwargs_formula.get("external_terms", True) and not data.system.force_internal_terms_only
+        if not ext:
+            raise NotImplementedError("the Ryoo spin current needs the external terms (covariant.py:747-749)")
+        E = data.E_K
+        J = -1j * (E[:, None, :, None, None] * data.Xbar("SA") - data.Xbar("SHA"))
+        J = J + np.einsum("kmls,klna->kmnas", data.Xbar("SS"), data.Xbar("Ham", 1))
+        J = (J + J.swapaxes(1, 2).conj()) / 2
+        A = 1j * data.D_H + data.Xbar("AA")
+        B = -1j * A
+        imAB = np.imag(J[:, :, :, :, None, :] * B.swapaxes(1, 2)[:, :, :, None, :, None])
+        restot = np.zeros((len(self.omega), len(self.Efermi) * 27), dtype=complex)
+        for ik in range(data.nk):
+            groups = data.get_bands_in_range_groups_ik(ik, -np.inf, np.inf, degen_thresh=self.degen_thresh,
+                                                       degen_Kramers=self.degen_Kramers)
+            pairs = [(m, n, Em, En) for m, Em in groups.items() for n, En in groups.items() if self.nonzero(Em, En)]
+            if not pairs:
+                continue
+            M = np.array([imAB[ik, p[0][0]:p[0][1]].sum(axis=0)[p[1][0]:p[1][1]].sum(axis=0) for p in pairs])
+            fE = np.array([fermi_dirac(p[3], self.Efermi, self.kBT) - fermi_dirac(p[2], self.Efermi, self.kBT)
+                           for p in pairs])
+            fw = np.array([self.factor_omega(p[2], p[3]) for p in pairs]).T
+            restot += fw @ (fE[:, :, None] * M.reshape(len(pairs), -1)[:, None, :]).reshape(len(pairs), -1)
+        restot = restot.reshape((len(self.omega), len(self.Efermi), 3, 3, 3)).swapaxes(0, 1)
+        return OracleResult(self.Efermi, restot * FACTOR_SHC / (data.nk * data.system.cell_volume))
+
+
+class JDOS:
+    """Joint density of states (dynamic.py:26-131 with JDOS 145-167): per k, every pair
+    (m, n) of band groups with E_m < max(Efermi), E_n > min(Efermi) and
+    min(omega) - 5 eta < E_m - E_n < max(omega) + 5 eta adds
+    |m| |n| (f(E_n) - f(E_m)) * smear(E_m - E_n - omega).  Result [nEF, nomega] real,
+    / (nk V)."""
+
+    def __init__(self, Efermi, omega, kBT=0, smr_fixed_width=0.1, smr_type="Lorentzian", degen_thresh=1e-4,
+                 degen_Kramers=False):
+        self.Efermi = np.array(Efermi, dtype=float).reshape(-1)
+        self.omega = np.array(omega, dtype=float)
+        self.kBT = kBT
+        self.eta = smr_fixed_width
+        self.smr_type = smr_type
+        self.degen_thresh = degen_thresh
+        self.degen_Kramers = degen_Kramers
+
+    def nonzero(self, E1, E2):
+        return (E1 < self.Efermi.max()) and (E2 > self.Efermi.min()) and (
+            self.omega.min() - 5 * self.eta < E1 - E2 < self.omega.max() + 5 * self.eta)
+
+    def smear(self, x):
+        if self.smr_type == "Lorentzian":
+            return lorentzian(x, self.eta)
+        return gaussian(x, self.eta)
+
+    def __call__(self, data):
+        restot = np.zeros((len(self.omega), len(self.Efermi)))
+        for ik in range(data.nk):
+            groups = data.get_bands_in_range_groups_ik(ik, -np.inf, np.inf, degen_thresh=self.degen_thresh,
+                                                       degen_Kramers=self.degen_Kramers)
+            pairs = [(m, n, Em, En) for m, Em in groups.items() for n, En in groups.items() if self.nonzero(Em, En)]
+            if not pairs:
+                continue
+            M = np.array([(p[0][1] - p[0][0]) * (p[1][1] - p[1][0]) for p in pairs], dtype=float)
+            fE = np.array([fermi_dirac(p[3], self.Efermi, self.kBT) - fermi_dirac(p[2], self.Efermi, self.kBT)
+                           for p in pairs])
+            fw = np.array([self.smear(p[2] - p[3] - self.omega) for p in pairs]).T
+            restot += fw @ (fE * M[:, None])
+        return OracleResult(self.Efermi, restot.T / (data.nk * data.system.cell_volume))
 
 
 def grid_blocks(NK=None, NKFFT=None, NKdiv=None, periodic=(True, True, True)):
@@ -1315,7 +1408,7 @@ def run_adaptive(system, calculators, NK=None, NKFFT=None, adpt_num_iter=1, adpt
 def load_golden_system(path):
     """Read a system written by tests/golden/make_golden.py (already `[R, m, n, ...]`)."""
     with np.load(path, allow_pickle=False) as f:
-        mats = {k: f[k] for k in f.files if k in ('Ham', 'AA', 'BB', 'CC', 'OO', 'SS')}
+        mats = {k: f[k] for k in f.files if k in ('Ham', 'AA', 'BB', 'CC', 'OO', 'SS', 'SA', 'SHA')}
         periodic = f['periodic'] if 'periodic' in f.files else (True, True, True)
         fito = bool(f['force_internal_terms_only']) if 'force_internal_terms_only' in f.files else False
         return OracleSystem(f['real_lattice'], f['iRvec'], f['wannier_centers_cart'], mats,

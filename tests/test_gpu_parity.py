@@ -286,6 +286,32 @@ def test_chiral_optical_conductivity(wb, results_run):
     assert rel_err(r.results["oc"].data, results_run["chiral_opt_conductivity"]) < 1e-8
 
 
+def test_chiral_jdos(wb, results_run):
+    """Joint density of states against the reference's fixture (Chiral model, Gaussian
+    smearing 0.2 eV, 27 Fermi levels x 11 frequencies; tests/test_run.py:183-202)."""
+    say("test_chiral_jdos")
+    s = gsys(wb, "chiral_left")
+    calc = wb.calculators.dynamic.JDOS(Efermi=np.linspace(-5, 8, 27), omega=np.linspace(0, 1., 11),
+                                       smr_fixed_width=0.20, smr_type="Gaussian")
+    r = run_gpu(wb, s, {"jdos": calc}, [10, 10, 4], [5, 5, 2])
+    assert r.results["jdos"].data.shape == (27, 11)
+    assert rel_err(r.results["jdos"].data, results_run["chiral_jdos"]) < 1e-8
+
+
+def test_synthetic_jdos(wb, small_synth):
+    """JDOS with Lorentzian smearing at finite temperature vs the oracle (nw = 8, k-grid 4^3)."""
+    say("test_synthetic_jdos")
+    s, o = small_synth
+    Ef = np.linspace(-4, 4, 9)
+    om = np.linspace(0.0, 3, 13)
+    kw = dict(omega=om, smr_fixed_width=0.1, smr_type="Lorentzian", kBT=0.05)
+    r = run_gpu(wb, s, {"jdos": wb.calculators.dynamic.JDOS(Efermi=Ef, **kw)}, [4, 4, 4], [2, 2, 2])
+    ro = O.run(o, {"jdos": O.JDOS(Ef, **kw)}, NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    e = rel_err(r.results["jdos"].data, ro["jdos"].data)
+    say(f"  jdos rel err {e:.2e}")
+    assert e < 1e-12
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-6)])
 def test_synthetic_optical_conductivity(wb, small_synth, precision, tol):
     """External terms (A = i D_H + Abar), Lorentzian smearing, finite temperature, vs the oracle."""

@@ -242,3 +242,25 @@ def test_chiral_optical_conductivity(results_run):
                                  smr_type="Gaussian")
     r = O.run(s, {"oc": calc}, NK=[10, 10, 4], NKFFT=[5, 5, 2])
     assert rel_err(r["oc"].data, results_run["chiral_opt_conductivity"]) < 1e-8
+
+
+def test_chiral_jdos(results_run):
+    """Joint density of states (dynamic.py:145-167) on the Chiral model, Gaussian smearing
+    0.2 eV, 27 Fermi levels x 11 frequencies (tests/test_run.py:183-202)."""
+    s = gsys("chiral_left")
+    calc = O.JDOS(np.linspace(-5, 8, 27), np.linspace(0, 1, 11), smr_fixed_width=0.2, smr_type="Gaussian")
+    r = O.run(s, {"jdos": calc}, NK=[10, 10, 4], NKFFT=[5, 5, 2])
+    assert rel_err(r["jdos"].data, results_run["chiral_jdos"]) < 1e-8
+
+
+def test_random_optical_and_spin_hall(results_run):
+    """Kubo optical conductivity (external terms) and spin Hall conductivity with the Ryoo
+    spin current on the random model, Gaussian 0.2 eV, Efermi 17 and 18 eV, omega 0..7 eV
+    (tests/test_run.py:653-667, default grid 6^3 / 3^3)."""
+    s = gsys("random")
+    kw = dict(smr_fixed_width=0.2, smr_type="Gaussian")
+    Ef, om = np.array([17.0, 18.0]), np.arange(0.0, 7.1, 1.0)
+    r = O.run(s, {"oc": O.OpticalConductivity(Ef, om, **kw), "shc": O.SHC(Ef, om, SHC_type="ryoo", **kw)},
+              NK=[6, 6, 6], NKFFT=[3, 3, 3])
+    assert rel_err(r["oc"].data, results_run["random_opt_conductivity"]) < 1e-8
+    assert rel_err(r["shc"].data, results_run["random_opt_SHCryoo"]) < 1e-8

@@ -37,7 +37,7 @@ SYMBOLS = [
     "wbmi_last_error", "wbmi_build_channels", "wbmi_r2k_plan_create", "wbmi_r2k_plan_destroy",
     "wbmi_r2k_plan_sizes", "wbmi_r2k_box", "wbmi_r2k_exec", "wbmi_r2k_klist", "wbmi_eigh",
     "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_optical_conductivity", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
-    "wbmi_static_scan_tetra", "wbmi_finalize_tetra",
+    "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos",
 ]
 
 
@@ -62,7 +62,8 @@ class ScanParams(ctypes.Structure):
 class OptcondParams(ctypes.Structure):
     _fields_ = [
         ("nEF", ctypes.c_int), ("nomega", ctypes.c_int), ("Efermi", ctypes.c_void_p), ("omega", ctypes.c_void_p),
-        ("EFmin", ctypes.c_double), ("EFmax", ctypes.c_double), ("kBT", ctypes.c_double), ("eta", ctypes.c_double),
+        ("EFmin", ctypes.c_double), ("EFmax", ctypes.c_double), ("omega_min", ctypes.c_double),
+        ("omega_max", ctypes.c_double), ("kBT", ctypes.c_double), ("eta", ctypes.c_double),
         ("smr_gaussian", ctypes.c_int), ("degen_thresh", ctypes.c_double), ("degen_Kramers", ctypes.c_int),
         ("external_terms", ctypes.c_int), ("ch", ctypes.c_int * NROLES),
     ]
@@ -97,6 +98,7 @@ _PROTOS = {
     "wbmi_static_scan_tetra": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp, _i, _d,
                                     _vp]),
     "wbmi_finalize_tetra": (_i, [_vp, _vp, _i, _i, _d, _vp]),
+    "wbmi_jdos": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(OptcondParams), _vp]),
 }
 
 
@@ -264,6 +266,11 @@ class Context:
         self.check(self._timed("optcond", (nk, nw), lib().wbmi_optical_conductivity, self.h, nk, nw, X.shape[1],
                                _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(out)),
                    "wbmi_optical_conductivity")
+
+    def jdos(self, E, params, out):
+        nk, nw = E.shape
+        self.check(self._timed("jdos", (nk, nw), lib().wbmi_jdos, self.h, nk, nw, _ptr(E), ctypes.byref(params),
+                               _ptr(out)), "wbmi_jdos")
 
     def finalize(self, hist, fder, dEF, scale, out):
         nEF, ncomp = hist.shape

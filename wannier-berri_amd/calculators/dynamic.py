@@ -1,11 +1,12 @@
 """Frequency-dependent calculators on the MI355X.
 
-OpticalConductivity keeps the constructor of
-wannierberri/calculators/dynamic.py:26-58,184-196 (Efermi, omega, kBT,
+OpticalConductivity and JDOS keep the constructors of
+wannierberri/calculators/dynamic.py:26-58,145-196 (Efermi, omega, kBT,
 smr_fixed_width, smr_type, kwargs_formula, degen_thresh, degen_Kramers) and
-its result: EnergyResult on (Efermi, omega) of complex [nEF, nomega, 3, 3]
-tensors, times factor_opt / (nk V).  The per-k pair sums and the
-(Fermi level, frequency) accumulation run in wbmi_optical_conductivity.
+their results: EnergyResult on (Efermi, omega) of complex [nEF, nomega, 3, 3]
+tensors times factor_opt / (nk V), resp. real [nEF, nomega] / (nk V).  The
+per-k pair sums and the (Fermi level, frequency) accumulation run in
+wbmi_optical_conductivity and wbmi_jdos.
 """
 import numpy as np
 
@@ -13,6 +14,27 @@ from .. import _lib
 from .. import factors
 from ..result import EnergyResult
 from .static import Calculator, _external
+
+
+def _params(calc, data_K):
+    """wbmi_optcond_params of a dynamic calculator (device copies of Efermi and omega kept alive
+    on the calculator)."""
+    torch = data_K.torch
+    dev = data_K.ctx.device
+    key = str(dev)
+    if key not in calc._dev_arrays:
+        calc._dev_arrays[key] = (torch.from_numpy(calc.Efermi).to(dev), torch.from_numpy(calc.omega).to(dev))
+    ef, om = calc._dev_arrays[key]
+    p = _lib.OptcondParams()
+    p.nEF, p.nomega = len(calc.Efermi), len(calc.omega)
+    p.Efermi, p.omega = ef.data_ptr(), om.data_ptr()
+    p.EFmin, p.EFmax = float(calc.Efermi.min()), float(calc.Efermi.max())
+    p.omega_min, p.omega_max = float(calc.omega.min()), float(calc.omega.max())
+    p.kBT, p.eta = calc.kBT, calc.smr_fixed_width
+    p.smr_gaussian = int(calc.smr_type == "Gaussian")
+    p.degen_thresh = float(calc.degen_thresh)
+    p.degen_Kramers = int(bool(calc.degen_Kramers))
+    return p
 
 
 class OpticalConductivity(Calculator):
@@ -33,6 +55,7 @@ class OpticalConductivity(Calculator):
             if k != "external_terms":
                 raise NotImplementedError(f"kwargs_formula option '{k}' is not available on the MI355X path")
         self.constant_factor = factors.factor_opt
+        self._dev_arrays = {}
 
     def needed_channels(self, system):
         need = {("Ham", 0), ("Ham", 1)}
@@ -45,16 +68,7 @@ class OpticalConductivity(Calculator):
         data_K.ensure(self.needed_channels(data_K.system))
         dev = data_K.ctx.device
         nEF, nw_ = len(self.Efermi), len(self.omega)
-        ef = torch.from_numpy(self.Efermi).to(dev)
-        om = torch.from_numpy(self.omega).to(dev)
-        p = _lib.OptcondParams()
-        p.nEF, p.nomega = nEF, nw_
-        p.Efermi, p.omega = ef.data_ptr(), om.data_ptr()
-        p.EFmin, p.EFmax = float(self.Efermi.min()), float(self.Efermi.max())
-        p.kBT, p.eta = self.kBT, self.smr_fixed_width
-        p.smr_gaussian = int(self.smr_type == "Gaussian")
-        p.degen_thresh = float(self.degen_thresh)
-        p.degen_Kramers = int(bool(self.degen_Kramers))
+        p = _params(self, data_K)
         p.external_terms = int(_external(self.kwargs_formula, data_K.system))
         for i, v in enumerate(data_K.role_table()):
             p.ch[i] = v
@@ -66,4 +80,36 @@ class OpticalConductivity(Calculator):
                             E_titles=("Efermi", "Omega"), comment=self.comment, save_mode=self.save_mode)
 
 
-__all__ = ["OpticalConductivity"]
+class JDOS(Calculator):
+    r"""Joint density of states (dynamic.py:145-167)
+
+    :math:`\rho(\omega) = \sum_{\mathbf{k}} \sum_{m,n} \delta(E_{m\mathbf{k}} - E_{n\mathbf{k}} - \omega)
+    \times \left(f(E_{n\mathbf{k}}) - f(E_{m\mathbf{k}})\right)`"""
+
+    def __init__(self, Efermi=None, omega=None, kBT=0, smr_fixed_width=0.1, smr_type="Lorentzian",
+                 kwargs_formula=None, dtype=float, **kwargs):
+        super().__init__(**kwargs)
+        if smr_type not in ("Lorentzian", "Gaussian"):
+            raise ValueError(f"Invalid smearing type {smr_type}")
+        self.Efermi = np.array(Efermi, dtype=float).reshape(-1)
+        self.omega = np.array(omega, dtype=float).reshape(-1)
+        self.kBT = float(kBT)
+        self.smr_fixed_width = float(smr_fixed_width)
+        self.smr_type = smr_type
+        self._dev_arrays = {}
+
+    def needed_channels(self, system):
+        return {("Ham", 0)}
+
+    def __call__(self, data_K):
+        torch = data_K.torch
+        data_K.ensure(self.needed_channels(data_K.system))
+        p = _params(self, data_K)
+        out = torch.zeros((len(self.Efermi), len(self.omega)), dtype=torch.float64, device=data_K.ctx.device)
+        data_K.ctx.jdos(data_K.E_dev, p, out)
+        data = out * (1.0 / (data_K.nk * data_K.cell_volume))
+        return EnergyResult([self.Efermi, self.omega], data, transformTR=1, transformInv=1,
+                            E_titles=("Efermi", "Omega"), comment=self.comment, save_mode=self.save_mode)
+
+
+__all__ = ["OpticalConductivity", "JDOS"]

@@ -1,5 +1,6 @@
-// Kubo interband optical conductivity of libwbmi (calculators/dynamic.py:26-196,
-// Formula_OptCond 170-181).
+// Frequency-dependent calculators of libwbmi (calculators/dynamic.py:26-196):
+// the Kubo interband optical conductivity (Formula_OptCond 170-181) and the
+// joint density of states (JDOS 145-167).
 //
 // Per k: the generalized Berry connection A^a = i D_H^a (+ Abar^a with external
 // terms), D_H = -V o dEinv (data_K.py:290-334); for every ordered pair (m, n)
@@ -25,6 +26,29 @@ struct OcArgs {
     wbmi_optcond_params p;
     int nw, nch;
 };
+
+// band groups of one k (grid/tetrahedron.py:132-136) into gb[0..ng], their mean energies
+// into gm; returns ng.  One thread.
+__device__ inline int band_groups(const double* E, int nw, double degen_thresh, int degen_Kramers, int* gb,
+                                  double* gm) {
+    int nb = 0;
+    gb[nb++] = 0;
+    for (int i = 1; i < nw; ++i)
+        if (E[i] - E[i - 1] > degen_thresh) gb[nb++] = i;
+    gb[nb++] = nw;
+    if (degen_Kramers) {
+        int m = 0;
+        for (int i = 0; i < nb; ++i)
+            if (gb[i] % 2 == 0) gb[m++] = gb[i];
+        nb = m;
+    }
+    for (int g = 0; g < nb - 1; ++g) {
+        double s = 0.0;
+        for (int i = gb[g]; i < gb[g + 1]; ++i) s += E[i];
+        gm[g] = s / (gb[g + 1] - gb[g]);
+    }
+    return nb - 1;
+}
 
 __device__ inline double fermi_dirac(double E, double mu, double kBT) {
     if (kBT == 0.0) return E <= mu ? 1.0 : 0.0;
@@ -60,26 +84,7 @@ __global__ void __launch_bounds__(OC_THREADS) k_optcond(int nk, OcArgs a, const 
     for (int k = blockIdx.y * OC_KCHUNK; k < k1; ++k) {
         for (int i = tid; i < nw; i += OC_THREADS) sE[i] = Eall[(int64_t)k * nw + i];
         __syncthreads();
-        if (tid == 0) {
-            // band groups (grid/tetrahedron.py:132-136) and their mean energies
-            int nb = 0;
-            sGb[nb++] = 0;
-            for (int i = 1; i < nw; ++i)
-                if (sE[i] - sE[i - 1] > p.degen_thresh) sGb[nb++] = i;
-            sGb[nb++] = nw;
-            if (p.degen_Kramers) {
-                int m = 0;
-                for (int i = 0; i < nb; ++i)
-                    if (sGb[i] % 2 == 0) sGb[m++] = sGb[i];
-                nb = m;
-            }
-            sNg = nb - 1;
-            for (int g = 0; g < nb - 1; ++g) {
-                double s = 0.0;
-                for (int i = sGb[g]; i < sGb[g + 1]; ++i) s += sE[i];
-                sGm[g] = s / (sGb[g + 1] - sGb[g]);
-            }
-        }
+        if (tid == 0) sNg = band_groups(sE, nw, p.degen_thresh, p.degen_Kramers, sGb, sGm);
         // A^a_ij = i D_H^a_ij (+ Abar^a_ij), D_H = -V dEinv (0 where |E_i - E_j| < 1e-7)
         const cplx<T>* Xk = Xall + (int64_t)k * a.nch * nw2;
         for (int t = tid; t < 3 * nw2; t += OC_THREADS) {
@@ -189,6 +194,76 @@ extern "C" int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch,
     } else {
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_optical_conductivity: bad precision");
     }
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+// Joint density of states (dynamic.py:145-167): per k, every ordered pair (m, n) of band
+// groups with E_m < max(Efermi), E_n > min(Efermi) and
+// min(omega) - 5 eta < E_m - E_n < max(omega) + 5 eta adds
+//   out[EF][w] += |m| |n| (f(E_n) - f(E_m)) smear(E_m - E_n - w).
+// Workgroup = (256 (EF, w) outputs, one per thread) x (a chunk of k-points); the pair
+// list of a k is shared, each thread accumulates its own output.
+#define JD_THREADS 256
+#define JD_KCHUNK 32
+#define JD_NWMAX 256
+
+__global__ void __launch_bounds__(JD_THREADS) k_jdos(int nk, wbmi_optcond_params p, int nw,
+                                                     const double* __restrict__ Eall, double* __restrict__ out) {
+    __shared__ double sE[JD_NWMAX], sGm[JD_NWMAX];
+    __shared__ int sGb[JD_NWMAX + 1];
+    __shared__ int sNg;
+    const int tid = threadIdx.x;
+    const int o = blockIdx.x * JD_THREADS + tid;  // output index EF * nomega + w
+    const bool has_out = o < p.nEF * p.nomega;
+    const double mu = has_out ? p.Efermi[o / p.nomega] : 0.0;
+    const double w = has_out ? p.omega[o % p.nomega] : 0.0;
+    const double wlo = p.omega_min - 5.0 * p.eta, whi = p.omega_max + 5.0 * p.eta;
+    double acc = 0.0;
+    const int k1 = min(nk, (int)(blockIdx.y + 1) * JD_KCHUNK);
+    for (int k = blockIdx.y * JD_KCHUNK; k < k1; ++k) {
+        for (int i = tid; i < nw; i += JD_THREADS) sE[i] = Eall[(int64_t)k * nw + i];
+        __syncthreads();
+        if (tid == 0) sNg = band_groups(sE, nw, p.degen_thresh, p.degen_Kramers, sGb, sGm);
+        __syncthreads();
+        const int ng = sNg;
+        if (has_out)
+            for (int gm = 0; gm < ng; ++gm) {
+                const double Em = sGm[gm];
+                if (!(Em < p.EFmax)) continue;
+                for (int gn = 0; gn < ng; ++gn) {
+                    const double En = sGm[gn], d = Em - En;
+                    if (!(En > p.EFmin) || !(d > wlo) || !(d < whi)) continue;
+                    const double fE = fermi_dirac(En, mu, p.kBT) - fermi_dirac(Em, mu, p.kBT);
+                    if (fE == 0.0) continue;
+                    const double x = d - w;
+                    double sm;
+                    if (p.smr_gaussian) {
+                        const double y = x / p.eta;
+                        sm = fabs(x) < p.eta * sqrt(200.0) ? exp(-y * y) / (sqrt(M_PI) * p.eta) : 0.0;
+                    } else {
+                        sm = 1.0 / (M_PI * p.eta) * (p.eta * p.eta) / (x * x + p.eta * p.eta);
+                    }
+                    acc += (double)((sGb[gm + 1] - sGb[gm]) * (sGb[gn + 1] - sGb[gn])) * fE * sm;
+                }
+            }
+        __syncthreads();
+    }
+    if (has_out) unsafeAtomicAdd(out + o, acc);
+}
+
+extern "C" int wbmi_jdos(wbmi_ctx* ctx, int nk, int nw, const double* E_dev, const wbmi_optcond_params* params,
+                         double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (nk <= 0 || nw <= 0 || !E_dev || !params || !out_dev || !params->Efermi || !params->omega ||
+        params->nEF <= 0 || params->nomega <= 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_jdos: bad arguments");
+    if (nw > JD_NWMAX) return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_jdos: nw <= %d", JD_NWMAX);
+    if (!(params->eta > 0) || params->kBT < 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_jdos: smearing width must be positive");
+    const int64_t nout = (int64_t)params->nEF * params->nomega;
+    dim3 g((unsigned)ceil_div(nout, JD_THREADS), (unsigned)ceil_div(nk, JD_KCHUNK));
+    hipLaunchKernelGGL(k_jdos, g, dim3(JD_THREADS), 0, ctx->stream, nk, *params, nw, E_dev, out_dev);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
 }
