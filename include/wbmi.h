@@ -95,6 +95,16 @@ typedef struct wbmi_optcond_params {
     int ch[WBMI_NROLES];  /* channel slots: WBMI_ROLE_V (and WBMI_ROLE_A)     */
 } wbmi_optcond_params;
 
+/* Parameters of the spin Hall conductivity (calculators/dynamic.py:204-241):
+ * the optical-conductivity parameters (V and A roles) plus the channel slots
+ * of Xbar('SS')[s], Xbar('SA')[a, s], Xbar('SHA')[a, s] (row-major (a, s)). */
+typedef struct wbmi_shc_params {
+    wbmi_optcond_params base;
+    int ch_S[3];
+    int ch_SA[9];
+    int ch_SHA[9];
+} wbmi_shc_params;
+
 /* ---- context ------------------------------------------------------------ */
 int wbmi_ctx_create(int device, void* stream, wbmi_ctx** out);
 int wbmi_ctx_destroy(wbmi_ctx* ctx);
@@ -200,6 +210,22 @@ int wbmi_eigh_guess(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hs
 int wbmi_jdos(wbmi_ctx* ctx, int nk, int nw, const double* E_dev, const wbmi_optcond_params* params,
               double* out_dev);
 
+/* Kubo spin Hall conductivity with the Ryoo spin current (dynamic.py:204-241,
+ * formula/covariant.py:689-756; SHC_type='ryoo', external terms):
+ *   J^{as} = (J' + J'^+)/2,  J'^{as}_mn = -i (E_n SA^{as}_mn - SHA^{as}_mn)
+ *            + sum_l SS^s_ml V^a_ln,
+ *   B^b = -i (i D_H + Abar)^b,
+ *   out[EF][w][a][b][s] += (f(E_n) - f(E_m)) sum_{i in m, j in n}
+ *                          Im(J^{as}_ij B^b_ji) / (2 (E_m - E_n - w - i eta))
+ * over band-group pairs (m, n) (Gaussian: imaginary part pi G / 2).
+ *   out_dev  : float64 [nEF, nomega, 27, 2] (complex), accumulated; the caller
+ *              scales by factor_shc / (nk V).
+ *   work_dev : wbmi_spin_hall_work_bytes(nk, nw) bytes (J per k).            */
+size_t wbmi_spin_hall_work_bytes(int nk, int nw);
+int wbmi_spin_hall(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                   int prec, const wbmi_shc_params* params, void* work_dev, size_t work_bytes,
+                   double* out_dev);
+
 int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
                               int prec, const wbmi_optcond_params* params, double* out_dev);
 
@@ -252,6 +278,17 @@ int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double*
                            const double* Ecorners_dev, const void* Xbar_dev, int prec,
                            const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
                            double weight, double* hist_dev);
+
+/* ---- per-band table (k-resolved tabulation) ------------------------------
+ * Replaces Tabulator.__call__ (calculators/tabulate.py:20-48): for every k and
+ * band, the trace of its band group (degen_thresh, degen_Kramers; inn = the
+ * group, out = the rest) divided by the group size; for WBMI_FORMULA_IDENTITY
+ * the group's mean energy (formula Eavln).  params: formula, sign,
+ * internal/external terms, degen_thresh, degen_Kramers, ch (the Fermi-level
+ * fields are not read).
+ *   out_dev : float64 [nk, nw, 3^ndim] (ndim 0 for the identity formula)     */
+int wbmi_tabulate(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                  int prec, const wbmi_scan_params* params, double* out_dev);
 
 /* out_dev [nEF, ncomp] = scale * (prefix sums of rows [0, nEF) + rows [nEF, 2 nEF))
  * of a wbmi_static_scan_tetra histogram (static.py:149-155, no fder stencil).

@@ -1265,6 +1265,24 @@ class SHC(OpticalConductivity):
         return OracleResult(self.Efermi, restot * FACTOR_SHC / (data.nk * data.system.cell_volume))
 
 
+def tabulate(data, Formula, ibands, degen_thresh=1e-4, degen_Kramers=False, constant_factor=1., **kw):
+    """Tabulator.__call__ (calculators/tabulate.py:20-48): per k and band of `ibands`, the
+    trace of the band's group (inn = the group, out = the rest) divided by the group size."""
+    formula = Formula(data, **kw)
+    NB = data.num_wann
+    out = []
+    for ik in range(data.nk):
+        groups = band_groups(data.E_K[ik], degen_thresh, degen_Kramers)
+        rows = []
+        for b in ibands:
+            a, c = next((a, c) for a, c in groups if a <= b < c)
+            inn = np.arange(a, c)
+            rest = np.concatenate((np.arange(0, a), np.arange(c, NB)))
+            rows.append(formula.trace(ik, inn, rest) / (c - a))
+        out.append(rows)
+    return np.array(out) * constant_factor
+
+
 class JDOS:
     """Joint density of states (dynamic.py:26-131 with JDOS 145-167): per k, every pair
     (m, n) of band groups with E_m < max(Efermi), E_n > min(Efermi) and

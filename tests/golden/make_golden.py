@@ -94,6 +94,11 @@ def main():
              "haldane_adpt3_tetra_dos_iter0": I + "Haldane-dos-PythTB-adpt3-tetra-sym_iter-0000.npz",
              "haldane_adpt3_tetra_dos_iter1": I + "Haldane-dos-PythTB-adpt3-tetra-sym_iter-0001.npz",
              }, "results_run.npz")
+    # k-resolved tabulation, Fe (Wannier90), grid 4^3 / 2^3, bands 5 and 6
+    # (tests/test_run.py:271-288,313-345)
+    with np.load(os.path.join(REF, I + "Fe_W90-tabulate.npz"), allow_pickle=False) as f:
+        np.savez_compressed(os.path.join(HERE, "results_tabulate.npz"),
+                            **{k: f[k] for k in ["Energy", "berry", "Der_berry", "morb", "Der_morb"]})
     # per-formula gauge-invariant traces (tests/reference/formula/*.npz)
     forms = {}
     for name in ["Dcov", "DerDcov", "DEinv_ln", "InvMass", "Omega", "DerOmega", "Morb_H", "morb",

@@ -298,6 +298,28 @@ def test_chiral_jdos(wb, results_run):
     assert rel_err(r.results["jdos"].data, results_run["chiral_jdos"]) < 1e-8
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_random_optical_and_spin_hall(wb, results_run, precision, tol):
+    """Kubo optical conductivity (external terms) and spin Hall conductivity with the Ryoo
+    spin current against the reference's fixtures on the random model (Gaussian 0.2 eV,
+    Efermi 17 and 18 eV, omega 0..7 eV, grid 6^3 / 3^3; tests/test_run.py:653-667)."""
+    say(f"test_random_optical_and_spin_hall {precision}")
+    dyn = wb.calculators.dynamic
+    s = gsys(wb, "random")
+    kw = dict(Efermi=np.array([17.0, 18.0]), omega=np.arange(0.0, 7.1, 1.0), smr_fixed_width=0.2,
+              smr_type="Gaussian")
+    r = run_gpu(wb, s, {"oc": dyn.OpticalConductivity(**kw), "shc": dyn.SHC(SHC_type="ryoo", **kw)},
+                [6, 6, 6], [3, 3, 3], precision=precision)
+    e_oc = rel_err(r.results["oc"].data, results_run["random_opt_conductivity"])
+    e_shc = rel_err(r.results["shc"].data, results_run["random_opt_SHCryoo"])
+    say(f"  opt_conductivity {e_oc:.2e}, SHC ryoo {e_shc:.2e}")
+    assert e_oc < tol and e_shc < tol
+    one = run_gpu(wb, s, {"shc": dyn.SHC(SHC_type="ryoo", shc_abc=(1, 2, 3), **kw)}, [6, 6, 6], [3, 3, 3],
+                  precision=precision)
+    assert np.allclose(one.results["shc"].data, r.results["shc"].data[:, :, 0, 1, 2], rtol=0, atol=1e-12 *
+                       np.abs(r.results["shc"].data).max())
+
+
 def test_synthetic_jdos(wb, small_synth):
     """JDOS with Lorentzian smearing at finite temperature vs the oracle (nw = 8, k-grid 4^3)."""
     say("test_synthetic_jdos")
@@ -600,3 +622,30 @@ def test_finalize_4001_levels(wb, small_synth):
         e = rel_err(r.results[k].data, ro[k].data)
         say(f"  {k}: rel err {e:.2e}")
         assert e < 1e-8, k
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1.0), ("fp32", 200.0)])
+def test_fe_tabulate(wb, precision, tol):
+    """k-resolved tabulation (TabulatorAll: Energy, Berry curvature and its derivative,
+    orbital moment and its derivative) on Fe (Wannier90), grid 4^3 / 2^3, bands 5 and 6,
+    against the reference's fixture at its tolerances (2e-8, Der_berry 5e-8, relative to
+    the largest value; tests/test_run.py:271-345); the mixed form at 200x those."""
+    say(f"test_fe_tabulate {precision}")
+    tab = wb.calculators.tabulate
+    s = gsys(wb, "fe_w90")
+    pt = {"degen_thresh": 5e-2}
+    calc = tab.TabulatorAll({"Energy": tab.Energy(), "berry": tab.BerryCurvature(**pt),
+                             "Der_berry": tab.DerBerryCurvature(**pt), "morb": tab.OrbitalMoment(**pt),
+                             "Der_morb": tab.DerOrbitalMoment(**pt)}, ibands=[5, 6])
+    grid = wb.Grid(system=s, NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    r = wb.run(s, grid, {"tabulate": calc}, use_irred_kpt=False, symmetrize=False,
+               parameters_K={"precision": precision}, save_files=False, verbose=False)
+    t = r.results["tabulate"]
+    with np.load(os.path.join(GOLDEN, "results_tabulate.npz"), allow_pickle=False) as f:
+        ref = {k: f[k] for k in f.files}
+    for k in ref:
+        d = t.get_data(k)
+        e = rel_err(d, ref[k])
+        say(f"  {k}: {e:.2e}")
+        assert d.shape == ref[k].shape
+        assert e < tol * (5e-8 if k == "Der_berry" else 2e-8), k

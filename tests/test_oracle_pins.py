@@ -264,3 +264,39 @@ def test_random_optical_and_spin_hall(results_run):
               NK=[6, 6, 6], NKFFT=[3, 3, 3])
     assert rel_err(r["oc"].data, results_run["random_opt_conductivity"]) < 1e-8
     assert rel_err(r["shc"].data, results_run["random_opt_SHCryoo"]) < 1e-8
+
+
+def _tabulate_grid(s, quantities, ibands, NK, NKFFT):
+    """Oracle tabulation over all K-blocks, assembled on the full grid (k index
+    (i0*N1 + i1)*N2 + i2, tabresult.py:134-170)."""
+    NKdiv, NKFFT, blocks = O.grid_blocks(NK, NKFFT)
+    NK = NKdiv * NKFFT
+    out = {}
+    for dK, _ in blocks:
+        d = O.OracleDataK(s, dK=dK, NKFFT=NKFFT)
+        ki = np.rint(d.kpoints_all * NK[None, :]).astype(int) % NK[None, :]
+        idx = (ki[:, 0] * NK[1] + ki[:, 1]) * NK[2] + ki[:, 2]
+        for key, (F, kw) in quantities.items():
+            v = O.tabulate(d, F, ibands, **kw)
+            if key not in out:
+                out[key] = np.zeros((int(np.prod(NK)),) + v.shape[1:])
+            out[key][idx] = v
+    return {k: v.reshape(tuple(NK) + v.shape[1:]) for k, v in out.items()}
+
+
+TAB_QUANTITIES = {"Energy": (O.Eavln, {}),
+                  "berry": (O.Omega, {"degen_thresh": 5e-2}),
+                  "Der_berry": (O.DerOmega, {"degen_thresh": 5e-2}),
+                  "morb": (O.morb, {"degen_thresh": 5e-2, "constant_factor": O.FACTOR_MORB_EVA2_TO_MUB}),
+                  "Der_morb": (O.Dermorb, {"degen_thresh": 5e-2, "constant_factor": O.FACTOR_MORB_EVA2_TO_MUB})}
+
+
+def test_fe_tabulate():
+    """k-resolved tabulation (tabulate.py:20-48, tabresult.py:134-170) on Fe (Wannier90),
+    grid 4^3 / 2^3, bands 5 and 6 (tests/test_run.py:271-288); tolerances of the reference
+    test (2e-8, Der_berry 5e-8, relative to the largest value)."""
+    with np.load(os.path.join(GOLDEN, "results_tabulate.npz"), allow_pickle=False) as f:
+        ref = {k: f[k] for k in f.files}
+    r = _tabulate_grid(gsys("fe_w90"), TAB_QUANTITIES, [5, 6], [4, 4, 4], [2, 2, 2])
+    for k in TAB_QUANTITIES:
+        assert rel_err(r[k], ref[k]) < (5e-8 if k == "Der_berry" else 2e-8), k

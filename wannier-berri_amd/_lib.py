@@ -37,7 +37,8 @@ SYMBOLS = [
     "wbmi_last_error", "wbmi_build_channels", "wbmi_r2k_plan_create", "wbmi_r2k_plan_destroy",
     "wbmi_r2k_plan_sizes", "wbmi_r2k_box", "wbmi_r2k_exec", "wbmi_r2k_klist", "wbmi_eigh",
     "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_optical_conductivity", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
-    "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos",
+    "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos", "wbmi_spin_hall", "wbmi_spin_hall_work_bytes",
+    "wbmi_tabulate",
 ]
 
 
@@ -67,6 +68,11 @@ class OptcondParams(ctypes.Structure):
         ("smr_gaussian", ctypes.c_int), ("degen_thresh", ctypes.c_double), ("degen_Kramers", ctypes.c_int),
         ("external_terms", ctypes.c_int), ("ch", ctypes.c_int * NROLES),
     ]
+
+
+class ShcParams(ctypes.Structure):
+    _fields_ = [("base", OptcondParams), ("ch_S", ctypes.c_int * 3), ("ch_SA", ctypes.c_int * 9),
+                ("ch_SHA", ctypes.c_int * 9)]
 
 
 _lib = None
@@ -99,6 +105,9 @@ _PROTOS = {
                                     _vp]),
     "wbmi_finalize_tetra": (_i, [_vp, _vp, _i, _i, _d, _vp]),
     "wbmi_jdos": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(OptcondParams), _vp]),
+    "wbmi_spin_hall_work_bytes": (ctypes.c_size_t, [_i, _i]),
+    "wbmi_tabulate": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp]),
+    "wbmi_spin_hall": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t, _vp]),
 }
 
 
@@ -271,6 +280,19 @@ class Context:
         nk, nw = E.shape
         self.check(self._timed("jdos", (nk, nw), lib().wbmi_jdos, self.h, nk, nw, _ptr(E), ctypes.byref(params),
                                _ptr(out)), "wbmi_jdos")
+
+    def spin_hall(self, E, X, prec, params, out):
+        nk, nw = E.shape
+        wbytes = int(lib().wbmi_spin_hall_work_bytes(nk, nw))
+        work = self._workspace(wbytes)
+        self.check(self._timed("shc", (nk, nw), lib().wbmi_spin_hall, self.h, nk, nw, X.shape[1], _ptr(E), _ptr(X),
+                               prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)), "wbmi_spin_hall")
+
+    def tabulate(self, E, X, prec, params, out):
+        nk, nw = E.shape
+        nch = X.shape[1] if X is not None else 1
+        self.check(self._timed(f"tab_f{params.formula}", (nk, nw), lib().wbmi_tabulate, self.h, nk, nw, nch, _ptr(E),
+                               _ptr(X), prec, ctypes.byref(params), _ptr(out)), "wbmi_tabulate")
 
     def finalize(self, hist, fder, dEF, scale, out):
         nEF, ncomp = hist.shape

@@ -1,3 +1,4 @@
-"""Calculators of the MI355X path (static Fermi-level scans, optical conductivity)."""
-from . import dynamic, static
+"""Calculators of the MI355X path (static Fermi-level scans, dynamic calculators, tabulation)."""
+from . import dynamic, static, tabulate
+from .tabulate import TabulatorAll
 from .static import Calculator, StaticCalculator

@@ -1,12 +1,13 @@
 """Frequency-dependent calculators on the MI355X.
 
-OpticalConductivity and JDOS keep the constructors of
-wannierberri/calculators/dynamic.py:26-58,145-196 (Efermi, omega, kBT,
-smr_fixed_width, smr_type, kwargs_formula, degen_thresh, degen_Kramers) and
-their results: EnergyResult on (Efermi, omega) of complex [nEF, nomega, 3, 3]
-tensors times factor_opt / (nk V), resp. real [nEF, nomega] / (nk V).  The
-per-k pair sums and the (Fermi level, frequency) accumulation run in
-wbmi_optical_conductivity and wbmi_jdos.
+OpticalConductivity, JDOS and SHC keep the constructors of
+wannierberri/calculators/dynamic.py:26-58,145-241 (Efermi, omega, kBT,
+smr_fixed_width, smr_type, kwargs_formula, degen_thresh, degen_Kramers;
+SHC_type, shc_abc) and their results: EnergyResult on (Efermi, omega) of
+complex [nEF, nomega, 3, 3] tensors times factor_opt / (nk V), real
+[nEF, nomega] / (nk V), complex [nEF, nomega, 3, 3, 3] times
+factor_shc / (nk V).  The per-k pair sums and the (Fermi level, frequency)
+accumulation run in wbmi_optical_conductivity, wbmi_jdos and wbmi_spin_hall.
 """
 import numpy as np
 
@@ -112,4 +113,61 @@ class JDOS(Calculator):
                             E_titles=("Efermi", "Omega"), comment=self.comment, save_mode=self.save_mode)
 
 
-__all__ = ["OpticalConductivity", "JDOS"]
+class SHC(Calculator):
+    r"""Kubo spin Hall conductivity (dynamic.py:224-241) with the Ryoo spin current
+    (formula/covariant.py:746-756); SHC_type 'qiao' and 'simple' are not available."""
+
+    def __init__(self, SHC_type="ryoo", shc_abc=None, Efermi=None, omega=None, kBT=0, smr_fixed_width=0.1,
+                 smr_type="Lorentzian", kwargs_formula=None, dtype=complex, **kwargs):
+        super().__init__(**kwargs)
+        if SHC_type != "ryoo":
+            raise NotImplementedError(f"SHC_type='{SHC_type}' is not available on the MI355X path (only 'ryoo')")
+        if smr_type not in ("Lorentzian", "Gaussian"):
+            raise ValueError(f"Invalid smearing type {smr_type}")
+        self.kwargs_formula = dict(kwargs_formula or {})
+        for k in self.kwargs_formula:
+            if k != "external_terms":
+                raise NotImplementedError(f"kwargs_formula option '{k}' is not available on the MI355X path")
+        if shc_abc is not None:
+            assert len(shc_abc) == 3
+        self.shc_abc = None if shc_abc is None else tuple(int(x) - 1 for x in shc_abc)
+        self.Efermi = np.array(Efermi, dtype=float).reshape(-1)
+        self.omega = np.array(omega, dtype=float).reshape(-1)
+        self.kBT = float(kBT)
+        self.smr_fixed_width = float(smr_fixed_width)
+        self.smr_type = smr_type
+        self.constant_factor = factors.factor_shc
+        self._dev_arrays = {}
+
+    def needed_channels(self, system):
+        return {("Ham", 0), ("Ham", 1), ("AA", 0), ("SS", 0), ("SA", 0), ("SHA", 0)}
+
+    def __call__(self, data_K):
+        torch = data_K.torch
+        if not _external(self.kwargs_formula, data_K.system):
+            raise NotImplementedError("spin Hall ryoo without external terms is not implemented yet. "
+                                      "Use `SHC_type='simple'`")
+        data_K.ensure(self.needed_channels(data_K.system))
+        p = _lib.ShcParams()
+        p.base = _params(self, data_K)
+        p.base.external_terms = 1
+        for i, v in enumerate(data_K.role_table()):
+            p.base.ch[i] = v
+        sl = data_K._slots
+        for c in range(3):
+            p.ch_S[c] = sl[("SS", 0)] + c
+        for c in range(9):
+            p.ch_SA[c] = sl[("SA", 0)] + c
+            p.ch_SHA[c] = sl[("SHA", 0)] + c
+        nEF, nom = len(self.Efermi), len(self.omega)
+        out = torch.zeros((nEF, nom, 3, 3, 3, 2), dtype=torch.float64, device=data_K.ctx.device)
+        data_K.ctx.spin_hall(data_K.E_dev, data_K.Xk, data_K.prec, p, out)
+        data = torch.view_as_complex(out) * (self.constant_factor / (data_K.nk * data_K.cell_volume))
+        if self.shc_abc is not None:
+            a, b, c = self.shc_abc
+            data = data[:, :, a, b, c].contiguous()
+        return EnergyResult([self.Efermi, self.omega], data, transformTR=1, transformInv=1,
+                            E_titles=("Efermi", "Omega"), comment=self.comment, save_mode=self.save_mode)
+
+
+__all__ = ["OpticalConductivity", "JDOS", "SHC"]

@@ -1,0 +1,141 @@
+"""k-resolved tabulation on the MI355X (wannierberri/calculators/tabulate.py:12-221).
+
+Each tabulator returns, per k-point of the K-block and per band of `ibands`,
+the trace of the band's group (degen_thresh) divided by the group size, from
+one wbmi_tabulate launch per K-block; TabulatorAll packs them into a
+TABresult, which run() puts on the full grid at the end.
+"""
+import numpy as np
+
+from .. import _lib
+from .. import factors
+from ..result import KBandResult, TABresult
+from .static import Calculator, StaticCalculator, _PARITY, _external
+
+
+class Tabulator(Calculator):
+    """Tabulator (tabulate.py:12-48): formula id, rank and constant factor per subclass."""
+    formula = None
+    ndim = 0
+    sign = 0
+    factor = 1.
+
+    def __init__(self, ibands=None, kwargs_formula=None, constant_factor=None, **kwargs):
+        super().__init__(**kwargs)
+        self.ibands = np.array(ibands) if ibands is not None else None
+        self.kwargs_formula = dict(kwargs_formula or {})
+        for k in self.kwargs_formula:
+            if k not in ("internal_terms", "external_terms"):
+                raise NotImplementedError(f"kwargs_formula option '{k}' is not available on the MI355X path")
+        self.constant_factor = self.factor if constant_factor is None else constant_factor
+
+    def needed_channels(self, system):
+        return StaticCalculator.needed_channels(self, system)
+
+    def __call__(self, data_K):
+        torch = data_K.torch
+        data_K.ensure(self.needed_channels(data_K.system))
+        p = _lib.ScanParams()
+        p.formula = self.formula
+        p.sign = self.sign
+        p.internal_terms = int(bool(self.kwargs_formula.get("internal_terms", True)))
+        p.external_terms = int(_external(self.kwargs_formula, data_K.system))
+        p.fder, p.nEF_extra, p.EFmin, p.EFmax, p.dEF = 1, 1, 0.0, 0.0, 1.0
+        p.degen_thresh = float(self.degen_thresh)
+        p.degen_Kramers = int(bool(self.degen_Kramers))
+        ncomp = 3 ** self.ndim
+        out = torch.empty((data_K.nk, data_K.num_wann, ncomp), dtype=torch.float64, device=data_K.ctx.device)
+        data_K.tabulate(p, out)
+        ib = np.arange(data_K.num_wann) if self.ibands is None else self.ibands
+        data = out.cpu().numpy()[:, ib].reshape((data_K.nk, len(ib)) + (3,) * self.ndim) * self.constant_factor
+        tr, inv = _PARITY[self.formula]
+        return KBandResult(data, transformTR=tr, transformInv=inv)
+
+
+class Energy(Tabulator):
+    """Band energies, the mean of each degenerate group (formula Eavln)"""
+    formula = _lib.FORMULA_IDENTITY
+
+
+class BerryCurvature(Tabulator):
+    r""" Berry curvature :math:`\Omega_a` in units of angstrom^2"""
+    formula = _lib.FORMULA_OMEGA
+    ndim = 1
+
+
+class DerBerryCurvature(Tabulator):
+    r"Derivative of Berry curvature :math:`X_{ab}\partial_b\Omega_a` in units of angstrom^3"
+    formula = _lib.FORMULA_DEROMEGA
+    ndim = 2
+
+
+class OrbitalMoment(Tabulator):
+    r"Orbital moment :math:`m_{orb}` in units of Bohr magneton"
+    formula = _lib.FORMULA_MORB_HPM
+    ndim = 1
+    sign = -1
+    factor = factors.factor_morb_evA2_to_muB
+
+
+class DerOrbitalMoment(Tabulator):
+    r"Derivative of orbital moment :math:`\partial_a m_{orb}` in units of Bohr magneton*angstrom"
+    formula = _lib.FORMULA_DERMORB
+    ndim = 2
+    sign = -1
+    factor = factors.factor_morb_evA2_to_muB
+
+
+def _not_available(name):
+    class _NA(Calculator):
+        def __init__(self, *args, **kwargs):
+            raise NotImplementedError(f"the tabulator {name} is not available on the MI355X path")
+    _NA.__name__ = name
+    return _NA
+
+
+Velocity = _not_available("Velocity")
+InvMass = _not_available("InvMass")
+Spin = _not_available("Spin")
+SpinBerry = _not_available("SpinBerry")
+
+
+class TabulatorAll(Calculator):
+    """A pack of tabulators (tabulate.py:55-105); "Energy" is always included."""
+
+    def __init__(self, tabulators, ibands=None, mode="grid", save_mode="bin", print_comment=False):
+        super().__init__(save_mode=save_mode, print_comment=print_comment)
+        mode = mode.lower()
+        if mode != "grid":
+            raise NotImplementedError("tabulation along a path is not available on the MI355X path")
+        self.mode = mode
+        self.tabulators = dict(tabulators)
+        if "Energy" not in self.tabulators:
+            self.tabulators["Energy"] = Energy()
+        ibands = None if ibands is None else np.array(ibands)
+        for k, v in self.tabulators.items():
+            if v.ibands is not None:
+                if ibands is None or len(v.ibands) != len(ibands) or np.any(v.ibands != ibands):
+                    raise ValueError(f"tabulator {k} has ibands={v.ibands} not equal to ibands={ibands} "
+                                     "required in TabulatorAll")
+            else:
+                v.ibands = ibands
+
+    def needed_channels(self, system):
+        need = set()
+        for v in self.tabulators.values():
+            need |= set(v.needed_channels(system))
+        return need
+
+    @property
+    def allow_grid(self):
+        return True
+
+    def __call__(self, data_K):
+        data_K.ensure(self.needed_channels(data_K.system))
+        return TABresult(kpoints=np.array(data_K.kpoints_all, dtype=float), recip_lattice=data_K.system.recip_lattice,
+                         results={k: v(data_K) for k, v in self.tabulators.items()}, mode=self.mode,
+                         save_mode=self.save_mode)
+
+
+__all__ = ["Tabulator", "TabulatorAll", "Energy", "BerryCurvature", "DerBerryCurvature", "OrbitalMoment",
+           "DerOrbitalMoment"]

@@ -1,6 +1,6 @@
-// Frequency-dependent calculators of libwbmi (calculators/dynamic.py:26-196):
-// the Kubo interband optical conductivity (Formula_OptCond 170-181) and the
-// joint density of states (JDOS 145-167).
+// Frequency-dependent calculators of libwbmi (calculators/dynamic.py:26-241):
+// the Kubo interband optical conductivity (Formula_OptCond 170-181), the joint
+// density of states (JDOS 145-167) and the spin Hall conductivity (SHC 204-241).
 //
 // Per k: the generalized Berry connection A^a = i D_H^a (+ Abar^a with external
 // terms), D_H = -V o dEinv (data_K.py:290-334); for every ordered pair (m, n)
@@ -266,4 +266,204 @@ extern "C" int wbmi_jdos(wbmi_ctx* ctx, int nk, int nw, const double* E_dev, con
     hipLaunchKernelGGL(k_jdos, g, dim3(JD_THREADS), 0, ctx->stream, nk, *params, nw, E_dev, out_dev);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Kubo spin Hall conductivity (dynamic.py:204-241) with the Ryoo spin current
+// (formula/covariant.py:689-756).
+// ---------------------------------------------------------------------------
+
+// J^{as} = (J' + J'^+) / 2,  J'^{as}_ij = -i (E_j SA^{as}_ij - SHA^{as}_ij) + sum_l SS^s_il V^a_lj,
+// one workgroup per (k, component a*3+s), one thread per (i, j); both J'_ij and J'_ji are
+// formed by the same thread, so no barrier is needed.  J: complex128 [nk, 9, nw, nw].
+template <typename T>
+__global__ void __launch_bounds__(256) k_spin_current(int nk, int nw, int nch, wbmi_shc_params p,
+                                                      const double* __restrict__ Eall,
+                                                      const cplx<T>* __restrict__ Xall, cplx<double>* __restrict__ J) {
+    const int k = blockIdx.x, comp = blockIdx.y, a = comp / 3, sp = comp % 3;
+    const int64_t nw2 = (int64_t)nw * nw;
+    const cplx<T>* Xk = Xall + (int64_t)k * nch * nw2;
+    const cplx<T>* SA = Xk + p.ch_SA[comp] * nw2;
+    const cplx<T>* SHA = Xk + p.ch_SHA[comp] * nw2;
+    const cplx<T>* S = Xk + p.ch_S[sp] * nw2;
+    const cplx<T>* V = Xk + p.base.ch[WBMI_ROLE_V + a] * nw2;
+    const double* E = Eall + (int64_t)k * nw;
+    auto ld = [](const cplx<T>* X, int64_t t) { return mk((double)X[t].re, (double)X[t].im); };
+    auto jprime = [&](int i, int j) {
+        const cplx<double> sa = ld(SA, i * nw + j), sha = ld(SHA, i * nw + j);
+        // -i (E_j SA - SHA)
+        cplx<double> v = mk(E[j] * sa.im - sha.im, -(E[j] * sa.re - sha.re));
+        for (int l = 0; l < nw; ++l) v += ld(S, i * nw + l) * ld(V, l * nw + j);
+        return v;
+    };
+    for (int t = threadIdx.x; t < nw2; t += blockDim.x) {
+        const int i = t / nw, j = t % nw;
+        const cplx<double> x = jprime(i, j), y = jprime(j, i);
+        J[((int64_t)k * 9 + comp) * nw2 + t] = mk(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
+    }
+}
+
+// Per k: B^b = -i A^b = -m V^b - i Abar^b (A = i D_H + Abar, m = 1/(E_i - E_j), 0 below
+// 1e-7) in LDS; for every pair (m, n) of band groups that can change occupation,
+//   M^{abs} = sum_{i in m, j in n} Im(J^{as}_ij B^b_ji)
+// into an LDS slot (256 pairs at a time), then every thread adds all slots into its own
+// (EF, w) output:  out[EF][w][a][b][s] += (f(E_n) - f(E_m)) M^{abs} / (2 (E_m - E_n - w - i eta))
+// (Gaussian smearing: imaginary part pi G / 2).  One FP64 atomic add per output per workgroup.
+template <typename T>
+__global__ void __launch_bounds__(OC_THREADS) k_shc(int nk, wbmi_shc_params sp, int nw, int nch,
+                                                    const double* __restrict__ Eall, const cplx<T>* __restrict__ Xall,
+                                                    const cplx<double>* __restrict__ J, double* __restrict__ out) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const wbmi_optcond_params& p = sp.base;
+    const int nw2 = nw * nw, tid = threadIdx.x;
+    cplx<double>* sB = (cplx<double>*)smem;  // [3][nw][nw]
+    double* sM = (double*)(sB + 3 * nw2);    // [OC_THREADS][27]
+    double* sEm = sM + OC_THREADS * 27;
+    double* sEn = sEm + OC_THREADS;
+    double* sE = sEn + OC_THREADS;
+    double* sGm = sE + OC_NWMAX;
+    int* sOk = (int*)(sGm + OC_NWMAX);
+    int* sGb = sOk + OC_THREADS;
+    int& sNg = sGb[OC_NWMAX + 1];
+    const int o = blockIdx.x * OC_THREADS + tid;
+    const bool has_out = o < p.nEF * p.nomega;
+    const double mu = has_out ? p.Efermi[o / p.nomega] : 0.0;
+    const double w = has_out ? p.omega[o % p.nomega] : 0.0;
+    const double occ_lo = p.EFmin - 30.0 * p.kBT, occ_hi = p.EFmax + 30.0 * p.kBT;
+    cplx<double> acc[27];
+#pragma unroll
+    for (int c = 0; c < 27; ++c) acc[c] = mk(0.0, 0.0);
+    const int k1 = min(nk, (int)(blockIdx.y + 1) * OC_KCHUNK);
+    for (int k = blockIdx.y * OC_KCHUNK; k < k1; ++k) {
+        for (int i = tid; i < nw; i += OC_THREADS) sE[i] = Eall[(int64_t)k * nw + i];
+        __syncthreads();
+        if (tid == 0) sNg = band_groups(sE, nw, p.degen_thresh, p.degen_Kramers, sGb, sGm);
+        const cplx<T>* Xk = Xall + (int64_t)k * nch * nw2;
+        for (int t = tid; t < 3 * nw2; t += OC_THREADS) {
+            const int c = t / nw2, i = (t % nw2) / nw, j = t % nw;
+            const double de = sE[i] - sE[j];
+            const double m = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+            const cplx<T> v = Xk[(int64_t)p.ch[WBMI_ROLE_V + c] * nw2 + i * nw + j];
+            const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + i * nw + j];
+            // -m V - i Abar
+            sB[t] = mk(-m * (double)v.re + (double)x.im, -m * (double)v.im - (double)x.re);
+        }
+        __syncthreads();
+        const cplx<double>* Jk = J + (int64_t)k * 9 * nw2;
+        const int ng = sNg, ncand = ng * ng;
+        for (int c0 = 0; c0 < ncand; c0 += OC_THREADS) {
+            const int cnd = c0 + tid;
+            int ok = 0;
+            if (cnd < ncand) {
+                const int gm = cnd / ng, gn = cnd % ng;
+                const double Em = sGm[gm], En = sGm[gn];
+                ok = !((Em < occ_lo && En < occ_lo) || (Em > occ_hi && En > occ_hi));
+                if (ok) {
+                    sEm[tid] = Em;
+                    sEn[tid] = En;
+                    for (int a = 0; a < 3; ++a)
+                        for (int s = 0; s < 3; ++s) {
+                            const cplx<double>* Ja = Jk + (a * 3 + s) * nw2;
+                            double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+                            for (int i = sGb[gm]; i < sGb[gm + 1]; ++i)
+                                for (int j = sGb[gn]; j < sGb[gn + 1]; ++j) {
+                                    const cplx<double> jv = Ja[i * nw + j];
+                                    const cplx<double> b0 = sB[j * nw + i], b1 = sB[nw2 + j * nw + i],
+                                                       b2 = sB[2 * nw2 + j * nw + i];
+                                    m0 += jv.re * b0.im + jv.im * b0.re;
+                                    m1 += jv.re * b1.im + jv.im * b1.re;
+                                    m2 += jv.re * b2.im + jv.im * b2.re;
+                                }
+                            sM[tid * 27 + (a * 3 + 0) * 3 + s] = m0;
+                            sM[tid * 27 + (a * 3 + 1) * 3 + s] = m1;
+                            sM[tid * 27 + (a * 3 + 2) * 3 + s] = m2;
+                        }
+                }
+            }
+            sOk[tid] = ok;
+            __syncthreads();
+            const int ns = ncand - c0 < OC_THREADS ? ncand - c0 : OC_THREADS;
+            if (has_out) {
+                for (int s = 0; s < ns; ++s) {
+                    if (!sOk[s]) continue;
+                    const double Em = sEm[s], En = sEn[s];
+                    const double fE = fermi_dirac(En, mu, p.kBT) - fermi_dirac(Em, mu, p.kBT);
+                    if (fE == 0.0) continue;
+                    // 1 / (2 (x - i eta)), x = E_m - E_n - w
+                    const double x = Em - En - w;
+                    const double den = 1.0 / (x * x + p.eta * p.eta);
+                    double fr = x * den, fi = p.eta * den;
+                    if (p.smr_gaussian) {
+                        const double y = x / p.eta;
+                        fi = fabs(x) < p.eta * sqrt(200.0) ? M_PI / (sqrt(M_PI) * p.eta) * exp(-y * y) : 0.0;
+                    }
+                    const cplx<double> f = mk(0.5 * fE * fr, 0.5 * fE * fi);
+#pragma unroll
+                    for (int c = 0; c < 27; ++c) {
+                        const double mv = sM[s * 27 + c];
+                        acc[c].re += f.re * mv;
+                        acc[c].im += f.im * mv;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (has_out) {
+#pragma unroll
+        for (int c = 0; c < 27; ++c) {
+            unsafeAtomicAdd(out + ((int64_t)o * 27 + c) * 2, acc[c].re);
+            unsafeAtomicAdd(out + ((int64_t)o * 27 + c) * 2 + 1, acc[c].im);
+        }
+    }
+}
+
+extern "C" size_t wbmi_spin_hall_work_bytes(int nk, int nw) {
+    return sizeof(cplx<double>) * 9 * (size_t)nk * nw * nw;
+}
+
+template <typename T>
+static int launch_shc(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E, const void* X,
+                      const wbmi_shc_params* params, cplx<double>* J, double* out) {
+    hipLaunchKernelGGL(k_spin_current<T>, dim3(nk, 9), dim3(256), 0, ctx->stream, nk, nw, nch, *params, E,
+                       (const cplx<T>*)X, J);
+    WBMI_LAUNCH_CHECK(ctx);
+    const int64_t nout = (int64_t)params->base.nEF * params->base.nomega;
+    dim3 g((unsigned)ceil_div(nout, OC_THREADS), (unsigned)ceil_div(nk, OC_KCHUNK));
+    const size_t lds = sizeof(cplx<double>) * 3 * (size_t)nw * nw + sizeof(double) * OC_THREADS * 27 +
+                       sizeof(double) * (2 * OC_THREADS + 2 * OC_NWMAX) + sizeof(int) * (OC_THREADS + OC_NWMAX + 2);
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_shc<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                            (int)lds));
+    hipLaunchKernelGGL(k_shc<T>, g, dim3(OC_THREADS), lds, ctx->stream, nk, *params, nw, nch, E, (const cplx<T>*)X,
+                       (const cplx<double>*)J, out);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+extern "C" int wbmi_spin_hall(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                              int prec, const wbmi_shc_params* params, void* work_dev, size_t work_bytes,
+                              double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (nk <= 0 || nw <= 0 || nch <= 0 || !E_dev || !Xbar_dev || !params || !out_dev || !params->base.Efermi ||
+        !params->base.omega || params->base.nEF <= 0 || params->base.nomega <= 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: bad arguments");
+    if (nw > OC_NWMAX) return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_spin_hall: nw <= %d", OC_NWMAX);
+    if (!(params->base.eta > 0) || params->base.kBT < 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: smearing width must be positive");
+    if (!work_dev || work_bytes < wbmi_spin_hall_work_bytes(nk, nw))
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: workspace of %zu bytes needed",
+                         wbmi_spin_hall_work_bytes(nk, nw));
+    for (int c = 0; c < 3; ++c) {
+        const int v = params->base.ch[WBMI_ROLE_V + c], x = params->base.ch[WBMI_ROLE_A + c], s = params->ch_S[c];
+        if (v < 0 || v >= nch || x < 0 || x >= nch || s < 0 || s >= nch)
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: missing V, A or SS channel");
+    }
+    for (int c = 0; c < 9; ++c)
+        if (params->ch_SA[c] < 0 || params->ch_SA[c] >= nch || params->ch_SHA[c] < 0 || params->ch_SHA[c] >= nch)
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: missing SA or SHA channel");
+    if (prec == WBMI_F64)
+        return launch_shc<double>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev);
+    if (prec == WBMI_F32)
+        return launch_shc<float>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev);
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: bad precision");
 }

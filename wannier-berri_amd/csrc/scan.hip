@@ -1,7 +1,7 @@
 // Entry points of the covariant formula traces + Fermi-level scan of libwbmi
 // (calculators/static.py:60-169) and the histogram finalisation.  The kernels
-// are in scan_kernels.h; scan_f64.hip, scan_f32.hip, scan_tetra_f64.hip and
-// scan_tetra_f32.hip instantiate them.
+// are in scan_kernels.h; scan_{f64,f32}.hip, scan_tetra_{f64,f32}.hip and
+// scan_tab_{f64,f32}.hip instantiate them.
 #include "scan_kernels.h"
 
 static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
@@ -99,6 +99,31 @@ extern "C" int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, co
     if (prec == WBMI_F64) return scan_dispatch_tetra_f64(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
     if (prec == WBMI_F32) return scan_dispatch_tetra_f32(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad precision");
+}
+
+extern "C" int wbmi_tabulate(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                             int prec, const wbmi_scan_params* params, double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (nk <= 0 || nw <= 0 || nw > SC_NWMAX || nch <= 0 || !E_dev || !params || !out_dev ||
+        (params->formula != WBMI_FORMULA_IDENTITY && !Xbar_dev))
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate: bad arguments (nw <= %d)", SC_NWMAX);
+    int rc = check_roles(ctx, params, nch);
+    if (rc) return rc;
+    ScanArgs a;
+    a.p = *params;
+    a.p.fder = 1;  // no Fermi-sea group
+    a.nw = nw;
+    a.nch = nch;
+    a.stage_va = nw <= SC_LDS_NW;
+    a.weight = 1.0;
+    a.tetra = 0;
+    a.nEF = 0;
+    a.Ec = nullptr;
+    a.ec_stride = 0;
+    a.Ef = nullptr;
+    if (prec == WBMI_F64) return scan_dispatch_tab_f64(ctx, nk, a, E_dev, Xbar_dev, out_dev);
+    if (prec == WBMI_F32) return scan_dispatch_tab_f32(ctx, nk, a, E_dev, Xbar_dev, out_dev);
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate: bad precision");
 }
 
 // ---------------------------------------------------------------------------

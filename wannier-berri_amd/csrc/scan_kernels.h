@@ -972,7 +972,10 @@ __device__ inline int fermi_bin(const wbmi_scan_params& p, double E) {
     return -1;
 }
 
-template <typename T, int F, bool LM, bool ST, bool TET>
+// MODE: 0 = Fermi-level bins, 1 = tetrahedron weights, 2 = per-band table (hist is then
+// out[nk][nw][ncomp]: each band gets its group's trace / group size, or, for the identity
+// formula, the group's mean energy)
+template <typename T, int F, bool LM, bool ST, int MODE>
 __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
                                                             const cplx<T>* __restrict__ Xall,
                                                             double* __restrict__ hist) {
@@ -985,7 +988,8 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     static_assert(tasks || F == WBMI_FORMULA_OMEGA || !ST, "V and A are staged for DerOmega, DerMorb and Omega");
     constexpr bool stage_v = ST;
     const bool stage_a = ST && p.external_terms;
-    constexpr bool tetra = TET;   // compile-time: the Fermi-bin kernels carry no tetrahedron code
+    constexpr bool tetra = MODE == 1;  // compile-time: the Fermi-bin kernels carry no tetrahedron code
+    constexpr bool tab = MODE == 2;
     const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tasks, tetra);
     for (int n = tid; n < nw; n += SC_THREADS) S.E[n] = Eall[k * nw + n];
     for (int r = tid; r < WBMI_NROLES; r += SC_THREADS) S.ch[r] = p.ch[r];
@@ -1057,7 +1061,9 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
         for (int g = 0; g < ng; ++g) {
             int a = S.gb[g], b = S.gb[g + 1];
             int inr;
-            if (tetra) {
+            if (tab) {
+                inr = 1;  // every group (Tabulator, tabulate.py:25-31)
+            } else if (tetra) {
                 // range over centre and corners against [Ef[0], Ef[-1]] (tetrahedron.py:140-152)
                 double mn = S.tEmn[a], mx = S.tEmx[a];
                 for (int i = a; i < b; ++i) {
@@ -1076,7 +1082,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             S.gw[g] = sum / (b - a);
         }
         int bandmax = 0;
-        if (p.fder == 0) {
+        if (p.fder == 0 && !tab) {
             // tetra: bands whose highest energy lies below Ef[0] (tetrahedron.py:155-162)
             for (int i = 0; i < nw; ++i)
                 if ((tetra ? S.tEmx[i] : S.E[i]) < p.EFmin) bandmax = i + 1;
@@ -1102,8 +1108,9 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     K.sV = S.V;
     K.sA = S.A;
 
-    if (additive) {
-        // main partition: every group at once
+    if (additive || tab) {
+        // main partition: every group at once (each group's trace with inn = the group,
+        // out = the rest: the additive scan and the per-band table of any formula)
         for (int g = 0; g < ng; ++g)
             for (int n = S.gb[g] + tid; n < S.gb[g + 1]; n += SC_THREADS) {
                 S.lab[n] = g;
@@ -1115,6 +1122,18 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
         eval_partition<F>(S, K, p);
         for (int t = tid; t < ng * ncomp; t += SC_THREADS) {
             const int g = t / ncomp, c = t % ncomp;
+            if (tab) {
+                double v;
+                if (F == WBMI_FORMULA_IDENTITY) {
+                    v = S.gw[g];  // Eavln: mean energy of the group
+                } else {
+                    v = 0.0;
+                    for (int n = S.gb[g]; n < S.gb[g + 1]; ++n) v += S.acc[n * 9 + c].re;
+                    v /= (S.gb[g + 1] - S.gb[g]);
+                }
+                for (int n = S.gb[g]; n < S.gb[g + 1]; ++n) hist[(k * nw + n) * ncomp + c] = v;
+                continue;
+            }
             if (tetra) {
                 if (!S.gin[g]) continue;
                 double s = 0.0;
@@ -1194,15 +1213,15 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     }
 }
 
-template <typename T, int F, bool LM, bool ST, bool TET>
+template <typename T, int F, bool LM, bool ST, int MODE>
 static int launch_scan_t(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
     const bool stage_a = ST && a.p.external_terms;
-    size_t lds = smem_offsets(a.nw, ST, stage_a, tasks, TET).total;
+    size_t lds = smem_offsets(a.nw, ST, stage_a, tasks, MODE == 1).total;
     if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM, ST, TET>,
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM, ST, MODE>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_static_scan<T, F, LM, ST, TET>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
+    hipLaunchKernelGGL((k_static_scan<T, F, LM, ST, MODE>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
                        (const cplx<T>*)X, hist);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
@@ -1210,25 +1229,25 @@ static int launch_scan_t(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
 
 // dEinv o Off in LDS up to SC_M_NW (the layout of smem_offsets), formed on the fly above;
 // V and A in LDS up to SC_LDS_NW for the formulas with wave tasks
-template <typename T, int F, bool TET>
+template <typename T, int F, int MODE>
 static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
     constexpr bool staged = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB || F == WBMI_FORMULA_OMEGA;
     if constexpr (staged)
-        if (a.stage_va) return launch_scan_t<T, F, true, true, TET>(ctx, nk, a, E, X, hist);
-    if (a.nw <= SC_M_NW) return launch_scan_t<T, F, true, false, TET>(ctx, nk, a, E, X, hist);
-    return launch_scan_t<T, F, false, false, TET>(ctx, nk, a, E, X, hist);
+        if (a.stage_va) return launch_scan_t<T, F, true, true, MODE>(ctx, nk, a, E, X, hist);
+    if (a.nw <= SC_M_NW) return launch_scan_t<T, F, true, false, MODE>(ctx, nk, a, E, X, hist);
+    return launch_scan_t<T, F, false, false, MODE>(ctx, nk, a, E, X, hist);
 }
 
-template <typename T, bool TET>
+template <typename T, int MODE>
 static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
     switch (a.p.formula) {
-        case WBMI_FORMULA_IDENTITY: return launch_scan<T, WBMI_FORMULA_IDENTITY, TET>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_OMEGA: return launch_scan<T, WBMI_FORMULA_OMEGA, TET>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_DEROMEGA: return launch_scan<T, WBMI_FORMULA_DEROMEGA, TET>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_VELOMEGA: return launch_scan<T, WBMI_FORMULA_VELOMEGA, TET>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_MORB_HPM: return launch_scan<T, WBMI_FORMULA_MORB_HPM, TET>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_VELHPLUS: return launch_scan<T, WBMI_FORMULA_VELHPLUS, TET>(ctx, nk, a, E, X, hist);
-        case WBMI_FORMULA_DERMORB: return launch_scan<T, WBMI_FORMULA_DERMORB, TET>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_IDENTITY: return launch_scan<T, WBMI_FORMULA_IDENTITY, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_OMEGA: return launch_scan<T, WBMI_FORMULA_OMEGA, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_DEROMEGA: return launch_scan<T, WBMI_FORMULA_DEROMEGA, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_VELOMEGA: return launch_scan<T, WBMI_FORMULA_VELOMEGA, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_MORB_HPM: return launch_scan<T, WBMI_FORMULA_MORB_HPM, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_VELHPLUS: return launch_scan<T, WBMI_FORMULA_VELHPLUS, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_DERMORB: return launch_scan<T, WBMI_FORMULA_DERMORB, MODE>(ctx, nk, a, E, X, hist);
         default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
     }
 }
@@ -1238,3 +1257,5 @@ int scan_dispatch_f64(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E,
 int scan_dispatch_f32(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist);
 int scan_dispatch_tetra_f64(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist);
 int scan_dispatch_tetra_f32(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist);
+int scan_dispatch_tab_f64(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist);
+int scan_dispatch_tab_f32(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist);

@@ -15,7 +15,8 @@ numerics run in libwbmi (HIP, gfx950):
   formula traces + Fermi scan               -> wbmi_static_scan (called by calculators)
 
 A "channel" is one complex nw x nw matrix per k: a component of
-Xbar(name, der) for name in Ham, AA, rotAA, BB, CC and der in 0, 1 (2 for Ham).
+Xbar(name, der) for name in Ham, AA, rotAA, BB, CC and der in 0, 1 (2 for Ham),
+or of the spin matrices Xbar('SS'), Xbar('SA'), Xbar('SHA') (der 0).
 """
 import numpy as np
 
@@ -25,11 +26,12 @@ from ._lib import WBMI_F32, WBMI_F64
 # components of each Xbar(name, der)
 CHANNEL_NCOMP = {("Ham", 0): 1, ("Ham", 1): 3, ("Ham", 2): 9,
                  ("AA", 0): 3, ("AA", 1): 9, ("rotAA", 0): 3, ("rotAA", 1): 9,
-                 ("BB", 0): 3, ("BB", 1): 9, ("CC", 0): 3, ("CC", 1): 9}
+                 ("BB", 0): 3, ("BB", 1): 9, ("CC", 0): 3, ("CC", 1): 9,
+                 ("SS", 0): 3, ("SA", 0): 9, ("SHA", 0): 9}
 CHANNEL_ORDER = list(CHANNEL_NCOMP)
 # matrices forced Hermitian after R->k (data_K/data_K_R.py:87)
-HERMITIAN = {"AA", "rotAA"}
-BASES = ["Ham", "AA", "BB", "CC"]
+HERMITIAN = {"AA", "rotAA", "SS"}
+BASES = ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA"]
 # At most this many FP64 Ogita-Aishima steps after the FP32 Jacobi (the
 # kernel stops at ||E||_F < 1e-8 (complex64 vectors) or 1e-10 (complex128):
 # two or three steps for separated spectra, up to about ten for a pair
@@ -61,7 +63,8 @@ def channel_terms(name, der, base_offset):
             return [[[1, b0, a, -1], [0, 0, -1, -1]] for a in range(3)]
         return [[[1, b0, a, b], [0, 0, -1, -1]] for a in range(3) for b in range(3)]
     if der == 0:
-        return [[[1, b0 + a, -1, -1], [0, 0, -1, -1]] for a in range(3)]
+        # SA and SHA carry two Cartesian indices (a, s), row-major
+        return [[[1, b0 + a, -1, -1], [0, 0, -1, -1]] for a in range(CHANNEL_NCOMP[(name, 0)])]
     return [[[1, b0 + a, d, -1], [0, 0, -1, -1]] for a in range(3) for d in range(3)]
 
 
@@ -296,6 +299,12 @@ class Data_K:
                 del H
             self._Ec = Ec
         return self._Ec
+
+    def tabulate(self, params, out):
+        """Per-band table [nk, nw, ncomp] (see wbmi_tabulate)."""
+        for i, v in enumerate(self.role_table()):
+            params.ch[i] = v
+        self.ctx.tabulate(self.E_dev, self.Xk, self.prec, params, out)
 
     def static_scan_tetra(self, params, Efermi, hist, weight=1.0):
         """Tetrahedron-weighted scan into hist [2 nEF, ncomp] (see wbmi_static_scan_tetra)."""

@@ -13,4 +13,5 @@ factor_morb_evA2_to_muB = -(elementary_charge * angstrom ** 2) * elementary_char
 factor_gme_orb = factor_morb_evA2_to_muB * bohr_magneton / angstrom ** 2
 factor_ahc = -(elementary_charge ** 2 / hbar / angstrom)
 factor_opt = -factor_ahc   # factors.py:36
+factor_shc = -factor_ahc   # factors.py:37
 factor_nlahc = elementary_charge ** 3 / hbar ** 2 * TAU_UNIT

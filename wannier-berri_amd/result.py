@@ -205,6 +205,131 @@ class EnergyResult(Result):
             return cls(Energies=E, data=f["data"], rank=int(f["rank"]) if "rank" in f.files else None)
 
 
+class KBandResult(Result):
+    """Per-(k, band) data [nk, nband, 3, ...] (result/kbandresult.py): K-blocks concatenate
+    along k; K-point factors do not apply."""
+
+    def __init__(self, data, transformTR=None, transformInv=None):
+        self.data = np.asarray(data)
+        self.transformTR = transformTR
+        self.transformInv = transformInv
+
+    @property
+    def nk(self):
+        return self.data.shape[0]
+
+    @property
+    def nband(self):
+        return self.data.shape[1]
+
+    @property
+    def rank(self):
+        return self.data.ndim - 2
+
+    def __add__(self, other):
+        if other is None or (isinstance(other, (int, float)) and other == 0):
+            return self
+        return KBandResult(np.concatenate((self.data, other.data), axis=0), self.transformTR, self.transformInv)
+
+    def __mul__(self, other):
+        return self
+
+    __truediv__ = __mul__
+
+    def to_grid(self, k_map):
+        """Grid point i gets the mean over the k-points k_map[i] (kbandresult.py)."""
+        return KBandResult(np.array([self.data[km].mean(axis=0) for km in k_map]), self.transformTR,
+                           self.transformInv)
+
+    def get_component(self, component=None):
+        """'x', 'xy', 'trace', ... of the Cartesian indices (kbandresult.py)."""
+        if component is None or component == "":
+            return self.data
+        xyz = {"x": 0, "y": 1, "z": 2}
+        if component == "trace":
+            assert self.rank == 2
+            return self.data[:, :, 0, 0] + self.data[:, :, 1, 1] + self.data[:, :, 2, 2]
+        assert len(component) == self.rank
+        return self.data[(slice(None), slice(None)) + tuple(xyz[c] for c in component)]
+
+
+class TABresult(Result):
+    """k-resolved results of TabulatorAll (result/tabresult.py:11-212): k-points and a
+    KBandResult per quantity; K-blocks concatenate, and run() puts the k-points on the
+    grid at the end (`self_to_grid`)."""
+
+    def __init__(self, kpoints, recip_lattice, results=None, mode="grid", save_mode="bin"):
+        self.results = results or {}
+        self.nband = self.results["Energy"].nband
+        self.mode = mode
+        self.grid = None
+        self.gridorder = None
+        self.recip_lattice = recip_lattice
+        self.kpoints = np.array(kpoints, dtype=float) % 1
+        self.save_mode = save_mode
+        for k, r in self.results.items():
+            assert len(self.kpoints) == r.nk, f"{len(self.kpoints)} k-points but {r.nk} in '{k}'"
+
+    @property
+    def Enk(self):
+        return self.results["Energy"]
+
+    def __mul__(self, other):
+        return self   # K-point factors do not apply to tabulated quantities
+
+    __truediv__ = __mul__
+
+    def __add__(self, other):
+        if other is None or (isinstance(other, (int, float)) and other == 0):
+            return self
+        results = {r: self.results[r] + other.results[r] for r in self.results if r in other.results}
+        return TABresult(np.vstack((self.kpoints, other.kpoints)), self.recip_lattice, results, self.mode,
+                         self.save_mode)
+
+    @property
+    def max(self):
+        return np.zeros(0)
+
+    def transform(self, sym):
+        raise NotImplementedError("tabulation with symmetrisation is not available on the MI355X path "
+                                  "(run with use_irred_kpt=False, symmetrize=False)")
+
+    def savedata(self, name, prefix, suffix, i_iter):
+        pass   # as the reference (tabresult.py:56-60)
+
+    def to_grid(self, grid, order="C"):
+        """Results on the full grid, k index (i0*N1 + i1)*N2 + i2 (tabresult.py:134-170)."""
+        grid = np.array(grid, dtype=int)
+        g1 = [np.linspace(0., 1., g, False) for g in grid]
+        k_new = np.array(np.meshgrid(g1[0], g1[1], g1[2], indexing="ij")).reshape((3, -1), order=order).T
+        ki = np.rint(self.kpoints * grid[None, :]).astype(int)
+        on_grid = np.all(abs(ki / grid[None, :] - self.kpoints) < 1e-5, axis=1)
+        ki = ki % grid[None, :]
+        ind = ki[:, 2] + grid[2] * (ki[:, 1] + grid[1] * ki[:, 0])
+        k_map = [[] for _ in range(int(np.prod(grid)))]
+        for ik in range(len(self.kpoints)):
+            if on_grid[ik]:
+                k_map[ind[ik]].append(ik)
+        res = TABresult(k_new, self.recip_lattice, {r: v.to_grid(k_map) for r, v in self.results.items()},
+                        self.mode, self.save_mode)
+        res.grid = grid
+        res.gridorder = order
+        return res
+
+    def self_to_grid(self, grid):
+        self.__dict__.update(self.to_grid(grid).__dict__)
+
+    def get_data(self, quantity, iband=None, component=None):
+        """Data of one quantity on the grid [N0, N1, N2, (bands), ...] (tabresult.py:186-230)."""
+        if iband is None:
+            iband = np.arange(self.nband)
+        r = self.Enk if quantity == "Energy" else self.results[quantity]
+        d = r.get_component(component)[:, iband]
+        if self.grid is None:
+            return d
+        return d.reshape(tuple(self.grid) + d.shape[1:])
+
+
 class ResultDict(Result):
     """Dictionary of results (result/resultdict.py:18-66)."""
 

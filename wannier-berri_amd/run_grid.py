@@ -14,7 +14,7 @@ import numpy as np
 
 from .data_k import Data_K
 from .grid import exclude_equiv_points
-from .result import ResultDict
+from .result import ResultDict, TABresult
 
 
 def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetrize=True, fout_name="result",
@@ -102,6 +102,10 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
             exclude_equiv_points(K_list, new_points=len(K_list) - l1)
     if verbose:
         print(f"run(): processed {counter} K-blocks in {time.time() - t0:.3f} s", flush=True)
+    # k-resolved results onto the full grid (run_grid.py:394-401)
+    for val in result_all.results.values():
+        if isinstance(val, TABresult):
+            val.self_to_grid(grid.dense)
     return result_all
 
 
