@@ -96,13 +96,19 @@ typedef struct wbmi_optcond_params {
 } wbmi_optcond_params;
 
 /* Parameters of the spin Hall conductivity (calculators/dynamic.py:204-241):
- * the optical-conductivity parameters (V and A roles) plus the channel slots
- * of Xbar('SS')[s], Xbar('SA')[a, s], Xbar('SHA')[a, s] (row-major (a, s)). */
+ * the optical-conductivity parameters (V and A roles), the spin current
+ * (0: Ryoo, 1: Qiao) and the channel slots of Xbar('SS')[s] and, for Ryoo,
+ * Xbar('SA')[a, s], Xbar('SHA')[a, s], for Qiao Xbar('SH')[s],
+ * Xbar('SR')[a, s], Xbar('SHR')[a, s] (row-major (a, s)).                  */
 typedef struct wbmi_shc_params {
     wbmi_optcond_params base;
+    int qiao;
     int ch_S[3];
     int ch_SA[9];
     int ch_SHA[9];
+    int ch_SH[3];
+    int ch_SR[9];
+    int ch_SHR[9];
 } wbmi_shc_params;
 
 /* ---- context ------------------------------------------------------------ */
@@ -210,10 +216,13 @@ int wbmi_eigh_guess(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hs
 int wbmi_jdos(wbmi_ctx* ctx, int nk, int nw, const double* E_dev, const wbmi_optcond_params* params,
               double* out_dev);
 
-/* Kubo spin Hall conductivity with the Ryoo spin current (dynamic.py:204-241,
- * formula/covariant.py:689-756; SHC_type='ryoo', external terms):
- *   J^{as} = (J' + J'^+)/2,  J'^{as}_mn = -i (E_n SA^{as}_mn - SHA^{as}_mn)
- *            + sum_l SS^s_ml V^a_ln,
+/* Kubo spin Hall conductivity with the Ryoo or Qiao spin current
+ * (dynamic.py:204-241, formula/covariant.py:689-756; external terms):
+ *   J^{as} = (J' + J'^+)/2,
+ *   Ryoo: J'^{as}_mn = -i (E_n SA^{as}_mn - SHA^{as}_mn) + sum_l SS^s_ml V^a_ln,
+ *   Qiao: J'^{as}_mn = v^a_n SS^s_mn + E_n (-i SR^{as}_mn + sum_l SS^s_ml D^a_ln)
+ *                      - (-i SHR^{as}_mn + sum_l SH^s_ml D^a_ln),
+ *         v^a_n = Re V^a_nn, D = D_H,
  *   B^b = -i (i D_H + Abar)^b,
  *   out[EF][w][a][b][s] += (f(E_n) - f(E_m)) sum_{i in m, j in n}
  *                          Im(J^{as}_ij B^b_ji) / (2 (E_m - E_n - w - i eta))

@@ -1216,10 +1216,13 @@ class OpticalConductivity:
 
 
 class SHC(OpticalConductivity):
-    """Kubo spin Hall conductivity (dynamic.py:204-241) with the Ryoo spin current
+    """Kubo spin Hall conductivity (dynamic.py:204-241) with the Ryoo or Qiao spin current
     (formula/covariant.py:689-756).
 
-    J^{as} = (J' + J'^+)/2, J'^{as}_mn = -i (E_n SA^{as}_mn - SHA^{as}_mn) + sum_l SS^s_ml V^a_ln;
+    J^{as} = (J' + J'^+)/2,
+      Ryoo: J'^{as}_mn = -i (E_n SA^{as}_mn - SHA^{as}_mn) + sum_l SS^s_ml V^a_ln,
+      Qiao: J'^{as}_mn = v^a_n SS^s_mn + E_n (-i SR^{as}_mn + sum_l SS^s_ml D^a_ln)
+                         - (-i SHR^{as}_mn + sum_l SH^s_ml D^a_ln),  v^a_n = Re V^a_nn;
     B^b = -i A^b, A = i D_H (+ Abar); per pair (m, n) of band groups
     M^{abs} = sum_{i in m, j in n} Im(J^{as}_ij B^b_ji), times (f(E_n) - f(E_m)) per
     Fermi level and 1 / (2 (E_m - E_n - omega - i eta)) per frequency (Gaussian: the
@@ -1228,8 +1231,9 @@ class SHC(OpticalConductivity):
 
     def __init__(self, Efermi, omega, SHC_type="ryoo", **kw):
         super().__init__(Efermi, omega, **kw)
-        if SHC_type != "ryoo":
+        if SHC_type not in ("ryoo", "qiao"):
             raise NotImplementedError(SHC_type)
+        self.SHC_type = SHC_type
 
     def factor_omega(self, E1, E2):
         arg = E1 - E2 - self.omega
@@ -1243,8 +1247,16 @@ class SHC(OpticalConductivity):
         if not ext:
             raise NotImplementedError("the Ryoo spin current needs the external terms (covariant.py:747-749)")
         E = data.E_K
-        J = -1j * (E[:, None, :, None, None] * data.Xbar("SA") - data.Xbar("SHA"))
-        J = J + np.einsum("kmls,klna->kmnas", data.Xbar("SS"), data.Xbar("Ham", 1))
+        if self.SHC_type == "ryoo":
+            J = -1j * (E[:, None, :, None, None] * data.Xbar("SA") - data.Xbar("SHA"))
+            J = J + np.einsum("kmls,klna->kmnas", data.Xbar("SS"), data.Xbar("Ham", 1))
+        else:
+            # formula/covariant.py:729-744; delE_K = Re diag V (data_K.py:236-242)
+            S, D = data.Xbar("SS"), data.D_H
+            v = np.einsum("klla->kla", data.Xbar("Ham", 1)).real
+            K = -1j * data.Xbar("SR") + np.einsum("kmls,klna->kmnas", S, D)
+            L = -1j * data.Xbar("SHR") + np.einsum("kmls,klna->kmnas", data.Xbar("SH"), D)
+            J = v[:, None, :, :, None] * S[:, :, :, None, :] + E[:, None, :, None, None] * K - L
         J = (J + J.swapaxes(1, 2).conj()) / 2
         A = 1j * data.D_H + data.Xbar("AA")
         B = -1j * A
@@ -1426,7 +1438,7 @@ def run_adaptive(system, calculators, NK=None, NKFFT=None, adpt_num_iter=1, adpt
 def load_golden_system(path):
     """Read a system written by tests/golden/make_golden.py (already `[R, m, n, ...]`)."""
     with np.load(path, allow_pickle=False) as f:
-        mats = {k: f[k] for k in f.files if k in ('Ham', 'AA', 'BB', 'CC', 'OO', 'SS', 'SA', 'SHA')}
+        mats = {k: f[k] for k in f.files if k in ('Ham', 'AA', 'BB', 'CC', 'OO', 'SS', 'SA', 'SHA', 'SR', 'SH', 'SHR')}
         periodic = f['periodic'] if 'periodic' in f.files else (True, True, True)
         fito = bool(f['force_internal_terms_only']) if 'force_internal_terms_only' in f.files else False
         return OracleSystem(f['real_lattice'], f['iRvec'], f['wannier_centers_cart'], mats,

@@ -55,7 +55,8 @@ def main():
            extra={"force_internal_terms_only": True})
     system("reference/systems/Chiral_left", "sys_chiral_left.npz", ["Ham"], legacy=True,
            extra={"force_internal_terms_only": True})
-    system("data/random", "sys_random.npz", ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA"], legacy=True)
+    system("data/random", "sys_random.npz", ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SR", "SH", "SHR"],
+           legacy=True)
     system("reference/systems/Fe_W90", "sys_fe_w90.npz", ["Ham", "AA", "BB", "CC"], legacy=True)
     system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz", ["Ham", "AA", "BB", "CC"], legacy=False)
     results({"haldane_ahc": I + "Haldane-ahc_iter-0000.npz",
@@ -82,6 +83,7 @@ def main():
              # (tests/test_run.py:653-667)
              "random_opt_conductivity": I + "random-opt_conductivity_iter-0000.npz",
              "random_opt_SHCryoo": I + "random-opt_SHCryoo_iter-0000.npz",
+             "random_opt_SHCqiao": I + "random-opt_SHCqiao_iter-0000.npz",
              # tetrahedron method: Chiral model with C3z (tests/test_run.py:227-238,835-845)
              # and the Haldane C3z refinement with adpt_mesh 3 (tests/test_run.py:730-735)
              "chiral_tetra_ahc": I + "Chiral_tetra-ahc_iter-0000.npz",

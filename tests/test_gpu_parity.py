@@ -301,7 +301,7 @@ def test_chiral_jdos(wb, results_run):
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
 def test_random_optical_and_spin_hall(wb, results_run, precision, tol):
     """Kubo optical conductivity (external terms) and spin Hall conductivity with the Ryoo
-    spin current against the reference's fixtures on the random model (Gaussian 0.2 eV,
+    and Qiao spin currents against the reference's fixtures on the random model (Gaussian 0.2 eV,
     Efermi 17 and 18 eV, omega 0..7 eV, grid 6^3 / 3^3; tests/test_run.py:653-667)."""
     say(f"test_random_optical_and_spin_hall {precision}")
     dyn = wb.calculators.dynamic
@@ -314,6 +314,10 @@ def test_random_optical_and_spin_hall(wb, results_run, precision, tol):
     e_shc = rel_err(r.results["shc"].data, results_run["random_opt_SHCryoo"])
     say(f"  opt_conductivity {e_oc:.2e}, SHC ryoo {e_shc:.2e}")
     assert e_oc < tol and e_shc < tol
+    rq = run_gpu(wb, s, {"shc": dyn.SHC(SHC_type="qiao", **kw)}, [6, 6, 6], [3, 3, 3], precision=precision)
+    e_q = rel_err(rq.results["shc"].data, results_run["random_opt_SHCqiao"])
+    say(f"  SHC qiao {e_q:.2e}")
+    assert e_q < tol
     one = run_gpu(wb, s, {"shc": dyn.SHC(SHC_type="ryoo", shc_abc=(1, 2, 3), **kw)}, [6, 6, 6], [3, 3, 3],
                   precision=precision)
     assert np.allclose(one.results["shc"].data, r.results["shc"].data[:, :, 0, 1, 2], rtol=0, atol=1e-12 *

@@ -255,7 +255,7 @@ def test_chiral_jdos(results_run):
 
 def test_random_optical_and_spin_hall(results_run):
     """Kubo optical conductivity (external terms) and spin Hall conductivity with the Ryoo
-    spin current on the random model, Gaussian 0.2 eV, Efermi 17 and 18 eV, omega 0..7 eV
+    and Qiao spin currents on the random model, Gaussian 0.2 eV, Efermi 17 and 18 eV, omega 0..7 eV
     (tests/test_run.py:653-667, default grid 6^3 / 3^3)."""
     s = gsys("random")
     kw = dict(smr_fixed_width=0.2, smr_type="Gaussian")
@@ -264,6 +264,8 @@ def test_random_optical_and_spin_hall(results_run):
               NK=[6, 6, 6], NKFFT=[3, 3, 3])
     assert rel_err(r["oc"].data, results_run["random_opt_conductivity"]) < 1e-8
     assert rel_err(r["shc"].data, results_run["random_opt_SHCryoo"]) < 1e-8
+    rq = O.run(s, {"shc": O.SHC(Ef, om, SHC_type="qiao", **kw)}, NK=[6, 6, 6], NKFFT=[3, 3, 3])
+    assert rel_err(rq["shc"].data, results_run["random_opt_SHCqiao"]) < 1e-8
 
 
 def _tabulate_grid(s, quantities, ibands, NK, NKFFT):

@@ -71,8 +71,9 @@ class OptcondParams(ctypes.Structure):
 
 
 class ShcParams(ctypes.Structure):
-    _fields_ = [("base", OptcondParams), ("ch_S", ctypes.c_int * 3), ("ch_SA", ctypes.c_int * 9),
-                ("ch_SHA", ctypes.c_int * 9)]
+    _fields_ = [("base", OptcondParams), ("qiao", ctypes.c_int), ("ch_S", ctypes.c_int * 3),
+                ("ch_SA", ctypes.c_int * 9), ("ch_SHA", ctypes.c_int * 9), ("ch_SH", ctypes.c_int * 3),
+                ("ch_SR", ctypes.c_int * 9), ("ch_SHR", ctypes.c_int * 9)]
 
 
 _lib = None

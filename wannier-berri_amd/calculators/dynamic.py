@@ -114,14 +114,16 @@ class JDOS(Calculator):
 
 
 class SHC(Calculator):
-    r"""Kubo spin Hall conductivity (dynamic.py:224-241) with the Ryoo spin current
-    (formula/covariant.py:746-756); SHC_type 'qiao' and 'simple' are not available."""
+    r"""Kubo spin Hall conductivity (dynamic.py:224-241) with the Ryoo or Qiao spin current
+    (formula/covariant.py:729-756); SHC_type 'simple' is not available."""
 
     def __init__(self, SHC_type="ryoo", shc_abc=None, Efermi=None, omega=None, kBT=0, smr_fixed_width=0.1,
                  smr_type="Lorentzian", kwargs_formula=None, dtype=complex, **kwargs):
         super().__init__(**kwargs)
-        if SHC_type != "ryoo":
-            raise NotImplementedError(f"SHC_type='{SHC_type}' is not available on the MI355X path (only 'ryoo')")
+        if SHC_type not in ("ryoo", "qiao"):
+            raise NotImplementedError(f"SHC_type='{SHC_type}' is not available on the MI355X path "
+                                      "(only 'ryoo' and 'qiao')")
+        self.SHC_type = SHC_type
         if smr_type not in ("Lorentzian", "Gaussian"):
             raise ValueError(f"Invalid smearing type {smr_type}")
         self.kwargs_formula = dict(kwargs_formula or {})
@@ -140,12 +142,13 @@ class SHC(Calculator):
         self._dev_arrays = {}
 
     def needed_channels(self, system):
-        return {("Ham", 0), ("Ham", 1), ("AA", 0), ("SS", 0), ("SA", 0), ("SHA", 0)}
+        spin = [("SA", 0), ("SHA", 0)] if self.SHC_type == "ryoo" else [("SH", 0), ("SR", 0), ("SHR", 0)]
+        return {("Ham", 0), ("Ham", 1), ("AA", 0), ("SS", 0), *spin}
 
     def __call__(self, data_K):
         torch = data_K.torch
         if not _external(self.kwargs_formula, data_K.system):
-            raise NotImplementedError("spin Hall ryoo without external terms is not implemented yet. "
+            raise NotImplementedError(f"spin Hall {self.SHC_type} without external terms is not implemented yet. "
                                       "Use `SHC_type='simple'`")
         data_K.ensure(self.needed_channels(data_K.system))
         p = _lib.ShcParams()
@@ -154,11 +157,15 @@ class SHC(Calculator):
         for i, v in enumerate(data_K.role_table()):
             p.base.ch[i] = v
         sl = data_K._slots
+        p.qiao = int(self.SHC_type == "qiao")
         for c in range(3):
             p.ch_S[c] = sl[("SS", 0)] + c
+            p.ch_SH[c] = sl[("SH", 0)] + c if ("SH", 0) in sl else -1
         for c in range(9):
-            p.ch_SA[c] = sl[("SA", 0)] + c
-            p.ch_SHA[c] = sl[("SHA", 0)] + c
+            p.ch_SA[c] = sl[("SA", 0)] + c if ("SA", 0) in sl else -1
+            p.ch_SHA[c] = sl[("SHA", 0)] + c if ("SHA", 0) in sl else -1
+            p.ch_SR[c] = sl[("SR", 0)] + c if ("SR", 0) in sl else -1
+            p.ch_SHR[c] = sl[("SHR", 0)] + c if ("SHR", 0) in sl else -1
         nEF, nom = len(self.Efermi), len(self.omega)
         out = torch.zeros((nEF, nom, 3, 3, 3, 2), dtype=torch.float64, device=data_K.ctx.device)
         data_K.ctx.spin_hall(data_K.E_dev, data_K.Xk, data_K.prec, p, out)

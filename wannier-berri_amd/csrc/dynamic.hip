@@ -273,9 +273,13 @@ extern "C" int wbmi_jdos(wbmi_ctx* ctx, int nk, int nw, const double* E_dev, con
 // (formula/covariant.py:689-756).
 // ---------------------------------------------------------------------------
 
-// J^{as} = (J' + J'^+) / 2,  J'^{as}_ij = -i (E_j SA^{as}_ij - SHA^{as}_ij) + sum_l SS^s_il V^a_lj,
-// one workgroup per (k, component a*3+s), one thread per (i, j); both J'_ij and J'_ji are
-// formed by the same thread, so no barrier is needed.  J: complex128 [nk, 9, nw, nw].
+// J^{as} = (J' + J'^+) / 2, one workgroup per (k, component a*3+s), one thread per (i, j);
+// both J'_ij and J'_ji are formed by the same thread, so no barrier is needed.
+//   Ryoo (covariant.py:746-756): J'^{as}_ij = -i (E_j SA^{as}_ij - SHA^{as}_ij) + sum_l SS^s_il V^a_lj
+//   Qiao (covariant.py:729-744): J'^{as}_ij = v^a_j SS^s_ij + E_j (-i SR^{as}_ij + sum_l SS^s_il D^a_lj)
+//                                             - (-i SHR^{as}_ij + sum_l SH^s_il D^a_lj),
+//     v^a_j = Re V^a_jj, D^a_lj = -V^a_lj / (E_l - E_j) (0 below 1e-7).
+// J: complex128 [nk, 9, nw, nw].
 template <typename T>
 __global__ void __launch_bounds__(256) k_spin_current(int nk, int nw, int nch, wbmi_shc_params p,
                                                       const double* __restrict__ Eall,
@@ -283,17 +287,37 @@ __global__ void __launch_bounds__(256) k_spin_current(int nk, int nw, int nch, w
     const int k = blockIdx.x, comp = blockIdx.y, a = comp / 3, sp = comp % 3;
     const int64_t nw2 = (int64_t)nw * nw;
     const cplx<T>* Xk = Xall + (int64_t)k * nch * nw2;
-    const cplx<T>* SA = Xk + p.ch_SA[comp] * nw2;
-    const cplx<T>* SHA = Xk + p.ch_SHA[comp] * nw2;
+    const bool qiao = p.qiao != 0;
+    // Ryoo: SA, SHA; Qiao: SR, SHR (the same roles in the formula), and SH
+    const cplx<T>* SA = Xk + (qiao ? p.ch_SR[comp] : p.ch_SA[comp]) * nw2;
+    const cplx<T>* SHA = Xk + (qiao ? p.ch_SHR[comp] : p.ch_SHA[comp]) * nw2;
     const cplx<T>* S = Xk + p.ch_S[sp] * nw2;
+    const cplx<T>* SH = Xk + (qiao ? p.ch_SH[sp] : p.ch_S[sp]) * nw2;
     const cplx<T>* V = Xk + p.base.ch[WBMI_ROLE_V + a] * nw2;
     const double* E = Eall + (int64_t)k * nw;
     auto ld = [](const cplx<T>* X, int64_t t) { return mk((double)X[t].re, (double)X[t].im); };
+    auto dh = [&](int l, int j) {  // D^a_lj
+        const double de = E[l] - E[j];
+        const double m = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+        const cplx<double> v = ld(V, l * nw + j);
+        return mk(-m * v.re, -m * v.im);
+    };
     auto jprime = [&](int i, int j) {
         const cplx<double> sa = ld(SA, i * nw + j), sha = ld(SHA, i * nw + j);
-        // -i (E_j SA - SHA)
+        // -i (E_j SA - SHA)  (Qiao: -i (E_j SR - SHR))
         cplx<double> v = mk(E[j] * sa.im - sha.im, -(E[j] * sa.re - sha.re));
-        for (int l = 0; l < nw; ++l) v += ld(S, i * nw + l) * ld(V, l * nw + j);
+        if (!qiao) {
+            for (int l = 0; l < nw; ++l) v += ld(S, i * nw + l) * ld(V, l * nw + j);
+        } else {
+            cplx<double> ks = mk(0.0, 0.0), ls = mk(0.0, 0.0);
+            for (int l = 0; l < nw; ++l) {
+                const cplx<double> d = dh(l, j);
+                ks += ld(S, i * nw + l) * d;
+                ls += ld(SH, i * nw + l) * d;
+            }
+            v += E[j] * ks - ls;
+            v += ld(V, j * nw + j).re * ld(S, i * nw + j);
+        }
         return v;
     };
     for (int t = threadIdx.x; t < nw2; t += blockDim.x) {
@@ -458,9 +482,13 @@ extern "C" int wbmi_spin_hall(wbmi_ctx* ctx, int nk, int nw, int nch, const doub
         if (v < 0 || v >= nch || x < 0 || x >= nch || s < 0 || s >= nch)
             return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: missing V, A or SS channel");
     }
-    for (int c = 0; c < 9; ++c)
-        if (params->ch_SA[c] < 0 || params->ch_SA[c] >= nch || params->ch_SHA[c] < 0 || params->ch_SHA[c] >= nch)
+    auto bad = [&](int c) { return c < 0 || c >= nch; };
+    for (int c = 0; c < 9; ++c) {
+        if (!params->qiao && (bad(params->ch_SA[c]) || bad(params->ch_SHA[c])))
             return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: missing SA or SHA channel");
+        if (params->qiao && (bad(params->ch_SR[c]) || bad(params->ch_SHR[c]) || (c < 3 && bad(params->ch_SH[c]))))
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: missing SH, SR or SHR channel");
+    }
     if (prec == WBMI_F64)
         return launch_shc<double>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev);
     if (prec == WBMI_F32)

@@ -16,7 +16,8 @@ numerics run in libwbmi (HIP, gfx950):
 
 A "channel" is one complex nw x nw matrix per k: a component of
 Xbar(name, der) for name in Ham, AA, rotAA, BB, CC and der in 0, 1 (2 for Ham),
-or of the spin matrices Xbar('SS'), Xbar('SA'), Xbar('SHA') (der 0).
+or of the spin matrices Xbar('SS'), Xbar('SA'), Xbar('SHA'), Xbar('SH'),
+Xbar('SR'), Xbar('SHR') (der 0).
 """
 import numpy as np
 
@@ -27,11 +28,11 @@ from ._lib import WBMI_F32, WBMI_F64
 CHANNEL_NCOMP = {("Ham", 0): 1, ("Ham", 1): 3, ("Ham", 2): 9,
                  ("AA", 0): 3, ("AA", 1): 9, ("rotAA", 0): 3, ("rotAA", 1): 9,
                  ("BB", 0): 3, ("BB", 1): 9, ("CC", 0): 3, ("CC", 1): 9,
-                 ("SS", 0): 3, ("SA", 0): 9, ("SHA", 0): 9}
+                 ("SS", 0): 3, ("SA", 0): 9, ("SHA", 0): 9, ("SH", 0): 3, ("SR", 0): 9, ("SHR", 0): 9}
 CHANNEL_ORDER = list(CHANNEL_NCOMP)
 # matrices forced Hermitian after R->k (data_K/data_K_R.py:87)
 HERMITIAN = {"AA", "rotAA", "SS"}
-BASES = ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA"]
+BASES = ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SH", "SR", "SHR"]
 # At most this many FP64 Ogita-Aishima steps after the FP32 Jacobi (the
 # kernel stops at ||E||_F < 1e-8 (complex64 vectors) or 1e-10 (complex128):
 # two or three steps for separated spectra, up to about ten for a pair

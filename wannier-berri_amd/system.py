@@ -213,7 +213,7 @@ class System_R:
     def from_arrays_npz(cls, file_npz):
         """Single `.npz` with keys real_lattice, iRvec, wannier_centers_cart, Ham, AA, ... (`[R, m, n, ...]`)."""
         with np.load(file_npz, allow_pickle=False) as f:
-            mats = {k: f[k] for k in f.files if k in ("Ham", "AA", "BB", "CC", "OO", "SS", "SA", "SHA")}
+            mats = {k: f[k] for k in f.files if k in ("Ham", "AA", "BB", "CC", "OO", "SS", "SA", "SHA", "SR", "SH", "SHR")}
             periodic = f["periodic"] if "periodic" in f.files else (True, True, True)
             fito = bool(f["force_internal_terms_only"]) if "force_internal_terms_only" in f.files else False
             return cls(f["real_lattice"], f["iRvec"], f["wannier_centers_cart"], mats, periodic=periodic,
