@@ -299,6 +299,14 @@ int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double*
 int wbmi_tabulate(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
                   int prec, const wbmi_scan_params* params, double* out_dev);
 
+/* Per-band table of a channel diagonal (tabulate.py Velocity and Spin:
+ * formula/covariant.py:321-336, whose group traces are sums of diagonal
+ * elements): out[k][n][c] = mean over the band group of n of Re X^{ch3[c]}_mm.
+ *   ch3_host : int32 [3] channel slots (host);  out_dev : float64 [nk, nw, 3] */
+int wbmi_tabulate_diag(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                       int prec, const int32_t* ch3_host, double degen_thresh, int degen_Kramers,
+                       double* out_dev);
+
 /* out_dev [nEF, ncomp] = scale * (prefix sums of rows [0, nEF) + rows [nEF, 2 nEF))
  * of a wbmi_static_scan_tetra histogram (static.py:149-155, no fder stencil).
  * The step rows of hist_dev are overwritten with their prefix sums.         */

@@ -57,7 +57,7 @@ def main():
            extra={"force_internal_terms_only": True})
     system("data/random", "sys_random.npz", ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SR", "SH", "SHR"],
            legacy=True)
-    system("reference/systems/Fe_W90", "sys_fe_w90.npz", ["Ham", "AA", "BB", "CC"], legacy=True)
+    system("reference/systems/Fe_W90", "sys_fe_w90.npz", ["Ham", "AA", "BB", "CC", "SS"], legacy=True)
     system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz", ["Ham", "AA", "BB", "CC"], legacy=False)
     results({"haldane_ahc": I + "Haldane-ahc_iter-0000.npz",
              "chiral_ahc": I + "Chiral-ahc_iter-0000.npz",
@@ -100,7 +100,7 @@ def main():
     # (tests/test_run.py:271-288,313-345)
     with np.load(os.path.join(REF, I + "Fe_W90-tabulate.npz"), allow_pickle=False) as f:
         np.savez_compressed(os.path.join(HERE, "results_tabulate.npz"),
-                            **{k: f[k] for k in ["Energy", "berry", "Der_berry", "morb", "Der_morb"]})
+                            **{k: f[k] for k in ["Energy", "V", "spin", "berry", "Der_berry", "morb", "Der_morb"]})
     # per-formula gauge-invariant traces (tests/reference/formula/*.npz)
     forms = {}
     for name in ["Dcov", "DerDcov", "DEinv_ln", "InvMass", "Omega", "DerOmega", "Morb_H", "morb",

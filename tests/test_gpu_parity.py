@@ -630,15 +630,16 @@ def test_finalize_4001_levels(wb, small_synth):
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1.0), ("fp32", 200.0)])
 def test_fe_tabulate(wb, precision, tol):
-    """k-resolved tabulation (TabulatorAll: Energy, Berry curvature and its derivative,
-    orbital moment and its derivative) on Fe (Wannier90), grid 4^3 / 2^3, bands 5 and 6,
+    """k-resolved tabulation (TabulatorAll: Energy, velocity, spin, Berry curvature and its
+    derivative, orbital moment and its derivative) on Fe (Wannier90), grid 4^3 / 2^3, bands 5 and 6,
     against the reference's fixture at its tolerances (2e-8, Der_berry 5e-8, relative to
     the largest value; tests/test_run.py:271-345); the mixed form at 200x those."""
     say(f"test_fe_tabulate {precision}")
     tab = wb.calculators.tabulate
     s = gsys(wb, "fe_w90")
     pt = {"degen_thresh": 5e-2}
-    calc = tab.TabulatorAll({"Energy": tab.Energy(), "berry": tab.BerryCurvature(**pt),
+    calc = tab.TabulatorAll({"Energy": tab.Energy(), "V": tab.Velocity(**pt), "spin": tab.Spin(**pt),
+                             "berry": tab.BerryCurvature(**pt),
                              "Der_berry": tab.DerBerryCurvature(**pt), "morb": tab.OrbitalMoment(**pt),
                              "Der_morb": tab.DerOrbitalMoment(**pt)}, ibands=[5, 6])
     grid = wb.Grid(system=s, NK=[4, 4, 4], NKFFT=[2, 2, 2])

@@ -287,6 +287,9 @@ def _tabulate_grid(s, quantities, ibands, NK, NKFFT):
 
 
 TAB_QUANTITIES = {"Energy": (O.Eavln, {}),
+                  # Velocity = covariant('Ham', gender=1), Spin = covariant('SS') (covariant.py:321-336)
+                  "V": (lambda d: d.covariant("Ham", gender=1), {"degen_thresh": 5e-2}),
+                  "spin": (lambda d: O.Matrix_ln(d.Xbar("SS")), {"degen_thresh": 5e-2}),
                   "berry": (O.Omega, {"degen_thresh": 5e-2}),
                   "Der_berry": (O.DerOmega, {"degen_thresh": 5e-2}),
                   "morb": (O.morb, {"degen_thresh": 5e-2, "constant_factor": O.FACTOR_MORB_EVA2_TO_MUB}),

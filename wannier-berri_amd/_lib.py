@@ -38,7 +38,7 @@ SYMBOLS = [
     "wbmi_r2k_plan_sizes", "wbmi_r2k_box", "wbmi_r2k_exec", "wbmi_r2k_klist", "wbmi_eigh",
     "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_optical_conductivity", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
     "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos", "wbmi_spin_hall", "wbmi_spin_hall_work_bytes",
-    "wbmi_tabulate",
+    "wbmi_tabulate", "wbmi_tabulate_diag",
 ]
 
 
@@ -108,6 +108,7 @@ _PROTOS = {
     "wbmi_jdos": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(OptcondParams), _vp]),
     "wbmi_spin_hall_work_bytes": (ctypes.c_size_t, [_i, _i]),
     "wbmi_tabulate": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp]),
+    "wbmi_tabulate_diag": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _d, _i, _vp]),
     "wbmi_spin_hall": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t, _vp]),
 }
 
@@ -294,6 +295,13 @@ class Context:
         nch = X.shape[1] if X is not None else 1
         self.check(self._timed(f"tab_f{params.formula}", (nk, nw), lib().wbmi_tabulate, self.h, nk, nw, nch, _ptr(E),
                                _ptr(X), prec, ctypes.byref(params), _ptr(out)), "wbmi_tabulate")
+
+    def tabulate_diag(self, E, X, prec, ch3, degen_thresh, degen_Kramers, out):
+        nk, nw = E.shape
+        c_a, c_p = _host_i32(np.asarray(ch3, dtype=np.int32))
+        self.check(self._timed("tab_diag", (nk, nw), lib().wbmi_tabulate_diag, self.h, nk, nw, X.shape[1], _ptr(E),
+                               _ptr(X), prec, c_p, float(degen_thresh), int(bool(degen_Kramers)), _ptr(out)),
+                   "wbmi_tabulate_diag")
 
     def finalize(self, hist, fder, dEF, scale, out):
         nEF, ncomp = hist.shape

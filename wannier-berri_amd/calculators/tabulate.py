@@ -85,6 +85,39 @@ class DerOrbitalMoment(Tabulator):
     factor = factors.factor_morb_evA2_to_muB
 
 
+class _DiagTabulator(Tabulator):
+    """Tabulators whose group traces are sums of diagonal elements of one channel triple
+    (formula/covariant.py:321-336): one wbmi_tabulate_diag launch."""
+    key = None
+    ndim = 1
+
+    def needed_channels(self, system):
+        return {("Ham", 0), self.key}
+
+    def __call__(self, data_K):
+        torch = data_K.torch
+        data_K.ensure(self.needed_channels(data_K.system))
+        s0 = data_K._slots[self.key]
+        out = torch.empty((data_K.nk, data_K.num_wann, 3), dtype=torch.float64, device=data_K.ctx.device)
+        data_K.ctx.tabulate_diag(data_K.E_dev, data_K.Xk, data_K.prec, [s0, s0 + 1, s0 + 2], self.degen_thresh,
+                                 self.degen_Kramers, out)
+        ib = np.arange(data_K.num_wann) if self.ibands is None else self.ibands
+        data = out.cpu().numpy()[:, ib] * self.constant_factor
+        return KBandResult(data, transformTR=self.parity[0], transformInv=self.parity[1])
+
+
+class Velocity(_DiagTabulator):
+    """Band velocity dE/dk (Re of the diagonal of Xbar('Ham', 1))"""
+    key = ("Ham", 1)
+    parity = (-1, -1)
+
+
+class Spin(_DiagTabulator):
+    r""" Spin expectation :math:` \langle u | \mathbf{\sigma} | u \rangle`"""
+    key = ("SS", 0)
+    parity = (-1, 1)
+
+
 def _not_available(name):
     class _NA(Calculator):
         def __init__(self, *args, **kwargs):
@@ -93,9 +126,7 @@ def _not_available(name):
     return _NA
 
 
-Velocity = _not_available("Velocity")
 InvMass = _not_available("InvMass")
-Spin = _not_available("Spin")
 SpinBerry = _not_available("SpinBerry")
 
 
@@ -137,5 +168,5 @@ class TabulatorAll(Calculator):
                          save_mode=self.save_mode)
 
 
-__all__ = ["Tabulator", "TabulatorAll", "Energy", "BerryCurvature", "DerBerryCurvature", "OrbitalMoment",
-           "DerOrbitalMoment"]
+__all__ = ["Tabulator", "TabulatorAll", "Energy", "Velocity", "Spin", "BerryCurvature", "DerBerryCurvature",
+           "OrbitalMoment", "DerOrbitalMoment"]
