@@ -57,7 +57,8 @@ def main():
            extra={"force_internal_terms_only": True})
     system("data/random", "sys_random.npz", ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SR", "SH", "SHR"],
            legacy=True)
-    system("reference/systems/Fe_W90", "sys_fe_w90.npz", ["Ham", "AA", "BB", "CC", "SS"], legacy=True)
+    system("reference/systems/Fe_W90", "sys_fe_w90.npz", ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SR", "SH", "SHR"],
+           legacy=True)
     system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz", ["Ham", "AA", "BB", "CC"], legacy=False)
     results({"haldane_ahc": I + "Haldane-ahc_iter-0000.npz",
              "chiral_ahc": I + "Chiral-ahc_iter-0000.npz",
@@ -84,6 +85,10 @@ def main():
              "random_opt_conductivity": I + "random-opt_conductivity_iter-0000.npz",
              "random_opt_SHCryoo": I + "random-opt_SHCryoo_iter-0000.npz",
              "random_opt_SHCqiao": I + "random-opt_SHCqiao_iter-0000.npz",
+             # the same on Fe (Wannier90), grid 4^3 / 2^3 (tests/test_run.py:271-311)
+             "fe_opt_conductivity": I + "Fe_W90-opt_conductivity_iter-0000.npz",
+             "fe_opt_SHCryoo": I + "Fe_W90-opt_SHCryoo_iter-0000.npz",
+             "fe_opt_SHCqiao": I + "Fe_W90-opt_SHCqiao_iter-0000.npz",
              # tetrahedron method: Chiral model with C3z (tests/test_run.py:227-238,835-845)
              # and the Haldane C3z refinement with adpt_mesh 3 (tests/test_run.py:730-735)
              "chiral_tetra_ahc": I + "Chiral_tetra-ahc_iter-0000.npz",

@@ -324,6 +324,23 @@ def test_random_optical_and_spin_hall(wb, results_run, precision, tol):
                        np.abs(r.results["shc"].data).max())
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_fe_optical_and_spin_hall(wb, results_run, precision, tol):
+    """Kubo optical and spin Hall (Ryoo, Qiao) conductivity on Fe (Wannier90, nw = 18), grid
+    4^3 / 2^3, against the reference's fixtures (tests/test_run.py:271-311)."""
+    say(f"test_fe_optical_and_spin_hall {precision}")
+    dyn = wb.calculators.dynamic
+    s = gsys(wb, "fe_w90")
+    kw = dict(Efermi=np.array([17.0, 18.0]), omega=np.arange(0.0, 7.1, 1.0), smr_fixed_width=0.2,
+              smr_type="Gaussian")
+    r = run_gpu(wb, s, {"oc": dyn.OpticalConductivity(**kw), "ry": dyn.SHC(SHC_type="ryoo", **kw),
+                        "qi": dyn.SHC(SHC_type="qiao", **kw)}, [4, 4, 4], [2, 2, 2], precision=precision)
+    for k, ref in (("oc", "fe_opt_conductivity"), ("ry", "fe_opt_SHCryoo"), ("qi", "fe_opt_SHCqiao")):
+        e = rel_err(r.results[k].data, results_run[ref])
+        say(f"  {ref}: {e:.2e}")
+        assert e < tol, k
+
+
 def test_synthetic_jdos(wb, small_synth):
     """JDOS with Lorentzian smearing at finite temperature vs the oracle (nw = 8, k-grid 4^3)."""
     say("test_synthetic_jdos")

@@ -305,3 +305,15 @@ def test_fe_tabulate():
     r = _tabulate_grid(gsys("fe_w90"), TAB_QUANTITIES, [5, 6], [4, 4, 4], [2, 2, 2])
     for k in TAB_QUANTITIES:
         assert rel_err(r[k], ref[k]) < (5e-8 if k == "Der_berry" else 2e-8), k
+
+
+def test_fe_optical_and_spin_hall(results_run):
+    """Kubo optical and spin Hall (Ryoo, Qiao) conductivity on Fe (Wannier90), grid 4^3 / 2^3,
+    Gaussian 0.2 eV, Efermi 17 and 18 eV, omega 0..7 eV (tests/test_run.py:271-311)."""
+    s = gsys("fe_w90")
+    kw = dict(smr_fixed_width=0.2, smr_type="Gaussian")
+    Ef, om = np.array([17.0, 18.0]), np.arange(0.0, 7.1, 1.0)
+    r = O.run(s, {"oc": O.OpticalConductivity(Ef, om, **kw), "ry": O.SHC(Ef, om, SHC_type="ryoo", **kw),
+                  "qi": O.SHC(Ef, om, SHC_type="qiao", **kw)}, NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    for k, ref in (("oc", "fe_opt_conductivity"), ("ry", "fe_opt_SHCryoo"), ("qi", "fe_opt_SHCqiao")):
+        assert rel_err(r[k].data, results_run[ref]) < 1e-8, k
