@@ -235,6 +235,23 @@ int wbmi_spin_hall(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, 
                    int prec, const wbmi_shc_params* params, void* work_dev, size_t work_bytes,
                    double* out_dev);
 
+/* Shift current (dynamic.py:244-324) and injection current (dynamic.py:327-361)
+ * with the pair walk of wbmi_spin_hall:
+ *   shift:     out[EF][w][a][b][c] += (f(E_n) - f(E_m)) sum_{i in m, j in n}
+ *              (-Im A'^{ca}_ij A^b_ji - Im A'^{ba}_ij A^c_ji) (g(E_m - E_n - w) + g(E_n - E_m - w)),
+ *              A' the generalised derivative of A regularised by sc_eta (roles V, W, and
+ *              with external terms A, AD);
+ *   injection: out[EF][w][a][b][c] += (f(E_n) - f(E_m)) sum (v^a_i - v^a_j) A^b_ij A^c_ji g(E_m - E_n - w),
+ * A = i D_H (+ Abar), g = the smearing function.
+ *   out_dev : float64 [nEF, nomega, 27, 2] (complex), accumulated; the caller scales.
+ *   work_dev: wbmi_spin_hall_work_bytes(nk, nw) bytes.                        */
+int wbmi_shift_current(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                       int prec, const wbmi_optcond_params* params, double sc_eta, void* work_dev,
+                       size_t work_bytes, double* out_dev);
+int wbmi_injection_current(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                           int prec, const wbmi_optcond_params* params, void* work_dev, size_t work_bytes,
+                           double* out_dev);
+
 int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
                               int prec, const wbmi_optcond_params* params, double* out_dev);
 

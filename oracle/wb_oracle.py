@@ -1295,6 +1295,97 @@ def tabulate(data, Formula, ibands, degen_thresh=1e-4, degen_Kramers=False, cons
     return np.array(out) * constant_factor
 
 
+FACTOR_SHIFT_CURRENT = _hbar / _e * np.pi * _e ** 3 / (4 * _hbar ** 2)       # factors.py:50
+FACTOR_INJECTION_CURRENT = -np.pi * _e ** 3 / _hbar ** 2 * 1e-15            # factors.py:51 (TAU_UNIT 1 fs)
+
+
+class _PairCalculator(OpticalConductivity):
+    """Common pair walk of the real-valued dynamic calculators (dynamic.py:70-131):
+    per k, pairs (m, n) of band groups, matrix elements `self.I[k, m, n, ...]` (27 reals)
+    summed over the two groups, times (f(E_n) - f(E_m)) and factor_omega(E_m, E_n)."""
+    factor = 1.
+    smear_kind = None
+
+    def smear(self, x):
+        return gaussian(x, self.eta) if self.smr_type != "Lorentzian" else lorentzian(x, self.eta)
+
+    def matrix(self, data):
+        raise NotImplementedError
+
+    def __call__(self, data):
+        I = self.matrix(data)
+        restot = np.zeros((len(self.omega), len(self.Efermi) * 27), dtype=I.dtype)
+        for ik in range(data.nk):
+            groups = data.get_bands_in_range_groups_ik(ik, -np.inf, np.inf, degen_thresh=self.degen_thresh,
+                                                       degen_Kramers=self.degen_Kramers)
+            pairs = [(m, n, Em, En) for m, Em in groups.items() for n, En in groups.items() if self.nonzero(Em, En)]
+            if not pairs:
+                continue
+            M = np.array([I[ik, p[0][0]:p[0][1]].sum(axis=0)[p[1][0]:p[1][1]].sum(axis=0) for p in pairs])
+            fE = np.array([fermi_dirac(p[3], self.Efermi, self.kBT) - fermi_dirac(p[2], self.Efermi, self.kBT)
+                           for p in pairs])
+            fw = np.array([self.factor_omega(p[2], p[3]) for p in pairs]).T
+            restot += fw @ (fE[:, :, None] * M.reshape(len(pairs), -1)[:, None, :]).reshape(len(pairs), -1)
+        restot = restot.reshape((len(self.omega), len(self.Efermi), 3, 3, 3)).swapaxes(0, 1)
+        return OracleResult(self.Efermi, restot * self.factor / (data.nk * data.system.cell_volume))
+
+
+class ShiftCurrent(_PairCalculator):
+    """Shift current (dynamic.py:244-324): I^{abc}_nm = -Im(A'^{ca}_nm A^b_mn) + (b <-> c), with the
+    generalised derivative A' of the Berry connection regularised by sc_eta; factor_omega =
+    smear(E_m - E_n - w) + smear(E_n - E_m - w)."""
+    factor = FACTOR_SHIFT_CURRENT
+
+    def __init__(self, Efermi, omega, sc_eta, **kw):
+        super().__init__(Efermi, omega, **kw)
+        self.sc_eta = sc_eta
+
+    def factor_omega(self, E1, E2):
+        return self.smear(E1 - E2 - self.omega) + self.smear(E2 - E1 - self.omega)
+
+    def matrix(self, data):
+        ext = self.kwargs_formula.get("external_terms", True) and not data.system.force_internal_terms_only
+        D, V, W = data.D_H, data.Xbar("Ham", 1), data.Xbar("Ham", 2)
+        dEinv = data.dEig_inv.swapaxes(2, 1)
+        E = data.E_K
+        dE = E[:, :, None] - E[:, None, :]
+        Dp = -V * (dE / (dE ** 2 + self.sc_eta ** 2))[:, :, :, None]
+        es = np.einsum
+        sum_HD = (es('knlc,klma->knmca', V, Dp) - es('knnc,knma->knmca', V, Dp)
+                  - es('knla,klmc->knmca', Dp, V) + es('knma,kmmc->knmca', Dp, V))
+        DV = (es('knmc,knna->knmca', D, V) - es('knmc,kmma->knmca', D, V)
+              + es('knma,knnc->knmca', D, V) - es('knma,kmmc->knmca', D, V))
+        Ag = 1j * (W + sum_HD + DV) * dEinv[:, :, :, None, None]
+        A_H = 1j * D
+        if ext:
+            A, dA = data.Xbar("AA"), data.Xbar("AA", 1)
+            sum_AD = (es('knlc,klma->knmca', A, Dp) - es('knnc,knma->knmca', A, Dp)
+                      - es('knla,klmc->knmca', Dp, A) + es('knma,kmmc->knmca', Dp, A))
+            AD = (es('knnc,knma->knmac', A, D) - es('kmmc,knma->knmac', A, D)
+                  + es('knna,knmc->knmac', A, D) - es('kmma,knmc->knmac', A, D))
+            AA = es('knnb,knma->knmab', A, A) - es('kmmb,knma->knmab', A, A)
+            Ag = Ag + dA + AD - 1j * AA + sum_AD
+            A_H = A_H + A
+        I = -es('knmca,kmnb->knmabc', Ag, A_H).imag
+        return I + I.swapaxes(4, 5)
+
+
+class InjectionCurrent(_PairCalculator):
+    """Injection current (dynamic.py:327-361): I^{abc}_mn = (v^a_m - v^a_n) A^b_mn A^c_nm (real part
+    of the sum), factor_omega = smear(E_m - E_n - w)."""
+    factor = FACTOR_INJECTION_CURRENT
+
+    def factor_omega(self, E1, E2):
+        return self.smear(E1 - E2 - self.omega)
+
+    def matrix(self, data):
+        ext = self.kwargs_formula.get("external_terms", True) and not data.system.force_internal_terms_only
+        A = 1j * data.D_H + (data.Xbar("AA") if ext else 0)
+        v = np.einsum("klla->kla", data.Xbar("Ham", 1)).real
+        dv = v[:, :, None, :] - v[:, None, :, :]
+        return np.einsum('kmna,kmnb,knmc->kmnabc', dv, A, A)
+
+
 class JDOS:
     """Joint density of states (dynamic.py:26-131 with JDOS 145-167): per k, every pair
     (m, n) of band groups with E_m < max(Efermi), E_n > min(Efermi) and

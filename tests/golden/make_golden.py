@@ -57,6 +57,7 @@ def main():
            extra={"force_internal_terms_only": True})
     system("data/random", "sys_random.npz", ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SR", "SH", "SHR"],
            legacy=True)
+    system("reference/systems/GaAs_W90", "sys_gaas_w90.npz", ["Ham", "AA"], legacy=True)
     system("reference/systems/Fe_W90", "sys_fe_w90.npz", ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SR", "SH", "SHR"],
            legacy=True)
     system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz", ["Ham", "AA", "BB", "CC"], legacy=False)
@@ -87,6 +88,9 @@ def main():
              "random_opt_SHCqiao": I + "random-opt_SHCqiao_iter-0000.npz",
              # the same on Fe (Wannier90), grid 4^3 / 2^3 (tests/test_run.py:271-311)
              "fe_opt_conductivity": I + "Fe_W90-opt_conductivity_iter-0000.npz",
+             # shift and injection current, GaAs (Wannier90), grid 6^3 / 3^3 (tests/test_run.py:584-611)
+             "gaas_shift_current": I + "dynamic_GaAs_W90-shift_current_iter-0000.npz",
+             "gaas_injection_current": I + "dynamic_GaAs_W90-injection_current_iter-0000.npz",
              "fe_opt_SHCryoo": I + "Fe_W90-opt_SHCryoo_iter-0000.npz",
              "fe_opt_SHCqiao": I + "Fe_W90-opt_SHCqiao_iter-0000.npz",
              # tetrahedron method: Chiral model with C3z (tests/test_run.py:227-238,835-845)

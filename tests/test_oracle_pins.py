@@ -317,3 +317,18 @@ def test_fe_optical_and_spin_hall(results_run):
                   "qi": O.SHC(Ef, om, SHC_type="qiao", **kw)}, NK=[4, 4, 4], NKFFT=[2, 2, 2])
     for k, ref in (("oc", "fe_opt_conductivity"), ("ry", "fe_opt_SHCryoo"), ("qi", "fe_opt_SHCqiao")):
         assert rel_err(r[k].data, results_run[ref]) < 1e-8, k
+
+
+def test_gaas_shift_injection_current(results_run):
+    """Shift and injection current (dynamic.py:244-361) on GaAs (Wannier90, nw = 16), grid
+    6^3 / 3^3, Gaussian 0.2 eV, kBT 0.01 eV, Efermi 7..9 eV, omega 1..5 eV
+    (tests/test_run.py:584-611), at the reference test's own absolute tolerances
+    (shift 1e-9, injection 1e-7 against maxima of 5.7e-5 and 1.0e-5): the fixture was made
+    from the Wannier90 files, the stored system reproduces it to about 1e-10 / 4e-8."""
+    s = gsys("gaas_w90")
+    kw = dict(smr_fixed_width=0.2, smr_type="Gaussian", kBT=0.01)
+    Ef, om = np.linspace(7, 9, 11), np.arange(1.0, 5.1, 0.5)
+    r = O.run(s, {"sc": O.ShiftCurrent(Ef, om, sc_eta=0.1, **kw), "ic": O.InjectionCurrent(Ef, om, **kw)},
+              NK=[6, 6, 6], NKFFT=[3, 3, 3])
+    assert np.abs(r["sc"].data - results_run["gaas_shift_current"]).max() < 1e-9
+    assert np.abs(r["ic"].data - results_run["gaas_injection_current"]).max() < 1e-7

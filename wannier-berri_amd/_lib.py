@@ -38,7 +38,7 @@ SYMBOLS = [
     "wbmi_r2k_plan_sizes", "wbmi_r2k_box", "wbmi_r2k_exec", "wbmi_r2k_klist", "wbmi_eigh",
     "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_optical_conductivity", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
     "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos", "wbmi_spin_hall", "wbmi_spin_hall_work_bytes",
-    "wbmi_tabulate", "wbmi_tabulate_diag",
+    "wbmi_tabulate", "wbmi_tabulate_diag", "wbmi_shift_current", "wbmi_injection_current",
 ]
 
 
@@ -109,6 +109,10 @@ _PROTOS = {
     "wbmi_spin_hall_work_bytes": (ctypes.c_size_t, [_i, _i]),
     "wbmi_tabulate": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp]),
     "wbmi_tabulate_diag": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _d, _i, _vp]),
+    "wbmi_shift_current": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(OptcondParams), _d, _vp,
+                                ctypes.c_size_t, _vp]),
+    "wbmi_injection_current": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(OptcondParams), _vp,
+                                    ctypes.c_size_t, _vp]),
     "wbmi_spin_hall": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t, _vp]),
 }
 
@@ -302,6 +306,22 @@ class Context:
         self.check(self._timed("tab_diag", (nk, nw), lib().wbmi_tabulate_diag, self.h, nk, nw, X.shape[1], _ptr(E),
                                _ptr(X), prec, c_p, float(degen_thresh), int(bool(degen_Kramers)), _ptr(out)),
                    "wbmi_tabulate_diag")
+
+    def shift_current(self, E, X, prec, params, sc_eta, out):
+        nk, nw = E.shape
+        wbytes = int(lib().wbmi_spin_hall_work_bytes(nk, nw))
+        work = self._workspace(wbytes)
+        self.check(self._timed("shift", (nk, nw), lib().wbmi_shift_current, self.h, nk, nw, X.shape[1], _ptr(E),
+                               _ptr(X), prec, ctypes.byref(params), float(sc_eta), _ptr(work), wbytes, _ptr(out)),
+                   "wbmi_shift_current")
+
+    def injection_current(self, E, X, prec, params, out):
+        nk, nw = E.shape
+        wbytes = int(lib().wbmi_spin_hall_work_bytes(nk, nw))
+        work = self._workspace(wbytes)
+        self.check(self._timed("injection", (nk, nw), lib().wbmi_injection_current, self.h, nk, nw, X.shape[1],
+                               _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)),
+                   "wbmi_injection_current")
 
     def finalize(self, hist, fder, dEF, scale, out):
         nEF, ncomp = hist.shape

@@ -177,4 +177,85 @@ class SHC(Calculator):
                             E_titles=("Efermi", "Omega"), comment=self.comment, save_mode=self.save_mode)
 
 
-__all__ = ["OpticalConductivity", "JDOS", "SHC"]
+class _PairCurrent(Calculator):
+    """Shift and injection current: the pair walk of wbmi_shift_current / wbmi_injection_current."""
+    external_ok = True
+
+    def __init__(self, Efermi=None, omega=None, kBT=0, smr_fixed_width=0.1, smr_type="Lorentzian",
+                 kwargs_formula=None, **kwargs):
+        super().__init__(**kwargs)
+        if smr_type not in ("Lorentzian", "Gaussian"):
+            raise ValueError(f"Invalid smearing type {smr_type}")
+        self.kwargs_formula = dict(kwargs_formula or {})
+        for k in self.kwargs_formula:
+            if k != "external_terms":
+                raise NotImplementedError(f"kwargs_formula option '{k}' is not available on the MI355X path")
+        self.Efermi = np.array(Efermi, dtype=float).reshape(-1)
+        self.omega = np.array(omega, dtype=float).reshape(-1)
+        self.kBT = float(kBT)
+        self.smr_fixed_width = float(smr_fixed_width)
+        self.smr_type = smr_type
+        self._dev_arrays = {}
+
+    def _params(self, data_K):
+        p = _params(self, data_K)
+        p.external_terms = int(_external(self.kwargs_formula, data_K.system))
+        for i, v in enumerate(data_K.role_table()):
+            p.ch[i] = v
+        return p
+
+    def _out(self, data_K):
+        return data_K.torch.zeros((len(self.Efermi), len(self.omega), 3, 3, 3, 2), dtype=data_K.torch.float64,
+                                  device=data_K.ctx.device)
+
+
+class ShiftCurrent(_PairCurrent):
+    r"""Shift current (dynamic.py:244-324), real [nEF, nomega, 3, 3, 3]"""
+
+    def __init__(self, sc_eta, **kwargs):
+        super().__init__(**kwargs)
+        self.sc_eta = float(sc_eta)
+        self.constant_factor = factors.factor_shift_current
+
+    def needed_channels(self, system):
+        need = {("Ham", 0), ("Ham", 1), ("Ham", 2)}
+        if _external(self.kwargs_formula, system):
+            need |= {("AA", 0), ("AA", 1)}
+        return need
+
+    def __call__(self, data_K):
+        data_K.ensure(self.needed_channels(data_K.system))
+        p = self._params(data_K)
+        out = self._out(data_K)
+        data_K.ctx.shift_current(data_K.E_dev, data_K.Xk, data_K.prec, p, self.sc_eta, out)
+        data = out[..., 0] * (self.constant_factor / (data_K.nk * data_K.cell_volume))
+        return EnergyResult([self.Efermi, self.omega], data, transformTR=1, transformInv=-1,
+                            E_titles=("Efermi", "Omega"), comment=self.comment, save_mode=self.save_mode)
+
+
+class InjectionCurrent(_PairCurrent):
+    r"""Injection current (dynamic.py:327-361), complex [nEF, nomega, 3, 3, 3]"""
+
+    def __init__(self, **kwargs):
+        super().__init__(**kwargs)
+        self.constant_factor = factors.factor_injection_current
+
+    def needed_channels(self, system):
+        need = {("Ham", 0), ("Ham", 1)}
+        if _external(self.kwargs_formula, system):
+            need.add(("AA", 0))
+        return need
+
+    def __call__(self, data_K):
+        torch = data_K.torch
+        data_K.ensure(self.needed_channels(data_K.system))
+        p = self._params(data_K)
+        out = self._out(data_K)
+        data_K.ctx.injection_current(data_K.E_dev, data_K.Xk, data_K.prec, p, out)
+        data = torch.view_as_complex(out) * (self.constant_factor / (data_K.nk * data_K.cell_volume))
+        # transformTR of the reference transposes (b, c): not a factor-only parity
+        return EnergyResult([self.Efermi, self.omega], data, transformTR=None, transformInv=-1,
+                            E_titles=("Efermi", "Omega"), comment=self.comment, save_mode=self.save_mode)
+
+
+__all__ = ["OpticalConductivity", "JDOS", "SHC", "ShiftCurrent", "InjectionCurrent"]

@@ -327,33 +327,134 @@ __global__ void __launch_bounds__(256) k_spin_current(int nk, int nw, int nch, w
     }
 }
 
-// Per k: B^b = -i A^b = -m V^b - i Abar^b (A = i D_H + Abar, m = 1/(E_i - E_j), 0 below
-// 1e-7) in LDS; for every pair (m, n) of band groups that can change occupation,
-//   M^{abs} = sum_{i in m, j in n} Im(J^{as}_ij B^b_ji)
-// into an LDS slot (256 pairs at a time), then every thread adds all slots into its own
-// (EF, w) output:  out[EF][w][a][b][s] += (f(E_n) - f(E_m)) M^{abs} / (2 (E_m - E_n - w - i eta))
-// (Gaussian smearing: imaginary part pi G / 2).  One FP64 atomic add per output per workgroup.
+// Shift current: generalised derivative of the Berry connection (dynamic.py:246-288),
+// Ag[n][m][p][q] for one (p, q) per workgroup, one thread per (n, m):
+//   Ag = i (W^{pq} + HD^{pq} + DV^{pq}) / (E_m - E_n)   (0 below 1e-7)
+//        [+ Abar'^{pq} + AD^{pq} - i AA^{pq} + AD'^{pq}  with external terms]
+//   HD^{ca}_nm = sum_l V^c_nl P^a_lm - V^c_nn P^a_nm - sum_l P^a_nl V^c_lm + P^a_nm V^c_mm,
+//   DV^{ca}_nm = D^c_nm (V^a_nn - V^a_mm) + D^a_nm (V^c_nn - V^c_mm),
+//   AD^{ac}_nm = D^a_nm (A^c_nn - A^c_mm) + D^c_nm (A^a_nn - A^a_mm),
+//   AA^{ab}_nm = A^a_nm (A^b_nn - A^b_mm),  AD'^{ca} = HD^{ca} with A in place of V,
+//   P^a_nm = -V^a_nm (E_n - E_m) / ((E_n - E_m)^2 + sc_eta^2),  D = D_H.
+// Out: complex128 [nk, 9, nw, nw] (component p*3 + q).
 template <typename T>
-__global__ void __launch_bounds__(OC_THREADS) k_shc(int nk, wbmi_shc_params sp, int nw, int nch,
-                                                    const double* __restrict__ Eall, const cplx<T>* __restrict__ Xall,
-                                                    const cplx<double>* __restrict__ J, double* __restrict__ out) {
+__global__ void __launch_bounds__(256) k_shift_ag(int nk, int nw, int nch, wbmi_optcond_params p, double sc_eta,
+                                                  const double* __restrict__ Eall, const cplx<T>* __restrict__ Xall,
+                                                  cplx<double>* __restrict__ Ag) {
+    const int k = blockIdx.x, comp = blockIdx.y, cp = comp / 3, cq = comp % 3;
+    const int64_t nw2 = (int64_t)nw * nw;
+    const cplx<T>* Xk = Xall + (int64_t)k * nch * nw2;
+    const double* E = Eall + (int64_t)k * nw;
+    const bool ext = p.external_terms != 0;
+    auto ld = [&](int role, int i, int j) {
+        const cplx<T> x = Xk[(int64_t)p.ch[role] * nw2 + (int64_t)i * nw + j];
+        return mk((double)x.re, (double)x.im);
+    };
+    auto dinv = [&](int i, int j) {  // 1 / (E_i - E_j), 0 below 1e-7
+        const double de = E[i] - E[j];
+        return fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+    };
+    auto Dh = [&](int a, int i, int j) { return -dinv(i, j) * ld(WBMI_ROLE_V + a, i, j); };
+    auto Pv = [&](int a, int i, int j) {
+        const double de = E[i] - E[j];
+        return -(de / (de * de + sc_eta * sc_eta)) * ld(WBMI_ROLE_V + a, i, j);
+    };
+    // HD-type term with X (V or A) in the c slot and P in the a slot, positions (c, a) = (cp, cq)
+    auto xp = [&](int role0, int n, int m) {
+        const int c = cp, a = cq;
+        cplx<double> s = mk(0.0, 0.0);
+        for (int l = 0; l < nw; ++l) s += ld(role0 + c, n, l) * Pv(a, l, m) - Pv(a, n, l) * ld(role0 + c, l, m);
+        s -= ld(role0 + c, n, n) * Pv(a, n, m);
+        s += Pv(a, n, m) * ld(role0 + c, m, m);
+        return s;
+    };
+    for (int t = threadIdx.x; t < nw2; t += blockDim.x) {
+        const int n = t / nw, m = t % nw;
+        const int c = cp, a = cq;
+        cplx<double> g = ld(WBMI_ROLE_W + c * 3 + a, n, m) + xp(WBMI_ROLE_V, n, m);
+        g += Dh(c, n, m) * (ld(WBMI_ROLE_V + a, n, n) - ld(WBMI_ROLE_V + a, m, m));
+        g += Dh(a, n, m) * (ld(WBMI_ROLE_V + c, n, n) - ld(WBMI_ROLE_V + c, m, m));
+        const double di = dinv(m, n);
+        cplx<double> v = mk(-g.im * di, g.re * di);  // i g / (E_m - E_n)
+        if (ext) {
+            // positions (p, q) = (cp, cq): Abar'[.., a = p, d = q], AD[.., a = p, c = q],
+            // AA[.., a = p, b = q], AD' (HD with A) at (c = p, a = q)
+            const int pp = cp, qq = cq;
+            v += ld(WBMI_ROLE_AD + pp * 3 + qq, n, m);
+            v += Dh(pp, n, m) * (ld(WBMI_ROLE_A + qq, n, n) - ld(WBMI_ROLE_A + qq, m, m));
+            v += Dh(qq, n, m) * (ld(WBMI_ROLE_A + pp, n, n) - ld(WBMI_ROLE_A + pp, m, m));
+            const cplx<double> aa = ld(WBMI_ROLE_A + pp, n, m) * (ld(WBMI_ROLE_A + qq, n, n) - ld(WBMI_ROLE_A + qq, m, m));
+            v += mk(aa.im, -aa.re);  // -i AA
+            v += xp(WBMI_ROLE_A, n, m);
+        }
+        Ag[((int64_t)k * 9 + comp) * nw2 + t] = v;
+    }
+}
+
+// Injection current (dynamic.py:327-361): X^{ab}_nm = (v^a_n - v^a_m) A^b_nm, v^a_n = Re V^a_nn,
+// A = i D_H (+ Abar).  Out: complex128 [nk, 9, nw, nw] (component a*3 + b).
+template <typename T>
+__global__ void __launch_bounds__(256) k_inject_x(int nk, int nw, int nch, wbmi_optcond_params p,
+                                                  const double* __restrict__ Eall, const cplx<T>* __restrict__ Xall,
+                                                  cplx<double>* __restrict__ X) {
+    const int k = blockIdx.x, comp = blockIdx.y, a = comp / 3, b = comp % 3;
+    const int64_t nw2 = (int64_t)nw * nw;
+    const cplx<T>* Xk = Xall + (int64_t)k * nch * nw2;
+    const double* E = Eall + (int64_t)k * nw;
+    const cplx<T>* Va = Xk + (int64_t)p.ch[WBMI_ROLE_V + a] * nw2;
+    const cplx<T>* Vb = Xk + (int64_t)p.ch[WBMI_ROLE_V + b] * nw2;
+    const cplx<T>* Ab = Xk + (int64_t)p.ch[WBMI_ROLE_A + b] * nw2;
+    for (int t = threadIdx.x; t < nw2; t += blockDim.x) {
+        const int n = t / nw, m = t % nw;
+        const double de = E[n] - E[m];
+        const double mi = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+        // A = i (-mi V) (+ Abar)
+        cplx<double> A = mk(mi * (double)Vb[t].im, -mi * (double)Vb[t].re);
+        if (p.external_terms) A += mk((double)Ab[t].re, (double)Ab[t].im);
+        const double dv = (double)Va[(int64_t)n * nw + n].re - (double)Va[(int64_t)m * nw + m].re;
+        X[((int64_t)k * 9 + comp) * nw2 + t] = dv * A;
+    }
+}
+
+#define P27_PAIRS 128  // pair slots per round (27 complex sums each)
+
+// Pair walk shared by the spin Hall conductivity (KIND 0), the shift current (1) and the
+// injection current (2).  Per k: Y^b in LDS (KIND 0: -i A, else A; A = i D_H (+ Abar));
+// for every pair (m, n) of band groups that can change occupation (128 at a time),
+//   P^{x,b} = sum_{i in m, j in n} X^x_ij Y^b_ji   (x = 0..8, from the per-k workspace)
+// into an LDS slot; then every thread adds all slots into its (EF, w) output:
+//   KIND 0: out[abs] += (f_n - f_m) Im P^{as,b} / (2 (E_m - E_n - w - i eta))   (Gaussian: pi G / 2)
+//   KIND 1: out[abc] += (f_n - f_m) (-Im P^{ca,b} - Im P^{ba,c}) (g(E_m - E_n - w) + g(E_n - E_m - w))
+//   KIND 2: out[abc] += (f_n - f_m) P^{ab,c} g(E_m - E_n - w)
+// (g = the smearing function).  One FP64 atomic add per output per workgroup.
+template <typename T, int KIND>
+__global__ void __launch_bounds__(OC_THREADS) k_pair27(int nk, wbmi_optcond_params p, int nw, int nch,
+                                                       const double* __restrict__ Eall,
+                                                       const cplx<T>* __restrict__ Xall,
+                                                       const cplx<double>* __restrict__ W, double* __restrict__ out) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
-    const wbmi_optcond_params& p = sp.base;
     const int nw2 = nw * nw, tid = threadIdx.x;
-    cplx<double>* sB = (cplx<double>*)smem;  // [3][nw][nw]
-    double* sM = (double*)(sB + 3 * nw2);    // [OC_THREADS][27]
-    double* sEm = sM + OC_THREADS * 27;
-    double* sEn = sEm + OC_THREADS;
-    double* sE = sEn + OC_THREADS;
+    cplx<double>* sY = (cplx<double>*)smem;           // [3][nw][nw]
+    cplx<double>* sP = sY + 3 * nw2;                 // [P27_PAIRS][27]
+    double* sEm = (double*)(sP + P27_PAIRS * 27);
+    double* sEn = sEm + P27_PAIRS;
+    double* sE = sEn + P27_PAIRS;
     double* sGm = sE + OC_NWMAX;
     int* sOk = (int*)(sGm + OC_NWMAX);
-    int* sGb = sOk + OC_THREADS;
+    int* sGb = sOk + P27_PAIRS;
     int& sNg = sGb[OC_NWMAX + 1];
     const int o = blockIdx.x * OC_THREADS + tid;
     const bool has_out = o < p.nEF * p.nomega;
     const double mu = has_out ? p.Efermi[o / p.nomega] : 0.0;
     const double w = has_out ? p.omega[o % p.nomega] : 0.0;
     const double occ_lo = p.EFmin - 30.0 * p.kBT, occ_hi = p.EFmax + 30.0 * p.kBT;
+    auto smear = [&](double x) {
+        if (p.smr_gaussian) {
+            const double y = x / p.eta;
+            return fabs(x) < p.eta * sqrt(200.0) ? exp(-y * y) / (sqrt(M_PI) * p.eta) : 0.0;
+        }
+        return 1.0 / (M_PI * p.eta) * (p.eta * p.eta) / (x * x + p.eta * p.eta);
+    };
     cplx<double> acc[27];
 #pragma unroll
     for (int c = 0; c < 27; ++c) acc[c] = mk(0.0, 0.0);
@@ -368,65 +469,84 @@ __global__ void __launch_bounds__(OC_THREADS) k_shc(int nk, wbmi_shc_params sp, 
             const double de = sE[i] - sE[j];
             const double m = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
             const cplx<T> v = Xk[(int64_t)p.ch[WBMI_ROLE_V + c] * nw2 + i * nw + j];
-            const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + i * nw + j];
-            // -m V - i Abar
-            sB[t] = mk(-m * (double)v.re + (double)x.im, -m * (double)v.im - (double)x.re);
+            cplx<double> a = mk(m * (double)v.im, -m * (double)v.re);  // i D_H = i (-m V)
+            if (p.external_terms) {
+                const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + i * nw + j];
+                a += mk((double)x.re, (double)x.im);
+            }
+            sY[t] = KIND == 0 ? mk(a.im, -a.re) : a;  // KIND 0: -i A
         }
         __syncthreads();
-        const cplx<double>* Jk = J + (int64_t)k * 9 * nw2;
+        const cplx<double>* Wk = W + (int64_t)k * 9 * nw2;
         const int ng = sNg, ncand = ng * ng;
-        for (int c0 = 0; c0 < ncand; c0 += OC_THREADS) {
+        for (int c0 = 0; c0 < ncand; c0 += P27_PAIRS) {
             const int cnd = c0 + tid;
-            int ok = 0;
-            if (cnd < ncand) {
-                const int gm = cnd / ng, gn = cnd % ng;
-                const double Em = sGm[gm], En = sGm[gn];
-                ok = !((Em < occ_lo && En < occ_lo) || (Em > occ_hi && En > occ_hi));
-                if (ok) {
-                    sEm[tid] = Em;
-                    sEn[tid] = En;
-                    for (int a = 0; a < 3; ++a)
-                        for (int s = 0; s < 3; ++s) {
-                            const cplx<double>* Ja = Jk + (a * 3 + s) * nw2;
-                            double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+            if (tid < P27_PAIRS) {
+                int ok = 0;
+                if (cnd < ncand) {
+                    const int gm = cnd / ng, gn = cnd % ng;
+                    const double Em = sGm[gm], En = sGm[gn];
+                    ok = !((Em < occ_lo && En < occ_lo) || (Em > occ_hi && En > occ_hi));
+                    if (ok) {
+                        sEm[tid] = Em;
+                        sEn[tid] = En;
+                        for (int x = 0; x < 9; ++x) {
+                            const cplx<double>* Wx = Wk + x * nw2;
+                            cplx<double> q0 = mk(0.0, 0.0), q1 = mk(0.0, 0.0), q2 = mk(0.0, 0.0);
                             for (int i = sGb[gm]; i < sGb[gm + 1]; ++i)
                                 for (int j = sGb[gn]; j < sGb[gn + 1]; ++j) {
-                                    const cplx<double> jv = Ja[i * nw + j];
-                                    const cplx<double> b0 = sB[j * nw + i], b1 = sB[nw2 + j * nw + i],
-                                                       b2 = sB[2 * nw2 + j * nw + i];
-                                    m0 += jv.re * b0.im + jv.im * b0.re;
-                                    m1 += jv.re * b1.im + jv.im * b1.re;
-                                    m2 += jv.re * b2.im + jv.im * b2.re;
+                                    const cplx<double> xv = Wx[i * nw + j];
+                                    q0 += xv * sY[j * nw + i];
+                                    q1 += xv * sY[nw2 + j * nw + i];
+                                    q2 += xv * sY[2 * nw2 + j * nw + i];
                                 }
-                            sM[tid * 27 + (a * 3 + 0) * 3 + s] = m0;
-                            sM[tid * 27 + (a * 3 + 1) * 3 + s] = m1;
-                            sM[tid * 27 + (a * 3 + 2) * 3 + s] = m2;
+                            sP[tid * 27 + x * 3 + 0] = q0;
+                            sP[tid * 27 + x * 3 + 1] = q1;
+                            sP[tid * 27 + x * 3 + 2] = q2;
                         }
+                    }
                 }
+                sOk[tid] = ok;
             }
-            sOk[tid] = ok;
             __syncthreads();
-            const int ns = ncand - c0 < OC_THREADS ? ncand - c0 : OC_THREADS;
+            const int ns = ncand - c0 < P27_PAIRS ? ncand - c0 : P27_PAIRS;
             if (has_out) {
                 for (int s = 0; s < ns; ++s) {
                     if (!sOk[s]) continue;
                     const double Em = sEm[s], En = sEn[s];
                     const double fE = fermi_dirac(En, mu, p.kBT) - fermi_dirac(Em, mu, p.kBT);
                     if (fE == 0.0) continue;
-                    // 1 / (2 (x - i eta)), x = E_m - E_n - w
-                    const double x = Em - En - w;
-                    const double den = 1.0 / (x * x + p.eta * p.eta);
-                    double fr = x * den, fi = p.eta * den;
-                    if (p.smr_gaussian) {
-                        const double y = x / p.eta;
-                        fi = fabs(x) < p.eta * sqrt(200.0) ? M_PI / (sqrt(M_PI) * p.eta) * exp(-y * y) : 0.0;
-                    }
-                    const cplx<double> f = mk(0.5 * fE * fr, 0.5 * fE * fi);
+                    const cplx<double>* P = sP + s * 27;
+                    if (KIND == 0) {
+                        const double x = Em - En - w;
+                        const double den = 1.0 / (x * x + p.eta * p.eta);
+                        const double fr = x * den;
+                        const double fi = p.smr_gaussian ? M_PI * smear(x) : p.eta * den;
+                        const cplx<double> f = mk(0.5 * fE * fr, 0.5 * fE * fi);
 #pragma unroll
-                    for (int c = 0; c < 27; ++c) {
-                        const double mv = sM[s * 27 + c];
-                        acc[c].re += f.re * mv;
-                        acc[c].im += f.im * mv;
+                        for (int a = 0; a < 3; ++a)
+#pragma unroll
+                            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                                for (int sp = 0; sp < 3; ++sp) {
+                                    const double mv = P[(a * 3 + sp) * 3 + b].im;
+                                    acc[(a * 3 + b) * 3 + sp] += mv * f;
+                                }
+                    } else if (KIND == 1) {
+                        const double f = fE * (smear(Em - En - w) + smear(En - Em - w));
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+#pragma unroll
+                            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) {
+                                    const double mv = -P[(c * 3 + a) * 3 + b].im - P[(b * 3 + a) * 3 + c].im;
+                                    acc[(a * 3 + b) * 3 + c].re += f * mv;
+                                }
+                    } else {
+                        const double f = fE * smear(Em - En - w);
+#pragma unroll
+                        for (int q = 0; q < 27; ++q) acc[q] += f * P[q];
                     }
                 }
             }
@@ -442,6 +562,90 @@ __global__ void __launch_bounds__(OC_THREADS) k_shc(int nk, wbmi_shc_params sp, 
     }
 }
 
+template <typename T, int KIND>
+static int launch_pair27(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E, const void* X,
+                         const wbmi_optcond_params* params, const cplx<double>* Wk, double* out) {
+    const int64_t nout = (int64_t)params->nEF * params->nomega;
+    dim3 g((unsigned)ceil_div(nout, OC_THREADS), (unsigned)ceil_div(nk, OC_KCHUNK));
+    const size_t lds = sizeof(cplx<double>) * (3 * (size_t)nw * nw + P27_PAIRS * 27) +
+                       sizeof(double) * (2 * P27_PAIRS + 2 * OC_NWMAX) + sizeof(int) * (P27_PAIRS + OC_NWMAX + 2);
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_pair27<T, KIND>,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL((k_pair27<T, KIND>), g, dim3(OC_THREADS), lds, ctx->stream, nk, *params, nw, nch, E,
+                       (const cplx<T>*)X, Wk, out);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+// arguments shared by the three pair-walk calculators
+static int check_pair27(wbmi_ctx* ctx, const char* what, int nk, int nw, int nch, const double* E, const void* X,
+                        const wbmi_optcond_params* p, void* work, size_t work_bytes, double* out, bool need_w) {
+    if (nk <= 0 || nw <= 0 || nch <= 0 || !E || !X || !p || !out || !p->Efermi || !p->omega || p->nEF <= 0 ||
+        p->nomega <= 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "%s: bad arguments", what);
+    if (nw > OC_NWMAX) return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "%s: nw <= %d", what, OC_NWMAX);
+    if (!(p->eta > 0) || p->kBT < 0) return wbmi_fail(ctx, WBMI_ERR_ARG, "%s: smearing width must be positive", what);
+    if (!work || work_bytes < sizeof(cplx<double>) * 9 * (size_t)nk * nw * nw)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "%s: workspace of %zu bytes needed", what,
+                         sizeof(cplx<double>) * 9 * (size_t)nk * nw * nw);
+    auto bad = [&](int c) { return c < 0 || c >= nch; };
+    for (int c = 0; c < 3; ++c)
+        if (bad(p->ch[WBMI_ROLE_V + c]) || (p->external_terms && bad(p->ch[WBMI_ROLE_A + c])))
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "%s: missing V or A channel", what);
+    if (need_w)
+        for (int c = 0; c < 9; ++c)
+            if (bad(p->ch[WBMI_ROLE_W + c]) || (p->external_terms && bad(p->ch[WBMI_ROLE_AD + c])))
+                return wbmi_fail(ctx, WBMI_ERR_ARG, "%s: missing Ham'' or AA' channel", what);
+    return WBMI_OK;
+}
+
+extern "C" int wbmi_shift_current(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                                  int prec, const wbmi_optcond_params* params, double sc_eta, void* work_dev,
+                                  size_t work_bytes, double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    int rc = check_pair27(ctx, "wbmi_shift_current", nk, nw, nch, E_dev, Xbar_dev, params, work_dev, work_bytes,
+                          out_dev, true);
+    if (rc) return rc;
+    if (!(sc_eta > 0)) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_shift_current: sc_eta must be positive");
+    cplx<double>* Wk = (cplx<double>*)work_dev;
+    if (prec == WBMI_F64) {
+        hipLaunchKernelGGL(k_shift_ag<double>, dim3(nk, 9), dim3(256), 0, ctx->stream, nk, nw, nch, *params, sc_eta,
+                           E_dev, (const cplx<double>*)Xbar_dev, Wk);
+        WBMI_LAUNCH_CHECK(ctx);
+        return launch_pair27<double, 1>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, Wk, out_dev);
+    }
+    if (prec == WBMI_F32) {
+        hipLaunchKernelGGL(k_shift_ag<float>, dim3(nk, 9), dim3(256), 0, ctx->stream, nk, nw, nch, *params, sc_eta,
+                           E_dev, (const cplx<float>*)Xbar_dev, Wk);
+        WBMI_LAUNCH_CHECK(ctx);
+        return launch_pair27<float, 1>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, Wk, out_dev);
+    }
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_shift_current: bad precision");
+}
+
+extern "C" int wbmi_injection_current(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                                      const void* Xbar_dev, int prec, const wbmi_optcond_params* params,
+                                      void* work_dev, size_t work_bytes, double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    int rc = check_pair27(ctx, "wbmi_injection_current", nk, nw, nch, E_dev, Xbar_dev, params, work_dev, work_bytes,
+                          out_dev, false);
+    if (rc) return rc;
+    cplx<double>* Wk = (cplx<double>*)work_dev;
+    if (prec == WBMI_F64) {
+        hipLaunchKernelGGL(k_inject_x<double>, dim3(nk, 9), dim3(256), 0, ctx->stream, nk, nw, nch, *params, E_dev,
+                           (const cplx<double>*)Xbar_dev, Wk);
+        WBMI_LAUNCH_CHECK(ctx);
+        return launch_pair27<double, 2>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, Wk, out_dev);
+    }
+    if (prec == WBMI_F32) {
+        hipLaunchKernelGGL(k_inject_x<float>, dim3(nk, 9), dim3(256), 0, ctx->stream, nk, nw, nch, *params, E_dev,
+                           (const cplx<float>*)Xbar_dev, Wk);
+        WBMI_LAUNCH_CHECK(ctx);
+        return launch_pair27<float, 2>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, Wk, out_dev);
+    }
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_injection_current: bad precision");
+}
+
 extern "C" size_t wbmi_spin_hall_work_bytes(int nk, int nw) {
     return sizeof(cplx<double>) * 9 * (size_t)nk * nw * nw;
 }
@@ -452,16 +656,7 @@ static int launch_shc(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E, c
     hipLaunchKernelGGL(k_spin_current<T>, dim3(nk, 9), dim3(256), 0, ctx->stream, nk, nw, nch, *params, E,
                        (const cplx<T>*)X, J);
     WBMI_LAUNCH_CHECK(ctx);
-    const int64_t nout = (int64_t)params->base.nEF * params->base.nomega;
-    dim3 g((unsigned)ceil_div(nout, OC_THREADS), (unsigned)ceil_div(nk, OC_KCHUNK));
-    const size_t lds = sizeof(cplx<double>) * 3 * (size_t)nw * nw + sizeof(double) * OC_THREADS * 27 +
-                       sizeof(double) * (2 * OC_THREADS + 2 * OC_NWMAX) + sizeof(int) * (OC_THREADS + OC_NWMAX + 2);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_shc<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                            (int)lds));
-    hipLaunchKernelGGL(k_shc<T>, g, dim3(OC_THREADS), lds, ctx->stream, nk, *params, nw, nch, E, (const cplx<T>*)X,
-                       (const cplx<double>*)J, out);
-    WBMI_LAUNCH_CHECK(ctx);
-    return WBMI_OK;
+    return launch_pair27<T, 0>(ctx, nk, nw, nch, E, X, &params->base, J, out);
 }
 
 extern "C" int wbmi_spin_hall(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,

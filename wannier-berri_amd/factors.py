@@ -3,6 +3,7 @@
 Same definitions as wannierberri/factors.py:6-28 (SciPy CODATA constants), so
 that results agree with the reference to the last bit of the constants.
 """
+import numpy as np
 from scipy.constants import elementary_charge, hbar, electron_mass, physical_constants, angstrom
 
 TAU_UNIT = 1E-15
@@ -14,4 +15,6 @@ factor_gme_orb = factor_morb_evA2_to_muB * bohr_magneton / angstrom ** 2
 factor_ahc = -(elementary_charge ** 2 / hbar / angstrom)
 factor_opt = -factor_ahc   # factors.py:36
 factor_shc = -factor_ahc   # factors.py:37
+factor_shift_current = hbar / elementary_charge * np.pi * elementary_charge ** 3 / (4 * hbar ** 2)  # factors.py:50
+factor_injection_current = -np.pi * elementary_charge ** 3 / (hbar ** 2) * TAU_UNIT               # factors.py:51
 factor_nlahc = elementary_charge ** 3 / hbar ** 2 * TAU_UNIT
