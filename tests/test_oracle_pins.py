@@ -9,6 +9,11 @@ Every oracle function on the path is checked by at least one of:
   DerOmega, Morb_H, morb, DerMorb, Dermorb, VelOmega, Hamiltonian, Velocity);
 * P6 `calculator-Fe-ident-fder=0..3` (band grouping, Fermi scan, all fder stencils);
 * P7 `evaluate_k` (eigenvalues, per-band Berry curvature, AHC, Morb at one k);
+* P8 dynamic calculators: `Chiral-opt_conductivity`, `Chiral-jdos`, `random-opt_conductivity`,
+  `random-opt_SHC{ryoo,qiao}`, `Fe_W90-opt_{conductivity,SHCryoo,SHCqiao}`,
+  `dynamic_GaAs_W90-{shift,injection}_current` (at that test's absolute tolerances);
+* P9 k-resolved tabulation: `Fe_W90-tabulate` (Energy, V, spin, berry, Der_berry, morb, Der_morb);
+* the tetrahedron method is pinned in tests/test_oracle_tetra.py;
 * closed forms and invariants: brute-force DFT == FFT (r2k_grid_slow), U^+HU = diag(E),
   sum_n Omega_n = 0 over all bands (internal terms), Tr dOmega = 0 (internal).
 
