@@ -715,3 +715,26 @@ def test_fe_tabulate(wb, precision, tol):
         say(f"  {k}: {e:.2e}")
         assert d.shape == ref[k].shape
         assert e < tol * (5e-8 if k == "Der_berry" else 2e-8), k
+
+
+def test_haldane_tabulate_symmetric(wb):
+    """Tabulation with the C3z group (irreducible K-blocks, every result mapped by the group,
+    images put on the grid, tabresult.py:128-170) equals the tabulation of the full grid for
+    the exactly symmetric Haldane model: energies, Berry curvature (axial, TR-odd) and
+    orbital moment, 12 x 12 grid."""
+    say("test_haldane_tabulate_symmetric")
+    tab = wb.calculators.tabulate
+    s = gsys(wb, "haldane")
+    s.set_pointgroup(["C3z"])
+    grid = wb.Grid(system=s, NK=[12, 12, 1], NKFFT=[4, 4, 1])
+    res = {}
+    for sym in (False, True):
+        calc = tab.TabulatorAll({"berry": tab.BerryCurvature(), "morb": tab.OrbitalMoment()})
+        r = wb.run(s, grid, {"tab": calc}, use_irred_kpt=sym, symmetrize=sym, save_files=False, verbose=False)
+        res[sym] = r.results["tab"]
+    for q in ("Energy", "berry", "morb"):
+        a, b = res[True].get_data(q), res[False].get_data(q)
+        assert a.shape == b.shape
+        e = rel_err(a, b)
+        say(f"  {q}: {e:.2e}")
+        assert e < 1e-10, q

@@ -236,6 +236,16 @@ class KBandResult(Result):
 
     __truediv__ = __mul__
 
+    def transform(self, sym):
+        """Image under a point operation (kbandresult.py:150-159): the Cartesian axes rotated,
+        times the TR / inversion parity."""
+        for flag, f, what in ((sym.TR, self.transformTR, "time reversal"), (sym.Inv, self.transformInv, "inversion")):
+            if flag and f is None and self.rank > 0:
+                raise ValueError(f"the parity of a tabulated quantity under {what} is not known")
+        fTR = self.transformTR if self.transformTR is not None else 1
+        fI = self.transformInv if self.transformInv is not None else 1
+        return KBandResult(sym.transform_tensor(self.data, self.rank, fTR, fI), self.transformTR, self.transformInv)
+
     def to_grid(self, k_map):
         """Grid point i gets the mean over the k-points k_map[i] (kbandresult.py)."""
         return KBandResult(np.array([self.data[km].mean(axis=0) for km in k_map]), self.transformTR,
@@ -291,8 +301,12 @@ class TABresult(Result):
         return np.zeros(0)
 
     def transform(self, sym):
-        raise NotImplementedError("tabulation with symmetrisation is not available on the MI355X path "
-                                  "(run with use_irred_kpt=False, symmetrize=False)")
+        """The k-points and every quantity mapped by the operation (tabresult.py:128-132); the
+        symmetrised sum over the group then holds every image, and to_grid averages those
+        that land on the same grid point."""
+        kp = sym.transform_reduced_vector(self.kpoints, self.recip_lattice)
+        return TABresult(kp, self.recip_lattice, {r: v.transform(sym) for r, v in self.results.items()}, self.mode,
+                         self.save_mode)
 
     def savedata(self, name, prefix, suffix, i_iter):
         pass   # as the reference (tabresult.py:56-60)
