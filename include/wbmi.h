@@ -295,7 +295,9 @@ int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev
  *                  (2l-1)) * dK_fullBZ / 2, corner i*4 + j*2 + l
  *                  (data_K/data_K_R.py:126-141)
  *   Efermi_dev   : float64 [nEF], ascending
- *   params       : fder = 0..3; EFmin = Efermi[0], EFmax = Efermi[nEF-1];
+ *   params       : fder = 0..3, or -1 for hole_like (weights 1 - occupation and
+ *                  the group of bands above the range, tetrahedron.py:165-172);
+ *                  EFmin = Efermi[0], EFmax = Efermi[nEF-1];
  *                  nEF_extra and dEF are not read
  *   hist_dev     : float64 [2 * nEF, 3^ndim]: rows [0, nEF) are steps
  *                  (prefix-summed by wbmi_finalize_tetra), rows [nEF, 2 nEF)

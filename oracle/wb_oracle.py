@@ -1076,7 +1076,7 @@ class Morb(StaticCalculator):
     def __init__(self, Efermi, **kw):
         super().__init__(Efermi, **kw)
         kw = dict(kw)
-        kw.setdefault('constant_factor', self.constant_factor)
+        kw.setdefault('constant_factor', self.constant_factor_default)  # the constructor's value, before the hole_like sign
         self.AHC = AHC(Efermi, **kw)
 
     def __call__(self, data_K):
@@ -1094,7 +1094,7 @@ class GME_orb_FermiSea(StaticCalculator):
     def __init__(self, Efermi, **kw):
         super().__init__(Efermi, **kw)
         kw = dict(kw)
-        kw.setdefault('constant_factor', self.constant_factor)
+        kw.setdefault('constant_factor', self.constant_factor_default)  # the constructor's value, before the hole_like sign
         self.BerryDipole = BerryDipole_FermiSea(Efermi, **kw)
 
     def __call__(self, data_K):
@@ -1113,7 +1113,7 @@ class GME_orb_FermiSurf(StaticCalculator):
     def __init__(self, Efermi, **kw):
         super().__init__(Efermi, **kw)
         kw = dict(kw)
-        kw.setdefault('constant_factor', self.constant_factor)
+        kw.setdefault('constant_factor', self.constant_factor_default)  # the constructor's value, before the hole_like sign
         self.BerryDipole = BerryDipole_FermiSurf(Efermi, **kw)
 
     def __call__(self, data_K):

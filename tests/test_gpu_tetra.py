@@ -122,3 +122,29 @@ def test_tetra_random_vs_oracle(wb, precision, tol):
         err = rel_err(r.results[k].data, ref[k].data)
         say(f"  {k}: {err:.2e}")
         assert err < tol, k
+
+
+def test_tetra_hole_like_vs_oracle(wb):
+    """hole_like with the tetrahedron method (weights 1 - occupation and the group of bands
+    above the range, tetrahedron.py:165-172,246-253): AHC (additive), Morb (non-additive) and
+    CumDOS on a random Hermitian model vs the oracle, with a window that leaves bands above."""
+    say("test_tetra_hole_like_vs_oracle")
+    st = wb.calculators.static
+    from wannierberri_amd import models
+    s = models.synthetic(num_wann=8, Rmax=2, seed=11, with_BBCC=True)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA", "BB", "CC")})
+    E0 = np.linalg.eigvalsh(s.get_R_mat("Ham")[np.where((s.rvec.iRvec == 0).all(1))[0][0]])
+    Ef = np.linspace(E0.min() - 0.5, 0.5 * (E0.min() + E0.max()), 17)
+    gpu = {"ahc": st.AHC(Efermi=Ef, tetra=True, hole_like=True),
+           "morb": st.Morb(Efermi=Ef, tetra=True, hole_like=True),
+           "cumdos": st.CumDOS(Efermi=Ef, tetra=True, hole_like=True)}
+    ora = {"ahc": O.AHC(Ef, tetra=True, hole_like=True), "morb": O.Morb(Ef, tetra=True, hole_like=True),
+           "cumdos": O.CumDOS(Ef, tetra=True, hole_like=True)}
+    NK, NKFFT = [4, 4, 4], [2, 2, 2]
+    r = wb.run(s, wb.Grid(system=s, NK=NK, NKFFT=NKFFT), gpu, save_files=False, verbose=False)
+    ref = _oracle_tetra(o, ora, NK, NKFFT)
+    for k in gpu:
+        err = rel_err(r.results[k].data, ref[k].data)
+        say(f"  {k}: {err:.2e}")
+        assert err < 1e-10, k

@@ -6,6 +6,7 @@
 * `tetra_band_weight` (12 tetrahedra around the centre of the parallelepiped) against
   the exact fraction of a cube below a plane, for a linear band, where the linear
   interpolation is exact;
+* hole_like: cumulative DOS of electrons minus that of holes = number of bands;
 * the whole chain (corner energies, group weights, weighted sum, symmetrisation,
   refinement) through the package's `run()` with the oracle evaluating each K-block,
   against the reference's fixtures `Chiral_tetra-*` (C3z, tests/test_run.py:835-845)
@@ -127,3 +128,19 @@ def test_haldane_tetra_refinement(wb, results_run):
                    data_k_class=_OracleBlock, save_files=False, verbose=False)
         for k in calcs:
             assert rel_err(r.results[k].data, results_run[f"haldane_adpt3_tetra_{k}_iter{it}"]) < 1e-8, (k, it)
+
+
+def test_tetra_hole_like_counts(wb):
+    """hole_like (weights 1 - occupation, the group of bands above the range added with
+    weight 1, constant factor negated): the cumulative DOS of holes H and of electrons C
+    satisfy C - H = number of bands at every Fermi level (any level, bands inside or
+    outside the window)."""
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_chiral_left.npz"))
+    _OracleBlock.osys = O.load_golden_system(os.path.join(GOLDEN, "sys_chiral_left.npz"))
+    Ef = np.linspace(-3, 1, 9)     # the upper band stays above the window at most k
+    calcs = {"c": _OracleCalc(wb, O.CumDOS(Ef, tetra=True), 1, 1),
+             "h": _OracleCalc(wb, O.CumDOS(Ef, tetra=True, hole_like=True), 1, 1)}
+    grid = wb.Grid(system=s, NK=[6, 6, 4], NKFFT=[3, 3, 2])
+    r = wb.run(s, grid, calcs, use_irred_kpt=False, symmetrize=False, data_k_class=_OracleBlock,
+               save_files=False, verbose=False)
+    assert np.allclose(r.results["c"].data - r.results["h"].data, s.num_wann, rtol=0, atol=1e-12)

@@ -53,7 +53,7 @@ class StaticCalculator(Calculator):
     """Fermi-level scan integrator (calculators/static.py:24-169).
 
     Subclasses set `formula` (a WBMI_FORMULA_* id), `ndim` and `fder`.
-    Not supported: k_resolved, select_bands, finite Emin/Emax, hole_like with tetra.
+    Not supported: k_resolved, select_bands, finite Emin/Emax.
     """
     formula = None
     ndim = 0
@@ -63,8 +63,6 @@ class StaticCalculator(Calculator):
                  Emin=-np.inf, Emax=np.inf, hole_like=False, k_resolved=False, fder=None, select_bands=None,
                  **kwargs):
         super().__init__(**kwargs)
-        if tetra and hole_like:
-            raise NotImplementedError("hole_like with the tetrahedron method is not available on the MI355X path")
         if k_resolved:
             raise NotImplementedError("k-resolved output is not available on the MI355X path")
         if select_bands is not None:
@@ -141,6 +139,8 @@ class StaticCalculator(Calculator):
         p = self.scan_params(data_K.system)
         p.EFmin, p.EFmax = float(self.Efermi[0]), float(self.Efermi[-1])
         p.nEF_extra = nEF
+        if self.hole_like:
+            p.fder = -1   # 1 - occupation (static.py:81-82)
         hist = torch.zeros((2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
         data_K.static_scan_tetra(p, Ef, hist, 1.0)
         out = torch.empty((nEF, self.ncomp), dtype=torch.float64, device=dev)

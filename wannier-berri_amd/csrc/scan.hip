@@ -81,7 +81,7 @@ extern "C" int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, co
     if (nk <= 0 || nw <= 0 || nw > SC_NWMAX || nch <= 0 || !E_dev || !Ecorners_dev || !params || !hist_dev ||
         !Efermi_dev || nEF <= 0 || (params->formula != WBMI_FORMULA_IDENTITY && !Xbar_dev))
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad arguments (nw <= %d)", SC_NWMAX);
-    if (params->fder < 0 || params->fder > 3 || !(params->EFmax >= params->EFmin))
+    if (params->fder < -1 || params->fder > 3 || !(params->EFmax >= params->EFmin))
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad fder or Fermi range");
     int rc = check_roles(ctx, params, nch);
     if (rc) return rc;

@@ -80,7 +80,7 @@ struct Smem {
     int* act;
     int* gb;    // group borders [nw+1]
     int* gin;   // group in range
-    int* misc;  // [0] = ngroups, [1] = bandmax
+    int* misc;  // [0] = ngroups, [1], [2] = sea group [lo, hi) (empty: lo == hi)
     int* ch;    // channel slot per role [WBMI_NROLES]
     // tetrahedron method only (empty regions otherwise)
     double* tC;    // [nw][8] corner energies
@@ -930,9 +930,15 @@ static __device__ void tetra_accumulate(const Smem& S, const ScanArgs& args, int
         if (S.gin[g]) {
             lo = ef_bound(args.Ef, nEF, S.tEmn[b], false);
             hi = ef_bound(args.Ef, nEF, S.tEmx[b] + TETRA_PAD, true);
-            if (der == 0 && hi < nEF) {
-                const double f = args.weight / (S.gb[g + 1] - S.gb[g]);
+            const double f = args.weight / (S.gb[g + 1] - S.gb[g]);
+            if (der == 0 && hi < nEF)
                 for (int c = 0; c < ncomp; ++c) unsafeAtomicAdd(hist + (int64_t)hi * ncomp + c, f * S.tv[g * 9 + c]);
+            if (der == -1) {
+                // 1 - occupation: 1 below the band's span (rows 0 .. hi-1), minus the occupation inside
+                for (int c = 0; c < ncomp; ++c) {
+                    unsafeAtomicAdd(hist + c, f * S.tv[g * 9 + c]);
+                    if (hi < nEF) unsafeAtomicAdd(hist + (int64_t)hi * ncomp + c, -f * S.tv[g * 9 + c]);
+                }
             }
         }
         S.tlo[b] = lo;
@@ -954,7 +960,8 @@ static __device__ void tetra_accumulate(const Smem& S, const ScanArgs& args, int
             else hi = m;
         }
         const int b = lo, e = S.tlo[b] + (t - S.tpre[b]), g = S.tg[b];
-        const double w = tetra_band(args.Ef[e], S.E[b], S.tC + b * 8, der);
+        const double w = der == -1 ? -tetra_band(args.Ef[e], S.E[b], S.tC + b * 8, 0)
+                                   : tetra_band(args.Ef[e], S.E[b], S.tC + b * 8, der);
         const double f = args.weight * w / (S.gb[g + 1] - S.gb[g]);
         for (int c = 0; c < ncomp; ++c)
             unsafeAtomicAdd(hist + (int64_t)(nEF + e) * ncomp + c, f * S.tv[g * 9 + c]);
@@ -1057,7 +1064,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             nb = m;
         }
         const int ng = nb - 1;
-        int first_in = -1;
+        int first_in = -1, last_in = 0;
         for (int g = 0; g < ng; ++g) {
             int a = S.gb[g], b = S.gb[g + 1];
             int inr;
@@ -1077,23 +1084,37 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             }
             S.gin[g] = inr;
             if (inr && first_in < 0) first_in = a;
+            if (inr) last_in = b;
             double sum = 0.0;
             for (int i = a; i < b; ++i) sum += S.E[i];
             S.gw[g] = sum / (b - a);
         }
-        int bandmax = 0;
+        int sea_lo = 0, sea_hi = 0;
         if (p.fder == 0 && !tab) {
-            // tetra: bands whose highest energy lies below Ef[0] (tetrahedron.py:155-162)
+            // bands below the range, up to the first group in it (data_K.py:178-185;
+            // tetra: highest energy over centre and corners below Ef[0], tetrahedron.py:155-162)
             for (int i = 0; i < nw; ++i)
-                if ((tetra ? S.tEmx[i] : S.E[i]) < p.EFmin) bandmax = i + 1;
-            if (first_in >= 0 && first_in < bandmax) bandmax = first_in;
+                if ((tetra ? S.tEmx[i] : S.E[i]) < p.EFmin) sea_hi = i + 1;
+            if (first_in >= 0 && first_in < sea_hi) sea_hi = first_in;
+        } else if (tetra && p.fder == -1) {
+            // hole-like: bands whose lowest energy lies above Ef[-1], from the end of the last
+            // group in range on (tetrahedron.py:165-172,246-253)
+            int bandmin = nw;
+            for (int i = nw - 1; i >= 0; --i)
+                if (S.tEmn[i] > p.EFmax) bandmin = i;
+            if (bandmin < last_in) bandmin = last_in;
+            if (bandmin < nw) {
+                sea_lo = bandmin;
+                sea_hi = nw;
+            }
         }
         S.misc[0] = ng;
-        S.misc[1] = bandmax;
+        S.misc[1] = sea_lo;
+        S.misc[2] = sea_hi;
     }
     __syncthreads();
     const int ng = S.misc[0];
-    const int bandmax = S.misc[1];
+    const int sea_lo = S.misc[1], sea_hi = S.misc[2];
 
     KCtx<T, LM, ST> K;
     K.X = Xk;
@@ -1148,28 +1169,30 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s);
         }
         __syncthreads();
-        if (bandmax > 0) {
-            // Fermi-sea group (0, bandmax) with weight -inf
+        if (sea_hi > sea_lo) {
+            // Fermi-sea group [sea_lo, sea_hi) (weight -inf; hole-like tetra: the bands above),
+            // added at every level; the other bands, on one side of it, are its "out"
+            const int out_lo = sea_lo == 0 ? sea_hi : 0, out_hi = sea_lo == 0 ? nw : sea_lo;
             for (int n = tid; n < nw; n += SC_THREADS) {
-                int inn = n < bandmax;
+                int inn = n >= sea_lo && n < sea_hi;
                 S.lab[n] = inn ? 0 : 1;
-                S.gs[n] = inn ? 0 : bandmax;
-                S.ge[n] = inn ? bandmax : nw;
+                S.gs[n] = inn ? sea_lo : out_lo;
+                S.ge[n] = inn ? sea_hi : out_hi;
                 S.act[n] = inn;
             }
             __syncthreads();
             eval_partition<F>(S, K, p);
             for (int c = tid; c < ncomp; c += SC_THREADS) {
                 double s = 0.0;
-                for (int n = 0; n < bandmax; ++n) s += S.acc[n * 9 + c].re;
-                unsafeAtomicAdd(hist + c, args.weight * s);  // bin 0: below EFmin
+                for (int n = sea_lo; n < sea_hi; ++n) s += S.acc[n * 9 + c].re;
+                unsafeAtomicAdd(hist + c, args.weight * s);  // bin 0: every level
             }
             __syncthreads();
         }
     } else {
         // non-additive: traces of the cumulative cuts [0, c) (static.py:109-117)
         for (int cidx = 1; cidx <= nw; ++cidx) {
-            bool need = (bandmax == cidx);
+            bool need = sea_hi > sea_lo && (cidx == sea_hi || cidx == sea_lo);
             for (int g = 0; g < ng; ++g)
                 if (S.gin[g] && (S.gb[g] == cidx || S.gb[g + 1] == cidx)) need = true;
             if (!need) continue;
@@ -1203,9 +1226,9 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             const double s = S.cut[S.gb[g + 1] * 9 + c] - S.cut[S.gb[g] * 9 + c];
             unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s);
         }
-        if (bandmax > 0)
+        if (sea_hi > sea_lo)
             for (int c = tid; c < ncomp; c += SC_THREADS)
-                unsafeAtomicAdd(hist + c, args.weight * S.cut[bandmax * 9 + c]);
+                unsafeAtomicAdd(hist + c, args.weight * (S.cut[sea_hi * 9 + c] - S.cut[sea_lo * 9 + c]));
     }
     if (tetra) {
         __syncthreads();
