@@ -283,6 +283,12 @@ int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev
                      const void* Xbar_dev, int prec, const wbmi_scan_params* params,
                      double weight, double* hist_dev);
 
+/* k-resolved scan (static.py:60-101,156-164, k_resolved=True): k-point k
+ * accumulates into its own histogram, hist_dev float64 [nk, nEF_extra, 3^ndim]. */
+int wbmi_static_scan_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                       const void* Xbar_dev, int prec, const wbmi_scan_params* params,
+                       double weight, double* hist_dev);
+
 /* ---- the same with tetrahedron weights -----------------------------------
  * Replaces the tetra=True branch of StaticCalculator.__call__
  * (calculators/static.py:80-85,119-125) with TetraWeightsParal
@@ -336,6 +342,9 @@ int wbmi_finalize_tetra(wbmi_ctx* ctx, double* hist_dev, int nEF, int ncomp, dou
  * Prefix sum over the Fermi levels, fder finite-difference stencil and scaling
  * (calculators/static.py:126-155).  out_dev: float64 [nEF_extra-2*extra, ncomp].
  * hist_dev is overwritten with its prefix sums (any nEF_extra).             */
+/* wbmi_finalize of nk histograms [nk, nEF_extra, ncomp] -> out [nk, nEF, ncomp]. */
+int wbmi_finalize_k(wbmi_ctx* ctx, double* hist_dev, int nk, int nEF_extra, int ncomp, int fder,
+                    double dEF, double scale, double* out_dev);
 int wbmi_finalize(wbmi_ctx* ctx, double* hist_dev, int nEF_extra, int ncomp,
                   int fder, double dEF, double scale, double* out_dev);
 

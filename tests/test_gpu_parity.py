@@ -738,3 +738,28 @@ def test_haldane_tabulate_symmetric(wb):
         e = rel_err(a, b)
         say(f"  {q}: {e:.2e}")
         assert e < 1e-10, q
+
+
+@pytest.mark.parametrize("use_sym", [False, True])
+def test_chiral_k_resolved(wb, use_sym):
+    """k-resolved AHC and Morb inside TabulatorAll: the mean over the grid of the per-k
+    results equals the integrated result (tests/test_run.py:873-909), with and without
+    the C3z group."""
+    say(f"test_chiral_k_resolved sym={use_sym}")
+    st, tab = wb.calculators.static, wb.calculators.tabulate
+    s = gsys(wb, "chiral_left")
+    if use_sym:
+        s.set_pointgroup(["C3z"])
+    Ef = np.linspace(-5, 8, 27)
+    calcs = {"AHC": st.AHC(Efermi=Ef), "Morb": st.Morb(Efermi=Ef),
+             "tabulate": tab.TabulatorAll({"AHC": st.AHC(Efermi=Ef, k_resolved=True),
+                                           "Morb": st.Morb(Efermi=Ef, k_resolved=True)}, ibands=(0, 1))}
+    grid = wb.Grid(system=s, NK=[10, 10, 4], NKFFT=[5, 5, 2])
+    r = wb.run(s, grid, calcs, use_irred_kpt=use_sym, symmetrize=use_sym, save_files=False, verbose=False)
+    for key in ("AHC", "Morb"):
+        d_int = r.results[key].data
+        d_tab = r.results["tabulate"].results[key].data
+        assert d_tab.shape == (400,) + d_int.shape
+        e = np.abs(d_tab.mean(axis=0) - d_int).max() / np.abs(d_int).max()
+        say(f"  {key}: {e:.2e}")
+        assert e < 1e-8, key

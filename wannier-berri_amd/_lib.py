@@ -39,6 +39,7 @@ SYMBOLS = [
     "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_optical_conductivity", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
     "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos", "wbmi_spin_hall", "wbmi_spin_hall_work_bytes",
     "wbmi_tabulate", "wbmi_tabulate_diag", "wbmi_shift_current", "wbmi_injection_current",
+    "wbmi_static_scan_k", "wbmi_finalize_k",
 ]
 
 
@@ -102,6 +103,8 @@ _PROTOS = {
     "wbmi_rotate_work_bytes": (ctypes.c_size_t, [_i, _i, _i]),
     "wbmi_static_scan": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _d, _vp]),
     "wbmi_finalize": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _vp]),
+    "wbmi_static_scan_k": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _d, _vp]),
+    "wbmi_finalize_k": (_i, [_vp, _vp, _i, _i, _i, _i, _d, _d, _vp]),
     "wbmi_static_scan_tetra": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp, _i, _d,
                                     _vp]),
     "wbmi_finalize_tetra": (_i, [_vp, _vp, _i, _i, _d, _vp]),
@@ -322,6 +325,18 @@ class Context:
         self.check(self._timed("injection", (nk, nw), lib().wbmi_injection_current, self.h, nk, nw, X.shape[1],
                                _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)),
                    "wbmi_injection_current")
+
+    def static_scan_k(self, E, X, prec, params, weight, hist):
+        nk, nw = E.shape
+        nch = X.shape[1] if X is not None else 1
+        self.check(self._timed(f"scan_k_f{params.formula}", (nk, nw), lib().wbmi_static_scan_k, self.h, nk, nw, nch,
+                               _ptr(E), _ptr(X), prec, ctypes.byref(params), float(weight), _ptr(hist)),
+                   "wbmi_static_scan_k")
+
+    def finalize_k(self, hist, fder, dEF, scale, out):
+        nk, nEF, ncomp = hist.shape
+        self.check(lib().wbmi_finalize_k(self.h, _ptr(hist), nk, nEF, ncomp, fder, float(dEF), float(scale),
+                                         _ptr(out)), "wbmi_finalize_k")
 
     def finalize(self, hist, fder, dEF, scale, out):
         nEF, ncomp = hist.shape

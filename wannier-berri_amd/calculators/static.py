@@ -11,7 +11,7 @@ import numpy as np
 
 from .. import _lib
 from .. import factors
-from ..result import EnergyResult, VoidResult
+from ..result import EnergyResult, KBandResult, VoidResult
 
 
 class Calculator:
@@ -53,7 +53,7 @@ class StaticCalculator(Calculator):
     """Fermi-level scan integrator (calculators/static.py:24-169).
 
     Subclasses set `formula` (a WBMI_FORMULA_* id), `ndim` and `fder`.
-    Not supported: k_resolved, select_bands, finite Emin/Emax.
+    Not supported: select_bands, finite Emin/Emax, k_resolved with tetra.
     """
     formula = None
     ndim = 0
@@ -63,8 +63,9 @@ class StaticCalculator(Calculator):
                  Emin=-np.inf, Emax=np.inf, hole_like=False, k_resolved=False, fder=None, select_bands=None,
                  **kwargs):
         super().__init__(**kwargs)
-        if k_resolved:
-            raise NotImplementedError("k-resolved output is not available on the MI355X path")
+        if k_resolved and tetra:
+            raise NotImplementedError("k-resolved output with the tetrahedron method is not available on the MI355X path")
+        self.k_resolved = bool(k_resolved)
         if select_bands is not None:
             raise NotImplementedError("select_bands is not available on the MI355X path")
         if np.isfinite(Emin) or np.isfinite(Emax):
@@ -148,9 +149,26 @@ class StaticCalculator(Calculator):
         data_K.ctx.finalize_tetra(hist, cf / data_K.cell_volume / data_K.nk, out)
         return out
 
+    def _call_k_resolved(self, data_K):
+        """Per k-point results [nk, nEF, 3^ndim] (static.py:60-101,156-164): each k its own
+        histogram, / cell volume (no / nk), x factor."""
+        torch = data_K.torch
+        dev = data_K.ctx.device
+        nk = data_K.nk
+        hist = torch.zeros((nk, self.nEF_extra, self.ncomp), dtype=torch.float64, device=dev)
+        data_K.static_scan_k(self.scan_params(data_K.system), hist, 1.0)
+        nout = self.Efermi.shape[0]
+        out = torch.empty((nk, nout, self.ncomp), dtype=torch.float64, device=dev)
+        cf = self.constant_factor if self.use_factor else float(np.sign(self.constant_factor))
+        data_K.ctx.finalize_k(hist, self.fder, self.dEF, cf / data_K.cell_volume, out)
+        tr, inv = _PARITY[self.formula]
+        return KBandResult(out.cpu().numpy().reshape((nk, nout) + (3,) * self.ndim), transformTR=tr, transformInv=inv)
+
     def __call__(self, data_K):
         torch = data_K.torch
         data_K.ensure(self.needed_channels(data_K.system))
+        if self.k_resolved:
+            return self._call_k_resolved(data_K)
         if self.tetra:
             out = self._call_tetra(data_K)
             data = out.reshape((self.Efermi.shape[0],) + (3,) * self.ndim)
@@ -282,6 +300,8 @@ class GME_orb_FermiSurf(StaticCalculator):
     sign = 1
 
     def __init__(self, constant_factor=factors.factor_gme_orb, **kwargs):
+        if kwargs.get("k_resolved"):
+            raise NotImplementedError("k-resolved GME is not available on the MI355X path")
         super().__init__(constant_factor=constant_factor, **kwargs)
         self.BerryDipole = BerryDipole_FermiSurf(constant_factor=constant_factor, print_comment=False, **kwargs)
 
@@ -306,6 +326,8 @@ class GME_orb_FermiSea(StaticCalculator):
     sign = 1
 
     def __init__(self, constant_factor=factors.factor_gme_orb, **kwargs):
+        if kwargs.get("k_resolved"):
+            raise NotImplementedError("k-resolved GME is not available on the MI355X path")
         super().__init__(constant_factor=constant_factor, **kwargs)
         self.BerryDipole = BerryDipole_FermiSea(constant_factor=constant_factor, print_comment=False, **kwargs)
 

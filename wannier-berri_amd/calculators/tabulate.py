@@ -144,6 +144,8 @@ class TabulatorAll(Calculator):
             self.tabulators["Energy"] = Energy()
         ibands = None if ibands is None else np.array(ibands)
         for k, v in self.tabulators.items():
+            if not hasattr(v, "ibands"):
+                continue   # k-resolved static calculators (static.py, k_resolved=True)
             if v.ibands is not None:
                 if ibands is None or len(v.ibands) != len(ibands) or np.any(v.ibands != ibands):
                     raise ValueError(f"tabulator {k} has ibands={v.ibands} not equal to ibands={ibands} "

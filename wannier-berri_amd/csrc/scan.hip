@@ -68,6 +68,7 @@ extern "C" int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const do
     a.Ec = nullptr;
     a.ec_stride = 0;
     a.Ef = nullptr;
+    a.hist_stride = 0;
     if (prec == WBMI_F64) return scan_dispatch_f64(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
     if (prec == WBMI_F32) return scan_dispatch_f32(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: bad precision");
@@ -96,9 +97,46 @@ extern "C" int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, co
     a.Ec = Ecorners_dev;
     a.ec_stride = (int64_t)nk * nw;
     a.Ef = Efermi_dev;
+    a.hist_stride = 0;
     if (prec == WBMI_F64) return scan_dispatch_tetra_f64(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
     if (prec == WBMI_F32) return scan_dispatch_tetra_f32(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad precision");
+}
+
+static int ncomp_of_formula(int f) {
+    switch (f) {
+        case WBMI_FORMULA_IDENTITY: return 1;
+        case WBMI_FORMULA_OMEGA:
+        case WBMI_FORMULA_MORB_HPM: return 3;
+        default: return 9;
+    }
+}
+
+extern "C" int wbmi_static_scan_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                                  int prec, const wbmi_scan_params* params, double weight, double* hist_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (nk <= 0 || nw <= 0 || nw > SC_NWMAX || nch <= 0 || !E_dev || !params || !hist_dev ||
+        (params->formula != WBMI_FORMULA_IDENTITY && !Xbar_dev))
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_k: bad arguments (nw <= %d)", SC_NWMAX);
+    if (params->nEF_extra <= 0 || !(params->dEF > 0) || params->fder < 0 || params->fder > 3)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_k: bad Fermi grid");
+    int rc = check_roles(ctx, params, nch);
+    if (rc) return rc;
+    ScanArgs a;
+    a.p = *params;
+    a.nw = nw;
+    a.nch = nch;
+    a.stage_va = nw <= SC_LDS_NW;
+    a.weight = weight;
+    a.tetra = 0;
+    a.nEF = 0;
+    a.Ec = nullptr;
+    a.ec_stride = 0;
+    a.Ef = nullptr;
+    a.hist_stride = (int64_t)params->nEF_extra * ncomp_of_formula(params->formula);
+    if (prec == WBMI_F64) return scan_dispatch_f64(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
+    if (prec == WBMI_F32) return scan_dispatch_f32(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_k: bad precision");
 }
 
 extern "C" int wbmi_tabulate(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
@@ -121,6 +159,7 @@ extern "C" int wbmi_tabulate(wbmi_ctx* ctx, int nk, int nw, int nch, const doubl
     a.Ec = nullptr;
     a.ec_stride = 0;
     a.Ef = nullptr;
+    a.hist_stride = 0;
     if (prec == WBMI_F64) return scan_dispatch_tab_f64(ctx, nk, a, E_dev, Xbar_dev, out_dev);
     if (prec == WBMI_F32) return scan_dispatch_tab_f32(ctx, nk, a, E_dev, Xbar_dev, out_dev);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate: bad precision");
@@ -134,8 +173,12 @@ extern "C" int wbmi_tabulate(wbmi_ctx* ctx, int nk, int nw, int nch, const doubl
 // One workgroup: per component, an in-place block prefix sum over the Fermi
 // levels (each thread owns a contiguous run), then the fder stencil.
 __global__ void __launch_bounds__(FIN_THREADS) k_finalize(double* __restrict__ hist, int nEF, int ncomp, int fder,
-                                                         double dEF, double scale, double* __restrict__ out) {
+                                                         double dEF, double scale, double* __restrict__ out,
+                                                         int64_t hstride, int64_t ostride) {
     __shared__ double part[FIN_THREADS];
+    // one histogram per workgroup (k-resolved: one per k-point)
+    hist += (int64_t)blockIdx.x * hstride;
+    out += (int64_t)blockIdx.x * ostride;
     const int tid = threadIdx.x;
     const int L = (nEF + FIN_THREADS - 1) / FIN_THREADS;
     const int i0 = tid * L < nEF ? tid * L : nEF, i1 = i0 + L < nEF ? i0 + L : nEF;
@@ -188,7 +231,7 @@ extern "C" int wbmi_finalize_tetra(wbmi_ctx* ctx, double* hist_dev, int nEF, int
     if (!hist_dev || !out_dev || ncomp <= 0 || nEF <= 0)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize_tetra: bad arguments");
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF, ncomp, 0, 1.0, scale,
-                       out_dev);
+                       out_dev, (int64_t)0, (int64_t)0);
     WBMI_LAUNCH_CHECK(ctx);
     const int64_t n = (int64_t)nEF * ncomp;
     hipLaunchKernelGGL(k_add_scaled, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, out_dev,
@@ -204,7 +247,20 @@ extern "C" int wbmi_finalize(wbmi_ctx* ctx, double* hist_dev, int nEF_extra, int
     if (!hist_dev || !out_dev || ncomp <= 0 || fder < 0 || fder > 3 || nEF_extra - 2 * extra <= 0)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize: bad arguments");
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF_extra, ncomp, fder, dEF,
-                       scale, out_dev);
+                       scale, out_dev, (int64_t)0, (int64_t)0);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+extern "C" int wbmi_finalize_k(wbmi_ctx* ctx, double* hist_dev, int nk, int nEF_extra, int ncomp, int fder, double dEF,
+                               double scale, double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    int extra = fder == 0 ? 0 : (fder == 3 ? 2 : 1);
+    if (!hist_dev || !out_dev || nk <= 0 || ncomp <= 0 || fder < 0 || fder > 3 || nEF_extra - 2 * extra <= 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize_k: bad arguments");
+    const int64_t hs = (int64_t)nEF_extra * ncomp, os = (int64_t)(nEF_extra - 2 * extra) * ncomp;
+    hipLaunchKernelGGL(k_finalize, dim3(nk), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF_extra, ncomp, fder, dEF,
+                       scale, out_dev, hs, os);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
 }

@@ -62,6 +62,8 @@ struct ScanArgs {
     const double* Ec;
     int64_t ec_stride;
     const double* Ef;
+    // k-resolved scans: k-point k accumulates into hist + k * hist_stride (0: one histogram)
+    int64_t hist_stride;
 };
 
 // dynamic shared-memory layout
@@ -991,6 +993,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const int nw = args.nw, ld = nw + 1;
     const int64_t k = blockIdx.x;
     const int tid = threadIdx.x;
+    if (MODE != 2) hist += k * args.hist_stride;
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
     static_assert(tasks || F == WBMI_FORMULA_OMEGA || !ST, "V and A are staged for DerOmega, DerMorb and Omega");
     constexpr bool stage_v = ST;

@@ -341,6 +341,12 @@ class Data_K:
             params.ch[i] = v
         self.ctx.static_scan(self.E_dev, self.Xk, self.prec, params, weight, hist)
 
+    def static_scan_k(self, params, hist, weight=1.0):
+        """k-resolved scan into hist [nk, nEF_extra, ncomp] (FP64, device)."""
+        for i, v in enumerate(self.role_table()):
+            params.ch[i] = v
+        self.ctx.static_scan_k(self.E_dev, self.Xk, self.prec, params, weight, hist)
+
     def subset(self, k0, k1):
         """The k-points [k0, k1) of an evaluated k-list block as a Data_K of their own.
 

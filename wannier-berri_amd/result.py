@@ -232,9 +232,24 @@ class KBandResult(Result):
         return KBandResult(np.concatenate((self.data, other.data), axis=0), self.transformTR, self.transformInv)
 
     def __mul__(self, other):
-        return self
+        return KBandResult(self.data * other, self.transformTR, self.transformInv)
 
-    __truediv__ = __mul__
+    def __truediv__(self, other):
+        return KBandResult(self.data / other, self.transformTR, self.transformInv)
+
+    def mul_array(self, arr, axes=0):
+        """Multiply along the axis after k (the Fermi levels of a k-resolved static result)."""
+        a = np.asarray(arr)
+        return KBandResult(self.data * a.reshape((1, -1) + (1,) * (self.data.ndim - 2)), self.transformTR,
+                           self.transformInv)
+
+    def add(self, other):
+        """Elementwise, in place (kbandresult.py:66-67)."""
+        self.data = self.data + other.data
+
+    def swapaxes(self, a, b):
+        """Swap tensor axes a, b counted as for EnergyResult (0 = the Fermi-level axis)."""
+        return KBandResult(self.data.swapaxes(a + 1, b + 1), self.transformTR, self.transformInv)
 
     def transform(self, sym):
         """Image under a point operation (kbandresult.py:150-159): the Cartesian axes rotated,
