@@ -742,7 +742,7 @@ def test_haldane_tabulate_symmetric(wb):
 
 @pytest.mark.parametrize("use_sym", [False, True])
 def test_chiral_k_resolved(wb, use_sym):
-    """k-resolved AHC and Morb inside TabulatorAll: the mean over the grid of the per-k
+    """k-resolved AHC, Morb, BCD, GME and DOS inside TabulatorAll: the mean over the grid of the per-k
     results equals the integrated result (tests/test_run.py:873-909), with and without
     the C3z group."""
     say(f"test_chiral_k_resolved sym={use_sym}")
@@ -751,12 +751,16 @@ def test_chiral_k_resolved(wb, use_sym):
     if use_sym:
         s.set_pointgroup(["C3z"])
     Ef = np.linspace(-5, 8, 27)
-    calcs = {"AHC": st.AHC(Efermi=Ef), "Morb": st.Morb(Efermi=Ef),
+    calcs = {"AHC": st.AHC(Efermi=Ef), "Morb": st.Morb(Efermi=Ef), "BCD": st.BerryDipole_FermiSea(Efermi=Ef),
+             "GME": st.GME_orb_FermiSea(Efermi=Ef), "DOS": st.DOS(Efermi=Ef),
              "tabulate": tab.TabulatorAll({"AHC": st.AHC(Efermi=Ef, k_resolved=True),
-                                           "Morb": st.Morb(Efermi=Ef, k_resolved=True)}, ibands=(0, 1))}
+                                           "Morb": st.Morb(Efermi=Ef, k_resolved=True),
+                                           "BCD": st.BerryDipole_FermiSea(Efermi=Ef, k_resolved=True),
+                                           "GME": st.GME_orb_FermiSea(Efermi=Ef, k_resolved=True),
+                                           "DOS": st.DOS(Efermi=Ef, k_resolved=True)}, ibands=(0, 1))}
     grid = wb.Grid(system=s, NK=[10, 10, 4], NKFFT=[5, 5, 2])
     r = wb.run(s, grid, calcs, use_irred_kpt=use_sym, symmetrize=use_sym, save_files=False, verbose=False)
-    for key in ("AHC", "Morb"):
+    for key in ("AHC", "Morb", "BCD", "GME", "DOS"):
         d_int = r.results[key].data
         d_tab = r.results["tabulate"].results[key].data
         assert d_tab.shape == (400,) + d_int.shape

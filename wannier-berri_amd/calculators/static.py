@@ -300,8 +300,6 @@ class GME_orb_FermiSurf(StaticCalculator):
     sign = 1
 
     def __init__(self, constant_factor=factors.factor_gme_orb, **kwargs):
-        if kwargs.get("k_resolved"):
-            raise NotImplementedError("k-resolved GME is not available on the MI355X path")
         super().__init__(constant_factor=constant_factor, **kwargs)
         self.BerryDipole = BerryDipole_FermiSurf(constant_factor=constant_factor, print_comment=False, **kwargs)
 
@@ -312,6 +310,9 @@ class GME_orb_FermiSurf(StaticCalculator):
         data_K.ensure(self.needed_channels(data_K.system))
         Hplus = super().__call__(data_K)
         Omega = self.BerryDipole(data_K).mul_array(self.Efermi)
+        if self.k_resolved:
+            Hplus.add(-2 * Omega)
+            return Hplus
         return Hplus - 2 * Omega
 
 
@@ -326,8 +327,6 @@ class GME_orb_FermiSea(StaticCalculator):
     sign = 1
 
     def __init__(self, constant_factor=factors.factor_gme_orb, **kwargs):
-        if kwargs.get("k_resolved"):
-            raise NotImplementedError("k-resolved GME is not available on the MI355X path")
         super().__init__(constant_factor=constant_factor, **kwargs)
         self.BerryDipole = BerryDipole_FermiSea(constant_factor=constant_factor, print_comment=False, **kwargs)
 
@@ -338,6 +337,9 @@ class GME_orb_FermiSea(StaticCalculator):
         data_K.ensure(self.needed_channels(data_K.system))
         Hplus = super().__call__(data_K).swapaxes(1, 2)
         Omega = self.BerryDipole(data_K).mul_array(self.Efermi)
+        if self.k_resolved:
+            Hplus.add(-2 * Omega)
+            return Hplus
         return Hplus - 2 * Omega
 
 
