@@ -332,6 +332,15 @@ int wbmi_tabulate_diag(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_d
                        int prec, const int32_t* ch3_host, double degen_thresh, int degen_Kramers,
                        double* out_dev);
 
+/* k-resolved: hist_dev [nk, 2 nEF, 3^ndim], one histogram per k-point */
+int wbmi_static_scan_tetra_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                             const double* Ecorners_dev, const void* Xbar_dev, int prec,
+                             const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
+                             double weight, double* hist_dev);
+/* wbmi_finalize_tetra of nk histograms [nk, 2 nEF, ncomp] -> out [nk, nEF, ncomp] */
+int wbmi_finalize_tetra_k(wbmi_ctx* ctx, double* hist_dev, int nk, int nEF, int ncomp, double scale,
+                          double* out_dev);
+
 /* out_dev [nEF, ncomp] = scale * (prefix sums of rows [0, nEF) + rows [nEF, 2 nEF))
  * of a wbmi_static_scan_tetra histogram (static.py:149-155, no fder stencil).
  * The step rows of hist_dev are overwritten with their prefix sums.         */

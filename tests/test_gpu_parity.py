@@ -740,24 +740,25 @@ def test_haldane_tabulate_symmetric(wb):
         assert e < 1e-10, q
 
 
-@pytest.mark.parametrize("use_sym", [False, True])
-def test_chiral_k_resolved(wb, use_sym):
+@pytest.mark.parametrize("use_sym,tetra", [(False, False), (True, False), (False, True), (True, True)])
+def test_chiral_k_resolved(wb, use_sym, tetra):
     """k-resolved AHC, Morb, BCD, GME and DOS inside TabulatorAll: the mean over the grid of the per-k
     results equals the integrated result (tests/test_run.py:873-909), with and without
     the C3z group."""
-    say(f"test_chiral_k_resolved sym={use_sym}")
+    say(f"test_chiral_k_resolved sym={use_sym} tetra={tetra}")
     st, tab = wb.calculators.static, wb.calculators.tabulate
     s = gsys(wb, "chiral_left")
     if use_sym:
         s.set_pointgroup(["C3z"])
     Ef = np.linspace(-5, 8, 27)
-    calcs = {"AHC": st.AHC(Efermi=Ef), "Morb": st.Morb(Efermi=Ef), "BCD": st.BerryDipole_FermiSea(Efermi=Ef),
-             "GME": st.GME_orb_FermiSea(Efermi=Ef), "DOS": st.DOS(Efermi=Ef),
-             "tabulate": tab.TabulatorAll({"AHC": st.AHC(Efermi=Ef, k_resolved=True),
-                                           "Morb": st.Morb(Efermi=Ef, k_resolved=True),
-                                           "BCD": st.BerryDipole_FermiSea(Efermi=Ef, k_resolved=True),
-                                           "GME": st.GME_orb_FermiSea(Efermi=Ef, k_resolved=True),
-                                           "DOS": st.DOS(Efermi=Ef, k_resolved=True)}, ibands=(0, 1))}
+    kw = dict(Efermi=Ef, tetra=tetra)
+    calcs = {"AHC": st.AHC(**kw), "Morb": st.Morb(**kw), "BCD": st.BerryDipole_FermiSea(**kw),
+             "GME": st.GME_orb_FermiSea(**kw), "DOS": st.DOS(**kw),
+             "tabulate": tab.TabulatorAll({"AHC": st.AHC(**kw, k_resolved=True),
+                                           "Morb": st.Morb(**kw, k_resolved=True),
+                                           "BCD": st.BerryDipole_FermiSea(**kw, k_resolved=True),
+                                           "GME": st.GME_orb_FermiSea(**kw, k_resolved=True),
+                                           "DOS": st.DOS(**kw, k_resolved=True)}, ibands=(0, 1))}
     grid = wb.Grid(system=s, NK=[10, 10, 4], NKFFT=[5, 5, 2])
     r = wb.run(s, grid, calcs, use_irred_kpt=use_sym, symmetrize=use_sym, save_files=False, verbose=False)
     for key in ("AHC", "Morb", "BCD", "GME", "DOS"):

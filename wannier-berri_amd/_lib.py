@@ -39,7 +39,7 @@ SYMBOLS = [
     "wbmi_eigh_work_bytes", "wbmi_eigh_guess", "wbmi_optical_conductivity", "wbmi_rotate", "wbmi_rotate_work_bytes", "wbmi_static_scan", "wbmi_finalize",
     "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos", "wbmi_spin_hall", "wbmi_spin_hall_work_bytes",
     "wbmi_tabulate", "wbmi_tabulate_diag", "wbmi_shift_current", "wbmi_injection_current",
-    "wbmi_static_scan_k", "wbmi_finalize_k",
+    "wbmi_static_scan_k", "wbmi_finalize_k", "wbmi_static_scan_tetra_k", "wbmi_finalize_tetra_k",
 ]
 
 
@@ -108,6 +108,9 @@ _PROTOS = {
     "wbmi_static_scan_tetra": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp, _i, _d,
                                     _vp]),
     "wbmi_finalize_tetra": (_i, [_vp, _vp, _i, _i, _d, _vp]),
+    "wbmi_static_scan_tetra_k": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp, _i, _d,
+                                      _vp]),
+    "wbmi_finalize_tetra_k": (_i, [_vp, _vp, _i, _i, _i, _d, _vp]),
     "wbmi_jdos": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(OptcondParams), _vp]),
     "wbmi_spin_hall_work_bytes": (ctypes.c_size_t, [_i, _i]),
     "wbmi_tabulate": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp]),
@@ -273,6 +276,21 @@ class Context:
                                nw, nch, _ptr(E), _ptr(Ec), _ptr(X), prec, ctypes.byref(params), _ptr(Efermi),
                                int(Efermi.shape[0]), float(weight), _ptr(hist)),
                    "wbmi_static_scan_tetra")
+
+    def static_scan_tetra_k(self, E, Ec, X, prec, params, Efermi, weight, hist):
+        """k-resolved tetrahedron scan: hist [nk, 2 nEF, ncomp]."""
+        nk, nw = E.shape
+        nch = X.shape[1] if X is not None else 1
+        if tuple(Ec.shape) != (8, nk, nw) or tuple(hist.shape[:2]) != (nk, 2 * Efermi.shape[0]):
+            raise ValueError("static_scan_tetra_k: Ec must be [8, nk, nw] and hist [nk, 2 nEF, ncomp]")
+        self.check(self._timed(f"scan_tetra_k_f{params.formula}", (nk, nw), lib().wbmi_static_scan_tetra_k, self.h,
+                               nk, nw, nch, _ptr(E), _ptr(Ec), _ptr(X), prec, ctypes.byref(params), _ptr(Efermi),
+                               int(Efermi.shape[0]), float(weight), _ptr(hist)), "wbmi_static_scan_tetra_k")
+
+    def finalize_tetra_k(self, hist, scale, out):
+        nk, nEF2, ncomp = hist.shape
+        self.check(lib().wbmi_finalize_tetra_k(self.h, _ptr(hist), nk, nEF2 // 2, ncomp, float(scale), _ptr(out)),
+                   "wbmi_finalize_tetra_k")
 
     def finalize_tetra(self, hist, scale, out):
         nEF2, ncomp = hist.shape

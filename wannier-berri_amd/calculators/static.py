@@ -53,7 +53,7 @@ class StaticCalculator(Calculator):
     """Fermi-level scan integrator (calculators/static.py:24-169).
 
     Subclasses set `formula` (a WBMI_FORMULA_* id), `ndim` and `fder`.
-    Not supported: select_bands, finite Emin/Emax, k_resolved with tetra.
+    Not supported: select_bands, finite Emin/Emax.
     """
     formula = None
     ndim = 0
@@ -63,8 +63,6 @@ class StaticCalculator(Calculator):
                  Emin=-np.inf, Emax=np.inf, hole_like=False, k_resolved=False, fder=None, select_bands=None,
                  **kwargs):
         super().__init__(**kwargs)
-        if k_resolved and tetra:
-            raise NotImplementedError("k-resolved output with the tetrahedron method is not available on the MI355X path")
         self.k_resolved = bool(k_resolved)
         if select_bands is not None:
             raise NotImplementedError("select_bands is not available on the MI355X path")
@@ -130,7 +128,8 @@ class StaticCalculator(Calculator):
         return p
 
     def _call_tetra(self, data_K):
-        """Tetrahedron weights (static.py:80-85,119-125): no extended Fermi grid, no stencil."""
+        """Tetrahedron weights (static.py:80-85,119-125): no extended Fermi grid, no stencil.
+        Returns [nEF, ncomp], or [nk, nEF, ncomp] k-resolved (/ cell volume only)."""
         torch = data_K.torch
         dev = data_K.ctx.device
         Ef = self._Ef_dev.get(dev)
@@ -142,10 +141,17 @@ class StaticCalculator(Calculator):
         p.nEF_extra = nEF
         if self.hole_like:
             p.fder = -1   # 1 - occupation (static.py:81-82)
+        cf = self.constant_factor if self.use_factor else float(np.sign(self.constant_factor))
+        if self.k_resolved:
+            nk = data_K.nk
+            hist = torch.zeros((nk, 2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
+            data_K.static_scan_tetra_k(p, Ef, hist, 1.0)
+            out = torch.empty((nk, nEF, self.ncomp), dtype=torch.float64, device=dev)
+            data_K.ctx.finalize_tetra_k(hist, cf / data_K.cell_volume, out)
+            return out
         hist = torch.zeros((2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
         data_K.static_scan_tetra(p, Ef, hist, 1.0)
         out = torch.empty((nEF, self.ncomp), dtype=torch.float64, device=dev)
-        cf = self.constant_factor if self.use_factor else float(np.sign(self.constant_factor))
         data_K.ctx.finalize_tetra(hist, cf / data_K.cell_volume / data_K.nk, out)
         return out
 
@@ -167,10 +173,14 @@ class StaticCalculator(Calculator):
     def __call__(self, data_K):
         torch = data_K.torch
         data_K.ensure(self.needed_channels(data_K.system))
-        if self.k_resolved:
+        if self.k_resolved and not self.tetra:
             return self._call_k_resolved(data_K)
         if self.tetra:
             out = self._call_tetra(data_K)
+            if self.k_resolved:
+                tr, inv = _PARITY[self.formula]
+                return KBandResult(out.cpu().numpy().reshape((data_K.nk, self.Efermi.shape[0]) + (3,) * self.ndim),
+                                   transformTR=tr, transformInv=inv)
             data = out.reshape((self.Efermi.shape[0],) + (3,) * self.ndim)
             tr, inv = _PARITY[self.formula]
             return EnergyResult(self.Efermi, data, smoothers=[self.smoother], transformTR=tr, transformInv=inv,

@@ -74,10 +74,31 @@ extern "C" int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const do
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: bad precision");
 }
 
+static int scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const double* Ecorners_dev,
+                      const void* Xbar_dev, int prec, const wbmi_scan_params* params, const double* Efermi_dev,
+                      int nEF, double weight, double* hist_dev, bool kres);
+
 extern "C" int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
                                       const double* Ecorners_dev, const void* Xbar_dev, int prec,
                                       const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
                                       double weight, double* hist_dev) {
+    return scan_tetra(ctx, nk, nw, nch, E_dev, Ecorners_dev, Xbar_dev, prec, params, Efermi_dev, nEF, weight, hist_dev,
+                      false);
+}
+
+extern "C" int wbmi_static_scan_tetra_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                                        const double* Ecorners_dev, const void* Xbar_dev, int prec,
+                                        const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
+                                        double weight, double* hist_dev) {
+    return scan_tetra(ctx, nk, nw, nch, E_dev, Ecorners_dev, Xbar_dev, prec, params, Efermi_dev, nEF, weight, hist_dev,
+                      true);
+}
+
+static int ncomp_of_formula(int f);
+
+static int scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const double* Ecorners_dev,
+                      const void* Xbar_dev, int prec, const wbmi_scan_params* params, const double* Efermi_dev,
+                      int nEF, double weight, double* hist_dev, bool kres) {
     if (!ctx) return WBMI_ERR_ARG;
     if (nk <= 0 || nw <= 0 || nw > SC_NWMAX || nch <= 0 || !E_dev || !Ecorners_dev || !params || !hist_dev ||
         !Efermi_dev || nEF <= 0 || (params->formula != WBMI_FORMULA_IDENTITY && !Xbar_dev))
@@ -97,7 +118,7 @@ extern "C" int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, co
     a.Ec = Ecorners_dev;
     a.ec_stride = (int64_t)nk * nw;
     a.Ef = Efermi_dev;
-    a.hist_stride = 0;
+    a.hist_stride = kres ? 2 * (int64_t)nEF * ncomp_of_formula(params->formula) : 0;
     if (prec == WBMI_F64) return scan_dispatch_tetra_f64(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
     if (prec == WBMI_F32) return scan_dispatch_tetra_f32(ctx, nk, a, E_dev, Xbar_dev, hist_dev);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad precision");
@@ -219,10 +240,12 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(double* __restrict__ h
     }
 }
 
-// out += scale * add (the directly accumulated rows of the tetrahedron histogram)
-__global__ void k_add_scaled(double* __restrict__ out, const double* __restrict__ add, double scale, int64_t n) {
+// out += scale * add (the directly accumulated rows of the tetrahedron histogram); n elements
+// in batches of `per`, batch b of `add` at b * hstride
+__global__ void k_add_scaled(double* __restrict__ out, const double* __restrict__ add, double scale, int64_t n,
+                             int64_t per, int64_t hstride) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] += scale * add[i];
+    if (i < n) out[i] += scale * add[(i / per) * hstride + i % per];
 }
 
 extern "C" int wbmi_finalize_tetra(wbmi_ctx* ctx, double* hist_dev, int nEF, int ncomp, double scale,
@@ -235,7 +258,23 @@ extern "C" int wbmi_finalize_tetra(wbmi_ctx* ctx, double* hist_dev, int nEF, int
     WBMI_LAUNCH_CHECK(ctx);
     const int64_t n = (int64_t)nEF * ncomp;
     hipLaunchKernelGGL(k_add_scaled, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, out_dev,
-                       hist_dev + n, scale, n);
+                       hist_dev + n, scale, n, n, 2 * n);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+extern "C" int wbmi_finalize_tetra_k(wbmi_ctx* ctx, double* hist_dev, int nk, int nEF, int ncomp, double scale,
+                                     double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (!hist_dev || !out_dev || nk <= 0 || ncomp <= 0 || nEF <= 0)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize_tetra_k: bad arguments");
+    const int64_t per = (int64_t)nEF * ncomp;
+    hipLaunchKernelGGL(k_finalize, dim3(nk), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF, ncomp, 0, 1.0, scale,
+                       out_dev, 2 * per, per);
+    WBMI_LAUNCH_CHECK(ctx);
+    const int64_t n = per * nk;
+    hipLaunchKernelGGL(k_add_scaled, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, ctx->stream, out_dev,
+                       hist_dev + per, scale, n, per, 2 * per);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
 }

@@ -307,6 +307,13 @@ class Data_K:
             params.ch[i] = v
         self.ctx.tabulate(self.E_dev, self.Xk, self.prec, params, out)
 
+    def static_scan_tetra_k(self, params, Efermi, hist, weight=1.0):
+        """k-resolved tetrahedron-weighted scan into hist [nk, 2 nEF, ncomp]."""
+        Ec = self.corner_energies()
+        for i, v in enumerate(self.role_table()):
+            params.ch[i] = v
+        self.ctx.static_scan_tetra_k(self.E_dev, Ec, self.Xk, self.prec, params, Efermi, weight, hist)
+
     def static_scan_tetra(self, params, Efermi, hist, weight=1.0):
         """Tetrahedron-weighted scan into hist [2 nEF, ncomp] (see wbmi_static_scan_tetra)."""
         Ec = self.corner_energies()
