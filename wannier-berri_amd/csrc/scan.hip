@@ -197,13 +197,15 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(double* __restrict__ h
                                                          double dEF, double scale, double* __restrict__ out,
                                                          int64_t hstride, int64_t ostride) {
     __shared__ double part[FIN_THREADS];
-    // one histogram per workgroup (k-resolved: one per k-point)
+    // one histogram (k-resolved: one per k-point, blockIdx.x) and one component (blockIdx.y)
+    // per workgroup
     hist += (int64_t)blockIdx.x * hstride;
     out += (int64_t)blockIdx.x * ostride;
     const int tid = threadIdx.x;
     const int L = (nEF + FIN_THREADS - 1) / FIN_THREADS;
     const int i0 = tid * L < nEF ? tid * L : nEF, i1 = i0 + L < nEF ? i0 + L : nEF;
-    for (int c = 0; c < ncomp; ++c) {
+    {
+        const int c = blockIdx.y;
         double sum = 0.0;
         for (int i = i0; i < i1; ++i) sum += hist[(int64_t)i * ncomp + c];
         part[tid] = sum;
@@ -224,8 +226,9 @@ __global__ void __launch_bounds__(FIN_THREADS) k_finalize(double* __restrict__ h
     }
     const int extra = fder == 0 ? 0 : (fder == 3 ? 2 : 1);
     const int nout = nEF - 2 * extra;
-    for (int t = tid; t < nout * ncomp; t += FIN_THREADS) {
-        const int i = t / ncomp, c = t % ncomp;
+    const int c = blockIdx.y;
+    for (int i = tid; i < nout; i += FIN_THREADS) {
+        const int t = i * ncomp + c;
         const double* r = hist + c;
         double v;
         if (fder == 0) v = r[(int64_t)i * ncomp];
@@ -253,7 +256,7 @@ extern "C" int wbmi_finalize_tetra(wbmi_ctx* ctx, double* hist_dev, int nEF, int
     if (!ctx) return WBMI_ERR_ARG;
     if (!hist_dev || !out_dev || ncomp <= 0 || nEF <= 0)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize_tetra: bad arguments");
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF, ncomp, 0, 1.0, scale,
+    hipLaunchKernelGGL(k_finalize, dim3(1, ncomp), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF, ncomp, 0, 1.0, scale,
                        out_dev, (int64_t)0, (int64_t)0);
     WBMI_LAUNCH_CHECK(ctx);
     const int64_t n = (int64_t)nEF * ncomp;
@@ -269,7 +272,7 @@ extern "C" int wbmi_finalize_tetra_k(wbmi_ctx* ctx, double* hist_dev, int nk, in
     if (!hist_dev || !out_dev || nk <= 0 || ncomp <= 0 || nEF <= 0)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize_tetra_k: bad arguments");
     const int64_t per = (int64_t)nEF * ncomp;
-    hipLaunchKernelGGL(k_finalize, dim3(nk), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF, ncomp, 0, 1.0, scale,
+    hipLaunchKernelGGL(k_finalize, dim3(nk, ncomp), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF, ncomp, 0, 1.0, scale,
                        out_dev, 2 * per, per);
     WBMI_LAUNCH_CHECK(ctx);
     const int64_t n = per * nk;
@@ -285,7 +288,7 @@ extern "C" int wbmi_finalize(wbmi_ctx* ctx, double* hist_dev, int nEF_extra, int
     int extra = fder == 0 ? 0 : (fder == 3 ? 2 : 1);
     if (!hist_dev || !out_dev || ncomp <= 0 || fder < 0 || fder > 3 || nEF_extra - 2 * extra <= 0)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize: bad arguments");
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF_extra, ncomp, fder, dEF,
+    hipLaunchKernelGGL(k_finalize, dim3(1, ncomp), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF_extra, ncomp, fder, dEF,
                        scale, out_dev, (int64_t)0, (int64_t)0);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
@@ -298,7 +301,7 @@ extern "C" int wbmi_finalize_k(wbmi_ctx* ctx, double* hist_dev, int nk, int nEF_
     if (!hist_dev || !out_dev || nk <= 0 || ncomp <= 0 || fder < 0 || fder > 3 || nEF_extra - 2 * extra <= 0)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_finalize_k: bad arguments");
     const int64_t hs = (int64_t)nEF_extra * ncomp, os = (int64_t)(nEF_extra - 2 * extra) * ncomp;
-    hipLaunchKernelGGL(k_finalize, dim3(nk), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF_extra, ncomp, fder, dEF,
+    hipLaunchKernelGGL(k_finalize, dim3(nk, ncomp), dim3(FIN_THREADS), 0, ctx->stream, hist_dev, nEF_extra, ncomp, fder, dEF,
                        scale, out_dev, hs, os);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
