@@ -223,9 +223,12 @@ class OracleDataK:
     (arbitrary k-points, no expdK phase: rvectors.py:481-482).
     """
 
-    def __init__(self, system, dK=None, NKFFT=None, k_list=None, dK_block=None):
+    def __init__(self, system, dK=None, NKFFT=None, k_list=None, dK_block=None, tetra_vertices=None):
         assert (dK is None) != (k_list is None)
         self.system = system
+        # tetrahedral K-block: the 4 corners of the tetrahedron around each k, relative
+        # to it, in units of the reciprocal lattice (KpointBZtetra.vertices_fullBZ)
+        self.tetra_vertices = None if tetra_vertices is None else np.array(tetra_vertices, dtype=float)
         # size of the K-block in the full BZ (Kpoint.dK); the tetrahedron corners
         # lie at +-dK_block / NKFFT / 2 around each k (Kpoint.py:107-109)
         self.dK_block = None if dK_block is None else np.array(dK_block, dtype=float)
@@ -305,6 +308,17 @@ class OracleDataK:
     @property
     def E_K(self):
         return self._eig[0]
+
+    @cached_property
+    def E_K_corners_tetra(self):
+        """Band energies [nk, 4, nw] at the vertices k + v_i of the tetrahedron around each k
+        (data_K_R.py:99-119)."""
+        H_R = self.get_R_mat('Ham')
+        out = np.zeros((self.nk, 4, self.num_wann))
+        for i, v in enumerate(self.tetra_vertices):
+            ph = np.exp(2j * np.pi * self.system.iRvec.dot(v))
+            out[:, i, :] = np.linalg.eigvalsh(r2k_grid(H_R * ph[:, None, None], self.system.iRvec, self.NKFFT, True))
+        return out
 
     @cached_property
     def E_K_corners_parallel(self):
@@ -497,16 +511,24 @@ def tetra_group_weights(data_K, Efermi, der, degen_thresh=-1, degen_Kramers=Fals
     its weight is the mean of its bands' weights.  der=0 adds the group of all bands
     entirely below Efermi[0] with weight 1 (155-162), der=-1 the group above Efermi[-1]."""
     Ec = data_K.E_K
-    Ecorn = data_K.E_K_corners_parallel
+    tetragrid = getattr(data_K, "tetra_vertices", None) is not None
+    Ecorn = data_K.E_K_corners_tetra if tetragrid else data_K.E_K_corners_parallel
     out = []
     for ik in range(data_K.nk):
-        Eall = np.concatenate((Ec[ik][None], Ecorn[ik].reshape(8, -1)), axis=0)
+        Eall = np.concatenate((Ec[ik][None], Ecorn[ik].reshape(4 if tetragrid else 8, -1)), axis=0)
         Emin, Emax = Eall.min(axis=0), Eall.max(axis=0)
         nb = len(Ec[ik])
         in_range = [(a, b) for a, b in band_groups(Ec[ik], degen_thresh, degen_Kramers)
                     if Emax[a:b].max() >= Efermi[0] and Emin[a:b].min() <= Efermi[-1]]
-        w = {(a, b): sum(tetra_band_weight(Efermi, Ec[ik, i], Ecorn[ik, :, :, :, i], der)
-                         for i in range(a, b)) / (b - a) for a, b in in_range}
+        if tetragrid:
+            # one tetrahedron per k (TetraWeights, tetrahedron.py:204-209)
+            def bw(i):
+                occ = tetra_occupation(Efermi, Ecorn[ik, :, i], 0 if der == -1 else der)
+                return 1 - occ if der == -1 else occ
+        else:
+            def bw(i):
+                return tetra_band_weight(Efermi, Ec[ik, i], Ecorn[ik, :, :, :, i], der)
+        w = {(a, b): sum(bw(i) for i in range(a, b)) / (b - a) for a, b in in_range}
         if der == 0:
             below = np.where(Emax < Efermi[0])[0]
             bandmax = below[-1] + 1 if len(below) else 0

@@ -100,6 +100,12 @@ def main():
              "chiral_tetra_berry_dipole_fsurf": I + "Chiral_tetra-berry_dipole_fsurf_iter-0000.npz",
              "chiral_tetra_dos": I + "Chiral_tetra-dos_iter-0000.npz",
              "chiral_tetra_cumdos": I + "Chiral_tetra-cumdos_iter-0000.npz",
+             # the tetrahedral grid: GridTetra(length=8, NKFFT=[5, 5, 2]) (tests/test_run.py:958-968)
+             "chiral_tetragrid_ahc": I + "Chiral_tetragrid-ahc_iter-0000.npz",
+             "chiral_tetragrid_berry_dipole": I + "Chiral_tetragrid-berry_dipole_iter-0000.npz",
+             "chiral_tetragrid_berry_dipole_fsurf": I + "Chiral_tetragrid-berry_dipole_fsurf_iter-0000.npz",
+             "chiral_tetragrid_dos": I + "Chiral_tetragrid-dos_iter-0000.npz",
+             "chiral_tetragrid_cumdos": I + "Chiral_tetragrid-cumdos_iter-0000.npz",
              "haldane_adpt3_tetra_ahc_iter0": I + "Haldane-ahc-PythTB-adpt3-tetra-sym_iter-0000.npz",
              "haldane_adpt3_tetra_ahc_iter1": I + "Haldane-ahc-PythTB-adpt3-tetra-sym_iter-0001.npz",
              "haldane_adpt3_tetra_dos_iter0": I + "Haldane-dos-PythTB-adpt3-tetra-sym_iter-0000.npz",

@@ -144,3 +144,34 @@ def test_tetra_hole_like_counts(wb):
     r = wb.run(s, grid, calcs, use_irred_kpt=False, symmetrize=False, data_k_class=_OracleBlock,
                save_files=False, verbose=False)
     assert np.allclose(r.results["c"].data - r.results["h"].data, s.num_wann, rtol=0, atol=1e-12)
+
+
+class _OracleBlockAny:
+    """data_k_class for run() on parallelepiped or tetrahedral K-blocks."""
+    osys = None
+
+    def __init__(self, system, dK=None, grid=None, Kpoint=None, **kw):
+        if hasattr(Kpoint, "vertices_fullBZ"):
+            self.d = O.OracleDataK(self.osys, dK=dK, NKFFT=grid.FFT, tetra_vertices=Kpoint.vertices_fullBZ)
+        else:
+            self.d = O.OracleDataK(self.osys, dK=dK, NKFFT=grid.FFT, dK_block=Kpoint.dK)
+
+
+def test_chiral_tetragrid(wb, results_run):
+    """The tetrahedral grid (grid_tetra.py, Kpoint_tetra.py; one tetrahedron per k-point,
+    tetrahedron.py:185-209) against the reference's Chiral_tetragrid fixtures
+    (tests/test_run.py:958-968; C3z symmetrisation of every block's result)."""
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_chiral_left.npz"))
+    s.set_pointgroup(["C3z"])
+    _OracleBlockAny.osys = O.load_golden_system(os.path.join(GOLDEN, "sys_chiral_left.npz"))
+    Ef = np.linspace(-5, 8, 27)
+    calcs = {"ahc": _OracleCalc(wb, O.AHC(Ef, tetra=True), -1, 1),
+             "berry_dipole": _OracleCalc(wb, O.BerryDipole_FermiSea(Ef, tetra=True, use_factor=False), 1, -1),
+             "berry_dipole_fsurf": _OracleCalc(wb, O.BerryDipole_FermiSurf(Ef, tetra=True, use_factor=False), 1, -1),
+             "dos": _OracleCalc(wb, O.DOS(Ef, tetra=True), 1, 1),
+             "cumdos": _OracleCalc(wb, O.CumDOS(Ef, tetra=True), 1, 1)}
+    grid = wb.grid.GridTetra(s, length=8, NKFFT=[5, 5, 2])
+    r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, data_k_class=_OracleBlockAny,
+               save_files=False, verbose=False)
+    for k in calcs:
+        assert rel_err(r.results[k].data, results_run["chiral_tetragrid_" + k]) < 1e-8, k

@@ -290,3 +290,137 @@ class Grid:
                                 KP.absorb(K[k[0]][k[1]][k[2]])
                                 K[k[0]][k[1]][k[2]] = None
         return [Kp for Kyz in K for Kz in Kyz for Kp in Kz if Kp is not None]
+
+
+# ---------------------------------------------------------------------------
+# tetrahedral K-blocks (grid/grid_tetra.py:8-152, grid/Kpoint_tetra.py:9-86)
+# ---------------------------------------------------------------------------
+
+# edges of a tetrahedron by their vertices, and the two vertices off each edge
+TETRA_EDGES = [[0, 1], [0, 2], [0, 3], [1, 2], [1, 3], [2, 3]]
+TETRA_EDGES_COMPLEMENT = [sorted({0, 1, 2, 3} - set(e)) for e in TETRA_EDGES]
+
+
+def tetra_volume(vertices):
+    return abs(np.linalg.det(vertices[1:] - vertices[0][None, :])) / 6.
+
+
+class KpointBZtetra(KpointBZparallel):
+    """A K-block of the tetrahedral grid: the NKFFT sub-grid shifted to the centre of a
+    tetrahedron, whose vertices (relative to the centre, in units of the reciprocal lattice
+    divided by NKFFT) are the corners of the tetrahedron method (Kpoint_tetra.py:13-86)."""
+
+    def __init__(self, vertices, basis=np.eye(3), K=0, NKFFT=np.ones(3), factor=1., refinement_level=0,
+                 split_level=0):
+        vertices = np.array(vertices, dtype=float)
+        assert vertices.shape == (4, 3)
+        cntr = vertices.mean(axis=0)
+        super().__init__(K=np.array(K, dtype=float) + cntr, dK=np.ones(3), NKFFT=NKFFT, factor=factor,
+                         pointgroup=None, refinement_level=refinement_level)
+        self.basis = basis
+        self.vertices = vertices - cntr
+        self.split_level = split_level
+
+    def copy(self):
+        return KpointBZtetra(vertices=self.vertices, K=self.K, NKFFT=self.NKFFT, factor=self.factor, basis=self.basis,
+                             refinement_level=self.refinement_level, split_level=self.split_level)
+
+    @property
+    def vertices_fullBZ(self):
+        return self.vertices / self.NKFFT
+
+    @property
+    def _edge_lengths(self):
+        edges = np.array([self.vertices[j] - self.vertices[i] for i, j in TETRA_EDGES]).dot(self.basis)
+        return np.linalg.norm(edges, axis=1)
+
+    @property
+    def size(self):
+        return max(self._edge_lengths)
+
+    def _i_max_edge(self):
+        """The longest edge; among edges equal to 1e-6, the first in TETRA_EDGES (Kpoint_tetra.py:38-48)."""
+        lengths = self._edge_lengths
+        srt = np.argsort(lengths)
+        return min(srt[np.where(lengths[srt[-1]] - lengths[srt] < 1e-6)])
+
+    def divide(self, ndiv=2, periodic=(True, True, True), use_symmetry=True, refine=True):
+        """Split along the longest edge into ndiv tetrahedra (Kpoint_tetra.py:54-86)."""
+        if not np.all(periodic):
+            raise ValueError("tetrahedron grid can be used only for 3D-periodic systems")
+        if isinstance(ndiv, Iterable):
+            ndiv = ndiv[0]
+        ie = self._i_max_edge()
+        edge, comp = TETRA_EDGES[ie], TETRA_EDGES_COMPLEMENT[ie]
+        v0 = self.vertices[edge[0]]
+        dv = (self.vertices[edge[1]] - v0) / ndiv
+        out = [KpointBZtetra(K=self.K, vertices=np.array([self.vertices[comp[0]], self.vertices[comp[1]],
+                                                          v0 + i * dv, v0 + (i + 1) * dv]),
+                             factor=self.factor / ndiv, basis=self.basis,
+                             refinement_level=self.refinement_level + int(refine),
+                             split_level=self.split_level + int(not refine), NKFFT=self.NKFFT)
+               for i in range(ndiv)]
+        self.set_factor(0)
+        return out
+
+
+class GridTetra:
+    """K-grid of tetrahedra (grid/grid_tetra.py:8-152): the reciprocal cell (or `IBZ_tetra`)
+    split into tetrahedra by volume and by edge length, each the centre of an NKFFT
+    sub-grid; use with the tetrahedron method."""
+
+    def __init__(self, system, length, NKFFT=None, length_FFT=None, IBZ_tetra=None, weights=None,
+                 refine_by_volume=True, refine_by_size=True, length_size=None):
+        _, self.FFT = determineNK(system.periodic, 1, NKFFT, None, system.NKFFT_recommended, system.recip_lattice,
+                                  length=None, length_FFT=length_FFT, pointgroup=system.pointgroup)
+        self.div = np.ones(3, dtype=int)
+        self.recip_lattice_reduced = system.recip_lattice / self.FFT[:, None]
+        if IBZ_tetra is None:   # the reciprocal cell as 5 tetrahedra (grid_tetra.py:44-50)
+            tetrahedra = np.array([[[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]],
+                                   [[1, 0, 1], [0, 0, 1], [1, 0, 0], [1, 1, 1]],
+                                   [[1, 1, 0], [1, 0, 0], [0, 1, 0], [1, 1, 1]],
+                                   [[0, 1, 1], [0, 0, 1], [0, 1, 0], [1, 1, 1]],
+                                   [[0, 0, 1], [0, 1, 0], [1, 0, 0], [1, 1, 1]]], dtype=float) - 0.5
+        else:
+            tetrahedra = np.array(IBZ_tetra, dtype=float)
+        vol = np.array([tetra_volume(t) for t in tetrahedra])
+        weights = vol / vol.sum() if weights is None else np.array(weights) * vol
+        self.K_list = [KpointBZtetra(vertices=t, K=0, NKFFT=self.FFT, factor=w, basis=self.recip_lattice_reduced)
+                       for t, w in zip(tetrahedra, weights)]
+        if refine_by_volume:
+            dkmax = 2 * np.pi / length
+            self._split_volume(dkmax ** 3 / np.linalg.det(self.recip_lattice_reduced))
+        if refine_by_size:
+            if length_size is None:
+                length_size = 0.5 * length
+            self._split_size((2 * np.pi / length_size) * np.sqrt(2))
+
+    def _split_size(self, dkmax):
+        while max(K.size for K in self.K_list) >= dkmax:
+            kl = []
+            for K in self.K_list:
+                kl += K.divide(ndiv=2, refine=False) if K.size > dkmax else [K]
+            self.K_list = kl
+
+    def _split_volume(self, vmax):
+        while True:
+            vols = [tetra_volume(K.vertices) for K in self.K_list]
+            if max(vols) < vmax:
+                break
+            kl = []
+            for K, v in zip(self.K_list, vols):
+                kl += K.divide(ndiv=2, refine=False) if v > vmax else [K]
+            self.K_list = kl
+
+    @property
+    def dense(self):
+        return self.FFT
+
+    @property
+    def points_FFT(self):
+        N = self.FFT
+        return np.array([[i / N[0], j / N[1], l / N[2]] for i in range(N[0]) for j in range(N[1])
+                         for l in range(N[2])])
+
+    def get_K_list(self, use_symmetry=True, k_batch=None):
+        return [K.copy() for K in self.K_list]
