@@ -297,9 +297,11 @@ int wbmi_static_scan_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_d
  * or their fder-th derivative (1..3).  Groups are kept, and the fder-0 group
  * of bands below the range is formed, from each band's energy range over the
  * centre and the 8 corners.
- *   Ecorners_dev : float64 [8, nk, nw] band energies at k + ((2i-1), (2j-1),
- *                  (2l-1)) * dK_fullBZ / 2, corner i*4 + j*2 + l
- *                  (data_K/data_K_R.py:126-141)
+ *   ncorner      : 8 (parallelepiped K-blocks, 12 tetrahedra around each k) or
+ *                  4 (tetrahedral K-blocks of GridTetra, one tetrahedron)
+ *   Ecorners_dev : float64 [ncorner, nk, nw] band energies at the corners:
+ *                  k + ((2i-1), (2j-1), (2l-1)) * dK_fullBZ / 2, corner i*4 + j*2 + l
+ *                  (data_K/data_K_R.py:126-141), or k + vertex_i (99-119)
  *   Efermi_dev   : float64 [nEF], ascending
  *   params       : fder = 0..3, or -1 for hole_like (weights 1 - occupation and
  *                  the group of bands above the range, tetrahedron.py:165-172);
@@ -308,7 +310,7 @@ int wbmi_static_scan_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_d
  *   hist_dev     : float64 [2 * nEF, 3^ndim]: rows [0, nEF) are steps
  *                  (prefix-summed by wbmi_finalize_tetra), rows [nEF, 2 nEF)
  *                  are added as they are.                                    */
-int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, int ncorner,
                            const double* Ecorners_dev, const void* Xbar_dev, int prec,
                            const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
                            double weight, double* hist_dev);
@@ -333,7 +335,7 @@ int wbmi_tabulate_diag(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_d
                        double* out_dev);
 
 /* k-resolved: hist_dev [nk, 2 nEF, 3^ndim], one histogram per k-point */
-int wbmi_static_scan_tetra_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+int wbmi_static_scan_tetra_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, int ncorner,
                              const double* Ecorners_dev, const void* Xbar_dev, int prec,
                              const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
                              double weight, double* hist_dev);

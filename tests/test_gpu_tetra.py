@@ -148,3 +148,27 @@ def test_tetra_hole_like_vs_oracle(wb):
         err = rel_err(r.results[k].data, ref[k].data)
         say(f"  {k}: {err:.2e}")
         assert err < 1e-10, k
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_chiral_tetragrid(wb, results_run, precision, tol):
+    """The tetrahedral grid (GridTetra(length=8, NKFFT=[5, 5, 2]), one tetrahedron per
+    k-point, 4 corner energies) against the reference's Chiral_tetragrid fixtures
+    (tests/test_run.py:958-968)."""
+    say(f"test_chiral_tetragrid {precision}")
+    st = wb.calculators.static
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_chiral_left.npz"))
+    s.set_pointgroup(["C3z"])
+    Ef = np.linspace(-5, 8, 27)
+    calcs = {"ahc": st.AHC(Efermi=Ef, tetra=True),
+             "berry_dipole": st.BerryDipole_FermiSea(Efermi=Ef, tetra=True, use_factor=False),
+             "berry_dipole_fsurf": st.BerryDipole_FermiSurf(Efermi=Ef, tetra=True, use_factor=False),
+             "dos": st.DOS(Efermi=Ef, tetra=True),
+             "cumdos": st.CumDOS(Efermi=Ef, tetra=True)}
+    grid = wb.grid.GridTetra(s, length=8, NKFFT=[5, 5, 2])
+    r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, parameters_K={"precision": precision},
+               save_files=False, verbose=False)
+    for k in calcs:
+        err = rel_err(r.results[k].data, results_run["chiral_tetragrid_" + k])
+        say(f"  {k}: {err:.2e}")
+        assert err < tol, k

@@ -105,11 +105,11 @@ _PROTOS = {
     "wbmi_finalize": (_i, [_vp, _vp, _i, _i, _i, _d, _d, _vp]),
     "wbmi_static_scan_k": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _d, _vp]),
     "wbmi_finalize_k": (_i, [_vp, _vp, _i, _i, _i, _i, _d, _d, _vp]),
-    "wbmi_static_scan_tetra": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp, _i, _d,
-                                    _vp]),
+    "wbmi_static_scan_tetra": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp, _i,
+                                    _d, _vp]),
     "wbmi_finalize_tetra": (_i, [_vp, _vp, _i, _i, _d, _vp]),
-    "wbmi_static_scan_tetra_k": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp, _i, _d,
-                                      _vp]),
+    "wbmi_static_scan_tetra_k": (_i, [_vp, _i, _i, _i, _vp, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp, _i,
+                                      _d, _vp]),
     "wbmi_finalize_tetra_k": (_i, [_vp, _vp, _i, _i, _i, _d, _vp]),
     "wbmi_jdos": (_i, [_vp, _i, _i, _vp, ctypes.POINTER(OptcondParams), _vp]),
     "wbmi_spin_hall_work_bytes": (ctypes.c_size_t, [_i, _i]),
@@ -270,10 +270,12 @@ class Context:
         hist [2 nEF, ncomp]."""
         nk, nw = E.shape
         nch = X.shape[1] if X is not None else 1
-        if tuple(Ec.shape) != (8, nk, nw) or tuple(hist.shape) != (2 * Efermi.shape[0], hist.shape[1]):
-            raise ValueError("static_scan_tetra: Ec must be [8, nk, nw] and hist [2 nEF, ncomp]")
+        if tuple(Ec.shape[1:]) != (nk, nw) or Ec.shape[0] not in (4, 8) or \
+                tuple(hist.shape) != (2 * Efermi.shape[0], hist.shape[1]):
+            raise ValueError("static_scan_tetra: Ec must be [8 or 4, nk, nw] and hist [2 nEF, ncomp]")
         self.check(self._timed(f"scan_tetra_f{params.formula}", (nk, nw), lib().wbmi_static_scan_tetra, self.h, nk,
-                               nw, nch, _ptr(E), _ptr(Ec), _ptr(X), prec, ctypes.byref(params), _ptr(Efermi),
+                               nw, nch, _ptr(E), int(Ec.shape[0]), _ptr(Ec), _ptr(X), prec, ctypes.byref(params),
+                               _ptr(Efermi),
                                int(Efermi.shape[0]), float(weight), _ptr(hist)),
                    "wbmi_static_scan_tetra")
 
@@ -281,10 +283,12 @@ class Context:
         """k-resolved tetrahedron scan: hist [nk, 2 nEF, ncomp]."""
         nk, nw = E.shape
         nch = X.shape[1] if X is not None else 1
-        if tuple(Ec.shape) != (8, nk, nw) or tuple(hist.shape[:2]) != (nk, 2 * Efermi.shape[0]):
-            raise ValueError("static_scan_tetra_k: Ec must be [8, nk, nw] and hist [nk, 2 nEF, ncomp]")
+        if tuple(Ec.shape[1:]) != (nk, nw) or Ec.shape[0] not in (4, 8) or \
+                tuple(hist.shape[:2]) != (nk, 2 * Efermi.shape[0]):
+            raise ValueError("static_scan_tetra_k: Ec must be [8 or 4, nk, nw] and hist [nk, 2 nEF, ncomp]")
         self.check(self._timed(f"scan_tetra_k_f{params.formula}", (nk, nw), lib().wbmi_static_scan_tetra_k, self.h,
-                               nk, nw, nch, _ptr(E), _ptr(Ec), _ptr(X), prec, ctypes.byref(params), _ptr(Efermi),
+                               nk, nw, nch, _ptr(E), int(Ec.shape[0]), _ptr(Ec), _ptr(X), prec, ctypes.byref(params),
+                               _ptr(Efermi),
                                int(Efermi.shape[0]), float(weight), _ptr(hist)), "wbmi_static_scan_tetra_k")
 
     def finalize_tetra_k(self, hist, scale, out):

@@ -74,31 +74,33 @@ extern "C" int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const do
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: bad precision");
 }
 
-static int scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const double* Ecorners_dev,
-                      const void* Xbar_dev, int prec, const wbmi_scan_params* params, const double* Efermi_dev,
-                      int nEF, double weight, double* hist_dev, bool kres);
+static int scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, int ncorner,
+                      const double* Ecorners_dev, const void* Xbar_dev, int prec, const wbmi_scan_params* params,
+                      const double* Efermi_dev, int nEF, double weight, double* hist_dev, bool kres);
 
-extern "C" int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+extern "C" int wbmi_static_scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, int ncorner,
                                       const double* Ecorners_dev, const void* Xbar_dev, int prec,
                                       const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
                                       double weight, double* hist_dev) {
-    return scan_tetra(ctx, nk, nw, nch, E_dev, Ecorners_dev, Xbar_dev, prec, params, Efermi_dev, nEF, weight, hist_dev,
-                      false);
+    return scan_tetra(ctx, nk, nw, nch, E_dev, ncorner, Ecorners_dev, Xbar_dev, prec, params, Efermi_dev, nEF, weight,
+                      hist_dev, false);
 }
 
-extern "C" int wbmi_static_scan_tetra_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+extern "C" int wbmi_static_scan_tetra_k(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, int ncorner,
                                         const double* Ecorners_dev, const void* Xbar_dev, int prec,
                                         const wbmi_scan_params* params, const double* Efermi_dev, int nEF,
                                         double weight, double* hist_dev) {
-    return scan_tetra(ctx, nk, nw, nch, E_dev, Ecorners_dev, Xbar_dev, prec, params, Efermi_dev, nEF, weight, hist_dev,
-                      true);
+    return scan_tetra(ctx, nk, nw, nch, E_dev, ncorner, Ecorners_dev, Xbar_dev, prec, params, Efermi_dev, nEF, weight,
+                      hist_dev, true);
 }
 
 static int ncomp_of_formula(int f);
 
-static int scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const double* Ecorners_dev,
-                      const void* Xbar_dev, int prec, const wbmi_scan_params* params, const double* Efermi_dev,
-                      int nEF, double weight, double* hist_dev, bool kres) {
+static int scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, int ncorner,
+                      const double* Ecorners_dev, const void* Xbar_dev, int prec, const wbmi_scan_params* params,
+                      const double* Efermi_dev, int nEF, double weight, double* hist_dev, bool kres) {
+    if (ncorner != 8 && ncorner != 4)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: ncorner must be 8 or 4");
     if (!ctx) return WBMI_ERR_ARG;
     if (nk <= 0 || nw <= 0 || nw > SC_NWMAX || nch <= 0 || !E_dev || !Ecorners_dev || !params || !hist_dev ||
         !Efermi_dev || nEF <= 0 || (params->formula != WBMI_FORMULA_IDENTITY && !Xbar_dev))
@@ -113,7 +115,7 @@ static int scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_de
     a.nch = nch;
     a.stage_va = nw <= SC_LDS_NW;
     a.weight = weight;
-    a.tetra = 1;
+    a.tetra = ncorner;
     a.nEF = nEF;
     a.Ec = Ecorners_dev;
     a.ec_stride = (int64_t)nk * nw;

@@ -55,9 +55,10 @@ struct ScanArgs {
     wbmi_scan_params p;
     int nw, nch, stage_va;
     double weight;
-    // tetrahedron method (tetra != 0): corner energies [8][nk][nw] (corner ix*4 + iy*2 + iz at
-    // k + (2i-1) dK/2), ascending Fermi levels Ef[nEF]; hist holds nEF step rows (prefix-summed
-    // later) followed by nEF rows added as they are
+    // tetrahedron method: `tetra` = number of corners, 8 (parallelepiped: corner ix*4 + iy*2 + iz
+    // at k + (2i-1) dK/2, 12 tetrahedra around the centre) or 4 (tetrahedral K-block: the vertices,
+    // one tetrahedron); corner energies [tetra][nk][nw], ascending Fermi levels Ef[nEF]; hist holds
+    // nEF step rows (prefix-summed later) followed by nEF rows added as they are
     int tetra, nEF;
     const double* Ec;
     int64_t ec_stride;
@@ -891,7 +892,9 @@ __device__ inline double tetra_occ(double ef, double a, double b, double c, doub
 // Weight of one band at one k: the mean over the 12 tetrahedra joining the centre to the
 // two triangles (F00, F01, F11), (F00, F10, F11) of each face of the parallelepiped, faces
 // taken as x = side, y = side, z = side for side 0 then 1.  C[ix*4 + iy*2 + iz].
-__device__ inline double tetra_band(double ef, double ec, const double* C, int der) {
+__device__ inline double tetra_band(double ef, double ec, const double* C, int der, int ncorner) {
+    // tetrahedral K-blocks: one tetrahedron, the 4 corners themselves (tetrahedron.py:204-209)
+    if (ncorner == 4) return tetra_occ(ef, C[0], C[1], C[2], C[3], der);
     double occ = 0.0;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
@@ -962,8 +965,8 @@ static __device__ void tetra_accumulate(const Smem& S, const ScanArgs& args, int
             else hi = m;
         }
         const int b = lo, e = S.tlo[b] + (t - S.tpre[b]), g = S.tg[b];
-        const double w = der == -1 ? -tetra_band(args.Ef[e], S.E[b], S.tC + b * 8, 0)
-                                   : tetra_band(args.Ef[e], S.E[b], S.tC + b * 8, der);
+        const double w = der == -1 ? -tetra_band(args.Ef[e], S.E[b], S.tC + b * 8, 0, args.tetra)
+                                   : tetra_band(args.Ef[e], S.E[b], S.tC + b * 8, der, args.tetra);
         const double f = args.weight * w / (S.gb[g + 1] - S.gb[g]);
         for (int c = 0; c < ncomp; ++c)
             unsafeAtomicAdd(hist + (int64_t)(nEF + e) * ncomp + c, f * S.tv[g * 9 + c]);
@@ -1004,7 +1007,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     for (int n = tid; n < nw; n += SC_THREADS) S.E[n] = Eall[k * nw + n];
     for (int r = tid; r < WBMI_NROLES; r += SC_THREADS) S.ch[r] = p.ch[r];
     if (tetra)
-        for (int t = tid; t < 8 * nw; t += SC_THREADS) {
+        for (int t = tid; t < args.tetra * nw; t += SC_THREADS) {
             const int c = t / nw, b = t % nw;
             S.tC[b * 8 + c] = args.Ec[c * args.ec_stride + k * nw + b];
         }
@@ -1013,7 +1016,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
         // band energy range over the centre and the corners (tetrahedron.py:189-193)
         for (int b = tid; b < nw; b += SC_THREADS) {
             double mn = S.E[b], mx = S.E[b];
-            for (int c = 0; c < 8; ++c) {
+            for (int c = 0; c < args.tetra; ++c) {
                 mn = fmin(mn, S.tC[b * 8 + c]);
                 mx = fmax(mx, S.tC[b * 8 + c]);
             }

@@ -281,21 +281,25 @@ class Data_K:
     def corner_energies(self):
         """Band energies [8, nk, nw] (FP64, device) at the corners
         k + ((2i-1), (2j-1), (2l-1)) * dK_fullBZ / 2 of the parallelepiped around each k,
-        corner i*4 + j*2 + l (data_K_R.py:126-141): the Hamiltonian transformed to the
-        shifted grid and diagonalised (FP32 Jacobi + FP64 refinement, FP64-accurate
-        energies) once per corner."""
+        corner i*4 + j*2 + l (data_K_R.py:126-141), or [4, nk, nw] at the vertices of a
+        tetrahedral K-block (99-119): the Hamiltonian transformed to the shifted grid and
+        diagonalised (FP32 Jacobi + FP64 refinement, FP64-accurate energies) once per corner."""
         if self._Ec is None:
             if self.k_list is not None or self.Kpoint is None:
                 raise ValueError("the tetrahedron method needs a grid K-block (Kpoint with dK)")
             self.ensure(set())
             torch = self.torch
             nw, nk = self.num_wann, self.nk
-            half = np.asarray(self.Kpoint.dK_fullBZ, dtype=float) / 2
-            Ec = torch.empty((8, nk, nw), dtype=torch.float64, device=self.ctx.device)
+            if hasattr(self.Kpoint, "vertices_fullBZ"):
+                # tetrahedral K-block: the 4 vertices (data_K_R.py:99-119)
+                shifts = [self.dK + v for v in np.asarray(self.Kpoint.vertices_fullBZ, dtype=float)]
+            else:
+                half = np.asarray(self.Kpoint.dK_fullBZ, dtype=float) / 2
+                shifts = [self.dK + (np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1]) * 2 - 1) * half for c in range(8)]
+            Ec = torch.empty((len(shifts), nk, nw), dtype=torch.float64, device=self.ctx.device)
             U = torch.empty((nk, nw, nw), dtype=torch.complex128, device=self.ctx.device)
-            for c in range(8):
-                sgn = np.array([(c >> 2) & 1, (c >> 1) & 1, c & 1]) * 2 - 1
-                H = self._transform([("Ham", 0)], torch.complex128, WBMI_F64, shift=self.dK + sgn * half)
+            for c, shift in enumerate(shifts):
+                H = self._transform([("Ham", 0)], torch.complex128, WBMI_F64, shift=shift)
                 self.ctx.eigh(H[:, 0], nw * nw, Ec[c], U, WBMI_F64, nw, refine=OA_STEPS)
                 del H
             self._Ec = Ec
