@@ -343,6 +343,15 @@ int wbmi_static_scan_tetra_k(wbmi_ctx* ctx, int nk, int nw, int nch, const doubl
 int wbmi_finalize_tetra_k(wbmi_ctx* ctx, double* hist_dev, int nk, int nEF, int ncomp, double scale,
                           double* out_dev);
 
+/* Per-band table of the inverse effective mass (tabulate.py InvMass,
+ * formula/elementary.py:28-33): out[k][n][b*3+d] = group mean of
+ * Re [W^{bd}_nn + sum_{l outside the group} (V^b_nl D^d_ln - D^d_nl V^b_ln)].
+ *   chV3_host, chW9_host : int32 channel slots of V and Xbar('Ham', 2) (host)
+ *   out_dev : float64 [nk, nw, 9]                                               */
+int wbmi_tabulate_invmass(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                          int prec, const int32_t* chV3_host, const int32_t* chW9_host,
+                          double degen_thresh, int degen_Kramers, double* out_dev);
+
 /* out_dev [nEF, ncomp] = scale * (prefix sums of rows [0, nEF) + rows [nEF, 2 nEF))
  * of a wbmi_static_scan_tetra histogram (static.py:149-155, no fder stencil).
  * The step rows of hist_dev are overwritten with their prefix sums.         */

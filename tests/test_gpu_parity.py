@@ -768,3 +768,30 @@ def test_chiral_k_resolved(wb, use_sym, tetra):
         e = np.abs(d_tab.mean(axis=0) - d_int).max() / np.abs(d_int).max()
         say(f"  {key}: {e:.2e}")
         assert e < 1e-8, key
+
+
+def test_fe_tabulate_invmass(wb):
+    """InvMass tabulator (generalised derivative of the velocity, elementary.py:28-33) on Fe
+    (Wannier90), grid 4^3 / 2^3, vs the oracle's Tabulator with InvMass (pinned by the
+    reference's formula fixture InvMass.npz)."""
+    say("test_fe_tabulate_invmass")
+    tab = wb.calculators.tabulate
+    s = gsys(wb, "fe_w90")
+    ib = [4, 5, 6, 7]
+    calc = tab.TabulatorAll({"im": tab.InvMass(degen_thresh=5e-2)}, ibands=ib)
+    grid = wb.Grid(system=s, NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    r = wb.run(s, grid, {"tab": calc}, use_irred_kpt=False, symmetrize=False, save_files=False, verbose=False)
+    got = r.results["tab"].get_data("im")
+    o = osys("fe_w90")
+    NKdiv, NKFFT, blocks = O.grid_blocks([4, 4, 4], [2, 2, 2])
+    NK = NKdiv * NKFFT
+    ref = np.zeros((int(np.prod(NK)), len(ib), 3, 3))
+    for dK, _ in blocks:
+        d = O.OracleDataK(o, dK=dK, NKFFT=NKFFT)
+        ki = np.rint(d.kpoints_all * NK[None, :]).astype(int) % NK[None, :]
+        idx = (ki[:, 0] * NK[1] + ki[:, 1]) * NK[2] + ki[:, 2]
+        ref[idx] = O.tabulate(d, O.InvMass, ib, degen_thresh=5e-2)
+    ref = ref.reshape(tuple(NK) + (len(ib), 3, 3))
+    e = rel_err(got, ref)
+    say(f"  InvMass: {e:.2e}")
+    assert e < 1e-10

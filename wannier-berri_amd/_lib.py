@@ -40,6 +40,7 @@ SYMBOLS = [
     "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos", "wbmi_spin_hall", "wbmi_spin_hall_work_bytes",
     "wbmi_tabulate", "wbmi_tabulate_diag", "wbmi_shift_current", "wbmi_injection_current",
     "wbmi_static_scan_k", "wbmi_finalize_k", "wbmi_static_scan_tetra_k", "wbmi_finalize_tetra_k",
+    "wbmi_tabulate_invmass",
 ]
 
 
@@ -115,6 +116,7 @@ _PROTOS = {
     "wbmi_spin_hall_work_bytes": (ctypes.c_size_t, [_i, _i]),
     "wbmi_tabulate": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ScanParams), _vp]),
     "wbmi_tabulate_diag": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _d, _i, _vp]),
+    "wbmi_tabulate_invmass": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _d, _i, _vp]),
     "wbmi_shift_current": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(OptcondParams), _d, _vp,
                                 ctypes.c_size_t, _vp]),
     "wbmi_injection_current": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(OptcondParams), _vp,
@@ -359,6 +361,14 @@ class Context:
         nk, nEF, ncomp = hist.shape
         self.check(lib().wbmi_finalize_k(self.h, _ptr(hist), nk, nEF, ncomp, fder, float(dEF), float(scale),
                                          _ptr(out)), "wbmi_finalize_k")
+
+    def tabulate_invmass(self, E, X, prec, chV, chW, degen_thresh, degen_Kramers, out):
+        nk, nw = E.shape
+        v_a, v_p = _host_i32(np.asarray(chV, dtype=np.int32))
+        w_a, w_p = _host_i32(np.asarray(chW, dtype=np.int32))
+        self.check(self._timed("tab_invmass", (nk, nw), lib().wbmi_tabulate_invmass, self.h, nk, nw, X.shape[1],
+                               _ptr(E), _ptr(X), prec, v_p, w_p, float(degen_thresh), int(bool(degen_Kramers)),
+                               _ptr(out)), "wbmi_tabulate_invmass")
 
     def finalize(self, hist, fder, dEF, scale, out):
         nEF, ncomp = hist.shape

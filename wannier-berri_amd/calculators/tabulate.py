@@ -118,6 +118,25 @@ class Spin(_DiagTabulator):
     parity = (-1, 1)
 
 
+class InvMass(Tabulator):
+    r""" second derivative of energy :math:`\partial_a\partial_b E` in units of eV*angstrom^2"""
+    ndim = 2
+
+    def needed_channels(self, system):
+        return {("Ham", 0), ("Ham", 1), ("Ham", 2)}
+
+    def __call__(self, data_K):
+        torch = data_K.torch
+        data_K.ensure(self.needed_channels(data_K.system))
+        v0, w0 = data_K._slots[("Ham", 1)], data_K._slots[("Ham", 2)]
+        out = torch.empty((data_K.nk, data_K.num_wann, 9), dtype=torch.float64, device=data_K.ctx.device)
+        data_K.ctx.tabulate_invmass(data_K.E_dev, data_K.Xk, data_K.prec, [v0 + i for i in range(3)],
+                                    [w0 + i for i in range(9)], self.degen_thresh, self.degen_Kramers, out)
+        ib = np.arange(data_K.num_wann) if self.ibands is None else self.ibands
+        data = out.cpu().numpy()[:, ib].reshape((data_K.nk, len(ib), 3, 3)) * self.constant_factor
+        return KBandResult(data, transformTR=1, transformInv=1)
+
+
 def _not_available(name):
     class _NA(Calculator):
         def __init__(self, *args, **kwargs):
@@ -126,7 +145,6 @@ def _not_available(name):
     return _NA
 
 
-InvMass = _not_available("InvMass")
 SpinBerry = _not_available("SpinBerry")
 
 
@@ -170,5 +188,5 @@ class TabulatorAll(Calculator):
                          save_mode=self.save_mode)
 
 
-__all__ = ["Tabulator", "TabulatorAll", "Energy", "Velocity", "Spin", "BerryCurvature", "DerBerryCurvature",
+__all__ = ["Tabulator", "TabulatorAll", "Energy", "Velocity", "Spin", "InvMass", "BerryCurvature", "DerBerryCurvature",
            "OrbitalMoment", "DerOrbitalMoment"]
