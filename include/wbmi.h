@@ -47,6 +47,8 @@ extern "C" {
 #define WBMI_FORMULA_MORB_HPM 4   /* formula/covariant.py:424-449          */
 #define WBMI_FORMULA_DERMORB 5    /* formula/covariant.py:522-552          */
 #define WBMI_FORMULA_VELHPLUS 6   /* formula/covariant.py:804-808          */
+#define WBMI_FORMULA_DERSPIN 7    /* formula/covariant.py:338-342 (GME spin, sea) */
+#define WBMI_FORMULA_VELSPIN 8    /* formula/covariant.py:811-814 (GME spin, surface) */
 
 /* channel roles: index into wbmi_scan_params.ch (value = channel slot in the
  * per-k stack, -1 = absent).  Two-index roles are row-major over (x, y). */
@@ -60,7 +62,9 @@ extern "C" {
 #define WBMI_ROLE_BD 39   /* Xbar('BB',1)[..,a,d]           9 */
 #define WBMI_ROLE_C 48    /* Xbar('CC',0)[..,a]             3 */
 #define WBMI_ROLE_CD 51   /* Xbar('CC',1)[..,a,d]           9 */
-#define WBMI_NROLES 60
+#define WBMI_ROLE_S 60    /* Xbar('SS',0)[..,b]             3 */
+#define WBMI_ROLE_SD 63   /* Xbar('SS',1)[..,b,d]           9 */
+#define WBMI_NROLES 72
 
 typedef struct wbmi_ctx wbmi_ctx;
 typedef struct wbmi_r2k_plan wbmi_r2k_plan;

@@ -43,6 +43,7 @@ FACTOR_MORB_EVA2_TO_MUB = -(_e * _ang ** 2) * _e / (2 * _hbar) / BOHR_MAGNETON
 FACTOR_GME_ORB = FACTOR_MORB_EVA2_TO_MUB * BOHR_MAGNETON / _ang ** 2
 FACTOR_AHC = -(_e ** 2 / _hbar / _ang)
 FACTOR_NLAHC = _e ** 3 / _hbar ** 2 * 1e-15
+FACTOR_GME_SPIN = -BOHR_MAGNETON / _ang ** 2
 
 # dEig_inv threshold (wannierberri/data_K/data_K.py:292)
 DEIG_THRESHOLD = 1e-7
@@ -900,6 +901,21 @@ def VelHplus(data_K, **kw):
     return FormulaProduct([data_K.covariant('Ham', commader=1), Morb_Hpm(data_K, sign=+1, **kw)])
 
 
+def Spin(data_K, **kw):
+    # formula/covariant.py:331-335: Xbar('SS') as a matrix
+    return data_K.covariant('SS')
+
+
+def DerSpin(data_K, **kw):
+    # formula/covariant.py:338-342: generalised derivative of Xbar('SS'), index order [b, d]
+    return data_K.covariant('SS', gender=1)
+
+
+def VelSpin(data_K, **kw):
+    # formula/covariant.py:811-814
+    return FormulaProduct([data_K.covariant('Ham', commader=1), Spin(data_K)])
+
+
 # ---------------------------------------------------------------------------
 # results
 # ---------------------------------------------------------------------------
@@ -1142,6 +1158,25 @@ class GME_orb_FermiSurf(StaticCalculator):
         H = super().__call__(data_K)
         Om = self.BerryDipole(data_K).mul_array(self.Efermi)
         return H - 2 * Om
+
+
+class GME_spin_FermiSea(StaticCalculator):
+    # static.py:322-335
+    Formula = staticmethod(DerSpin)
+    fder = 0
+    constant_factor_default = FACTOR_GME_SPIN
+
+    def __call__(self, data_K):
+        r = super().__call__(data_K)
+        r.data = r.data.swapaxes(1, 2)
+        return r
+
+
+class GME_spin_FermiSurf(StaticCalculator):
+    # static.py:338-349
+    Formula = staticmethod(VelSpin)
+    fder = 1
+    constant_factor_default = FACTOR_GME_SPIN
 
 
 # ---------------------------------------------------------------------------

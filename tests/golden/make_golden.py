@@ -49,6 +49,12 @@ def results(pairs, out):
     np.savez_compressed(os.path.join(HERE, out), **d)
 
 
+def dat_columns(rel):
+    """Numeric rows of an EnergyResult .dat file: energy, data..., smoothed data..."""
+    rows = [line.split() for line in open(os.path.join(REF, rel)) if line.strip() and not line.startswith("#")]
+    return np.array(rows, dtype=float)
+
+
 def main():
     I = "reference/integrate_files/"
     system("reference/systems/Haldane", "sys_haldane.npz", ["Ham"], legacy=True,
@@ -60,7 +66,11 @@ def main():
     system("reference/systems/GaAs_W90", "sys_gaas_w90.npz", ["Ham", "AA"], legacy=True)
     system("reference/systems/Fe_W90", "sys_fe_w90.npz", ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SR", "SH", "SHR"],
            legacy=True)
-    system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz", ["Ham", "AA", "BB", "CC"], legacy=False)
+    system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz", ["Ham", "AA", "BB", "CC", "SS"],
+           legacy=False)
+    # tellurium from Quantum ESPRESSO (24 WF, 65 R, legacy [m, n, R]); the tests set its point group
+    # from the generators C3z, C2x, TimeReversal (tests/common_systems.py:29,1022-1026)
+    system("data/Te_qe/system", "sys_te_qe.npz", ["Ham", "AA", "BB", "CC", "SS"], legacy=True)
     results({"haldane_ahc": I + "Haldane-ahc_iter-0000.npz",
              "chiral_ahc": I + "Chiral-ahc_iter-0000.npz",
              "chiral_berry_dipole": I + "Chiral-berry_dipole_iter-0000.npz",
@@ -111,6 +121,16 @@ def main():
              "haldane_adpt3_tetra_dos_iter0": I + "Haldane-dos-PythTB-adpt3-tetra-sym_iter-0000.npz",
              "haldane_adpt3_tetra_dos_iter1": I + "Haldane-dos-PythTB-adpt3-tetra-sym_iter-0001.npz",
              }, "results_run.npz")
+    # FermiDiracSmoother(Efermi, T_Kelvin=1200, maxdE=8) on the Chiral AHC and Fermi-sea BCD
+    # (tests/test_run.py:181-193): the .dat files hold the data and the smoothed data
+    np.savez_compressed(os.path.join(HERE, "results_smooth.npz"),
+                        chiral_ahc_dat=dat_columns(I + "Chiral-ahc_iter-0000.dat"),
+                        chiral_berry_dipole_dat=dat_columns(I + "Chiral-berry_dipole_iter-0000.dat"))
+    # Te (QE): tetra=True, irreducible K-points and symmetrisation, NK [3, 3, 4], NKFFT [1, 1, 4],
+    # Efermi = linspace(4, 8, 11) (tests/test_run.py:154-170,1102-1119; common_systems.py:35)
+    results({"te_" + k: I + f"Te_QE-{k}_iter-0000.npz"
+             for k in ["GME_orb_FermiSea", "GME_orb_FermiSurf", "BerryDipole_FermiSea", "berry_dipole",
+                       "dos", "cumdos"]}, "results_te_qe.npz")
     # k-resolved tabulation, Fe (Wannier90), grid 4^3 / 2^3, bands 5 and 6
     # (tests/test_run.py:271-288,313-345)
     with np.load(os.path.join(REF, I + "Fe_W90-tabulate.npz"), allow_pickle=False) as f:
@@ -119,7 +139,7 @@ def main():
     # per-formula gauge-invariant traces (tests/reference/formula/*.npz)
     forms = {}
     for name in ["Dcov", "DerDcov", "DEinv_ln", "InvMass", "Omega", "DerOmega", "Morb_H", "morb",
-                 "DerMorb_upper", "Dermorb_lower", "VelOmega", "Hamiltonian", "Velocity"]:
+                 "DerMorb_upper", "Dermorb_lower", "VelOmega", "Hamiltonian", "Velocity", "Spin", "VelSpin"]:
         with np.load(os.path.join(REF, "reference/formula", name + ".npz"), allow_pickle=False) as f:
             for k in f.files:
                 forms[name + "/" + k] = f[k]

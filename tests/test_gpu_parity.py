@@ -795,3 +795,116 @@ def test_fe_tabulate_invmass(wb):
     e = rel_err(got, ref)
     say(f"  InvMass: {e:.2e}")
     assert e < 1e-10
+
+
+@pytest.fixture(scope="module")
+def bcc32(wb):
+    """The bcc nw = 32 model of the throughput workload (seed 2, |R_i| <= 4: 729 R-vectors,
+    Ham and AA), 2001 Fermi levels over the band range [-22, 22] eV, and the oracle's AHC
+    and Fermi-sea Berry-curvature dipole on two 6x6x4 K-blocks (NK 6x6x8)."""
+    from wannierberri_amd import models
+    s = models.synthetic(num_wann=32, Rmax=4, seed=2)
+    assert s.rvec.nRvec == 729
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA")})
+    Ef = np.linspace(-22, 22, 2001)
+    NK, NKFFT = [6, 6, 8], [6, 6, 4]
+    ro = O.run(o, {"ahc": O.AHC(Ef), "bcd": O.BerryDipole_FermiSea(Ef)}, NK=NK, NKFFT=NKFFT)
+    return s, Ef, NK, NKFFT, ro
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", {"ahc": 1e-8, "bcd": 1e-8}), ("fp32", {"ahc": 1e-6, "bcd": 1e-6})])
+def test_bcc_nw32_workload(wb, bcc32, precision, tol):
+    """The throughput workload's model and Fermi grid (nw 32, 729 R, external terms, 2001
+    levels) on two K-blocks: AHC at 1e-8 (FP64) and 1e-6 (mixed, the metric's bar), BCD sea
+    at 1e-8 / 1e-6, relative to max|oracle|."""
+    say(f"test_bcc_nw32_workload {precision}")
+    st = wb.calculators.static
+    s, Ef, NK, NKFFT, ro = bcc32
+    r = run_gpu(wb, s, {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef)}, NK, NKFFT, precision)
+    for k in ("ahc", "bcd"):
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  {precision} {k}: rel err {e:.2e}")
+        assert e < tol[k], k
+
+
+@pytest.fixture(scope="module")
+def large128(wb):
+    """nw = 128 model of the large workload's recipe (seed 5). The R cube is |R_i| <= 2
+    (125 vectors) instead of 6 (2197): the oracle forms every derivative channel in
+    R-space ([nR, nw, nw, 3, 3] complex128 per second-derivative channel, 5 GB at 2197 R)
+    and needs ~9 s per k-point for the orbital GME at nw = 128, so the grid is 1x2x2.
+    The kernels' code paths depend on nw, not on nR.  4001 Fermi levels in [-1, 1] eV
+    (a window inside the band range [-16.5, 15.4] eV, so most bands form the Fermi-sea
+    group and the cumulative cuts of the GME stay few)."""
+    from wannierberri_amd import models
+    s = models.synthetic(num_wann=128, Rmax=2, seed=5)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA")})
+    Ef = np.linspace(-1, 1, 4001)
+    ki = {"external_terms": False}
+    NK = NKFFT = [1, 2, 2]
+    ro = O.run(o, {"ahc": O.AHC(Ef), "bcd": O.BerryDipole_FermiSea(Ef), "bcds": O.BerryDipole_FermiSurf(Ef),
+                   "gme": O.GME_orb_FermiSea(Ef, kwargs_formula=ki),
+                   "gmes": O.GME_orb_FermiSurf(Ef, kwargs_formula=ki)}, NK=NK, NKFFT=NKFFT)
+    return s, Ef, ki, NK, NKFFT, ro
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_large_nw128(wb, large128, precision, tol):
+    """nw = 128, 4001 Fermi levels: AHC and BCD (sea, surface) with external terms, GME orbital
+    (sea, surface) with internal terms (the model has no BB/CC) against the oracle: the
+    row-chunked Jacobi, the global-operand rotation and the scan's on-the-fly dEinv at nw = 128."""
+    say(f"test_large_nw128 {precision}")
+    st = wb.calculators.static
+    s, Ef, ki, NK, NKFFT, ro = large128
+    calcs = {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef),
+             "bcds": st.BerryDipole_FermiSurf(Efermi=Ef),
+             "gme": st.GME_orb_FermiSea(Efermi=Ef, kwargs_formula=ki),
+             "gmes": st.GME_orb_FermiSurf(Efermi=Ef, kwargs_formula=ki)}
+    r = run_gpu(wb, s, calcs, NK, NKFFT, precision)
+    for k in calcs:
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  {precision} {k}: rel err {e:.2e}")
+        assert e < (1e-6 if (k == "ahc" and precision == "fp32") else tol), k
+
+
+def test_factor_nlahc(wb):
+    """NLAHC_FermiSurf = BerryDipole_FermiSurf x factor_nlahc, and with use_factor=False equal to
+    BerryDipole_FermiSurf (tests/test_run.py:1168-1201, GaAs from Wannier90, default grid 6^3 / 3^3),
+    and the same for the Fermi-sea forms."""
+    say("test_factor_nlahc")
+    st = wb.calculators.static
+    s = gsys(wb, "gaas_w90")
+    Ef = np.linspace(7, 9, 11)
+    calcs = {"bcd": st.BerryDipole_FermiSurf(Efermi=Ef), "nlahc": st.NLAHC_FermiSurf(Efermi=Ef),
+             "nlahc_nf": st.NLAHC_FermiSurf(Efermi=Ef, use_factor=False),
+             "bcds": st.BerryDipole_FermiSea(Efermi=Ef), "nlahcs": st.NLAHC_FermiSea(Efermi=Ef)}
+    r = run_gpu(wb, s, calcs, [6, 6, 6], [3, 3, 3])
+    d = {k: r.results[k].data for k in calcs}
+    prec = max(np.average(np.abs(d["bcd"])) / 1e10, 1e-8)
+    assert np.abs(d["bcd"]).max() > 1e3 * prec
+    assert np.abs(d["nlahc_nf"] - d["bcd"]).max() < prec
+    assert np.abs(d["nlahc"] - d["bcd"] * wb.factors.factor_nlahc).max() < prec
+    assert np.abs(d["nlahcs"] - d["bcds"] * wb.factors.factor_nlahc).max() < 1e-12 * np.abs(d["nlahcs"]).max()
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_fe_gme_spin(wb, precision, tol):
+    """Spin part of the gyrotropic tensor, Fermi sea (DerSpin) and surface (VelSpin), on Fe
+    (Wannier90, spin matrices SS), grid 4^3 / 2^3, against the oracle (whose DerSpin is
+    pinned by central differences of the band spin, its VelSpin and Spin by the
+    reference's formula fixtures)."""
+    say(f"test_fe_gme_spin {precision}")
+    st = wb.calculators.static
+    s = gsys(wb, "fe_w90")
+    Ef = np.linspace(17, 18, 11)
+    NK, NKFFT = [4, 4, 4], [2, 2, 2]
+    calcs = {"sea": st.GME_spin_FermiSea(Efermi=Ef), "surf": st.GME_spin_FermiSurf(Efermi=Ef)}
+    r = run_gpu(wb, s, calcs, NK, NKFFT, precision)
+    ro = O.run(osys("fe_w90"), {"sea": O.GME_spin_FermiSea(Ef), "surf": O.GME_spin_FermiSurf(Ef)}, NK=NK, NKFFT=NKFFT)
+    for k in calcs:
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  {precision} {k}: rel err {e:.2e}")
+        assert np.abs(ro[k].data).max() > 0
+        assert e < tol, k

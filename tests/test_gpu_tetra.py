@@ -172,3 +172,31 @@ def test_chiral_tetragrid(wb, results_run, precision, tol):
         err = rel_err(r.results[k].data, results_run["chiral_tetragrid_" + k])
         say(f"  {k}: {err:.2e}")
         assert err < tol, k
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_te_qe_gme(wb, precision, tol):
+    """Tellurium (QE): tetra=True, point group C3z x C2x x TR, irreducible K-points,
+    NK [3, 3, 4], NKFFT [1, 1, 4] (tests/test_run.py:1102-1119) against the reference's
+    Te_QE fixtures: GME_orb sea and surface, BerryDipole_FermiSea, NLAHC_FermiSea, DOS,
+    cumulative DOS.  Bar 1e-8 x max|ref| (FP64); the mixed form's error is printed."""
+    say(f"test_te_qe_gme {precision}")
+    st = wb.calculators.static
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_te_qe.npz"))
+    s.set_pointgroup(["C3z", "C2x", "TimeReversal"])
+    with np.load(os.path.join(GOLDEN, "results_te_qe.npz"), allow_pickle=False) as f:
+        ref = {k: f[k] for k in f.files}
+    Ef = np.linspace(4, 8, 11)
+    calcs = {"dos": st.DOS(Efermi=Ef, tetra=True),
+             "cumdos": st.CumDOS(Efermi=Ef, tetra=True),
+             "berry_dipole": st.NLAHC_FermiSea(Efermi=Ef, tetra=True),
+             "BerryDipole_FermiSea": st.BerryDipole_FermiSea(Efermi=Ef, tetra=True),
+             "GME_orb_FermiSea": st.GME_orb_FermiSea(Efermi=Ef, tetra=True),
+             "GME_orb_FermiSurf": st.GME_orb_FermiSurf(Efermi=Ef, tetra=True)}
+    grid = wb.Grid(system=s, NK=[3, 3, 4], NKFFT=[1, 1, 4])
+    r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, parameters_K={"precision": precision},
+               save_files=False, verbose=False)
+    for k in calcs:
+        err = rel_err(r.results[k].data, ref["te_" + k])
+        say(f"  {k}: {err:.2e}")
+        assert err < tol, k

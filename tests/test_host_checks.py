@@ -1,0 +1,131 @@
+"""Host-side logic that needs no GPU.
+
+* smoothers (smoother.py:9-175): the reference's Chiral run smooths AHC and the Fermi-sea
+  BCD with FermiDiracSmoother(Efermi, 1200 K, maxdE=8) (tests/test_run.py:181-193); its
+  .dat files hold both the raw and the smoothed columns, written with 7 significant
+  digits.  The package's smoother applied to the raw .npz data must give the .dat
+  smoothed columns to that rounding, and .dat / .max of an EnergyResult must use them;
+* the k-derivative formulas (DerOmega, DerMorb) refuse a system whose Ham(-R) is not
+  Ham(R)^+ (the reference's `random` system: R-vectors without their -R partner), before
+  any GPU work.
+"""
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def wb():
+    import wannierberri_amd
+    return wannierberri_amd
+
+
+@pytest.fixture(scope="module")
+def golden():
+    out = {}
+    for name in ("results_run.npz", "results_smooth.npz"):
+        with np.load(os.path.join(GOLDEN, name), allow_pickle=False) as f:
+            out.update({k: f[k] for k in f.files})
+    return out
+
+
+EF_CHIRAL = np.linspace(-5, 8, 27)
+
+
+@pytest.mark.parametrize("key,ncomp", [("chiral_ahc", 3), ("chiral_berry_dipole", 9)])
+def test_fermi_dirac_smoother_chiral(wb, golden, key, ncomp):
+    dat = golden[key + "_dat"]
+    assert np.allclose(dat[:, 0], EF_CHIRAL, atol=1e-6)
+    raw, smooth_ref = dat[:, 1:1 + ncomp], dat[:, 1 + ncomp:]
+    data = golden[key].reshape(27, ncomp)
+    # the .dat raw columns are the npz data rounded to 7 digits
+    assert np.abs(raw - data).max() <= 1e-6 * np.abs(data).max()
+    sm = wb.smoother.FermiDiracSmoother(EF_CHIRAL, T_Kelvin=1200, maxdE=8)
+    assert sm.NE1 == int(8 * 1200 * 8.617333262e-5 / 0.5)
+    got = sm(data, axis=0)
+    scale = np.abs(smooth_ref).max()
+    assert np.abs(got - smooth_ref).max() <= 1e-6 * scale, np.abs(got - smooth_ref).max() / scale
+    # a smoother that did nothing would fail: the smoothing moves the data well beyond rounding
+    assert np.abs(data - smooth_ref).max() > 1e-3 * scale
+    # EnergyResult: .dataSmooth, the .dat columns and the refinement metric use the smoothed data
+    r = wb.EnergyResult(EF_CHIRAL, golden[key], smoothers=[sm], transformTR=1, transformInv=1)
+    assert np.allclose(r.dataSmooth.reshape(27, ncomp), got)
+    m = r.max
+    assert np.isclose(m[0], np.abs(got).max()) and np.isclose(m[1], np.linalg.norm(got))
+
+
+def test_smoother_dat_roundtrip(wb, golden, tmp_path):
+    sm = wb.smoother.FermiDiracSmoother(EF_CHIRAL, T_Kelvin=1200, maxdE=8)
+    r = wb.EnergyResult(EF_CHIRAL, golden["chiral_ahc"], smoothers=[sm], transformTR=-1, transformInv=1)
+    fn = str(tmp_path / "x.dat")
+    r.savetxt(fn)
+    rows = np.array([line.split() for line in open(fn) if line.strip() and not line.startswith("#")], dtype=float)
+    ref = golden["chiral_ahc_dat"]
+    assert rows.shape == ref.shape
+    assert np.abs(rows - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+def test_smoother_torch_matches_numpy(wb, golden):
+    import torch
+    sm = wb.smoother.GaussianSmoother(EF_CHIRAL, smear=0.7)
+    d = golden["chiral_berry_dipole"]
+    a = sm(d, axis=0)
+    b = sm(torch.from_numpy(d), axis=0).numpy()
+    assert np.abs(a - b).max() < 1e-14 * np.abs(a).max()
+    # the Gaussian's window sums to one far from the ends: a constant stays constant
+    c = sm(np.ones((27, 2)), axis=0)
+    assert np.allclose(c, 1)
+
+
+def test_smoother_mismatch_refused(wb, golden):
+    a = wb.EnergyResult(EF_CHIRAL, golden["chiral_ahc"], smoothers=[wb.smoother.FermiDiracSmoother(EF_CHIRAL, 1200)])
+    b = wb.EnergyResult(EF_CHIRAL, golden["chiral_ahc"])
+    with pytest.raises(RuntimeError):
+        a + b
+    assert isinstance(wb.get_smoother(EF_CHIRAL, 300, "Fermi-Dirac"), wb.smoother.FermiDiracSmoother)
+    assert isinstance(wb.get_smoother(EF_CHIRAL, None), wb.smoother.VoidSmoother)
+
+
+@pytest.mark.parametrize("calc", ["BerryDipole_FermiSea", "GME_orb_FermiSea", "NLAHC_FermiSea"])
+def test_non_hermitian_ham_refused(wb, calc):
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_random.npz"))
+    assert s.hermiticity_error("Ham") > 1
+    Ef = np.linspace(-2, 2, 5)
+    grid = wb.Grid(system=s, NK=[6, 6, 6], NKFFT=[3, 3, 3])
+    c = getattr(wb.calculators.static, calc)(Efermi=Ef)
+    with pytest.raises(NotImplementedError, match="Ham\\(-R\\)"):
+        wb.run(s, grid, {"x": c}, use_irred_kpt=False, save_files=False, verbose=False)
+
+
+def test_hermitian_ham_accepted(wb):
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_te_qe.npz"))
+    assert s.hermiticity_error("Ham") < 1e-14
+    c = wb.calculators.static.BerryDipole_FermiSea(Efermi=np.linspace(4, 8, 11))
+    assert ("Ham", 2) in c.needed_channels(s)
+
+
+def test_tabresult_to_grid(wb):
+    """k-resolved results onto the grid: each grid point holds the mean over the k-points on it
+    (images of one point from several symmetry operations), off-grid points are dropped, grid
+    index (i0 N1 + i1) N2 + i2.  Checked against a direct loop."""
+    from wannierberri_amd.result import KBandResult, TABresult
+    rng = np.random.default_rng(3)
+    grid = np.array([3, 4, 2])
+    pts = np.array([[i, j, l] for i in range(3) for j in range(4) for l in range(2)], dtype=float) / grid
+    kp = np.vstack([pts, pts[::-1] + 1.0, pts[:5] - 2.0, rng.random((4, 3)) * 0.3 + 0.01])
+    nk = kp.shape[0]
+    E = rng.random((nk, 2))
+    V = rng.random((nk, 2, 3))
+    t = TABresult(kp, np.eye(3), {"Energy": KBandResult(E, 1, 1), "V": KBandResult(V, -1, -1)})
+    g = t.to_grid(grid)
+    assert np.allclose(g.kpoints, pts)
+    for i, p in enumerate(pts):
+        d = np.abs(((kp - p) + 0.5) % 1 - 0.5).max(axis=1)
+        sel = np.where(d < 1e-9)[0]
+        assert len(sel) >= 2
+        assert np.allclose(g.results["Energy"].data[i], E[sel].mean(axis=0))
+        assert np.allclose(g.results["V"].data[i], V[sel].mean(axis=0))
+    assert g.get_data("V", component="y").shape == (3, 4, 2, 2)

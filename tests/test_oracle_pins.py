@@ -151,9 +151,10 @@ FORMULAS = {
     # the reference's Velocity copies only the attributes of the covariant V,
     # so its ln block is the full off-diagonal block (formula/covariant.py:321-328)
     "Velocity": lambda d: O.Matrix_ln(d.Xbar("Ham", 1)),
+    "Spin": O.Spin, "VelSpin": O.VelSpin,
 }
 LN_ONLY = {"Dcov", "DerDcov", "DEinv_ln"}
-NN_ONLY = {"Omega", "DerOmega", "Morb_H", "morb", "DerMorb_upper", "Dermorb_lower", "VelOmega"}
+NN_ONLY = {"Omega", "DerOmega", "Morb_H", "morb", "DerMorb_upper", "Dermorb_lower", "VelOmega", "VelSpin"}
 
 
 @pytest.fixture(scope="module")
@@ -201,6 +202,33 @@ def test_formula_traces(datak_fe_sym, name):
             assert np.abs(o).max() < atol, k
         else:
             assert np.abs(r - o).max() / mx < rtol, k
+
+
+def test_derspin_finite_difference():
+    """DerSpin (the generalised derivative of Xbar('SS'), formula/covariant.py:338-342) traced
+    over an isolated band is the k-derivative of that band's spin expectation <n|S^b|n>:
+    central differences in each Cartesian direction on Fe (W90, spin matrices), all bands
+    whose neighbours are at least 0.05 eV away.  A dropped or sign-flipped D term, or the
+    derivative index in the wrong slot, fails this."""
+    o = gsys("fe_sym_osd")
+    k0 = np.array([0.1, 0.2, -0.3])
+    d0 = O.OracleDataK(o, k_list=k0[None])
+    E = d0.E_K[0]
+    gap = np.minimum(np.diff(E, prepend=-np.inf), np.diff(E, append=np.inf))
+    bands = np.where(gap > 0.05)[0]
+    assert len(bands) >= 6
+    ds = O.DerSpin(d0)
+    h = 1e-5
+    binv = np.linalg.inv(o.recip_lattice)
+    for d in range(3):
+        dk = h * binv[d]           # Cartesian step h along axis d, in reduced coordinates
+        sp = O.OracleDataK(o, k_list=np.array([k0 + dk, k0 - dk]))
+        S = np.einsum("knnb->knb", sp.Xbar("SS")).real
+        fd = (S[0] - S[1]) / (2 * h)
+        for n in bands:
+            out = [m for m in range(o.num_wann) if m != n]
+            tr = ds.trace(0, [n], out)     # [b, d]
+            assert np.abs(tr[:, d] - fd[n]).max() < 1e-5 * max(1, np.abs(fd[n]).max()), (n, d)
 
 
 # -- closed forms and invariants ------------------------------------------

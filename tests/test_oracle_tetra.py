@@ -175,3 +175,49 @@ def test_chiral_tetragrid(wb, results_run):
                save_files=False, verbose=False)
     for k in calcs:
         assert rel_err(r.results[k].data, results_run["chiral_tetragrid_" + k]) < 1e-8, k
+
+
+TE_CALCS = {
+    # name: (oracle calculator, TR parity, inversion parity) -- tests/test_run.py:154-170
+    "dos": (lambda Ef: O.DOS(Ef, tetra=True), 1, 1),
+    "cumdos": (lambda Ef: O.CumDOS(Ef, tetra=True), 1, 1),
+    "berry_dipole": (lambda Ef: O.NLAHC_FermiSea(Ef, tetra=True), 1, -1),
+    "BerryDipole_FermiSea": (lambda Ef: O.BerryDipole_FermiSea(Ef, tetra=True), 1, -1),
+    "GME_orb_FermiSea": (lambda Ef: O.GME_orb_FermiSea(Ef, tetra=True), 1, -1),
+    "GME_orb_FermiSurf": (lambda Ef: O.GME_orb_FermiSurf(Ef, tetra=True), 1, -1),
+}
+
+
+def test_te_qe_gme(wb):
+    """Tellurium (QE, 24 WF, 65 R, external terms from AA, BB, CC) with the tetrahedron
+    method, the point group generated by C3z, C2x and time reversal, irreducible
+    K-points and symmetrisation, NK [3, 3, 4], NKFFT [1, 1, 4], Efermi = linspace(4, 8, 11)
+    (tests/test_run.py:1102-1119) against the reference's Te_QE fixtures.
+
+    This pins the composition of the orbital gyrotropic tensor: GME_orb_FermiSea =
+    swapaxes(DerMorb) - 2 E_F BCD_sea and GME_orb_FermiSurf = VelHplus - 2 E_F BCD_surf,
+    both times factor_gme_orb (static.py:266-300), and NLAHC = BCD_sea * factor_nlahc
+    (static.py:490-498): the fixtures of BerryDipole_FermiSea (factor 1) and of
+    berry_dipole (NLAHC_FermiSea) differ by that factor alone."""
+    path = os.path.join(GOLDEN, "sys_te_qe.npz")
+    s = wb.System_R.from_arrays_npz(path)
+    s.set_pointgroup(["C3z", "C2x", "TimeReversal"])
+    assert s.pointgroup.size == 12
+    _OracleBlock.osys = O.load_golden_system(path)
+    with np.load(os.path.join(GOLDEN, "results_te_qe.npz"), allow_pickle=False) as f:
+        ref = {k: f[k] for k in f.files}
+    Ef = np.linspace(4, 8, 11)
+    calcs = {k: _OracleCalc(wb, c(Ef), tr, inv) for k, (c, tr, inv) in TE_CALCS.items()}
+    grid = wb.Grid(system=s, NK=[3, 3, 4], NKFFT=[1, 1, 4])
+    r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, data_k_class=_OracleBlock,
+               save_files=False, verbose=False)
+    for k in calcs:
+        assert np.allclose(ref["te_" + k + "__Efermi"], Ef)
+        assert np.abs(ref["te_" + k]).max() > 0, k
+        assert rel_err(r.results[k].data, ref["te_" + k]) < 1e-8, (k, rel_err(r.results[k].data, ref["te_" + k]))
+    # NLAHC_FermiSea = BerryDipole_FermiSea x factor_nlahc (elements allowed by symmetry;
+    # the forbidden ones are rounding noise in both files)
+    bcd = ref["te_BerryDipole_FermiSea"]
+    big = np.abs(bcd) > 1e-6 * np.abs(bcd).max()
+    assert big.sum() > 20
+    assert np.abs(ref["te_berry_dipole"][big] / bcd[big] / O.FACTOR_NLAHC - 1).max() < 1e-8

@@ -26,10 +26,13 @@ FORMULA_VELOMEGA = 3
 FORMULA_MORB_HPM = 4
 FORMULA_DERMORB = 5
 FORMULA_VELHPLUS = 6
+FORMULA_DERSPIN = 7
+FORMULA_VELSPIN = 8
 
 ROLE_V, ROLE_W, ROLE_A, ROLE_AD, ROLE_O, ROLE_OD = 0, 3, 12, 15, 24, 27
 ROLE_B, ROLE_BD, ROLE_C, ROLE_CD = 36, 39, 48, 51
-NROLES = 60
+ROLE_S, ROLE_SD = 60, 63
+NROLES = 72
 
 # exported symbols (must match include/wbmi.h)
 SYMBOLS = [

@@ -41,7 +41,26 @@ class Calculator:
 # velocity, odd under both, formula/formula.py:127-128, data_K.py:16-46)
 _PARITY = {_lib.FORMULA_IDENTITY: (1, 1), _lib.FORMULA_OMEGA: (-1, 1), _lib.FORMULA_DEROMEGA: (1, -1),
            _lib.FORMULA_VELOMEGA: (1, -1), _lib.FORMULA_MORB_HPM: (-1, 1), _lib.FORMULA_DERMORB: (1, -1),
-           _lib.FORMULA_VELHPLUS: (1, -1)}
+           _lib.FORMULA_VELHPLUS: (1, -1), _lib.FORMULA_DERSPIN: (1, -1), _lib.FORMULA_VELSPIN: (1, -1)}
+# formulas that read only the Hamiltonian's derivatives and the spin, never A, B, C
+_SPIN = (_lib.FORMULA_DERSPIN, _lib.FORMULA_VELSPIN)
+
+
+# relative Hermiticity defect of Ham_R above which the k-derivative formulas refuse a system
+HERMITIAN_TOL = 1e-8
+
+
+def require_hermitian_ham(system, what):
+    """DerOmega and DerMorb on the GPU take dH/dk and d2H/dk2 Hermitian (one product serves
+    a term and its transpose, and W(y, x) is read as conj W(x, y)).  That holds when
+    Ham(-R) = Ham(R)^+ for every R-vector; the reference does not force it
+    (data_K_R.py:84-87), so a system without it is refused rather than given a
+    different answer."""
+    err = system.hermiticity_error("Ham")
+    if err > HERMITIAN_TOL:
+        raise NotImplementedError(
+            f"{what} on the MI355X path needs Ham(-R) = Ham(R)^+ for every R-vector (an R-vector without its -R "
+            f"partner breaks it); this system deviates by {err:.2e} relative to max|Ham|")
 
 
 def _external(kwargs_formula, system):
@@ -101,8 +120,11 @@ class StaticCalculator(Calculator):
         f = self.formula
         if f == _lib.FORMULA_IDENTITY:
             return {("Ham", 0)}
+        if f in _SPIN:
+            return {("Ham", 0), ("Ham", 1), ("SS", 0)} | ({("SS", 1)} if f == _lib.FORMULA_DERSPIN else set())
         need = {("Ham", 0), ("Ham", 1)}
         if f in (_lib.FORMULA_DEROMEGA, _lib.FORMULA_DERMORB):
+            require_hermitian_ham(system, type(self).__name__)
             need.add(("Ham", 2))
         if ext:
             need |= {("AA", 0), ("rotAA", 0)}
@@ -119,7 +141,7 @@ class StaticCalculator(Calculator):
         p.formula = self.formula
         p.sign = int(self.kwargs_formula.get("sign", self.sign))
         p.internal_terms = int(bool(self.kwargs_formula.get("internal_terms", True)))
-        p.external_terms = int(_external(self.kwargs_formula, system))
+        p.external_terms = int(_external(self.kwargs_formula, system)) if self.formula not in _SPIN else 0
         p.fder = self.fder
         p.nEF_extra = self.nEF_extra
         p.EFmin, p.EFmax, p.dEF = float(self.EFmin), float(self.EFmax), float(self.dEF)
@@ -353,5 +375,34 @@ class GME_orb_FermiSea(StaticCalculator):
         return Hplus - 2 * Omega
 
 
+class GME_spin_FermiSea(StaticCalculator):
+    r"""Gyrotropic tensor, spin part, as a Fermi-sea integral (calculators/static.py:322-335)
+
+        | Output: :math:`K^{spin}_{\alpha :\mu} = -\int [dk] \partial_\alpha s_\mu f`"""
+    formula = _lib.FORMULA_DERSPIN
+    ndim = 2
+    fder = 0
+
+    def __init__(self, constant_factor=factors.factor_gme_spin, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+    def __call__(self, data_K):
+        # swap axes as eq. (29) of DOI:10.1038/s41524-021-00498-5 (static.py:331-335)
+        return super().__call__(data_K).swapaxes(1, 2)
+
+
+class GME_spin_FermiSurf(StaticCalculator):
+    r"""Gyrotropic tensor, spin part, as a Fermi-surface integral (calculators/static.py:338-349)
+
+        | Output: :math:`K^{spin}_{\alpha :\mu} = \tau \int [dk] v_\alpha s_\mu f'`"""
+    formula = _lib.FORMULA_VELSPIN
+    ndim = 2
+    fder = 1
+
+    def __init__(self, constant_factor=factors.factor_gme_spin, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
 __all__ = ["Calculator", "StaticCalculator", "DOS", "CumDOS", "AHC", "BerryDipole_FermiSurf", "NLAHC_FermiSurf",
-           "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "GME_orb_FermiSurf", "GME_orb_FermiSea", "VoidResult"]
+           "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "GME_orb_FermiSurf", "GME_orb_FermiSea",
+           "GME_spin_FermiSea", "GME_spin_FermiSurf", "VoidResult"]

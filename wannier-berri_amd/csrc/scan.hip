@@ -34,6 +34,12 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
                        (rc = need(WBMI_ROLE_C, 3)) || (rc = need(WBMI_ROLE_CD, 9))))
                 return rc;
             break;
+        case WBMI_FORMULA_DERSPIN:
+            if ((rc = need(WBMI_ROLE_V, 3)) || (rc = need(WBMI_ROLE_S, 3)) || (rc = need(WBMI_ROLE_SD, 9))) return rc;
+            break;
+        case WBMI_FORMULA_VELSPIN:
+            if ((rc = need(WBMI_ROLE_V, 3)) || (rc = need(WBMI_ROLE_S, 3))) return rc;
+            break;
         case WBMI_FORMULA_MORB_HPM:
         case WBMI_FORMULA_VELHPLUS:
             if ((rc = need(WBMI_ROLE_V, 3))) return rc;

@@ -210,6 +210,8 @@ struct KCtx {
     __device__ inline cplx<double> C(int a, int x, int y) const { return get(WBMI_ROLE_C + a, x, y); }
     __device__ inline cplx<double> Bd(int a, int d, int x, int y) const { return get(WBMI_ROLE_BD + 3 * a + d, x, y); }
     __device__ inline cplx<double> Cd(int c, int d, int x, int y) const { return get(WBMI_ROLE_CD + 3 * c + d, x, y); }
+    __device__ inline cplx<double> S(int b, int x, int y) const { return get(WBMI_ROLE_S + b, x, y); }
+    __device__ inline cplx<double> Sd(int b, int d, int x, int y) const { return get(WBMI_ROLE_SD + 3 * b + d, x, y); }
     // X o On: elements between bands of the same group
     __device__ inline cplx<double> Ain(int a, int x, int y) const { return lab[x] == lab[y] ? A(a, x, y) : mk(0.0, 0.0); }
     __device__ inline cplx<double> Vin(int a, int x, int y) const { return lab[x] == lab[y] ? V(a, x, y) : mk(0.0, 0.0); }
@@ -822,6 +824,28 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
             }
             S.acc[t] = mk(v.re, 0.0);
         }
+    } else if (F == WBMI_FORMULA_DERSPIN) {
+        // generalised derivative of the spin (formula/formula.py:95-107 with A = Xbar('SS'),
+        // dA = Xbar('SS', 1)): Re [S'^{bd}_nn - sum_{l off} D^d_nl S^b_ln + sum_{l off} S^b_nl D^d_ln]
+        for (int t = tid; t < nw * 9; t += SC_THREADS) {
+            const int n = t / 9, b = (t % 9) / 3, d = t % 3;
+            if (!S.act[n]) continue;
+            cplx<double> v = K.Sd(b, d, n, n);
+            for (int l = 0; l < nw; ++l) {
+                v -= K.Dp(d, n, l) * K.S(b, l, n);
+                v += K.S(b, n, l) * K.Dp(d, l, n);
+            }
+            S.acc[t] = mk(v.re, 0.0);
+        }
+    } else if (F == WBMI_FORMULA_VELSPIN) {
+        // Re sum_{n' in G} V^a_nn' S^b_n'n (formula/formula.py:121-148, covariant.py:811-814)
+        for (int t = tid; t < nw * 9; t += SC_THREADS) {
+            const int n = t / 9, a = (t % 9) / 3, b = t % 3;
+            if (!S.act[n]) continue;
+            cplx<double> v = mk(0.0, 0.0);
+            for (int m = S.gs[n]; m < S.ge[n]; ++m) v += K.V(a, n, m) * K.S(b, m, n);
+            S.acc[t] = mk(v.re, 0.0);
+        }
     } else if (F == WBMI_FORMULA_DEROMEGA) {
         eval_deromega(S, K, in, ex, false, 0);
     } else if (F == WBMI_FORMULA_DERMORB) {
@@ -932,7 +956,7 @@ static __device__ void tetra_accumulate(const Smem& S, const ScanArgs& args, int
     for (int b = tid; b < nw; b += SC_THREADS) {
         const int g = S.tg[b];
         int lo = 0, hi = 0;
-        if (S.gin[g]) {
+        if (g >= 0 && S.gin[g]) {  // g < 0: a band in no group (the odd last band under degen_Kramers)
             lo = ef_bound(args.Ef, nEF, S.tEmn[b], false);
             hi = ef_bound(args.Ef, nEF, S.tEmx[b] + TETRA_PAD, true);
             const double f = args.weight / (S.gb[g + 1] - S.gb[g]);
@@ -1054,7 +1078,8 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     constexpr int ncomp = ncomp_of<F>();
     constexpr bool additive = F == WBMI_FORMULA_IDENTITY || F == WBMI_FORMULA_OMEGA ||
                               F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
-                              F == WBMI_FORMULA_VELHPLUS;
+                              F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERSPIN ||
+                              F == WBMI_FORMULA_VELSPIN;
 
     // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186)
     if (tid == 0) {
@@ -1070,6 +1095,9 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             nb = m;
         }
         const int ng = nb - 1;
+        // degen_Kramers keeps only even borders, so with odd nw the last band is in no group
+        // (grid/tetrahedron.py:132-136): it is an "out" band of every group and adds nothing itself
+        for (int i = 0; i < nw; ++i) S.tg[i] = -1;
         int first_in = -1, last_in = 0;
         for (int g = 0; g < ng; ++g) {
             int a = S.gb[g], b = S.gb[g + 1];
@@ -1137,7 +1165,15 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
 
     if (additive || tab) {
         // main partition: every group at once (each group's trace with inn = the group,
-        // out = the rest: the additive scan and the per-band table of any formula)
+        // out = the rest: the additive scan and the per-band table of any formula); a band in
+        // no group (label -1) is an out band of every group
+        for (int n = tid; n < nw; n += SC_THREADS) {
+            S.lab[n] = -1;
+            S.gs[n] = n;
+            S.ge[n] = n;
+            S.act[n] = 0;
+        }
+        __syncthreads();
         for (int g = 0; g < ng; ++g)
             for (int n = S.gb[g] + tid; n < S.gb[g + 1]; n += SC_THREADS) {
                 S.lab[n] = g;
@@ -1277,6 +1313,8 @@ static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
         case WBMI_FORMULA_MORB_HPM: return launch_scan<T, WBMI_FORMULA_MORB_HPM, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_VELHPLUS: return launch_scan<T, WBMI_FORMULA_VELHPLUS, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_DERMORB: return launch_scan<T, WBMI_FORMULA_DERMORB, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_DERSPIN: return launch_scan<T, WBMI_FORMULA_DERSPIN, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_VELSPIN: return launch_scan<T, WBMI_FORMULA_VELSPIN, MODE>(ctx, nk, a, E, X, hist);
         default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
     }
 }

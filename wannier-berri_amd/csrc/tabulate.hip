@@ -42,6 +42,9 @@ __global__ void __launch_bounds__(TD_THREADS) k_tab_diag(int nw, int nch, int c0
                 if (sGb[i] % 2 == 0) sGb[m++] = sGb[i];
             nb = m;
         }
+        // a band in no group (the odd last band under degen_Kramers) keeps label -1: an out band
+        // of every group, tabulated as 0 (tabulate.py:20-48 fills only the groups' bands)
+        for (int n = 0; n < nw; ++n) sLab[n] = -1;
         for (int g = 0; g + 1 < nb; ++g)
             for (int n = sGb[g]; n < sGb[g + 1]; ++n) sLab[n] = g;
     }
@@ -51,6 +54,10 @@ __global__ void __launch_bounds__(TD_THREADS) k_tab_diag(int nw, int nch, int c0
     const int ch[3] = {c0, c1, c2};
     for (int t = tid; t < 3 * nw; t += TD_THREADS) {
         const int n = t / 3, c = t % 3, g = sLab[n];
+        if (g < 0) {
+            out[(k * nw + n) * 3 + c] = 0.0;
+            continue;
+        }
         double s = 0.0;
         for (int m = sGb[g]; m < sGb[g + 1]; ++m) s += (double)Xk[ch[c] * nw2 + (int64_t)m * nw + m].re;
         out[(k * nw + n) * 3 + c] = s / (sGb[g + 1] - sGb[g]);
@@ -109,6 +116,9 @@ __global__ void __launch_bounds__(TD_THREADS) k_tab_invmass(int nw, int nch, wbm
                 if (sGb[i] % 2 == 0) sGb[m++] = sGb[i];
             nb = m;
         }
+        // a band in no group (the odd last band under degen_Kramers) keeps label -1: an out band
+        // of every group, tabulated as 0 (tabulate.py:20-48 fills only the groups' bands)
+        for (int n = 0; n < nw; ++n) sLab[n] = -1;
         for (int g = 0; g + 1 < nb; ++g)
             for (int n = sGb[g]; n < sGb[g + 1]; ++n) sLab[n] = g;
     }
@@ -136,6 +146,10 @@ __global__ void __launch_bounds__(TD_THREADS) k_tab_invmass(int nw, int nch, wbm
     __syncthreads();
     for (int t = tid; t < 9 * nw; t += TD_THREADS) {
         const int n = t / 9, c = t % 9, g = sLab[n];
+        if (g < 0) {
+            out[(k * nw + n) * 9 + c] = 0.0;
+            continue;
+        }
         double s = 0.0;
         for (int m = sGb[g]; m < sGb[g + 1]; ++m) s += sv[m * 9 + c];
         out[(k * nw + n) * 9 + c] = s / (sGb[g + 1] - sGb[g]);

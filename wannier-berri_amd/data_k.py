@@ -28,7 +28,7 @@ from ._lib import WBMI_F32, WBMI_F64
 CHANNEL_NCOMP = {("Ham", 0): 1, ("Ham", 1): 3, ("Ham", 2): 9,
                  ("AA", 0): 3, ("AA", 1): 9, ("rotAA", 0): 3, ("rotAA", 1): 9,
                  ("BB", 0): 3, ("BB", 1): 9, ("CC", 0): 3, ("CC", 1): 9,
-                 ("SS", 0): 3, ("SA", 0): 9, ("SHA", 0): 9, ("SH", 0): 3, ("SR", 0): 9, ("SHR", 0): 9}
+                 ("SS", 0): 3, ("SS", 1): 9, ("SA", 0): 9, ("SHA", 0): 9, ("SH", 0): 3, ("SR", 0): 9, ("SHR", 0): 9}
 CHANNEL_ORDER = list(CHANNEL_NCOMP)
 # matrices forced Hermitian after R->k (data_K/data_K_R.py:87)
 HERMITIAN = {"AA", "rotAA", "SS"}
@@ -344,6 +344,8 @@ class Data_K:
         put(_lib.ROLE_BD, ("BB", 1), 9)
         put(_lib.ROLE_C, ("CC", 0), 3)
         put(_lib.ROLE_CD, ("CC", 1), 9)
+        put(_lib.ROLE_S, ("SS", 0), 3)
+        put(_lib.ROLE_SD, ("SS", 1), 9)
         return ch
 
     def static_scan(self, params, hist, weight=1.0):

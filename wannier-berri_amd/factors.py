@@ -12,6 +12,7 @@ bohr = physical_constants['Bohr radius'][0] / angstrom
 eV_au = physical_constants['electron volt-hartree relationship'][0]
 factor_morb_evA2_to_muB = -(elementary_charge * angstrom ** 2) * elementary_charge / (2 * hbar) / bohr_magneton
 factor_gme_orb = factor_morb_evA2_to_muB * bohr_magneton / angstrom ** 2
+factor_gme_spin = -bohr_magneton / angstrom ** 2   # factors.py:22
 factor_ahc = -(elementary_charge ** 2 / hbar / angstrom)
 factor_opt = -factor_ahc   # factors.py:36
 factor_shc = -factor_ahc   # factors.py:37

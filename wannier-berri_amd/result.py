@@ -9,6 +9,8 @@ import os
 
 import numpy as np
 
+from .smoother import VoidSmoother
+
 
 def _is_tensor(x):
     return hasattr(x, "device") and hasattr(x, "cpu") and not isinstance(x, np.ndarray)
@@ -65,11 +67,20 @@ class EnergyResult(Result):
         self.rank = nd - self.N_energies if rank is None else rank
         for i in range(self.N_energies):
             assert self.Energies[i].shape[0] == data.shape[i], "Energy grid does not match the data"
-        self.smoothers = smoothers
+        self.set_smoother(smoothers)
         self.transformTR = transformTR
         self.transformInv = transformInv
         self.comment = comment
         self.save_mode = set(s for s in ("bin", "txt") if s in save_mode) if isinstance(save_mode, str) else set(save_mode)
+
+    def set_smoother(self, smoothers):
+        """One smoother per energy axis, None meaning none (energyresult.py:116-122)."""
+        if smoothers is None:
+            smoothers = (None,) * self.N_energies
+        if not isinstance(smoothers, (list, tuple)):
+            smoothers = [smoothers]
+        assert len(smoothers) == self.N_energies
+        self.smoothers = [VoidSmoother() if sm is None else sm for sm in smoothers]
 
     # -- data access ----------------------------------------------------------
     @property
@@ -109,6 +120,9 @@ class EnergyResult(Result):
         for i in range(self.N_energies):
             if np.linalg.norm(self.Energies[i] - other.Energies[i]) > 1e-8:
                 raise RuntimeError(f"Adding results with different energies {i} ({self.E_titles[i]}) - not allowed")
+            if self.smoothers[i] != other.smoothers[i]:
+                raise RuntimeError(f"Adding results with different smoothers [{i}]: {self.smoothers[i]} and "
+                                   f"{other.smoothers[i]}")
         a, b = self._data, other._data
         if _is_tensor(a) != _is_tensor(b):
             a, b = self.data, other.data
@@ -152,7 +166,13 @@ class EnergyResult(Result):
     # -- refinement criteria (energyresult.py:250-264) ------------------------
     @property
     def dataSmooth(self):
-        return self.data
+        """The data smoothed along the first energy axis by its smoother (energyresult.py:124-129:
+        the reference's loop over the axes leaves the first axis' smoothing of the raw data)."""
+        if getattr(self, "_np_smooth", None) is None or self._np_smooth[0] is not self._data:
+            sm = self.smoothers[0](self._data, axis=0)
+            sm = sm.detach().cpu().numpy() if _is_tensor(sm) else np.asarray(sm)
+            self._np_smooth = (self._data, sm)
+        return self._np_smooth[1]
 
     @property
     def max(self):
@@ -172,8 +192,9 @@ class EnergyResult(Result):
 
     def savetxt(self, name):
         """One line per point of the energy grid(s): the energies, then the data and the
-        smoothed data (here the same), complex values as `re im` (energyresult.py:202-225)."""
+        smoothed data, complex values as `re im` (energyresult.py:202-225)."""
         data = self.data
+        smooth = self.dataSmooth
         cpx = np.iscomplexobj(data)
         frmt = "{0:^31s}" if cpx else "{0:^15s}"
         head = "".join("#### " + s + "\n" for s in self.comment.split("\n"))
@@ -182,7 +203,7 @@ class EnergyResult(Result):
             "    ".join(frmt.format(b) for b in labels * 2) + "\n"
         lines = []
         for idx in np.ndindex(*data.shape[:self.N_energies]):
-            row = list(data[idx].reshape(-1)) * 2
+            row = list(data[idx].reshape(-1)) + list(smooth[idx].reshape(-1))
             Es = "    ".join(f"{self.Energies[i][j]:15.6e}" for i, j in enumerate(idx))
             vals = "    ".join((f"{x.real:15.6e} {x.imag:15.6e}" if cpx else f"{x:15.6e}") for x in row)
             lines.append(Es + "    " + vals)
@@ -261,10 +282,18 @@ class KBandResult(Result):
         fI = self.transformInv if self.transformInv is not None else 1
         return KBandResult(sym.transform_tensor(self.data, self.rank, fTR, fI), self.transformTR, self.transformInv)
 
-    def to_grid(self, k_map):
-        """Grid point i gets the mean over the k-points k_map[i] (kbandresult.py)."""
-        return KBandResult(np.array([self.data[km].mean(axis=0) for km in k_map]), self.transformTR,
-                           self.transformInv)
+    def to_grid(self, index, npoints):
+        """Segment mean: grid point i gets the mean of the rows whose `index` is i (rows with
+        index -1 are dropped; a point no row lands on is NaN, the mean of nothing)."""
+        keep = index >= 0
+        idx = index[keep]
+        rows = self.data[keep].reshape(idx.shape[0], -1)
+        count = np.bincount(idx, minlength=npoints).astype(float)
+        total = np.zeros((npoints, rows.shape[1]), dtype=rows.dtype)
+        np.add.at(total, idx, rows)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            mean = total / count[:, None]
+        return KBandResult(mean.reshape((npoints,) + self.data.shape[1:]), self.transformTR, self.transformInv)
 
     def get_component(self, component=None):
         """'x', 'xy', 'trace', ... of the Cartesian indices (kbandresult.py)."""
@@ -327,19 +356,18 @@ class TABresult(Result):
         pass   # as the reference (tabresult.py:56-60)
 
     def to_grid(self, grid, order="C"):
-        """Results on the full grid, k index (i0*N1 + i1)*N2 + i2 (tabresult.py:134-170)."""
+        """Results on the full grid (the format of tabresult.py:134-170): grid point
+        (i0, i1, i2) at k = i / grid, listed in `order`, holding the mean of the k-points
+        (symmetry images included) that fall on it within 1e-5; other k-points are dropped."""
         grid = np.array(grid, dtype=int)
-        g1 = [np.linspace(0., 1., g, False) for g in grid]
-        k_new = np.array(np.meshgrid(g1[0], g1[1], g1[2], indexing="ij")).reshape((3, -1), order=order).T
-        ki = np.rint(self.kpoints * grid[None, :]).astype(int)
-        on_grid = np.all(abs(ki / grid[None, :] - self.kpoints) < 1e-5, axis=1)
-        ki = ki % grid[None, :]
-        ind = ki[:, 2] + grid[2] * (ki[:, 1] + grid[1] * ki[:, 0])
-        k_map = [[] for _ in range(int(np.prod(grid)))]
-        for ik in range(len(self.kpoints)):
-            if on_grid[ik]:
-                k_map[ind[ik]].append(ik)
-        res = TABresult(k_new, self.recip_lattice, {r: v.to_grid(k_map) for r, v in self.results.items()},
+        npoints = int(np.prod(grid))
+        pos = self.kpoints * grid[None, :]
+        nearest = np.rint(pos)
+        on_grid = np.all(np.abs(pos - nearest) < 1e-5 * grid[None, :], axis=1)
+        flat = np.ravel_multi_index(tuple((nearest.astype(int) % grid[None, :]).T), tuple(grid), order=order)
+        flat = np.where(on_grid, flat, -1)
+        k_new = np.stack(np.unravel_index(np.arange(npoints), tuple(grid), order=order), axis=1) / grid[None, :]
+        res = TABresult(k_new, self.recip_lattice, {r: v.to_grid(flat, npoints) for r, v in self.results.items()},
                         self.mode, self.save_mode)
         res.grid = grid
         res.gridorder = order

@@ -103,6 +103,17 @@ class System_R:
             raise RuntimeError(f"setting {key} for the second time without explicit permission")
         self._XX_R[key] = value
         self._device_cache.clear()
+        self._herm_err = {}
+
+    def hermiticity_error(self, key="Ham"):
+        """max |X(R) - X(-R)^+| / max |X|, an R-vector without its -R partner counting
+        with its whole X(R) (the pairing of fourier/rvectors.py:424-430)."""
+        cache = self.__dict__.setdefault("_herm_err", {})
+        if key not in cache:
+            X = self.get_R_mat(key)
+            scale = np.abs(X).max()
+            cache[key] = float(np.abs(X - self.conj_XX_R(X)).max() / scale) if scale > 0 else 0.
+        return cache[key]
 
     def conj_XX_R(self, X):
         """X(-R)^dagger (fourier/rvectors.py:424-430)."""
