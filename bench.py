@@ -61,31 +61,58 @@ def calculators(wb):
 
 # -- CPU baseline: the FP64 oracle, one worker process per host core ------------
 
-def _cpu_worker(args):
-    dK, nkfft = args
+def _cpu_worker(rank, dKs, nkfft, barrier, queue):
+    """One process: imports, model and oracle system built first, then (after all workers are
+    ready) the timed compute of its K-blocks: R->k, eigh, rotation, AHC and BCD traces, scan."""
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
     from oracle import wb_oracle as O
     s = make_system()
     o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
                        {"Ham": s.get_R_mat("Ham"), "AA": s.get_R_mat("AA")})
     Ef = np.linspace(*EFERMI)
-    t = time.time()
-    d = O.OracleDataK(o, dK=dK, NKFFT=nkfft)
-    ahc = O.AHC(Ef)(d).data
-    bcd = O.BerryDipole_FermiSea(Ef)(d).data
-    return time.time() - t, d.nk, ahc, bcd
+    barrier.wait()
+    t0 = time.time()
+    out = []
+    for dK in dKs:
+        d = O.OracleDataK(o, dK=dK, NKFFT=nkfft)
+        out.append((d.nk, O.AHC(Ef)(d).data, O.BerryDipole_FermiSea(Ef)(d).data))
+    queue.put((rank, t0, time.time(), out))
 
 
-def cpu_baseline(nproc, nkfft, blocks_per_proc=1):
+def host_cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(nproc=len(os.sched_getaffinity(0)), cpu_count=os.cpu_count(), model=model)
+
+
+def cpu_baseline(nproc, nkfft, blocks_per_proc=2):
+    """The oracle on `nproc` processes at once, `blocks_per_proc` K-blocks each.  The rate is the
+    total k-points over the window from the first worker's start to the last one's end, after
+    every worker has imported its modules and built its model (a barrier)."""
     rng = np.random.default_rng(7)
     dKs = [rng.uniform(0, 1, 3) / np.array(nkfft) for _ in range(nproc * blocks_per_proc)]
     ctx = mp.get_context("spawn")
-    t = time.time()
-    with ctx.Pool(nproc) as pool:
-        out = pool.map(_cpu_worker, [(dk, nkfft) for dk in dKs])
-    wall = time.time() - t
-    nk = sum(o[1] for o in out)
-    return dict(wall=wall, nk=nk, kps=nk / wall, dKs=dKs, ahc=[o[2] for o in out], bcd=[o[3] for o in out])
+    barrier, queue = ctx.Barrier(nproc), ctx.Queue()
+    procs = [ctx.Process(target=_cpu_worker, args=(r, dKs[r::nproc], nkfft, barrier, queue)) for r in range(nproc)]
+    for p in procs:
+        p.start()
+    got = sorted(queue.get() for _ in procs)
+    for p in procs:
+        p.join()
+    t_start, t_end = min(g[1] for g in got), max(g[2] for g in got)
+    blocks = [b for g in got for b in g[3]]
+    order = [i for r in range(nproc) for i in range(r, len(dKs), nproc)]
+    nk = sum(b[0] for b in blocks)
+    wall = t_end - t_start
+    return dict(wall=wall, nk=nk, kps=nk / wall, dKs=[dKs[i] for i in order],
+                ahc=[b[1] for b in blocks], bcd=[b[2] for b in blocks])
 
 
 # -- GPU -------------------------------------------------------------------------
@@ -288,9 +315,11 @@ def main():
     out_err = {}
     cb = None
     if not args.no_cpu:
-        nproc = args.cpu_procs or min(16, os.cpu_count() or 1)
+        hw = host_cpu_info()
+        # one process per core of this process's CPU set, at most 16: a one-GPU share of the host
+        nproc = args.cpu_procs or min(16, hw["nproc"])
         cnk = [int(x) for x in args.cpu_nkfft.split(",")]
-        log(f"[bench] cpu baseline: oracle on {nproc} processes, one {cnk} K-block each")
+        log(f"[bench] cpu baseline: oracle on {nproc} processes ({hw}), two {cnk} K-blocks each")
         cb = cpu_baseline(nproc, cnk)
         # GPU on the same K-blocks: error of each GPU form vs the FP64 oracle
         g = wb.Grid(system=system, NKFFT=cnk, NKdiv=[1, 1, 1])
@@ -338,9 +367,11 @@ def main():
     if cb is not None:
         out["rel_err_vs_oracle_on_cpu_sample"] = {("mixed" if p == "fp32" else p): e for p, e in out_err.items()}
         out["cpu_baseline"] = {"value": cb["kps"], "unit": "k-points/s", "cores": nproc, "kind": "port",
-                               "sample": f"{nproc} K-blocks of {cnk[0]}x{cnk[1]}x{cnk[2]} k-points (same model, "
-                                         f"AHC+BCD, 2001 levels), FP64 NumPy oracle, one process per core, "
-                                         f"OPENBLAS_NUM_THREADS=1, wall {cb['wall']:.1f} s"}
+                               "sample": f"{len(cb['dKs'])} K-blocks of {cnk[0]}x{cnk[1]}x{cnk[2]} k-points ({cb['nk']} "
+                                         f"k, same model, AHC+BCD, 2001 levels), FP64 NumPy oracle, one process per "
+                                         f"core, OPENBLAS_NUM_THREADS=1, compute window {cb['wall']:.1f} s after "
+                                         f"imports and model build; host: {hw['model']}, {hw['nproc']} CPUs in "
+                                         f"this process's set (os.cpu_count {hw['cpu_count']})"}
         out["gpu_over_cpu"] = m["kps"] / cb["kps"]
     print(json.dumps(out), flush=True)
 

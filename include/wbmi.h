@@ -38,6 +38,9 @@ extern "C" {
 
 #define WBMI_F64 0
 #define WBMI_F32 1
+/* wbmi_rotate only: complex64 storage, products on the fp16 matrix cores with operands
+ * split in two halves (about FP32 accuracy) for nw <= 32; WBMI_F32 above */
+#define WBMI_F32_SPLIT 2
 
 /* formula ids for wbmi_static_scan */
 #define WBMI_FORMULA_IDENTITY 0   /* formula/covariant.py:10-22            */

@@ -188,6 +188,39 @@ def test_rotate(wb, ctx, nw, prec):
             assert (np.abs(got[:, c] - ref) / np.abs(ref)).max() < 1.2e-7, c
 
 
+@pytest.mark.parametrize("nw", [5, 18, 31, 32])
+def test_rotate_split(wb, ctx, nw):
+    """Split-fp16 matrix-core rotation (WBMI_F32_SPLIT, complex64 channels): normwise error of a
+    product accumulated in FP32, below 2e-6 x max|X| (a product in plain fp16 would be ~1e-3, a wrong
+    operand layout O(1)), with Hermitisation and skip flags; channels of very different magnitude
+    (1e-6 .. 1e4) each keep that relative accuracy (per-channel scaling)."""
+    import torch
+    say(f"test_rotate_split nw={nw}")
+    rng = np.random.default_rng(11)
+    nk, nch = 9, 6
+    X = rng.standard_normal((nk, nch, nw, nw)) + 1j * rng.standard_normal((nk, nch, nw, nw))
+    X *= np.array([1.0, 1e4, 1e-6, 3.0, 1.0, 0.5])[None, :, None, None]
+    U = np.linalg.qr(rng.standard_normal((nk, nw, nw)) + 1j * rng.standard_normal((nk, nw, nw)))[0]
+    X = X.astype(np.complex64).astype(np.complex128)
+    U = U.astype(np.complex64).astype(np.complex128)
+    herm = np.array([0, 1, 0, 1, 0, 1], dtype=np.int32)
+    skip = np.array([0, 0, 0, 0, 1, 0], dtype=np.int32)
+    Xt = torch.from_numpy(X.copy()).to(torch.complex64).cuda()
+    ctx.rotate(torch.from_numpy(U).to(torch.complex64).cuda(), Xt, herm, skip, wb._lib.WBMI_F32_SPLIT)
+    got = Xt.cpu().numpy().astype(np.complex128)
+    for c in range(nch):
+        x = X[:, c]
+        if skip[c]:
+            assert np.array_equal(got[:, c], x)
+            continue
+        if herm[c]:
+            x = 0.5 * (x + x.conj().swapaxes(1, 2))
+        ref = np.einsum("kba,kbc,kcd->kad", U.conj(), x, U)
+        for k in range(nk):
+            e = np.abs(got[k, c] - ref[k]).max() / np.abs(x[k]).max()
+            assert e < 2e-6, (c, k, e)
+
+
 # -- end to end against the reference fixtures ----------------------------------
 
 def test_haldane_ahc(wb, results_run):

@@ -18,6 +18,7 @@ WBMI_ERR_HIP = 2
 WBMI_ERR_UNSUPPORTED = 3
 WBMI_F64 = 0
 WBMI_F32 = 1
+WBMI_F32_SPLIT = 2
 
 FORMULA_IDENTITY = 0
 FORMULA_OMEGA = 1
@@ -260,7 +261,8 @@ class Context:
         nrot = int(nch - (0 if s_a is None else np.count_nonzero(s_a)))
         wbytes = int(lib().wbmi_rotate_work_bytes(nk, nw, prec))
         work = self._workspace(wbytes)
-        self.check(self._timed("rotate", (nk, nw, nrot), lib().wbmi_rotate, self.h, nk, nw, nch, _ptr(U), _ptr(X),
+        stage = "rotate_split" if prec == WBMI_F32_SPLIT else "rotate"
+        self.check(self._timed(stage, (nk, nw, nrot), lib().wbmi_rotate, self.h, nk, nw, nch, _ptr(U), _ptr(X),
                                h_p, s_p, prec, _ptr(work), wbytes), "wbmi_rotate")
 
     def static_scan(self, E, X, prec, params, weight, hist):

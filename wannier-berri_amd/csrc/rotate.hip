@@ -313,6 +313,181 @@ __global__ void __launch_bounds__(ROTG_THREADS) k_rotate_g(int nw, int nch, cons
     }
 }
 
+// Split-half rotation (complex64 storage, nw <= 32): Xbar = U^+ X U with both products
+// on the fp16 matrix cores (v_mfma_f32_16x16x16_f16).  Each real operand is scaled by a
+// power of two and split into two fp16 numbers, x = hi + lo (hi = fp16(x), lo = fp16(x - hi):
+// 22 significant bits), and a product is Ah Bh + Ah Bl + Al Bh accumulated in FP32; the
+// dropped Al Bl is 2^-22 of the terms.  So a product carries FP32-accumulation rounding,
+// eps32 sqrt(nw) |X| |U| per element, where the FP64-arithmetic path carries only the
+// rounding of the stored result.  Used for every channel but the velocity dH/dk (whose
+// elements between close bands enter the Berry curvature through 1/(E_n - E_m)^2).
+//
+// One wave per (k, channel), no LDS and no barriers: X is loaded from HBM straight into
+// the A-operand registers, U (split once per k) serves as the B operand of T = X U and,
+// conjugated, as the A operand of U^+ T, and the accumulators of T are the B operand of
+// the second product.  That last step relies on the output layout of the 16 x 16 MFMA:
+// lane l, register v holds D[RH_ROW(l / 16, v)][l % 16], so the k index of every operand
+// is enumerated in that same order (any order common to A and B gives the same sum).
+typedef _Float16 wbmi_h4 __attribute__((ext_vector_type(4)));
+typedef float wbmi_f4 __attribute__((ext_vector_type(4)));
+#define ROTH_WAVES 4
+#define RH_ROW(g, v) (4 * (g) + (v))
+
+struct SplitH {
+    wbmi_h4 hi, lo;
+};
+
+__device__ inline SplitH split_h(const float* x, float s) {
+    SplitH r;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float v = x[j] * s;
+        const _Float16 h = (_Float16)v;
+        r.hi[j] = h;
+        r.lo[j] = (_Float16)(v - (float)h);
+    }
+    return r;
+}
+
+__device__ inline wbmi_h4 neg_h(wbmi_h4 a) { return -a; }
+
+// acc += (Ah + Al)(Bh + Bl) without Al Bl
+__device__ inline wbmi_f4 mfma3(const SplitH& a, const SplitH& b, wbmi_f4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a.hi, b.lo, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(a.lo, b.hi, acc, 0, 0, 0);
+    return acc;
+}
+
+__device__ inline SplitH neg_split(const SplitH& a) { return SplitH{neg_h(a.hi), neg_h(a.lo)}; }
+
+__global__ void __launch_bounds__(64 * ROTH_WAVES) k_rotate_h(int nw, int nch, const cplx<float>* __restrict__ U,
+                                                             cplx<float>* X, uint64_t herm_lo, uint64_t herm_hi,
+                                                             uint64_t skip_lo, uint64_t skip_hi) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, g = lane >> 4;
+    const int64_t k = blockIdx.x;
+    const int nw2 = nw * nw;
+    const cplx<float>* Uk = U + k * nw2;
+    // U as the B operand of X U: block (ks, bj) holds U[16 ks + RH_ROW(g, j)][16 bj + li]; the same
+    // registers are the A operand of U^+ T for output row block bj (conjugated)
+    const float su = 16384.f;
+    SplitH ur[2][2], ui[2][2];
+#pragma unroll
+    for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            float re[4], im[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int m = 16 * ks + RH_ROW(g, j), c = 16 * bj + li;
+                const cplx<float> u = (m < nw && c < nw) ? Uk[m * nw + c] : mk(0.f, 0.f);
+                re[j] = u.re;
+                im[j] = u.im;
+            }
+            ur[ks][bj] = split_h(re, su);
+            ui[ks][bj] = split_h(im, su);
+        }
+    // the waves take the channels to rotate in turn (rotation index q = wave, wave + ROTH_WAVES, ...)
+    int q = -1;
+    for (int ch = 0; ch < nch; ++ch) {
+        if (bit_of(skip_lo, skip_hi, ch)) continue;
+        if (++q % ROTH_WAVES != wave) continue;
+        cplx<float>* Xc = X + (k * nch + ch) * nw2;
+        const bool herm = bit_of(herm_lo, herm_hi, ch);
+        // X as the A operand: block (bi, ks) holds X[16 bi + li][16 ks + RH_ROW(g, j)]
+        float xre[2][2][4], xim[2][2][4];
+        float mx = 0.f;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int r = 16 * bi + li, m = 16 * ks + RH_ROW(g, j);
+                    cplx<float> x = mk(0.f, 0.f);
+                    if (r < nw && m < nw) {
+                        x = Xc[r * nw + m];
+                        if (herm) {
+                            const cplx<float> w = Xc[m * nw + r];
+                            x = mk(0.5f * (x.re + w.re), 0.5f * (x.im - w.im));
+                        }
+                    }
+                    xre[bi][ks][j] = x.re;
+                    xim[bi][ks][j] = x.im;
+                    mx = fmaxf(mx, fmaxf(fabsf(x.re), fabsf(x.im)));
+                }
+        for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        // scale: max |x| s < 2^14 (fp16 holds up to 65504)
+        int e = 0;
+        frexpf(mx, &e);
+        const float sx = mx > 0.f ? ldexpf(1.f, 14 - e) : 1.f;
+        SplitH xr[2][2], xi[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                xr[bi][ks] = split_h(xre[bi][ks], sx);
+                xi[bi][ks] = split_h(xim[bi][ks], sx);
+            }
+        // T = X U:  Tr = Xr Ur - Xi Ui,  Ti = Xr Ui + Xi Ur   (scaled by sx su)
+        wbmi_f4 tr[2][2], ti[2][2];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                wbmi_f4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    ar = mfma3(xr[bi][ks], ur[ks][bj], ar);
+                    ar = mfma3(neg_split(xi[bi][ks]), ui[ks][bj], ar);
+                    ai = mfma3(xr[bi][ks], ui[ks][bj], ai);
+                    ai = mfma3(xi[bi][ks], ur[ks][bj], ai);
+                }
+                tr[bi][bj] = ar;
+                ti[bi][bj] = ai;
+            }
+        // T as the B operand of U^+ T: block (ks = bi, bj), element j = v; scale st = sx / 8
+        // (|T| <= sqrt(nw) max|X| < 8 max|X|), i.e. the accumulators times 1 / (8 su)
+        SplitH bre[2][2], bim[2][2];
+        const float tsc = 1.f / (8.f * su);
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                float vr[4], vi[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    vr[v] = tr[bi][bj][v];
+                    vi[v] = ti[bi][bj][v];
+                }
+                bre[bi][bj] = split_h(vr, tsc);
+                bim[bi][bj] = split_h(vi, tsc);
+            }
+        // Xbar = U^+ T: rows bo; A = conj(U)^T: Ar = Ur^T, Ai = -Ui^T
+        //   Xr = Ur^T Tr + Ui^T Ti,  Xi = Ur^T Ti - Ui^T Tr   (scaled by su sx / 8)
+        const float osc = 8.f / (su * sx);
+#pragma unroll
+        for (int bo = 0; bo < 2; ++bo)
+#pragma unroll
+            for (int bj = 0; bj < 2; ++bj) {
+                wbmi_f4 ar = {0.f, 0.f, 0.f, 0.f}, ai = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < 2; ++ks) {
+                    ar = mfma3(ur[ks][bo], bre[ks][bj], ar);
+                    ar = mfma3(ui[ks][bo], bim[ks][bj], ar);
+                    ai = mfma3(ur[ks][bo], bim[ks][bj], ai);
+                    ai = mfma3(neg_split(ui[ks][bo]), bre[ks][bj], ai);
+                }
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = 16 * bo + RH_ROW(g, v), cc = 16 * bj + li;
+                    if (r < nw && cc < nw) Xc[r * nw + cc] = mk(ar[v] * osc, ai[v] * osc);
+                }
+            }
+    }
+}
+
 static size_t rotate_work_bytes(int nk, int nw) { return nw > 64 ? sizeof(cplx<double>) * (size_t)nw * nw * nk : 0; }
 
 extern "C" size_t wbmi_rotate_work_bytes(int nk, int nw, int prec) {
@@ -366,5 +541,13 @@ extern "C" int wbmi_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U
     }
     if (prec == WBMI_F64) return launch_rotate<double>(ctx, nk, nw, nch, U_dev, X_dev, h, s, work_dev, work_bytes);
     if (prec == WBMI_F32) return launch_rotate<float>(ctx, nk, nw, nch, U_dev, X_dev, h, s, work_dev, work_bytes);
+    if (prec == WBMI_F32_SPLIT) {
+        if (nw > 32)  // above one 32 x 32 block per operand: the FP64-arithmetic path
+            return launch_rotate<float>(ctx, nk, nw, nch, U_dev, X_dev, h, s, work_dev, work_bytes);
+        hipLaunchKernelGGL(k_rotate_h, dim3(nk), dim3(64 * ROTH_WAVES), 0, ctx->stream, nw, nch,
+                           (const cplx<float>*)U_dev, (cplx<float>*)X_dev, h[0], h[1], s[0], s[1]);
+        WBMI_LAUNCH_CHECK(ctx);
+        return WBMI_OK;
+    }
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_rotate: bad precision");
 }

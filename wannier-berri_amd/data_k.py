@@ -22,7 +22,7 @@ Xbar('SR'), Xbar('SHR') (der 0).
 import numpy as np
 
 from . import _lib
-from ._lib import WBMI_F32, WBMI_F64
+from ._lib import WBMI_F32, WBMI_F32_SPLIT, WBMI_F64
 
 # components of each Xbar(name, der)
 CHANNEL_NCOMP = {("Ham", 0): 1, ("Ham", 1): 3, ("Ham", 2): 9,
@@ -150,13 +150,17 @@ class Data_K:
 
     Parameters (beyond the reference's): `precision` = 'fp64' (default) or
     'fp32' (complex64 k-space channels; energies, denominators and sums stay
-    FP64); `channels` = iterable of (name, der) computed together on first use.
+    FP64); `channels` = iterable of (name, der) computed together on first use;
+    `split_rotation` (mixed form only, default True): rotate every channel but
+    the velocity dH/dk with split-fp16 matrix-core products (FP32-accumulation
+    accuracy) instead of FP64 arithmetic.
     """
 
     accepts_channels = True
 
     def __init__(self, system, dK=None, grid=None, Kpoint=None, k_list=None, Emin=-np.inf, Emax=np.inf,
-                 fftlib=None, precision="fp64", channels=None, device=0, random_gauge=False, **unused):
+                 fftlib=None, precision="fp64", channels=None, device=0, random_gauge=False, split_rotation=True,
+                 **unused):
         import torch
         self.torch = torch
         if not torch.cuda.is_available():
@@ -190,6 +194,7 @@ class Data_K:
         self.precision = precision
         self.prec = WBMI_F64 if precision == "fp64" else WBMI_F32
         self.cdtype = torch.complex128 if precision == "fp64" else torch.complex64
+        self.split_rotation = bool(split_rotation) and precision == "fp32"
         self._requested = set(channels) if channels else set()
         self._slots = None
         self.Xk = None
@@ -274,7 +279,17 @@ class Data_K:
                     herm[sl:sl + CHANNEL_NCOMP[key]] = 1
             if ("Ham", 0) in slots:
                 skip[slots[("Ham", 0)]] = 1
-            if skip.sum() < nch:
+            if self.split_rotation:
+                # the velocity in FP64 arithmetic (its elements between close bands enter the
+                # Berry curvature through 1 / (E_n - E_m)^2), the rest on split-fp16 products
+                vel = np.zeros(nch, dtype=np.int32)
+                if ("Ham", 1) in slots:
+                    vel[slots[("Ham", 1)]:slots[("Ham", 1)] + 3] = 1
+                if (skip | (1 - vel)).sum() < nch:
+                    self.ctx.rotate(U, X, herm, skip | (1 - vel), self.prec)
+                if (skip | vel).sum() < nch:
+                    self.ctx.rotate(U, X, herm, skip | vel, WBMI_F32_SPLIT)
+            elif skip.sum() < nch:
                 self.ctx.rotate(U, X, herm, skip, self.prec)
         self.Xk, self.E_dev, self.U_dev, self._slots, self.nch = X, E, U, slots, nch
 
