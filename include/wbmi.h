@@ -191,6 +191,18 @@ int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride,
               void* work_dev, size_t work_bytes);
 size_t wbmi_eigh_work_bytes(int nk, int nw, int prec, int refine);
 
+/* ---- the eigensolver without allocation ------------------------------------
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops).  wbmi_eigh with its
+ * workspace taken from a fixed scratch array in the library, one per GPU, of
+ * wbmi_eigh_noalloc_max_bytes() = 64 MiB: sizes whose wbmi_eigh_work_bytes
+ * exceed it are refused (at nw = 128 with refinement: 64 k-points per call;
+ * nw <= 44 needs none).  Allocates nothing; calls on one GPU share the scratch,
+ * so calls that overlap on one GPU race.                                    */
+int wbmi_eigh_noalloc(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride,
+                      double* E_dev, void* U_dev, int prec, int refine, const void* H64_dev);
+size_t wbmi_eigh_noalloc_max_bytes(void);
+
 /* ---- batched eigensolver with a starting basis ---------------------------
  * No counterpart in the reference (its eigh starts from nothing,
  * data_K.py:211-218): the same result as wbmi_eigh with refine >= 1, started

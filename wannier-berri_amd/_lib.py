@@ -44,7 +44,7 @@ SYMBOLS = [
     "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos", "wbmi_spin_hall", "wbmi_spin_hall_work_bytes",
     "wbmi_tabulate", "wbmi_tabulate_diag", "wbmi_shift_current", "wbmi_injection_current",
     "wbmi_static_scan_k", "wbmi_finalize_k", "wbmi_static_scan_tetra_k", "wbmi_finalize_tetra_k",
-    "wbmi_tabulate_invmass",
+    "wbmi_tabulate_invmass", "wbmi_eigh_noalloc", "wbmi_eigh_noalloc_max_bytes",
 ]
 
 
@@ -101,6 +101,8 @@ _PROTOS = {
     "wbmi_r2k_exec": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
     "wbmi_r2k_klist": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _vp, _i]),
     "wbmi_eigh": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp, _vp, ctypes.c_size_t]),
+    "wbmi_eigh_noalloc": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp]),
+    "wbmi_eigh_noalloc_max_bytes": (ctypes.c_size_t, []),
     "wbmi_eigh_work_bytes": (ctypes.c_size_t, [_i, _i, _i, _i]),
     "wbmi_eigh_guess": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp, _vp, _vp, ctypes.c_size_t]),
     "wbmi_optical_conductivity": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(OptcondParams), _vp]),
@@ -253,6 +255,12 @@ class Context:
             self.check(self._timed("eigh", (nk, nw), lib().wbmi_eigh_guess, self.h, nk, nw, _ptr(H), Hstride,
                                    _ptr(E), _ptr(U), prec, int(refine), _ptr(H64), _ptr(U0), _ptr(work), wbytes),
                        "wbmi_eigh_guess")
+
+    def eigh_noalloc(self, H, Hstride, E, U, prec, nw, refine=0, H64=None):
+        """wbmi_eigh_noalloc: eigh with the library's fixed scratch instead of a workspace."""
+        nk = E.shape[0]
+        self.check(lib().wbmi_eigh_noalloc(self.h, nk, nw, _ptr(H), Hstride, _ptr(E), _ptr(U), prec, int(refine),
+                                           _ptr(H64)), "wbmi_eigh_noalloc")
 
     def rotate(self, U, X, herm, skip, prec):
         nk, nch, nw = X.shape[0], X.shape[1], X.shape[2]

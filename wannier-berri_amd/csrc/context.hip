@@ -51,6 +51,18 @@ int wbmi_ctx_create(int device, void* stream, wbmi_ctx** out) {
     wbmi_ctx* c = new wbmi_ctx();
     c->device = device;
     c->stream = (hipStream_t)stream;
+    c->eigh_scratch = nullptr;
+    // the scratch array's address on this device, looked up with the device current
+    int prev = 0;
+    if (hipGetDevice(&prev) != hipSuccess || hipSetDevice(device) != hipSuccess) {
+        delete c;
+        return WBMI_ERR_HIP;
+    }
+    const int rc = wbmi_eigh_scratch_init(&c->eigh_scratch);
+    if (hipSetDevice(prev) != hipSuccess || rc != 0) {
+        delete c;
+        return WBMI_ERR_HIP;
+    }
     *out = c;
     return WBMI_OK;
 }

@@ -1057,6 +1057,21 @@ __global__ void __launch_bounds__(OAG_THREADS) k_eigh_refine_g(int n, const cplx
     for (int i = tid; i < n; i += OAG_THREADS) Eout[k * n + rank[i]] = lam[i];
 }
 
+// Scratch of wbmi_eigh_noalloc: the per-k refinement workspace (X, T, S/E, R for
+// 44 < nw <= 128) in a file-scope device array, so that callers that cannot allocate
+// (graph capture, serving loops) get the refined solver without a workspace.  Each GPU
+// holds its own copy for the life of the process; calls on one GPU share it (their
+// kernels write every element before reading it, so nothing needs clearing, and
+// overlapping calls on one GPU race).  Host code uses the per-device address from
+// hipGetSymbolAddress, never the array's name.
+#define WBMI_EIGH_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) char scratch[WBMI_EIGH_SCRATCH_BYTES];
+
+int wbmi_eigh_scratch_init(void** ptr) {
+    *ptr = nullptr;
+    return hipGetSymbolAddress(ptr, HIP_SYMBOL(scratch)) == hipSuccess && *ptr ? 0 : 1;
+}
+
 // bytes of global workspace wbmi_eigh needs (refinement for nw > 44)
 static size_t eigh_work_bytes(int nk, int nw, int refine) {
     return (refine && nw > 44) ? sizeof(cplx<double>) * 4 * (size_t)nw * nw * nk : 0;
@@ -1171,6 +1186,23 @@ extern "C" int wbmi_eigh(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64
     }
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
+}
+
+extern "C" size_t wbmi_eigh_noalloc_max_bytes(void) { return WBMI_EIGH_SCRATCH_BYTES; }
+
+extern "C" int wbmi_eigh_noalloc(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride, double* E_dev,
+                                 void* U_dev, int prec, int refine, const void* H64_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    const size_t need = nk > 0 && nw > 0 ? eigh_work_bytes(nk, nw, refine) : 0;
+    if (need > WBMI_EIGH_SCRATCH_BYTES)
+        return wbmi_fail(ctx, WBMI_ERR_ARG,
+                         "wbmi_eigh_noalloc: %d k-points at nw = %d need %zu bytes of scratch, more than its %zu "
+                         "(at most %zu k-points per call at this nw)",
+                         nk, nw, need, WBMI_EIGH_SCRATCH_BYTES,
+                         WBMI_EIGH_SCRATCH_BYTES / eigh_work_bytes(1, nw, refine));
+    if (need > 0 && !ctx->eigh_scratch) return wbmi_fail(ctx, WBMI_ERR_HIP, "wbmi_eigh_noalloc: no scratch address");
+    return wbmi_eigh(ctx, nk, nw, H_dev, Hstride, E_dev, U_dev, prec, refine, H64_dev, ctx->eigh_scratch,
+                     WBMI_EIGH_SCRATCH_BYTES);
 }
 
 extern "C" int wbmi_eigh_guess(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hstride, double* E_dev,

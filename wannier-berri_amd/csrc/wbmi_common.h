@@ -13,7 +13,11 @@ struct wbmi_ctx {
     int device;
     hipStream_t stream;
     std::string err;
+    void* eigh_scratch;  // this device's address of the no-allocation eigensolver's scratch
 };
+
+// Device address of the eigensolver scratch on the current device (eigh.hip); 0 on success.
+int wbmi_eigh_scratch_init(void** ptr);
 
 // Record an error message on the context and return the code.
 int wbmi_fail(wbmi_ctx* ctx, int code, const char* fmt, ...);
