@@ -186,10 +186,12 @@ def bench_klists(args):
     ~ N(centre, 0.02) around 16 centres, seed 4), list i on model i % 5
     (synthetic recipe, nw = 8, 16, 24, 36, 48, R cube |R_i| <= 4), AHC with
     external terms at 2001 levels, every point weighted 1 / (its model's
-    points).  A step evaluates all lists of all five models with run_klists."""
+    points).  The k-list transform's workgroup tile is chosen per nw by the
+    autotuner (untimed), then all lists of all five models are captured as one
+    HIP graph (KlistGraph); a step is one replay of that graph."""
     import torch
     import wannierberri_amd as wb
-    from wannierberri_amd import _lib, models
+    from wannierberri_amd import _lib, autotune, models
 
     lists = models.klist_batch()
     systems = [models.synthetic(num_wann=nw, Rmax=RMAX, seed=40 + i) for i, nw in enumerate(KL_NW)]
@@ -199,21 +201,41 @@ def bench_klists(args):
     calcs = {"ahc": wb.calculators.static.AHC(Efermi=Ef)}
     ctx = _lib.get_context(0)
     prec = args.precisions.split(",")[0]
+    wprec = _lib.WBMI_F64 if prec == "fp64" else _lib.WBMI_F32
+    tiles = {}
+    for s in systems:
+        need = sorted(calcs["ahc"].needed_channels(s), key=list(wb.data_k.CHANNEL_NCOMP).index)
+        need = [k for k in need if not (prec == "fp32" and k == ("Ham", 0))]
+        best, times = autotune.tune_klist(s, need, nk=8192, prec=wprec)
+        tiles[s.num_wann] = {"tile": autotune.tile_shapes()[best], "ms": {str(autotune.tile_shapes()[t]): round(v, 3)
+                                                                          for t, v in times.items()}}
+        log(f"[bench] nw={s.num_wann}: tile {autotune.tile_shapes()[best]} ({times})")
+    pk = {"precision": prec}
+    t0 = time.perf_counter()
+    g = wb.KlistGraph(list(zip(systems, groups)), calcs, parameters_K=pk)
+    log(f"[bench] graph built in {time.perf_counter() - t0:.1f} s ({npts} points)")
 
-    def step():
-        return [wb.run_klists(s, g, calcs, parameters_K={"precision": prec}) for s, g in zip(systems, groups)]
+    def eager_step():
+        return [wb.run_klists(s, gl, calcs, parameters_K=pk) for s, gl in zip(systems, groups)]
 
     for _ in range(args.warmup):
-        step()
+        g.replay()
+        eager_step()
     torch.cuda.synchronize()
     ctx.timer = {}
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        r = step()
+        eager_step()
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+    dt_eager = (time.perf_counter() - t0) / args.steps
     stats = stage_stats(ctx.timer)
     ctx.timer = None
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = g.replay()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    log(f"[bench] klists: graph {dt * 1e3:.1f} ms/step, eager {dt_eager * 1e3:.1f} ms/step")
     for st, (n, ms, _) in sorted(stats.items(), key=lambda x: -x[1][1]):
         log(f"    {st:12s} calls/step={n // args.steps:5d} ms/step={ms / args.steps:9.2f} avg={ms / n:.3f} ms")
     n, ms, units = stats["r2k_klist"]
@@ -225,29 +247,37 @@ def bench_klists(args):
         from oracle import wb_oracle as O
         t = time.time()
         nk_cpu, err = 0, 0.0
-        for s, g in zip(systems, groups):
+        for s, gl, r in zip(systems, groups, res):
             o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
                                {"Ham": s.get_R_mat("Ham"), "AA": s.get_R_mat("AA")})
-            sample = g[0][:200]
+            sample = gl[0][:200]
             ref = O.AHC(Ef)(O.OracleDataK(o, k_list=sample)).data
             nk_cpu += len(sample)
-            got = wb.run_klists(s, [sample], calcs, parameters_K={"precision": prec}).results["ahc"].data
+            got = wb.run_klists(s, [sample], calcs, parameters_K=pk).results["ahc"].data
             err = max(err, float(np.abs(got - ref).max() / np.abs(ref).max()))
         wall = time.time() - t
         cb = {"value": nk_cpu / wall, "unit": "k-points/s", "cores": 1, "kind": "port",
               "sample": f"<= 200 points of the first list of each model ({nk_cpu} points), FP64 NumPy "
                         f"oracle, one process, "
                         f"wall {wall:.1f} s; GPU rel err on it {err:.2e}"}
+    # the graph's totals against the eager run_klists totals
+    eg = eager_step()
+    gerr = max(float(np.abs(r.results["ahc"].data - e.results["ahc"].data).max() /
+                     np.abs(e.results["ahc"].data).max()) for r, e in zip(g.replay(), eg))
     out = {
         "metric": "k-pts/s AHC over 512 weighted k-lists, 5 models nw 8-48, on 1 MI355X",
         "value": npts / dt, "unit": "k-points/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"512 k-lists (log-uniform 8..4096 points, {npts} total) on 5 synthetic models "
-                               f"nw {KL_NW}, R cube |R_i|<={RMAX} (729), AHC at {EFERMI[2]} levels",
+                               f"nw {KL_NW}, R cube |R_i|<={RMAX} (729), AHC at {EFERMI[2]} levels, one HIP graph "
+                               f"replay per step, k-list tiles autotuned per nw",
                    "nk": npts, "precision": "mixed" if prec == "fp32" else "fp64"},
         "roofline": {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64, "unit": "TFLOP/s",
                      "frac": ach / PEAK_FP64, "kernel": "r2k_klist", "traffic": traffic_of("r2k_klist", prec)},
+        "eager_ms_per_step": dt_eager * 1e3,
+        "graph_vs_eager_rel_err": gerr,
+        "tiles": tiles,
         "stages_ms_per_step": {st: v[1] / args.steps for st, v in stats.items()},
     }
     if cb is not None:

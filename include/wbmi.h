@@ -165,6 +165,15 @@ int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* plan, const void* XR_dev,
  *   XR_dev    : complex128 [nR, nelem];  out_dev : [nk, nelem] (prec)       */
 int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nelem,
                    const void* XR_dev, int nk, const double* klist_dev, void* out_dev, int prec);
+/* The same transform with an explicit workgroup tile (0 .. WBMI_KLIST_NTILES-1; 0 is what
+ * wbmi_r2k_klist uses): no counterpart in the reference, for the per-nw autotuner
+ * (wannierberri_amd/autotune.py) that times the tiles on the caller's shapes.
+ * wbmi_r2k_klist_tile_shape gives a tile's k-points x elements per workgroup.         */
+#define WBMI_KLIST_NTILES 5
+int wbmi_r2k_klist_tile(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nelem,
+                        const void* XR_dev, int nk, const double* klist_dev, void* out_dev, int prec,
+                        int tile);
+int wbmi_r2k_klist_tile_shape(int tile, int* k_rows, int* elements);
 
 /* ---- batched Hermitian eigensolver ----------------------------------------
  * Replaces np.linalg.eigh in Data_K.E_K (data_K/data_K.py:211-218).

@@ -129,3 +129,60 @@ def test_tabresult_to_grid(wb):
         assert np.allclose(g.results["Energy"].data[i], E[sel].mean(axis=0))
         assert np.allclose(g.results["V"].data[i], V[sel].mean(axis=0))
     assert g.get_data("V", component="y").shape == (3, 4, 2, 2)
+
+
+def test_ws_dist_closed_form(wb):
+    """MDRS fold (system_R.py:572-588) on a one-orbital cubic model, mp_grid 4 along x: the hopping
+    at R = 2 sits at the Wigner-Seitz boundary of the 4-cell supercell, so it is shared by
+    R = 2 and R = -2 with weight 1/2 each; R = 1 and R = -1 stay; R = 3 folds to R = -1."""
+    L = np.eye(3)
+    iR = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [2, 0, 0], [3, 0, 0]])
+    H = np.zeros((5, 1, 1), dtype=complex)
+    H[:, 0, 0] = [0.5, 1.0, 1.0, 0.3, 0.2]
+    s = wb.System_R(L, iR, np.zeros((1, 3)), {"Ham": H})
+    s.do_ws_dist(mp_grid=[4, 1, 1])
+    got = {tuple(r): s.get_R_mat("Ham")[i, 0, 0] for i, r in enumerate(s.rvec.iRvec)}
+    assert got == pytest.approx({(0, 0, 0): 0.5, (1, 0, 0): 1.0, (-1, 0, 0): 1.2, (2, 0, 0): 0.15,
+                                 (-2, 0, 0): 0.15})
+    assert list(s.NKFFT_recommended) == [4, 1, 1]
+
+
+def test_ws_dist_invariants(wb):
+    """MDRS fold of a random 6-band model with Wannier centres off the lattice points:
+    (a) H(k) on the mp_grid points is unchanged (the fold only adds supercell vectors to R);
+    (b) Hermiticity H(-R) = H(R)^+ survives; (c) every element sits on the translates closest to
+    |R + t_n - t_m| (no other translate of the same cell vector is shorter); (d) per element and
+    cell vector the weights add up to the folded value."""
+    rng = np.random.default_rng(9)
+    L = np.array([[3.0, 0.2, 0.0], [0.4, 2.7, 0.1], [0.0, 0.3, 3.3]])
+    nw = 6
+    iR = np.array([[i, j, k] for i in range(-2, 3) for j in range(-2, 3) for k in range(-1, 2)])
+    H = rng.standard_normal((len(iR), nw, nw)) + 1j * rng.standard_normal((len(iR), nw, nw))
+    wcc = rng.random((nw, 3)).dot(L)
+    s = wb.System_R(L, iR, wcc, {"Ham": H})
+    H = 0.5 * (H + s.conj_XX_R(H))
+    s = wb.System_R(L, iR, wcc, {"Ham": H})
+    mp = np.array([3, 4, 2])
+    kgrid = np.array([[i, j, k] for i in range(3) for j in range(4) for k in range(2)]) / mp
+
+    def hk(sys_):
+        ph = np.exp(2j * np.pi * kgrid.dot(sys_.rvec.iRvec.T))
+        return np.einsum("kr,rmn->kmn", ph, sys_.get_R_mat("Ham"))
+    before = hk(s)
+    s.do_ws_dist(mp_grid=mp)
+    assert np.abs(hk(s) - before).max() < 1e-12
+    assert s.hermiticity_error("Ham") < 1e-14
+    Hn, Rn = s.get_R_mat("Ham"), s.rvec.iRvec
+    T = np.array([[a, b, c] for a in range(-3, 4) for b in range(-3, 4) for c in range(-3, 4)]) * mp
+    t = wcc
+    for m in range(nw):
+        for n in range(nw):
+            nz = np.abs(Hn[:, m, n]) > 1e-12
+            for R in Rn[nz]:
+                d = np.linalg.norm((R[None] + T - (R % mp)[None] + (R % mp)[None]).dot(L) + t[n] - t[m], axis=1)
+                assert np.linalg.norm(R.dot(L) + t[n] - t[m]) <= d.min() + 1e-9
+            # weights: the images of each cell vector add up to the folded value
+            for c in {tuple(x) for x in (Rn[nz] % mp)}:
+                img = np.all(Rn % mp == np.array(c), axis=1)
+                orig = np.all(iR % mp == np.array(c), axis=1)
+                assert abs(Hn[img, m, n].sum() - H[orig, m, n].sum()) < 1e-12

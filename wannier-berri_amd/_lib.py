@@ -44,7 +44,8 @@ SYMBOLS = [
     "wbmi_static_scan_tetra", "wbmi_finalize_tetra", "wbmi_jdos", "wbmi_spin_hall", "wbmi_spin_hall_work_bytes",
     "wbmi_tabulate", "wbmi_tabulate_diag", "wbmi_shift_current", "wbmi_injection_current",
     "wbmi_static_scan_k", "wbmi_finalize_k", "wbmi_static_scan_tetra_k", "wbmi_finalize_tetra_k",
-    "wbmi_tabulate_invmass", "wbmi_eigh_noalloc", "wbmi_eigh_noalloc_max_bytes",
+    "wbmi_tabulate_invmass", "wbmi_eigh_noalloc", "wbmi_eigh_noalloc_max_bytes", "wbmi_r2k_klist_tile",
+    "wbmi_r2k_klist_tile_shape",
 ]
 
 
@@ -100,6 +101,8 @@ _PROTOS = {
     "wbmi_r2k_box": (_i, [_vp, _vp, _vp, _vp]),
     "wbmi_r2k_exec": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
     "wbmi_r2k_klist": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _vp, _i]),
+    "wbmi_r2k_klist_tile": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _vp, _i, _i]),
+    "wbmi_r2k_klist_tile_shape": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "wbmi_eigh": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp, _vp, ctypes.c_size_t]),
     "wbmi_eigh_noalloc": (_i, [_vp, _i, _i, _vp, _i64, _vp, _vp, _i, _i, _vp]),
     "wbmi_eigh_noalloc_max_bytes": (ctypes.c_size_t, []),
@@ -179,6 +182,7 @@ class Context:
             raise WbmiError(f"wbmi_ctx_create failed with code {rc} (device {device}; gfx950 required)")
         self.h = h
         self.timer = None   # optional {stage: [(start_event, end_event, units)]}
+        self.tuning = {}    # (knob, nw, prec) -> choice, filled by wannierberri_amd.autotune
         self._work = None   # device workspace shared by the calls on this context's stream
 
     def __del__(self):
@@ -213,6 +217,13 @@ class Context:
     def sync(self):
         self.check(lib().wbmi_sync(self.h), "wbmi_sync")
 
+    def set_stream(self, stream):
+        """Launch on `stream` (a torch stream) from now on; returns the previous one."""
+        prev = self.stream
+        self.check(lib().wbmi_ctx_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream)), "wbmi_ctx_set_stream")
+        self.stream = stream
+        return prev
+
     # -- stages --------------------------------------------------------------
     def build_channels(self, cRshift, base, spec, out):
         nR, nw = cRshift.shape[0], cRshift.shape[1]
@@ -223,12 +234,16 @@ class Context:
     def r2k_plan(self, iRvec, NKFFT, nelem):
         return R2KPlan(self, iRvec, NKFFT, nelem)
 
-    def r2k_klist(self, iRvec_dev, XR, klist_dev, out, prec):
+    def r2k_klist(self, iRvec_dev, XR, klist_dev, out, prec, tile=None):
+        """Phase-GEMM R -> k; `tile` defaults to the tuned tile for out's nw and `prec`
+        (self.tuning[("klist_tile", nw, prec)], set by wannierberri_amd.autotune), else tile 0."""
         nR = iRvec_dev.shape[0]
         nelem = XR[0].numel()
-        self.check(self._timed("r2k_klist", (klist_dev.shape[0], nelem, nR), lib().wbmi_r2k_klist, self.h, nR,
+        if tile is None:
+            tile = self.tuning.get(("klist_tile", int(out.shape[-1]), int(prec)), 0)
+        self.check(self._timed("r2k_klist", (klist_dev.shape[0], nelem, nR), lib().wbmi_r2k_klist_tile, self.h, nR,
                                _ptr(iRvec_dev), nelem, _ptr(XR), klist_dev.shape[0], _ptr(klist_dev), _ptr(out),
-                               prec), "wbmi_r2k_klist")
+                               prec, int(tile)), "wbmi_r2k_klist")
 
     def _workspace(self, nbytes):
         """Device workspace of at least nbytes (kept and grown on demand; None for 0)."""

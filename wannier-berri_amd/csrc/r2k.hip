@@ -348,29 +348,31 @@ int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* XR_dev, voi
 // (nR x 128 x 16 B) through that XCD's L2.
 // ---------------------------------------------------------------------------
 
-#define KLM_THREADS 512
-#define KLM_TK 64
-#define KLM_TE 128
 #define KLM_RC 32
 
-template <typename Tout>
-__global__ void __launch_bounds__(KLM_THREADS) k_r2k_klist(int nR, const int32_t* __restrict__ iR, int64_t nelem,
+// Workgroup tiles (waves along k x waves along the elements), chosen per nw by the tuning
+// table of the context (wbmi_tune_set, WBMI_KNOB_KLIST_TILE); index 0 is the default.
+static const int klm_tiles[WBMI_KLIST_NTILES][2] = {{2, 4}, {4, 2}, {1, 8}, {2, 2}, {1, 4}};
+
+template <typename Tout, int WK, int WE>
+__global__ void __launch_bounds__(64 * WK * WE) k_r2k_klist(int nR, const int32_t* __restrict__ iR, int64_t nelem,
                                                            const cplx<double>* __restrict__ X, int nk,
                                                            const double* __restrict__ kl,
                                                            cplx<Tout>* __restrict__ out) {
-    __shared__ cplx<double> ph[KLM_TK][KLM_RC + 1];
+    constexpr int TK = 32 * WK, TE = 32 * WE, NT = 64 * WK * WE;
+    __shared__ cplx<double> ph[TK][KLM_RC + 1];
     const int wave = threadIdx.x >> 6;
-    const int k0 = blockIdx.x * KLM_TK;
-    const int64_t e0 = (int64_t)blockIdx.y * KLM_TE;
-    const int kw = k0 + 32 * (wave >> 2);          // first k of this wave's tile
-    const int64_t ew = e0 + 32 * (wave & 3);       // first element of this wave's tile
+    const int k0 = blockIdx.x * TK;
+    const int64_t e0 = (int64_t)blockIdx.y * TE;
+    const int kw = k0 + 32 * (wave / WE);          // first k of this wave's tile
+    const int64_t ew = e0 + 32 * (wave % WE);      // first element of this wave's tile
     const int nrow = nk - kw < 32 ? nk - kw : 32;
     const int ncol = nelem - ew < 32 ? (int)(nelem - ew) : 32;
     MfmaBlock C;
     C.zero();
     for (int r0 = 0; r0 < nR; r0 += KLM_RC) {
         __syncthreads();
-        for (int t = threadIdx.x; t < KLM_TK * KLM_RC; t += KLM_THREADS) {
+        for (int t = threadIdx.x; t < TK * KLM_RC; t += NT) {
             const int q = t / KLM_RC, rr = t % KLM_RC;
             const int k = k0 + q, r = r0 + rr;
             cplx<double> v = mk(0.0, 0.0);
@@ -386,7 +388,7 @@ __global__ void __launch_bounds__(KLM_THREADS) k_r2k_klist(int nR, const int32_t
         __syncthreads();
         if (nrow > 0 && ncol > 0) {
             const int rc = nR - r0 < KLM_RC ? nR - r0 : KLM_RC;
-            const int prow = 32 * (wave >> 2);
+            const int prow = 32 * (wave / WE);
             const cplx<double>* Xr = X + (int64_t)r0 * nelem + ew;
             mfma_block(
                 C, nrow, ncol, rc, 0, 0, [&](int r, int k) { return ph[prow + r][k]; },
@@ -408,23 +410,54 @@ __global__ void __launch_bounds__(KLM_THREADS) k_r2k_klist(int nR, const int32_t
             }
 }
 
-extern "C" int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nelem, const void* XR_dev,
-                              int nk, const double* klist_dev, void* out_dev, int prec) {
+template <typename Tout, int WK, int WE>
+static int launch_klist(wbmi_ctx* ctx, int nR, const int32_t* iR, int64_t nelem, const void* XR, int nk,
+                        const double* kl, void* out) {
+    constexpr int TK = 32 * WK, TE = 32 * WE;
+    if (ceil_div(nelem, TE) > 65535)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: more than %d elements per k-point", 65535 * TE);
+    dim3 g((unsigned)ceil_div(nk, TK), (unsigned)ceil_div(nelem, TE));
+    hipLaunchKernelGGL((k_r2k_klist<Tout, WK, WE>), g, dim3(64 * WK * WE), 0, ctx->stream, nR, iR, nelem,
+                       (const cplx<double>*)XR, nk, kl, (cplx<Tout>*)out);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+template <typename Tout>
+static int launch_klist_tile(wbmi_ctx* ctx, int tile, int nR, const int32_t* iR, int64_t nelem, const void* XR,
+                             int nk, const double* kl, void* out) {
+    switch (tile) {
+        case 1: return launch_klist<Tout, 4, 2>(ctx, nR, iR, nelem, XR, nk, kl, out);
+        case 2: return launch_klist<Tout, 1, 8>(ctx, nR, iR, nelem, XR, nk, kl, out);
+        case 3: return launch_klist<Tout, 2, 2>(ctx, nR, iR, nelem, XR, nk, kl, out);
+        case 4: return launch_klist<Tout, 1, 4>(ctx, nR, iR, nelem, XR, nk, kl, out);
+        default: return launch_klist<Tout, 2, 4>(ctx, nR, iR, nelem, XR, nk, kl, out);
+    }
+}
+
+extern "C" int wbmi_r2k_klist_tile(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nelem,
+                                   const void* XR_dev, int nk, const double* klist_dev, void* out_dev, int prec,
+                                   int tile) {
     if (!ctx) return WBMI_ERR_ARG;
     if (nR <= 0 || nelem <= 0 || nk <= 0 || !iRvec_dev || !XR_dev || !klist_dev || !out_dev)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: bad arguments");
-    if (ceil_div(nelem, KLM_TE) > 65535)
-        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: more than %d elements per k-point",
-                         65535 * KLM_TE);
-    dim3 g((unsigned)ceil_div(nk, KLM_TK), (unsigned)ceil_div(nelem, KLM_TE));
+    if (tile < 0 || tile >= WBMI_KLIST_NTILES)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist_tile: tile must be 0..%d", WBMI_KLIST_NTILES - 1);
     if (prec == WBMI_F64)
-        hipLaunchKernelGGL(k_r2k_klist<double>, g, dim3(KLM_THREADS), 0, ctx->stream, nR, iRvec_dev, nelem,
-                           (const cplx<double>*)XR_dev, nk, klist_dev, (cplx<double>*)out_dev);
-    else if (prec == WBMI_F32)
-        hipLaunchKernelGGL(k_r2k_klist<float>, g, dim3(KLM_THREADS), 0, ctx->stream, nR, iRvec_dev, nelem,
-                           (const cplx<double>*)XR_dev, nk, klist_dev, (cplx<float>*)out_dev);
-    else
-        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: bad precision");
-    WBMI_LAUNCH_CHECK(ctx);
+        return launch_klist_tile<double>(ctx, tile, nR, iRvec_dev, nelem, XR_dev, nk, klist_dev, out_dev);
+    if (prec == WBMI_F32)
+        return launch_klist_tile<float>(ctx, tile, nR, iRvec_dev, nelem, XR_dev, nk, klist_dev, out_dev);
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_klist: bad precision");
+}
+
+extern "C" int wbmi_r2k_klist(wbmi_ctx* ctx, int nR, const int32_t* iRvec_dev, int64_t nelem, const void* XR_dev,
+                              int nk, const double* klist_dev, void* out_dev, int prec) {
+    return wbmi_r2k_klist_tile(ctx, nR, iRvec_dev, nelem, XR_dev, nk, klist_dev, out_dev, prec, 0);
+}
+
+extern "C" int wbmi_r2k_klist_tile_shape(int tile, int* k_rows, int* elements) {
+    if (tile < 0 || tile >= WBMI_KLIST_NTILES || !k_rows || !elements) return WBMI_ERR_ARG;
+    *k_rows = 32 * klm_tiles[tile][0];
+    *elements = 32 * klm_tiles[tile][1];
     return WBMI_OK;
 }

@@ -140,7 +140,7 @@ __global__ void __launch_bounds__(ROT_THREADS) k_rotate(int nw, int nch, const c
 // Barriers stay in uniform control flow.
 #define ROTW_WAVES 8
 
-__device__ inline bool bit_of(uint64_t lo, uint64_t hi, int c) { return c < 64 ? (lo >> c) & 1 : (hi >> (c - 64)) & 1; }
+__host__ __device__ inline bool bit_of(uint64_t lo, uint64_t hi, int c) { return c < 64 ? (lo >> c) & 1 : (hi >> (c - 64)) & 1; }
 
 __host__ __device__ inline size_t rotw_tile_bytes(int nw) { return sizeof(cplx<double>) * nw * (nw + 1); }
 
@@ -155,7 +155,8 @@ __global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, c
     cplx<double>* tile = sU + (1 + wave) * nw * ld;
     const int64_t k = blockIdx.x;
     const cplx<T>* Uk = U + k * nw2;
-    for (int t = tid; t < nw2; t += 64 * ROTW_WAVES) {
+    const int nwave = (int)(blockDim.x >> 6);
+    for (int t = tid; t < nw2; t += (int)blockDim.x) {
         const cplx<T> u = Uk[t];
         sU[(t / nw) * ld + t % nw] = mk((double)u.re, (double)u.im);
     }
@@ -175,7 +176,7 @@ __global__ void __launch_bounds__(64 * ROTW_WAVES) k_rotate_w(int nw, int nch, c
     // From here on every wave works on its own tile: the phases of a channel are
     // ordered by wave-level fences only, so the waves drift apart and one wave's
     // loads and stores overlap the other waves' products on the matrix cores.
-    for (int base = 0; base < nrot; base += ROTW_WAVES) {
+    for (int base = 0; base < nrot; base += nwave) {
         const int c = base + wave < nrot ? channel(base + wave) : -1;
         if (c < 0) break;
         cplx<T>* Xc = X + (k * nch + c) * nw2;
@@ -395,28 +396,45 @@ __global__ void __launch_bounds__(64 * ROTH_WAVES) k_rotate_h(int nw, int nch, c
         if (++q % ROTH_WAVES != wave) continue;
         cplx<float>* Xc = X + (k * nch + ch) * nw2;
         const bool herm = bit_of(herm_lo, herm_hi, ch);
-        // X as the A operand: block (bi, ks) holds X[16 bi + li][16 ks + RH_ROW(g, j)]
+        // X as the A operand: block (bi, ks) holds X[16 bi + li][16 ks + RH_ROW(g, j)], four
+        // consecutive elements of a row (at nw = 32 two 16-byte loads: the row offset is a
+        // multiple of 4 elements and a channel starts 8 KB-aligned)
         float xre[2][2][4], xim[2][2][4];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                const int r = 16 * bi + li, m0 = 16 * ks + RH_ROW(g, 0);
+                if (nw == 32) {
+                    const float4* p = (const float4*)(Xc + r * 32 + m0);
+                    const float4 a = p[0], b = p[1];
+                    xre[bi][ks][0] = a.x; xim[bi][ks][0] = a.y; xre[bi][ks][1] = a.z; xim[bi][ks][1] = a.w;
+                    xre[bi][ks][2] = b.x; xim[bi][ks][2] = b.y; xre[bi][ks][3] = b.z; xim[bi][ks][3] = b.w;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const cplx<float> x = (r < nw && m0 + j < nw) ? Xc[r * nw + m0 + j] : mk(0.f, 0.f);
+                        xre[bi][ks][j] = x.re;
+                        xim[bi][ks][j] = x.im;
+                    }
+                }
+                if (herm) {
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int m = m0 + j;
+                        const cplx<float> w = (r < nw && m < nw) ? Xc[m * nw + r] : mk(0.f, 0.f);
+                        xre[bi][ks][j] = 0.5f * (xre[bi][ks][j] + w.re);
+                        xim[bi][ks][j] = 0.5f * (xim[bi][ks][j] - w.im);
+                    }
+                }
+            }
         float mx = 0.f;
 #pragma unroll
         for (int bi = 0; bi < 2; ++bi)
 #pragma unroll
             for (int ks = 0; ks < 2; ++ks)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int r = 16 * bi + li, m = 16 * ks + RH_ROW(g, j);
-                    cplx<float> x = mk(0.f, 0.f);
-                    if (r < nw && m < nw) {
-                        x = Xc[r * nw + m];
-                        if (herm) {
-                            const cplx<float> w = Xc[m * nw + r];
-                            x = mk(0.5f * (x.re + w.re), 0.5f * (x.im - w.im));
-                        }
-                    }
-                    xre[bi][ks][j] = x.re;
-                    xim[bi][ks][j] = x.im;
-                    mx = fmaxf(mx, fmaxf(fabsf(x.re), fabsf(x.im)));
-                }
+                for (int j = 0; j < 4; ++j) mx = fmaxf(mx, fmaxf(fabsf(xre[bi][ks][j]), fabsf(xim[bi][ks][j])));
         for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
         // scale: max |x| s < 2^14 (fp16 holds up to 65504)
         int e = 0;
@@ -508,10 +526,15 @@ static int launch_rotate(wbmi_ctx* ctx, int nk, int nw, int nch, const void* U_d
         return WBMI_OK;
     }
     if (nw <= 32) {
-        const size_t lds = (1 + ROTW_WAVES) * rotw_tile_bytes(nw);
+        // one wave (and one LDS tile) per channel to rotate, at most ROTW_WAVES: a call with few
+        // channels (the mixed form's velocity) runs small workgroups, several per CU
+        int nrot = 0;
+        for (int c = 0; c < nch; ++c) nrot += !bit_of(s[0], s[1], c);
+        const int waves = nrot < ROTW_WAVES ? (nrot > 0 ? nrot : 1) : ROTW_WAVES;
+        const size_t lds = (1 + waves) * rotw_tile_bytes(nw);
         WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_rotate_w<T>,
                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_rotate_w<T>, dim3(nk), dim3(64 * ROTW_WAVES), lds, ctx->stream, nw, nch,
+        hipLaunchKernelGGL(k_rotate_w<T>, dim3(nk), dim3(64 * waves), lds, ctx->stream, nw, nch,
                            (const cplx<T>*)U_dev, (cplx<T>*)X_dev, h[0], h[1], s[0], s[1]);
         WBMI_LAUNCH_CHECK(ctx);
         return WBMI_OK;

@@ -42,8 +42,9 @@
 #include "mfma_c64.h"
 #include "wbmi_common.h"
 
-#define SC_THREADS 512
-#define SC_WAVES (SC_THREADS / 64)
+#define SC_THREADS 512       // workgroup size (and launch bound) of the scan kernels
+#define SC_TASK_THREADS 256  // DerOmega / DerMorb on complex64 channels: 4-wave workgroups, two per CU
+#define SC_WAVES_MAX (SC_THREADS / 64)
 #define SC_NWMAX 128
 #define SC_M_NW 64    // dEinv o Off lives in LDS up to this nw, else it is formed on the fly
 #define SC_LDS_NW 32  // V and A live in LDS up to this nw
@@ -69,9 +70,9 @@ struct ScanArgs {
 
 // dynamic shared-memory layout
 struct Smem {
-    cplx<double>* V;    // [3][nw][ld] (staged) or null
-    cplx<double>* A;    // [3][nw][ld] (staged, external terms) or null
-    double* accw;       // [SC_WAVES][nw][9] per-wave DerOmega accumulators
+    char* V;            // [3][nw][ld] staged V, in the storage type of the channels
+    char* A;            // [3][nw][ld] staged A (external terms)
+    double* accw;       // [waves][nw][9] per-wave DerOmega accumulators
     cplx<double>* acc;  // [nw][9]
     double* M;          // [nw][ld] dEinv o Off
     double* E;
@@ -103,17 +104,18 @@ struct SmemOffsets {
     size_t V, A, accw, acc, M, E, cut, gw, ints, tet, tints, total;
 };
 
+// vsz: bytes of one staged V / A element (the channels' storage type)
 __host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool stage_a, bool stage_w,
-                                                    bool tetra) {
+                                                    bool tetra, size_t vsz, int nwaves) {
     const size_t ld = nw + 1;
     SmemOffsets o;
     size_t off = 0;
     o.V = off;
-    off = al16(off + (stage_v ? sizeof(cplx<double>) * 3 * nw * ld : 0));
+    off = al16(off + (stage_v ? vsz * 3 * nw * ld : 0));
     o.A = off;
-    off = al16(off + (stage_a ? sizeof(cplx<double>) * 3 * nw * ld : 0));
+    off = al16(off + (stage_a ? vsz * 3 * nw * ld : 0));
     o.accw = off;
-    off = al16(off + (stage_w ? sizeof(double) * SC_WAVES * nw * 9 : 0));
+    off = al16(off + (stage_w ? sizeof(double) * nwaves * nw * 9 : 0));
     o.acc = off;
     off = al16(off + sizeof(cplx<double>) * nw * 9);
     o.M = off;
@@ -135,12 +137,12 @@ __host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool s
 }
 
 __device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int nw, bool stage_v, bool stage_a,
-                                                                bool stage_w, bool tetra) {
-    const SmemOffsets o = smem_offsets(nw, stage_v, stage_a, stage_w, tetra);
+                                                                bool stage_w, bool tetra, size_t vsz) {
+    const SmemOffsets o = smem_offsets(nw, stage_v, stage_a, stage_w, tetra, vsz, (int)(blockDim.x >> 6));
     Smem s;
     // LDS addresses even where a region is empty (never read then): no null tests on LDS pointers
-    s.V = (cplx<double>*)(base + o.V);
-    s.A = (cplx<double>*)(base + o.A);
+    s.V = base + o.V;
+    s.A = base + o.A;
     s.accw = (double*)(base + o.accw);
     s.acc = (cplx<double>*)(base + o.acc);
     s.M = (double*)(base + o.M);  // empty region (never read) when nw > SC_M_NW
@@ -171,10 +173,11 @@ __device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int 
 
 // Element access to the channels of one k-point.
 // LM: dEinv o Off read from LDS (nw <= SC_M_NW) or formed from E and the labels.
-// ST: V and A read from their LDS copies (DerOmega, DerMorb and Omega at nw <= SC_LDS_NW;
-// A only with external terms, the only case that reads it) or from the channels.
+// ST: V read from its LDS copy (DerOmega, DerMorb and Omega at nw <= SC_LDS_NW) or from the
+// channels; SA: the same for A (staged together with V, and only with external terms, the only
+// case that reads it).
 // Both are compile-time so that every access is a plain LDS or global load.
-template <typename T, bool LM, bool ST>
+template <typename T, bool LM, bool ST, bool SA>
 struct KCtx {
     const cplx<T>* X;  // channels of this k: [nch][nw][nw]
     int nw, ld;
@@ -184,23 +187,27 @@ struct KCtx {
     const int* lab;
     const int* gs;
     const int* ge;
-    const cplx<double>* sV;  // LDS copies of V and A (ST)
-    const cplx<double>* sA;
+    const cplx<T>* sV;  // LDS copies of V and A (ST), in the storage type
+    const cplx<T>* sA;
     __device__ inline cplx<double> get(int role, int x, int y) const {
         cplx<T> v = X[((int64_t)ch[role] * nw + x) * nw + y];
         return mk((double)v.re, (double)v.im);
     }
     __device__ inline cplx<double> V(int a, int x, int y) const {
-        if constexpr (ST)
-            return sV[(a * nw + x) * ld + y];
-        else
+        if constexpr (ST) {
+            const cplx<T> v = sV[(a * nw + x) * ld + y];
+            return mk((double)v.re, (double)v.im);
+        } else {
             return get(WBMI_ROLE_V + a, x, y);
+        }
     }
     __device__ inline cplx<double> A(int a, int x, int y) const {
-        if constexpr (ST)
-            return sA[(a * nw + x) * ld + y];
-        else
+        if constexpr (SA) {
+            const cplx<T> v = sA[(a * nw + x) * ld + y];
+            return mk((double)v.re, (double)v.im);
+        } else {
             return get(WBMI_ROLE_A + a, x, y);
+        }
     }
     __device__ inline cplx<double> W(int a, int b, int x, int y) const { return get(WBMI_ROLE_W + 3 * a + b, x, y); }
     __device__ inline cplx<double> Ad(int a, int d, int x, int y) const { return get(WBMI_ROLE_AD + 3 * a + d, x, y); }
@@ -231,8 +238,8 @@ struct KCtx {
 // pre-Hermitian Omega summand s^c_xy for x, y in the same group
 // (formula/covariant.py:175-200): Omega.nn = s + s^+.  The sum over the
 // off-group bands l runs over [l0, l1); `tail` adds the within-group and O terms.
-template <typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal,
+template <typename T, bool LM, bool ST, bool SA>
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA>& K, int c, int x, int y, bool internal,
                                                               bool external, int l0, int l1, bool tail) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
@@ -248,16 +255,16 @@ __device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM,
     }
     return s;
 }
-template <typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal,
+template <typename T, bool LM, bool ST, bool SA>
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA>& K, int c, int x, int y, bool internal,
                                                               bool external) {
     return omega_s(K, c, x, y, internal, external, 0, K.nw, true);
 }
 
 // pre-Hermitian Morb_H summand h^c_xy for x, y in the same group
 // (formula/covariant.py:390-414)
-template <typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM, ST>& K, int c, int x, int y, bool internal, bool external) {
+template <typename T, bool LM, bool ST, bool SA>
+__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM, ST, SA>& K, int c, int x, int y, bool internal, bool external) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
     for (int l = 0; l < K.nw; ++l) {
@@ -309,7 +316,7 @@ __device__ inline Pair pair_first(int x, int which) {
 // build M = dEinv o Off for the current labels (data_K.py:290-298)
 __device__ __attribute__((always_inline)) void build_M(const Smem& S, int nw) {
     const int ld = nw + 1;
-    for (int t = threadIdx.x; t < nw * nw; t += SC_THREADS) {
+    for (int t = threadIdx.x; t < nw * nw; t += (int)blockDim.x) {
         int x = t / nw, y = t % nw;
         double m = 0.0;
         if (S.lab[x] != S.lab[y]) {
@@ -361,8 +368,8 @@ __host__ __device__ constexpr int task_nt() {
 // kind 1's transposed product is -conj of kind 0's at the same element, and
 // its elementwise channel W^{db}_yx = conj(W^{bd}_xy): one product and one
 // load serve both consumers.
-template <int KIND, bool M1, typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST>& K, const TaskCtx& tc_,
+template <int KIND, bool M1, typename T, bool LM, bool ST, bool SA>
+__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST, SA>& K, const TaskCtx& tc_,
                                                            double* accw, int b, int d, int rb, int cb) {
     const int nw = K.nw, lane = threadIdx.x & 63;
     const int r0 = rb * 32, c0 = cb * 32;
@@ -644,12 +651,12 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
 // DerMorb = DerMorb_H + sign [Eav o dOmega + (Omega V + V Omega)/2] on cut
 // partitions; its trace terms (s1 ... s7, the Omega block) are listed at
 // formula_task.
-template <typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST>& K, bool in, bool ex,
+template <typename T, bool LM, bool ST, bool SA>
+__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST, SA>& K, bool in, bool ex,
                                                             bool dm, int sign) {
     const int nw = K.nw, tid = threadIdx.x, wave = tid >> 6;
     // per-band terms, lanes running over (n, c, d)
-    for (int t = tid; t < nw * 9; t += SC_THREADS) {
+    for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
         const int n = t % nw, c = (t / nw) / 3, d = (t / nw) % 3;
         double v = 0.0;
         if (S.act[n] && ex) {
@@ -690,7 +697,8 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
         }
         S.acc[n * 9 + c * 3 + d] = mk(v, 0.0);
     }
-    for (int t = tid; t < SC_WAVES * nw * 9; t += SC_THREADS) S.accw[t] = 0.0;
+    const int nwaves = (int)(blockDim.x >> 6);
+    for (int t = tid; t < nwaves * nw * 9; t += (int)blockDim.x) S.accw[t] = 0.0;
     __syncthreads();
     // wave tasks (kind, b, d, row block, column block), round-robin over the waves
     const TaskCtx tcx{dm, in, ex, (double)sign};
@@ -712,7 +720,7 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
 #pragma unroll
     for (int q = 0; q < 8; ++q) ntask += cnt[q] * nb2;
     double* accw = S.accw + wave * nw * 9;
-    for (int t = wave; t < ntask; t += SC_WAVES) {
+    for (int t = wave; t < ntask; t += nwaves) {
         int kind = -1, rem = 0, base = 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -742,31 +750,31 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
         }
     }
     __syncthreads();
-    for (int t = tid; t < nw * 9; t += SC_THREADS) {
+    for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
         double v = S.acc[t].re;
-        for (int w = 0; w < SC_WAVES; ++w) v += S.accw[w * nw * 9 + t];
+        for (int w = 0; w < nwaves; ++w) v += S.accw[w * nw * 9 + t];
         S.acc[t] = mk(v, 0.0);
     }
 }
 
 // Per-band values for the current partition into S.acc[n][comp] (.re).
-template <int F, typename T, bool LM, bool ST>
-__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM, ST>& K, const wbmi_scan_params& p) {
+template <int F, typename T, bool LM, bool ST, bool SA>
+__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM, ST, SA>& K, const wbmi_scan_params& p) {
     const int nw = K.nw;
     const int tid = threadIdx.x;
     const bool in = p.internal_terms, ex = p.external_terms;
     if constexpr (LM) build_M(S, nw);
-    for (int t = tid; t < nw * 9; t += SC_THREADS) S.acc[t] = mk(0.0, 0.0);
+    for (int t = tid; t < nw * 9; t += (int)blockDim.x) S.acc[t] = mk(0.0, 0.0);
     __syncthreads();
     if (F == WBMI_FORMULA_IDENTITY) {
-        for (int n = tid; n < nw; n += SC_THREADS)
+        for (int n = tid; n < nw; n += (int)blockDim.x)
             if (S.act[n]) S.acc[n * 9] = mk(1.0, 0.0);
     } else if (F == WBMI_FORMULA_OMEGA) {
         if constexpr (ST) {
             // V and A in LDS: four adjacent lanes share (n, c) and split the sum over l,
             // reduced by two shuffles (a uniform loop, so every lane reaches them)
             const int L4 = (nw + 3) / 4;
-            for (int t0 = 0; t0 < nw * 12; t0 += SC_THREADS) {
+            for (int t0 = 0; t0 < nw * 12; t0 += (int)blockDim.x) {
                 const int t = t0 + tid;
                 const bool ok = t < nw * 12;
                 const int n = ok ? t / 12 : 0, c = (t / 4) % 3, qq = t % 4;
@@ -781,14 +789,14 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
                 if (act && qq == 0) S.acc[n * 9 + c] = mk(2.0 * v, 0.0);
             }
         } else {
-            for (int t = tid; t < nw * 3; t += SC_THREADS) {
+            for (int t = tid; t < nw * 3; t += (int)blockDim.x) {
                 int n = t / 3, c = t % 3;
                 if (!S.act[n]) continue;
                 S.acc[n * 9 + c] = mk(2.0 * omega_s(K, c, n, n, in, ex).re, 0.0);
             }
         }
     } else if (F == WBMI_FORMULA_MORB_HPM) {
-        for (int t = tid; t < nw * 3; t += SC_THREADS) {
+        for (int t = tid; t < nw * 3; t += (int)blockDim.x) {
             int n = t / 3, c = t % 3;
             if (!S.act[n]) continue;
             double v = 2.0 * morbh_s(K, c, n, n, in, ex).re;
@@ -797,7 +805,7 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
         }
     } else if (F == WBMI_FORMULA_VELOMEGA) {
         // Re sum_{n' in G} V^a_nn' Omega^b_n'n, Omega = s + s^+ (formula/formula.py:141-148)
-        for (int t = tid; t < nw * 9; t += SC_THREADS) {
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
             int n = t / 9, a = (t % 9) / 3, b = t % 3;
             if (!S.act[n]) continue;
             cplx<double> v = mk(0.0, 0.0);
@@ -811,7 +819,7 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
         // Re sum_{n' in G} V^a_nn' Hpm^b_n'n with the group block of Morb_Hpm,
         // Hpm = Morb_H.nn + sign Eav o Omega.nn (formula/covariant.py:424-449,
         // formula/formula.py:141-148; FormulaProduct is additive)
-        for (int t = tid; t < nw * 9; t += SC_THREADS) {
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
             int n = t / 9, a = (t % 9) / 3, b = t % 3;
             if (!S.act[n]) continue;
             cplx<double> v = mk(0.0, 0.0);
@@ -827,7 +835,7 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
     } else if (F == WBMI_FORMULA_DERSPIN) {
         // generalised derivative of the spin (formula/formula.py:95-107 with A = Xbar('SS'),
         // dA = Xbar('SS', 1)): Re [S'^{bd}_nn - sum_{l off} D^d_nl S^b_ln + sum_{l off} S^b_nl D^d_ln]
-        for (int t = tid; t < nw * 9; t += SC_THREADS) {
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
             const int n = t / 9, b = (t % 9) / 3, d = t % 3;
             if (!S.act[n]) continue;
             cplx<double> v = K.Sd(b, d, n, n);
@@ -839,7 +847,7 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
         }
     } else if (F == WBMI_FORMULA_VELSPIN) {
         // Re sum_{n' in G} V^a_nn' S^b_n'n (formula/formula.py:121-148, covariant.py:811-814)
-        for (int t = tid; t < nw * 9; t += SC_THREADS) {
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
             const int n = t / 9, a = (t % 9) / 3, b = t % 3;
             if (!S.act[n]) continue;
             cplx<double> v = mk(0.0, 0.0);
@@ -953,7 +961,7 @@ __device__ inline int ef_bound(const double* __restrict__ Ef, int n, double x, b
 // in between are evaluated one by one, spread over all threads.
 static __device__ void tetra_accumulate(const Smem& S, const ScanArgs& args, int ncomp, double* __restrict__ hist) {
     const int nw = args.nw, nEF = args.nEF, der = args.p.fder, tid = threadIdx.x;
-    for (int b = tid; b < nw; b += SC_THREADS) {
+    for (int b = tid; b < nw; b += (int)blockDim.x) {
         const int g = S.tg[b];
         int lo = 0, hi = 0;
         if (g >= 0 && S.gin[g]) {  // g < 0: a band in no group (the odd last band under degen_Kramers)
@@ -980,7 +988,7 @@ static __device__ void tetra_accumulate(const Smem& S, const ScanArgs& args, int
     }
     __syncthreads();
     const int total = S.tpre[nw];
-    for (int t = tid; t < total; t += SC_THREADS) {
+    for (int t = tid; t < total; t += (int)blockDim.x) {
         // band b with tpre[b] <= t < tpre[b + 1]
         int lo = 0, hi = nw;
         while (hi - lo > 1) {
@@ -1023,22 +1031,23 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     if (MODE != 2) hist += k * args.hist_stride;
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
     static_assert(tasks || F == WBMI_FORMULA_OMEGA || !ST, "V and A are staged for DerOmega, DerMorb and Omega");
+    constexpr bool SA = ST;
     constexpr bool stage_v = ST;
-    const bool stage_a = ST && p.external_terms;
+    const bool stage_a = SA && p.external_terms;
     constexpr bool tetra = MODE == 1;  // compile-time: the Fermi-bin kernels carry no tetrahedron code
     constexpr bool tab = MODE == 2;
-    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tasks, tetra);
-    for (int n = tid; n < nw; n += SC_THREADS) S.E[n] = Eall[k * nw + n];
-    for (int r = tid; r < WBMI_NROLES; r += SC_THREADS) S.ch[r] = p.ch[r];
+    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tasks, tetra, sizeof(cplx<T>));
+    for (int n = tid; n < nw; n += (int)blockDim.x) S.E[n] = Eall[k * nw + n];
+    for (int r = tid; r < WBMI_NROLES; r += (int)blockDim.x) S.ch[r] = p.ch[r];
     if (tetra)
-        for (int t = tid; t < args.tetra * nw; t += SC_THREADS) {
+        for (int t = tid; t < args.tetra * nw; t += (int)blockDim.x) {
             const int c = t / nw, b = t % nw;
             S.tC[b * 8 + c] = args.Ec[c * args.ec_stride + k * nw + b];
         }
     __syncthreads();
     if (tetra) {
         // band energy range over the centre and the corners (tetrahedron.py:189-193)
-        for (int b = tid; b < nw; b += SC_THREADS) {
+        for (int b = tid; b < nw; b += (int)blockDim.x) {
             double mn = S.E[b], mx = S.E[b];
             for (int c = 0; c < args.tetra; ++c) {
                 mn = fmin(mn, S.tC[b * 8 + c]);
@@ -1051,14 +1060,14 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     }
     const cplx<T>* Xk = Xall + k * (int64_t)args.nch * nw * nw;
     if (stage_v || stage_a) {
-        // staged only for nw <= SC_LDS_NW: 3 nw^2 <= 3072 = 6 x SC_THREADS elements per array;
+        // staged only for nw <= SC_LDS_NW: 3 nw^2 <= 3072 = 12 x SC_TASK_THREADS elements per array;
         // all of a thread's loads are issued before its LDS stores
-        constexpr int NS = (3 * SC_LDS_NW * SC_LDS_NW + SC_THREADS - 1) / SC_THREADS;
+        constexpr int NS = (3 * SC_LDS_NW * SC_LDS_NW + SC_TASK_THREADS - 1) / SC_TASK_THREADS;
         const int nw2 = nw * nw;
         cplx<T> bv[NS], ba[NS];
 #pragma unroll
         for (int u = 0; u < NS; ++u) {
-            const int t = tid + SC_THREADS * u;
+            const int t = tid + (int)blockDim.x * u;
             if (t < 3 * nw2) {
                 const int a = t / nw2, xy = t % nw2;
                 if (stage_v) bv[u] = Xk[(int64_t)S.ch[WBMI_ROLE_V + a] * nw2 + xy];
@@ -1067,11 +1076,11 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
         }
 #pragma unroll
         for (int u = 0; u < NS; ++u) {
-            const int t = tid + SC_THREADS * u;
+            const int t = tid + (int)blockDim.x * u;
             if (t < 3 * nw2) {
                 const int a = t / nw2, x = (t % nw2) / nw, y = t % nw;
-                if (stage_v) S.V[(a * nw + x) * ld + y] = mk((double)bv[u].re, (double)bv[u].im);
-                if (stage_a) S.A[(a * nw + x) * ld + y] = mk((double)ba[u].re, (double)ba[u].im);
+                if (stage_v) ((cplx<T>*)S.V)[(a * nw + x) * ld + y] = bv[u];
+                if (stage_a) ((cplx<T>*)S.A)[(a * nw + x) * ld + y] = ba[u];
             }
         }
     }
@@ -1081,26 +1090,35 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                               F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERSPIN ||
                               F == WBMI_FORMULA_VELSPIN;
 
-    // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186)
-    if (tid == 0) {
-        S.gb[0] = 0;
-        int nb = 1;
-        for (int i = 1; i < nw; ++i)
-            if (S.E[i] - S.E[i - 1] > p.degen_thresh) S.gb[nb++] = i;
-        S.gb[nb++] = nw;
-        if (p.degen_Kramers) {
-            int m = 0;
-            for (int i = 0; i < nb; ++i)
-                if (S.gb[i] % 2 == 0) S.gb[m++] = S.gb[i];
-            nb = m;
-        }
+    // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186), formed by wave 0: each lane
+    // owns bands lane and lane + 64 (nw <= 128); borders from two ballots and popcounts, one
+    // lane per group for its range test and mean energy, wave reductions for the sea group
+    if (tid < 64) {
+        const int lane = tid;
+        const double thr = p.degen_thresh;
+        auto is_border = [&](int i) {  // i in [0, nw]
+            bool b = i == 0 || i == nw || (i < nw && S.E[i] - S.E[i - 1] > thr);
+            if (p.degen_Kramers && (i & 1)) b = false;
+            return b;
+        };
+        const bool b0 = lane <= nw && is_border(lane);
+        const bool b1 = lane + 64 <= nw && is_border(lane + 64);
+        const bool b2 = lane == 0 && nw == 128 && is_border(128);
+        const uint64_t m0 = __ballot(b0), m1 = __ballot(b1);
+        const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+        const int n0 = __popcll(m0), n1 = __popcll(m1);
+        if (b0) S.gb[__popcll(m0 & below)] = lane;
+        if (b1) S.gb[n0 + __popcll(m1 & below)] = lane + 64;
+        if (b2) S.gb[n0 + n1] = 128;
+        const int nb = n0 + n1 + (nw == 128 && is_border(128) ? 1 : 0);
         const int ng = nb - 1;
         // degen_Kramers keeps only even borders, so with odd nw the last band is in no group
         // (grid/tetrahedron.py:132-136): it is an "out" band of every group and adds nothing itself
-        for (int i = 0; i < nw; ++i) S.tg[i] = -1;
-        int first_in = -1, last_in = 0;
-        for (int g = 0; g < ng; ++g) {
-            int a = S.gb[g], b = S.gb[g + 1];
+        for (int i = lane; i < nw; i += 64) S.tg[i] = -1;
+        wave_lds_sync();
+        int first_in = nw, last_in = 0;
+        for (int g = lane; g < ng; g += 64) {
+            const int a = S.gb[g], b = S.gb[g + 1];
             int inr;
             if (tab) {
                 inr = 1;  // every group (Tabulator, tabulate.py:25-31)
@@ -1117,40 +1135,51 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 inr = (S.E[b - 1] >= p.EFmin) && (S.E[a] <= p.EFmax);
             }
             S.gin[g] = inr;
-            if (inr && first_in < 0) first_in = a;
-            if (inr) last_in = b;
+            if (inr) {
+                first_in = min(first_in, a);
+                last_in = max(last_in, b);
+            }
             double sum = 0.0;
             for (int i = a; i < b; ++i) sum += S.E[i];
             S.gw[g] = sum / (b - a);
+        }
+        for (int o = 32; o >= 1; o >>= 1) {
+            first_in = min(first_in, __shfl_xor(first_in, o));
+            last_in = max(last_in, __shfl_xor(last_in, o));
         }
         int sea_lo = 0, sea_hi = 0;
         if (p.fder == 0 && !tab) {
             // bands below the range, up to the first group in it (data_K.py:178-185;
             // tetra: highest energy over centre and corners below Ef[0], tetrahedron.py:155-162)
-            for (int i = 0; i < nw; ++i)
-                if ((tetra ? S.tEmx[i] : S.E[i]) < p.EFmin) sea_hi = i + 1;
-            if (first_in >= 0 && first_in < sea_hi) sea_hi = first_in;
+            int hi = 0;
+            for (int i = lane; i < nw; i += 64)
+                if ((tetra ? S.tEmx[i] : S.E[i]) < p.EFmin) hi = i + 1;
+            for (int o = 32; o >= 1; o >>= 1) hi = max(hi, __shfl_xor(hi, o));
+            sea_hi = first_in < hi ? first_in : hi;
         } else if (tetra && p.fder == -1) {
             // hole-like: bands whose lowest energy lies above Ef[-1], from the end of the last
             // group in range on (tetrahedron.py:165-172,246-253)
             int bandmin = nw;
-            for (int i = nw - 1; i >= 0; --i)
-                if (S.tEmn[i] > p.EFmax) bandmin = i;
+            for (int i = lane; i < nw; i += 64)
+                if (S.tEmn[i] > p.EFmax) bandmin = min(bandmin, i);
+            for (int o = 32; o >= 1; o >>= 1) bandmin = min(bandmin, __shfl_xor(bandmin, o));
             if (bandmin < last_in) bandmin = last_in;
             if (bandmin < nw) {
                 sea_lo = bandmin;
                 sea_hi = nw;
             }
         }
-        S.misc[0] = ng;
-        S.misc[1] = sea_lo;
-        S.misc[2] = sea_hi;
+        if (lane == 0) {
+            S.misc[0] = ng;
+            S.misc[1] = sea_lo;
+            S.misc[2] = sea_hi;
+        }
     }
     __syncthreads();
     const int ng = S.misc[0];
     const int sea_lo = S.misc[1], sea_hi = S.misc[2];
 
-    KCtx<T, LM, ST> K;
+    KCtx<T, LM, ST, SA> K;
     K.X = Xk;
     K.nw = nw;
     K.ld = ld;
@@ -1160,14 +1189,14 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     K.lab = S.lab;
     K.gs = S.gs;
     K.ge = S.ge;
-    K.sV = S.V;
-    K.sA = S.A;
+    K.sV = (const cplx<T>*)S.V;
+    K.sA = (const cplx<T>*)S.A;
 
     if (additive || tab) {
         // main partition: every group at once (each group's trace with inn = the group,
         // out = the rest: the additive scan and the per-band table of any formula); a band in
         // no group (label -1) is an out band of every group
-        for (int n = tid; n < nw; n += SC_THREADS) {
+        for (int n = tid; n < nw; n += (int)blockDim.x) {
             S.lab[n] = -1;
             S.gs[n] = n;
             S.ge[n] = n;
@@ -1175,7 +1204,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
         }
         __syncthreads();
         for (int g = 0; g < ng; ++g)
-            for (int n = S.gb[g] + tid; n < S.gb[g + 1]; n += SC_THREADS) {
+            for (int n = S.gb[g] + tid; n < S.gb[g + 1]; n += (int)blockDim.x) {
                 S.lab[n] = g;
                 S.gs[n] = S.gb[g];
                 S.ge[n] = S.gb[g + 1];
@@ -1183,7 +1212,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             }
         __syncthreads();
         eval_partition<F>(S, K, p);
-        for (int t = tid; t < ng * ncomp; t += SC_THREADS) {
+        for (int t = tid; t < ng * ncomp; t += (int)blockDim.x) {
             const int g = t / ncomp, c = t % ncomp;
             if (tab) {
                 double v;
@@ -1215,7 +1244,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             // Fermi-sea group [sea_lo, sea_hi) (weight -inf; hole-like tetra: the bands above),
             // added at every level; the other bands, on one side of it, are its "out"
             const int out_lo = sea_lo == 0 ? sea_hi : 0, out_hi = sea_lo == 0 ? nw : sea_lo;
-            for (int n = tid; n < nw; n += SC_THREADS) {
+            for (int n = tid; n < nw; n += (int)blockDim.x) {
                 int inn = n >= sea_lo && n < sea_hi;
                 S.lab[n] = inn ? 0 : 1;
                 S.gs[n] = inn ? sea_lo : out_lo;
@@ -1224,7 +1253,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             }
             __syncthreads();
             eval_partition<F>(S, K, p);
-            for (int c = tid; c < ncomp; c += SC_THREADS) {
+            for (int c = tid; c < ncomp; c += (int)blockDim.x) {
                 double s = 0.0;
                 for (int n = sea_lo; n < sea_hi; ++n) s += S.acc[n * 9 + c].re;
                 unsafeAtomicAdd(hist + c, args.weight * s);  // bin 0: every level
@@ -1238,7 +1267,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             for (int g = 0; g < ng; ++g)
                 if (S.gin[g] && (S.gb[g] == cidx || S.gb[g + 1] == cidx)) need = true;
             if (!need) continue;
-            for (int n = tid; n < nw; n += SC_THREADS) {
+            for (int n = tid; n < nw; n += (int)blockDim.x) {
                 int inn = n < cidx;
                 S.lab[n] = inn ? 0 : 1;
                 S.gs[n] = inn ? 0 : cidx;
@@ -1255,9 +1284,9 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
                 }
             __syncthreads();
         }
-        for (int c = tid; c < ncomp; c += SC_THREADS) S.cut[c] = 0.0;
+        for (int c = tid; c < ncomp; c += (int)blockDim.x) S.cut[c] = 0.0;
         __syncthreads();
-        for (int t = tid; t < ng * ncomp; t += SC_THREADS) {
+        for (int t = tid; t < ng * ncomp; t += (int)blockDim.x) {
             const int g = t / ncomp, c = t % ncomp;
             if (tetra) {
                 if (S.gin[g]) S.tv[g * 9 + c] = S.cut[S.gb[g + 1] * 9 + c] - S.cut[S.gb[g] * 9 + c];
@@ -1269,7 +1298,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
             unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s);
         }
         if (sea_hi > sea_lo)
-            for (int c = tid; c < ncomp; c += SC_THREADS)
+            for (int c = tid; c < ncomp; c += (int)blockDim.x)
                 unsafeAtomicAdd(hist + c, args.weight * (S.cut[sea_hi * 9 + c] - S.cut[sea_lo * 9 + c]));
     }
     if (tetra) {
@@ -1282,11 +1311,15 @@ template <typename T, int F, bool LM, bool ST, int MODE>
 static int launch_scan_t(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
     const bool stage_a = ST && a.p.external_terms;
-    size_t lds = smem_offsets(a.nw, ST, stage_a, tasks, MODE == 1).total;
+    // complex64 DerOmega / DerMorb: 4-wave workgroups, so that two of them (LDS ~75 KB each with V
+    // and A staged as complex64) share a CU; complex128: 8 waves, one workgroup per CU
+    const int threads = tasks && sizeof(T) == 4 ? SC_TASK_THREADS : SC_THREADS;
+    size_t lds = smem_offsets(a.nw, ST, stage_a, tasks, MODE == 1, sizeof(cplx<T>), threads / 64).total;
     if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
     WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM, ST, MODE>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_static_scan<T, F, LM, ST, MODE>), dim3(nk), dim3(SC_THREADS), lds, ctx->stream, a, E,
+    hipLaunchKernelGGL((k_static_scan<T, F, LM, ST, MODE>), dim3(nk), dim3(threads), lds,
+                       ctx->stream, a, E,
                        (const cplx<T>*)X, hist);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;

@@ -138,7 +138,9 @@ def _is_aliased(iRvec, NKFFT):
 
 
 def device_system(system, ctx):
-    key = (ctx.device.index, ctx.stream.cuda_stream)
+    # one copy per device: the context launches on whatever stream it holds at the time
+    # (a graph capture switches it to the capture stream)
+    key = ctx.device.index
     cache = system._device_cache
     if key not in cache:
         cache[key] = DeviceSystem(system, ctx)
@@ -160,7 +162,7 @@ class Data_K:
 
     def __init__(self, system, dK=None, grid=None, Kpoint=None, k_list=None, Emin=-np.inf, Emax=np.inf,
                  fftlib=None, precision="fp64", channels=None, device=0, random_gauge=False, split_rotation=True,
-                 **unused):
+                 k_list_dev=None, **unused):
         import torch
         self.torch = torch
         if not torch.cuda.is_available():
@@ -181,6 +183,8 @@ class Data_K:
         self.grid = grid
         self.Kpoint = Kpoint
         self.k_list = None if k_list is None else np.array(k_list, dtype=float).reshape(-1, 3)
+        # the k-list already on the device (float64 [nk, 3]): no host-to-device copy per evaluation
+        self.k_list_dev = k_list_dev
         if self.k_list is None:
             self.dK = np.array(dK, dtype=float)
             self.NKFFT = np.array(grid.FFT, dtype=int)
@@ -229,7 +233,7 @@ class Data_K:
             plan.execute(stack if aliased else None, box, self.dK if shift is None else shift, work, X, prec)
             del work
         else:
-            kl = torch.from_numpy(self.k_list).to(self.ctx.device)
+            kl = self.k_list_dev if self.k_list_dev is not None else torch.from_numpy(self.k_list).to(self.ctx.device)
             self.ctx.r2k_klist(self.dev.iRvec_dev, stack, kl, X, prec)
         return X
 

@@ -109,6 +109,20 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
     return result_all
 
 
+def _klist_batches(sizes, batch_points):
+    """Consecutive lists packed into batches of up to `batch_points` points (a longer list is a
+    batch of its own)."""
+    batches, cur, npts = [], [], 0
+    for i, n in enumerate(sizes):
+        if cur and npts + n > batch_points:
+            batches.append(cur)
+            cur, npts = [], 0
+        cur.append(i)
+        npts += int(n)
+    batches.append(cur)
+    return batches
+
+
 def run_klists(system, k_lists, calculators, weights=None, parameters_K=None, batch_points=1 << 15,
                per_list=False):
     """Static calculators over many k-lists, batched into few large launches.
@@ -142,14 +156,7 @@ def run_klists(system, k_lists, calculators, weights=None, parameters_K=None, ba
     weights = sizes / ntot if uniform else np.asarray(weights, dtype=float).reshape(-1)
     if weights.shape[0] != len(k_lists):
         raise ValueError(f"{weights.shape[0]} weights for {len(k_lists)} k-lists")
-    batches, cur, npts = [], [], 0
-    for i, n in enumerate(sizes):
-        if cur and npts + n > batch_points:
-            batches.append(cur)
-            cur, npts = [], 0
-        cur.append(i)
-        npts += int(n)
-    batches.append(cur)
+    batches = _klist_batches(sizes, batch_points)
     total, each = None, [None] * len(k_lists)
     for b in batches:
         data = Data_K(system, k_list=np.concatenate([k_lists[i] for i in b]), **parameters_K)
@@ -167,3 +174,82 @@ def run_klists(system, k_lists, calculators, weights=None, parameters_K=None, ba
                 total = contrib if total is None else total + contrib
         del data
     return (total, each) if per_list else total
+
+
+class KlistGraph:
+    """Uniformly weighted k-lists of several systems (any mix of nw) as one HIP graph.
+
+    `jobs` is a list of (system, k_lists); each system's total is the run_klists
+    total over its lists (every point weighted 1 / its system's points).  The
+    constructor packs each system's lists into batches (run_klists), puts every
+    batch's k-points on the device once, runs the whole step eagerly (device
+    systems, channel stacks, workspaces and Fermi grids are created there) and
+    captures a second pass into one graph on a side stream: R -> k, eigh,
+    rotation, scan and finalisation of every batch of every system, with the
+    k-sum.  `replay()` launches that graph and returns one ResultDict per system,
+    whose device arrays are the graph's outputs (overwritten by the next replay;
+    read or copy them before).  The library calls inside the graph allocate
+    nothing and clear accumulators with fill kernels, so replays are
+    independent.
+    """
+
+    def __init__(self, jobs, calculators, parameters_K=None, batch_points=1 << 15, device=0):
+        import torch
+        from . import _lib
+        self.torch = torch
+        self.calculators = calculators
+        self.parameters_K = dict(parameters_K or {})
+        self.ctx = _lib.get_context(device)
+        self.batches = []   # per system: [(k_list, k_list_dev, weight)]
+        for system, k_lists in jobs:
+            k_lists = [np.asarray(k, dtype=float).reshape(-1, 3) for k in k_lists]
+            sizes = np.array([len(k) for k in k_lists], dtype=np.int64)
+            if len(k_lists) == 0 or sizes.min() <= 0:
+                raise ValueError("KlistGraph needs at least one list per system and no empty list")
+            ntot = int(sizes.sum())
+            need = set()
+            for calc in calculators.values():
+                if hasattr(calc, "needed_channels"):
+                    need |= set(calc.needed_channels(system))
+            rows = []
+            for b in _klist_batches(sizes, batch_points):
+                kl = np.concatenate([k_lists[i] for i in b])
+                rows.append((kl, torch.from_numpy(kl).to(self.ctx.device), len(kl) / ntot))
+            self.batches.append((system, need, rows))
+        self.npoints = sum(len(kl) for _, _, rows in self.batches for kl, _, _ in rows)
+        self._step()                       # eager: set-up and first-use allocations
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(device=self.ctx.device)
+        side.wait_stream(torch.cuda.current_stream(self.ctx.device))
+        prev = self.ctx.set_stream(side)
+        try:
+            with torch.cuda.graph(self.graph, stream=side):
+                self.results = self._step()
+        finally:
+            self.ctx.set_stream(prev)
+        torch.cuda.current_stream(self.ctx.device).wait_stream(side)
+
+    def _step(self):
+        out = []
+        for system, need, rows in self.batches:
+            params = dict(self.parameters_K)
+            params.setdefault("channels", need)
+            total = None
+            for kl, kl_dev, w in rows:
+                data = Data_K(system, k_list=kl, k_list_dev=kl_dev, **params)
+                res = ResultDict({k: c(data) for k, c in self.calculators.items()}) * w
+                total = res if total is None else total + res
+                del data
+            out.append(total)
+        return out
+
+    def replay(self):
+        """Launch the graph on the current stream; returns the per-system ResultDicts."""
+        self.graph.replay()
+        for rd in self.results:
+            for r in rd.results.values():
+                # host copies of the previous replay's outputs are stale now
+                r._np = None
+                r._np_smooth = None
+        return self.results

@@ -153,6 +153,69 @@ class System_R:
         cR = self.rvec.iRvec.dot(self.real_lattice)
         return cR[:, None, None, :] + (-shifts[:, None, :] + shifts[None, :, :])[None]
 
+    # -- Wigner-Seitz (MDRS) fold ----------------------------------------------
+    def do_ws_dist(self, mp_grid, wannier_centers_cart=None, ws_dist_tol=1e-5):
+        """Minimal-distance replica selection (system_R.py:572-588, rvectors.py:57-100,126-200,570-609).
+
+        Every matrix is first folded onto the mp_grid cell (X_grid[R mod mp] = sum of X_R).
+        Then, for the element (m, n) and each cell vector R0, the translates R0 + mp * T
+        (T in {-3..3}^3) that minimise |R + t_n - t_m| (Cartesian; ties within ws_dist_tol)
+        each get X_grid[R0][m, n] / (number of ties).  The new R-list is the union over all
+        element shifts, R-vectors whose matrices all vanish (< 1e-8) are dropped, and
+        NKFFT_recommended becomes mp_grid.  A negative ws_dist_tol is used as |ws_dist_tol|,
+        with the shifts compared to 8 decimals (rvectors.py:78-86).
+        """
+        mp = np.array(mp_grid, dtype=int).reshape(-1)
+        if mp.size == 1:
+            mp = np.repeat(mp, 3)
+        if mp.size != 3 or np.any(mp <= 0):
+            raise ValueError(f"mp_grid should be one integer, or three integers. found {mp_grid}")
+        wcc = self.wannier_centers_cart if wannier_centers_cart is None else np.asarray(wannier_centers_cart, float)
+        tol = abs(ws_dist_tol)
+        digits = 8 if ws_dist_tol < 0 else int(np.ceil(-np.log10(ws_dist_tol))) + 1
+        L = self.real_lattice
+        t_red = wcc.dot(np.linalg.inv(L))
+        nw = t_red.shape[0]
+        # element (m, n) needs R + t_n - t_m; elements with equal shifts share one selection
+        shift = np.round(-t_red[:, None, :] + t_red[None, :, :], digits).reshape(-1, 3)
+        shifts, shift_of = np.unique(shift, axis=0, return_inverse=True)
+        shift_of = shift_of.reshape(nw, nw)
+        cell = np.stack(np.meshgrid(*[np.arange(n) for n in mp], indexing="ij"), axis=-1).reshape(-1, 3)
+        T = np.stack(np.meshgrid(*[np.arange(-3, 4)] * 3, indexing="ij"), axis=-1).reshape(-1, 3)
+        cand = cell[:, None, :] + (T * mp)[None, :, :]            # [ncell, 343, 3]
+        cand_cart = cand.dot(L)
+        sel = []                                                    # per shift: (R [n, 3], cell index, 1/ndeg)
+        for s in shifts:
+            dist = np.linalg.norm(cand_cart + s.dot(L), axis=2)
+            near = np.abs(dist - dist.min(axis=1, keepdims=True)) < tol
+            ic, it = np.nonzero(near)
+            nd = near.sum(axis=1)
+            sel.append((cand[ic, it], ic, 1.0 / nd[ic]))
+        allR = np.unique(np.concatenate([r for r, _, _ in sel]), axis=0)
+        index = {tuple(r): i for i, r in enumerate(allR)}
+        sel = [(np.array([index[tuple(r)] for r in R]), ic, w) for R, ic, w in sel]
+        ncell = cell.shape[0]
+        cell_of_R = np.ravel_multi_index(tuple((self.rvec.iRvec % mp).T), tuple(mp))
+        new = {}
+        for key, X in self._XX_R.items():
+            grid = np.zeros((ncell,) + X.shape[1:], dtype=X.dtype)
+            np.add.at(grid, cell_of_R, X)
+            Y = np.zeros((allR.shape[0],) + X.shape[1:], dtype=X.dtype)
+            for isft, (iR, ic, w) in enumerate(sel):
+                m_idx, n_idx = np.nonzero(shift_of == isft)
+                wx = w.reshape((-1,) + (1,) * (X.ndim - 3))
+                for m, n in zip(m_idx, n_idx):
+                    Y[iR, m, n] += grid[ic, m, n] * wx
+            new[key] = Y
+        keep = np.zeros(allR.shape[0], dtype=bool)
+        for Y in new.values():
+            keep |= np.abs(Y.reshape(Y.shape[0], -1)).max(axis=1) > 1e-8
+        self.rvec = _Rvec(allR[keep])
+        self._XX_R = {k: Y[keep] for k, Y in new.items()}
+        self._device_cache.clear()
+        self._herm_err = {}
+        self._NKFFT_recommended = mp
+
     @property
     def NKFFT_recommended(self):
         """Smallest FFT grid on which the R-vectors do not overlap (rvectors.py:280-292)."""
