@@ -68,3 +68,48 @@ __device__ __attribute__((always_inline)) void mfma_block(MfmaBlock& C, int nr, 
             }
     }
 }
+
+// One 16 x 32 half of such a block: rows r0 .. r0 + 15 (a single output row
+// block), columns c0 .. c0 + 31; lane l owns row l / 16 + 4 v, column 16 bj + l % 16.
+struct MfmaHalf {
+    wbmi_v4d re[2], im[2];
+    __device__ inline void zero() {
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            re[bj] = wbmi_v4d{0.0, 0.0, 0.0, 0.0};
+            im[bj] = wbmi_v4d{0.0, 0.0, 0.0, 0.0};
+        }
+    }
+    __device__ inline cplx<double> get(int bj, int v) const { return mk(re[bj][v], im[bj][v]); }
+};
+
+__device__ inline int mfma_hrow(int v) { return (int)(threadIdx.x & 63) / 16 + 4 * v; }
+
+template <class FX, class FY>
+__device__ __attribute__((always_inline)) void mfma_half(MfmaHalf& C, int nr, int nc, int nk, int r0, int c0, FX X, FY Y) {
+    const int li = threadIdx.x & 15, lk = (threadIdx.x & 63) >> 4;
+    const bool rok = r0 + li < nr;
+    bool cok[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) cok[b] = c0 + 16 * b + li < nc;
+#pragma unroll 2
+    for (int k0 = 0; k0 < nk; k0 += 4) {
+        const int k = k0 + lk;
+        const bool kok = k < nk;
+        const cplx<double> a = (kok && rok) ? X(r0 + li, k) : mk(0.0, 0.0);
+        cplx<double> b[2];
+#pragma unroll
+        for (int q = 0; q < 2; ++q) b[q] = (kok && cok[q]) ? Y(k, c0 + 16 * q + li) : mk(0.0, 0.0);
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            C.re[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b[bj].re, C.re[bj], 0, 0, 0);
+            C.im[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.re, b[bj].im, C.im[bj], 0, 0, 0);
+        }
+#pragma unroll
+        for (int bj = 0; bj < 2; ++bj) {
+            C.re[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.im, b[bj].im, C.re[bj], 0, 0, 0);
+            C.im[bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.im, b[bj].re, C.im[bj], 0, 0, 0);
+        }
+    }
+}
+

@@ -1,8 +1,9 @@
 // Covariant formula traces + Fermi-level scan of libwbmi.
 //
 // Replaces the per-(k, band group) loop of StaticCalculator.__call__
-// (calculators/static.py:60-136).  One 512-thread workgroup per k-point; the
-// formula is a template parameter, so every calculator gets its own kernel.
+// (calculators/static.py:60-136).  One workgroup per k-point (512 threads;
+// DerOmega / DerMorb: scan_threads); the formula is a template parameter, so
+// every calculator gets its own kernel.
 //
 // Band partition.  The reference evaluates, for every band group G, the
 // formula with inn = G and out = all other bands.  Every sum in the formulas
@@ -25,11 +26,12 @@
 // symmetric in (b, d), so the elementwise terms are folded half-and-half into
 // each product: Q1 = G + (W - S)/2, Q2 = -H + (W + R)/2, Y1 = K + A' - D'A''.
 // The traces use only diagonal elements of D'^a X1, X2 A^b and D'^a Y1, so
-// every product element is consumed where it is computed: each wave forms
-// one 32 x 32 block of one of the 27 products with a 4 x 4 register tile per
-// lane, adds the elementwise terms, multiplies by the consumer weights and
-// reduces over the summed index with cross-lane shuffles into its own
-// accumulator rows.  No product is stored and no barrier separates them.
+// every product element is consumed where it is computed: a wave task forms
+// one 16 x 32 block of one of the products on the FP64 matrix cores, adds the
+// elementwise terms, multiplies by the consumer weights, reduces over the
+// summed index with cross-lane shuffles and adds the result to the per-band
+// rows with LDS atomics.  The waves take tasks from an LDS counter.  No
+// product is stored and no barrier separates them.
 //
 // For DerOmega the operands V and A stay in LDS (rows padded by one element)
 // when nw <= 32.
@@ -42,9 +44,10 @@
 #include "mfma_c64.h"
 #include "wbmi_common.h"
 
-#define SC_THREADS 512       // workgroup size (and launch bound) of the scan kernels
-#define SC_TASK_THREADS 256  // DerOmega / DerMorb on complex64 channels: 4-wave workgroups, two per CU
-#define SC_WAVES_MAX (SC_THREADS / 64)
+#define SC_THREADS 512            // workgroup size (and launch bound) of the scan kernels
+#ifndef SC_TASK_THREADS
+#define SC_TASK_THREADS 768       // DerOmega / DerMorb: 12 waves (3 per SIMD), one workgroup per CU
+#endif
 #define SC_NWMAX 128
 #define SC_M_NW 64    // dEinv o Off lives in LDS up to this nw, else it is formed on the fly
 #define SC_LDS_NW 32  // V and A live in LDS up to this nw
@@ -72,7 +75,6 @@ struct ScanArgs {
 struct Smem {
     char* V;            // [3][nw][ld] staged V, in the storage type of the channels
     char* A;            // [3][nw][ld] staged A (external terms)
-    double* accw;       // [waves][nw][9] per-wave DerOmega accumulators
     cplx<double>* acc;  // [nw][9]
     double* M;          // [nw][ld] dEinv o Off
     double* E;
@@ -101,12 +103,11 @@ __host__ __device__ inline size_t al16(size_t x) { return (x + 15) & ~(size_t)15
 
 // byte offsets of the LDS regions and their total size
 struct SmemOffsets {
-    size_t V, A, accw, acc, M, E, cut, gw, ints, tet, tints, total;
+    size_t V, A, acc, M, E, cut, gw, ints, tet, tints, total;
 };
 
 // vsz: bytes of one staged V / A element (the channels' storage type)
-__host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool stage_a, bool stage_w,
-                                                    bool tetra, size_t vsz, int nwaves) {
+__host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool stage_a, bool tetra, size_t vsz) {
     const size_t ld = nw + 1;
     SmemOffsets o;
     size_t off = 0;
@@ -114,8 +115,6 @@ __host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool s
     off = al16(off + (stage_v ? vsz * 3 * nw * ld : 0));
     o.A = off;
     off = al16(off + (stage_a ? vsz * 3 * nw * ld : 0));
-    o.accw = off;
-    off = al16(off + (stage_w ? sizeof(double) * nwaves * nw * 9 : 0));
     o.acc = off;
     off = al16(off + sizeof(cplx<double>) * nw * 9);
     o.M = off;
@@ -137,13 +136,12 @@ __host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool s
 }
 
 __device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int nw, bool stage_v, bool stage_a,
-                                                                bool stage_w, bool tetra, size_t vsz) {
-    const SmemOffsets o = smem_offsets(nw, stage_v, stage_a, stage_w, tetra, vsz, (int)(blockDim.x >> 6));
+                                                                bool tetra, size_t vsz) {
+    const SmemOffsets o = smem_offsets(nw, stage_v, stage_a, tetra, vsz);
     Smem s;
     // LDS addresses even where a region is empty (never read then): no null tests on LDS pointers
     s.V = base + o.V;
     s.A = base + o.A;
-    s.accw = (double*)(base + o.accw);
     s.acc = (cplx<double>*)(base + o.acc);
     s.M = (double*)(base + o.M);  // empty region (never read) when nw > SC_M_NW
     s.E = (double*)(base + o.E);
@@ -334,11 +332,11 @@ __device__ inline double wave_sum(double v, int mask) {
     return v;
 }
 
-// Wave tasks.  One wave forms block (rb, cb) (32 x 32) of a dense product
+// Wave tasks.  One wave forms the 16-row x 32-column block (rh, cb) of a dense product
 // P = X Y on the matrix cores (FP64 MFMA, mfma_c64.h), adds the elementwise terms of
 // each element (x, y) and accumulates the consumers' weighted elements into
-// its accumulator rows accw[band][comp] after a shuffle reduction over the
-// summed index.  The band of an element is its column y (kind 1 forms its
+// the workgroup's per-band rows acc[band][comp] (LDS FP64 atomic adds, .re) after a
+// shuffle reduction over the summed index.  The band of an element is its column y (kind 1 forms its
 // product transposed for this).  Kinds (task indices b, d; for kinds 5, 6 b = c):
 //   0  Q1^{bd} = V^b D'^d + (W - D'^b V''^d - D'^d V''^b)/2, y in the partition, x off-side:
 //      DerOmega T1; DerMorb s2 (x 2 E_x), s7 (conj B^a_xy), s1 (raw V^b D'^d)
@@ -370,30 +368,30 @@ __host__ __device__ constexpr int task_nt() {
 // load serve both consumers.
 template <int KIND, bool M1, typename T, bool LM, bool ST, bool SA>
 __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST, SA>& K, const TaskCtx& tc_,
-                                                           double* accw, int b, int d, int rb, int cb) {
+                                                           cplx<double>* acc, int b, int d, int rh, int cb) {
     const int nw = K.nw, lane = threadIdx.x & 63;
-    const int r0 = rb * 32, c0 = cb * 32;
+    const int r0 = rh * 16, c0 = cb * 32;  // a 16 x 32 half block
     const bool dm = tc_.dm, in = tc_.in, ex = tc_.ex;
     const double sign = tc_.sign;
     const int al = c_alpha[b], be = c_beta[b];  // kinds 5, 6: b = c
-    MfmaBlock P;
+    MfmaHalf P;
     P.zero();
     if constexpr (KIND == 0)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.V(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
+        mfma_half(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.V(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
     else if constexpr (KIND == 1)  // transposed: P_xy = (D'^b V^d)_yx, so the band n = y is a column as for the others
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.V(d, m, r); }, [&](int m, int c) { return K.Dp(b, c, m); });
+        mfma_half(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.V(d, m, r); }, [&](int m, int c) { return K.Dp(b, c, m); });
     else if constexpr (KIND == 2)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.A(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
+        mfma_half(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.A(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
     else if constexpr (KIND == 3)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.B(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
+        mfma_half(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.B(b, r, m); }, [&](int m, int c) { return K.Dp(d, m, c); });
     else if constexpr (KIND == 4)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(d, r, m); }, [&](int m, int c) { return K.A(b, m, c); });
+        mfma_half(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(d, r, m); }, [&](int m, int c) { return K.A(b, m, c); });
     else if constexpr (KIND == 5)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(al, r, m); }, [&](int m, int c) { return K.Dp(be, m, c); });
+        mfma_half(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Dp(al, r, m); }, [&](int m, int c) { return K.Dp(be, m, c); });
     else if constexpr (KIND == 6)
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Ain(al, r, m); }, [&](int m, int c) { return K.Ain(be, m, c); });
+        mfma_half(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Ain(al, r, m); }, [&](int m, int c) { return K.Ain(be, m, c); });
     else
-        mfma_block(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Vin(b, r, m); }, [&](int m, int c) { return K.Ain(d, m, c); });
+        mfma_half(P, nw, nw, nw, r0, c0, [&](int r, int m) { return K.Vin(b, r, m); }, [&](int m, int c) { return K.Ain(d, m, c); });
 
     // targets: partner component tp, accumulator column tcol, factor tw
     constexpr int NT = task_nt<KIND>();
@@ -499,8 +497,7 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
 #pragma unroll
         for (int w = 0; w < NT1; ++w) part1[u][w] = 0.0;
     }
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi) {
+    {
         // the elementwise channel (W, A', B') of this row half: all loads issued before use
         cplx<double> gz[2][4];
         if constexpr (KIND <= 3) {
@@ -508,7 +505,7 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
             for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int x = r0 + mfma_row(bi, v), y = c0 + mfma_col(bj);
+                    const int x = r0 + mfma_hrow(v), y = c0 + mfma_col(bj);
                     const bool ok = x < nw && y < nw;
                     const int xx = ok ? x : 0, yy = ok ? y : 0;
                     if constexpr (KIND == 0)
@@ -525,10 +522,10 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
         for (int bj = 0; bj < 2; ++bj)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int x = r0 + mfma_row(bi, v), y = c0 + mfma_col(bj);
+            const int x = r0 + mfma_hrow(v), y = c0 + mfma_col(bj);
             if (x >= nw || y >= nw) continue;
             const int i = bj, j = bj;  // part slot
-            const cplx<double> pv = P.get(bi, bj, v);
+            const cplx<double> pv = P.get(bj, v);
             const int n = y, l = x;
             if (!S.act[n]) continue;
             const bool same = K.lab[l] == K.lab[n];
@@ -634,13 +631,13 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
 #pragma unroll
         for (int w = 0; w < NT; ++w) {
             const double sum = wave_sum(part[u][w], mask);
-            if (writer && tw[w] != 0.0 && n < nw) accw[n * 9 + tcol[w]] += tw[w] * sum;
+            if (writer && tw[w] != 0.0 && n < nw) atomicAdd(&acc[n * 9 + tcol[w]].re, tw[w] * sum);
         }
         if constexpr (M1) {
 #pragma unroll
             for (int w = 0; w < NT1; ++w) {
                 const double sum = wave_sum(part1[u][w], mask);
-                if (writer && tw1[w] != 0.0 && n < nw) accw[n * 9 + tcol1[w]] += tw1[w] * sum;
+                if (writer && tw1[w] != 0.0 && n < nw) atomicAdd(&acc[n * 9 + tcol1[w]].re, tw1[w] * sum);
             }
         }
     }
@@ -654,7 +651,7 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
 template <typename T, bool LM, bool ST, bool SA>
 __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST, SA>& K, bool in, bool ex,
                                                             bool dm, int sign) {
-    const int nw = K.nw, tid = threadIdx.x, wave = tid >> 6;
+    const int nw = K.nw, tid = threadIdx.x;
     // per-band terms, lanes running over (n, c, d)
     for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
         const int n = t % nw, c = (t / nw) / 3, d = (t / nw) % 3;
@@ -697,8 +694,7 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
         }
         S.acc[n * 9 + c * 3 + d] = mk(v, 0.0);
     }
-    const int nwaves = (int)(blockDim.x >> 6);
-    for (int t = tid; t < nwaves * nw * 9; t += (int)blockDim.x) S.accw[t] = 0.0;
+    if (tid == 0) S.misc[3] = 0;  // task counter
     __syncthreads();
     // wave tasks (kind, b, d, row block, column block), round-robin over the waves
     const TaskCtx tcx{dm, in, ex, (double)sign};
@@ -715,12 +711,19 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
     // kind-1 tasks ride on the kind-0 tasks of the transposed index pair when both run
     const bool merge01 = cnt[0] > 0 && cnt[1] > 0;
     if (merge01) cnt[1] = 0;
-    const int nblk = (nw + 31) / 32, nb2 = nblk * nblk;
+    // a task is one 16-row x 32-column half block of one product; the waves take the next
+    // task from an LDS counter, kinds in the order above (the heaviest, kind 0 with its
+    // carried kind-1 task, first), so the waves finish together whatever the mix of kinds
+    const int nblk = (nw + 31) / 32, nrh = (nw + 15) / 16, nb2 = nrh * nblk;
     int ntask = 0;
 #pragma unroll
     for (int q = 0; q < 8; ++q) ntask += cnt[q] * nb2;
-    double* accw = S.accw + wave * nw * 9;
-    for (int t = wave; t < ntask; t += nwaves) {
+    const int lane = tid & 63;
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(&S.misc[3], 1);
+        t = __builtin_amdgcn_readfirstlane(t);
+        if (t >= ntask) break;
         int kind = -1, rem = 0, base = 0;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -731,29 +734,23 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
             }
             base += nq;
         }
-        const int idx = rem / nb2, rb = (rem % nb2) / nblk, cb = rem % nblk;
+        const int idx = rem / nb2, rh = (rem % nb2) / nblk, cb = rem % nblk;
         const int b = kind == 5 || kind == 6 ? idx : idx / 3, d = kind == 5 || kind == 6 ? 0 : idx % 3;
         switch (kind) {
             case 0:
                 if (merge01)
-                    formula_task<0, true>(S, K, tcx, accw, b, d, rb, cb);
+                    formula_task<0, true>(S, K, tcx, S.acc, b, d, rh, cb);
                 else
-                    formula_task<0, false>(S, K, tcx, accw, b, d, rb, cb);
+                    formula_task<0, false>(S, K, tcx, S.acc, b, d, rh, cb);
                 break;
-            case 1: formula_task<1, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 2: formula_task<2, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 3: formula_task<3, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 4: formula_task<4, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 5: formula_task<5, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            case 6: formula_task<6, false>(S, K, tcx, accw, b, d, rb, cb); break;
-            default: formula_task<7, false>(S, K, tcx, accw, b, d, rb, cb); break;
+            case 1: formula_task<1, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+            case 2: formula_task<2, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+            case 3: formula_task<3, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+            case 4: formula_task<4, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+            case 5: formula_task<5, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+            case 6: formula_task<6, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+            default: formula_task<7, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
         }
-    }
-    __syncthreads();
-    for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
-        double v = S.acc[t].re;
-        for (int w = 0; w < nwaves; ++w) v += S.accw[w * nw * 9 + t];
-        S.acc[t] = mk(v, 0.0);
     }
 }
 
@@ -1019,8 +1016,20 @@ __device__ inline int fermi_bin(const wbmi_scan_params& p, double E) {
 // MODE: 0 = Fermi-level bins, 1 = tetrahedron weights, 2 = per-band table (hist is then
 // out[nk][nw][ncomp]: each band gets its group's trace / group size, or, for the identity
 // formula, the group's mean energy)
+// workgroup size of k_static_scan<T, F, .., MODE> (its launch bound).  The tetrahedron
+// variants (MODE 1) of DerOmega / DerMorb keep 4-wave (complex64) and 8-wave (complex128)
+// workgroups: with their weight code inlined, the 168-register bound of 12-wave workgroups
+// spills several hundred registers
+template <typename T, int F, int MODE>
+__host__ __device__ constexpr int scan_threads() {
+    constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
+    if (!tasks) return SC_THREADS;
+    if (MODE == 1) return sizeof(T) == 4 ? 256 : SC_THREADS;
+    return SC_TASK_THREADS;
+}
+
 template <typename T, int F, bool LM, bool ST, int MODE>
-__global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
+__global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
                                                             const cplx<T>* __restrict__ Xall,
                                                             double* __restrict__ hist) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
@@ -1036,7 +1045,7 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
     const bool stage_a = SA && p.external_terms;
     constexpr bool tetra = MODE == 1;  // compile-time: the Fermi-bin kernels carry no tetrahedron code
     constexpr bool tab = MODE == 2;
-    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tasks, tetra, sizeof(cplx<T>));
+    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tetra, sizeof(cplx<T>));
     for (int n = tid; n < nw; n += (int)blockDim.x) S.E[n] = Eall[k * nw + n];
     for (int r = tid; r < WBMI_NROLES; r += (int)blockDim.x) S.ch[r] = p.ch[r];
     if (tetra)
@@ -1059,28 +1068,39 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
         __syncthreads();
     }
     const cplx<T>* Xk = Xall + k * (int64_t)args.nch * nw * nw;
-    if (stage_v || stage_a) {
-        // staged only for nw <= SC_LDS_NW: 3 nw^2 <= 3072 = 12 x SC_TASK_THREADS elements per array;
-        // all of a thread's loads are issued before its LDS stores
-        constexpr int NS = (3 * SC_LDS_NW * SC_LDS_NW + SC_TASK_THREADS - 1) / SC_TASK_THREADS;
-        const int nw2 = nw * nw;
-        cplx<T> bv[NS], ba[NS];
+    if (stage_v) {
+        // staged only for nw <= SC_LDS_NW: 3 nw^2 <= 3072 elements per array, NS per thread of
+        // the workgroup.  Each copy loop is branch-free (indices clamped into range; a thread past
+        // the end rewrites the last element with its own value), so it is one basic block and the
+        // scheduler issues all of a thread's loads before its first LDS store
+        constexpr int NTH = scan_threads<T, F, MODE>();
+        constexpr int NS = (3 * SC_LDS_NW * SC_LDS_NW + NTH - 1) / NTH;
+        const int nw2 = nw * nw, n3 = 3 * nw2;
+        int so[NS], lo[NS];  // source element (channel-relative) and LDS element of each slot
 #pragma unroll
         for (int u = 0; u < NS; ++u) {
-            const int t = tid + (int)blockDim.x * u;
-            if (t < 3 * nw2) {
-                const int a = t / nw2, xy = t % nw2;
-                if (stage_v) bv[u] = Xk[(int64_t)S.ch[WBMI_ROLE_V + a] * nw2 + xy];
-                if (stage_a) ba[u] = Xk[(int64_t)S.ch[WBMI_ROLE_A + a] * nw2 + xy];
-            }
+            const int t = min(tid + NTH * u, n3 - 1);
+            const int a = t >= 2 * nw2 ? 2 : t >= nw2 ? 1 : 0, xy = t - a * nw2;
+            const int x = xy / nw, y = xy - x * nw;
+            so[u] = a * 4096 + xy;  // a in the bits above xy (nw^2 <= 1024)
+            lo[u] = (a * nw + x) * ld + y;
         }
+        const cplx<T>* srcv[3] = {Xk + (int64_t)S.ch[WBMI_ROLE_V] * nw2, Xk + (int64_t)S.ch[WBMI_ROLE_V + 1] * nw2,
+                                  Xk + (int64_t)S.ch[WBMI_ROLE_V + 2] * nw2};
+        cplx<T>* dv = (cplx<T>*)S.V;
 #pragma unroll
         for (int u = 0; u < NS; ++u) {
-            const int t = tid + (int)blockDim.x * u;
-            if (t < 3 * nw2) {
-                const int a = t / nw2, x = (t % nw2) / nw, y = t % nw;
-                if (stage_v) ((cplx<T>*)S.V)[(a * nw + x) * ld + y] = bv[u];
-                if (stage_a) ((cplx<T>*)S.A)[(a * nw + x) * ld + y] = ba[u];
+            const int a = so[u] >> 12, xy = so[u] & 4095;
+            dv[lo[u]] = (a == 0 ? srcv[0] : a == 1 ? srcv[1] : srcv[2])[xy];
+        }
+        if (stage_a) {
+            const cplx<T>* srca[3] = {Xk + (int64_t)S.ch[WBMI_ROLE_A] * nw2, Xk + (int64_t)S.ch[WBMI_ROLE_A + 1] * nw2,
+                                      Xk + (int64_t)S.ch[WBMI_ROLE_A + 2] * nw2};
+            cplx<T>* da = (cplx<T>*)S.A;
+#pragma unroll
+            for (int u = 0; u < NS; ++u) {
+                const int a = so[u] >> 12, xy = so[u] & 4095;
+                da[lo[u]] = (a == 0 ? srca[0] : a == 1 ? srca[1] : srca[2])[xy];
             }
         }
     }
@@ -1309,12 +1329,12 @@ __global__ void __launch_bounds__(SC_THREADS) k_static_scan(ScanArgs args, const
 
 template <typename T, int F, bool LM, bool ST, int MODE>
 static int launch_scan_t(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
     const bool stage_a = ST && a.p.external_terms;
-    // complex64 DerOmega / DerMorb: 4-wave workgroups, so that two of them (LDS ~75 KB each with V
-    // and A staged as complex64) share a CU; complex128: 8 waves, one workgroup per CU
-    const int threads = tasks && sizeof(T) == 4 ? SC_TASK_THREADS : SC_THREADS;
-    size_t lds = smem_offsets(a.nw, ST, stage_a, tasks, MODE == 1, sizeof(cplx<T>), threads / 64).total;
+    // DerOmega / DerMorb: 12 waves (3 per SIMD at <= 168 VGPRs), one workgroup per CU (at nw = 32
+    // LDS holds V and A: 118 KB as complex128, 66 KB as complex64; measured faster than two
+    // 4-wave workgroups per CU for complex64)
+    const int threads = scan_threads<T, F, MODE>();
+    size_t lds = smem_offsets(a.nw, ST, stage_a, MODE == 1, sizeof(cplx<T>)).total;
     if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
     WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM, ST, MODE>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
