@@ -866,6 +866,39 @@ def test_bcc_nw32_workload(wb, bcc32, precision, tol):
 
 
 @pytest.fixture(scope="module")
+def gme32(wb):
+    """nw = 32 model with BB and CC (seed 12, |R_i| <= 1), 61 Fermi levels across the middle of
+    the band range, and the oracle's AHC, BCD sea, Morb and GME (sea, surface) with external
+    terms on one 2x2x2 K-block: nw = 32 runs the Fermi-bin kernels whose size is fixed at
+    compile time, the only path the throughput workload takes."""
+    from wannierberri_amd import models
+    s = models.synthetic(num_wann=32, Rmax=1, seed=12, with_BBCC=True)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA", "BB", "CC")})
+    E0 = np.linalg.eigvalsh(s.get_R_mat("Ham")[np.where((s.rvec.iRvec == 0).all(1))[0][0]])
+    Ef = np.linspace(np.percentile(E0, 30), np.percentile(E0, 70), 61)
+    NK = NKFFT = [2, 2, 2]
+    ro = O.run(o, {"ahc": O.AHC(Ef), "bcd": O.BerryDipole_FermiSea(Ef), "morb": O.Morb(Ef),
+                   "gme": O.GME_orb_FermiSea(Ef), "gmes": O.GME_orb_FermiSurf(Ef)}, NK=NK, NKFFT=NKFFT)
+    return s, Ef, NK, NKFFT, ro
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_nw32_all_formulas(wb, gme32, precision, tol):
+    """AHC, BCD sea, Morb and GME (sea, surface) at nw = 32 with external terms vs the oracle."""
+    say(f"test_nw32_all_formulas {precision}")
+    st = wb.calculators.static
+    s, Ef, NK, NKFFT, ro = gme32
+    calcs = {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef), "morb": st.Morb(Efermi=Ef),
+             "gme": st.GME_orb_FermiSea(Efermi=Ef), "gmes": st.GME_orb_FermiSurf(Efermi=Ef)}
+    r = run_gpu(wb, s, calcs, NK, NKFFT, precision)
+    for k in calcs:
+        e = rel_err(r.results[k].data, ro[k].data)
+        say(f"  {precision} {k}: rel err {e:.2e}")
+        assert e < tol, k
+
+
+@pytest.fixture(scope="module")
 def large128(wb):
     """nw = 128 model of the large workload's recipe (seed 5). The R cube is |R_i| <= 2
     (125 vectors) instead of 6 (2197): the oracle forms every derivative channel in

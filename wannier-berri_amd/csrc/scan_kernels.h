@@ -175,10 +175,18 @@ __device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int 
 // channels; SA: the same for A (staged together with V, and only with external terms, the only
 // case that reads it).
 // Both are compile-time so that every access is a plain LDS or global load.
-template <typename T, bool LM, bool ST, bool SA>
+template <typename T, bool LM, bool ST, bool SA, int NW = 0>
 struct KCtx {
     const cplx<T>* X;  // channels of this k: [nch][nw][nw]
-    int nw, ld;
+    int nw, ld;  // run-time size (NW = 0); NW > 0 fixes both at compile time
+    __device__ inline int n_() const {
+        if constexpr (NW > 0) return NW;
+        else return nw;
+    }
+    __device__ inline int ld_() const {
+        if constexpr (NW > 0) return NW + 1;
+        else return ld;
+    }
     const int* ch;
     const double* E;
     const double* M;  // dEinv o Off, row stride ld
@@ -188,12 +196,12 @@ struct KCtx {
     const cplx<T>* sV;  // LDS copies of V and A (ST), in the storage type
     const cplx<T>* sA;
     __device__ inline cplx<double> get(int role, int x, int y) const {
-        cplx<T> v = X[((int64_t)ch[role] * nw + x) * nw + y];
+        cplx<T> v = X[((int64_t)ch[role] * n_() + x) * n_() + y];
         return mk((double)v.re, (double)v.im);
     }
     __device__ inline cplx<double> V(int a, int x, int y) const {
         if constexpr (ST) {
-            const cplx<T> v = sV[(a * nw + x) * ld + y];
+            const cplx<T> v = sV[(a * n_() + x) * ld_() + y];
             return mk((double)v.re, (double)v.im);
         } else {
             return get(WBMI_ROLE_V + a, x, y);
@@ -201,7 +209,7 @@ struct KCtx {
     }
     __device__ inline cplx<double> A(int a, int x, int y) const {
         if constexpr (SA) {
-            const cplx<T> v = sA[(a * nw + x) * ld + y];
+            const cplx<T> v = sA[(a * n_() + x) * ld_() + y];
             return mk((double)v.re, (double)v.im);
         } else {
             return get(WBMI_ROLE_A + a, x, y);
@@ -223,7 +231,7 @@ struct KCtx {
     // dEinv_xy Off(x, y) (data_K.py:290-298): from LDS, or formed from E and the labels
     __device__ inline double m(int x, int y) const {
         if constexpr (LM) {
-            return M[x * ld + y];
+            return M[x * ld_() + y];
         } else {
             const double de = E[x] - E[y];
             return (lab[x] != lab[y] && fabs(de) >= 1e-7) ? 1.0 / de : 0.0;
@@ -236,8 +244,8 @@ struct KCtx {
 // pre-Hermitian Omega summand s^c_xy for x, y in the same group
 // (formula/covariant.py:175-200): Omega.nn = s + s^+.  The sum over the
 // off-group bands l runs over [l0, l1); `tail` adds the within-group and O terms.
-template <typename T, bool LM, bool ST, bool SA>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA>& K, int c, int x, int y, bool internal,
+template <typename T, bool LM, bool ST, bool SA, int NW>
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA, NW>& K, int c, int x, int y, bool internal,
                                                               bool external, int l0, int l1, bool tail) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
@@ -253,19 +261,19 @@ __device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM,
     }
     return s;
 }
-template <typename T, bool LM, bool ST, bool SA>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA>& K, int c, int x, int y, bool internal,
+template <typename T, bool LM, bool ST, bool SA, int NW>
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA, NW>& K, int c, int x, int y, bool internal,
                                                               bool external) {
-    return omega_s(K, c, x, y, internal, external, 0, K.nw, true);
+    return omega_s(K, c, x, y, internal, external, 0, K.n_(), true);
 }
 
 // pre-Hermitian Morb_H summand h^c_xy for x, y in the same group
 // (formula/covariant.py:390-414)
-template <typename T, bool LM, bool ST, bool SA>
-__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM, ST, SA>& K, int c, int x, int y, bool internal, bool external) {
+template <typename T, bool LM, bool ST, bool SA, int NW>
+__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM, ST, SA, NW>& K, int c, int x, int y, bool internal, bool external) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
-    for (int l = 0; l < K.nw; ++l) {
+    for (int l = 0; l < K.n_(); ++l) {
         if (K.lab[l] == K.lab[x]) continue;
         cplx<double> dxa = K.Dp(al, x, l);
         if (internal) s += times_i(-(K.E[l] * (dxa * K.Dp(be, l, y))));
@@ -366,10 +374,10 @@ __host__ __device__ constexpr int task_nt() {
 // kind 1's transposed product is -conj of kind 0's at the same element, and
 // its elementwise channel W^{db}_yx = conj(W^{bd}_xy): one product and one
 // load serve both consumers.
-template <int KIND, bool M1, typename T, bool LM, bool ST, bool SA>
-__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST, SA>& K, const TaskCtx& tc_,
+template <int KIND, bool M1, typename T, bool LM, bool ST, bool SA, int NW>
+__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST, SA, NW>& K, const TaskCtx& tc_,
                                                            cplx<double>* acc, int b, int d, int rh, int cb) {
-    const int nw = K.nw, lane = threadIdx.x & 63;
+    const int nw = K.n_(), lane = threadIdx.x & 63;
     const int r0 = rh * 16, c0 = cb * 32;  // a 16 x 32 half block
     const bool dm = tc_.dm, in = tc_.in, ex = tc_.ex;
     const double sign = tc_.sign;
@@ -648,10 +656,10 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
 // DerMorb = DerMorb_H + sign [Eav o dOmega + (Omega V + V Omega)/2] on cut
 // partitions; its trace terms (s1 ... s7, the Omega block) are listed at
 // formula_task.
-template <typename T, bool LM, bool ST, bool SA>
-__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST, SA>& K, bool in, bool ex,
+template <typename T, bool LM, bool ST, bool SA, int NW>
+__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST, SA, NW>& K, bool in, bool ex,
                                                             bool dm, int sign) {
-    const int nw = K.nw, tid = threadIdx.x;
+    const int nw = K.n_(), tid = threadIdx.x;
     // per-band terms, lanes running over (n, c, d)
     for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
         const int n = t % nw, c = (t / nw) / 3, d = (t / nw) % 3;
@@ -755,9 +763,9 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
 }
 
 // Per-band values for the current partition into S.acc[n][comp] (.re).
-template <int F, typename T, bool LM, bool ST, bool SA>
-__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM, ST, SA>& K, const wbmi_scan_params& p) {
-    const int nw = K.nw;
+template <int F, typename T, bool LM, bool ST, bool SA, int NW>
+__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM, ST, SA, NW>& K, const wbmi_scan_params& p) {
+    const int nw = K.n_();
     const int tid = threadIdx.x;
     const bool in = p.internal_terms, ex = p.external_terms;
     if constexpr (LM) build_M(S, nw);
@@ -1028,13 +1036,14 @@ __host__ __device__ constexpr int scan_threads() {
     return SC_TASK_THREADS;
 }
 
-template <typename T, int F, bool LM, bool ST, int MODE>
+// NW > 0: the kernel for nw = NW only, every size and row stride a compile-time constant
+template <typename T, int F, bool LM, bool ST, int MODE, int NW>
 __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
                                                             const cplx<T>* __restrict__ Xall,
                                                             double* __restrict__ hist) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
     const wbmi_scan_params& p = args.p;
-    const int nw = args.nw, ld = nw + 1;
+    const int nw = NW > 0 ? NW : args.nw, ld = nw + 1;
     const int64_t k = blockIdx.x;
     const int tid = threadIdx.x;
     if (MODE != 2) hist += k * args.hist_stride;
@@ -1199,7 +1208,7 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
     const int ng = S.misc[0];
     const int sea_lo = S.misc[1], sea_hi = S.misc[2];
 
-    KCtx<T, LM, ST, SA> K;
+    KCtx<T, LM, ST, SA, NW> K;
     K.X = Xk;
     K.nw = nw;
     K.ld = ld;
@@ -1327,7 +1336,7 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
     }
 }
 
-template <typename T, int F, bool LM, bool ST, int MODE>
+template <typename T, int F, bool LM, bool ST, int MODE, int NW = 0>
 static int launch_scan_t(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
     const bool stage_a = ST && a.p.external_terms;
     // DerOmega / DerMorb: 12 waves (3 per SIMD at <= 168 VGPRs), one workgroup per CU (at nw = 32
@@ -1336,9 +1345,9 @@ static int launch_scan_t(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
     const int threads = scan_threads<T, F, MODE>();
     size_t lds = smem_offsets(a.nw, ST, stage_a, MODE == 1, sizeof(cplx<T>)).total;
     if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM, ST, MODE>,
+    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM, ST, MODE, NW>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_static_scan<T, F, LM, ST, MODE>), dim3(nk), dim3(threads), lds,
+    hipLaunchKernelGGL((k_static_scan<T, F, LM, ST, MODE, NW>), dim3(nk), dim3(threads), lds,
                        ctx->stream, a, E,
                        (const cplx<T>*)X, hist);
     WBMI_LAUNCH_CHECK(ctx);
@@ -1350,8 +1359,13 @@ static int launch_scan_t(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
 template <typename T, int F, int MODE>
 static int launch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
     constexpr bool staged = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB || F == WBMI_FORMULA_OMEGA;
-    if constexpr (staged)
+    if constexpr (staged) {
+        // nw = 32 (a single 32-column block, the spin blocks of the spinor models) has its own
+        // Fermi-bin kernels with the size fixed at compile time
+        if constexpr (MODE == 0)
+            if (a.stage_va && a.nw == 32) return launch_scan_t<T, F, true, true, MODE, 32>(ctx, nk, a, E, X, hist);
         if (a.stage_va) return launch_scan_t<T, F, true, true, MODE>(ctx, nk, a, E, X, hist);
+    }
     if (a.nw <= SC_M_NW) return launch_scan_t<T, F, true, false, MODE>(ctx, nk, a, E, X, hist);
     return launch_scan_t<T, F, false, false, MODE>(ctx, nk, a, E, X, hist);
 }
