@@ -63,11 +63,13 @@ def run_gpu(wb, system, calcs, NK, NKFFT, precision="fp64"):
 # -- stage parity --------------------------------------------------------------
 
 @pytest.mark.parametrize("NKFFT,prec", [((3, 3, 3), 0), ((9, 9, 9), 0), ((2, 4, 3), 0), ((20, 17, 3), 0),
-                                        ((9, 9, 9), 1), ((20, 17, 3), 1), ((18, 3, 2), 1)])
+                                        ((40, 3, 2), 0), ((70, 2, 3), 0), ((2, 70, 3), 0),
+                                        ((9, 9, 9), 1), ((20, 17, 3), 1), ((18, 3, 2), 1), ((70, 2, 3), 1)])
 def test_r2k_channels(wb, ctx, NKFFT, prec):
-    """R->k of Ham, Ham', AA, rotAA, AA' (phase, derivatives, curl) vs the oracle's FFT; the y and
-    x passes fused on the matrix cores, with several 16-row blocks of j0 / j1 (N > 16) and a
-    ragged last column tile; complex64 output to its rounding (2e-7 of the largest value)."""
+    """R->k of Ham, Ham', AA, rotAA, AA' (phase, derivatives, curl) vs the oracle's FFT.  The
+    passes run on the matrix cores with 1, 2 or 4 accumulator blocks of 16 rows (N <= 16, 32, 64)
+    and chunks of 64 rows beyond (N = 70), ragged last column tiles, aliased axes (N < the R
+    extent); complex64 output to its rounding (2e-7 of the largest value)."""
     import torch
     say(f"test_r2k_channels NKFFT={NKFFT}")
     from wannierberri_amd.data_k import device_system, CHANNEL_NCOMP
@@ -93,6 +95,31 @@ def test_r2k_channels(wb, ctx, NKFFT, prec):
         got = np.moveaxis(X[:, sl:sl + n], 1, -1).reshape(ref.shape)
         assert rel_err(got, ref) < (1e-12 if prec == 0 else 2e-7), (name, der)
         sl += n
+
+
+def test_r2k_odd_columns(wb, ctx):
+    """complex64 passes with an odd column count (nw = 5, one channel, odd NKFFT): the one-column
+    form of the complex64 pass (pairs of columns need an even count) against the complex128
+    transform of the same plan, to complex64 rounding."""
+    import torch
+    say("test_r2k_odd_columns")
+    from wannierberri_amd import models
+    from wannierberri_amd.data_k import device_system
+    s = models.synthetic(num_wann=5, Rmax=2, seed=21)
+    keys = [("Ham", 0)]
+    dev = device_system(s, ctx)
+    stack = dev.stack(keys)
+    NKFFT = (3, 5, 7)
+    plan, box, aliased = dev.plan(keys, NKFFT)
+    dK = np.array([0.11, 0.02, -0.3])
+    nk = int(np.prod(NKFFT))
+    outs = []
+    for prec, cdt in ((0, torch.complex128), (1, torch.complex64)):
+        X = torch.empty((nk, stack.shape[1], 5, 5), dtype=cdt, device="cuda")
+        work = torch.empty(plan.work_bytes, dtype=torch.uint8, device="cuda")
+        plan.execute(stack if aliased else None, box, dK, work, X, prec)
+        outs.append(X.cpu().numpy().astype(np.complex128))
+    assert rel_err(outs[1], outs[0]) < 2e-7
 
 
 def test_r2k_klist(wb, ctx):

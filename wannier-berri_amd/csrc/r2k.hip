@@ -2,9 +2,10 @@
 //
 //  * wbmi_build_channels: R-space channel stack (derivatives, curl) once per system.
 //  * wbmi_r2k_*: the K-block transform on an NKFFT grid as three separable DFT
-//    passes over the R bounding box.  Every pass is HBM-bound: it streams the
-//    [A][L][B] input once and writes [A][N][B]; threads run along the
-//    contiguous B index so loads and stores are fully coalesced.
+//    passes over the R bounding box.  Each pass is a small complex GEMM on the
+//    FP64 matrix cores that streams the [A][L][B] input once and writes
+//    [A][N][B]; lanes run along the contiguous B index so loads and stores
+//    are coalesced.
 //  * wbmi_r2k_klist: phase-matrix transform for arbitrary k-points (FP64 MFMA GEMM).
 #include <algorithm>
 #include <cmath>
@@ -136,42 +137,101 @@ __global__ void k_pass_phases(int N0, int N1, int N2, int L0, int L1, int L2, in
     }
 }
 
-// out[a][j][b] = sum_l P[j][l] in[a][l][b]; one thread per (a, b), inputs held in registers
-template <typename Tin, typename Tout>
-__global__ void __launch_bounds__(256) k_dft_pass(const cplx<Tin>* __restrict__ in, cplx<Tout>* __restrict__ out,
-                                                  const cplx<double>* __restrict__ P, int64_t A, int L, int N,
-                                                  int64_t B) {
-    extern __shared__ cplx<double> sP[];
+// out[a][j][b] = sum_l P[j][l] in[a][l][b] on the FP64 matrix cores (v_mfma_f64_16x16x4f64:
+// A[l%16][l/16], B[l/16][l%16], D[l/16 + 4v][l%16]).  One wave per 16 consecutive b of one a:
+// the B operand is in[a][l][b] (16 b per row, 4 rows l per step), every 16-row block of j is an
+// accumulator pair (re, im) fed from the phases in LDS (the A operand), so each input element is
+// read once and each output element written once.  Rows j come in chunks of 16 NB (NB = 1, 2, 4
+// accumulator pairs, chosen from N at launch); K = l in chunks of DM_LCHUNK, whose loads are
+// issued before their products.
+#define DM_WAVES 4
+#define DM_LCHUNK 16
+
+template <typename Tin, typename Tout, int NB, bool PAIR>
+__global__ void __launch_bounds__(64 * DM_WAVES) k_dft_mfma(const cplx<Tin>* __restrict__ in, cplx<Tout>* __restrict__ out,
+                                                          const cplx<double>* __restrict__ P, int64_t A, int L, int N,
+                                                          int64_t B) {
+    // PAIR (complex64 in and out, B even): a lane owns the adjacent columns b, b + 1 (16-byte
+    // loads and stores) as two MFMA column tiles, so a wave covers 32 columns
+    static_assert(!PAIR || (sizeof(Tin) == 4 && sizeof(Tout) == 4), "column pairs are for complex64");
+    constexpr int CW = PAIR ? 2 : 1;
+    extern __shared__ cplx<double> sP[];  // [N][L]
     for (int t = threadIdx.x; t < N * L; t += blockDim.x) sP[t] = P[t];
     __syncthreads();
-    int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t a = blockIdx.y + (int64_t)blockIdx.z * gridDim.y;
-    if (b >= B || a >= A) return;
-    const cplx<Tin>* src = in + (a * L) * B + b;
-    cplx<Tout>* dst = out + (a * N) * B + b;
-    constexpr int LC = 16;
-    for (int l0 = 0; l0 < L; l0 += LC) {
-        cplx<double> x[LC];
+    const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
+    const int64_t tiles_b = (B + 16 * CW - 1) / (16 * CW);
+    const int64_t tile = (int64_t)blockIdx.x * DM_WAVES + (threadIdx.x >> 6);
+    if (tile >= A * tiles_b) return;  // whole wave; no barrier follows
+    const int64_t a = tile / tiles_b, b = (tile % tiles_b) * 16 * CW + CW * li;
+    const bool bok = b < B;  // PAIR: B is even, so b + 1 < B as well
+    const cplx<Tin>* src = in + a * L * B + b;
+    cplx<Tout>* dst = out + a * N * B + b;
+    for (int j0 = 0; j0 < N; j0 += 16 * NB) {
+        wbmi_v4d cr[CW][NB], ci[CW][NB];
 #pragma unroll
-        for (int l = 0; l < LC; ++l) {
-            if (l0 + l < L) {
-                cplx<Tin> v = src[(int64_t)(l0 + l) * B];
-                x[l] = mk((double)v.re, (double)v.im);
-            } else {
-                x[l] = mk(0.0, 0.0);
+        for (int c = 0; c < CW; ++c)
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                cr[c][q] = wbmi_v4d{0.0, 0.0, 0.0, 0.0};
+                ci[c][q] = wbmi_v4d{0.0, 0.0, 0.0, 0.0};
+            }
+        const int nb = min(16 * NB, N - j0);
+        for (int l0 = 0; l0 < L; l0 += DM_LCHUNK) {
+            cplx<double> x[DM_LCHUNK / 4][CW];
+#pragma unroll
+            for (int ks = 0; ks < DM_LCHUNK / 4; ++ks) {
+                const int l = l0 + 4 * ks + g;
+#pragma unroll
+                for (int c = 0; c < CW; ++c) x[ks][c] = mk(0.0, 0.0);
+                if (l < L && bok) {
+                    if constexpr (PAIR) {
+                        const float4 v = *(const float4*)(src + (int64_t)l * B);
+                        x[ks][0] = mk((double)v.x, (double)v.y);
+                        x[ks][CW - 1] = mk((double)v.z, (double)v.w);
+                    } else {
+                        const cplx<Tin> v = src[(int64_t)l * B];
+                        x[ks][0] = mk((double)v.re, (double)v.im);
+                    }
+                }
+            }
+#pragma unroll
+            for (int ks = 0; ks < DM_LCHUNK / 4; ++ks) {
+                const int l = l0 + 4 * ks + g;
+                if (l0 + 4 * ks >= L) break;
+#pragma unroll
+                for (int q = 0; q < NB; ++q) {
+                    if (16 * q >= nb) break;
+                    const int j = j0 + 16 * q + li;
+                    const cplx<double> ph = (j < N && l < L) ? sP[j * L + l] : mk(0.0, 0.0);
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) {
+                        cr[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.re, x[ks][c].re, cr[c][q], 0, 0, 0);
+                        ci[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.re, x[ks][c].im, ci[c][q], 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int c = 0; c < CW; ++c) {
+                        cr[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ph.im, x[ks][c].im, cr[c][q], 0, 0, 0);
+                        ci[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.im, x[ks][c].re, ci[c][q], 0, 0, 0);
+                    }
+                }
             }
         }
-        for (int j = 0; j < N; ++j) {
-            cplx<double> acc = mk(0.0, 0.0);
-            if (l0 > 0) {
-                cplx<Tout> p = dst[(int64_t)j * B];
-                acc = mk((double)p.re, (double)p.im);
-            }
-            const cplx<double>* pj = sP + j * L + l0;
+        if (bok) {
 #pragma unroll
-            for (int l = 0; l < LC; ++l)
-                if (l0 + l < L) acc += pj[l] * x[l];
-            dst[(int64_t)j * B] = mk((Tout)acc.re, (Tout)acc.im);
+            for (int q = 0; q < NB; ++q) {
+                if (16 * q >= nb) break;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int j = j0 + 16 * q + g + 4 * v;
+                    if (j >= N) continue;
+                    if constexpr (PAIR) {
+                        *(float4*)(dst + (int64_t)j * B) = make_float4((float)cr[0][q][v], (float)ci[0][q][v],
+                                                                      (float)cr[CW - 1][q][v], (float)ci[CW - 1][q][v]);
+                    } else {
+                        dst[(int64_t)j * B] = mk((Tout)cr[0][q][v], (Tout)ci[0][q][v]);
+                    }
+                }
+            }
         }
     }
 }
@@ -184,16 +244,40 @@ static void grid_2d(int64_t A, dim3& g) {
     g.z = (unsigned)gz;
 }
 
+template <typename Tin, typename Tout, int NB, bool PAIR>
+static int launch_pass_nb(wbmi_ctx* ctx, const void* in, void* out, const cplx<double>* P, int64_t A, int L, int N,
+                          int64_t B) {
+    constexpr int CW = PAIR ? 2 : 1;  // columns per lane
+    const int64_t tiles = A * ceil_div(B, 16 * CW);
+    const size_t lds = sizeof(cplx<double>) * (size_t)N * L;
+    if (lds > 160 * 1024 || ceil_div(tiles, DM_WAVES) > 0x7fffffff)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_exec: pass of %lld x %d x %d x %lld too large", (long long)A, L, N,
+                         (long long)B);
+    if (lds > 64 * 1024) {
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_dft_mfma<Tin, Tout, NB, PAIR>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    hipLaunchKernelGGL((k_dft_mfma<Tin, Tout, NB, PAIR>), dim3((unsigned)ceil_div(tiles, DM_WAVES)), dim3(64 * DM_WAVES), lds,
+                       ctx->stream, (const cplx<Tin>*)in, (cplx<Tout>*)out, P, A, L, N, B);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+template <typename Tin, typename Tout, bool PAIR>
+static int launch_pass_p(wbmi_ctx* ctx, const void* in, void* out, const cplx<double>* P, int64_t A, int L, int N,
+                         int64_t B) {
+    if (N <= 16) return launch_pass_nb<Tin, Tout, 1, PAIR>(ctx, in, out, P, A, L, N, B);
+    if (N <= 32) return launch_pass_nb<Tin, Tout, 2, PAIR>(ctx, in, out, P, A, L, N, B);
+    return launch_pass_nb<Tin, Tout, 4, PAIR>(ctx, in, out, P, A, L, N, B);
+}
+
 template <typename Tin, typename Tout>
 static int launch_pass(wbmi_ctx* ctx, const void* in, void* out, const cplx<double>* P, int64_t A, int L, int N,
                        int64_t B) {
-    dim3 g((unsigned)ceil_div(B, 256));
-    grid_2d(A, g);
-    size_t lds = sizeof(cplx<double>) * (size_t)N * L;
-    hipLaunchKernelGGL((k_dft_pass<Tin, Tout>), g, dim3(256), lds, ctx->stream, (const cplx<Tin>*)in,
-                       (cplx<Tout>*)out, P, A, L, N, B);
-    WBMI_LAUNCH_CHECK(ctx);
-    return WBMI_OK;
+    // complex64 -> complex64 with an even column count (every row 16-byte aligned): column pairs
+    if constexpr (sizeof(Tin) == 4 && sizeof(Tout) == 4)
+        if ((B & 1) == 0) return launch_pass_p<Tin, Tout, true>(ctx, in, out, P, A, L, N, B);
+    return launch_pass_p<Tin, Tout, false>(ctx, in, out, P, A, L, N, B);
 }
 
 extern "C" {
