@@ -4,7 +4,7 @@ The API surface of the path is the reference's: System_R -> Data_K ->
 calculators / run().  All numerics run in the HIP library libwbmi.so
 (include/wbmi.h); see `wannierberri_amd.build_lib` to build it.
 """
-from . import calculators, factors, smoother, symmetry
+from . import autotune, calculators, factors, smoother, symmetry
 from .calculators import static
 from .data_k import Data_K
 from .grid import Grid, KpointBZparallel
