@@ -145,7 +145,7 @@ __global__ void k_pass_phases(int N0, int N1, int N2, int L0, int L1, int L2, in
 // accumulator pairs, chosen from N at launch); K = l in chunks of DM_LCHUNK, whose loads are
 // issued before their products.
 #define DM_WAVES 4
-#define DM_LCHUNK 16
+#define DM_LCHUNK 12
 
 template <typename Tin, typename Tout, int NB, bool PAIR>
 __global__ void __launch_bounds__(64 * DM_WAVES) k_dft_mfma(const cplx<Tin>* __restrict__ in, cplx<Tout>* __restrict__ out,
@@ -177,20 +177,20 @@ __global__ void __launch_bounds__(64 * DM_WAVES) k_dft_mfma(const cplx<Tin>* __r
             }
         const int nb = min(16 * NB, N - j0);
         for (int l0 = 0; l0 < L; l0 += DM_LCHUNK) {
-            cplx<double> x[DM_LCHUNK / 4][CW];
+            // the chunk's inputs in their storage type (converted at their products)
+            cplx<Tin> x[DM_LCHUNK / 4][CW];
 #pragma unroll
             for (int ks = 0; ks < DM_LCHUNK / 4; ++ks) {
                 const int l = l0 + 4 * ks + g;
 #pragma unroll
-                for (int c = 0; c < CW; ++c) x[ks][c] = mk(0.0, 0.0);
+                for (int c = 0; c < CW; ++c) x[ks][c] = mk((Tin)0, (Tin)0);
                 if (l < L && bok) {
                     if constexpr (PAIR) {
                         const float4 v = *(const float4*)(src + (int64_t)l * B);
-                        x[ks][0] = mk((double)v.x, (double)v.y);
-                        x[ks][CW - 1] = mk((double)v.z, (double)v.w);
+                        x[ks][0] = mk((Tin)v.x, (Tin)v.y);
+                        x[ks][CW - 1] = mk((Tin)v.z, (Tin)v.w);
                     } else {
-                        const cplx<Tin> v = src[(int64_t)l * B];
-                        x[ks][0] = mk((double)v.re, (double)v.im);
+                        x[ks][0] = src[(int64_t)l * B];
                     }
                 }
             }
@@ -205,13 +205,13 @@ __global__ void __launch_bounds__(64 * DM_WAVES) k_dft_mfma(const cplx<Tin>* __r
                     const cplx<double> ph = (j < N && l < L) ? sP[j * L + l] : mk(0.0, 0.0);
 #pragma unroll
                     for (int c = 0; c < CW; ++c) {
-                        cr[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.re, x[ks][c].re, cr[c][q], 0, 0, 0);
-                        ci[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.re, x[ks][c].im, ci[c][q], 0, 0, 0);
+                        cr[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.re, (double)x[ks][c].re, cr[c][q], 0, 0, 0);
+                        ci[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.re, (double)x[ks][c].im, ci[c][q], 0, 0, 0);
                     }
 #pragma unroll
                     for (int c = 0; c < CW; ++c) {
-                        cr[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ph.im, x[ks][c].im, cr[c][q], 0, 0, 0);
-                        ci[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.im, x[ks][c].re, ci[c][q], 0, 0, 0);
+                        cr[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ph.im, (double)x[ks][c].im, cr[c][q], 0, 0, 0);
+                        ci[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.im, (double)x[ks][c].re, ci[c][q], 0, 0, 0);
                     }
                 }
             }
