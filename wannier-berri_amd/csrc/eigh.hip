@@ -214,6 +214,8 @@ __host__ __device__ constexpr size_t jacobi_c_lds() {
 // the rotations accumulated onto V = U0: a nearly diagonal start needs about
 // three active sweeps instead of six or seven.  Any unitary U0 gives the same
 // eigen-decomposition; a poor one only costs sweeps.
+// (no minimum waves per SIMD: with 4 the kernel keeps 116 VGPRs and no AGPRs but runs 15 %
+// slower than at 2 waves per SIMD with 64 AGPRs, its LDS traffic being the limit)
 template <int NT, typename Tin, typename TO>
 __global__ void __launch_bounds__(64) k_eigh_jacobi_c(int nw, const cplx<Tin>* __restrict__ H, int64_t Hstride,
                                                       double* __restrict__ Eout, cplx<TO>* Uout,
@@ -749,8 +751,9 @@ __device__ void cluster_jacobi(int n, FS S, FX X, char* scratch, double theta, d
 // re-orthogonalised there; the following steps separate it, converging
 // quadratically once the mixing angle is small (six to eight steps for a
 // relative gap of 4e-6 at nw = 18..40).
+// (at least 5 waves per SIMD: 72 VGPRs and no AGPRs, 7 waves per SIMD)
 template <typename TU>
-__global__ void __launch_bounds__(OA_THREADS) k_eigh_refine(int nw, const cplx<double>* __restrict__ H64,
+__global__ void __launch_bounds__(OA_THREADS, 5) k_eigh_refine(int nw, const cplx<double>* __restrict__ H64,
                                                             int64_t Hstride, double* __restrict__ Eout,
                                                             cplx<TU>* __restrict__ U, int iters, double tol) {
     extern __shared__ __attribute__((aligned(16))) char smem[];

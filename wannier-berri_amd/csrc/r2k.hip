@@ -147,8 +147,9 @@ __global__ void k_pass_phases(int N0, int N1, int N2, int L0, int L1, int L2, in
 #define DM_WAVES 4
 #define DM_LCHUNK 12
 
+// (minimum waves per SIMD per variant: the accumulators stay in VGPRs and nothing spills)
 template <typename Tin, typename Tout, int NB, bool PAIR>
-__global__ void __launch_bounds__(64 * DM_WAVES) k_dft_mfma(const cplx<Tin>* __restrict__ in, cplx<Tout>* __restrict__ out,
+__global__ void __launch_bounds__(64 * DM_WAVES, (PAIR ? (NB == 1 ? 6 : NB == 2 ? 4 : 2) : (NB == 1 ? 8 : NB == 2 ? 5 : 4))) k_dft_mfma(const cplx<Tin>* __restrict__ in, cplx<Tout>* __restrict__ out,
                                                           const cplx<double>* __restrict__ P, int64_t A, int L, int N,
                                                           int64_t B) {
     // PAIR (complex64 in and out, B even): a lane owns the adjacent columns b, b + 1 (16-byte

@@ -362,7 +362,8 @@ __device__ inline wbmi_f4 mfma3(const SplitH& a, const SplitH& b, wbmi_f4 acc) {
 
 __device__ inline SplitH neg_split(const SplitH& a) { return SplitH{neg_h(a.hi), neg_h(a.lo)}; }
 
-__global__ void __launch_bounds__(64 * ROTH_WAVES) k_rotate_h(int nw, int nch, const cplx<float>* __restrict__ U,
+// (4 waves per SIMD: the accumulators in VGPRs rather than AGPRs, 128 registers in all)
+__global__ void __launch_bounds__(64 * ROTH_WAVES, 4) k_rotate_h(int nw, int nch, const cplx<float>* __restrict__ U,
                                                              cplx<float>* X, uint64_t herm_lo, uint64_t herm_hi,
                                                              uint64_t skip_lo, uint64_t skip_hi) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
