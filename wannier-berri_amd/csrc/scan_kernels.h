@@ -40,6 +40,7 @@
 #pragma once
 
 #include <cmath>
+#include <type_traits>
 
 #include "mfma_c64.h"
 #include "wbmi_common.h"
@@ -73,7 +74,7 @@ struct ScanArgs {
 
 // dynamic shared-memory layout
 struct Smem {
-    char* V;            // [3][nw][ld] staged V, in the storage type of the channels
+    char* V;            // [3][nw][ld] staged V (element type scan_stage)
     char* A;            // [3][nw][ld] staged A (external terms)
     cplx<double>* acc;  // [nw][9]
     double* M;          // [nw][ld] dEinv o Off
@@ -106,7 +107,7 @@ struct SmemOffsets {
     size_t V, A, acc, M, E, cut, gw, ints, tet, tints, total;
 };
 
-// vsz: bytes of one staged V / A element (the channels' storage type)
+// vsz: bytes of one staged V / A element
 __host__ __device__ inline SmemOffsets smem_offsets(int nw, bool stage_v, bool stage_a, bool tetra, size_t vsz) {
     const size_t ld = nw + 1;
     SmemOffsets o;
@@ -175,7 +176,7 @@ __device__ __attribute__((always_inline)) inline Smem make_smem(char* base, int 
 // channels; SA: the same for A (staged together with V, and only with external terms, the only
 // case that reads it).
 // Both are compile-time so that every access is a plain LDS or global load.
-template <typename T, bool LM, bool ST, bool SA, int NW = 0>
+template <typename T, bool LM, bool ST, bool SA, int NW = 0, typename TS = T>
 struct KCtx {
     const cplx<T>* X;  // channels of this k: [nch][nw][nw]
     int nw, ld;  // run-time size (NW = 0); NW > 0 fixes both at compile time
@@ -193,15 +194,15 @@ struct KCtx {
     const int* lab;
     const int* gs;
     const int* ge;
-    const cplx<T>* sV;  // LDS copies of V and A (ST), in the storage type
-    const cplx<T>* sA;
+    const cplx<TS>* sV;  // LDS copies of V and A (ST), element type TS
+    const cplx<TS>* sA;
     __device__ inline cplx<double> get(int role, int x, int y) const {
         cplx<T> v = X[((int64_t)ch[role] * n_() + x) * n_() + y];
         return mk((double)v.re, (double)v.im);
     }
     __device__ inline cplx<double> V(int a, int x, int y) const {
         if constexpr (ST) {
-            const cplx<T> v = sV[(a * n_() + x) * ld_() + y];
+            const cplx<TS> v = sV[(a * n_() + x) * ld_() + y];
             return mk((double)v.re, (double)v.im);
         } else {
             return get(WBMI_ROLE_V + a, x, y);
@@ -209,7 +210,7 @@ struct KCtx {
     }
     __device__ inline cplx<double> A(int a, int x, int y) const {
         if constexpr (SA) {
-            const cplx<T> v = sA[(a * n_() + x) * ld_() + y];
+            const cplx<TS> v = sA[(a * n_() + x) * ld_() + y];
             return mk((double)v.re, (double)v.im);
         } else {
             return get(WBMI_ROLE_A + a, x, y);
@@ -244,8 +245,8 @@ struct KCtx {
 // pre-Hermitian Omega summand s^c_xy for x, y in the same group
 // (formula/covariant.py:175-200): Omega.nn = s + s^+.  The sum over the
 // off-group bands l runs over [l0, l1); `tail` adds the within-group and O terms.
-template <typename T, bool LM, bool ST, bool SA, int NW>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA, NW>& K, int c, int x, int y, bool internal,
+template <typename T, bool LM, bool ST, bool SA, int NW, typename TS>
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA, NW, TS>& K, int c, int x, int y, bool internal,
                                                               bool external, int l0, int l1, bool tail) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
@@ -261,16 +262,16 @@ __device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM,
     }
     return s;
 }
-template <typename T, bool LM, bool ST, bool SA, int NW>
-__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA, NW>& K, int c, int x, int y, bool internal,
+template <typename T, bool LM, bool ST, bool SA, int NW, typename TS>
+__device__ __attribute__((always_inline)) cplx<double> omega_s(const KCtx<T, LM, ST, SA, NW, TS>& K, int c, int x, int y, bool internal,
                                                               bool external) {
     return omega_s(K, c, x, y, internal, external, 0, K.n_(), true);
 }
 
 // pre-Hermitian Morb_H summand h^c_xy for x, y in the same group
 // (formula/covariant.py:390-414)
-template <typename T, bool LM, bool ST, bool SA, int NW>
-__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM, ST, SA, NW>& K, int c, int x, int y, bool internal, bool external) {
+template <typename T, bool LM, bool ST, bool SA, int NW, typename TS>
+__device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM, ST, SA, NW, TS>& K, int c, int x, int y, bool internal, bool external) {
     const int al = c_alpha[c], be = c_beta[c];
     cplx<double> s = mk(0.0, 0.0);
     for (int l = 0; l < K.n_(); ++l) {
@@ -374,8 +375,8 @@ __host__ __device__ constexpr int task_nt() {
 // kind 1's transposed product is -conj of kind 0's at the same element, and
 // its elementwise channel W^{db}_yx = conj(W^{bd}_xy): one product and one
 // load serve both consumers.
-template <int KIND, bool M1, typename T, bool LM, bool ST, bool SA, int NW>
-__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST, SA, NW>& K, const TaskCtx& tc_,
+template <int KIND, bool M1, typename T, bool LM, bool ST, bool SA, int NW, typename TS>
+__device__ __attribute__((always_inline)) void formula_task(const Smem& S, const KCtx<T, LM, ST, SA, NW, TS>& K, const TaskCtx& tc_,
                                                            cplx<double>* acc, int b, int d, int rh, int cb) {
     const int nw = K.n_(), lane = threadIdx.x & 63;
     const int r0 = rh * 16, c0 = cb * 32;  // a 16 x 32 half block
@@ -656,8 +657,8 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
 // DerMorb = DerMorb_H + sign [Eav o dOmega + (Omega V + V Omega)/2] on cut
 // partitions; its trace terms (s1 ... s7, the Omega block) are listed at
 // formula_task.
-template <typename T, bool LM, bool ST, bool SA, int NW>
-__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST, SA, NW>& K, bool in, bool ex,
+template <typename T, bool LM, bool ST, bool SA, int NW, typename TS>
+__device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST, SA, NW, TS>& K, bool in, bool ex,
                                                             bool dm, int sign) {
     const int nw = K.n_(), tid = threadIdx.x;
     // per-band terms, lanes running over (n, c, d)
@@ -763,8 +764,8 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
 }
 
 // Per-band values for the current partition into S.acc[n][comp] (.re).
-template <int F, typename T, bool LM, bool ST, bool SA, int NW>
-__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM, ST, SA, NW>& K, const wbmi_scan_params& p) {
+template <int F, typename T, bool LM, bool ST, bool SA, int NW, typename TS>
+__device__ __attribute__((always_inline)) void eval_partition(const Smem& S, const KCtx<T, LM, ST, SA, NW, TS>& K, const wbmi_scan_params& p) {
     const int nw = K.n_();
     const int tid = threadIdx.x;
     const bool in = p.internal_terms, ex = p.external_terms;
@@ -1024,15 +1025,21 @@ __device__ inline int fermi_bin(const wbmi_scan_params& p, double E) {
 // MODE: 0 = Fermi-level bins, 1 = tetrahedron weights, 2 = per-band table (hist is then
 // out[nk][nw][ncomp]: each band gets its group's trace / group size, or, for the identity
 // formula, the group's mean energy)
+// element type of the staged V and A: complex128 for DerOmega / DerMorb (one workgroup per CU
+// holds them anyway; converted once at staging instead of at every access), the channels'
+// storage type for Omega (two complex64 workgroups per CU)
+template <typename T, int F>
+struct scan_stage {
+    typedef typename std::conditional<F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB, double, T>::type type;
+};
+
 // workgroup size of k_static_scan<T, F, .., MODE> (its launch bound).  The tetrahedron
-// variants (MODE 1) of DerOmega / DerMorb keep 4-wave (complex64) and 8-wave (complex128)
-// workgroups: with their weight code inlined, the 168-register bound of 12-wave workgroups
-// spills several hundred registers
+// variants (MODE 1) of DerOmega / DerMorb keep 8-wave workgroups: with their weight code
+// inlined, the 168-register bound of 12-wave workgroups spills several hundred registers
 template <typename T, int F, int MODE>
 __host__ __device__ constexpr int scan_threads() {
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
-    if (!tasks) return SC_THREADS;
-    if (MODE == 1) return sizeof(T) == 4 ? 256 : SC_THREADS;
+    if (!tasks || MODE == 1) return SC_THREADS;
     return SC_TASK_THREADS;
 }
 
@@ -1049,12 +1056,13 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
     if (MODE != 2) hist += k * args.hist_stride;
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
     static_assert(tasks || F == WBMI_FORMULA_OMEGA || !ST, "V and A are staged for DerOmega, DerMorb and Omega");
+    using TS = typename scan_stage<T, F>::type;
     constexpr bool SA = ST;
     constexpr bool stage_v = ST;
     const bool stage_a = SA && p.external_terms;
     constexpr bool tetra = MODE == 1;  // compile-time: the Fermi-bin kernels carry no tetrahedron code
     constexpr bool tab = MODE == 2;
-    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tetra, sizeof(cplx<T>));
+    const Smem S = make_smem(smem_raw, nw, stage_v, stage_a, tetra, sizeof(cplx<TS>));
     for (int n = tid; n < nw; n += (int)blockDim.x) S.E[n] = Eall[k * nw + n];
     for (int r = tid; r < WBMI_NROLES; r += (int)blockDim.x) S.ch[r] = p.ch[r];
     if (tetra)
@@ -1096,20 +1104,22 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
         }
         const cplx<T>* srcv[3] = {Xk + (int64_t)S.ch[WBMI_ROLE_V] * nw2, Xk + (int64_t)S.ch[WBMI_ROLE_V + 1] * nw2,
                                   Xk + (int64_t)S.ch[WBMI_ROLE_V + 2] * nw2};
-        cplx<T>* dv = (cplx<T>*)S.V;
+        cplx<TS>* dv = (cplx<TS>*)S.V;
 #pragma unroll
         for (int u = 0; u < NS; ++u) {
             const int a = so[u] >> 12, xy = so[u] & 4095;
-            dv[lo[u]] = (a == 0 ? srcv[0] : a == 1 ? srcv[1] : srcv[2])[xy];
+            const cplx<T> v = (a == 0 ? srcv[0] : a == 1 ? srcv[1] : srcv[2])[xy];
+            dv[lo[u]] = mk((TS)v.re, (TS)v.im);
         }
         if (stage_a) {
             const cplx<T>* srca[3] = {Xk + (int64_t)S.ch[WBMI_ROLE_A] * nw2, Xk + (int64_t)S.ch[WBMI_ROLE_A + 1] * nw2,
                                       Xk + (int64_t)S.ch[WBMI_ROLE_A + 2] * nw2};
-            cplx<T>* da = (cplx<T>*)S.A;
+            cplx<TS>* da = (cplx<TS>*)S.A;
 #pragma unroll
             for (int u = 0; u < NS; ++u) {
                 const int a = so[u] >> 12, xy = so[u] & 4095;
-                da[lo[u]] = (a == 0 ? srca[0] : a == 1 ? srca[1] : srca[2])[xy];
+                const cplx<T> v = (a == 0 ? srca[0] : a == 1 ? srca[1] : srca[2])[xy];
+                da[lo[u]] = mk((TS)v.re, (TS)v.im);
             }
         }
     }
@@ -1208,7 +1218,7 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
     const int ng = S.misc[0];
     const int sea_lo = S.misc[1], sea_hi = S.misc[2];
 
-    KCtx<T, LM, ST, SA, NW> K;
+    KCtx<T, LM, ST, SA, NW, TS> K;
     K.X = Xk;
     K.nw = nw;
     K.ld = ld;
@@ -1218,8 +1228,8 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
     K.lab = S.lab;
     K.gs = S.gs;
     K.ge = S.ge;
-    K.sV = (const cplx<T>*)S.V;
-    K.sA = (const cplx<T>*)S.A;
+    K.sV = (const cplx<TS>*)S.V;
+    K.sA = (const cplx<TS>*)S.A;
 
     if (additive || tab) {
         // main partition: every group at once (each group's trace with inn = the group,
@@ -1340,10 +1350,9 @@ template <typename T, int F, bool LM, bool ST, int MODE, int NW = 0>
 static int launch_scan_t(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
     const bool stage_a = ST && a.p.external_terms;
     // DerOmega / DerMorb: 12 waves (3 per SIMD at <= 168 VGPRs), one workgroup per CU (at nw = 32
-    // LDS holds V and A: 118 KB as complex128, 66 KB as complex64; measured faster than two
-    // 4-wave workgroups per CU for complex64)
+    // LDS holds V and A, 118 KB as complex128)
     const int threads = scan_threads<T, F, MODE>();
-    size_t lds = smem_offsets(a.nw, ST, stage_a, MODE == 1, sizeof(cplx<T>)).total;
+    size_t lds = smem_offsets(a.nw, ST, stage_a, MODE == 1, sizeof(cplx<typename scan_stage<T, F>::type>)).total;
     if (lds > 160 * 1024) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: %zu bytes of LDS needed", lds);
     WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_static_scan<T, F, LM, ST, MODE, NW>,
                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
