@@ -149,7 +149,9 @@ int wbmi_build_channels(wbmi_ctx* ctx, int nR, int nw, const double* cRshift_dev
 int wbmi_r2k_plan_create(wbmi_ctx* ctx, int nR, const int32_t* iRvec_host,
                          const int32_t NKFFT[3], int64_t nelem, wbmi_r2k_plan** out);
 int wbmi_r2k_plan_destroy(wbmi_r2k_plan* plan);
-/* bytes of box storage (complex128) and of pass workspace (complex of `prec`) */
+/* bytes of box storage (complex128) and of the per-call workspace (the pass phase tables and
+ * the intermediate pass outputs): calls that do not share a workspace (and, for an aliased
+ * plan, a box) may run concurrently on different streams                     */
 int wbmi_r2k_plan_sizes(const wbmi_r2k_plan* plan, size_t* box_bytes, size_t* work_bytes);
 /* scatter X[R, e] (complex128) into the box; needed once per system when no
  * axis aliases, and is repeated by wbmi_r2k_exec otherwise                  */

@@ -79,6 +79,8 @@ def test_r2k_channels(wb, ctx, NKFFT, prec):
     dev = device_system(s, ctx)
     stack = dev.stack(keys)
     plan, box, aliased = dev.plan(keys, NKFFT)
+    if box is None:   # aliased plan: the box is rebuilt per call into the caller's buffer
+        box = torch.empty(plan.box_bytes // 16, dtype=torch.complex128, device="cuda")
     dK = np.array([0.13, -0.21, 0.37])
     nk = int(np.prod(NKFFT))
     nw = s.num_wann
@@ -111,6 +113,8 @@ def test_r2k_odd_columns(wb, ctx):
     stack = dev.stack(keys)
     NKFFT = (3, 5, 7)
     plan, box, aliased = dev.plan(keys, NKFFT)
+    if box is None:   # aliased plan: the box is rebuilt per call into the caller's buffer
+        box = torch.empty(plan.box_bytes // 16, dtype=torch.complex128, device="cuda")
     dK = np.array([0.11, 0.02, -0.3])
     nk = int(np.prod(NKFFT))
     outs = []
@@ -890,6 +894,30 @@ def test_bcc_nw32_workload(wb, bcc32, precision, tol):
         e = rel_err(r.results[k].data, ro[k].data)
         say(f"  {precision} {k}: rel err {e:.2e}")
         assert e < tol[k], k
+
+
+@pytest.mark.parametrize("precision,NKFFT", [("fp64", (4, 4, 4)), ("fp32", (4, 4, 4)), ("fp32", (3, 2, 4))])
+def test_run_prefetch(wb, precision, NKFFT):
+    """run() with the next K-block's R -> k prefetched on a side stream equals run() without it
+    (27 K-blocks; AHC, BCD sea, optical conductivity; (3, 2, 4) aliases every axis of the
+    |R_i| <= 2 model, so each call rebuilds its own box)."""
+    say(f"test_run_prefetch {precision} {NKFFT}")
+    from wannierberri_amd import models
+    s = models.synthetic(num_wann=12, Rmax=2, seed=31)
+    Ef = np.linspace(-3, 3, 61)
+    calcs = {"ahc": wb.calculators.static.AHC(Efermi=Ef), "bcd": wb.calculators.static.BerryDipole_FermiSea(Efermi=Ef),
+             "opt": wb.calculators.dynamic.OpticalConductivity(Efermi=np.array([-0.5, 0.5]), omega=np.linspace(0, 2, 5),
+                                                               smr_fixed_width=0.2, smr_type="Gaussian")}
+    NK = [3 * n for n in NKFFT]
+    grid = wb.Grid(system=s, NK=NK, NKFFT=list(NKFFT))
+    out = []
+    for pf in (False, True):
+        r = wb.run(s, grid, calcs, parameters_K={"precision": precision}, save_files=False, verbose=False, prefetch=pf)
+        out.append({k: np.array(v.data) for k, v in r.results.items()})
+    for k in calcs:
+        e = rel_err(out[1][k], out[0][k])
+        say(f"  {k}: {e:.1e}")
+        assert e < 1e-12, k
 
 
 @pytest.fixture(scope="module")

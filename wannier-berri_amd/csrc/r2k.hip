@@ -90,7 +90,6 @@ struct wbmi_r2k_plan {
     int32_t* slot_start;  // device CSR: R indices per box slot
     int32_t* slot_list;
     int32_t* iR;          // device copy of iRvec
-    cplx<double>* phase;  // device [3][NMAX*LMAX] pass phase tables
     int tab;              // stride of one axis table
 };
 
@@ -275,9 +274,11 @@ static int launch_pass_p(wbmi_ctx* ctx, const void* in, void* out, const cplx<do
 template <typename Tin, typename Tout>
 static int launch_pass(wbmi_ctx* ctx, const void* in, void* out, const cplx<double>* P, int64_t A, int L, int N,
                        int64_t B) {
-    // complex64 -> complex64 with an even column count (every row 16-byte aligned): column pairs
+    // complex64 -> complex64 with an even column count and 16-byte aligned arrays (so every row
+    // is 16-byte aligned): column pairs
     if constexpr (sizeof(Tin) == 4 && sizeof(Tout) == 4)
-        if ((B & 1) == 0) return launch_pass_p<Tin, Tout, true>(ctx, in, out, P, A, L, N, B);
+        if ((B & 1) == 0 && (((uintptr_t)in | (uintptr_t)out) & 15) == 0)
+            return launch_pass_p<Tin, Tout, true>(ctx, in, out, P, A, L, N, B);
     return launch_pass_p<Tin, Tout, false>(ctx, in, out, P, A, L, N, B);
 }
 
@@ -334,7 +335,6 @@ int wbmi_r2k_plan_create(wbmi_ctx* ctx, int nR, const int32_t* iRvec_host, const
     hipError_t e = hipMalloc(&p->slot_start, sizeof(int32_t) * (p->nslot + 1));
     if (e == hipSuccess) e = hipMalloc(&p->slot_list, sizeof(int32_t) * nR);
     if (e == hipSuccess) e = hipMalloc(&p->iR, sizeof(int32_t) * 3 * nR);
-    if (e == hipSuccess) e = hipMalloc(&p->phase, sizeof(cplx<double>) * 3 * p->tab);
     if (e == hipSuccess) e = hipMemcpy(p->slot_start, start.data(), sizeof(int32_t) * (p->nslot + 1), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->slot_list, flat.data(), sizeof(int32_t) * nR, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->iR, iRvec_host, sizeof(int32_t) * 3 * nR, hipMemcpyHostToDevice);
@@ -351,18 +351,24 @@ int wbmi_r2k_plan_destroy(wbmi_r2k_plan* p) {
     if (p->slot_start) (void)hipFree(p->slot_start);
     if (p->slot_list) (void)hipFree(p->slot_list);
     if (p->iR) (void)hipFree(p->iR);
-    if (p->phase) (void)hipFree(p->phase);
     delete p;
     return WBMI_OK;
+}
+
+// bytes at the head of the workspace for the pass phase tables [3][tab] (256-byte multiple):
+// per call, so executions of one plan on different streams share no state
+static size_t r2k_phase_bytes(const wbmi_r2k_plan* p) {
+    return (sizeof(cplx<double>) * 3 * (size_t)p->tab + 255) & ~(size_t)255;
 }
 
 int wbmi_r2k_plan_sizes(const wbmi_r2k_plan* p, size_t* box_bytes, size_t* work_bytes) {
     if (!p) return WBMI_ERR_ARG;
     if (box_bytes) *box_bytes = sizeof(cplx<double>) * (size_t)p->nslot * p->nelem;
-    // two intermediate pass outputs, sized for complex128 (also enough for complex64)
+    // the three pass phase tables, then two intermediate pass outputs, sized for complex128
+    // (also enough for complex64)
     size_t w1 = (size_t)p->L[0] * p->L[1] * p->N[2];
     size_t w2 = (size_t)p->L[0] * p->N[1] * p->N[2];
-    if (work_bytes) *work_bytes = sizeof(cplx<double>) * (w1 + w2) * p->nelem;
+    if (work_bytes) *work_bytes = r2k_phase_bytes(p) + sizeof(cplx<double>) * (w1 + w2) * p->nelem;
     return WBMI_OK;
 }
 
@@ -392,29 +398,30 @@ int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* XR_dev, voi
         WBMI_LAUNCH_CHECK(ctx);
     }
     for (int i = 0; i < 3; ++i) dpass[i] = p->alias[i] ? 0.0 : dK[i];
+    cplx<double>* phase = (cplx<double>*)work_dev;
     hipLaunchKernelGGL(k_pass_phases, dim3(4, 3), dim3(256), 0, ctx->stream, p->N[0], p->N[1], p->N[2], p->L[0],
                        p->L[1], p->L[2], p->off[0], p->off[1], p->off[2], dpass[0], dpass[1], dpass[2], p->tab,
-                       p->phase);
+                       phase);
     WBMI_LAUNCH_CHECK(ctx);
     const int64_t E = p->nelem;
     const int L0 = p->L[0], L1 = p->L[1], L2 = p->L[2], N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
     size_t esz = prec == WBMI_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
-    char* w1 = (char*)work_dev;
+    char* w1 = (char*)work_dev + r2k_phase_bytes(p);
     char* w2 = w1 + esz * (size_t)L0 * L1 * N2 * E;
     int rc;
     // z pass: [L0 L1][L2][E] -> [L0 L1][N2][E]
     if (prec == WBMI_F64) {
-        rc = launch_pass<double, double>(ctx, box_dev, w1, p->phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
+        rc = launch_pass<double, double>(ctx, box_dev, w1, phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
         if (rc) return rc;
-        rc = launch_pass<double, double>(ctx, w1, w2, p->phase + 1 * p->tab, L0, L1, N1, (int64_t)N2 * E);
+        rc = launch_pass<double, double>(ctx, w1, w2, phase + 1 * p->tab, L0, L1, N1, (int64_t)N2 * E);
         if (rc) return rc;
-        rc = launch_pass<double, double>(ctx, w2, out_dev, p->phase, 1, L0, N0, (int64_t)N1 * N2 * E);
+        rc = launch_pass<double, double>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * N2 * E);
     } else {
-        rc = launch_pass<double, float>(ctx, box_dev, w1, p->phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
+        rc = launch_pass<double, float>(ctx, box_dev, w1, phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
         if (rc) return rc;
-        rc = launch_pass<float, float>(ctx, w1, w2, p->phase + 1 * p->tab, L0, L1, N1, (int64_t)N2 * E);
+        rc = launch_pass<float, float>(ctx, w1, w2, phase + 1 * p->tab, L0, L1, N1, (int64_t)N2 * E);
         if (rc) return rc;
-        rc = launch_pass<float, float>(ctx, w2, out_dev, p->phase, 1, L0, N0, (int64_t)N1 * N2 * E);
+        rc = launch_pass<float, float>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * N2 * E);
     }
     return rc;
 }

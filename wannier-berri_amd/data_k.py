@@ -124,9 +124,10 @@ class DeviceSystem:
             st = self.stack(keys)
             nelem = st[0].numel()
             plan = self.ctx.r2k_plan(self.iRvec, NKFFT, nelem)
-            box = self.torch.empty(plan.box_bytes // 16, dtype=self.torch.complex128, device=self.ctx.device)
             aliased = _is_aliased(self.iRvec, NKFFT)
+            box = None   # aliased: rebuilt per call into a box of the caller's
             if not aliased:
+                box = self.torch.empty(plan.box_bytes // 16, dtype=self.torch.complex128, device=self.ctx.device)
                 plan.box(st, box)
             self._plans[key] = (plan, box, aliased)
         return self._plans[key]
@@ -205,6 +206,7 @@ class Data_K:
         self.E_dev = None
         self.U_dev = None
         self._Ec = None
+        self._pref = None
 
     # -- evaluation -------------------------------------------------------------
     def request(self, keys):
@@ -220,9 +222,10 @@ class Data_K:
             self._requested |= set(self._slots)
         self._evaluate(sorted(self._requested, key=CHANNEL_ORDER.index))
 
-    def _transform(self, keys, dtype, prec, shift=None):
+    def _transform(self, keys, dtype, prec, shift=None, keep=None):
         """R -> k of the channel keys into a new [nk, nch, nw, nw] tensor (grid blocks:
-        at the points j/NKFFT + dK, or + shift in place of dK)."""
+        at the points j/NKFFT + dK, or + shift in place of dK).  `keep`: a list that takes the
+        work buffer instead of freeing it (work launched on another stream)."""
         torch = self.torch
         nw, nk = self.num_wann, self.nk
         stack = self.dev.stack(keys)
@@ -230,12 +233,74 @@ class Data_K:
         if self.k_list is None:
             plan, box, aliased = self.dev.plan(keys, self.NKFFT)
             work = torch.empty(plan.work_bytes, dtype=torch.uint8, device=self.ctx.device)
+            if aliased:
+                box = torch.empty(plan.box_bytes // 16, dtype=torch.complex128, device=self.ctx.device)
             plan.execute(stack if aliased else None, box, self.dK if shift is None else shift, work, X, prec)
+            if keep is not None:
+                keep.extend([work, box])
             del work
         else:
             kl = self.k_list_dev if self.k_list_dev is not None else torch.from_numpy(self.k_list).to(self.ctx.device)
             self.ctx.r2k_klist(self.dev.iRvec_dev, stack, kl, X, prec)
         return X
+
+    def _channel_keys(self):
+        return sorted(self._requested | {("Ham", 0)}, key=CHANNEL_ORDER.index)
+
+    def _transforms(self, keys, keep=None):
+        """R -> k of `keys`: (the stack in the form's type, the FP64 Hamiltonian of the mixed form)."""
+        torch = self.torch
+        mixed = self.prec == WBMI_F32
+        main = [k for k in keys if not (mixed and k == ("Ham", 0))]
+        X = self._transform(main, self.cdtype, self.prec, keep=keep) if main else None
+        H64 = self._transform([("Ham", 0)], torch.complex128, WBMI_F64, keep=keep) if mixed else None
+        return X, H64
+
+    def prefetch(self, stream):
+        """Start this grid K-block's R -> k transforms of the requested channels on `stream`
+        (a side stream), so that they run while the previous K-block is still being evaluated
+        on the context's stream; the evaluation waits for them with an event.
+
+        Every buffer comes from the context stream's memory pool: `stream` first waits for the
+        work queued on the context's stream so far (so memory freed there is no longer in use),
+        and the work buffers stay referenced until the context's stream has waited for the
+        event, after which they are freed in its order."""
+        torch = self.torch
+        if self.k_list is not None or self._slots is not None or self._pref is not None:
+            return
+        keys = self._channel_keys()
+        main = self.ctx.stream
+        stream.wait_stream(main)
+        keep = []
+        prev = self.ctx.set_stream(stream)
+        try:
+            X, H64 = self._transforms(keys, keep=keep)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        finally:
+            self.ctx.set_stream(prev)
+        self._pref = (keys, X, H64, ev, keep)
+
+    def __del__(self):
+        # a prefetched block dropped unevaluated: its buffers return to the context stream's
+        # pool only after the side-stream work that writes them
+        pref = getattr(self, "_pref", None)
+        if pref is not None:
+            try:
+                self.ctx.stream.wait_event(pref[3])
+            except Exception:
+                pass
+
+    def _take_prefetch(self, keys):
+        """The prefetched transforms when they hold exactly `keys`, ordered before the context
+        stream's next work; else (None, None)."""
+        pref, self._pref = self._pref, None
+        if pref is None:
+            return None, None
+        self.ctx.stream.wait_event(pref[3])
+        if pref[0] != list(keys):
+            return None, None
+        return pref[1], pref[2]
 
     def _evaluate(self, keys):
         """Channels `keys` (always including the Hamiltonian) -> E, U and the rotated stack.
@@ -258,7 +323,9 @@ class Data_K:
         nch = s
         E = torch.empty((nk, nw), dtype=torch.float64, device=self.ctx.device)
         U = torch.empty((nk, nw, nw), dtype=self.cdtype, device=self.ctx.device)
-        X = self._transform(main, self.cdtype, self.prec) if main else None
+        X, H64 = self._take_prefetch(keys)
+        if X is None and H64 is None:
+            X, H64 = self._transforms(keys)
         # starting eigenvectors: the previous K-block's at the same k index (one grid step
         # away on the NK grid), when it was a grid block of the same shape and form
         U0 = None
@@ -266,7 +333,6 @@ class Data_K:
         if self.k_list is None and hint is not None and hint.shape == U.shape and hint.dtype == U.dtype:
             U0 = hint
         if mixed:
-            H64 = self._transform([("Ham", 0)], torch.complex128, WBMI_F64)
             self.ctx.eigh(None, nw * nw, E, U, self.prec, nw, refine=OA_STEPS, H64=H64, U0=U0)
             del H64
         else:

@@ -21,14 +21,16 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
         suffix="", parameters_K=None, file_Klist_path=None, restart=False, allow_restart=False, dump_results=False,
         restart_iteration=-1, Klist_part=10, parallel=True, print_Kpoints=False, adpt_mesh=2, adpt_fac=1,
         print_progress_step_time=5, print_progress_step_percent=1, data_k_class=None, k_batch=50,
-        save_files=True, verbose=True):
+        save_files=True, verbose=True, prefetch=True):
     """Integrate `calculators` (dict name -> calculator) over `grid`.
 
     With `use_irred_kpt` the K-list holds one K-block per star of the grid's
     point group (and refinement merges equivalent sub-blocks), which implies
     `symmetrize`: every K-block's result is averaged over the group
     (run_grid.py:225-236,258-265,382-386).  `parallel` is accepted for
-    compatibility: one process drives one GPU.
+    compatibility: one process drives one GPU.  With `prefetch` (and a data_k_class that has
+    `prefetch`) the next K-block's R -> k runs on a side stream while the current one is
+    evaluated.
     """
     if restart or dump_results:
         raise NotImplementedError("restart / dump_results are not available on the MI355X path")
@@ -62,12 +64,23 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
     factors = np.array([Kp.factor for Kp in K_list])
     t0 = time.time()
     counter = 0
+    side = None
+    if prefetch and hasattr(data_k_class, "prefetch"):
+        import torch
+        side = torch.cuda.Stream()   # default priority
     for i_iter in range(adpt_num_iter + 1):
         result_iter = None
-        for Kp in K_list:
-            if Kp.was_evaluated_flag:
-                continue
-            data = data_k_class(system, dK=Kp.Kp_fullBZ, grid=grid, Kpoint=Kp, **parameters_K)
+        todo = [Kp for Kp in K_list if not Kp.was_evaluated_flag]
+        nxt = None
+        for i, Kp in enumerate(todo):
+            data = nxt if nxt is not None else data_k_class(system, dK=Kp.Kp_fullBZ, grid=grid, Kpoint=Kp,
+                                                            **parameters_K)
+            nxt = None
+            if side is not None and i + 1 < len(todo):
+                nKp = todo[i + 1]
+                nxt = data_k_class(system, dK=nKp.Kp_fullBZ, grid=grid, Kpoint=nKp, **parameters_K)
+                if i > 0:   # the first block sets up the system's stacks and plans on the main stream
+                    nxt.prefetch(side)
             res = ResultDict({k: v(data) for k, v in calculators.items()})
             del data
             res = system.pointgroup.symmetrize(res) if symmetrize else res
