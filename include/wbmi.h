@@ -52,6 +52,7 @@ extern "C" {
 #define WBMI_FORMULA_VELHPLUS 6   /* formula/covariant.py:804-808          */
 #define WBMI_FORMULA_DERSPIN 7    /* formula/covariant.py:338-342 (GME spin, sea) */
 #define WBMI_FORMULA_VELSPIN 8    /* formula/covariant.py:811-814 (GME spin, surface) */
+#define WBMI_FORMULA_SPIN 9       /* formula/covariant.py:331-335 (Spin: Re S^s_nn)   */
 
 /* channel roles: index into wbmi_scan_params.ch (value = channel slot in the
  * per-k stack, -1 = absent).  Two-index roles are row-major over (x, y). */
@@ -103,13 +104,17 @@ typedef struct wbmi_optcond_params {
 } wbmi_optcond_params;
 
 /* Parameters of the spin Hall conductivity (calculators/dynamic.py:204-241):
- * the optical-conductivity parameters (V and A roles), the spin current
- * (0: Ryoo, 1: Qiao) and the channel slots of Xbar('SS')[s] and, for Ryoo,
- * Xbar('SA')[a, s], Xbar('SHA')[a, s], for Qiao Xbar('SH')[s],
- * Xbar('SR')[a, s], Xbar('SHR')[a, s] (row-major (a, s)).                  */
+ * the optical-conductivity parameters (V and A roles; with external_terms 0 A
+ * is not read), the spin current `kind` (formula/covariant.py:703-756) and the
+ * channel slots of Xbar('SS')[s] and, for Ryoo, Xbar('SA')[a, s],
+ * Xbar('SHA')[a, s], for Qiao Xbar('SH')[s], Xbar('SR')[a, s],
+ * Xbar('SHR')[a, s] (row-major (a, s)).  Ryoo and Qiao need external terms. */
+#define WBMI_SHC_RYOO 0
+#define WBMI_SHC_QIAO 1
+#define WBMI_SHC_SIMPLE 2   /* {S, v} / 2 with the Velocity (covariant.py:721-727) */
 typedef struct wbmi_shc_params {
     wbmi_optcond_params base;
-    int qiao;
+    int kind;
     int ch_S[3];
     int ch_SA[9];
     int ch_SHA[9];

@@ -1105,6 +1105,15 @@ class DOS(CumDOS):
     fder = 1
 
 
+class SpinCalc(StaticCalculator):
+    # static.py:217-228: the Spin formula (covariant.py:331-335) per unit cell
+    Formula = staticmethod(Spin)
+    fder = 0
+
+    def __call__(self, data_K):
+        return super().__call__(data_K) * data_K.cell_volume
+
+
 class Morb(StaticCalculator):
     # static.py:231-247
     Formula = Morb_Hpm
@@ -1273,10 +1282,12 @@ class OpticalConductivity:
 
 
 class SHC(OpticalConductivity):
-    """Kubo spin Hall conductivity (dynamic.py:204-241) with the Ryoo or Qiao spin current
-    (formula/covariant.py:689-756).
+    """Kubo spin Hall conductivity (dynamic.py:204-241) with the Ryoo, Qiao or 'simple' spin
+    current (formula/covariant.py:689-756).
 
     J^{as} = (J' + J'^+)/2,
+      simple: J'^{as}_mn = sum_l SS^s_ml v^a_ln, v the Velocity (covariant.py:321-328:
+              V^a + i Abar^a_ln (E_l - E_n) with external terms),
       Ryoo: J'^{as}_mn = -i (E_n SA^{as}_mn - SHA^{as}_mn) + sum_l SS^s_ml V^a_ln,
       Qiao: J'^{as}_mn = v^a_n SS^s_mn + E_n (-i SR^{as}_mn + sum_l SS^s_ml D^a_ln)
                          - (-i SHR^{as}_mn + sum_l SH^s_ml D^a_ln),  v^a_n = Re V^a_nn;
@@ -1288,7 +1299,7 @@ class SHC(OpticalConductivity):
 
     def __init__(self, Efermi, omega, SHC_type="ryoo", **kw):
         super().__init__(Efermi, omega, **kw)
-        if SHC_type not in ("ryoo", "qiao"):
+        if SHC_type not in ("ryoo", "qiao", "simple"):
             raise NotImplementedError(SHC_type)
         self.SHC_type = SHC_type
 
@@ -1301,10 +1312,16 @@ class SHC(OpticalConductivity):
 
     def __call__(self, data):
         ext = self.kwargs_formula.get("external_terms", True) and not data.system.force_internal_terms_only
-        if not ext:
+        if not ext and self.SHC_type != "simple":
             raise NotImplementedError("the Ryoo spin current needs the external terms (covariant.py:747-749)")
         E = data.E_K
-        if self.SHC_type == "ryoo":
+        if self.SHC_type == "simple":
+            # covariant.py:721-727 with Velocity (321-328)
+            V = data.Xbar("Ham", 1)
+            if ext:
+                V = V + 1j * data.Xbar("AA") * (E[:, :, None, None] - E[:, None, :, None])
+            J = np.einsum("kmls,klna->kmnas", data.Xbar("SS"), V)
+        elif self.SHC_type == "ryoo":
             J = -1j * (E[:, None, :, None, None] * data.Xbar("SA") - data.Xbar("SHA"))
             J = J + np.einsum("kmls,klna->kmnas", data.Xbar("SS"), data.Xbar("Ham", 1))
         else:
@@ -1315,7 +1332,7 @@ class SHC(OpticalConductivity):
             L = -1j * data.Xbar("SHR") + np.einsum("kmls,klna->kmnas", data.Xbar("SH"), D)
             J = v[:, None, :, :, None] * S[:, :, :, None, :] + E[:, None, :, None, None] * K - L
         J = (J + J.swapaxes(1, 2).conj()) / 2
-        A = 1j * data.D_H + data.Xbar("AA")
+        A = 1j * data.D_H + data.Xbar("AA") if ext else 1j * data.D_H
         B = -1j * A
         imAB = np.imag(J[:, :, :, :, None, :] * B.swapaxes(1, 2)[:, :, :, None, :, None])
         restot = np.zeros((len(self.omega), len(self.Efermi) * 27), dtype=complex)

@@ -68,6 +68,9 @@ def main():
            legacy=True)
     system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz", ["Ham", "AA", "BB", "CC", "SS"],
            legacy=False)
+    # Fe_sym (Wannier90, projections) after set_spin_interlaced (tests/common_systems.py:276-279):
+    # Ham and AA of Fe_W90_proj and the SS that set_spin_interlaced wrote
+    system("reference/systems/Fe_W90_proj_set_spin", "sys_fe_w90_proj_set_spin.npz", ["Ham", "AA", "SS"], legacy=True)
     # tellurium from Quantum ESPRESSO (24 WF, 65 R, legacy [m, n, R]); the tests set its point group
     # from the generators C3z, C2x, TimeReversal (tests/common_systems.py:29,1022-1026)
     system("data/Te_qe/system", "sys_te_qe.npz", ["Ham", "AA", "BB", "CC", "SS"], legacy=True)
@@ -103,6 +106,12 @@ def main():
              "gaas_injection_current": I + "dynamic_GaAs_W90-injection_current_iter-0000.npz",
              "fe_opt_SHCryoo": I + "Fe_W90-opt_SHCryoo_iter-0000.npz",
              "fe_opt_SHCqiao": I + "Fe_W90-opt_SHCqiao_iter-0000.npz",
+             # Fe_sym (Wannier90) with set_spin_interlaced, grid 4^3 / 2^3, no symmetry: Spin,
+             # CumDOS and SHC 'simple' with and without external terms (tests/test_run.py:462-484)
+             "fe_set_spin_spin": I + "Fe_set_spin-spin_iter-0000.npz",
+             "fe_set_spin_cumdos": I + "Fe_set_spin-cumdos_iter-0000.npz",
+             "fe_set_spin_SHCsimple": I + "Fe_set_spin-opt_SHCsimple_iter-0000.npz",
+             "fe_set_spin_SHCsimple_internal": I + "Fe_set_spin-opt_SHCsimple_internal_iter-0000.npz",
              # tetrahedron method: Chiral model with C3z (tests/test_run.py:227-238,835-845)
              # and the Haldane C3z refinement with adpt_mesh 3 (tests/test_run.py:730-735)
              "chiral_tetra_ahc": I + "Chiral_tetra-ahc_iter-0000.npz",

@@ -409,6 +409,28 @@ def test_fe_optical_and_spin_hall(wb, results_run, precision, tol):
         assert e < tol, k
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_fe_set_spin(wb, results_run, precision, tol):
+    """Spin, CumDOS and SHC 'simple' (with and without external terms) on Fe (Wannier90,
+    projections) with interlaced spins, grid 4^3 / 2^3, no symmetry, against the reference's
+    fixtures (tests/test_run.py:462-484)."""
+    say(f"test_fe_set_spin {precision}")
+    st, dyn = wb.calculators.static, wb.calculators.dynamic
+    s = gsys(wb, "fe_w90_proj_set_spin")
+    Ef = np.linspace(17, 18, 11)
+    kw = dict(Efermi=np.array([Ef[0], Ef[-1]]), omega=np.arange(0.0, 7.1, 1.0), smr_fixed_width=0.2,
+              smr_type="Gaussian", SHC_type="simple")
+    calcs = {"spin": st.Spin(Efermi=Ef), "cumdos": st.CumDOS(Efermi=Ef),
+             "shc": dyn.SHC(kwargs_formula={"external_terms": True}, **kw),
+             "shc_int": dyn.SHC(kwargs_formula={"external_terms": False}, **kw)}
+    r = run_gpu(wb, s, calcs, [4, 4, 4], [2, 2, 2], precision=precision)
+    for k, ref in (("spin", "fe_set_spin_spin"), ("cumdos", "fe_set_spin_cumdos"), ("shc", "fe_set_spin_SHCsimple"),
+                   ("shc_int", "fe_set_spin_SHCsimple_internal")):
+        e = rel_err(r.results[k].data, results_run[ref])
+        say(f"  {ref}: {e:.2e}")
+        assert e < tol, k
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_gaas_shift_injection_current(wb, results_run, precision):
     """Shift and injection current on GaAs (Wannier90, nw = 16, grid 6^3 / 3^3) against the

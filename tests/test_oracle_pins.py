@@ -365,3 +365,34 @@ def test_gaas_shift_injection_current(results_run):
               NK=[6, 6, 6], NKFFT=[3, 3, 3])
     assert np.abs(r["sc"].data - results_run["gaas_shift_current"]).max() < 1e-9
     assert np.abs(r["ic"].data - results_run["gaas_injection_current"]).max() < 1e-7
+
+
+def test_set_spin_interlaced():
+    """System_R.set_spin_interlaced (system_R.py:500-545) on the stored Fe_W90_proj system
+    reproduces the SS that the reference's set_spin_interlaced wrote there."""
+    import warnings
+    import wannierberri_amd as wb
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_fe_w90_proj_set_spin.npz"))
+    ref = s.get_R_mat("SS").copy()
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")   # the pairs' centres differ, as the reference warns too
+        s.set_spin_interlaced()
+    assert np.abs(s.get_R_mat("SS") - ref).max() < 1e-14
+
+
+def test_fe_set_spin(results_run):
+    """Spin, CumDOS and SHC 'simple' with and without external terms on Fe (Wannier90,
+    projections) with interlaced spins, grid 4^3 / 2^3, no symmetry (tests/test_run.py:462-484):
+    the 'simple' spin current (formula/covariant.py:721-727) with the Velocity's external term
+    (321-328), at the reference test's 1e-8."""
+    o = gsys("fe_w90_proj_set_spin")
+    Ef = np.linspace(17, 18, 11)
+    kw = dict(smr_fixed_width=0.2, smr_type="Gaussian")
+    Ef2, om = np.array([Ef[0], Ef[-1]]), np.arange(0.0, 7.1, 1.0)
+    r = O.run(o, {"spin": O.SpinCalc(Ef), "cumdos": O.CumDOS(Ef),
+                  "shc": O.SHC(Ef2, om, SHC_type="simple", kwargs_formula={"external_terms": True}, **kw),
+                  "shc_int": O.SHC(Ef2, om, SHC_type="simple", kwargs_formula={"external_terms": False}, **kw)},
+              NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    for k, ref in (("spin", "fe_set_spin_spin"), ("cumdos", "fe_set_spin_cumdos"), ("shc", "fe_set_spin_SHCsimple"),
+                   ("shc_int", "fe_set_spin_SHCsimple_internal")):
+        assert rel_err(r[k].data, results_run[ref]) < 1e-8, k

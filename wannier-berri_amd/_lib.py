@@ -29,6 +29,8 @@ FORMULA_DERMORB = 5
 FORMULA_VELHPLUS = 6
 FORMULA_DERSPIN = 7
 FORMULA_VELSPIN = 8
+FORMULA_SPIN = 9
+WBMI_SHC_RYOO, WBMI_SHC_QIAO, WBMI_SHC_SIMPLE = 0, 1, 2
 
 ROLE_V, ROLE_W, ROLE_A, ROLE_AD, ROLE_O, ROLE_OD = 0, 3, 12, 15, 24, 27
 ROLE_B, ROLE_BD, ROLE_C, ROLE_CD = 36, 39, 48, 51
@@ -78,7 +80,7 @@ class OptcondParams(ctypes.Structure):
 
 
 class ShcParams(ctypes.Structure):
-    _fields_ = [("base", OptcondParams), ("qiao", ctypes.c_int), ("ch_S", ctypes.c_int * 3),
+    _fields_ = [("base", OptcondParams), ("kind", ctypes.c_int), ("ch_S", ctypes.c_int * 3),
                 ("ch_SA", ctypes.c_int * 9), ("ch_SHA", ctypes.c_int * 9), ("ch_SH", ctypes.c_int * 3),
                 ("ch_SR", ctypes.c_int * 9), ("ch_SHR", ctypes.c_int * 9)]
 

@@ -114,15 +114,14 @@ class JDOS(Calculator):
 
 
 class SHC(Calculator):
-    r"""Kubo spin Hall conductivity (dynamic.py:224-241) with the Ryoo or Qiao spin current
-    (formula/covariant.py:729-756); SHC_type 'simple' is not available."""
+    r"""Kubo spin Hall conductivity (dynamic.py:224-241) with the Ryoo, Qiao or 'simple' spin
+    current (formula/covariant.py:703-756); 'simple' also without external terms."""
 
     def __init__(self, SHC_type="ryoo", shc_abc=None, Efermi=None, omega=None, kBT=0, smr_fixed_width=0.1,
                  smr_type="Lorentzian", kwargs_formula=None, dtype=complex, **kwargs):
         super().__init__(**kwargs)
-        if SHC_type not in ("ryoo", "qiao"):
-            raise NotImplementedError(f"SHC_type='{SHC_type}' is not available on the MI355X path "
-                                      "(only 'ryoo' and 'qiao')")
+        if SHC_type not in ("ryoo", "qiao", "simple"):
+            raise ValueError(f"spin_current_type must be `qiao` or `ryoo` or `simple`, not {SHC_type}")
         self.SHC_type = SHC_type
         if smr_type not in ("Lorentzian", "Gaussian"):
             raise ValueError(f"Invalid smearing type {smr_type}")
@@ -142,22 +141,25 @@ class SHC(Calculator):
         self._dev_arrays = {}
 
     def needed_channels(self, system):
+        if self.SHC_type == "simple":
+            return {("Ham", 0), ("Ham", 1), ("SS", 0)} | ({("AA", 0)} if _external(self.kwargs_formula, system) else set())
         spin = [("SA", 0), ("SHA", 0)] if self.SHC_type == "ryoo" else [("SH", 0), ("SR", 0), ("SHR", 0)]
         return {("Ham", 0), ("Ham", 1), ("AA", 0), ("SS", 0), *spin}
 
     def __call__(self, data_K):
         torch = data_K.torch
-        if not _external(self.kwargs_formula, data_K.system):
+        ext = _external(self.kwargs_formula, data_K.system)
+        if not ext and self.SHC_type != "simple":
             raise NotImplementedError(f"spin Hall {self.SHC_type} without external terms is not implemented yet. "
                                       "Use `SHC_type='simple'`")
         data_K.ensure(self.needed_channels(data_K.system))
         p = _lib.ShcParams()
         p.base = _params(self, data_K)
-        p.base.external_terms = 1
+        p.base.external_terms = int(ext)
         for i, v in enumerate(data_K.role_table()):
             p.base.ch[i] = v
         sl = data_K._slots
-        p.qiao = int(self.SHC_type == "qiao")
+        p.kind = {"ryoo": _lib.WBMI_SHC_RYOO, "qiao": _lib.WBMI_SHC_QIAO, "simple": _lib.WBMI_SHC_SIMPLE}[self.SHC_type]
         for c in range(3):
             p.ch_S[c] = sl[("SS", 0)] + c
             p.ch_SH[c] = sl[("SH", 0)] + c if ("SH", 0) in sl else -1

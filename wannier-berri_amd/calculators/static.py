@@ -41,9 +41,10 @@ class Calculator:
 # velocity, odd under both, formula/formula.py:127-128, data_K.py:16-46)
 _PARITY = {_lib.FORMULA_IDENTITY: (1, 1), _lib.FORMULA_OMEGA: (-1, 1), _lib.FORMULA_DEROMEGA: (1, -1),
            _lib.FORMULA_VELOMEGA: (1, -1), _lib.FORMULA_MORB_HPM: (-1, 1), _lib.FORMULA_DERMORB: (1, -1),
-           _lib.FORMULA_VELHPLUS: (1, -1), _lib.FORMULA_DERSPIN: (1, -1), _lib.FORMULA_VELSPIN: (1, -1)}
+           _lib.FORMULA_VELHPLUS: (1, -1), _lib.FORMULA_DERSPIN: (1, -1), _lib.FORMULA_VELSPIN: (1, -1),
+           _lib.FORMULA_SPIN: (-1, 1)}
 # formulas that read only the Hamiltonian's derivatives and the spin, never A, B, C
-_SPIN = (_lib.FORMULA_DERSPIN, _lib.FORMULA_VELSPIN)
+_SPIN = (_lib.FORMULA_DERSPIN, _lib.FORMULA_VELSPIN, _lib.FORMULA_SPIN)
 
 
 # relative Hermiticity defect of Ham_R above which the k-derivative formulas refuse a system
@@ -120,6 +121,8 @@ class StaticCalculator(Calculator):
         f = self.formula
         if f == _lib.FORMULA_IDENTITY:
             return {("Ham", 0)}
+        if f == _lib.FORMULA_SPIN:
+            return {("Ham", 0), ("SS", 0)}
         if f in _SPIN:
             return {("Ham", 0), ("Ham", 1), ("SS", 0)} | ({("SS", 1)} if f == _lib.FORMULA_DERSPIN else set())
         need = {("Ham", 0), ("Ham", 1)}
@@ -375,6 +378,18 @@ class GME_orb_FermiSea(StaticCalculator):
         return Hplus - 2 * Omega
 
 
+class Spin(StaticCalculator):
+    r"""Spin per unit cell (dimensionless) (calculators/static.py:217-228)
+
+       | Output: :math:`\int [dk] s f`"""
+    formula = _lib.FORMULA_SPIN
+    ndim = 1
+    fder = 0
+
+    def __call__(self, data_K):
+        return super().__call__(data_K) * data_K.cell_volume
+
+
 class GME_spin_FermiSea(StaticCalculator):
     r"""Gyrotropic tensor, spin part, as a Fermi-sea integral (calculators/static.py:322-335)
 
@@ -405,4 +420,4 @@ class GME_spin_FermiSurf(StaticCalculator):
 
 __all__ = ["Calculator", "StaticCalculator", "DOS", "CumDOS", "AHC", "BerryDipole_FermiSurf", "NLAHC_FermiSurf",
            "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "GME_orb_FermiSurf", "GME_orb_FermiSea",
-           "GME_spin_FermiSea", "GME_spin_FermiSurf", "VoidResult"]
+           "GME_spin_FermiSea", "GME_spin_FermiSurf", "Spin", "VoidResult"]

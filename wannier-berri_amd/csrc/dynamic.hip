@@ -275,6 +275,8 @@ extern "C" int wbmi_jdos(wbmi_ctx* ctx, int nk, int nw, const double* E_dev, con
 
 // J^{as} = (J' + J'^+) / 2, one workgroup per (k, component a*3+s), one thread per (i, j);
 // both J'_ij and J'_ji are formed by the same thread, so no barrier is needed.
+//   simple (covariant.py:721-727): J'^{as}_ij = sum_l SS^s_il v^a_lj, the Velocity
+//     v^a_lj = V^a_lj (+ i Abar^a_lj (E_l - E_j) with external terms, covariant.py:321-328)
 //   Ryoo (covariant.py:746-756): J'^{as}_ij = -i (E_j SA^{as}_ij - SHA^{as}_ij) + sum_l SS^s_il V^a_lj
 //   Qiao (covariant.py:729-744): J'^{as}_ij = v^a_j SS^s_ij + E_j (-i SR^{as}_ij + sum_l SS^s_il D^a_lj)
 //                                             - (-i SHR^{as}_ij + sum_l SH^s_il D^a_lj),
@@ -287,10 +289,12 @@ __global__ void __launch_bounds__(256) k_spin_current(int nk, int nw, int nch, w
     const int k = blockIdx.x, comp = blockIdx.y, a = comp / 3, sp = comp % 3;
     const int64_t nw2 = (int64_t)nw * nw;
     const cplx<T>* Xk = Xall + (int64_t)k * nch * nw2;
-    const bool qiao = p.qiao != 0;
-    // Ryoo: SA, SHA; Qiao: SR, SHR (the same roles in the formula), and SH
-    const cplx<T>* SA = Xk + (qiao ? p.ch_SR[comp] : p.ch_SA[comp]) * nw2;
-    const cplx<T>* SHA = Xk + (qiao ? p.ch_SHR[comp] : p.ch_SHA[comp]) * nw2;
+    const bool qiao = p.kind == WBMI_SHC_QIAO, simple = p.kind == WBMI_SHC_SIMPLE;
+    const bool ext = p.base.external_terms != 0;
+    // Ryoo: SA, SHA; Qiao: SR, SHR (the same roles in the formula), and SH; simple: none
+    const cplx<T>* SA = simple ? nullptr : Xk + (qiao ? p.ch_SR[comp] : p.ch_SA[comp]) * nw2;
+    const cplx<T>* SHA = simple ? nullptr : Xk + (qiao ? p.ch_SHR[comp] : p.ch_SHA[comp]) * nw2;
+    const cplx<T>* Ab = simple && ext ? Xk + p.base.ch[WBMI_ROLE_A + a] * nw2 : nullptr;
     const cplx<T>* S = Xk + p.ch_S[sp] * nw2;
     const cplx<T>* SH = Xk + (qiao ? p.ch_SH[sp] : p.ch_S[sp]) * nw2;
     const cplx<T>* V = Xk + p.base.ch[WBMI_ROLE_V + a] * nw2;
@@ -303,6 +307,19 @@ __global__ void __launch_bounds__(256) k_spin_current(int nk, int nw, int nch, w
         return mk(-m * v.re, -m * v.im);
     };
     auto jprime = [&](int i, int j) {
+        if (simple) {
+            cplx<double> v = mk(0.0, 0.0);
+            for (int l = 0; l < nw; ++l) {
+                cplx<double> vel = ld(V, l * nw + j);
+                if (ext) {
+                    const cplx<double> x = ld(Ab, l * nw + j);
+                    const double de = E[l] - E[j];
+                    vel += mk(-x.im * de, x.re * de);  // i Abar (E_l - E_j)
+                }
+                v += ld(S, i * nw + l) * vel;
+            }
+            return v;
+        }
         const cplx<double> sa = ld(SA, i * nw + j), sha = ld(SHA, i * nw + j);
         // -i (E_j SA - SHA)  (Qiao: -i (E_j SR - SHR))
         cplx<double> v = mk(E[j] * sa.im - sha.im, -(E[j] * sa.re - sha.re));
@@ -672,16 +689,21 @@ extern "C" int wbmi_spin_hall(wbmi_ctx* ctx, int nk, int nw, int nch, const doub
     if (!work_dev || work_bytes < wbmi_spin_hall_work_bytes(nk, nw))
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: workspace of %zu bytes needed",
                          wbmi_spin_hall_work_bytes(nk, nw));
+    const int kind = params->kind;
+    if (kind != WBMI_SHC_RYOO && kind != WBMI_SHC_QIAO && kind != WBMI_SHC_SIMPLE)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: bad spin current kind %d", kind);
+    if (kind != WBMI_SHC_SIMPLE && !params->base.external_terms)
+        return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_spin_hall: the Ryoo and Qiao spin currents need external terms");
+    auto bad = [&](int c) { return c < 0 || c >= nch; };
     for (int c = 0; c < 3; ++c) {
-        const int v = params->base.ch[WBMI_ROLE_V + c], x = params->base.ch[WBMI_ROLE_A + c], s = params->ch_S[c];
-        if (v < 0 || v >= nch || x < 0 || x >= nch || s < 0 || s >= nch)
+        if (bad(params->base.ch[WBMI_ROLE_V + c]) || bad(params->ch_S[c]) ||
+            (params->base.external_terms && bad(params->base.ch[WBMI_ROLE_A + c])))
             return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: missing V, A or SS channel");
     }
-    auto bad = [&](int c) { return c < 0 || c >= nch; };
     for (int c = 0; c < 9; ++c) {
-        if (!params->qiao && (bad(params->ch_SA[c]) || bad(params->ch_SHA[c])))
+        if (kind == WBMI_SHC_RYOO && (bad(params->ch_SA[c]) || bad(params->ch_SHA[c])))
             return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: missing SA or SHA channel");
-        if (params->qiao && (bad(params->ch_SR[c]) || bad(params->ch_SHR[c]) || (c < 3 && bad(params->ch_SH[c]))))
+        if (kind == WBMI_SHC_QIAO && (bad(params->ch_SR[c]) || bad(params->ch_SHR[c]) || (c < 3 && bad(params->ch_SH[c]))))
             return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: missing SH, SR or SHR channel");
     }
     if (prec == WBMI_F64)

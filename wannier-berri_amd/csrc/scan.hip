@@ -40,6 +40,9 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
         case WBMI_FORMULA_VELSPIN:
             if ((rc = need(WBMI_ROLE_V, 3)) || (rc = need(WBMI_ROLE_S, 3))) return rc;
             break;
+        case WBMI_FORMULA_SPIN:
+            if ((rc = need(WBMI_ROLE_S, 3))) return rc;
+            break;
         case WBMI_FORMULA_MORB_HPM:
         case WBMI_FORMULA_VELHPLUS:
             if ((rc = need(WBMI_ROLE_V, 3))) return rc;
@@ -136,7 +139,8 @@ static int ncomp_of_formula(int f) {
     switch (f) {
         case WBMI_FORMULA_IDENTITY: return 1;
         case WBMI_FORMULA_OMEGA:
-        case WBMI_FORMULA_MORB_HPM: return 3;
+        case WBMI_FORMULA_MORB_HPM:
+        case WBMI_FORMULA_SPIN: return 3;
         default: return 9;
     }
 }

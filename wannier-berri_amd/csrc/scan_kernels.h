@@ -289,7 +289,9 @@ __device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM,
 
 template <int F>
 __host__ __device__ constexpr int ncomp_of() {
-    return F == WBMI_FORMULA_IDENTITY ? 1 : (F == WBMI_FORMULA_OMEGA || F == WBMI_FORMULA_MORB_HPM) ? 3 : 9;
+    return F == WBMI_FORMULA_IDENTITY ? 1
+           : (F == WBMI_FORMULA_OMEGA || F == WBMI_FORMULA_MORB_HPM || F == WBMI_FORMULA_SPIN) ? 3
+                                                                                                : 9;
 }
 
 template <int F>
@@ -851,6 +853,12 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
             }
             S.acc[t] = mk(v.re, 0.0);
         }
+    } else if (F == WBMI_FORMULA_SPIN) {
+        // Re S^s_nn: the group trace of Xbar('SS') (formula/covariant.py:331-335)
+        for (int t = tid; t < nw * 3; t += (int)blockDim.x) {
+            const int n = t / 3, sp = t % 3;
+            if (S.act[n]) S.acc[n * 9 + sp] = mk(K.S(sp, n, n).re, 0.0);
+        }
     } else if (F == WBMI_FORMULA_VELSPIN) {
         // Re sum_{n' in G} V^a_nn' S^b_n'n (formula/formula.py:121-148, covariant.py:811-814)
         for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
@@ -1127,7 +1135,7 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
     constexpr bool additive = F == WBMI_FORMULA_IDENTITY || F == WBMI_FORMULA_OMEGA ||
                               F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
                               F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERSPIN ||
-                              F == WBMI_FORMULA_VELSPIN;
+                              F == WBMI_FORMULA_VELSPIN || F == WBMI_FORMULA_SPIN;
 
     // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186), formed by wave 0: each lane
     // owns bands lane and lane + 64 (nw <= 128); borders from two ballots and popcounts, one
@@ -1391,6 +1399,7 @@ static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
         case WBMI_FORMULA_DERMORB: return launch_scan<T, WBMI_FORMULA_DERMORB, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_DERSPIN: return launch_scan<T, WBMI_FORMULA_DERSPIN, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_VELSPIN: return launch_scan<T, WBMI_FORMULA_VELSPIN, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_SPIN: return launch_scan<T, WBMI_FORMULA_SPIN, MODE>(ctx, nk, a, E, X, hist);
         default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
     }
 }

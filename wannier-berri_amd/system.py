@@ -9,6 +9,7 @@ centres used as the derivative shifts (system_R.py:776-780), `periodic`,
 device and cached here.
 """
 import glob
+import warnings
 import os
 
 import numpy as np
@@ -104,6 +105,34 @@ class System_R:
         self._XX_R[key] = value
         self._device_cache.clear()
         self._herm_err = {}
+
+    def set_spin_pairs(self, pairs):
+        """SS_R for Wannier functions that are Sz eigenstates in pairs (up, down)
+        (system/system_R.py:500-531): at R = 0, the Pauli matrices on each pair's 2x2 block,
+        zero elsewhere (states in no pair carry no spin)."""
+        pairs = [tuple(int(x) for x in p) for p in pairs]
+        assert all(len(p) == 2 for p in pairs)
+        states = [x for p in pairs for x in p]
+        assert all(0 <= x < self.num_wann for x in states), "indices of states should be 0 <= i < num_wann"
+        assert len(set(states)) == len(states), "some states appear more than once in pairs"
+        if len(pairs) < self.num_wann / 2:
+            warnings.warn(f"number of spin pairs {len(pairs)} is less than num_wann/2 = {self.num_wann / 2}; "
+                          "the other states get no spin")
+        # (sigma_x, sigma_y, sigma_z)[a, b] of the pair's (up, down) = (0, 1) indices
+        pauli = np.array([[[0, 0, 1], [1, -1j, 0]], [[1, 1j, 0], [0, 0, -1]]], dtype=complex)
+        S0 = np.zeros((self.num_wann, self.num_wann, 3), dtype=complex)
+        for i, j in pairs:
+            if np.linalg.norm(self.wannier_centers_cart[i] - self.wannier_centers_cart[j]) > 1e-3:
+                warnings.warn(f"setting a spin pair for Wannier functions {i} and {j} with different centres")
+            S0[i, i], S0[i, j], S0[j, i], S0[j, j] = pauli[0, 0], pauli[0, 1], pauli[1, 0], pauli[1, 1]
+        SS = np.zeros((self.rvec.nRvec, self.num_wann, self.num_wann, 3), dtype=complex)
+        SS[self.rvec.iR0] = S0
+        self.set_R_mat("SS", SS, reset=True)
+
+    def set_spin_interlaced(self):
+        """set_spin_pairs with the spins interlaced, pairs (2i, 2i + 1) (system/system_R.py:533-545)."""
+        assert self.num_wann % 2 == 0, f"odd number of Wannier functions {self.num_wann} cannot be grouped into spin pairs"
+        self.set_spin_pairs([(2 * i, 2 * i + 1) for i in range(self.num_wann // 2)])
 
     def hermiticity_error(self, key="Ham"):
         """max |X(R) - X(-R)^+| / max |X|, an R-vector without its -R partner counting
