@@ -387,6 +387,18 @@ int wbmi_tabulate_invmass(wbmi_ctx* ctx, int nk, int nw, int nch, const double* 
                           int prec, const int32_t* chV3_host, const int32_t* chW9_host,
                           double degen_thresh, int degen_Kramers, double* out_dev);
 
+/* Per-band spin Berry curvature (tabulate.py:218-221 SpinBerry, formula/covariant.py:759-789
+ * SpinOmega): out[k][n][a*9+b*3+s] = group mean over m of
+ *   -2 Im sum_{l outside the group} J^{as}_ml / (E_m - E_l) (D^b_lm - i Abar^b_lm),
+ * J the spin current of params->kind (as in wbmi_spin_hall), D = D_H, Abar only with
+ * params->base.external_terms.  Reads params->base.ch (V, A), degen_thresh and
+ * degen_Kramers; Efermi / omega are not used.
+ *   work_dev: wbmi_spin_hall_work_bytes(nk, nw) bytes (J per k).
+ *   out_dev : float64 [nk, nw, 27]                                                 */
+int wbmi_tabulate_spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                             int prec, const wbmi_shc_params* params, void* work_dev, size_t work_bytes,
+                             double* out_dev);
+
 /* out_dev [nEF, ncomp] = scale * (prefix sums of rows [0, nEF) + rows [nEF, 2 nEF))
  * of a wbmi_static_scan_tetra histogram (static.py:149-155, no fder stencil).
  * The step rows of hist_dev are overwritten with their prefix sums.         */

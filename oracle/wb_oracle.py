@@ -1281,6 +1281,54 @@ class OpticalConductivity:
         return OracleResult(self.Efermi, restot * FACTOR_OPT / (data.nk * data.system.cell_volume))
 
 
+def spin_current(data, kind, ext):
+    """J[k, m, n, a, s] = (J' + J'^+) / 2, the spin current of type `kind` (SpinVelocity,
+    formula/covariant.py:703-756); Ryoo and Qiao need the external terms."""
+    if not ext and kind != "simple":
+        raise NotImplementedError("the Ryoo and Qiao spin currents need the external terms (covariant.py:730-749)")
+    E = data.E_K
+    if kind == "simple":
+        # covariant.py:721-727 with Velocity (321-328)
+        V = data.Xbar("Ham", 1)
+        if ext:
+            V = V + 1j * data.Xbar("AA") * (E[:, :, None, None] - E[:, None, :, None])
+        J = np.einsum("kmls,klna->kmnas", data.Xbar("SS"), V)
+    elif kind == "ryoo":
+        J = -1j * (E[:, None, :, None, None] * data.Xbar("SA") - data.Xbar("SHA"))
+        J = J + np.einsum("kmls,klna->kmnas", data.Xbar("SS"), data.Xbar("Ham", 1))
+    elif kind == "qiao":
+        # formula/covariant.py:729-744; delE_K = Re diag V (data_K.py:236-242)
+        S, D = data.Xbar("SS"), data.D_H
+        v = np.einsum("klla->kla", data.Xbar("Ham", 1)).real
+        K = -1j * data.Xbar("SR") + np.einsum("kmls,klna->kmnas", S, D)
+        L = -1j * data.Xbar("SHR") + np.einsum("kmls,klna->kmnas", data.Xbar("SH"), D)
+        J = v[:, None, :, :, None] * S[:, :, :, None, :] + E[:, None, :, None, None] * K - L
+    else:
+        raise ValueError(f"spin_current_type must be `qiao` or `ryoo` or `simple`, not {kind}")
+    return (J + J.swapaxes(1, 2).conj()) / 2
+
+
+class SpinOmega(Formula_ln):
+    """Spin Berry curvature (formula/covariant.py:759-789): per (m, n) in the group,
+    -2 Im sum_{l out} J^{as}_ml / (E_m - E_l) (D^b_ln - i A^b_ln)."""
+
+    def __init__(self, data_K, spin_current_type="ryoo", **kw):
+        self._set_terms(data_K, **kw)
+        if self.external_terms:
+            self.A = data_K.covariant('AA')
+        self.D = data_K.Dcov
+        self.J = Matrix_ln(spin_current(data_K, spin_current_type, self.external_terms))
+        self.dEinv = DEinv_ln(data_K)
+        self.ndim = 3
+
+    def nn(self, ik, inn, out):
+        v_over_de = self.D.ln(ik, inn, out)
+        if self.external_terms:
+            v_over_de = v_over_de - 1j * self.A.ln(ik, inn, out)
+        j_over_de = self.J.nl(ik, inn, out) * self.dEinv.nl(ik, inn, out)[:, :, None, None]
+        return -2 * np.einsum("mlas,lnb->mnabs", j_over_de, v_over_de).imag + 0j
+
+
 class SHC(OpticalConductivity):
     """Kubo spin Hall conductivity (dynamic.py:204-241) with the Ryoo, Qiao or 'simple' spin
     current (formula/covariant.py:689-756).
@@ -1300,7 +1348,7 @@ class SHC(OpticalConductivity):
     def __init__(self, Efermi, omega, SHC_type="ryoo", **kw):
         super().__init__(Efermi, omega, **kw)
         if SHC_type not in ("ryoo", "qiao", "simple"):
-            raise NotImplementedError(SHC_type)
+            raise ValueError(SHC_type)
         self.SHC_type = SHC_type
 
     def factor_omega(self, E1, E2):
@@ -1312,26 +1360,7 @@ class SHC(OpticalConductivity):
 
     def __call__(self, data):
         ext = self.kwargs_formula.get("external_terms", True) and not data.system.force_internal_terms_only
-        if not ext and self.SHC_type != "simple":
-            raise NotImplementedError("the Ryoo spin current needs the external terms (covariant.py:747-749)")
-        E = data.E_K
-        if self.SHC_type == "simple":
-            # covariant.py:721-727 with Velocity (321-328)
-            V = data.Xbar("Ham", 1)
-            if ext:
-                V = V + 1j * data.Xbar("AA") * (E[:, :, None, None] - E[:, None, :, None])
-            J = np.einsum("kmls,klna->kmnas", data.Xbar("SS"), V)
-        elif self.SHC_type == "ryoo":
-            J = -1j * (E[:, None, :, None, None] * data.Xbar("SA") - data.Xbar("SHA"))
-            J = J + np.einsum("kmls,klna->kmnas", data.Xbar("SS"), data.Xbar("Ham", 1))
-        else:
-            # formula/covariant.py:729-744; delE_K = Re diag V (data_K.py:236-242)
-            S, D = data.Xbar("SS"), data.D_H
-            v = np.einsum("klla->kla", data.Xbar("Ham", 1)).real
-            K = -1j * data.Xbar("SR") + np.einsum("kmls,klna->kmnas", S, D)
-            L = -1j * data.Xbar("SHR") + np.einsum("kmls,klna->kmnas", data.Xbar("SH"), D)
-            J = v[:, None, :, :, None] * S[:, :, :, None, :] + E[:, None, :, None, None] * K - L
-        J = (J + J.swapaxes(1, 2).conj()) / 2
+        J = spin_current(data, self.SHC_type, ext)
         A = 1j * data.D_H + data.Xbar("AA") if ext else 1j * data.D_H
         B = -1j * A
         imAB = np.imag(J[:, :, :, :, None, :] * B.swapaxes(1, 2)[:, :, :, None, :, None])

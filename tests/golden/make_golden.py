@@ -144,7 +144,8 @@ def main():
     # (tests/test_run.py:271-288,313-345)
     with np.load(os.path.join(REF, I + "Fe_W90-tabulate.npz"), allow_pickle=False) as f:
         np.savez_compressed(os.path.join(HERE, "results_tabulate.npz"),
-                            **{k: f[k] for k in ["Energy", "V", "spin", "berry", "Der_berry", "morb", "Der_morb"]})
+                            **{k: f[k] for k in ["Energy", "V", "spin", "berry", "Der_berry", "morb", "Der_morb",
+                                                  "spin_berry"]})
     # per-formula gauge-invariant traces (tests/reference/formula/*.npz)
     forms = {}
     for name in ["Dcov", "DerDcov", "DEinv_ln", "InvMass", "Omega", "DerOmega", "Morb_H", "morb",

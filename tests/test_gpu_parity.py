@@ -782,7 +782,7 @@ def test_finalize_4001_levels(wb, small_synth):
 @pytest.mark.parametrize("precision,tol", [("fp64", 1.0), ("fp32", 200.0)])
 def test_fe_tabulate(wb, precision, tol):
     """k-resolved tabulation (TabulatorAll: Energy, velocity, spin, Berry curvature and its
-    derivative, orbital moment and its derivative) on Fe (Wannier90), grid 4^3 / 2^3, bands 5 and 6,
+    derivative, orbital moment and its derivative, spin Berry curvature) on Fe (Wannier90), grid 4^3 / 2^3, bands 5 and 6,
     against the reference's fixture at its tolerances (2e-8, Der_berry 5e-8, relative to
     the largest value; tests/test_run.py:271-345); the mixed form at 200x those."""
     say(f"test_fe_tabulate {precision}")
@@ -792,7 +792,8 @@ def test_fe_tabulate(wb, precision, tol):
     calc = tab.TabulatorAll({"Energy": tab.Energy(), "V": tab.Velocity(**pt), "spin": tab.Spin(**pt),
                              "berry": tab.BerryCurvature(**pt),
                              "Der_berry": tab.DerBerryCurvature(**pt), "morb": tab.OrbitalMoment(**pt),
-                             "Der_morb": tab.DerOrbitalMoment(**pt)}, ibands=[5, 6])
+                             "Der_morb": tab.DerOrbitalMoment(**pt), "spin_berry": tab.SpinBerry(**pt)},
+                            ibands=[5, 6])
     grid = wb.Grid(system=s, NK=[4, 4, 4], NKFFT=[2, 2, 2])
     r = wb.run(s, grid, {"tabulate": calc}, use_irred_kpt=False, symmetrize=False,
                parameters_K={"precision": precision}, save_files=False, verbose=False)
@@ -884,6 +885,37 @@ def test_fe_tabulate_invmass(wb):
     ref = ref.reshape(tuple(NK) + (len(ib), 3, 3))
     e = rel_err(got, ref)
     say(f"  InvMass: {e:.2e}")
+    assert e < 1e-10
+
+
+@pytest.mark.parametrize("kind,ext", [("qiao", True), ("simple", True), ("simple", False)])
+def test_fe_tabulate_spin_berry_kinds(wb, kind, ext):
+    """SpinBerry tabulator with the Qiao and 'simple' spin currents (formula/covariant.py:703-789),
+    the latter also without external terms, on Fe (Wannier90), grid 4^3 / 2^3, bands 4..7, vs the
+    oracle's Tabulator with SpinOmega (its Ryoo form is pinned by the reference's tabulate fixture,
+    the spin currents by the Qiao / Ryoo SHC fixtures)."""
+    say(f"test_fe_tabulate_spin_berry_kinds {kind} ext={ext}")
+    tab = wb.calculators.tabulate
+    s = gsys(wb, "fe_w90")
+    ib = [4, 5, 6, 7]
+    kf = {"spin_current_type": kind, "external_terms": ext}
+    calc = tab.TabulatorAll({"sb": tab.SpinBerry(degen_thresh=5e-2, kwargs_formula=kf)}, ibands=ib)
+    grid = wb.Grid(system=s, NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    r = wb.run(s, grid, {"tab": calc}, use_irred_kpt=False, symmetrize=False, save_files=False, verbose=False)
+    got = r.results["tab"].get_data("sb")
+    o = osys("fe_w90")
+    NKdiv, NKFFT, blocks = O.grid_blocks([4, 4, 4], [2, 2, 2])
+    NK = NKdiv * NKFFT
+    ref = np.zeros((int(np.prod(NK)), len(ib), 3, 3, 3))
+    for dK, _ in blocks:
+        d = O.OracleDataK(o, dK=dK, NKFFT=NKFFT)
+        ki = np.rint(d.kpoints_all * NK[None, :]).astype(int) % NK[None, :]
+        idx = (ki[:, 0] * NK[1] + ki[:, 1]) * NK[2] + ki[:, 2]
+        ref[idx] = O.tabulate(d, O.SpinOmega, ib, degen_thresh=5e-2, spin_current_type=kind,
+                              external_terms=ext).real
+    ref = ref.reshape(tuple(NK) + (len(ib), 3, 3, 3))
+    e = rel_err(got, ref)
+    say(f"  SpinBerry {kind} ext={ext}: {e:.2e}")
     assert e < 1e-10
 
 

@@ -326,7 +326,9 @@ TAB_QUANTITIES = {"Energy": (O.Eavln, {}),
                   "berry": (O.Omega, {"degen_thresh": 5e-2}),
                   "Der_berry": (O.DerOmega, {"degen_thresh": 5e-2}),
                   "morb": (O.morb, {"degen_thresh": 5e-2, "constant_factor": O.FACTOR_MORB_EVA2_TO_MUB}),
-                  "Der_morb": (O.Dermorb, {"degen_thresh": 5e-2, "constant_factor": O.FACTOR_MORB_EVA2_TO_MUB})}
+                  "Der_morb": (O.Dermorb, {"degen_thresh": 5e-2, "constant_factor": O.FACTOR_MORB_EVA2_TO_MUB}),
+                  # SpinOmega with the Ryoo spin current (covariant.py:759-789)
+                  "spin_berry": (O.SpinOmega, {"degen_thresh": 5e-2})}
 
 
 def test_fe_tabulate():

@@ -47,7 +47,7 @@ SYMBOLS = [
     "wbmi_tabulate", "wbmi_tabulate_diag", "wbmi_shift_current", "wbmi_injection_current",
     "wbmi_static_scan_k", "wbmi_finalize_k", "wbmi_static_scan_tetra_k", "wbmi_finalize_tetra_k",
     "wbmi_tabulate_invmass", "wbmi_eigh_noalloc", "wbmi_eigh_noalloc_max_bytes", "wbmi_r2k_klist_tile",
-    "wbmi_r2k_klist_tile_shape",
+    "wbmi_r2k_klist_tile_shape", "wbmi_tabulate_spin_berry",
 ]
 
 
@@ -133,6 +133,8 @@ _PROTOS = {
     "wbmi_injection_current": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(OptcondParams), _vp,
                                     ctypes.c_size_t, _vp]),
     "wbmi_spin_hall": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t, _vp]),
+    "wbmi_tabulate_spin_berry": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t,
+                                      _vp]),
 }
 
 
@@ -350,6 +352,14 @@ class Context:
         work = self._workspace(wbytes)
         self.check(self._timed("shc", (nk, nw), lib().wbmi_spin_hall, self.h, nk, nw, X.shape[1], _ptr(E), _ptr(X),
                                prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)), "wbmi_spin_hall")
+
+    def tabulate_spin_berry(self, E, X, prec, params, out):
+        nk, nw = E.shape
+        wbytes = int(lib().wbmi_spin_hall_work_bytes(nk, nw))
+        work = self._workspace(wbytes)
+        self.check(self._timed("tab_spin_berry", (nk, nw), lib().wbmi_tabulate_spin_berry, self.h, nk, nw, X.shape[1],
+                               _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)),
+                   "wbmi_tabulate_spin_berry")
 
     def tabulate(self, E, X, prec, params, out):
         nk, nw = E.shape

@@ -137,15 +137,54 @@ class InvMass(Tabulator):
         return KBandResult(data, transformTR=1, transformInv=1)
 
 
-def _not_available(name):
-    class _NA(Calculator):
-        def __init__(self, *args, **kwargs):
-            raise NotImplementedError(f"the tabulator {name} is not available on the MI355X path")
-    _NA.__name__ = name
-    return _NA
+class SpinBerry(Tabulator):
+    r"""Spin Berry curvature per band (tabulate.py:218-221, formula/covariant.py:759-789):
+    group mean of :math:`-2 \mathrm{Im} \sum_{l \notin G} J^{as}_{ml} (D^b_{lm} - i \bar A^b_{lm}) / (E_m - E_l)`,
+    indices [a, b, s]; `kwargs_formula` may set spin_current_type ('ryoo', 'qiao', 'simple')."""
+    ndim = 3
 
+    def __init__(self, ibands=None, kwargs_formula=None, **kwargs):
+        kf = dict(kwargs_formula or {})
+        self.spin_current_type = kf.pop("spin_current_type", "ryoo")
+        if self.spin_current_type not in ("ryoo", "qiao", "simple"):
+            raise ValueError(f"spin_current_type must be `qiao` or `ryoo` or `simple`, not {self.spin_current_type}")
+        super().__init__(ibands=ibands, kwargs_formula=kf, **kwargs)
 
-SpinBerry = _not_available("SpinBerry")
+    def needed_channels(self, system):
+        ext = _external(self.kwargs_formula, system)
+        need = {("Ham", 0), ("Ham", 1), ("SS", 0)} | ({("AA", 0)} if ext else set())
+        if self.spin_current_type == "ryoo":
+            need |= {("SA", 0), ("SHA", 0)}
+        elif self.spin_current_type == "qiao":
+            need |= {("SH", 0), ("SR", 0), ("SHR", 0)}
+        return need
+
+    def __call__(self, data_K):
+        torch = data_K.torch
+        ext = _external(self.kwargs_formula, data_K.system)
+        if not ext and self.spin_current_type != "simple":
+            raise NotImplementedError(f"the {self.spin_current_type} spin current needs the external terms")
+        data_K.ensure(self.needed_channels(data_K.system))
+        p = _lib.ShcParams()
+        p.base.degen_thresh = float(self.degen_thresh)
+        p.base.degen_Kramers = int(bool(self.degen_Kramers))
+        p.base.external_terms = int(ext)
+        for i, v in enumerate(data_K.role_table()):
+            p.base.ch[i] = v
+        sl = data_K._slots
+        p.kind = {"ryoo": _lib.WBMI_SHC_RYOO, "qiao": _lib.WBMI_SHC_QIAO,
+                  "simple": _lib.WBMI_SHC_SIMPLE}[self.spin_current_type]
+        for c in range(3):
+            p.ch_S[c] = sl[("SS", 0)] + c
+            p.ch_SH[c] = sl[("SH", 0)] + c if ("SH", 0) in sl else -1
+        for c in range(9):
+            for name, arr in (("SA", p.ch_SA), ("SHA", p.ch_SHA), ("SR", p.ch_SR), ("SHR", p.ch_SHR)):
+                arr[c] = sl[(name, 0)] + c if (name, 0) in sl else -1
+        out = torch.empty((data_K.nk, data_K.num_wann, 27), dtype=torch.float64, device=data_K.ctx.device)
+        data_K.ctx.tabulate_spin_berry(data_K.E_dev, data_K.Xk, data_K.prec, p, out)
+        ib = np.arange(data_K.num_wann) if self.ibands is None else self.ibands
+        data = out.cpu().numpy()[:, ib].reshape((data_K.nk, len(ib), 3, 3, 3)) * self.constant_factor
+        return KBandResult(data, transformTR=1, transformInv=1)
 
 
 class TabulatorAll(Calculator):
@@ -189,4 +228,4 @@ class TabulatorAll(Calculator):
 
 
 __all__ = ["Tabulator", "TabulatorAll", "Energy", "Velocity", "Spin", "InvMass", "BerryCurvature", "DerBerryCurvature",
-           "OrbitalMoment", "DerOrbitalMoment"]
+           "OrbitalMoment", "DerOrbitalMoment", "SpinBerry"]

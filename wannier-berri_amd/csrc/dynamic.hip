@@ -712,3 +712,122 @@ extern "C" int wbmi_spin_hall(wbmi_ctx* ctx, int nk, int nw, int nch, const doub
         return launch_shc<float>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_spin_hall: bad precision");
 }
+
+// Spin Berry curvature per band (tabulate.py SpinBerry, formula/covariant.py:759-789):
+//   c_m^{abs} = -2 Im sum_{l outside m's group} J^{as}_ml / (E_m - E_l) (D^b_lm - i Abar^b_lm),
+// D^b_lm = -V^b_lm / (E_l - E_m) (both inverses 0 below 1e-7; Abar only with external terms),
+// out[k][n][a*9+b*3+s] = the mean of c over n's group.  One workgroup per k: W = D - i Abar in
+// LDS, one thread per (m, abs) for c, then the group means.  J from k_spin_current.
+template <typename T>
+__global__ void __launch_bounds__(OC_THREADS) k_tab_spin_berry(int nk, int nw, int nch, wbmi_optcond_params p,
+                                                               const double* __restrict__ Eall,
+                                                               const cplx<T>* __restrict__ Xall,
+                                                               const cplx<double>* __restrict__ J,
+                                                               double* __restrict__ out) {
+    extern __shared__ __attribute__((aligned(16))) char smem[];
+    const int k = blockIdx.x, tid = threadIdx.x, nw2 = nw * nw;
+    cplx<double>* sW = (cplx<double>*)smem;  // [3][nw][nw]
+    double* sC = (double*)(sW + 3 * nw2);    // [nw][27]
+    double* sE = sC + nw * 27;               // [OC_NWMAX]
+    double* sGm = sE + OC_NWMAX;             // [OC_NWMAX]
+    int* sGb = (int*)(sGm + OC_NWMAX);       // [OC_NWMAX + 1]
+    int* sG = sGb + OC_NWMAX + 1;            // band -> group [OC_NWMAX]
+    for (int i = tid; i < nw; i += OC_THREADS) sE[i] = Eall[(int64_t)k * nw + i];
+    __syncthreads();
+    if (tid == 0) {
+        const int ng = band_groups(sE, nw, p.degen_thresh, p.degen_Kramers, sGb, sGm);
+        for (int i = 0; i < nw; ++i) sG[i] = -1;  // bands past the last Kramers border: no group
+        for (int g = 0; g < ng; ++g)
+            for (int i = sGb[g]; i < sGb[g + 1]; ++i) sG[i] = g;
+    }
+    const cplx<T>* Xk = Xall + (int64_t)k * nch * nw2;
+    for (int t = tid; t < 3 * nw2; t += OC_THREADS) {
+        const int c = t / nw2, l = (t % nw2) / nw, m = t % nw;
+        const double de = sE[l] - sE[m];
+        const double inv = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+        const cplx<T> v = Xk[(int64_t)p.ch[WBMI_ROLE_V + c] * nw2 + l * nw + m];
+        cplx<double> w = mk(-inv * (double)v.re, -inv * (double)v.im);
+        if (p.external_terms) {
+            const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + l * nw + m];
+            w.re += (double)x.im;  // - i Abar
+            w.im -= (double)x.re;
+        }
+        sW[t] = w;
+    }
+    __syncthreads();
+    const cplx<double>* Jk = J + (int64_t)k * 9 * nw2;
+    for (int t = tid; t < nw * 27; t += OC_THREADS) {
+        const int m = t / 27, r = t % 27, a = r / 9, b = (r / 3) % 3, s = r % 3;
+        const cplx<double>* Jm = Jk + (int64_t)(a * 3 + s) * nw2 + m * nw;
+        const cplx<double>* Wb = sW + b * nw2;
+        const int gm = sG[m];
+        double sum = 0.0;
+        for (int l = 0; l < nw && gm >= 0; ++l) {
+            if (sG[l] == gm) continue;
+            const double de = sE[m] - sE[l];
+            const double inv = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+            const cplx<double> j = Jm[l], w = Wb[l * nw + m];
+            sum += inv * (j.re * w.im + j.im * w.re);
+        }
+        sC[t] = -2.0 * sum;
+    }
+    __syncthreads();
+    for (int t = tid; t < nw * 27; t += OC_THREADS) {
+        const int n = t / 27, r = t % 27, g = sG[n];
+        double s = 0.0;
+        if (g >= 0) {
+            for (int m = sGb[g]; m < sGb[g + 1]; ++m) s += sC[m * 27 + r];
+            s /= sGb[g + 1] - sGb[g];
+        }
+        out[((int64_t)k * nw + n) * 27 + r] = s;
+    }
+}
+
+template <typename T>
+static int launch_spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E, const void* X,
+                             const wbmi_shc_params* params, cplx<double>* J, double* out) {
+    hipLaunchKernelGGL(k_spin_current<T>, dim3(nk, 9), dim3(256), 0, ctx->stream, nk, nw, nch, *params, E,
+                       (const cplx<T>*)X, J);
+    WBMI_LAUNCH_CHECK(ctx);
+    const size_t lds = sizeof(cplx<double>) * 3 * nw * nw + sizeof(double) * (27 * nw + 2 * OC_NWMAX) +
+                       sizeof(int) * (2 * OC_NWMAX + 1);
+    hipLaunchKernelGGL(k_tab_spin_berry<T>, dim3(nk), dim3(OC_THREADS), lds, ctx->stream, nk, nw, nch, params->base,
+                       E, (const cplx<T>*)X, (const cplx<double>*)J, out);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+extern "C" int wbmi_tabulate_spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                                        const void* Xbar_dev, int prec, const wbmi_shc_params* params,
+                                        void* work_dev, size_t work_bytes, double* out_dev) {
+    if (!ctx) return WBMI_ERR_ARG;
+    if (nk <= 0 || nw <= 0 || nch <= 0 || !E_dev || !Xbar_dev || !params || !out_dev)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: bad arguments");
+    if (nw > OC_NWMAX) return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_tabulate_spin_berry: nw <= %d", OC_NWMAX);
+    if (!work_dev || work_bytes < wbmi_spin_hall_work_bytes(nk, nw))
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: workspace of %zu bytes needed",
+                         wbmi_spin_hall_work_bytes(nk, nw));
+    const int kind = params->kind;
+    if (kind != WBMI_SHC_RYOO && kind != WBMI_SHC_QIAO && kind != WBMI_SHC_SIMPLE)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: bad spin current kind %d", kind);
+    if (kind != WBMI_SHC_SIMPLE && !params->base.external_terms)
+        return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED,
+                         "wbmi_tabulate_spin_berry: the Ryoo and Qiao spin currents need external terms");
+    auto bad = [&](int c) { return c < 0 || c >= nch; };
+    for (int c = 0; c < 3; ++c) {
+        if (bad(params->base.ch[WBMI_ROLE_V + c]) || bad(params->ch_S[c]) ||
+            (params->base.external_terms && bad(params->base.ch[WBMI_ROLE_A + c])))
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: missing V, A or SS channel");
+    }
+    for (int c = 0; c < 9; ++c) {
+        if (kind == WBMI_SHC_RYOO && (bad(params->ch_SA[c]) || bad(params->ch_SHA[c])))
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: missing SA or SHA channel");
+        if (kind == WBMI_SHC_QIAO && (bad(params->ch_SR[c]) || bad(params->ch_SHR[c]) || (c < 3 && bad(params->ch_SH[c]))))
+            return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: missing SH, SR or SHR channel");
+    }
+    if (prec == WBMI_F64)
+        return launch_spin_berry<double>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev);
+    if (prec == WBMI_F32)
+        return launch_spin_berry<float>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev);
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: bad precision");
+}
