@@ -154,6 +154,9 @@ def main():
             for k in f.files:
                 forms[name + "/" + k] = f[k]
     np.savez_compressed(os.path.join(HERE, "results_formula.npz"), **forms)
+    # k-path refined by a factor 4 (tests/test_path.py:76-95)
+    with np.load(os.path.join(REF, "reference/path_refined.npz"), allow_pickle=False) as f:
+        np.savez_compressed(os.path.join(HERE, "results_path.npz"), **{"refined/" + k: f[k] for k in f.files})
     with np.load(os.path.join(REF, "reference/evaluate_k.npz"), allow_pickle=False) as f:
         np.savez_compressed(os.path.join(HERE, "results_evaluate_k.npz"),
                             **{k: f[k] for k in ["kpoint", "energy", "berry_curvature", "ahc", "Morb"]})

@@ -808,6 +808,40 @@ def test_fe_tabulate(wb, precision, tol):
         assert e < tol * (5e-8 if k == "Der_berry" else 2e-8), k
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("fp32", 2e-5)])
+def test_haldane_tabulate_path(wb, precision, tol):
+    """Tabulation along a path (TabulatorAll mode='path' on Path.from_nodes, dk = 1; the
+    configuration of tests/test_path.py:200-245: Energy, V, berry, Der_berry, morb, Der_morb of
+    band 0 on Haldane), in K-blocks of 4 path points (k-list R -> k), put back in path order,
+    against the oracle's Tabulator on the same k-list (the reference's own values for this
+    case are a pickle, which is not read here)."""
+    say(f"test_haldane_tabulate_path {precision}")
+    tab = wb.calculators.tabulate
+    s = gsys(wb, "haldane")
+    pt = {"degen_thresh": 5e-2}
+    calc = tab.TabulatorAll({"Energy": tab.Energy(), "V": tab.Velocity(**pt), "Der_berry": tab.DerBerryCurvature(**pt),
+                             "berry": tab.BerryCurvature(**pt), "morb": tab.OrbitalMoment(**pt),
+                             "Der_morb": tab.DerOrbitalMoment(**pt)}, ibands=[0], mode="path")
+    path = wb.Path.from_nodes(s, nodes=[[0.0, 0.0, 0.5], [0.0, 0.0, 0.0], [0.5, 0.5, 0.5]], dk=1.0)
+    r = wb.run(s, path, {"tabulate": calc}, use_irred_kpt=True, k_batch=4, parameters_K={"precision": precision},
+               save_files=False, verbose=False)
+    t = r.results["tabulate"]
+    assert t.mode == "path" and np.allclose(t.kpoints, path.K_list)
+    d = O.OracleDataK(osys("haldane"), k_list=path.K_list)
+    quants = {"Energy": (O.Eavln, {}), "V": (lambda d: d.covariant("Ham", gender=1), pt), "berry": (O.Omega, pt),
+              "Der_berry": (O.DerOmega, pt),
+              "morb": (O.morb, dict(pt, constant_factor=O.FACTOR_MORB_EVA2_TO_MUB)),
+              "Der_morb": (O.Dermorb, dict(pt, constant_factor=O.FACTOR_MORB_EVA2_TO_MUB))}
+    for k, (F, kw) in quants.items():
+        ref = O.tabulate(d, F, [0], **kw)
+        got = t.get_data(k)
+        e = rel_err(got, ref)
+        say(f"  {k}: {e:.2e}")
+        assert got.shape == ref.shape, k
+        assert e < tol, k
+    assert t.get_data("berry", iband=0, component="z").shape == (len(path.K_list),)
+
+
 def test_haldane_tabulate_symmetric(wb):
     """Tabulation with the C3z group (irreducible K-blocks, every result mapped by the group,
     images put on the grid, tabresult.py:128-170) equals the tabulation of the full grid for
@@ -862,30 +896,32 @@ def test_chiral_k_resolved(wb, use_sym, tetra):
 
 
 def test_fe_tabulate_invmass(wb):
-    """InvMass tabulator (generalised derivative of the velocity, elementary.py:28-33) on Fe
-    (Wannier90), grid 4^3 / 2^3, vs the oracle's Tabulator with InvMass (pinned by the
-    reference's formula fixture InvMass.npz)."""
+    """InvMass tabulator (generalised derivative of the velocity, elementary.py:28-33) and DerSpin
+    tabulator (generalised derivative of the spin, covariant.py:338-342) on Fe (Wannier90),
+    grid 4^3 / 2^3, vs the oracle's Tabulator with InvMass (pinned by the reference's formula
+    fixture InvMass.npz) and DerSpin (pinned through the reference's GME_spin results)."""
     say("test_fe_tabulate_invmass")
     tab = wb.calculators.tabulate
     s = gsys(wb, "fe_w90")
     ib = [4, 5, 6, 7]
-    calc = tab.TabulatorAll({"im": tab.InvMass(degen_thresh=5e-2)}, ibands=ib)
+    calc = tab.TabulatorAll({"im": tab.InvMass(degen_thresh=5e-2), "dspin": tab.DerSpin(degen_thresh=5e-2)}, ibands=ib)
     grid = wb.Grid(system=s, NK=[4, 4, 4], NKFFT=[2, 2, 2])
     r = wb.run(s, grid, {"tab": calc}, use_irred_kpt=False, symmetrize=False, save_files=False, verbose=False)
-    got = r.results["tab"].get_data("im")
     o = osys("fe_w90")
     NKdiv, NKFFT, blocks = O.grid_blocks([4, 4, 4], [2, 2, 2])
     NK = NKdiv * NKFFT
-    ref = np.zeros((int(np.prod(NK)), len(ib), 3, 3))
-    for dK, _ in blocks:
-        d = O.OracleDataK(o, dK=dK, NKFFT=NKFFT)
-        ki = np.rint(d.kpoints_all * NK[None, :]).astype(int) % NK[None, :]
-        idx = (ki[:, 0] * NK[1] + ki[:, 1]) * NK[2] + ki[:, 2]
-        ref[idx] = O.tabulate(d, O.InvMass, ib, degen_thresh=5e-2)
-    ref = ref.reshape(tuple(NK) + (len(ib), 3, 3))
-    e = rel_err(got, ref)
-    say(f"  InvMass: {e:.2e}")
-    assert e < 1e-10
+    for key, F in (("im", O.InvMass), ("dspin", O.DerSpin)):
+        got = r.results["tab"].get_data(key)
+        ref = np.zeros((int(np.prod(NK)), len(ib), 3, 3))
+        for dK, _ in blocks:
+            d = O.OracleDataK(o, dK=dK, NKFFT=NKFFT)
+            ki = np.rint(d.kpoints_all * NK[None, :]).astype(int) % NK[None, :]
+            idx = (ki[:, 0] * NK[1] + ki[:, 1]) * NK[2] + ki[:, 2]
+            ref[idx] = O.tabulate(d, F, ib, degen_thresh=5e-2)
+        ref = ref.reshape(tuple(NK) + (len(ib), 3, 3))
+        e = rel_err(got, ref)
+        say(f"  {key}: {e:.2e}")
+        assert e < 1e-10, key
 
 
 @pytest.mark.parametrize("kind,ext", [("qiao", True), ("simple", True), ("simple", False)])

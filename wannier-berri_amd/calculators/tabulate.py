@@ -85,6 +85,12 @@ class DerOrbitalMoment(Tabulator):
     factor = factors.factor_morb_evA2_to_muB
 
 
+class DerSpin(Tabulator):
+    r"Derivative of the spin :math:`\partial_b \langle u | \sigma_s | u \rangle` (tabulate.py:176-180), indices [s, b]"
+    formula = _lib.FORMULA_DERSPIN
+    ndim = 2
+
+
 class _DiagTabulator(Tabulator):
     """Tabulators whose group traces are sums of diagonal elements of one channel triple
     (formula/covariant.py:321-336): one wbmi_tabulate_diag launch."""
@@ -188,13 +194,14 @@ class SpinBerry(Tabulator):
 
 
 class TabulatorAll(Calculator):
-    """A pack of tabulators (tabulate.py:55-105); "Energy" is always included."""
+    """A pack of tabulators (tabulate.py:55-105); "Energy" is always included.  mode 'grid'
+    (run on a Grid, tables put on the full grid) or 'path' (run on a Path, tables in path order)."""
 
     def __init__(self, tabulators, ibands=None, mode="grid", save_mode="bin", print_comment=False):
         super().__init__(save_mode=save_mode, print_comment=print_comment)
         mode = mode.lower()
-        if mode != "grid":
-            raise NotImplementedError("tabulation along a path is not available on the MI355X path")
+        if mode not in ("grid", "path"):
+            raise ValueError(f"mode must be 'grid' or 'path', not '{mode}'")
         self.mode = mode
         self.tabulators = dict(tabulators)
         if "Energy" not in self.tabulators:
@@ -217,8 +224,12 @@ class TabulatorAll(Calculator):
         return need
 
     @property
+    def allow_path(self):
+        return self.mode == "path"
+
+    @property
     def allow_grid(self):
-        return True
+        return self.mode == "grid"
 
     def __call__(self, data_K):
         data_K.ensure(self.needed_channels(data_K.system))
@@ -228,4 +239,4 @@ class TabulatorAll(Calculator):
 
 
 __all__ = ["Tabulator", "TabulatorAll", "Energy", "Velocity", "Spin", "InvMass", "BerryCurvature", "DerBerryCurvature",
-           "OrbitalMoment", "DerOrbitalMoment", "SpinBerry"]
+           "OrbitalMoment", "DerOrbitalMoment", "SpinBerry", "DerSpin"]

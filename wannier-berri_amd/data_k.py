@@ -23,6 +23,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import WBMI_F32, WBMI_F32_SPLIT, WBMI_F64
+from .path import KpointBZpath
 
 # components of each Xbar(name, der)
 CHANNEL_NCOMP = {("Ham", 0): 1, ("Ham", 1): 3, ("Ham", 2): 9,
@@ -172,6 +173,8 @@ class Data_K:
             raise NotImplementedError("band selection by Emin/Emax is not supported")
         if random_gauge:
             raise NotImplementedError("random_gauge is a test-only option of the reference and is not supported")
+        if k_list is None and isinstance(Kpoint, KpointBZpath):
+            k_list, dK = Kpoint.K, None   # a batch of path points is a k-list (data_K.py:84-87)
         if k_list is None and dK is None:
             raise AssertionError("either dK or k_list must be specified, but not both")
         self.system = system

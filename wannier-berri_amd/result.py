@@ -295,6 +295,10 @@ class KBandResult(Result):
             mean = total / count[:, None]
         return KBandResult(mean.reshape((npoints,) + self.data.shape[1:]), self.transformTR, self.transformInv)
 
+    def to_path(self, k_map):
+        """Rows reordered (and repeated) by `k_map` (kbandresult.py:133-141)."""
+        return KBandResult(self.data[np.asarray(k_map, dtype=int)], self.transformTR, self.transformInv)
+
     def get_component(self, component=None):
         """'x', 'xy', 'trace', ... of the Cartesian indices (kbandresult.py)."""
         if component is None or component == "":
@@ -375,6 +379,20 @@ class TABresult(Result):
 
     def self_to_grid(self, grid):
         self.__dict__.update(self.to_grid(grid).__dict__)
+
+    def self_to_path(self, path):
+        """Rows put in the order of the path's k-points (tabresult.py:171-185): each path point
+        takes the computed k-point nearest to it modulo reciprocal lattice vectors, which must
+        agree within 1e-5."""
+        kp = path.get_kpoints()
+        diff = self.kpoints[:, None, :] - kp[None, :, :]
+        diff -= np.round(diff)
+        k_map = np.argmin(np.linalg.norm(diff, axis=2), axis=0)
+        d = self.kpoints[k_map] - kp
+        d -= np.round(d)
+        assert np.allclose(d, 0, atol=1e-5), f"kpoints from path and result differ by {np.max(abs(d))}"
+        self.results = {r: v.to_path(k_map) for r, v in self.results.items()}
+        self.kpoints = kp.copy()
 
     def get_data(self, quantity, iband=None, component=None):
         """Data of one quantity on the grid [N0, N1, N2, (bands), ...] (tabresult.py:186-230)."""

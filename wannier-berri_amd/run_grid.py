@@ -14,6 +14,7 @@ import numpy as np
 
 from .data_k import Data_K
 from .grid import exclude_equiv_points
+from .path import Path
 from .result import ResultDict, TABresult
 
 
@@ -28,20 +29,33 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
     point group (and refinement merges equivalent sub-blocks), which implies
     `symmetrize`: every K-block's result is averaged over the group
     (run_grid.py:225-236,258-265,382-386).  `parallel` is accepted for
-    compatibility: one process drives one GPU.  With `prefetch` (and a data_k_class that has
+    compatibility: one process drives one GPU.  On a Path (tabulation only) the K-blocks are
+    batches of `k_batch` path points, each a Data_K on that k-list, and the tables come back in
+    path order.  With `prefetch` (and a data_k_class that has
     `prefetch`) the next K-block's R -> k runs on a side stream while the current one is
     evaluated.
     """
-    if restart or dump_results:
-        raise NotImplementedError("restart / dump_results are not available on the MI355X path")
     if data_k_class is None:
         data_k_class = Data_K
-    if use_irred_kpt:
-        symmetrize = True
     parameters_K = dict(parameters_K or {})
-    for key, calc in calculators.items():
-        if not calc.allow_grid:
-            raise ValueError(f"Calculation on Grid is running, but calculator `{key}` is not compatible with a Grid")
+    on_path = isinstance(grid, Path)
+    if on_path:
+        # only tabulation along a path; no symmetry, refinement or restart (run_grid.py:217-229)
+        for key, calc in calculators.items():
+            if not calc.allow_path:
+                raise ValueError(f"Calculation along a Path is running, but calculator `{key}` is not compatible "
+                                 "with a Path")
+        symmetrize = False
+        adpt_num_iter = 0
+        restart = dump_results = False
+    else:
+        if use_irred_kpt:
+            symmetrize = True
+        for key, calc in calculators.items():
+            if not calc.allow_grid:
+                raise ValueError(f"Calculation on Grid is running, but calculator `{key}` is not compatible with a Grid")
+    if restart or dump_results:
+        raise NotImplementedError("restart / dump_results are not available on the MI355X path")
     if getattr(data_k_class, "accepts_channels", False):
         need = set()
         for calc in calculators.values():
@@ -49,7 +63,7 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
                 need |= set(calc.needed_channels(system))
         parameters_K.setdefault("channels", need)
 
-    K_list = grid.get_K_list(use_symmetry=use_irred_kpt)
+    K_list = grid.get_K_list(use_symmetry=use_irred_kpt and not on_path, k_batch=k_batch if on_path else None)
     if adpt_num_iter < 0:
         adpt_num_iter = -adpt_num_iter * np.prod(grid.div) / np.prod(adpt_mesh) / adpt_fac / 3
     adpt_num_iter = int(round(adpt_num_iter))
@@ -115,10 +129,13 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
             exclude_equiv_points(K_list, new_points=len(K_list) - l1)
     if verbose:
         print(f"run(): processed {counter} K-blocks in {time.time() - t0:.3f} s", flush=True)
-    # k-resolved results onto the full grid (run_grid.py:394-401)
+    # k-resolved results onto the full grid, or in path order (run_grid.py:394-401)
     for val in result_all.results.values():
         if isinstance(val, TABresult):
-            val.self_to_grid(grid.dense)
+            if on_path:
+                val.self_to_path(grid)
+            else:
+                val.self_to_grid(grid.dense)
     return result_all
 
 
