@@ -53,6 +53,8 @@ extern "C" {
 #define WBMI_FORMULA_DERSPIN 7    /* formula/covariant.py:338-342 (GME spin, sea) */
 #define WBMI_FORMULA_VELSPIN 8    /* formula/covariant.py:811-814 (GME spin, surface) */
 #define WBMI_FORMULA_SPIN 9       /* formula/covariant.py:331-335 (Spin: Re S^s_nn)   */
+#define WBMI_FORMULA_INVMASS 10   /* formula/elementary.py:28-33 (Ohmic, Fermi sea)  */
+#define WBMI_FORMULA_VELVEL 11    /* formula/covariant.py:817-820 (Ohmic, Fermi surface) */
 
 /* channel roles: index into wbmi_scan_params.ch (value = channel slot in the
  * per-k stack, -1 = absent).  Two-index roles are row-major over (x, y). */

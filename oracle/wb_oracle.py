@@ -43,6 +43,7 @@ FACTOR_MORB_EVA2_TO_MUB = -(_e * _ang ** 2) * _e / (2 * _hbar) / BOHR_MAGNETON
 FACTOR_GME_ORB = FACTOR_MORB_EVA2_TO_MUB * BOHR_MAGNETON / _ang ** 2
 FACTOR_AHC = -(_e ** 2 / _hbar / _ang)
 FACTOR_NLAHC = _e ** 3 / _hbar ** 2 * 1e-15
+FACTOR_OHMIC = _e ** 2 / _hbar / _ang * 1e-15 * _e / _hbar   # factors.py:25-26
 FACTOR_GME_SPIN = -BOHR_MAGNETON / _ang ** 2
 
 # dEig_inv threshold (wannierberri/data_K/data_K.py:292)
@@ -911,6 +912,11 @@ def DerSpin(data_K, **kw):
     return data_K.covariant('SS', gender=1)
 
 
+def VelVel(data_K, **kw):
+    # formula/covariant.py:817-820
+    return FormulaProduct([data_K.covariant('Ham', commader=1), data_K.covariant('Ham', commader=1)])
+
+
 def VelSpin(data_K, **kw):
     # formula/covariant.py:811-814
     return FormulaProduct([data_K.covariant('Ham', commader=1), Spin(data_K)])
@@ -1079,6 +1085,20 @@ class BerryDipole_FermiSea(StaticCalculator):
 class NLAHC_FermiSea(BerryDipole_FermiSea):
     # static.py:490-498
     constant_factor_default = FACTOR_NLAHC
+
+
+class Ohmic_FermiSea(StaticCalculator):
+    # static.py:379-390
+    Formula = InvMass
+    fder = 0
+    constant_factor_default = FACTOR_OHMIC
+
+
+class Ohmic_FermiSurf(StaticCalculator):
+    # static.py:393-404
+    Formula = staticmethod(VelVel)
+    fder = 1
+    constant_factor_default = FACTOR_OHMIC
 
 
 class BerryDipole_FermiSurf(StaticCalculator):

@@ -129,6 +129,17 @@ def main():
              "haldane_adpt3_tetra_ahc_iter1": I + "Haldane-ahc-PythTB-adpt3-tetra-sym_iter-0001.npz",
              "haldane_adpt3_tetra_dos_iter0": I + "Haldane-dos-PythTB-adpt3-tetra-sym_iter-0000.npz",
              "haldane_adpt3_tetra_dos_iter1": I + "Haldane-dos-PythTB-adpt3-tetra-sym_iter-0001.npz",
+             # Ohmic conductivity, Fermi sea (InvMass) and surface (VelVel): Chiral (10x10x4 / 5x5x2,
+             # also with the tetrahedron method and C3z), random (6^3 / 3^3), Fe (4^3 / 2^3)
+             # (tests/test_run.py:119-128,186-238,271-303,653-669)
+             "chiral_ohmic": I + "Chiral-conductivity_ohmic_iter-0000.npz",
+             "chiral_ohmic_fsurf": I + "Chiral-conductivity_ohmic_fsurf_iter-0000.npz",
+             "chiral_tetra_ohmic": I + "Chiral_tetra-conductivity_ohmic_iter-0000.npz",
+             "chiral_tetra_ohmic_fsurf": I + "Chiral_tetra-conductivity_ohmic_fsurf_iter-0000.npz",
+             "random_ohmic": I + "random-conductivity_ohmic_iter-0000.npz",
+             "random_ohmic_fsurf": I + "random-conductivity_ohmic_fsurf_iter-0000.npz",
+             "fe_ohmic": I + "Fe_W90-conductivity_ohmic_iter-0000.npz",
+             "fe_ohmic_fsurf": I + "Fe_W90-conductivity_ohmic_fsurf_iter-0000.npz",
              }, "results_run.npz")
     # FermiDiracSmoother(Efermi, T_Kelvin=1200, maxdE=8) on the Chiral AHC and Fermi-sea BCD
     # (tests/test_run.py:181-193): the .dat files hold the data and the smoothed data

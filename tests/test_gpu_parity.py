@@ -289,6 +289,26 @@ def test_random_ahc_morb(wb, results_run):
     assert rel_err(r.results["Morb"].data, results_run["random_Morb"]) < 1e-8
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_ohmic(wb, results_run, precision, tol):
+    """Ohmic conductivity, Fermi sea (InvMass) and Fermi surface (VelVel), against the reference's
+    Chiral (10x10x4 / 5x5x2), random (6^3 / 3^3) and Fe (Wannier90, 4^3 / 2^3) fixtures
+    (tests/test_run.py:119-128,186-190,271-303,653-669)."""
+    st = wb.calculators.static
+    for name, Ef, NK, NKFFT in (("chiral_left", np.linspace(-5, 8, 27), [10, 10, 4], [5, 5, 2]),
+                                ("random", np.linspace(-2, 2, 5), [6, 6, 6], [3, 3, 3]),
+                                ("fe_w90", np.linspace(17, 18, 11), [4, 4, 4], [2, 2, 2])):
+        say(f"test_ohmic {name} {precision}")
+        s = gsys(wb, name)
+        r = run_gpu(wb, s, {"sea": st.Ohmic_FermiSea(Efermi=Ef), "surf": st.Ohmic_FermiSurf(Efermi=Ef)}, NK, NKFFT,
+                    precision=precision)
+        key = name.split("_")[0]
+        for k, ref in (("sea", key + "_ohmic"), ("surf", key + "_ohmic_fsurf")):
+            e = rel_err(r.results[k].data, results_run[ref])
+            say(f"  {k}: {e:.2e}")
+            assert e < tol, (name, k)
+
+
 def test_refinement_random(wb):
     """Adaptive refinement (two iterations, 2x2x2 sub-blocks) against the oracle's
     restatement; the selected blocks' metrics are separated by >= 2 %, so both

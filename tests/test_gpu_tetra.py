@@ -57,7 +57,9 @@ def test_chiral_tetra(wb, results_run, precision, tol):
              "berry_dipole": st.BerryDipole_FermiSea(Efermi=Ef, tetra=True, use_factor=False),
              "berry_dipole_fsurf": st.BerryDipole_FermiSurf(Efermi=Ef, tetra=True, use_factor=False),
              "dos": st.DOS(Efermi=Ef, tetra=True),
-             "cumdos": st.CumDOS(Efermi=Ef, tetra=True)}
+             "cumdos": st.CumDOS(Efermi=Ef, tetra=True),
+             "ohmic": st.Ohmic_FermiSea(Efermi=Ef, tetra=True),
+             "ohmic_fsurf": st.Ohmic_FermiSurf(Efermi=Ef, tetra=True)}
     grid = wb.Grid(system=s, NK=[10, 10, 4], NKFFT=[5, 5, 2])
     r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, parameters_K={"precision": precision},
                save_files=False, verbose=False)

@@ -119,6 +119,19 @@ def test_random_ahc_morb(results_run):
     assert rel_err(r["Morb"].data, results_run["random_Morb"]) < 1e-8
 
 
+def test_ohmic_chiral_random_fe(results_run):
+    """Ohmic conductivity, Fermi sea (InvMass, fder 0) and surface (VelVel, fder 1), against the
+    reference's Chiral (10x10x4 / 5x5x2), random (6^3 / 3^3) and Fe (4^3 / 2^3) fixtures
+    (tests/test_run.py:119-128,186-190,271-303,653-669)."""
+    for name, Ef, NK, NKFFT in (("chiral_left", np.linspace(-5, 8, 27), [10, 10, 4], [5, 5, 2]),
+                                ("random", np.linspace(-2, 2, 5), [6, 6, 6], [3, 3, 3]),
+                                ("fe_w90", np.linspace(17, 18, 11), [4, 4, 4], [2, 2, 2])):
+        r = O.run(gsys(name), {"sea": O.Ohmic_FermiSea(Ef), "surf": O.Ohmic_FermiSurf(Ef)}, NK=NK, NKFFT=NKFFT)
+        key = name.split("_")[0]
+        assert rel_err(r["sea"].data, results_run[key + "_ohmic"]) < 1e-8, name
+        assert rel_err(r["surf"].data, results_run[key + "_ohmic_fsurf"]) < 1e-8, name
+
+
 @pytest.mark.parametrize("fder", [0, 1, 2, 3])
 def test_fe_scan_stencils(results_run, fder):
     s = gsys("fe_w90")

@@ -30,6 +30,8 @@ FORMULA_VELHPLUS = 6
 FORMULA_DERSPIN = 7
 FORMULA_VELSPIN = 8
 FORMULA_SPIN = 9
+FORMULA_INVMASS = 10
+FORMULA_VELVEL = 11
 WBMI_SHC_RYOO, WBMI_SHC_QIAO, WBMI_SHC_SIMPLE = 0, 1, 2
 
 ROLE_V, ROLE_W, ROLE_A, ROLE_AD, ROLE_O, ROLE_OD = 0, 3, 12, 15, 24, 27

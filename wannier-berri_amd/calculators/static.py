@@ -42,7 +42,7 @@ class Calculator:
 _PARITY = {_lib.FORMULA_IDENTITY: (1, 1), _lib.FORMULA_OMEGA: (-1, 1), _lib.FORMULA_DEROMEGA: (1, -1),
            _lib.FORMULA_VELOMEGA: (1, -1), _lib.FORMULA_MORB_HPM: (-1, 1), _lib.FORMULA_DERMORB: (1, -1),
            _lib.FORMULA_VELHPLUS: (1, -1), _lib.FORMULA_DERSPIN: (1, -1), _lib.FORMULA_VELSPIN: (1, -1),
-           _lib.FORMULA_SPIN: (-1, 1)}
+           _lib.FORMULA_SPIN: (-1, 1), _lib.FORMULA_INVMASS: (1, 1), _lib.FORMULA_VELVEL: (1, 1)}
 # formulas that read only the Hamiltonian's derivatives and the spin, never A, B, C
 _SPIN = (_lib.FORMULA_DERSPIN, _lib.FORMULA_VELSPIN, _lib.FORMULA_SPIN)
 
@@ -125,6 +125,10 @@ class StaticCalculator(Calculator):
             return {("Ham", 0), ("SS", 0)}
         if f in _SPIN:
             return {("Ham", 0), ("Ham", 1), ("SS", 0)} | ({("SS", 1)} if f == _lib.FORMULA_DERSPIN else set())
+        if f == _lib.FORMULA_VELVEL:
+            return {("Ham", 0), ("Ham", 1)}
+        if f == _lib.FORMULA_INVMASS:
+            return {("Ham", 0), ("Ham", 1), ("Ham", 2)}
         need = {("Ham", 0), ("Ham", 1)}
         if f in (_lib.FORMULA_DEROMEGA, _lib.FORMULA_DERMORB):
             require_hermitian_ham(system, type(self).__name__)
@@ -261,6 +265,32 @@ class AHC(StaticCalculator):
     fder = 0
 
     def __init__(self, constant_factor=factors.factor_ahc, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class Ohmic_FermiSea(StaticCalculator):
+    r"""Ohmic conductivity (:math:`S/m`), Fermi sea (static.py:379-390)
+
+        | Output: :math:`\sigma_{\alpha\beta} = e^2/\hbar \tau \int [dk] \partial_\beta v_\alpha f` for tau = 1 fs
+        | Instruction: :math:`j_\alpha = \sigma_{\alpha\beta} E_\beta`"""
+    formula = _lib.FORMULA_INVMASS
+    ndim = 2
+    fder = 0
+
+    def __init__(self, constant_factor=factors.factor_ohmic, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class Ohmic_FermiSurf(StaticCalculator):
+    r"""Ohmic conductivity (:math:`S/m`), Fermi surface (static.py:393-404)
+
+        | Output: :math:`\sigma_{\alpha\beta} = -e^2/\hbar \tau \int [dk] v_\alpha v_\beta f'` for tau = 1 fs
+        | Instruction: :math:`j_\alpha = \sigma_{\alpha\beta} E_\beta`"""
+    formula = _lib.FORMULA_VELVEL
+    ndim = 2
+    fder = 1
+
+    def __init__(self, constant_factor=factors.factor_ohmic, **kwargs):
         super().__init__(constant_factor=constant_factor, **kwargs)
 
 
@@ -420,4 +450,4 @@ class GME_spin_FermiSurf(StaticCalculator):
 
 __all__ = ["Calculator", "StaticCalculator", "DOS", "CumDOS", "AHC", "BerryDipole_FermiSurf", "NLAHC_FermiSurf",
            "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "GME_orb_FermiSurf", "GME_orb_FermiSea",
-           "GME_spin_FermiSea", "GME_spin_FermiSurf", "Spin", "VoidResult"]
+           "GME_spin_FermiSea", "GME_spin_FermiSurf", "Spin", "Ohmic_FermiSea", "Ohmic_FermiSurf", "VoidResult"]

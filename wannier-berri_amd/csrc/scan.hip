@@ -43,6 +43,12 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
         case WBMI_FORMULA_SPIN:
             if ((rc = need(WBMI_ROLE_S, 3))) return rc;
             break;
+        case WBMI_FORMULA_INVMASS:
+            if ((rc = need(WBMI_ROLE_V, 3)) || (rc = need(WBMI_ROLE_W, 9))) return rc;
+            break;
+        case WBMI_FORMULA_VELVEL:
+            if ((rc = need(WBMI_ROLE_V, 3))) return rc;
+            break;
         case WBMI_FORMULA_MORB_HPM:
         case WBMI_FORMULA_VELHPLUS:
             if ((rc = need(WBMI_ROLE_V, 3))) return rc;

@@ -868,6 +868,29 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
             for (int m = S.gs[n]; m < S.ge[n]; ++m) v += K.V(a, n, m) * K.S(b, m, n);
             S.acc[t] = mk(v.re, 0.0);
         }
+    } else if (F == WBMI_FORMULA_INVMASS) {
+        // inverse effective mass, the generalised derivative of the velocity (formula/elementary.py:28-33,
+        // formula/formula.py:95-107 with A = V, dA = W): Re [W^{bd}_nn - sum_{l off} D^d_nl V^b_ln
+        // + sum_{l off} V^b_nl D^d_ln]
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
+            const int n = t / 9, b = (t % 9) / 3, d = t % 3;
+            if (!S.act[n]) continue;
+            cplx<double> v = K.W(b, d, n, n);
+            for (int l = 0; l < nw; ++l) {
+                v -= K.Dp(d, n, l) * K.V(b, l, n);
+                v += K.V(b, n, l) * K.Dp(d, l, n);
+            }
+            S.acc[t] = mk(v.re, 0.0);
+        }
+    } else if (F == WBMI_FORMULA_VELVEL) {
+        // Re sum_{n' in G} V^a_nn' V^b_n'n (FormulaProduct of two velocities, formula/covariant.py:817-820)
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
+            const int n = t / 9, a = (t % 9) / 3, b = t % 3;
+            if (!S.act[n]) continue;
+            cplx<double> v = mk(0.0, 0.0);
+            for (int m = S.gs[n]; m < S.ge[n]; ++m) v += K.V(a, n, m) * K.V(b, m, n);
+            S.acc[t] = mk(v.re, 0.0);
+        }
     } else if (F == WBMI_FORMULA_DEROMEGA) {
         eval_deromega(S, K, in, ex, false, 0);
     } else if (F == WBMI_FORMULA_DERMORB) {
@@ -1135,7 +1158,8 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
     constexpr bool additive = F == WBMI_FORMULA_IDENTITY || F == WBMI_FORMULA_OMEGA ||
                               F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
                               F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERSPIN ||
-                              F == WBMI_FORMULA_VELSPIN || F == WBMI_FORMULA_SPIN;
+                              F == WBMI_FORMULA_VELSPIN || F == WBMI_FORMULA_SPIN || F == WBMI_FORMULA_INVMASS ||
+                              F == WBMI_FORMULA_VELVEL;
 
     // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186), formed by wave 0: each lane
     // owns bands lane and lane + 64 (nw <= 128); borders from two ballots and popcounts, one
@@ -1400,6 +1424,8 @@ static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
         case WBMI_FORMULA_DERSPIN: return launch_scan<T, WBMI_FORMULA_DERSPIN, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_VELSPIN: return launch_scan<T, WBMI_FORMULA_VELSPIN, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_SPIN: return launch_scan<T, WBMI_FORMULA_SPIN, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_INVMASS: return launch_scan<T, WBMI_FORMULA_INVMASS, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_VELVEL: return launch_scan<T, WBMI_FORMULA_VELVEL, MODE>(ctx, nk, a, E, X, hist);
         default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
     }
 }

@@ -19,3 +19,5 @@ factor_shc = -factor_ahc   # factors.py:37
 factor_shift_current = hbar / elementary_charge * np.pi * elementary_charge ** 3 / (4 * hbar ** 2)  # factors.py:50
 factor_injection_current = -np.pi * elementary_charge ** 3 / (hbar ** 2) * TAU_UNIT               # factors.py:51
 factor_nlahc = elementary_charge ** 3 / hbar ** 2 * TAU_UNIT
+# tau^1 E^1, velocities converted from eV*m to m/s (factors.py:25-26)
+factor_ohmic = elementary_charge ** 2 / hbar / angstrom * TAU_UNIT * elementary_charge / hbar
