@@ -55,6 +55,14 @@ extern "C" {
 #define WBMI_FORMULA_SPIN 9       /* formula/covariant.py:331-335 (Spin: Re S^s_nn)   */
 #define WBMI_FORMULA_INVMASS 10   /* formula/elementary.py:28-33 (Ohmic, Fermi sea)  */
 #define WBMI_FORMULA_VELVEL 11    /* formula/covariant.py:817-820 (Ohmic, Fermi surface) */
+/* rank-3 formulas: 27 components, computed 9 at a time; wbmi_scan_params.comp_slice = a
+ * selects the components [a][b][c] of one call (hist / out rows of 9, index b*3+c) */
+#define WBMI_FORMULA_VELVELVEL 12 /* formula/covariant.py:823-827 (NLDrude, f'')     */
+#define WBMI_FORMULA_MASSVEL 13   /* formula/covariant.py:830-833 (NLDrude, Fermi surface) */
+/* classic Hall conductivity, the rank-4 products contracted to [j][i] as
+ * calculators/static.py:407-446 contracts them (9 components) */
+#define WBMI_FORMULA_HALL_SURF 14 /* VelMassVel, covariant.py:842-846, static.py:407-424 */
+#define WBMI_FORMULA_HALL_SEA 15  /* MassMass, covariant.py:836-839, static.py:427-446   */
 
 /* channel roles: index into wbmi_scan_params.ch (value = channel slot in the
  * per-k stack, -1 = absent).  Two-index roles are row-major over (x, y). */
@@ -87,6 +95,7 @@ typedef struct wbmi_scan_params {
     double degen_thresh;  /* calculator.py:20                                 */
     int degen_Kramers;
     int ch[WBMI_NROLES];
+    int comp_slice;       /* rank-3 formulas: first tensor index of this call  */
 } wbmi_scan_params;
 
 /* Parameters of the Kubo optical conductivity (calculators/dynamic.py:26-196). */

@@ -44,6 +44,8 @@ FACTOR_GME_ORB = FACTOR_MORB_EVA2_TO_MUB * BOHR_MAGNETON / _ang ** 2
 FACTOR_AHC = -(_e ** 2 / _hbar / _ang)
 FACTOR_NLAHC = _e ** 3 / _hbar ** 2 * 1e-15
 FACTOR_OHMIC = _e ** 2 / _hbar / _ang * 1e-15 * _e / _hbar   # factors.py:25-26
+FACTOR_HALL_CLASSIC = -(_e ** 3 / _hbar ** 2 * _ang * 1e-15 ** 2 * _e ** 2 / _hbar ** 2)   # factors.py:28-29
+FACTOR_NLDRUDE = -(_e ** 3 / _hbar ** 2 * 1e-15 ** 2 * _e / _hbar)                        # factors.py:30-31
 FACTOR_GME_SPIN = -BOHR_MAGNETON / _ang ** 2
 
 # dEig_inv threshold (wannierberri/data_K/data_K.py:292)
@@ -917,6 +919,28 @@ def VelVel(data_K, **kw):
     return FormulaProduct([data_K.covariant('Ham', commader=1), data_K.covariant('Ham', commader=1)])
 
 
+def VelVelVel(data_K, **kw):
+    # formula/covariant.py:823-827
+    v = data_K.covariant('Ham', commader=1)
+    return FormulaProduct([v, v, v])
+
+
+def MassVel(data_K, **kw):
+    # formula/covariant.py:830-833
+    return FormulaProduct([InvMass(data_K), data_K.covariant('Ham', commader=1)])
+
+
+def VelMassVel(data_K, **kw):
+    # formula/covariant.py:842-846
+    v = data_K.covariant('Ham', commader=1)
+    return FormulaProduct([v, InvMass(data_K), v])
+
+
+def MassMass(data_K, **kw):
+    # formula/covariant.py:836-839
+    return FormulaProduct([InvMass(data_K), InvMass(data_K)])
+
+
 def VelSpin(data_K, **kw):
     # formula/covariant.py:811-814
     return FormulaProduct([data_K.covariant('Ham', commader=1), Spin(data_K)])
@@ -1099,6 +1123,54 @@ class Ohmic_FermiSurf(StaticCalculator):
     Formula = staticmethod(VelVel)
     fder = 1
     constant_factor_default = FACTOR_OHMIC
+
+
+_ALPHA, _BETA = np.array([1, 2, 0]), np.array([2, 0, 1])   # utility.py:45-46
+
+
+def _hall_contract(d):
+    """[nEF, a, b, c, d] -> [nEF, j, i] as static.py:418-423: antisymmetrise (c, d) into i,
+    then (a, b) into j, times 1/2."""
+    d = d[:, :, :, _BETA, _ALPHA] - d[:, :, :, _ALPHA, _BETA]
+    return 0.5 * (d[:, _ALPHA, _BETA, :] - d[:, _BETA, _ALPHA, :])
+
+
+class Hall_classic_FermiSurf(StaticCalculator):
+    # static.py:407-424
+    Formula = staticmethod(VelMassVel)
+    fder = 1
+    constant_factor_default = FACTOR_HALL_CLASSIC
+
+    def __call__(self, data_K):
+        r = super().__call__(data_K)
+        r.data = _hall_contract(r.data)
+        return r
+
+
+class Hall_classic_FermiSea(StaticCalculator):
+    # static.py:427-446
+    Formula = staticmethod(MassMass)
+    fder = 0
+    constant_factor_default = FACTOR_HALL_CLASSIC
+
+    def __call__(self, data_K):
+        r = super().__call__(data_K)
+        r.data = _hall_contract(r.data.transpose(0, 4, 1, 2, 3))
+        return r
+
+
+class NLDrude_FermiSurf(StaticCalculator):
+    # static.py:532-542
+    Formula = staticmethod(MassVel)
+    fder = 1
+    constant_factor_default = FACTOR_NLDRUDE
+
+
+class NLDrude_Fermider2(StaticCalculator):
+    # static.py:545-557
+    Formula = staticmethod(VelVelVel)
+    fder = 2
+    constant_factor_default = FACTOR_NLDRUDE / 2
 
 
 class BerryDipole_FermiSurf(StaticCalculator):

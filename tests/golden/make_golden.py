@@ -140,6 +140,11 @@ def main():
              "random_ohmic_fsurf": I + "random-conductivity_ohmic_fsurf_iter-0000.npz",
              "fe_ohmic": I + "Fe_W90-conductivity_ohmic_iter-0000.npz",
              "fe_ohmic_fsurf": I + "Fe_W90-conductivity_ohmic_fsurf_iter-0000.npz",
+             # classic Hall conductivity, Fermi surface and sea, Chiral (also tetrahedra)
+             "chiral_hall_classic": I + "Chiral-Hall_classic_iter-0000.npz",
+             "chiral_hall_classic_fsurf": I + "Chiral-Hall_classic_fsurf_iter-0000.npz",
+             "chiral_tetra_hall_classic": I + "Chiral_tetra-Hall_classic_iter-0000.npz",
+             "chiral_tetra_hall_classic_fsurf": I + "Chiral_tetra-Hall_classic_fsurf_iter-0000.npz",
              }, "results_run.npz")
     # FermiDiracSmoother(Efermi, T_Kelvin=1200, maxdE=8) on the Chiral AHC and Fermi-sea BCD
     # (tests/test_run.py:181-193): the .dat files hold the data and the smoothed data
@@ -150,7 +155,7 @@ def main():
     # Efermi = linspace(4, 8, 11) (tests/test_run.py:154-170,1102-1119; common_systems.py:35)
     results({"te_" + k: I + f"Te_QE-{k}_iter-0000.npz"
              for k in ["GME_orb_FermiSea", "GME_orb_FermiSurf", "BerryDipole_FermiSea", "berry_dipole",
-                       "dos", "cumdos"]}, "results_te_qe.npz")
+                       "dos", "cumdos", "NLDrude_FermiSurf", "NLDrude_Fermider2"]}, "results_te_qe.npz")
     # k-resolved tabulation, Fe (Wannier90), grid 4^3 / 2^3, bands 5 and 6
     # (tests/test_run.py:271-288,313-345)
     with np.load(os.path.join(REF, I + "Fe_W90-tabulate.npz"), allow_pickle=False) as f:

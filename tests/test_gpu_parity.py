@@ -309,6 +309,33 @@ def test_ohmic(wb, results_run, precision, tol):
             assert e < tol, (name, k)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_hall_nldrude(wb, results_run, precision, tol):
+    """Classic Hall conductivity (Fermi surface: VelMassVel, sea: MassMass, contracted as
+    static.py:407-446) against the reference's Chiral fixtures, and the nonlinear Drude
+    conductivity (Fermi surface: MassVel, f'': VelVelVel; 27 components, computed 9 at a time)
+    against the oracle on the same Chiral run (10x10x4 / 5x5x2)."""
+    say(f"test_hall_nldrude {precision}")
+    st = wb.calculators.static
+    s = gsys(wb, "chiral_left")
+    Ef = np.linspace(-5, 8, 27)
+    calcs = {"hall_classic_fsurf": st.Hall_classic_FermiSurf(Efermi=Ef), "hall_classic": st.Hall_classic_FermiSea(Efermi=Ef),
+             "nls": st.NLDrude_FermiSurf(Efermi=Ef), "nl2": st.NLDrude_Fermider2(Efermi=Ef)}
+    r = run_gpu(wb, s, calcs, [10, 10, 4], [5, 5, 2], precision=precision)
+    for k in ("hall_classic_fsurf", "hall_classic"):
+        e = rel_err(r.results[k].data, results_run["chiral_" + k])
+        say(f"  {k}: {e:.2e}")
+        assert e < tol, k
+    ref = O.run(osys("chiral_left"), {"nls": O.NLDrude_FermiSurf(Ef), "nl2": O.NLDrude_Fermider2(Ef)},
+                NK=[10, 10, 4], NKFFT=[5, 5, 2])
+    for k in ("nls", "nl2"):
+        got = r.results[k].data
+        assert got.shape == (len(Ef), 3, 3, 3)
+        e = rel_err(got, ref[k].data)
+        say(f"  {k}: {e:.2e}")
+        assert e < tol, k
+
+
 def test_refinement_random(wb):
     """Adaptive refinement (two iterations, 2x2x2 sub-blocks) against the oracle's
     restatement; the selected blocks' metrics are separated by >= 2 %, so both

@@ -59,7 +59,9 @@ def test_chiral_tetra(wb, results_run, precision, tol):
              "dos": st.DOS(Efermi=Ef, tetra=True),
              "cumdos": st.CumDOS(Efermi=Ef, tetra=True),
              "ohmic": st.Ohmic_FermiSea(Efermi=Ef, tetra=True),
-             "ohmic_fsurf": st.Ohmic_FermiSurf(Efermi=Ef, tetra=True)}
+             "ohmic_fsurf": st.Ohmic_FermiSurf(Efermi=Ef, tetra=True),
+             "hall_classic": st.Hall_classic_FermiSea(Efermi=Ef, tetra=True),
+             "hall_classic_fsurf": st.Hall_classic_FermiSurf(Efermi=Ef, tetra=True)}
     grid = wb.Grid(system=s, NK=[10, 10, 4], NKFFT=[5, 5, 2])
     r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, parameters_K={"precision": precision},
                save_files=False, verbose=False)
@@ -194,11 +196,26 @@ def test_te_qe_gme(wb, precision, tol):
              "berry_dipole": st.NLAHC_FermiSea(Efermi=Ef, tetra=True),
              "BerryDipole_FermiSea": st.BerryDipole_FermiSea(Efermi=Ef, tetra=True),
              "GME_orb_FermiSea": st.GME_orb_FermiSea(Efermi=Ef, tetra=True),
-             "GME_orb_FermiSurf": st.GME_orb_FermiSurf(Efermi=Ef, tetra=True)}
+             "GME_orb_FermiSurf": st.GME_orb_FermiSurf(Efermi=Ef, tetra=True),
+             "NLDrude_FermiSurf": st.NLDrude_FermiSurf(Efermi=Ef, tetra=True),
+             "NLDrude_Fermider2": st.NLDrude_Fermider2(Efermi=Ef, tetra=True)}
     grid = wb.Grid(system=s, NK=[3, 3, 4], NKFFT=[1, 1, 4])
     r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, parameters_K={"precision": precision},
                save_files=False, verbose=False)
     for k in calcs:
+        if k.startswith("NLDrude"):
+            continue
         err = rel_err(r.results[k].data, ref["te_" + k])
         say(f"  {k}: {err:.2e}")
         assert err < tol, k
+    # the nonlinear Drude tensors are odd under time reversal, which is in the group: the
+    # symmetrised results vanish; the reference's files hold its rounding noise (~1e-19)
+    # (relative to the same tensors on 64 random k-points, a set with no symmetry)
+    nl = {"NLDrude_FermiSurf": st.NLDrude_FermiSurf(Efermi=Ef), "NLDrude_Fermider2": st.NLDrude_Fermider2(Efermi=Ef)}
+    r_ns = wb.run_klists(s, [np.random.default_rng(3).random((64, 3))], nl, parameters_K={"precision": precision})
+    for k in nl:
+        got, scale = r.results[k].data, np.abs(r_ns.results[k].data).max()
+        say(f"  {k}: max {np.abs(got).max():.1e}, unsymmetrised {scale:.1e}, "
+            f"reference file {np.abs(ref['te_' + k]).max():.1e}")
+        assert got.shape == ref["te_" + k].shape and scale > 0
+        assert np.abs(got).max() < 1e-12 * scale and np.abs(ref["te_" + k]).max() < 1e-12 * scale

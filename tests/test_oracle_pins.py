@@ -132,6 +132,53 @@ def test_ohmic_chiral_random_fe(results_run):
         assert rel_err(r["surf"].data, results_run[key + "_ohmic_fsurf"]) < 1e-8, name
 
 
+def test_velocity_products_finite_difference():
+    """VelVelVel and MassVel (formula/covariant.py:823-833) traced over an isolated band are
+    v_a v_b v_c and (d_b v_a) v_c of that band, v = dE/dk: central differences of the energies
+    on Fe (W90) at a general k, every band whose neighbours are at least 0.05 eV away.  A
+    factor in the wrong slot of the product or a wrong InvMass index fails this."""
+    o = gsys("fe_sym_osd")
+    k0 = np.array([0.1, 0.2, -0.3])
+    d0 = O.OracleDataK(o, k_list=k0[None])
+    E0 = d0.E_K[0]
+    gap = np.minimum(np.diff(E0, prepend=-np.inf), np.diff(E0, append=np.inf))
+    bands = np.where(gap > 0.05)[0]
+    assert len(bands) >= 6
+    h = 1e-4
+    binv = np.linalg.inv(o.recip_lattice)
+    step = [h * binv[a] for a in range(3)]     # Cartesian step h along axis a, reduced coordinates
+
+    def energies(shifts):
+        return O.OracleDataK(o, k_list=np.array([k0 + s for s in shifts])).E_K
+
+    v = np.stack([(lambda e: (e[0] - e[1]) / (2 * h))(energies([step[a], -step[a]])) for a in range(3)], axis=-1)
+    m = np.zeros((o.num_wann, 3, 3))
+    for a in range(3):
+        for b in range(3):
+            e = energies([step[a] + step[b], step[a] - step[b], -step[a] + step[b], -step[a] - step[b]])
+            m[:, a, b] = (e[0] - e[1] - e[2] + e[3]) / (4 * h * h)
+    vvv, mv = O.VelVelVel(d0), O.MassVel(d0)
+    for n in bands:
+        out = [x for x in range(o.num_wann) if x != n]
+        t3 = vvv.trace(0, [n], out)
+        tm = mv.trace(0, [n], out)
+        ref3 = np.einsum("a,b,c->abc", v[n], v[n], v[n])
+        refm = np.einsum("ab,c->abc", m[n], v[n])
+        assert np.abs(t3 - ref3).max() < 1e-5 * max(1, np.abs(ref3).max()), n
+        assert np.abs(tm - refm).max() < 1e-4 * max(1, np.abs(refm).max()), n
+
+
+def test_chiral_hall_classic(results_run):
+    """Classic Hall conductivity on Chiral (10x10x4 / 5x5x2): Fermi surface (VelMassVel) and
+    sea (MassMass), the rank-4 products contracted as static.py:407-446
+    (tests/test_run.py:186-200)."""
+    Ef = np.linspace(-5, 8, 27)
+    r = O.run(gsys("chiral_left"), {"surf": O.Hall_classic_FermiSurf(Ef), "sea": O.Hall_classic_FermiSea(Ef)},
+              NK=[10, 10, 4], NKFFT=[5, 5, 2])
+    assert rel_err(r["surf"].data, results_run["chiral_hall_classic_fsurf"]) < 1e-8
+    assert rel_err(r["sea"].data, results_run["chiral_hall_classic"]) < 1e-8
+
+
 @pytest.mark.parametrize("fder", [0, 1, 2, 3])
 def test_fe_scan_stencils(results_run, fder):
     s = gsys("fe_w90")

@@ -32,6 +32,10 @@ FORMULA_VELSPIN = 8
 FORMULA_SPIN = 9
 FORMULA_INVMASS = 10
 FORMULA_VELVEL = 11
+FORMULA_VELVELVEL = 12
+FORMULA_MASSVEL = 13
+FORMULA_HALL_SURF = 14
+FORMULA_HALL_SEA = 15
 WBMI_SHC_RYOO, WBMI_SHC_QIAO, WBMI_SHC_SIMPLE = 0, 1, 2
 
 ROLE_V, ROLE_W, ROLE_A, ROLE_AD, ROLE_O, ROLE_OD = 0, 3, 12, 15, 24, 27
@@ -67,7 +71,7 @@ class ScanParams(ctypes.Structure):
         ("external_terms", ctypes.c_int), ("fder", ctypes.c_int), ("nEF_extra", ctypes.c_int),
         ("EFmin", ctypes.c_double), ("EFmax", ctypes.c_double), ("dEF", ctypes.c_double),
         ("degen_thresh", ctypes.c_double), ("degen_Kramers", ctypes.c_int),
-        ("ch", ctypes.c_int * NROLES),
+        ("ch", ctypes.c_int * NROLES), ("comp_slice", ctypes.c_int),
     ]
 
 

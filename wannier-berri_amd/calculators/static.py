@@ -42,7 +42,11 @@ class Calculator:
 _PARITY = {_lib.FORMULA_IDENTITY: (1, 1), _lib.FORMULA_OMEGA: (-1, 1), _lib.FORMULA_DEROMEGA: (1, -1),
            _lib.FORMULA_VELOMEGA: (1, -1), _lib.FORMULA_MORB_HPM: (-1, 1), _lib.FORMULA_DERMORB: (1, -1),
            _lib.FORMULA_VELHPLUS: (1, -1), _lib.FORMULA_DERSPIN: (1, -1), _lib.FORMULA_VELSPIN: (1, -1),
-           _lib.FORMULA_SPIN: (-1, 1), _lib.FORMULA_INVMASS: (1, 1), _lib.FORMULA_VELVEL: (1, 1)}
+           _lib.FORMULA_SPIN: (-1, 1), _lib.FORMULA_INVMASS: (1, 1), _lib.FORMULA_VELVEL: (1, 1),
+           # products of velocities (odd, odd) and inverse masses (even, even) (formula.py:127-128);
+           # the Hall tensors keep the parity of their rank-4 products (static.py:418-423)
+           _lib.FORMULA_VELVELVEL: (-1, -1), _lib.FORMULA_MASSVEL: (-1, -1), _lib.FORMULA_HALL_SURF: (1, 1),
+           _lib.FORMULA_HALL_SEA: (1, 1)}
 # formulas that read only the Hamiltonian's derivatives and the spin, never A, B, C
 _SPIN = (_lib.FORMULA_DERSPIN, _lib.FORMULA_VELSPIN, _lib.FORMULA_SPIN)
 
@@ -113,7 +117,12 @@ class StaticCalculator(Calculator):
 
     @property
     def ncomp(self):
-        return 3 ** self.ndim
+        """Components of one library call: rank-3 formulas run as three calls of 9 (comp_slice)."""
+        return min(3 ** self.ndim, 9)
+
+    @property
+    def nslices(self):
+        return 3 if self.ndim == 3 else 1
 
     def needed_channels(self, system):
         """Channels Xbar(name, der) the formula reads."""
@@ -125,9 +134,9 @@ class StaticCalculator(Calculator):
             return {("Ham", 0), ("SS", 0)}
         if f in _SPIN:
             return {("Ham", 0), ("Ham", 1), ("SS", 0)} | ({("SS", 1)} if f == _lib.FORMULA_DERSPIN else set())
-        if f == _lib.FORMULA_VELVEL:
+        if f in (_lib.FORMULA_VELVEL, _lib.FORMULA_VELVELVEL):
             return {("Ham", 0), ("Ham", 1)}
-        if f == _lib.FORMULA_INVMASS:
+        if f in (_lib.FORMULA_INVMASS, _lib.FORMULA_MASSVEL, _lib.FORMULA_HALL_SURF, _lib.FORMULA_HALL_SEA):
             return {("Ham", 0), ("Ham", 1), ("Ham", 2)}
         need = {("Ham", 0), ("Ham", 1)}
         if f in (_lib.FORMULA_DEROMEGA, _lib.FORMULA_DERMORB):
@@ -154,7 +163,15 @@ class StaticCalculator(Calculator):
         p.EFmin, p.EFmax, p.dEF = float(self.EFmin), float(self.EFmax), float(self.dEF)
         p.degen_thresh = float(self.degen_thresh)
         p.degen_Kramers = int(bool(self.degen_Kramers))
+        p.comp_slice = 0
         return p
+
+    def _slices(self, data_K, run):
+        """run(comp_slice) -> [..., nEF, 9] per slice, joined to [..., nEF, 27] (first index slowest)."""
+        if self.nslices == 1:
+            return run(0)
+        outs = [run(a) for a in range(3)]
+        return data_K.torch.stack(outs, dim=-2).reshape(outs[0].shape[:-1] + (27,))
 
     def _call_tetra(self, data_K):
         """Tetrahedron weights (static.py:80-85,119-125): no extended Fermi grid, no stencil.
@@ -165,24 +182,29 @@ class StaticCalculator(Calculator):
         if Ef is None:
             Ef = self._Ef_dev[dev] = torch.from_numpy(self.Efermi).to(dev)
         nEF = self.Efermi.shape[0]
-        p = self.scan_params(data_K.system)
-        p.EFmin, p.EFmax = float(self.Efermi[0]), float(self.Efermi[-1])
-        p.nEF_extra = nEF
-        if self.hole_like:
-            p.fder = -1   # 1 - occupation (static.py:81-82)
         cf = self.constant_factor if self.use_factor else float(np.sign(self.constant_factor))
-        if self.k_resolved:
-            nk = data_K.nk
-            hist = torch.zeros((nk, 2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
-            data_K.static_scan_tetra_k(p, Ef, hist, 1.0)
-            out = torch.empty((nk, nEF, self.ncomp), dtype=torch.float64, device=dev)
-            data_K.ctx.finalize_tetra_k(hist, cf / data_K.cell_volume, out)
+
+        def run(sl):
+            p = self.scan_params(data_K.system)
+            p.comp_slice = sl
+            p.EFmin, p.EFmax = float(self.Efermi[0]), float(self.Efermi[-1])
+            p.nEF_extra = nEF
+            if self.hole_like:
+                p.fder = -1   # 1 - occupation (static.py:81-82)
+            if self.k_resolved:
+                nk = data_K.nk
+                hist = torch.zeros((nk, 2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
+                data_K.static_scan_tetra_k(p, Ef, hist, 1.0)
+                out = torch.empty((nk, nEF, self.ncomp), dtype=torch.float64, device=dev)
+                data_K.ctx.finalize_tetra_k(hist, cf / data_K.cell_volume, out)
+                return out
+            hist = torch.zeros((2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
+            data_K.static_scan_tetra(p, Ef, hist, 1.0)
+            out = torch.empty((nEF, self.ncomp), dtype=torch.float64, device=dev)
+            data_K.ctx.finalize_tetra(hist, cf / data_K.cell_volume / data_K.nk, out)
             return out
-        hist = torch.zeros((2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
-        data_K.static_scan_tetra(p, Ef, hist, 1.0)
-        out = torch.empty((nEF, self.ncomp), dtype=torch.float64, device=dev)
-        data_K.ctx.finalize_tetra(hist, cf / data_K.cell_volume / data_K.nk, out)
-        return out
+
+        return self._slices(data_K, run)
 
     def _call_k_resolved(self, data_K):
         """Per k-point results [nk, nEF, 3^ndim] (static.py:60-101,156-164): each k its own
@@ -190,12 +212,19 @@ class StaticCalculator(Calculator):
         torch = data_K.torch
         dev = data_K.ctx.device
         nk = data_K.nk
-        hist = torch.zeros((nk, self.nEF_extra, self.ncomp), dtype=torch.float64, device=dev)
-        data_K.static_scan_k(self.scan_params(data_K.system), hist, 1.0)
         nout = self.Efermi.shape[0]
-        out = torch.empty((nk, nout, self.ncomp), dtype=torch.float64, device=dev)
         cf = self.constant_factor if self.use_factor else float(np.sign(self.constant_factor))
-        data_K.ctx.finalize_k(hist, self.fder, self.dEF, cf / data_K.cell_volume, out)
+
+        def run(sl):
+            p = self.scan_params(data_K.system)
+            p.comp_slice = sl
+            hist = torch.zeros((nk, self.nEF_extra, self.ncomp), dtype=torch.float64, device=dev)
+            data_K.static_scan_k(p, hist, 1.0)
+            out = torch.empty((nk, nout, self.ncomp), dtype=torch.float64, device=dev)
+            data_K.ctx.finalize_k(hist, self.fder, self.dEF, cf / data_K.cell_volume, out)
+            return out
+
+        out = self._slices(data_K, run)
         tr, inv = _PARITY[self.formula]
         return KBandResult(out.cpu().numpy().reshape((nk, nout) + (3,) * self.ndim), transformTR=tr, transformInv=inv)
 
@@ -215,13 +244,20 @@ class StaticCalculator(Calculator):
             return EnergyResult(self.Efermi, data, smoothers=[self.smoother], transformTR=tr, transformInv=inv,
                                 comment=self.comment, save_mode=self.save_mode)
         dev = data_K.ctx.device
-        hist = torch.zeros((self.nEF_extra, self.ncomp), dtype=torch.float64, device=dev)
-        data_K.static_scan(self.scan_params(data_K.system), hist, 1.0)
         nout = self.Efermi.shape[0]
-        out = torch.empty((nout, self.ncomp), dtype=torch.float64, device=dev)
         cf = self.constant_factor if self.use_factor else float(np.sign(self.constant_factor))
         scale = cf / data_K.cell_volume / data_K.nk
-        data_K.ctx.finalize(hist, self.fder, self.dEF, scale, out)
+
+        def run(sl):
+            p = self.scan_params(data_K.system)
+            p.comp_slice = sl
+            hist = torch.zeros((self.nEF_extra, self.ncomp), dtype=torch.float64, device=dev)
+            data_K.static_scan(p, hist, 1.0)
+            out = torch.empty((nout, self.ncomp), dtype=torch.float64, device=dev)
+            data_K.ctx.finalize(hist, self.fder, self.dEF, scale, out)
+            return out
+
+        out = self._slices(data_K, run)
         data = out.reshape((nout,) + (3,) * self.ndim)
         tr, inv = _PARITY[self.formula]
         return EnergyResult(self.Efermi, data, smoothers=[self.smoother], transformTR=tr, transformInv=inv,
@@ -291,6 +327,59 @@ class Ohmic_FermiSurf(StaticCalculator):
     fder = 1
 
     def __init__(self, constant_factor=factors.factor_ohmic, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class Hall_classic_FermiSurf(StaticCalculator):
+    r"""Classic Hall conductivity (:math:`S/m/T`), Fermi surface (static.py:407-424)
+
+        | Output: :math:`\sigma_{\alpha\beta :\mu} = e^3/\hbar^2 \tau^2 \epsilon_{\gamma\mu\rho} \int [dk] v_\alpha \partial_\rho v_\beta v_\gamma f'`
+        | Instruction: :math:`j_\alpha = \sigma_{\alpha\beta :\mu} E_\beta B_\mu`"""
+    formula = _lib.FORMULA_HALL_SURF
+    ndim = 2
+    fder = 1
+
+    def __init__(self, constant_factor=factors.factor_hall_classic, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class Hall_classic_FermiSea(StaticCalculator):
+    r"""Classic Hall conductivity (:math:`S/m/T`), Fermi sea (static.py:427-446)
+
+        | Output: :math:`\sigma_{\alpha\beta :\mu} = -e^3/\hbar^2 \tau^2 \epsilon_{\gamma\mu\rho} \int [dk] \partial_\gamma v_\alpha \partial_\rho v_\beta f`
+        | Instruction: :math:`j_\alpha = \sigma_{\alpha\beta :\mu} E_\beta B_\mu`"""
+    formula = _lib.FORMULA_HALL_SEA
+    ndim = 2
+    fder = 0
+
+    def __init__(self, constant_factor=factors.factor_hall_classic, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class NLDrude_FermiSurf(StaticCalculator):
+    r"""Nonlinear Drude conductivity (:math:`S^2/A`), Fermi surface (static.py:532-542)
+
+        | Output: :math:`\sigma_{\alpha\beta\gamma} = e^3/\hbar^2 \tau^2 \int [dk] \partial_\beta v_\alpha v_\gamma f'`
+        | Instruction: :math:`j_\alpha = \sigma_{\alpha\beta\gamma} E_\beta E\gamma`"""
+    formula = _lib.FORMULA_MASSVEL
+    ndim = 3
+    fder = 1
+
+    def __init__(self, constant_factor=factors.factor_nldrude, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class NLDrude_Fermider2(StaticCalculator):
+    r"""Nonlinear Drude conductivity (:math:`S^2/A`), second derivative of the occupation
+    (static.py:545-557)
+
+        | Output: :math:`\sigma_{\alpha\beta\gamma} = -e^3/\hbar^2 \tau^2 \int [dk] v_\beta v_\alpha v_\gamma f''`
+        | Instruction: :math:`j_\alpha = \sigma_{\alpha\beta\gamma} E_\beta E\gamma`"""
+    formula = _lib.FORMULA_VELVELVEL
+    ndim = 3
+    fder = 2
+
+    def __init__(self, constant_factor=factors.factor_nldrude / 2, **kwargs):
         super().__init__(constant_factor=constant_factor, **kwargs)
 
 
@@ -450,4 +539,5 @@ class GME_spin_FermiSurf(StaticCalculator):
 
 __all__ = ["Calculator", "StaticCalculator", "DOS", "CumDOS", "AHC", "BerryDipole_FermiSurf", "NLAHC_FermiSurf",
            "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "GME_orb_FermiSurf", "GME_orb_FermiSea",
-           "GME_spin_FermiSea", "GME_spin_FermiSurf", "Spin", "Ohmic_FermiSea", "Ohmic_FermiSurf", "VoidResult"]
+           "GME_spin_FermiSea", "GME_spin_FermiSurf", "Spin", "Ohmic_FermiSea", "Ohmic_FermiSurf",
+           "Hall_classic_FermiSurf", "Hall_classic_FermiSea", "NLDrude_FermiSurf", "NLDrude_Fermider2", "VoidResult"]

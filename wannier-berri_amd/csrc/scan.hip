@@ -49,6 +49,17 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
         case WBMI_FORMULA_VELVEL:
             if ((rc = need(WBMI_ROLE_V, 3))) return rc;
             break;
+        case WBMI_FORMULA_VELVELVEL:
+        case WBMI_FORMULA_MASSVEL:
+            if (p->comp_slice < 0 || p->comp_slice > 2)
+                return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: comp_slice must be 0, 1 or 2");
+            if ((rc = need(WBMI_ROLE_V, 3))) return rc;
+            if (p->formula == WBMI_FORMULA_MASSVEL && (rc = need(WBMI_ROLE_W, 9))) return rc;
+            break;
+        case WBMI_FORMULA_HALL_SURF:
+        case WBMI_FORMULA_HALL_SEA:
+            if ((rc = need(WBMI_ROLE_V, 3)) || (rc = need(WBMI_ROLE_W, 9))) return rc;
+            break;
         case WBMI_FORMULA_MORB_HPM:
         case WBMI_FORMULA_VELHPLUS:
             if ((rc = need(WBMI_ROLE_V, 3))) return rc;

@@ -287,6 +287,34 @@ __device__ __attribute__((always_inline)) cplx<double> morbh_s(const KCtx<T, LM,
     return s;
 }
 
+// Group block of the inverse effective mass (formula/elementary.py:28-33 through
+// formula/formula.py:95-107 with A = V, dA = W): for x, y in one band group,
+//   M^{bd}_xy = W^{bd}_xy - sum_{l off} D^d_xl V^b_ly + sum_{l off} V^b_xl D^d_ly
+// (D' carries the off-group mask, so the sums run over every l).
+template <typename T, bool LM, bool ST, bool SA, int NW, typename TS>
+__device__ __attribute__((always_inline)) cplx<double> mass_xy(const KCtx<T, LM, ST, SA, NW, TS>& K, int b, int d, int x,
+                                                               int y) {
+    cplx<double> v = K.W(b, d, x, y);
+    for (int l = 0; l < K.n_(); ++l) {
+        v -= K.Dp(d, x, l) * K.V(b, l, y);
+        v += K.V(b, x, l) * K.Dp(d, l, y);
+    }
+    return v;
+}
+
+// Contraction of a rank-4 product X[a][b][c][d] to the Hall tensor [j][i]
+// (calculators/static.py:419-423): 0.5 sum over (a, b) = (al_j, be_j) (+), (be_j, al_j) (-)
+// and (c, d) = (be_i, al_i) (+), (al_i, be_i) (-).  `term(a, b, c, d)` gives X.
+template <typename Fn>
+__device__ __attribute__((always_inline)) cplx<double> hall_contract(int j, int i, Fn term) {
+    const int aj = c_alpha[j], bj = c_beta[j], ai = c_alpha[i], bi = c_beta[i];
+    cplx<double> v = term(aj, bj, bi, ai);
+    v -= term(aj, bj, ai, bi);
+    v -= term(bj, aj, bi, ai);
+    v += term(bj, aj, ai, bi);
+    return mk(0.5 * v.re, 0.5 * v.im);
+}
+
 template <int F>
 __host__ __device__ constexpr int ncomp_of() {
     return F == WBMI_FORMULA_IDENTITY ? 1
@@ -891,6 +919,53 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
             for (int m = S.gs[n]; m < S.ge[n]; ++m) v += K.V(a, n, m) * K.V(b, m, n);
             S.acc[t] = mk(v.re, 0.0);
         }
+    } else if (F == WBMI_FORMULA_VELVELVEL) {
+        // [a][b][c] = Re sum_{m, l in G} V^a_nm V^b_ml V^c_ln (formula/formula.py:121-148), a = comp_slice
+        const int a = p.comp_slice;
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
+            const int n = t / 9, b = (t % 9) / 3, c = t % 3;
+            if (!S.act[n]) continue;
+            cplx<double> v = mk(0.0, 0.0);
+            for (int m = S.gs[n]; m < S.ge[n]; ++m)
+                for (int l = S.gs[n]; l < S.ge[n]; ++l) v += K.V(a, n, m) * K.V(b, m, l) * K.V(c, l, n);
+            S.acc[t] = mk(v.re, 0.0);
+        }
+    } else if (F == WBMI_FORMULA_MASSVEL) {
+        // [a][b][c] = Re sum_{m in G} M^{ab}_nm V^c_mn, a = comp_slice
+        const int a = p.comp_slice;
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
+            const int n = t / 9, b = (t % 9) / 3, c = t % 3;
+            if (!S.act[n]) continue;
+            cplx<double> v = mk(0.0, 0.0);
+            for (int m = S.gs[n]; m < S.ge[n]; ++m) v += mass_xy(K, a, b, n, m) * K.V(c, m, n);
+            S.acc[t] = mk(v.re, 0.0);
+        }
+    } else if (F == WBMI_FORMULA_HALL_SURF) {
+        // VelMassVel X[a][b][c][d] = Re sum_{m, l in G} V^a_nm M^{bc}_ml V^d_ln, contracted
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
+            const int n = t / 9, j = (t % 9) / 3, i = t % 3;
+            if (!S.act[n]) continue;
+            const cplx<double> v = hall_contract(j, i, [&](int a, int b, int c, int d) {
+                cplx<double> x = mk(0.0, 0.0);
+                for (int m = S.gs[n]; m < S.ge[n]; ++m)
+                    for (int l = S.gs[n]; l < S.ge[n]; ++l) x += K.V(a, n, m) * mass_xy(K, b, c, m, l) * K.V(d, l, n);
+                return x;
+            });
+            S.acc[t] = mk(v.re, 0.0);
+        }
+    } else if (F == WBMI_FORMULA_HALL_SEA) {
+        // MassMass X[a][b][c][d] = Re sum_{m in G} M^{ab}_nm M^{cd}_mn, its axes moved as
+        // transpose(0, 4, 1, 2, 3) (Y[p][q][r][s] = X[q][r][s][p]), contracted
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
+            const int n = t / 9, j = (t % 9) / 3, i = t % 3;
+            if (!S.act[n]) continue;
+            const cplx<double> v = hall_contract(j, i, [&](int p0, int q, int r, int s4) {
+                cplx<double> x = mk(0.0, 0.0);
+                for (int m = S.gs[n]; m < S.ge[n]; ++m) x += mass_xy(K, q, r, n, m) * mass_xy(K, s4, p0, m, n);
+                return x;
+            });
+            S.acc[t] = mk(v.re, 0.0);
+        }
     } else if (F == WBMI_FORMULA_DEROMEGA) {
         eval_deromega(S, K, in, ex, false, 0);
     } else if (F == WBMI_FORMULA_DERMORB) {
@@ -1159,7 +1234,8 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
                               F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
                               F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERSPIN ||
                               F == WBMI_FORMULA_VELSPIN || F == WBMI_FORMULA_SPIN || F == WBMI_FORMULA_INVMASS ||
-                              F == WBMI_FORMULA_VELVEL;
+                              F == WBMI_FORMULA_VELVEL || F == WBMI_FORMULA_VELVELVEL || F == WBMI_FORMULA_MASSVEL ||
+                              F == WBMI_FORMULA_HALL_SURF || F == WBMI_FORMULA_HALL_SEA;
 
     // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186), formed by wave 0: each lane
     // owns bands lane and lane + 64 (nw <= 128); borders from two ballots and popcounts, one
@@ -1426,6 +1502,10 @@ static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
         case WBMI_FORMULA_SPIN: return launch_scan<T, WBMI_FORMULA_SPIN, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_INVMASS: return launch_scan<T, WBMI_FORMULA_INVMASS, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_VELVEL: return launch_scan<T, WBMI_FORMULA_VELVEL, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_VELVELVEL: return launch_scan<T, WBMI_FORMULA_VELVELVEL, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_MASSVEL: return launch_scan<T, WBMI_FORMULA_MASSVEL, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_HALL_SURF: return launch_scan<T, WBMI_FORMULA_HALL_SURF, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_HALL_SEA: return launch_scan<T, WBMI_FORMULA_HALL_SEA, MODE>(ctx, nk, a, E, X, hist);
         default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
     }
 }
