@@ -63,6 +63,10 @@ extern "C" {
  * calculators/static.py:407-446 contracts them (9 components) */
 #define WBMI_FORMULA_HALL_SURF 14 /* VelMassVel, covariant.py:842-846, static.py:407-424 */
 #define WBMI_FORMULA_HALL_SEA 15  /* MassMass, covariant.py:836-839, static.py:427-446   */
+/* per-band values computed by the caller: Xbar_dev is float64 [nk, nw, nch] (nch = 9 or 27,
+ * `prec` is not read), and a band's value is row comp_slice * 9 .. + 8 of its nch; the
+ * trace of a group is the sum over its bands (additive), as for the other formulas */
+#define WBMI_FORMULA_BANDVALUES 16
 
 /* channel roles: index into wbmi_scan_params.ch (value = channel slot in the
  * per-k stack, -1 = absent).  Two-index roles are row-major over (x, y). */
@@ -409,6 +413,12 @@ int wbmi_tabulate_invmass(wbmi_ctx* ctx, int nk, int nw, int nch, const double* 
 int wbmi_tabulate_spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
                              int prec, const wbmi_shc_params* params, void* work_dev, size_t work_bytes,
                              double* out_dev);
+/* The same per band, before the group mean: out[k][n][abs] = c_n^{abs}, the band's share of its
+ * group's trace (the trace of a group is the sum over its bands).  Input of the static spin Hall
+ * conductivity (calculators/static.py:647-660), which bins these with WBMI_FORMULA_BANDVALUES. */
+int wbmi_spin_berry_bands(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                          int prec, const wbmi_shc_params* params, void* work_dev, size_t work_bytes,
+                          double* out_dev);
 
 /* out_dev [nEF, ncomp] = scale * (prefix sums of rows [0, nEF) + rows [nEF, 2 nEF))
  * of a wbmi_static_scan_tetra histogram (static.py:149-155, no fder stencil).

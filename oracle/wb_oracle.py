@@ -1159,6 +1159,13 @@ class Hall_classic_FermiSea(StaticCalculator):
         return r
 
 
+class SHC_static(StaticCalculator):
+    # static.py:647-660 (the static SHC; `SHC` here is the frequency-dependent one)
+    Formula = staticmethod(lambda data_K, **kw: SpinOmega(data_K, **kw))   # defined below
+    fder = 0
+    constant_factor_default = -FACTOR_AHC / 2
+
+
 class NLDrude_FermiSurf(StaticCalculator):
     # static.py:532-542
     Formula = staticmethod(MassVel)

@@ -336,6 +336,53 @@ def test_hall_nldrude(wb, results_run, precision, tol):
         assert e < tol, k
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_shc_static(wb, precision, tol):
+    """dc spin Hall conductivity (static.SHC: per-band spin Berry curvature binned by the Fermi
+    scan as caller values) on Fe (Wannier90), grid 4^3 / 2^3, Efermi 17 and 18 eV: equal to the
+    GPU Kubo SHC at omega = 0 with a 1e-10 eV Gaussian at the reference's own tolerance
+    (tests/test_run.py:1120-1151; FP64 form), and to the oracle's static SHC (Qiao, Ryoo, and
+    'simple' without external terms; tetrahedron weights for Ryoo)."""
+    st, dy = wb.calculators.static, wb.calculators.dynamic
+    s = gsys(wb, "fe_w90")
+    Ef = np.array([17.0, 18.0])
+    pk = {"precision": precision}
+    o = osys("fe_w90")
+    for kind in ("qiao", "ryoo"):
+        say(f"test_shc_static {kind} {precision}")
+        calcs = {"st": st.SHC(Efermi=Ef, kwargs_formula={"spin_current_type": kind}),
+                 "dyn": dy.SHC(SHC_type=kind, Efermi=Ef, omega=np.arange(0.0, 7.1, 1.0), smr_fixed_width=1e-10,
+                               smr_type="Gaussian")}
+        r = run_gpu(wb, s, calcs, [4, 4, 4], [2, 2, 2], precision=precision)
+        got, dyn = r.results["st"].data, r.results["dyn"].data[:, 0].real
+        assert got.shape == (2, 3, 3, 3)
+        if precision == "fp64":
+            assert np.abs(got - dyn).max() <= max(np.average(np.abs(got)) / 1e10, 1e-8), kind
+        ref = O.run(o, {"st": O.SHC_static(Ef, kwargs_formula={"spin_current_type": kind})}, NK=[4, 4, 4],
+                    NKFFT=[2, 2, 2])["st"].data
+        e = rel_err(got, ref)
+        say(f"  vs oracle: {e:.2e}, vs Kubo at omega=0: {np.abs(got - dyn).max():.1e}")
+        assert e < tol, kind
+    # 'simple' spin current without external terms, and Ryoo with tetrahedron weights
+    calcs = {"simple": st.SHC(Efermi=Ef, kwargs_formula={"spin_current_type": "simple", "external_terms": False}),
+             "tetra": st.SHC(Efermi=Ef, tetra=True, kwargs_formula={"spin_current_type": "ryoo"})}
+    r = run_gpu(wb, s, calcs, [4, 4, 4], [2, 2, 2], precision=precision)
+    ref = O.run(o, {"simple": O.SHC_static(Ef, kwargs_formula={"spin_current_type": "simple",
+                                                               "external_terms": False})},
+                NK=[4, 4, 4], NKFFT=[2, 2, 2])["simple"].data
+    e = rel_err(r.results["simple"].data, ref)
+    say(f"  simple, internal terms: {e:.2e}")
+    assert e < tol
+    NKdiv, NKFFT, blocks = O.grid_blocks([4, 4, 4], [2, 2, 2])
+    ref = 0
+    for dK, factor in blocks:
+        d = O.OracleDataK(o, dK=dK, NKFFT=NKFFT, dK_block=1.0 / NKdiv)
+        ref = ref + O.SHC_static(Ef, tetra=True, kwargs_formula={"spin_current_type": "ryoo"})(d).data * factor
+    e = rel_err(r.results["tetra"].data, ref)
+    say(f"  Ryoo, tetrahedra: {e:.2e}")
+    assert e < tol
+
+
 def test_refinement_random(wb):
     """Adaptive refinement (two iterations, 2x2x2 sub-blocks) against the oracle's
     restatement; the selected blocks' metrics are separated by >= 2 %, so both

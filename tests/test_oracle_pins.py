@@ -168,6 +168,22 @@ def test_velocity_products_finite_difference():
         assert np.abs(tm - refm).max() < 1e-4 * max(1, np.abs(refm).max()), n
 
 
+@pytest.mark.parametrize("kind", ["qiao", "ryoo"])
+def test_shc_static_equals_dynamic_at_zero_frequency(kind):
+    """The dc spin Hall conductivity (static SHC: Fermi-sea integral of the spin Berry
+    curvature, static.py:647-660) equals the Kubo SHC at omega = 0 with a vanishing Gaussian
+    width (1e-10 eV), on Fe (Wannier90), grid 4^3 / 2^3, Efermi 17 and 18 eV, at the reference's
+    own tolerance (tests/test_run.py:1120-1151).  The Kubo SHC is pinned by the Fe fixtures."""
+    Ef = np.array([17.0, 18.0])
+    r = O.run(gsys("fe_w90"), {"dyn": O.SHC(Ef, np.arange(0.0, 7.1, 1.0), SHC_type=kind, smr_fixed_width=1e-10,
+                                           smr_type="Gaussian"),
+                               "st": O.SHC_static(Ef, kwargs_formula={"spin_current_type": kind})},
+              NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    st, dyn = r["st"].data, r["dyn"].data[:, 0].real
+    assert np.abs(st).max() > 0
+    assert np.abs(st - dyn).max() <= max(np.average(np.abs(st)) / 1e10, 1e-8)
+
+
 def test_chiral_hall_classic(results_run):
     """Classic Hall conductivity on Chiral (10x10x4 / 5x5x2): Fermi surface (VelMassVel) and
     sea (MassMass), the rank-4 products contracted as static.py:407-446

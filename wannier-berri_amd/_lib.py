@@ -36,6 +36,7 @@ FORMULA_VELVELVEL = 12
 FORMULA_MASSVEL = 13
 FORMULA_HALL_SURF = 14
 FORMULA_HALL_SEA = 15
+FORMULA_BANDVALUES = 16
 WBMI_SHC_RYOO, WBMI_SHC_QIAO, WBMI_SHC_SIMPLE = 0, 1, 2
 
 ROLE_V, ROLE_W, ROLE_A, ROLE_AD, ROLE_O, ROLE_OD = 0, 3, 12, 15, 24, 27
@@ -53,7 +54,7 @@ SYMBOLS = [
     "wbmi_tabulate", "wbmi_tabulate_diag", "wbmi_shift_current", "wbmi_injection_current",
     "wbmi_static_scan_k", "wbmi_finalize_k", "wbmi_static_scan_tetra_k", "wbmi_finalize_tetra_k",
     "wbmi_tabulate_invmass", "wbmi_eigh_noalloc", "wbmi_eigh_noalloc_max_bytes", "wbmi_r2k_klist_tile",
-    "wbmi_r2k_klist_tile_shape", "wbmi_tabulate_spin_berry",
+    "wbmi_r2k_klist_tile_shape", "wbmi_tabulate_spin_berry", "wbmi_spin_berry_bands",
 ]
 
 
@@ -141,6 +142,8 @@ _PROTOS = {
     "wbmi_spin_hall": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t, _vp]),
     "wbmi_tabulate_spin_berry": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t,
                                       _vp]),
+    "wbmi_spin_berry_bands": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t,
+                                   _vp]),
 }
 
 
@@ -298,18 +301,20 @@ class Context:
         self.check(self._timed(stage, (nk, nw, nrot), lib().wbmi_rotate, self.h, nk, nw, nch, _ptr(U), _ptr(X),
                                h_p, s_p, prec, _ptr(work), wbytes), "wbmi_rotate")
 
-    def static_scan(self, E, X, prec, params, weight, hist):
+    def static_scan(self, E, X, prec, params, weight, hist, nch=None):
         nk, nw = E.shape
-        nch = X.shape[1] if X is not None else 1
+        if nch is None:
+            nch = X.shape[1] if X is not None else 1
         self.check(self._timed(f"scan_f{params.formula}", (nk, nw), lib().wbmi_static_scan, self.h, nk, nw, nch,
                                _ptr(E), _ptr(X), prec, ctypes.byref(params), float(weight), _ptr(hist)),
                    "wbmi_static_scan")
 
-    def static_scan_tetra(self, E, Ec, X, prec, params, Efermi, weight, hist):
+    def static_scan_tetra(self, E, Ec, X, prec, params, Efermi, weight, hist, nch=None):
         """Tetrahedron-weighted scan: Ec [8, nk, nw], Efermi [nEF] (device, ascending),
         hist [2 nEF, ncomp]."""
         nk, nw = E.shape
-        nch = X.shape[1] if X is not None else 1
+        if nch is None:
+            nch = X.shape[1] if X is not None else 1
         if tuple(Ec.shape[1:]) != (nk, nw) or Ec.shape[0] not in (4, 8) or \
                 tuple(hist.shape) != (2 * Efermi.shape[0], hist.shape[1]):
             raise ValueError("static_scan_tetra: Ec must be [8 or 4, nk, nw] and hist [2 nEF, ncomp]")
@@ -319,10 +324,11 @@ class Context:
                                int(Efermi.shape[0]), float(weight), _ptr(hist)),
                    "wbmi_static_scan_tetra")
 
-    def static_scan_tetra_k(self, E, Ec, X, prec, params, Efermi, weight, hist):
+    def static_scan_tetra_k(self, E, Ec, X, prec, params, Efermi, weight, hist, nch=None):
         """k-resolved tetrahedron scan: hist [nk, 2 nEF, ncomp]."""
         nk, nw = E.shape
-        nch = X.shape[1] if X is not None else 1
+        if nch is None:
+            nch = X.shape[1] if X is not None else 1
         if tuple(Ec.shape[1:]) != (nk, nw) or Ec.shape[0] not in (4, 8) or \
                 tuple(hist.shape[:2]) != (nk, 2 * Efermi.shape[0]):
             raise ValueError("static_scan_tetra_k: Ec must be [8 or 4, nk, nw] and hist [nk, 2 nEF, ncomp]")
@@ -359,13 +365,14 @@ class Context:
         self.check(self._timed("shc", (nk, nw), lib().wbmi_spin_hall, self.h, nk, nw, X.shape[1], _ptr(E), _ptr(X),
                                prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)), "wbmi_spin_hall")
 
-    def tabulate_spin_berry(self, E, X, prec, params, out):
+    def tabulate_spin_berry(self, E, X, prec, params, out, per_band=False):
         nk, nw = E.shape
         wbytes = int(lib().wbmi_spin_hall_work_bytes(nk, nw))
         work = self._workspace(wbytes)
-        self.check(self._timed("tab_spin_berry", (nk, nw), lib().wbmi_tabulate_spin_berry, self.h, nk, nw, X.shape[1],
-                               _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)),
-                   "wbmi_tabulate_spin_berry")
+        fn = lib().wbmi_spin_berry_bands if per_band else lib().wbmi_tabulate_spin_berry
+        self.check(self._timed("spin_berry_bands" if per_band else "tab_spin_berry", (nk, nw), fn, self.h, nk, nw,
+                               X.shape[1], _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)),
+                   "wbmi_spin_berry_bands" if per_band else "wbmi_tabulate_spin_berry")
 
     def tabulate(self, E, X, prec, params, out):
         nk, nw = E.shape
@@ -396,9 +403,10 @@ class Context:
                                _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)),
                    "wbmi_injection_current")
 
-    def static_scan_k(self, E, X, prec, params, weight, hist):
+    def static_scan_k(self, E, X, prec, params, weight, hist, nch=None):
         nk, nw = E.shape
-        nch = X.shape[1] if X is not None else 1
+        if nch is None:
+            nch = X.shape[1] if X is not None else 1
         self.check(self._timed(f"scan_k_f{params.formula}", (nk, nw), lib().wbmi_static_scan_k, self.h, nk, nw, nch,
                                _ptr(E), _ptr(X), prec, ctypes.byref(params), float(weight), _ptr(hist)),
                    "wbmi_static_scan_k")

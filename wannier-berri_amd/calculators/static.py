@@ -124,6 +124,11 @@ class StaticCalculator(Calculator):
     def nslices(self):
         return 3 if self.ndim == 3 else 1
 
+    @property
+    def parity(self):
+        """(time reversal, inversion) parity of the result (+1 even, -1 odd)."""
+        return _PARITY[self.formula]
+
     def needed_channels(self, system):
         """Channels Xbar(name, der) the formula reads."""
         ext = _external(self.kwargs_formula, system)
@@ -173,7 +178,7 @@ class StaticCalculator(Calculator):
         outs = [run(a) for a in range(3)]
         return data_K.torch.stack(outs, dim=-2).reshape(outs[0].shape[:-1] + (27,))
 
-    def _call_tetra(self, data_K):
+    def _call_tetra(self, data_K, bands=None):
         """Tetrahedron weights (static.py:80-85,119-125): no extended Fermi grid, no stencil.
         Returns [nEF, ncomp], or [nk, nEF, ncomp] k-resolved (/ cell volume only)."""
         torch = data_K.torch
@@ -194,19 +199,19 @@ class StaticCalculator(Calculator):
             if self.k_resolved:
                 nk = data_K.nk
                 hist = torch.zeros((nk, 2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
-                data_K.static_scan_tetra_k(p, Ef, hist, 1.0)
+                data_K.static_scan_tetra_k(p, Ef, hist, 1.0, bands=bands)
                 out = torch.empty((nk, nEF, self.ncomp), dtype=torch.float64, device=dev)
                 data_K.ctx.finalize_tetra_k(hist, cf / data_K.cell_volume, out)
                 return out
             hist = torch.zeros((2 * nEF, self.ncomp), dtype=torch.float64, device=dev)
-            data_K.static_scan_tetra(p, Ef, hist, 1.0)
+            data_K.static_scan_tetra(p, Ef, hist, 1.0, bands=bands)
             out = torch.empty((nEF, self.ncomp), dtype=torch.float64, device=dev)
             data_K.ctx.finalize_tetra(hist, cf / data_K.cell_volume / data_K.nk, out)
             return out
 
         return self._slices(data_K, run)
 
-    def _call_k_resolved(self, data_K):
+    def _call_k_resolved(self, data_K, bands=None):
         """Per k-point results [nk, nEF, 3^ndim] (static.py:60-101,156-164): each k its own
         histogram, / cell volume (no / nk), x factor."""
         torch = data_K.torch
@@ -219,28 +224,34 @@ class StaticCalculator(Calculator):
             p = self.scan_params(data_K.system)
             p.comp_slice = sl
             hist = torch.zeros((nk, self.nEF_extra, self.ncomp), dtype=torch.float64, device=dev)
-            data_K.static_scan_k(p, hist, 1.0)
+            data_K.static_scan_k(p, hist, 1.0, bands=bands)
             out = torch.empty((nk, nout, self.ncomp), dtype=torch.float64, device=dev)
             data_K.ctx.finalize_k(hist, self.fder, self.dEF, cf / data_K.cell_volume, out)
             return out
 
         out = self._slices(data_K, run)
-        tr, inv = _PARITY[self.formula]
+        tr, inv = self.parity
         return KBandResult(out.cpu().numpy().reshape((nk, nout) + (3,) * self.ndim), transformTR=tr, transformInv=inv)
+
+    def band_values(self, data_K):
+        """Per-band values [nk, nw, 3^ndim] (float64, device) for WBMI_FORMULA_BANDVALUES
+        calculators, formed by the calculator before the scan; None for the others."""
+        return None
 
     def __call__(self, data_K):
         torch = data_K.torch
         data_K.ensure(self.needed_channels(data_K.system))
+        bands = self.band_values(data_K)
         if self.k_resolved and not self.tetra:
-            return self._call_k_resolved(data_K)
+            return self._call_k_resolved(data_K, bands)
         if self.tetra:
-            out = self._call_tetra(data_K)
+            out = self._call_tetra(data_K, bands)
             if self.k_resolved:
-                tr, inv = _PARITY[self.formula]
+                tr, inv = self.parity
                 return KBandResult(out.cpu().numpy().reshape((data_K.nk, self.Efermi.shape[0]) + (3,) * self.ndim),
                                    transformTR=tr, transformInv=inv)
             data = out.reshape((self.Efermi.shape[0],) + (3,) * self.ndim)
-            tr, inv = _PARITY[self.formula]
+            tr, inv = self.parity
             return EnergyResult(self.Efermi, data, smoothers=[self.smoother], transformTR=tr, transformInv=inv,
                                 comment=self.comment, save_mode=self.save_mode)
         dev = data_K.ctx.device
@@ -252,16 +263,53 @@ class StaticCalculator(Calculator):
             p = self.scan_params(data_K.system)
             p.comp_slice = sl
             hist = torch.zeros((self.nEF_extra, self.ncomp), dtype=torch.float64, device=dev)
-            data_K.static_scan(p, hist, 1.0)
+            data_K.static_scan(p, hist, 1.0, bands=bands)
             out = torch.empty((nout, self.ncomp), dtype=torch.float64, device=dev)
             data_K.ctx.finalize(hist, self.fder, self.dEF, scale, out)
             return out
 
         out = self._slices(data_K, run)
         data = out.reshape((nout,) + (3,) * self.ndim)
-        tr, inv = _PARITY[self.formula]
+        tr, inv = self.parity
         return EnergyResult(self.Efermi, data, smoothers=[self.smoother], transformTR=tr, transformInv=inv,
                             comment=self.comment, save_mode=self.save_mode)
+
+
+SPIN_CURRENT_KINDS = {"ryoo": _lib.WBMI_SHC_RYOO, "qiao": _lib.WBMI_SHC_QIAO, "simple": _lib.WBMI_SHC_SIMPLE}
+
+
+def spin_current_channels(kind, ext):
+    """Channels of the spin current J of `kind` (formula/covariant.py:703-756) and of the velocity
+    and Berry connection next to it."""
+    need = {("Ham", 0), ("Ham", 1), ("SS", 0)} | ({("AA", 0)} if ext else set())
+    if kind == "ryoo":
+        need |= {("SA", 0), ("SHA", 0)}
+    elif kind == "qiao":
+        need |= {("SH", 0), ("SR", 0), ("SHR", 0)}
+    return need
+
+
+def spin_current_params(data_K, kind, ext, degen_thresh, degen_Kramers):
+    """wbmi_shc_params for the spin Berry curvature of the Data_K's channels (no Fermi levels)."""
+    if kind not in SPIN_CURRENT_KINDS:
+        raise ValueError(f"spin_current_type must be `qiao` or `ryoo` or `simple`, not {kind}")
+    if not ext and kind != "simple":
+        raise NotImplementedError(f"the {kind} spin current needs the external terms")
+    p = _lib.ShcParams()
+    p.base.degen_thresh = float(degen_thresh)
+    p.base.degen_Kramers = int(bool(degen_Kramers))
+    p.base.external_terms = int(ext)
+    for i, v in enumerate(data_K.role_table()):
+        p.base.ch[i] = v
+    sl = data_K._slots
+    p.kind = SPIN_CURRENT_KINDS[kind]
+    for c in range(3):
+        p.ch_S[c] = sl[("SS", 0)] + c
+        p.ch_SH[c] = sl[("SH", 0)] + c if ("SH", 0) in sl else -1
+    for c in range(9):
+        for name, arr in (("SA", p.ch_SA), ("SHA", p.ch_SHA), ("SR", p.ch_SR), ("SHR", p.ch_SHR)):
+            arr[c] = sl[(name, 0)] + c if (name, 0) in sl else -1
+    return p
 
 
 # -- implemented calculators (calculators/static.py:192-498) --------------------
@@ -328,6 +376,42 @@ class Ohmic_FermiSurf(StaticCalculator):
 
     def __init__(self, constant_factor=factors.factor_ohmic, **kwargs):
         super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class SHC(StaticCalculator):
+    r"""Spin Hall conductivity, dc (:math:`S/m`), Fermi sea (static.py:647-660)
+
+        | Output: :math:`\sigma_{\alpha\beta}^s = -e^2/\hbar \int [dk] Im(j_{nm,\alpha} v_{mn,\beta})/(\epsilon_n - \epsilon_m)^2 f`
+        | kwargs_formula={'spin_current_type': 'ryoo' | 'qiao' | 'simple'}
+
+    The spin Berry curvature of every band (wbmi_spin_berry_bands: the spin current, then each
+    band's share of its group's trace) is binned by the Fermi scan as caller values
+    (WBMI_FORMULA_BANDVALUES), 9 components per launch."""
+    formula = _lib.FORMULA_BANDVALUES
+    ndim = 3
+    fder = 0
+
+    def __init__(self, constant_factor=-factors.factor_ahc / 2, kwargs_formula=None, **kwargs):
+        kf = dict(kwargs_formula or {})
+        self.spin_current_type = kf.pop("spin_current_type", "ryoo")
+        if self.spin_current_type not in SPIN_CURRENT_KINDS:
+            raise ValueError(f"spin_current_type must be `qiao` or `ryoo` or `simple`, not {self.spin_current_type}")
+        super().__init__(constant_factor=constant_factor, kwargs_formula=kf, **kwargs)
+
+    @property
+    def parity(self):
+        return (1, 1)   # SpinOmega: transform_ident for both (covariant.py:770-771)
+
+    def needed_channels(self, system):
+        return spin_current_channels(self.spin_current_type, _external(self.kwargs_formula, system))
+
+    def band_values(self, data_K):
+        ext = _external(self.kwargs_formula, data_K.system)
+        p = spin_current_params(data_K, self.spin_current_type, ext, self.degen_thresh, self.degen_Kramers)
+        out = data_K.torch.empty((data_K.nk, data_K.num_wann, 27), dtype=data_K.torch.float64,
+                                 device=data_K.ctx.device)
+        data_K.ctx.tabulate_spin_berry(data_K.E_dev, data_K.Xk, data_K.prec, p, out, per_band=True)
+        return out
 
 
 class Hall_classic_FermiSurf(StaticCalculator):
@@ -540,4 +624,5 @@ class GME_spin_FermiSurf(StaticCalculator):
 __all__ = ["Calculator", "StaticCalculator", "DOS", "CumDOS", "AHC", "BerryDipole_FermiSurf", "NLAHC_FermiSurf",
            "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "GME_orb_FermiSurf", "GME_orb_FermiSea",
            "GME_spin_FermiSea", "GME_spin_FermiSurf", "Spin", "Ohmic_FermiSea", "Ohmic_FermiSurf",
-           "Hall_classic_FermiSurf", "Hall_classic_FermiSea", "NLDrude_FermiSurf", "NLDrude_Fermider2", "VoidResult"]
+           "Hall_classic_FermiSurf", "Hall_classic_FermiSea", "NLDrude_FermiSurf", "NLDrude_Fermider2", "SHC",
+           "VoidResult"]

@@ -10,7 +10,7 @@ import numpy as np
 from .. import _lib
 from .. import factors
 from ..result import KBandResult, TABresult
-from .static import Calculator, StaticCalculator, _PARITY, _external
+from .static import Calculator, StaticCalculator, _PARITY, _external, spin_current_channels, spin_current_params
 
 
 class Tabulator(Calculator):
@@ -157,35 +157,13 @@ class SpinBerry(Tabulator):
         super().__init__(ibands=ibands, kwargs_formula=kf, **kwargs)
 
     def needed_channels(self, system):
-        ext = _external(self.kwargs_formula, system)
-        need = {("Ham", 0), ("Ham", 1), ("SS", 0)} | ({("AA", 0)} if ext else set())
-        if self.spin_current_type == "ryoo":
-            need |= {("SA", 0), ("SHA", 0)}
-        elif self.spin_current_type == "qiao":
-            need |= {("SH", 0), ("SR", 0), ("SHR", 0)}
-        return need
+        return spin_current_channels(self.spin_current_type, _external(self.kwargs_formula, system))
 
     def __call__(self, data_K):
         torch = data_K.torch
-        ext = _external(self.kwargs_formula, data_K.system)
-        if not ext and self.spin_current_type != "simple":
-            raise NotImplementedError(f"the {self.spin_current_type} spin current needs the external terms")
         data_K.ensure(self.needed_channels(data_K.system))
-        p = _lib.ShcParams()
-        p.base.degen_thresh = float(self.degen_thresh)
-        p.base.degen_Kramers = int(bool(self.degen_Kramers))
-        p.base.external_terms = int(ext)
-        for i, v in enumerate(data_K.role_table()):
-            p.base.ch[i] = v
-        sl = data_K._slots
-        p.kind = {"ryoo": _lib.WBMI_SHC_RYOO, "qiao": _lib.WBMI_SHC_QIAO,
-                  "simple": _lib.WBMI_SHC_SIMPLE}[self.spin_current_type]
-        for c in range(3):
-            p.ch_S[c] = sl[("SS", 0)] + c
-            p.ch_SH[c] = sl[("SH", 0)] + c if ("SH", 0) in sl else -1
-        for c in range(9):
-            for name, arr in (("SA", p.ch_SA), ("SHA", p.ch_SHA), ("SR", p.ch_SR), ("SHR", p.ch_SHR)):
-                arr[c] = sl[(name, 0)] + c if (name, 0) in sl else -1
+        ext = _external(self.kwargs_formula, data_K.system)
+        p = spin_current_params(data_K, self.spin_current_type, ext, self.degen_thresh, self.degen_Kramers)
         out = torch.empty((data_K.nk, data_K.num_wann, 27), dtype=torch.float64, device=data_K.ctx.device)
         data_K.ctx.tabulate_spin_berry(data_K.E_dev, data_K.Xk, data_K.prec, p, out)
         ib = np.arange(data_K.num_wann) if self.ibands is None else self.ibands

@@ -716,14 +716,15 @@ extern "C" int wbmi_spin_hall(wbmi_ctx* ctx, int nk, int nw, int nch, const doub
 // Spin Berry curvature per band (tabulate.py SpinBerry, formula/covariant.py:759-789):
 //   c_m^{abs} = -2 Im sum_{l outside m's group} J^{as}_ml / (E_m - E_l) (D^b_lm - i Abar^b_lm),
 // D^b_lm = -V^b_lm / (E_l - E_m) (both inverses 0 below 1e-7; Abar only with external terms),
-// out[k][n][a*9+b*3+s] = the mean of c over n's group.  One workgroup per k: W = D - i Abar in
-// LDS, one thread per (m, abs) for c, then the group means.  J from k_spin_current.
+// out[k][n][a*9+b*3+s] = the mean of c over n's group (per_band: c_n itself, the band's share
+// of its group's trace).  One workgroup per k: W = D - i Abar in LDS, one thread per (m, abs)
+// for c, then the group means.  J from k_spin_current.
 template <typename T>
 __global__ void __launch_bounds__(OC_THREADS) k_tab_spin_berry(int nk, int nw, int nch, wbmi_optcond_params p,
                                                                const double* __restrict__ Eall,
                                                                const cplx<T>* __restrict__ Xall,
                                                                const cplx<double>* __restrict__ J,
-                                                               double* __restrict__ out) {
+                                                               double* __restrict__ out, int per_band) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int k = blockIdx.x, tid = threadIdx.x, nw2 = nw * nw;
     cplx<double>* sW = (cplx<double>*)smem;  // [3][nw][nw]
@@ -775,7 +776,9 @@ __global__ void __launch_bounds__(OC_THREADS) k_tab_spin_berry(int nk, int nw, i
     for (int t = tid; t < nw * 27; t += OC_THREADS) {
         const int n = t / 27, r = t % 27, g = sG[n];
         double s = 0.0;
-        if (g >= 0) {
+        if (per_band) {
+            s = sC[t];
+        } else if (g >= 0) {
             for (int m = sGb[g]; m < sGb[g + 1]; ++m) s += sC[m * 27 + r];
             s /= sGb[g + 1] - sGb[g];
         }
@@ -785,21 +788,20 @@ __global__ void __launch_bounds__(OC_THREADS) k_tab_spin_berry(int nk, int nw, i
 
 template <typename T>
 static int launch_spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E, const void* X,
-                             const wbmi_shc_params* params, cplx<double>* J, double* out) {
+                             const wbmi_shc_params* params, cplx<double>* J, double* out, int per_band) {
     hipLaunchKernelGGL(k_spin_current<T>, dim3(nk, 9), dim3(256), 0, ctx->stream, nk, nw, nch, *params, E,
                        (const cplx<T>*)X, J);
     WBMI_LAUNCH_CHECK(ctx);
     const size_t lds = sizeof(cplx<double>) * 3 * nw * nw + sizeof(double) * (27 * nw + 2 * OC_NWMAX) +
                        sizeof(int) * (2 * OC_NWMAX + 1);
     hipLaunchKernelGGL(k_tab_spin_berry<T>, dim3(nk), dim3(OC_THREADS), lds, ctx->stream, nk, nw, nch, params->base,
-                       E, (const cplx<T>*)X, (const cplx<double>*)J, out);
+                       E, (const cplx<T>*)X, (const cplx<double>*)J, out, per_band);
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
 }
 
-extern "C" int wbmi_tabulate_spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
-                                        const void* Xbar_dev, int prec, const wbmi_shc_params* params,
-                                        void* work_dev, size_t work_bytes, double* out_dev) {
+static int spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev, int prec,
+                      const wbmi_shc_params* params, void* work_dev, size_t work_bytes, double* out_dev, int per_band) {
     if (!ctx) return WBMI_ERR_ARG;
     if (nk <= 0 || nw <= 0 || nch <= 0 || !E_dev || !Xbar_dev || !params || !out_dev)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: bad arguments");
@@ -826,8 +828,22 @@ extern "C" int wbmi_tabulate_spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, 
             return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: missing SH, SR or SHR channel");
     }
     if (prec == WBMI_F64)
-        return launch_spin_berry<double>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev);
+        return launch_spin_berry<double>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev,
+                                         per_band);
     if (prec == WBMI_F32)
-        return launch_spin_berry<float>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev);
+        return launch_spin_berry<float>(ctx, nk, nw, nch, E_dev, Xbar_dev, params, (cplx<double>*)work_dev, out_dev,
+                                        per_band);
     return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: bad precision");
+}
+
+extern "C" int wbmi_tabulate_spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                                        const void* Xbar_dev, int prec, const wbmi_shc_params* params,
+                                        void* work_dev, size_t work_bytes, double* out_dev) {
+    return spin_berry(ctx, nk, nw, nch, E_dev, Xbar_dev, prec, params, work_dev, work_bytes, out_dev, 0);
+}
+
+extern "C" int wbmi_spin_berry_bands(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                                     const void* Xbar_dev, int prec, const wbmi_shc_params* params, void* work_dev,
+                                     size_t work_bytes, double* out_dev) {
+    return spin_berry(ctx, nk, nw, nch, E_dev, Xbar_dev, prec, params, work_dev, work_bytes, out_dev, 1);
 }

@@ -60,6 +60,11 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
         case WBMI_FORMULA_HALL_SEA:
             if ((rc = need(WBMI_ROLE_V, 3)) || (rc = need(WBMI_ROLE_W, 9))) return rc;
             break;
+        case WBMI_FORMULA_BANDVALUES:
+            if (!((nch == 9 && p->comp_slice == 0) || (nch == 27 && p->comp_slice >= 0 && p->comp_slice <= 2)))
+                return wbmi_fail(ctx, WBMI_ERR_ARG,
+                                 "wbmi_static_scan: band values need nch 9 (comp_slice 0) or 27 (comp_slice 0..2)");
+            break;
         case WBMI_FORMULA_MORB_HPM:
         case WBMI_FORMULA_VELHPLUS:
             if ((rc = need(WBMI_ROLE_V, 3))) return rc;

@@ -196,6 +196,8 @@ struct KCtx {
     const int* ge;
     const cplx<TS>* sV;  // LDS copies of V and A (ST), element type TS
     const cplx<TS>* sA;
+    const double* bv;    // WBMI_FORMULA_BANDVALUES: this k's caller values [nw][nbv]
+    int nbv;
     __device__ inline cplx<double> get(int role, int x, int y) const {
         cplx<T> v = X[((int64_t)ch[role] * n_() + x) * n_() + y];
         return mk((double)v.re, (double)v.im);
@@ -966,6 +968,12 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
             });
             S.acc[t] = mk(v.re, 0.0);
         }
+    } else if (F == WBMI_FORMULA_BANDVALUES) {
+        // the caller's per-band shares of the group traces, 9 of them from row comp_slice * 9
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
+            const int n = t / 9, c = t % 9;
+            if (S.act[n]) S.acc[t] = mk(K.bv[n * K.nbv + p.comp_slice * 9 + c], 0.0);
+        }
     } else if (F == WBMI_FORMULA_DEROMEGA) {
         eval_deromega(S, K, in, ex, false, 0);
     } else if (F == WBMI_FORMULA_DERMORB) {
@@ -1235,7 +1243,8 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
                               F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERSPIN ||
                               F == WBMI_FORMULA_VELSPIN || F == WBMI_FORMULA_SPIN || F == WBMI_FORMULA_INVMASS ||
                               F == WBMI_FORMULA_VELVEL || F == WBMI_FORMULA_VELVELVEL || F == WBMI_FORMULA_MASSVEL ||
-                              F == WBMI_FORMULA_HALL_SURF || F == WBMI_FORMULA_HALL_SEA;
+                              F == WBMI_FORMULA_HALL_SURF || F == WBMI_FORMULA_HALL_SEA ||
+                              F == WBMI_FORMULA_BANDVALUES;
 
     // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186), formed by wave 0: each lane
     // owns bands lane and lane + 64 (nw <= 128); borders from two ballots and popcounts, one
@@ -1338,6 +1347,8 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
     K.ge = S.ge;
     K.sV = (const cplx<TS>*)S.V;
     K.sA = (const cplx<TS>*)S.A;
+    K.bv = (const double*)(const void*)Xall + k * (int64_t)nw * args.nch;
+    K.nbv = args.nch;
 
     if (additive || tab) {
         // main partition: every group at once (each group's trace with inn = the group,
@@ -1506,6 +1517,7 @@ static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
         case WBMI_FORMULA_MASSVEL: return launch_scan<T, WBMI_FORMULA_MASSVEL, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_HALL_SURF: return launch_scan<T, WBMI_FORMULA_HALL_SURF, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_HALL_SEA: return launch_scan<T, WBMI_FORMULA_HALL_SEA, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_BANDVALUES: return launch_scan<T, WBMI_FORMULA_BANDVALUES, MODE>(ctx, nk, a, E, X, hist);
         default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
     }
 }

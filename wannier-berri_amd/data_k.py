@@ -399,19 +399,26 @@ class Data_K:
             params.ch[i] = v
         self.ctx.tabulate(self.E_dev, self.Xk, self.prec, params, out)
 
-    def static_scan_tetra_k(self, params, Efermi, hist, weight=1.0):
+    def _scan_input(self, params, bands):
+        """(X, prec, nch) of a scan: the channel stack, or per-band values [nk, nw, nvals]
+        (float64) for WBMI_FORMULA_BANDVALUES."""
+        for i, v in enumerate(self.role_table()):
+            params.ch[i] = v
+        if bands is None:
+            return self.Xk, self.prec, None
+        return bands, WBMI_F64, int(bands.shape[2])
+
+    def static_scan_tetra_k(self, params, Efermi, hist, weight=1.0, bands=None):
         """k-resolved tetrahedron-weighted scan into hist [nk, 2 nEF, ncomp]."""
         Ec = self.corner_energies()
-        for i, v in enumerate(self.role_table()):
-            params.ch[i] = v
-        self.ctx.static_scan_tetra_k(self.E_dev, Ec, self.Xk, self.prec, params, Efermi, weight, hist)
+        X, prec, nch = self._scan_input(params, bands)
+        self.ctx.static_scan_tetra_k(self.E_dev, Ec, X, prec, params, Efermi, weight, hist, nch=nch)
 
-    def static_scan_tetra(self, params, Efermi, hist, weight=1.0):
+    def static_scan_tetra(self, params, Efermi, hist, weight=1.0, bands=None):
         """Tetrahedron-weighted scan into hist [2 nEF, ncomp] (see wbmi_static_scan_tetra)."""
         Ec = self.corner_energies()
-        for i, v in enumerate(self.role_table()):
-            params.ch[i] = v
-        self.ctx.static_scan_tetra(self.E_dev, Ec, self.Xk, self.prec, params, Efermi, weight, hist)
+        X, prec, nch = self._scan_input(params, bands)
+        self.ctx.static_scan_tetra(self.E_dev, Ec, X, prec, params, Efermi, weight, hist, nch=nch)
 
     def role_table(self):
         """Channel slot per formula role (include/wbmi.h WBMI_ROLE_*), -1 if absent."""
@@ -436,17 +443,15 @@ class Data_K:
         put(_lib.ROLE_SD, ("SS", 1), 9)
         return ch
 
-    def static_scan(self, params, hist, weight=1.0):
+    def static_scan(self, params, hist, weight=1.0, bands=None):
         """Accumulate weight * (group traces) into hist [nEF_extra, ncomp] (FP64, device)."""
-        for i, v in enumerate(self.role_table()):
-            params.ch[i] = v
-        self.ctx.static_scan(self.E_dev, self.Xk, self.prec, params, weight, hist)
+        X, prec, nch = self._scan_input(params, bands)
+        self.ctx.static_scan(self.E_dev, X, prec, params, weight, hist, nch=nch)
 
-    def static_scan_k(self, params, hist, weight=1.0):
+    def static_scan_k(self, params, hist, weight=1.0, bands=None):
         """k-resolved scan into hist [nk, nEF_extra, ncomp] (FP64, device)."""
-        for i, v in enumerate(self.role_table()):
-            params.ch[i] = v
-        self.ctx.static_scan_k(self.E_dev, self.Xk, self.prec, params, weight, hist)
+        X, prec, nch = self._scan_input(params, bands)
+        self.ctx.static_scan_k(self.E_dev, X, prec, params, weight, hist, nch=nch)
 
     def subset(self, k0, k1):
         """The k-points [k0, k1) of an evaluated k-list block as a Data_K of their own.
