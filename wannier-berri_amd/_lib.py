@@ -10,7 +10,8 @@ import threading
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libwbmi.so")
+# WBMI_LIB: another build of the library (e.g. one compiled with different tuning macros)
+LIB_PATH = os.environ.get("WBMI_LIB") or os.path.join(HERE, "libwbmi.so")
 
 WBMI_OK = 0
 WBMI_ERR_ARG = 1

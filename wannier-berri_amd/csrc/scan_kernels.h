@@ -50,6 +50,9 @@
 #define SC_TASK_THREADS 768       // DerOmega / DerMorb: 12 waves (3 per SIMD), one workgroup per CU
 #endif
 #define SC_NWMAX 128
+#ifndef SC_STAGE_A
+#define SC_STAGE_A 1  // A staged in LDS next to V (1) or read from the channels (0)
+#endif
 #define SC_M_NW 64    // dEinv o Off lives in LDS up to this nw, else it is formed on the fly
 #define SC_LDS_NW 32  // V and A live in LDS up to this nw
 
@@ -689,9 +692,12 @@ __device__ __attribute__((always_inline)) void formula_task(const Smem& S, const
 // DerMorb = DerMorb_H + sign [Eav o dOmega + (Omega V + V Omega)/2] on cut
 // partitions; its trace terms (s1 ... s7, the Omega block) are listed at
 // formula_task.
-template <typename T, bool LM, bool ST, bool SA, int NW, typename TS>
+// DM is compile-time so that the DerOmega kernel carries only the kinds it runs (0 - 2): its
+// register allocation and code size are not those of the DerMorb kinds.
+template <bool DM, typename T, bool LM, bool ST, bool SA, int NW, typename TS>
 __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, const KCtx<T, LM, ST, SA, NW, TS>& K, bool in, bool ex,
-                                                            bool dm, int sign) {
+                                                            int sign) {
+    constexpr bool dm = DM;
     const int nw = K.n_(), tid = threadIdx.x;
     // per-band terms, lanes running over (n, c, d)
     for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
@@ -786,11 +792,17 @@ __device__ __attribute__((always_inline)) void eval_deromega(const Smem& S, cons
                 break;
             case 1: formula_task<1, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
             case 2: formula_task<2, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
-            case 3: formula_task<3, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
-            case 4: formula_task<4, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
-            case 5: formula_task<5, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
-            case 6: formula_task<6, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
-            default: formula_task<7, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+            default:
+                if constexpr (DM) {
+                    switch (kind) {
+                        case 3: formula_task<3, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+                        case 4: formula_task<4, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+                        case 5: formula_task<5, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+                        case 6: formula_task<6, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+                        default: formula_task<7, false>(S, K, tcx, S.acc, b, d, rh, cb); break;
+                    }
+                }
+                break;
         }
     }
 }
@@ -975,9 +987,9 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
             if (S.act[n]) S.acc[t] = mk(K.bv[n * K.nbv + p.comp_slice * 9 + c], 0.0);
         }
     } else if (F == WBMI_FORMULA_DEROMEGA) {
-        eval_deromega(S, K, in, ex, false, 0);
+        eval_deromega<false>(S, K, in, ex, 0);
     } else if (F == WBMI_FORMULA_DERMORB) {
-        eval_deromega(S, K, in, ex, true, p.sign);
+        eval_deromega<true>(S, K, in, ex, p.sign);
     }
     __syncthreads();
 }
@@ -1157,9 +1169,18 @@ __host__ __device__ constexpr int scan_threads() {
     return SC_TASK_THREADS;
 }
 
+#ifndef SC_TASK_MINW
+#define SC_TASK_MINW 1  // minimum waves per SIMD asked of the DerOmega / DerMorb Fermi-bin kernels
+#endif
+template <typename T, int F, int MODE>
+__host__ __device__ constexpr int scan_min_waves() {
+    constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
+    return tasks && MODE != 1 ? SC_TASK_MINW : 1;
+}
+
 // NW > 0: the kernel for nw = NW only, every size and row stride a compile-time constant
 template <typename T, int F, bool LM, bool ST, int MODE, int NW>
-__global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
+__global__ void __launch_bounds__((scan_threads<T, F, MODE>()), (scan_min_waves<T, F, MODE>())) k_static_scan(ScanArgs args, const double* __restrict__ Eall,
                                                             const cplx<T>* __restrict__ Xall,
                                                             double* __restrict__ hist) {
     extern __shared__ __attribute__((aligned(16))) char smem_raw[];
@@ -1171,7 +1192,7 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
     constexpr bool tasks = F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_DERMORB;
     static_assert(tasks || F == WBMI_FORMULA_OMEGA || !ST, "V and A are staged for DerOmega, DerMorb and Omega");
     using TS = typename scan_stage<T, F>::type;
-    constexpr bool SA = ST;
+    constexpr bool SA = ST && SC_STAGE_A;
     constexpr bool stage_v = ST;
     const bool stage_a = SA && p.external_terms;
     constexpr bool tetra = MODE == 1;  // compile-time: the Fermi-bin kernels carry no tetrahedron code
@@ -1467,7 +1488,7 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>())) k_static_scan(Sc
 
 template <typename T, int F, bool LM, bool ST, int MODE, int NW = 0>
 static int launch_scan_t(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double* E, const void* X, double* hist) {
-    const bool stage_a = ST && a.p.external_terms;
+    const bool stage_a = ST && SC_STAGE_A && a.p.external_terms;
     // DerOmega / DerMorb: 12 waves (3 per SIMD at <= 168 VGPRs), one workgroup per CU (at nw = 32
     // LDS holds V and A, 118 KB as complex128)
     const int threads = scan_threads<T, F, MODE>();
