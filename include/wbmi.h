@@ -413,6 +413,17 @@ int wbmi_tabulate_invmass(wbmi_ctx* ctx, int nk, int nw, int nch, const double* 
 int wbmi_tabulate_spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
                              int prec, const wbmi_shc_params* params, void* work_dev, size_t work_bytes,
                              double* out_dev);
+/* Third k-derivative of the band energies (tabulate.py Der3E, formula/covariant.py:126-150 with
+ * InvMass, DerDcov, DerWln of formula/elementary.py:28-70), nw <= 32:
+ *   mode 0: out[k][n][a*9+b*3+c] = group mean of Re Der3E.nn[m, m, a, b, c] (the tabulator),
+ *   mode 1: the band's own Re Der3E.nn[n, n, a, b, c], its share of the group's trace (input of
+ *           NLDrude_FermiSea, calculators/static.py:519-529, binned with WBMI_FORMULA_BANDVALUES).
+ *   chV3_host, chW9_host, chW27_host : int32 channel slots of Xbar('Ham', 1), ('Ham', 2), ('Ham', 3)
+ *   out_dev : float64 [nk, nw, 27]                                                           */
+int wbmi_der3e_bands(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev, int prec,
+                     const int32_t* chV3_host, const int32_t* chW9_host, const int32_t* chW27_host,
+                     double degen_thresh, int degen_Kramers, int mode, double* out_dev);
+
 /* The same per band, before the group mean: out[k][n][abs] = c_n^{abs}, the band's share of its
  * group's trace (the trace of a group is the sum over its bands).  Input of the static spin Hall
  * conductivity (calculators/static.py:647-660), which bins these with WBMI_FORMULA_BANDVALUES. */

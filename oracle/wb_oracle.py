@@ -664,6 +664,34 @@ class InvMass(Matrix_GenDer_ln):
         super().__init__(data_K.covariant('Ham', commader=1), data_K.covariant('Ham', commader=2), data_K.Dcov)
 
 
+class DerWln(Matrix_GenDer_ln):
+    # formula/elementary.py:37-43: generalised derivative of W = Xbar('Ham', 2) with Xbar('Ham', 3)
+    def __init__(self, data_K):
+        super().__init__(data_K.covariant('Ham', commader=2), data_K.covariant('Ham', commader=3), data_K.Dcov)
+
+
+class Der3E(Formula_ln):
+    """Third k-derivative of the band energy (formula/covariant.py:126-150):
+    dW.nn + dV.nl D.ln + V.nl dD.ln - dD.nl V.ln - D.nl dV.ln, indices [a, b, c]."""
+
+    def __init__(self, data_K, **kw):
+        self._set_terms(data_K, **kw)
+        self.V = data_K.covariant('Ham', commader=1)
+        self.D = data_K.Dcov
+        self.dV = InvMass(data_K)
+        self.dD = DerDcov(data_K)
+        self.dW = DerWln(data_K)
+        self.ndim = 3
+
+    def nn(self, ik, inn, out):
+        s = self.dW.nn(ik, inn, out).astype(complex)
+        s = s + np.einsum("mlac,lnb->mnabc", self.dV.nl(ik, inn, out), self.D.ln(ik, inn, out))
+        s = s + np.einsum("mla,lnbc->mnabc", self.V.nl(ik, inn, out), self.dD.ln(ik, inn, out))
+        s = s - np.einsum("mlbc,lna->mnabc", self.dD.nl(ik, inn, out), self.V.ln(ik, inn, out))
+        s = s - np.einsum("mlb,lnac->mnabc", self.D.nl(ik, inn, out), self.dV.ln(ik, inn, out))
+        return s
+
+
 class Identity(Formula_ln):
     # formula/covariant.py:10-22
     def __init__(self, data_K=None, **kw):
@@ -1164,6 +1192,13 @@ class SHC_static(StaticCalculator):
     Formula = staticmethod(lambda data_K, **kw: SpinOmega(data_K, **kw))   # defined below
     fder = 0
     constant_factor_default = -FACTOR_AHC / 2
+
+
+class NLDrude_FermiSea(StaticCalculator):
+    # static.py:519-529
+    Formula = Der3E
+    fder = 0
+    constant_factor_default = FACTOR_NLDRUDE
 
 
 class NLDrude_FermiSurf(StaticCalculator):

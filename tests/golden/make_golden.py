@@ -66,8 +66,8 @@ def main():
     system("reference/systems/GaAs_W90", "sys_gaas_w90.npz", ["Ham", "AA"], legacy=True)
     system("reference/systems/Fe_W90", "sys_fe_w90.npz", ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SR", "SH", "SHR"],
            legacy=True)
-    system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz", ["Ham", "AA", "BB", "CC", "SS"],
-           legacy=False)
+    system("reference/systems/system_Fe_sym_W90_OSD", "sys_fe_sym_osd.npz",
+           ["Ham", "AA", "BB", "CC", "SS", "SA", "SHA", "SH", "SR", "SHR"], legacy=False)
     # Fe_sym (Wannier90, projections) after set_spin_interlaced (tests/common_systems.py:276-279):
     # Ham and AA of Fe_W90_proj and the SS that set_spin_interlaced wrote
     system("reference/systems/Fe_W90_proj_set_spin", "sys_fe_w90_proj_set_spin.npz", ["Ham", "AA", "SS"], legacy=True)
@@ -145,6 +145,9 @@ def main():
              "chiral_hall_classic_fsurf": I + "Chiral-Hall_classic_fsurf_iter-0000.npz",
              "chiral_tetra_hall_classic": I + "Chiral_tetra-Hall_classic_iter-0000.npz",
              "chiral_tetra_hall_classic_fsurf": I + "Chiral_tetra-Hall_classic_fsurf_iter-0000.npz",
+             # nonlinear Drude, Fermi sea (Der3E), Chiral (also tetrahedra)
+             "chiral_der3e": I + "Chiral-Der3E_iter-0000.npz",
+             "chiral_tetra_der3e": I + "Chiral_tetra-Der3E_iter-0000.npz",
              }, "results_run.npz")
     # FermiDiracSmoother(Efermi, T_Kelvin=1200, maxdE=8) on the Chiral AHC and Fermi-sea BCD
     # (tests/test_run.py:181-193): the .dat files hold the data and the smoothed data
@@ -165,7 +168,8 @@ def main():
     # per-formula gauge-invariant traces (tests/reference/formula/*.npz)
     forms = {}
     for name in ["Dcov", "DerDcov", "DEinv_ln", "InvMass", "Omega", "DerOmega", "Morb_H", "morb",
-                 "DerMorb_upper", "Dermorb_lower", "VelOmega", "Hamiltonian", "Velocity", "Spin", "VelSpin"]:
+                 "DerMorb_upper", "Dermorb_lower", "VelOmega", "Hamiltonian", "Velocity", "Spin", "VelSpin",
+                 "DerWln", "Der3E", "SpinOmega", "VelVel", "VelVelVel", "VelMassVel"]:
         with np.load(os.path.join(REF, "reference/formula", name + ".npz"), allow_pickle=False) as f:
             for k in f.files:
                 forms[name + "/" + k] = f[k]

@@ -312,17 +312,19 @@ def test_ohmic(wb, results_run, precision, tol):
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
 def test_hall_nldrude(wb, results_run, precision, tol):
     """Classic Hall conductivity (Fermi surface: VelMassVel, sea: MassMass, contracted as
-    static.py:407-446) against the reference's Chiral fixtures, and the nonlinear Drude
-    conductivity (Fermi surface: MassVel, f'': VelVelVel; 27 components, computed 9 at a time)
-    against the oracle on the same Chiral run (10x10x4 / 5x5x2)."""
+    static.py:407-446) and the Fermi-sea nonlinear Drude conductivity (Der3E per band, binned as
+    caller values) against the reference's Chiral fixtures, and the nonlinear Drude conductivity
+    (Fermi surface: MassVel, f'': VelVelVel; 27 components, computed 9 at a time) against the
+    oracle on the same Chiral run (10x10x4 / 5x5x2)."""
     say(f"test_hall_nldrude {precision}")
     st = wb.calculators.static
     s = gsys(wb, "chiral_left")
     Ef = np.linspace(-5, 8, 27)
     calcs = {"hall_classic_fsurf": st.Hall_classic_FermiSurf(Efermi=Ef), "hall_classic": st.Hall_classic_FermiSea(Efermi=Ef),
-             "nls": st.NLDrude_FermiSurf(Efermi=Ef), "nl2": st.NLDrude_Fermider2(Efermi=Ef)}
+             "nls": st.NLDrude_FermiSurf(Efermi=Ef), "nl2": st.NLDrude_Fermider2(Efermi=Ef),
+             "der3e": st.NLDrude_FermiSea(Efermi=Ef)}
     r = run_gpu(wb, s, calcs, [10, 10, 4], [5, 5, 2], precision=precision)
-    for k in ("hall_classic_fsurf", "hall_classic"):
+    for k in ("hall_classic_fsurf", "hall_classic", "der3e"):
         e = rel_err(r.results[k].data, results_run["chiral_" + k])
         say(f"  {k}: {e:.2e}")
         assert e < tol, k
@@ -990,29 +992,32 @@ def test_chiral_k_resolved(wb, use_sym, tetra):
 
 
 def test_fe_tabulate_invmass(wb):
-    """InvMass tabulator (generalised derivative of the velocity, elementary.py:28-33) and DerSpin
-    tabulator (generalised derivative of the spin, covariant.py:338-342) on Fe (Wannier90),
-    grid 4^3 / 2^3, vs the oracle's Tabulator with InvMass (pinned by the reference's formula
-    fixture InvMass.npz) and DerSpin (pinned through the reference's GME_spin results)."""
+    """InvMass tabulator (generalised derivative of the velocity, elementary.py:28-33), DerSpin
+    tabulator (generalised derivative of the spin, covariant.py:338-342) and Der3E tabulator
+    (covariant.py:126-150) on Fe (Wannier90), grid 4^3 / 2^3, vs the oracle's Tabulator with
+    InvMass and Der3E (pinned by the reference's formula fixtures InvMass.npz, Der3E.npz) and
+    DerSpin (pinned by finite differences of the spin expectation)."""
     say("test_fe_tabulate_invmass")
     tab = wb.calculators.tabulate
     s = gsys(wb, "fe_w90")
     ib = [4, 5, 6, 7]
-    calc = tab.TabulatorAll({"im": tab.InvMass(degen_thresh=5e-2), "dspin": tab.DerSpin(degen_thresh=5e-2)}, ibands=ib)
+    calc = tab.TabulatorAll({"im": tab.InvMass(degen_thresh=5e-2), "dspin": tab.DerSpin(degen_thresh=5e-2),
+                             "d3e": tab.Der3E(degen_thresh=5e-2)}, ibands=ib)
     grid = wb.Grid(system=s, NK=[4, 4, 4], NKFFT=[2, 2, 2])
     r = wb.run(s, grid, {"tab": calc}, use_irred_kpt=False, symmetrize=False, save_files=False, verbose=False)
     o = osys("fe_w90")
     NKdiv, NKFFT, blocks = O.grid_blocks([4, 4, 4], [2, 2, 2])
     NK = NKdiv * NKFFT
-    for key, F in (("im", O.InvMass), ("dspin", O.DerSpin)):
+    for key, F in (("im", O.InvMass), ("dspin", O.DerSpin), ("d3e", O.Der3E)):
         got = r.results["tab"].get_data(key)
-        ref = np.zeros((int(np.prod(NK)), len(ib), 3, 3))
+        rank = 3 if key == "d3e" else 2
+        ref = np.zeros((int(np.prod(NK)), len(ib)) + (3,) * rank)
         for dK, _ in blocks:
             d = O.OracleDataK(o, dK=dK, NKFFT=NKFFT)
             ki = np.rint(d.kpoints_all * NK[None, :]).astype(int) % NK[None, :]
             idx = (ki[:, 0] * NK[1] + ki[:, 1]) * NK[2] + ki[:, 2]
             ref[idx] = O.tabulate(d, F, ib, degen_thresh=5e-2)
-        ref = ref.reshape(tuple(NK) + (len(ib), 3, 3))
+        ref = ref.reshape(tuple(NK) + (len(ib),) + (3,) * rank)
         e = rel_err(got, ref)
         say(f"  {key}: {e:.2e}")
         assert e < 1e-10, key

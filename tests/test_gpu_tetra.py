@@ -61,7 +61,8 @@ def test_chiral_tetra(wb, results_run, precision, tol):
              "ohmic": st.Ohmic_FermiSea(Efermi=Ef, tetra=True),
              "ohmic_fsurf": st.Ohmic_FermiSurf(Efermi=Ef, tetra=True),
              "hall_classic": st.Hall_classic_FermiSea(Efermi=Ef, tetra=True),
-             "hall_classic_fsurf": st.Hall_classic_FermiSurf(Efermi=Ef, tetra=True)}
+             "hall_classic_fsurf": st.Hall_classic_FermiSurf(Efermi=Ef, tetra=True),
+             "der3e": st.NLDrude_FermiSea(Efermi=Ef, tetra=True)}
     grid = wb.Grid(system=s, NK=[10, 10, 4], NKFFT=[5, 5, 2])
     r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, parameters_K={"precision": precision},
                save_files=False, verbose=False)

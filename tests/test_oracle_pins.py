@@ -195,6 +195,14 @@ def test_chiral_hall_classic(results_run):
     assert rel_err(r["sea"].data, results_run["chiral_hall_classic"]) < 1e-8
 
 
+def test_chiral_nldrude_fermi_sea(results_run):
+    """Nonlinear Drude conductivity, Fermi sea (Der3E, static.py:519-529), on Chiral
+    (10x10x4 / 5x5x2) against the reference's Chiral-Der3E fixture."""
+    Ef = np.linspace(-5, 8, 27)
+    r = O.run(gsys("chiral_left"), {"d3": O.NLDrude_FermiSea(Ef)}, NK=[10, 10, 4], NKFFT=[5, 5, 2])
+    assert rel_err(r["d3"].data, results_run["chiral_der3e"]) < 1e-8
+
+
 @pytest.mark.parametrize("fder", [0, 1, 2, 3])
 def test_fe_scan_stencils(results_run, fder):
     s = gsys("fe_w90")
@@ -228,9 +236,13 @@ FORMULAS = {
     # so its ln block is the full off-diagonal block (formula/covariant.py:321-328)
     "Velocity": lambda d: O.Matrix_ln(d.Xbar("Ham", 1)),
     "Spin": O.Spin, "VelSpin": O.VelSpin,
+    "DerWln": O.DerWln, "Der3E": O.Der3E, "VelVel": O.VelVel, "VelVelVel": O.VelVelVel, "VelMassVel": O.VelMassVel,
+    # tests/test_formula.py:175-176: the Ryoo spin current
+    "SpinOmega": lambda d: O.SpinOmega(d, spin_current_type="ryoo"),
 }
 LN_ONLY = {"Dcov", "DerDcov", "DEinv_ln"}
-NN_ONLY = {"Omega", "DerOmega", "Morb_H", "morb", "DerMorb_upper", "Dermorb_lower", "VelOmega", "VelSpin"}
+NN_ONLY = {"Omega", "DerOmega", "Morb_H", "morb", "DerMorb_upper", "Dermorb_lower", "VelOmega", "VelSpin", "Der3E",
+           "VelVel", "VelVelVel", "VelMassVel", "SpinOmega"}
 
 
 @pytest.fixture(scope="module")
@@ -265,7 +277,7 @@ def test_formula_traces(datak_fe_sym, name):
                 val[f"Xnn_ik={ik}"].append(np.einsum("nn...->...", Xnn))
                 val[f"XllXll_ik={ik}"].append(np.einsum("ll...,ll...->...", Xll, Xll))
                 val[f"XnnXnn_ik={ik}"].append(np.einsum("nn...,nn...->...", Xnn, Xnn))
-    der = name.startswith("Der") or name == "VelOmega"
+    der = name.startswith("Der") or name in ("VelOmega", "SpinOmega")
     rtol, atol = (1e-5, 2e-6) if der else (1e-6, 1e-10)
     assert len(ref) == 36
     for k, r in ref.items():

@@ -111,7 +111,8 @@ def test_chiral_tetra(wb, results_run):
              "ohmic": _OracleCalc(wb, O.Ohmic_FermiSea(Ef, tetra=True), 1, 1),
              "ohmic_fsurf": _OracleCalc(wb, O.Ohmic_FermiSurf(Ef, tetra=True), 1, 1),
              "hall_classic": _OracleCalc(wb, O.Hall_classic_FermiSea(Ef, tetra=True), 1, 1),
-             "hall_classic_fsurf": _OracleCalc(wb, O.Hall_classic_FermiSurf(Ef, tetra=True), 1, 1)}
+             "hall_classic_fsurf": _OracleCalc(wb, O.Hall_classic_FermiSurf(Ef, tetra=True), 1, 1),
+             "der3e": _OracleCalc(wb, O.NLDrude_FermiSea(Ef, tetra=True), -1, -1)}
     grid = wb.Grid(system=s, NK=[10, 10, 4], NKFFT=[5, 5, 2])
     r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, data_k_class=_OracleBlock,
                save_files=False, verbose=False)

@@ -55,7 +55,7 @@ SYMBOLS = [
     "wbmi_tabulate", "wbmi_tabulate_diag", "wbmi_shift_current", "wbmi_injection_current",
     "wbmi_static_scan_k", "wbmi_finalize_k", "wbmi_static_scan_tetra_k", "wbmi_finalize_tetra_k",
     "wbmi_tabulate_invmass", "wbmi_eigh_noalloc", "wbmi_eigh_noalloc_max_bytes", "wbmi_r2k_klist_tile",
-    "wbmi_r2k_klist_tile_shape", "wbmi_tabulate_spin_berry", "wbmi_spin_berry_bands",
+    "wbmi_r2k_klist_tile_shape", "wbmi_tabulate_spin_berry", "wbmi_spin_berry_bands", "wbmi_der3e_bands",
 ]
 
 
@@ -145,6 +145,7 @@ _PROTOS = {
                                       _vp]),
     "wbmi_spin_berry_bands": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t,
                                    _vp]),
+    "wbmi_der3e_bands": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _i, _vp]),
 }
 
 
@@ -380,6 +381,15 @@ class Context:
         nch = X.shape[1] if X is not None else 1
         self.check(self._timed(f"tab_f{params.formula}", (nk, nw), lib().wbmi_tabulate, self.h, nk, nw, nch, _ptr(E),
                                _ptr(X), prec, ctypes.byref(params), _ptr(out)), "wbmi_tabulate")
+
+    def der3e_bands(self, E, X, prec, chV, chW, chW3, degen_thresh, degen_Kramers, mode, out):
+        nk, nw = E.shape
+        v_a, v_p = _host_i32(np.asarray(chV, dtype=np.int32))
+        w_a, w_p = _host_i32(np.asarray(chW, dtype=np.int32))
+        w3_a, w3_p = _host_i32(np.asarray(chW3, dtype=np.int32))
+        self.check(self._timed("der3e", (nk, nw), lib().wbmi_der3e_bands, self.h, nk, nw, X.shape[1], _ptr(E), _ptr(X),
+                               prec, v_p, w_p, w3_p, float(degen_thresh), int(bool(degen_Kramers)), int(mode),
+                               _ptr(out)), "wbmi_der3e_bands")
 
     def tabulate_diag(self, E, X, prec, ch3, degen_thresh, degen_Kramers, out):
         nk, nw = E.shape

@@ -440,6 +440,43 @@ class Hall_classic_FermiSea(StaticCalculator):
         super().__init__(constant_factor=constant_factor, **kwargs)
 
 
+def der3e_bands(data_K, degen_thresh, degen_Kramers, mode):
+    """Der3E per band [nk, nw, 27] (wbmi_der3e_bands; mode 0: group means, 1: per-band shares)."""
+    torch = data_K.torch
+    sl = data_K._slots
+    v0, w0, w3 = sl[("Ham", 1)], sl[("Ham", 2)], sl[("Ham", 3)]
+    out = torch.empty((data_K.nk, data_K.num_wann, 27), dtype=torch.float64, device=data_K.ctx.device)
+    data_K.ctx.der3e_bands(data_K.E_dev, data_K.Xk, data_K.prec, [v0 + i for i in range(3)], [w0 + i for i in range(9)],
+                           [w3 + i for i in range(27)], degen_thresh, degen_Kramers, mode, out)
+    return out
+
+
+class NLDrude_FermiSea(StaticCalculator):
+    r"""Nonlinear Drude conductivity (:math:`S^2/A`), Fermi sea (static.py:519-529)
+
+        | Output: :math:`\sigma_{\alpha\beta\gamma} = -e^3/\hbar^2 \tau^2 \int [dk] \partial_{\beta\gamma} v_\alpha f`
+        | Instruction: :math:`j_\alpha = \sigma_{\alpha\beta\gamma} E_\beta E\gamma`
+
+    The third derivative of the band energies (Der3E) per band (wbmi_der3e_bands) is binned by the
+    Fermi scan as caller values (WBMI_FORMULA_BANDVALUES), 9 components per launch."""
+    formula = _lib.FORMULA_BANDVALUES
+    ndim = 3
+    fder = 0
+
+    def __init__(self, constant_factor=factors.factor_nldrude, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+    @property
+    def parity(self):
+        return (-1, -1)   # Der3E: transform_odd for both (covariant.py:135-136)
+
+    def needed_channels(self, system):
+        return {("Ham", 0), ("Ham", 1), ("Ham", 2), ("Ham", 3)}
+
+    def band_values(self, data_K):
+        return der3e_bands(data_K, self.degen_thresh, self.degen_Kramers, 1)
+
+
 class NLDrude_FermiSurf(StaticCalculator):
     r"""Nonlinear Drude conductivity (:math:`S^2/A`), Fermi surface (static.py:532-542)
 
@@ -624,5 +661,5 @@ class GME_spin_FermiSurf(StaticCalculator):
 __all__ = ["Calculator", "StaticCalculator", "DOS", "CumDOS", "AHC", "BerryDipole_FermiSurf", "NLAHC_FermiSurf",
            "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "GME_orb_FermiSurf", "GME_orb_FermiSea",
            "GME_spin_FermiSea", "GME_spin_FermiSurf", "Spin", "Ohmic_FermiSea", "Ohmic_FermiSurf",
-           "Hall_classic_FermiSurf", "Hall_classic_FermiSea", "NLDrude_FermiSurf", "NLDrude_Fermider2", "SHC",
+           "Hall_classic_FermiSurf", "Hall_classic_FermiSea", "NLDrude_FermiSurf", "NLDrude_Fermider2", "NLDrude_FermiSea", "SHC",
            "VoidResult"]

@@ -10,7 +10,8 @@ import numpy as np
 from .. import _lib
 from .. import factors
 from ..result import KBandResult, TABresult
-from .static import Calculator, StaticCalculator, _PARITY, _external, spin_current_channels, spin_current_params
+from .static import (Calculator, StaticCalculator, _PARITY, _external, der3e_bands, spin_current_channels,
+                     spin_current_params)
 
 
 class Tabulator(Calculator):
@@ -83,6 +84,21 @@ class DerOrbitalMoment(Tabulator):
     ndim = 2
     sign = -1
     factor = factors.factor_morb_evA2_to_muB
+
+
+class Der3E(Tabulator):
+    r"Third derivative of the band energy :math:`\partial_a\partial_b\partial_c E` (tabulate.py:141-145), eV*angstrom^3"
+    ndim = 3
+
+    def needed_channels(self, system):
+        return {("Ham", 0), ("Ham", 1), ("Ham", 2), ("Ham", 3)}
+
+    def __call__(self, data_K):
+        data_K.ensure(self.needed_channels(data_K.system))
+        out = der3e_bands(data_K, self.degen_thresh, self.degen_Kramers, 0)
+        ib = np.arange(data_K.num_wann) if self.ibands is None else self.ibands
+        data = out.cpu().numpy()[:, ib].reshape((data_K.nk, len(ib), 3, 3, 3)) * self.constant_factor
+        return KBandResult(data, transformTR=-1, transformInv=-1)
 
 
 class DerSpin(Tabulator):
@@ -217,4 +233,4 @@ class TabulatorAll(Calculator):
 
 
 __all__ = ["Tabulator", "TabulatorAll", "Energy", "Velocity", "Spin", "InvMass", "BerryCurvature", "DerBerryCurvature",
-           "OrbitalMoment", "DerOrbitalMoment", "SpinBerry", "DerSpin"]
+           "OrbitalMoment", "DerOrbitalMoment", "SpinBerry", "DerSpin", "Der3E"]

@@ -26,7 +26,7 @@ from ._lib import WBMI_F32, WBMI_F32_SPLIT, WBMI_F64
 from .path import KpointBZpath
 
 # components of each Xbar(name, der)
-CHANNEL_NCOMP = {("Ham", 0): 1, ("Ham", 1): 3, ("Ham", 2): 9,
+CHANNEL_NCOMP = {("Ham", 0): 1, ("Ham", 1): 3, ("Ham", 2): 9, ("Ham", 3): 27,
                  ("AA", 0): 3, ("AA", 1): 9, ("rotAA", 0): 3, ("rotAA", 1): 9,
                  ("BB", 0): 3, ("BB", 1): 9, ("CC", 0): 3, ("CC", 1): 9,
                  ("SS", 0): 3, ("SS", 1): 9, ("SA", 0): 9, ("SHA", 0): 9, ("SH", 0): 3, ("SR", 0): 9, ("SHR", 0): 9}
@@ -63,6 +63,10 @@ def channel_terms(name, der, base_offset):
             return [[[1, b0, -1, -1], [0, 0, -1, -1]]]
         if der == 1:
             return [[[1, b0, a, -1], [0, 0, -1, -1]] for a in range(3)]
+        if der == 3:
+            # the first factor i r_a is in the base (Ham_d = i r_a Ham, DeviceSystem.stack)
+            h0 = base_offset["Ham_d"]
+            return [[[1, h0 + a, b, c], [0, 0, -1, -1]] for a in range(3) for b in range(3) for c in range(3)]
         return [[[1, b0, a, b], [0, 0, -1, -1]] for a in range(3) for b in range(3)]
     if der == 0:
         # SA and SHA carry two Cartesian indices (a, s), row-major
@@ -104,6 +108,12 @@ class DeviceSystem:
     def stack(self, keys):
         """R-space channel stack [nR, nch, nw, nw] (complex128) for the channel keys."""
         keys = tuple(keys)
+        if ("Ham", 3) in keys and "Ham_d" not in self.base_offset:
+            # third derivative: one factor i r_a applied here, two by the channel builder
+            h = self.base[..., self.base_offset["Ham"]:self.base_offset["Ham"] + 1]
+            hamd = h * (1j * self.cR)
+            self.base_offset["Ham_d"] = self.base.shape[3]
+            self.base = self.torch.cat([self.base, hamd], dim=3).contiguous()
         if keys not in self._stacks:
             spec = []
             for name, der in keys:
