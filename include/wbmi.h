@@ -67,6 +67,11 @@ extern "C" {
  * `prec` is not read), and a band's value is row comp_slice * 9 .. + 8 of its nch; the
  * trace of a group is the sum over its bands (additive), as for the other formulas */
 #define WBMI_FORMULA_BANDVALUES 16
+/* products of the group blocks of Omega with the spin, Omega and Morb_Hpm (sign +1)
+ * (formula/covariant.py:849-866), Fermi-surface Zeeman terms of the AHC (static.py:605-645) */
+#define WBMI_FORMULA_OMEGASPIN 17
+#define WBMI_FORMULA_OMEGAOMEGA 18
+#define WBMI_FORMULA_OMEGAHPLUS 19
 
 /* channel roles: index into wbmi_scan_params.ch (value = channel slot in the
  * per-k stack, -1 = absent).  Two-index roles are row-major over (x, y). */

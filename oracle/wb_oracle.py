@@ -969,6 +969,21 @@ def MassMass(data_K, **kw):
     return FormulaProduct([InvMass(data_K), InvMass(data_K)])
 
 
+def OmegaS(data_K, **kw):
+    # formula/covariant.py:849-852
+    return FormulaProduct([Omega(data_K, **kw), Spin(data_K)])
+
+
+def OmegaOmega(data_K, **kw):
+    # formula/covariant.py:855-858
+    return FormulaProduct([Omega(data_K, **kw), Omega(data_K, **kw)])
+
+
+def OmegaHplus(data_K, **kw):
+    # formula/covariant.py:861-865
+    return FormulaProduct([Omega(data_K, **kw), Morb_Hpm(data_K, sign=+1, **kw)])
+
+
 def VelSpin(data_K, **kw):
     # formula/covariant.py:811-814
     return FormulaProduct([data_K.covariant('Ham', commader=1), Spin(data_K)])
@@ -1192,6 +1207,38 @@ class SHC_static(StaticCalculator):
     Formula = staticmethod(lambda data_K, **kw: SpinOmega(data_K, **kw))   # defined below
     fder = 0
     constant_factor_default = -FACTOR_AHC / 2
+
+
+FAC_SPIN_Z = _hbar / (2 * _me)                  # factors.py:17
+FAC_ORB_Z = _e / 2 / _hbar * _ang ** 2           # factors.py:16
+
+
+class AHC_Zeeman_spin(StaticCalculator):
+    # static.py:605-615
+    Formula = staticmethod(OmegaS)
+    fder = 1
+    constant_factor_default = FAC_SPIN_Z * FACTOR_AHC
+
+
+class OmegaOmegaCalc(StaticCalculator):
+    # static.py:618-623 (OmegaOmega)
+    Formula = staticmethod(OmegaOmega)
+    fder = 1
+
+
+class AHC_Zeeman_orb(StaticCalculator):
+    # static.py:626-644: Hplus - 2 E_F OmegaOmega, both with the same factor
+    Formula = staticmethod(OmegaHplus)
+    fder = 1
+    constant_factor_default = FAC_ORB_Z * FACTOR_AHC
+
+    def __call__(self, data_K):
+        r = super().__call__(data_K)
+        oo = OmegaOmegaCalc(self.Efermi, constant_factor=self.constant_factor, use_factor=self.use_factor,
+                            kwargs_formula=self.kwargs_formula, degen_thresh=self.degen_thresh,
+                            degen_Kramers=self.degen_Kramers, tetra=self.tetra)(data_K)
+        r.data = r.data - 2 * oo.data * self.Efermi[:, None, None]
+        return r
 
 
 class NLDrude_FermiSea(StaticCalculator):

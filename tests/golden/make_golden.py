@@ -158,7 +158,8 @@ def main():
     # Efermi = linspace(4, 8, 11) (tests/test_run.py:154-170,1102-1119; common_systems.py:35)
     results({"te_" + k: I + f"Te_QE-{k}_iter-0000.npz"
              for k in ["GME_orb_FermiSea", "GME_orb_FermiSurf", "BerryDipole_FermiSea", "berry_dipole",
-                       "dos", "cumdos", "NLDrude_FermiSurf", "NLDrude_Fermider2"]}, "results_te_qe.npz")
+                       "dos", "cumdos", "NLDrude_FermiSurf", "NLDrude_Fermider2", "AHC_Zeeman_spin"]},
+            "results_te_qe.npz")
     # k-resolved tabulation, Fe (Wannier90), grid 4^3 / 2^3, bands 5 and 6
     # (tests/test_run.py:271-288,313-345)
     with np.load(os.path.join(REF, I + "Fe_W90-tabulate.npz"), allow_pickle=False) as f:
@@ -169,7 +170,7 @@ def main():
     forms = {}
     for name in ["Dcov", "DerDcov", "DEinv_ln", "InvMass", "Omega", "DerOmega", "Morb_H", "morb",
                  "DerMorb_upper", "Dermorb_lower", "VelOmega", "Hamiltonian", "Velocity", "Spin", "VelSpin",
-                 "DerWln", "Der3E", "SpinOmega", "VelVel", "VelVelVel", "VelMassVel"]:
+                 "DerWln", "Der3E", "SpinOmega", "VelVel", "VelVelVel", "VelMassVel", "OmegaS", "OmegaOmega"]:
         with np.load(os.path.join(REF, "reference/formula", name + ".npz"), allow_pickle=False) as f:
             for k in f.files:
                 forms[name + "/" + k] = f[k]

@@ -385,6 +385,27 @@ def test_shc_static(wb, precision, tol):
     assert e < tol
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
+def test_ahc_zeeman(wb, precision, tol):
+    """Zeeman corrections to the AHC (static.py:605-644): spin (Omega x spin), orbital
+    (Omega x Morb_Hpm - 2 E_F Omega x Omega) and OmegaOmega, Fermi-surface integrals, on Fe
+    (Wannier90, external terms with BB and CC), grid 4^3 / 2^3, Efermi 17..18 eV, vs the
+    oracle (its products pinned by the reference's OmegaS / OmegaOmega formula fixtures and
+    the Te_QE AHC_Zeeman_spin fixture)."""
+    say(f"test_ahc_zeeman {precision}")
+    st = wb.calculators.static
+    s = gsys(wb, "fe_w90")
+    Ef = np.linspace(17, 18, 11)
+    calcs = {"zs": st.AHC_Zeeman_spin(Efermi=Ef), "zo": st.AHC_Zeeman_orb(Efermi=Ef), "oo": st.OmegaOmega(Efermi=Ef)}
+    r = run_gpu(wb, s, calcs, [4, 4, 4], [2, 2, 2], precision=precision)
+    ref = O.run(osys("fe_w90"), {"zs": O.AHC_Zeeman_spin(Ef), "zo": O.AHC_Zeeman_orb(Ef), "oo": O.OmegaOmegaCalc(Ef)},
+                NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    for k in calcs:
+        e = rel_err(r.results[k].data, ref[k].data)
+        say(f"  {k}: {e:.2e}")
+        assert np.abs(ref[k].data).max() > 0 and e < tol, k
+
+
 def test_refinement_random(wb):
     """Adaptive refinement (two iterations, 2x2x2 sub-blocks) against the oracle's
     restatement; the selected blocks' metrics are separated by >= 2 %, so both

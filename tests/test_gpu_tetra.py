@@ -184,7 +184,7 @@ def test_te_qe_gme(wb, precision, tol):
     """Tellurium (QE): tetra=True, point group C3z x C2x x TR, irreducible K-points,
     NK [3, 3, 4], NKFFT [1, 1, 4] (tests/test_run.py:1102-1119) against the reference's
     Te_QE fixtures: GME_orb sea and surface, BerryDipole_FermiSea, NLAHC_FermiSea, DOS,
-    cumulative DOS.  Bar 1e-8 x max|ref| (FP64); the mixed form's error is printed."""
+    cumulative DOS, AHC_Zeeman_spin; NLDrude (Fermi surface, f'') vanish by time reversal.  Bar 1e-8 x max|ref| (FP64); the mixed form's error is printed."""
     say(f"test_te_qe_gme {precision}")
     st = wb.calculators.static
     s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_te_qe.npz"))
@@ -199,7 +199,8 @@ def test_te_qe_gme(wb, precision, tol):
              "GME_orb_FermiSea": st.GME_orb_FermiSea(Efermi=Ef, tetra=True),
              "GME_orb_FermiSurf": st.GME_orb_FermiSurf(Efermi=Ef, tetra=True),
              "NLDrude_FermiSurf": st.NLDrude_FermiSurf(Efermi=Ef, tetra=True),
-             "NLDrude_Fermider2": st.NLDrude_Fermider2(Efermi=Ef, tetra=True)}
+             "NLDrude_Fermider2": st.NLDrude_Fermider2(Efermi=Ef, tetra=True),
+             "AHC_Zeeman_spin": st.AHC_Zeeman_spin(Efermi=Ef, tetra=True)}
     grid = wb.Grid(system=s, NK=[3, 3, 4], NKFFT=[1, 1, 4])
     r = wb.run(s, grid, calcs, use_irred_kpt=True, symmetrize=True, parameters_K={"precision": precision},
                save_files=False, verbose=False)

@@ -239,10 +239,11 @@ FORMULAS = {
     "DerWln": O.DerWln, "Der3E": O.Der3E, "VelVel": O.VelVel, "VelVelVel": O.VelVelVel, "VelMassVel": O.VelMassVel,
     # tests/test_formula.py:175-176: the Ryoo spin current
     "SpinOmega": lambda d: O.SpinOmega(d, spin_current_type="ryoo"),
+    "OmegaS": O.OmegaS, "OmegaOmega": O.OmegaOmega,
 }
 LN_ONLY = {"Dcov", "DerDcov", "DEinv_ln"}
 NN_ONLY = {"Omega", "DerOmega", "Morb_H", "morb", "DerMorb_upper", "Dermorb_lower", "VelOmega", "VelSpin", "Der3E",
-           "VelVel", "VelVelVel", "VelMassVel", "SpinOmega"}
+           "VelVel", "VelVelVel", "VelMassVel", "SpinOmega", "OmegaS", "OmegaOmega"}
 
 
 @pytest.fixture(scope="module")

@@ -190,6 +190,7 @@ TE_CALCS = {
     "BerryDipole_FermiSea": (lambda Ef: O.BerryDipole_FermiSea(Ef, tetra=True), 1, -1),
     "GME_orb_FermiSea": (lambda Ef: O.GME_orb_FermiSea(Ef, tetra=True), 1, -1),
     "GME_orb_FermiSurf": (lambda Ef: O.GME_orb_FermiSurf(Ef, tetra=True), 1, -1),
+    "AHC_Zeeman_spin": (lambda Ef: O.AHC_Zeeman_spin(Ef, tetra=True), 1, 1),
 }
 
 

@@ -38,6 +38,9 @@ FORMULA_MASSVEL = 13
 FORMULA_HALL_SURF = 14
 FORMULA_HALL_SEA = 15
 FORMULA_BANDVALUES = 16
+FORMULA_OMEGASPIN = 17
+FORMULA_OMEGAOMEGA = 18
+FORMULA_OMEGAHPLUS = 19
 WBMI_SHC_RYOO, WBMI_SHC_QIAO, WBMI_SHC_SIMPLE = 0, 1, 2
 
 ROLE_V, ROLE_W, ROLE_A, ROLE_AD, ROLE_O, ROLE_OD = 0, 3, 12, 15, 24, 27

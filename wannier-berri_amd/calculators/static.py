@@ -46,7 +46,9 @@ _PARITY = {_lib.FORMULA_IDENTITY: (1, 1), _lib.FORMULA_OMEGA: (-1, 1), _lib.FORM
            # products of velocities (odd, odd) and inverse masses (even, even) (formula.py:127-128);
            # the Hall tensors keep the parity of their rank-4 products (static.py:418-423)
            _lib.FORMULA_VELVELVEL: (-1, -1), _lib.FORMULA_MASSVEL: (-1, -1), _lib.FORMULA_HALL_SURF: (1, 1),
-           _lib.FORMULA_HALL_SEA: (1, 1)}
+           _lib.FORMULA_HALL_SEA: (1, 1),
+           # Omega (odd, even) times the spin (odd, even), Omega or Morb_Hpm (odd, even)
+           _lib.FORMULA_OMEGASPIN: (1, 1), _lib.FORMULA_OMEGAOMEGA: (1, 1), _lib.FORMULA_OMEGAHPLUS: (1, 1)}
 # formulas that read only the Hamiltonian's derivatives and the spin, never A, B, C
 _SPIN = (_lib.FORMULA_DERSPIN, _lib.FORMULA_VELSPIN, _lib.FORMULA_SPIN)
 
@@ -147,11 +149,13 @@ class StaticCalculator(Calculator):
         if f in (_lib.FORMULA_DEROMEGA, _lib.FORMULA_DERMORB):
             require_hermitian_ham(system, type(self).__name__)
             need.add(("Ham", 2))
+        if f == _lib.FORMULA_OMEGASPIN:
+            need.add(("SS", 0))
         if ext:
             need |= {("AA", 0), ("rotAA", 0)}
             if f in (_lib.FORMULA_DEROMEGA, _lib.FORMULA_DERMORB):
                 need |= {("AA", 1), ("rotAA", 1)}
-            if f in (_lib.FORMULA_MORB_HPM, _lib.FORMULA_VELHPLUS, _lib.FORMULA_DERMORB):
+            if f in (_lib.FORMULA_MORB_HPM, _lib.FORMULA_VELHPLUS, _lib.FORMULA_DERMORB, _lib.FORMULA_OMEGAHPLUS):
                 need |= {("BB", 0), ("CC", 0)}
             if f == _lib.FORMULA_DERMORB:
                 need |= {("BB", 1), ("CC", 1)}
@@ -440,6 +444,51 @@ class Hall_classic_FermiSea(StaticCalculator):
         super().__init__(constant_factor=constant_factor, **kwargs)
 
 
+class AHC_Zeeman_spin(StaticCalculator):
+    r"""Zeeman (spin) correction to the AHC (:math:`S/m/T`), Fermi surface (static.py:605-615)
+
+        | Output: :math:`ZAHC^{spin}_{\alpha\beta :\mu} = e^2/\hbar \int [dk] \Omega_\delta s_\mu f'`"""
+    formula = _lib.FORMULA_OMEGASPIN
+    ndim = 2
+    fder = 1
+
+    def __init__(self, constant_factor=factors.fac_spin_Z * factors.factor_ahc, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+
+
+class OmegaOmega(StaticCalculator):
+    """Fermi-surface integral of Omega_a Omega_b (static.py:618-623)."""
+    formula = _lib.FORMULA_OMEGAOMEGA
+    ndim = 2
+    fder = 1
+
+
+class AHC_Zeeman_orb(StaticCalculator):
+    r"""Zeeman (orbital) correction to the AHC (:math:`S/m/T`), Fermi surface (static.py:626-644)
+
+        | Output: :math:`ZAHC^{orb}_{\alpha\beta :\mu} = e^2/\hbar \int [dk] \Omega_\delta m_\mu f'`,
+        | :math:`m = H + G - 2E_f \Omega`"""
+    formula = _lib.FORMULA_OMEGAHPLUS
+    ndim = 2
+    fder = 1
+
+    def __init__(self, constant_factor=factors.fac_orb_Z * factors.factor_ahc, **kwargs):
+        super().__init__(constant_factor=constant_factor, **kwargs)
+        self.OmegaOmega = OmegaOmega(constant_factor=constant_factor, print_comment=False, **kwargs)
+
+    def needed_channels(self, system):
+        return super().needed_channels(system) | self.OmegaOmega.needed_channels(system)
+
+    def __call__(self, data_K):
+        data_K.ensure(self.needed_channels(data_K.system))
+        Hplus = super().__call__(data_K)
+        Omega = self.OmegaOmega(data_K).mul_array(self.Efermi)
+        if self.k_resolved:
+            Hplus.add(-2 * Omega)
+            return Hplus
+        return Hplus - 2 * Omega
+
+
 def der3e_bands(data_K, degen_thresh, degen_Kramers, mode):
     """Der3E per band [nk, nw, 27] (wbmi_der3e_bands; mode 0: group means, 1: per-band shares)."""
     torch = data_K.torch
@@ -662,4 +711,5 @@ __all__ = ["Calculator", "StaticCalculator", "DOS", "CumDOS", "AHC", "BerryDipol
            "BerryDipole_FermiSea", "NLAHC_FermiSea", "Morb", "GME_orb_FermiSurf", "GME_orb_FermiSea",
            "GME_spin_FermiSea", "GME_spin_FermiSurf", "Spin", "Ohmic_FermiSea", "Ohmic_FermiSurf",
            "Hall_classic_FermiSurf", "Hall_classic_FermiSea", "NLDrude_FermiSurf", "NLDrude_Fermider2", "NLDrude_FermiSea", "SHC",
+           "AHC_Zeeman_spin", "OmegaOmega", "AHC_Zeeman_orb",
            "VoidResult"]

@@ -65,6 +65,13 @@ static int check_roles(wbmi_ctx* ctx, const wbmi_scan_params* p, int nch) {
                 return wbmi_fail(ctx, WBMI_ERR_ARG,
                                  "wbmi_static_scan: band values need nch 9 (comp_slice 0) or 27 (comp_slice 0..2)");
             break;
+        case WBMI_FORMULA_OMEGASPIN:
+        case WBMI_FORMULA_OMEGAOMEGA:
+            if ((rc = need(WBMI_ROLE_V, 3))) return rc;
+            if (ex && ((rc = need(WBMI_ROLE_A, 3)) || (rc = need(WBMI_ROLE_O, 3)))) return rc;
+            if (p->formula == WBMI_FORMULA_OMEGASPIN && (rc = need(WBMI_ROLE_S, 3))) return rc;
+            break;
+        case WBMI_FORMULA_OMEGAHPLUS:
         case WBMI_FORMULA_MORB_HPM:
         case WBMI_FORMULA_VELHPLUS:
             if ((rc = need(WBMI_ROLE_V, 3))) return rc;

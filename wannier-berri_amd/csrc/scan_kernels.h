@@ -330,7 +330,8 @@ __host__ __device__ constexpr int ncomp_of() {
 template <int F>
 __host__ __device__ constexpr bool uses_A() {
     return F == WBMI_FORMULA_OMEGA || F == WBMI_FORMULA_DEROMEGA || F == WBMI_FORMULA_VELOMEGA ||
-           F == WBMI_FORMULA_MORB_HPM || F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERMORB;
+           F == WBMI_FORMULA_MORB_HPM || F == WBMI_FORMULA_VELHPLUS || F == WBMI_FORMULA_DERMORB ||
+           F == WBMI_FORMULA_OMEGASPIN || F == WBMI_FORMULA_OMEGAOMEGA || F == WBMI_FORMULA_OMEGAHPLUS;
 }
 
 // The two curl components c whose index pair (alpha_c, beta_c) contains x.
@@ -980,6 +981,30 @@ __device__ __attribute__((always_inline)) void eval_partition(const Smem& S, con
             });
             S.acc[t] = mk(v.re, 0.0);
         }
+    } else if (F == WBMI_FORMULA_OMEGASPIN || F == WBMI_FORMULA_OMEGAOMEGA || F == WBMI_FORMULA_OMEGAHPLUS) {
+        // Re sum_{n' in G} Omega^a_nn' Y^b_n'n, Omega = s + s^+, Y = S (OmegaS), Omega (OmegaOmega)
+        // or Morb_Hpm with sign +1 (OmegaHplus) (formula/formula.py:121-148, covariant.py:849-866)
+        for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
+            const int n = t / 9, a = (t % 9) / 3, b = t % 3;
+            if (!S.act[n]) continue;
+            cplx<double> v = mk(0.0, 0.0);
+            for (int m = S.gs[n]; m < S.ge[n]; ++m) {
+                const cplx<double> om = omega_s(K, a, n, m, in, ex) + conj(omega_s(K, a, m, n, in, ex));
+                cplx<double> y;
+                if (F == WBMI_FORMULA_OMEGASPIN) {
+                    y = K.S(b, m, n);
+                } else {
+                    const cplx<double> ob = omega_s(K, b, m, n, in, ex) + conj(omega_s(K, b, n, m, in, ex));
+                    if (F == WBMI_FORMULA_OMEGAOMEGA)
+                        y = ob;
+                    else
+                        y = morbh_s(K, b, m, n, in, ex) + conj(morbh_s(K, b, n, m, in, ex)) +
+                            (0.5 * (S.E[m] + S.E[n])) * ob;
+                }
+                v += om * y;
+            }
+            S.acc[t] = mk(v.re, 0.0);
+        }
     } else if (F == WBMI_FORMULA_BANDVALUES) {
         // the caller's per-band shares of the group traces, 9 of them from row comp_slice * 9
         for (int t = tid; t < nw * 9; t += (int)blockDim.x) {
@@ -1265,7 +1290,8 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>()), (scan_min_waves<
                               F == WBMI_FORMULA_VELSPIN || F == WBMI_FORMULA_SPIN || F == WBMI_FORMULA_INVMASS ||
                               F == WBMI_FORMULA_VELVEL || F == WBMI_FORMULA_VELVELVEL || F == WBMI_FORMULA_MASSVEL ||
                               F == WBMI_FORMULA_HALL_SURF || F == WBMI_FORMULA_HALL_SEA ||
-                              F == WBMI_FORMULA_BANDVALUES;
+                              F == WBMI_FORMULA_BANDVALUES || F == WBMI_FORMULA_OMEGASPIN ||
+                              F == WBMI_FORMULA_OMEGAOMEGA || F == WBMI_FORMULA_OMEGAHPLUS;
 
     // band groups (grid/tetrahedron.py:132-162, data_K.py:172-186), formed by wave 0: each lane
     // owns bands lane and lane + 64 (nw <= 128); borders from two ballots and popcounts, one
@@ -1539,6 +1565,9 @@ static int dispatch_scan(wbmi_ctx* ctx, int nk, const ScanArgs& a, const double*
         case WBMI_FORMULA_HALL_SURF: return launch_scan<T, WBMI_FORMULA_HALL_SURF, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_HALL_SEA: return launch_scan<T, WBMI_FORMULA_HALL_SEA, MODE>(ctx, nk, a, E, X, hist);
         case WBMI_FORMULA_BANDVALUES: return launch_scan<T, WBMI_FORMULA_BANDVALUES, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_OMEGASPIN: return launch_scan<T, WBMI_FORMULA_OMEGASPIN, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_OMEGAOMEGA: return launch_scan<T, WBMI_FORMULA_OMEGAOMEGA, MODE>(ctx, nk, a, E, X, hist);
+        case WBMI_FORMULA_OMEGAHPLUS: return launch_scan<T, WBMI_FORMULA_OMEGAHPLUS, MODE>(ctx, nk, a, E, X, hist);
         default: return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: formula %d", a.p.formula);
     }
 }

@@ -21,6 +21,8 @@ factor_injection_current = -np.pi * elementary_charge ** 3 / (hbar ** 2) * TAU_U
 factor_nlahc = elementary_charge ** 3 / hbar ** 2 * TAU_UNIT
 # tau^1 E^1, velocities converted from eV*m to m/s (factors.py:25-26)
 factor_ohmic = elementary_charge ** 2 / hbar / angstrom * TAU_UNIT * elementary_charge / hbar
+fac_orb_Z = elementary_charge / 2 / hbar * angstrom ** 2   # m_orb * B in eV (factors.py:16)
+fac_spin_Z = hbar / (2 * electron_mass)                      # m_spin * B in eV (factors.py:17)
 # tau^2 E^1 B^1 and tau^2 E^2 B^0, velocities in m/s (factors.py:28-31)
 factor_hall_classic = -(elementary_charge ** 3 / hbar ** 2 * angstrom * TAU_UNIT ** 2 * elementary_charge ** 2 / hbar ** 2)
 factor_nldrude = -(elementary_charge ** 3 / hbar ** 2 * TAU_UNIT ** 2 * elementary_charge / hbar)
