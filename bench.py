@@ -285,10 +285,92 @@ def bench_klists(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_config(args):
+    """The other model configurations at their own sizes, as one more JSON line each (not the
+    headline): `haldane` (2-band Haldane model, AHC at 1001 levels in [-3, 3] eV, 200 x 200 x 1 grid
+    as NKFFT 50 x 50 x 1, FP64, the whole grid per step), `spinor64` (nw = 64 spinor model, R cube
+    |R_i| <= 5 (1331): AHC with external terms, Morb and GME sea / surface with internal terms (the
+    model has no BB, CC), 128^3 grid as NKFFT 16^3 K-blocks) and `large128` (nw = 128, R cube
+    |R_i| <= 6 (2197): AHC, BCD sea with external terms and GME sea / surface with internal terms,
+    4001 levels over the band range, 200^3 grid as NKFFT 8^3 K-blocks).  For the two large
+    configurations a step is the first `--blocks` K-blocks of the grid's K-list (every K-block
+    of a grid does the same work); value = k-points of those blocks / step time."""
+    import torch
+    import wannierberri_amd as wb
+    from wannierberri_amd import models
+    st = wb.calculators.static
+    prec = args.precisions.split(",")[-1]
+    ki = {"external_terms": False}
+    if args.workload == "haldane":
+        system = models.haldane()
+        Ef = np.linspace(-3, 3, 1001)
+        calcs = {"ahc": st.AHC(Efermi=Ef)}
+        grid = wb.Grid(system=system, NKFFT=[50, 50, 1], NKdiv=[4, 4, 1])
+        prec = "fp64"
+        quant, nwdesc = "AHC", "2-band Haldane model, 1001 levels in [-3, 3] eV, 200x200x1 grid as NKFFT 50x50x1"
+    elif args.workload == "spinor64":
+        system = models.spinor(num_wann_half=32, Rmax=5, seed=3)
+        E0 = np.linalg.eigvalsh(system.get_R_mat("Ham")[np.where((system.rvec.iRvec == 0).all(1))[0][0]])
+        Ef = np.linspace(E0.min() - 1, E0.max() + 1, 2001)
+        calcs = {"ahc": st.AHC(Efermi=Ef), "morb": st.Morb(Efermi=Ef, kwargs_formula=ki),
+                 "gme": st.GME_orb_FermiSea(Efermi=Ef, kwargs_formula=ki),
+                 "gmes": st.GME_orb_FermiSurf(Efermi=Ef, kwargs_formula=ki)}
+        grid = wb.Grid(system=system, NKFFT=[16, 16, 16], NKdiv=[8, 8, 8])
+        quant = "AHC+Morb+GME"
+        nwdesc = (f"nw=64 spinor model (2x32 spin blocks, SOC block std 0.05 eV, seed 3), R cube |R_i|<=5 "
+                  f"({system.rvec.nRvec}), 2001 levels over the onsite band range, 128^3 grid as NKFFT 16^3")
+    else:
+        system = models.synthetic(num_wann=128, Rmax=6, seed=5)
+        Ef = np.linspace(-16.0, 16.0, 4001)
+        calcs = {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef),
+                 "gme": st.GME_orb_FermiSea(Efermi=Ef, kwargs_formula=ki),
+                 "gmes": st.GME_orb_FermiSurf(Efermi=Ef, kwargs_formula=ki)}
+        grid = wb.Grid(system=system, NKFFT=[8, 8, 8], NKdiv=[25, 25, 25])
+        quant = "AHC+BCD+GME"
+        nwdesc = (f"nw=128 synthetic model (seed 5), R cube |R_i|<=6 ({system.rvec.nRvec}), 4001 levels in "
+                  f"[-16, 16] eV, 200^3 grid as NKFFT 8^3")
+    K_list = grid.get_K_list(use_symmetry=False)
+    blocks = K_list if args.workload == "haldane" else K_list[:args.blocks]
+    need = set()
+    for c in calcs.values():
+        need |= set(c.needed_channels(system))
+    pk = {"precision": prec, "channels": need}
+
+    def step():
+        tot = None
+        for Kp in blocks:
+            d = wb.Data_K(system, dK=Kp.Kp_fullBZ, grid=grid, Kpoint=Kp, **pk)
+            r = wb.ResultDict({k: c(d) for k, c in calcs.items()}) * Kp.factor
+            tot = r if tot is None else tot + r
+            del d
+        return tot
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    nk = len(blocks) * int(np.prod(grid.FFT))
+    log(f"[bench] {args.workload}: {len(blocks)} of {len(K_list)} K-blocks, {nk} k-points, {dt * 1e3:.1f} ms/step")
+    out = {"metric": f"k-pts/s {quant} at nw={system.num_wann} on 1 MI355X", "value": nk / dt, "unit": "k-points/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
+           "config": {"workload": nwdesc, "nk_measured": nk, "k_blocks_measured": len(blocks),
+                      "k_blocks_grid": len(K_list), "nk_grid": int(np.prod(grid.dense)),
+                      "precision": "mixed" if prec == "fp32" else "fp64"}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["grid", "klists"], default="grid",
-                    help="grid: AHC+BCD on the 96^3 grid (default); klists: AHC over 512 k-lists")
+    ap.add_argument("--workload", choices=["grid", "klists", "haldane", "spinor64", "large128"], default="grid",
+                    help="grid: AHC+BCD on the 96^3 grid (default); klists: AHC over 512 k-lists; haldane, "
+                         "spinor64, large128: the other model configurations (one JSON line each)")
+    ap.add_argument("--blocks", type=int, default=8, help="K-blocks per step of spinor64 / large128")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
@@ -307,6 +389,9 @@ def main():
         return
     if args.workload == "klists":
         bench_klists(args)
+        return
+    if args.workload in ("haldane", "spinor64", "large128"):
+        bench_config(args)
         return
 
     import torch
