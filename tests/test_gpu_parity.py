@@ -406,6 +406,27 @@ def test_ahc_zeeman(wb, precision, tol):
         assert np.abs(ref[k].data).max() > 0 and e < tol, k
 
 
+def test_restart_refinement_gpu(wb, tmp_path):
+    """Restart files (run_grid.py:242-348): two refinement iterations of AHC + DOS on Haldane with
+    C3z in one run equal one iteration written with dump_results followed by one read back with
+    restart=True (1e-12)."""
+    say("test_restart_refinement_gpu")
+    st = wb.calculators.static
+    s = gsys(wb, "haldane")
+    s.set_pointgroup(["C3z"])
+    Ef = np.linspace(-3, 3, 31)
+    calcs = {"ahc": st.AHC(Efermi=Ef), "dos": st.DOS(Efermi=Ef)}
+    grid = wb.Grid(system=s, NK=[12, 12, 1], NKFFT=[4, 4, 1])
+    kw = dict(use_irred_kpt=True, symmetrize=True, save_files=False, verbose=False)
+    full = wb.run(s, grid, calcs, adpt_num_iter=2, **kw)
+    rec = str(tmp_path / "rec")
+    wb.run(s, grid, calcs, adpt_num_iter=1, dump_results=True, file_Klist_path=rec, **kw)
+    cont = wb.run(s, grid, calcs, adpt_num_iter=1, restart=True, file_Klist_path=rec, **kw)
+    for k in calcs:
+        a, b = full.results[k].data, cont.results[k].data
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max(), k
+
+
 def test_refinement_random(wb):
     """Adaptive refinement (two iterations, 2x2x2 sub-blocks) against the oracle's
     restatement; the selected blocks' metrics are separated by >= 2 %, so both

@@ -93,6 +93,7 @@ class KpointBZparallel:
         self.pointgroup = pointgroup
         self.refinement_level = refinement_level
         self.result = None
+        self.result_loader = None   # restart: reads the result from its file (restart.py)
         self.was_evaluated_flag = False
         self._max = None
         self._star = None
@@ -126,6 +127,8 @@ class KpointBZparallel:
 
     def get_result(self):
         if self.result is None:
+            if self.result_loader is not None:
+                return self.result_loader()
             raise RuntimeError("result for a K-point is called, which was never evaluated")
         return self.result
 

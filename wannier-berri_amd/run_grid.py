@@ -13,7 +13,8 @@ from collections.abc import Iterable
 import numpy as np
 
 from .data_k import Data_K
-from .grid import exclude_equiv_points
+from . import restart as restart_files
+from .grid import GridTetra, exclude_equiv_points
 from .path import Path
 from .result import ResultDict, TABresult
 
@@ -33,7 +34,9 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
     batches of `k_batch` path points, each a Data_K on that k-list, and the tables come back in
     path order.  With `prefetch` (and a data_k_class that has
     `prefetch`) the next K-block's R -> k runs on a side stream while the current one is
-    evaluated.
+    evaluated.  `dump_results` / `allow_restart` keep a record in `file_Klist_path` (default
+    `_tmp_wb`) from which `restart=True` continues (restart_iteration: -1 = the last
+    iteration written); the files are npz / npy arrays (restart.py), regular grids only.
     """
     if data_k_class is None:
         data_k_class = Data_K
@@ -47,15 +50,20 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
                                  "with a Path")
         symmetrize = False
         adpt_num_iter = 0
-        restart = dump_results = False
+        restart = dump_results = allow_restart = False
     else:
         if use_irred_kpt:
             symmetrize = True
         for key, calc in calculators.items():
             if not calc.allow_grid:
                 raise ValueError(f"Calculation on Grid is running, but calculator `{key}` is not compatible with a Grid")
-    if restart or dump_results:
-        raise NotImplementedError("restart / dump_results are not available on the MI355X path")
+    if (restart or allow_restart or dump_results) and isinstance(grid, GridTetra):
+        raise NotImplementedError("restart files are written for regular grids only, not for GridTetra")
+    if dump_results:
+        allow_restart = True
+    if restart:
+        allow_restart = True
+    rpath = file_Klist_path if file_Klist_path is not None else "_tmp_wb"
     if getattr(data_k_class, "accepts_channels", False):
         need = set()
         for calc in calculators.values():
@@ -63,7 +71,28 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
                 need |= set(calc.needed_channels(system))
         parameters_K.setdefault("channels", need)
 
-    K_list = grid.get_K_list(use_symmetry=use_irred_kpt and not on_path, k_batch=k_batch if on_path else None)
+    if restart:
+        # run_grid.py:272-286: the K-points, their weights and results as the record left them
+        K_list = restart_files.read_klist(rpath, grid, calculators)
+        start_iter, factors = restart_files.read_factors(rpath, restart_iteration)
+        factors = np.concatenate([factors, np.zeros(len(K_list) - len(factors))])
+        for Kp, fac in zip(K_list, factors):
+            Kp.set_factor(float(fac))
+        result_all = None
+        for Kp in K_list:
+            if Kp.was_evaluated_flag:
+                r = Kp.get_result() * Kp.factor
+                result_all = r if result_all is None else result_all + r
+        if verbose:
+            print(f"run(): {len(K_list)} K-points read from {rpath}, iteration {start_iter}", flush=True)
+    else:
+        K_list = grid.get_K_list(use_symmetry=use_irred_kpt and not on_path, k_batch=k_batch if on_path else None)
+        start_iter = 0
+        result_all = None
+        if allow_restart:
+            restart_files.prepare(rpath)
+            restart_files.write_factors(rpath, [Kp.factor for Kp in K_list], 0)
+    store_results = allow_restart or adpt_num_iter != 0
     if adpt_num_iter < 0:
         adpt_num_iter = -adpt_num_iter * np.prod(grid.div) / np.prod(adpt_mesh) / adpt_fac / 3
     adpt_num_iter = int(round(adpt_num_iter))
@@ -74,7 +103,6 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
             adpt_mesh = [adpt_mesh] * 3
         adpt_mesh = np.array(adpt_mesh)
 
-    result_all = None
     factors = np.array([Kp.factor for Kp in K_list])
     t0 = time.time()
     counter = 0
@@ -84,24 +112,29 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
         side = torch.cuda.Stream()   # default priority
     for i_iter in range(adpt_num_iter + 1):
         result_iter = None
-        todo = [Kp for Kp in K_list if not Kp.was_evaluated_flag]
+        todo = [(ik, Kp) for ik, Kp in enumerate(K_list) if not Kp.was_evaluated_flag]
         nxt = None
-        for i, Kp in enumerate(todo):
+        for i, (ik, Kp) in enumerate(todo):
             data = nxt if nxt is not None else data_k_class(system, dK=Kp.Kp_fullBZ, grid=grid, Kpoint=Kp,
                                                             **parameters_K)
             nxt = None
             if side is not None and i + 1 < len(todo):
-                nKp = todo[i + 1]
+                nKp = todo[i + 1][1]
                 nxt = data_k_class(system, dK=nKp.Kp_fullBZ, grid=grid, Kpoint=nKp, **parameters_K)
                 if i > 0:   # the first block sets up the system's stacks and plans on the main stream
                     nxt.prefetch(side)
             res = ResultDict({k: v(data) for k, v in calculators.items()})
             del data
             res = system.pointgroup.symmetrize(res) if symmetrize else res
-            if adpt_num_iter > 0:
+            if store_results:
                 Kp.set_result(res)
             else:
                 Kp.was_evaluated_flag = True
+            if allow_restart:
+                restart_files.dump_result(rpath, ik, res)
+                if dump_results:   # kept in the file only (Kpoint.py:56-61)
+                    Kp.result = None
+                    Kp.result_loader = (lambda ik=ik: restart_files.load_result(rpath, ik, calculators))
             contrib = res * Kp.factor
             result_iter = contrib if result_iter is None else result_iter + contrib
             counter += 1
@@ -116,8 +149,11 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
             for i, fac in enumerate(diff):
                 if abs(fac) > 1e-8:
                     result_all = result_all + K_list[i].get_result() * fac
-        if save_files:
-            result_all.savedata(prefix=fout_name, suffix=suffix, i_iter=i_iter)
+        if allow_restart:
+            restart_files.write_klist(rpath, K_list)
+            restart_files.write_factors(rpath, [Kp.factor for Kp in K_list], i_iter + start_iter)
+        if save_files and not (restart and i_iter == 0):
+            result_all.savedata(prefix=fout_name, suffix=suffix, i_iter=i_iter + start_iter)
         if i_iter >= adpt_num_iter:
             break
         Kmax = np.array([K.max for K in K_list]).T
