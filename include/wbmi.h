@@ -260,7 +260,7 @@ int wbmi_eigh_guess(wbmi_ctx* ctx, int nk, int nw, const void* H_dev, int64_t Hs
  * added as (f(E_n) - f(E_m)) (E_n - E_m) / (E_n - E_m - w - i eta) M^{ab}
  * (Gaussian: imaginary part pi G) into out_dev [nEF, nomega, 3, 3] complex128
  * (interleaved doubles), accumulated across calls; the caller scales by
- * factor_opt / (nk V).  nw <= 32.                                          */
+ * factor_opt / (nk V).  nw <= 32 (larger: wbmi_optical_conductivity_ws). */
 /* Joint density of states (calculators/dynamic.py:145-167): per k, band-group
  * pairs (m, n) with E_m < EFmax, E_n > EFmin and omega_min - 5 eta <
  * E_m - E_n < omega_max + 5 eta add |m| |n| (f(E_n) - f(E_m)) smear(E_m - E_n - w).
@@ -309,6 +309,15 @@ int wbmi_injection_current(wbmi_ctx* ctx, int nk, int nw, int nch, const double*
 
 int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
                               int prec, const wbmi_optcond_params* params, double* out_dev);
+/* The same for nw > 32 (up to 128), where A does not fit in LDS: work_dev holds A for every k,
+ * wbmi_optical_conductivity_work_bytes(nk, nw) bytes (0 for nw <= 32, where
+ * wbmi_optical_conductivity needs none).  The pair-walk calculators (wbmi_spin_hall,
+ * wbmi_shift_current, wbmi_injection_current) take nw <= 128 the same way, in the
+ * larger workspace that wbmi_spin_hall_work_bytes reports for nw > 32.            */
+size_t wbmi_optical_conductivity_work_bytes(int nk, int nw);
+int wbmi_optical_conductivity_ws(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev,
+                                 int prec, const wbmi_optcond_params* params, void* work_dev, size_t work_bytes,
+                                 double* out_dev);
 
 /* ---- gauge rotation U^+ X U -----------------------------------------------
  * Replaces Data_K._rotate (data_K/data_K.py:130-132) for the channels of a

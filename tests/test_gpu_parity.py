@@ -427,6 +427,35 @@ def test_restart_refinement_gpu(wb, tmp_path):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max(), k
 
 
+@pytest.mark.parametrize("nw,precision,tol", [(40, "fp64", 1e-8), (40, "fp32", 1e-5), (64, "fp64", 1e-8)])
+def test_dynamic_large_nw(wb, nw, precision, tol):
+    """Dynamic calculators above nw = 32, where the per-k matrices A / Y live in the workspace
+    (wbmi_optical_conductivity_ws, the larger pair-walk workspace): Kubo optical conductivity,
+    shift and injection current and the 'simple' spin Hall conductivity (SS from spin pairs)
+    on a synthetic model with external terms, grid 2^3, vs the oracle."""
+    say(f"test_dynamic_large_nw nw={nw} {precision}")
+    from wannierberri_amd import models
+    dyn = wb.calculators.dynamic
+    s = models.synthetic(num_wann=nw, Rmax=1, seed=60 + nw)
+    s.set_spin_pairs([(2 * i, 2 * i + 1) for i in range(nw // 2)])
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA", "SS")})
+    Ef, om = np.linspace(-2, 2, 5), np.linspace(0.0, 4.0, 9)
+    kw = dict(smr_fixed_width=0.2, smr_type="Gaussian", kBT=0.01)
+    calcs = {"oc": dyn.OpticalConductivity(Efermi=Ef, omega=om, **kw),
+             "sc": dyn.ShiftCurrent(sc_eta=0.1, Efermi=Ef, omega=om, **kw),
+             "ic": dyn.InjectionCurrent(Efermi=Ef, omega=om, **kw),
+             "shc": dyn.SHC(SHC_type="simple", Efermi=Ef, omega=om, **kw)}
+    r = run_gpu(wb, s, calcs, [2, 2, 2], [2, 2, 2], precision=precision)
+    ref = O.run(o, {"oc": O.OpticalConductivity(Ef, om, **kw), "sc": O.ShiftCurrent(Ef, om, sc_eta=0.1, **kw),
+                    "ic": O.InjectionCurrent(Ef, om, **kw), "shc": O.SHC(Ef, om, SHC_type="simple", **kw)},
+                NK=[2, 2, 2], NKFFT=[2, 2, 2])
+    for k in calcs:
+        e = rel_err(r.results[k].data, ref[k].data)
+        say(f"  {k}: {e:.2e}")
+        assert np.abs(ref[k].data).max() > 0 and e < tol, k
+
+
 def test_refinement_random(wb):
     """Adaptive refinement (two iterations, 2x2x2 sub-blocks) against the oracle's
     restatement; the selected blocks' metrics are separated by >= 2 %, so both

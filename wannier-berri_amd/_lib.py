@@ -59,6 +59,7 @@ SYMBOLS = [
     "wbmi_static_scan_k", "wbmi_finalize_k", "wbmi_static_scan_tetra_k", "wbmi_finalize_tetra_k",
     "wbmi_tabulate_invmass", "wbmi_eigh_noalloc", "wbmi_eigh_noalloc_max_bytes", "wbmi_r2k_klist_tile",
     "wbmi_r2k_klist_tile_shape", "wbmi_tabulate_spin_berry", "wbmi_spin_berry_bands", "wbmi_der3e_bands",
+    "wbmi_optical_conductivity_work_bytes", "wbmi_optical_conductivity_ws",
 ]
 
 
@@ -149,6 +150,9 @@ _PROTOS = {
     "wbmi_spin_berry_bands": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(ShcParams), _vp, ctypes.c_size_t,
                                    _vp]),
     "wbmi_der3e_bands": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, _vp, _vp, _vp, _d, _i, _i, _vp]),
+    "wbmi_optical_conductivity_work_bytes": (ctypes.c_size_t, [_i, _i]),
+    "wbmi_optical_conductivity_ws": (_i, [_vp, _i, _i, _i, _vp, _vp, _i, ctypes.POINTER(OptcondParams), _vp,
+                                          ctypes.c_size_t, _vp]),
 }
 
 
@@ -354,9 +358,16 @@ class Context:
 
     def optical_conductivity(self, E, X, prec, params, out):
         nk, nw = E.shape
-        self.check(self._timed("optcond", (nk, nw), lib().wbmi_optical_conductivity, self.h, nk, nw, X.shape[1],
-                               _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(out)),
-                   "wbmi_optical_conductivity")
+        wbytes = int(lib().wbmi_optical_conductivity_work_bytes(nk, nw))
+        if wbytes == 0:
+            self.check(self._timed("optcond", (nk, nw), lib().wbmi_optical_conductivity, self.h, nk, nw, X.shape[1],
+                                   _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(out)),
+                       "wbmi_optical_conductivity")
+            return
+        work = self._workspace(wbytes)   # nw > 32: A of every k
+        self.check(self._timed("optcond", (nk, nw), lib().wbmi_optical_conductivity_ws, self.h, nk, nw, X.shape[1],
+                               _ptr(E), _ptr(X), prec, ctypes.byref(params), _ptr(work), wbytes, _ptr(out)),
+                   "wbmi_optical_conductivity_ws")
 
     def jdos(self, E, params, out):
         nk, nw = E.shape

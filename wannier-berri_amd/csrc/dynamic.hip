@@ -20,7 +20,8 @@
 
 #define OC_THREADS 256
 #define OC_KCHUNK 16
-#define OC_NWMAX 32
+#define OC_NWMAX 128  // band groups of one k (LDS arrays of this size)
+#define OC_LDS_NW 32  // the per-k matrices (A, Y) live in LDS up to this nw, else in the workspace
 
 struct OcArgs {
     wbmi_optcond_params p;
@@ -57,14 +58,41 @@ __device__ inline double fermi_dirac(double E, double mu, double kBT) {
     return 1.0 / (exp((E - mu) / kBT) + 1.0);
 }
 
-template <typename T>
+// A^a_ij = i D_H^a_ij (+ Abar^a_ij), D_H = -V dEinv (0 where |E_i - E_j| < 1e-7), of every k into
+// Y[k][3][nw][nw] (KIND 0: -i A, the spin Hall Y), for nw > OC_LDS_NW where it does not fit in LDS.
+template <typename T, int KIND>
+__global__ void __launch_bounds__(256) k_build_y(int nw, int nch, wbmi_optcond_params p,
+                                                 const double* __restrict__ Eall, const cplx<T>* __restrict__ Xall,
+                                                 cplx<double>* __restrict__ Y) {
+    const int64_t k = blockIdx.x;
+    const int nw2 = nw * nw;
+    const double* E = Eall + k * nw;
+    const cplx<T>* Xk = Xall + k * nch * (int64_t)nw2;
+    for (int t = threadIdx.x; t < 3 * nw2; t += blockDim.x) {
+        const int c = t / nw2, i = (t % nw2) / nw, j = t % nw;
+        const double de = E[i] - E[j];
+        const double m = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+        const cplx<T> v = Xk[(int64_t)p.ch[WBMI_ROLE_V + c] * nw2 + i * nw + j];
+        cplx<double> av = mk((double)v.im * m, -(double)v.re * m);  // i * (-m V)
+        if (p.external_terms) {
+            const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + i * nw + j];
+            av.re += (double)x.re;
+            av.im += (double)x.im;
+        }
+        Y[k * 3 * nw2 + t] = KIND == 0 ? mk(av.im, -av.re) : av;
+    }
+}
+
+// GA: A read from the workspace (nw > OC_LDS_NW, filled by k_build_y) instead of LDS.
+template <typename T, bool GA>
 __global__ void __launch_bounds__(OC_THREADS) k_optcond(int nk, OcArgs a, const double* __restrict__ Eall,
-                                                        const cplx<T>* __restrict__ Xall, double* __restrict__ out) {
+                                                        const cplx<T>* __restrict__ Xall,
+                                                        const cplx<double>* __restrict__ Aws, double* __restrict__ out) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const wbmi_optcond_params& p = a.p;
     const int nw = a.nw, nw2 = nw * nw, tid = threadIdx.x;
-    cplx<double>* sA = (cplx<double>*)smem;  // [3][nw][nw]
-    cplx<double>* sM = sA + 3 * nw2;         // [OC_THREADS][9]
+    cplx<double>* sA = (cplx<double>*)smem;       // [3][nw][nw] (LDS form only)
+    cplx<double>* sM = sA + (GA ? 0 : 3 * nw2);  // [OC_THREADS][9]
     double* sEm = (double*)(sM + OC_THREADS * 9);
     double* sEn = sEm + OC_THREADS;
     double* sE = sEn + OC_THREADS;
@@ -86,19 +114,22 @@ __global__ void __launch_bounds__(OC_THREADS) k_optcond(int nk, OcArgs a, const 
         __syncthreads();
         if (tid == 0) sNg = band_groups(sE, nw, p.degen_thresh, p.degen_Kramers, sGb, sGm);
         // A^a_ij = i D_H^a_ij (+ Abar^a_ij), D_H = -V dEinv (0 where |E_i - E_j| < 1e-7)
-        const cplx<T>* Xk = Xall + (int64_t)k * a.nch * nw2;
-        for (int t = tid; t < 3 * nw2; t += OC_THREADS) {
-            const int c = t / nw2, i = (t % nw2) / nw, j = t % nw;
-            const double de = sE[i] - sE[j];
-            const double m = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
-            const cplx<T> v = Xk[(int64_t)p.ch[WBMI_ROLE_V + c] * nw2 + i * nw + j];
-            cplx<double> av = mk((double)v.im * m, -(double)v.re * m);  // i * (-m V)
-            if (p.external_terms) {
-                const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + i * nw + j];
-                av.re += (double)x.re;
-                av.im += (double)x.im;
+        const cplx<double>* Ak = GA ? Aws + (int64_t)k * 3 * nw2 : sA;
+        if (!GA) {
+            const cplx<T>* Xk = Xall + (int64_t)k * a.nch * nw2;
+            for (int t = tid; t < 3 * nw2; t += OC_THREADS) {
+                const int c = t / nw2, i = (t % nw2) / nw, j = t % nw;
+                const double de = sE[i] - sE[j];
+                const double m = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+                const cplx<T> v = Xk[(int64_t)p.ch[WBMI_ROLE_V + c] * nw2 + i * nw + j];
+                cplx<double> av = mk((double)v.im * m, -(double)v.re * m);  // i * (-m V)
+                if (p.external_terms) {
+                    const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + i * nw + j];
+                    av.re += (double)x.re;
+                    av.im += (double)x.im;
+                }
+                sA[t] = av;
             }
-            sA[t] = av;
         }
         __syncthreads();
         const int ng = sNg, ncand = ng * ng;
@@ -117,7 +148,7 @@ __global__ void __launch_bounds__(OC_THREADS) k_optcond(int nk, OcArgs a, const 
                         cplx<double> s = mk(0.0, 0.0);
                         for (int i = sGb[gm]; i < sGb[gm + 1]; ++i)
                             for (int j = sGb[gn]; j < sGb[gn + 1]; ++j)
-                                s += sA[(ca * nw + i) * nw + j] * sA[(cb * nw + j) * nw + i];
+                                s += Ak[(ca * nw + i) * nw + j] * Ak[(cb * nw + j) * nw + i];
                         sM[tid * 9 + ab] = times_i(s);
                     }
                 }
@@ -157,15 +188,50 @@ __global__ void __launch_bounds__(OC_THREADS) k_optcond(int nk, OcArgs a, const 
     }
 }
 
-extern "C" int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
-                                         const void* Xbar_dev, int prec, const wbmi_optcond_params* params,
-                                         double* out_dev) {
+extern "C" size_t wbmi_optical_conductivity_work_bytes(int nk, int nw) {
+    return nw > OC_LDS_NW ? sizeof(cplx<double>) * 3 * (size_t)nk * nw * nw : 0;
+}
+
+template <typename T>
+static int launch_optcond(wbmi_ctx* ctx, int nk, const OcArgs& a, const double* E, const void* X, void* work,
+                          double* out) {
+    const int nw = a.nw;
+    const bool ga = nw > OC_LDS_NW;
+    const int64_t nout = (int64_t)a.p.nEF * a.p.nomega;
+    dim3 g((unsigned)ceil_div(nout, OC_THREADS), (unsigned)ceil_div(nk, OC_KCHUNK));
+    const size_t lds = sizeof(cplx<double>) * ((ga ? 0 : 3 * (size_t)nw * nw) + OC_THREADS * 9) +
+                       sizeof(double) * (2 * OC_THREADS + 2 * OC_NWMAX) + sizeof(int) * (OC_THREADS + OC_NWMAX + 2);
+    if (ga) {
+        hipLaunchKernelGGL((k_build_y<T, 1>), dim3(nk), dim3(256), 0, ctx->stream, nw, a.nch, a.p, E,
+                           (const cplx<T>*)X, (cplx<double>*)work);
+        WBMI_LAUNCH_CHECK(ctx);
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_optcond<T, true>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_optcond<T, true>), g, dim3(OC_THREADS), lds, ctx->stream, nk, a, E, (const cplx<T>*)X,
+                           (const cplx<double>*)work, out);
+    } else {
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_optcond<T, false>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_optcond<T, false>), g, dim3(OC_THREADS), lds, ctx->stream, nk, a, E,
+                           (const cplx<T>*)X, (const cplx<double>*)nullptr, out);
+    }
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+static int optcond(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev, const void* Xbar_dev, int prec,
+                   const wbmi_optcond_params* params, void* work_dev, size_t work_bytes, double* out_dev) {
     if (!ctx) return WBMI_ERR_ARG;
     if (nk <= 0 || nw <= 0 || nch <= 0 || !E_dev || !Xbar_dev || !params || !out_dev || !params->Efermi ||
         !params->omega || params->nEF <= 0 || params->nomega <= 0)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_optical_conductivity: bad arguments");
     if (nw > OC_NWMAX)
         return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_optical_conductivity: nw <= %d", OC_NWMAX);
+    const size_t need = wbmi_optical_conductivity_work_bytes(nk, nw);
+    if (need > 0 && (!work_dev || work_bytes < need))
+        return wbmi_fail(ctx, WBMI_ERR_ARG,
+                         "wbmi_optical_conductivity: nw = %d > %d needs a workspace of %zu bytes "
+                         "(wbmi_optical_conductivity_ws)", nw, OC_LDS_NW, need);
     if (!(params->eta > 0) || params->kBT < 0)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_optical_conductivity: smearing width must be positive");
     for (int c = 0; c < 3; ++c) {
@@ -177,25 +243,21 @@ extern "C" int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch,
     a.p = *params;
     a.nw = nw;
     a.nch = nch;
-    const int64_t nout = (int64_t)params->nEF * params->nomega;
-    dim3 g((unsigned)ceil_div(nout, OC_THREADS), (unsigned)ceil_div(nk, OC_KCHUNK));
-    const size_t lds = sizeof(cplx<double>) * (3 * (size_t)nw * nw + OC_THREADS * 9) +
-                       sizeof(double) * (2 * OC_THREADS + 2 * OC_NWMAX) + sizeof(int) * (OC_THREADS + OC_NWMAX + 2);
-    if (prec == WBMI_F64) {
-        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_optcond<double>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_optcond<double>, g, dim3(OC_THREADS), lds, ctx->stream, nk, a, E_dev,
-                           (const cplx<double>*)Xbar_dev, out_dev);
-    } else if (prec == WBMI_F32) {
-        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_optcond<float>,
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_optcond<float>, g, dim3(OC_THREADS), lds, ctx->stream, nk, a, E_dev,
-                           (const cplx<float>*)Xbar_dev, out_dev);
-    } else {
-        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_optical_conductivity: bad precision");
-    }
-    WBMI_LAUNCH_CHECK(ctx);
-    return WBMI_OK;
+    if (prec == WBMI_F64) return launch_optcond<double>(ctx, nk, a, E_dev, Xbar_dev, work_dev, out_dev);
+    if (prec == WBMI_F32) return launch_optcond<float>(ctx, nk, a, E_dev, Xbar_dev, work_dev, out_dev);
+    return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_optical_conductivity: bad precision");
+}
+
+extern "C" int wbmi_optical_conductivity(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                                         const void* Xbar_dev, int prec, const wbmi_optcond_params* params,
+                                         double* out_dev) {
+    return optcond(ctx, nk, nw, nch, E_dev, Xbar_dev, prec, params, nullptr, 0, out_dev);
+}
+
+extern "C" int wbmi_optical_conductivity_ws(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_dev,
+                                            const void* Xbar_dev, int prec, const wbmi_optcond_params* params,
+                                            void* work_dev, size_t work_bytes, double* out_dev) {
+    return optcond(ctx, nk, nw, nch, E_dev, Xbar_dev, prec, params, work_dev, work_bytes, out_dev);
 }
 
 // Joint density of states (dynamic.py:145-167): per k, every ordered pair (m, n) of band
@@ -444,15 +506,17 @@ __global__ void __launch_bounds__(256) k_inject_x(int nk, int nw, int nch, wbmi_
 //   KIND 1: out[abc] += (f_n - f_m) (-Im P^{ca,b} - Im P^{ba,c}) (g(E_m - E_n - w) + g(E_n - E_m - w))
 //   KIND 2: out[abc] += (f_n - f_m) P^{ab,c} g(E_m - E_n - w)
 // (g = the smearing function).  One FP64 atomic add per output per workgroup.
-template <typename T, int KIND>
+// GY: Y read from the workspace (nw > OC_LDS_NW, filled by k_build_y) instead of LDS.
+template <typename T, int KIND, bool GY>
 __global__ void __launch_bounds__(OC_THREADS) k_pair27(int nk, wbmi_optcond_params p, int nw, int nch,
                                                        const double* __restrict__ Eall,
                                                        const cplx<T>* __restrict__ Xall,
-                                                       const cplx<double>* __restrict__ W, double* __restrict__ out) {
+                                                       const cplx<double>* __restrict__ W,
+                                                       const cplx<double>* __restrict__ Yws, double* __restrict__ out) {
     extern __shared__ __attribute__((aligned(16))) char smem[];
     const int nw2 = nw * nw, tid = threadIdx.x;
-    cplx<double>* sY = (cplx<double>*)smem;           // [3][nw][nw]
-    cplx<double>* sP = sY + 3 * nw2;                 // [P27_PAIRS][27]
+    cplx<double>* sYl = (cplx<double>*)smem;             // [3][nw][nw] (LDS form only)
+    cplx<double>* sP = sYl + (GY ? 0 : 3 * nw2);        // [P27_PAIRS][27]
     double* sEm = (double*)(sP + P27_PAIRS * 27);
     double* sEn = sEm + P27_PAIRS;
     double* sE = sEn + P27_PAIRS;
@@ -480,18 +544,21 @@ __global__ void __launch_bounds__(OC_THREADS) k_pair27(int nk, wbmi_optcond_para
         for (int i = tid; i < nw; i += OC_THREADS) sE[i] = Eall[(int64_t)k * nw + i];
         __syncthreads();
         if (tid == 0) sNg = band_groups(sE, nw, p.degen_thresh, p.degen_Kramers, sGb, sGm);
-        const cplx<T>* Xk = Xall + (int64_t)k * nch * nw2;
-        for (int t = tid; t < 3 * nw2; t += OC_THREADS) {
-            const int c = t / nw2, i = (t % nw2) / nw, j = t % nw;
-            const double de = sE[i] - sE[j];
-            const double m = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
-            const cplx<T> v = Xk[(int64_t)p.ch[WBMI_ROLE_V + c] * nw2 + i * nw + j];
-            cplx<double> a = mk(m * (double)v.im, -m * (double)v.re);  // i D_H = i (-m V)
-            if (p.external_terms) {
-                const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + i * nw + j];
-                a += mk((double)x.re, (double)x.im);
+        const cplx<double>* sY = GY ? Yws + (int64_t)k * 3 * nw2 : sYl;
+        if (!GY) {
+            const cplx<T>* Xk = Xall + (int64_t)k * nch * nw2;
+            for (int t = tid; t < 3 * nw2; t += OC_THREADS) {
+                const int c = t / nw2, i = (t % nw2) / nw, j = t % nw;
+                const double de = sE[i] - sE[j];
+                const double m = fabs(de) < 1e-7 ? 0.0 : 1.0 / de;
+                const cplx<T> v = Xk[(int64_t)p.ch[WBMI_ROLE_V + c] * nw2 + i * nw + j];
+                cplx<double> a = mk(m * (double)v.im, -m * (double)v.re);  // i D_H = i (-m V)
+                if (p.external_terms) {
+                    const cplx<T> x = Xk[(int64_t)p.ch[WBMI_ROLE_A + c] * nw2 + i * nw + j];
+                    a += mk((double)x.re, (double)x.im);
+                }
+                sYl[t] = KIND == 0 ? mk(a.im, -a.re) : a;  // KIND 0: -i A
             }
-            sY[t] = KIND == 0 ? mk(a.im, -a.re) : a;  // KIND 0: -i A
         }
         __syncthreads();
         const cplx<double>* Wk = W + (int64_t)k * 9 * nw2;
@@ -579,17 +646,30 @@ __global__ void __launch_bounds__(OC_THREADS) k_pair27(int nk, wbmi_optcond_para
     }
 }
 
+// Wk: the per-k workspace (X^x, 9 nw^2 per k); for nw > OC_LDS_NW, Y follows it (3 nw^2 per k).
 template <typename T, int KIND>
 static int launch_pair27(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E, const void* X,
                          const wbmi_optcond_params* params, const cplx<double>* Wk, double* out) {
+    const bool gy = nw > OC_LDS_NW;
     const int64_t nout = (int64_t)params->nEF * params->nomega;
     dim3 g((unsigned)ceil_div(nout, OC_THREADS), (unsigned)ceil_div(nk, OC_KCHUNK));
-    const size_t lds = sizeof(cplx<double>) * (3 * (size_t)nw * nw + P27_PAIRS * 27) +
+    const size_t lds = sizeof(cplx<double>) * ((gy ? 0 : 3 * (size_t)nw * nw) + P27_PAIRS * 27) +
                        sizeof(double) * (2 * P27_PAIRS + 2 * OC_NWMAX) + sizeof(int) * (P27_PAIRS + OC_NWMAX + 2);
-    WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_pair27<T, KIND>,
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL((k_pair27<T, KIND>), g, dim3(OC_THREADS), lds, ctx->stream, nk, *params, nw, nch, E,
-                       (const cplx<T>*)X, Wk, out);
+    if (gy) {
+        cplx<double>* Y = const_cast<cplx<double>*>(Wk) + 9 * (size_t)nk * nw * nw;
+        hipLaunchKernelGGL((k_build_y<T, KIND == 0 ? 0 : 1>), dim3(nk), dim3(256), 0, ctx->stream, nw, nch, *params,
+                           E, (const cplx<T>*)X, Y);
+        WBMI_LAUNCH_CHECK(ctx);
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_pair27<T, KIND, true>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_pair27<T, KIND, true>), g, dim3(OC_THREADS), lds, ctx->stream, nk, *params, nw, nch, E,
+                           (const cplx<T>*)X, Wk, (const cplx<double>*)Y, out);
+    } else {
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_pair27<T, KIND, false>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL((k_pair27<T, KIND, false>), g, dim3(OC_THREADS), lds, ctx->stream, nk, *params, nw, nch,
+                           E, (const cplx<T>*)X, Wk, (const cplx<double>*)nullptr, out);
+    }
     WBMI_LAUNCH_CHECK(ctx);
     return WBMI_OK;
 }
@@ -602,9 +682,8 @@ static int check_pair27(wbmi_ctx* ctx, const char* what, int nk, int nw, int nch
         return wbmi_fail(ctx, WBMI_ERR_ARG, "%s: bad arguments", what);
     if (nw > OC_NWMAX) return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "%s: nw <= %d", what, OC_NWMAX);
     if (!(p->eta > 0) || p->kBT < 0) return wbmi_fail(ctx, WBMI_ERR_ARG, "%s: smearing width must be positive", what);
-    if (!work || work_bytes < sizeof(cplx<double>) * 9 * (size_t)nk * nw * nw)
-        return wbmi_fail(ctx, WBMI_ERR_ARG, "%s: workspace of %zu bytes needed", what,
-                         sizeof(cplx<double>) * 9 * (size_t)nk * nw * nw);
+    if (!work || work_bytes < wbmi_spin_hall_work_bytes(nk, nw))
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "%s: workspace of %zu bytes needed", what, wbmi_spin_hall_work_bytes(nk, nw));
     auto bad = [&](int c) { return c < 0 || c >= nch; };
     for (int c = 0; c < 3; ++c)
         if (bad(p->ch[WBMI_ROLE_V + c]) || (p->external_terms && bad(p->ch[WBMI_ROLE_A + c])))
@@ -664,7 +743,8 @@ extern "C" int wbmi_injection_current(wbmi_ctx* ctx, int nk, int nw, int nch, co
 }
 
 extern "C" size_t wbmi_spin_hall_work_bytes(int nk, int nw) {
-    return sizeof(cplx<double>) * 9 * (size_t)nk * nw * nw;
+    // X^x per k (9 nw^2), then Y per k (3 nw^2) when it does not fit in LDS
+    return sizeof(cplx<double>) * (nw > OC_LDS_NW ? 12 : 9) * (size_t)nk * nw * nw;
 }
 
 template <typename T>
@@ -805,7 +885,7 @@ static int spin_berry(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_de
     if (!ctx) return WBMI_ERR_ARG;
     if (nk <= 0 || nw <= 0 || nch <= 0 || !E_dev || !Xbar_dev || !params || !out_dev)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: bad arguments");
-    if (nw > OC_NWMAX) return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_tabulate_spin_berry: nw <= %d", OC_NWMAX);
+    if (nw > OC_LDS_NW) return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_tabulate_spin_berry: nw <= %d", OC_LDS_NW);
     if (!work_dev || work_bytes < wbmi_spin_hall_work_bytes(nk, nw))
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate_spin_berry: workspace of %zu bytes needed",
                          wbmi_spin_hall_work_bytes(nk, nw));
