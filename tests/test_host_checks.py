@@ -186,3 +186,14 @@ def test_ws_dist_invariants(wb):
                 img = np.all(Rn % mp == np.array(c), axis=1)
                 orig = np.all(iR % mp == np.array(c), axis=1)
                 assert abs(Hn[img, m, n].sum() - H[orig, m, n].sum()) < 1e-12
+
+
+def test_static_calculator_band_options(wb):
+    """Emin / Emax of a static calculator are accepted without effect, as in the reference
+    (calculators/static.py:83 passes them to data_K.py:172-186, which does not use them);
+    select_bands is refused."""
+    st = wb.calculators.static
+    c = st.AHC(Efermi=np.linspace(-1, 1, 5), Emin=-0.5, Emax=0.5)
+    assert c.Emin == -0.5 and c.Emax == 0.5
+    with pytest.raises(NotImplementedError):
+        st.AHC(Efermi=np.linspace(-1, 1, 5), select_bands=[0, 1])

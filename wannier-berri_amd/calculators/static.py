@@ -79,7 +79,8 @@ class StaticCalculator(Calculator):
     """Fermi-level scan integrator (calculators/static.py:24-169).
 
     Subclasses set `formula` (a WBMI_FORMULA_* id), `ndim` and `fder`.
-    Not supported: select_bands, finite Emin/Emax.
+    Not supported: select_bands.  Emin / Emax are accepted and have no effect, as in the
+    reference (data_K.py:172-186 receives them but does not use them).
     """
     formula = None
     ndim = 0
@@ -92,8 +93,9 @@ class StaticCalculator(Calculator):
         self.k_resolved = bool(k_resolved)
         if select_bands is not None:
             raise NotImplementedError("select_bands is not available on the MI355X path")
-        if np.isfinite(Emin) or np.isfinite(Emax):
-            raise NotImplementedError("Emin/Emax band windows are not available on the MI355X path")
+        # accepted and, as in the reference, without effect: get_bands_in_range_groups_ik takes
+        # them but selects groups by the Fermi range alone (data_K.py:172-186)
+        self.Emin, self.Emax = Emin, Emax
         self.kwargs_formula = dict(kwargs_formula or {})
         for k in self.kwargs_formula:
             if k not in ("internal_terms", "external_terms", "sign"):
