@@ -139,16 +139,24 @@ __global__ void k_pass_phases(int N0, int N1, int N2, int L0, int L1, int L2, in
 // out[a][j][b] = sum_l P[j][l] in[a][l][b] on the FP64 matrix cores (v_mfma_f64_16x16x4f64:
 // A[l%16][l/16], B[l/16][l%16], D[l/16 + 4v][l%16]).  One wave per 16 consecutive b of one a:
 // the B operand is in[a][l][b] (16 b per row, 4 rows l per step), every 16-row block of j is an
-// accumulator pair (re, im) fed from the phases in LDS (the A operand), so each input element is
+// accumulator triple (the 3M form, cmac3) fed from the phases in LDS (the A operand), so each input element is
 // read once and each output element written once.  Rows j come in chunks of 16 NB (NB = 1, 2, 4
-// accumulator pairs, chosen from N at launch); K = l in chunks of DM_LCHUNK, whose loads are
+// accumulator triples, chosen from N at launch); K = l in chunks of DM_LCHUNK, whose loads are
 // issued before their products.
+// Complex multiply-accumulate with three real products on the matrix cores (the 3M form):
+// s += Pr Xr, t += Pi Xi, u += (Pr + Pi)(Xr + Xi); at the end re = s - t, im = u - s - t.
+__device__ inline void cmac3(double pr, double pi, double xr, double xi, wbmi_v4d& s, wbmi_v4d& t, wbmi_v4d& u) {
+    s = __builtin_amdgcn_mfma_f64_16x16x4f64(pr, xr, s, 0, 0, 0);
+    t = __builtin_amdgcn_mfma_f64_16x16x4f64(pi, xi, t, 0, 0, 0);
+    u = __builtin_amdgcn_mfma_f64_16x16x4f64(pr + pi, xr + xi, u, 0, 0, 0);
+}
+
 #define DM_WAVES 4
 #define DM_LCHUNK 12
 
 // (minimum waves per SIMD per variant: the accumulators stay in VGPRs and nothing spills)
 template <typename Tin, typename Tout, int NB, bool PAIR>
-__global__ void __launch_bounds__(64 * DM_WAVES, (PAIR ? (NB == 1 ? 6 : NB == 2 ? 4 : 2) : (NB == 1 ? 8 : NB == 2 ? 5 : 4))) k_dft_mfma(const cplx<Tin>* __restrict__ in, cplx<Tout>* __restrict__ out,
+__global__ void __launch_bounds__(64 * DM_WAVES, (PAIR ? (NB == 1 ? 3 : NB == 2 ? 2 : 1) : (NB == 1 ? 8 : NB == 2 ? 5 : 2))) k_dft_mfma(const cplx<Tin>* __restrict__ in, cplx<Tout>* __restrict__ out,
                                                           const cplx<double>* __restrict__ P, int64_t A, int L, int N,
                                                           int64_t B) {
     // PAIR (complex64 in and out, B even): a lane owns the adjacent columns b, b + 1 (16-byte
@@ -167,13 +175,14 @@ __global__ void __launch_bounds__(64 * DM_WAVES, (PAIR ? (NB == 1 ? 6 : NB == 2 
     const cplx<Tin>* src = in + a * L * B + b;
     cplx<Tout>* dst = out + a * N * B + b;
     for (int j0 = 0; j0 < N; j0 += 16 * NB) {
-        wbmi_v4d cr[CW][NB], ci[CW][NB];
+        wbmi_v4d as[CW][NB], at[CW][NB], au[CW][NB];
 #pragma unroll
         for (int c = 0; c < CW; ++c)
 #pragma unroll
             for (int q = 0; q < NB; ++q) {
-                cr[c][q] = wbmi_v4d{0.0, 0.0, 0.0, 0.0};
-                ci[c][q] = wbmi_v4d{0.0, 0.0, 0.0, 0.0};
+                as[c][q] = wbmi_v4d{0.0, 0.0, 0.0, 0.0};
+                at[c][q] = as[c][q];
+                au[c][q] = as[c][q];
             }
         const int nb = min(16 * NB, N - j0);
         for (int l0 = 0; l0 < L; l0 += DM_LCHUNK) {
@@ -204,15 +213,8 @@ __global__ void __launch_bounds__(64 * DM_WAVES, (PAIR ? (NB == 1 ? 6 : NB == 2 
                     const int j = j0 + 16 * q + li;
                     const cplx<double> ph = (j < N && l < L) ? sP[j * L + l] : mk(0.0, 0.0);
 #pragma unroll
-                    for (int c = 0; c < CW; ++c) {
-                        cr[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.re, (double)x[ks][c].re, cr[c][q], 0, 0, 0);
-                        ci[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.re, (double)x[ks][c].im, ci[c][q], 0, 0, 0);
-                    }
-#pragma unroll
-                    for (int c = 0; c < CW; ++c) {
-                        cr[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ph.im, (double)x[ks][c].im, cr[c][q], 0, 0, 0);
-                        ci[c][q] = __builtin_amdgcn_mfma_f64_16x16x4f64(ph.im, (double)x[ks][c].re, ci[c][q], 0, 0, 0);
-                    }
+                    for (int c = 0; c < CW; ++c)
+                        cmac3(ph.re, ph.im, (double)x[ks][c].re, (double)x[ks][c].im, as[c][q], at[c][q], au[c][q]);
                 }
             }
         }
@@ -224,16 +226,163 @@ __global__ void __launch_bounds__(64 * DM_WAVES, (PAIR ? (NB == 1 ? 6 : NB == 2 
                 for (int v = 0; v < 4; ++v) {
                     const int j = j0 + 16 * q + g + 4 * v;
                     if (j >= N) continue;
+                    const double r0 = as[0][q][v] - at[0][q][v], i0 = au[0][q][v] - as[0][q][v] - at[0][q][v];
                     if constexpr (PAIR) {
-                        *(float4*)(dst + (int64_t)j * B) = make_float4((float)cr[0][q][v], (float)ci[0][q][v],
-                                                                      (float)cr[CW - 1][q][v], (float)ci[CW - 1][q][v]);
+                        const double r1 = as[CW - 1][q][v] - at[CW - 1][q][v];
+                        const double i1 = au[CW - 1][q][v] - as[CW - 1][q][v] - at[CW - 1][q][v];
+                        *(float4*)(dst + (int64_t)j * B) = make_float4((float)r0, (float)i0, (float)r1, (float)i1);
                     } else {
-                        dst[(int64_t)j * B] = mk((Tout)cr[0][q][v], (Tout)ci[0][q][v]);
+                        dst[(int64_t)j * B] = mk((Tout)r0, (Tout)i0);
                     }
                 }
             }
         }
     }
+}
+
+// The y and x passes in one kernel: out[j0][j1][c] = sum_{a0,a1} P0[j0][a0] P1[j1][a1] in[a0][a1][c].
+// A workgroup owns 16 CW consecutive columns c (CW = 2: complex64 column pairs, as in k_dft_mfma).
+// Its waves first form T[a0][j1][c] (the y pass, one task per a0 and 16 rows j1) into LDS as
+// complex128, then the x pass reads T from LDS as the MFMA B operand (one task per j1 and 16 rows
+// j0), so the y-pass output never goes to HBM: per column the kernel reads L0 L1 and writes N0 N1
+// elements.  LDS: P0 [N0][L0], P1 [N1][L1], T [L0][N1p][16 CW] (N1p = N1 rounded up to 16).
+#ifndef DM2_WAVES
+#define DM2_WAVES 8
+#endif
+#ifndef DM2_FUSE
+#define DM2_FUSE 1
+#endif
+
+template <typename Tin, typename Tout, bool PAIR>
+__global__ void __launch_bounds__(64 * DM2_WAVES, 2) k_dft2_mfma(const cplx<Tin>* __restrict__ in, cplx<Tout>* __restrict__ out,
+                                                            const cplx<double>* __restrict__ P0,
+                                                            const cplx<double>* __restrict__ P1, int L0, int L1, int N0,
+                                                            int N1, int64_t C) {
+    static_assert(!PAIR || (sizeof(Tin) == 4 && sizeof(Tout) == 4), "column pairs are for complex64");
+    constexpr int CW = PAIR ? 2 : 1;
+    constexpr int COLS = 16 * CW;
+    extern __shared__ cplx<double> lds2[];
+    const int N1p = (N1 + 15) & ~15;
+    cplx<double>* sP0 = lds2;                 // [N0][L0]
+    cplx<double>* sP1 = sP0 + N0 * L0;        // [N1][L1]
+    cplx<double>* sT = sP1 + N1 * L1;         // [L0][N1p][COLS]
+    for (int t = threadIdx.x; t < N0 * L0; t += blockDim.x) sP0[t] = P0[t];
+    for (int t = threadIdx.x; t < N1 * L1; t += blockDim.x) sP1[t] = P1[t];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4, wv = threadIdx.x >> 6;
+    const int64_t c0 = (int64_t)blockIdx.x * COLS;
+    const int64_t b = c0 + CW * li;
+    const bool bok = b < C;  // PAIR: C is even, so b + 1 < C as well
+
+    // y pass: task (a0, 16-row block of j1)
+    const int nj1 = N1p / 16;
+    for (int task = wv; task < L0 * nj1; task += DM2_WAVES) {
+        const int a0 = task / nj1, j1b = 16 * (task % nj1);
+        const cplx<Tin>* src = in + (int64_t)a0 * L1 * C + b;
+        wbmi_v4d as[CW], at[CW], au[CW];
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            as[c] = wbmi_v4d{0.0, 0.0, 0.0, 0.0};
+            at[c] = as[c];
+            au[c] = as[c];
+        }
+        for (int l0 = 0; l0 < L1; l0 += DM_LCHUNK) {
+            cplx<Tin> x[DM_LCHUNK / 4][CW];
+#pragma unroll
+            for (int ks = 0; ks < DM_LCHUNK / 4; ++ks) {
+                const int l = l0 + 4 * ks + g;
+#pragma unroll
+                for (int c = 0; c < CW; ++c) x[ks][c] = mk((Tin)0, (Tin)0);
+                if (l < L1 && bok) {
+                    if constexpr (PAIR) {
+                        const float4 v = *(const float4*)(src + (int64_t)l * C);
+                        x[ks][0] = mk((Tin)v.x, (Tin)v.y);
+                        x[ks][CW - 1] = mk((Tin)v.z, (Tin)v.w);
+                    } else {
+                        x[ks][0] = src[(int64_t)l * C];
+                    }
+                }
+            }
+#pragma unroll
+            for (int ks = 0; ks < DM_LCHUNK / 4; ++ks) {
+                if (l0 + 4 * ks >= L1) break;
+                const int l = l0 + 4 * ks + g, j = j1b + li;
+                const cplx<double> ph = (j < N1 && l < L1) ? sP1[j * L1 + l] : mk(0.0, 0.0);
+#pragma unroll
+                for (int c = 0; c < CW; ++c)
+                    cmac3(ph.re, ph.im, (double)x[ks][c].re, (double)x[ks][c].im, as[c], at[c], au[c]);
+            }
+        }
+        // rows j1b + g + 4v of T (rows up to N1p exist; those >= N1 hold zeros and are not read)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+#pragma unroll
+            for (int c = 0; c < CW; ++c)
+                sT[((int64_t)a0 * N1p + j1b + g + 4 * v) * COLS + CW * li + c] =
+                    mk(as[c][v] - at[c][v], au[c][v] - as[c][v] - at[c][v]);
+    }
+    __syncthreads();
+
+    // x pass: task (j1, 16-row block of j0); the B operand T[a0][j1][col] comes from LDS
+    const int nj0 = (N0 + 15) / 16;
+    for (int task = wv; task < N1 * nj0; task += DM2_WAVES) {
+        const int j1 = task / nj0, j0b = 16 * (task % nj0);
+        wbmi_v4d as[CW], at[CW], au[CW];
+#pragma unroll
+        for (int c = 0; c < CW; ++c) {
+            as[c] = wbmi_v4d{0.0, 0.0, 0.0, 0.0};
+            at[c] = as[c];
+            au[c] = as[c];
+        }
+        for (int a = 0; a < L0; a += 4) {
+            const int a0 = a + g, j = j0b + li;
+            const cplx<double> ph = (j < N0 && a0 < L0) ? sP0[j * L0 + a0] : mk(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < CW; ++c) {
+                const cplx<double> x = a0 < L0 ? sT[((int64_t)a0 * N1p + j1) * COLS + CW * li + c] : mk(0.0, 0.0);
+                cmac3(ph.re, ph.im, x.re, x.im, as[c], at[c], au[c]);
+            }
+        }
+        if (bok) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int j0 = j0b + g + 4 * v;
+                if (j0 >= N0) continue;
+                cplx<Tout>* dst = out + ((int64_t)j0 * N1 + j1) * C + b;
+                const double r0 = as[0][v] - at[0][v], i0 = au[0][v] - as[0][v] - at[0][v];
+                if constexpr (PAIR) {
+                    const double r1 = as[CW - 1][v] - at[CW - 1][v], i1 = au[CW - 1][v] - as[CW - 1][v] - at[CW - 1][v];
+                    *(float4*)dst = make_float4((float)r0, (float)i0, (float)r1, (float)i1);
+                } else {
+                    *dst = mk((Tout)r0, (Tout)i0);
+                }
+            }
+        }
+    }
+}
+
+// LDS bytes of k_dft2_mfma
+static size_t dft2_lds(int L0, int L1, int N0, int N1, int cw) {
+    const size_t N1p = (size_t)((N1 + 15) & ~15);
+    return sizeof(cplx<double>) * ((size_t)N0 * L0 + (size_t)N1 * L1 + (size_t)L0 * N1p * 16 * cw);
+}
+
+template <typename Tin, typename Tout, bool PAIR>
+static int launch_pass2(wbmi_ctx* ctx, const void* in, void* out, const cplx<double>* P0, const cplx<double>* P1,
+                        int L0, int L1, int N0, int N1, int64_t C) {
+    constexpr int CW = PAIR ? 2 : 1;
+    const size_t lds = dft2_lds(L0, L1, N0, N1, CW);
+    const int64_t blocks = ceil_div(C, 16 * CW);
+    if (lds > 160 * 1024 || blocks > 0x7fffffff)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_exec: fused pass of %d x %d -> %d x %d too large", L0, L1, N0, N1);
+    if (lds > 64 * 1024) {
+        WBMI_HIP_CHECK(ctx, hipFuncSetAttribute((const void*)k_dft2_mfma<Tin, Tout, PAIR>,
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    hipLaunchKernelGGL((k_dft2_mfma<Tin, Tout, PAIR>), dim3((unsigned)blocks), dim3(64 * DM2_WAVES), lds, ctx->stream,
+                       (const cplx<Tin>*)in, (cplx<Tout>*)out, P0, P1, L0, L1, N0, N1, C);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
 }
 
 static void grid_2d(int64_t A, dim3& g) {
@@ -409,19 +558,28 @@ int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* XR_dev, voi
     char* w1 = (char*)work_dev + r2k_phase_bytes(p);
     char* w2 = w1 + esz * (size_t)L0 * L1 * N2 * E;
     int rc;
-    // z pass: [L0 L1][L2][E] -> [L0 L1][N2][E]
+    // z pass: [L0 L1][L2][E] -> [L0 L1][N2][E]; then the y and x passes, fused through LDS when
+    // their intermediate fits ([L0][L1][N2 E] -> [N0][N1][N2 E]), else as two passes through w2
+    const int64_t C = (int64_t)N2 * E;
+    const bool pair = prec != WBMI_F64 && (C & 1) == 0 && (((uintptr_t)w1 | (uintptr_t)out_dev) & 15) == 0;
+    const bool fuse = DM2_FUSE && dft2_lds(L0, L1, N0, N1, pair ? 2 : 1) <= 160 * 1024;
     if (prec == WBMI_F64) {
         rc = launch_pass<double, double>(ctx, box_dev, w1, phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
         if (rc) return rc;
-        rc = launch_pass<double, double>(ctx, w1, w2, phase + 1 * p->tab, L0, L1, N1, (int64_t)N2 * E);
+        if (fuse) return launch_pass2<double, double, false>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
+        rc = launch_pass<double, double>(ctx, w1, w2, phase + 1 * p->tab, L0, L1, N1, C);
         if (rc) return rc;
-        rc = launch_pass<double, double>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * N2 * E);
+        rc = launch_pass<double, double>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * C);
     } else {
         rc = launch_pass<double, float>(ctx, box_dev, w1, phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
         if (rc) return rc;
-        rc = launch_pass<float, float>(ctx, w1, w2, phase + 1 * p->tab, L0, L1, N1, (int64_t)N2 * E);
+        if (fuse) {
+            if (pair) return launch_pass2<float, float, true>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
+            return launch_pass2<float, float, false>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
+        }
+        rc = launch_pass<float, float>(ctx, w1, w2, phase + 1 * p->tab, L0, L1, N1, C);
         if (rc) return rc;
-        rc = launch_pass<float, float>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * N2 * E);
+        rc = launch_pass<float, float>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * C);
     }
     return rc;
 }
