@@ -185,6 +185,18 @@ int wbmi_r2k_box(wbmi_ctx* ctx, const wbmi_r2k_plan* plan, const void* XR_dev, v
  * aliases (R extent > NKFFT), in which case box_dev is rebuilt per call.    */
 int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* plan, const void* XR_dev,
                   void* box_dev, const double dK[3], void* work_dev, void* out_dev, int prec);
+/* wbmi_r2k_exec in two steps, for plans whose box does not depend on dK (no axis aliases):
+ * the z pass box -> w1 ([L0*L1][N2][nelem] complex of `prec`, wbmi_r2k_w1_bytes) depends only
+ * on dK[2], so a caller that evaluates many K-blocks of one grid can keep w1 resident per
+ * distinct dK[2] and run only the y and x passes per K-block (wbmi_r2k_yx).  The reference
+ * recomputes the whole FFT per K-block (fourier/fft.py:58-192, grid/Kpoint.py K-block loop);
+ * the split is this library's, the result equals wbmi_r2k_exec's.  Both steps take the
+ * workspace of wbmi_r2k_plan_sizes.                                                      */
+int wbmi_r2k_w1_bytes(const wbmi_r2k_plan* plan, int prec, size_t* bytes);
+int wbmi_r2k_zpass(wbmi_ctx* ctx, const wbmi_r2k_plan* plan, const void* box_dev, const double dK[3],
+                   void* work_dev, void* w1_dev, int prec);
+int wbmi_r2k_yx(wbmi_ctx* ctx, const wbmi_r2k_plan* plan, const void* w1_dev, const double dK[3],
+                void* work_dev, void* out_dev, int prec);
 
 /* ---- R -> k on an arbitrary k-list ---------------------------------------
  * Phase-matrix transform (fourier/fft.py:125-130,170-171), no expdK phase.

@@ -126,6 +126,75 @@ def test_r2k_odd_columns(wb, ctx):
     assert rel_err(outs[1], outs[0]) < 2e-7
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_r2k_split_zpass(wb, ctx, prec):
+    """The z pass kept apart (wbmi_r2k_zpass once, wbmi_r2k_yx per K-block) gives exactly the
+    whole transform (wbmi_r2k_exec) for shifts sharing dK[2]; a z pass of another dK[2] does
+    not (the cache must be keyed on dK[2]); an aliased plan refuses the split."""
+    import torch
+    say(f"test_r2k_split_zpass prec={prec}")
+    from wannierberri_amd import models
+    from wannierberri_amd.data_k import device_system
+    s = models.synthetic(num_wann=6, Rmax=2, seed=23)
+    keys = [("Ham", 0), ("AA", 0)]
+    dev = device_system(s, ctx)
+    stack = dev.stack(keys)
+    NKFFT = (6, 5, 7)
+    plan, box, aliased = dev.plan(keys, NKFFT)
+    assert not aliased
+    nk = int(np.prod(NKFFT))
+    cdt = torch.complex128 if prec == 0 else torch.complex64
+    work = torch.empty(plan.work_bytes, dtype=torch.uint8, device="cuda")
+    w1 = torch.empty(plan.w1_bytes(prec), dtype=torch.uint8, device="cuda")
+    plan.zpass(box, [0.1, 0.2, 0.3], work, w1, prec)
+    for dK in ([0.1, 0.2, 0.3], [-0.05, 0.4, 0.3], [0.1, 0.2, 0.35]):
+        X1 = torch.empty((nk, stack.shape[1], 6, 6), dtype=cdt, device="cuda")
+        X2 = torch.empty_like(X1)
+        plan.yx(w1, dK, work, X1, prec)
+        plan.execute(None, box, dK, work, X2, prec)
+        if dK[2] == 0.3:
+            assert torch.equal(X1, X2), dK
+        else:
+            assert rel_err(X1.cpu().numpy(), X2.cpu().numpy()) > 1e-3
+    plan_a, _, aliased_a = dev.plan(keys, (3, 5, 7))
+    assert aliased_a
+    with pytest.raises(ValueError, match="aliases R"):
+        plan_a.zpass(box, [0.1, 0.2, 0.3], work, w1, prec)
+
+
+def test_r2k_zcache_run(wb, ctx, monkeypatch):
+    """run() on a grid with the resident z-pass outputs equals run() without them (FP64, 1e-12;
+    each K-block's transform is bitwise the same, test_r2k_split_zpass), and the K-blocks after
+    the first of each dK[2] take their z pass from the cache."""
+    say("test_r2k_zcache_run")
+    from wannierberri_amd import models
+    from wannierberri_amd.data_k import DeviceSystem
+    s = models.synthetic(num_wann=8, Rmax=2, seed=24)
+    Ef = np.linspace(-4, 4, 17)
+    st = wb.calculators.static
+
+    def calcs():
+        return {"ahc": st.AHC(Efermi=Ef), "bcd": st.BerryDipole_FermiSea(Efermi=Ef)}
+    uses = []
+    orig = DeviceSystem.zpass_cached
+
+    def counted(self, keys, NKFFT, plan, box, dK, prec, work):
+        key = (tuple(keys), tuple(int(x) for x in NKFFT), int(prec), float(dK[2]))
+        hit = key in self._zcache
+        r = orig(self, keys, NKFFT, plan, box, dK, prec, work)
+        uses.append((key, hit, r is not None))
+        return r
+    monkeypatch.setattr(DeviceSystem, "zpass_cached", counted)
+    r1 = run_gpu(wb, s, calcs(), NK=(10, 10, 10), NKFFT=(5, 5, 5))
+    # 8 K-blocks, 2 distinct dK[2]: one z pass per (channel set, dK[2]), every other use a hit
+    assert len(uses) >= 8 and all(u[2] for u in uses)
+    assert sum(not u[1] for u in uses) == len({u[0] for u in uses}) and len({u[0][3] for u in uses}) == 2
+    monkeypatch.setattr(DeviceSystem, "zcache_limit", 0)
+    r2 = run_gpu(wb, s, calcs(), NK=(10, 10, 10), NKFFT=(5, 5, 5))
+    for k in ("ahc", "bcd"):   # not bitwise: the eigensolver's warm start and the bin sums' order vary
+        assert rel_err(r1.results[k].data, r2.results[k].data) < 1e-12, k
+
+
 def test_r2k_klist(wb, ctx):
     import torch
     say("test_r2k_klist")

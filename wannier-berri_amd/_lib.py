@@ -59,7 +59,8 @@ SYMBOLS = [
     "wbmi_static_scan_k", "wbmi_finalize_k", "wbmi_static_scan_tetra_k", "wbmi_finalize_tetra_k",
     "wbmi_tabulate_invmass", "wbmi_eigh_noalloc", "wbmi_eigh_noalloc_max_bytes", "wbmi_r2k_klist_tile",
     "wbmi_r2k_klist_tile_shape", "wbmi_tabulate_spin_berry", "wbmi_spin_berry_bands", "wbmi_der3e_bands",
-    "wbmi_optical_conductivity_work_bytes", "wbmi_optical_conductivity_ws",
+    "wbmi_optical_conductivity_work_bytes", "wbmi_optical_conductivity_ws", "wbmi_r2k_w1_bytes", "wbmi_r2k_zpass",
+    "wbmi_r2k_yx",
 ]
 
 
@@ -114,6 +115,9 @@ _PROTOS = {
     "wbmi_r2k_plan_sizes": (_i, [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     "wbmi_r2k_box": (_i, [_vp, _vp, _vp, _vp]),
     "wbmi_r2k_exec": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "wbmi_r2k_w1_bytes": (_i, [_vp, _i, ctypes.POINTER(ctypes.c_size_t)]),
+    "wbmi_r2k_zpass": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "wbmi_r2k_yx": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i]),
     "wbmi_r2k_klist": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _vp, _i]),
     "wbmi_r2k_klist_tile": (_i, [_vp, _i, _vp, _i64, _vp, _i, _vp, _vp, _i, _i]),
     "wbmi_r2k_klist_tile_shape": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
@@ -484,6 +488,23 @@ class R2KPlan:
         self.ctx.check(self.ctx._timed("r2k", (out.shape[0], out[0].numel(), self.box_bytes, self.work_bytes),
                                        lib().wbmi_r2k_exec, self.ctx.h, self.h, _ptr(XR), _ptr(box), dk, _ptr(work),
                                        _ptr(out), prec), "wbmi_r2k_exec")
+
+    def w1_bytes(self, prec):
+        """Bytes of the z-pass output (wbmi_r2k_zpass) in the form `prec`."""
+        b = ctypes.c_size_t()
+        self.ctx.check(lib().wbmi_r2k_w1_bytes(self.h, prec, ctypes.byref(b)), "wbmi_r2k_w1_bytes")
+        return b.value
+
+    def zpass(self, box, dK, work, w1, prec):
+        dk = (ctypes.c_double * 3)(*[float(x) for x in dK])
+        self.ctx.check(self.ctx._timed("r2k_z", (w1.numel(), self.box_bytes), lib().wbmi_r2k_zpass, self.ctx.h, self.h,
+                                       _ptr(box), dk, _ptr(work), _ptr(w1), prec), "wbmi_r2k_zpass")
+
+    def yx(self, w1, dK, work, out, prec):
+        dk = (ctypes.c_double * 3)(*[float(x) for x in dK])
+        self.ctx.check(self.ctx._timed("r2k", (out.shape[0], out[0].numel(), self.box_bytes, self.work_bytes),
+                                       lib().wbmi_r2k_yx, self.ctx.h, self.h, _ptr(w1), dk, _ptr(work), _ptr(out),
+                                       prec), "wbmi_r2k_yx")
 
 
 _contexts = {}

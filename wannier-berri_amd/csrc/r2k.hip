@@ -531,12 +531,56 @@ int wbmi_r2k_box(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* XR_dev, void
     return WBMI_OK;
 }
 
+// pass phase tables of the shift dK into the head of the workspace
+static int r2k_phases(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const double dK[3], void* work_dev) {
+    double dpass[3];
+    for (int i = 0; i < 3; ++i) dpass[i] = p->alias[i] ? 0.0 : dK[i];
+    hipLaunchKernelGGL(k_pass_phases, dim3(4, 3), dim3(256), 0, ctx->stream, p->N[0], p->N[1], p->N[2], p->L[0],
+                       p->L[1], p->L[2], p->off[0], p->off[1], p->off[2], dpass[0], dpass[1], dpass[2], p->tab,
+                       (cplx<double>*)work_dev);
+    WBMI_LAUNCH_CHECK(ctx);
+    return WBMI_OK;
+}
+
+static size_t r2k_w1_elems(const wbmi_r2k_plan* p) { return (size_t)p->L[0] * p->L[1] * p->N[2] * p->nelem; }
+
+// z pass: box [L0 L1][L2][E] (complex128) -> w1 [L0 L1][N2][E] in the form's type
+static int r2k_z(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* box_dev, const cplx<double>* phase, void* w1,
+                 int prec) {
+    const int64_t A = (int64_t)p->L[0] * p->L[1];
+    if (prec == WBMI_F64) return launch_pass<double, double>(ctx, box_dev, w1, phase + 2 * p->tab, A, p->L[2], p->N[2], p->nelem);
+    return launch_pass<double, float>(ctx, box_dev, w1, phase + 2 * p->tab, A, p->L[2], p->N[2], p->nelem);
+}
+
+// y and x passes: w1 [L0][L1][N2 E] -> out [N0][N1][N2 E], fused through LDS when the
+// intermediate fits, else as two passes through w2 ([L0][N1][N2 E])
+static int r2k_yx(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* w1, const cplx<double>* phase, void* w2,
+                  void* out_dev, int prec) {
+    const int L0 = p->L[0], L1 = p->L[1], N0 = p->N[0], N1 = p->N[1];
+    const int64_t C = (int64_t)p->N[2] * p->nelem;
+    const bool pair = prec != WBMI_F64 && (C & 1) == 0 && (((uintptr_t)w1 | (uintptr_t)out_dev) & 15) == 0;
+    const bool fuse = DM2_FUSE && dft2_lds(L0, L1, N0, N1, pair ? 2 : 1) <= 160 * 1024;
+    int rc;
+    if (prec == WBMI_F64) {
+        if (fuse) return launch_pass2<double, double, false>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
+        rc = launch_pass<double, double>(ctx, w1, w2, phase + p->tab, L0, L1, N1, C);
+        if (rc) return rc;
+        return launch_pass<double, double>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * C);
+    }
+    if (fuse) {
+        if (pair) return launch_pass2<float, float, true>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
+        return launch_pass2<float, float, false>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
+    }
+    rc = launch_pass<float, float>(ctx, w1, w2, phase + p->tab, L0, L1, N1, C);
+    if (rc) return rc;
+    return launch_pass<float, float>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * C);
+}
+
 int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* XR_dev, void* box_dev, const double dK[3],
                   void* work_dev, void* out_dev, int prec) {
     if (!ctx || !p || !box_dev || !work_dev || !out_dev || !dK)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_exec: null argument");
     if (prec != WBMI_F64 && prec != WBMI_F32) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_exec: bad precision");
-    double dpass[3];
     if (p->any_alias) {
         if (!XR_dev) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_exec: aliased plan needs XR_dev");
         dim3 g((unsigned)ceil_div(p->nelem, 256));
@@ -546,42 +590,47 @@ int wbmi_r2k_exec(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* XR_dev, voi
                            p->alias[1] ? dK[1] : 0.0, p->alias[2] ? dK[2] : 0.0, (cplx<double>*)box_dev);
         WBMI_LAUNCH_CHECK(ctx);
     }
-    for (int i = 0; i < 3; ++i) dpass[i] = p->alias[i] ? 0.0 : dK[i];
-    cplx<double>* phase = (cplx<double>*)work_dev;
-    hipLaunchKernelGGL(k_pass_phases, dim3(4, 3), dim3(256), 0, ctx->stream, p->N[0], p->N[1], p->N[2], p->L[0],
-                       p->L[1], p->L[2], p->off[0], p->off[1], p->off[2], dpass[0], dpass[1], dpass[2], p->tab,
-                       phase);
-    WBMI_LAUNCH_CHECK(ctx);
-    const int64_t E = p->nelem;
-    const int L0 = p->L[0], L1 = p->L[1], L2 = p->L[2], N0 = p->N[0], N1 = p->N[1], N2 = p->N[2];
-    size_t esz = prec == WBMI_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+    int rc = r2k_phases(ctx, p, dK, work_dev);
+    if (rc) return rc;
+    const cplx<double>* phase = (const cplx<double>*)work_dev;
+    const size_t esz = prec == WBMI_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
     char* w1 = (char*)work_dev + r2k_phase_bytes(p);
-    char* w2 = w1 + esz * (size_t)L0 * L1 * N2 * E;
-    int rc;
-    // z pass: [L0 L1][L2][E] -> [L0 L1][N2][E]; then the y and x passes, fused through LDS when
-    // their intermediate fits ([L0][L1][N2 E] -> [N0][N1][N2 E]), else as two passes through w2
-    const int64_t C = (int64_t)N2 * E;
-    const bool pair = prec != WBMI_F64 && (C & 1) == 0 && (((uintptr_t)w1 | (uintptr_t)out_dev) & 15) == 0;
-    const bool fuse = DM2_FUSE && dft2_lds(L0, L1, N0, N1, pair ? 2 : 1) <= 160 * 1024;
-    if (prec == WBMI_F64) {
-        rc = launch_pass<double, double>(ctx, box_dev, w1, phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
-        if (rc) return rc;
-        if (fuse) return launch_pass2<double, double, false>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
-        rc = launch_pass<double, double>(ctx, w1, w2, phase + 1 * p->tab, L0, L1, N1, C);
-        if (rc) return rc;
-        rc = launch_pass<double, double>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * C);
-    } else {
-        rc = launch_pass<double, float>(ctx, box_dev, w1, phase + 2 * p->tab, (int64_t)L0 * L1, L2, N2, E);
-        if (rc) return rc;
-        if (fuse) {
-            if (pair) return launch_pass2<float, float, true>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
-            return launch_pass2<float, float, false>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
-        }
-        rc = launch_pass<float, float>(ctx, w1, w2, phase + 1 * p->tab, L0, L1, N1, C);
-        if (rc) return rc;
-        rc = launch_pass<float, float>(ctx, w2, out_dev, phase, 1, L0, N0, (int64_t)N1 * C);
-    }
-    return rc;
+    char* w2 = w1 + esz * r2k_w1_elems(p);
+    rc = r2k_z(ctx, p, box_dev, phase, w1, prec);
+    if (rc) return rc;
+    return r2k_yx(ctx, p, w1, phase, w2, out_dev, prec);
+}
+
+int wbmi_r2k_w1_bytes(const wbmi_r2k_plan* p, int prec, size_t* bytes) {
+    if (!p || !bytes || (prec != WBMI_F64 && prec != WBMI_F32)) return WBMI_ERR_ARG;
+    *bytes = (prec == WBMI_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>)) * r2k_w1_elems(p);
+    return WBMI_OK;
+}
+
+int wbmi_r2k_zpass(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* box_dev, const double dK[3], void* work_dev,
+                   void* w1_dev, int prec) {
+    if (!ctx || !p || !box_dev || !work_dev || !w1_dev || !dK)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_zpass: null argument");
+    if (prec != WBMI_F64 && prec != WBMI_F32) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_zpass: bad precision");
+    if (p->any_alias)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_zpass: the plan aliases R (its box depends on dK); use wbmi_r2k_exec");
+    int rc = r2k_phases(ctx, p, dK, work_dev);
+    if (rc) return rc;
+    return r2k_z(ctx, p, box_dev, (const cplx<double>*)work_dev, w1_dev, prec);
+}
+
+int wbmi_r2k_yx(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* w1_dev, const double dK[3], void* work_dev,
+                void* out_dev, int prec) {
+    if (!ctx || !p || !w1_dev || !work_dev || !out_dev || !dK)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_yx: null argument");
+    if (prec != WBMI_F64 && prec != WBMI_F32) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_yx: bad precision");
+    if (p->any_alias)
+        return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_yx: the plan aliases R (its box depends on dK); use wbmi_r2k_exec");
+    int rc = r2k_phases(ctx, p, dK, work_dev);
+    if (rc) return rc;
+    const size_t esz = prec == WBMI_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
+    char* w2 = (char*)work_dev + r2k_phase_bytes(p) + esz * r2k_w1_elems(p);
+    return r2k_yx(ctx, p, w1_dev, (const cplx<double>*)work_dev, w2, out_dev, prec);
 }
 
 }  // extern "C"

@@ -19,6 +19,8 @@ Xbar(name, der) for name in Ham, AA, rotAA, BB, CC and der in 0, 1 (2 for Ham),
 or of the spin matrices Xbar('SS'), Xbar('SA'), Xbar('SHA'), Xbar('SH'),
 Xbar('SR'), Xbar('SHR') (der 0).
 """
+import collections
+
 import numpy as np
 
 from . import _lib
@@ -77,6 +79,9 @@ def channel_terms(name, der, base_offset):
 class DeviceSystem:
     """Device copies of a System_R's R-space data (cached per system and device)."""
 
+    # bytes of z-pass outputs kept resident for the K-blocks of grids (zpass_cached); 0 = none
+    zcache_limit = 16 << 30
+
     def __init__(self, system, ctx):
         import torch
         self.torch = torch
@@ -100,6 +105,8 @@ class DeviceSystem:
         self.iRvec_dev = torch.from_numpy(self.iRvec).to(dev)
         self._stacks = {}
         self._plans = {}
+        self._zcache = collections.OrderedDict()   # z-pass outputs, least recently used first
+        self._zcache_used = 0
         self.eig_hint = None   # the last grid K-block's eigenvectors (starting basis of the next)
 
     def has(self, name):
@@ -142,6 +149,48 @@ class DeviceSystem:
                 plan.box(st, box)
             self._plans[key] = (plan, box, aliased)
         return self._plans[key]
+
+
+    def zcache_clear(self):
+        self._zcache.clear()
+        self._zcache_used = 0
+
+    def zpass_cached(self, keys, NKFFT, plan, box, dK, prec, work):
+        """The z-pass output of the grid transform of `keys` for shifts with this dK[2]
+        (wbmi_r2k_zpass): the K-blocks of a grid share a few distinct dK[2], so it is kept
+        resident (up to `zcache_limit` bytes, least recently used dropped first) and each
+        K-block runs only the y and x passes.  None when it does not fit (or under graph
+        capture): the caller then runs the whole transform.
+
+        Each use is ordered after the stream that wrote the buffer (an event) and recorded on
+        the using stream, so a dropped buffer is reused only after every stream that read it."""
+        torch = self.torch
+        if self.zcache_limit <= 0 or torch.cuda.is_current_stream_capturing():
+            return None
+        key = (tuple(keys), tuple(int(x) for x in NKFFT), int(prec), float(dK[2]))
+        cur = self.ctx.stream
+        ent = self._zcache.get(key)
+        if ent is not None:
+            self._zcache.move_to_end(key)
+            w1, ev = ent
+            cur.wait_event(ev)
+            w1.record_stream(cur)
+            return w1
+        nb = plan.w1_bytes(prec)
+        if nb > self.zcache_limit:
+            return None
+        while self._zcache_used + nb > self.zcache_limit:
+            _, (old, _) = self._zcache.popitem(last=False)
+            self._zcache_used -= old.numel()
+        w1 = torch.empty(nb, dtype=torch.uint8, device=self.ctx.device)
+        w1.record_stream(cur)
+        plan.zpass(box, dK, work, w1, prec)
+        ev = torch.cuda.Event()
+        ev.record(cur)
+        self._zcache[key] = (w1, ev)
+        self._zcache_used += nb
+        return w1
+
 
 
 def _is_aliased(iRvec, NKFFT):
@@ -245,10 +294,15 @@ class Data_K:
         X = torch.empty((nk, stack.shape[1], nw, nw), dtype=dtype, device=self.ctx.device)
         if self.k_list is None:
             plan, box, aliased = self.dev.plan(keys, self.NKFFT)
+            dK = self.dK if shift is None else shift
             work = torch.empty(plan.work_bytes, dtype=torch.uint8, device=self.ctx.device)
-            if aliased:
-                box = torch.empty(plan.box_bytes // 16, dtype=torch.complex128, device=self.ctx.device)
-            plan.execute(stack if aliased else None, box, self.dK if shift is None else shift, work, X, prec)
+            w1 = None if aliased else self.dev.zpass_cached(keys, self.NKFFT, plan, box, dK, prec, work)
+            if w1 is not None:
+                plan.yx(w1, dK, work, X, prec)
+            else:
+                if aliased:
+                    box = torch.empty(plan.box_bytes // 16, dtype=torch.complex128, device=self.ctx.device)
+                plan.execute(stack if aliased else None, box, dK, work, X, prec)
             if keep is not None:
                 keep.extend([work, box])
             del work

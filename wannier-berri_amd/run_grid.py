@@ -104,6 +104,7 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
         adpt_mesh = np.array(adpt_mesh)
 
     factors = np.array([Kp.factor for Kp in K_list])
+    _clear_zcache(system)   # the resident R -> k z-pass outputs live for one run
     t0 = time.time()
     counter = 0
     side = None
@@ -163,6 +164,7 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
             K_list += K_list[iK].divide(ndiv=adpt_mesh, periodic=system.periodic, use_symmetry=use_irred_kpt)
         if use_irred_kpt:
             exclude_equiv_points(K_list, new_points=len(K_list) - l1)
+    _clear_zcache(system)
     if verbose:
         print(f"run(): processed {counter} K-blocks in {time.time() - t0:.3f} s", flush=True)
     # k-resolved results onto the full grid, or in path order (run_grid.py:394-401)
@@ -173,6 +175,11 @@ def run(system, grid, calculators, adpt_num_iter=0, use_irred_kpt=True, symmetri
             else:
                 val.self_to_grid(grid.dense)
     return result_all
+
+
+def _clear_zcache(system):
+    for dev in getattr(system, "_device_cache", {}).values():
+        dev.zcache_clear()
 
 
 def _klist_batches(sizes, batch_points):
