@@ -249,6 +249,9 @@ __global__ void __launch_bounds__(64 * DM_WAVES, (PAIR ? (NB == 1 ? 3 : NB == 2 
 #ifndef DM2_WAVES
 #define DM2_WAVES 8
 #endif
+#ifndef DM2_NT
+#define DM2_NT 1
+#endif
 #ifndef DM2_FUSE
 #define DM2_FUSE 1
 #endif
@@ -352,9 +355,19 @@ __global__ void __launch_bounds__(64 * DM2_WAVES, 2) k_dft2_mfma(const cplx<Tin>
                 const double r0 = as[0][v] - at[0][v], i0 = au[0][v] - as[0][v] - at[0][v];
                 if constexpr (PAIR) {
                     const double r1 = as[CW - 1][v] - at[CW - 1][v], i1 = au[CW - 1][v] - as[CW - 1][v] - at[CW - 1][v];
+#if DM2_NT
+                    typedef float f4 __attribute__((ext_vector_type(4)));
+                    __builtin_nontemporal_store(f4{(float)r0, (float)i0, (float)r1, (float)i1}, (f4*)dst);
+#else
                     *(float4*)dst = make_float4((float)r0, (float)i0, (float)r1, (float)i1);
+#endif
                 } else {
+#if DM2_NT
+                    typedef Tout t2 __attribute__((ext_vector_type(2)));
+                    __builtin_nontemporal_store(t2{(Tout)r0, (Tout)i0}, (t2*)dst);
+#else
                     *dst = mk((Tout)r0, (Tout)i0);
+#endif
                 }
             }
         }
