@@ -105,6 +105,13 @@ typedef struct wbmi_scan_params {
     int degen_Kramers;
     int ch[WBMI_NROLES];
     int comp_slice;       /* rank-3 formulas: first tensor index of this call  */
+    /* select_bands (calculators/static.py:39,133,136; utility.py:398-403): device
+     * [nw] float64, the number of times each band is listed; a band group adds its
+     * value times the sum over its bands / its size (groups with none listed add
+     * nothing).  NULL = every band.  Fermi-level bins with fder >= 1 only (the
+     * reference refuses it for the Fermi sea, data_K.py:179-180); the tetrahedron
+     * and table entry points refuse a non-NULL value.                           */
+    const double* band_weight;
 } wbmi_scan_params;
 
 /* Parameters of the Kubo optical conductivity (calculators/dynamic.py:26-196). */

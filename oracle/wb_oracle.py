@@ -394,12 +394,18 @@ class OracleDataK:
         return self._cov[key]
 
     # -- band groups ------------------------------------------------------
-    def get_bands_in_range_groups_ik(self, ik, emin, emax, degen_thresh=-1, degen_Kramers=False, sea=False):
-        """data_K.py:172-186 with grid/tetrahedron.py:132-162."""
+    def get_bands_in_range_groups_ik(self, ik, emin, emax, degen_thresh=-1, degen_Kramers=False, sea=False,
+                                     select_bands=None):
+        """data_K.py:172-186 with grid/tetrahedron.py:132-162 (groups holding none of
+        `select_bands` are dropped, tetrahedron.py:147-149)."""
         E = self.E_K[ik]
         groups = band_groups(E, degen_thresh, degen_Kramers)
+        if select_bands is not None:
+            groups = [(a, b) for a, b in groups if set(range(a, b)) & set(int(x) for x in select_bands)]
         in_range = [(a, b) for a, b in groups if E[a:b].max() >= emin and E[a:b].min() <= emax]
         weights = {(a, b): E[a:b].mean() for a, b in in_range}
+        if sea and select_bands is not None:
+            raise NotImplementedError("Selection of bands for Fermi sea is not implemented")
         if sea:
             below = np.where(E < emin)[0]
             bandmax = below[-1] + 1 if len(below) > 0 else 0
@@ -409,9 +415,19 @@ class OracleDataK:
                 weights[(0, bandmax)] = -np.inf
         return weights
 
-    def get_bands_in_range_groups(self, emin, emax, degen_thresh=-1, degen_Kramers=False, sea=False):
-        return [self.get_bands_in_range_groups_ik(ik, emin, emax, degen_thresh, degen_Kramers, sea)
+    def get_bands_in_range_groups(self, emin, emax, degen_thresh=-1, degen_Kramers=False, sea=False,
+                                  select_bands=None):
+        return [self.get_bands_in_range_groups_ik(ik, emin, emax, degen_thresh, degen_Kramers, sea, select_bands)
                 for ik in range(self.nk)]
+
+
+def weight_select_bands(ib1, ib2, select_bands=None):
+    """Share of the group [ib1, ib2) that `select_bands` lists (utility.py:398-403); every
+    listed entry counts, repeats included."""
+    if select_bands is None:
+        return 1.0
+    select_bands = np.asarray(select_bands)
+    return np.sum((select_bands >= ib1) * (select_bands < ib2)) / (ib2 - ib1)
 
 
 def band_groups(E, degen_thresh, degen_Kramers=False):
@@ -1042,8 +1058,9 @@ class StaticCalculator:
 
     def __init__(self, Efermi, constant_factor=None, use_factor=True, kwargs_formula=None,
                  degen_thresh=1e-4, degen_Kramers=False, hole_like=False, Formula=None, fder=None,
-                 tetra=False):
+                 tetra=False, select_bands=None):
         self.Efermi = np.array(Efermi, dtype=float)
+        self.select_bands = None if select_bands is None else np.array(select_bands).reshape(-1)
         self.tetra = tetra
         self.hole_like = hole_like
         if Formula is not None:
@@ -1073,7 +1090,8 @@ class StaticCalculator:
                                           self.degen_thresh, self.degen_Kramers)
         else:
             weights = data_K.get_bands_in_range_groups(self.EFmin, self.EFmax, degen_thresh=self.degen_thresh,
-                                                       degen_Kramers=self.degen_Kramers, sea=(self.fder == 0))
+                                                       degen_Kramers=self.degen_Kramers, sea=(self.fder == 0),
+                                                       select_bands=self.select_bands)
         out = []
         for ik, bnd in enumerate(weights):
             vals = {}
@@ -1106,6 +1124,7 @@ class StaticCalculator:
         # Fermi scan (static.py:126-136)
         for groups in per_k:
             for n, (E, v) in sorted(groups.items()):
+                v = v * weight_select_bands(n[0], n[1], self.select_bands)
                 if E < self.EFmin:
                     restot += v[None]
                 elif E <= self.EFmax:

@@ -378,6 +378,35 @@ def test_ohmic(wb, results_run, precision, tol):
             assert e < tol, (name, k)
 
 
+def test_select_bands(wb):
+    """select_bands (static.py:133,136; utility.py:398-403) on the random model with pairs of
+    bands as groups (degen_Kramers) against the oracle: DOS, Berry curvature dipole and Ohmic
+    conductivity on the Fermi surface with the even bands and with the odd bands listed twice;
+    FP64 at 1e-8.  The Fermi sea and tetrahedra refuse it."""
+    st = wb.calculators.static
+    s = gsys(wb, "random")
+    o = osys("random")
+    nw = s.num_wann
+    E = np.linspace(-3, 3, 31)
+    NK, NKFFT = [4, 4, 4], [2, 2, 2]
+    kw = dict(degen_Kramers=True)
+    for sel in (list(range(0, nw, 2)), list(range(1, nw, 2)) * 2):
+        say(f"test_select_bands {sel[:4]}...")
+        r = run_gpu(wb, s, {"dos": st.DOS(Efermi=E, select_bands=sel, **kw),
+                            "bcd": st.BerryDipole_FermiSurf(Efermi=E, select_bands=sel, **kw),
+                            "ohm": st.Ohmic_FermiSurf(Efermi=E, select_bands=sel, **kw)}, NK, NKFFT)
+        ro = O.run(o, {"dos": O.DOS(E, select_bands=sel, **kw), "bcd": O.BerryDipole_FermiSurf(E, select_bands=sel, **kw),
+                       "ohm": O.Ohmic_FermiSurf(E, select_bands=sel, **kw)}, NK=NK, NKFFT=NKFFT)
+        for k in ("dos", "bcd", "ohm"):
+            e = rel_err(r.results[k].data, ro[k].data)
+            say(f"  {k}: {e:.2e}")
+            assert np.abs(ro[k].data).max() > 0 and e < 1e-8, k
+    with pytest.raises(NotImplementedError, match="Fermi sea"):
+        run_gpu(wb, s, {"ahc": st.AHC(Efermi=E, select_bands=[0])}, NK, NKFFT)
+    with pytest.raises(NotImplementedError):
+        st.DOS(Efermi=E, select_bands=[0], tetra=True)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
 def test_hall_nldrude(wb, results_run, precision, tol):
     """Classic Hall conductivity (Fermi surface: VelMassVel, sea: MassMass, contracted as

@@ -487,3 +487,30 @@ def test_fe_set_spin(results_run):
     for k, ref in (("spin", "fe_set_spin_spin"), ("cumdos", "fe_set_spin_cumdos"), ("shc", "fe_set_spin_SHCsimple"),
                    ("shc_int", "fe_set_spin_SHCsimple_internal")):
         assert rel_err(r[k].data, results_run[ref]) < 1e-8, k
+
+
+def test_select_bands_partition():
+    """select_bands (static.py:133,136; utility.py:398-403; tetrahedron.py:147-149) on the
+    random model with pairs of bands as groups (degen_Kramers): all bands selected gives the
+    unselected result; the even and the odd bands, selected apart, each give half of it (each
+    group of two adds half its value), and sum to it; listing the even bands twice counts twice;
+    the Fermi sea refuses a selection."""
+    s = gsys("random")
+    nw = s.matrices["Ham"].shape[-1]
+    E = np.linspace(-3, 3, 31)
+    kw = dict(degen_Kramers=True)   # groups of two bands (even borders only)
+
+    def run(sel):
+        c = {"dos": O.DOS(E, select_bands=sel, **kw), "bcd": O.BerryDipole_FermiSurf(E, select_bands=sel, **kw)}
+        return O.run(s, c, NK=[4, 4, 4], NKFFT=[2, 2, 2])
+    full, every = run(None), run(list(range(nw)))
+    even, odd = run(list(range(0, nw, 2))), run(list(range(1, nw, 2)))
+    twice = run(list(range(0, nw, 2)) * 2)
+    for k in ("dos", "bcd"):
+        assert np.abs(full[k].data).max() > 0
+        assert np.allclose(every[k].data, full[k].data, rtol=0, atol=1e-12 * np.abs(full[k].data).max())
+        assert np.allclose(even[k].data + odd[k].data, full[k].data, rtol=0, atol=1e-12 * np.abs(full[k].data).max())
+        assert np.allclose(even[k].data, 0.5 * full[k].data, rtol=0, atol=1e-12 * np.abs(full[k].data).max())
+        assert np.allclose(twice[k].data, full[k].data, rtol=0, atol=1e-12 * np.abs(full[k].data).max())
+    with pytest.raises(NotImplementedError, match="Fermi sea"):
+        O.run(s, {"ahc": O.AHC(E, select_bands=[0])}, NK=[2, 2, 2], NKFFT=[2, 2, 2])

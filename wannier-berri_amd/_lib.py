@@ -78,7 +78,7 @@ class ScanParams(ctypes.Structure):
         ("external_terms", ctypes.c_int), ("fder", ctypes.c_int), ("nEF_extra", ctypes.c_int),
         ("EFmin", ctypes.c_double), ("EFmax", ctypes.c_double), ("dEF", ctypes.c_double),
         ("degen_thresh", ctypes.c_double), ("degen_Kramers", ctypes.c_int),
-        ("ch", ctypes.c_int * NROLES), ("comp_slice", ctypes.c_int),
+        ("ch", ctypes.c_int * NROLES), ("comp_slice", ctypes.c_int), ("band_weight", ctypes.c_void_p),
     ]
 
 

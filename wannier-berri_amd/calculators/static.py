@@ -7,6 +7,8 @@ on the K-block and the prefix-sum / fder-stencil / scaling kernel
 (wbmi_static_scan_tetra, corner energies from Data_K.corner_energies) and
 wbmi_finalize_tetra.  The result stays on the GPU until read.
 """
+import functools
+
 import numpy as np
 
 from .. import _lib
@@ -75,11 +77,28 @@ def _external(kwargs_formula, system):
     return bool(ext) and not system.force_internal_terms_only
 
 
+def _with_band_weights(call):
+    """Every __call__ of a static calculator (composites included) runs with the device
+    select_bands weights of the call in place (StaticCalculator.scan_params reads them); a
+    nested call reuses the outer call's."""
+    @functools.wraps(call)
+    def wrapped(self, data_K):
+        if self.select_bands is None or getattr(self, "_bw_now", None) is not None:
+            return call(self, data_K)
+        self._bw_now = self._band_weights(data_K)
+        try:
+            return call(self, data_K)
+        finally:
+            self._bw_now = None
+    return wrapped
+
+
 class StaticCalculator(Calculator):
     """Fermi-level scan integrator (calculators/static.py:24-169).
 
     Subclasses set `formula` (a WBMI_FORMULA_* id), `ndim` and `fder`.
-    Not supported: select_bands.  Emin / Emax are accepted and have no effect, as in the
+    select_bands (Fermi-level bins, fder >= 1): each band group's value is weighted by the
+    share of its bands listed (static.py:133,136).  Emin / Emax are accepted and have no effect, as in the
     reference (data_K.py:172-186 receives them but does not use them).
     """
     formula = None
@@ -91,8 +110,10 @@ class StaticCalculator(Calculator):
                  **kwargs):
         super().__init__(**kwargs)
         self.k_resolved = bool(k_resolved)
-        if select_bands is not None:
-            raise NotImplementedError("select_bands is not available on the MI355X path")
+        self.select_bands = None if select_bands is None else np.array(select_bands, dtype=np.int64).reshape(-1)
+        self._band_w = {}
+        if self.select_bands is not None and tetra:
+            raise NotImplementedError("select_bands with the tetrahedron method is not available on the MI355X path")
         # accepted and, as in the reference, without effect: get_bands_in_range_groups_ik takes
         # them but selects groups by the Fermi range alone (data_K.py:172-186)
         self.Emin, self.Emax = Emin, Emax
@@ -175,7 +196,25 @@ class StaticCalculator(Calculator):
         p.degen_thresh = float(self.degen_thresh)
         p.degen_Kramers = int(bool(self.degen_Kramers))
         p.comp_slice = 0
+        bw = getattr(self, "_bw_now", None)
+        p.band_weight = bw.data_ptr() if bw is not None else None
         return p
+
+    def _band_weights(self, data_K):
+        """select_bands as a device [nw] float64 of listing counts (utility.py:398-403 counts
+        every listed entry inside a group, so repeated bands count repeatedly); None without a
+        selection.  The reference refuses a selection for the Fermi sea (data_K.py:179-180)."""
+        if self.select_bands is None:
+            return None
+        if self.fder == 0:
+            raise NotImplementedError("Selection of bands for Fermi sea is not implemented")
+        nw = data_K.num_wann
+        key = (data_K.ctx.device, nw)
+        if key not in self._band_w:
+            sel = self.select_bands[(self.select_bands >= 0) & (self.select_bands < nw)]
+            w = np.bincount(sel, minlength=nw).astype(np.float64)
+            self._band_w[key] = data_K.torch.from_numpy(w).to(data_K.ctx.device)
+        return self._band_w[key]
 
     def _slices(self, data_K, run):
         """run(comp_slice) -> [..., nEF, 9] per slice, joined to [..., nEF, 27] (first index slowest)."""
@@ -243,6 +282,11 @@ class StaticCalculator(Calculator):
         """Per-band values [nk, nw, 3^ndim] (float64, device) for WBMI_FORMULA_BANDVALUES
         calculators, formed by the calculator before the scan; None for the others."""
         return None
+
+    def __init_subclass__(cls, **kwargs):
+        super().__init_subclass__(**kwargs)
+        if "__call__" in cls.__dict__:
+            cls.__call__ = _with_band_weights(cls.__dict__["__call__"])
 
     def __call__(self, data_K):
         torch = data_K.torch
@@ -319,6 +363,9 @@ def spin_current_params(data_K, kind, ext, degen_thresh, degen_Kramers):
 
 
 # -- implemented calculators (calculators/static.py:192-498) --------------------
+
+
+StaticCalculator.__call__ = _with_band_weights(StaticCalculator.__call__)
 
 class _DOS(StaticCalculator):
     formula = _lib.FORMULA_IDENTITY

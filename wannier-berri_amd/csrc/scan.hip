@@ -93,6 +93,8 @@ extern "C" int wbmi_static_scan(wbmi_ctx* ctx, int nk, int nw, int nch, const do
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: bad arguments (nw <= %d)", SC_NWMAX);
     if (params->nEF_extra <= 0 || !(params->dEF > 0) || params->fder < 0 || params->fder > 3)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan: bad Fermi grid");
+    if (params->band_weight && params->fder == 0)
+        return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan: selection of bands for the Fermi sea is not implemented");
     int rc = check_roles(ctx, params, nch);
     if (rc) return rc;
     ScanArgs a;
@@ -145,6 +147,8 @@ static int scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_de
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad arguments (nw <= %d)", SC_NWMAX);
     if (params->fder < -1 || params->fder > 3 || !(params->EFmax >= params->EFmin))
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad fder or Fermi range");
+    if (params->band_weight)
+        return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan_tetra: band selection is not available with tetrahedra");
     int rc = check_roles(ctx, params, nch);
     if (rc) return rc;
     ScanArgs a;
@@ -182,6 +186,8 @@ extern "C" int wbmi_static_scan_k(wbmi_ctx* ctx, int nk, int nw, int nch, const 
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_k: bad arguments (nw <= %d)", SC_NWMAX);
     if (params->nEF_extra <= 0 || !(params->dEF > 0) || params->fder < 0 || params->fder > 3)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_k: bad Fermi grid");
+    if (params->band_weight && params->fder == 0)
+        return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan_k: selection of bands for the Fermi sea is not implemented");
     int rc = check_roles(ctx, params, nch);
     if (rc) return rc;
     ScanArgs a;
@@ -207,6 +213,7 @@ extern "C" int wbmi_tabulate(wbmi_ctx* ctx, int nk, int nw, int nch, const doubl
     if (nk <= 0 || nw <= 0 || nw > SC_NWMAX || nch <= 0 || !E_dev || !params || !out_dev ||
         (params->formula != WBMI_FORMULA_IDENTITY && !Xbar_dev))
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate: bad arguments (nw <= %d)", SC_NWMAX);
+    if (params->band_weight) return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_tabulate: band_weight must be NULL");
     int rc = check_roles(ctx, params, nch);
     if (rc) return rc;
     ScanArgs a;

@@ -1162,6 +1162,14 @@ static __device__ void tetra_accumulate(const Smem& S, const ScanArgs& args, int
     }
 }
 
+// select_bands share of the group [a, b) (utility.py:398-403): 1 without a selection
+__device__ inline double group_selected(const wbmi_scan_params& p, int a, int b) {
+    if (!p.band_weight) return 1.0;
+    double s = 0.0;
+    for (int n = a; n < b; ++n) s += p.band_weight[n];
+    return s / (double)(b - a);
+}
+
 // Fermi-level bin of a group with mean energy E (static.py:131-136): 0 for
 // E < EFmin (added to every level), ceil((E - EFmin)/dEF) up to EFmax, -1 = none.
 __device__ inline int fermi_bin(const wbmi_scan_params& p, double E) {
@@ -1442,7 +1450,7 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>()), (scan_min_waves<
             if (bin < 0) continue;
             double s = 0.0;
             for (int n = S.gb[g]; n < S.gb[g + 1]; ++n) s += S.acc[n * 9 + c].re;
-            unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s);
+            unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s * group_selected(p, S.gb[g], S.gb[g + 1]));
         }
         __syncthreads();
         if (sea_hi > sea_lo) {
@@ -1500,7 +1508,7 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>()), (scan_min_waves<
             const int bin = S.gin[g] ? fermi_bin(p, S.gw[g]) : -1;
             if (bin < 0) continue;
             const double s = S.cut[S.gb[g + 1] * 9 + c] - S.cut[S.gb[g] * 9 + c];
-            unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s);
+            unsafeAtomicAdd(hist + (int64_t)bin * ncomp + c, args.weight * s * group_selected(p, S.gb[g], S.gb[g + 1]));
         }
         if (sea_hi > sea_lo)
             for (int c = tid; c < ncomp; c += (int)blockDim.x)
