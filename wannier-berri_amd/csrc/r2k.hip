@@ -9,6 +9,7 @@
 //  * wbmi_r2k_klist: phase-matrix transform for arbitrary k-points (FP64 MFMA GEMM).
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "mfma_c64.h"
@@ -252,6 +253,9 @@ __global__ void __launch_bounds__(64 * DM_WAVES, (PAIR ? (NB == 1 ? 3 : NB == 2 
 #ifndef DM2_NT
 #define DM2_NT 1
 #endif
+#ifndef DM2_MIXED_PAIR
+#define DM2_MIXED_PAIR 1
+#endif
 #ifndef DM2_FUSE
 #define DM2_FUSE 1
 #endif
@@ -264,11 +268,14 @@ __global__ void __launch_bounds__(64 * DM2_WAVES, 2) k_dft2_mfma(const cplx<Tin>
     static_assert(!PAIR || (sizeof(Tin) == 4 && sizeof(Tout) == 4), "column pairs are for complex64");
     constexpr int CW = PAIR ? 2 : 1;
     constexpr int COLS = 16 * CW;
+    // the slab in the storage type of the form (complex64 in the mixed form, as the pass outputs it
+    // replaces), so that with one column per lane it fits beside a scan workgroup on a CU
+    using TT = typename std::conditional<sizeof(Tin) == 4, float, double>::type;
     extern __shared__ cplx<double> lds2[];
     const int N1p = (N1 + 15) & ~15;
-    cplx<double>* sP0 = lds2;                 // [N0][L0]
-    cplx<double>* sP1 = sP0 + N0 * L0;        // [N1][L1]
-    cplx<double>* sT = sP1 + N1 * L1;         // [L0][N1p][COLS]
+    cplx<double>* sP0 = lds2;                          // [N0][L0]
+    cplx<double>* sP1 = sP0 + N0 * L0;                 // [N1][L1]
+    cplx<TT>* sT = (cplx<TT>*)(sP1 + N1 * L1);         // [L0][N1p][COLS]
     for (int t = threadIdx.x; t < N0 * L0; t += blockDim.x) sP0[t] = P0[t];
     for (int t = threadIdx.x; t < N1 * L1; t += blockDim.x) sP1[t] = P1[t];
     __syncthreads();
@@ -322,7 +329,7 @@ __global__ void __launch_bounds__(64 * DM2_WAVES, 2) k_dft2_mfma(const cplx<Tin>
 #pragma unroll
             for (int c = 0; c < CW; ++c)
                 sT[((int64_t)a0 * N1p + j1b + g + 4 * v) * COLS + CW * li + c] =
-                    mk(as[c][v] - at[c][v], au[c][v] - as[c][v] - at[c][v]);
+                    mk((TT)(as[c][v] - at[c][v]), (TT)(au[c][v] - as[c][v] - at[c][v]));
     }
     __syncthreads();
 
@@ -342,8 +349,8 @@ __global__ void __launch_bounds__(64 * DM2_WAVES, 2) k_dft2_mfma(const cplx<Tin>
             const cplx<double> ph = (j < N0 && a0 < L0) ? sP0[j * L0 + a0] : mk(0.0, 0.0);
 #pragma unroll
             for (int c = 0; c < CW; ++c) {
-                const cplx<double> x = a0 < L0 ? sT[((int64_t)a0 * N1p + j1) * COLS + CW * li + c] : mk(0.0, 0.0);
-                cmac3(ph.re, ph.im, x.re, x.im, as[c], at[c], au[c]);
+                const cplx<TT> x = a0 < L0 ? sT[((int64_t)a0 * N1p + j1) * COLS + CW * li + c] : mk((TT)0, (TT)0);
+                cmac3(ph.re, ph.im, (double)x.re, (double)x.im, as[c], at[c], au[c]);
             }
         }
         if (bok) {
@@ -374,17 +381,17 @@ __global__ void __launch_bounds__(64 * DM2_WAVES, 2) k_dft2_mfma(const cplx<Tin>
     }
 }
 
-// LDS bytes of k_dft2_mfma
-static size_t dft2_lds(int L0, int L1, int N0, int N1, int cw) {
+// LDS bytes of k_dft2_mfma (tsz: bytes of a slab element, 16 or 8)
+static size_t dft2_lds(int L0, int L1, int N0, int N1, int cw, size_t tsz) {
     const size_t N1p = (size_t)((N1 + 15) & ~15);
-    return sizeof(cplx<double>) * ((size_t)N0 * L0 + (size_t)N1 * L1 + (size_t)L0 * N1p * 16 * cw);
+    return sizeof(cplx<double>) * ((size_t)N0 * L0 + (size_t)N1 * L1) + tsz * (size_t)L0 * N1p * 16 * cw;
 }
 
 template <typename Tin, typename Tout, bool PAIR>
 static int launch_pass2(wbmi_ctx* ctx, const void* in, void* out, const cplx<double>* P0, const cplx<double>* P1,
                         int L0, int L1, int N0, int N1, int64_t C) {
     constexpr int CW = PAIR ? 2 : 1;
-    const size_t lds = dft2_lds(L0, L1, N0, N1, CW);
+    const size_t lds = dft2_lds(L0, L1, N0, N1, CW, 2 * sizeof(Tin));
     const int64_t blocks = ceil_div(C, 16 * CW);
     if (lds > 160 * 1024 || blocks > 0x7fffffff)
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_r2k_exec: fused pass of %d x %d -> %d x %d too large", L0, L1, N0, N1);
@@ -571,8 +578,9 @@ static int r2k_yx(wbmi_ctx* ctx, const wbmi_r2k_plan* p, const void* w1, const c
                   void* out_dev, int prec) {
     const int L0 = p->L[0], L1 = p->L[1], N0 = p->N[0], N1 = p->N[1];
     const int64_t C = (int64_t)p->N[2] * p->nelem;
-    const bool pair = prec != WBMI_F64 && (C & 1) == 0 && (((uintptr_t)w1 | (uintptr_t)out_dev) & 15) == 0;
-    const bool fuse = DM2_FUSE && dft2_lds(L0, L1, N0, N1, pair ? 2 : 1) <= 160 * 1024;
+    const bool pair = DM2_MIXED_PAIR && prec != WBMI_F64 && (C & 1) == 0 &&
+                      (((uintptr_t)w1 | (uintptr_t)out_dev) & 15) == 0;
+    const bool fuse = DM2_FUSE && dft2_lds(L0, L1, N0, N1, pair ? 2 : 1, prec == WBMI_F64 ? 16 : 8) <= 160 * 1024;
     int rc;
     if (prec == WBMI_F64) {
         if (fuse) return launch_pass2<double, double, false>(ctx, w1, out_dev, phase, phase + p->tab, L0, L1, N0, N1, C);
