@@ -112,6 +112,12 @@ typedef struct wbmi_scan_params {
      * reference refuses it for the Fermi sea, data_K.py:179-180); the tetrahedron
      * and table entry points refuse a non-NULL value.                           */
     const double* band_weight;
+    /* Emin / Emax of the tetrahedron method (grid/tetrahedron.py:246-258), read when
+     * erange != 0: the Fermi-sea group (fder 0) leaves out the bands whose energies over
+     * centre and corners all lie below Emin; the hole-like group (fder -1) those above
+     * Emax.  Fermi-level bins ignore them, as the reference does (data_K.py:172-186). */
+    int erange;
+    double Emin, Emax;
 } wbmi_scan_params;
 
 /* Parameters of the Kubo optical conductivity (calculators/dynamic.py:26-196). */

@@ -524,7 +524,7 @@ def tetra_band_weight(Efermi, e_center, e_corners, der=0):
     return occ / 12.
 
 
-def tetra_group_weights(data_K, Efermi, der, degen_thresh=-1, degen_Kramers=False):
+def tetra_group_weights(data_K, Efermi, der, degen_thresh=-1, degen_Kramers=False, Erange=(-np.inf, np.inf)):
     """Per k: {(ib1, ib2): weight over Efermi} (TetraWeights.weights_all_band_groups,
     tetrahedron.py:212-255).  Groups are split on the central energies; a group is kept
     when its energies over the centre and corners reach [Efermi[0], Efermi[-1]] (140-152);
@@ -554,15 +554,19 @@ def tetra_group_weights(data_K, Efermi, der, degen_thresh=-1, degen_Kramers=Fals
             bandmax = below[-1] + 1 if len(below) else 0
             if in_range:
                 bandmax = min(bandmax, in_range[0][0])
-            if bandmax > 0:
-                w[(0, bandmax)] = np.ones(len(Efermi))
+            lowest = np.where(Emax < Erange[0])[0]    # bands entirely below Emin (246-251)
+            bandmin = lowest[-1] + 1 if len(lowest) else 0
+            if bandmax > bandmin:
+                w[(bandmin, bandmax)] = np.ones(len(Efermi))
         if der == -1:
             above = np.where(Emin > Efermi[-1])[0]
             bandmin = above[0] if len(above) else nb
             if in_range:
                 bandmin = max(bandmin, in_range[-1][1])
-            if nb > bandmin:
-                w[(bandmin, nb)] = np.ones(len(Efermi))
+            highest = np.where(Emin > Erange[1])[0]   # bands entirely above Emax (252-258)
+            bandmax = highest[0] if len(highest) else nb
+            if bandmax > bandmin:
+                w[(bandmin, bandmax)] = np.ones(len(Efermi))
         out.append(w)
     return out
 
@@ -1058,8 +1062,9 @@ class StaticCalculator:
 
     def __init__(self, Efermi, constant_factor=None, use_factor=True, kwargs_formula=None,
                  degen_thresh=1e-4, degen_Kramers=False, hole_like=False, Formula=None, fder=None,
-                 tetra=False, select_bands=None):
+                 tetra=False, select_bands=None, Emin=-np.inf, Emax=np.inf):
         self.Efermi = np.array(Efermi, dtype=float)
+        self.Erange = (Emin, Emax)   # tetra only (tetrahedron.py:246-258); bins ignore them
         self.select_bands = None if select_bands is None else np.array(select_bands).reshape(-1)
         self.tetra = tetra
         self.hole_like = hole_like
@@ -1087,7 +1092,7 @@ class StaticCalculator:
         formula = self.Formula(data_K, **self.kwargs_formula)
         if self.tetra:
             weights = tetra_group_weights(data_K, self.Efermi, -1 if self.hole_like else self.fder,
-                                          self.degen_thresh, self.degen_Kramers)
+                                          self.degen_thresh, self.degen_Kramers, self.Erange)
         else:
             weights = data_K.get_bands_in_range_groups(self.EFmin, self.EFmax, degen_thresh=self.degen_thresh,
                                                        degen_Kramers=self.degen_Kramers, sea=(self.fder == 0),

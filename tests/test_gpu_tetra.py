@@ -155,6 +155,38 @@ def test_tetra_hole_like_vs_oracle(wb):
         assert err < 1e-10, k
 
 
+def test_tetra_erange_vs_oracle(wb):
+    """Emin / Emax with the tetrahedron method (tetrahedron.py:246-258): the Fermi sea leaves out
+    the bands entirely below Emin (so its group has "out" bands on both sides), the hole-like
+    group those entirely above Emax.  AHC (additive), Morb (non-additive) and CumDOS on a random
+    Hermitian model vs the oracle, FP64 at 1e-10."""
+    say("test_tetra_erange_vs_oracle")
+    st = wb.calculators.static
+    from wannierberri_amd import models
+    s = models.synthetic(num_wann=8, Rmax=2, seed=11, with_BBCC=True)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA", "BB", "CC")})
+    E0 = np.linalg.eigvalsh(s.get_R_mat("Ham")[np.where((s.rvec.iRvec == 0).all(1))[0][0]])
+    lo, hi = E0.min(), E0.max()
+    NK, NKFFT = [4, 4, 4], [2, 2, 2]
+    for name, Ef, kw in (("sea", np.linspace(0.5 * (lo + hi), hi + 0.5, 17), dict(Emin=lo + 0.3 * (hi - lo))),
+                         ("hole", np.linspace(lo - 0.5, 0.5 * (lo + hi), 17),
+                          dict(hole_like=True, Emax=hi - 0.3 * (hi - lo)))):
+        say(f"  {name}: {kw}")
+        gpu = {"ahc": st.AHC(Efermi=Ef, tetra=True, **kw), "morb": st.Morb(Efermi=Ef, tetra=True, **kw),
+               "cumdos": st.CumDOS(Efermi=Ef, tetra=True, **kw)}
+        ora = {"ahc": O.AHC(Ef, tetra=True, **kw), "morb": O.Morb(Ef, tetra=True, **kw),
+               "cumdos": O.CumDOS(Ef, tetra=True, **kw)}
+        r = wb.run(s, wb.Grid(system=s, NK=NK, NKFFT=NKFFT), gpu, save_files=False, verbose=False)
+        ref = _oracle_tetra(o, ora, NK, NKFFT)
+        full = _oracle_tetra(o, {"cumdos": O.CumDOS(Ef, tetra=True, hole_like=kw.get("hole_like", False))}, NK, NKFFT)
+        assert np.abs(full["cumdos"].data - ref["cumdos"].data).max() > 0.1   # the bound removes bands
+        for k in gpu:
+            err = rel_err(r.results[k].data, ref[k].data)
+            say(f"  {k}: {err:.2e}")
+            assert err < 1e-10, (name, k)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])
 def test_chiral_tetragrid(wb, results_run, precision, tol):
     """The tetrahedral grid (GridTetra(length=8, NKFFT=[5, 5, 2]), one tetrahedron per

@@ -227,3 +227,29 @@ def test_te_qe_gme(wb):
     big = np.abs(bcd) > 1e-6 * np.abs(bcd).max()
     assert big.sum() > 20
     assert np.abs(ref["te_berry_dipole"][big] / bcd[big] / O.FACTOR_NLAHC - 1).max() < 1e-8
+
+
+def test_tetra_erange_cumdos(wb):
+    """Emin / Emax of the tetrahedron method (tetrahedron.py:246-258) on the Haldane model
+    (bands in [-3.01, -0.58] and [0.58, 3.01]), Fermi levels in the gap: the CumDOS Fermi sea
+    is the valence band, one state per k; Emin = -0.3 (above the valence band) leaves it out
+    (CumDOS 0), Emin = -3.5 keeps it; the hole-like CumDOS loses the conduction band with
+    Emax = 0.3 and keeps it with Emax = 3.5; infinite bounds change nothing."""
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_haldane.npz"))
+    _OracleBlock.osys = O.load_golden_system(os.path.join(GOLDEN, "sys_haldane.npz"))
+    Ef = np.linspace(-0.4, 0.4, 9)
+    grid = wb.Grid(system=s, NK=[6, 6, 1], NKFFT=[3, 3, 1])
+
+    def cum(**c):
+        calcs = {"c": _OracleCalc(wb, O.CumDOS(Ef, tetra=True, **c), 1, 1)}
+        r = wb.run(s, grid, calcs, data_k_class=_OracleBlock, save_files=False, verbose=False)
+        return r.results["c"].data
+    base = cum()
+    assert np.allclose(base, 1.0, rtol=0, atol=1e-12)
+    assert np.allclose(cum(Emin=-np.inf, Emax=np.inf), base, rtol=0, atol=1e-12)
+    assert np.allclose(cum(Emin=-3.5), base, rtol=0, atol=1e-12)
+    assert np.allclose(cum(Emin=-0.3), 0.0, rtol=0, atol=1e-12)
+    hb = cum(hole_like=True)
+    assert np.allclose(np.abs(hb), 1.0, rtol=0, atol=1e-12)
+    assert np.allclose(cum(hole_like=True, Emax=3.5), hb, rtol=0, atol=1e-12)
+    assert np.allclose(cum(hole_like=True, Emax=0.3), 0.0, rtol=0, atol=1e-12)

@@ -79,6 +79,7 @@ class ScanParams(ctypes.Structure):
         ("EFmin", ctypes.c_double), ("EFmax", ctypes.c_double), ("dEF", ctypes.c_double),
         ("degen_thresh", ctypes.c_double), ("degen_Kramers", ctypes.c_int),
         ("ch", ctypes.c_int * NROLES), ("comp_slice", ctypes.c_int), ("band_weight", ctypes.c_void_p),
+        ("erange", ctypes.c_int), ("Emin", ctypes.c_double), ("Emax", ctypes.c_double),
     ]
 
 

@@ -98,7 +98,8 @@ class StaticCalculator(Calculator):
 
     Subclasses set `formula` (a WBMI_FORMULA_* id), `ndim` and `fder`.
     select_bands (Fermi-level bins, fder >= 1): each band group's value is weighted by the
-    share of its bands listed (static.py:133,136).  Emin / Emax are accepted and have no effect, as in the
+    share of its bands listed (static.py:133,136).  Emin / Emax bound the Fermi-sea (hole-like) group of the
+    tetrahedron method (tetrahedron.py:246-258) and have no effect on Fermi-level bins, as in the
     reference (data_K.py:172-186 receives them but does not use them).
     """
     formula = None
@@ -114,9 +115,10 @@ class StaticCalculator(Calculator):
         self._band_w = {}
         if self.select_bands is not None and tetra:
             raise NotImplementedError("select_bands with the tetrahedron method is not available on the MI355X path")
-        # accepted and, as in the reference, without effect: get_bands_in_range_groups_ik takes
-        # them but selects groups by the Fermi range alone (data_K.py:172-186)
-        self.Emin, self.Emax = Emin, Emax
+        # Fermi-level bins: without effect, as in the reference (get_bands_in_range_groups_ik
+        # takes them but selects groups by the Fermi range alone, data_K.py:172-186); tetra:
+        # they bound the Fermi-sea / hole-like group (tetrahedron.py:246-258)
+        self.Emin, self.Emax = float(Emin), float(Emax)
         self.kwargs_formula = dict(kwargs_formula or {})
         for k in self.kwargs_formula:
             if k not in ("internal_terms", "external_terms", "sign"):
@@ -198,6 +200,8 @@ class StaticCalculator(Calculator):
         p.comp_slice = 0
         bw = getattr(self, "_bw_now", None)
         p.band_weight = bw.data_ptr() if bw is not None else None
+        p.erange = int(self.tetra and (np.isfinite(self.Emin) or np.isfinite(self.Emax)))
+        p.Emin, p.Emax = self.Emin, self.Emax
         return p
 
     def _band_weights(self, data_K):

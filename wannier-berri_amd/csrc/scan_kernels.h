@@ -1362,22 +1362,35 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>()), (scan_min_waves<
         if (p.fder == 0 && !tab) {
             // bands below the range, up to the first group in it (data_K.py:178-185;
             // tetra: highest energy over centre and corners below Ef[0], tetrahedron.py:155-162)
-            int hi = 0;
-            for (int i = lane; i < nw; i += 64)
+            int hi = 0, lo = 0;
+            for (int i = lane; i < nw; i += 64) {
                 if ((tetra ? S.tEmx[i] : S.E[i]) < p.EFmin) hi = i + 1;
-            for (int o = 32; o >= 1; o >>= 1) hi = max(hi, __shfl_xor(hi, o));
+                // tetra with Emin: bands entirely below Emin stay out (tetrahedron.py:246-251)
+                if (tetra && p.erange && S.tEmx[i] < p.Emin) lo = i + 1;
+            }
+            for (int o = 32; o >= 1; o >>= 1) {
+                hi = max(hi, __shfl_xor(hi, o));
+                lo = max(lo, __shfl_xor(lo, o));
+            }
             sea_hi = first_in < hi ? first_in : hi;
+            sea_lo = lo < sea_hi ? lo : sea_hi;
         } else if (tetra && p.fder == -1) {
             // hole-like: bands whose lowest energy lies above Ef[-1], from the end of the last
             // group in range on (tetrahedron.py:165-172,246-253)
-            int bandmin = nw;
-            for (int i = lane; i < nw; i += 64)
+            int bandmin = nw, bandmax = nw;
+            for (int i = lane; i < nw; i += 64) {
                 if (S.tEmn[i] > p.EFmax) bandmin = min(bandmin, i);
-            for (int o = 32; o >= 1; o >>= 1) bandmin = min(bandmin, __shfl_xor(bandmin, o));
+                // with Emax: bands entirely above Emax stay out (tetrahedron.py:252-258)
+                if (p.erange && S.tEmn[i] > p.Emax) bandmax = min(bandmax, i);
+            }
+            for (int o = 32; o >= 1; o >>= 1) {
+                bandmin = min(bandmin, __shfl_xor(bandmin, o));
+                bandmax = min(bandmax, __shfl_xor(bandmax, o));
+            }
             if (bandmin < last_in) bandmin = last_in;
-            if (bandmin < nw) {
+            if (bandmin < bandmax) {
                 sea_lo = bandmin;
-                sea_hi = nw;
+                sea_hi = bandmax;
             }
         }
         if (lane == 0) {
@@ -1455,13 +1468,13 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>()), (scan_min_waves<
         __syncthreads();
         if (sea_hi > sea_lo) {
             // Fermi-sea group [sea_lo, sea_hi) (weight -inf; hole-like tetra: the bands above),
-            // added at every level; the other bands, on one side of it, are its "out"
-            const int out_lo = sea_lo == 0 ? sea_hi : 0, out_hi = sea_lo == 0 ? nw : sea_lo;
+            // added at every level; every other band is its "out" (with Emin / Emax on both
+            // sides of it: each side its own block)
             for (int n = tid; n < nw; n += (int)blockDim.x) {
                 int inn = n >= sea_lo && n < sea_hi;
                 S.lab[n] = inn ? 0 : 1;
-                S.gs[n] = inn ? sea_lo : out_lo;
-                S.ge[n] = inn ? sea_hi : out_hi;
+                S.gs[n] = inn ? sea_lo : (n < sea_lo ? 0 : sea_hi);
+                S.ge[n] = inn ? sea_hi : (n < sea_lo ? sea_lo : nw);
                 S.act[n] = inn;
             }
             __syncthreads();
