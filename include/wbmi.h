@@ -108,9 +108,10 @@ typedef struct wbmi_scan_params {
     /* select_bands (calculators/static.py:39,133,136; utility.py:398-403): device
      * [nw] float64, the number of times each band is listed; a band group adds its
      * value times the sum over its bands / its size (groups with none listed add
-     * nothing).  NULL = every band.  Fermi-level bins with fder >= 1 only (the
-     * reference refuses it for the Fermi sea, data_K.py:179-180); the tetrahedron
-     * and table entry points refuse a non-NULL value.                           */
+     * nothing).  NULL = every band.  fder >= 1 only, in Fermi-level bins and with
+     * tetrahedra (tetrahedron.py:238-255; the reference refuses it for the Fermi
+     * sea and the hole-like group, data_K.py:179-180); the table entry points
+     * refuse a non-NULL value.                                                   */
     const double* band_weight;
     /* Emin / Emax of the tetrahedron method (grid/tetrahedron.py:246-258), read when
      * erange != 0: the Fermi-sea group (fder 0) leaves out the bands whose energies over

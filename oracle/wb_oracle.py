@@ -524,7 +524,8 @@ def tetra_band_weight(Efermi, e_center, e_corners, der=0):
     return occ / 12.
 
 
-def tetra_group_weights(data_K, Efermi, der, degen_thresh=-1, degen_Kramers=False, Erange=(-np.inf, np.inf)):
+def tetra_group_weights(data_K, Efermi, der, degen_thresh=-1, degen_Kramers=False, Erange=(-np.inf, np.inf),
+                        select_bands=None):
     """Per k: {(ib1, ib2): weight over Efermi} (TetraWeights.weights_all_band_groups,
     tetrahedron.py:212-255).  Groups are split on the central energies; a group is kept
     when its energies over the centre and corners reach [Efermi[0], Efermi[-1]] (140-152);
@@ -548,7 +549,14 @@ def tetra_group_weights(data_K, Efermi, der, degen_thresh=-1, degen_Kramers=Fals
         else:
             def bw(i):
                 return tetra_band_weight(Efermi, Ec[ik, i], Ecorn[ik, :, :, :, i], der)
-        w = {(a, b): sum(bw(i) for i in range(a, b)) / (b - a) for a, b in in_range}
+        if select_bands is not None:
+            # groups without a listed band are dropped; the others weighted by their listed share
+            # (tetrahedron.py:147-149,238-241); the (inverse) Fermi sea refuses a selection
+            if der in (0, -1):
+                raise NotImplementedError("Selection of bands for Fermi sea is not implemented")
+            in_range = [(a, b) for a, b in in_range if set(range(a, b)) & set(int(x) for x in select_bands)]
+        w = {(a, b): sum(bw(i) for i in range(a, b)) / (b - a) * weight_select_bands(a, b, select_bands)
+             for a, b in in_range}
         if der == 0:
             below = np.where(Emax < Efermi[0])[0]
             bandmax = below[-1] + 1 if len(below) else 0
@@ -1092,7 +1100,7 @@ class StaticCalculator:
         formula = self.Formula(data_K, **self.kwargs_formula)
         if self.tetra:
             weights = tetra_group_weights(data_K, self.Efermi, -1 if self.hole_like else self.fder,
-                                          self.degen_thresh, self.degen_Kramers, self.Erange)
+                                          self.degen_thresh, self.degen_Kramers, self.Erange, self.select_bands)
         else:
             weights = data_K.get_bands_in_range_groups(self.EFmin, self.EFmax, degen_thresh=self.degen_thresh,
                                                        degen_Kramers=self.degen_Kramers, sea=(self.fder == 0),

@@ -382,7 +382,7 @@ def test_select_bands(wb):
     """select_bands (static.py:133,136; utility.py:398-403) on the random model with pairs of
     bands as groups (degen_Kramers) against the oracle: DOS, Berry curvature dipole and Ohmic
     conductivity on the Fermi surface with the even bands and with the odd bands listed twice;
-    FP64 at 1e-8.  The Fermi sea and tetrahedra refuse it."""
+    FP64 at 1e-8.  The Fermi sea refuses it (tetrahedra: test_gpu_tetra.py)."""
     st = wb.calculators.static
     s = gsys(wb, "random")
     o = osys("random")
@@ -403,8 +403,6 @@ def test_select_bands(wb):
             assert np.abs(ro[k].data).max() > 0 and e < 1e-8, k
     with pytest.raises(NotImplementedError, match="Fermi sea"):
         run_gpu(wb, s, {"ahc": st.AHC(Efermi=E, select_bands=[0])}, NK, NKFFT)
-    with pytest.raises(NotImplementedError):
-        st.DOS(Efermi=E, select_bands=[0], tetra=True)
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-8), ("fp32", 1e-5)])

@@ -155,6 +155,34 @@ def test_tetra_hole_like_vs_oracle(wb):
         assert err < 1e-10, k
 
 
+def test_tetra_select_bands_vs_oracle(wb):
+    """select_bands with the tetrahedron method (tetrahedron.py:147-149,238-241): DOS and the
+    Fermi-surface Berry curvature dipole with pairs of bands as groups (degen_Kramers), the even
+    bands and the odd bands listed twice, vs the oracle (FP64, 1e-10); the Fermi sea refuses it."""
+    say("test_tetra_select_bands_vs_oracle")
+    st = wb.calculators.static
+    from wannierberri_amd import models
+    s = models.synthetic(num_wann=8, Rmax=2, seed=11, with_BBCC=True)
+    o = O.OracleSystem(s.real_lattice, s.rvec.iRvec, s.wannier_centers_cart,
+                       {k: s.get_R_mat(k) for k in ("Ham", "AA", "BB", "CC")})
+    E0 = np.linalg.eigvalsh(s.get_R_mat("Ham")[np.where((s.rvec.iRvec == 0).all(1))[0][0]])
+    Ef = np.linspace(E0.min() - 1, E0.max() + 1, 23)
+    NK, NKFFT = [4, 4, 4], [2, 2, 2]
+    for sel in (list(range(0, 8, 2)), list(range(1, 8, 2)) * 2):
+        kw = dict(tetra=True, degen_Kramers=True, select_bands=sel)
+        gpu = {"dos": st.DOS(Efermi=Ef, **kw), "bcd": st.BerryDipole_FermiSurf(Efermi=Ef, **kw)}
+        ora = {"dos": O.DOS(Ef, **kw), "bcd": O.BerryDipole_FermiSurf(Ef, **kw)}
+        r = wb.run(s, wb.Grid(system=s, NK=NK, NKFFT=NKFFT), gpu, save_files=False, verbose=False)
+        ref = _oracle_tetra(o, ora, NK, NKFFT)
+        for k in gpu:
+            err = rel_err(r.results[k].data, ref[k].data)
+            say(f"  {sel[:3]} {k}: {err:.2e}")
+            assert np.abs(ref[k].data).max() > 0 and err < 1e-10, k
+    with pytest.raises(NotImplementedError, match="Fermi sea"):
+        wb.run(s, wb.Grid(system=s, NK=NK, NKFFT=NKFFT), {"c": st.CumDOS(Efermi=Ef, tetra=True, select_bands=[0])},
+               save_files=False, verbose=False)
+
+
 def test_tetra_erange_vs_oracle(wb):
     """Emin / Emax with the tetrahedron method (tetrahedron.py:246-258): the Fermi sea leaves out
     the bands entirely below Emin (so its group has "out" bands on both sides), the hole-like

@@ -191,12 +191,12 @@ def test_ws_dist_invariants(wb):
 def test_static_calculator_band_options(wb):
     """Emin / Emax of a static calculator are accepted without effect, as in the reference
     (calculators/static.py:83 passes them to data_K.py:172-186, which does not use them);
-    select_bands is refused for the Fermi sea (data_K.py:179-180) and, on this path, with tetra."""
+    select_bands is refused for the Fermi sea (data_K.py:179-180) and the tetrahedron hole-like group."""
     st = wb.calculators.static
     c = st.AHC(Efermi=np.linspace(-1, 1, 5), Emin=-0.5, Emax=0.5)
     assert c.Emin == -0.5 and c.Emax == 0.5
     with pytest.raises(NotImplementedError, match="Fermi sea"):
         st.AHC(Efermi=np.linspace(-1, 1, 5), select_bands=[0, 1])._band_weights(None)
-    with pytest.raises(NotImplementedError):
-        st.DOS(Efermi=np.linspace(-1, 1, 5), select_bands=[0, 1], tetra=True)
+    with pytest.raises(NotImplementedError, match="inverse Fermi sea"):
+        st.DOS(Efermi=np.linspace(-1, 1, 5), select_bands=[0, 1], tetra=True, hole_like=True)._band_weights(None)
     assert list(st.DOS(Efermi=np.linspace(-1, 1, 5), select_bands=[3, 1, 1]).select_bands) == [3, 1, 1]

@@ -253,3 +253,23 @@ def test_tetra_erange_cumdos(wb):
     assert np.allclose(np.abs(hb), 1.0, rtol=0, atol=1e-12)
     assert np.allclose(cum(hole_like=True, Emax=3.5), hb, rtol=0, atol=1e-12)
     assert np.allclose(cum(hole_like=True, Emax=0.3), 0.0, rtol=0, atol=1e-12)
+
+
+def test_tetra_select_bands_partition(wb):
+    """select_bands with the tetrahedron method (tetrahedron.py:147-149,238-241) on the Haldane
+    model with its two bands as one group (degen_Kramers): DOS with band 0 listed is half the
+    DOS, both bands listed the whole; the Fermi sea refuses a selection."""
+    s = wb.System_R.from_arrays_npz(os.path.join(GOLDEN, "sys_haldane.npz"))
+    _OracleBlock.osys = O.load_golden_system(os.path.join(GOLDEN, "sys_haldane.npz"))
+    Ef = np.linspace(-2.5, 2.5, 11)
+    grid = wb.Grid(system=s, NK=[6, 6, 1], NKFFT=[3, 3, 1])
+
+    def dos(sel, calc=O.DOS):
+        c = {"c": _OracleCalc(wb, calc(Ef, tetra=True, degen_Kramers=True, select_bands=sel), 1, 1)}
+        return wb.run(s, grid, c, data_k_class=_OracleBlock, save_files=False, verbose=False).results["c"].data
+    full = dos(None)
+    assert np.abs(full).max() > 0
+    assert np.allclose(dos([0]), 0.5 * full, rtol=0, atol=1e-12 * np.abs(full).max())
+    assert np.allclose(dos([0, 1]), full, rtol=0, atol=1e-12 * np.abs(full).max())
+    with pytest.raises(NotImplementedError, match="Fermi sea"):
+        dos([0], calc=O.CumDOS)

@@ -113,8 +113,6 @@ class StaticCalculator(Calculator):
         self.k_resolved = bool(k_resolved)
         self.select_bands = None if select_bands is None else np.array(select_bands, dtype=np.int64).reshape(-1)
         self._band_w = {}
-        if self.select_bands is not None and tetra:
-            raise NotImplementedError("select_bands with the tetrahedron method is not available on the MI355X path")
         # Fermi-level bins: without effect, as in the reference (get_bands_in_range_groups_ik
         # takes them but selects groups by the Fermi range alone, data_K.py:172-186); tetra:
         # they bound the Fermi-sea / hole-like group (tetrahedron.py:246-258)
@@ -212,6 +210,8 @@ class StaticCalculator(Calculator):
             return None
         if self.fder == 0:
             raise NotImplementedError("Selection of bands for Fermi sea is not implemented")
+        if self.tetra and self.hole_like:
+            raise NotImplementedError("Selection of bands for inverse Fermi sea is not implemented")
         nw = data_K.num_wann
         key = (data_K.ctx.device, nw)
         if key not in self._band_w:

@@ -147,8 +147,9 @@ static int scan_tetra(wbmi_ctx* ctx, int nk, int nw, int nch, const double* E_de
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad arguments (nw <= %d)", SC_NWMAX);
     if (params->fder < -1 || params->fder > 3 || !(params->EFmax >= params->EFmin))
         return wbmi_fail(ctx, WBMI_ERR_ARG, "wbmi_static_scan_tetra: bad fder or Fermi range");
-    if (params->band_weight)
-        return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED, "wbmi_static_scan_tetra: band selection is not available with tetrahedra");
+    if (params->band_weight && (params->fder == 0 || params->fder == -1))
+        return wbmi_fail(ctx, WBMI_ERR_UNSUPPORTED,
+                         "wbmi_static_scan_tetra: selection of bands for the Fermi sea (or the hole-like group) is not implemented");
     int rc = check_roles(ctx, params, nch);
     if (rc) return rc;
     ScanArgs a;

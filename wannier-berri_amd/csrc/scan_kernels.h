@@ -1456,7 +1456,7 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>()), (scan_min_waves<
                 if (!S.gin[g]) continue;
                 double s = 0.0;
                 for (int n = S.gb[g]; n < S.gb[g + 1]; ++n) s += S.acc[n * 9 + c].re;
-                S.tv[g * 9 + c] = s;
+                S.tv[g * 9 + c] = s * group_selected(p, S.gb[g], S.gb[g + 1]);
                 continue;
             }
             const int bin = S.gin[g] ? fermi_bin(p, S.gw[g]) : -1;
@@ -1515,7 +1515,9 @@ __global__ void __launch_bounds__((scan_threads<T, F, MODE>()), (scan_min_waves<
         for (int t = tid; t < ng * ncomp; t += (int)blockDim.x) {
             const int g = t / ncomp, c = t % ncomp;
             if (tetra) {
-                if (S.gin[g]) S.tv[g * 9 + c] = S.cut[S.gb[g + 1] * 9 + c] - S.cut[S.gb[g] * 9 + c];
+                if (S.gin[g])
+                    S.tv[g * 9 + c] = (S.cut[S.gb[g + 1] * 9 + c] - S.cut[S.gb[g] * 9 + c]) *
+                                      group_selected(p, S.gb[g], S.gb[g + 1]);
                 continue;
             }
             const int bin = S.gin[g] ? fermi_bin(p, S.gw[g]) : -1;
