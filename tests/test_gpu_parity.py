@@ -523,6 +523,28 @@ def test_restart_refinement_gpu(wb, tmp_path):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max(), k
 
 
+def test_restart_tabulate_gpu(wb, tmp_path):
+    """Restart files with tabulated results: energies and Berry curvature tabulated on a Haldane
+    grid, written with dump_results and read back with restart=True, equal the direct run."""
+    say("test_restart_tabulate_gpu")
+    tab = wb.calculators.tabulate
+    s = gsys(wb, "haldane")
+
+    def calcs():
+        return {"tab": tab.TabulatorAll({"Energy": tab.Energy(), "berry": tab.BerryCurvature()}, mode="grid")}
+    grid = wb.Grid(system=s, NK=[8, 8, 1], NKFFT=[4, 4, 1])
+    kw = dict(save_files=False, verbose=False)
+    full = wb.run(s, grid, calcs(), **kw)
+    rec = str(tmp_path / "rec")
+    wb.run(s, grid, calcs(), dump_results=True, file_Klist_path=rec, **kw)
+    back = wb.run(s, grid, calcs(), restart=True, file_Klist_path=rec, **kw)
+    a, b = full.results["tab"], back.results["tab"]
+    assert np.allclose(a.kpoints, b.kpoints)
+    for q in ("Energy", "berry"):
+        x, y = a.results[q].data, b.results[q].data
+        assert x.shape == y.shape and np.abs(x - y).max() <= 1e-12 * np.abs(x).max(), q
+
+
 @pytest.mark.parametrize("nw,precision,tol", [(40, "fp64", 1e-8), (40, "fp32", 1e-5), (64, "fp64", 1e-8)])
 def test_dynamic_large_nw(wb, nw, precision, tol):
     """Dynamic calculators above nw = 32, where the per-k matrices A / Y live in the workspace

@@ -77,8 +77,31 @@ def test_restart_refused(wb, tmp_path):
     g = wb.grid.GridTetra.__new__(wb.grid.GridTetra)   # refused before the grid is used
     with pytest.raises(NotImplementedError):
         wb.run(s, g, {}, dump_results=True, file_Klist_path=str(tmp_path / "r"), save_files=False, verbose=False)
-    # tabulated (k-resolved) results are not written
+    # other result types are not written
     from wannierberri_amd import restart
-    tab = wb.result.TABresult(np.zeros((1, 3)), np.eye(3), {"Energy": wb.result.KBandResult(np.zeros((1, 2)))})
+    kb = wb.result.KBandResult(np.zeros((1, 2)))
     with pytest.raises(NotImplementedError):
-        restart.dump_result(str(tmp_path), 0, wb.ResultDict({"tab": tab}))
+        restart.dump_result(str(tmp_path), 0, wb.ResultDict({"kb": kb}))
+
+
+def test_restart_tabresult_round_trip(wb, tmp_path):
+    """A K-point's tabulated result (TABresult of KBandResults with their parities, path or grid
+    mode) written to its restart record and read back equals the original, without pickling."""
+    from wannierberri_amd import restart
+    rng = np.random.default_rng(3)
+    k = rng.random((5, 3))
+    res = {"Energy": wb.result.KBandResult(rng.random((5, 4)), transformTR=1, transformInv=1),
+           "berry": wb.result.KBandResult(rng.random((5, 4, 3)), transformTR=-1, transformInv=1),
+           "plain": wb.result.KBandResult(rng.random((5, 4)))}
+    tab = wb.result.TABresult(k, 2 * np.eye(3), results=res, mode="path", save_mode="txt")
+    e = wb.result.EnergyResult(np.linspace(0, 1, 3), rng.random(3))
+    restart.dump_result(str(tmp_path), 7, wb.ResultDict({"tab": tab, "e": e}))
+    back = restart.load_result(str(tmp_path), 7, {"tab": None, "e": None}).results
+    t = back["tab"]
+    assert isinstance(t, wb.result.TABresult) and t.mode == "path" and t.save_mode == "txt"
+    assert np.array_equal(t.kpoints, tab.kpoints) and np.array_equal(t.recip_lattice, tab.recip_lattice)
+    assert sorted(t.results) == sorted(res)
+    for q, kb in res.items():
+        assert np.array_equal(t.results[q].data, kb.data)
+        assert (t.results[q].transformTR, t.results[q].transformInv) == (kb.transformTR, kb.transformInv)
+    assert np.array_equal(back["e"].data, e.data)

@@ -7,11 +7,14 @@ Directory `file_Klist_path` (default `_tmp_wb`):
 * `K_list.npz`: the K-points (K, dK, NKFFT, refinement level), whether each was
   evaluated and its refinement metric (`max`), rewritten after every iteration;
 * `_Kp-<ik>.npz`: the (symmetrised) result of K-point ik, one group of keys per
-  calculator (`<name>/Energies_0`, `<name>/data`, `<name>/rank`).
+  calculator: EnergyResult `<name>/Energies_0`, `<name>/data`, `<name>/rank`;
+  TABresult `<name>/tab/kpoints`, `<name>/tab/recip_lattice`, `<name>/tab/mode`,
+  `<name>/tab/save_mode` and per quantity q `<name>/tab/q/<q>/data`, `.../tr`, `.../inv`
+  (the KBandResult's time-reversal and inversion parities, NaN for none).
 
 The reference pickles its K-points and results; these files hold plain arrays
 only, so this build neither reads nor writes the reference's restart files.
-Only regular grids (KpointBZparallel) and EnergyResult results are supported.
+Only regular grids (KpointBZparallel) and EnergyResult / TABresult results are supported.
 """
 import glob
 import os
@@ -19,7 +22,7 @@ import os
 import numpy as np
 
 from .grid import KpointBZparallel
-from .result import EnergyResult, ResultDict
+from .result import EnergyResult, KBandResult, ResultDict, TABresult
 
 KLIST = "K_list.npz"
 
@@ -56,13 +59,36 @@ def result_file(path, ik):
 def dump_result(path, ik, res):
     d = {}
     for name, r in res.results.items():
+        if isinstance(r, TABresult):
+            t = f"{name}/tab/"
+            d[t + "kpoints"], d[t + "recip_lattice"] = r.kpoints, np.asarray(r.recip_lattice, dtype=float)
+            d[t + "mode"], d[t + "save_mode"] = np.array(r.mode), np.array(r.save_mode)
+            for q, kb in r.results.items():
+                d[f"{t}q/{q}/data"] = kb.data
+                d[f"{t}q/{q}/tr"] = np.array(np.nan if kb.transformTR is None else kb.transformTR, dtype=float)
+                d[f"{t}q/{q}/inv"] = np.array(np.nan if kb.transformInv is None else kb.transformInv, dtype=float)
+            continue
         if not isinstance(r, EnergyResult):
-            raise NotImplementedError(f"restart files hold EnergyResult results only, not {type(r).__name__} "
-                                      f"('{name}')")
+            raise NotImplementedError(f"restart files hold EnergyResult and TABresult results only, not "
+                                      f"{type(r).__name__} ('{name}')")
         for key, val in r.as_dict().items():
             if key != "comment":
                 d[f"{name}/{key}"] = val
     np.savez_compressed(result_file(path, ik), **d)
+
+
+def _parity(x):
+    x = float(x)
+    return None if np.isnan(x) else int(x)
+
+
+def _load_tab(f, name):
+    t = f"{name}/tab/"
+    qs = sorted({k[len(t) + 2:].rsplit("/", 1)[0] for k in f.files if k.startswith(t + "q/")})
+    results = {q: KBandResult(f[f"{t}q/{q}/data"], transformTR=_parity(f[f"{t}q/{q}/tr"]),
+                              transformInv=_parity(f[f"{t}q/{q}/inv"])) for q in qs}
+    return TABresult(f[t + "kpoints"], f[t + "recip_lattice"], results=results, mode=str(f[t + "mode"]),
+                     save_mode=str(f[t + "save_mode"]))
 
 
 def load_result(path, ik, calculators):
@@ -70,6 +96,9 @@ def load_result(path, ik, calculators):
     out = {}
     with np.load(result_file(path, ik), allow_pickle=False) as f:
         for name, calc in calculators.items():
+            if f"{name}/tab/kpoints" in f.files:
+                out[name] = _load_tab(f, name)
+                continue
             keys = [k for k in f.files if k.startswith(name + "/")]
             E = [f[k] for k in sorted(keys) if k.split("/", 1)[1].startswith("Energies_")]
             rank = int(f[name + "/rank"])
